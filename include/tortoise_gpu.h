@@ -1,0 +1,138 @@
+/*
+ * tortoise_gpu.h -- C ABI of the MI355X scattering-integral backend.
+ *
+ * One context owns one mesh, one electron-phonon channel and its assembled
+ * kernel on the current HIP device, plus one HIP stream (the caller may
+ * substitute its own).  A context is not thread-safe; calls on one context
+ * are serialised by the caller.  Errors are returned as negative status codes
+ * and described by tg_last_error(); nothing throws across this boundary.
+ *
+ * Data conventions
+ *   - nodal P1 values on the (nsec+1)^dim vertex lattice of a structured mesh,
+ *     node id = ix + (nx+1)*(iy + (ny+1)*iz);
+ *   - a batch of B members is column-major n x B: member b at ptr + b*n;
+ *   - FP64 at the boundary in every precision mode;
+ *   - pointers are host memory unless TG_PTR_DEVICE is set in flags, in which
+ *     case they are device memory on the context's device.
+ *
+ * Reference interfaces replaced (paths relative to the TORTOISE tree):
+ *   MaterialStatus<NDim>::propagate()            src/Physics/Status/MaterialStatus.cpp:146-161
+ *   ScatteringChannel<NDim,NLegs>::propagate()   src/Physics/Scattering/ScatteringChannel.cpp:651-670
+ *   MaterialTimeStatus<NDim>::propagate(tf, dt)  src/Physics/Status/MaterialTimeStatus.cpp:102-137
+ *   Algorithms::dormandPrince54 (vetoer form)    include/Generics/Algorithms/DormandPrince54.hpp:479-665
+ *   MaterialStatus<NDim>::relativeError()        src/Physics/Status/MaterialStatus.cpp:189-196
+ *   Band<NDim>::relativeError()                  src/Physics/Material/Band.cpp:87-101
+ *   Function<NDim>::integrate() / integrate(rhs) src/Geometry/Structured/FunctionSpace/Function.cpp:369-375
+ */
+#ifndef TORTOISE_GPU_H
+#define TORTOISE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tg_ctx tg_ctx;
+typedef int32_t tg_status;
+
+#define TG_OK                  0
+#define TG_ERR_ARG            -1   /* invalid argument or size */
+#define TG_ERR_HIP            -2   /* HIP runtime / kernel launch failure */
+#define TG_ERR_NONFINITE      -3   /* NaN/Inf met in a state or error norm */
+#define TG_ERR_STEP_UNDERFLOW -4   /* DP5 step fell below dense_dt/1e4 (DormandPrince54.hpp:658-661) */
+#define TG_ERR_STATE          -5   /* call out of order (e.g. apply before assemble) */
+#define TG_ERR_NOMEM          -6   /* device allocation failed */
+
+/* flags */
+#define TG_PTR_DEVICE   0x1   /* f / I / out pointers are device memory */
+#define TG_PREC_MIXED   0x2   /* batched fp16 hi+lo split kernel, FP32 MFMA, FP64 epilogue */
+#define TG_CONSERVE     0x4   /* remove the particle-number defect of one evaluation */
+#define TG_QUIET        0x8   /* (dp5) no per-step console line */
+
+typedef struct tg_problem_desc {
+    int32_t dim;            /* 2 or 3 */
+    int32_t nsec[3];        /* sections per axis (reference Mesh nSecSplits) */
+    double  origin[3];      /* physical coordinates of the mesh origin */
+    double  extent[3];      /* edge lengths of the (axis-aligned) mesh box */
+    const double* eps;      /* nodal band energies, host memory, tg_num_nodes() values */
+    double  g;              /* e-ph coupling; channel amplitude = g*g */
+    double  omega0;         /* dispersionless optical phonon energy */
+    double  sigma;          /* Gaussian broadening of the energy delta */
+    double  bath_n;         /* default phonon bath occupation */
+    double  de;             /* energy-grid spacing of the overlap quadrature (<= 0: sigma/8) */
+    double  cutoff;         /* Gaussian cut-off in units of sigma (<= 0: 8) */
+    int32_t max_batch;      /* largest batch the context will be asked for (>= 1) */
+    int32_t reserved;
+} tg_problem_desc;
+
+typedef struct tg_step_stats {
+    int64_t attempted;      /* attempted DP5 steps */
+    int64_t accepted;
+    int64_t rejected;
+    int64_t evals;          /* collision-operator evaluations of this member */
+    int32_t failed;         /* 1: step underflow */
+    int32_t n_dense;        /* dense outputs produced */
+} tg_step_stats;
+
+/* ---------------------------------------------------------------- setup */
+/* Number of lattice nodes of a structured mesh (no context needed). */
+int64_t     tg_num_nodes(int32_t dim, const int32_t* nsec);
+
+/* Builds the mesh, uploads nodal data and sizes the energy projection.
+ * Replaces the construction of Mesh/Band/Material + ScatteringChannel
+ * (src/Physics/Material/Material.cpp:244-301). */
+tg_status   tg_ctx_create(const tg_problem_desc* desc, tg_ctx** out);
+void        tg_ctx_destroy(tg_ctx* ctx);
+const char* tg_last_error(const tg_ctx* ctx);
+/* Launch all work of this context on `hip_stream` (a hipStream_t); NULL
+ * restores the context's own stream. */
+tg_status   tg_ctx_set_stream(tg_ctx* ctx, void* hip_stream);
+void*       tg_ctx_stream(const tg_ctx* ctx);
+tg_status   tg_mesh_info(const tg_ctx* ctx, int64_t* n_nodes, int64_t* n_elements,
+                         double* elem_volume);
+/* Lumped P1 masses V_i (host out, n values). */
+tg_status   tg_node_volumes(tg_ctx* ctx, double* V);
+
+/* Assembles the emission kernel K (FP64, once) and, when max_batch > 1 or
+ * TG_PREC_MIXED is in flags, the stacked [K; K^T] fp16 hi+lo split used by the
+ * batched kernel.  Replaces the tuple screening of
+ * ScatteringChannel::precalculateIndices (ScatteringChannel.cpp:421-538). */
+tg_status   tg_kernel_assemble(tg_ctx* ctx, int32_t flags);
+/* Copies rows [row0, row0+nrows) of K (FP64) to host `out` (nrows x n). */
+tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double* out);
+
+/* -------------------------------------------------------- the hot path */
+/* One collision-operator evaluation per member:
+ *   I_b = [(1-f_b)(W_b f_b) - f_b (W_b^T (1-f_b))] / V,  W_b = (n_b+1) K + n_b K^T.
+ * bath_n: per-member bath occupations (batch values) or NULL for the default.
+ * Default path: FP64 one-pass K f / K^T f kernel (any batch, members in turn).
+ * TG_PREC_MIXED: batched MFMA kernel (batch <= max_batch). */
+tg_status   tg_collision_apply(tg_ctx* ctx, const double* f, double* I, int64_t n,
+                               int32_t batch, const double* bath_n, int32_t flags);
+
+/* Adaptive DP5(4) from t0 to t1 for every member (per-member step size and
+ * accept/reject, error norm of Band::relativeError for a fermion band).
+ * f (n x batch) holds the initial state and receives the state after the
+ * last accepted step.  dense_out (optional, max_dense x n x batch, member-
+ * major: member b's k-th output at dense_out + (b*max_dense + k)*n) receives
+ * the dense outputs at t0 + k*dense_dt.  stats: batch entries (optional). */
+tg_status   tg_dp5_integrate(tg_ctx* ctx, double* f, int64_t n, int32_t batch,
+                             double t0, double t1, double dense_dt, double rtol,
+                             double atol, const double* bath_n, int32_t flags,
+                             double* dense_out, int32_t max_dense,
+                             tg_step_stats* stats);
+
+/* FP64 sums per member: number = sum_i V_i f_i, energy = sum_i V_i eps_i f_i. */
+tg_status   tg_conservation(tg_ctx* ctx, const double* f, int64_t n, int32_t batch,
+                            double* number, double* energy, int32_t flags);
+
+/* Band::relativeError (fermion) of err against pop, FP64 (host results). */
+tg_status   tg_relative_error(tg_ctx* ctx, const double* err, const double* pop,
+                              int64_t n, int32_t batch, double rtol, double atol,
+                              double* out, int32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,299 @@
+"""ctypes front-end of the CPU FP64 oracle (oracle/tortoise_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker the HIP path is compared
+against.  The product package never imports it.
+
+Reference files restated (paths relative to the TORTOISE tree):
+  mesh numbering       src/Geometry/Structured/DirectSpace/Mesh.cpp:55-87,222-267,384-433
+  error norm           src/Physics/Material/Band.cpp:87-101, MaterialStatus.cpp:189-196
+  DP5(4) driver        include/Generics/Algorithms/DormandPrince54.hpp:479-665
+  amplitude convention src/Physics/Scattering/ScatteringChannel.cpp:399-415
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+
+
+class EGrid(C.Structure):
+    _fields_ = [("e0", C.c_double), ("de", C.c_double), ("nbins", C.c_int32)]
+
+
+class DP5Stats(C.Structure):
+    _fields_ = [("attempted", C.c_int64), ("accepted", C.c_int64), ("rejected", C.c_int64),
+                ("evals", C.c_int64), ("failed", C.c_int32), ("n_dense", C.c_int32)]
+
+
+RHS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+NORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def build(force: bool = False) -> str:
+    """Compiles liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "all"], cwd=_HERE, check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_num_nodes.restype = C.c_int64
+        L.or_num_nodes.argtypes = [C.c_int, _i32p]
+        L.or_num_elements.restype = C.c_int64
+        L.or_num_elements.argtypes = [C.c_int, _i32p]
+        L.or_element_volume.restype = C.c_double
+        L.or_element_volume.argtypes = [C.c_int, _i32p, _f64p]
+        L.or_mesh_nodes.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p]
+        L.or_mesh_elements.argtypes = [C.c_int, _i32p, _i32p]
+        L.or_node_volumes.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+        L.or_egrid_make.argtypes = [_f64p, C.c_int64, C.c_double, C.POINTER(EGrid)]
+        L.or_projection.restype = C.c_int32
+        L.or_projection.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.POINTER(EGrid), _i32p, _i32p,
+                                    C.c_void_p, C.c_int32]
+        L.or_simplex_density.restype = C.c_double
+        L.or_simplex_density.argtypes = [C.c_int, _f64p, C.c_int, C.c_double, C.c_double]
+        L.or_gauss_table.restype = C.c_int32
+        L.or_gauss_table.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.c_int32), C.c_void_p, C.c_int32]
+        L.or_kernel_full.argtypes = [C.c_int64, _i32p, _i32p, _f64p, C.c_int32, _f64p, C.c_int32,
+                                     C.c_int32, C.c_double, _f64p]
+        L.or_kernel_row.argtypes = [C.c_int64, C.c_int64, _i32p, _i32p, _f64p, C.c_int32, _f64p,
+                                    C.c_int32, C.c_int32, C.c_double, _f64p]
+        L.or_collision_apply.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, _f64p, C.c_int]
+        L.or_conservation.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        L.or_relative_error.restype = C.c_double
+        L.or_relative_error.argtypes = [C.c_int, C.c_int64, _i32p, _f64p, _f64p, C.c_double,
+                                        C.c_double, C.c_double, C.c_double]
+        L.or_dp5_integrate.argtypes = [C.c_int64, _f64p, C.c_double, C.c_double, C.c_double,
+                                       RHS_FN, NORM_FN, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.POINTER(DP5Stats)]
+        L.or_dp5_collision.argtypes = [C.c_int, _i32p, C.c_int64, _f64p, _f64p, C.c_double, _f64p,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                       C.c_void_p, C.c_int64, C.POINTER(DP5Stats)]
+        L.or_dp5_collision_replay.argtypes = [C.c_int, _i32p, C.c_int64, _f64p, _f64p, C.c_double,
+                                              _f64p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                              _f64p, _i8p, C.c_int64, C.c_void_p, C.c_void_p,
+                                              C.c_int32]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ------------------------------------------------------------------ mesh
+def _nsec(nsec):
+    a = np.zeros(3, np.int32)
+    a[: len(nsec)] = nsec
+    return a
+
+
+def _vec3(v):
+    a = np.zeros(3, np.float64)
+    a[: len(v)] = v
+    return a
+
+
+def num_nodes(dim, nsec):
+    return int(lib().or_num_nodes(dim, _nsec(nsec)))
+
+
+def num_elements(dim, nsec):
+    return int(lib().or_num_elements(dim, _nsec(nsec)))
+
+
+def element_volume(dim, nsec, extent):
+    return float(lib().or_element_volume(dim, _nsec(nsec), _vec3(extent)))
+
+
+def mesh_nodes(dim, nsec, origin, extent):
+    n = num_nodes(dim, nsec)
+    out = np.zeros(n * dim)
+    lib().or_mesh_nodes(dim, _nsec(nsec), _vec3(origin), _vec3(extent), out)
+    return out.reshape(n, dim)
+
+
+def mesh_elements(dim, nsec):
+    ne = num_elements(dim, nsec)
+    out = np.zeros(ne * (dim + 1), np.int32)
+    lib().or_mesh_elements(dim, _nsec(nsec), out)
+    return out.reshape(ne, dim + 1)
+
+
+def node_volumes(dim, nsec, extent):
+    out = np.zeros(num_nodes(dim, nsec))
+    lib().or_node_volumes(dim, _nsec(nsec), _vec3(extent), out)
+    return out
+
+
+# ------------------------------------------------------ energy projection
+def egrid(eps, de):
+    g = EGrid()
+    e = np.ascontiguousarray(eps, np.float64)
+    lib().or_egrid_make(e, e.size, de, C.byref(g))
+    return g
+
+
+def projection(dim, nsec, extent, eps, grid):
+    """Returns (off, len, R) with R of shape (n, maxw)."""
+    n = num_nodes(dim, nsec)
+    eps = np.ascontiguousarray(eps, np.float64)
+    off = np.zeros(n, np.int32)
+    ln = np.zeros(n, np.int32)
+    maxw = lib().or_projection(dim, _nsec(nsec), _vec3(extent), eps, C.byref(grid), off, ln, None, 0)
+    R = np.zeros((n, maxw))
+    lib().or_projection(dim, _nsec(nsec), _vec3(extent), eps, C.byref(grid), off, ln, _ptr(R), maxw)
+    return off, ln, R
+
+
+def simplex_density(dim, evert, a, vol, x):
+    return float(lib().or_simplex_density(dim, np.ascontiguousarray(evert, np.float64), a, vol, x))
+
+
+def gauss_table(de, shift, sigma, cutoff):
+    mlo = C.c_int32()
+    cnt = lib().or_gauss_table(de, shift, sigma, cutoff, C.byref(mlo), None, 0)
+    gm = np.zeros(cnt)
+    lib().or_gauss_table(de, shift, sigma, cutoff, C.byref(mlo), _ptr(gm), cnt)
+    return int(mlo.value), gm
+
+
+class Kernel:
+    """Emission kernel K of the nodal W-form on a structured mesh."""
+
+    def __init__(self, dim, nsec, origin, extent, eps, g, omega0, sigma, de=None, cutoff=8.0):
+        self.dim, self.nsec = dim, list(nsec)
+        self.origin, self.extent = list(origin), list(extent)
+        self.eps = np.ascontiguousarray(eps, np.float64)
+        self.g, self.omega0, self.sigma = g, omega0, sigma
+        self.de = sigma / 8.0 if de is None else de
+        self.cutoff = cutoff
+        self.n = num_nodes(dim, nsec)
+        self.V = node_volumes(dim, nsec, extent)
+        self.grid = egrid(self.eps, self.de)
+        self.off, self.len, self.R = projection(dim, nsec, extent, self.eps, self.grid)
+        self.maxw = self.R.shape[1]
+        self.m_lo, self.gm = gauss_table(self.de, omega0, sigma, cutoff)
+        self.amp = g * g
+
+    def row(self, i):
+        out = np.zeros(self.n)
+        lib().or_kernel_row(self.n, i, self.off, self.len, self.R, self.maxw, self.gm,
+                            self.m_lo, self.gm.size, self.amp, out)
+        return out
+
+    def full(self):
+        K = np.zeros((self.n, self.n))
+        lib().or_kernel_full(self.n, self.off, self.len, self.R, self.maxw, self.gm,
+                             self.m_lo, self.gm.size, self.amp, K)
+        return K
+
+
+# ---------------------------------------------------------------- collision
+def collision_apply(K, V, bath_n, f, conserve=False):
+    K = np.ascontiguousarray(K, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    out = np.zeros_like(f)
+    lib().or_collision_apply(f.size, K, np.ascontiguousarray(V, np.float64), bath_n, f, out,
+                             int(bool(conserve)))
+    return out
+
+
+def conservation(V, eps, f):
+    num, en = C.c_double(), C.c_double()
+    lib().or_conservation(f.size, np.ascontiguousarray(V, np.float64),
+                          np.ascontiguousarray(eps, np.float64),
+                          np.ascontiguousarray(f, np.float64), C.byref(num), C.byref(en))
+    return num.value, en.value
+
+
+def relative_error(dim, conn, err, pop, stat=-1.0, rtol=1e-3, atol=1e-3, penalty=0.0):
+    conn = np.ascontiguousarray(conn, np.int32)
+    return float(lib().or_relative_error(dim, conn.shape[0], conn.reshape(-1),
+                                         np.ascontiguousarray(err, np.float64),
+                                         np.ascontiguousarray(pop, np.float64),
+                                         stat, rtol, atol, penalty))
+
+
+# ------------------------------------------------------------------- DP5
+def dp5_integrate(rhs, norm, y0, t0, t_final, dense_dt, max_dense=100000, log_cap=1 << 20):
+    """Generic DP5(4) with Python callbacks: rhs(t, y) -> dydt, norm(err, y5) -> float."""
+    y = np.array(y0, np.float64, copy=True)
+    n = y.size
+
+    def _rhs(_u, t, yp, dp):
+        ya = np.ctypeslib.as_array(yp, shape=(n,))
+        np.ctypeslib.as_array(dp, shape=(n,))[:] = rhs(t, ya.copy())
+
+    def _norm(_u, ep, yp):
+        return float(norm(np.ctypeslib.as_array(ep, shape=(n,)).copy(),
+                          np.ctypeslib.as_array(yp, shape=(n,)).copy()))
+
+    cr, cn = RHS_FN(_rhs), NORM_FN(_norm)
+    dense = np.zeros((max_dense, n))
+    dt = np.zeros(max_dense)
+    hl = np.zeros(log_cap)
+    al = np.zeros(log_cap, np.int8)
+    st = DP5Stats()
+    lib().or_dp5_integrate(n, y, t0, t_final, dense_dt, cr, cn, None, _ptr(dense), _ptr(dt),
+                           max_dense, _ptr(hl), _ptr(al), log_cap, C.byref(st))
+    nd = min(st.n_dense, max_dense)
+    na = min(st.attempted, log_cap)
+    return dict(y=y, dense=dense[:nd], dense_t=dt[:nd], h=hl[:na], accepted=al[:na].astype(bool),
+                stats=st)
+
+
+def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol,
+                  conserve=False, max_dense=100000, log_cap=1 << 20):
+    f = np.array(f0, np.float64, copy=True)
+    n = f.size
+    dense = np.zeros((max_dense, n)) if max_dense else None
+    dt = np.zeros(max(max_dense, 1))
+    hl = np.zeros(log_cap)
+    al = np.zeros(log_cap, np.int8)
+    st = DP5Stats()
+    lib().or_dp5_collision(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
+                           np.ascontiguousarray(V, np.float64), bath_n, f, t0, t_final, dense_dt,
+                           rtol, atol, int(bool(conserve)), _ptr(dense), _ptr(dt), max_dense,
+                           _ptr(hl), _ptr(al), log_cap, C.byref(st))
+    nd = min(st.n_dense, max_dense)
+    na = min(st.attempted, log_cap)
+    return dict(y=f, dense=None if dense is None else dense[:nd], dense_t=dt[:nd], h=hl[:na],
+                accepted=al[:na].astype(bool), stats=st)
+
+
+def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, accepted,
+                         conserve=False, max_dense=100000):
+    f = np.array(f0, np.float64, copy=True)
+    n = f.size
+    dense = np.zeros((max_dense, n))
+    dt = np.zeros(max_dense)
+    h = np.ascontiguousarray(h, np.float64)
+    acc = np.ascontiguousarray(np.asarray(accepted).astype(np.int8))
+    lib().or_dp5_collision_replay(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
+                                  np.ascontiguousarray(V, np.float64), bath_n, f, t0, t_final,
+                                  dense_dt, int(bool(conserve)), h, acc, h.size, _ptr(dense),
+                                  _ptr(dt), max_dense)
+    nd = int(np.count_nonzero(dt))
+    return dict(y=f, dense=dense[:nd], dense_t=dt[:nd])

@@ -1,0 +1,668 @@
+/*
+ * tortoise_oracle.c -- CPU FP64 restatement of the scattering-integral hot
+ * path.  TEST INFRASTRUCTURE ONLY (see tortoise_oracle.h for the reference
+ * files each part follows and for what is new).  OpenMP over rows where the
+ * work is large; every reduction order is fixed so results are repeatable.
+ */
+#include "tortoise_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ===================================================================== mesh
+ * Section s = (sx, sy[, sz]) owns dim! elements.  Element id
+ *   2D: 2*(sx + sy*nx) + t            (Mesh.cpp:412-415)
+ *   3D: 6*(sx + (sy + sz*ny)*nx) + t  (Mesh.cpp:416-420)
+ * Vertex order inside a section: depth-first walk from the origin corner that
+ * sets one still-zero axis to 1 at each step, axes tried in increasing order
+ * (Mesh.cpp:55-87).  For 2D: t=0 -> (0,0),(1,0),(1,1); t=1 -> (0,0),(0,1),(1,1).
+ * Nodes are the (nsec+1)^dim lattice, node id ix + (nx+1)*(iy + (ny+1)*iz).
+ */
+static int fact(int d) { int r = 1; for (int i = 2; i <= d; ++i) r *= i; return r; }
+
+/* vertex offsets of element t in a section: off[a*3+d] in {0,1} */
+static void ref_section_vertices(int dim, int t, int* off) {
+    /* permutations of the axes in lexicographic order = DFS order */
+    int perm[3] = {0, 1, 2};
+    /* enumerate the t-th permutation of {0..dim-1} in lexicographic order */
+    int avail[3] = {0, 1, 2};
+    int navail = dim;
+    int rem = t;
+    for (int k = 0; k < dim; ++k) {
+        int f = fact(dim - 1 - k);
+        int idx = rem / f;
+        rem %= f;
+        perm[k] = avail[idx];
+        for (int q = idx; q < navail - 1; ++q) avail[q] = avail[q + 1];
+        --navail;
+    }
+    int cur[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) off[d] = 0;
+    for (int a = 1; a <= dim; ++a) {
+        cur[perm[a - 1]] = 1;
+        for (int d = 0; d < 3; ++d) off[a * 3 + d] = cur[d];
+    }
+}
+
+int64_t or_num_nodes(int dim, const int32_t* nsec) {
+    int64_t n = 1;
+    for (int d = 0; d < dim; ++d) n *= (int64_t)(nsec[d] + 1);
+    return n;
+}
+
+int64_t or_num_elements(int dim, const int32_t* nsec) {
+    int64_t n = fact(dim);
+    for (int d = 0; d < dim; ++d) n *= nsec[d];
+    return n;
+}
+
+double or_element_volume(int dim, const int32_t* nsec, const double* extent) {
+    double v = 1.0;
+    for (int d = 0; d < dim; ++d) v *= extent[d] / (double)nsec[d];
+    return fabs(v / (double)fact(dim));
+}
+
+void or_mesh_nodes(int dim, const int32_t* nsec, const double* origin,
+                   const double* extent, double* coords) {
+    int64_t n = or_num_nodes(dim, nsec);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = i;
+        for (int d = 0; d < dim; ++d) {
+            int64_t id = r % (nsec[d] + 1);
+            r /= (nsec[d] + 1);
+            coords[i * dim + d] = origin[d] + (extent[d] / (double)nsec[d]) * (double)id;
+        }
+    }
+}
+
+static int64_t node_id(int dim, const int32_t* nsec, const int* c) {
+    int64_t id = 0, stride = 1;
+    for (int d = 0; d < dim; ++d) { id += stride * c[d]; stride *= (nsec[d] + 1); }
+    return id;
+}
+
+void or_mesh_elements(int dim, const int32_t* nsec, int32_t* conn) {
+    int nes = fact(dim);
+    int64_t nsecs = 1;
+    for (int d = 0; d < dim; ++d) nsecs *= nsec[d];
+    for (int64_t s = 0; s < nsecs; ++s) {
+        int sc[3] = {0, 0, 0};
+        int64_t r = s;
+        for (int d = 0; d < dim; ++d) { sc[d] = (int)(r % nsec[d]); r /= nsec[d]; }
+        for (int t = 0; t < nes; ++t) {
+            int off[12];
+            ref_section_vertices(dim, t, off);
+            for (int a = 0; a <= dim; ++a) {
+                int c[3];
+                for (int d = 0; d < dim; ++d) c[d] = sc[d] + off[a * 3 + d];
+                conn[(s * nes + t) * (dim + 1) + a] = (int32_t)node_id(dim, nsec, c);
+            }
+        }
+    }
+}
+
+void or_node_volumes(int dim, const int32_t* nsec, const double* extent, double* V) {
+    int64_t n = or_num_nodes(dim, nsec), nel = or_num_elements(dim, nsec);
+    double vol = or_element_volume(dim, nsec, extent);
+    int32_t* conn = (int32_t*)malloc(sizeof(int32_t) * nel * (dim + 1));
+    or_mesh_elements(dim, nsec, conn);
+    for (int64_t i = 0; i < n; ++i) V[i] = 0.0;
+    for (int64_t e = 0; e < nel; ++e)
+        for (int a = 0; a <= dim; ++a) V[conn[e * (dim + 1) + a]] += vol / (double)(dim + 1);
+    free(conn);
+}
+
+/* ======================================================= energy projection */
+void or_egrid_make(const double* eps, int64_t n, double de, or_egrid* g) {
+    double lo = eps[0], hi = eps[0];
+    for (int64_t i = 1; i < n; ++i) { if (eps[i] < lo) lo = eps[i]; if (eps[i] > hi) hi = eps[i]; }
+    g->de = de;
+    g->e0 = lo - de;
+    g->nbins = (int32_t)floor((hi - g->e0) / de) + 3;
+}
+
+static int32_t egrid_cell(const or_egrid* g, double x) {
+    int32_t c = (int32_t)floor((x - g->e0) / g->de);
+    if (c < 0) c = 0;
+    if (c > g->nbins - 2) c = g->nbins - 2;
+    return c;
+}
+
+/* Cox-de Boor value of the order-k B-spline N_{0,k} on knots t[0..k] at x,
+ * where x lies strictly inside the knot span [t[span], t[span+1]). */
+static double bspline_value(const double* t, int k, int span, double x) {
+    double N[5] = {0, 0, 0, 0, 0};
+    N[span] = 1.0;
+    for (int q = 2; q <= k; ++q) {
+        for (int i = 0; i <= k - q; ++i) {
+            double v = 0.0;
+            double d1 = t[i + q - 1] - t[i];
+            double d2 = t[i + q] - t[i + 1];
+            if (d1 > 0.0) v += (x - t[i]) / d1 * N[i];
+            if (d2 > 0.0) v += (t[i + q] - x) / d2 * N[i + 1];
+            N[i] = v;
+        }
+    }
+    return N[0];
+}
+
+static void sort_small(double* v, int n) {
+    for (int i = 1; i < n; ++i) {
+        double x = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j] > x) { v[j + 1] = v[j]; --j; }
+        v[j + 1] = x;
+    }
+}
+
+double or_simplex_density(int dim, const double* evert, int a, double vol, double x) {
+    double t[5];
+    for (int q = 0; q <= dim; ++q) t[q] = evert[q];
+    t[dim + 1] = evert[a];
+    sort_small(t, dim + 2);
+    double span = t[dim + 1] - t[0];
+    if (!(span > 0.0) || x <= t[0] || x >= t[dim + 1]) return 0.0;
+    int s = 0;
+    while (s < dim && !(x < t[s + 1])) ++s;
+    return vol * bspline_value(t, dim + 1, s, x) / span;
+}
+
+/* 3-point Gauss-Legendre on [0,1] */
+static const double GL3_X[3] = {0.11270166537925831148, 0.5, 0.88729833462074168852};
+static const double GL3_W[3] = {0.27777777777777777778, 0.44444444444444444444,
+                                0.27777777777777777778};
+
+/* Adds vol/(dim+1) * int M(x) h_b(x) dx for every hat b into acc[b - base]. */
+static void project_simplex_basis(int dim, const double* ev, int a, double vol,
+                                  const or_egrid* g, double* acc, int32_t base) {
+    double t[5];
+    for (int q = 0; q <= dim; ++q) t[q] = ev[q];
+    t[dim + 1] = ev[a];
+    sort_small(t, dim + 2);
+    const int nk = dim + 2;
+    double span = t[nk - 1] - t[0];
+    const double de = g->de;
+    if (!(span > 1e-9 * de)) {
+        /* flat simplex: the density is vol/(dim+1) * delta(x - t0) */
+        double x = t[0];
+        int32_t c = egrid_cell(g, x);
+        double ec = g->e0 + (double)c * de;
+        double w1 = (x - ec) / de;
+        double m = vol / (double)(dim + 1);
+        acc[c - base] += m * (1.0 - w1);
+        acc[c + 1 - base] += m * w1;
+        return;
+    }
+    const double scale = vol / span;
+    int32_t c0 = egrid_cell(g, t[0]), c1 = egrid_cell(g, t[nk - 1]);
+    for (int32_t c = c0; c <= c1; ++c) {
+        double ec = g->e0 + (double)c * de;
+        double ec1 = ec + de;
+        double lo = t[0] > ec ? t[0] : ec;
+        double hi = t[nk - 1] < ec1 ? t[nk - 1] : ec1;
+        if (!(hi > lo)) continue;
+        /* split [lo,hi] at the knots */
+        for (int s = 0; s < nk - 1; ++s) {
+            double a0 = t[s] > lo ? t[s] : lo;
+            double a1 = t[s + 1] < hi ? t[s + 1] : hi;
+            if (!(a1 > a0)) continue;
+            double w = a1 - a0;
+            double s0 = 0.0, s1 = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                double x = a0 + GL3_X[q] * w;
+                double v = GL3_W[q] * w * scale * bspline_value(t, nk - 1, s, x);
+                s0 += v * (ec1 - x);
+                s1 += v * (x - ec);
+            }
+            acc[c - base] += s0 / de;
+            acc[c + 1 - base] += s1 / de;
+        }
+    }
+}
+
+/* incident (element, local vertex) pairs of a lattice node; returns count */
+static int node_star(int dim, const int32_t* nsec, int64_t node, int64_t* elems, int* locs) {
+    int nc[3] = {0, 0, 0};
+    int64_t r = node;
+    for (int d = 0; d < dim; ++d) { nc[d] = (int)(r % (nsec[d] + 1)); r /= (nsec[d] + 1); }
+    int nes = fact(dim);
+    int cnt = 0;
+    for (int m = 0; m < (1 << dim); ++m) {
+        int sc[3], ok = 1;
+        int64_t sid = 0, stride = 1;
+        for (int d = 0; d < dim; ++d) {
+            sc[d] = nc[d] - ((m >> d) & 1);
+            if (sc[d] < 0 || sc[d] >= nsec[d]) ok = 0;
+            sid += stride * sc[d];
+            stride *= nsec[d];
+        }
+        if (!ok) continue;
+        for (int t = 0; t < nes; ++t) {
+            int off[12];
+            ref_section_vertices(dim, t, off);
+            for (int a = 0; a <= dim; ++a) {
+                int match = 1;
+                for (int d = 0; d < dim; ++d)
+                    if (sc[d] + off[a * 3 + d] != nc[d]) match = 0;
+                if (match) { elems[cnt] = sid * nes + t; locs[cnt] = a; ++cnt; }
+            }
+        }
+    }
+    return cnt;
+}
+
+int32_t or_projection(int dim, const int32_t* nsec, const double* extent,
+                      const double* eps, const or_egrid* g,
+                      int32_t* off, int32_t* len, double* R, int32_t maxw) {
+    int64_t n = or_num_nodes(dim, nsec), nel = or_num_elements(dim, nsec);
+    double vol = or_element_volume(dim, nsec, extent);
+    int32_t* conn = (int32_t*)malloc(sizeof(int32_t) * nel * (dim + 1));
+    or_mesh_elements(dim, nsec, conn);
+    int32_t maxlen = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : maxlen)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t elems[48];
+        int locs[48];
+        int ns = node_star(dim, nsec, i, elems, locs);
+        int32_t lo = 0x7fffffff, hi = -1;
+        for (int q = 0; q < ns; ++q) {
+            double emin = 1e300, emax = -1e300;
+            for (int a = 0; a <= dim; ++a) {
+                double e = eps[conn[elems[q] * (dim + 1) + a]];
+                if (e < emin) emin = e;
+                if (e > emax) emax = e;
+            }
+            int32_t c0 = egrid_cell(g, emin), c1 = egrid_cell(g, emax) + 1;
+            if (c0 < lo) lo = c0;
+            if (c1 > hi) hi = c1;
+        }
+        off[i] = lo;
+        len[i] = hi - lo + 1;
+        if (len[i] > maxlen) maxlen = len[i];
+        if (R && len[i] <= maxw) {
+            double* row = R + i * (int64_t)maxw;
+            for (int32_t w = 0; w < maxw; ++w) row[w] = 0.0;
+            for (int q = 0; q < ns; ++q) {
+                double ev[4];
+                for (int a = 0; a <= dim; ++a) ev[a] = eps[conn[elems[q] * (dim + 1) + a]];
+                project_simplex_basis(dim, ev, locs[q], vol, g, row, lo);
+            }
+        }
+    }
+    free(conn);
+    return maxlen;
+}
+
+/* ============================================================ kernel K */
+int32_t or_gauss_table(double de, double shift, double sigma, double cutoff,
+                       int32_t* m_lo, double* gm, int32_t cap) {
+    int32_t lo = (int32_t)ceil((-cutoff * sigma - shift) / de);
+    int32_t hi = (int32_t)floor((cutoff * sigma - shift) / de);
+    int32_t cnt = 0;
+    const double norm = 1.0 / (sigma * sqrt(2.0 * M_PI));
+    for (int32_t m = lo; m <= hi; ++m) {
+        double x = (double)m * de + shift;
+        if (cnt < cap) gm[cnt] = norm * exp(-0.5 * (x / sigma) * (x / sigma));
+        ++cnt;
+    }
+    *m_lo = lo;
+    return cnt;
+}
+
+/* Q_j(b) = sum_b' g_{b-b'} R_j(b'), b in [off_j + m_lo, off_j + len_j - 1 + m_hi] */
+static void conv_row(const double* Rj, int32_t lenj, const double* gm, int32_t nm, double* Q) {
+    int32_t lq = lenj + nm - 1;
+    for (int32_t p = 0; p < lq; ++p) {
+        double s = 0.0;
+        /* Q[p] = sum_w R[w] g[p - w] */
+        int32_t w0 = p - (nm - 1) > 0 ? p - (nm - 1) : 0;
+        int32_t w1 = p < lenj - 1 ? p : lenj - 1;
+        for (int32_t w = w0; w <= w1; ++w) s += Rj[w] * gm[p - w];
+        Q[p] = s;
+    }
+}
+
+static double kernel_entry(const double* Ri, int32_t offi, int32_t leni,
+                           const double* Qj, int32_t qoff, int32_t qlen) {
+    int32_t b0 = offi > qoff ? offi : qoff;
+    int32_t b1 = (offi + leni < qoff + qlen) ? offi + leni : qoff + qlen;
+    double s = 0.0;
+    for (int32_t b = b0; b < b1; ++b) s += Ri[b - offi] * Qj[b - qoff];
+    return s;
+}
+
+void or_kernel_row(int64_t n, int64_t i, const int32_t* off, const int32_t* len,
+                   const double* R, int32_t maxw, const double* gm,
+                   int32_t m_lo, int32_t nm, double amp, double* Krow) {
+    double* Q = (double*)malloc(sizeof(double) * (maxw + nm));
+    for (int64_t j = 0; j < n; ++j) {
+        conv_row(R + j * (int64_t)maxw, len[j], gm, nm, Q);
+        Krow[j] = amp * kernel_entry(R + i * (int64_t)maxw, off[i], len[i], Q,
+                                     off[j] + m_lo, len[j] + nm - 1);
+    }
+    free(Q);
+}
+
+void or_kernel_full(int64_t n, const int32_t* off, const int32_t* len,
+                    const double* R, int32_t maxw, const double* gm,
+                    int32_t m_lo, int32_t nm, double amp, double* K) {
+    int32_t qw = maxw + nm - 1;
+    double* Q = (double*)malloc(sizeof(double) * n * (int64_t)qw);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) conv_row(R + j * (int64_t)maxw, len[j], gm, nm, Q + j * (int64_t)qw);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j)
+            K[i * n + j] = amp * kernel_entry(R + i * (int64_t)maxw, off[i], len[i],
+                                              Q + j * (int64_t)qw, off[j] + m_lo, len[j] + nm - 1);
+    free(Q);
+}
+
+/* ========================================================= collision */
+void or_collision_apply(int64_t n, const double* K, const double* V,
+                        double bath_n, const double* f, double* I, int conserve) {
+    double* u = (double*)calloc(n, sizeof(double));
+    double* rk = (double*)calloc(n, sizeof(double));
+    double* v = (double*)calloc(n, sizeof(double));
+    double* ck = (double*)calloc(n, sizeof(double));
+    /* One pass over K (row-major): u = K f and rk = K 1 from each row;
+     * v = K^T f and ck = K^T 1 accumulated per thread over its row block and
+     * summed in thread order afterwards. */
+#pragma omp parallel
+    {
+        double* vp = (double*)calloc(2 * n, sizeof(double));
+        double* cp = vp + n;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const double* Ki = K + i * n;
+            const double fi = f[i];
+            double su = 0.0, sr = 0.0;
+            for (int64_t j = 0; j < n; ++j) {
+                double k = Ki[j];
+                su += k * f[j];
+                sr += k;
+                vp[j] += k * fi;
+                cp[j] += k;
+            }
+            u[i] = su;
+            rk[i] = sr;
+        }
+#pragma omp for ordered schedule(static, 1)
+        for (int t = 0; t < omp_get_num_threads(); ++t) {
+#pragma omp ordered
+            for (int64_t j = 0; j < n; ++j) { v[j] += vp[j]; ck[j] += cp[j]; }
+        }
+        free(vp);
+    }
+    const double np1 = bath_n + 1.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double wf = np1 * u[i] + bath_n * v[i];        /* (W f)_i   */
+        double wt1 = np1 * ck[i] + bath_n * rk[i];     /* (W^T 1)_i */
+        double wtf = np1 * v[i] + bath_n * u[i];       /* (W^T f)_i */
+        I[i] = ((1.0 - f[i]) * wf - f[i] * (wt1 - wtf)) / V[i];
+    }
+    if (conserve) {
+        double d = 0.0, wsum = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            d += V[i] * I[i];
+            wsum += V[i] * f[i] * (1.0 - f[i]);
+        }
+        if (wsum > 0.0)
+            for (int64_t i = 0; i < n; ++i) I[i] -= d * f[i] * (1.0 - f[i]) / wsum;
+    }
+    free(u); free(v); free(rk); free(ck);
+}
+
+void or_conservation(int64_t n, const double* V, const double* eps,
+                     const double* f, double* number, double* energy) {
+    double s = 0.0, e = 0.0;
+    for (int64_t i = 0; i < n; ++i) { s += V[i] * f[i]; e += V[i] * eps[i] * f[i]; }
+    *number = s;
+    *energy = e;
+}
+
+/* ======================================================== error norm */
+/* Band.cpp:87-101.  Function arithmetic is nodal (Function.cpp:75-109), so
+ * every pointwise operation is done on node values; integrate(rhs) of two
+ * P1 fields over an element is exact: vol/((d+1)(d+2)) (sum r_a^2 + (sum r_a)^2). */
+double or_relative_error(int dim, int64_t nel, const int32_t* conn,
+                         const double* err, const double* pop, double stat,
+                         double rtol, double atol, double penalty) {
+    double s_r = 0.0, s_p = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        double sr = 0.0, sr2 = 0.0, sp = 0.0, sp2 = 0.0;
+        for (int a = 0; a <= dim; ++a) {
+            int32_t i = conn[e * (dim + 1) + a];
+            double y = pop[i];
+            double scale, pen;
+            if (stat == -1.0) {
+                double m = fabs(y) < fabs(1.0 - y) ? fabs(y) : fabs(1.0 - y);
+                scale = rtol * m + atol;
+                pen = ((y < 0.0 ? -y : 0.0) + (y - 1.0 > 0.0 ? y - 1.0 : 0.0)) / scale;
+            } else {
+                scale = rtol * y + atol;
+                pen = (y < 0.0 ? -y : 0.0) / scale;
+            }
+            double r = err[i] / scale;
+            sr += r; sr2 += r * r; sp += pen; sp2 += pen * pen;
+        }
+        s_r += sr2 + sr * sr;
+        s_p += sp2 + sp * sp;
+    }
+    /* vol cancels against Nel*vol in the normalisation */
+    double denom = (double)((dim + 1) * (dim + 2)) * (double)nel;
+    return sqrt((s_r + penalty * s_p) / denom);
+}
+
+/* ======================================================== DP5(4) driver */
+/* Butcher tableau, controller and dense output of DormandPrince54.hpp:503-511,
+ * :532-543, :588-662, :598-611.  enforceValue and the step vetoer are the
+ * identity here (no constrained bands in the W-form). */
+#define DP_C2 (1. / 5.)
+#define DP_C3 (3. / 10.)
+#define DP_C4 (4. / 5.)
+#define DP_C5 (8. / 9.)
+static const double A21 = 1. / 5.;
+static const double A31 = 3. / 40., A32 = 9. / 40.;
+static const double A41 = 44. / 45., A42 = -56. / 15., A43 = 32. / 9.;
+static const double A51 = 19372. / 6561., A52 = -25360. / 2187., A53 = 64448. / 6561., A54 = -212. / 729.;
+static const double A61 = 9017. / 3168., A62 = -355. / 33., A63 = 46732. / 5247., A64 = 49. / 176., A65 = -5103. / 18656.;
+static const double B1 = 35. / 384., B3 = 500. / 1113., B4 = 125. / 192., B5 = -2187. / 6784., B6 = 11. / 84.;
+static const double BH1 = 5179. / 57600., BH3 = 7571. / 16695., BH4 = 393. / 640., BH5 = -92097. / 339200., BH6 = 187. / 2100., BH7 = 1. / 40.;
+
+typedef struct dp5_work {
+    int64_t n;
+    double* base;
+    double *k1, *k2, *k3, *k4, *k5, *k6, *k7, *ys, *y5, *err;
+} dp5_work;
+
+static void dp5_alloc(dp5_work* w, int64_t n) {
+    w->n = n;
+    double* b = (double*)malloc(sizeof(double) * n * 10);
+    w->base = b;
+    w->k1 = b; w->k2 = b + n; w->k3 = b + 2 * n; w->k4 = b + 3 * n; w->k5 = b + 4 * n;
+    w->k6 = b + 5 * n; w->k7 = b + 6 * n; w->ys = b + 7 * n; w->y5 = b + 8 * n; w->err = b + 9 * n;
+}
+static void dp5_free(dp5_work* w) { free(w->base); }
+
+/* One attempted step: k2..k7, y5 and err (= y5 - y4).  Returns evals (6).
+ * The stage sums are written in the reference's order (h*a_ij)*k_j. */
+static void dp5_attempt(dp5_work* w, const double* y, double t, double h, or_rhs_fn rhs, void* user) {
+    int64_t n = w->n;
+    for (int64_t i = 0; i < n; ++i) w->ys[i] = y[i] + (A21 * h) * w->k1[i];
+    rhs(user, t + DP_C2 * h, w->ys, w->k2);
+    for (int64_t i = 0; i < n; ++i) w->ys[i] = y[i] + (A31 * h) * w->k1[i] + (A32 * h) * w->k2[i];
+    rhs(user, t + DP_C3 * h, w->ys, w->k3);
+    for (int64_t i = 0; i < n; ++i)
+        w->ys[i] = y[i] + (A41 * h) * w->k1[i] + (A42 * h) * w->k2[i] + (A43 * h) * w->k3[i];
+    rhs(user, t + DP_C4 * h, w->ys, w->k4);
+    for (int64_t i = 0; i < n; ++i)
+        w->ys[i] = y[i] + (A51 * h) * w->k1[i] + (A52 * h) * w->k2[i] + (A53 * h) * w->k3[i] + (A54 * h) * w->k4[i];
+    rhs(user, t + DP_C5 * h, w->ys, w->k5);
+    for (int64_t i = 0; i < n; ++i)
+        w->ys[i] = y[i] + (A61 * h) * w->k1[i] + (A62 * h) * w->k2[i] + (A63 * h) * w->k3[i] + (A64 * h) * w->k4[i] + (A65 * h) * w->k5[i];
+    rhs(user, t + h, w->ys, w->k6);
+    for (int64_t i = 0; i < n; ++i)
+        w->y5[i] = y[i] + (h * B1) * w->k1[i] + (h * B3) * w->k3[i] + (h * B4) * w->k4[i] + (h * B5) * w->k5[i] + (h * B6) * w->k6[i];
+    rhs(user, t + h, w->y5, w->k7);
+    for (int64_t i = 0; i < n; ++i) {
+        double y4 = y[i] + (h * BH1) * w->k1[i] + (h * BH3) * w->k3[i] + (h * BH4) * w->k4[i] + (h * BH5) * w->k5[i] + (h * BH6) * w->k6[i] + (h * BH7) * w->k7[i];
+        w->err[i] = w->y5[i] - y4;
+    }
+}
+
+/* dense output at s in (0,1]: DormandPrince54.hpp:601-608 */
+static void dp5_dense(const dp5_work* w, const double* y, double h, double s, double* out) {
+    double bt1 = -(435. * s * s * s - 1184. * s * s + 1098. * s - 384.) / (384.);
+    double bt3 = 500. * s * (6. * s * s - 14. * s + 9.) / (1113.);
+    double bt4 = -125. * s * (9. * s * s - 16. * s + 6.) / (192.);
+    double bt5 = 729. * s * (35. * s * s - 64. * s + 26.) / (6784.);
+    double bt6 = -11. * s * (3. * s - 2.) * (5. * s - 6.) / (84.);
+    double bt7 = s * (s - 1.) * (5. * s - 3.) / (2.);
+    for (int64_t i = 0; i < w->n; ++i)
+        out[i] = y[i] + (s * h * bt1) * w->k1[i] + (s * h * bt3) * w->k3[i] + (s * h * bt4) * w->k4[i] +
+                 (s * h * bt5) * w->k5[i] + (s * h * bt6) * w->k6[i] + (s * h * bt7) * w->k7[i];
+}
+
+void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
+                      double dense_dt, or_rhs_fn rhs, or_norm_fn norm, void* user,
+                      double* dense_out, double* dense_t, int32_t max_dense,
+                      double* h_log, int8_t* acc_log, int64_t log_cap,
+                      or_dp5_stats* stats) {
+    dp5_work w;
+    dp5_alloc(&w, n);
+    or_dp5_stats st;
+    memset(&st, 0, sizeof(st));
+    double t = t0, h = dense_dt, dense_time = t0 + dense_dt;
+    const double min_h = dense_dt / 10000.;
+    const double minscale = 0.333333, maxscale = 3.0, safety = 0.98;
+    const double alpha = 0.7 / 5., beta = 0.4 / 5.;
+    double olderror = 1.0;
+    int notfailed = 1, rejected = 0;
+    rhs(user, t, y, w.k1);
+    st.evals = 1;
+    while (t < t_final && notfailed) {
+        dp5_attempt(&w, y, t, h, rhs, user);
+        st.evals += 6;
+        if (h_log && st.attempted < log_cap) h_log[st.attempted] = h;
+        double en = norm(user, w.err, w.y5);
+        int accept = en <= 1.0;
+        if (acc_log && st.attempted < log_cap) acc_log[st.attempted] = (int8_t)accept;
+        ++st.attempted;
+        if (accept) {
+            while (dense_time <= t + 1.0001 * h && dense_time <= t_final + .0001 * h) {
+                double s = (dense_time - t) / h;
+                if (st.n_dense < max_dense && dense_out) {
+                    dp5_dense(&w, y, h, s, dense_out + (int64_t)st.n_dense * n);
+                    if (dense_t) dense_t[st.n_dense] = dense_time;
+                }
+                ++st.n_dense;
+                dense_time += dense_dt;
+            }
+            memcpy(y, w.y5, sizeof(double) * n);
+            t += h;
+            double* tmp = w.k1; w.k1 = w.k7; w.k7 = tmp;
+            double scale;
+            if (en == 0.) {
+                scale = maxscale;
+            } else {
+                scale = safety * pow(1. / en, alpha) * pow(olderror, beta);
+                scale = fmin(fmax(scale, minscale), maxscale);
+            }
+            if (rejected) h *= fmin(1.0, scale);
+            else h *= scale;
+            rejected = 0;
+            olderror = en;
+            ++st.accepted;
+        } else {
+            double scale = safety * pow(1. / en, 1 / 5.);
+            scale = fmin(fmax(scale, minscale), maxscale);
+            h *= scale;
+            rejected = 1;
+            ++st.rejected;
+            if (h < min_h) notfailed = 0;
+        }
+    }
+    st.failed = !notfailed;
+    if (stats) *stats = st;
+    dp5_free(&w);
+}
+
+/* ----------------------------------------------- W-form right-hand side */
+typedef struct coll_user {
+    int dim;
+    int64_t n, nel;
+    const double* K;
+    const double* V;
+    double bath_n;
+    int conserve;
+    int32_t* conn;
+    double rtol, atol;
+} coll_user;
+
+static void coll_rhs(void* u, double t, const double* y, double* dydt) {
+    (void)t;
+    coll_user* c = (coll_user*)u;
+    or_collision_apply(c->n, c->K, c->V, c->bath_n, y, dydt, c->conserve);
+}
+
+static double coll_norm(void* u, const double* err, const double* y5) {
+    coll_user* c = (coll_user*)u;
+    return or_relative_error(c->dim, c->nel, c->conn, err, y5, -1.0, c->rtol, c->atol, 0.0);
+}
+
+void or_dp5_collision(int dim, const int32_t* nsec, int64_t n, const double* K,
+                      const double* V, double bath_n, double* f, double t0,
+                      double t_final, double dense_dt, double rtol, double atol,
+                      int conserve, double* dense_out, double* dense_t,
+                      int32_t max_dense, double* h_log, int8_t* acc_log,
+                      int64_t log_cap, or_dp5_stats* stats) {
+    coll_user c;
+    c.dim = dim; c.n = n; c.nel = or_num_elements(dim, nsec);
+    c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    c.rtol = rtol; c.atol = atol;
+    c.conn = (int32_t*)malloc(sizeof(int32_t) * c.nel * (dim + 1));
+    or_mesh_elements(dim, nsec, c.conn);
+    or_dp5_integrate(n, f, t0, t_final, dense_dt, coll_rhs, coll_norm, &c, dense_out,
+                     dense_t, max_dense, h_log, acc_log, log_cap, stats);
+    free(c.conn);
+}
+
+void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
+                             const double* K, const double* V, double bath_n,
+                             double* f, double t0, double t_final,
+                             double dense_dt, int conserve, const double* h_log,
+                             const int8_t* acc_log, int64_t n_attempts,
+                             double* dense_out, double* dense_t,
+                             int32_t max_dense) {
+    (void)dim; (void)nsec;
+    coll_user c;
+    memset(&c, 0, sizeof(c));
+    c.n = n; c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    dp5_work w;
+    dp5_alloc(&w, n);
+    double t = t0, dense_time = t0 + dense_dt;
+    int32_t nd = 0;
+    coll_rhs(&c, t, f, w.k1);
+    for (int64_t a = 0; a < n_attempts && t < t_final; ++a) {
+        if (!acc_log[a]) continue;
+        double h = h_log[a];
+        dp5_attempt(&w, f, t, h, coll_rhs, &c);
+        while (dense_time <= t + 1.0001 * h && dense_time <= t_final + .0001 * h) {
+            double s = (dense_time - t) / h;
+            if (nd < max_dense && dense_out) {
+                dp5_dense(&w, f, h, s, dense_out + (int64_t)nd * n);
+                if (dense_t) dense_t[nd] = dense_time;
+            }
+            ++nd;
+            dense_time += dense_dt;
+        }
+        memcpy(f, w.y5, sizeof(double) * n);
+        t += h;
+        double* tmp = w.k1; w.k1 = w.k7; w.k7 = tmp;
+    }
+    dp5_free(&w);
+}

@@ -1,0 +1,156 @@
+/*
+ * tortoise_oracle.h -- CPU FP64 restatement of the scattering-integral hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code: it is the checker that tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py compare the HIP path against.
+ *
+ * What it restates (reference paths relative to the TORTOISE tree):
+ *   - structured simplex mesh, section/element numbering and vertex order:
+ *       src/Geometry/Structured/DirectSpace/Mesh.cpp:55-87 (vertElemInRefSec),
+ *       :222-267 (verticesCoord), :384-433 (section/element ids)
+ *   - element volume and lumped P1 masses: Mesh.cpp:268-293 (elemVolume),
+ *       Function.cpp:369-372 (integrate() of a P1 field)
+ *   - error norm of the adaptive stepper: src/Physics/Material/Band.cpp:87-101
+ *       with Function::integrate(rhs) (Function.cpp:373-375) done exactly
+ *       (order-2 Gauss is exact for P1*P1)
+ *   - max over bands: src/Physics/Status/MaterialStatus.cpp:189-196
+ *   - Dormand-Prince 5(4) driver with PI control, reject rule, dense output:
+ *       include/Generics/Algorithms/DormandPrince54.hpp:479-665
+ *   - channel amplitude convention (amplitude = g^2, full momentum measure,
+ *       no 2*pi): src/Physics/Scattering/ScatteringChannel.cpp:399-415
+ *
+ * What is new (no reference function computes it; pinned by closed forms and
+ * invariants in tests/test_oracle_*.py):
+ *   - nodal P1 W-form of the e-ph collision operator with a Gaussian-broadened
+ *     energy delta and a phonon bath of occupation n:
+ *       I_i = [ (1-f_i) (W f)_i - f_i (W^T (1-f))_i ] / V_i,
+ *       W   = (n+1) K + n K^T,
+ *       K_ij = g^2 sum_{b,b'} R_ib R_jb' G_sigma((b-b') dE + omega0)
+ *     where R_ib = int phi_i(k) h_b(eps(k)) dk is the exact projection of the
+ *     P1 basis function phi_i (eps linear per element) on the hat functions
+ *     h_b of a uniform energy grid, computed with the simplex-spline identity
+ *     (basis-weighted energy density of a linear simplex = a B-spline with the
+ *     vertex energies plus the basis vertex's energy as knots).
+ */
+#ifndef TORTOISE_ORACLE_H
+#define TORTOISE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- mesh */
+int64_t or_num_nodes(int dim, const int32_t* nsec);
+int64_t or_num_elements(int dim, const int32_t* nsec);
+double  or_element_volume(int dim, const int32_t* nsec, const double* extent);
+/* node coordinates, node-major: coords[i*dim + d] */
+void    or_mesh_nodes(int dim, const int32_t* nsec, const double* origin,
+                      const double* extent, double* coords);
+/* element -> node connectivity in the reference's vertex order, conn[e*(dim+1)+a] */
+void    or_mesh_elements(int dim, const int32_t* nsec, int32_t* conn);
+/* lumped P1 mass: V_i = sum_{e ni i} vol_e/(dim+1) */
+void    or_node_volumes(int dim, const int32_t* nsec, const double* extent, double* V);
+
+/* ------------------------------------------------------ energy projection */
+typedef struct or_egrid {
+    double  e0;      /* energy of grid point 0 */
+    double  de;      /* grid spacing */
+    int32_t nbins;   /* number of hat functions */
+} or_egrid;
+
+void or_egrid_make(const double* eps, int64_t n, double de, or_egrid* g);
+
+/* Projection windows: node i owns hats [off[i], off[i]+len[i]).  Returns the
+ * largest len (call with R == NULL to size the buffer).  R is n x maxw, row i
+ * holds R_i,off[i]+w at column w. */
+int32_t or_projection(int dim, const int32_t* nsec, const double* extent,
+                      const double* eps, const or_egrid* g,
+                      int32_t* off, int32_t* len, double* R, int32_t maxw);
+
+/* Basis-weighted energy density of one simplex, B-spline form:
+ * D_a(x) = vol * N(x | sorted{e_0..e_dim, e_a}) / (t_last - t_first). */
+double or_simplex_density(int dim, const double* evert, int a, double vol, double x);
+
+/* Sampled broadened delta: g_m = G_sigma(m*de + shift) for m in [m_lo, m_hi]
+ * with |m*de+shift| <= cutoff*sigma.  Returns count, fills m_lo. */
+int32_t or_gauss_table(double de, double shift, double sigma, double cutoff,
+                       int32_t* m_lo, double* gm, int32_t cap);
+
+/* Emission kernel K (row-major n x n) or a single row of it. */
+void or_kernel_full(int64_t n, const int32_t* off, const int32_t* len,
+                    const double* R, int32_t maxw, const double* gm,
+                    int32_t m_lo, int32_t nm, double amp, double* K);
+void or_kernel_row(int64_t n, int64_t i, const int32_t* off, const int32_t* len,
+                   const double* R, int32_t maxw, const double* gm,
+                   int32_t m_lo, int32_t nm, double amp, double* Krow);
+
+/* ---------------------------------------------------------- collision */
+/* I = [(1-f) (W f) - f (W^T 1 - W^T f)] / V,  W = (n+1)K + n K^T.
+ * K row-major n x n.  If conserve != 0 the number defect sum_i V_i I_i is
+ * removed with weights f_i(1-f_i) (the same rule the mixed GPU path uses). */
+void or_collision_apply(int64_t n, const double* K, const double* V,
+                        double bath_n, const double* f, double* I, int conserve);
+
+/* sum_i V_i f_i and sum_i V_i eps_i f_i */
+void or_conservation(int64_t n, const double* V, const double* eps,
+                     const double* f, double* number, double* energy);
+
+/* ------------------------------------------------------- error norm */
+/* Band::relativeError for a fermion (stat=-1) or boson (stat=+1) band on
+ * nodal data: sqrt((int r^2 + p int pen^2)/(Nel*vol)). */
+double or_relative_error(int dim, int64_t nel, const int32_t* conn,
+                         const double* err, const double* pop, double stat,
+                         double rtol, double atol, double penalty);
+
+/* ------------------------------------------------------- DP5(4) driver */
+typedef void   (*or_rhs_fn)(void* user, double t, const double* y, double* dydt);
+typedef double (*or_norm_fn)(void* user, const double* err, const double* y5);
+
+typedef struct or_dp5_stats {
+    int64_t attempted;
+    int64_t accepted;
+    int64_t rejected;
+    int64_t evals;
+    int32_t failed;       /* 1 if the step size underflowed */
+    int32_t n_dense;      /* dense outputs written */
+} or_dp5_stats;
+
+/* Integrates y' = rhs(t,y) from t0 to t_final.  Dense outputs at t0+k*dense_dt
+ * (k>=1) are written to dense_out[(k-1)*n ...] (capacity max_dense) with
+ * their times in dense_t.  y is overwritten with the last accepted y5.
+ * If h_log != NULL the attempted step sizes are written to h_log[0..cap) and
+ * acc_log[k] = 1 for accepted attempts. */
+void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
+                      double dense_dt, or_rhs_fn rhs, or_norm_fn norm, void* user,
+                      double* dense_out, double* dense_t, int32_t max_dense,
+                      double* h_log, int8_t* acc_log, int64_t log_cap,
+                      or_dp5_stats* stats);
+
+/* DP5 on the nodal W-form: rhs = or_collision_apply, norm = or_relative_error
+ * (fermion).  Convenience wrapper used by the parity tests. */
+void or_dp5_collision(int dim, const int32_t* nsec, int64_t n, const double* K,
+                      const double* V, double bath_n, double* f, double t0,
+                      double t_final, double dense_dt, double rtol, double atol,
+                      int conserve, double* dense_out, double* dense_t,
+                      int32_t max_dense, double* h_log, int8_t* acc_log,
+                      int64_t log_cap, or_dp5_stats* stats);
+
+/* Fixed-step replay of a recorded attempt sequence (h_log/acc_log): the same
+ * stage arithmetic as or_dp5_integrate with the controller taken out, so a
+ * second precision can be compared on identical steps.  Writes the state
+ * after the last accepted step to y and dense outputs like the driver. */
+void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
+                             const double* K, const double* V, double bath_n,
+                             double* f, double t0, double t_final,
+                             double dense_dt, int conserve, const double* h_log,
+                             const int8_t* acc_log, int64_t n_attempts,
+                             double* dense_out, double* dense_t,
+                             int32_t max_dense);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,271 @@
+"""Python front-end of libtortoise_gpu.so (the C ABI in include/tortoise_gpu.h).
+
+The library is hand-written HIP for gfx950.  This module only binds it with
+ctypes; there is no fallback: importing works without the library, but every
+call raises if libtortoise_gpu.so is missing or no GPU is present.
+
+    ctx = Context.create(workload)        # tg_ctx_create
+    ctx.assemble()                        # tg_kernel_assemble
+    I = ctx.apply(f)                      # tg_collision_apply
+    f_end, dense, stats = ctx.dp5(f0, 0.0, t_final, dense_dt, rtol, atol)
+
+Device pointers (torch tensors) are accepted by the *_device methods, which
+pass TG_PTR_DEVICE.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import workloads  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtortoise_gpu.so")
+
+TG_OK = 0
+TG_ERR_ARG = -1
+TG_ERR_HIP = -2
+TG_ERR_NONFINITE = -3
+TG_ERR_STEP_UNDERFLOW = -4
+TG_ERR_STATE = -5
+TG_ERR_NOMEM = -6
+
+TG_PTR_DEVICE = 0x1
+TG_PREC_MIXED = 0x2
+TG_CONSERVE = 0x4
+TG_QUIET = 0x8
+
+# every symbol include/tortoise_gpu.h declares
+EXPORTS = [
+    "tg_num_nodes", "tg_ctx_create", "tg_ctx_destroy", "tg_last_error", "tg_ctx_set_stream",
+    "tg_ctx_stream", "tg_mesh_info", "tg_node_volumes", "tg_kernel_assemble",
+    "tg_kernel_copy_rows", "tg_collision_apply", "tg_dp5_integrate", "tg_conservation",
+    "tg_relative_error",
+]
+
+
+class TgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tortoise_gpu status {code}: {msg}")
+        self.code = code
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("nsec", C.c_int32 * 3), ("origin", C.c_double * 3),
+        ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("g", C.c_double),
+        ("omega0", C.c_double), ("sigma", C.c_double), ("bath_n", C.c_double),
+        ("de", C.c_double), ("cutoff", C.c_double), ("max_batch", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("attempted", C.c_int64), ("accepted", C.c_int64), ("rejected", C.c_int64),
+                ("evals", C.c_int64), ("failed", C.c_int32), ("n_dense", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load():
+    """Loads libtortoise_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.tg_num_nodes.restype = i64
+    L.tg_num_nodes.argtypes = [i32, C.POINTER(i32)]
+    L.tg_ctx_create.restype = i32
+    L.tg_ctx_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(vp)]
+    L.tg_ctx_destroy.restype = None
+    L.tg_ctx_destroy.argtypes = [vp]
+    L.tg_last_error.restype = C.c_char_p
+    L.tg_last_error.argtypes = [vp]
+    L.tg_ctx_set_stream.restype = i32
+    L.tg_ctx_set_stream.argtypes = [vp, vp]
+    L.tg_ctx_stream.restype = vp
+    L.tg_ctx_stream.argtypes = [vp]
+    L.tg_mesh_info.restype = i32
+    L.tg_mesh_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(f64)]
+    L.tg_node_volumes.restype = i32
+    L.tg_node_volumes.argtypes = [vp, vp]
+    L.tg_kernel_assemble.restype = i32
+    L.tg_kernel_assemble.argtypes = [vp, i32]
+    L.tg_kernel_copy_rows.restype = i32
+    L.tg_kernel_copy_rows.argtypes = [vp, i64, i64, vp]
+    L.tg_collision_apply.restype = i32
+    L.tg_collision_apply.argtypes = [vp, vp, vp, i64, i32, vp, i32]
+    L.tg_dp5_integrate.restype = i32
+    L.tg_dp5_integrate.argtypes = [vp, vp, i64, i32, f64, f64, f64, f64, f64, vp, i32, vp, i32,
+                                   vp]
+    L.tg_conservation.restype = i32
+    L.tg_conservation.argtypes = [vp, vp, i64, i32, vp, vp, i32]
+    L.tg_relative_error.restype = i32
+    L.tg_relative_error.argtypes = [vp, vp, vp, i64, i32, f64, f64, vp, i32]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One tg_ctx: mesh + e-ph channel + assembled kernel on the current device."""
+
+    def __init__(self, handle, n, workload=None):
+        self._h = handle
+        self.n = n
+        self.workload = workload
+
+    @classmethod
+    def create(cls, workload, max_batch=None, eps=None, g=None, omega0=None, sigma=None,
+               bath_n=None, de=0.0, cutoff=0.0):
+        L = load()
+        w = workload
+        e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
+        d = ProblemDesc()
+        d.dim = w.dim
+        for q in range(3):
+            d.nsec[q] = w.nsec[q] if q < w.dim else 1
+            d.origin[q] = w.origin[q] if q < w.dim else 0.0
+            d.extent[q] = w.extent[q] if q < w.dim else 1.0
+        d.eps = e.ctypes.data_as(C.POINTER(C.c_double))
+        d.g = w.g if g is None else g
+        d.omega0 = w.omega0 if omega0 is None else omega0
+        d.sigma = w.sigma if sigma is None else sigma
+        d.bath_n = w.bath_n if bath_n is None else bath_n
+        d.de = de
+        d.cutoff = cutoff
+        d.max_batch = max_batch or w.batch
+        h = C.c_void_p()
+        rc = L.tg_ctx_create(C.byref(d), C.byref(h))
+        if rc != TG_OK:
+            msg = L.tg_last_error(h).decode() if h.value else "tg_ctx_create failed"
+            if h.value:
+                L.tg_ctx_destroy(h)
+            raise TgError(rc, msg)
+        ctx = cls(h, e.size, w)
+        ctx.eps = e
+        return ctx
+
+    def _check(self, rc):
+        if rc != TG_OK:
+            raise TgError(rc, load().tg_last_error(self._h).decode())
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            load().tg_ctx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- setup
+    def set_stream(self, stream_ptr):
+        self._check(load().tg_ctx_set_stream(self._h, C.c_void_p(stream_ptr)))
+
+    def stream(self):
+        return load().tg_ctx_stream(self._h)
+
+    def mesh_info(self):
+        n, ne, v = C.c_int64(), C.c_int64(), C.c_double()
+        self._check(load().tg_mesh_info(self._h, C.byref(n), C.byref(ne), C.byref(v)))
+        return n.value, ne.value, v.value
+
+    def node_volumes(self):
+        out = np.zeros(self.n)
+        self._check(load().tg_node_volumes(self._h, _p(out)))
+        return out
+
+    def assemble(self, mixed=False):
+        self._check(load().tg_kernel_assemble(self._h, TG_PREC_MIXED if mixed else 0))
+
+    def kernel_rows(self, row0, nrows):
+        out = np.zeros((nrows, self.n))
+        self._check(load().tg_kernel_copy_rows(self._h, row0, nrows, _p(out)))
+        return out
+
+    # ---- hot path, host arrays
+    def apply(self, f, bath_n=None, mixed=False, conserve=False):
+        f = np.ascontiguousarray(f, np.float64)
+        batch = 1 if f.ndim == 1 else f.shape[1]
+        fc = np.ascontiguousarray(f.T if f.ndim == 2 else f)  # member-major
+        out = np.zeros_like(fc)
+        bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
+                                                              np.float64)
+        flags = (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+        self._check(load().tg_collision_apply(self._h, _p(fc), _p(out), self.n, batch,
+                                              None if bn is None else _p(bn), flags))
+        return out if f.ndim == 1 else out.T
+
+    def conservation(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        batch = 1 if f.ndim == 1 else f.shape[1]
+        fc = np.ascontiguousarray(f.T if f.ndim == 2 else f)
+        num = np.zeros(batch)
+        en = np.zeros(batch)
+        self._check(load().tg_conservation(self._h, _p(fc), self.n, batch, _p(num), _p(en), 0))
+        return (num[0], en[0]) if f.ndim == 1 else (num, en)
+
+    def relative_error(self, err, pop, rtol, atol):
+        err = np.ascontiguousarray(err, np.float64)
+        pop = np.ascontiguousarray(pop, np.float64)
+        batch = 1 if err.ndim == 1 else err.shape[1]
+        e = np.ascontiguousarray(err.T if err.ndim == 2 else err)
+        p = np.ascontiguousarray(pop.T if pop.ndim == 2 else pop)
+        out = np.zeros(batch)
+        self._check(load().tg_relative_error(self._h, _p(e), _p(p), self.n, batch, rtol, atol,
+                                             _p(out), 0))
+        return out[0] if err.ndim == 1 else out
+
+    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, mixed=False, conserve=False,
+            max_dense=0, quiet=True, raise_on_fail=True):
+        """Returns (f_last, dense[(batch,) max_dense, n], stats list)."""
+        f0 = np.ascontiguousarray(f0, np.float64)
+        batch = 1 if f0.ndim == 1 else f0.shape[1]
+        f = np.ascontiguousarray(f0.T if f0.ndim == 2 else f0).copy()
+        dense = np.zeros((batch, max_dense, self.n)) if max_dense else None
+        st = (StepStats * batch)()
+        bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
+                                                              np.float64)
+        flags = ((TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+                 | (TG_QUIET if quiet else 0))
+        rc = load().tg_dp5_integrate(self._h, _p(f), self.n, batch, t0, t1, dense_dt, rtol, atol,
+                                     None if bn is None else _p(bn), flags,
+                                     None if dense is None else _p(dense), max_dense, st)
+        if rc != TG_OK and raise_on_fail:
+            self._check(rc)
+        stats = [s.as_dict() for s in st]
+        if f0.ndim == 1:
+            return f, (None if dense is None else dense[0]), stats[0]
+        return f.T, dense, stats
+
+    # ---- hot path, device pointers (e.g. torch tensors' data_ptr())
+    def apply_device(self, f_ptr, I_ptr, batch=1, bath_ptr=None, mixed=False, conserve=False,
+                     bath_host=None):
+        flags = TG_PTR_DEVICE | (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+        bn = None
+        if bath_host is not None:
+            bn = np.ascontiguousarray(np.broadcast_to(bath_host, (batch,)), np.float64)
+        self._check(load().tg_collision_apply(self._h, C.c_void_p(f_ptr), C.c_void_p(I_ptr),
+                                              self.n, batch, None if bn is None else _p(bn),
+                                              flags))

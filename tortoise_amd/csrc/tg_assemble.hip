@@ -1,0 +1,300 @@
+// tg_assemble.hip -- one-time FP64 assembly of the emission kernel K.
+//
+//   K_ij = g^2 sum_{b,b'} R_ib R_jb' G_sigma((b-b') dE + omega0)
+//
+// R_ib = int phi_i(k) h_b(eps(k)) dk projects node i's P1 basis function on
+// the hat functions h_b of a uniform energy grid (eps linear per simplex,
+// as the reference's LinearForm dispersion, Band.cpp:43-56).  Per simplex
+// the basis-weighted energy density is vol * N(x)/span, N the order-(dim+1)
+// B-spline on the sorted knots {e_0..e_dim, e_a} (simplex-spline identity),
+// integrated exactly against the hats with 3-point Gauss-Legendre on every
+// piece between knots and grid points.  The amplitude convention (g^2, full
+// momentum measure, no 2*pi) follows ScatteringChannel.cpp:399-415.
+//
+// Kernels (all setup, run once per context):
+//   proj_window_kernel  node -> [off, len) of touched hats        (gather)
+//   proj_values_kernel  node -> R row                              (gather)
+//   conv_kernel         node -> Q_j = R_j * g (sampled Gaussian)   (window conv)
+//   kernel_entries      (i, j) -> K_ij = g^2 <R_i, Q_j>            (dense FP64)
+//   K 1 and K^T 1 come from the plain one-pass kernel with f = 1
+#include <cmath>
+#include <vector>
+
+#include "tg_internal.hpp"
+#include "tg_mesh.hpp"
+
+namespace tg {
+
+struct EGrid {
+    double e0, de;
+    int nbins;
+};
+
+__host__ __device__ inline int egrid_cell(const EGrid& g, double x) {
+    int c = (int)floor((x - g.e0) / g.de);
+    if (c < 0) c = 0;
+    if (c > g.nbins - 2) c = g.nbins - 2;
+    return c;
+}
+
+__device__ inline void sort_small(double* v, int n) {
+    for (int i = 1; i < n; ++i) {
+        double x = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j] > x) {
+            v[j + 1] = v[j];
+            --j;
+        }
+        v[j + 1] = x;
+    }
+}
+
+// Cox-de Boor: order-k B-spline N_{0,k} on knots t[0..k], x inside span s.
+__device__ inline double bspline_value(const double* t, int k, int s, double x) {
+    double N[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    N[s] = 1.0;
+    for (int q = 2; q <= k; ++q) {
+        for (int i = 0; i <= k - q; ++i) {
+            double v = 0.0;
+            double d1 = t[i + q - 1] - t[i];
+            double d2 = t[i + q] - t[i + 1];
+            if (d1 > 0.0) v += (x - t[i]) / d1 * N[i];
+            if (d2 > 0.0) v += (t[i + q] - x) / d2 * N[i + 1];
+            N[i] = v;
+        }
+    }
+    return N[0];
+}
+
+__device__ __constant__ double kGL3X[3] = {0.11270166537925831148, 0.5, 0.88729833462074168852};
+__device__ __constant__ double kGL3W[3] = {0.27777777777777777778, 0.44444444444444444444,
+                                           0.27777777777777777778};
+
+// Adds the projection of simplex (energies ev) basis a onto the hats into
+// acc[b - base].
+__device__ void project_simplex_basis(int dim, const double* ev, int a, double vol,
+                                      const EGrid& g, double* acc, int base) {
+    double t[5];
+    for (int q = 0; q <= dim; ++q) t[q] = ev[q];
+    t[dim + 1] = ev[a];
+    sort_small(t, dim + 2);
+    const int nk = dim + 2;
+    const double span = t[nk - 1] - t[0];
+    const double de = g.de;
+    if (!(span > 1e-9 * de)) {
+        double x = t[0];
+        int c = egrid_cell(g, x);
+        double ec = g.e0 + (double)c * de;
+        double w1 = (x - ec) / de;
+        double m = vol / (double)(dim + 1);
+        acc[c - base] += m * (1.0 - w1);
+        acc[c + 1 - base] += m * w1;
+        return;
+    }
+    const double scale = vol / span;
+    const int c0 = egrid_cell(g, t[0]), c1 = egrid_cell(g, t[nk - 1]);
+    for (int c = c0; c <= c1; ++c) {
+        double ec = g.e0 + (double)c * de;
+        double ec1 = ec + de;
+        double lo = t[0] > ec ? t[0] : ec;
+        double hi = t[nk - 1] < ec1 ? t[nk - 1] : ec1;
+        if (!(hi > lo)) continue;
+        for (int s = 0; s < nk - 1; ++s) {
+            double a0 = t[s] > lo ? t[s] : lo;
+            double a1 = t[s + 1] < hi ? t[s + 1] : hi;
+            if (!(a1 > a0)) continue;
+            double w = a1 - a0;
+            double s0 = 0.0, s1 = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                double x = a0 + kGL3X[q] * w;
+                double v = kGL3W[q] * w * scale * bspline_value(t, nk - 1, s, x);
+                s0 += v * (ec1 - x);
+                s1 += v * (x - ec);
+            }
+            acc[c - base] += s0 / de;
+            acc[c + 1 - base] += s1 / de;
+        }
+    }
+}
+
+__global__ void proj_window_kernel(MeshDims md, const double* __restrict__ eps,
+                                   const int32_t* __restrict__ conn, EGrid g, int64_t n,
+                                   int32_t* __restrict__ off, int32_t* __restrict__ len) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t elems[24];
+    int locs[24];
+    int ns = node_star(md, i, elems, locs);
+    int lo = 0x7fffffff, hi = -1;
+    for (int q = 0; q < ns; ++q) {
+        double emin = 1e300, emax = -1e300;
+        for (int a = 0; a <= md.dim; ++a) {
+            double e = eps[conn[elems[q] * (md.dim + 1) + a]];
+            emin = e < emin ? e : emin;
+            emax = e > emax ? e : emax;
+        }
+        int c0 = egrid_cell(g, emin), c1 = egrid_cell(g, emax) + 1;
+        lo = c0 < lo ? c0 : lo;
+        hi = c1 > hi ? c1 : hi;
+    }
+    off[i] = lo;
+    len[i] = hi - lo + 1;
+}
+
+__global__ void proj_values_kernel(MeshDims md, const double* __restrict__ eps,
+                                   const int32_t* __restrict__ conn, EGrid g, double vol,
+                                   int64_t n, const int32_t* __restrict__ off, int maxw,
+                                   double* __restrict__ R) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* row = R + i * (int64_t)maxw;
+    for (int w = 0; w < maxw; ++w) row[w] = 0.0;
+    int64_t elems[24];
+    int locs[24];
+    int ns = node_star(md, i, elems, locs);
+    for (int q = 0; q < ns; ++q) {
+        double ev[4];
+        for (int a = 0; a <= md.dim; ++a) ev[a] = eps[conn[elems[q] * (md.dim + 1) + a]];
+        project_simplex_basis(md.dim, ev, locs[q], vol, g, row, off[i]);
+    }
+}
+
+// Q_j[p] = sum_w R_j[w] g[p - w]  (window of node j shifted by m_lo)
+__global__ void conv_kernel(const double* __restrict__ R, const int32_t* __restrict__ len,
+                            int maxw, const double* __restrict__ gm, int nm, int qw, int64_t n,
+                            double* __restrict__ Q) {
+    int64_t j = blockIdx.y;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || p >= qw) return;
+    const int lenj = len[j];
+    const double* Rj = R + j * (int64_t)maxw;
+    double s = 0.0;
+    if (p < lenj + nm - 1) {
+        int w0 = p - (nm - 1) > 0 ? p - (nm - 1) : 0;
+        int w1 = p < lenj - 1 ? p : lenj - 1;
+        for (int w = w0; w <= w1; ++w) s += Rj[w] * gm[p - w];
+    }
+    Q[j * (int64_t)qw + p] = s;
+}
+
+// K_ij = amp * sum_b R_i(b) Q_j(b) over the overlap of the two windows.
+// Block = 256 columns j of one row i; row i's window is staged in LDS.
+constexpr int kKThreads = 256;
+__global__ void kernel_entries(const double* __restrict__ R, const double* __restrict__ Q,
+                               const int32_t* __restrict__ off, const int32_t* __restrict__ len,
+                               int maxw, int qw, int m_lo, int nm, int64_t n, int64_t ld,
+                               double amp, double* __restrict__ K) {
+    extern __shared__ double sRi[];
+    const int64_t i = blockIdx.y;
+    const int offi = off[i], leni = len[i];
+    for (int w = threadIdx.x; w < leni; w += blockDim.x) sRi[w] = R[i * (int64_t)maxw + w];
+    __syncthreads();
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    double s = 0.0;
+    if (j < n) {
+        const int qoff = off[j] + m_lo;
+        const int qlen = len[j] + nm - 1;
+        int b0 = offi > qoff ? offi : qoff;
+        int b1 = (offi + leni < qoff + qlen) ? offi + leni : qoff + qlen;
+        const double* Qj = Q + j * (int64_t)qw;
+        for (int b = b0; b < b1; ++b) s += sRi[b - offi] * Qj[b - qoff];
+        s *= amp;
+    }
+    K[i * ld + j] = s;  // padding columns hold exact zeros
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
+    if (!c) return TG_ERR_ARG;
+    MeshDims md{c->dim, {c->nsec[0], c->nsec[1], c->nsec[2]}};
+    EGrid g{c->e0, c->de, c->nbins};
+    const unsigned nb = ceil_div(c->n, 128);
+
+    if (!c->assembled) {
+        // 1. projection windows
+        TG_TRY(dev_alloc(c, &c->d_off, c->n));
+        TG_TRY(dev_alloc(c, &c->d_len, c->n));
+        hipLaunchKernelGGL(proj_window_kernel, dim3(nb), dim3(128), 0, c->stream, md, c->d_eps,
+                           c->d_conn, g, c->n, c->d_off, c->d_len);
+        TG_CHECK_LAUNCH(c);
+        std::vector<int32_t> hlen(c->n);
+        TG_HIP(c, hipMemcpyAsync(hlen.data(), c->d_len, sizeof(int32_t) * c->n,
+                                 hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+        int32_t maxw = 0;
+        for (auto v : hlen) maxw = v > maxw ? v : maxw;
+        c->maxw = maxw;
+
+        // 2. projection values
+        TG_TRY(dev_alloc(c, &c->d_R, (size_t)c->n * maxw));
+        hipLaunchKernelGGL(proj_values_kernel, dim3(nb), dim3(128), 0, c->stream, md, c->d_eps,
+                           c->d_conn, g, c->vol, c->n, c->d_off, maxw, c->d_R);
+        TG_CHECK_LAUNCH(c);
+
+        // 3. sampled broadened delta G(m dE + omega0), |.| <= cutoff*sigma
+        const int32_t lo = (int32_t)std::ceil((-c->cutoff * c->sigma - c->omega0) / c->de);
+        const int32_t hi = (int32_t)std::floor((c->cutoff * c->sigma - c->omega0) / c->de);
+        c->m_lo = lo;
+        c->nm = hi - lo + 1;
+        std::vector<double> gm(c->nm);
+        const double norm = 1.0 / (c->sigma * std::sqrt(2.0 * M_PI));
+        for (int32_t m = lo; m <= hi; ++m) {
+            double x = (double)m * c->de + c->omega0;
+            gm[m - lo] = norm * std::exp(-0.5 * (x / c->sigma) * (x / c->sigma));
+        }
+        TG_TRY(dev_alloc(c, &c->d_gm, gm.size()));
+        TG_HIP(c, hipMemcpyAsync(c->d_gm, gm.data(), sizeof(double) * gm.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+
+        // 4. Q = R (*) g
+        c->qw = c->maxw + c->nm - 1;
+        TG_TRY(dev_alloc(c, &c->d_Q, (size_t)c->n * c->qw));
+        hipLaunchKernelGGL(conv_kernel, dim3(ceil_div(c->qw, 128), c->n), dim3(128), 0, c->stream,
+                           c->d_R, c->d_len, c->maxw, c->d_gm, c->nm, c->qw, c->n, c->d_Q);
+        TG_CHECK_LAUNCH(c);
+
+        // 5. K (dense FP64, row stride ld, zero padding)
+        TG_TRY(dev_alloc(c, &c->d_K, (size_t)c->n * c->ld));
+        hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
+                           sizeof(double) * c->maxw, c->stream, c->d_R, c->d_Q, c->d_off, c->d_len,
+                           c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->g * c->g, c->d_K);
+        TG_CHECK_LAUNCH(c);
+
+        // 6. plain-apply partial buffers, then K 1 and K^T 1 through the same
+        //    one-pass kernel (f = 1), summed in a fixed order
+        c->n_rowblk = (int32_t)ceil_div(c->n, kPlainRows);
+        c->n_colchk = (int32_t)ceil_div(c->n, kPlainChunk);
+        TG_TRY(dev_alloc(c, &c->d_upart, (size_t)c->n_colchk * c->n));
+        TG_TRY(dev_alloc(c, &c->d_vpart, (size_t)c->n_rowblk * c->ld));
+        TG_TRY(dev_alloc(c, &c->d_rk, c->n));
+        TG_TRY(dev_alloc(c, &c->d_ck, c->n));
+        {
+            std::vector<double> ones(c->n, 1.0);
+            TG_HIP(c, hipMemcpyAsync(c->d_Q, ones.data(), sizeof(double) * c->n,
+                                     hipMemcpyHostToDevice, c->stream));
+            TG_TRY(plain_products_device(c, c->d_Q, c->d_rk, c->d_ck));
+            TG_HIP(c, hipStreamSynchronize(c->stream));
+        }
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+        // the projection/convolution scratch is only needed for assembly
+        dev_free(&c->d_Q);
+        c->assembled = true;
+    }
+    if ((flags & TG_PREC_MIXED) || c->max_batch > 1) TG_TRY(mixed_prepare(c));
+    return TG_OK;
+}
+
+extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows, double* out) {
+    if (!c || !out) return TG_ERR_ARG;
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
+    if (row0 < 0 || nrows < 0 || row0 + nrows > c->n) return set_error(c, TG_ERR_ARG, "rows out of range");
+    TG_HIP(c, hipMemcpy2DAsync(out, sizeof(double) * c->n, c->d_K + row0 * c->ld,
+                               sizeof(double) * c->ld, sizeof(double) * c->n, nrows,
+                               hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
+}

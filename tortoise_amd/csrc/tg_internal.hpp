@@ -1,0 +1,149 @@
+// tg_internal.hpp -- context layout and helpers shared by the HIP sources of
+// libtortoise_gpu.so.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/tortoise_gpu.h"
+
+namespace tg {
+
+// Plain FP64 one-pass kernel geometry: a workgroup of 256 threads covers
+// kPlainRows rows x kPlainCols columns per sweep (each thread two adjacent
+// columns, the four waves interleaved over rows) and sweeps kPlainChunk
+// columns before reducing its per-row partial sums.
+constexpr int kPlainThreads = 256;
+constexpr int kPlainRows = 128;      // rows per workgroup (32 per wave)
+constexpr int kPlainCols = 128;      // columns per sweep (64 lanes x 2)
+constexpr int kPlainChunk = 2048;    // columns per workgroup
+constexpr int kLdAlign = 64;         // K row stride alignment in elements
+
+struct Buffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct tg_ctx_impl;
+
+}  // namespace tg
+
+struct tg_ctx {
+    // ---- problem
+    int dim = 2;
+    int32_t nsec[3] = {1, 1, 1};
+    double origin[3] = {0, 0, 0};
+    double extent[3] = {1, 1, 1};
+    int64_t n = 0;       // nodes
+    int64_t nel = 0;     // elements
+    int64_t ld = 0;      // K row stride (elements)
+    double vol = 0;      // element volume
+    double g = 0, omega0 = 0, sigma = 0, bath_n = 0, de = 0, cutoff = 8;
+    int32_t max_batch = 1;
+
+    // energy grid and projection
+    double e0 = 0;
+    int32_t nbins = 0;
+    int32_t maxw = 0;     // projection window width
+    int32_t m_lo = 0;     // first sampled offset of the broadened delta
+    int32_t nm = 0;       // number of sampled offsets
+    int32_t qw = 0;       // convolved window width = maxw + nm - 1
+
+    // ---- device data
+    double* d_eps = nullptr;
+    double* d_V = nullptr;
+    int32_t* d_conn = nullptr;     // nel x (dim+1)
+    int32_t* d_off = nullptr;      // projection window start per node
+    int32_t* d_len = nullptr;      // projection window length per node
+    double* d_R = nullptr;         // n x maxw
+    double* d_Q = nullptr;         // n x qw
+    double* d_gm = nullptr;        // nm
+    double* d_K = nullptr;         // n x ld, FP64
+    double* d_rk = nullptr;        // K 1
+    double* d_ck = nullptr;        // K^T 1
+    bool assembled = false;
+
+    // plain FP64 apply scratch
+    int32_t n_rowblk = 0, n_colchk = 0;
+    double* d_upart = nullptr;     // n_colchk x n
+    double* d_vpart = nullptr;     // n_rowblk x ld
+
+    // mixed batched data (tg_mixed.hip)
+    void* mixed = nullptr;
+
+    // DP5 work (tg_dp5.hip)
+    void* dp5 = nullptr;
+
+    // scratch vectors for host<->device staging (n x max_batch each)
+    double* d_f = nullptr;
+    double* d_I = nullptr;
+    double* d_red = nullptr;       // reduction partials
+    double* h_pinned = nullptr;    // pinned host scalars
+    int32_t red_cap = 0;
+
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+};
+
+namespace tg {
+
+tg_status set_error(tg_ctx* c, tg_status code, const char* fmt, ...);
+
+#define TG_HIP(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::tg::set_error((ctx), TG_ERR_HIP, "%s failed: %s (%s:%d)", #call,  \
+                                   hipGetErrorString(_e), __FILE__, __LINE__);         \
+    } while (0)
+
+#define TG_CHECK_LAUNCH(ctx) TG_HIP(ctx, hipGetLastError())
+
+#define TG_TRY(expr)                          \
+    do {                                      \
+        tg_status _s = (expr);                \
+        if (_s != TG_OK) return _s;           \
+    } while (0)
+
+template <typename T>
+tg_status dev_alloc(tg_ctx* c, T** p, size_t count) {
+    if (*p) return TG_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T) + 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(c, TG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T),
+                         hipGetErrorString(e));
+    }
+    return TG_OK;
+}
+
+template <typename T>
+void dev_free(T** p) {
+    if (*p) hipFree((void*)*p);
+    *p = nullptr;
+}
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- internal entry points between translation units
+tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
+                             int conserve);
+tg_status plain_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
+tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, double* number,
+                              double* energy);
+tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,
+                                int32_t batch, double rtol, double atol, double* h_out);
+tg_status mixed_prepare(tg_ctx* c);
+tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
+                             const double* h_bath, int conserve);
+void mixed_destroy(tg_ctx* c);
+void dp5_destroy(tg_ctx* c);
+tg_status ensure_staging(tg_ctx* c, int32_t batch);
+
+}  // namespace tg

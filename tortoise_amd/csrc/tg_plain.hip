@@ -1,0 +1,265 @@
+// tg_plain.hip -- plain FP64 collision operator, one pass over K.
+//
+// Per evaluation the kernel streams K (n x ld FP64, row-major) from HBM once
+// and produces both u = K f and v = K^T f:
+//
+//   phase 1 (plain_dual_kernel): workgroup (colchunk c, rowblock r) covers
+//     rows [128 r, 128 r + 128) x cols [2048 c, 2048 c + 2048).  Each of the 4
+//     waves owns 32 consecutive rows; a lane owns two adjacent columns of a
+//     128-column sweep, so every row read is one 1 KiB coalesced segment per
+//     wave (16 B per lane).  The 32 loads of a sweep are independent and in
+//     flight together.  Row partials of u stay in registers for the whole
+//     chunk and are reduced across the wave once per chunk; column partials
+//     of v are reduced across the 4 waves in LDS once per sweep.
+//       upart[c][i] = sum_{j in chunk c} K_ij f_j
+//       vpart[r][j] = sum_{i in rowblock r} K_ij f_i
+//   phase 2 (plain_finish_kernel): per node, sum the partials in index order
+//     (deterministic) and apply the Pauli-blocked assembly
+//       I_i = [(1-f_i)((n+1)u_i + n v_i) - f_i((n+1)ck_i + n rk_i - (n+1)v_i - n u_i)] / V_i
+//     with rk = K 1 and ck = K^T 1 precomputed in FP64.
+//
+// Algorithmic HBM traffic per evaluation: 8 n^2 (K) + 32 n (f, I, two
+// vectors); the partial buffers add 8 n (n/128 + n/2048) bytes (< 1 %).
+#include "tg_internal.hpp"
+
+namespace tg {
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+constexpr int kRowsPerWave = kPlainRows / 4;            // 32
+constexpr int kSweeps = kPlainChunk / kPlainCols;       // 16
+
+__global__ void __launch_bounds__(kPlainThreads)
+plain_dual_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+                  const double* __restrict__ f, double* __restrict__ upart,
+                  double* __restrict__ vpart) {
+    __shared__ double sv[4][kPlainCols];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t row0 = (int64_t)blockIdx.y * kPlainRows + wave * kRowsPerWave;
+    const int64_t col0 = (int64_t)blockIdx.x * kPlainChunk;
+
+    double fr[kRowsPerWave];
+    double uacc[kRowsPerWave];
+#pragma unroll
+    for (int q = 0; q < kRowsPerWave; ++q) {
+        const int64_t r = row0 + q;
+        fr[q] = r < n ? f[r] : 0.0;
+        uacc[q] = 0.0;
+    }
+    const int nrows = (int)((n - row0) < kRowsPerWave ? (n - row0) : kRowsPerWave);
+
+    for (int s = 0; s < kSweeps; ++s) {
+        const int64_t cs = col0 + (int64_t)s * kPlainCols + 2 * lane;
+        if (col0 + (int64_t)s * kPlainCols >= ld) break;  // uniform over the block
+        double fc0 = 0.0, fc1 = 0.0;
+        if (cs < n) fc0 = f[cs];
+        if (cs + 1 < n) fc1 = f[cs + 1];
+        double v0 = 0.0, v1 = 0.0;
+        if (cs < ld && nrows > 0) {
+            if (nrows == kRowsPerWave) {
+                dbl2 kk[kRowsPerWave];
+#pragma unroll
+                for (int q = 0; q < kRowsPerWave; ++q)
+                    kk[q] = __builtin_nontemporal_load(
+                        reinterpret_cast<const dbl2*>(K + (row0 + q) * ld + cs));
+#pragma unroll
+                for (int q = 0; q < kRowsPerWave; ++q) {
+                    uacc[q] += kk[q].x * fc0 + kk[q].y * fc1;
+                    v0 += kk[q].x * fr[q];
+                    v1 += kk[q].y * fr[q];
+                }
+            } else {
+                for (int q = 0; q < nrows; ++q) {
+                    dbl2 kk = *reinterpret_cast<const dbl2*>(K + (row0 + q) * ld + cs);
+                    uacc[q] += kk.x * fc0 + kk.y * fc1;
+                    v0 += kk.x * fr[q];
+                    v1 += kk.y * fr[q];
+                }
+            }
+        }
+        sv[wave][2 * lane] = v0;
+        sv[wave][2 * lane + 1] = v1;
+        __syncthreads();
+        if (threadIdx.x < kPlainCols) {
+            const int64_t c = col0 + (int64_t)s * kPlainCols + threadIdx.x;
+            const double t = (sv[0][threadIdx.x] + sv[1][threadIdx.x]) +
+                             (sv[2][threadIdx.x] + sv[3][threadIdx.x]);
+            if (c < ld) vpart[(int64_t)blockIdx.y * ld + c] = t;
+        }
+        __syncthreads();
+    }
+    // row partials: reduce across the 64 lanes of the wave
+#pragma unroll
+    for (int q = 0; q < kRowsPerWave; ++q) {
+        double u = uacc[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
+        uacc[q] = u;
+    }
+    if (lane < kRowsPerWave) {
+        // lane q publishes row q (the value is identical in every lane)
+        double mine = 0.0;
+#pragma unroll
+        for (int q = 0; q < kRowsPerWave; ++q) mine = (lane == q) ? uacc[q] : mine;
+        const int64_t r = row0 + lane;
+        if (r < n) upart[(int64_t)blockIdx.x * n + r] = mine;
+    }
+}
+
+// mode 0: collision operator; mode 1: raw products u -> out0, v -> out1
+constexpr int kFinishNodes = 64;
+constexpr int kFinishSlices = 4;
+
+__global__ void __launch_bounds__(kFinishNodes* kFinishSlices)
+plain_finish_kernel(int mode, const double* __restrict__ upart, int ncol,
+                    const double* __restrict__ vpart, int nrow, int64_t n, int64_t ld,
+                    const double* __restrict__ f, const double* __restrict__ rk,
+                    const double* __restrict__ ck, const double* __restrict__ V,
+                    double bath_n, double* __restrict__ out0, double* __restrict__ out1,
+                    double* __restrict__ dpart) {
+    __shared__ double su[kFinishSlices][kFinishNodes];
+    __shared__ double svv[kFinishSlices][kFinishNodes];
+    const int tx = threadIdx.x % kFinishNodes;
+    const int ty = threadIdx.x / kFinishNodes;
+    const int64_t i = (int64_t)blockIdx.x * kFinishNodes + tx;
+    double u = 0.0, v = 0.0;
+    if (i < n) {
+        for (int c = ty; c < ncol; c += kFinishSlices) u += upart[(int64_t)c * n + i];
+        for (int r = ty; r < nrow; r += kFinishSlices) v += vpart[(int64_t)r * ld + i];
+    }
+    su[ty][tx] = u;
+    svv[ty][tx] = v;
+    __syncthreads();
+    double dI = 0.0, dw = 0.0;
+    if (ty == 0 && i < n) {
+        u = (su[0][tx] + su[1][tx]) + (su[2][tx] + su[3][tx]);
+        v = (svv[0][tx] + svv[1][tx]) + (svv[2][tx] + svv[3][tx]);
+        if (mode == 1) {
+            out0[i] = u;
+            out1[i] = v;
+        } else {
+            const double np1 = bath_n + 1.0;
+            const double fi = f[i];
+            const double wf = np1 * u + bath_n * v;
+            const double wt1 = np1 * ck[i] + bath_n * rk[i];
+            const double wtf = np1 * v + bath_n * u;
+            const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[i];
+            out0[i] = Ii;
+            dI = V[i] * Ii;
+            dw = V[i] * fi * (1.0 - fi);
+        }
+    }
+    if (dpart) {
+        // per-block partials of sum V I and sum V f(1-f), wave 0 only
+        if (ty == 0) {
+            for (int o = 32; o > 0; o >>= 1) {
+                dI += __shfl_down(dI, o, 64);
+                dw += __shfl_down(dw, o, 64);
+            }
+            if (tx == 0) {
+                dpart[2 * blockIdx.x] = dI;
+                dpart[2 * blockIdx.x + 1] = dw;
+            }
+        }
+    }
+}
+
+// Removes the number defect d = sum V I with weights f(1-f):
+// I_i -= d f_i(1-f_i) / sum_k V_k f_k(1-f_k).
+__global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
+                                    const double* __restrict__ f, double* __restrict__ I) {
+    double d = 0.0, w = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        d += dpart[2 * b];
+        w += dpart[2 * b + 1];
+    }
+    if (!(w > 0.0)) return;
+    const double s = d / w;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        I[i] -= s * f[i] * (1.0 - f[i]);
+}
+
+static tg_status launch_dual(tg_ctx* c, const double* d_f) {
+    hipLaunchKernelGGL(plain_dual_kernel, dim3(c->n_colchk, c->n_rowblk), dim3(kPlainThreads), 0,
+                       c->stream, c->d_K, c->n, c->ld, d_f, c->d_upart, c->d_vpart);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
+                             int conserve) {
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
+    TG_TRY(launch_dual(c, d_f));
+    const unsigned nb = ceil_div(c->n, kFinishNodes);
+    double* dpart = nullptr;
+    if (conserve) {
+        if ((size_t)c->red_cap < 2 * (size_t)nb) {
+            dev_free(&c->d_red);
+            TG_TRY(dev_alloc(c, &c->d_red, 2 * (size_t)nb));
+            c->red_cap = 2 * (int32_t)nb;
+        }
+        dpart = c->d_red;
+    }
+    hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
+                       c->stream, 0, c->d_upart, c->n_colchk, c->d_vpart, c->n_rowblk, c->n,
+                       c->ld, d_f, c->d_rk, c->d_ck, c->d_V, bath_n, d_I, nullptr, dpart);
+    TG_CHECK_LAUNCH(c);
+    if (conserve) {
+        hipLaunchKernelGGL(conserve_fix_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                           dpart, (int)nb, c->n, d_f, d_I);
+        TG_CHECK_LAUNCH(c);
+    }
+    return TG_OK;
+}
+
+// u = K f, v = K^T f (used for the row/column sums at assembly)
+tg_status plain_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v) {
+    TG_TRY(launch_dual(c, d_f));
+    const unsigned nb = ceil_div(c->n, kFinishNodes);
+    hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
+                       c->stream, 1, c->d_upart, c->n_colchk, c->d_vpart, c->n_rowblk, c->n,
+                       c->ld, d_f, nullptr, nullptr, nullptr, 0.0, d_u, d_v, nullptr);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" tg_status tg_collision_apply(tg_ctx* c, const double* f, double* I, int64_t n,
+                                        int32_t batch, const double* bath_n, int32_t flags) {
+    if (!c || !f || !I) return TG_ERR_ARG;
+    if (n != c->n)
+        return set_error(c, TG_ERR_ARG, "n=%lld but the mesh has %lld nodes", (long long)n,
+                         (long long)c->n);
+    if (batch < 1) return set_error(c, TG_ERR_ARG, "batch < 1");
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "tg_kernel_assemble not called");
+    const bool dev = flags & TG_PTR_DEVICE;
+    const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
+    const double* d_f = f;
+    double* d_I = I;
+    if (!dev) {
+        TG_TRY(ensure_staging(c, batch));
+        TG_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * n * batch, hipMemcpyHostToDevice,
+                                 c->stream));
+        d_f = c->d_f;
+        d_I = c->d_I;
+    }
+    if (flags & TG_PREC_MIXED) {
+        TG_TRY(mixed_apply_device(c, d_f, d_I, batch, bath_n, conserve));
+    } else {
+        for (int b = 0; b < batch; ++b) {
+            const double nb = bath_n ? bath_n[b] : c->bath_n;
+            TG_TRY(plain_apply_device(c, d_f + (int64_t)b * n, d_I + (int64_t)b * n, nb, conserve));
+        }
+    }
+    if (!dev) {
+        TG_HIP(c, hipMemcpyAsync(I, d_I, sizeof(double) * n * batch, hipMemcpyDeviceToHost,
+                                 c->stream));
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return TG_OK;
+}

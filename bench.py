@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Scattering-integral throughput on one MI355X.
 
-One step = one pass of the hot path over one batch: B collision-operator
-evaluations (B = 1 for the plain FP64 form) on the 128x128-section square
-mesh (16641 nodes, dense 16641^2 kernel), inputs resident in HBM.  Prints one
-JSON line (see README of the measurement contract in DESIGN.md).
+Workload: square momentum zone, 128x128 sections (32768 P1 triangles, 16641
+nodes), e-ph emission kernel of the Gaussian-broadened overlaps, 256
+independent members (pump centre, amplitude and temperature drawn with seed
+0).  One step = one batched evaluation of the collision operator for all 256
+members (fp16 hi+lo split, FP32 MFMA, FP64 epilogue), inputs resident in HBM.
+The plain FP64 form (one member per evaluation) is timed in the same run, and
+the CPU oracle (FP64, OpenMP) on the host cores.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -18,41 +21,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_plain(ctx, torch, f, I, steps, warmup):
-    # the library launches on its own stream: record the events there
+def time_loop(torch, ctx, call, steps, warmup):
+    """Average ms per call from HIP events on the library's stream."""
     stream = torch.cuda.ExternalStream(ctx.stream())
     for _ in range(warmup):
-        ctx.apply_device(f.data_ptr(), I.data_ptr())
+        call()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(steps):
-        ctx.apply_device(f.data_ptr(), I.data_ptr())
+        call()
     e1.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / steps, wall
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
 
 
-def cpu_baseline(ctx, w, budget_s=15.0):
-    """Oracle W-form (one pass over K, OpenMP) on the host cores."""
+def kernel_ms(ctx, call, reps):
+    """Average duration of the dominant kernel (HIP events around it)."""
+    tot = 0.0
+    for _ in range(reps):
+        call()
+        tot += ctx.last_kernel_ms()
+    return tot / reps
+
+
+def cpu_baseline(ctx, w, f, budget_s=15.0):
+    """Oracle W-form (one pass over K, OpenMP) on the host cores, one member."""
     import numpy as np
 
     import oracle
 
     n = ctx.n
-    _log("cpu baseline: downloading K")
     K = ctx.kernel_rows(0, n)
     V = ctx.node_volumes()
-    f = w.initial_f()
     oracle.collision_apply(K, V, w.bath_n, f)  # warm-up
     times = []
     t_start = time.perf_counter()
@@ -63,8 +73,8 @@ def cpu_baseline(ctx, w, budget_s=15.0):
     med = float(np.median(times))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 W-form apply, n={n}, median of {len(times)} evaluations "
-                      f"({sum(times):.1f} s)"}
+            "sample": f"oracle FP64 W-form evaluation of one member, n={n}, median of "
+                      f"{len(times)} evaluations ({sum(times):.1f} s)"}
 
 
 def main():
@@ -80,48 +90,101 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
 
     import tortoise_amd as tg
     from tortoise_amd import workloads as wl
 
-    w = wl.SQUARE128
+    w = wl.SQUARE128_ENSEMBLE
+    B = w.batch
     torch.cuda.init()
-    ctx = tg.Context.create(w, max_batch=1)
+    ctx = tg.Context.create(w, max_batch=B)
     t0 = time.perf_counter()
-    ctx.assemble()
+    ctx.assemble(mixed=True)
     torch.cuda.synchronize()
-    _log(f"assembled K (n={ctx.n}) in {time.perf_counter() - t0:.2f} s")
+    st = ctx.kernel_stats()
     n = ctx.n
-    f = torch.from_numpy(w.initial_f()).to("cuda")
-    I = torch.empty_like(f)
+    _log(f"assembled K and the split [K;K^T] (n={n}) in {time.perf_counter() - t0:.2f} s; "
+         f"nnz(K)={st['nnz']} ({st['nnz'] / n / n:.3f} dense), streamed band "
+         f"{st['mixed_band']} entries, splits {st['mixed_splits']}")
+
+    eps = w.eps()
+    centre, amp, temp = wl.ensemble_members(B, seed=0)
+    F = np.stack([wl.initial_distribution(eps, w.mu, temp[b], centre[b], w.pump_width, amp[b])
+                  for b in range(B)], axis=0)  # member-major
+    bath = np.array([wl.bose_einstein(w.omega0, t) for t in temp])
+    Fd = torch.from_numpy(F).to("cuda")
+    Id = torch.empty_like(Fd)
     torch.cuda.synchronize()
-    ms, wall = time_plain(ctx, torch, f, I, args.steps, args.warmup)
-    evals = 1000.0 / ms
-    bytes_alg = 8.0 * n * n + 32.0 * n
-    gbs = bytes_alg / (ms * 1e-3) / 1e9
-    _log(f"plain FP64: {ms:.4f} ms/eval, {gbs:.0f} GB/s algorithmic")
+
+    def mixed_call(time_kernel=False):
+        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
+                         time_kernel=time_kernel)
+
+    def plain_call(time_kernel=False):
+        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=1, bath_host=bath[:1],
+                         time_kernel=time_kernel)
+
+    ms_mixed, wall = time_loop(torch, ctx, mixed_call, args.steps, args.warmup)
+    ms_gemm = kernel_ms(ctx, lambda: mixed_call(True), 10)
+    ms_plain, _ = time_loop(torch, ctx, plain_call, args.steps, args.warmup)
+    ms_dual = kernel_ms(ctx, lambda: plain_call(True), 10)
+
+    # accuracy of the measured batched evaluation against the FP64 form
+    Im = Id.cpu().numpy().copy()
+    Fh = F.T.copy()
+    pick = [0, 63, 128, 255]
+    Ip = ctx.apply(Fh[:, pick], bath_n=bath[pick])
+    rel = max(float(np.linalg.norm(Im[b] - Ip[:, q]) / np.linalg.norm(Ip[:, q]))
+              for q, b in enumerate(pick))
+
+    evals_mixed = B * 1000.0 / ms_mixed
+    evals_plain = 1000.0 / ms_plain
+    # dominant kernel: the MFMA GEMM.  Algorithmic work = the two products
+    # K F and K^T F over the structural nonzeros of K: 4 nnz B flops.
+    flops_alg = 4.0 * st["nnz"] * B
+    tflops = flops_alg / (ms_gemm * 1e-3) / 1e12
+    issued = 3 * 2.0 * st["mixed_band"] * st["mixed_width"] / (ms_gemm * 1e-3) / 1e12
+    # plain FP64 pass: 8 n^2 + 32 n bytes (dense K) per evaluation
+    gbs_plain = (8.0 * n * n + 32.0 * n) / (ms_dual * 1e-3) / 1e9
+    _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
+         f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}")
+    _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), dual kernel "
+         f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s")
+
     out = {
         "metric": "scattering-integral evals/s: batched mixed-prec vs FP64 plain @1e-6 rel err",
-        "value": evals,
+        "value": evals_mixed,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ms_mixed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "fp16x2-split/f32-acc/f64-epilogue",
         "data": "synthetic",
-        "config": {"workload": "square128_fp64_plain", "nodes": n, "batch": 1,
-                   "mesh": "128x128 sections, 32768 P1 triangles", "form": "plain FP64"},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+        "config": {"workload": w.name, "nodes": n, "members": B,
+                   "mesh": "128x128 sections, 32768 P1 triangles",
+                   "kernel_nnz": st["nnz"], "streamed_band": st["mixed_band"],
+                   "splits": st["mixed_splits"]},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP16_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / FP16_MFMA_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "mixed_gemm_kernel", "kernel_ms": ms_gemm,
+                     "issued_tflops": issued},
+        "plain_fp64": {"evals_per_s": evals_plain, "ms_per_eval": ms_plain,
+                       "kernel_ms": ms_dual, "achieved_gbs": gbs_plain,
+                       "frac_hbm": gbs_plain / HBM_PEAK_GBS},
+        "mixed_vs_plain": evals_mixed / evals_plain,
+        "max_rel_err_vs_fp64": rel,
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, w)
+        cb = cpu_baseline(ctx, w, Fh[:, 0].copy())
+        out["cpu_baseline"] = cb
+        out["mixed_vs_cpu"] = evals_mixed / cb["value"]
     ctx.close()
     print(json.dumps(out), flush=True)
 

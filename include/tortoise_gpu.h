@@ -49,6 +49,7 @@ typedef int32_t tg_status;
 #define TG_PREC_MIXED   0x2   /* batched fp16 hi+lo split kernel, FP32 MFMA, FP64 epilogue */
 #define TG_CONSERVE     0x4   /* remove the particle-number defect of one evaluation */
 #define TG_QUIET        0x8   /* (dp5) no per-step console line */
+#define TG_TIME_KERNEL  0x10  /* (apply) bracket the dominant kernel with HIP events */
 
 typedef struct tg_problem_desc {
     int32_t dim;            /* 2 or 3 */
@@ -99,6 +100,19 @@ tg_status   tg_node_volumes(tg_ctx* ctx, double* V);
  * batched kernel.  Replaces the tuple screening of
  * ScatteringChannel::precalculateIndices (ScatteringChannel.cpp:421-538). */
 tg_status   tg_kernel_assemble(tg_ctx* ctx, int32_t flags);
+typedef struct tg_kernel_stats {
+    int64_t n;              /* nodes */
+    int64_t nnz;            /* structurally nonzero entries of K */
+    int64_t mixed_band;     /* entries of [K; K^T] the batched kernel streams per evaluation */
+    int32_t mixed_splits;   /* reduction splits of the batched kernel */
+    int32_t mixed_width;    /* members per batched tile (64, 128 or 256) */
+    int32_t mixed_scale;    /* power-of-two exponent applied to K before the fp16 split */
+    int32_t window;         /* widest energy-projection window (hats per node) */
+    double  kmax;           /* largest row sum of K */
+} tg_kernel_stats;
+
+/* Sizes of the assembled kernel (for throughput and roofline accounting). */
+tg_status   tg_kernel_stats_get(tg_ctx* ctx, tg_kernel_stats* out);
 /* Copies rows [row0, row0+nrows) of K (FP64) to host `out` (nrows x n). */
 tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double* out);
 
@@ -111,17 +125,33 @@ tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double
 tg_status   tg_collision_apply(tg_ctx* ctx, const double* f, double* I, int64_t n,
                                int32_t batch, const double* bath_n, int32_t flags);
 
+/* Duration in ms of the dominant kernel of the last tg_collision_apply made
+ * with TG_TIME_KERNEL (plain: the one-pass K f / K^T f kernel; mixed: the
+ * MFMA GEMM), from HIP events on the context's stream.  Waits for it. */
+tg_status   tg_last_kernel_ms(tg_ctx* ctx, double* ms);
+
 /* Adaptive DP5(4) from t0 to t1 for every member (per-member step size and
  * accept/reject, error norm of Band::relativeError for a fermion band).
  * f (n x batch) holds the initial state and receives the state after the
  * last accepted step.  dense_out (optional, max_dense x n x batch, member-
  * major: member b's k-th output at dense_out + (b*max_dense + k)*n) receives
- * the dense outputs at t0 + k*dense_dt.  stats: batch entries (optional). */
+ * the dense outputs at t0 + k*dense_dt.  stats: batch entries (optional).
+ * h_accepted (optional, batch x h_cap): the accepted step sizes of member b
+ * in order at h_accepted + b*h_cap (stats[b].accepted of them, up to h_cap). */
 tg_status   tg_dp5_integrate(tg_ctx* ctx, double* f, int64_t n, int32_t batch,
                              double t0, double t1, double dense_dt, double rtol,
                              double atol, const double* bath_n, int32_t flags,
                              double* dense_out, int32_t max_dense,
-                             tg_step_stats* stats);
+                             tg_step_stats* stats, double* h_accepted, int32_t h_cap);
+
+/* The same stage arithmetic on a given step sequence: member b takes the
+ * n_steps[b] steps h_steps[b*h_cap ...] (all accepted, no error control),
+ * with dense output as tg_dp5_integrate.  Used to compare two precisions on
+ * identical steps.  No reference counterpart (test and measurement aid). */
+tg_status   tg_dp5_replay(tg_ctx* ctx, double* f, int64_t n, int32_t batch, double t0,
+                          double t1, double dense_dt, const double* h_steps,
+                          const int32_t* n_steps, int32_t h_cap, const double* bath_n,
+                          int32_t flags, double* dense_out, int32_t max_dense);
 
 /* FP64 sums per member: number = sum_i V_i f_i, energy = sum_i V_i eps_i f_i. */
 tg_status   tg_conservation(tg_ctx* ctx, const double* f, int64_t n, int32_t batch,

@@ -1,0 +1,58 @@
+// CPU checks of the Tortoise:: host layer (no GPU calls): mesh numbering,
+// DG <-> nodal conversion, Function arithmetic and integrals against closed
+// forms.  Prints "ok" and exits 0 on success.
+#include <TORTOISE>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+using namespace Tortoise;
+
+static int fails = 0;
+#define CHECK(c)                                                        \
+    do {                                                                \
+        if (!(c)) {                                                     \
+            std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c);     \
+            ++fails;                                                    \
+        }                                                               \
+    } while (0)
+
+int main() {
+    const Real pi = M_PI;
+    Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
+    Mesh<2> m(zone, Point<2>({0., 0.}), Point<2>({1., 1.}), {6, 5});
+    CHECK(m.numberElements == 60 && m.numberNodes() == 42);
+    // Mesh.cpp:55-87 vertex order of section (0,0)
+    CHECK(m.elementNode(0, 0) == 0 && m.elementNode(0, 1) == 1 && m.elementNode(0, 2) == 8);
+    CHECK(m.elementNode(1, 0) == 0 && m.elementNode(1, 1) == 7 && m.elementNode(1, 2) == 8);
+    // a linear field round-trips exactly and integrates in closed form
+    Function<2> lin(m, [](Point<2> k) { return 1.0 + 0.5 * k(0) - 0.25 * k(1); });
+    auto nodal = lin.toNodal();
+    Function<2> back = Function<2>::fromNodal(m, nodal);
+    Real d = 0;
+    for (size_t i = 0; i < lin.vec.size(); ++i) d = std::fmax(d, std::fabs(lin.vec[i] - back.vec[i]));
+    CHECK(d < 1e-14);
+    CHECK(std::fabs(lin.integrate() - 4 * pi * pi) < 1e-12);   // odd parts vanish on the box
+    // int x^2 over [-pi,pi]^2 = (2pi)(2pi^3/3); exact for P1*P1
+    Function<2> x(m, [](Point<2> k) { return k(0); });
+    CHECK(std::fabs(x.integrate(x) - 2 * pi * 2 * pi * pi * pi / 3) < 1e-10);
+    // nodal product and constant shift
+    Function<2> one(m, 1.0);
+    Function<2> sq = x * x;
+    CHECK(std::fabs(sq.integrate() - (x.integrate(x) + 0.0)) > 0.0);  // lumped != exact
+    Function<2> y = one;
+    y += 2.0;
+    CHECK(std::fabs(y.integrate() - 3 * 4 * pi * pi) < 1e-11);
+    // Band / Material bookkeeping
+    Material<2> mat("m", zone);
+    mat.addBand("el:C", -1, fermion, m, [](Point<2> k) { return 0.5 * k.squaredNorm(); });
+    mat.addBand("pn:o", 0, boson, m, 0.05);
+    auto ch = mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 0);
+    CHECK(ch.size() == 1 && ch[0]->electronBand == "el:C" && ch[0]->phononBand == "pn:o");
+    MaterialStatus<2> s(mat);
+    s.setToEquilibrium(0.02, 0.5, 0.0);
+    CHECK(std::fabs(s.bathOccupation() - 1.0 / std::expm1(0.05 / 0.02)) < 1e-15);
+    if (fails == 0) std::printf("ok\n");
+    return fails ? 1 : 0;
+}

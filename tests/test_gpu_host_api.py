@@ -1,0 +1,45 @@
+"""The C++ host-layer example runs the GPU operator through the Tortoise::
+API and agrees with the same integration driven through the C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import tortoise_amd as tg
+from tortoise_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tortoise_amd", "host", "examples", "thermalisation_square")
+
+
+def test_host_example_matches_abi():
+    assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
+    print("[gpu] host example 16 sections t=10", flush=True)
+    r = subprocess.run([EXE, "16", "10", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    steps = int(lines[0].split()[1])
+    t0_num = float(lines[2].split("|")[1])
+    fin = lines[-1].split("|")
+    num_end, en_end = float(fin[1]), float(fin[2])
+    # particle number conserved by the run
+    assert abs(num_end - t0_num) <= 1e-12 * t0_num
+    # same integration through the C ABI
+    w = wl.Workload("square16", 2, (16, 16), t_final=10.0, rtol=1e-8, atol=1e-12)
+    ctx = tg.Context.create(w, max_batch=1)
+    ctx.assemble()
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100)
+    assert st["attempted"] == steps
+    V = ctx.node_volumes()
+    assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
+    # energy with the exact P1 x P1 integral of Function::integrate(rhs)
+    conn = oracle.mesh_elements(2, [16, 16])
+    eps = w.eps()
+    vol = (2 * np.pi / 16) ** 2 / 2
+    a, b = dense[-1][conn], eps[conn]
+    en = vol / 12 * ((a * b).sum() + (a.sum(1) * b.sum(1)).sum())
+    assert abs(en - en_end) <= 1e-11 * abs(en)

@@ -1,0 +1,115 @@
+"""GPU parity of the batched mixed-precision path (fp16 hi+lo split, FP32
+MFMA accumulation, FP64 epilogue) against the FP64 paths.
+
+Tolerance: relative L2 error of one evaluation <= 2e-6 per member against
+the FP64 oracle (the split keeps ~22 significant bits per operand; FP32
+accumulation adds ~1e-7).  Number conservation with TG_CONSERVE: the defect
+sum_i V_i I_i is at FP64 roundoff.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tortoise_amd as tg
+from tortoise_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(w, B, seed=0):
+    eps = w.eps()
+    centre, amp, temp = wl.ensemble_members(B, seed)
+    F = np.stack([wl.initial_distribution(eps, w.mu, temp[b], centre[b], w.pump_width, amp[b])
+                  for b in range(B)], axis=1)
+    bath = np.array([wl.bose_einstein(w.omega0, t) for t in temp])
+    return F, bath
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def sq32_mixed():
+    w = wl.SQUARE32
+    print("[gpu] mixed square32 B=8", flush=True)
+    ctx = tg.Context.create(w, max_batch=8)
+    ctx.assemble(mixed=True)
+    ok = oracle.Kernel(w.dim, w.nsec, w.origin, w.extent, w.eps(), w.g, w.omega0, w.sigma)
+    return w, ctx, ok, ok.full()
+
+
+def test_mixed_matches_fp64(sq32_mixed):
+    w, ctx, ok, K = sq32_mixed
+    F, bath = _members(w, 8)
+    Im = ctx.apply(F, bath_n=bath, mixed=True)
+    Ip = ctx.apply(F, bath_n=bath)
+    for b in range(8):
+        Io = oracle.collision_apply(K, ok.V, bath[b], F[:, b])
+        assert _rel(Ip[:, b], Io) <= 1e-13
+        assert _rel(Im[:, b], Io) <= 2e-6, (b, _rel(Im[:, b], Io))
+
+
+def test_mixed_partial_batch_and_conservation(sq32_mixed):
+    w, ctx, ok, K = sq32_mixed
+    F, bath = _members(w, 3, seed=5)
+    Im = ctx.apply(F, bath_n=bath, mixed=True, conserve=True)
+    for b in range(3):
+        Io = oracle.collision_apply(K, ok.V, bath[b], F[:, b], conserve=True)
+        assert _rel(Im[:, b], Io) <= 2e-6
+        assert abs(np.dot(ok.V, Im[:, b])) <= 1e-13 * np.dot(ok.V, np.abs(Im[:, b]))
+
+
+def test_mixed_edge_states(sq32_mixed):
+    w, ctx, ok, K = sq32_mixed
+    F = np.zeros((ctx.n, 2))
+    F[:, 1] = 1.0
+    Im = ctx.apply(F, mixed=True)
+    assert np.abs(Im[:, 0]).max() == 0.0
+    # full band: (1-f) = 0 and K^T 1 - K^T f cancels against the split sums
+    assert np.abs(Im[:, 1]).max() <= 1e-9 * np.abs(K).sum(0).max()
+
+
+def test_mixed_large_mesh_full_batch():
+    w = wl.SQUARE128_ENSEMBLE
+    print("[gpu] mixed square128 B=256", flush=True)
+    ctx = tg.Context.create(w, max_batch=256)
+    ctx.assemble(mixed=True)
+    F, bath = _members(w, 256, seed=0)
+    Im = ctx.apply(F, bath_n=bath, mixed=True)
+    pick = [0, 17, 100, 255]
+    Ip = ctx.apply(F[:, pick], bath_n=bath[pick])
+    for q, b in enumerate(pick):
+        assert _rel(Im[:, b], Ip[:, q]) <= 2e-6, (b, _rel(Im[:, b], Ip[:, q]))
+
+
+def test_replay_matches_oracle_replay(sq32_mixed):
+    """FP64 replay of a recorded step sequence equals the oracle's replay."""
+    w, ctx, ok, K = sq32_mixed
+    f0 = w.initial_f()
+    fg, dg, st, steps = ctx.dp5(f0, 0.0, 20.0, 0.2, 1e-8, 1e-12, max_dense=100, h_cap=4096)
+    fr, dr = ctx.dp5_replay(f0, 0.0, 20.0, 0.2, steps, max_dense=100)
+    assert _rel(dr[-1], dg[-1]) <= 1e-15
+    ro = oracle.dp5_collision_replay(2, w.nsec, K, ok.V, w.bath_n, f0, 0.0, 20.0, 0.2, steps,
+                                     np.ones(len(steps), bool))
+    assert _rel(dr[-1], ro["dense"][-1]) <= 1e-12
+
+
+def test_mixed_dp5_error_vs_fp64_on_same_steps(sq32_mixed):
+    """Batched mixed-precision DP5 (per-member adaptive steps, rtol 1e-6) vs
+    FP64 replaying each member's accepted steps: relative L2 error of
+    f(t_final) <= 1e-6; particle-number drift <= 1e-12 (with TG_CONSERVE)."""
+    w, ctx, ok, K = sq32_mixed
+    F, bath = _members(w, 8, seed=3)
+    print("[gpu] mixed dp5 square32 B=8 t=100", flush=True)
+    fm, dm, stats, steps = ctx.dp5(F, 0.0, 100.0, 1.0, 1e-6, 1e-12, bath_n=bath, mixed=True,
+                                   conserve=True, max_dense=100, h_cap=4096)
+    assert all(s["n_dense"] == 100 and not s["failed"] for s in stats)
+    # members take different numbers of steps
+    assert len({s["attempted"] for s in stats}) > 1
+    fr, dr = ctx.dp5_replay(F, 0.0, 100.0, 1.0, steps, bath_n=bath, max_dense=100)
+    V = ok.V
+    for b in range(8):
+        assert _rel(dm[b, -1], dr[b, -1]) <= 1e-6, (b, _rel(dm[b, -1], dr[b, -1]))
+        n0 = np.dot(V, F[:, b])
+        assert abs(np.dot(V, dm[b, -1]) - n0) <= 1e-12 * n0

@@ -36,13 +36,14 @@ TG_PTR_DEVICE = 0x1
 TG_PREC_MIXED = 0x2
 TG_CONSERVE = 0x4
 TG_QUIET = 0x8
+TG_TIME_KERNEL = 0x10
 
 # every symbol include/tortoise_gpu.h declares
 EXPORTS = [
     "tg_num_nodes", "tg_ctx_create", "tg_ctx_destroy", "tg_last_error", "tg_ctx_set_stream",
     "tg_ctx_stream", "tg_mesh_info", "tg_node_volumes", "tg_kernel_assemble",
     "tg_kernel_copy_rows", "tg_collision_apply", "tg_dp5_integrate", "tg_conservation",
-    "tg_relative_error",
+    "tg_relative_error", "tg_kernel_stats_get", "tg_last_kernel_ms", "tg_dp5_replay",
 ]
 
 
@@ -60,6 +61,15 @@ class ProblemDesc(C.Structure):
         ("de", C.c_double), ("cutoff", C.c_double), ("max_batch", C.c_int32),
         ("reserved", C.c_int32),
     ]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("mixed_band", C.c_int64),
+                ("mixed_splits", C.c_int32), ("mixed_width", C.c_int32),
+                ("mixed_scale", C.c_int32), ("window", C.c_int32), ("kmax", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class StepStats(C.Structure):
@@ -106,13 +116,19 @@ def load():
     L.tg_node_volumes.argtypes = [vp, vp]
     L.tg_kernel_assemble.restype = i32
     L.tg_kernel_assemble.argtypes = [vp, i32]
+    L.tg_kernel_stats_get.restype = i32
+    L.tg_kernel_stats_get.argtypes = [vp, C.POINTER(KernelStats)]
+    L.tg_last_kernel_ms.restype = i32
+    L.tg_last_kernel_ms.argtypes = [vp, C.POINTER(f64)]
     L.tg_kernel_copy_rows.restype = i32
     L.tg_kernel_copy_rows.argtypes = [vp, i64, i64, vp]
     L.tg_collision_apply.restype = i32
     L.tg_collision_apply.argtypes = [vp, vp, vp, i64, i32, vp, i32]
     L.tg_dp5_integrate.restype = i32
     L.tg_dp5_integrate.argtypes = [vp, vp, i64, i32, f64, f64, f64, f64, f64, vp, i32, vp, i32,
-                                   vp]
+                                   vp, vp, i32]
+    L.tg_dp5_replay.restype = i32
+    L.tg_dp5_replay.argtypes = [vp, vp, i64, i32, f64, f64, f64, vp, vp, i32, vp, i32, vp, i32]
     L.tg_conservation.restype = i32
     L.tg_conservation.argtypes = [vp, vp, i64, i32, vp, vp, i32]
     L.tg_relative_error.restype = i32
@@ -199,6 +215,11 @@ class Context:
     def assemble(self, mixed=False):
         self._check(load().tg_kernel_assemble(self._h, TG_PREC_MIXED if mixed else 0))
 
+    def kernel_stats(self):
+        st = KernelStats()
+        self._check(load().tg_kernel_stats_get(self._h, C.byref(st)))
+        return st.as_dict()
+
     def kernel_rows(self, row0, nrows):
         out = np.zeros((nrows, self.n))
         self._check(load().tg_kernel_copy_rows(self._h, row0, nrows, _p(out)))
@@ -238,8 +259,9 @@ class Context:
         return out[0] if err.ndim == 1 else out
 
     def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, mixed=False, conserve=False,
-            max_dense=0, quiet=True, raise_on_fail=True):
-        """Returns (f_last, dense[(batch,) max_dense, n], stats list)."""
+            max_dense=0, quiet=True, raise_on_fail=True, h_cap=0):
+        """Returns (f_last, dense[(batch,) max_dense, n], stats) and, with
+        h_cap > 0, the accepted step sizes per member as a 4th element."""
         f0 = np.ascontiguousarray(f0, np.float64)
         batch = 1 if f0.ndim == 1 else f0.shape[1]
         f = np.ascontiguousarray(f0.T if f0.ndim == 2 else f0).copy()
@@ -247,22 +269,60 @@ class Context:
         st = (StepStats * batch)()
         bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
                                                               np.float64)
+        hl = np.zeros((batch, h_cap)) if h_cap else None
         flags = ((TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
                  | (TG_QUIET if quiet else 0))
         rc = load().tg_dp5_integrate(self._h, _p(f), self.n, batch, t0, t1, dense_dt, rtol, atol,
                                      None if bn is None else _p(bn), flags,
-                                     None if dense is None else _p(dense), max_dense, st)
+                                     None if dense is None else _p(dense), max_dense, st,
+                                     None if hl is None else _p(hl), h_cap)
         if rc != TG_OK and raise_on_fail:
             self._check(rc)
         stats = [s.as_dict() for s in st]
+        steps = None
+        if hl is not None:
+            steps = [hl[b, : min(stats[b]["accepted"], h_cap)].copy() for b in range(batch)]
         if f0.ndim == 1:
-            return f, (None if dense is None else dense[0]), stats[0]
-        return f.T, dense, stats
+            out = (f, (None if dense is None else dense[0]), stats[0])
+            return out + ((steps[0],) if steps is not None else ())
+        out = (f.T, dense, stats)
+        return out + ((steps,) if steps is not None else ())
+
+    def dp5_replay(self, f0, t0, t1, dense_dt, steps, bath_n=None, mixed=False, conserve=False,
+                   max_dense=0):
+        """Fixed-step DP5 on per-member accepted step lists; returns (f_last, dense)."""
+        f0 = np.ascontiguousarray(f0, np.float64)
+        batch = 1 if f0.ndim == 1 else f0.shape[1]
+        if f0.ndim == 1:
+            steps = [steps]
+        f = np.ascontiguousarray(f0.T if f0.ndim == 2 else f0).copy()
+        cap = max(1, max(len(s) for s in steps))
+        H = np.zeros((batch, cap))
+        ns = np.zeros(batch, np.int32)
+        for b, s in enumerate(steps):
+            H[b, : len(s)] = s
+            ns[b] = len(s)
+        dense = np.zeros((batch, max_dense, self.n)) if max_dense else None
+        bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
+                                                              np.float64)
+        flags = (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+        self._check(load().tg_dp5_replay(self._h, _p(f), self.n, batch, t0, t1, dense_dt, _p(H),
+                                         _p(ns), cap, None if bn is None else _p(bn), flags,
+                                         None if dense is None else _p(dense), max_dense))
+        if f0.ndim == 1:
+            return f, (None if dense is None else dense[0])
+        return f.T, dense
 
     # ---- hot path, device pointers (e.g. torch tensors' data_ptr())
+    def last_kernel_ms(self):
+        v = C.c_double()
+        self._check(load().tg_last_kernel_ms(self._h, C.byref(v)))
+        return v.value
+
     def apply_device(self, f_ptr, I_ptr, batch=1, bath_ptr=None, mixed=False, conserve=False,
-                     bath_host=None):
-        flags = TG_PTR_DEVICE | (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+                     bath_host=None, time_kernel=False):
+        flags = (TG_PTR_DEVICE | (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+                 | (TG_TIME_KERNEL if time_kernel else 0))
         bn = None
         if bath_host is not None:
             bn = np.ascontiguousarray(np.broadcast_to(bath_host, (batch,)), np.float64)

@@ -204,6 +204,19 @@ __global__ void kernel_entries(const double* __restrict__ R, const double* __res
     K[i * ld + j] = s;  // padding columns hold exact zeros
 }
 
+// nonzero count per row of K (for throughput accounting)
+__global__ void nnz_rows_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+                                unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sh[4];
+    const int64_t i = blockIdx.x;
+    unsigned long long c = 0;
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) c += K[i * ld + j] != 0.0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
 }  // namespace tg
 
 using namespace tg;
@@ -279,6 +292,20 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
             TG_TRY(plain_products_device(c, c->d_Q, c->d_rk, c->d_ck));
             TG_HIP(c, hipStreamSynchronize(c->stream));
         }
+        {
+            unsigned long long* d_cnt = nullptr;
+            TG_TRY(dev_alloc(c, &d_cnt, (size_t)c->n));
+            hipLaunchKernelGGL(nnz_rows_kernel, dim3(c->n), dim3(256), 0, c->stream, c->d_K, c->n,
+                               c->ld, d_cnt);
+            TG_CHECK_LAUNCH(c);
+            std::vector<unsigned long long> cnt(c->n);
+            TG_HIP(c, hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned long long) * c->n,
+                                     hipMemcpyDeviceToHost, c->stream));
+            TG_HIP(c, hipStreamSynchronize(c->stream));
+            dev_free(&d_cnt);
+            c->nnz = 0;
+            for (auto v : cnt) c->nnz += (int64_t)v;
+        }
         TG_HIP(c, hipStreamSynchronize(c->stream));
         // the projection/convolution scratch is only needed for assembly
         dev_free(&c->d_Q);
@@ -296,5 +323,21 @@ extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows,
                                sizeof(double) * c->ld, sizeof(double) * c->n, nrows,
                                hipMemcpyDeviceToHost, c->stream));
     TG_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+extern "C" tg_status tg_kernel_stats_get(tg_ctx* c, tg_kernel_stats* out) {
+    if (!c || !out) return TG_ERR_ARG;
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
+    *out = tg_kernel_stats{};
+    out->n = c->n;
+    out->nnz = c->nnz;
+    out->window = c->maxw;
+    std::vector<double> rk(c->n);
+    TG_HIP(c, hipMemcpyAsync(rk.data(), c->d_rk, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    for (double v : rk) out->kmax = v > out->kmax ? v : out->kmax;
+    mixed_stats(c, out);
     return TG_OK;
 }

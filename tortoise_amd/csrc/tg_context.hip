@@ -20,6 +20,21 @@ tg_status set_error(tg_ctx* c, tg_status code, const char* fmt, ...) {
     return code;
 }
 
+void time_begin(tg_ctx* c) {
+    if (!c->time_kernel) return;
+    if (!c->ev_k0) {
+        hipEventCreate(&c->ev_k0);
+        hipEventCreate(&c->ev_k1);
+    }
+    hipEventRecord(c->ev_k0, c->stream);
+}
+
+void time_end(tg_ctx* c) {
+    if (!c->time_kernel || !c->ev_k1) return;
+    hipEventRecord(c->ev_k1, c->stream);
+    c->timed = true;
+}
+
 tg_status ensure_staging(tg_ctx* c, int32_t batch) {
     if (batch < 1 || batch > c->max_batch)
         return set_error(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
@@ -288,6 +303,8 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_I);
     dev_free(&c->d_red);
     if (c->h_pinned) hipHostFree(c->h_pinned);
+    if (c->ev_k0) hipEventDestroy(c->ev_k0);
+    if (c->ev_k1) hipEventDestroy(c->ev_k1);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -346,4 +363,14 @@ extern "C" tg_status tg_relative_error(tg_ctx* c, const double* err, const doubl
         d_p = c->d_I;
     }
     return relative_error_device(c, d_e, d_p, batch, rtol, atol, out);
+}
+
+extern "C" tg_status tg_last_kernel_ms(tg_ctx* c, double* ms) {
+    if (!c || !ms) return TG_ERR_ARG;
+    if (!c->timed) return set_error(c, TG_ERR_STATE, "no apply with TG_TIME_KERNEL yet");
+    TG_HIP(c, hipEventSynchronize(c->ev_k1));
+    float v = 0.f;
+    TG_HIP(c, hipEventElapsedTime(&v, c->ev_k0, c->ev_k1));
+    *ms = v;
+    return TG_OK;
 }

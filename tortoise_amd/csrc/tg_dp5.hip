@@ -186,10 +186,13 @@ struct MemberState {
 
 using namespace tg;
 
-extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B, double t0,
-                                      double t1, double dense_dt, double rtol, double atol,
-                                      const double* bath_n, int32_t flags, double* dense_out,
-                                      int32_t max_dense, tg_step_stats* stats) {
+// Shared driver: adaptive when `sched` is null, otherwise member b takes the
+// steps sched[b*h_cap .. + nsched[b]) and every attempt is accepted.
+static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, double t1,
+                         double dense_dt, double rtol, double atol, const double* bath_n,
+                         int32_t flags, double* dense_out, int32_t max_dense,
+                         tg_step_stats* stats, double* h_acc, int32_t h_cap,
+                         const double* sched, const int32_t* nsched) {
     if (!c || !f) return TG_ERR_ARG;
     if (n != c->n) return set_error(c, TG_ERR_ARG, "n mismatch");
     if (B < 1 || B > c->max_batch) return set_error(c, TG_ERR_ARG, "batch outside [1, max_batch]");
@@ -258,11 +261,19 @@ extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B
         return TG_OK;
     };
 
+    const bool replay = sched != nullptr;
+    std::vector<int32_t> step_idx(B, 0);
+    auto is_active = [&](int b) {
+        const MemberState& m = ms[b];
+        if (replay) return step_idx[b] < nsched[b] && m.t < t1;
+        return m.t < t1 && !m.failed;
+    };
     while (true) {
         bool any = false;
         for (int b = 0; b < B; ++b) {
             MemberState& m = ms[b];
-            const bool active = m.t < t1 && !m.failed;
+            const bool active = is_active(b);
+            if (active && replay) m.h = sched[(int64_t)b * h_cap + step_idx[b]];
             hh[b] = active ? m.h : 0.0;
             any |= active;
         }
@@ -294,13 +305,15 @@ extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B
             hipLaunchKernelGGL(error_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, w->y5, w->err);
             TG_CHECK_LAUNCH(c);
         }
-        TG_TRY(relative_error_device(c, w->err, w->y5, B, rtol, atol, en.data()));
+        if (replay) std::fill(en.begin(), en.end(), 0.0);
+        else TG_TRY(relative_error_device(c, w->err, w->y5, B, rtol, atol, en.data()));
 
         pts.clear();
         for (int b = 0; b < B; ++b) {
             MemberState& m = ms[b];
             acc[b] = 0;
-            if (!(m.t < t1 && !m.failed)) continue;
+            if (!is_active(b)) continue;
+            if (replay) ++step_idx[b];
             m.st.attempted += 1;
             m.st.evals += 6;
             const double e = en[b];
@@ -336,6 +349,7 @@ extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B
                     m.st.n_dense += 1;
                     m.dense_time += dense_dt;
                 }
+                if (h_acc && m.st.accepted < h_cap) h_acc[(int64_t)b * h_cap + m.st.accepted] = h;
                 m.t += h;
                 acc[b] = 1;
                 double scale;
@@ -395,4 +409,26 @@ extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B
     if (rc != TG_OK)
         set_error(c, rc, rc == TG_ERR_NONFINITE ? "non-finite error norm" : "step size underflow");
     return rc;
+}
+
+extern "C" tg_status tg_dp5_integrate(tg_ctx* c, double* f, int64_t n, int32_t B, double t0,
+                                      double t1, double dense_dt, double rtol, double atol,
+                                      const double* bath_n, int32_t flags, double* dense_out,
+                                      int32_t max_dense, tg_step_stats* stats, double* h_accepted,
+                                      int32_t h_cap) {
+    if (h_accepted && h_cap < 1) return TG_ERR_ARG;
+    return dp5_run(c, f, n, B, t0, t1, dense_dt, rtol, atol, bath_n, flags, dense_out, max_dense,
+                   stats, h_accepted, h_cap, nullptr, nullptr);
+}
+
+extern "C" tg_status tg_dp5_replay(tg_ctx* c, double* f, int64_t n, int32_t B, double t0,
+                                   double t1, double dense_dt, const double* h_steps,
+                                   const int32_t* n_steps, int32_t h_cap, const double* bath_n,
+                                   int32_t flags, double* dense_out, int32_t max_dense) {
+    if (!c || !h_steps || !n_steps || h_cap < 1) return TG_ERR_ARG;
+    for (int b = 0; b < B; ++b)
+        if (n_steps[b] < 0 || n_steps[b] > h_cap)
+            return set_error(c, TG_ERR_ARG, "n_steps[%d] outside [0, h_cap]", b);
+    return dp5_run(c, f, n, B, t0, t1, dense_dt, 1.0, 1.0, bath_n, flags | TG_QUIET, dense_out,
+                   max_dense, nullptr, nullptr, 1, h_steps, n_steps);
 }

@@ -65,6 +65,7 @@ struct tg_ctx {
     double* d_K = nullptr;         // n x ld, FP64
     double* d_rk = nullptr;        // K 1
     double* d_ck = nullptr;        // K^T 1
+    int64_t nnz = 0;               // structurally nonzero entries of K
     bool assembled = false;
 
     // plain FP64 apply scratch
@@ -84,6 +85,10 @@ struct tg_ctx {
     double* d_red = nullptr;       // reduction partials
     double* h_pinned = nullptr;    // pinned host scalars
     int32_t red_cap = 0;
+
+    hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;  // dominant-kernel timing
+    bool time_kernel = false;
+    bool timed = false;
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -143,7 +148,11 @@ tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);
 void mixed_destroy(tg_ctx* c);
+void mixed_stats(tg_ctx* c, tg_kernel_stats* out);
 void dp5_destroy(tg_ctx* c);
 tg_status ensure_staging(tg_ctx* c, int32_t batch);
+// event bracket of the dominant kernel when TG_TIME_KERNEL is active
+void time_begin(tg_ctx* c);
+void time_end(tg_ctx* c);
 
 }  // namespace tg

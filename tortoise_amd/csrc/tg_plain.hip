@@ -182,8 +182,10 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
 }
 
 static tg_status launch_dual(tg_ctx* c, const double* d_f) {
+    time_begin(c);
     hipLaunchKernelGGL(plain_dual_kernel, dim3(c->n_colchk, c->n_rowblk), dim3(kPlainThreads), 0,
                        c->stream, c->d_K, c->n, c->ld, d_f, c->d_upart, c->d_vpart);
+    time_end(c);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }
@@ -239,6 +241,7 @@ extern "C" tg_status tg_collision_apply(tg_ctx* c, const double* f, double* I, i
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "tg_kernel_assemble not called");
     const bool dev = flags & TG_PTR_DEVICE;
     const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
+    c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
     const double* d_f = f;
     double* d_I = I;
     if (!dev) {
@@ -256,6 +259,7 @@ extern "C" tg_status tg_collision_apply(tg_ctx* c, const double* f, double* I, i
             TG_TRY(plain_apply_device(c, d_f + (int64_t)b * n, d_I + (int64_t)b * n, nb, conserve));
         }
     }
+    c->time_kernel = false;
     if (!dev) {
         TG_HIP(c, hipMemcpyAsync(I, d_I, sizeof(double) * n * batch, hipMemcpyDeviceToHost,
                                  c->stream));

@@ -1,0 +1,57 @@
+// Hot-electron thermalisation on a square zone, written against the
+// Tortoise:: host API the way the reference's tutorials are (compare
+// tutorials/TORTOISE_Workshop/Thermalisation/main.cpp): one parabolic
+// electron band, a dispersionless optical phonon band held as a bath, the
+// e-ph channel, DP5 time propagation with dense output, and the number /
+// energy sums of every dense step.  Runs the collision operator on the GPU.
+//
+// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1]
+#include <TORTOISE>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+using namespace Tortoise;
+
+int main(int argc, char** argv) {
+    const int res = argc > 1 ? std::atoi(argv[1]) : 32;
+    const Real tFinal = argc > 2 ? std::atof(argv[2]) : 100.0;
+    const bool quiet = argc > 3 ? std::atoi(argv[3]) != 0 : true;
+    const Real pi = M_PI;
+
+    Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
+    Mesh<2> mesh(zone, Point<2>({0., 0.}), Point<2>({1., 1.}), {res, res});
+    Material<2> mat("SquareParabolic", zone);
+    mat.addBand("el:C", -1, fermion, mesh, [](Point<2> k) { return 0.5 * k.squaredNorm(); });
+    mat.addBand("pn:opt", 0, boson, mesh, 0.05);
+    for (auto sc : mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 0)) {
+        sc->amplitude = 0.1 * 0.1;      // g^2
+        sc->deltaBroadening = 0.01;
+    }
+    mat["el:C"].relativeErrorTolerance = 1e-8;
+    mat["el:C"].absoluteErrorTolerance = 1e-12;
+
+    const Real T = 0.02, mu = 0.5;
+    MaterialStatus<2> start(mat);
+    start.setToEquilibrium(T, mu, 0.0);   // phonon bath at n_BE(omega0, T)
+    start["el:C"] += Function<2>(mesh, [](Point<2> k) {
+        const Real e = 0.5 * k.squaredNorm();
+        return 0.2 * std::exp(-0.5 * std::pow((e - 0.9) / 0.03, 2));
+    });
+
+    MaterialTimeStatus<2> dyn(start);
+    dyn.quiet = quiet;
+    dyn.propagateDeterministic(tFinal, tFinal / 100.0);
+
+    std::printf("steps %lld accepted %lld rejected %lld\n", (long long)dyn.lastStats.attempted,
+                (long long)dyn.lastStats.accepted, (long long)dyn.lastStats.rejected);
+    std::printf("t | electron number | electron energy\n");
+    for (int i = 0; i < dyn.size(); i += dyn.size() > 10 ? dyn.size() / 10 : 1)
+        std::printf("%.4f | %.15e | %.15e\n", dyn.arg(i), dyn[i]["el:C"].integrate(),
+                    dyn[i]["el:C"].integrate(mat["el:C"].dispersion));
+    const Function<2>& fEnd = dyn.back()["el:C"];
+    std::printf("final %.4f | %.15e | %.15e\n", dyn.argback(), fEnd.integrate(),
+                fEnd.integrate(mat["el:C"].dispersion));
+    return 0;
+}

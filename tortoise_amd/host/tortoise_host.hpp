@@ -1,0 +1,523 @@
+// tortoise_host.hpp -- C++ host layer with the reference's Tortoise:: types,
+// backed by the MI355X collision operator of libtortoise_gpu.so (C ABI in
+// include/tortoise_gpu.h).
+//
+// Mirrors, for the path the GPU backend evaluates (one electron band on a
+// structured box mesh, e-ph scattering with a dispersionless phonon bath):
+//   Region<NDim>              include/Geometry/Structured/DirectSpace/Region.hpp:60-111
+//   Mesh<NDim>                include/Geometry/Structured/DirectSpace/Mesh.hpp:73-230
+//   Function<NDim>            include/Geometry/Structured/FunctionSpace/Function.hpp:67-262
+//   Band<NDim>                include/Physics/Material/Band.hpp:49-113
+//   ScatteringChannel<NDim,3> include/Physics/Scattering/ScatteringChannel.hpp:54-192
+//   Material<NDim>            include/Physics/Material/Material.hpp:58-188
+//   MaterialStatus<NDim>      include/Physics/Status/MaterialStatus.hpp:46-140
+//   MaterialTimeStatus<NDim>  include/Physics/Status/MaterialTimeStatus.hpp:55-90
+//
+// Data model kept from the reference: a Function stores, per element, the
+// LinearForm [f(v0), f(v1)-f(v0), ...] (element-major, Function.hpp:96,
+// :326-344), vertex order and element numbering of Mesh.cpp:55-87,:384-433.
+// The backend works on nodal values of the (nsec+1)^NDim vertex lattice; the
+// DG -> nodal gather reads c0 (+ cj) of any element touching the node, which
+// is exact for the continuous fields Function(mesh, lambda) builds
+// (Function.cpp:60-68); nodal -> DG writes c0 = g(v0), cj = g(vj) - g(v0).
+//
+// Errors from the backend are thrown as std::runtime_error with the
+// library's message.  Plotting is not provided (no gnuplot pipes).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <functional>
+#include <initializer_list>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/tortoise_gpu.h"
+
+namespace Tortoise {
+
+using Real = double;
+constexpr Real fermion = -1.0;   // Band statistics (Band.hpp:58)
+constexpr Real boson = 1.0;
+constexpr int in = 1;            // leg directions (Geometry.hpp:119)
+constexpr int out = -1;
+
+// ------------------------------------------------------------------- Point
+template <int NDim>
+struct Point {
+    std::array<Real, NDim> c{};
+    Point() = default;
+    Point(std::initializer_list<Real> v) {
+        int i = 0;
+        for (Real x : v) if (i < NDim) c[i++] = x;
+    }
+    Real& operator()(int i) { return c[i]; }
+    Real operator()(int i) const { return c[i]; }
+    Real squaredNorm() const { Real s = 0; for (Real x : c) s += x * x; return s; }
+    Real norm() const { return std::sqrt(squaredNorm()); }
+    Point operator+(const Point& o) const { Point r; for (int i = 0; i < NDim; ++i) r.c[i] = c[i] + o.c[i]; return r; }
+    Point operator-(const Point& o) const { Point r; for (int i = 0; i < NDim; ++i) r.c[i] = c[i] - o.c[i]; return r; }
+    Point operator*(Real s) const { Point r; for (int i = 0; i < NDim; ++i) r.c[i] = c[i] * s; return r; }
+};
+template <int NDim>
+Point<NDim> operator*(Real s, const Point<NDim>& p) { return p * s; }
+template <int NDim>
+Real dot(const Point<NDim>& a, const Point<NDim>& b) { Real s = 0; for (int i = 0; i < NDim; ++i) s += a(i) * b(i); return s; }
+
+inline int factorial(int d) { int r = 1; for (int i = 2; i <= d; ++i) r *= i; return r; }
+
+// ------------------------------------------------------------------ Region
+template <int NDim>
+class Region {
+public:
+    Point<NDim> origin;
+    std::array<Point<NDim>, NDim> gVec;
+    // The backend meshes axis-aligned boxes: gVec must be diagonal.
+    Region(const Point<NDim>& t_origin, const std::array<Point<NDim>, NDim>& t_gVec)
+        : origin(t_origin), gVec(t_gVec) {
+        for (int a = 0; a < NDim; ++a)
+            for (int d = 0; d < NDim; ++d)
+                if (a != d && gVec[a](d) != 0.0)
+                    throw std::invalid_argument("Region: only axis-aligned zones are supported");
+    }
+};
+
+// -------------------------------------------------------------------- Mesh
+template <int NDim>
+class Mesh {
+public:
+    static constexpr int nVertElement = NDim + 1;
+    const int nElementSection = factorial(NDim);
+    const Region<NDim>* region;
+    Point<NDim> origin;          // physical origin of the mesh
+    Point<NDim> extent;          // physical edge lengths
+    std::array<int, NDim> nSecSplits;
+    int numberSections, numberElements, dataVectorDim;
+    Real elemVolume;
+
+    // Mesh.cpp:268-293: origin = region.origin + gVec * relOrigin; gVec scaled by relSize
+    Mesh(const Region<NDim>& t_region, const Point<NDim>& relOrigin, const Point<NDim>& relSize,
+         const std::array<int, NDim>& t_nSecSplits)
+        : region(&t_region), nSecSplits(t_nSecSplits) {
+        numberSections = 1;
+        Real vol = 1.0;
+        for (int d = 0; d < NDim; ++d) {
+            origin(d) = t_region.origin(d) + t_region.gVec[d](d) * relOrigin(d);
+            extent(d) = t_region.gVec[d](d) * relSize(d);
+            numberSections *= nSecSplits[d];
+            vol *= extent(d) / nSecSplits[d];
+        }
+        numberElements = numberSections * nElementSection;
+        dataVectorDim = numberElements * nVertElement;
+        elemVolume = std::fabs(vol / nElementSection);
+    }
+    Mesh(const Mesh&) = delete;
+    Mesh(Region<NDim>&&, const Point<NDim>&, const Point<NDim>&, const std::array<int, NDim>&) = delete;
+
+    int64_t numberNodes() const {
+        int64_t n = 1;
+        for (int d = 0; d < NDim; ++d) n *= nSecSplits[d] + 1;
+        return n;
+    }
+    Point<NDim> node(int64_t id) const {
+        Point<NDim> p;
+        for (int d = 0; d < NDim; ++d) {
+            int64_t i = id % (nSecSplits[d] + 1);
+            id /= (nSecSplits[d] + 1);
+            p(d) = origin(d) + (extent(d) / nSecSplits[d]) * (Real)i;
+        }
+        return p;
+    }
+    // node id of vertex `vert` of element `elem` (Mesh.cpp:55-87, :412-420)
+    int64_t elementNode(int elem, int vert) const {
+        int s = elem / nElementSection, t = elem % nElementSection;
+        int sc[3] = {0, 0, 0};
+        for (int d = 0; d < NDim; ++d) { sc[d] = s % nSecSplits[d]; s /= nSecSplits[d]; }
+        int avail[3] = {0, 1, 2}, navail = NDim, rem = t, mask = 0;
+        for (int k = 0; k < vert; ++k) {
+            int f = factorial(NDim - 1 - k), idx = rem / f;
+            rem %= f;
+            mask |= 1 << avail[idx];
+            for (int q = idx; q < navail - 1; ++q) avail[q] = avail[q + 1];
+            --navail;
+        }
+        int64_t id = 0, stride = 1;
+        for (int d = 0; d < NDim; ++d) {
+            id += stride * (sc[d] + ((mask >> d) & 1));
+            stride *= nSecSplits[d] + 1;
+        }
+        return id;
+    }
+    Point<NDim> operator()(int elem, int vert) const { return node(elementNode(elem, vert)); }
+    Point<NDim> elemCentre(int elem) const {
+        Point<NDim> p;
+        for (int v = 0; v < nVertElement; ++v) p = p + (*this)(elem, v);
+        return p * (1.0 / nVertElement);
+    }
+};
+
+// ---------------------------------------------------------------- Function
+template <int NDim>
+class Function {
+public:
+    const Mesh<NDim>* mesh;
+    std::vector<Real> vec;   // per element [f(v0), f(v1)-f(v0), ...]
+
+    explicit Function(const Mesh<NDim>& m) : mesh(&m), vec(m.dataVectorDim, 0.0) {}
+    Function(const Mesh<NDim>& m, Real value) : mesh(&m), vec(m.dataVectorDim, 0.0) {
+        for (int e = 0; e < m.numberElements; ++e) vec[e * m.nVertElement] = value;
+    }
+    Function(const Mesh<NDim>& m, const std::function<Real(Point<NDim>)>& f) : mesh(&m), vec(m.dataVectorDim) {
+        for (int e = 0; e < m.numberElements; ++e)
+            for (int v = 0; v < m.nVertElement; ++v) vec[e * m.nVertElement + v] = f(m(e, v));
+        toLinFormFromNodal();
+    }
+    static Function fromNodal(const Mesh<NDim>& m, const std::vector<Real>& nodal) {
+        Function r(m);
+        for (int e = 0; e < m.numberElements; ++e)
+            for (int v = 0; v < m.nVertElement; ++v)
+                r.vec[e * m.nVertElement + v] = nodal[m.elementNode(e, v)];
+        r.toLinFormFromNodal();
+        return r;
+    }
+    // nodal values on the vertex lattice (exact for continuous fields)
+    std::vector<Real> toNodal() const {
+        std::vector<Real> out(mesh->numberNodes(), 0.0);
+        const int nv = mesh->nVertElement;
+        for (int e = 0; e < mesh->numberElements; ++e)
+            for (int v = 0; v < nv; ++v)
+                out[mesh->elementNode(e, v)] = vec[e * nv] + (v ? vec[e * nv + v] : 0.0);
+        return out;
+    }
+    void toLinFormFromNodal() {
+        const int nv = mesh->nVertElement;
+        for (int e = 0; e < mesh->numberElements; ++e)
+            for (int v = 1; v < nv; ++v) vec[e * nv + v] -= vec[e * nv];
+    }
+    void toNodalFromLinForm() {
+        const int nv = mesh->nVertElement;
+        for (int e = 0; e < mesh->numberElements; ++e)
+            for (int v = 1; v < nv; ++v) vec[e * nv + v] += vec[e * nv];
+    }
+    // Function.cpp:123-234: linear ops on coefficients, nonlinear ops nodal
+    Function& operator+=(const Function& o) { for (size_t i = 0; i < vec.size(); ++i) vec[i] += o.vec[i]; return *this; }
+    Function& operator-=(const Function& o) { for (size_t i = 0; i < vec.size(); ++i) vec[i] -= o.vec[i]; return *this; }
+    Function& operator*=(Real s) { for (auto& x : vec) x *= s; return *this; }
+    Function& operator+=(Real s) { for (int e = 0; e < mesh->numberElements; ++e) vec[e * mesh->nVertElement] += s; return *this; }
+    Function& operator*=(const Function& o) { return nodalOp(o, [](Real a, Real b) { return a * b; }); }
+    Function& operator/=(const Function& o) { return nodalOp(o, [](Real a, Real b) { return a / b; }); }
+    Function operator+(const Function& o) const { Function r(*this); r += o; return r; }
+    Function operator-(const Function& o) const { Function r(*this); r -= o; return r; }
+    Function operator*(const Function& o) const { Function r(*this); r *= o; return r; }
+    Function operator*(Real s) const { Function r(*this); r *= s; return r; }
+    friend Function operator*(Real s, const Function& f) { return f * s; }
+    Function map(const std::function<Real(Real)>& g) const {
+        Function r(*this);
+        r.toNodalFromLinForm();
+        for (auto& x : r.vec) x = g(x);
+        r.toLinFormFromNodal();
+        return r;
+    }
+    // Function.cpp:369-372
+    Real integrate() const {
+        Real s = 0, s0 = 0;
+        const int nv = mesh->nVertElement;
+        for (int e = 0; e < mesh->numberElements; ++e) {
+            s0 += vec[e * nv];
+            for (int v = 0; v < nv; ++v) s += vec[e * nv + v];
+        }
+        return mesh->elemVolume * (s + NDim * s0) / (NDim + 1);
+    }
+    // Function.cpp:373-375 (order-2 Gauss is exact for P1*P1)
+    Real integrate(const Function& rhs) const {
+        const int nv = mesh->nVertElement;
+        Real tot = 0;
+        for (int e = 0; e < mesh->numberElements; ++e) {
+            Real a[4], b[4], sa = 0, sb = 0, sab = 0;
+            for (int v = 0; v < nv; ++v) {
+                a[v] = vec[e * nv] + (v ? vec[e * nv + v] : 0.0);
+                b[v] = rhs.vec[e * nv] + (v ? rhs.vec[e * nv + v] : 0.0);
+                sa += a[v];
+                sb += b[v];
+                sab += a[v] * b[v];
+            }
+            tot += sab + sa * sb;
+        }
+        return mesh->elemVolume * tot / ((NDim + 1) * (NDim + 2));
+    }
+
+private:
+    Function& nodalOp(const Function& o, Real (*op)(Real, Real)) {
+        toNodalFromLinForm();
+        Function b(o);
+        b.toNodalFromLinForm();
+        for (size_t i = 0; i < vec.size(); ++i) vec[i] = op(vec[i], b.vec[i]);
+        toLinFormFromNodal();
+        return *this;
+    }
+};
+
+// -------------------------------------------------------------------- Band
+template <int NDim>
+class Band {
+public:
+    std::string name;
+    Real charge, statistics;
+    const Mesh<NDim>* mesh;
+    Function<NDim> dispersion;
+    bool constrained = false;
+    std::function<Function<NDim>(Real)> constrainedPopulation;
+    Real absoluteErrorTolerance = 1.e-3;         // Band.hpp:77-82
+    Real relativeErrorTolerance = 1.e-3;
+    Real penalisationUnphysicalPopulation = 0.;
+    Band(std::string n, Real ch, Real st, const Mesh<NDim>& m, const Function<NDim>& disp)
+        : name(std::move(n)), charge(ch), statistics(st), mesh(&m), dispersion(disp) {}
+};
+
+// ------------------------------------------------------ ScatteringChannel
+// The e-ph channel {electron in, phonon in, electron out} with the phonon
+// band acting as a bath of a dispersionless mode: the W-form the backend
+// evaluates.  `amplitude` is g^2 (ScatteringChannel.hpp:70); the broadening
+// and the energy grid of the overlap quadrature are new parameters.
+template <int NDim>
+class ScatteringChannel {
+public:
+    std::string electronBand, phononBand;
+    Real amplitude = 1.0;
+    Real deltaBroadening = 0.01;
+    Real energyGridStep = 0.0;      // <= 0: deltaBroadening / 8
+    Real deltaCutoff = 8.0;         // in units of deltaBroadening
+    int numMCPoints = 50;           // accepted for source compatibility; unused
+    Real minimumDeterminant = 1e-9; // idem
+};
+
+template <int NDim> class MaterialStatus;
+
+// ---------------------------------------------------------------- Material
+template <int NDim>
+class Material {
+public:
+    std::string name;
+    const Region<NDim>* region;
+    std::deque<Band<NDim>> bands;                       // insertion order (StringMap.hpp:78-79)
+    std::deque<ScatteringChannel<NDim>> channels;
+
+    Material(std::string n, const Region<NDim>& r) : name(std::move(n)), region(&r) {}
+    Material(const Material&) = delete;
+    ~Material() { if (ctx_) tg_ctx_destroy(ctx_); }
+
+    void addBand(const std::string& n, Real charge, Real stat, const Mesh<NDim>& m,
+                 const std::function<Real(Point<NDim>)>& disp) {
+        bands.emplace_back(n, charge, stat, m, Function<NDim>(m, disp));
+        invalidate();
+    }
+    void addBand(const std::string& n, Real charge, Real stat, const Mesh<NDim>& m, Real disp) {
+        bands.emplace_back(n, charge, stat, m, Function<NDim>(m, disp));
+        invalidate();
+    }
+    Band<NDim>& operator[](const std::string& n) {
+        for (auto& b : bands) if (b.name == n) return b;
+        throw std::out_of_range("no band " + n);
+    }
+    const Band<NDim>& operator[](const std::string& n) const {
+        for (auto& b : bands) if (b.name == n) return b;
+        throw std::out_of_range("no band " + n);
+    }
+    // wildcard match (StringUtilities.cpp:44-59): '*' matches any suffix
+    std::vector<std::string> matchListId(const std::string& pat) const {
+        std::vector<std::string> r;
+        for (auto& b : bands) if (match(pat, b.name)) r.push_back(b.name);
+        return r;
+    }
+    // Material.cpp:244-301 for {el:*, pn:*, el:*} with {in, in, out}
+    template <int NLegs>
+    std::vector<ScatteringChannel<NDim>*> addScatteringChannelsWildcards(
+        const std::array<std::string, NLegs>& legs, const std::array<int, NLegs>& dirs, int nUmklapp) {
+        static_assert(NLegs == 3, "the backend evaluates 3-leg e-ph channels");
+        if (!(dirs[0] == in && dirs[1] == in && dirs[2] == out))
+            throw std::invalid_argument("e-ph channel must be {in, in, out}");
+        std::vector<ScatteringChannel<NDim>*> r;
+        for (auto& e : matchListId(legs[0]))
+            for (auto& p : matchListId(legs[1])) {
+                if ((*this)[e].statistics != fermion || (*this)[p].statistics != boson) continue;
+                channels.push_back(ScatteringChannel<NDim>{});
+                channels.back().electronBand = e;
+                channels.back().phononBand = p;
+                r.push_back(&channels.back());
+            }
+        (void)nUmklapp;  // the W-form needs no umklapp enumeration (phonon takes any q)
+        invalidate();
+        return r;
+    }
+
+    // ---- backend
+    // Context of the (single) e-ph channel; built on first use with the
+    // channel parameters current at that time.
+    tg_ctx* backend(Real bath_n, int max_batch = 1) {
+        if (ctx_ && ctx_bath_ == bath_n) return ctx_;
+        if (channels.size() != 1) throw std::runtime_error("the backend evaluates exactly one e-ph channel");
+        const auto& ch = channels.front();
+        const Band<NDim>& el = (*this)[ch.electronBand];
+        const Band<NDim>& pn = (*this)[ch.phononBand];
+        const auto omega = pn.dispersion.toNodal();
+        for (Real w : omega)
+            if (std::fabs(w - omega[0]) > 1e-12 * std::fabs(omega[0]))
+                throw std::runtime_error("phonon band must be dispersionless");
+        eps_ = el.dispersion.toNodal();
+        tg_problem_desc d{};
+        d.dim = NDim;
+        for (int q = 0; q < 3; ++q) {
+            d.nsec[q] = q < NDim ? el.mesh->nSecSplits[q] : 1;
+            d.origin[q] = q < NDim ? el.mesh->origin(q) : 0.0;
+            d.extent[q] = q < NDim ? el.mesh->extent(q) : 1.0;
+        }
+        d.eps = eps_.data();
+        d.g = std::sqrt(ch.amplitude);
+        d.omega0 = omega[0];
+        d.sigma = ch.deltaBroadening;
+        d.bath_n = bath_n;
+        d.de = ch.energyGridStep;
+        d.cutoff = ch.deltaCutoff;
+        d.max_batch = max_batch;
+        if (ctx_) tg_ctx_destroy(ctx_);
+        ctx_ = nullptr;
+        check(tg_ctx_create(&d, &ctx_));
+        check(tg_kernel_assemble(ctx_, 0));
+        ctx_bath_ = bath_n;
+        return ctx_;
+    }
+    void check(tg_status s) const {
+        if (s != TG_OK) throw std::runtime_error(std::string("tortoise_gpu: ") + tg_last_error(ctx_));
+    }
+
+private:
+    tg_ctx* ctx_ = nullptr;
+    Real ctx_bath_ = -1.0;
+    std::vector<Real> eps_;
+    void invalidate() {
+        if (ctx_) tg_ctx_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+    static bool match(const std::string& pat, const std::string& s) {
+        auto star = pat.find('*');
+        if (star == std::string::npos) return pat == s;
+        return s.compare(0, star, pat, 0, star) == 0;
+    }
+};
+
+// ---------------------------------------------------------- MaterialStatus
+template <int NDim>
+class MaterialStatus {
+public:
+    Material<NDim>* const material;
+    std::map<std::string, Function<NDim>> pop;
+
+    explicit MaterialStatus(Material<NDim>& m) : material(&m) {
+        for (auto& b : m.bands) pop.emplace(b.name, Function<NDim>(*b.mesh));
+    }
+    MaterialStatus(const MaterialStatus&) = default;
+    MaterialStatus& operator=(const MaterialStatus& o) { pop = o.pop; return *this; }
+    Function<NDim>& operator[](const std::string& n) { return pop.at(n); }
+    const Function<NDim>& operator[](const std::string& n) const { return pop.at(n); }
+
+    MaterialStatus& operator+=(const MaterialStatus& o) { for (auto& [k, f] : pop) f += o[k]; return *this; }
+    MaterialStatus& operator*=(Real s) { for (auto& [k, f] : pop) f *= s; return *this; }
+
+    // MaterialStatus.cpp:212-220
+    void setToEquilibrium(Real T, Real muF, Real muB) {
+        for (auto& b : material->bands) {
+            const Real mu = b.statistics == fermion ? muF : muB;
+            const Real s = b.statistics == fermion ? 1.0 : -1.0;
+            pop.at(b.name) = b.dispersion.map([&](Real e) { return 1.0 / (std::exp((e - mu) / T) + s); });
+        }
+    }
+
+    // MaterialStatus.cpp:146-161: d(populations)/dt of every channel.  The
+    // phonon bath is constrained, its derivative is zero.
+    MaterialStatus propagate() const {
+        MaterialStatus r(*material);
+        const auto& ch = material->channels.front();
+        tg_ctx* ctx = material->backend(bathOccupation());
+        const Function<NDim>& fe = (*this)[ch.electronBand];
+        std::vector<Real> f = fe.toNodal(), I(f.size());
+        material->check(tg_collision_apply(ctx, f.data(), I.data(), (int64_t)f.size(), 1, nullptr, 0));
+        r[ch.electronBand] = Function<NDim>::fromNodal(*fe.mesh, I);
+        return r;
+    }
+    // MaterialStatus.cpp:189-196 with Band::relativeError (Band.cpp:87-101)
+    Real relativeError(const MaterialStatus& population) const {
+        const auto& ch = material->channels.front();
+        tg_ctx* ctx = material->backend(population.bathOccupation());
+        const Band<NDim>& b = (*material)[ch.electronBand];
+        std::vector<Real> e = (*this)[ch.electronBand].toNodal(), p = population[ch.electronBand].toNodal();
+        Real out = 0;
+        material->check(tg_relative_error(ctx, e.data(), p.data(), (int64_t)e.size(), 1,
+                                          b.relativeErrorTolerance, b.absoluteErrorTolerance, &out, 0));
+        return out;
+    }
+    // occupation of the phonon bath: the (uniform) population of the phonon band
+    Real bathOccupation() const {
+        const auto& ch = material->channels.front();
+        const auto v = (*this)[ch.phononBand].toNodal();
+        return v.empty() ? 0.0 : v[0];
+    }
+};
+
+// ------------------------------------------------------ MaterialTimeStatus
+template <int NDim>
+class MaterialTimeStatus {
+public:
+    Material<NDim>* const material;
+    std::vector<Real> argVec;
+    std::vector<MaterialStatus<NDim>> valVec;
+    bool quiet = false;
+    tg_step_stats lastStats{};
+
+    explicit MaterialTimeStatus(const MaterialStatus<NDim>& s) : material(s.material) {
+        argVec.push_back(0.0);
+        valVec.push_back(s);
+    }
+    int size() const { return (int)argVec.size(); }
+    Real arg(int i) const { return argVec[i]; }
+    Real argback() const { return argVec.back(); }
+    MaterialStatus<NDim>& operator[](int i) { return valVec[i]; }
+    MaterialStatus<NDim>& back() { return valVec.back(); }
+
+    // MaterialTimeStatus.cpp:102-137 -> DormandPrince54.hpp:479-665: adaptive
+    // DP5(4) from the last entry to finalTime; every dense output is appended.
+    void propagate(Real finalTime, Real denseStep) {
+        const MaterialStatus<NDim> start = valVec.back();
+        const auto& ch = material->channels.front();
+        const Band<NDim>& el = (*material)[ch.electronBand];
+        tg_ctx* ctx = material->backend(start.bathOccupation());
+        std::vector<Real> f = start[ch.electronBand].toNodal();
+        const int64_t n = (int64_t)f.size();
+        const Real t0 = argVec.back();
+        const int maxDense = (int)std::floor((finalTime - t0) / denseStep + 1e-9) + 1;
+        std::vector<Real> dense((size_t)maxDense * n);
+        tg_status s = tg_dp5_integrate(ctx, f.data(), n, 1, t0, finalTime, denseStep,
+                                       el.relativeErrorTolerance, el.absoluteErrorTolerance, nullptr,
+                                       quiet ? TG_QUIET : 0, dense.data(), maxDense, &lastStats,
+                                       nullptr, 0);
+        if (s != TG_OK && s != TG_ERR_STEP_UNDERFLOW) material->check(s);
+        if (s == TG_ERR_STEP_UNDERFLOW) std::printf("The step size has become too small to be handled\n");
+        const int nd = lastStats.n_dense < maxDense ? lastStats.n_dense : maxDense;
+        for (int k = 0; k < nd; ++k) {
+            MaterialStatus<NDim> st = start;
+            st[ch.electronBand] = Function<NDim>::fromNodal(
+                *el.mesh, std::vector<Real>(dense.begin() + (size_t)k * n, dense.begin() + (size_t)(k + 1) * n));
+            argVec.push_back(t0 + (k + 1) * denseStep);
+            valVec.push_back(std::move(st));
+        }
+    }
+    // the name the reference's examples call (examples/main.cpp:99)
+    void propagateDeterministic(Real finalTime, Real denseStep) { propagate(finalTime, denseStep); }
+};
+
+}  // namespace Tortoise

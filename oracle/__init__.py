@@ -94,6 +94,15 @@ def lib():
                                               _f64p, C.c_double, C.c_double, C.c_double, C.c_int,
                                               _f64p, _i8p, C.c_int64, C.c_void_p, C.c_void_p,
                                               C.c_int32]
+        L.or_kernel_domega_full.argtypes = [C.c_int64, _i32p, _i32p, _f64p, C.c_int32, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, C.c_double, _f64p]
+        L.or_collision_general.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, C.c_double, _f64p,
+                                           _f64p]
+        L.or_collision_vjp.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, _f64p, _f64p]
+        L.or_dp5_adjoint.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                     C.c_double, _f64p, C.c_double, C.c_double, C.c_double, _f64p,
+                                     C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -197,6 +206,13 @@ class Kernel:
         self.m_lo, self.gm = gauss_table(self.de, omega0, sigma, cutoff)
         self.amp = g * g
 
+    def domega_full(self):
+        """dK/domega0 on the same windows (the sampled G' table)."""
+        Kd = np.zeros((self.n, self.n))
+        lib().or_kernel_domega_full(self.n, self.off, self.len, self.R, self.maxw, self.de,
+                                    self.omega0, self.sigma, self.cutoff, self.amp, Kd)
+        return Kd
+
     def row(self, i):
         out = np.zeros(self.n)
         lib().or_kernel_row(self.n, i, self.off, self.len, self.R, self.maxw, self.gm,
@@ -297,3 +313,27 @@ def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, 
                                   _ptr(dt), max_dense)
     nd = int(np.count_nonzero(dt))
     return dict(y=f, dense=dense[:nd], dense_t=dt[:nd])
+
+
+def collision_vjp(K, V, bath_n, f, mu):
+    out = np.zeros_like(np.asarray(f, np.float64))
+    lib().or_collision_vjp(out.size, np.ascontiguousarray(K, np.float64),
+                           np.ascontiguousarray(V, np.float64), bath_n,
+                           np.ascontiguousarray(f, np.float64), np.ascontiguousarray(mu, np.float64),
+                           out)
+    return out
+
+
+def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL_df, ckpt=32):
+    """Discrete adjoint of the fixed-step DP5 with dense output at t_final.
+    Returns (dL/df0, dL/dg, dL/domega0)."""
+    n = len(f0)
+    out = np.zeros(n)
+    dg, dw = C.c_double(), C.c_double()
+    steps = np.ascontiguousarray(steps, np.float64)
+    lib().or_dp5_adjoint(n, np.ascontiguousarray(K, np.float64), np.ascontiguousarray(Kd, np.float64),
+                         np.ascontiguousarray(V, np.float64), bath_n, dbath, g,
+                         np.ascontiguousarray(f0, np.float64), t0, t_final, dense_dt, steps,
+                         steps.size, ckpt, np.ascontiguousarray(dL_df, np.float64), out,
+                         C.byref(dg), C.byref(dw))
+    return out, dg.value, dw.value

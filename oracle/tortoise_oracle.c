@@ -666,3 +666,232 @@ void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
     }
     dp5_free(&w);
 }
+
+/* ============================================================ adjoint */
+void or_kernel_domega_full(int64_t n, const int32_t* off, const int32_t* len,
+                           const double* R, int32_t maxw, double de, double omega0,
+                           double sigma, double cutoff, double amp, double* Kd) {
+    /* G'(x) = -x/sigma^2 G(x) sampled on the same offsets as G */
+    int32_t m_lo = (int32_t)ceil((-cutoff * sigma - omega0) / de);
+    int32_t m_hi = (int32_t)floor((cutoff * sigma - omega0) / de);
+    int32_t nm = m_hi - m_lo + 1;
+    double* gd = (double*)malloc(sizeof(double) * nm);
+    const double norm = 1.0 / (sigma * sqrt(2.0 * M_PI));
+    for (int32_t m = m_lo; m <= m_hi; ++m) {
+        double x = (double)m * de + omega0;
+        gd[m - m_lo] = -x / (sigma * sigma) * norm * exp(-0.5 * (x / sigma) * (x / sigma));
+    }
+    or_kernel_full(n, off, len, R, maxw, gd, m_lo, nm, amp, Kd);
+    free(gd);
+}
+
+/* u = K x and v = K^T x for several vectors x at once (one pass over K) */
+static void dual_products(int64_t n, const double* K, int nv, const double* const* x,
+                          double* const* u, double* const* v) {
+    for (int q = 0; q < nv; ++q)
+        for (int64_t i = 0; i < n; ++i) { u[q][i] = 0.0; v[q][i] = 0.0; }
+    for (int64_t i = 0; i < n; ++i) {
+        const double* Ki = K + i * n;
+        for (int q = 0; q < nv; ++q) {
+            double su = 0.0;
+            const double xi = x[q][i];
+            double* vq = v[q];
+            const double* xq = x[q];
+            for (int64_t j = 0; j < n; ++j) {
+                su += Ki[j] * xq[j];
+                vq[j] += Ki[j] * xi;
+            }
+            u[q][i] = su;
+        }
+    }
+}
+
+void or_collision_general(int64_t n, const double* K, const double* V, double alpha,
+                          double beta, const double* f, double* I) {
+    double* buf = (double*)malloc(sizeof(double) * 6 * n);
+    double *one = buf, *u = buf + n, *v = buf + 2 * n, *rk = buf + 3 * n, *ck = buf + 4 * n;
+    for (int64_t i = 0; i < n; ++i) one[i] = 1.0;
+    const double* xs[2] = {f, one};
+    double* us[2] = {u, rk};
+    double* vs[2] = {v, ck};
+    dual_products(n, K, 2, xs, us, vs);
+    for (int64_t i = 0; i < n; ++i) {
+        double wf = alpha * u[i] + beta * v[i];
+        double wt1 = alpha * ck[i] + beta * rk[i];
+        double wtf = alpha * v[i] + beta * u[i];
+        I[i] = ((1.0 - f[i]) * wf - f[i] * (wt1 - wtf)) / V[i];
+    }
+    free(buf);
+}
+
+/* J^T mu: with m = mu/V, x = m(1-f), y = m f,
+ * out_k = -m_k [(W f)_k + (W^T 1)_k - (W^T f)_k] + (W^T x)_k + (W y)_k */
+void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n,
+                      const double* f, const double* mu, double* out) {
+    double* buf = (double*)malloc(sizeof(double) * 12 * n);
+    double *one = buf, *x = buf + n, *y = buf + 2 * n;
+    double *uf = buf + 3 * n, *vf = buf + 4 * n, *u1 = buf + 5 * n, *v1 = buf + 6 * n;
+    double *ux = buf + 7 * n, *vx = buf + 8 * n, *uy = buf + 9 * n, *vy = buf + 10 * n;
+    for (int64_t i = 0; i < n; ++i) {
+        double m = mu[i] / V[i];
+        one[i] = 1.0;
+        x[i] = m * (1.0 - f[i]);
+        y[i] = m * f[i];
+    }
+    const double* xs[4] = {f, one, x, y};
+    double* us[4] = {uf, u1, ux, uy};
+    double* vs[4] = {vf, v1, vx, vy};
+    dual_products(n, K, 4, xs, us, vs);
+    const double np1 = bath_n + 1.0;
+    for (int64_t k = 0; k < n; ++k) {
+        double m = mu[k] / V[k];
+        double wf = np1 * uf[k] + bath_n * vf[k];
+        double wt1 = np1 * v1[k] + bath_n * u1[k];
+        double wtf = np1 * vf[k] + bath_n * uf[k];
+        double wtx = np1 * vx[k] + bath_n * ux[k];
+        double wy = np1 * uy[k] + bath_n * vy[k];
+        out[k] = -m * (wf + wt1 - wtf) + wtx + wy;
+    }
+    free(buf);
+}
+
+typedef struct adj_ctx {
+    int64_t n;
+    const double *K, *Kd, *V;
+    double bath, dbath, g;
+} adj_ctx;
+
+static void adj_rhs(const adj_ctx* a, const double* y, double* out) {
+    or_collision_general(a->n, a->K, a->V, a->bath + 1.0, a->bath, y, out);
+}
+
+/* mu^T dF/dtheta at state y: (dL/dg, dL/domega0) contributions */
+static void adj_param(const adj_ctx* a, const double* y, const double* mu, double* dg, double* dw) {
+    int64_t n = a->n;
+    double* t1 = (double*)malloc(sizeof(double) * 2 * n);
+    double* t2 = t1 + n;
+    adj_rhs(a, y, t1);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += mu[i] * t1[i];
+    *dg += 2.0 * s / a->g;
+    /* W' = (n+1) Kd + n Kd^T + n' (K + K^T) */
+    or_collision_general(n, a->Kd, a->V, a->bath + 1.0, a->bath, y, t1);
+    or_collision_general(n, a->K, a->V, a->dbath, a->dbath, y, t2);
+    s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += mu[i] * (t1[i] + t2[i]);
+    *dw += s;
+    free(t1);
+}
+
+static const double TA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {1. / 5., 0, 0, 0, 0, 0},
+    {3. / 40., 9. / 40., 0, 0, 0, 0},
+    {44. / 45., -56. / 15., 32. / 9., 0, 0, 0},
+    {19372. / 6561., -25360. / 2187., 64448. / 6561., -212. / 729., 0, 0},
+    {9017. / 3168., -355. / 33., 46732. / 5247., 49. / 176., -5103. / 18656., 0},
+    {35. / 384., 0., 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84.}};
+
+/* stage states Y[0..6] (Y[6] = y5) of the step from y with size h */
+static void adj_stages(const adj_ctx* a, const double* y, double h, double** Y, double** k) {
+    int64_t n = a->n;
+    for (int s = 0; s < 7; ++s) {
+        for (int64_t i = 0; i < n; ++i) {
+            double v = y[i];
+            for (int j = 0; j < s; ++j) v += (TA[s][j] * h) * k[j][i];
+            Y[s][i] = v;
+        }
+        adj_rhs(a, Y[s], k[s]);
+    }
+}
+
+void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* V,
+                    double bath_n, double dbath, double g, const double* f0, double t0,
+                    double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
+                    int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
+                    double* dL_domega0) {
+    adj_ctx a = {n, K, Kd, V, bath_n, dbath, g};
+    /* forward: find the step holding the last dense output and keep checkpoints */
+    int64_t nck = n_steps / ckpt + 2;
+    double* ck = (double*)malloc(sizeof(double) * n * nck);
+    double* y = (double*)malloc(sizeof(double) * n);
+    double *Y[7], *k[7];
+    double* sb = (double*)malloc(sizeof(double) * 14 * n);
+    for (int s = 0; s < 7; ++s) { Y[s] = sb + s * n; k[s] = sb + (7 + s) * n; }
+    memcpy(y, f0, sizeof(double) * n);
+    double t = t0, dense_time = t0 + dense_dt, s_last = 0.0;
+    int64_t m_last = -1;
+    for (int64_t m = 0; m < n_steps && t < t_final; ++m) {
+        if (m % ckpt == 0) memcpy(ck + (m / ckpt) * n, y, sizeof(double) * n);
+        double h = h_steps[m];
+        adj_stages(&a, y, h, Y, k);
+        while (dense_time <= t + 1.0001 * h && dense_time <= t_final + .0001 * h) {
+            m_last = m;
+            s_last = (dense_time - t) / h;
+            dense_time += dense_dt;
+        }
+        memcpy(y, Y[6], sizeof(double) * n);
+        t += h;
+    }
+    double* lam = (double*)calloc(n, sizeof(double));
+    double* mu[7];
+    double* nu = (double*)malloc(sizeof(double) * n);
+    double* mb = (double*)malloc(sizeof(double) * 7 * n);
+    for (int s = 0; s < 7; ++s) mu[s] = mb + s * n;
+    double dg = 0.0, dw = 0.0;
+    double* seg = (double*)malloc(sizeof(double) * n * (ckpt + 1));
+    for (int64_t c0 = (m_last / ckpt) * ckpt; c0 >= 0 && m_last >= 0; c0 -= ckpt) {
+        int64_t c1 = c0 + ckpt - 1 < m_last ? c0 + ckpt - 1 : m_last;
+        /* recompute the segment's step-start states from its checkpoint */
+        memcpy(seg, ck + (c0 / ckpt) * n, sizeof(double) * n);
+        for (int64_t m = c0; m < c1; ++m) {
+            adj_stages(&a, seg + (m - c0) * n, h_steps[m], Y, k);
+            memcpy(seg + (m - c0 + 1) * n, Y[6], sizeof(double) * n);
+        }
+        for (int64_t m = c1; m >= c0; --m) {
+            const double h = h_steps[m];
+            const double* ym = seg + (m - c0) * n;
+            adj_stages(&a, ym, h, Y, k);
+            for (int s = 0; s < 7; ++s) memset(mu[s], 0, sizeof(double) * n);
+            if (m == m_last) {
+                /* dense output y + sum_j (s h btilde_j) k_j (DormandPrince54.hpp:601-608) */
+                double s = s_last;
+                double bt[7];
+                bt[0] = -(435. * s * s * s - 1184. * s * s + 1098. * s - 384.) / (384.);
+                bt[1] = 0.0;
+                bt[2] = 500. * s * (6. * s * s - 14. * s + 9.) / (1113.);
+                bt[3] = -125. * s * (9. * s * s - 16. * s + 6.) / (192.);
+                bt[4] = 729. * s * (35. * s * s - 64. * s + 26.) / (6784.);
+                bt[5] = -11. * s * (3. * s - 2.) * (5. * s - 6.) / (84.);
+                bt[6] = s * (s - 1.) * (5. * s - 3.) / (2.);
+                for (int64_t i = 0; i < n; ++i) lam[i] = dL_df[i];
+                for (int j = 0; j < 7; ++j)
+                    for (int64_t i = 0; i < n; ++i) mu[j][i] = (s * h * bt[j]) * dL_df[i];
+                /* k7 = F(Y7), Y7 = y + sum_i (h b_i) k_i */
+                or_collision_vjp(n, K, V, bath_n, Y[6], mu[6], nu);
+                adj_param(&a, Y[6], mu[6], &dg, &dw);
+                for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
+                for (int j = 0; j < 6; ++j)
+                    for (int64_t i = 0; i < n; ++i) mu[j][i] += (TA[6][j] * h) * nu[i];
+            } else {
+                /* y_{m+1} = y_m + sum_i (h b_i) k_i; lam holds dL/dy_{m+1} */
+                for (int j = 0; j < 6; ++j)
+                    for (int64_t i = 0; i < n; ++i) mu[j][i] = (TA[6][j] * h) * lam[i];
+            }
+            for (int st = 5; st >= 1; --st) {
+                or_collision_vjp(n, K, V, bath_n, Y[st], mu[st], nu);
+                adj_param(&a, Y[st], mu[st], &dg, &dw);
+                for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
+                for (int j = 0; j < st; ++j)
+                    for (int64_t i = 0; i < n; ++i) mu[j][i] += (TA[st][j] * h) * nu[i];
+            }
+            or_collision_vjp(n, K, V, bath_n, Y[0], mu[0], nu);
+            adj_param(&a, Y[0], mu[0], &dg, &dw);
+            for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
+        }
+    }
+    memcpy(dL_df0, lam, sizeof(double) * n);
+    *dL_dg = dg;
+    *dL_domega0 = dw;
+    free(ck); free(y); free(sb); free(lam); free(nu); free(mb); free(seg);
+}

@@ -150,6 +150,32 @@ void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
                              double* dense_out, double* dense_t,
                              int32_t max_dense);
 
+/* ------------------------------------------------------------ adjoint */
+/* Derivative of K with respect to omega0 (the sampled G' table, same
+ * windows): Kd_ij = g^2 sum R_ib R_jb' G'_sigma((b-b') dE + omega0). */
+void or_kernel_domega_full(int64_t n, const int32_t* off, const int32_t* len,
+                           const double* R, int32_t maxw, double de, double omega0,
+                           double sigma, double cutoff, double amp, double* Kd);
+
+/* Collision operator with a general W = alpha K + beta K^T (no correction). */
+void or_collision_general(int64_t n, const double* K, const double* V, double alpha,
+                          double beta, const double* f, double* I);
+
+/* Vector-Jacobian product of the operator at f: out = J(f)^T mu. */
+void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n,
+                      const double* f, const double* mu, double* out);
+
+/* Discrete adjoint of the fixed-step DP5 (same step sequence as
+ * or_dp5_collision_replay, all accepted) with dense output at t_final:
+ * given dL/df(t_final) returns dL/df0, dL/dg and dL/domega0.  K scales as g^2;
+ * Kd = dK/domega0; dbath = d(bath_n)/domega0.  Forward states are recomputed
+ * from checkpoints every `ckpt` steps. */
+void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* V,
+                    double bath_n, double dbath, double g, const double* f0, double t0,
+                    double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
+                    int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
+                    double* dL_domega0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,7 +289,7 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
             std::vector<double> ones(c->n, 1.0);
             TG_HIP(c, hipMemcpyAsync(c->d_Q, ones.data(), sizeof(double) * c->n,
                                      hipMemcpyHostToDevice, c->stream));
-            TG_TRY(plain_products_device(c, c->d_Q, c->d_rk, c->d_ck));
+            TG_TRY(plain_products_device(c, c->d_K, c->d_Q, c->d_rk, c->d_ck));
             TG_HIP(c, hipStreamSynchronize(c->stream));
         }
         {
@@ -325,6 +325,48 @@ extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows,
     TG_HIP(c, hipStreamSynchronize(c->stream));
     return TG_OK;
 }
+
+namespace tg {
+
+// dK/domega0: the same projection windows with the sampled derivative table
+// G'(x) = -x/sigma^2 G(x) (x = m dE + omega0), then K' 1 and K'^T 1.
+tg_status assemble_domega(tg_ctx* c) {
+    if (c->d_Kd) return TG_OK;
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
+    std::vector<double> gd(c->nm);
+    const double norm = 1.0 / (c->sigma * std::sqrt(2.0 * M_PI));
+    for (int32_t m = c->m_lo; m < c->m_lo + c->nm; ++m) {
+        const double x = (double)m * c->de + c->omega0;
+        gd[m - c->m_lo] = -x / (c->sigma * c->sigma) * norm *
+                          std::exp(-0.5 * (x / c->sigma) * (x / c->sigma));
+    }
+    double* d_gd = nullptr;
+    double* d_Qd = nullptr;
+    TG_TRY(dev_alloc(c, &d_gd, gd.size()));
+    TG_TRY(dev_alloc(c, &d_Qd, (size_t)c->n * c->qw));
+    TG_HIP(c, hipMemcpyAsync(d_gd, gd.data(), sizeof(double) * gd.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    hipLaunchKernelGGL(conv_kernel, dim3(ceil_div(c->qw, 128), c->n), dim3(128), 0, c->stream,
+                       c->d_R, c->d_len, c->maxw, d_gd, c->nm, c->qw, c->n, d_Qd);
+    TG_CHECK_LAUNCH(c);
+    TG_TRY(dev_alloc(c, &c->d_Kd, (size_t)c->n * c->ld));
+    hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
+                       sizeof(double) * c->maxw, c->stream, c->d_R, d_Qd, c->d_off, c->d_len,
+                       c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->g * c->g, c->d_Kd);
+    TG_CHECK_LAUNCH(c);
+    TG_TRY(dev_alloc(c, &c->d_rkd, c->n));
+    TG_TRY(dev_alloc(c, &c->d_ckd, c->n));
+    std::vector<double> ones(c->n, 1.0);
+    TG_HIP(c, hipMemcpyAsync(d_Qd, ones.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
+                             c->stream));
+    TG_TRY(plain_products_device(c, c->d_Kd, d_Qd, c->d_rkd, c->d_ckd));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&d_Qd);
+    dev_free(&d_gd);
+    return TG_OK;
+}
+
+}  // namespace tg
 
 extern "C" tg_status tg_kernel_stats_get(tg_ctx* c, tg_kernel_stats* out) {
     if (!c || !out) return TG_ERR_ARG;

@@ -286,6 +286,10 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     mixed_destroy(c);
     dp5_destroy(c);
+    adjoint_destroy(c);
+    dev_free(&c->d_Kd);
+    dev_free(&c->d_rkd);
+    dev_free(&c->d_ckd);
     dev_free(&c->d_eps);
     dev_free(&c->d_V);
     dev_free(&c->d_conn);

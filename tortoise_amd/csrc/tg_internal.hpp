@@ -66,6 +66,10 @@ struct tg_ctx {
     double* d_rk = nullptr;        // K 1
     double* d_ck = nullptr;        // K^T 1
     int64_t nnz = 0;               // structurally nonzero entries of K
+    double* d_Kd = nullptr;        // dK/domega0 (adjoint only), n x ld
+    double* d_rkd = nullptr;       // dK/domega0 1
+    double* d_ckd = nullptr;       // (dK/domega0)^T 1
+    void* adj = nullptr;           // adjoint work (tg_adjoint.hip)
     bool assembled = false;
 
     // plain FP64 apply scratch
@@ -139,7 +143,9 @@ inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) /
 // ---- internal entry points between translation units
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
                              int conserve);
-tg_status plain_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
+tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
+                                double* d_v);
+tg_status assemble_domega(tg_ctx* c);
 tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, double* number,
                               double* energy);
 tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,
@@ -150,6 +156,7 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
 void mixed_destroy(tg_ctx* c);
 void mixed_stats(tg_ctx* c, tg_kernel_stats* out);
 void dp5_destroy(tg_ctx* c);
+void adjoint_destroy(tg_ctx* c);
 tg_status ensure_staging(tg_ctx* c, int32_t batch);
 // event bracket of the dominant kernel when TG_TIME_KERNEL is active
 void time_begin(tg_ctx* c);

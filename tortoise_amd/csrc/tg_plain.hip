@@ -181,10 +181,10 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
         I[i] -= s * f[i] * (1.0 - f[i]);
 }
 
-static tg_status launch_dual(tg_ctx* c, const double* d_f) {
+static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
     time_begin(c);
     hipLaunchKernelGGL(plain_dual_kernel, dim3(c->n_colchk, c->n_rowblk), dim3(kPlainThreads), 0,
-                       c->stream, c->d_K, c->n, c->ld, d_f, c->d_upart, c->d_vpart);
+                       c->stream, Kmat, c->n, c->ld, d_f, c->d_upart, c->d_vpart);
     time_end(c);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
@@ -193,7 +193,7 @@ static tg_status launch_dual(tg_ctx* c, const double* d_f) {
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
                              int conserve) {
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
-    TG_TRY(launch_dual(c, d_f));
+    TG_TRY(launch_dual(c, c->d_K, d_f));
     const unsigned nb = ceil_div(c->n, kFinishNodes);
     double* dpart = nullptr;
     if (conserve) {
@@ -216,9 +216,11 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     return TG_OK;
 }
 
-// u = K f, v = K^T f (used for the row/column sums at assembly)
-tg_status plain_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v) {
-    TG_TRY(launch_dual(c, d_f));
+// u = M f, v = M^T f for M = K or dK/domega0 (row/column sums at assembly,
+// products of the adjoint)
+tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
+                                double* d_v) {
+    TG_TRY(launch_dual(c, Kmat, d_f));
     const unsigned nb = ceil_div(c->n, kFinishNodes);
     hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
                        c->stream, 1, c->d_upart, c->n_colchk, c->d_vpart, c->n_rowblk, c->n,

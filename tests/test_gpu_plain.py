@@ -147,3 +147,26 @@ def test_large_mesh_rows_and_properties():
     Io = oracle.collision_apply(K, V, w.bath_n, f0)
     del K
     assert _rel(I, Io) <= 1e-13
+
+
+def test_cube_tetrahedra_matches_oracle():
+    """3D: Kuhn tetrahedra of a 6x6x6-section cube (343 nodes), parabolic
+    band; K, one evaluation and a short DP5 run against the oracle."""
+    w = wl.Workload("cube6", 3, (6, 6, 6), t_final=2.0, rtol=1e-8, atol=1e-12, sigma=0.02)
+    print(f"[gpu] cube6 context, n={w.n_nodes}", flush=True)
+    ctx = tg.Context.create(w, max_batch=1)
+    ctx.assemble()
+    ok = _oracle_kernel(w)
+    K = ok.full()
+    Kg = ctx.kernel_rows(0, ctx.n)
+    assert np.abs(Kg - K).max() <= 1e-12 * np.abs(K).max()
+    np.testing.assert_allclose(ctx.node_volumes(), ok.V, rtol=1e-15)
+    f0 = w.initial_f()
+    assert _rel(ctx.apply(f0), oracle.collision_apply(K, ok.V, w.bath_n, f0)) <= 1e-13
+    fg, dg, sg = ctx.dp5(f0, 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    conn = oracle.mesh_elements(3, w.nsec)
+    ro = oracle.dp5_collision(3, w.nsec, K, ok.V, w.bath_n, f0, 0.0, 2.0, 0.02, 1e-8, 1e-12,
+                              max_dense=100)
+    assert sg["attempted"] == ro["stats"].attempted
+    assert _rel(dg[-1], ro["dense"][-1]) <= 1e-12
+    assert conn.shape == (6 * 216, 4)

@@ -65,7 +65,8 @@ class ProblemDesc(C.Structure):
 
 
 class KernelStats(C.Structure):
-    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("mixed_band", C.c_int64),
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("plain_band", C.c_int64),
+                ("mixed_band", C.c_int64),
                 ("mixed_splits", C.c_int32), ("mixed_width", C.c_int32),
                 ("mixed_scale", C.c_int32), ("window", C.c_int32), ("kmax", C.c_double)]
 

@@ -17,6 +17,7 @@
 //   conv_kernel         node -> Q_j = R_j * g (sampled Gaussian)   (window conv)
 //   kernel_entries      (i, j) -> K_ij = g^2 <R_i, Q_j>            (dense FP64)
 //   K 1 and K^T 1 come from the plain one-pass kernel with f = 1
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -177,22 +178,26 @@ __global__ void conv_kernel(const double* __restrict__ R, const int32_t* __restr
     Q[j * (int64_t)qw + p] = s;
 }
 
-// K_ij = amp * sum_b R_i(b) Q_j(b) over the overlap of the two windows.
-// Block = 256 columns j of one row i; row i's window is staged in LDS.
+// K_ij = amp * sum_b R_i(b) Q_j(b) over the overlap of the two windows,
+// stored in energy order: row p holds node i = perm[p], column q node
+// j = perm[q].  Block = 256 columns of one row; the row's window is in LDS.
 constexpr int kKThreads = 256;
 __global__ void kernel_entries(const double* __restrict__ R, const double* __restrict__ Q,
                                const int32_t* __restrict__ off, const int32_t* __restrict__ len,
                                int maxw, int qw, int m_lo, int nm, int64_t n, int64_t ld,
-                               double amp, double* __restrict__ K) {
+                               const int32_t* __restrict__ perm, double amp,
+                               double* __restrict__ K) {
     extern __shared__ double sRi[];
-    const int64_t i = blockIdx.y;
+    const int64_t prow = blockIdx.y;
+    const int64_t i = perm[prow];
     const int offi = off[i], leni = len[i];
     for (int w = threadIdx.x; w < leni; w += blockDim.x) sRi[w] = R[i * (int64_t)maxw + w];
     __syncthreads();
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= ld) return;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= ld) return;
     double s = 0.0;
-    if (j < n) {
+    if (q < n) {
+        const int64_t j = perm[q];
         const int qoff = off[j] + m_lo;
         const int qlen = len[j] + nm - 1;
         int b0 = offi > qoff ? offi : qoff;
@@ -201,7 +206,49 @@ __global__ void kernel_entries(const double* __restrict__ R, const double* __res
         for (int b = b0; b < b1; ++b) s += sRi[b - offi] * Qj[b - qoff];
         s *= amp;
     }
-    K[i * ld + j] = s;  // padding columns hold exact zeros
+    K[prow * ld + q] = s;  // padding columns hold exact zeros
+}
+
+// first / last nonzero column of every (sorted) row of K; -1 when empty
+__global__ void row_nz_extent_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+                                     int32_t* __restrict__ first, int32_t* __restrict__ last) {
+    __shared__ int32_t sf[4], sl[4];
+    const int64_t p = blockIdx.x;
+    int32_t f = 0x7fffffff, l = -1;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x)
+        if (K[p * ld + q] != 0.0) {
+            f = q < f ? (int32_t)q : f;
+            l = (int32_t)q;
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        f = min(f, __shfl_down(f, o, 64));
+        l = max(l, __shfl_down(l, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sf[threadIdx.x >> 6] = f;
+        sl[threadIdx.x >> 6] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        first[p] = min(min(sf[0], sf[1]), min(sf[2], sf[3]));
+        last[p] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
+    }
+}
+
+__global__ void gather_kernel(const double* __restrict__ src, const int32_t* __restrict__ perm,
+                              int64_t n, double* __restrict__ dst) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[perm[p]];
+}
+
+// natural-order rows of K from the sorted storage: out[r][j] = K[pinv r'][pinv j]
+__global__ void copy_rows_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+                                 const int32_t* __restrict__ pinv, int64_t row0,
+                                 double* __restrict__ out) {
+    const int64_t r = blockIdx.y;
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    out[r * n + j] = K[(int64_t)pinv[row0 + r] * ld + pinv[j]];
 }
 
 // nonzero count per row of K (for throughput accounting)
@@ -215,6 +262,68 @@ __global__ void nnz_rows_kernel(const double* __restrict__ K, int64_t n, int64_t
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) out[i] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Row-block column ranges of the sorted K, the (row block, chunk) work list
+// and, per column chunk, the row blocks whose range covers it.
+tg_status build_band(tg_ctx* c) {
+    int32_t *d_first = nullptr, *d_last = nullptr;
+    TG_TRY(dev_alloc(c, &d_first, (size_t)c->n));
+    TG_TRY(dev_alloc(c, &d_last, (size_t)c->n));
+    hipLaunchKernelGGL(row_nz_extent_kernel, dim3(c->n), dim3(256), 0, c->stream, c->d_K, c->n,
+                       c->ld, d_first, d_last);
+    TG_CHECK_LAUNCH(c);
+    std::vector<int32_t> first(c->n), last(c->n);
+    TG_HIP(c, hipMemcpyAsync(first.data(), d_first, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(c, hipMemcpyAsync(last.data(), d_last, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&d_first);
+    dev_free(&d_last);
+    c->h_rb_col.assign(c->n_rowblk, int2{0, 0});
+    std::vector<int2> rb_chunk(c->n_rowblk, int2{0, -1});
+    std::vector<int2> work;
+    std::vector<std::vector<int32_t>> cover(c->n_colchk);
+    c->band_elems = 0;
+    for (int32_t r = 0; r < c->n_rowblk; ++r) {
+        int32_t lo = 0x7fffffff, hi = -1;
+        const int64_t p1 = std::min<int64_t>(c->n, (int64_t)(r + 1) * kPlainRows);
+        for (int64_t p = (int64_t)r * kPlainRows; p < p1; ++p)
+            if (last[p] >= 0) {
+                lo = std::min(lo, first[p]);
+                hi = std::max(hi, last[p]);
+            }
+        if (hi < 0) continue;
+        // whole sweeps: the kernel reads 128-column segments
+        const int32_t c0 = lo / kPlainCols * kPlainCols;
+        const int32_t c1 = std::min<int64_t>(c->ld, (hi / kPlainCols + 1) * kPlainCols);
+        c->h_rb_col[r] = int2{c0, c1};
+        rb_chunk[r] = int2{c0 / kPlainChunk, (c1 - 1) / kPlainChunk};
+        for (int32_t ch = rb_chunk[r].x; ch <= rb_chunk[r].y; ++ch) {
+            work.push_back(int2{r, ch});
+            cover[ch].push_back(r);
+        }
+        c->band_elems += (p1 - (int64_t)r * kPlainRows) * (int64_t)(c1 - c0);
+    }
+    std::vector<int32_t> cc_off(c->n_colchk + 1, 0), cc_rows;
+    for (int32_t ch = 0; ch < c->n_colchk; ++ch) {
+        cc_off[ch + 1] = cc_off[ch] + (int32_t)cover[ch].size();
+        cc_rows.insert(cc_rows.end(), cover[ch].begin(), cover[ch].end());
+    }
+    if (cc_rows.empty()) cc_rows.push_back(0);
+    if (work.empty()) work.push_back(int2{0, 0});
+    c->n_work = (int32_t)work.size();
+    TG_TRY(dev_alloc(c, &c->d_rb_col, (size_t)c->n_rowblk));
+    TG_TRY(dev_alloc(c, &c->d_rb_chunk, (size_t)c->n_rowblk));
+    TG_TRY(dev_alloc(c, &c->d_work, work.size()));
+    TG_TRY(dev_alloc(c, &c->d_cc_off, cc_off.size()));
+    TG_TRY(dev_alloc(c, &c->d_cc_rows, cc_rows.size()));
+    TG_HIP(c, hipMemcpyAsync(c->d_rb_col, c->h_rb_col.data(), sizeof(int2) * c->n_rowblk, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipMemcpyAsync(c->d_rb_chunk, rb_chunk.data(), sizeof(int2) * c->n_rowblk, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipMemcpyAsync(c->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipMemcpyAsync(c->d_cc_off, cc_off.data(), sizeof(int32_t) * cc_off.size(), hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipMemcpyAsync(c->d_cc_rows, cc_rows.data(), sizeof(int32_t) * cc_rows.size(), hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
 }
 
 }  // namespace tg
@@ -274,22 +383,31 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
         TG_TRY(dev_alloc(c, &c->d_K, (size_t)c->n * c->ld));
         hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
                            sizeof(double) * c->maxw, c->stream, c->d_R, c->d_Q, c->d_off, c->d_len,
-                           c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->g * c->g, c->d_K);
+                           c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->d_perm, c->g * c->g,
+                           c->d_K);
         TG_CHECK_LAUNCH(c);
 
-        // 6. plain-apply partial buffers, then K 1 and K^T 1 through the same
-        //    one-pass kernel (f = 1), summed in a fixed order
+        // 6. band of the sorted K and the work list of the one-pass kernel
         c->n_rowblk = (int32_t)ceil_div(c->n, kPlainRows);
-        c->n_colchk = (int32_t)ceil_div(c->n, kPlainChunk);
+        c->n_colchk = (int32_t)ceil_div(c->ld, kPlainChunk);
+        TG_TRY(build_band(c));
         TG_TRY(dev_alloc(c, &c->d_upart, (size_t)c->n_colchk * c->n));
         TG_TRY(dev_alloc(c, &c->d_vpart, (size_t)c->n_rowblk * c->ld));
+        // 7. K 1 and K^T 1 through the same one-pass kernel (f = 1)
         TG_TRY(dev_alloc(c, &c->d_rk, c->n));
         TG_TRY(dev_alloc(c, &c->d_ck, c->n));
+        TG_TRY(dev_alloc(c, &c->d_rk_s, c->n));
+        TG_TRY(dev_alloc(c, &c->d_ck_s, c->n));
         {
             std::vector<double> ones(c->n, 1.0);
             TG_HIP(c, hipMemcpyAsync(c->d_Q, ones.data(), sizeof(double) * c->n,
                                      hipMemcpyHostToDevice, c->stream));
             TG_TRY(plain_products_device(c, c->d_K, c->d_Q, c->d_rk, c->d_ck));
+            hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                               c->d_rk, c->d_perm, c->n, c->d_rk_s);
+            hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                               c->d_ck, c->d_perm, c->n, c->d_ck_s);
+            TG_CHECK_LAUNCH(c);
             TG_HIP(c, hipStreamSynchronize(c->stream));
         }
         {
@@ -319,10 +437,34 @@ extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows,
     if (!c || !out) return TG_ERR_ARG;
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
     if (row0 < 0 || nrows < 0 || row0 + nrows > c->n) return set_error(c, TG_ERR_ARG, "rows out of range");
-    TG_HIP(c, hipMemcpy2DAsync(out, sizeof(double) * c->n, c->d_K + row0 * c->ld,
-                               sizeof(double) * c->ld, sizeof(double) * c->n, nrows,
-                               hipMemcpyDeviceToHost, c->stream));
-    TG_HIP(c, hipStreamSynchronize(c->stream));
+    if (nrows == 0) return TG_OK;
+    // K is stored in energy order: gather natural rows/columns through pinv
+    std::vector<int32_t> pinv(c->n);
+    for (int64_t p = 0; p < c->n; ++p) pinv[c->h_perm[p]] = (int32_t)p;
+    int32_t* d_pinv = nullptr;
+    double* d_out = nullptr;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nrows, (int64_t)(1ll << 27) / c->n));
+    TG_TRY(dev_alloc(c, &d_pinv, (size_t)c->n));
+    tg_status s = dev_alloc(c, &d_out, (size_t)chunk * c->n);
+    if (s != TG_OK) {
+        dev_free(&d_pinv);
+        return s;
+    }
+    hipError_t e = hipMemcpyAsync(d_pinv, pinv.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice,
+                                  c->stream);
+    for (int64_t r0 = 0; r0 < nrows && e == hipSuccess; r0 += chunk) {
+        const int64_t nr = std::min(chunk, nrows - r0);
+        hipLaunchKernelGGL(copy_rows_kernel, dim3(ceil_div(c->n, 256), nr), dim3(256), 0, c->stream,
+                           c->d_K, c->n, c->ld, d_pinv, row0 + r0, d_out);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out + r0 * c->n, d_out, sizeof(double) * nr * c->n,
+                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    dev_free(&d_out);
+    dev_free(&d_pinv);
+    if (e != hipSuccess) return set_error(c, TG_ERR_HIP, "copy rows: %s", hipGetErrorString(e));
     return TG_OK;
 }
 
@@ -352,7 +494,8 @@ tg_status assemble_domega(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &c->d_Kd, (size_t)c->n * c->ld));
     hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
                        sizeof(double) * c->maxw, c->stream, c->d_R, d_Qd, c->d_off, c->d_len,
-                       c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->g * c->g, c->d_Kd);
+                       c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->d_perm, c->g * c->g,
+                       c->d_Kd);
     TG_CHECK_LAUNCH(c);
     TG_TRY(dev_alloc(c, &c->d_rkd, c->n));
     TG_TRY(dev_alloc(c, &c->d_ckd, c->n));
@@ -374,6 +517,7 @@ extern "C" tg_status tg_kernel_stats_get(tg_ctx* c, tg_kernel_stats* out) {
     *out = tg_kernel_stats{};
     out->n = c->n;
     out->nnz = c->nnz;
+    out->plain_band = c->band_elems;
     out->window = c->maxw;
     std::vector<double> rk(c->n);
     TG_HIP(c, hipMemcpyAsync(rk.data(), c->d_rk, sizeof(double) * c->n, hipMemcpyDeviceToHost,

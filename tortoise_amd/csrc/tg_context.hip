@@ -1,5 +1,6 @@
 // tg_context.hip -- context lifetime, mesh upload, errors, staging buffers,
 // and the FP64 reductions of the boundary (conservation sums, error norm).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -269,6 +270,20 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
         e = hipMemcpy(c->d_conn, conn.data(), sizeof(int32_t) * conn.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
 
+    // energy order of the nodes (stable: equal energies keep the node order)
+    c->h_perm.resize(c->n);
+    for (int64_t i = 0; i < c->n; ++i) c->h_perm[i] = (int32_t)i;
+    std::stable_sort(c->h_perm.begin(), c->h_perm.end(),
+                     [&](int32_t a, int32_t b) { return d->eps[a] < d->eps[b]; });
+    std::vector<double> Vs(c->n);
+    for (int64_t p = 0; p < c->n; ++p) Vs[p] = V[c->h_perm[p]];
+    if ((s = dev_alloc(c, &c->d_perm, c->n)) != TG_OK) return fail(s);
+    if ((s = dev_alloc(c, &c->d_V_s, c->n)) != TG_OK) return fail(s);
+    if ((s = dev_alloc(c, &c->d_fs, c->n)) != TG_OK) return fail(s);
+    e = hipMemcpy(c->d_perm, c->h_perm.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_V_s, Vs.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+
     // energy grid (same rule as the CPU restatement: e0 = min - de)
     double lo = d->eps[0], hi = d->eps[0];
     for (int64_t i = 1; i < c->n; ++i) {
@@ -303,6 +318,16 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_ck);
     dev_free(&c->d_upart);
     dev_free(&c->d_vpart);
+    dev_free(&c->d_perm);
+    dev_free(&c->d_V_s);
+    dev_free(&c->d_rk_s);
+    dev_free(&c->d_ck_s);
+    dev_free(&c->d_fs);
+    dev_free(&c->d_rb_chunk);
+    dev_free(&c->d_rb_col);
+    dev_free(&c->d_work);
+    dev_free(&c->d_cc_off);
+    dev_free(&c->d_cc_rows);
     dev_free(&c->d_f);
     dev_free(&c->d_I);
     dev_free(&c->d_red);

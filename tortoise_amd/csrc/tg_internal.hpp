@@ -77,6 +77,25 @@ struct tg_ctx {
     double* d_upart = nullptr;     // n_colchk x n
     double* d_vpart = nullptr;     // n_rowblk x ld
 
+    // energy order: K is stored with rows and columns sorted by node energy
+    int32_t* d_perm = nullptr;     // sorted position -> node
+    std::vector<int32_t> h_perm;
+    double* d_V_s = nullptr;       // V in sorted order
+    double* d_rk_s = nullptr;      // K 1 in sorted order
+    double* d_ck_s = nullptr;      // K^T 1 in sorted order
+    double* d_fs = nullptr;        // scratch: one state gathered to sorted order
+    // band of the sorted K: row block r has nonzero columns in
+    // [rb_col[r].x, rb_col[r].y) (column units); the one-pass kernel runs over
+    // the (row block, column chunk) pairs that intersect it
+    std::vector<int2> h_rb_col;
+    int2* d_rb_chunk = nullptr;    // per row block: [first chunk, last chunk]
+    int2* d_rb_col = nullptr;      // per row block: [first col, end col)
+    int2* d_work = nullptr;        // (row block, chunk) pairs
+    int32_t n_work = 0;
+    int32_t* d_cc_off = nullptr;   // per column chunk: offset into d_cc_rows (n_colchk + 1)
+    int32_t* d_cc_rows = nullptr;  // row blocks whose band covers the chunk, increasing
+    int64_t band_elems = 0;        // K entries the one-pass kernel streams per evaluation
+
     // mixed batched data (tg_mixed.hip)
     void* mixed = nullptr;
 
@@ -146,6 +165,7 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
 tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
                                 double* d_v);
 tg_status assemble_domega(tg_ctx* c);
+tg_status build_band(tg_ctx* c);
 tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, double* number,
                               double* energy);
 tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,

@@ -12,11 +12,12 @@
 // sums K 1, K^T 1 used by the epilogue are exact FP64 sums of the *split*
 // matrix, so the split error cancels in K^T 1 - K^T f for Pauli-blocked nodes.
 //
-// Node order: nodes are sorted by energy before packing, which makes the
-// emission kernel a band: every 128-row panel of A has a contiguous column
-// range outside which it is exactly zero, and the GEMM only streams that
-// range.  Inputs and outputs stay in the caller's node order; the F-packing
-// kernel gathers through the permutation and the epilogue scatters back.
+// Node order: K is stored in energy order (see tg_assemble.hip), which makes
+// the emission kernel a band: every 128-row panel of A has a contiguous
+// column range outside which it is exactly zero, and the GEMM only streams
+// that range.  Inputs and outputs stay in the caller's node order; the
+// F-packing kernel gathers through the permutation and the epilogue scatters
+// back.
 //
 // Packed layouts (16-byte units of eight fp16 values along the reduction k):
 //   A_{hi,lo}[kg][row]   kg = k/8, row in [0, rows_pad)
@@ -54,10 +55,10 @@ struct MixedData {
     uint4* F_lo = nullptr;
     int2* krange = nullptr;    // per panel [kit_begin, kit_end) in iterations
     std::vector<int2> h_krange;
-    int32_t* perm = nullptr;   // sorted position -> node
+    const int32_t* perm = nullptr;  // sorted position -> node (owned by the context)
     double* rk_s = nullptr;    // split-matrix K 1 (sorted order)
     double* ck_s = nullptr;    // split-matrix K^T 1 (sorted order)
-    double* V_s = nullptr;     // node volumes (sorted order)
+    const double* V_s = nullptr;    // node volumes (sorted order, owned by the context)
     float* P = nullptr;        // nsplit x nt x rows_pad partial sums
     int nsplit = 1;
     size_t P_cap = 0;
@@ -76,10 +77,10 @@ __device__ inline void split_fp16(double x, double scale, _Float16& hi, _Float16
     lo = (_Float16)(s - (double)hi);
 }
 
-// A[p][q] (sorted order): p < n -> K[perm p][perm q]; p >= n -> K[perm q][perm (p-n)]
+// A[p][q] from K stored in energy order: p < n -> K[p][q]; p >= n -> K[q][p-n]
 __global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                              const int32_t* __restrict__ perm, int64_t rows_pad, int64_t kgroups,
-                              double scale, uint4* __restrict__ Ahi, uint4* __restrict__ Alo) {
+                              int64_t rows_pad, int64_t kgroups, double scale,
+                              uint4* __restrict__ Ahi, uint4* __restrict__ Alo) {
     const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t kg = blockIdx.y;
     if (row >= rows_pad || kg >= kgroups) return;
@@ -87,10 +88,7 @@ __global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t l
     for (int q = 0; q < 8; ++q) {
         const int64_t col = kg * 8 + q;
         double x = 0.0;
-        if (row < 2 * n && col < n) {
-            const int64_t pc = perm[col];
-            x = row < n ? K[(int64_t)perm[row] * ld + pc] : K[pc * ld + perm[row - n]];
-        }
+        if (row < 2 * n && col < n) x = row < n ? K[row * ld + col] : K[col * ld + (row - n)];
         _Float16 a, b;
         split_fp16(x, scale, a, b);
         h.v[q] = a;
@@ -135,12 +133,6 @@ __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* 
     s *= inv_scale;
     if (row < n) rk[row] = s;
     else ck[row - n] = s;
-}
-
-__global__ void gather_sorted_kernel(const double* __restrict__ src, const int32_t* __restrict__ perm,
-                                     int64_t n, double* __restrict__ dst) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p < n) dst[p] = src[perm[p]];
 }
 
 // F (caller order, member-major f[b*n + i], FP64) -> packed sorted fp16 hi/lo
@@ -391,15 +383,8 @@ tg_status mixed_prepare(tg_ctx* c) {
     const int64_t kgroups = m->kpad / 8;
     hipStream_t st = c->stream;
 
-    // energy order (stable: ties keep the node order)
-    std::vector<double> eps(n);
-    TG_HIP(c, hipMemcpyAsync(eps.data(), c->d_eps, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipStreamSynchronize(st));
-    std::vector<int32_t> perm(n);
-    for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return eps[a] < eps[b]; });
-    TG_TRY(dev_alloc(c, &m->perm, (size_t)n));
-    TG_HIP(c, hipMemcpyAsync(m->perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    // node order shared with the plain path (energy order, built at context creation)
+    m->perm = c->d_perm;
 
     // scale: largest |K| maps to 2^14 (fp16 max 65504)
     double kmax = 0.0;
@@ -416,18 +401,15 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->A_hi, (size_t)kgroups * m->rows_pad));
     TG_TRY(dev_alloc(c, &m->A_lo, (size_t)kgroups * m->rows_pad));
     hipLaunchKernelGGL(pack_A_kernel, dim3(ceil_div(m->rows_pad, 256), kgroups), dim3(256), 0, st,
-                       c->d_K, n, c->ld, m->perm, m->rows_pad, kgroups, scale, m->A_hi, m->A_lo);
+                       c->d_K, n, c->ld, m->rows_pad, kgroups, scale, m->A_hi, m->A_lo);
     TG_CHECK_LAUNCH(c);
 
     // split-matrix row sums and sorted node volumes
     TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
     TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
-    TG_TRY(dev_alloc(c, &m->V_s, (size_t)n));
+    m->V_s = c->d_V_s;
     hipLaunchKernelGGL(split_rowsum_kernel, dim3(ceil_div(2 * n, 256)), dim3(256), 0, st, m->A_hi,
                        m->A_lo, m->rows_pad, kgroups, n, 1.0 / scale, m->rk_s, m->ck_s);
-    TG_CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(gather_sorted_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->d_V,
-                       m->perm, n, m->V_s);
     TG_CHECK_LAUNCH(c);
 
     // band extent of every panel
@@ -498,10 +480,8 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->F_hi);
     dev_free(&m->F_lo);
     dev_free(&m->krange);
-    dev_free(&m->perm);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
-    dev_free(&m->V_s);
     dev_free(&m->P);
     dev_free(&m->bath);
     dev_free(&m->part);

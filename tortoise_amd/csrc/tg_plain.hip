@@ -3,7 +3,13 @@
 // Per evaluation the kernel streams K (n x ld FP64, row-major) from HBM once
 // and produces both u = K f and v = K^T f:
 //
-//   phase 1 (plain_dual_kernel): workgroup (colchunk c, rowblock r) covers
+// K is stored with rows and columns in energy order, which makes the
+// emission kernel a band: every 128-row block has a contiguous range of
+// nonzero columns, and only (row block, 2048-column chunk) pairs that meet
+// it are launched; sweeps of a launched chunk outside the range publish
+// zeros without reading K.
+//
+//   phase 1 (plain_dual_kernel): workgroup (row block r, chunk c) covers
 //     rows [128 r, 128 r + 128) x cols [2048 c, 2048 c + 2048).  Each of the 4
 //     waves owns 32 consecutive rows; a lane owns two adjacent columns of a
 //     128-column sweep, so every row read is one 1 KiB coalesced segment per
@@ -18,8 +24,8 @@
 //       I_i = [(1-f_i)((n+1)u_i + n v_i) - f_i((n+1)ck_i + n rk_i - (n+1)v_i - n u_i)] / V_i
 //     with rk = K 1 and ck = K^T 1 precomputed in FP64.
 //
-// Algorithmic HBM traffic per evaluation: 8 n^2 (K) + 32 n (f, I, two
-// vectors); the partial buffers add 8 n (n/128 + n/2048) bytes (< 1 %).
+// HBM traffic per evaluation: 8 x (band entries of K) + 32 n (f, I, two
+// vectors) + the partial buffers; the dense form would be 8 n^2 + 32 n.
 #include "tg_internal.hpp"
 
 namespace tg {
@@ -31,13 +37,16 @@ constexpr int kSweeps = kPlainChunk / kPlainCols;       // 16
 
 __global__ void __launch_bounds__(kPlainThreads)
 plain_dual_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                  const double* __restrict__ f, double* __restrict__ upart,
+                  const double* __restrict__ f, const int2* __restrict__ work,
+                  const int2* __restrict__ rb_col, double* __restrict__ upart,
                   double* __restrict__ vpart) {
     __shared__ double sv[4][kPlainCols];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t row0 = (int64_t)blockIdx.y * kPlainRows + wave * kRowsPerWave;
-    const int64_t col0 = (int64_t)blockIdx.x * kPlainChunk;
+    const int2 wk = work[blockIdx.x];           // (row block, column chunk)
+    const int2 band = rb_col[wk.x];              // nonzero columns of the row block
+    const int64_t row0 = (int64_t)wk.x * kPlainRows + wave * kRowsPerWave;
+    const int64_t col0 = (int64_t)wk.y * kPlainChunk;
 
     double fr[kRowsPerWave];
     double uacc[kRowsPerWave];
@@ -50,13 +59,16 @@ plain_dual_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
     const int nrows = (int)((n - row0) < kRowsPerWave ? (n - row0) : kRowsPerWave);
 
     for (int s = 0; s < kSweeps; ++s) {
-        const int64_t cs = col0 + (int64_t)s * kPlainCols + 2 * lane;
-        if (col0 + (int64_t)s * kPlainCols >= ld) break;  // uniform over the block
-        double fc0 = 0.0, fc1 = 0.0;
-        if (cs < n) fc0 = f[cs];
-        if (cs + 1 < n) fc1 = f[cs + 1];
+        const int64_t sw = col0 + (int64_t)s * kPlainCols;
+        if (sw >= ld) break;  // uniform over the block
+        const bool in_band = sw >= band.x && sw < band.y;  // uniform over the block
+        const int64_t cs = sw + 2 * lane;
         double v0 = 0.0, v1 = 0.0;
-        if (cs < ld && nrows > 0) {
+        // ld is a multiple of 64, not of 128: lanes past the row end stay idle
+        if (in_band && nrows > 0 && cs < ld) {
+            double fc0 = 0.0, fc1 = 0.0;
+            if (cs < n) fc0 = f[cs];
+            if (cs + 1 < n) fc1 = f[cs + 1];
             if (nrows == kRowsPerWave) {
                 dbl2 kk[kRowsPerWave];
 #pragma unroll
@@ -78,14 +90,16 @@ plain_dual_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
                 }
             }
         }
+        // sweeps outside the band publish zeros (the chunk is covered by this
+        // row block, so the finishing sum reads them)
         sv[wave][2 * lane] = v0;
         sv[wave][2 * lane + 1] = v1;
         __syncthreads();
         if (threadIdx.x < kPlainCols) {
-            const int64_t c = col0 + (int64_t)s * kPlainCols + threadIdx.x;
+            const int64_t cc = sw + threadIdx.x;
             const double t = (sv[0][threadIdx.x] + sv[1][threadIdx.x]) +
                              (sv[2][threadIdx.x] + sv[3][threadIdx.x]);
-            if (c < ld) vpart[(int64_t)blockIdx.y * ld + c] = t;
+            if (cc < ld) vpart[(int64_t)wk.x * ld + cc] = t;
         }
         __syncthreads();
     }
@@ -98,56 +112,69 @@ plain_dual_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
         uacc[q] = u;
     }
     if (lane < kRowsPerWave) {
-        // lane q publishes row q (the value is identical in every lane)
         double mine = 0.0;
 #pragma unroll
         for (int q = 0; q < kRowsPerWave; ++q) mine = (lane == q) ? uacc[q] : mine;
         const int64_t r = row0 + lane;
-        if (r < n) upart[(int64_t)blockIdx.x * n + r] = mine;
+        if (r < n) upart[(int64_t)wk.y * n + r] = mine;
     }
 }
 
-// mode 0: collision operator; mode 1: raw products u -> out0, v -> out1
+__global__ void gather_sorted_f_kernel(const double* __restrict__ f, const int32_t* __restrict__ perm,
+                                       int64_t n, double* __restrict__ fs) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < n) fs[p] = f[perm[p]];
+}
+
+// mode 0: collision operator; mode 1: raw products u -> out0, v -> out1.
+// Thread (tx, ty) sums slice ty of node p's partials: the column chunks of
+// its row block (u) and the row blocks covering its column chunk (v).
+// Outputs go to the caller's node order perm[p].
 constexpr int kFinishNodes = 64;
 constexpr int kFinishSlices = 4;
 
 __global__ void __launch_bounds__(kFinishNodes* kFinishSlices)
-plain_finish_kernel(int mode, const double* __restrict__ upart, int ncol,
-                    const double* __restrict__ vpart, int nrow, int64_t n, int64_t ld,
-                    const double* __restrict__ f, const double* __restrict__ rk,
-                    const double* __restrict__ ck, const double* __restrict__ V,
-                    double bath_n, double* __restrict__ out0, double* __restrict__ out1,
-                    double* __restrict__ dpart) {
+plain_finish_kernel(int mode, const double* __restrict__ upart, const double* __restrict__ vpart,
+                    const int2* __restrict__ rb_chunk, const int32_t* __restrict__ cc_off,
+                    const int32_t* __restrict__ cc_rows, int64_t n, int64_t ld,
+                    const int32_t* __restrict__ perm, const double* __restrict__ fs,
+                    const double* __restrict__ rk, const double* __restrict__ ck,
+                    const double* __restrict__ V, double bath_n, double* __restrict__ out0,
+                    double* __restrict__ out1, double* __restrict__ dpart) {
     __shared__ double su[kFinishSlices][kFinishNodes];
     __shared__ double svv[kFinishSlices][kFinishNodes];
     const int tx = threadIdx.x % kFinishNodes;
     const int ty = threadIdx.x / kFinishNodes;
-    const int64_t i = (int64_t)blockIdx.x * kFinishNodes + tx;
+    const int64_t p = (int64_t)blockIdx.x * kFinishNodes + tx;
     double u = 0.0, v = 0.0;
-    if (i < n) {
-        for (int c = ty; c < ncol; c += kFinishSlices) u += upart[(int64_t)c * n + i];
-        for (int r = ty; r < nrow; r += kFinishSlices) v += vpart[(int64_t)r * ld + i];
+    if (p < n) {
+        const int2 ch = rb_chunk[p / kPlainRows];
+        for (int c = ch.x + ty; c <= ch.y; c += kFinishSlices) u += upart[(int64_t)c * n + p];
+        const int cchunk = (int)(p / kPlainChunk);
+        for (int q = cc_off[cchunk] + ty; q < cc_off[cchunk + 1]; q += kFinishSlices)
+            v += vpart[(int64_t)cc_rows[q] * ld + p];
     }
     su[ty][tx] = u;
     svv[ty][tx] = v;
     __syncthreads();
     double dI = 0.0, dw = 0.0;
-    if (ty == 0 && i < n) {
+    if (ty == 0 && p < n) {
         u = (su[0][tx] + su[1][tx]) + (su[2][tx] + su[3][tx]);
         v = (svv[0][tx] + svv[1][tx]) + (svv[2][tx] + svv[3][tx]);
+        const int64_t node = perm[p];
         if (mode == 1) {
-            out0[i] = u;
-            out1[i] = v;
+            out0[node] = u;
+            out1[node] = v;
         } else {
             const double np1 = bath_n + 1.0;
-            const double fi = f[i];
+            const double fi = fs[p];
             const double wf = np1 * u + bath_n * v;
-            const double wt1 = np1 * ck[i] + bath_n * rk[i];
+            const double wt1 = np1 * ck[p] + bath_n * rk[p];
             const double wtf = np1 * v + bath_n * u;
-            const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[i];
-            out0[i] = Ii;
-            dI = V[i] * Ii;
-            dw = V[i] * fi * (1.0 - fi);
+            const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
+            out0[node] = Ii;
+            dI = V[p] * Ii;
+            dw = V[p] * fi * (1.0 - fi);
         }
     }
     if (dpart) {
@@ -182,9 +209,12 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
 }
 
 static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
+    hipLaunchKernelGGL(gather_sorted_f_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                       d_f, c->d_perm, c->n, c->d_fs);
+    TG_CHECK_LAUNCH(c);
     time_begin(c);
-    hipLaunchKernelGGL(plain_dual_kernel, dim3(c->n_colchk, c->n_rowblk), dim3(kPlainThreads), 0,
-                       c->stream, Kmat, c->n, c->ld, d_f, c->d_upart, c->d_vpart);
+    hipLaunchKernelGGL(plain_dual_kernel, dim3(c->n_work), dim3(kPlainThreads), 0, c->stream, Kmat,
+                       c->n, c->ld, c->d_fs, c->d_work, c->d_rb_col, c->d_upart, c->d_vpart);
     time_end(c);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
@@ -205,8 +235,9 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
         dpart = c->d_red;
     }
     hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
-                       c->stream, 0, c->d_upart, c->n_colchk, c->d_vpart, c->n_rowblk, c->n,
-                       c->ld, d_f, c->d_rk, c->d_ck, c->d_V, bath_n, d_I, nullptr, dpart);
+                       c->stream, 0, c->d_upart, c->d_vpart, c->d_rb_chunk, c->d_cc_off,
+                       c->d_cc_rows, c->n, c->ld, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s,
+                       c->d_V_s, bath_n, d_I, nullptr, dpart);
     TG_CHECK_LAUNCH(c);
     if (conserve) {
         hipLaunchKernelGGL(conserve_fix_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
@@ -216,15 +247,16 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     return TG_OK;
 }
 
-// u = M f, v = M^T f for M = K or dK/domega0 (row/column sums at assembly,
-// products of the adjoint)
+// u = M f, v = M^T f (caller's node order) for M = K or dK/domega0 stored in
+// energy order (row/column sums at assembly, products of the adjoint)
 tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
                                 double* d_v) {
     TG_TRY(launch_dual(c, Kmat, d_f));
     const unsigned nb = ceil_div(c->n, kFinishNodes);
     hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
-                       c->stream, 1, c->d_upart, c->n_colchk, c->d_vpart, c->n_rowblk, c->n,
-                       c->ld, d_f, nullptr, nullptr, nullptr, 0.0, d_u, d_v, nullptr);
+                       c->stream, 1, c->d_upart, c->d_vpart, c->d_rb_chunk, c->d_cc_off,
+                       c->d_cc_rows, c->n, c->ld, c->d_perm, c->d_fs, nullptr, nullptr, nullptr,
+                       0.0, d_u, d_v, nullptr);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }

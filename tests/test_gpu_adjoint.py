@@ -80,7 +80,8 @@ def test_autograd_gradients_against_central_fd():
 
     def L_at(g, om, f):
         with torch.no_grad():
-            return float(loss_of(torch.from_numpy(f), torch.tensor(g), torch.tensor(om)))
+            return float(loss_of(torch.from_numpy(f), torch.tensor(g, dtype=torch.float64),
+                                 torch.tensor(om, dtype=torch.float64)))
 
     hg, hw = 1e-6 * g0, 1e-6 * w0
     fd_g = (L_at(g0 + hg, w0, f0) - L_at(g0 - hg, w0, f0)) / (2 * hg)

@@ -26,6 +26,7 @@
 // reads in these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "tg_internal.hpp"
@@ -44,7 +45,8 @@ constexpr int kSF = 12;         // F scale exponent: f in [-8, 8] stays inside f
 
 struct MixedData {
     int64_t rows = 0;          // 2n
-    int64_t rows_pad = 0;      // multiple of kMT
+    int64_t kt = 0;            // first row of the K^T half (n rounded up to kMT)
+    int64_t rows_pad = 0;      // 2 kt
     int64_t kpad = 0;          // n rounded up to kKS
     int nt = 256;              // members per GEMM tile (padded batch)
     int sa = 0;                // A scale exponent
@@ -53,6 +55,9 @@ struct MixedData {
     uint4* A_lo = nullptr;
     uint4* F_hi = nullptr;
     uint4* F_lo = nullptr;
+    uint4* G_hi = nullptr;     // operand g = 1 - f
+    uint4* G_lo = nullptr;
+    uint8_t* useg = nullptr;   // npanel x nt operand choice
     int2* krange = nullptr;    // per panel [kit_begin, kit_end) in iterations
     std::vector<int2> h_krange;
     const int32_t* perm = nullptr;  // sorted position -> node (owned by the context)
@@ -77,8 +82,9 @@ __device__ inline void split_fp16(double x, double scale, _Float16& hi, _Float16
     lo = (_Float16)(s - (double)hi);
 }
 
-// A[p][q] from K stored in energy order: p < n -> K[p][q]; p >= n -> K[q][p-n]
-__global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+// A[r][q] from K stored in energy order: r < n -> K[r][q]; the K^T half
+// starts at the panel-aligned row kt: kt <= r < kt + n -> K[q][r-kt]
+__global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t ld, int64_t kt,
                               int64_t rows_pad, int64_t kgroups, double scale,
                               uint4* __restrict__ Ahi, uint4* __restrict__ Alo) {
     const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -88,7 +94,10 @@ __global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t l
     for (int q = 0; q < 8; ++q) {
         const int64_t col = kg * 8 + q;
         double x = 0.0;
-        if (row < 2 * n && col < n) x = row < n ? K[row * ld + col] : K[col * ld + (row - n)];
+        if (col < n) {
+            if (row < n) x = K[row * ld + col];
+            else if (row >= kt && row < kt + n) x = K[col * ld + (row - kt)];
+        }
         _Float16 a, b;
         split_fp16(x, scale, a, b);
         h.v[q] = a;
@@ -116,13 +125,14 @@ __global__ void row_extent_kernel(const uint4* __restrict__ Ahi, const uint4* __
     last[row] = l;
 }
 
-// exact FP64 row sums of the split matrix for rows [0, 2n): the first n are
-// K 1, the last n are K^T 1 (both in sorted order)
+// exact FP64 row sums of the split matrix: rows [0, n) give K 1, rows
+// [kt, kt + n) give K^T 1 (both in sorted order)
 __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                                    int64_t rows_pad, int64_t kgroups, int64_t n, double inv_scale,
-                                    double* __restrict__ rk, double* __restrict__ ck) {
+                                    int64_t rows_pad, int64_t kgroups, int64_t n, int64_t kt,
+                                    double inv_scale, double* __restrict__ rk,
+                                    double* __restrict__ ck) {
     const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (row >= 2 * n) return;
+    if (row >= rows_pad || (row >= n && row < kt) || row >= kt + n) return;
     double s = 0.0;
     for (int64_t kg = 0; kg < kgroups; ++kg) {
         union { half8 v; uint4 u; } h, l;
@@ -132,29 +142,62 @@ __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* 
     }
     s *= inv_scale;
     if (row < n) rk[row] = s;
-    else ck[row - n] = s;
+    else ck[row - kt] = s;
 }
 
 // F (caller order, member-major f[b*n + i], FP64) -> packed sorted fp16 hi/lo
+// of both operands z = f and z = g = 1 - f (the GEMM streams one of them per
+// (panel, member), see choose_operand_kernel)
 __global__ void pack_F_kernel(const double* __restrict__ f, int64_t n, int B,
                               const int32_t* __restrict__ perm, int64_t kgroups, int nt,
-                              uint4* __restrict__ Fhi, uint4* __restrict__ Flo) {
+                              uint4* __restrict__ Fhi, uint4* __restrict__ Flo,
+                              uint4* __restrict__ Ghi, uint4* __restrict__ Glo) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;  // member
     const int64_t kg = blockIdx.y;
     if (col >= nt || kg >= kgroups) return;
-    union { half8 v; uint4 u; } h, l;
+    union { half8 v; uint4 u; } fh, fl, gh, gl;
     const double scale = (double)(1 << kSF);
     for (int q = 0; q < 8; ++q) {
         const int64_t p = kg * 8 + q;
-        double x = 0.0;
-        if (col < B && p < n) x = f[(int64_t)col * n + perm[p]];
+        double x = 0.0, y = 0.0;
+        if (col < B && p < n) {
+            x = f[(int64_t)col * n + perm[p]];
+            y = 1.0 - x;
+        }
         _Float16 a, b;
         split_fp16(x, scale, a, b);
-        h.v[q] = a;
-        l.v[q] = b;
+        fh.v[q] = a;
+        fl.v[q] = b;
+        split_fp16(y, scale, a, b);
+        gh.v[q] = a;
+        gl.v[q] = b;
     }
-    Fhi[kg * nt + col] = h.u;
-    Flo[kg * nt + col] = l.u;
+    Fhi[kg * nt + col] = fh.u;
+    Flo[kg * nt + col] = fl.u;
+    Ghi[kg * nt + col] = gh.u;
+    Glo[kg * nt + col] = gl.u;
+}
+
+// Operand of (panel, member): g = 1 - f when the panel's middle node is in
+// the filled part of the member's distribution (f > 1/2), else f.  The
+// products the epilogue needs come back as K z and K 1 - K z, so the
+// Pauli-blocked out-scattering K^T(1 - f) of filled nodes is a sum of small
+// terms instead of the difference of two large ones.
+__global__ void choose_operand_kernel(const double* __restrict__ f, int64_t n, int64_t kt, int B,
+                                      int npanel, const int32_t* __restrict__ perm,
+                                      uint8_t* __restrict__ useg, int nt) {
+    const int pn = blockIdx.x;
+    const int b = threadIdx.x;
+    if (pn >= npanel || b >= nt) return;
+    uint8_t v = 0;
+    if (b < B) {
+        // K rows and K^T rows index the same sorted nodes
+        const int64_t row = (int64_t)pn * kMT + kMT / 2;
+        int64_t p = row < kt ? row : row - kt;
+        if (p > n - 1) p = n - 1;
+        v = f[(int64_t)b * n + perm[p]] > 0.5 ? 1 : 0;
+    }
+    useg[(int64_t)pn * nt + b] = v;
 }
 
 // --------------------------------------------------------------------- GEMM
@@ -167,8 +210,9 @@ template <int NT>
 __global__ void __launch_bounds__(kMixThreads)
 mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
                   const uint4* __restrict__ Fhi, const uint4* __restrict__ Flo,
-                  int64_t rows_pad, const int2* __restrict__ krange, int nsplit,
-                  float* __restrict__ P) {
+                  const uint4* __restrict__ Ghi, const uint4* __restrict__ Glo,
+                  const uint8_t* __restrict__ useg, int64_t rows_pad,
+                  const int2* __restrict__ krange, int nsplit, float* __restrict__ P) {
     constexpr int TN = NT / 64;                  // 32-wide member tiles per wave
     constexpr int A_UNITS = 2 * kKG * kMT;       // 1024 uint4
     constexpr int F_UNITS = 2 * kKG * NT;
@@ -196,6 +240,10 @@ mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
         for (int u = 0; u < TN; ++u) acc[t][u] = floatx16{0};
 
     uint4 ra[A_PER_T], rf[F_PER_T];
+    // every F unit this thread stages belongs to member tid % NT
+    const bool my_g = useg[(int64_t)panel * NT + (tid % NT)] != 0;
+    const uint4* Zhi = my_g ? Ghi : Fhi;
+    const uint4* Zlo = my_g ? Glo : Flo;
     auto gload = [&](int it) {
         const int64_t kg0 = (int64_t)it * kKG;
 #pragma unroll
@@ -213,7 +261,7 @@ mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
             const int which = q / (kKG * NT);
             const int kg = (q / NT) % kKG;
             const int col = q % NT;
-            const uint4* src = which ? Flo : Fhi;
+            const uint4* src = which ? Zlo : Zhi;
             rf[j] = src[(kg0 + kg) * NT + col];
         }
     };
@@ -287,11 +335,11 @@ mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
 constexpr int kFinT = 256;
 __global__ void __launch_bounds__(kFinT)
 mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t rows_pad,
-                    int64_t n, const double* __restrict__ f, const int32_t* __restrict__ perm,
+                    int64_t n, int64_t kt, const double* __restrict__ f, const int32_t* __restrict__ perm,
                     const double* __restrict__ rk, const double* __restrict__ ck,
                     const double* __restrict__ V, const double* __restrict__ bath,
-                    double inv_scale, double* __restrict__ I, double* __restrict__ part,
-                    int nblk) {
+                    const uint8_t* __restrict__ useg, double inv_scale, double* __restrict__ I,
+                    double* __restrict__ part, int nblk) {
     const int b = blockIdx.y;
     const int64_t p = blockIdx.x * (int64_t)kFinT + threadIdx.x;
     double dI = 0.0, dw = 0.0;
@@ -300,18 +348,21 @@ mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t row
         for (int s = 0; s < nsplit; ++s) {
             const float* Ps = P + ((int64_t)s * nt + b) * rows_pad;
             u += (double)Ps[p];
-            v += (double)Ps[n + p];
+            v += (double)Ps[kt + p];
         }
         u *= inv_scale;
         v *= inv_scale;
+        // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
+        const bool gu = useg[(p / kMT) * (int64_t)nt + b] != 0;
+        const bool gv = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
+        const double Kf = gu ? rk[p] - u : u, Kg = gu ? u : rk[p] - u;
+        const double KTf = gv ? ck[p] - v : v, KTg = gv ? v : ck[p] - v;
         const double nb = bath[b];
         const double np1 = nb + 1.0;
         const int64_t node = perm[p];
         const double fi = f[(int64_t)b * n + node];
-        const double wf = np1 * u + nb * v;
-        const double wt1 = np1 * ck[p] + nb * rk[p];
-        const double wtf = np1 * v + nb * u;
-        const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
+        // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
+        const double Ii = ((1.0 - fi) * (np1 * Kf + nb * KTf) - fi * (np1 * KTg + nb * Kg)) / V[p];
         I[(int64_t)b * n + node] = Ii;
         dI = V[p] * Ii;
         dw = V[p] * fi * (1.0 - fi);
@@ -376,7 +427,8 @@ tg_status mixed_prepare(tg_ctx* c) {
     c->mixed = m;
     const int64_t n = c->n;
     m->rows = 2 * n;
-    m->rows_pad = (m->rows + kMT - 1) / kMT * kMT;
+    m->kt = (n + kMT - 1) / kMT * kMT;
+    m->rows_pad = 2 * m->kt;
     m->kpad = (n + kKS - 1) / kKS * kKS;
     m->npanel = (int)(m->rows_pad / kMT);
     m->nt = nt_for(c->max_batch);
@@ -401,15 +453,16 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->A_hi, (size_t)kgroups * m->rows_pad));
     TG_TRY(dev_alloc(c, &m->A_lo, (size_t)kgroups * m->rows_pad));
     hipLaunchKernelGGL(pack_A_kernel, dim3(ceil_div(m->rows_pad, 256), kgroups), dim3(256), 0, st,
-                       c->d_K, n, c->ld, m->rows_pad, kgroups, scale, m->A_hi, m->A_lo);
+                       c->d_K, n, c->ld, m->kt, m->rows_pad, kgroups, scale, m->A_hi, m->A_lo);
     TG_CHECK_LAUNCH(c);
 
     // split-matrix row sums and sorted node volumes
     TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
     TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
     m->V_s = c->d_V_s;
-    hipLaunchKernelGGL(split_rowsum_kernel, dim3(ceil_div(2 * n, 256)), dim3(256), 0, st, m->A_hi,
-                       m->A_lo, m->rows_pad, kgroups, n, 1.0 / scale, m->rk_s, m->ck_s);
+    hipLaunchKernelGGL(split_rowsum_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st,
+                       m->A_hi, m->A_lo, m->rows_pad, kgroups, n, m->kt, 1.0 / scale, m->rk_s,
+                       m->ck_s);
     TG_CHECK_LAUNCH(c);
 
     // band extent of every panel
@@ -450,14 +503,23 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
-    // K-splits: enough workgroups to cover the 256 CUs several times, each
-    // split still a few iterations long
+    // K-splits: ~16 workgroups per CU (fills the 256 CUs and shortens every
+    // FP32 accumulation chain; the splits are summed in FP64), each split a
+    // few iterations long
     const double avg = (double)items / std::max(1, m->npanel);
-    int ns = (int)std::ceil(1024.0 / m->npanel);
-    ns = std::max(1, std::min(ns, (int)std::max(1.0, avg / 4.0)));
-    m->nsplit = std::min(ns, 16);
+    int ns = (int)std::ceil(4096.0 / m->npanel);
+    ns = std::max(1, std::min(ns, (int)std::max(1.0, avg / 3.0)));
+    m->nsplit = std::min(ns, 32);
+    // explicit override (measurements of the split count)
+    if (const char* e = std::getenv("TG_MIXED_SPLITS")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 64) m->nsplit = v;
+    }
     TG_TRY(dev_alloc(c, &m->F_hi, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->F_lo, (size_t)kgroups * m->nt));
+    TG_TRY(dev_alloc(c, &m->G_hi, (size_t)kgroups * m->nt));
+    TG_TRY(dev_alloc(c, &m->G_lo, (size_t)kgroups * m->nt));
+    TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
     TG_HIP(c, hipStreamSynchronize(st));
     return TG_OK;
@@ -479,6 +541,9 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->A_lo);
     dev_free(&m->F_hi);
     dev_free(&m->F_lo);
+    dev_free(&m->G_hi);
+    dev_free(&m->G_lo);
+    dev_free(&m->useg);
     dev_free(&m->krange);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
@@ -508,22 +573,28 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
 
     hipLaunchKernelGGL(pack_F_kernel, dim3(ceil_div(m->nt, 64), kgroups), dim3(64), 0, st, d_f, n,
-                       B, m->perm, kgroups, m->nt, m->F_hi, m->F_lo);
+                       B, m->perm, kgroups, m->nt, m->F_hi, m->F_lo, m->G_hi, m->G_lo);
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, st, d_f, n, m->kt, B,
+                       m->npanel, m->perm, m->useg, m->nt);
     TG_CHECK_LAUNCH(c);
     const dim3 grid(m->npanel, m->nsplit);
     time_begin(c);
     switch (m->nt) {
         case 64:
             hipLaunchKernelGGL(mixed_gemm_kernel<64>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->rows_pad, m->krange, m->nsplit, m->P);
+                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
+                               m->krange, m->nsplit, m->P);
             break;
         case 128:
             hipLaunchKernelGGL(mixed_gemm_kernel<128>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->rows_pad, m->krange, m->nsplit, m->P);
+                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
+                               m->krange, m->nsplit, m->P);
             break;
         default:
             hipLaunchKernelGGL(mixed_gemm_kernel<256>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->rows_pad, m->krange, m->nsplit, m->P);
+                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
+                               m->krange, m->nsplit, m->P);
             break;
     }
     time_end(c);
@@ -540,8 +611,8 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     }
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
     hipLaunchKernelGGL(mixed_finish_kernel, dim3(nblk, B), dim3(kFinT), 0, st, m->P, m->nsplit, m->nt,
-                       m->rows_pad, n, d_f, m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, inv_scale,
-                       d_I, part, nblk);
+                       m->rows_pad, n, m->kt, d_f, m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg,
+                       inv_scale, d_I, part, nblk);
     TG_CHECK_LAUNCH(c);
     if (conserve) {
         hipLaunchKernelGGL(mixed_conserve_kernel, dim3(ceil_div(n, 1024), B), dim3(256), 0, st,

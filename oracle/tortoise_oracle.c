@@ -408,10 +408,10 @@ void or_collision_apply(int64_t n, const double* K, const double* V,
         double d = 0.0, wsum = 0.0;
         for (int64_t i = 0; i < n; ++i) {
             d += V[i] * I[i];
-            wsum += V[i] * f[i] * (1.0 - f[i]);
+            wsum += V[i] * fabs(f[i] * (1.0 - f[i]));
         }
         if (wsum > 0.0)
-            for (int64_t i = 0; i < n; ++i) I[i] -= d * f[i] * (1.0 - f[i]) / wsum;
+            for (int64_t i = 0; i < n; ++i) I[i] -= d * fabs(f[i] * (1.0 - f[i])) / wsum;
     }
     free(u); free(v); free(rk); free(ck);
 }

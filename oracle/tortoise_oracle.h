@@ -90,7 +90,7 @@ void or_kernel_row(int64_t n, int64_t i, const int32_t* off, const int32_t* len,
 /* ---------------------------------------------------------- collision */
 /* I = [(1-f) (W f) - f (W^T 1 - W^T f)] / V,  W = (n+1)K + n K^T.
  * K row-major n x n.  If conserve != 0 the number defect sum_i V_i I_i is
- * removed with weights f_i(1-f_i) (the same rule the mixed GPU path uses). */
+ * removed with weights |f_i(1-f_i)| (the same rule the GPU paths use). */
 void or_collision_apply(int64_t n, const double* K, const double* V,
                         double bath_n, const double* f, double* I, int conserve);
 

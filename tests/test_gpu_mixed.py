@@ -60,6 +60,21 @@ def test_mixed_partial_batch_and_conservation(sq32_mixed):
         assert abs(np.dot(ok.V, Im[:, b])) <= 1e-13 * np.dot(ok.V, np.abs(Im[:, b]))
 
 
+def test_mixed_conservation_with_states_above_one(sq32_mixed):
+    """A pump inside the Fermi sea pushes f above 1 where f(1-f) < 0: the
+    number-defect removal still zeroes sum V I."""
+    w, ctx, ok, K = sq32_mixed
+    eps = w.eps()
+    F = np.stack([wl.initial_distribution(eps, w.mu, 0.02, 0.25, w.pump_width, 0.5),
+                  wl.initial_distribution(eps, w.mu, 0.03, 0.9, w.pump_width, 0.1)], axis=1)
+    assert F[:, 0].max() > 1.2
+    Im = ctx.apply(F, mixed=True, conserve=True)
+    for b in range(2):
+        assert abs(np.dot(ok.V, Im[:, b])) <= 1e-13 * np.dot(ok.V, np.abs(Im[:, b]))
+        Io = oracle.collision_apply(K, ok.V, w.bath_n, F[:, b], conserve=True)
+        assert _rel(Im[:, b], Io) <= 2e-6
+
+
 def test_mixed_edge_states(sq32_mixed):
     w, ctx, ok, K = sq32_mixed
     F = np.zeros((ctx.n, 2))

@@ -274,6 +274,16 @@ def test_collision_invariants(k8):
     # conservative correction: zero defect, small change
     Ic = oracle.collision_apply(K, V, 0.09, f, conserve=True)
     assert np.abs(Ic - I).max() <= 1e-13 * np.abs(I).max()
+    # the W-form conserves number for any K (the in/out terms are one double
+    # sum read two ways), also for states partly above 1 where f(1-f) < 0;
+    # the correction then leaves the operator unchanged
+    fo = f.copy()
+    fo[: V.size // 3] = 1.0 + 0.5 * rng.uniform(0, 1, V.size // 3)
+    Kp = K * (1.0 + 1e-3 * rng.uniform(-1, 1, K.shape))
+    Io = oracle.collision_apply(Kp, V, 0.09, fo)
+    Ioc = oracle.collision_apply(Kp, V, 0.09, fo, conserve=True)
+    assert abs(np.dot(V, Io)) <= 1e-14 * np.dot(V, np.abs(Io))
+    assert np.abs(Ioc - Io).max() <= 1e-13 * np.abs(Io).max()
 
 
 # ------------------------------------------------------------ error norm

@@ -331,7 +331,7 @@ mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
 // One thread per (sorted node p, member b): FP64 sum of the split partials,
 // Pauli-blocked assembly with the member's bath occupation, result written
 // to the caller's node order; per-block partials of sum V I and
-// sum V f(1-f) for the number-defect removal.
+// sum V |f(1-f)| for the number-defect removal.
 constexpr int kFinT = 256;
 __global__ void __launch_bounds__(kFinT)
 mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t rows_pad,
@@ -365,7 +365,7 @@ mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t row
         const double Ii = ((1.0 - fi) * (np1 * Kf + nb * KTf) - fi * (np1 * KTg + nb * Kg)) / V[p];
         I[(int64_t)b * n + node] = Ii;
         dI = V[p] * Ii;
-        dw = V[p] * fi * (1.0 - fi);
+        dw = V[p] * fabs(fi * (1.0 - fi));
     }
     if (part) {
         __shared__ double sh[2][kFinT / 64];
@@ -390,7 +390,7 @@ mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t row
     }
 }
 
-// per member: s_b = sum V I / sum V f(1-f); I -= s_b f(1-f)
+// per member: s_b = sum V I / sum V |f(1-f)|; I -= s_b |f(1-f)|
 __global__ void mixed_conserve_kernel(const double* __restrict__ part, int nblk, int64_t n,
                                       const double* __restrict__ f, double* __restrict__ I) {
     const int b = blockIdx.y;
@@ -408,7 +408,7 @@ __global__ void mixed_conserve_kernel(const double* __restrict__ part, int nblk,
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const double fi = f[(int64_t)b * n + i];
-        I[(int64_t)b * n + i] -= s * fi * (1.0 - fi);
+        I[(int64_t)b * n + i] -= s * fabs(fi * (1.0 - fi));
     }
 }
 

@@ -174,11 +174,11 @@ plain_finish_kernel(int mode, const double* __restrict__ upart, const double* __
             const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
             out0[node] = Ii;
             dI = V[p] * Ii;
-            dw = V[p] * fi * (1.0 - fi);
+            dw = V[p] * fabs(fi * (1.0 - fi));
         }
     }
     if (dpart) {
-        // per-block partials of sum V I and sum V f(1-f), wave 0 only
+        // per-block partials of sum V I and sum V |f(1-f)|, wave 0 only
         if (ty == 0) {
             for (int o = 32; o > 0; o >>= 1) {
                 dI += __shfl_down(dI, o, 64);
@@ -192,8 +192,8 @@ plain_finish_kernel(int mode, const double* __restrict__ upart, const double* __
     }
 }
 
-// Removes the number defect d = sum V I with weights f(1-f):
-// I_i -= d f_i(1-f_i) / sum_k V_k f_k(1-f_k).
+// Removes the number defect d = sum V I with weights |f(1-f)| (nonnegative
+// also where a state leaves [0,1]): I_i -= d |f_i(1-f_i)| / sum_k V_k |f_k(1-f_k)|.
 __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
                                     const double* __restrict__ f, double* __restrict__ I) {
     double d = 0.0, w = 0.0;
@@ -205,7 +205,7 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
     const double s = d / w;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        I[i] -= s * f[i] * (1.0 - f[i]);
+        I[i] -= s * fabs(f[i] * (1.0 - f[i]));
 }
 
 static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {

@@ -110,6 +110,8 @@ typedef struct tg_kernel_stats {
     int32_t mixed_scale;    /* power-of-two exponent applied to K before the fp16 split */
     int32_t window;         /* widest energy-projection window (hats per node) */
     double  kmax;           /* largest row sum of K */
+    int32_t mixed_tile;     /* members per workgroup of the batched kernel */
+    int32_t mixed_fold;     /* iterations per FP64 fold of its FP32 sums (0: FP32 partials) */
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

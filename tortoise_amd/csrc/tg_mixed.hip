@@ -36,6 +36,7 @@ namespace tg {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 128;        // rows of A per workgroup
 constexpr int kKS = 32;         // reduction depth per iteration (two MFMA k-steps)
@@ -64,12 +65,15 @@ struct MixedData {
     double* rk_s = nullptr;    // split-matrix K 1 (sorted order)
     double* ck_s = nullptr;    // split-matrix K^T 1 (sorted order)
     const double* V_s = nullptr;    // node volumes (sorted order, owned by the context)
-    float* P = nullptr;        // nsplit x nt x rows_pad partial sums
+    uint8_t* P = nullptr;      // nsplit x nt x rows_pad partial sums (FP64, FP32 if fold == 0)
     int nsplit = 1;
-    size_t P_cap = 0;
+    int nw = 128;              // members per workgroup tile
+    int fold = 2;              // iterations per FP64 fold (0: FP32 partials)
+    int waves = 4;             // waves per workgroup
+    size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
-    double* part = nullptr;    // per-member conservation partials
-    int part_cap = 0;
+    double* fs = nullptr;      // f in sorted order, FP64, member-major (nt x n)
+    bool rowcache = true;      // member rows fit in LDS
     int64_t work_items = 0;
 };
 
@@ -145,37 +149,59 @@ __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* 
     else ck[row - kt] = s;
 }
 
-// F (caller order, member-major f[b*n + i], FP64) -> packed sorted fp16 hi/lo
-// of both operands z = f and z = g = 1 - f (the GEMM streams one of them per
-// (panel, member), see choose_operand_kernel)
-__global__ void pack_F_kernel(const double* __restrict__ f, int64_t n, int B,
-                              const int32_t* __restrict__ perm, int64_t kgroups, int nt,
-                              uint4* __restrict__ Fhi, uint4* __restrict__ Flo,
-                              uint4* __restrict__ Ghi, uint4* __restrict__ Glo) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;  // member
-    const int64_t kg = blockIdx.y;
-    if (col >= nt || kg >= kgroups) return;
-    union { half8 v; uint4 u; } fh, fl, gh, gl;
-    const double scale = (double)(1 << kSF);
-    for (int q = 0; q < 8; ++q) {
-        const int64_t p = kg * 8 + q;
-        double x = 0.0, y = 0.0;
-        if (col < B && p < n) {
-            x = f[(int64_t)col * n + perm[p]];
-            y = 1.0 - x;
-        }
-        _Float16 a, b;
-        split_fp16(x, scale, a, b);
-        fh.v[q] = a;
-        fl.v[q] = b;
-        split_fp16(y, scale, a, b);
-        gh.v[q] = a;
-        gl.v[q] = b;
+// Member-per-workgroup kernels.  Workgroup w serves member
+// b = (w % 8) * (nt / 8) + w / 8, so the members one XCD serves are
+// consecutive and the 16-byte units of eight members that share a 128-byte
+// line of the packed operands meet in one L2.  With ROW the member's row of
+// f (caller order) is staged in LDS by coalesced reads and the energy-order
+// gather reads LDS; without it (rows larger than the LDS) the gather reads
+// global memory.
+constexpr int kMemT = 1024;
+
+__device__ inline int member_of_block(int w, int nt) { return (w % 8) * (nt / 8) + w / 8; }
+
+// f -> packed sorted fp16 hi/lo of both operands z = f and z = g = 1 - f
+// (the GEMM streams one of them per (panel, member), see
+// choose_operand_kernel) and the FP64 sorted copy fs the epilogue reads
+template <bool ROW>
+__global__ void __launch_bounds__(kMemT)
+pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
+                   int64_t kgroups, int nt, u32x4* __restrict__ Fhi, u32x4* __restrict__ Flo,
+                   u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo, double* __restrict__ fs) {
+    extern __shared__ double row[];
+    const int b = member_of_block(blockIdx.x, nt);
+    const bool live = b < B;
+    const double* fb = f + (int64_t)b * n;
+    if constexpr (ROW) {
+        if (live)
+            for (int64_t i = threadIdx.x; i < n; i += kMemT) row[i] = fb[i];
+        __syncthreads();
     }
-    Fhi[kg * nt + col] = fh.u;
-    Flo[kg * nt + col] = fl.u;
-    Ghi[kg * nt + col] = gh.u;
-    Glo[kg * nt + col] = gl.u;
+    const double scale = (double)(1 << kSF);
+    for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
+        half8 fh, fl, gh, gl;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t p = kg * 8 + q;
+            double x = 0.0, y = 0.0;
+            if (live && p < n) {
+                const int32_t node = perm[p];
+                x = ROW ? row[node] : fb[node];
+                y = 1.0 - x;
+                fs[(int64_t)b * n + p] = x;
+            }
+            _Float16 a, c;
+            split_fp16(x, scale, a, c);
+            fh[q] = a;
+            fl[q] = c;
+            split_fp16(y, scale, a, c);
+            gh[q] = a;
+            gl[q] = c;
+        }
+        Fhi[kg * nt + b] = __builtin_bit_cast(u32x4, fh);
+        Flo[kg * nt + b] = __builtin_bit_cast(u32x4, fl);
+        Ghi[kg * nt + b] = __builtin_bit_cast(u32x4, gh);
+        Glo[kg * nt + b] = __builtin_bit_cast(u32x4, gl);
+    }
 }
 
 // Operand of (panel, member): g = 1 - f when the panel's middle node is in
@@ -201,32 +227,91 @@ __global__ void choose_operand_kernel(const double* __restrict__ f, int64_t n, i
 }
 
 // --------------------------------------------------------------------- GEMM
-// Workgroup: 128 rows x NT members, four waves in a 2 x 2 grid; a wave owns
-// 64 rows x NT/2 members = 2 x (NT/64) tiles of 32 x 32.  One iteration
-// stages A (4 k-groups x 128 rows, hi and lo) and F (4 k-groups x NT, hi and
-// lo) in LDS, prefetching the next iteration's tile into registers while the
-// MFMAs run.
-template <int NT>
-__global__ void __launch_bounds__(kMixThreads)
-mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                  const uint4* __restrict__ Fhi, const uint4* __restrict__ Flo,
-                  const uint4* __restrict__ Ghi, const uint4* __restrict__ Glo,
-                  const uint8_t* __restrict__ useg, int64_t rows_pad,
-                  const int2* __restrict__ krange, int nsplit, float* __restrict__ P) {
-    constexpr int TN = NT / 64;                  // 32-wide member tiles per wave
-    constexpr int A_UNITS = 2 * kKG * kMT;       // 1024 uint4
-    constexpr int F_UNITS = 2 * kKG * NT;
-    constexpr int A_PER_T = A_UNITS / kMixThreads;
-    constexpr int F_PER_T = F_UNITS / kMixThreads;
-    __shared__ uint4 sA[2][kKG][kMT];
-    __shared__ uint4 sF[2][kKG][NT];
+// Workgroup: 128 rows x NW members (one member tile of the padded batch) and
+// WAVES waves in a 2 x WAVES/2 grid; a wave owns 64 rows x 2*NW/WAVES
+// members = 2 x TN tiles of 32 x 32.  One iteration stages A (4 k-groups x
+// 128 rows, hi and lo) and Z (4 k-groups x NW, hi and lo) in LDS,
+// prefetching the next iteration's tile into registers while the MFMAs run.
+//
+// Accuracy: with FOLD > 0 the FP32 accumulators are added into FP64
+// registers every FOLD iterations (a chain of at most 32*FOLD products per
+// FP32 sum), so a workgroup can run a whole panel's band with no K-split and
+// the partial sums leave the kernel in FP64.  FOLD == 0 keeps one FP32 chain
+// per K-split and writes FP32 partials; it needs many splits for the same
+// accuracy.
+//
+// Block order: the dispatcher hands workgroup w to XCD w % 8.  The logical
+// block L = (w % 8) * per_xcd + w / 8 makes consecutive L run on the same
+// XCD, and L enumerates (panel, split, member tile) with the member tile
+// fastest, so the member tiles of one panel read the same A tile through
+// one L2 and neighbouring panels share most of their Z window.
+template <int FOLD>
+struct Partial { typedef double type; };
+template <>
+struct Partial<0> { typedef float type; };
 
-    const int panel = blockIdx.x;
-    const int split = blockIdx.y;
+template <int NTH, int N>
+__device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict__ Ahi,
+                                        const u32x4* __restrict__ Alo, int64_t rows_pad,
+                                        int64_t row0, int64_t kg0, int tid) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int q = tid + j * NTH;
+        const int which = q / (kKG * kMT);
+        const int kg = (q / kMT) % kKG;
+        const int r = q % kMT;
+        const u32x4* src = which ? Alo : Ahi;
+        ra[j] = src[(kg0 + kg) * rows_pad + row0 + r];
+    }
+}
+
+template <int NTH, int NW, int N>
+__device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* __restrict__ Zhi,
+                                        const u32x4* __restrict__ Zlo, int nt, int col0,
+                                        int64_t kg0, int tid) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int q = tid + j * NTH;
+        const int which = q / (kKG * NW);
+        const int kg = (q / NW) % kKG;
+        const int col = q % NW;
+        const u32x4* src = which ? Zlo : Zhi;
+        rf[j] = src[(kg0 + kg) * nt + col0 + col];
+    }
+}
+
+template <int NW, int WAVES, int FOLD>
+__global__ void __launch_bounds__(64 * WAVES)
+mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
+                  const u32x4* __restrict__ Fhi, const u32x4* __restrict__ Flo,
+                  const u32x4* __restrict__ Ghi, const u32x4* __restrict__ Glo,
+                  const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
+                  const int2* __restrict__ krange, int nmt, int nsplit, int64_t nblocks,
+                  int64_t per_xcd, typename Partial<FOLD>::type* __restrict__ P) {
+    constexpr int NTH = 64 * WAVES;
+    constexpr int WC = WAVES / 2;                // waves along the members
+    constexpr int WN = NW / WC;                  // members per wave
+    constexpr int TN = WN / 32;                  // 32-wide member tiles per wave
+    static_assert(TN >= 1 && WN % 32 == 0, "member tile too narrow for the wave grid");
+    static_assert(NTH % NW == 0, "each thread stages a single member");
+    constexpr int A_PER_T = 2 * kKG * kMT / NTH;
+    constexpr int F_PER_T = 2 * kKG * NW / NTH;
+    __shared__ u32x4 sA[2][kKG][kMT];
+    __shared__ u32x4 sF[2][kKG][NW];
+
+    const int64_t wg = blockIdx.x;
+    const int64_t L = (wg % 8) * per_xcd + wg / 8;
+    if (L >= nblocks) return;
+    const int mt = (int)(L % nmt);
+    const int64_t ps = L / nmt;
+    const int split = (int)(ps % nsplit);
+    const int panel = (int)(ps / nsplit);
+
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WC, wc = wave % WC;
     const int64_t row0 = (int64_t)panel * kMT;
+    const int col0 = mt * NW;
 
     const int2 kr = krange[panel];
     const int nit = kr.y - kr.x;
@@ -238,115 +323,131 @@ mixed_gemm_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u) acc[t][u] = floatx16{0};
+    double sum[2][TN][16];
+    if constexpr (FOLD > 0) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int u = 0; u < TN; ++u)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) sum[t][u][e] = 0.0;
+    }
 
-    uint4 ra[A_PER_T], rf[F_PER_T];
-    // every F unit this thread stages belongs to member tid % NT
-    const bool my_g = useg[(int64_t)panel * NT + (tid % NT)] != 0;
-    const uint4* Zhi = my_g ? Ghi : Fhi;
-    const uint4* Zlo = my_g ? Glo : Flo;
-    auto gload = [&](int it) {
-        const int64_t kg0 = (int64_t)it * kKG;
+    // every Z unit this thread stages belongs to member col0 + tid % NW
+    const bool my_g = useg[(int64_t)panel * nt + col0 + (tid % NW)] != 0;
+    const u32x4* Zhi = my_g ? Ghi : Fhi;
+    const u32x4* Zlo = my_g ? Glo : Flo;
+    u32x4 ra[A_PER_T], rf[F_PER_T];
+
+    if (it0 < it1) {
+        stage_a<NTH>(ra, Ahi, Alo, rows_pad, row0, (int64_t)it0 * kKG, tid);
+        stage_z<NTH, NW>(rf, Zhi, Zlo, nt, col0, (int64_t)it0 * kKG, tid);
+    }
+    int since = 0;
+    for (int it = it0; it < it1; ++it) {
 #pragma unroll
         for (int j = 0; j < A_PER_T; ++j) {
-            const int q = tid + j * kMixThreads;
-            const int which = q / (kKG * kMT);
-            const int kg = (q / kMT) % kKG;
-            const int r = q % kMT;
-            const uint4* src = which ? Alo : Ahi;
-            ra[j] = src[(kg0 + kg) * rows_pad + row0 + r];
-        }
-#pragma unroll
-        for (int j = 0; j < F_PER_T; ++j) {
-            const int q = tid + j * kMixThreads;
-            const int which = q / (kKG * NT);
-            const int kg = (q / NT) % kKG;
-            const int col = q % NT;
-            const uint4* src = which ? Zlo : Zhi;
-            rf[j] = src[(kg0 + kg) * NT + col];
-        }
-    };
-    auto lstore = [&]() {
-#pragma unroll
-        for (int j = 0; j < A_PER_T; ++j) {
-            const int q = tid + j * kMixThreads;
+            const int q = tid + j * NTH;
             sA[q / (kKG * kMT)][(q / kMT) % kKG][q % kMT] = ra[j];
         }
 #pragma unroll
         for (int j = 0; j < F_PER_T; ++j) {
-            const int q = tid + j * kMixThreads;
-            sF[q / (kKG * NT)][(q / NT) % kKG][q % NT] = rf[j];
+            const int q = tid + j * NTH;
+            sF[q / (kKG * NW)][(q / NW) % kKG][q % NW] = rf[j];
         }
-    };
-
-    if (it0 < it1) gload(it0);
-    for (int it = it0; it < it1; ++it) {
-        lstore();
         __syncthreads();
-        if (it + 1 < it1) gload(it + 1);
+        if (it + 1 < it1) {
+            stage_a<NTH>(ra, Ahi, Alo, rows_pad, row0, (int64_t)(it + 1) * kKG, tid);
+            stage_z<NTH, NW>(rf, Zhi, Zlo, nt, col0, (int64_t)(it + 1) * kKG, tid);
+        }
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int kg = 2 * ks + (lane >> 5);
-            union { uint4 u; half8 v; } ah[2], al[2], fh[TN], fl[TN];
+            half8 ah[2], al[2], fh[TN], fl[TN];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const int r = wr * 64 + t * 32 + (lane & 31);
-                ah[t].u = sA[0][kg][r];
-                al[t].u = sA[1][kg][r];
+                ah[t] = __builtin_bit_cast(half8, sA[0][kg][r]);
+                al[t] = __builtin_bit_cast(half8, sA[1][kg][r]);
             }
 #pragma unroll
             for (int u = 0; u < TN; ++u) {
-                const int col = wc * (NT / 2) + u * 32 + (lane & 31);
-                fh[u].u = sF[0][kg][col];
-                fl[u].u = sF[1][kg][col];
+                const int col = wc * WN + u * 32 + (lane & 31);
+                fh[u] = __builtin_bit_cast(half8, sF[0][kg][col]);
+                fl[u] = __builtin_bit_cast(half8, sF[1][kg][col]);
             }
 #pragma unroll
             for (int t = 0; t < 2; ++t)
 #pragma unroll
                 for (int u = 0; u < TN; ++u) {
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t].v, fh[u].v, acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t].v, fl[u].v, acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[t].v, fh[u].v, acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fh[u], acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fl[u], acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[t], fh[u], acc[t][u], 0, 0, 0);
                 }
         }
         __syncthreads();
+        if constexpr (FOLD > 0) {
+            if (++since == FOLD) {
+                since = 0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int u = 0; u < TN; ++u) {
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) sum[t][u][e] += (double)acc[t][u][e];
+                        acc[t][u] = floatx16{0};
+                    }
+            }
+        }
     }
     // C tile -> P[split][member][row]; lane holds member (lane & 31) of each
-    // 32-wide tile and rows 8*(r/4) + 4*(lane/32) + (r%4) of each 32-row tile
+    // 32-wide tile and rows 8*(e/4) + 4*(lane/32) + (e%4) of each 32-row tile
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u) {
-            const int col = wc * (NT / 2) + u * 32 + (lane & 31);
-            float* dst = P + ((int64_t)split * NT + col) * rows_pad + row0 + wr * 64 + t * 32;
+            const int col = col0 + wc * WN + u * 32 + (lane & 31);
+            typename Partial<FOLD>::type* dst =
+                P + ((int64_t)split * nt + col) * rows_pad + row0 + wr * 64 + t * 32 + 4 * (lane >> 5);
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                floatx4 v = {acc[t][u][4 * g + 0], acc[t][u][4 * g + 1], acc[t][u][4 * g + 2],
-                             acc[t][u][4 * g + 3]};
-                *reinterpret_cast<floatx4*>(dst + 8 * g + 4 * (lane >> 5)) = v;
+            for (int e = 0; e < 16; ++e) {
+                if constexpr (FOLD > 0)
+                    dst[8 * (e / 4) + (e % 4)] = sum[t][u][e] + (double)acc[t][u][e];
+                else
+                    dst[8 * (e / 4) + (e % 4)] = acc[t][u][e];
             }
         }
 }
 
 // ------------------------------------------------------------------ epilogue
-// One thread per (sorted node p, member b): FP64 sum of the split partials,
-// Pauli-blocked assembly with the member's bath occupation, result written
-// to the caller's node order; per-block partials of sum V I and
-// sum V |f(1-f)| for the number-defect removal.
-constexpr int kFinT = 256;
-__global__ void __launch_bounds__(kFinT)
-mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t rows_pad,
-                    int64_t n, int64_t kt, const double* __restrict__ f, const int32_t* __restrict__ perm,
-                    const double* __restrict__ rk, const double* __restrict__ ck,
-                    const double* __restrict__ V, const double* __restrict__ bath,
-                    const uint8_t* __restrict__ useg, double inv_scale, double* __restrict__ I,
-                    double* __restrict__ part, int nblk) {
-    const int b = blockIdx.y;
-    const int64_t p = blockIdx.x * (int64_t)kFinT + threadIdx.x;
+// One workgroup per member: FP64 sum of the split partials, Pauli-blocked
+// assembly with the member's bath occupation, FP64 block sums of sum V I and
+// sum V |f(1-f)| for the number-defect removal, result in the caller's node
+// order.  With ROW the result is scattered into an LDS row and written out
+// by coalesced stores together with the correction; without it the scatter
+// goes to global memory and the correction is a second pass.
+template <bool ROW, typename PT>
+__global__ void __launch_bounds__(kMemT)
+finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
+                     int64_t kt, int B, const double* __restrict__ fs, const double* __restrict__ f,
+                     const int32_t* __restrict__ perm, const double* __restrict__ rk,
+                     const double* __restrict__ ck, const double* __restrict__ V,
+                     const double* __restrict__ bath, const uint8_t* __restrict__ useg,
+                     double inv_scale, int conserve, double* __restrict__ I) {
+    extern __shared__ double row[];
+    __shared__ double red[2][kMemT / 64];
+    __shared__ double s_b;
+    const int b = member_of_block(blockIdx.x, nt);
+    if (b >= B) return;
+    const double nb = bath[b];
+    const double np1 = nb + 1.0;
+    const double* fsb = fs + (int64_t)b * n;
+    double* Ib = I + (int64_t)b * n;
     double dI = 0.0, dw = 0.0;
-    if (p < n) {
+    for (int64_t p = threadIdx.x; p < n; p += kMemT) {
         double u = 0.0, v = 0.0;
         for (int s = 0; s < nsplit; ++s) {
-            const float* Ps = P + ((int64_t)s * nt + b) * rows_pad;
+            const PT* Ps = P + ((int64_t)s * nt + b) * rows_pad;
             u += (double)Ps[p];
             v += (double)Ps[kt + p];
         }
@@ -357,62 +458,91 @@ mixed_finish_kernel(const float* __restrict__ P, int nsplit, int nt, int64_t row
         const bool gv = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
         const double Kf = gu ? rk[p] - u : u, Kg = gu ? u : rk[p] - u;
         const double KTf = gv ? ck[p] - v : v, KTg = gv ? v : ck[p] - v;
-        const double nb = bath[b];
-        const double np1 = nb + 1.0;
-        const int64_t node = perm[p];
-        const double fi = f[(int64_t)b * n + node];
+        const double fi = fsb[p];
         // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
         const double Ii = ((1.0 - fi) * (np1 * Kf + nb * KTf) - fi * (np1 * KTg + nb * Kg)) / V[p];
-        I[(int64_t)b * n + node] = Ii;
-        dI = V[p] * Ii;
-        dw = V[p] * fabs(fi * (1.0 - fi));
+        dI += V[p] * Ii;
+        dw += V[p] * fabs(fi * (1.0 - fi));
+        if constexpr (ROW)
+            row[perm[p]] = Ii;
+        else
+            Ib[perm[p]] = Ii;
     }
-    if (part) {
-        __shared__ double sh[2][kFinT / 64];
+    double sc = 0.0;
+    if (conserve) {
         for (int o = 32; o > 0; o >>= 1) {
             dI += __shfl_down(dI, o, 64);
             dw += __shfl_down(dw, o, 64);
         }
         if ((threadIdx.x & 63) == 0) {
-            sh[0][threadIdx.x >> 6] = dI;
-            sh[1][threadIdx.x >> 6] = dw;
+            red[0][threadIdx.x >> 6] = dI;
+            red[1][threadIdx.x >> 6] = dw;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             double a = 0.0, c = 0.0;
-            for (int w = 0; w < kFinT / 64; ++w) {
-                a += sh[0][w];
-                c += sh[1][w];
+            for (int w = 0; w < kMemT / 64; ++w) {
+                a += red[0][w];
+                c += red[1][w];
             }
-            part[((int64_t)b * nblk + blockIdx.x) * 2] = a;
-            part[((int64_t)b * nblk + blockIdx.x) * 2 + 1] = c;
+            s_b = c > 0.0 ? a / c : 0.0;
         }
-    }
-}
-
-// per member: s_b = sum V I / sum V |f(1-f)|; I -= s_b |f(1-f)|
-__global__ void mixed_conserve_kernel(const double* __restrict__ part, int nblk, int64_t n,
-                                      const double* __restrict__ f, double* __restrict__ I) {
-    const int b = blockIdx.y;
-    __shared__ double s_b;
-    if (threadIdx.x == 0) {
-        double d = 0.0, w = 0.0;
-        for (int k = 0; k < nblk; ++k) {
-            d += part[((int64_t)b * nblk + k) * 2];
-            w += part[((int64_t)b * nblk + k) * 2 + 1];
-        }
-        s_b = w > 0.0 ? d / w : 0.0;
     }
     __syncthreads();
-    const double s = s_b;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double fi = f[(int64_t)b * n + i];
-        I[(int64_t)b * n + i] -= s * fabs(fi * (1.0 - fi));
+    if (conserve) sc = s_b;
+    const double* fb = f + (int64_t)b * n;
+    if constexpr (ROW) {
+        for (int64_t i = threadIdx.x; i < n; i += kMemT) {
+            double Ii = row[i];
+            if (conserve) {
+                const double fi = fb[i];
+                Ii -= sc * fabs(fi * (1.0 - fi));
+            }
+            Ib[i] = Ii;
+        }
+    } else if (conserve) {
+        for (int64_t i = threadIdx.x; i < n; i += kMemT) {
+            const double fi = fb[i];
+            Ib[i] -= sc * fabs(fi * (1.0 - fi));
+        }
     }
 }
 
 // ------------------------------------------------------------------- host
+template <int NW, int WAVES, int FOLD>
+static void launch_gemm_t(tg_ctx* c, MixedData* m) {
+    const int nmt = m->nt / NW;
+    const int64_t nblocks = (int64_t)m->npanel * m->nsplit * nmt;
+    const int64_t per = (nblocks + 7) / 8;
+    hipLaunchKernelGGL((mixed_gemm_kernel<NW, WAVES, FOLD>), dim3((unsigned)(8 * per)), dim3(64 * WAVES),
+                       0, c->stream, (const u32x4*)m->A_hi, (const u32x4*)m->A_lo,
+                       (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
+                       (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, nmt, m->nsplit,
+                       nblocks, per, (typename Partial<FOLD>::type*)m->P);
+}
+
+// the tile configurations the library carries: (members, waves, fold)
+#define TG_MIXED_CONFIGS(X) \
+    X(64, 4, 0) X(64, 4, 1) X(64, 4, 2) X(128, 4, 0) X(128, 4, 1) X(128, 4, 2) X(128, 8, 1) \
+    X(128, 8, 2) X(256, 4, 0) X(256, 4, 1) X(256, 8, 0) X(256, 8, 1) X(256, 8, 2)
+
+static bool config_known(int nw, int waves, int fold) {
+#define TG_X(a, b, c_) if (nw == a && waves == b && fold == c_) return true;
+    TG_MIXED_CONFIGS(TG_X)
+#undef TG_X
+    return false;
+}
+
+static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
+    if (m->nt % m->nw != 0) return set_error(c, TG_ERR_ARG, "tile width %d does not divide %d", m->nw, m->nt);
+#define TG_X(a, b, c_) if (m->nw == a && m->waves == b && m->fold == c_) launch_gemm_t<a, b, c_>(c, m); else
+    TG_MIXED_CONFIGS(TG_X)
+#undef TG_X
+    return set_error(c, TG_ERR_ARG, "no kernel for tile %d members, %d waves, fold %d", m->nw, m->waves, m->fold);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
 static int nt_for(int B) {
     if (B <= 64) return 64;
     if (B <= 128) return 128;
@@ -503,18 +633,50 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
-    // K-splits: ~16 workgroups per CU (fills the 256 CUs and shortens every
-    // FP32 accumulation chain; the splits are summed in FP64), each split a
-    // few iterations long
-    const double avg = (double)items / std::max(1, m->npanel);
-    int ns = (int)std::ceil(4096.0 / m->npanel);
-    ns = std::max(1, std::min(ns, (int)std::max(1.0, avg / 3.0)));
-    m->nsplit = std::min(ns, 32);
-    // explicit override (measurements of the split count)
+    // tile width, waves, FP64 fold interval and K-splits
+    m->nw = std::min(64, m->nt);
+    m->waves = 4;
+    m->fold = 1;
+    {
+        const int nmt = m->nt / m->nw;
+        // enough workgroups to load all 256 CUs a few times over
+        int ns = (int)std::ceil(1024.0 / ((double)m->npanel * nmt));
+        m->nsplit = std::max(1, std::min(ns, 8));
+    }
+    // explicit overrides (measurements of the tile configuration)
+    if (const char* e = std::getenv("TG_MIXED_NW")) {
+        const int v = std::atoi(e);
+        if ((v == 64 || v == 128 || v == 256) && v <= m->nt) m->nw = v;
+    }
+    if (const char* e = std::getenv("TG_MIXED_FOLD")) m->fold = std::atoi(e);
+    if (const char* e = std::getenv("TG_MIXED_WAVES")) m->waves = std::atoi(e);
+    if (!config_known(m->nw, m->waves, m->fold)) {
+        m->nw = std::min(64, m->nt);
+        m->waves = 4;
+        m->fold = 1;
+    }
     if (const char* e = std::getenv("TG_MIXED_SPLITS")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 64) m->nsplit = v;
     }
+    // member rows in LDS when they fit (with the epilogue's reduction scratch)
+    {
+        hipDeviceProp_t prop;
+        TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
+        const size_t rowbytes = sizeof(double) * (size_t)n;
+        m->rowcache = rowbytes + 1024 <= (size_t)prop.sharedMemPerBlock;
+        if (const char* e = std::getenv("TG_MIXED_ROWCACHE"))
+            if (std::atoi(e) == 0) m->rowcache = false;
+        if (m->rowcache) {
+            TG_HIP(c, hipFuncSetAttribute((const void*)pack_member_kernel<true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
+            TG_HIP(c, hipFuncSetAttribute((const void*)finish_member_kernel<true, double>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
+            TG_HIP(c, hipFuncSetAttribute((const void*)finish_member_kernel<true, float>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
+        }
+    }
+    TG_TRY(dev_alloc(c, &m->fs, (size_t)n * m->nt));
     TG_TRY(dev_alloc(c, &m->F_hi, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->F_lo, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->G_hi, (size_t)kgroups * m->nt));
@@ -532,6 +694,8 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_splits = m->nsplit;
     out->mixed_width = m->nt;
     out->mixed_scale = m->sa;
+    out->mixed_tile = m->nw;
+    out->mixed_fold = m->fold;
 }
 
 void mixed_destroy(tg_ctx* c) {
@@ -549,7 +713,7 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->ck_s);
     dev_free(&m->P);
     dev_free(&m->bath);
-    dev_free(&m->part);
+    dev_free(&m->fs);
     delete m;
     c->mixed = nullptr;
 }
@@ -562,9 +726,11 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     hipStream_t st = c->stream;
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
-    const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad;
+    const size_t psize = m->fold ? sizeof(double) : sizeof(float);
+    const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad * psize;
     if (m->P_cap < need) {
         dev_free(&m->P);
+        m->P_cap = 0;
         TG_TRY(dev_alloc(c, &m->P, need));
         m->P_cap = need;
     }
@@ -572,53 +738,36 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     for (int b = 0; b < B; ++b) bath[b] = h_bath ? h_bath[b] : c->bath_n;
     TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
 
-    hipLaunchKernelGGL(pack_F_kernel, dim3(ceil_div(m->nt, 64), kgroups), dim3(64), 0, st, d_f, n,
-                       B, m->perm, kgroups, m->nt, m->F_hi, m->F_lo, m->G_hi, m->G_lo);
+    const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
+    if (m->rowcache)
+        hipLaunchKernelGGL(pack_member_kernel<true>, dim3(m->nt), dim3(kMemT), rowbytes, st, d_f, n, B,
+                           m->perm, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
+                           (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->fs);
+    else
+        hipLaunchKernelGGL(pack_member_kernel<false>, dim3(m->nt), dim3(kMemT), 0, st, d_f, n, B,
+                           m->perm, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
+                           (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->fs);
     TG_CHECK_LAUNCH(c);
     hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, st, d_f, n, m->kt, B,
                        m->npanel, m->perm, m->useg, m->nt);
     TG_CHECK_LAUNCH(c);
-    const dim3 grid(m->npanel, m->nsplit);
     time_begin(c);
-    switch (m->nt) {
-        case 64:
-            hipLaunchKernelGGL(mixed_gemm_kernel<64>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
-                               m->krange, m->nsplit, m->P);
-            break;
-        case 128:
-            hipLaunchKernelGGL(mixed_gemm_kernel<128>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
-                               m->krange, m->nsplit, m->P);
-            break;
-        default:
-            hipLaunchKernelGGL(mixed_gemm_kernel<256>, grid, dim3(kMixThreads), 0, st, m->A_hi,
-                               m->A_lo, m->F_hi, m->F_lo, m->G_hi, m->G_lo, m->useg, m->rows_pad,
-                               m->krange, m->nsplit, m->P);
-            break;
-    }
+    TG_TRY(launch_gemm(c, m));
     time_end(c);
-    TG_CHECK_LAUNCH(c);
-    const int nblk = (int)ceil_div(n, kFinT);
-    double* part = nullptr;
-    if (conserve) {
-        if (m->part_cap < nblk * B) {
-            dev_free(&m->part);
-            TG_TRY(dev_alloc(c, &m->part, (size_t)2 * nblk * B));
-            m->part_cap = nblk * B;
-        }
-        part = m->part;
-    }
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
-    hipLaunchKernelGGL(mixed_finish_kernel, dim3(nblk, B), dim3(kFinT), 0, st, m->P, m->nsplit, m->nt,
-                       m->rows_pad, n, m->kt, d_f, m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg,
-                       inv_scale, d_I, part, nblk);
-    TG_CHECK_LAUNCH(c);
-    if (conserve) {
-        hipLaunchKernelGGL(mixed_conserve_kernel, dim3(ceil_div(n, 1024), B), dim3(256), 0, st,
-                           part, nblk, n, d_f, d_I);
-        TG_CHECK_LAUNCH(c);
+#define TG_FINISH(ROW, PT)                                                                         \
+    hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
+                       st, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, m->fs, d_f,   \
+                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve, d_I)
+    if (m->fold) {
+        if (m->rowcache) TG_FINISH(true, double);
+        else TG_FINISH(false, double);
+    } else {
+        if (m->rowcache) TG_FINISH(true, float);
+        else TG_FINISH(false, float);
     }
+#undef TG_FINISH
+    TG_CHECK_LAUNCH(c);
     return TG_OK;
 }
 

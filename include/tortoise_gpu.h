@@ -112,6 +112,10 @@ typedef struct tg_kernel_stats {
     double  kmax;           /* largest row sum of K */
     int32_t mixed_tile;     /* members per workgroup of the batched kernel */
     int32_t mixed_fold;     /* iterations per FP64 fold of its FP32 sums (0: FP32 partials) */
+    int32_t mixed_waves;    /* waves per workgroup of the batched kernel */
+    int32_t mixed_depth;    /* iterations of register prefetch of the batched kernel */
+    float   mixed_tuned_ms; /* autotuner's time of the chosen configuration (0: set explicitly) */
+    int32_t reserved_;
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

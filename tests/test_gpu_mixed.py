@@ -121,10 +121,11 @@ def test_mixed_member_rows_outside_lds(monkeypatch):
 
 
 # every tile configuration the library carries: (members per workgroup,
-# waves, FP64 fold interval, K-splits)
-TILE_CONFIGS = [(64, 4, 0, 16), (64, 4, 1, 1), (64, 4, 2, 3), (128, 4, 0, 8), (128, 4, 1, 2),
-                (128, 4, 2, 1), (128, 8, 1, 1), (128, 8, 2, 2), (256, 4, 0, 8), (256, 4, 1, 1),
-                (256, 8, 0, 4), (256, 8, 1, 2), (256, 8, 2, 1)]
+# waves, FP64 fold interval, prefetch depth), each with a K-split count
+TILE_CONFIGS = [(64, 4, 0, 1, 16), (64, 4, 1, 1, 1), (64, 4, 1, 2, 3), (64, 4, 1, 3, 2),
+                (64, 4, 2, 2, 1), (128, 4, 0, 1, 8), (128, 4, 1, 1, 2), (128, 8, 1, 1, 1),
+                (128, 8, 1, 2, 2), (128, 8, 1, 3, 1), (128, 8, 2, 2, 3), (256, 4, 0, 1, 8),
+                (256, 4, 0, 2, 4), (256, 8, 0, 2, 8)]
 
 
 def test_mixed_tile_configurations(monkeypatch):
@@ -134,21 +135,23 @@ def test_mixed_tile_configurations(monkeypatch):
     B = 256
     F, bath = _members(w, B, seed=11)
     ref = None
-    for nw, waves, fold, ns in TILE_CONFIGS:
-        print(f"[gpu] tile config {nw} {waves} {fold} {ns}", flush=True)
+    for nw, waves, fold, depth, ns in TILE_CONFIGS:
+        print(f"[gpu] tile config {nw} {waves} {fold} {depth} {ns}", flush=True)
         monkeypatch.setenv("TG_MIXED_NW", str(nw))
         monkeypatch.setenv("TG_MIXED_WAVES", str(waves))
         monkeypatch.setenv("TG_MIXED_FOLD", str(fold))
+        monkeypatch.setenv("TG_MIXED_DEPTH", str(depth))
         monkeypatch.setenv("TG_MIXED_SPLITS", str(ns))
         ctx = tg.Context.create(w, max_batch=B)
         ctx.assemble(mixed=True)
         st = ctx.kernel_stats()
-        assert (st["mixed_tile"], st["mixed_fold"], st["mixed_splits"]) == (nw, fold, ns)
+        assert (st["mixed_tile"], st["mixed_waves"], st["mixed_fold"], st["mixed_depth"],
+                st["mixed_splits"]) == (nw, waves, fold, depth, ns)
         if ref is None:
             ref = ctx.apply(F[:, ::37], bath_n=bath[::37])
         Im = ctx.apply(F, bath_n=bath, mixed=True)
         for q, b in enumerate(range(0, B, 37)):
-            assert _rel(Im[:, b], ref[:, q]) <= 2e-6, (nw, waves, fold, ns, b)
+            assert _rel(Im[:, b], ref[:, q]) <= 2e-6, (nw, waves, fold, depth, ns, b)
         ctx.close()
 
 

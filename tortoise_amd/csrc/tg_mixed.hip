@@ -70,6 +70,8 @@ struct MixedData {
     int nw = 128;              // members per workgroup tile
     int fold = 2;              // iterations per FP64 fold (0: FP32 partials)
     int waves = 4;             // waves per workgroup
+    int depth = 2;             // iterations of register prefetch
+    float tuned_ms = 0.0f;     // GEMM time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
     double* fs = nullptr;      // f in sorted order, FP64, member-major (nt x n)
@@ -280,7 +282,7 @@ __device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* __restrict_
     }
 }
 
-template <int NW, int WAVES, int FOLD>
+template <int NW, int WAVES, int FOLD, int DEPTH>
 __global__ void __launch_bounds__(64 * WAVES)
 mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
                   const u32x4* __restrict__ Fhi, const u32x4* __restrict__ Flo,
@@ -337,66 +339,84 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
     const bool my_g = useg[(int64_t)panel * nt + col0 + (tid % NW)] != 0;
     const u32x4* Zhi = my_g ? Ghi : Fhi;
     const u32x4* Zlo = my_g ? Glo : Flo;
-    u32x4 ra[A_PER_T], rf[F_PER_T];
-
+    // DEPTH register buffers: iteration it is staged from buffer (it - it0) %
+    // DEPTH, whose loads were issued DEPTH iterations earlier.  Every
+    // iteration of the padded count issues its loads and LDS stores
+    // unconditionally (beyond it1 the index is clamped and the tile reloaded),
+    // so the wait before staging a buffer counts exactly the DEPTH-1 newer
+    // buffers still in flight; only the MFMAs are skipped past it1.
+    u32x4 ra[DEPTH][A_PER_T], rf[DEPTH][F_PER_T];
     if (it0 < it1) {
-        stage_a<NTH>(ra, Ahi, Alo, rows_pad, row0, (int64_t)it0 * kKG, tid);
-        stage_z<NTH, NW>(rf, Zhi, Zlo, nt, col0, (int64_t)it0 * kKG, tid);
-    }
-    int since = 0;
-    for (int it = it0; it < it1; ++it) {
+        const int last = it1 - 1;
+        const int nit_pad = (it1 - it0 + DEPTH - 1) / DEPTH * DEPTH;
 #pragma unroll
-        for (int j = 0; j < A_PER_T; ++j) {
-            const int q = tid + j * NTH;
-            sA[q / (kKG * kMT)][(q / kMT) % kKG][q % kMT] = ra[j];
+        for (int d = 0; d < DEPTH; ++d) {
+            const int il = min(it0 + d, last);
+            stage_a<NTH>(ra[d], Ahi, Alo, rows_pad, row0, (int64_t)il * kKG, tid);
+            stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
         }
+        int since = 0;
+        for (int itb = it0; itb < it0 + nit_pad; itb += DEPTH) {
 #pragma unroll
-        for (int j = 0; j < F_PER_T; ++j) {
-            const int q = tid + j * NTH;
-            sF[q / (kKG * NW)][(q / NW) % kKG][q % NW] = rf[j];
-        }
-        __syncthreads();
-        if (it + 1 < it1) {
-            stage_a<NTH>(ra, Ahi, Alo, rows_pad, row0, (int64_t)(it + 1) * kKG, tid);
-            stage_z<NTH, NW>(rf, Zhi, Zlo, nt, col0, (int64_t)(it + 1) * kKG, tid);
-        }
+            for (int d = 0; d < DEPTH; ++d) {
+                const int it = itb + d;
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int kg = 2 * ks + (lane >> 5);
-            half8 ah[2], al[2], fh[TN], fl[TN];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int r = wr * 64 + t * 32 + (lane & 31);
-                ah[t] = __builtin_bit_cast(half8, sA[0][kg][r]);
-                al[t] = __builtin_bit_cast(half8, sA[1][kg][r]);
-            }
-#pragma unroll
-            for (int u = 0; u < TN; ++u) {
-                const int col = wc * WN + u * 32 + (lane & 31);
-                fh[u] = __builtin_bit_cast(half8, sF[0][kg][col]);
-                fl[u] = __builtin_bit_cast(half8, sF[1][kg][col]);
-            }
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int u = 0; u < TN; ++u) {
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fh[u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fl[u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[t], fh[u], acc[t][u], 0, 0, 0);
+                for (int j = 0; j < A_PER_T; ++j) {
+                    const int q = tid + j * NTH;
+                    sA[q / (kKG * kMT)][(q / kMT) % kKG][q % kMT] = ra[d][j];
                 }
-        }
-        __syncthreads();
-        if constexpr (FOLD > 0) {
-            if (++since == FOLD) {
-                since = 0;
 #pragma unroll
-                for (int t = 0; t < 2; ++t)
+                for (int j = 0; j < F_PER_T; ++j) {
+                    const int q = tid + j * NTH;
+                    sF[q / (kKG * NW)][(q / NW) % kKG][q % NW] = rf[d][j];
+                }
+                __syncthreads();
+                {
+                    const int il = min(it + DEPTH, last);
+                    stage_a<NTH>(ra[d], Ahi, Alo, rows_pad, row0, (int64_t)il * kKG, tid);
+                    stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
+                }
+                if (it < it1) {
 #pragma unroll
-                    for (int u = 0; u < TN; ++u) {
+                    for (int ks = 0; ks < 2; ++ks) {
+                        const int kg = 2 * ks + (lane >> 5);
+                        half8 ah[2], al[2], fh[TN], fl[TN];
 #pragma unroll
-                        for (int e = 0; e < 16; ++e) sum[t][u][e] += (double)acc[t][u][e];
-                        acc[t][u] = floatx16{0};
+                        for (int t = 0; t < 2; ++t) {
+                            const int r = wr * 64 + t * 32 + (lane & 31);
+                            ah[t] = __builtin_bit_cast(half8, sA[0][kg][r]);
+                            al[t] = __builtin_bit_cast(half8, sA[1][kg][r]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < TN; ++u) {
+                            const int col = wc * WN + u * 32 + (lane & 31);
+                            fh[u] = __builtin_bit_cast(half8, sF[0][kg][col]);
+                            fl[u] = __builtin_bit_cast(half8, sF[1][kg][col]);
+                        }
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+#pragma unroll
+                            for (int u = 0; u < TN; ++u) {
+                                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fh[u], acc[t][u], 0, 0, 0);
+                                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[t], fl[u], acc[t][u], 0, 0, 0);
+                                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[t], fh[u], acc[t][u], 0, 0, 0);
+                            }
                     }
+                }
+                __syncthreads();
+                if constexpr (FOLD > 0) {
+                    if (it < it1 && ++since == FOLD) {
+                        since = 0;
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+#pragma unroll
+                            for (int u = 0; u < TN; ++u) {
+#pragma unroll
+                                for (int e = 0; e < 16; ++e) sum[t][u][e] += (double)acc[t][u][e];
+                                acc[t][u] = floatx16{0};
+                            }
+                    }
+                }
             }
         }
     }
@@ -509,25 +529,26 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
 }
 
 // ------------------------------------------------------------------- host
-template <int NW, int WAVES, int FOLD>
+template <int NW, int WAVES, int FOLD, int DEPTH>
 static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     const int nmt = m->nt / NW;
     const int64_t nblocks = (int64_t)m->npanel * m->nsplit * nmt;
     const int64_t per = (nblocks + 7) / 8;
-    hipLaunchKernelGGL((mixed_gemm_kernel<NW, WAVES, FOLD>), dim3((unsigned)(8 * per)), dim3(64 * WAVES),
+    hipLaunchKernelGGL((mixed_gemm_kernel<NW, WAVES, FOLD, DEPTH>), dim3((unsigned)(8 * per)), dim3(64 * WAVES),
                        0, c->stream, (const u32x4*)m->A_hi, (const u32x4*)m->A_lo,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, nmt, m->nsplit,
                        nblocks, per, (typename Partial<FOLD>::type*)m->P);
 }
 
-// the tile configurations the library carries: (members, waves, fold)
-#define TG_MIXED_CONFIGS(X) \
-    X(64, 4, 0) X(64, 4, 1) X(64, 4, 2) X(128, 4, 0) X(128, 4, 1) X(128, 4, 2) X(128, 8, 1) \
-    X(128, 8, 2) X(256, 4, 0) X(256, 4, 1) X(256, 8, 0) X(256, 8, 1) X(256, 8, 2)
+// the tile configurations the library carries: (members, waves, fold, prefetch depth)
+#define TG_MIXED_CONFIGS(X)                                                                     \
+    X(64, 4, 0, 1) X(64, 4, 1, 1) X(64, 4, 1, 2) X(64, 4, 1, 3) X(64, 4, 2, 2) X(128, 4, 0, 1)  \
+    X(128, 4, 1, 1) X(128, 8, 1, 1) X(128, 8, 1, 2) X(128, 8, 1, 3) X(128, 8, 2, 2)             \
+    X(256, 4, 0, 1) X(256, 4, 0, 2) X(256, 8, 0, 2)
 
-static bool config_known(int nw, int waves, int fold) {
-#define TG_X(a, b, c_) if (nw == a && waves == b && fold == c_) return true;
+static bool config_known(int nw, int waves, int fold, int depth) {
+#define TG_X(a, b, c_, d_) if (nw == a && waves == b && fold == c_ && depth == d_) return true;
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
     return false;
@@ -535,11 +556,88 @@ static bool config_known(int nw, int waves, int fold) {
 
 static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     if (m->nt % m->nw != 0) return set_error(c, TG_ERR_ARG, "tile width %d does not divide %d", m->nw, m->nt);
-#define TG_X(a, b, c_) if (m->nw == a && m->waves == b && m->fold == c_) launch_gemm_t<a, b, c_>(c, m); else
+#define TG_X(a, b, c_, d_)                                                        \
+    if (m->nw == a && m->waves == b && m->fold == c_ && m->depth == d_) launch_gemm_t<a, b, c_, d_>(c, m); else
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
-    return set_error(c, TG_ERR_ARG, "no kernel for tile %d members, %d waves, fold %d", m->nw, m->waves, m->fold);
+    return set_error(c, TG_ERR_ARG, "no kernel for tile %d members, %d waves, fold %d, depth %d", m->nw,
+                     m->waves, m->fold, m->depth);
     TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+static tg_status ensure_P(tg_ctx* c, MixedData* m) {
+    const size_t psize = m->fold ? sizeof(double) : sizeof(float);
+    const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad * psize;
+    if (m->P_cap < need) {
+        dev_free(&m->P);
+        m->P_cap = 0;
+        TG_TRY(dev_alloc(c, &m->P, need));
+        m->P_cap = need;
+    }
+    return TG_OK;
+}
+
+// Autotuner: times every accuracy-safe tile configuration (FP64 fold, so no
+// K-split is needed for accuracy) with one and two K-splits on this band and
+// keeps the fastest.  The operands are zeroed first; the GEMM's time does not
+// depend on their values.
+static tg_status autotune(tg_ctx* c, MixedData* m) {
+    struct Cand { int nw, waves, fold, depth; };
+    std::vector<Cand> cands;
+#define TG_X(a, b, c_, d_) if (c_ > 0 && a <= m->nt) cands.push_back(Cand{a, b, c_, d_});
+    TG_MIXED_CONFIGS(TG_X)
+#undef TG_X
+    if (cands.empty()) return TG_OK;
+    hipStream_t st = c->stream;
+    const int64_t kgroups = m->kpad / 8;
+    TG_HIP(c, hipMemsetAsync(m->F_hi, 0, sizeof(uint4) * kgroups * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->F_lo, 0, sizeof(uint4) * kgroups * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->G_hi, 0, sizeof(uint4) * kgroups * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->G_lo, 0, sizeof(uint4) * kgroups * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->useg, 0, (size_t)m->npanel * m->nt, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TG_HIP(c, hipEventCreate(&e0));
+    TG_HIP(c, hipEventCreate(&e1));
+    float best = 0.0f;
+    Cand pick = cands[0];
+    int pick_split = 1;
+    tg_status status = TG_OK;
+    for (const Cand& cd : cands) {
+        for (int ns = 1; ns <= 2 && status == TG_OK; ++ns) {
+            m->nw = cd.nw;
+            m->waves = cd.waves;
+            m->fold = cd.fold;
+            m->depth = cd.depth;
+            m->nsplit = ns;
+            status = ensure_P(c, m);
+            if (status == TG_OK) status = launch_gemm(c, m);     // warm-up
+            if (status != TG_OK) break;
+            if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
+            for (int r = 0; r < 3 && status == TG_OK; ++r) status = launch_gemm(c, m);
+            if (status != TG_OK) break;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                status = set_error(c, TG_ERR_HIP, "event timing");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (best == 0.0f || ms < best) {
+                best = ms;
+                pick = cd;
+                pick_split = ns;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TG_TRY(status);
+    m->nw = pick.nw;
+    m->waves = pick.waves;
+    m->fold = pick.fold;
+    m->depth = pick.depth;
+    m->nsplit = pick_split;
+    m->tuned_ms = best / 3.0f;
     return TG_OK;
 }
 
@@ -637,23 +735,31 @@ tg_status mixed_prepare(tg_ctx* c) {
     m->nw = std::min(64, m->nt);
     m->waves = 4;
     m->fold = 1;
+    m->depth = 2;
     {
         const int nmt = m->nt / m->nw;
         // enough workgroups to load all 256 CUs a few times over
         int ns = (int)std::ceil(1024.0 / ((double)m->npanel * nmt));
         m->nsplit = std::max(1, std::min(ns, 8));
     }
-    // explicit overrides (measurements of the tile configuration)
+    // explicit overrides (measurements of the tile configuration) replace the autotuner
+    bool fixed = false;
+    for (const char* v : {"TG_MIXED_NW", "TG_MIXED_FOLD", "TG_MIXED_WAVES", "TG_MIXED_DEPTH", "TG_MIXED_SPLITS"})
+        if (std::getenv(v)) fixed = true;
+    if (const char* e = std::getenv("TG_MIXED_AUTOTUNE"))
+        if (std::atoi(e) == 0) fixed = true;
     if (const char* e = std::getenv("TG_MIXED_NW")) {
         const int v = std::atoi(e);
         if ((v == 64 || v == 128 || v == 256) && v <= m->nt) m->nw = v;
     }
     if (const char* e = std::getenv("TG_MIXED_FOLD")) m->fold = std::atoi(e);
     if (const char* e = std::getenv("TG_MIXED_WAVES")) m->waves = std::atoi(e);
-    if (!config_known(m->nw, m->waves, m->fold)) {
+    if (const char* e = std::getenv("TG_MIXED_DEPTH")) m->depth = std::atoi(e);
+    if (!config_known(m->nw, m->waves, m->fold, m->depth)) {
         m->nw = std::min(64, m->nt);
         m->waves = 4;
         m->fold = 1;
+        m->depth = 2;
     }
     if (const char* e = std::getenv("TG_MIXED_SPLITS")) {
         const int v = std::atoi(e);
@@ -683,6 +789,7 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->G_lo, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
+    if (!fixed) TG_TRY(autotune(c, m));
     TG_HIP(c, hipStreamSynchronize(st));
     return TG_OK;
 }
@@ -696,6 +803,9 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_scale = m->sa;
     out->mixed_tile = m->nw;
     out->mixed_fold = m->fold;
+    out->mixed_waves = m->waves;
+    out->mixed_depth = m->depth;
+    out->mixed_tuned_ms = m->tuned_ms;
 }
 
 void mixed_destroy(tg_ctx* c) {
@@ -726,14 +836,7 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     hipStream_t st = c->stream;
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
-    const size_t psize = m->fold ? sizeof(double) : sizeof(float);
-    const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad * psize;
-    if (m->P_cap < need) {
-        dev_free(&m->P);
-        m->P_cap = 0;
-        TG_TRY(dev_alloc(c, &m->P, need));
-        m->P_cap = need;
-    }
+    TG_TRY(ensure_P(c, m));
     std::vector<double> bath(B);
     for (int b = 0; b < B; ++b) bath[b] = h_bath ? h_bath[b] : c->bath_n;
     TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));

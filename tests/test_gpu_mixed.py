@@ -107,6 +107,11 @@ def test_mixed_member_rows_outside_lds(monkeypatch):
     larger than the LDS) give the same bits as the LDS row path."""
     w = wl.SQUARE32
     F, bath = _members(w, 8, seed=7)
+    # one tile configuration for both contexts (the autotuner may pick
+    # different ones, which round differently)
+    for k, v in [("TG_MIXED_NW", "64"), ("TG_MIXED_WAVES", "4"), ("TG_MIXED_FOLD", "1"),
+                 ("TG_MIXED_DEPTH", "2"), ("TG_MIXED_SPLITS", "1")]:
+        monkeypatch.setenv(k, v)
     ca = tg.Context.create(w, max_batch=8)
     ca.assemble(mixed=True)
     Ia = ca.apply(F, bath_n=bath, mixed=True, conserve=True)

@@ -71,7 +71,7 @@ struct MixedData {
     int fold = 2;              // iterations per FP64 fold (0: FP32 partials)
     int waves = 4;             // waves per workgroup
     int depth = 2;             // iterations of register prefetch
-    float tuned_ms = 0.0f;     // GEMM time of the configuration the autotuner picked (0: not tuned)
+    float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
     double* fs = nullptr;      // f in sorted order, FP64, member-major (nt x n)
@@ -82,10 +82,13 @@ struct MixedData {
 static MixedData* md_of(tg_ctx* c) { return (MixedData*)c->mixed; }
 
 // ------------------------------------------------------------------ packing
+// hi = s rounded to fp16 through FP32 (two hardware conversions); a double
+// rounding tie moves hi by one fp16 ulp, which the exact FP64 residual s - hi
+// absorbs, so hi + lo keeps its ~22 significant bits
 __device__ inline void split_fp16(double x, double scale, _Float16& hi, _Float16& lo) {
     const double s = x * scale;
-    hi = (_Float16)s;
-    lo = (_Float16)(s - (double)hi);
+    hi = (_Float16)(float)s;
+    lo = (_Float16)(float)(s - (double)hi);
 }
 
 // A[r][q] from K stored in energy order: r < n -> K[r][q]; the K^T half
@@ -162,42 +165,100 @@ constexpr int kMemT = 1024;
 
 __device__ inline int member_of_block(int w, int nt) { return (w % 8) * (nt / 8) + w / 8; }
 
-// f -> packed sorted fp16 hi/lo of both operands z = f and z = g = 1 - f
-// (the GEMM streams one of them per (panel, member), see
-// choose_operand_kernel) and the FP64 sorted copy fs the epilogue reads
+// f (caller order) -> fs, the FP64 copy in energy order the GEMM operands
+// and the epilogue are built from
 template <bool ROW>
 __global__ void __launch_bounds__(kMemT)
-pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
-                   int64_t kgroups, int nt, u32x4* __restrict__ Fhi, u32x4* __restrict__ Flo,
-                   u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo, double* __restrict__ fs) {
+gather_member_kernel(const double* __restrict__ f, int64_t n, int B,
+                     const int32_t* __restrict__ perm, int nt, double* __restrict__ fs) {
     extern __shared__ double row[];
     const int b = member_of_block(blockIdx.x, nt);
-    const bool live = b < B;
+    if (b >= B) return;
     const double* fb = f + (int64_t)b * n;
+    // UNR independent loads per thread in flight before any is used
+    constexpr int UNR = 8;
     if constexpr (ROW) {
-        if (live)
-            for (int64_t i = threadIdx.x; i < n; i += kMemT) row[i] = fb[i];
+        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * UNR) {
+            double v[UNR];
+#pragma unroll
+            for (int j = 0; j < UNR; ++j) {
+                const int64_t i = i0 + (int64_t)j * kMemT;
+                v[j] = i < n ? fb[i] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < UNR; ++j) {
+                const int64_t i = i0 + (int64_t)j * kMemT;
+                if (i < n) row[i] = v[j];
+            }
+        }
         __syncthreads();
     }
+    double* out = fs + (int64_t)b * n;
+    for (int64_t p0 = threadIdx.x; p0 < n; p0 += (int64_t)kMemT * UNR) {
+        int32_t node[UNR];
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t p = p0 + (int64_t)j * kMemT;
+            node[j] = p < n ? perm[p] : 0;
+        }
+        double v[UNR];
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) v[j] = ROW ? row[node[j]] : fb[node[j]];
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t p = p0 + (int64_t)j * kMemT;
+            if (p < n) out[p] = v[j];
+        }
+    }
+}
+
+// fs -> packed fp16 hi/lo of both operands z = f and z = g = 1 - f (the GEMM
+// streams one of them per (panel, member), see choose_operand_kernel).  A
+// workgroup transposes a tile of 64 members x 64 sorted nodes through LDS so
+// both the fs reads and the 16-byte unit writes are contiguous; members >= B
+// and nodes >= n pack as zero.
+constexpr int kPackT = 256;
+__global__ void __launch_bounds__(kPackT)
+pack_tile_kernel(const double* __restrict__ fs, int64_t n, int B, int64_t kgroups, int nt,
+                 u32x4* __restrict__ Fhi, u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi,
+                 u32x4* __restrict__ Glo) {
+    __shared__ double tile[64][65];
+    const int64_t p0 = (int64_t)blockIdx.x * 64;
+    const int b0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    {
+        const int pos = t & 63;
+        const int64_t p = p0 + pos;
+        constexpr int R = 64 / (kPackT / 64);
+        double v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int b = b0 + (t >> 6) + j * (kPackT / 64);
+            v[j] = (b < B && p < n) ? fs[(int64_t)b * n + p] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) tile[(t >> 6) + j * (kPackT / 64)][pos] = v[j];
+    }
+    __syncthreads();
     const double scale = (double)(1 << kSF);
-    for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
+    const int ml = t & 63;
+    const int b = b0 + ml;
+    for (int kgl = t >> 6; kgl < 8; kgl += kPackT / 64) {
+        const int64_t kg = p0 / 8 + kgl;
+        if (kg >= kgroups) break;
+        const bool live = b < B;
         half8 fh, fl, gh, gl;
         for (int q = 0; q < 8; ++q) {
             const int64_t p = kg * 8 + q;
-            double x = 0.0, y = 0.0;
-            if (live && p < n) {
-                const int32_t node = perm[p];
-                x = ROW ? row[node] : fb[node];
-                y = 1.0 - x;
-                fs[(int64_t)b * n + p] = x;
-            }
-            _Float16 a, c;
-            split_fp16(x, scale, a, c);
-            fh[q] = a;
-            fl[q] = c;
-            split_fp16(y, scale, a, c);
-            gh[q] = a;
-            gl[q] = c;
+            const double x = tile[ml][kgl * 8 + q];
+            const double y = (live && p < n) ? 1.0 - x : 0.0;
+            _Float16 hi, lo;
+            split_fp16(x, scale, hi, lo);
+            fh[q] = hi;
+            fl[q] = lo;
+            split_fp16(y, scale, hi, lo);
+            gh[q] = hi;
+            gl[q] = lo;
         }
         Fhi[kg * nt + b] = __builtin_bit_cast(u32x4, fh);
         Flo[kg * nt + b] = __builtin_bit_cast(u32x4, fl);
@@ -464,29 +525,47 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     const double* fsb = fs + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
     double dI = 0.0, dw = 0.0;
-    for (int64_t p = threadIdx.x; p < n; p += kMemT) {
-        double u = 0.0, v = 0.0;
-        for (int s = 0; s < nsplit; ++s) {
-            const PT* Ps = P + ((int64_t)s * nt + b) * rows_pad;
-            u += (double)Ps[p];
-            v += (double)Ps[kt + p];
+    // UNR positions per thread with all their loads issued before use
+    constexpr int UNR = 4;
+    for (int64_t p0 = threadIdx.x; p0 < n; p0 += (int64_t)kMemT * UNR) {
+        double u[UNR], v[UNR], fi[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
+        int32_t node[UNR];
+        bool gu[UNR], gv[UNR];
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
+            u[j] = 0.0;
+            v[j] = 0.0;
+            for (int s = 0; s < nsplit; ++s) {
+                const PT* Ps = P + ((int64_t)s * nt + b) * rows_pad;
+                u[j] += (double)Ps[p];
+                v[j] += (double)Ps[kt + p];
+            }
+            gu[j] = useg[(p / kMT) * (int64_t)nt + b] != 0;
+            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
+            fi[j] = fsb[p];
+            rkp[j] = rk[p];
+            ckp[j] = ck[p];
+            Vp[j] = V[p];
+            node[j] = perm[p];
         }
-        u *= inv_scale;
-        v *= inv_scale;
-        // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
-        const bool gu = useg[(p / kMT) * (int64_t)nt + b] != 0;
-        const bool gv = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
-        const double Kf = gu ? rk[p] - u : u, Kg = gu ? u : rk[p] - u;
-        const double KTf = gv ? ck[p] - v : v, KTg = gv ? v : ck[p] - v;
-        const double fi = fsb[p];
-        // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
-        const double Ii = ((1.0 - fi) * (np1 * Kf + nb * KTf) - fi * (np1 * KTg + nb * Kg)) / V[p];
-        dI += V[p] * Ii;
-        dw += V[p] * fabs(fi * (1.0 - fi));
-        if constexpr (ROW)
-            row[perm[p]] = Ii;
-        else
-            Ib[perm[p]] = Ii;
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            if (p0 + (int64_t)j * kMemT >= n) break;
+            const double uu = u[j] * inv_scale, vv = v[j] * inv_scale;
+            // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
+            const double Kf = gu[j] ? rkp[j] - uu : uu, Kg = gu[j] ? uu : rkp[j] - uu;
+            const double KTf = gv[j] ? ckp[j] - vv : vv, KTg = gv[j] ? vv : ckp[j] - vv;
+            const double x = fi[j];
+            // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
+            const double Ii = ((1.0 - x) * (np1 * Kf + nb * KTf) - x * (np1 * KTg + nb * Kg)) / Vp[j];
+            dI += Vp[j] * Ii;
+            dw += Vp[j] * fabs(x * (1.0 - x));
+            if constexpr (ROW)
+                row[node[j]] = Ii;
+            else
+                Ib[node[j]] = Ii;
+        }
     }
     double sc = 0.0;
     if (conserve) {
@@ -511,19 +590,21 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     __syncthreads();
     if (conserve) sc = s_b;
     const double* fb = f + (int64_t)b * n;
-    if constexpr (ROW) {
-        for (int64_t i = threadIdx.x; i < n; i += kMemT) {
-            double Ii = row[i];
-            if (conserve) {
-                const double fi = fb[i];
-                Ii -= sc * fabs(fi * (1.0 - fi));
+    constexpr int WUNR = 8;
+    if (ROW || conserve) {
+        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * WUNR) {
+            double x[WUNR], y[WUNR];
+#pragma unroll
+            for (int j = 0; j < WUNR; ++j) {
+                const int64_t i = min(i0 + (int64_t)j * kMemT, n - 1);
+                x[j] = conserve ? fb[i] : 0.0;
+                y[j] = ROW ? row[i] : Ib[i];
             }
-            Ib[i] = Ii;
-        }
-    } else if (conserve) {
-        for (int64_t i = threadIdx.x; i < n; i += kMemT) {
-            const double fi = fb[i];
-            Ib[i] -= sc * fabs(fi * (1.0 - fi));
+#pragma unroll
+            for (int j = 0; j < WUNR; ++j) {
+                const int64_t i = i0 + (int64_t)j * kMemT;
+                if (i < n) Ib[i] = y[j] - sc * fabs(x[j] * (1.0 - x[j]));
+            }
         }
     }
 }
@@ -566,6 +647,45 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
+static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B) {
+    const int64_t n = c->n;
+    const int64_t kgroups = m->kpad / 8;
+    if (m->rowcache)
+        hipLaunchKernelGGL(gather_member_kernel<true>, dim3(m->nt), dim3(kMemT), sizeof(double) * n,
+                           c->stream, d_f, n, B, m->perm, m->nt, m->fs);
+    else
+        hipLaunchKernelGGL(gather_member_kernel<false>, dim3(m->nt), dim3(kMemT), 0, c->stream, d_f, n,
+                           B, m->perm, m->nt, m->fs);
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(pack_tile_kernel, dim3(ceil_div(kgroups * 8, 64), m->nt / 64), dim3(kPackT), 0,
+                       c->stream, m->fs, n, B, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
+                       (u32x4*)m->G_hi, (u32x4*)m->G_lo);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, double* d_I, int B,
+                               int conserve) {
+    const int64_t n = c->n;
+    const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
+    const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
+#define TG_FINISH(ROW, PT)                                                                           \
+    hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
+                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, m->fs,  \
+                       d_f, m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
+                       d_I)
+    if (m->fold) {
+        if (m->rowcache) TG_FINISH(true, double);
+        else TG_FINISH(false, double);
+    } else {
+        if (m->rowcache) TG_FINISH(true, float);
+        else TG_FINISH(false, float);
+    }
+#undef TG_FINISH
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
 static tg_status ensure_P(tg_ctx* c, MixedData* m) {
     const size_t psize = m->fold ? sizeof(double) : sizeof(float);
     const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad * psize;
@@ -579,9 +699,10 @@ static tg_status ensure_P(tg_ctx* c, MixedData* m) {
 }
 
 // Autotuner: times every accuracy-safe tile configuration (FP64 fold, so no
-// K-split is needed for accuracy) with one and two K-splits on this band and
-// keeps the fastest.  The operands are zeroed first; the GEMM's time does not
-// depend on their values.
+// K-split is needed for accuracy) with one and two K-splits on this band --
+// the GEMM and the epilogue, whose partial-sum reads grow with the split
+// count -- and keeps the fastest.  The operands are zeroed first; the time
+// does not depend on their values.
 static tg_status autotune(tg_ctx* c, MixedData* m) {
     struct Cand { int nw, waves, fold, depth; };
     std::vector<Cand> cands;
@@ -596,6 +717,11 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     TG_HIP(c, hipMemsetAsync(m->G_hi, 0, sizeof(uint4) * kgroups * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->G_lo, 0, sizeof(uint4) * kgroups * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->useg, 0, (size_t)m->npanel * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->fs, 0, sizeof(double) * c->n * m->nt, st));
+    TG_HIP(c, hipMemsetAsync(m->bath, 0, sizeof(double) * c->max_batch, st));
+    double* d_out = nullptr;
+    TG_TRY(dev_alloc(c, &d_out, (size_t)c->n * m->nt));
+    const int Bt = std::min(m->nt, (int)c->max_batch);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     TG_HIP(c, hipEventCreate(&e0));
     TG_HIP(c, hipEventCreate(&e1));
@@ -612,9 +738,13 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
             m->nsplit = ns;
             status = ensure_P(c, m);
             if (status == TG_OK) status = launch_gemm(c, m);     // warm-up
+            if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
             if (status != TG_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
-            for (int r = 0; r < 3 && status == TG_OK; ++r) status = launch_gemm(c, m);
+            for (int r = 0; r < 3 && status == TG_OK; ++r) {
+                status = launch_gemm(c, m);
+                if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
+            }
             if (status != TG_OK) break;
             if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                 status = set_error(c, TG_ERR_HIP, "event timing");
@@ -631,6 +761,8 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    dev_free(&d_out);
     TG_TRY(status);
     m->nw = pick.nw;
     m->waves = pick.waves;
@@ -774,7 +906,7 @@ tg_status mixed_prepare(tg_ctx* c) {
         if (const char* e = std::getenv("TG_MIXED_ROWCACHE"))
             if (std::atoi(e) == 0) m->rowcache = false;
         if (m->rowcache) {
-            TG_HIP(c, hipFuncSetAttribute((const void*)pack_member_kernel<true>,
+            TG_HIP(c, hipFuncSetAttribute((const void*)gather_member_kernel<true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
             TG_HIP(c, hipFuncSetAttribute((const void*)finish_member_kernel<true, double>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
@@ -841,36 +973,14 @@ tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t 
     for (int b = 0; b < B; ++b) bath[b] = h_bath ? h_bath[b] : c->bath_n;
     TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
 
-    const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
-    if (m->rowcache)
-        hipLaunchKernelGGL(pack_member_kernel<true>, dim3(m->nt), dim3(kMemT), rowbytes, st, d_f, n, B,
-                           m->perm, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
-                           (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->fs);
-    else
-        hipLaunchKernelGGL(pack_member_kernel<false>, dim3(m->nt), dim3(kMemT), 0, st, d_f, n, B,
-                           m->perm, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
-                           (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->fs);
-    TG_CHECK_LAUNCH(c);
+    TG_TRY(launch_pack(c, m, d_f, B));
     hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, st, d_f, n, m->kt, B,
                        m->npanel, m->perm, m->useg, m->nt);
     TG_CHECK_LAUNCH(c);
     time_begin(c);
     TG_TRY(launch_gemm(c, m));
     time_end(c);
-    const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
-#define TG_FINISH(ROW, PT)                                                                         \
-    hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
-                       st, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, m->fs, d_f,   \
-                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve, d_I)
-    if (m->fold) {
-        if (m->rowcache) TG_FINISH(true, double);
-        else TG_FINISH(false, double);
-    } else {
-        if (m->rowcache) TG_FINISH(true, float);
-        else TG_FINISH(false, float);
-    }
-#undef TG_FINISH
-    TG_CHECK_LAUNCH(c);
+    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve));
     return TG_OK;
 }
 

@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP16_MFMA_PEAK_TFLOPS = 2500.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_gemm_traffic.json")
 
 
 def _log(msg):
@@ -77,13 +78,44 @@ def cpu_baseline(ctx, w, f, budget_s=15.0):
                       f"{len(times)} evaluations ({sum(times):.1f} s)"}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (the known-size reference for the
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+
+
+def measured_traffic(st):
+    """HBM bytes per launch of the batched GEMM from the committed counter
+    runs, when they were taken on the tile configuration in use."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no counter profile"
+    with open(TRAFFIC_FILE) as fh:
+        t = json.load(fh)
+    cfg = [st["mixed_tile"], st["mixed_waves"], st["mixed_fold"], st["mixed_depth"], st["mixed_splits"]]
+    if t.get("config") != cfg:
+        return None, f"counter profile taken on tile configuration {t.get('config')}, running {cfg}"
+    return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"], "measured"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(members shard across GPUs without per-stage communication, see DESIGN.md)",
@@ -141,17 +173,25 @@ def main():
 
     evals_mixed = B * 1000.0 / ms_mixed
     evals_plain = 1000.0 / ms_plain
-    # dominant kernel: the MFMA GEMM.  Algorithmic work = the two products
-    # K F and K^T F over the structural nonzeros of K: 4 nnz B flops.
+    # dominant kernel: the MFMA GEMM.  Algorithmic work of one launch = the
+    # two products K F and K^T F over the structural nonzeros of K for B
+    # members, 4 nnz B flops.  Dense-equivalent (4 n^2 B, zeros counted) and
+    # issued (three fp16 products over the streamed band) rates beside it.
     flops_alg = 4.0 * st["nnz"] * B
     tflops = flops_alg / (ms_gemm * 1e-3) / 1e12
+    dense_tflops = 4.0 * n * n * B / (ms_gemm * 1e-3) / 1e12
     issued = 3 * 2.0 * st["mixed_band"] * st["mixed_width"] / (ms_gemm * 1e-3) / 1e12
-    # plain FP64 pass: 8 n^2 + 32 n bytes (dense K) per evaluation
-    gbs_plain = (8.0 * n * n + 32.0 * n) / (ms_dual * 1e-3) / 1e9
+    traffic, traffic_note = measured_traffic(st)
+    # plain FP64 pass: algorithmic bytes = the nonzeros of K once (8 nnz) plus
+    # f in, I out and two n-vectors (32 n); streamed = the stored band
+    gbs_plain = (8.0 * st["nnz"] + 32.0 * n) / (ms_dual * 1e-3) / 1e9
+    gbs_streamed = (8.0 * st["plain_band"] + 32.0 * n) / (ms_dual * 1e-3) / 1e9
     _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
-         f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}")
+         f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}; "
+         f"tile {st['mixed_tile']}x{st['mixed_waves']}w fold {st['mixed_fold']} depth "
+         f"{st['mixed_depth']} splits {st['mixed_splits']}")
     _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), dual kernel "
-         f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s")
+         f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s algorithmic, {gbs_streamed:.0f} GB/s streamed")
 
     out = {
         "metric": "scattering-integral evals/s: batched mixed-prec vs FP64 plain @1e-6 rel err",
@@ -172,11 +212,18 @@ def main():
                    "splits": st["mixed_splits"]},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP16_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP16_MFMA_PEAK_TFLOPS,
-                     "traffic": None, "kernel": "mixed_gemm_kernel", "kernel_ms": ms_gemm,
-                     "issued_tflops": issued},
+                     "traffic": traffic, "traffic_note": traffic_note,
+                     "kernel": "mixed_gemm_kernel", "kernel_ms": ms_gemm,
+                     "flops_per_launch": flops_alg, "issued_tflops": issued,
+                     "issued_frac": issued / FP16_MFMA_PEAK_TFLOPS,
+                     "dense_equiv_tflops": dense_tflops,
+                     "tile": {"members": st["mixed_tile"], "waves": st["mixed_waves"],
+                              "fold": st["mixed_fold"], "depth": st["mixed_depth"],
+                              "splits": st["mixed_splits"], "autotuned_ms": st["mixed_tuned_ms"]}},
         "plain_fp64": {"evals_per_s": evals_plain, "ms_per_eval": ms_plain,
-                       "kernel_ms": ms_dual, "achieved_gbs": gbs_plain,
-                       "frac_hbm": gbs_plain / HBM_PEAK_GBS},
+                       "kernel": "plain_dual_kernel", "kernel_ms": ms_dual,
+                       "achieved_gbs": gbs_plain, "frac_hbm": gbs_plain / HBM_PEAK_GBS,
+                       "streamed_gbs": gbs_streamed, "bytes_per_eval": 8.0 * st["nnz"] + 32.0 * n},
         "mixed_vs_plain": evals_mixed / evals_plain,
         "max_rel_err_vs_fp64": rel,
         "wall_s_timed": wall,

@@ -1,27 +1,27 @@
 // tg_mixed.hip -- batched mixed-precision collision operator on MFMA.
 //
-// For B members at once the operator needs U = K F and V = K^T F (F = n x B).
-// Both come from one GEMM with the stacked matrix A = [K; K^T] (2n x n):
+// For B members at once the operator needs U = K Z and V = K^T Z (Z = n x B,
+// z = f or 1 - f per panel and member).  Both come from one GEMM with the
+// stacked matrix A = [K; K^T] (2n x n):
 //
-//   C = A F,  A = 2^-sa (A_hi + A_lo),  F = 2^-sf (F_hi + F_lo)   (fp16 parts)
-//   C ~ 2^-(sa+sf) (A_hi F_hi + A_hi F_lo + A_lo F_hi)
+//   C = A Z,  A = 2^-sa (A_hi + A_lo),  Z = 2^-sf (Z_hi + Z_lo)   (fp16 parts)
+//   C ~ 2^-(sa+sf) (A_hi Z_hi + A_hi Z_lo + A_lo Z_hi)
 //
-// three v_mfma_f32_32x32x16_f16 products per tile, FP32 accumulation inside
-// a K-split, FP64 sum of the splits and FP64 Pauli assembly in the epilogue
-// (per-member bath occupation, number-defect removal).  The row and column
-// sums K 1, K^T 1 used by the epilogue are exact FP64 sums of the *split*
-// matrix, so the split error cancels in K^T 1 - K^T f for Pauli-blocked nodes.
+// three v_mfma_f32_32x32x16_f16 products per tile; the FP32 sums of at most
+// 32*FOLD products are folded into FP64 registers, the epilogue works in FP64
+// (per-member bath occupation, Pauli blocking, number-defect removal).  The
+// row and column sums K 1, K^T 1 used by the epilogue are exact FP64 sums of
+// the *split* matrix, so the split error cancels in K 1 - K g.
 //
 // Node order: K is stored in energy order (see tg_assemble.hip), which makes
 // the emission kernel a band: every 128-row panel of A has a contiguous
 // column range outside which it is exactly zero, and the GEMM only streams
-// that range.  Inputs and outputs stay in the caller's node order; the
-// F-packing kernel gathers through the permutation and the epilogue scatters
-// back.
+// that range.  Inputs and outputs stay in the caller's node order; the gather
+// kernel copies f into energy order and the epilogue scatters back.
 //
 // Packed layouts (16-byte units of eight fp16 values along the reduction k):
 //   A_{hi,lo}[kg][row]   kg = k/8, row in [0, rows_pad)
-//   F_{hi,lo}[kg][col]   col = member in [0, NT)
+//   Z_{hi,lo}[kg][col]   col = member in [0, nt)
 // A workgroup's LDS tile and every MFMA fragment are contiguous 16-byte
 // reads in these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1).
 #include <algorithm>
@@ -703,6 +703,8 @@ static tg_status ensure_P(tg_ctx* c, MixedData* m) {
 // the GEMM and the epilogue, whose partial-sum reads grow with the split
 // count -- and keeps the fastest.  The operands are zeroed first; the time
 // does not depend on their values.
+constexpr int kTuneReps = 5;
+
 static tg_status autotune(tg_ctx* c, MixedData* m) {
     struct Cand { int nw, waves, fold, depth; };
     std::vector<Cand> cands;
@@ -725,7 +727,11 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     TG_HIP(c, hipEventCreate(&e0));
     TG_HIP(c, hipEventCreate(&e1));
-    float best = 0.0f;
+    // the default keeps its place unless another candidate is >3% faster, so
+    // near-equal candidates do not flip from one context to the next
+    const Cand dflt{m->nw, m->waves, m->fold, m->depth};
+    const int dflt_split = m->nsplit;
+    float best = 0.0f, dflt_ms = 0.0f;
     Cand pick = cands[0];
     int pick_split = 1;
     tg_status status = TG_OK;
@@ -741,7 +747,7 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
             if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
             if (status != TG_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
-            for (int r = 0; r < 3 && status == TG_OK; ++r) {
+            for (int r = 0; r < kTuneReps && status == TG_OK; ++r) {
                 status = launch_gemm(c, m);
                 if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
             }
@@ -752,12 +758,21 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
             }
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= kTuneReps;
+            if (cd.nw == dflt.nw && cd.waves == dflt.waves && cd.fold == dflt.fold &&
+                cd.depth == dflt.depth && ns == dflt_split)
+                dflt_ms = ms;
             if (best == 0.0f || ms < best) {
                 best = ms;
                 pick = cd;
                 pick_split = ns;
             }
         }
+    }
+    if (status == TG_OK && dflt_ms > 0.0f && best > 0.97f * dflt_ms) {
+        best = dflt_ms;
+        pick = dflt;
+        pick_split = dflt_split;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -769,7 +784,7 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     m->fold = pick.fold;
     m->depth = pick.depth;
     m->nsplit = pick_split;
-    m->tuned_ms = best / 3.0f;
+    m->tuned_ms = best;
     return TG_OK;
 }
 
@@ -864,15 +879,17 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
     // tile width, waves, FP64 fold interval and K-splits
-    m->nw = std::min(64, m->nt);
-    m->waves = 4;
-    m->fold = 1;
+    // default (kept by the autotuner unless beaten by >3%): 128 members, 8
+    // waves, FP64 fold every 2 iterations, 2 iterations of prefetch; one
+    // K-split, or more when the panels alone leave CUs idle
+    m->nw = std::min(128, m->nt);
+    m->waves = m->nw == 128 ? 8 : 4;
+    m->fold = 2;
     m->depth = 2;
     {
         const int nmt = m->nt / m->nw;
-        // enough workgroups to load all 256 CUs a few times over
-        int ns = (int)std::ceil(1024.0 / ((double)m->npanel * nmt));
-        m->nsplit = std::max(1, std::min(ns, 8));
+        int ns = (int)std::ceil(512.0 / ((double)m->npanel * nmt));
+        m->nsplit = std::max(1, std::min(ns, 2));
     }
     // explicit overrides (measurements of the tile configuration) replace the autotuner
     bool fixed = false;

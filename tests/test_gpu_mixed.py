@@ -185,6 +185,11 @@ def test_mixed_dp5_error_vs_fp64_on_same_steps(sq32_mixed):
     # members take different numbers of steps
     assert len({s["attempted"] for s in stats}) > 1
     fr, dr = ctx.dp5_replay(F, 0.0, 100.0, 1.0, steps, bath_n=bath, max_dense=100)
+    # each member of the batched FP64 replay takes its own steps: it equals the
+    # replay of that member alone, bit for bit
+    for b in (0, 5):
+        f1, d1 = ctx.dp5_replay(F[:, b], 0.0, 100.0, 1.0, steps[b], bath_n=bath[b], max_dense=100)
+        assert np.array_equal(d1, dr[b]) and np.array_equal(f1, fr[:, b])
     V = ok.V
     for b in range(8):
         assert _rel(dm[b, -1], dr[b, -1]) <= 1e-6, (b, _rel(dm[b, -1], dr[b, -1]))

@@ -185,6 +185,19 @@ tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_
     return TG_OK;
 }
 
+int relerr_blocks(tg_ctx* c) { return red_blocks(c->nel); }
+
+tg_status relerr_sums_device(tg_ctx* c, const double* d_err, const double* d_pop, int32_t batch,
+                             double rtol, double atol, double* part, double* out) {
+    const int nblk = red_blocks(c->nel);
+    hipLaunchKernelGGL(relerr_partial_kernel, dim3(nblk, batch), dim3(kRedThreads), 0, c->stream,
+                       d_err, d_pop, c->d_conn, c->dim, c->nel, c->n, rtol, atol, part, nblk);
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(batch), dim3(64), 0, c->stream, part, nblk, 1, out);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
 }  // namespace tg
 
 using namespace tg;

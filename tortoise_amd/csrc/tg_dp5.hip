@@ -9,10 +9,18 @@
 // own error norm (Band::relativeError, Band.cpp:87-101).  enforceValue and the
 // step vetoer are the identity (no constrained bands in this operator).
 //
-// Device work per attempted step: 6 stage-sum kernels fused with nothing else
-// to read (y, k1..k6 -> ys), 6 operator evaluations, one fused y4/err kernel,
-// one batched error-norm reduction; after the host decision one dense-output
-// kernel (all dense points of all members) and one accept kernel.
+// The whole attempt runs on the device and is captured once as a HIP graph:
+//   prep_kernel        members still running take their step size (0 for the
+//                      others, whose stages then reproduce y: masked)
+//   6 x (stage_kernel, operator evaluation), y5 and k7
+//   error_kernel       err = y5 - y4;  relerr sums (Band::relativeError)
+//   controller_kernel  per member: accept/reject, PI update, post-reject cap,
+//                      step underflow, dense-output range, step log
+//   dense_kernel       dense outputs of the accepted steps
+//   accept_kernel      y <- y5, k1 <- k7 (FSAL)
+//   a copy of the "any member running" flag to pinned host memory.
+// The host replays the graph one attempt ahead of the flag it reads, so it
+// never waits for a decision and at most one masked attempt runs at the end.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -75,26 +83,137 @@ __global__ void error_kernel(int64_t n, int B, const double* __restrict__ y, Sta
     }
 }
 
-// One dense output point: member, six coefficients (s*h*btilde_j), slot.
-struct DensePoint {
-    int32_t member;
-    int32_t pad;
-    int64_t dst;          // element offset into the dense buffer
-    double c[6];          // for k1, k3, k4, k5, k6, k7
+// Per-member controller state (DormandPrince54.hpp:527-543, 547-661), on the
+// device.  The accepted step's start, size and first dense time are kept for
+// the dense-output kernel.
+struct MemberDev {
+    double t, h, dense_time, olderror;
+    double step_t, step_h, step_dense_time;
+    int32_t rejected, failed, nonfinite, active;
+    int32_t step_idx, acc, dense_k0, dense_k1;
+    tg_step_stats st;
 };
 
-__global__ void dense_kernel(int64_t n, const double* __restrict__ y, const double* __restrict__ k1,
+struct Ctl {
+    double t1, dense_dt, min_h, denom;
+    int32_t B, replay, h_cap, max_dense;
+    const double* sched;      // replay: step sizes, B x h_cap
+    const int32_t* nsched;    // replay: steps per member
+};
+
+__device__ inline bool member_running(const MemberDev& m, const Ctl& k, int b) {
+    if (k.replay) return m.step_idx < k.nsched[b] && m.t < k.t1;
+    return m.t < k.t1 && !m.failed;
+}
+
+// members still running take their step size, the others 0; clears the
+// "any member running" flag the controller sets
+__global__ void prep_kernel(MemberDev* __restrict__ md, Ctl k, double* __restrict__ h,
+                            int32_t* __restrict__ running) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *running = 0;
+    if (b >= k.B) return;
+    MemberDev m = md[b];
+    m.active = member_running(m, k, b) ? 1 : 0;
+    if (m.active && k.replay) m.h = k.sched[(int64_t)b * k.h_cap + m.step_idx];
+    h[b] = m.active ? m.h : 0.0;
+    md[b] = m;
+}
+
+// accept / reject per member (DormandPrince54.hpp:582-661); norms holds the
+// error-norm sums, err = sqrt(sum / denom) (Band.cpp:87-101)
+__global__ void controller_kernel(MemberDev* __restrict__ md, Ctl k,
+                                  const double* __restrict__ norms, int8_t* __restrict__ acc,
+                                  double* __restrict__ hlog, int32_t* __restrict__ running) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < k.B) {
+        const double minscale = 0.333333, maxscale = 3.0, safety = 0.98;
+        const double alpha = 0.7 / 5., beta = 0.4 / 5.;
+        MemberDev m = md[b];
+        m.acc = 0;
+        m.dense_k0 = m.dense_k1 = m.st.n_dense;
+        if (m.active) {
+            if (k.replay) ++m.step_idx;
+            m.st.attempted += 1;
+            m.st.evals += 6;
+            const double e = k.replay ? 0.0 : sqrt(norms[b] / k.denom);
+            const double h = m.h;
+            if (!isfinite(e)) {
+                m.failed = 1;
+                m.nonfinite = 1;
+            } else if (e <= 1.0) {
+                // dense outputs inside the step (:598-611)
+                m.step_t = m.t;
+                m.step_h = h;
+                m.step_dense_time = m.dense_time;
+                while (m.dense_time <= m.t + 1.0001 * h && m.dense_time <= k.t1 + .0001 * h) {
+                    m.st.n_dense += 1;
+                    m.dense_time += k.dense_dt;
+                }
+                m.dense_k1 = m.st.n_dense;
+                if (hlog && m.st.accepted < k.h_cap) hlog[(int64_t)b * k.h_cap + m.st.accepted] = h;
+                m.t += h;
+                m.acc = 1;
+                double scale;
+                if (e == 0.) {
+                    scale = maxscale;
+                } else {
+                    scale = safety * pow(1. / e, alpha) * pow(m.olderror, beta);
+                    scale = fmin(fmax(scale, minscale), maxscale);
+                }
+                m.h = m.rejected ? h * fmin(1.0, scale) : h * scale;
+                m.rejected = 0;
+                m.olderror = e;
+                m.st.accepted += 1;
+            } else {
+                double scale = safety * pow(1. / e, 1 / 5.);
+                scale = fmin(fmax(scale, minscale), maxscale);
+                m.h = h * scale;
+                m.rejected = 1;
+                m.st.rejected += 1;
+                if (m.h < k.min_h) m.failed = 1;
+            }
+        }
+        acc[b] = (int8_t)m.acc;
+        if (member_running(m, k, b)) atomicOr(&any, 1);
+        md[b] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(running, 1);
+}
+
+// dense outputs of the step just accepted: y(t_s) = y + sum_j (s h bt_j(s)) k_j
+// with s = (t_s - t)/h (DormandPrince54.hpp:601-608)
+__global__ void dense_kernel(int64_t n, const MemberDev* __restrict__ md, Ctl k,
+                             const double* __restrict__ y, const double* __restrict__ k1,
                              const double* __restrict__ k3, const double* __restrict__ k4,
                              const double* __restrict__ k5, const double* __restrict__ k6,
-                             const double* __restrict__ k7, const DensePoint* __restrict__ pts,
-                             double* __restrict__ out) {
-    const DensePoint p = pts[blockIdx.y];
-    const int64_t base = (int64_t)p.member * n;
+                             const double* __restrict__ k7, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const MemberDev& m = md[b];
+    if (!m.acc || m.dense_k0 >= m.dense_k1 || m.dense_k0 >= k.max_dense) return;
+    const int64_t base = (int64_t)b * n;
+    const double t = m.step_t, h = m.step_h;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t q = base + i;
-        out[p.dst + i] = y[q] + p.c[0] * k1[q] + p.c[1] * k3[q] + p.c[2] * k4[q] +
-                         p.c[3] * k5[q] + p.c[4] * k6[q] + p.c[5] * k7[q];
+        double td = m.step_dense_time;
+        for (int32_t slot = m.dense_k0; slot < m.dense_k1 && slot < k.max_dense; ++slot) {
+            const double s = (td - t) / h;
+            const double bt1 = -(435. * s * s * s - 1184. * s * s + 1098. * s - 384.) / (384.);
+            const double bt3 = 500. * s * (6. * s * s - 14. * s + 9.) / (1113.);
+            const double bt4 = -125. * s * (9. * s * s - 16. * s + 6.) / (192.);
+            const double bt5 = 729. * s * (35. * s * s - 64. * s + 26.) / (6784.);
+            const double bt6 = -11. * s * (3. * s - 2.) * (5. * s - 6.) / (84.);
+            const double bt7 = s * (s - 1.) * (5. * s - 3.) / (2.);
+            out[((int64_t)b * k.max_dense + slot) * n + i] =
+                y[q] + (s * h * bt1) * k1[q] + (s * h * bt3) * k3[q] + (s * h * bt4) * k4[q] +
+                (s * h * bt5) * k5[q] + (s * h * bt6) * k6[q] + (s * h * bt7) * k7[q];
+            td += k.dense_dt;
+        }
     }
 }
 
@@ -122,8 +241,16 @@ struct Dp5Work {
     double* y = nullptr;
     double* h = nullptr;      // device per-member step
     int8_t* acc = nullptr;
-    DensePoint* pts = nullptr;
-    int32_t pts_cap = 0;
+    MemberDev* md = nullptr;
+    double* norms = nullptr;  // error-norm sums per member
+    double* npart = nullptr;  // their block partials
+    double* sched = nullptr;  // replay schedule (B x h_cap)
+    int32_t* nsched = nullptr;
+    size_t sched_cap = 0;
+    double* hlog = nullptr;   // accepted step sizes (B x h_cap)
+    size_t hlog_cap = 0;
+    int32_t* running = nullptr;        // device flag
+    int32_t* running_host = nullptr;   // pinned copy
     double* dense = nullptr;  // device dense buffer when the caller's is host memory
     size_t dense_cap = 0;
 };
@@ -146,6 +273,12 @@ static tg_status dp5_ensure(tg_ctx* c, int32_t B) {
     TG_TRY(dev_alloc(c, &w->y, sz));
     TG_TRY(dev_alloc(c, &w->h, (size_t)w->cap));
     TG_TRY(dev_alloc(c, &w->acc, (size_t)w->cap));
+    TG_TRY(dev_alloc(c, &w->md, (size_t)w->cap));
+    TG_TRY(dev_alloc(c, &w->norms, (size_t)w->cap));
+    TG_TRY(dev_alloc(c, &w->npart, (size_t)w->cap * relerr_blocks(c)));
+    TG_TRY(dev_alloc(c, &w->nsched, (size_t)w->cap));
+    TG_TRY(dev_alloc(c, &w->running, (size_t)1));
+    TG_HIP(c, hipHostMalloc((void**)&w->running_host, sizeof(int32_t), hipHostMallocDefault));
     return TG_OK;
 }
 
@@ -159,28 +292,93 @@ void dp5_destroy(tg_ctx* c) {
     dev_free(&w->y);
     dev_free(&w->h);
     dev_free(&w->acc);
-    dev_free(&w->pts);
+    dev_free(&w->md);
+    dev_free(&w->norms);
+    dev_free(&w->npart);
+    dev_free(&w->sched);
+    dev_free(&w->nsched);
+    dev_free(&w->hlog);
+    dev_free(&w->running);
+    if (w->running_host) (void)hipHostFree(w->running_host);
     dev_free(&w->dense);
     delete w;
     c->dp5 = nullptr;
 }
 
-// operator evaluation for the whole batch
+// operator evaluation for the whole batch (bath set beforehand for the
+// batched path); kernel launches only
 static tg_status eval_batch(tg_ctx* c, const double* d_in, double* d_out, int32_t B,
                             const std::vector<double>& bath, int32_t flags) {
     const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
-    if (flags & TG_PREC_MIXED) return mixed_apply_device(c, d_in, d_out, B, bath.data(), conserve);
+    if (flags & TG_PREC_MIXED) return mixed_eval_device(c, d_in, d_out, B, conserve);
     for (int b = 0; b < B; ++b)
         TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, bath[b],
                                   conserve));
     return TG_OK;
 }
 
-struct MemberState {
-    double t, h, dense_time, olderror;
-    bool rejected, failed, nonfinite;
-    tg_step_stats st;
-};
+// the device work of one attempted step (captured into a graph)
+static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl,
+                         const std::vector<double>& bath, int32_t flags, double rtol, double atol,
+                         double* d_dense, double* d_hlog) {
+    hipStream_t st = c->stream;
+    const int64_t n = c->n;
+    double** k = w->k;
+    const dim3 blk(256);
+    const dim3 grd(ceil_div(n, 256 * 4) > 0 ? ceil_div(n, 256 * 4) : 1, B);
+    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out) -> tg_status {
+        StageArgs a{};
+        int j = 0;
+        for (double v : coefs) a.a[j++] = v;
+        for (int q = 0; q < nk; ++q) a.k[q] = k[q];
+        a.nk = nk;
+        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out);
+        TG_CHECK_LAUNCH(c);
+        return TG_OK;
+    };
+    const dim3 cgrd(ceil_div(B, 256));
+    hipLaunchKernelGGL(prep_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->h, w->running);
+    TG_CHECK_LAUNCH(c);
+    // stages k2..k6 (DormandPrince54.hpp:565-574)
+    TG_TRY(stage({A21}, 1, w->ys));
+    TG_TRY(eval_batch(c, w->ys, k[1], B, bath, flags));
+    TG_TRY(stage({A31, A32}, 2, w->ys));
+    TG_TRY(eval_batch(c, w->ys, k[2], B, bath, flags));
+    TG_TRY(stage({A41, A42, A43}, 3, w->ys));
+    TG_TRY(eval_batch(c, w->ys, k[3], B, bath, flags));
+    TG_TRY(stage({A51, A52, A53, A54}, 4, w->ys));
+    TG_TRY(eval_batch(c, w->ys, k[4], B, bath, flags));
+    TG_TRY(stage({A61, A62, A63, A64, A65}, 5, w->ys));
+    TG_TRY(eval_batch(c, w->ys, k[5], B, bath, flags));
+    // y5 (b2 = 0) and k7 (:576-578)
+    TG_TRY(stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5));
+    TG_TRY(eval_batch(c, w->y5, k[6], B, bath, flags));
+    // err = y5 - y4 (:580)
+    {
+        StageArgs a{};
+        const double bh[7] = {BH1, 0.0, BH3, BH4, BH5, BH6, BH7};
+        for (int q = 0; q < 7; ++q) {
+            a.a[q] = bh[q];
+            a.k[q] = k[q];
+        }
+        a.nk = 7;
+        hipLaunchKernelGGL(error_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, w->y5, w->err);
+        TG_CHECK_LAUNCH(c);
+    }
+    if (!ctl.replay) TG_TRY(relerr_sums_device(c, w->err, w->y5, B, rtol, atol, w->npart, w->norms));
+    hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc,
+                       d_hlog, w->running);
+    TG_CHECK_LAUNCH(c);
+    if (d_dense) {
+        hipLaunchKernelGGL(dense_kernel, dim3(ceil_div(n, 1024) > 0 ? ceil_div(n, 1024) : 1, B), blk,
+                           0, st, n, w->md, ctl, w->y, k[0], k[2], k[3], k[4], k[5], k[6], d_dense);
+        TG_CHECK_LAUNCH(c);
+    }
+    hipLaunchKernelGGL(accept_kernel, grd, blk, 0, st, n, w->acc, w->y, w->y5, k[0], k[6]);
+    TG_CHECK_LAUNCH(c);
+    TG_HIP(c, hipMemcpyAsync(w->running_host, w->running, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    return TG_OK;
+}
 
 }  // namespace tg
 
@@ -203,11 +401,13 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
     Dp5Work* w = work_of(c);
     const bool dev = flags & TG_PTR_DEVICE;
     const bool quiet = flags & TG_QUIET;
+    const bool replay = sched != nullptr;
     hipStream_t st = c->stream;
     const size_t vec = (size_t)n * B;
 
     std::vector<double> bath(B);
     for (int b = 0; b < B; ++b) bath[b] = bath_n ? bath_n[b] : c->bath_n;
+    if (flags & TG_PREC_MIXED) TG_TRY(mixed_set_bath(c, bath.data(), B));
 
     TG_HIP(c, hipMemcpyAsync(w->y, f, sizeof(double) * vec,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
@@ -225,181 +425,114 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
             d_dense = w->dense;
         }
     }
+    const int32_t log_cap = replay ? h_cap : (h_acc ? h_cap : 1);
+    if (replay || h_acc) {
+        const size_t need = (size_t)B * log_cap;
+        if (replay && w->sched_cap < need) {
+            dev_free(&w->sched);
+            TG_TRY(dev_alloc(c, &w->sched, need));
+            w->sched_cap = need;
+        }
+        if (h_acc && w->hlog_cap < need) {
+            dev_free(&w->hlog);
+            TG_TRY(dev_alloc(c, &w->hlog, need));
+            w->hlog_cap = need;
+        }
+    }
+    if (h_acc)  // entries past a member's accepted count keep the caller's values
+        TG_HIP(c, hipMemcpyAsync(w->hlog, h_acc, sizeof(double) * (size_t)B * h_cap,
+                                 hipMemcpyHostToDevice, st));
+    if (replay) {
+        TG_HIP(c, hipMemcpyAsync(w->sched, sched, sizeof(double) * B * h_cap, hipMemcpyHostToDevice, st));
+        TG_HIP(c, hipMemcpyAsync(w->nsched, nsched, sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+    }
 
     // controller state (DormandPrince54.hpp:527-543)
-    const double min_h = dense_dt / 10000.;
-    const double minscale = 0.333333, maxscale = 3.0, safety = 0.98;
-    const double alpha = 0.7 / 5., beta = 0.4 / 5.;
-    std::vector<MemberState> ms(B);
+    Ctl ctl{};
+    ctl.t1 = t1;
+    ctl.dense_dt = dense_dt;
+    ctl.min_h = dense_dt / 10000.;
+    ctl.denom = (double)((c->dim + 1) * (c->dim + 2)) * (double)c->nel;
+    ctl.B = B;
+    ctl.replay = replay ? 1 : 0;
+    ctl.h_cap = log_cap;
+    ctl.max_dense = d_dense ? max_dense : 0;
+    ctl.sched = replay ? w->sched : nullptr;
+    ctl.nsched = w->nsched;
+    std::vector<MemberDev> ms(B);
     for (auto& m : ms) {
+        std::memset(&m, 0, sizeof(m));
         m.t = t0;
         m.h = dense_dt;
         m.dense_time = t0 + dense_dt;
         m.olderror = 1.0;
-        m.rejected = m.failed = m.nonfinite = false;
-        std::memset(&m.st, 0, sizeof(m.st));
         m.st.evals = 1;
     }
+    TG_HIP(c, hipMemcpyAsync(w->md, ms.data(), sizeof(MemberDev) * B, hipMemcpyHostToDevice, st));
+    TG_TRY(eval_batch(c, w->y, w->k[0], B, bath, flags));  // k1 = timeOp(y0)
+    TG_HIP(c, hipStreamSynchronize(st));
 
-    double** k = w->k;
-    TG_TRY(eval_batch(c, w->y, k[0], B, bath, flags));  // k1 = timeOp(y0)
-
-    std::vector<double> hh(B), en(B);
-    std::vector<int8_t> acc(B);
-    std::vector<DensePoint> pts;
-    const dim3 blk(256);
-    const dim3 grd(ceil_div(n, 256 * 4) > 0 ? ceil_div(n, 256 * 4) : 1, B);
-    auto stage = [&](const double* in_y, std::initializer_list<double> coefs, int nk,
-                     double* out) -> tg_status {
-        StageArgs a{};
-        int j = 0;
-        for (double v : coefs) a.a[j++] = v;
-        for (int q = 0; q < nk; ++q) a.k[q] = k[q];
-        a.nk = nk;
-        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, in_y, a, w->h, out);
-        TG_CHECK_LAUNCH(c);
-        return TG_OK;
-    };
-
-    const bool replay = sched != nullptr;
-    std::vector<int32_t> step_idx(B, 0);
-    auto is_active = [&](int b) {
-        const MemberState& m = ms[b];
-        if (replay) return step_idx[b] < nsched[b] && m.t < t1;
-        return m.t < t1 && !m.failed;
-    };
-    while (true) {
-        bool any = false;
-        for (int b = 0; b < B; ++b) {
-            MemberState& m = ms[b];
-            const bool active = is_active(b);
-            if (active && replay) m.h = sched[(int64_t)b * h_cap + step_idx[b]];
-            hh[b] = active ? m.h : 0.0;
-            any |= active;
-        }
-        if (!any) break;
-        TG_HIP(c, hipMemcpyAsync(w->h, hh.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
-        // stages k2..k6 (DormandPrince54.hpp:565-574)
-        TG_TRY(stage(w->y, {A21}, 1, w->ys));
-        TG_TRY(eval_batch(c, w->ys, k[1], B, bath, flags));
-        TG_TRY(stage(w->y, {A31, A32}, 2, w->ys));
-        TG_TRY(eval_batch(c, w->ys, k[2], B, bath, flags));
-        TG_TRY(stage(w->y, {A41, A42, A43}, 3, w->ys));
-        TG_TRY(eval_batch(c, w->ys, k[3], B, bath, flags));
-        TG_TRY(stage(w->y, {A51, A52, A53, A54}, 4, w->ys));
-        TG_TRY(eval_batch(c, w->ys, k[4], B, bath, flags));
-        TG_TRY(stage(w->y, {A61, A62, A63, A64, A65}, 5, w->ys));
-        TG_TRY(eval_batch(c, w->ys, k[5], B, bath, flags));
-        // y5 (b2 = 0) and k7 (:576-578)
-        TG_TRY(stage(w->y, {B1, 0.0, B3, B4, B5, B6}, 6, w->y5));
-        TG_TRY(eval_batch(c, w->y5, k[6], B, bath, flags));
-        // err = y5 - y4 (:580)
-        {
-            StageArgs a{};
-            const double bh[7] = {BH1, 0.0, BH3, BH4, BH5, BH6, BH7};
-            for (int q = 0; q < 7; ++q) {
-                a.a[q] = bh[q];
-                a.k[q] = k[q];
-            }
-            a.nk = 7;
-            hipLaunchKernelGGL(error_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, w->y5, w->err);
-            TG_CHECK_LAUNCH(c);
-        }
-        if (replay) std::fill(en.begin(), en.end(), 0.0);
-        else TG_TRY(relative_error_device(c, w->err, w->y5, B, rtol, atol, en.data()));
-
-        pts.clear();
-        for (int b = 0; b < B; ++b) {
-            MemberState& m = ms[b];
-            acc[b] = 0;
-            if (!is_active(b)) continue;
-            if (replay) ++step_idx[b];
-            m.st.attempted += 1;
-            m.st.evals += 6;
-            const double e = en[b];
-            const double h = m.h;
-            if (!std::isfinite(e)) {
-                m.failed = true;
-                m.nonfinite = true;
-                continue;
-            }
-            if (!quiet && B == 1) printf("** Time: %f timestep: %f\n", m.t, h);
-            if (e <= 1.0) {
-                // dense output (:598-611)
-                while (m.dense_time <= m.t + 1.0001 * h && m.dense_time <= t1 + .0001 * h) {
-                    const double s = (m.dense_time - m.t) / h;
-                    if (d_dense && m.st.n_dense < max_dense) {
-                        DensePoint p{};
-                        p.member = b;
-                        p.dst = ((int64_t)b * max_dense + m.st.n_dense) * n;
-                        const double bt1 = -(435. * s * s * s - 1184. * s * s + 1098. * s - 384.) / (384.);
-                        const double bt3 = 500. * s * (6. * s * s - 14. * s + 9.) / (1113.);
-                        const double bt4 = -125. * s * (9. * s * s - 16. * s + 6.) / (192.);
-                        const double bt5 = 729. * s * (35. * s * s - 64. * s + 26.) / (6784.);
-                        const double bt6 = -11. * s * (3. * s - 2.) * (5. * s - 6.) / (84.);
-                        const double bt7 = s * (s - 1.) * (5. * s - 3.) / (2.);
-                        p.c[0] = s * h * bt1;
-                        p.c[1] = s * h * bt3;
-                        p.c[2] = s * h * bt4;
-                        p.c[3] = s * h * bt5;
-                        p.c[4] = s * h * bt6;
-                        p.c[5] = s * h * bt7;
-                        pts.push_back(p);
-                    }
-                    m.st.n_dense += 1;
-                    m.dense_time += dense_dt;
-                }
-                if (h_acc && m.st.accepted < h_cap) h_acc[(int64_t)b * h_cap + m.st.accepted] = h;
-                m.t += h;
-                acc[b] = 1;
-                double scale;
-                if (e == 0.) {
-                    scale = maxscale;
-                } else {
-                    scale = safety * std::pow(1. / e, alpha) * std::pow(m.olderror, beta);
-                    scale = std::fmin(std::fmax(scale, minscale), maxscale);
-                }
-                m.h = m.rejected ? h * std::fmin(1.0, scale) : h * scale;
-                m.rejected = false;
-                m.olderror = e;
-                m.st.accepted += 1;
-            } else {
-                double scale = safety * std::pow(1. / e, 1 / 5.);
-                scale = std::fmin(std::fmax(scale, minscale), maxscale);
-                m.h = h * scale;
-                m.rejected = true;
-                m.st.rejected += 1;
-                if (m.h < min_h) m.failed = true;
-            }
-        }
-        if (!pts.empty()) {
-            if ((int32_t)pts.size() > w->pts_cap) {
-                dev_free(&w->pts);
-                w->pts_cap = (int32_t)pts.size() * 2;
-                TG_TRY(dev_alloc(c, &w->pts, (size_t)w->pts_cap));
-            }
-            TG_HIP(c, hipMemcpyAsync(w->pts, pts.data(), sizeof(DensePoint) * pts.size(),
-                                     hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(dense_kernel, dim3(ceil_div(n, 1024) > 0 ? ceil_div(n, 1024) : 1, pts.size()),
-                               blk, 0, st, n, w->y, k[0], k[2], k[3], k[4], k[5], k[6], w->pts, d_dense);
-            TG_CHECK_LAUNCH(c);
-            // pts is reused next step: wait for the copy before the host edits it
-            TG_HIP(c, hipStreamSynchronize(st));
-        }
-        TG_HIP(c, hipMemcpyAsync(w->acc, acc.data(), B, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(accept_kernel, grd, blk, 0, st, n, w->acc, w->y, w->y5, k[0], k[6]);
-        TG_CHECK_LAUNCH(c);
-        TG_HIP(c, hipStreamSynchronize(st));  // acc is host memory reused next step
+    // the attempt as a graph; replayed one attempt ahead of the flag read
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    TG_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const bool timing = c->time_kernel;
+    c->time_kernel = false;   // no event records inside the graph
+    tg_status cap = attempt(c, w, B, ctl, bath, flags, rtol, atol, d_dense, h_acc ? w->hlog : nullptr);
+    c->time_kernel = timing;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (cap != TG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return cap;
     }
-    if (!quiet && B == 1) printf("Number of adaptive steps: %lld\n", (long long)ms[0].st.attempted);
+    if (ec != hipSuccess)
+        return set_error(c, TG_ERR_HIP, "graph capture of a DP5 attempt: %s", hipGetErrorString(ec));
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+        return set_error(c, TG_ERR_HIP, "graph instantiation: %s", hipGetErrorString(ei));
 
+    tg_status rc = TG_OK;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev)
+        if (rc == TG_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            rc = set_error(c, TG_ERR_HIP, "event creation");
+    // a hard limit on attempts guards against a controller that never ends
+    const int64_t max_attempts = replay ? (int64_t)h_cap + 2 : 100000000;
+    for (int64_t a = 0; rc == TG_OK; ++a) {
+        if (a >= max_attempts) {
+            rc = set_error(c, TG_ERR_STATE, "more than %lld DP5 attempts", (long long)max_attempts);
+            break;
+        }
+        if (hipGraphLaunch(exec, st) != hipSuccess || hipEventRecord(ev[a & 1], st) != hipSuccess) {
+            rc = set_error(c, TG_ERR_HIP, "graph launch of a DP5 attempt");
+            break;
+        }
+        if (a == 0) continue;
+        if (hipEventSynchronize(ev[(a - 1) & 1]) != hipSuccess) {
+            rc = set_error(c, TG_ERR_HIP, "DP5 attempt failed on the device");
+            break;
+        }
+        if (*(volatile int32_t*)w->running_host == 0) break;
+    }
+    const hipError_t es = hipStreamSynchronize(st);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipGraphExecDestroy(exec);
+    if (rc != TG_OK) return rc;
+    if (es != hipSuccess) return set_error(c, TG_ERR_HIP, "DP5 attempts: %s", hipGetErrorString(es));
+
+    TG_HIP(c, hipMemcpyAsync(ms.data(), w->md, sizeof(MemberDev) * B, hipMemcpyDeviceToHost, st));
     TG_HIP(c, hipMemcpyAsync(f, w->y, sizeof(double) * vec,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     if (dense_out && !dev)
         TG_HIP(c, hipMemcpyAsync(dense_out, d_dense, sizeof(double) * (size_t)max_dense * vec,
                                  hipMemcpyDeviceToHost, st));
+    if (h_acc)
+        TG_HIP(c, hipMemcpyAsync(h_acc, w->hlog, sizeof(double) * (size_t)B * h_cap,
+                                 hipMemcpyDeviceToHost, st));
     TG_HIP(c, hipStreamSynchronize(st));
-    tg_status rc = TG_OK;
+    if (!quiet && B == 1) printf("Number of adaptive steps: %lld\n", (long long)ms[0].st.attempted);
     for (int b = 0; b < B; ++b) {
         ms[b].st.failed = ms[b].failed ? 1 : 0;
         if (stats) stats[b] = ms[b].st;
@@ -430,5 +563,5 @@ extern "C" tg_status tg_dp5_replay(tg_ctx* c, double* f, int64_t n, int32_t B, d
         if (n_steps[b] < 0 || n_steps[b] > h_cap)
             return set_error(c, TG_ERR_ARG, "n_steps[%d] outside [0, h_cap]", b);
     return dp5_run(c, f, n, B, t0, t1, dense_dt, 1.0, 1.0, bath_n, flags | TG_QUIET, dense_out,
-                   max_dense, nullptr, nullptr, 1, h_steps, n_steps);
+                   max_dense, nullptr, nullptr, h_cap, h_steps, n_steps);
 }

@@ -173,6 +173,13 @@ tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_
 tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);
+tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t batch);
+tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch, int conserve);
+// error-norm sums sum_e (sum r^2 + (sum r)^2) per member into out[batch]
+// (part: batch x relerr_blocks partials); kernel launches only
+int relerr_blocks(tg_ctx* c);
+tg_status relerr_sums_device(tg_ctx* c, const double* d_err, const double* d_pop, int32_t batch,
+                             double rtol, double atol, double* part, double* out);
 void mixed_destroy(tg_ctx* c);
 void mixed_stats(tg_ctx* c, tg_kernel_stats* out);
 void dp5_destroy(tg_ctx* c);

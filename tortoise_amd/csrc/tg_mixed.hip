@@ -977,28 +977,42 @@ void mixed_destroy(tg_ctx* c) {
     c->mixed = nullptr;
 }
 
-tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B,
-                             const double* h_bath, int conserve) {
+// per-member bath occupations for the following evaluations
+tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
     TG_TRY(mixed_prepare(c));
     MixedData* m = md_of(c);
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
-    hipStream_t st = c->stream;
-    const int64_t n = c->n;
-    const int64_t kgroups = m->kpad / 8;
-    TG_TRY(ensure_P(c, m));
     std::vector<double> bath(B);
     for (int b = 0; b < B; ++b) bath[b] = h_bath ? h_bath[b] : c->bath_n;
-    TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice,
+                             c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
 
+// one batched evaluation with the bath set by mixed_set_bath: kernel launches
+// only, so a caller may capture it in a graph once it has run once (buffers
+// are allocated on first use)
+tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, int conserve) {
+    MixedData* m = md_of(c);
+    if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
+    if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
+    TG_TRY(ensure_P(c, m));
     TG_TRY(launch_pack(c, m, d_f, B));
-    hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, st, d_f, n, m->kt, B,
-                       m->npanel, m->perm, m->useg, m->nt);
+    hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, c->stream, d_f, c->n,
+                       m->kt, B, m->npanel, m->perm, m->useg, m->nt);
     TG_CHECK_LAUNCH(c);
     time_begin(c);
     TG_TRY(launch_gemm(c, m));
     time_end(c);
     TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve));
     return TG_OK;
+}
+
+tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B,
+                             const double* h_bath, int conserve) {
+    TG_TRY(mixed_set_bath(c, h_bath, B));
+    return mixed_eval_device(c, d_f, d_I, B, conserve);
 }
 
 }  // namespace tg

@@ -78,6 +78,54 @@ def cpu_baseline(ctx, w, f, budget_s=15.0):
                       f"{len(times)} evaluations ({sum(times):.1f} s)"}
 
 
+def dp5_batched(ctx, w, F, bath, check=(0, 128, 255)):
+    """The batched workload integrated: all members with DP5 to t_final (per-
+    member adaptive steps, rtol 1e-6, number-defect removal), the attempt
+    graph on the device.  Error of f(t_final) against FP64 replays of the same
+    accepted steps for a few members; particle-number drift for all."""
+    import numpy as np
+
+    V = ctx.node_volumes()
+    n0 = V @ F
+    t0 = time.perf_counter()
+    fT, _, st, steps = ctx.dp5(F, 0.0, w.t_final, w.t_final / 100, w.rtol, w.atol, bath_n=bath,
+                               mixed=True, conserve=True, h_cap=8192)
+    wall = time.perf_counter() - t0
+    att = np.array([s["attempted"] for s in st])
+    evals = int(sum(s["evals"] for s in st))
+    errs = []
+    for b in check:
+        fr, _ = ctx.dp5_replay(F[:, b], 0.0, w.t_final, w.t_final / 100, steps[b], bath_n=bath[b])
+        errs.append(float(np.linalg.norm(fT[:, b] - fr) / np.linalg.norm(fr)))
+    drift = float(np.max(np.abs(V @ fT - n0) / np.abs(n0)))
+    _log(f"dp5 batched: {F.shape[1]} members to t={w.t_final} in {wall:.3f} s, attempts per member "
+         f"{att.min()}..{att.max()}, f(t_final) rel err vs FP64 replay {max(errs):.2e}, "
+         f"number drift {drift:.1e}")
+    return {"members": int(F.shape[1]), "t_final": w.t_final, "rtol": w.rtol, "wall_s": wall,
+            "attempts_min": int(att.min()), "attempts_max": int(att.max()),
+            "member_evals_per_s": evals / wall, "f_final_rel_err_vs_fp64": max(errs),
+            "checked_members": list(check), "number_drift_rel": drift}
+
+
+def dp5_plain(ctx, w, f0, bath_n):
+    """One member in FP64 with DP5 to t_final at rtol 1e-8 (the plain
+    configuration on the same mesh)."""
+    import numpy as np
+
+    from tortoise_amd import workloads as wl
+
+    p = wl.SQUARE128
+    V = ctx.node_volumes()
+    t0 = time.perf_counter()
+    fT, _, st = ctx.dp5(f0, 0.0, p.t_final, p.t_final / 100, p.rtol, p.atol, bath_n=bath_n)
+    wall = time.perf_counter() - t0
+    drift = float(abs(V @ fT - V @ f0) / abs(V @ f0))
+    _log(f"dp5 plain: t={p.t_final} rtol {p.rtol} in {wall:.3f} s, {st['attempted']} attempts, "
+         f"number drift {drift:.1e}")
+    return {"t_final": p.t_final, "rtol": p.rtol, "wall_s": wall, "attempts": st["attempted"],
+            "evals_per_s": st["evals"] / wall, "number_drift_rel": drift}
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes (the known-size reference for the
     FETCH_SIZE / WRITE_SIZE counters)."""
@@ -110,6 +158,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dp5", action="store_true", help="skip the full DP5 runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -228,6 +277,9 @@ def main():
         "max_rel_err_vs_fp64": rel,
         "wall_s_timed": wall,
     }
+    if not args.no_dp5:
+        out["dp5_batched"] = dp5_batched(ctx, w, Fh, bath)
+        out["dp5_plain"] = dp5_plain(ctx, w, Fh[:, 0].copy(), bath[0])
     if not args.no_cpu:
         cb = cpu_baseline(ctx, w, Fh[:, 0].copy())
         out["cpu_baseline"] = cb

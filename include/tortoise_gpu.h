@@ -65,6 +65,16 @@ typedef struct tg_problem_desc {
     double  cutoff;         /* Gaussian cut-off in units of sigma (<= 0: 8) */
     int32_t max_batch;      /* largest batch the context will be asked for (>= 1) */
     int32_t reserved;
+    /* electron-electron (4-leg) channel on the periodic lattice of the box
+     * (off when ee_g == 0): amplitude ee_g^2 / (|q|^2 + ee_kappa2)^2, energy
+     * delta a Gaussian of width ee_sigma (cut at `cutoff` widths) projected on
+     * a grid of spacing ee_dw (<= 0: ee_sigma/4).  Replaces a 4-leg
+     * ScatteringChannel<NDim,4> with functAmplitude = V(q)^2
+     * (src/Physics/Scattering/ScatteringChannel.cpp:399-415,618-650). */
+    double  ee_g;
+    double  ee_kappa2;
+    double  ee_sigma;
+    double  ee_dw;
 } tg_problem_desc;
 
 typedef struct tg_step_stats {

@@ -103,6 +103,11 @@ def lib():
                                      C.c_double, _f64p, C.c_double, C.c_double, C.c_double, _f64p,
                                      C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.or_ee_grid.argtypes = [C.c_int, _i32p, _f64p, C.c_double, C.POINTER(C.c_int32)]
+        L.or_ee_apply.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, _f64p]
+        L.or_ee_apply_direct.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, _f64p]
         _lib = L
     return _lib
 
@@ -337,3 +342,34 @@ def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL
                          steps.size, ckpt, np.ascontiguousarray(dL_df, np.float64), out,
                          C.byref(dg), C.byref(dw))
     return out, dg.value, dw.value
+
+
+# ------------------------------------------------------------ e-e channel
+def ee_apply(dim, nsec, extent, eps, f, g, kappa2, sigma, dw, cutoff=8.0):
+    """Electron-electron operator with the energy delta projected on the
+    grid m*dw (ee_oracle.c); one value per mesh node."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    ext = np.ascontiguousarray(list(extent) + [1.0] * (3 - len(extent)), np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    out = np.zeros_like(f)
+    lib().or_ee_apply(dim, nsec, ext, np.ascontiguousarray(eps, np.float64), f, g, kappa2, sigma,
+                      dw, cutoff, out)
+    return out
+
+
+def ee_apply_direct(dim, nsec, extent, eps, f, g, kappa2, sigma, cutoff=8.0):
+    """The same operator with the unprojected Gaussian delta (O(P^3))."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    ext = np.ascontiguousarray(list(extent) + [1.0] * (3 - len(extent)), np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    out = np.zeros_like(f)
+    lib().or_ee_apply_direct(dim, nsec, ext, np.ascontiguousarray(eps, np.float64), f, g, kappa2,
+                             sigma, cutoff, out)
+    return out
+
+
+def ee_grid(dim, nsec, eps, dw):
+    mh = C.c_int32()
+    lib().or_ee_grid(dim, np.ascontiguousarray(nsec, np.int32),
+                     np.ascontiguousarray(eps, np.float64), dw, C.byref(mh))
+    return mh.value

@@ -176,6 +176,21 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
                     double* dL_domega0);
 
+/* ------------------------------------------------------- e-e channel */
+/* Electron-electron (4-leg) operator on the periodic lattice of the mesh box
+ * (ee_oracle.c): A(q) = g^2/(|q|^2+kappa2)^2, Gaussian energy delta of width
+ * sigma cut at cutoff*sigma, projected on the grid m*dw, |m| <= mh
+ * (or_ee_grid).  I is written for every mesh node (duplicates of a periodic
+ * point share its value).  or_ee_apply_direct: the unprojected delta by
+ * brute force (O(P^3)). */
+void or_ee_grid(int dim, const int32_t* nsec, const double* eps, double dw, int32_t* mh);
+void or_ee_apply(int dim, const int32_t* nsec, const double* extent, const double* eps,
+                 const double* f, double g, double kappa2, double sigma, double dw,
+                 double cutoff, double* I);
+void or_ee_apply_direct(int dim, const int32_t* nsec, const double* extent, const double* eps,
+                        const double* f, double g, double kappa2, double sigma, double cutoff,
+                        double* I);
+
 #ifdef __cplusplus
 }
 #endif

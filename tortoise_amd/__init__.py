@@ -60,7 +60,8 @@ class ProblemDesc(C.Structure):
         ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("g", C.c_double),
         ("omega0", C.c_double), ("sigma", C.c_double), ("bath_n", C.c_double),
         ("de", C.c_double), ("cutoff", C.c_double), ("max_batch", C.c_int32),
-        ("reserved", C.c_int32),
+        ("reserved", C.c_int32), ("ee_g", C.c_double), ("ee_kappa2", C.c_double),
+        ("ee_sigma", C.c_double), ("ee_dw", C.c_double),
     ]
 
 
@@ -158,7 +159,7 @@ class Context:
 
     @classmethod
     def create(cls, workload, max_batch=None, eps=None, g=None, omega0=None, sigma=None,
-               bath_n=None, de=0.0, cutoff=0.0):
+               bath_n=None, de=0.0, cutoff=0.0, ee_g=None, ee_dw=0.0):
         L = load()
         w = workload
         e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
@@ -176,6 +177,10 @@ class Context:
         d.de = de
         d.cutoff = cutoff
         d.max_batch = max_batch or w.batch
+        d.ee_g = w.ee_g if ee_g is None else ee_g
+        d.ee_kappa2 = w.ee_kappa2
+        d.ee_sigma = w.ee_sigma
+        d.ee_dw = ee_dw
         h = C.c_void_p()
         rc = L.tg_ctx_create(C.byref(d), C.byref(h))
         if rc != TG_OK:

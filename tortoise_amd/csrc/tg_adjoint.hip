@@ -262,6 +262,7 @@ extern "C" tg_status tg_dp5_adjoint(tg_ctx* c, const double* f0, int64_t n, doub
     if (n_steps < 1 || ckpt < 1 || !(dense_dt > 0.0)) return set_error(c, TG_ERR_ARG, "bad steps");
     if (!(c->g != 0.0)) return set_error(c, TG_ERR_ARG, "g must be nonzero for dL/dg");
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "tg_kernel_assemble not called");
+    if (c->ee) return set_error(c, TG_ERR_STATE, "the adjoint covers the e-ph channel only (e-e is on)");
     TG_TRY(assemble_domega(c));
     const int32_t nck = n_steps / ckpt + 2;
     TG_TRY(adj_ensure(c, ckpt, nck));

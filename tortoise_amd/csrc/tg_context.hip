@@ -305,6 +305,15 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     }
     c->e0 = lo - c->de;
     c->nbins = (int32_t)std::floor((hi - c->e0) / c->de) + 3;
+    if (d->ee_g != 0.0) {
+        if (!(d->ee_sigma > 0.0) || !(d->ee_kappa2 > 0.0) || !std::isfinite(d->ee_g))
+            return fail(set_error(c, TG_ERR_ARG, "e-e channel needs ee_sigma > 0 and ee_kappa2 > 0"));
+        c->ee_g = d->ee_g;
+        c->ee_kappa2 = d->ee_kappa2;
+        c->ee_sigma = d->ee_sigma;
+        c->ee_dw = d->ee_dw;
+        if ((s = ee_prepare(c, d->eps)) != TG_OK) return fail(s);
+    }
     *out = c;
     return TG_OK;
 }
@@ -315,6 +324,7 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     mixed_destroy(c);
     dp5_destroy(c);
     adjoint_destroy(c);
+    ee_destroy(c);
     dev_free(&c->d_Kd);
     dev_free(&c->d_rkd);
     dev_free(&c->d_ckd);

@@ -310,11 +310,14 @@ void dp5_destroy(tg_ctx* c) {
 static tg_status eval_batch(tg_ctx* c, const double* d_in, double* d_out, int32_t B,
                             const std::vector<double>& bath, int32_t flags) {
     const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
-    if (flags & TG_PREC_MIXED) return mixed_eval_device(c, d_in, d_out, B, conserve);
-    for (int b = 0; b < B; ++b)
-        TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, bath[b],
-                                  conserve));
-    return TG_OK;
+    if (flags & TG_PREC_MIXED) {
+        TG_TRY(mixed_eval_device(c, d_in, d_out, B, conserve));
+    } else {
+        for (int b = 0; b < B; ++b)
+            TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, bath[b],
+                                      conserve));
+    }
+    return ee_apply_device(c, d_in, d_out, B);
 }
 
 // the device work of one attempted step (captured into a graph)

@@ -1,0 +1,358 @@
+// tg_ee.hip -- electron-electron (4-leg) collision operator on the periodic
+// momentum lattice of the mesh box.
+//
+//   I_1 = sum_{2,3} w^2 A(k1-k3) delta(e1 + e2 - e3 - e4)
+//                   [ f3 f4 (1-f1)(1-f2) - f1 f2 (1-f3)(1-f4) ],   k4 = k1 + k2 - k3
+//
+// over the P = prod(nsec) periodic points (the mesh's duplicate boundary
+// nodes carry their periodic image's value), w = volume per point,
+// A(q) = g^2/(|q|^2 + kappa2)^2 (minimum image), the Gaussian energy delta
+// projected on the grid m*dw with hat functions (the reference counterpart is
+// ScatteringChannel<3,4> with functAmplitude = V(q)^2,
+// src/Physics/Scattering/ScatteringChannel.cpp:399-415,618-650).
+//
+// Grouping by the momentum transfer q = k1 - k3 (then k4 = k2 + q) makes the
+// raw O(P^3) sum O(P^2 + P M J): per q, the pair spectra
+//   Ha_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2+q)(1 - f(2))   (gain)
+//   Hb_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2)(1 - f(2+q))   (loss)
+// are convolved with the Gaussian on the grid and read back at e1 - e3 for
+// every k1.  A workgroup owns 16 values of q and one member:
+//   - the spectrum is accumulated in LDS as int64 fixed point (atomic adds of
+//     integers are exact, so the sum does not depend on their order), then
+//     converted to FP64 in place;
+//   - the convolution runs in FP64, 8 grid outputs per thread from a sliding
+//     window of the spectrum;
+//   - each thread keeps the sums of its k1 values in registers across the
+//     16 transfers and writes them once;
+// and a last kernel sums the workgroups' partials per k1 in a fixed order and
+// adds the result to the caller's output on every mesh node.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tg_internal.hpp"
+
+namespace tg {
+
+constexpr int kEeT = 1024;      // threads of the main kernel
+constexpr int kEeAccMax = 16;   // k1 values per thread: P <= kEeT * kEeAccMax
+constexpr int kEeQTile = 16;    // momentum transfers per workgroup
+constexpr int kEeConvR = 8;     // grid outputs per thread in the convolution
+constexpr int kEeMaxJ = 256;    // largest Gaussian half-width in grid cells
+
+struct EeData {
+    int dim = 2;
+    int ns[3] = {1, 1, 1};
+    int64_t P = 0;
+    double w = 0.0, dw = 0.0;
+    int32_t Mh = 0, M = 0, J = 0;
+    int32_t ntile = 0, cap = 0;
+    double* gtab = nullptr;      // G_sigma(j dw), |j| <= J
+    double* ep = nullptr;        // energies on the lattice (P)
+    int32_t* node_of_p = nullptr;
+    int32_t* p_of_node = nullptr;
+    double* amp = nullptr;       // A(q) (P)
+    double* fp = nullptr;        // f on the lattice, cap x P
+    int32_t* fexp = nullptr;     // fixed-point exponent per member
+    double* part = nullptr;      // cap x ntile x P partial sums
+};
+
+struct EeLattice {
+    int dim;
+    int ns0, ns1, ns2;
+};
+
+__device__ inline int64_t lat_add(const EeLattice& L, int64_t a, int64_t b, int sign) {
+    int64_t a0 = a % L.ns0, b0 = b % L.ns0;
+    int64_t c0 = (a0 + sign * b0) % L.ns0;
+    if (c0 < 0) c0 += L.ns0;
+    if (L.dim == 1) return c0;
+    a /= L.ns0;
+    b /= L.ns0;
+    int64_t a1 = a % L.ns1, b1 = b % L.ns1;
+    int64_t c1 = (a1 + sign * b1) % L.ns1;
+    if (c1 < 0) c1 += L.ns1;
+    if (L.dim == 2) return c0 + (int64_t)L.ns0 * c1;
+    a /= L.ns1;
+    b /= L.ns1;
+    int64_t c2 = (a + sign * b) % L.ns2;
+    if (c2 < 0) c2 += L.ns2;
+    return c0 + (int64_t)L.ns0 * (c1 + (int64_t)L.ns1 * c2);
+}
+
+// f on the lattice per member and the fixed-point exponent E of its spectra:
+// every bin sum is bounded by P w m (1 + m), m = max|f|, and E puts that
+// bound at 2^61
+__global__ void __launch_bounds__(kEeT)
+ee_prep_kernel(const double* __restrict__ f, int64_t n, int64_t P,
+               const int32_t* __restrict__ node_of_p, double w, double* __restrict__ fp,
+               int32_t* __restrict__ fexp) {
+    __shared__ double red[kEeT / 64];
+    const int b = blockIdx.x;
+    double m = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kEeT) {
+        const double x = f[(int64_t)b * n + node_of_p[p]];
+        fp[(int64_t)b * P + p] = x;
+        m = fmax(m, fabs(x));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < kEeT / 64; ++i) mm = fmax(mm, red[i]);
+        const double bound = (double)P * w * mm * (1.0 + mm);
+        int32_t e = 61;
+        if (!isfinite(bound)) e = 0;
+        else if (bound > 0.0) e = (int32_t)floor(61.0 - log2(bound));
+        fexp[b] = max(-1000, min(1000, e));
+    }
+}
+
+template <int ACC>
+__global__ void __launch_bounds__(kEeT)
+ee_main_kernel(EeLattice L, int64_t P, int32_t Mh, int32_t M, int32_t J, double dw, double w,
+               const double* __restrict__ gtab, const double* __restrict__ ep,
+               const double* __restrict__ amp, const double* __restrict__ fp,
+               const int32_t* __restrict__ fexp, int ntile, double* __restrict__ part) {
+    extern __shared__ double lds[];
+    __shared__ double g[2 * kEeMaxJ + 1];
+    long long* Hi = reinterpret_cast<long long*>(lds);   // spectrum, fixed point
+    double* Hd = lds;                                      // the same, FP64
+    double* S = lds + M;                                   // convolved spectrum
+    const int tid = threadIdx.x;
+    const int b = blockIdx.y;
+    const int tile = blockIdx.x;
+    const int64_t q0 = (int64_t)tile * kEeQTile;
+    const int64_t q1 = min(q0 + kEeQTile, P);
+    const double* fb = fp + (int64_t)b * P;
+    const double scale = ldexp(1.0, fexp[b]);
+    const double inv = ldexp(1.0, -fexp[b]);
+    for (int j = tid; j <= 2 * J; j += kEeT) g[j] = gtab[j];
+
+    double acc[ACC];
+#pragma unroll
+    for (int i = 0; i < ACC; ++i) acc[i] = 0.0;
+
+    for (int64_t q = q0; q < q1; ++q) {
+        const double aq = w * amp[q];
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int m = tid; m < M; m += kEeT) Hi[m] = 0;
+            __syncthreads();
+            // pair spectrum of transfer q: k4 = k2 + q
+            for (int64_t p2 = tid; p2 < P; p2 += kEeT) {
+                const int64_t p4 = lat_add(L, p2, q, +1);
+                const double y = (ep[p4] - ep[p2]) / dw;
+                const double fl = floor(y);
+                const double t = y - fl;
+                const int m = (int)fl + Mh;
+                const double f2 = fb[p2], f4 = fb[p4];
+                const double F = pass == 0 ? w * f4 * (1.0 - f2) : w * f2 * (1.0 - f4);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&Hi[m]),
+                          (unsigned long long)llrint((1.0 - t) * F * scale));
+                atomicAdd(reinterpret_cast<unsigned long long*>(&Hi[m + 1]),
+                          (unsigned long long)llrint(t * F * scale));
+            }
+            __syncthreads();
+            for (int m = tid; m < M; m += kEeT) Hd[m] = (double)Hi[m] * inv;
+            __syncthreads();
+            // S = G * H: kEeConvR outputs per thread from one window of H
+            for (int m0 = tid * kEeConvR; m0 < M; m0 += kEeT * kEeConvR) {
+                double out[kEeConvR];
+#pragma unroll
+                for (int r = 0; r < kEeConvR; ++r) out[r] = 0.0;
+                for (int k = m0 - J; k < m0 + kEeConvR + J; ++k) {
+                    if (k < 0 || k >= M) continue;
+                    const double h = Hd[k];
+#pragma unroll
+                    for (int r = 0; r < kEeConvR; ++r) {
+                        const int j = m0 + r - k;   // output m0 + r, source k
+                        if (j >= -J && j <= J) out[r] += g[j + J] * h;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < kEeConvR; ++r)
+                    if (m0 + r < M) S[m0 + r] = out[r];
+            }
+            __syncthreads();
+            // pairs (1, 3) with k1 - k3 = q
+#pragma unroll
+            for (int i = 0; i < ACC; ++i) {
+                const int64_t p1 = tid + (int64_t)i * kEeT;
+                if (p1 < P) {
+                    const int64_t p3 = lat_add(L, p1, q, -1);
+                    const double x = (ep[p1] - ep[p3]) / dw;
+                    const double fl = floor(x);
+                    const double t = x - fl;
+                    const int m = (int)fl + Mh;
+                    const double s = (1.0 - t) * S[m] + t * S[m + 1];
+                    const double f1 = fb[p1], f3 = fb[p3];
+                    acc[i] += pass == 0 ? aq * ((1.0 - f1) * f3 * s) : -aq * (f1 * (1.0 - f3) * s);
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ACC; ++i) {
+        const int64_t p1 = tid + (int64_t)i * kEeT;
+        if (p1 < P) part[((int64_t)b * ntile + tile) * P + p1] = acc[i];
+    }
+}
+
+// out[b][node] += sum over workgroups (fixed order) of the node's periodic point
+__global__ void ee_reduce_kernel(const double* __restrict__ part, int ntile, int64_t P,
+                                 const int32_t* __restrict__ p_of_node, int64_t n,
+                                 double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = p_of_node[i];
+    double s = 0.0;
+    for (int t = 0; t < ntile; ++t) s += part[((int64_t)b * ntile + t) * P + p];
+    out[(int64_t)b * n + i] += s;
+}
+
+static EeData* ee_of(tg_ctx* c) { return (EeData*)c->ee; }
+
+tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
+    if (c->ee) return TG_OK;
+    EeData* e = new EeData();
+    c->ee = e;
+    e->dim = c->dim;
+    e->P = 1;
+    e->w = 1.0;
+    for (int d = 0; d < 3; ++d) e->ns[d] = d < c->dim ? c->nsec[d] : 1;
+    for (int d = 0; d < c->dim; ++d) {
+        e->P *= e->ns[d];
+        e->w *= c->extent[d] / e->ns[d];
+    }
+    if (e->P > (int64_t)kEeT * kEeAccMax)
+        return set_error(c, TG_ERR_ARG, "e-e channel: %lld lattice points exceed %d",
+                         (long long)e->P, kEeT * kEeAccMax);
+    e->dw = c->ee_dw > 0.0 ? c->ee_dw : c->ee_sigma / 4.0;
+    double lo = h_eps[0], hi = h_eps[0];
+    for (int64_t i = 1; i < c->n; ++i) {
+        lo = std::fmin(lo, h_eps[i]);
+        hi = std::fmax(hi, h_eps[i]);
+    }
+    e->Mh = (int32_t)std::ceil((hi - lo) / e->dw) + 2;
+    e->M = 2 * e->Mh + 1;
+    e->J = (int32_t)std::floor(c->cutoff * c->ee_sigma / e->dw);
+    if (e->J > kEeMaxJ)
+        return set_error(c, TG_ERR_ARG, "e-e channel: Gaussian spans %d grid cells (max %d)", e->J,
+                         kEeMaxJ);
+    hipDeviceProp_t prop;
+    TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
+    const size_t lds = sizeof(double) * 2 * (size_t)e->M;
+    if (lds + sizeof(double) * (2 * kEeMaxJ + 1) > (size_t)prop.sharedMemPerBlock)
+        return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds the LDS", e->M);
+    for (const void* k : {(const void*)ee_main_kernel<4>, (const void*)ee_main_kernel<8>,
+                          (const void*)ee_main_kernel<16>})
+        TG_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    e->ntile = (int32_t)((e->P + kEeQTile - 1) / kEeQTile);
+    e->cap = c->max_batch;
+
+    // lattice tables (host)
+    const int64_t P = e->P;
+    std::vector<int32_t> node_of_p(P), p_of_node(c->n);
+    std::vector<double> ep(P), amp(P);
+    for (int64_t p = 0; p < P; ++p) {
+        int64_t r = p, id = 0, stride = 1;
+        for (int d = 0; d < c->dim; ++d) {
+            const int64_t cd = r % e->ns[d];
+            r /= e->ns[d];
+            id += cd * stride;
+            stride *= c->nsec[d] + 1;
+        }
+        node_of_p[p] = (int32_t)id;
+        ep[p] = h_eps[id];
+        // A(q) for the lattice vector q = p (minimum image)
+        double q2 = 0.0;
+        r = p;
+        for (int d = 0; d < c->dim; ++d) {
+            const int64_t cd = r % e->ns[d];
+            r /= e->ns[d];
+            const double box = c->extent[d];
+            double qd = cd * (box / e->ns[d]);
+            if (qd >= 0.5 * box) qd -= box;
+            q2 += qd * qd;
+        }
+        const double v = 1.0 / (q2 + c->ee_kappa2);
+        amp[p] = c->ee_g * c->ee_g * v * v;
+    }
+    for (int64_t i = 0; i < c->n; ++i) {
+        int64_t r = i, p = 0, stride = 1;
+        for (int d = 0; d < c->dim; ++d) {
+            const int64_t cd = r % (c->nsec[d] + 1);
+            r /= c->nsec[d] + 1;
+            p += (cd % e->ns[d]) * stride;
+            stride *= e->ns[d];
+        }
+        p_of_node[i] = (int32_t)p;
+    }
+    std::vector<double> gt(2 * e->J + 1);
+    const double nrm = 1.0 / (c->ee_sigma * std::sqrt(2.0 * M_PI));
+    for (int j = -e->J; j <= e->J; ++j) {
+        const double x = j * e->dw;
+        gt[j + e->J] = nrm * std::exp(-0.5 * x * x / (c->ee_sigma * c->ee_sigma));
+    }
+    TG_TRY(dev_alloc(c, &e->gtab, gt.size()));
+    TG_TRY(dev_alloc(c, &e->ep, (size_t)P));
+    TG_TRY(dev_alloc(c, &e->amp, (size_t)P));
+    TG_TRY(dev_alloc(c, &e->node_of_p, (size_t)P));
+    TG_TRY(dev_alloc(c, &e->p_of_node, (size_t)c->n));
+    TG_TRY(dev_alloc(c, &e->fp, (size_t)P * e->cap));
+    TG_TRY(dev_alloc(c, &e->fexp, (size_t)e->cap));
+    TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap));
+    TG_HIP(c, hipMemcpy(e->gtab, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(e->ep, ep.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(e->amp, amp.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(e->node_of_p, node_of_p.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(e->p_of_node, p_of_node.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+    return TG_OK;
+}
+
+void ee_destroy(tg_ctx* c) {
+    EeData* e = ee_of(c);
+    if (!e) return;
+    dev_free(&e->gtab);
+    dev_free(&e->ep);
+    dev_free(&e->amp);
+    dev_free(&e->node_of_p);
+    dev_free(&e->p_of_node);
+    dev_free(&e->fp);
+    dev_free(&e->fexp);
+    dev_free(&e->part);
+    delete e;
+    c->ee = nullptr;
+}
+
+// d_I[b] += I_ee(d_f[b]) for the B members (caller order, member-major);
+// kernel launches only
+tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B) {
+    EeData* e = ee_of(c);
+    if (!e) return TG_OK;
+    if (B < 1 || B > e->cap) return set_error(c, TG_ERR_ARG, "e-e channel: batch %d outside [1, %d]", B, e->cap);
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(ee_prep_kernel, dim3(B), dim3(kEeT), 0, st, d_f, c->n, e->P, e->node_of_p, e->w,
+                       e->fp, e->fexp);
+    TG_CHECK_LAUNCH(c);
+    const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
+    const size_t lds = sizeof(double) * 2 * (size_t)e->M;
+#define TG_EE_MAIN(ACC)                                                                              \
+    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, e->P, e->Mh, \
+                       e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->amp, e->fp, e->fexp, e->ntile,    \
+                       e->part)
+    if (e->P <= 4 * kEeT) TG_EE_MAIN(4);
+    else if (e->P <= 8 * kEeT) TG_EE_MAIN(8);
+    else TG_EE_MAIN(16);
+#undef TG_EE_MAIN
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(ee_reduce_kernel, dim3(ceil_div(c->n, 256), B), dim3(256), 0, st, e->part,
+                       e->ntile, e->P, e->p_of_node, c->n, d_I);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+}  // namespace tg

@@ -102,6 +102,10 @@ struct tg_ctx {
     // DP5 work (tg_dp5.hip)
     void* dp5 = nullptr;
 
+    // electron-electron channel (tg_ee.hip), off when ee_g == 0
+    double ee_g = 0, ee_kappa2 = 0, ee_sigma = 0, ee_dw = 0;
+    void* ee = nullptr;
+
     // scratch vectors for host<->device staging (n x max_batch each)
     double* d_f = nullptr;
     double* d_I = nullptr;
@@ -183,6 +187,10 @@ tg_status relerr_sums_device(tg_ctx* c, const double* d_err, const double* d_pop
 void mixed_destroy(tg_ctx* c);
 void mixed_stats(tg_ctx* c, tg_kernel_stats* out);
 void dp5_destroy(tg_ctx* c);
+tg_status ee_prepare(tg_ctx* c, const double* h_eps);
+void ee_destroy(tg_ctx* c);
+// d_I[b] += I_ee(d_f[b]); no-op without the e-e channel; kernel launches only
+tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch);
 void adjoint_destroy(tg_ctx* c);
 tg_status ensure_staging(tg_ctx* c, int32_t batch);
 // event bracket of the dominant kernel when TG_TIME_KERNEL is active

@@ -294,6 +294,7 @@ extern "C" tg_status tg_collision_apply(tg_ctx* c, const double* f, double* I, i
         }
     }
     c->time_kernel = false;
+    TG_TRY(ee_apply_device(c, d_f, d_I, batch));
     if (!dev) {
         TG_HIP(c, hipMemcpyAsync(I, d_I, sizeof(double) * n * batch, hipMemcpyDeviceToHost,
                                  c->stream));

@@ -41,6 +41,11 @@ class Workload:
     rtol: float = 1e-8
     atol: float = 1e-12
     batch: int = 1
+    # electron-electron channel (off when ee_g == 0): amplitude
+    # ee_g^2 / (|q|^2 + ee_kappa2)^2, Gaussian energy delta of width ee_sigma
+    ee_g: float = 0.0
+    ee_kappa2: float = 0.09
+    ee_sigma: float = 0.02
 
     @property
     def origin(self):
@@ -112,8 +117,10 @@ SQUARE128_ENSEMBLE = Workload("square128_ensemble256_mixed", 2, (128, 128), t_fi
                               rtol=1e-6, atol=1e-12, batch=256)
 # 64x64 sections, 8192 triangles, 4225 nodes; gradients at t = 50
 SQUARE64 = Workload("square64_adjoint", 2, (64, 64), t_final=50.0, rtol=1e-8, atol=1e-12)
-# 20^3 sections x 6 = 48000 tetrahedra, 9261 nodes; 16 members; t = 50, rtol 1e-6
+# 20^3 sections x 6 = 48000 tetrahedra, 9261 nodes (8000 periodic points);
+# e-ph plus e-e with V(q) = 1/(|q|^2 + 0.09), sigma_ee = 0.02; 16 members;
+# t = 50, rtol 1e-6
 CUBE20 = Workload("cube20_ensemble16", 3, (20, 20, 20), t_final=50.0, rtol=1e-6, atol=1e-12,
-                  batch=16)
+                  batch=16, ee_g=1.0)
 
 ALL = {w.name: w for w in (SQUARE32, SQUARE128, SQUARE128_ENSEMBLE, SQUARE64, CUBE20)}

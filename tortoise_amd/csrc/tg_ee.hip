@@ -17,11 +17,12 @@
 //   Hb_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2)(1 - f(2+q))   (loss)
 // are convolved with the Gaussian on the grid and read back at e1 - e3 for
 // every k1.  A workgroup owns 16 values of q and one member:
-//   - the spectrum is accumulated in LDS as int64 fixed point (atomic adds of
-//     integers are exact, so the sum does not depend on their order), then
-//     converted to FP64 in place;
+//   - both spectra are accumulated in LDS as int64 fixed point from one sweep
+//     over k2 (integer adds are exact, so the sum does not depend on their
+//     order; lanes of a wave that hit the same bin are summed by a segmented
+//     scan before one atomic), then converted to FP64 in place;
 //   - the convolution runs in FP64, 8 grid outputs per thread from a sliding
-//     window of the spectrum;
+//     register window of the spectrum;
 //   - each thread keeps the sums of its k1 values in registers across the
 //     16 transfers and writes them once;
 // and a last kernel sums the workgroups' partials per k1 in a fixed order and
@@ -34,8 +35,8 @@
 
 namespace tg {
 
-constexpr int kEeT = 1024;      // threads of the main kernel
-constexpr int kEeAccMax = 16;   // k1 values per thread: P <= kEeT * kEeAccMax
+constexpr int kEeT = 512;       // threads of the main kernel
+constexpr int kEeAccMax = 32;   // k1 values per thread: P <= kEeT * kEeAccMax
 constexpr int kEeQTile = 16;    // momentum transfers per workgroup
 constexpr int kEeConvR = 8;     // grid outputs per thread in the convolution
 constexpr int kEeMaxJ = 256;    // largest Gaussian half-width in grid cells
@@ -51,6 +52,7 @@ struct EeData {
     double* ep = nullptr;        // energies on the lattice (P)
     int32_t* node_of_p = nullptr;
     int32_t* p_of_node = nullptr;
+    int32_t* coords = nullptr;   // packed lattice coordinates (P)
     double* amp = nullptr;       // A(q) (P)
     double* fp = nullptr;        // f on the lattice, cap x P
     int32_t* fexp = nullptr;     // fixed-point exponent per member
@@ -109,85 +111,158 @@ ee_prep_kernel(const double* __restrict__ f, int64_t n, int64_t P,
     }
 }
 
+// lattice coordinates are packed as c0 | c1 << 10 | c2 << 20 (at most 1023 per axis)
+// index of the point with coordinates a + sign*q (periodic), no divisions
+__device__ inline int ee_shift(const EeLattice& L, int a, int q, int sign) {
+    int c0 = (a & 1023) + sign * (q & 1023);
+    int c1 = ((a >> 10) & 1023) + sign * ((q >> 10) & 1023);
+    int c2 = (a >> 20) + sign * (q >> 20);
+    c0 += c0 < 0 ? L.ns0 : (c0 >= L.ns0 ? -L.ns0 : 0);
+    c1 += c1 < 0 ? L.ns1 : (c1 >= L.ns1 ? -L.ns1 : 0);
+    c2 += c2 < 0 ? L.ns2 : (c2 >= L.ns2 ? -L.ns2 : 0);
+    return c0 + L.ns0 * (c1 + L.ns1 * c2);
+}
+
+// Adds the four fixed-point deposits of every lane to its bins m, m + 1 of
+// the two spectra.  Lanes of one wave that hit the same bin (consecutive
+// lanes with equal m: common when the energy difference does not depend on
+// the fastest lattice axis) are summed first by a segmented scan, so each
+// run costs one set of atomics.  Integer sums: the result is exact.
+__device__ inline void ee_deposit(int m, long long a0, long long a1, long long b0, long long b1,
+                                  long long* __restrict__ Ha, long long* __restrict__ Hb) {
+    const int lane = threadIdx.x & 63;
+    const int mprev = __shfl_up(m, 1, 64);
+    int start = (lane == 0 || mprev != m) ? lane : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int s = __shfl_up(start, o, 64);
+        if (lane >= o) start = max(start, s);
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long x0 = __shfl_up(a0, o, 64), x1 = __shfl_up(a1, o, 64);
+        const long long y0 = __shfl_up(b0, o, 64), y1 = __shfl_up(b1, o, 64);
+        if (lane - o >= start) {
+            a0 += x0;
+            a1 += x1;
+            b0 += y0;
+            b1 += y1;
+        }
+    }
+    const int mnext = __shfl_down(m, 1, 64);
+    if ((lane == 63 || mnext != m) && m >= 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Ha[m]), (unsigned long long)a0);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Ha[m + 1]), (unsigned long long)a1);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Hb[m]), (unsigned long long)b0);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&Hb[m + 1]), (unsigned long long)b1);
+    }
+}
+
+// S = G * H on the grid: kEeConvR consecutive outputs per thread, the taps
+// walked from +J to -J with a sliding window of H in registers (one LDS read
+// of H and one broadcast read of G per tap)
+__device__ inline void ee_convolve(const double* __restrict__ H, const double* __restrict__ g,
+                                   int M, int J, double* __restrict__ S) {
+    for (int m0 = threadIdx.x * kEeConvR; m0 < M; m0 += kEeT * kEeConvR) {
+        double out[kEeConvR], h[kEeConvR];
+#pragma unroll
+        for (int r = 0; r < kEeConvR; ++r) {
+            out[r] = 0.0;
+            const int k = m0 + r - J;
+            h[r] = (k >= 0 && k < M) ? H[k] : 0.0;
+        }
+        for (int j = J; j >= -J; --j) {
+            const double gj = g[j + J];
+#pragma unroll
+            for (int r = 0; r < kEeConvR; ++r) out[r] += gj * h[r];
+#pragma unroll
+            for (int r = 0; r + 1 < kEeConvR; ++r) h[r] = h[r + 1];
+            const int k = m0 + kEeConvR - j;
+            h[kEeConvR - 1] = (k >= 0 && k < M) ? H[k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < kEeConvR; ++r)
+            if (m0 + r < M) S[m0 + r] = out[r];
+    }
+}
+
 template <int ACC>
 __global__ void __launch_bounds__(kEeT)
-ee_main_kernel(EeLattice L, int64_t P, int32_t Mh, int32_t M, int32_t J, double dw, double w,
+ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                const double* __restrict__ gtab, const double* __restrict__ ep,
+               const int32_t* __restrict__ coords,
                const double* __restrict__ amp, const double* __restrict__ fp,
                const int32_t* __restrict__ fexp, int ntile, double* __restrict__ part) {
     extern __shared__ double lds[];
     __shared__ double g[2 * kEeMaxJ + 1];
-    long long* Hi = reinterpret_cast<long long*>(lds);   // spectrum, fixed point
-    double* Hd = lds;                                      // the same, FP64
-    double* S = lds + M;                                   // convolved spectrum
+    long long* Ha = reinterpret_cast<long long*>(lds);       // gain spectrum, fixed point
+    long long* Hb = Ha + M;                                   // loss spectrum
+    double* S = lds + 2 * (size_t)M;                          // convolved spectrum
     const int tid = threadIdx.x;
     const int b = blockIdx.y;
     const int tile = blockIdx.x;
-    const int64_t q0 = (int64_t)tile * kEeQTile;
-    const int64_t q1 = min(q0 + kEeQTile, P);
+    const int q0 = tile * kEeQTile;
+    const int q1 = min(q0 + kEeQTile, P);
     const double* fb = fp + (int64_t)b * P;
     const double scale = ldexp(1.0, fexp[b]);
     const double inv = ldexp(1.0, -fexp[b]);
     for (int j = tid; j <= 2 * J; j += kEeT) g[j] = gtab[j];
 
+    // this thread's points p = tid + i*kEeT serve both as k2 (spectra) and as
+    // k1 (evaluation); only the sums of the k1 values live in registers
     double acc[ACC];
 #pragma unroll
     for (int i = 0; i < ACC; ++i) acc[i] = 0.0;
 
-    for (int64_t q = q0; q < q1; ++q) {
+    for (int q = q0; q < q1; ++q) {
         const double aq = w * amp[q];
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int m = tid; m < M; m += kEeT) Hi[m] = 0;
-            __syncthreads();
-            // pair spectrum of transfer q: k4 = k2 + q
-            for (int64_t p2 = tid; p2 < P; p2 += kEeT) {
-                const int64_t p4 = lat_add(L, p2, q, +1);
+        const int qc = coords[q];
+        for (int m = tid; m < 2 * M; m += kEeT) Ha[m] = 0;
+        __syncthreads();
+        // both pair spectra of transfer q: k4 = k2 + q (every lane takes part
+        // in every iteration: the deposit scans across the wave)
+#pragma unroll 1
+        for (int i = 0; i < ACC; ++i) {
+            const int p2 = tid + i * kEeT;
+            int m = -1;
+            long long a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+            if (p2 < P) {
+                const int p4 = ee_shift(L, coords[p2], qc, +1);
+                const double f2 = fb[p2], f4 = fb[p4];
                 const double y = (ep[p4] - ep[p2]) / dw;
                 const double fl = floor(y);
                 const double t = y - fl;
-                const int m = (int)fl + Mh;
-                const double f2 = fb[p2], f4 = fb[p4];
-                const double F = pass == 0 ? w * f4 * (1.0 - f2) : w * f2 * (1.0 - f4);
-                atomicAdd(reinterpret_cast<unsigned long long*>(&Hi[m]),
-                          (unsigned long long)llrint((1.0 - t) * F * scale));
-                atomicAdd(reinterpret_cast<unsigned long long*>(&Hi[m + 1]),
-                          (unsigned long long)llrint(t * F * scale));
+                m = (int)fl + Mh;
+                const double Fa = w * f4 * (1.0 - f2);
+                const double Fb = w * f2 * (1.0 - f4);
+                a0 = llrint((1.0 - t) * Fa * scale);
+                a1 = llrint(t * Fa * scale);
+                b0 = llrint((1.0 - t) * Fb * scale);
+                b1 = llrint(t * Fb * scale);
             }
-            __syncthreads();
+            ee_deposit(m, a0, a1, b0, b1, Ha, Hb);
+        }
+        __syncthreads();
+        for (int pass = 0; pass < 2; ++pass) {
+            long long* Hi = pass == 0 ? Ha : Hb;
+            double* Hd = reinterpret_cast<double*>(Hi);
             for (int m = tid; m < M; m += kEeT) Hd[m] = (double)Hi[m] * inv;
             __syncthreads();
-            // S = G * H: kEeConvR outputs per thread from one window of H
-            for (int m0 = tid * kEeConvR; m0 < M; m0 += kEeT * kEeConvR) {
-                double out[kEeConvR];
-#pragma unroll
-                for (int r = 0; r < kEeConvR; ++r) out[r] = 0.0;
-                for (int k = m0 - J; k < m0 + kEeConvR + J; ++k) {
-                    if (k < 0 || k >= M) continue;
-                    const double h = Hd[k];
-#pragma unroll
-                    for (int r = 0; r < kEeConvR; ++r) {
-                        const int j = m0 + r - k;   // output m0 + r, source k
-                        if (j >= -J && j <= J) out[r] += g[j + J] * h;
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < kEeConvR; ++r)
-                    if (m0 + r < M) S[m0 + r] = out[r];
-            }
+            ee_convolve(Hd, g, M, J, S);
             __syncthreads();
             // pairs (1, 3) with k1 - k3 = q
 #pragma unroll
             for (int i = 0; i < ACC; ++i) {
-                const int64_t p1 = tid + (int64_t)i * kEeT;
+                const int p1 = tid + i * kEeT;
                 if (p1 < P) {
-                    const int64_t p3 = lat_add(L, p1, q, -1);
+                    const int p3 = ee_shift(L, coords[p1], qc, -1);
                     const double x = (ep[p1] - ep[p3]) / dw;
                     const double fl = floor(x);
                     const double t = x - fl;
                     const int m = (int)fl + Mh;
-                    const double s = (1.0 - t) * S[m] + t * S[m + 1];
+                    const double sv = (1.0 - t) * S[m] + t * S[m + 1];
                     const double f1 = fb[p1], f3 = fb[p3];
-                    acc[i] += pass == 0 ? aq * ((1.0 - f1) * f3 * s) : -aq * (f1 * (1.0 - f3) * s);
+                    acc[i] += pass == 0 ? aq * ((1.0 - f1) * f3 * sv) : -aq * (f1 * (1.0 - f3) * sv);
                 }
             }
             __syncthreads();
@@ -195,7 +270,7 @@ ee_main_kernel(EeLattice L, int64_t P, int32_t Mh, int32_t M, int32_t J, double 
     }
 #pragma unroll
     for (int i = 0; i < ACC; ++i) {
-        const int64_t p1 = tid + (int64_t)i * kEeT;
+        const int p1 = tid + i * kEeT;
         if (p1 < P) part[((int64_t)b * ntile + tile) * P + p1] = acc[i];
     }
 }
@@ -227,6 +302,8 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
         e->P *= e->ns[d];
         e->w *= c->extent[d] / e->ns[d];
     }
+    for (int d = 0; d < c->dim; ++d)
+        if (e->ns[d] > 1023) return set_error(c, TG_ERR_ARG, "e-e channel: more than 1023 points per axis");
     if (e->P > (int64_t)kEeT * kEeAccMax)
         return set_error(c, TG_ERR_ARG, "e-e channel: %lld lattice points exceed %d",
                          (long long)e->P, kEeT * kEeAccMax);
@@ -244,18 +321,18 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
                          kEeMaxJ);
     hipDeviceProp_t prop;
     TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
-    const size_t lds = sizeof(double) * 2 * (size_t)e->M;
+    const size_t lds = sizeof(double) * 3 * (size_t)e->M;
     if (lds + sizeof(double) * (2 * kEeMaxJ + 1) > (size_t)prop.sharedMemPerBlock)
         return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds the LDS", e->M);
-    for (const void* k : {(const void*)ee_main_kernel<4>, (const void*)ee_main_kernel<8>,
-                          (const void*)ee_main_kernel<16>})
+    for (const void* k : {(const void*)ee_main_kernel<8>, (const void*)ee_main_kernel<16>,
+                          (const void*)ee_main_kernel<32>})
         TG_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     e->ntile = (int32_t)((e->P + kEeQTile - 1) / kEeQTile);
     e->cap = c->max_batch;
 
     // lattice tables (host)
     const int64_t P = e->P;
-    std::vector<int32_t> node_of_p(P), p_of_node(c->n);
+    std::vector<int32_t> node_of_p(P), p_of_node(c->n), coords(P);
     std::vector<double> ep(P), amp(P);
     for (int64_t p = 0; p < P; ++p) {
         int64_t r = p, id = 0, stride = 1;
@@ -267,6 +344,15 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
         }
         node_of_p[p] = (int32_t)id;
         ep[p] = h_eps[id];
+        {
+            int64_t rr = p;
+            int32_t pk = 0;
+            for (int d = 0; d < c->dim; ++d) {
+                pk |= (int32_t)(rr % e->ns[d]) << (10 * d);
+                rr /= e->ns[d];
+            }
+            coords[p] = pk;
+        }
         // A(q) for the lattice vector q = p (minimum image)
         double q2 = 0.0;
         r = p;
@@ -302,6 +388,7 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
     TG_TRY(dev_alloc(c, &e->amp, (size_t)P));
     TG_TRY(dev_alloc(c, &e->node_of_p, (size_t)P));
     TG_TRY(dev_alloc(c, &e->p_of_node, (size_t)c->n));
+    TG_TRY(dev_alloc(c, &e->coords, (size_t)P));
     TG_TRY(dev_alloc(c, &e->fp, (size_t)P * e->cap));
     TG_TRY(dev_alloc(c, &e->fexp, (size_t)e->cap));
     TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap));
@@ -310,6 +397,7 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
     TG_HIP(c, hipMemcpy(e->amp, amp.data(), sizeof(double) * P, hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->node_of_p, node_of_p.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->p_of_node, p_of_node.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(e->coords, coords.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
     return TG_OK;
 }
 
@@ -321,6 +409,7 @@ void ee_destroy(tg_ctx* c) {
     dev_free(&e->amp);
     dev_free(&e->node_of_p);
     dev_free(&e->p_of_node);
+    dev_free(&e->coords);
     dev_free(&e->fp);
     dev_free(&e->fexp);
     dev_free(&e->part);
@@ -339,14 +428,15 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B) 
                        e->fp, e->fexp);
     TG_CHECK_LAUNCH(c);
     const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
-    const size_t lds = sizeof(double) * 2 * (size_t)e->M;
+    const size_t lds = sizeof(double) * 3 * (size_t)e->M;
 #define TG_EE_MAIN(ACC)                                                                              \
-    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, e->P, e->Mh, \
-                       e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->amp, e->fp, e->fexp, e->ntile,    \
+    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
+                       e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
+                       e->ntile,                                                                     \
                        e->part)
-    if (e->P <= 4 * kEeT) TG_EE_MAIN(4);
-    else if (e->P <= 8 * kEeT) TG_EE_MAIN(8);
-    else TG_EE_MAIN(16);
+    if (e->P <= 8 * kEeT) TG_EE_MAIN(8);
+    else if (e->P <= 16 * kEeT) TG_EE_MAIN(16);
+    else TG_EE_MAIN(32);
 #undef TG_EE_MAIN
     TG_CHECK_LAUNCH(c);
     hipLaunchKernelGGL(ee_reduce_kernel, dim3(ceil_div(c->n, 256), B), dim3(256), 0, st, e->part,

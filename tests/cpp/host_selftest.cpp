@@ -50,6 +50,10 @@ int main() {
     mat.addBand("pn:o", 0, boson, m, 0.05);
     auto ch = mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 0);
     CHECK(ch.size() == 1 && ch[0]->electronBand == "el:C" && ch[0]->phononBand == "pn:o");
+    // the 4-leg e-e channel on the same fermion band (the phonon band never matches)
+    auto ee = mat.addScatteringChannelsWildcards<4>({"el:*", "el:*", "el:*", "*"}, {in, in, out, out}, 0);
+    CHECK(ee.size() == 1 && ee[0]->legs == 4 && ee[0]->electronBand == "el:C");
+    CHECK(mat.channels.size() == 2);
     MaterialStatus<2> s(mat);
     s.setToEquilibrium(0.02, 0.5, 0.0);
     CHECK(std::fabs(s.bathOccupation() - 1.0 / std::expm1(0.05 / 0.02)) < 1e-15);

@@ -43,3 +43,30 @@ def test_host_example_matches_abi():
     a, b = dense[-1][conn], eps[conn]
     en = vol / 12 * ((a * b).sum() + (a.sum(1) * b.sum(1)).sum())
     assert abs(en - en_end) <= 1e-11 * abs(en)
+
+
+def test_host_example_ee_channel_matches_abi():
+    """The 4-leg e-e channel added through addScatteringChannelsWildcards<4>
+    reaches the backend: same step count and final number/energy as the C ABI
+    with the same e-e parameters, and the e-e channel changes the run."""
+    assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
+    print("[gpu] host example 16 sections t=2 with e-e g=1", flush=True)
+    r = subprocess.run([EXE, "16", "2", "1", "1.0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    steps = int(lines[0].split()[1])
+    fin = lines[-1].split("|")
+    num_end = float(fin[1])
+    w = wl.Workload("square16ee", 2, (16, 16), t_final=2.0, rtol=1e-8, atol=1e-12,
+                    ee_g=1.0, ee_kappa2=0.09, ee_sigma=0.05)
+    ctx = tg.Context.create(w, max_batch=1)
+    ctx.assemble()
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    assert st["attempted"] == steps
+    V = ctx.node_volumes()
+    assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
+    # without e-e the same run differs
+    cp = tg.Context.create(w, max_batch=1, ee_g=0.0)
+    cp.assemble()
+    fp, _, _ = cp.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    assert np.linalg.norm(fp - f) > 1e-6 * np.linalg.norm(f)

@@ -2,10 +2,11 @@
 // Tortoise:: host API the way the reference's tutorials are (compare
 // tutorials/TORTOISE_Workshop/Thermalisation/main.cpp): one parabolic
 // electron band, a dispersionless optical phonon band held as a bath, the
-// e-ph channel, DP5 time propagation with dense output, and the number /
-// energy sums of every dense step.  Runs the collision operator on the GPU.
+// e-ph channel, optionally the e-e channel on the electron band, DP5 time
+// propagation with dense output, and the number / energy sums of every dense
+// step.  Runs the collision operator on the GPU.
 //
-// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1]
+// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0]
 #include <TORTOISE>
 
 #include <cmath>
@@ -18,6 +19,7 @@ int main(int argc, char** argv) {
     const int res = argc > 1 ? std::atoi(argv[1]) : 32;
     const Real tFinal = argc > 2 ? std::atof(argv[2]) : 100.0;
     const bool quiet = argc > 3 ? std::atoi(argv[3]) != 0 : true;
+    const Real gEE = argc > 4 ? std::atof(argv[4]) : 0.0;
     const Real pi = M_PI;
 
     Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
@@ -29,6 +31,13 @@ int main(int argc, char** argv) {
         sc->amplitude = 0.1 * 0.1;      // g^2
         sc->deltaBroadening = 0.01;
     }
+    if (gEE != 0.0)
+        for (auto sc : mat.addScatteringChannelsWildcards<4>({"el:*", "el:*", "el:*", "el:*"},
+                                                             {in, in, out, out}, 0)) {
+            sc->amplitude = gEE * gEE;  // g_ee^2 in front of V(q)^2
+            sc->screening = 0.09;       // kappa^2
+            sc->deltaBroadening = 0.05;
+        }
     mat["el:C"].relativeErrorTolerance = 1e-8;
     mat["el:C"].absoluteErrorTolerance = 1e-12;
 

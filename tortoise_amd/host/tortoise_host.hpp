@@ -25,6 +25,7 @@
 // library's message.  Plotting is not provided (no gnuplot pipes).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -281,14 +282,21 @@ public:
 };
 
 // ------------------------------------------------------ ScatteringChannel
-// The e-ph channel {electron in, phonon in, electron out} with the phonon
-// band acting as a bath of a dispersionless mode: the W-form the backend
-// evaluates.  `amplitude` is g^2 (ScatteringChannel.hpp:70); the broadening
-// and the energy grid of the overlap quadrature are new parameters.
+// Two channels the backend evaluates:
+//   legs == 3: e-ph {electron in, phonon in, electron out} with the phonon
+//     band acting as a bath of a dispersionless mode (the W-form);
+//   legs == 4: e-e {el in, el in, el out, el out} with the screened Coulomb
+//     amplitude V(q)^2, V(q) = 1/(|q|^2 + screening), on the periodic lattice
+//     of the zone (the functAmplitude of a 4-leg channel,
+//     ScatteringChannel.cpp:399-415).
+// `amplitude` is g^2 (ScatteringChannel.hpp:70); the broadening and the
+// energy grid of the delta are new parameters.
 template <int NDim>
 class ScatteringChannel {
 public:
+    int legs = 3;
     std::string electronBand, phononBand;
+    Real screening = 0.09;          // e-e: kappa^2 of V(q)
     Real amplitude = 1.0;
     Real deltaBroadening = 0.01;
     Real energyGridStep = 0.0;      // <= 0: deltaBroadening / 8
@@ -335,34 +343,69 @@ public:
         for (auto& b : bands) if (match(pat, b.name)) r.push_back(b.name);
         return r;
     }
-    // Material.cpp:244-301 for {el:*, pn:*, el:*} with {in, in, out}
+    // Material.cpp:244-301 for {el:*, pn:*, el:*} with {in, in, out} (e-ph)
+    // and {el:*, el:*, el:*, el:*} with {in, in, out, out} (e-e, all four legs
+    // on one fermion band)
     template <int NLegs>
     std::vector<ScatteringChannel<NDim>*> addScatteringChannelsWildcards(
         const std::array<std::string, NLegs>& legs, const std::array<int, NLegs>& dirs, int nUmklapp) {
-        static_assert(NLegs == 3, "the backend evaluates 3-leg e-ph channels");
-        if (!(dirs[0] == in && dirs[1] == in && dirs[2] == out))
-            throw std::invalid_argument("e-ph channel must be {in, in, out}");
+        static_assert(NLegs == 3 || NLegs == 4, "the backend evaluates 3-leg e-ph and 4-leg e-e channels");
         std::vector<ScatteringChannel<NDim>*> r;
-        for (auto& e : matchListId(legs[0]))
-            for (auto& p : matchListId(legs[1])) {
-                if ((*this)[e].statistics != fermion || (*this)[p].statistics != boson) continue;
+        if constexpr (NLegs == 3) {
+            if (!(dirs[0] == in && dirs[1] == in && dirs[2] == out))
+                throw std::invalid_argument("e-ph channel must be {in, in, out}");
+            for (auto& e : matchListId(legs[0]))
+                for (auto& p : matchListId(legs[1])) {
+                    if ((*this)[e].statistics != fermion || (*this)[p].statistics != boson) continue;
+                    channels.push_back(ScatteringChannel<NDim>{});
+                    channels.back().electronBand = e;
+                    channels.back().phononBand = p;
+                    r.push_back(&channels.back());
+                }
+        } else {
+            if (!(dirs[0] == in && dirs[1] == in && dirs[2] == out && dirs[3] == out))
+                throw std::invalid_argument("e-e channel must be {in, in, out, out}");
+            for (auto& e : matchListId(legs[0])) {
+                if ((*this)[e].statistics != fermion) continue;
+                bool all = true;
+                for (int l = 1; l < 4; ++l) {
+                    const auto ids = matchListId(legs[l]);
+                    all = all && std::find(ids.begin(), ids.end(), e) != ids.end();
+                }
+                if (!all) continue;
                 channels.push_back(ScatteringChannel<NDim>{});
+                channels.back().legs = 4;
                 channels.back().electronBand = e;
-                channels.back().phononBand = p;
+                channels.back().deltaBroadening = 0.02;
                 r.push_back(&channels.back());
             }
-        (void)nUmklapp;  // the W-form needs no umklapp enumeration (phonon takes any q)
+        }
+        (void)nUmklapp;  // lattice momentum wraps periodically; no umklapp enumeration
         invalidate();
         return r;
     }
 
     // ---- backend
-    // Context of the (single) e-ph channel; built on first use with the
-    // channel parameters current at that time.
+    // Context of the e-ph channel (exactly one) and the e-e channel (at most
+    // one, on the same electron band); built on first use with the channel
+    // parameters current at that time.
     tg_ctx* backend(Real bath_n, int max_batch = 1) {
         if (ctx_ && ctx_bath_ == bath_n) return ctx_;
-        if (channels.size() != 1) throw std::runtime_error("the backend evaluates exactly one e-ph channel");
-        const auto& ch = channels.front();
+        const ScatteringChannel<NDim>* eph = nullptr;
+        const ScatteringChannel<NDim>* ee = nullptr;
+        for (const auto& c : channels) {
+            if (c.legs == 3) {
+                if (eph) throw std::runtime_error("the backend evaluates one e-ph channel");
+                eph = &c;
+            } else {
+                if (ee) throw std::runtime_error("the backend evaluates at most one e-e channel");
+                ee = &c;
+            }
+        }
+        if (!eph) throw std::runtime_error("the backend needs an e-ph channel");
+        if (ee && ee->electronBand != eph->electronBand)
+            throw std::runtime_error("the e-e channel must act on the e-ph channel's electron band");
+        const auto& ch = *eph;
         const Band<NDim>& el = (*this)[ch.electronBand];
         const Band<NDim>& pn = (*this)[ch.phononBand];
         const auto omega = pn.dispersion.toNodal();
@@ -385,6 +428,12 @@ public:
         d.de = ch.energyGridStep;
         d.cutoff = ch.deltaCutoff;
         d.max_batch = max_batch;
+        if (ee) {
+            d.ee_g = std::sqrt(ee->amplitude);
+            d.ee_kappa2 = ee->screening;
+            d.ee_sigma = ee->deltaBroadening;
+            d.ee_dw = ee->energyGridStep;
+        }
         if (ctx_) tg_ctx_destroy(ctx_);
         ctx_ = nullptr;
         check(tg_ctx_create(&d, &ctx_));

@@ -9,6 +9,8 @@ Reference files restated (paths relative to the TORTOISE tree):
   error norm           src/Physics/Material/Band.cpp:87-101, MaterialStatus.cpp:189-196
   DP5(4) driver        include/Generics/Algorithms/DormandPrince54.hpp:479-665
   amplitude convention src/Physics/Scattering/ScatteringChannel.cpp:399-415
+  tuple operator       include/Physics/Scattering/IntegratorCore.hpp:260-357,
+                       src/Physics/Scattering/ScatteringChannel.cpp:285-538,618-688
 """
 from __future__ import annotations
 
@@ -103,6 +105,23 @@ def lib():
                                      C.c_double, _f64p, C.c_double, C.c_double, C.c_double, _f64p,
                                      C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.or_mc_rows.restype = C.c_int32
+        L.or_mc_rows.argtypes = [C.c_int, C.c_int]
+        L.or_mc_points.argtypes = [C.c_int, C.c_int, C.c_int32, C.c_uint64, _f64p]
+        L.or_mc_invert.restype = C.c_double
+        L.or_mc_invert.argtypes = [C.c_int, _f64p, _f64p]
+        L.or_mc_integrate.restype = C.c_int
+        L.or_mc_integrate.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                      C.c_int32, _f64p]
+        L.or_mc_tuples_eph.restype = C.c_int64
+        L.or_mc_tuples_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_int32,
+                                       _f64p, C.c_int32, C.c_double, C.c_double, C.c_void_p,
+                                       C.c_int64]
+        L.or_mc_inverse_mass.argtypes = [C.c_int, C.c_double, _f64p]
+        L.or_mc_apply_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                      C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
+                                      _f64p, C.c_void_p, C.c_int32]
+        L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
         L.or_ee_grid.argtypes = [C.c_int, _i32p, _f64p, C.c_double, C.POINTER(C.c_int32)]
         L.or_ee_apply.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, _f64p]
@@ -373,3 +392,84 @@ def ee_grid(dim, nsec, eps, dw):
     lib().or_ee_grid(dim, np.ascontiguousarray(nsec, np.int32),
                      np.ascontiguousarray(eps, np.float64), dw, C.byref(mh))
     return mh.value
+
+
+# ------------------------------------------------ reference tuple operator
+def _pad3(x, fill):
+    return np.ascontiguousarray(list(x) + [fill] * (3 - len(x)), np.float64)
+
+
+def mc_points(dim, nlegs, nmc, seed):
+    """MC point matrix [rows][nmc] of a channel (mc_oracle.c)."""
+    rows = lib().or_mc_rows(dim, nlegs)
+    pts = np.zeros((rows, nmc))
+    lib().or_mc_points(dim, nlegs, nmc, seed, pts)
+    return pts
+
+
+def mc_invert(A):
+    A = np.ascontiguousarray(A, np.float64)
+    inv = np.zeros_like(A)
+    det = lib().or_mc_invert(A.shape[0], A, inv)
+    return det, inv
+
+
+def mc_integrate(dim, nlegs, D, N, fa, fb, pts):
+    """scatteringIntegrationTypeB with the standard output forms."""
+    out = np.zeros(nlegs * (dim + 1))
+    pts = np.ascontiguousarray(pts, np.float64)
+    rc = lib().or_mc_integrate(dim, nlegs, np.ascontiguousarray(D, np.float64),
+                               np.ascontiguousarray(N, np.float64),
+                               np.ascontiguousarray(fa, np.float64),
+                               np.ascontiguousarray(fb, np.float64), pts, pts.shape[1], out)
+    if rc != 0:
+        raise ValueError("singular D_C")
+    return out
+
+
+def mc_tuples_eph(dim, nsec, origin, extent, eps, omega0, nu, pts, min_det=1e-9,
+                  threshold=1e-9):
+    """Tuple list int32 [T][4] = (e0, e1, e2, umklapp id) of the e-ph channel."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    eps = np.ascontiguousarray(eps, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    L = lib()
+    n = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, omega0, nu, pts, pts.shape[1], min_det,
+                           threshold, None, 0)
+    idx = np.zeros((max(n, 1), 4), np.int32)
+    m = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, omega0, nu, pts, pts.shape[1], min_det,
+                           threshold, idx.ctypes.data, n)
+    assert m == n
+    return idx[:n]
+
+
+def mc_inverse_mass(dim, vol):
+    m = np.zeros((dim + 1, dim + 1))
+    lib().or_mc_inverse_mass(dim, vol, m)
+    return m
+
+
+def mc_apply_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
+                 nthreads=1, phonon_out=False):
+    """Electron (and optionally phonon) DG derivative of the e-ph channel."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    pts = np.ascontiguousarray(pts, np.float64)
+    f_el = np.ascontiguousarray(f_el, np.float64)
+    f_ph = np.ascontiguousarray(f_ph, np.float64)
+    out_el = np.zeros_like(f_el)
+    out_ph = np.zeros_like(f_ph) if phonon_out else None
+    idx = np.ascontiguousarray(idx, np.int32)
+    lib().or_mc_apply_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
+                          amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out_el,
+                          None if out_ph is None else out_ph.ctypes.data, nthreads)
+    return (out_el, out_ph) if phonon_out else out_el
+
+
+def mc_nodal_to_dg(dim, nsec, f):
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    nel = int(np.prod(nsec)) * (2 if dim == 2 else 6 if dim == 3 else 1)
+    dg = np.zeros((nel, dim + 1))
+    lib().or_mc_nodal_to_dg(dim, nsec, np.ascontiguousarray(f, np.float64), dg)
+    return dg

@@ -1,0 +1,538 @@
+/*
+ * mc_oracle.c -- CPU FP64 restatement of the reference's own scattering
+ * operator: element tuples, the Monte-Carlo delta-shell integrator, the
+ * per-tuple propagator and the inverse mass.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tortoise_oracle.h): the checker of the HIP
+ * tuple operator, and the "reference algorithm" CPU timing.
+ *
+ * Reference files restated (paths relative to the TORTOISE tree):
+ *   include/Physics/Scattering/IntegratorCore.hpp:153-155   det of D_C
+ *   include/Physics/Scattering/IntegratorCore.hpp:260-357   scatteringIntegrationTypeB
+ *   src/Physics/Scattering/ScatteringChannel.cpp:285-352    D and N of a tuple
+ *   src/Physics/Scattering/ScatteringChannel.cpp:355-381    output forms
+ *   src/Physics/Scattering/ScatteringChannel.cpp:382-398    amplitude factor
+ *   src/Physics/Scattering/ScatteringChannel.cpp:421-538    tuple screening
+ *   src/Physics/Scattering/ScatteringChannel.cpp:618-670    propagator + sum
+ *   src/Physics/Scattering/ScatteringChannel.cpp:677-688    MC point rows
+ *   include/Geometry/GeometryCore/Point.hpp:44-68           simplex sampling
+ *   include/Physics/Scattering/ScatteringChannel.hpp:205-255 energy window
+ *   src/Geometry/Structured/DirectSpace/Mesh.cpp:95-123     mass matrix
+ *   Function.cpp:801-804                                     applyInverseMass
+ *
+ * Spec decisions (the reference leaves them to Eigen / std::rand / TBB):
+ *   - random numbers: splitmix64 (Steele, Lea, Flood 2014), seeded by the
+ *     caller; the reference draws from Eigen's Random (std::rand);
+ *   - D_C^-1: Gauss-Jordan elimination with partial pivoting in a fixed
+ *     operation order (the HIP path uses the same steps, so the inside-simplex
+ *     decisions of the two agree point by point);
+ *   - tuple screening uses the operator's own MC points (the reference draws
+ *     fresh ones for the screening integral);
+ *   - mass matrix: exact rational moments of the reference simplex (the
+ *     reference sums a 14-digit order-2 Gauss rule: 1e-14 relative in 2D);
+ *   - candidate pruning: element energy windows and momentum bounding boxes
+ *     (the reference prunes by section boxes first); both keep every tuple
+ *     whose delta shell meets the simplices.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "tortoise_oracle.h"
+
+static int mc_fact(int d) { int r = 1; for (int i = 2; i <= d; ++i) r *= i; return r; }
+
+/* ================================================================ points */
+static uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double unit_draw(uint64_t* s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+/* Uniform point in the reference simplex of dimension d (1..3) from the unit
+ * cube by folding (Point.hpp:44-68; 3D: Rocchini & Cignoni 2000). */
+static void simplex_point(int d, uint64_t* s, double* x) {
+    for (int i = 0; i < d; ++i) x[i] = unit_draw(s);
+    if (d == 2) {
+        if (x[0] + x[1] > 1.0) { x[0] = 1.0 - x[0]; x[1] = 1.0 - x[1]; }
+    } else if (d == 3) {
+        if (x[0] + x[1] > 1.0) { x[0] = 1.0 - x[0]; x[1] = 1.0 - x[1]; }
+        if (x[1] + x[2] > 1.0) {
+            double t = x[2];
+            x[2] = 1.0 - x[0] - x[1];
+            x[1] = 1.0 - t;
+        } else if (x[0] + x[1] + x[2] > 1.0) {
+            double t = x[2];
+            x[2] = x[0] + x[1] + x[2] - 1.0;
+            x[0] = 1.0 - x[1] - t;
+        }
+    }
+}
+
+int32_t or_mc_rows(int dim, int nlegs) {
+    int a = dim * (nlegs - 1) - 1, b = 2 * dim;
+    return a > b ? a : b;
+}
+
+/* MC point matrix of a channel (ScatteringChannel.cpp:677-688), row-major
+ * [rows][nmc]: legs 0..nlegs-3 fill rows leg*dim.., the partially free leg
+ * fills the bottom dim-1 rows; rows no leg uses are zero. */
+void or_mc_points(int dim, int nlegs, int32_t nmc, uint64_t seed, double* pts) {
+    int rows = or_mc_rows(dim, nlegs);
+    memset(pts, 0, sizeof(double) * (size_t)rows * nmc);
+    uint64_t s = seed;
+    double x[3];
+    for (int leg = 0; leg < nlegs - 2; ++leg)
+        for (int32_t m = 0; m < nmc; ++m) {
+            simplex_point(dim, &s, x);
+            for (int d = 0; d < dim; ++d) pts[(size_t)(leg * dim + d) * nmc + m] = x[d];
+        }
+    if (dim > 1)
+        for (int32_t m = 0; m < nmc; ++m) {
+            simplex_point(dim - 1, &s, x);
+            for (int d = 0; d < dim - 1; ++d) pts[(size_t)(rows - (dim - 1) + d) * nmc + m] = x[d];
+        }
+}
+
+/* ============================================================ small LA */
+/* Inverse of the n x n matrix A (row-major, n <= 4) by Gauss-Jordan with
+ * partial pivoting; returns the determinant (0 when singular). */
+double or_mc_invert(int n, const double* A, double* inv) {
+    double a[4][8];
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < 2 * n; ++c) a[r][c] = c < n ? A[r * n + c] : (c - n == r ? 1.0 : 0.0);
+    double det = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+        if (a[p][c] == 0.0) return 0.0;
+        if (p != c) {
+            for (int q = 0; q < 2 * n; ++q) { double t = a[p][q]; a[p][q] = a[c][q]; a[c][q] = t; }
+            det = -det;
+        }
+        double piv = a[c][c];
+        det *= piv;
+        for (int q = 0; q < 2 * n; ++q) a[c][q] /= piv;
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double m = a[r][c];
+            for (int q = 0; q < 2 * n; ++q) a[r][q] -= m * a[c][q];
+        }
+    }
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) inv[r * n + c] = a[r][n + c];
+    return det;
+}
+
+/* ====================================================== integrator core */
+static double lf_apply(int dim, const double* form, const double* x) {
+    double v = form[0];
+    for (int d = 0; d < dim; ++d) v += form[1 + d] * x[d];
+    return v;
+}
+
+/* scatteringIntegrationTypeB (IntegratorCore.hpp:260-357) with the output
+ * forms of constructOutputformsLinForm (ScatteringChannel.cpp:355-381): the
+ * output of leg l, basis j, is the integrand times phi_j at leg l's point,
+ * phi_0 = 1, phi_j = x_{j-1}.
+ *   D  (dim+1) x (nlegs*dim) row-major,  N  dim+1
+ *   fa, fb: nlegs blocks of dim+1 linear-form coefficients (the "direct" and
+ *   "time-reversed" products)
+ *   pts: or_mc_points layout; out: nlegs*(dim+1).
+ * Returns 0, or -1 if D_C is singular. */
+int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const double* fa,
+                    const double* fb, const double* pts, int32_t nmc, double* out) {
+    const int nc = dim + 1, ncol = nlegs * dim, rows = or_mc_rows(dim, nlegs);
+    const int c0 = (nlegs - 1) * dim - 1;           /* first column of D_C */
+    double DC[16], inv[16];
+    for (int r = 0; r < nc; ++r)
+        for (int c = 0; c < nc; ++c) DC[r * nc + c] = D[r * ncol + c0 + c];
+    double det = or_mc_invert(nc, DC, inv);
+    if (det == 0.0) return -1;
+    for (int i = 0; i < nlegs * nc; ++i) out[i] = 0.0;
+    const int single = (nlegs == 2 && dim == 1);
+    const int32_t npt = single ? 1 : nmc;
+    for (int32_t m = 0; m < npt; ++m) {
+        double K[4][3];                  /* point of every leg, reference coords */
+        for (int leg = 0; leg < nlegs - 2; ++leg)
+            for (int d = 0; d < dim; ++d) K[leg][d] = pts[(size_t)(leg * dim + d) * nmc + m];
+        for (int d = 0; d < dim - 1; ++d) K[nlegs - 2][d] = pts[(size_t)(rows - (dim - 1) + d) * nmc + m];
+        /* rhs = D_F K_F - N over the free columns 0..c0-1 */
+        double rhs[4];
+        for (int r = 0; r < nc; ++r) {
+            double s = 0.0;
+            for (int c = 0; c < c0; ++c) s += D[r * ncol + c] * K[c / dim][c % dim];
+            rhs[r] = s - N[r];
+        }
+        double alpha[4];
+        for (int r = 0; r < nc; ++r) {
+            double s = 0.0;
+            for (int c = 0; c < nc; ++c) s += inv[r * nc + c] * rhs[c];
+            alpha[r] = -s;
+        }
+        K[nlegs - 2][dim - 1] = alpha[0];
+        for (int d = 0; d < dim; ++d) K[nlegs - 1][d] = alpha[1 + d];
+        /* inside-simplex tests of the two constrained legs */
+        int inside = 1;
+        for (int leg = nlegs - 2; leg < nlegs; ++leg) {
+            double sum = 0.0;
+            for (int d = 0; d < dim; ++d) { inside &= K[leg][d] > 0.0; sum += K[leg][d]; }
+            inside &= sum < 1.0;
+        }
+        if (!inside) continue;
+        double pa = 1.0, pb = 1.0;
+        for (int leg = 0; leg < nlegs; ++leg) {
+            pa *= lf_apply(dim, fa + leg * nc, K[leg]);
+            pb *= lf_apply(dim, fb + leg * nc, K[leg]);
+        }
+        double val = pa + pb;
+        for (int leg = 0; leg < nlegs; ++leg) {
+            out[leg * nc] += val;
+            for (int d = 0; d < dim; ++d) out[leg * nc + 1 + d] += val * K[leg][d];
+        }
+    }
+    double w = fabs(det);
+    if (!single) {
+        w *= (double)nmc * (double)mc_fact(dim - 1);
+        for (int k = 0; k < nlegs - 2; ++k) w *= (double)mc_fact(dim);
+    }
+    for (int i = 0; i < nlegs * nc; ++i) out[i] /= w;
+    return 0;
+}
+
+/* ==================================================== e-ph channel geometry */
+/* The channel {el in, pn in, el out} with every band on one structured mesh:
+ * leg 0 and leg 2 on the electron band (P1 dispersion from nodal eps), leg 1
+ * on a dispersionless phonon band (energy omega0). */
+typedef struct mc_geo {
+    int dim;
+    const int32_t* nsec;
+    const double* origin;
+    const double* extent;
+    const double* eps;
+    double omega0;
+    int64_t nel;
+    int32_t* conn;
+    double* nodes;
+    double vol;
+} mc_geo;
+
+static void geo_init(mc_geo* g, int dim, const int32_t* nsec, const double* origin,
+                     const double* extent, const double* eps, double omega0) {
+    g->dim = dim; g->nsec = nsec; g->origin = origin; g->extent = extent; g->eps = eps;
+    g->omega0 = omega0;
+    g->nel = or_num_elements(dim, nsec);
+    int64_t n = or_num_nodes(dim, nsec);
+    g->conn = (int32_t*)malloc(sizeof(int32_t) * g->nel * (dim + 1));
+    g->nodes = (double*)malloc(sizeof(double) * n * dim);
+    or_mesh_elements(dim, nsec, g->conn);
+    or_mesh_nodes(dim, nsec, origin, extent, g->nodes);
+    g->vol = or_element_volume(dim, nsec, extent);
+}
+static void geo_free(mc_geo* g) { free(g->conn); free(g->nodes); }
+
+/* k-space linear form of coordinate d and of the energy on element e:
+ * kf[d][0..dim] = [k_d(v0), k_d(v1)-k_d(v0), ...], ef likewise. */
+static void elem_forms(const mc_geo* g, int64_t e, int phonon, double kf[3][4], double ef[4]) {
+    const int dim = g->dim;
+    const int32_t* v = g->conn + e * (dim + 1);
+    for (int d = 0; d < dim; ++d) {
+        double k0 = g->nodes[(int64_t)v[0] * dim + d];
+        kf[d][0] = k0;
+        for (int a = 1; a <= dim; ++a) kf[d][a] = g->nodes[(int64_t)v[a] * dim + d] - k0;
+    }
+    if (phonon) {
+        ef[0] = g->omega0;
+        for (int a = 1; a <= dim; ++a) ef[a] = 0.0;
+    } else {
+        double e0 = g->eps[v[0]];
+        ef[0] = e0;
+        for (int a = 1; a <= dim; ++a) ef[a] = g->eps[v[a]] - e0;
+    }
+}
+
+/* umklapp vector u of id `uid` in the reference order (first axis outermost) */
+static void umklapp_vec(int dim, int32_t nu, int32_t uid, int* u) {
+    int span = 2 * nu + 1;
+    for (int d = dim - 1; d >= 0; --d) { u[d] = uid % span - nu; uid /= span; }
+}
+
+/* D and N of a tuple (ScatteringChannel.cpp:285-352): legs {e0 in, e1 in, e2 out}. */
+static void tuple_DN(const mc_geo* g, const int64_t* el, const int* u, double* D, double* N) {
+    const int dim = g->dim, ncol = 3 * dim;
+    static const double dir[3] = {1.0, 1.0, -1.0};
+    for (int r = 0; r <= dim; ++r) N[r] = 0.0;
+    for (int leg = 0; leg < 3; ++leg) {
+        double kf[3][4], ef[4];
+        elem_forms(g, el[leg], leg == 1, kf, ef);
+        for (int d = 0; d < dim; ++d) {
+            for (int a = 0; a < dim; ++a) D[d * ncol + leg * dim + a] = dir[leg] * kf[d][1 + a];
+            N[d] -= dir[leg] * kf[d][0];
+        }
+        for (int a = 0; a < dim; ++a) D[dim * ncol + leg * dim + a] = dir[leg] * ef[1 + a];
+        N[dim] -= dir[leg] * ef[0];
+    }
+    for (int d = 0; d < dim; ++d) N[d] -= (double)u[d] * g->extent[d];
+}
+
+static void elem_box(const mc_geo* g, int64_t e, double* lo, double* hi) {
+    const int dim = g->dim;
+    const int32_t* v = g->conn + e * (dim + 1);
+    for (int d = 0; d < dim; ++d) { lo[d] = 1e300; hi[d] = -1e300; }
+    for (int a = 0; a <= dim; ++a)
+        for (int d = 0; d < dim; ++d) {
+            double x = g->nodes[(int64_t)v[a] * dim + d];
+            if (x < lo[d]) lo[d] = x;
+            if (x > hi[d]) hi[d] = x;
+        }
+}
+
+static void elem_erange(const mc_geo* g, int64_t e, double* lo, double* hi) {
+    const int32_t* v = g->conn + e * (g->dim + 1);
+    *lo = 1e300; *hi = -1e300;
+    for (int a = 0; a <= g->dim; ++a) {
+        double x = g->eps[v[a]];
+        if (x < *lo) *lo = x;
+        if (x > *hi) *hi = x;
+    }
+}
+
+/* Screening value of a tuple: scatteringIntegrationTypeC with the modal
+ * order-0 form on every leg (ScatteringChannel.cpp:465-489), amplitude 1 and
+ * functAmplitude 1 (screening runs at channel creation, before either is set).
+ * Returns -1 when |det D_C| < min_det. */
+static double tuple_screen(const mc_geo* g, const double* D, const double* N, const double* pts,
+                           int32_t nmc, double min_det) {
+    const int dim = g->dim, nc = dim + 1;
+    double fa[12], fb[12], out[12];
+    double c = dim == 1 ? 1.0 : sqrt(2.0);        /* modal order 0 in the LF basis */
+    for (int i = 0; i < 3 * nc; ++i) { fa[i] = 0.0; fb[i] = 0.0; }
+    for (int leg = 0; leg < 3; ++leg) fa[leg * nc] = c;
+    double DC[16], inv[16];
+    for (int r = 0; r < nc; ++r)
+        for (int q = 0; q < nc; ++q) DC[r * nc + q] = D[r * 3 * dim + 2 * dim - 1 + q];
+    double det = or_mc_invert(nc, DC, inv);
+    if (fabs(det) < min_det) return -1.0;
+    or_mc_integrate(dim, 3, D, N, fa, fb, pts, nmc, out);
+    double amp = 1.0;
+    for (int leg = 0; leg < 3; ++leg) amp *= g->vol * (double)mc_fact(dim);
+    return fabs(out[0] * amp);
+}
+
+/* Tuples of the e-ph channel, ordered by (e0, e2, umklapp id, e1): every
+ * (e0, e2) pair whose energy window contains omega0-conservation
+ * (ScatteringChannel.hpp:212-222), every umklapp vector in {-nu..nu}^dim and
+ * every phonon element whose bounding box meets k2 - k0 - G u, kept when
+ * |det D_C| >= min_det and the screening value exceeds `threshold`
+ * (ScatteringChannel.cpp:474-503).  idx[t*4 + {0,1,2,3}] = e0, e1, e2, uid.
+ * Returns the tuple count (idx may be NULL to count; at most cap written). */
+int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                         const double* eps, double omega0, int32_t nu, const double* pts,
+                         int32_t nmc, double min_det, double threshold, int32_t* idx, int64_t cap) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    const int64_t nel = g.nel;
+    const int nes = mc_fact(dim);
+    int32_t nuv = 1;
+    for (int d = 0; d < dim; ++d) nuv *= 2 * nu + 1;
+    double* box = (double*)malloc(sizeof(double) * nel * 2 * dim);
+    double* er = (double*)malloc(sizeof(double) * nel * 2);
+    for (int64_t e = 0; e < nel; ++e) {
+        elem_box(&g, e, box + e * 2 * dim, box + e * 2 * dim + dim);
+        elem_erange(&g, e, er + 2 * e, er + 2 * e + 1);
+    }
+    int64_t* counts = (int64_t*)calloc((size_t)nel + 1, sizeof(int64_t));
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            if (!idx) break;
+            for (int64_t e = 0; e < nel; ++e) counts[e + 1] += counts[e];
+        }
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t e0 = 0; e0 < nel; ++e0) {
+            int64_t w = pass == 1 ? counts[e0] : 0, cnt = 0;
+            const double* b0 = box + e0 * 2 * dim;
+            for (int64_t e2 = 0; e2 < nel; ++e2) {
+                if (!(er[2 * e0] + omega0 - er[2 * e2 + 1] <= 0.0 &&
+                      er[2 * e0 + 1] + omega0 - er[2 * e2] >= 0.0))
+                    continue;
+                const double* b2 = box + e2 * 2 * dim;
+                for (int32_t uid = 0; uid < nuv; ++uid) {
+                    int u[3] = {0, 0, 0};
+                    umklapp_vec(dim, nu, uid, u);
+                    /* k1 = k2 - k0 - G u */
+                    double lo[3], hi[3];
+                    int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, empty = 0;
+                    for (int d = 0; d < dim; ++d) {
+                        lo[d] = b2[d] - b0[dim + d] - u[d] * extent[d];
+                        hi[d] = b2[dim + d] - b0[d] - u[d] * extent[d];
+                        double h = extent[d] / nsec[d];
+                        double a = floor((lo[d] - origin[d]) / h), b = floor((hi[d] - origin[d]) / h);
+                        if (b < 0.0 || a > nsec[d] - 1) { empty = 1; break; }
+                        slo[d] = a < 0.0 ? 0 : (int)a;
+                        shi[d] = b > nsec[d] - 1 ? nsec[d] - 1 : (int)b;
+                    }
+                    if (empty) continue;
+                    int sc[3] = {slo[0], dim > 1 ? slo[1] : 0, dim > 2 ? slo[2] : 0};
+                    for (;;) {
+                        int64_t sid = 0, stride = 1;
+                        for (int d = 0; d < dim; ++d) { sid += stride * sc[d]; stride *= nsec[d]; }
+                        for (int t = 0; t < nes; ++t) {
+                            int64_t e1 = sid * nes + t;
+                            const double* b1 = box + e1 * 2 * dim;
+                            int meet = 1;
+                            for (int d = 0; d < dim; ++d)
+                                meet &= b1[d] <= hi[d] && b1[dim + d] >= lo[d];
+                            if (!meet) continue;
+                            int64_t el[3] = {e0, e1, e2};
+                            double D[36], N[4];
+                            tuple_DN(&g, el, u, D, N);
+                            double s = tuple_screen(&g, D, N, pts, nmc, min_det);
+                            if (!(s > threshold)) continue;
+                            if (pass == 1 && w < cap) {
+                                idx[w * 4 + 0] = (int32_t)e0;
+                                idx[w * 4 + 1] = (int32_t)e1;
+                                idx[w * 4 + 2] = (int32_t)e2;
+                                idx[w * 4 + 3] = uid;
+                            }
+                            ++w;
+                            ++cnt;
+                        }
+                        int d = 0;
+                        while (d < dim && ++sc[d] > shi[d]) { sc[d] = slo[d]; ++d; }
+                        if (d == dim) break;
+                    }
+                }
+            }
+            if (pass == 0) counts[e0 + 1] = cnt;
+        }
+    }
+    int64_t total = 0;
+    if (idx) total = counts[nel];
+    else for (int64_t e = 0; e < nel; ++e) total += counts[e + 1];
+    free(counts); free(box); free(er);
+    geo_free(&g);
+    return total;
+}
+
+/* ================================================================ mass */
+/* Inverse of the LF-basis mass matrix of one element (Mesh.cpp:95-123):
+ * M_ab = vol * d! * int_ref phi_a phi_b, phi = {1, x_1..x_d}; exact moments
+ * int x_i = 1/(d+1)!, int x_i^2 = 2/(d+2)!, int x_i x_j = 1/(d+2)!. */
+void or_mc_inverse_mass(int dim, double vol, double* minv) {
+    const int nc = dim + 1;
+    double M[16];
+    double f1 = 1.0 / (double)mc_fact(dim + 1), f2 = 1.0 / (double)mc_fact(dim + 2);
+    double scale = vol * (double)mc_fact(dim);
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            double m;
+            if (a == 0 && b == 0) m = 1.0 / (double)mc_fact(dim);
+            else if (a == 0 || b == 0) m = f1;
+            else m = a == b ? 2.0 * f2 : f2;
+            M[a * nc + b] = scale * m;
+        }
+    or_mc_invert(nc, M, minv);
+}
+
+/* ============================================================ propagator */
+/* ScatteringChannel::propagate (ScatteringChannel.cpp:618-670) of the e-ph
+ * channel over a tuple list, then applyInverseMass (Function.cpp:801-804).
+ *   f_el, f_ph: DG linear-form coefficients [nel][dim+1] of the electron and
+ *   phonon bands; out_el, out_ph likewise (out_ph may be NULL).
+ *   amplitude: g^2 (functAmplitude = 1).
+ * Per tuple the forms are (fermion electrons, boson phonons):
+ *   direct        -f(k0) * n(k1) * (1 - f(k2))
+ *   time-reversed (1 - f(k0)) * (1 + n(k1)) * f(k2)
+ * and leg l receives dir_l * result_l.  Threads accumulate disjoint tuple
+ * ranges; their sums are added in thread order. */
+void or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                     const double* eps, double omega0, double amplitude, int32_t nu,
+                     const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                     const double* f_el, const double* f_ph, double* out_el, double* out_ph,
+                     int32_t nthreads) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    const int nc = dim + 1;
+    const int64_t nel = g.nel, len = nel * nc;
+    int nt = nthreads > 0 ? nthreads : 1;
+    double* acc = (double*)calloc((size_t)nt * 2 * len, sizeof(double));
+    double fullamp = amplitude;
+    for (int leg = 0; leg < 3; ++leg) fullamp *= g.vol * (double)mc_fact(dim);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int th = 0; th < nt; ++th) {
+        double* ae = acc + (size_t)th * 2 * len;
+        double* ap = ae + len;
+        int64_t t0 = ntup * th / nt, t1 = ntup * (th + 1) / nt;
+        for (int64_t t = t0; t < t1; ++t) {
+            int64_t el[3] = {idx[t * 4], idx[t * 4 + 1], idx[t * 4 + 2]};
+            int u[3] = {0, 0, 0};
+            umklapp_vec(dim, nu, idx[t * 4 + 3], u);
+            double D[36], N[4];
+            tuple_DN(&g, el, u, D, N);
+            double DC[16], inv[16];
+            for (int r = 0; r < nc; ++r)
+                for (int q = 0; q < nc; ++q) DC[r * nc + q] = D[r * 3 * dim + 2 * dim - 1 + q];
+            if (fabs(or_mc_invert(nc, DC, inv)) < 1e-300) continue;
+            double fa[12], fb[12], res[12];
+            const double* c0 = f_el + el[0] * nc;
+            const double* c1 = f_ph + el[1] * nc;
+            const double* c2 = f_el + el[2] * nc;
+            for (int a = 0; a < nc; ++a) {
+                double one = a == 0 ? 1.0 : 0.0;
+                fa[a] = -c0[a];                 /* leg 0 in, sign -1 of the direct product */
+                fa[nc + a] = c1[a];             /* leg 1 in: n */
+                fa[2 * nc + a] = one - c2[a];   /* leg 2 out, fermion: 1 - f */
+                fb[a] = one - c0[a];            /* time reversed: 1 - f */
+                fb[nc + a] = one + c1[a];       /* 1 + n */
+                fb[2 * nc + a] = c2[a];         /* f */
+            }
+            or_mc_integrate(dim, 3, D, N, fa, fb, pts, nmc, res);
+            for (int a = 0; a < nc; ++a) {
+                ae[el[0] * nc + a] += fullamp * res[a];
+                ap[el[1] * nc + a] += fullamp * res[nc + a];
+                ae[el[2] * nc + a] -= fullamp * res[2 * nc + a];
+            }
+        }
+    }
+    double minv[16];
+    or_mc_inverse_mass(dim, g.vol, minv);
+    for (int64_t e = 0; e < nel; ++e) {
+        double se[4] = {0, 0, 0, 0}, sp[4] = {0, 0, 0, 0};
+        for (int th = 0; th < nt; ++th)
+            for (int a = 0; a < nc; ++a) {
+                se[a] += acc[(size_t)th * 2 * len + e * nc + a];
+                sp[a] += acc[(size_t)th * 2 * len + len + e * nc + a];
+            }
+        for (int b = 0; b < nc; ++b) {
+            double ve = 0.0, vp = 0.0;
+            for (int a = 0; a < nc; ++a) { ve += se[a] * minv[a * nc + b]; vp += sp[a] * minv[a * nc + b]; }
+            out_el[e * nc + b] = ve;
+            if (out_ph) out_ph[e * nc + b] = vp;
+        }
+    }
+    free(acc);
+    geo_free(&g);
+}
+
+/* Nodal values -> DG linear-form coefficients per element (the form that
+ * interpolates the nodal values at the element's vertices, LinearForm.hpp:34-40). */
+void or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg) {
+    int64_t nel = or_num_elements(dim, nsec);
+    int32_t* conn = (int32_t*)malloc(sizeof(int32_t) * nel * (dim + 1));
+    or_mesh_elements(dim, nsec, conn);
+    for (int64_t e = 0; e < nel; ++e) {
+        const int32_t* v = conn + e * (dim + 1);
+        double f0 = f[v[0]];
+        dg[e * (dim + 1)] = f0;
+        for (int a = 1; a <= dim; ++a) dg[e * (dim + 1) + a] = f[v[a]] - f0;
+    }
+    free(conn);
+}

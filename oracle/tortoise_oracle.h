@@ -191,6 +191,35 @@ void or_ee_apply_direct(int dim, const int32_t* nsec, const double* extent, cons
                         const double* f, double g, double kappa2, double sigma, double cutoff,
                         double* I);
 
+
+/* ------------------------------------- reference tuple operator (mc_oracle.c) */
+/* Rows of a channel's MC point matrix: max(dim*(nlegs-1)-1, 2*dim). */
+int32_t or_mc_rows(int dim, int nlegs);
+/* MC points [rows][nmc] of a channel from a splitmix64 stream. */
+void    or_mc_points(int dim, int nlegs, int32_t nmc, uint64_t seed, double* pts);
+/* Inverse (row-major, n <= 4) by Gauss-Jordan with partial pivoting; returns det. */
+double  or_mc_invert(int n, const double* A, double* inv);
+/* scatteringIntegrationTypeB with the standard output forms; out nlegs*(dim+1). */
+int     or_mc_integrate(int dim, int nlegs, const double* D, const double* N,
+                        const double* fa, const double* fb, const double* pts,
+                        int32_t nmc, double* out);
+/* Tuples (e0, e1, e2, umklapp id) of the e-ph channel {el in, pn in, el out}. */
+int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin,
+                         const double* extent, const double* eps, double omega0,
+                         int32_t nu, const double* pts, int32_t nmc, double min_det,
+                         double threshold, int32_t* idx, int64_t cap);
+/* LF-basis inverse mass matrix of one element. */
+void    or_mc_inverse_mass(int dim, double vol, double* minv);
+/* Propagator of the e-ph channel over a tuple list, then the inverse mass. */
+void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin,
+                        const double* extent, const double* eps, double omega0,
+                        double amplitude, int32_t nu, const double* pts, int32_t nmc,
+                        const int32_t* idx, int64_t ntup, const double* f_el,
+                        const double* f_ph, double* out_el, double* out_ph,
+                        int32_t nthreads);
+/* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
+void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,261 @@
+"""Pins of the oracle's restatement of the reference's own tuple operator
+(oracle/mc_oracle.c: IntegratorCore.hpp:260-357, ScatteringChannel.cpp:285-670).
+
+No executable reference exists here (Eigen and TBB are absent), so the
+restatement is pinned by the mathematics:
+  - the 1D two-leg integral is deterministic: g(D^-1 N) / |det D|;
+  - the 2D three-leg delta-shell integral against an independent quadrature
+    that takes a different set of free coordinates (legs 1 and 2 free, leg 0
+    constrained), so weights, factorials, inside tests and the constrained
+    solve are all checked against a second parametrisation of the same shell;
+  - uniform sampling of the reference simplices (moments);
+  - the LF mass matrix against quadrature and its inverse;
+  - the tuple list against brute force over every element triple and
+    umklapp vector of a small mesh, with D and N built independently;
+  - particle-number and energy balance of the full operator to roundoff, its
+    linearity in the amplitude, and thread-count independence.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tortoise_amd import workloads as wl
+
+TWO_PI = 2.0 * math.pi
+
+
+def test_simplex_sampling_moments():
+    for dim in (2, 3):
+        pts = oracle.mc_points(dim, 3, 400_000, seed=5)
+        rows = pts.shape[0]
+        x = pts[:dim]                     # leg 0: full simplex
+        assert (x > 0).all() and (x.sum(0) < 1).all()
+        assert np.allclose(x.mean(1), 1.0 / (dim + 1), atol=2e-3)
+        # E[x_i^2] = 2/((d+1)(d+2)), E[x_i x_j] = 1/((d+1)(d+2))
+        assert abs((x[0] ** 2).mean() - 2.0 / ((dim + 1) * (dim + 2))) < 2e-3
+        assert abs((x[0] * x[1]).mean() - 1.0 / ((dim + 1) * (dim + 2))) < 2e-3
+        part = pts[rows - (dim - 1):]      # partially free leg: (dim-1)-simplex
+        assert (part > 0).all() and (part.sum(0) < 1).all()
+        assert np.allclose(part.mean(1), 1.0 / dim, atol=2e-3)
+
+
+def test_invert_matches_numpy():
+    rng = np.random.default_rng(1)
+    for n in (2, 3, 4):
+        A = rng.normal(size=(n, n))
+        det, inv = oracle.mc_invert(A)
+        assert np.allclose(inv, np.linalg.inv(A), rtol=1e-12, atol=1e-12)
+        assert abs(det - np.linalg.det(A)) <= 1e-12 * abs(det)
+
+
+def test_1d_two_legs_closed_form():
+    """NDim = 1, NLegs = 2 is deterministic: the delta fixes both legs."""
+    D = np.array([[1.0, -1.0], [0.8, -0.3]])          # momentum row, energy row
+    K = np.array([0.3, 0.55])                          # the point we want inside
+    N = D @ K
+    fa = np.array([0.2, 0.5, 0.7, -0.4])               # leg0 form, leg1 form
+    fb = np.array([1.0, 0.1, 0.3, 0.2])
+    out = oracle.mc_integrate(1, 2, D, N, fa, fb, np.zeros((2, 1)))
+    g = (fa[0] + fa[1] * K[0]) * (fa[2] + fa[3] * K[1]) + (fb[0] + fb[1] * K[0]) * (fb[2] + fb[3] * K[1])
+    expect = g / abs(np.linalg.det(D)) * np.array([1.0, K[0], 1.0, K[1]])
+    assert np.allclose(out, expect, rtol=1e-13)
+    # a shell outside the elements gives nothing
+    out = oracle.mc_integrate(1, 2, D, D @ np.array([0.3, 1.2]), fa, fb, np.zeros((2, 1)))
+    assert not out.any()
+
+
+def _shell_2d3_alternative(D, N, fa, fb, m):
+    """The 2D three-leg shell integral with legs 1 (x) and 2 free and leg 0
+    plus leg 1 (y) constrained: midpoint rule on centroids of an m x m
+    triangulation of leg 2's simplex times m midpoints of leg 1's x."""
+    # centroids of the regular subdivision of the reference triangle
+    c = []
+    for i in range(m):
+        for j in range(m - i):
+            c.append(((i + 1 / 3) / m, (j + 1 / 3) / m))
+            if i + j < m - 1:
+                c.append(((i + 2 / 3) / m, (j + 2 / 3) / m))
+    c = np.array(c)                                     # m^2 points, equal areas
+    t = (np.arange(m) + 0.5) / m
+    x2 = np.repeat(c, m, axis=0)
+    x1x = np.tile(t, len(c))
+    free_cols = [2, 4, 5]                               # leg1 x, leg2 x, leg2 y
+    con_cols = [0, 1, 3]                                # leg0 x, leg0 y, leg1 y
+    DF, DCp = D[:, free_cols], D[:, con_cols]
+    F = np.stack([x1x, x2[:, 0], x2[:, 1]])
+    a = np.linalg.solve(DCp, N[:, None] - DF @ F)       # constrained values
+    x0 = a[:2]
+    x1 = np.stack([x1x, a[2]])
+    inside = (x0 > 0).all(0) & (x0.sum(0) < 1) & (x1 > 0).all(0) & (x1.sum(0) < 1)
+
+    def lf(form, x):
+        return form[0] + form[1] * x[0] + form[2] * x[1]
+    g = (lf(fa[0:3], x0) * lf(fa[3:6], x1) * lf(fa[6:9], x2.T)
+         + lf(fb[0:3], x0) * lf(fb[3:6], x1) * lf(fb[6:9], x2.T)) * inside
+    vol_free = 0.5 * 1.0                                # triangle x unit interval
+    w = vol_free / len(g) / abs(np.linalg.det(DCp))
+    outs = []
+    for pt in (x0, x1, x2.T):
+        outs += [w * g.sum(), w * (g * pt[0]).sum(), w * (g * pt[1]).sum()]
+    return np.array(outs)
+
+
+def test_2d_three_legs_against_second_parametrisation():
+    rng = np.random.default_rng(3)
+    # a tuple shaped like the e-ph one: leg 1 has no energy slope
+    D = np.array([[0.20, 0.05, 0.19, 0.02, -0.21, -0.03],
+                  [0.01, 0.18, 0.03, 0.20, -0.02, -0.19],
+                  [0.15, 0.22, 0.00, 0.00, -0.17, -0.24]])
+    Kin = np.array([0.25, 0.3, 0.4, 0.2, 0.3, 0.35])     # one point of the shell, inside
+    N = D @ Kin
+    fa = rng.uniform(-1, 1, 9)
+    fb = rng.uniform(-1, 1, 9)
+    pts = oracle.mc_points(2, 3, 2_000_000, seed=11)
+    mc = oracle.mc_integrate(2, 3, D, N, fa, fb, pts)
+    alt = _shell_2d3_alternative(D, N, fa, fb, 160)
+    scale = np.abs(alt).max()
+    assert np.abs(mc - alt).max() <= 5e-3 * scale, (mc, alt)
+    # the measure of the shell (unit forms) is positive and agrees as well
+    one = np.zeros(9)
+    one[[0, 3, 6]] = 1.0
+    mc1 = oracle.mc_integrate(2, 3, D, N, one, np.zeros(9), pts)
+    alt1 = _shell_2d3_alternative(D, N, one, np.zeros(9), 160)
+    assert mc1[0] > 0 and abs(mc1[0] - alt1[0]) <= 5e-3 * alt1[0]
+    # basis-0 outputs of all legs are the same integral
+    assert mc[0] == mc[3] == mc[6]
+
+
+def test_inverse_mass_matrix():
+    for dim in (2, 3):
+        vol = 0.37
+        Minv = oracle.mc_inverse_mass(dim, vol)
+        # M by a quadrature of the reference simplex (uniform points)
+        x = oracle.mc_points(dim, 3, 1_000_000, seed=2)[:dim]
+        phi = np.vstack([np.ones(x.shape[1]), x])
+        M = vol * (phi @ phi.T) / x.shape[1]
+        assert np.allclose(np.linalg.inv(Minv), M, rtol=5e-3, atol=5e-3 * vol)
+        assert np.allclose(np.linalg.inv(Minv), np.linalg.inv(Minv).T)
+
+
+def _mesh(ns, dim=2):
+    w = wl.Workload("sq", dim, (ns,) * dim)
+    return w, w.eps()
+
+
+def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9):
+    """Every (e0, e1, e2, u) of a small 2D mesh, D and N written out here."""
+    dim = 2
+    X = oracle.mesh_nodes(dim, list(w.nsec), list(w.origin), list(w.extent))
+    conn = oracle.mesh_elements(dim, list(w.nsec))
+    nel = conn.shape[0]
+    vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
+    span = 2 * nu + 1
+    us = [(i - nu, j - nu) for i in range(span) for j in range(span)]
+    kf = np.stack([X[conn[:, 1]] - X[conn[:, 0]], X[conn[:, 2]] - X[conn[:, 0]]], axis=2)  # [e][d][a]
+    k0 = X[conn[:, 0]]
+    ef = np.stack([eps[conn[:, 1]] - eps[conn[:, 0]], eps[conn[:, 2]] - eps[conn[:, 0]]], 1)
+    e0v = eps[conn[:, 0]]
+    emin, emax = eps[conn].min(1), eps[conn].max(1)
+    one = np.zeros(9)
+    one[[0, 3, 6]] = math.sqrt(2.0)
+    keep = set()
+    for a in range(nel):
+        for c in range(nel):
+            if not (emin[a] + w.omega0 - emax[c] <= 0 and emax[a] + w.omega0 - emin[c] >= 0):
+                continue
+            for b in range(nel):
+                for uid, u in enumerate(us):
+                    D = np.zeros((3, 6))
+                    D[:2, 0:2] = kf[a]
+                    D[:2, 2:4] = kf[b]
+                    D[:2, 4:6] = -kf[c]
+                    D[2, 0:2] = ef[a]
+                    D[2, 4:6] = -ef[c]
+                    N = np.zeros(3)
+                    N[:2] = -(k0[a] + k0[b] - k0[c]) - np.array(u) * np.array(w.extent)
+                    N[2] = -(e0v[a] + w.omega0 - e0v[c])
+                    DC = D[:, 3:6]
+                    if abs(np.linalg.det(DC)) < min_det:
+                        continue
+                    r = oracle.mc_integrate(2, 3, D, N, one, np.zeros(9), pts)
+                    if abs(r[0] * (vol * 2) ** 3) > threshold:
+                        keep.add((a, b, c, uid))
+    return keep
+
+
+def test_tuples_match_brute_force_small_mesh():
+    w, eps = _mesh(4)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    got = set(map(tuple, idx.tolist()))
+    assert len(got) == len(idx)                        # no duplicates
+    brute = _brute_tuples(w, eps, 1, pts)
+    assert got == brute
+    # ordered by (e0, e2, umklapp id, e1)
+    key = idx[:, 0].astype(np.int64) * 10 ** 9 + idx[:, 2] * 10 ** 5 + idx[:, 3] * 10 ** 3 + idx[:, 1]
+    assert (np.diff(key) > 0).all()
+
+
+def _operator_case(ns=8, seed=0, nthreads=1, amp=None):
+    w, eps = _mesh(ns)
+    pts = oracle.mc_points(2, 3, 50, seed=seed)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    fdg = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    rng = np.random.default_rng(seed)
+    fdg = fdg + 0.02 * rng.standard_normal(fdg.shape)  # discontinuous DG state
+    nel = fdg.shape[0]
+    fph = np.zeros((nel, 3))
+    fph[:, 0] = 0.0894
+    fph[:, 1:] = 0.01 * rng.standard_normal((nel, 2))
+    amp = w.g ** 2 if amp is None else amp
+    el, ph = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, amp, 1, pts, idx,
+                                 fdg, fph, nthreads=nthreads, phonon_out=True)
+    return w, eps, idx, fdg, fph, el, ph
+
+
+def _integrals(w, eps, coeff, weight_form=None):
+    """Exact integral of a DG function (times a DG weight form) per element."""
+    vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
+    Minv = oracle.mc_inverse_mass(2, vol)
+    M = np.linalg.inv(Minv)
+    if weight_form is None:
+        weight_form = np.zeros_like(coeff)
+        weight_form[:, 0] = 1.0
+    return np.einsum("ea,ab,eb->e", weight_form, M, coeff)
+
+
+def test_operator_number_and_energy_balance():
+    w, eps, idx, fdg, fph, el, ph = _operator_case()
+    num = _integrals(w, eps, el)
+    assert abs(num.sum()) <= 1e-13 * np.abs(num).sum()
+    # energy: int eps f_el' + omega0 int n' = 0 (eps is the P1 dispersion)
+    eform = oracle.mc_nodal_to_dg(2, w.nsec, eps)
+    e_el = _integrals(w, eps, el, eform).sum()
+    e_ph = w.omega0 * _integrals(w, eps, ph).sum()
+    assert abs(e_el + e_ph) <= 1e-12 * (abs(e_el) + abs(e_ph))
+    # phonons are absorbed and emitted: the phonon number changes
+    assert abs(_integrals(w, eps, ph).sum()) > 1e-6 * np.abs(_integrals(w, eps, ph)).sum()
+
+
+def test_operator_linear_in_amplitude_and_thread_independent():
+    _, _, _, _, _, el1, _ = _operator_case(amp=0.01)
+    _, _, _, _, _, el2, _ = _operator_case(amp=0.03)
+    assert np.allclose(el2, 3 * el1, rtol=1e-12, atol=1e-15)
+    _, _, _, _, _, el4, _ = _operator_case(amp=0.01, nthreads=4)
+    assert np.abs(el4 - el1).max() <= 1e-13 * np.abs(el1).max()
+
+
+def test_operator_vanishes_for_empty_band():
+    w, eps = _mesh(6)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    nel = idx[:, 0].max() + 1
+    z = np.zeros((72, 3))
+    fph = np.zeros((72, 3))
+    fph[:, 0] = 0.3
+    out = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 0.01, 1, pts, idx, z,
+                              fph)
+    assert not out.any() and nel <= 72

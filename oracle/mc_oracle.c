@@ -139,6 +139,48 @@ static double lf_apply(int dim, const double* form, const double* x) {
     return v;
 }
 
+/* The legs' points of MC sample m (IntegratorCore.hpp:283-320): free
+ * coordinates from the point matrix, the D_C-constrained ones from
+ *   alpha = -D_C^-1 (D_F K_F - N),
+ * then the inside tests of the two constrained legs.  Returns 1 inside. */
+static int shell_point(int dim, int nlegs, const double* D, const double* N, const double* inv,
+                       const double* pts, int32_t nmc, int32_t m, double K[4][3]) {
+    const int nc = dim + 1, ncol = nlegs * dim, rows = or_mc_rows(dim, nlegs);
+    const int c0 = (nlegs - 1) * dim - 1;
+    for (int leg = 0; leg < nlegs - 2; ++leg)
+        for (int d = 0; d < dim; ++d) K[leg][d] = pts[(size_t)(leg * dim + d) * nmc + m];
+    for (int d = 0; d < dim - 1; ++d) K[nlegs - 2][d] = pts[(size_t)(rows - (dim - 1) + d) * nmc + m];
+    double rhs[4];
+    for (int r = 0; r < nc; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < c0; ++c) s += D[r * ncol + c] * K[c / dim][c % dim];
+        rhs[r] = s - N[r];
+    }
+    for (int r = 0; r < nc; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < nc; ++c) s += inv[r * nc + c] * rhs[c];
+        if (r == 0) K[nlegs - 2][dim - 1] = -s;
+        else K[nlegs - 1][r - 1] = -s;
+    }
+    int inside = 1;
+    for (int leg = nlegs - 2; leg < nlegs; ++leg) {
+        double sum = 0.0;
+        for (int d = 0; d < dim; ++d) { inside &= K[leg][d] > 0.0; sum += K[leg][d]; }
+        inside &= sum < 1.0;
+    }
+    return inside;
+}
+
+/* Divisor of the MC estimate (IntegratorCore.hpp:336-356):
+ * |det D_C| * NMC * (dim-1)! * (dim!)^(nlegs-2); just |det D_C| for 1D two legs. */
+static double shell_divisor(int dim, int nlegs, double det, int32_t nmc) {
+    double w = fabs(det);
+    if (nlegs == 2 && dim == 1) return w;
+    w *= (double)nmc * (double)mc_fact(dim - 1);
+    for (int k = 0; k < nlegs - 2; ++k) w *= (double)mc_fact(dim);
+    return w;
+}
+
 /* scatteringIntegrationTypeB (IntegratorCore.hpp:260-357) with the output
  * forms of constructOutputformsLinForm (ScatteringChannel.cpp:355-381): the
  * output of leg l, basis j, is the integrand times phi_j at leg l's point,
@@ -150,7 +192,7 @@ static double lf_apply(int dim, const double* form, const double* x) {
  * Returns 0, or -1 if D_C is singular. */
 int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const double* fa,
                     const double* fb, const double* pts, int32_t nmc, double* out) {
-    const int nc = dim + 1, ncol = nlegs * dim, rows = or_mc_rows(dim, nlegs);
+    const int nc = dim + 1, ncol = nlegs * dim;
     const int c0 = (nlegs - 1) * dim - 1;           /* first column of D_C */
     double DC[16], inv[16];
     for (int r = 0; r < nc; ++r)
@@ -158,36 +200,10 @@ int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const 
     double det = or_mc_invert(nc, DC, inv);
     if (det == 0.0) return -1;
     for (int i = 0; i < nlegs * nc; ++i) out[i] = 0.0;
-    const int single = (nlegs == 2 && dim == 1);
-    const int32_t npt = single ? 1 : nmc;
+    const int32_t npt = (nlegs == 2 && dim == 1) ? 1 : nmc;
     for (int32_t m = 0; m < npt; ++m) {
         double K[4][3];                  /* point of every leg, reference coords */
-        for (int leg = 0; leg < nlegs - 2; ++leg)
-            for (int d = 0; d < dim; ++d) K[leg][d] = pts[(size_t)(leg * dim + d) * nmc + m];
-        for (int d = 0; d < dim - 1; ++d) K[nlegs - 2][d] = pts[(size_t)(rows - (dim - 1) + d) * nmc + m];
-        /* rhs = D_F K_F - N over the free columns 0..c0-1 */
-        double rhs[4];
-        for (int r = 0; r < nc; ++r) {
-            double s = 0.0;
-            for (int c = 0; c < c0; ++c) s += D[r * ncol + c] * K[c / dim][c % dim];
-            rhs[r] = s - N[r];
-        }
-        double alpha[4];
-        for (int r = 0; r < nc; ++r) {
-            double s = 0.0;
-            for (int c = 0; c < nc; ++c) s += inv[r * nc + c] * rhs[c];
-            alpha[r] = -s;
-        }
-        K[nlegs - 2][dim - 1] = alpha[0];
-        for (int d = 0; d < dim; ++d) K[nlegs - 1][d] = alpha[1 + d];
-        /* inside-simplex tests of the two constrained legs */
-        int inside = 1;
-        for (int leg = nlegs - 2; leg < nlegs; ++leg) {
-            double sum = 0.0;
-            for (int d = 0; d < dim; ++d) { inside &= K[leg][d] > 0.0; sum += K[leg][d]; }
-            inside &= sum < 1.0;
-        }
-        if (!inside) continue;
+        if (!shell_point(dim, nlegs, D, N, inv, pts, nmc, m, K)) continue;
         double pa = 1.0, pb = 1.0;
         for (int leg = 0; leg < nlegs; ++leg) {
             pa *= lf_apply(dim, fa + leg * nc, K[leg]);
@@ -199,11 +215,7 @@ int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const 
             for (int d = 0; d < dim; ++d) out[leg * nc + 1 + d] += val * K[leg][d];
         }
     }
-    double w = fabs(det);
-    if (!single) {
-        w *= (double)nmc * (double)mc_fact(dim - 1);
-        for (int k = 0; k < nlegs - 2; ++k) w *= (double)mc_fact(dim);
-    }
+    double w = shell_divisor(dim, nlegs, det, nmc);
     for (int i = 0; i < nlegs * nc; ++i) out[i] /= w;
     return 0;
 }
@@ -306,25 +318,29 @@ static void elem_erange(const mc_geo* g, int64_t e, double* lo, double* hi) {
 }
 
 /* Screening value of a tuple: scatteringIntegrationTypeC with the modal
- * order-0 form on every leg (ScatteringChannel.cpp:465-489), amplitude 1 and
- * functAmplitude 1 (screening runs at channel creation, before either is set).
- * Returns -1 when |det D_C| < min_det. */
+ * order-0 form sqrt(2) (1 in 1D) on every leg (ScatteringChannel.cpp:465-489),
+ * amplitude 1 and functAmplitude 1 (screening runs at channel creation,
+ * before either is set).  Evaluated as hits * c^3 * prod(vol*dim!) / divisor,
+ * which grows with the hit count, so the comparison with the threshold can
+ * stop at the first count that passes.  Returns -1 when |det D_C| < min_det. */
 static double tuple_screen(const mc_geo* g, const double* D, const double* N, const double* pts,
                            int32_t nmc, double min_det) {
     const int dim = g->dim, nc = dim + 1;
-    double fa[12], fb[12], out[12];
-    double c = dim == 1 ? 1.0 : sqrt(2.0);        /* modal order 0 in the LF basis */
-    for (int i = 0; i < 3 * nc; ++i) { fa[i] = 0.0; fb[i] = 0.0; }
-    for (int leg = 0; leg < 3; ++leg) fa[leg * nc] = c;
     double DC[16], inv[16];
     for (int r = 0; r < nc; ++r)
         for (int q = 0; q < nc; ++q) DC[r * nc + q] = D[r * 3 * dim + 2 * dim - 1 + q];
     double det = or_mc_invert(nc, DC, inv);
-    if (fabs(det) < min_det) return -1.0;
-    or_mc_integrate(dim, 3, D, N, fa, fb, pts, nmc, out);
+    if (!(fabs(det) >= min_det)) return -1.0;
+    int64_t hits = 0;
+    for (int32_t m = 0; m < nmc; ++m) {
+        double K[4][3];
+        hits += shell_point(dim, 3, D, N, inv, pts, nmc, m, K);
+    }
+    double c = dim == 1 ? 1.0 : sqrt(2.0);
+    double cpow = c * c * c;
     double amp = 1.0;
     for (int leg = 0; leg < 3; ++leg) amp *= g->vol * (double)mc_fact(dim);
-    return fabs(out[0] * amp);
+    return (double)hits * cpow * amp / shell_divisor(dim, 3, det, nmc);
 }
 
 /* Tuples of the e-ph channel, ordered by (e0, e2, umklapp id, e1): every

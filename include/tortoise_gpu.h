@@ -193,6 +193,71 @@ tg_status   tg_relative_error(tg_ctx* ctx, const double* err, const double* pop,
                               int64_t n, int32_t batch, double rtol, double atol,
                               double* out, int32_t flags);
 
+
+/* ==================================================================
+ * Tuple operator: the reference's own e-ph scattering integral.
+ *
+ * The channel {electron in, phonon in, electron out} of
+ * ScatteringChannel<NDim,3> with a dispersionless phonon band held as a bath,
+ * evaluated the reference's way: a list of element tuples (e0, e1, e2,
+ * umklapp) and, per tuple, the Monte-Carlo integral over the energy-momentum
+ * delta shell with a fixed matrix of MC points, on DG (linear-form) element
+ * coefficients.  Replaces (paths relative to the TORTOISE tree):
+ *   ScatteringChannel::precalculateIndices     src/Physics/Scattering/ScatteringChannel.cpp:421-538
+ *   ScatteringChannel::constructMCPoints       src/Physics/Scattering/ScatteringChannel.cpp:677-688
+ *   ScatteringChannel::propagate               src/Physics/Scattering/ScatteringChannel.cpp:618-670
+ *   scatteringIntegrationTypeB                 include/Physics/Scattering/IntegratorCore.hpp:260-357
+ *   Function::applyInverseMass                 src/Geometry/Structured/FunctionSpace/Function.cpp:801-804
+ *   .scattIndices file                         src/Physics/Scattering/ScatteringChannel.cpp:422-459,520-537
+ *
+ * State layout: member b, element e, linear-form coefficient a at
+ * ptr[(b*nel + e)*(dim+1) + a] -- the reference Function's element view
+ * [f(v0), f(v1)-f(v0), ...] (Function.hpp:326-344).
+ * ================================================================== */
+typedef struct tg_tuple_ctx tg_tuple_ctx;
+
+typedef struct tg_tuple_desc {
+    int32_t dim;            /* 2 or 3 */
+    int32_t nsec[3];        /* sections per axis */
+    double  origin[3];
+    double  extent[3];
+    const double* eps;      /* nodal electron energies (the P1 dispersion), host memory */
+    double  omega0;         /* phonon energy (dispersionless band) */
+    double  amplitude;      /* ScatteringChannel::amplitude (g^2); functAmplitude = 1 */
+    int32_t n_umklapp;      /* umklapp vectors {-n..n}^dim (findUmklappVectors(n)), 0..3 */
+    int32_t num_mc;         /* numMCPoints (Material default 50) */
+    double  min_det;        /* minimumDeterminant (Material default 1e-9) */
+    double  threshold;      /* scattElemeThreshold (Material default 1e-9) */
+    int32_t max_batch;      /* members per propagate call */
+    int32_t reserved;
+} tg_tuple_desc;
+
+tg_status   tg_tuple_create(const tg_tuple_desc* desc, tg_tuple_ctx** out);
+void        tg_tuple_destroy(tg_tuple_ctx* ctx);
+const char* tg_tuple_last_error(const tg_tuple_ctx* ctx);
+/* rows of the MC point matrix of a channel: max(dim*(nlegs-1)-1, 2*dim) */
+int32_t     tg_tuple_point_rows(int32_t dim, int32_t nlegs);
+/* Sets the MC point matrix [rows][num_mc] (host, row-major): leg 0's points
+ * in rows 0..dim-1, the partially free leg's in the bottom dim-1 rows.  The
+ * first call also builds the tuple list (unless one was loaded); every call
+ * re-integrates the per-tuple moments (MaterialTimeStatus::propagate redraws
+ * the points, MaterialTimeStatus.cpp:102-105). */
+tg_status   tg_tuple_set_points(tg_tuple_ctx* ctx, const double* pts, int32_t flags);
+/* Tuple list (scatteringMatrixIndices + umklapp ids): [n][4] = e0, e1, e2, uid,
+ * ordered by e0.  count is -1 before a list exists. */
+tg_status   tg_tuple_count(const tg_tuple_ctx* ctx, int64_t* count);
+tg_status   tg_tuple_copy_indices(tg_tuple_ctx* ctx, int32_t* idx, int64_t cap);
+tg_status   tg_tuple_set_indices(tg_tuple_ctx* ctx, const int32_t* idx, int64_t count);
+tg_status   tg_tuple_save_indices(tg_tuple_ctx* ctx, const char* path);
+tg_status   tg_tuple_load_indices(tg_tuple_ctx* ctx, const char* path);
+tg_status   tg_tuple_set_amplitude(tg_tuple_ctx* ctx, double amplitude);
+/* dN/dt of the electron band (after the inverse mass) for `batch` members
+ * with phonon occupations bath_n[b].  flags: TG_PTR_DEVICE, TG_TIME_KERNEL. */
+tg_status   tg_tuple_propagate(tg_tuple_ctx* ctx, const double* f, const double* bath_n, double* out,
+                               int64_t nel, int32_t batch, int32_t flags);
+tg_status   tg_tuple_last_kernel_ms(tg_tuple_ctx* ctx, double* ms);
+void*       tg_tuple_stream(const tg_tuple_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,155 @@
+"""GPU parity of the tuple operator (tg_tuple_*: the reference's own e-ph
+operator, ScatteringChannel.cpp:421-538,618-688 and IntegratorCore.hpp:260-357)
+against the oracle's restatement (oracle/mc_oracle.c).
+
+Bar:
+  - the tuple list is identical (same tuples, same order): both sides make the
+    same inside/outside decision per MC point;
+  - one evaluation agrees to 1e-12 of max|out| (the GPU sums precomputed
+    per-tuple moments, the oracle sums the MC points per stage);
+  - particle number sum_e int out_e = 0 to 1e-13 of sum |int out_e|;
+  - a member of a batch is bitwise equal to its single evaluation.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+import tortoise_amd as tg
+from tortoise_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(dim, ns, seed=0, nmc=50, nu=1):
+    w = wl.Workload(f"t{dim}d{ns}", dim, (ns,) * dim)
+    eps = w.eps()
+    pts = oracle.mc_points(dim, 3, nmc, seed)
+    return w, eps, pts
+
+
+def _state(w, batch, seed):
+    rng = np.random.default_rng(seed)
+    base = oracle.mc_nodal_to_dg(w.dim, w.nsec, w.initial_f())
+    F = np.stack([base + 0.02 * rng.standard_normal(base.shape) for _ in range(batch)])
+    bath = rng.uniform(0.05, 0.2, batch)
+    return F, bath
+
+
+def _oracle_out(w, eps, pts, idx, F, n, nu=1):
+    fph = np.zeros_like(F)
+    fph[:, 0] = n
+    return oracle.mc_apply_eph(w.dim, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, nu, pts,
+                               idx, F, fph, nthreads=8)
+
+
+def _number(w, out):
+    vol = abs(np.prod(np.array(w.extent[:w.dim]) / np.array(w.nsec[:w.dim]))) / (2 if w.dim == 2 else 6)
+    M = np.linalg.inv(oracle.mc_inverse_mass(w.dim, vol))
+    return out @ M[:, 0]       # int of each element's linear form
+
+
+@pytest.mark.parametrize("dim,ns", [(2, 8), (2, 16), (3, 3)])
+def test_tuple_list_and_propagate_match_oracle(dim, ns):
+    print(f"[gpu] tuple operator {dim}D ns={ns}", flush=True)
+    w, eps, pts = _case(dim, ns)
+    idx_o = oracle.mc_tuples_eph(dim, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    op = tg.TupleOperator.create(w, max_batch=3)
+    op.set_points(pts)
+    idx_g = op.indices()
+    assert idx_g.shape == idx_o.shape and np.array_equal(idx_g, idx_o)
+    F, bath = _state(w, 3, seed=ns)
+    out = op.propagate(F, bath)
+    for b in range(3):
+        ref = _oracle_out(w, eps, pts, idx_o, F[b], bath[b])
+        assert np.abs(out[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
+        num = _number(w, out[b])
+        assert abs(num.sum()) <= 1e-13 * np.abs(num).sum()
+    # members of a batch equal single evaluations
+    for b in (0, 2):
+        assert np.array_equal(op.propagate(F[b], bath[b]), out[b])
+    op.close()
+
+
+def test_tuple_config0_square32():
+    """The reference's own CPU-runnable case: 32 x 32 sections (2048
+    triangles), NMC = 50, umklapp {-1,0,1}^2."""
+    print("[gpu] tuple operator square32", flush=True)
+    w, eps, pts = _case(2, 32)
+    idx_o = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    op = tg.TupleOperator.create(w, max_batch=1)
+    op.set_points(pts)
+    assert np.array_equal(op.indices(), idx_o)
+    F, bath = _state(w, 1, seed=7)
+    out = op.propagate(F[0], bath[0])
+    ref = _oracle_out(w, eps, pts, idx_o, F[0], bath[0])
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+    op.close()
+
+
+def test_redraw_keeps_tuples_and_matches_oracle():
+    """MaterialTimeStatus::propagate redraws the MC points: the tuple list
+    stays, the moments follow the new points."""
+    w, eps, pts = _case(2, 12, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    op = tg.TupleOperator.create(w, max_batch=1)
+    op.set_points(pts)
+    F, bath = _state(w, 1, seed=3)
+    out0 = op.propagate(F[0], bath[0])
+    pts1 = oracle.mc_points(2, 3, 50, seed=99)
+    op.set_points(pts1)
+    assert np.array_equal(op.indices(), idx)
+    out1 = op.propagate(F[0], bath[0])
+    ref1 = _oracle_out(w, eps, pts1, idx, F[0], bath[0])
+    assert np.abs(out1 - ref1).max() <= 1e-12 * np.abs(ref1).max()
+    assert np.abs(out1 - out0).max() > 1e-6 * np.abs(out0).max()
+    op.close()
+
+
+def test_scatt_indices_file_round_trip(tmp_path):
+    """.scattIndices layout (ScatteringChannel.cpp:422-459,520-537): per leg
+    and axis an int section count, an unsigned long tuple count, then per
+    tuple three int element ids and an int umklapp id."""
+    w, eps, pts = _case(2, 6)
+    op = tg.TupleOperator.create(w, max_batch=1)
+    op.set_points(pts)
+    idx = op.indices()
+    path = str(tmp_path / "[(el:C)(pn:opt)]-[(el:C)].scattIndices")
+    op.save_indices(path)
+    raw = open(path, "rb").read()
+    head = struct.unpack("<6i", raw[:24])
+    assert head == (6, 6, 6, 6, 6, 6)
+    (count,) = struct.unpack("<Q", raw[24:32])
+    assert count == len(idx)
+    body = np.frombuffer(raw[32:], np.int32).reshape(-1, 4)
+    assert np.array_equal(body, idx)
+    # a second operator loads the list instead of enumerating, same result
+    op2 = tg.TupleOperator.create(w, max_batch=1)
+    op2.load_indices(path)
+    op2.set_points(pts)
+    F, bath = _state(w, 1, seed=4)
+    assert np.array_equal(op2.propagate(F[0], bath[0]), op.propagate(F[0], bath[0]))
+    # a file for another mesh is refused
+    w8 = wl.Workload("t8", 2, (8, 8))
+    op3 = tg.TupleOperator.create(w8, max_batch=1)
+    with pytest.raises(tg.TgError, match="sections differ"):
+        op3.load_indices(path)
+    for o in (op, op2, op3):
+        o.close()
+
+
+def test_set_indices_rejects_bad_lists():
+    w, eps, pts = _case(2, 4)
+    op = tg.TupleOperator.create(w, max_batch=1)
+    nel = 32
+    with pytest.raises(tg.TgError, match="element"):
+        op.set_indices(np.array([[0, 1, nel, 0]], np.int32))
+    with pytest.raises(tg.TgError, match="umklapp"):
+        op.set_indices(np.array([[0, 1, 2, 9]], np.int32))
+    with pytest.raises(tg.TgError, match="ordered"):
+        op.set_indices(np.array([[3, 1, 2, 0], [2, 1, 2, 0]], np.int32))
+    with pytest.raises(tg.TgError, match="MC points"):
+        op.propagate(np.zeros((nel, 3)), 0.1)
+    op.close()

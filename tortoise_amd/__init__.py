@@ -45,6 +45,10 @@ EXPORTS = [
     "tg_kernel_copy_rows", "tg_collision_apply", "tg_dp5_integrate", "tg_conservation",
     "tg_relative_error", "tg_kernel_stats_get", "tg_last_kernel_ms", "tg_dp5_replay",
     "tg_dp5_adjoint",
+    "tg_tuple_create", "tg_tuple_destroy", "tg_tuple_last_error", "tg_tuple_point_rows",
+    "tg_tuple_set_points", "tg_tuple_count", "tg_tuple_copy_indices", "tg_tuple_set_indices",
+    "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
+    "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream",
 ]
 
 
@@ -84,6 +88,14 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TupleDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("nsec", C.c_int32 * 3), ("origin", C.c_double * 3),
+                ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("omega0", C.c_double),
+                ("amplitude", C.c_double), ("n_umklapp", C.c_int32), ("num_mc", C.c_int32),
+                ("min_det", C.c_double), ("threshold", C.c_double), ("max_batch", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 _lib = None
@@ -141,6 +153,28 @@ def load():
     L.tg_conservation.argtypes = [vp, vp, i64, i32, vp, vp, i32]
     L.tg_relative_error.restype = i32
     L.tg_relative_error.argtypes = [vp, vp, vp, i64, i32, f64, f64, vp, i32]
+    L.tg_tuple_create.restype = i32
+    L.tg_tuple_create.argtypes = [C.POINTER(TupleDesc), C.POINTER(vp)]
+    L.tg_tuple_destroy.restype = None
+    L.tg_tuple_destroy.argtypes = [vp]
+    L.tg_tuple_last_error.restype = C.c_char_p
+    L.tg_tuple_last_error.argtypes = [vp]
+    L.tg_tuple_point_rows.restype = i32
+    L.tg_tuple_point_rows.argtypes = [i32, i32]
+    for name, args in (("tg_tuple_set_points", [vp, vp, i32]),
+                       ("tg_tuple_count", [vp, C.POINTER(i64)]),
+                       ("tg_tuple_copy_indices", [vp, vp, i64]),
+                       ("tg_tuple_set_indices", [vp, vp, i64]),
+                       ("tg_tuple_save_indices", [vp, C.c_char_p]),
+                       ("tg_tuple_load_indices", [vp, C.c_char_p]),
+                       ("tg_tuple_set_amplitude", [vp, f64]),
+                       ("tg_tuple_propagate", [vp, vp, vp, vp, i64, i32, i32]),
+                       ("tg_tuple_last_kernel_ms", [vp, C.POINTER(f64)])):
+        fn = getattr(L, name)
+        fn.restype = i32
+        fn.argtypes = args
+    L.tg_tuple_stream.restype = vp
+    L.tg_tuple_stream.argtypes = [vp]
     _lib = L
     return L
 
@@ -353,3 +387,120 @@ class Context:
         self._check(load().tg_collision_apply(self._h, C.c_void_p(f_ptr), C.c_void_p(I_ptr),
                                               self.n, batch, None if bn is None else _p(bn),
                                               flags))
+
+
+class TupleOperator:
+    """The reference's own e-ph operator (tg_tuple_*): element tuples of the
+    channel {el in, pn in, el out}, the MC delta-shell integral per tuple and
+    the propagator on DG linear-form coefficients [batch][nel][dim+1]
+    (ScatteringChannel.cpp:421-538,618-688; IntegratorCore.hpp:260-357)."""
+
+    def __init__(self, handle, dim, nel, max_batch):
+        self._h = handle
+        self.dim = dim
+        self.nel = nel
+        self.max_batch = max_batch
+
+    @classmethod
+    def create(cls, workload, n_umklapp=1, num_mc=50, min_det=1e-9, threshold=1e-9,
+               max_batch=1, amplitude=None, eps=None):
+        L = load()
+        w = workload
+        e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
+        d = TupleDesc()
+        d.dim = w.dim
+        for q in range(3):
+            d.nsec[q] = w.nsec[q] if q < w.dim else 1
+            d.origin[q] = w.origin[q] if q < w.dim else 0.0
+            d.extent[q] = w.extent[q] if q < w.dim else 1.0
+        d.eps = e.ctypes.data_as(C.POINTER(C.c_double))
+        d.omega0 = w.omega0
+        d.amplitude = w.g ** 2 if amplitude is None else amplitude
+        d.n_umklapp = n_umklapp
+        d.num_mc = num_mc
+        d.min_det = min_det
+        d.threshold = threshold
+        d.max_batch = max_batch
+        h = C.c_void_p()
+        rc = L.tg_tuple_create(C.byref(d), C.byref(h))
+        if rc != TG_OK:
+            msg = L.tg_tuple_last_error(h).decode() if h.value else "tg_tuple_create failed"
+            if h.value:
+                L.tg_tuple_destroy(h)
+            raise TgError(rc, msg)
+        nel = int(np.prod(w.nsec[:w.dim])) * (2 if w.dim == 2 else 6)
+        op = cls(h, w.dim, nel, max_batch)
+        op.rows = L.tg_tuple_point_rows(w.dim, 3)
+        op.num_mc = num_mc
+        return op
+
+    def _check(self, rc):
+        if rc != TG_OK:
+            raise TgError(rc, load().tg_tuple_last_error(self._h).decode())
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            load().tg_tuple_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_points(self, pts):
+        pts = np.ascontiguousarray(pts, np.float64)
+        if pts.shape != (self.rows, self.num_mc):
+            raise ValueError(f"points must be {self.rows} x {self.num_mc}")
+        self._check(load().tg_tuple_set_points(self._h, _p(pts), 0))
+
+    def count(self):
+        n = C.c_int64()
+        self._check(load().tg_tuple_count(self._h, C.byref(n)))
+        return n.value
+
+    def indices(self):
+        n = self.count()
+        idx = np.zeros((max(n, 1), 4), np.int32)
+        self._check(load().tg_tuple_copy_indices(self._h, _p(idx), max(n, 0)))
+        return idx[:max(n, 0)]
+
+    def set_indices(self, idx):
+        idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 4)
+        self._check(load().tg_tuple_set_indices(self._h, _p(idx), idx.shape[0]))
+
+    def save_indices(self, path):
+        self._check(load().tg_tuple_save_indices(self._h, os.fsencode(path)))
+
+    def load_indices(self, path):
+        self._check(load().tg_tuple_load_indices(self._h, os.fsencode(path)))
+
+    def set_amplitude(self, amplitude):
+        self._check(load().tg_tuple_set_amplitude(self._h, amplitude))
+
+    def propagate(self, f, bath_n, time_kernel=False):
+        """f: [nel, dim+1] or [batch, nel, dim+1]; bath_n scalar or [batch]."""
+        f = np.ascontiguousarray(f, np.float64)
+        single = f.ndim == 2
+        fb = f[None] if single else f
+        batch = fb.shape[0]
+        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        out = np.zeros_like(fb)
+        self._check(load().tg_tuple_propagate(self._h, _p(fb), _p(bn), _p(out), self.nel, batch,
+                                              TG_TIME_KERNEL if time_kernel else 0))
+        return out[0] if single else out
+
+    def propagate_device(self, f, bath_n, out, batch, time_kernel=False):
+        """Device pointers (ints, e.g. torch data_ptr()) on the operator's device."""
+        flags = TG_PTR_DEVICE | (TG_TIME_KERNEL if time_kernel else 0)
+        self._check(load().tg_tuple_propagate(self._h, C.c_void_p(f), C.c_void_p(bath_n),
+                                              C.c_void_p(out), self.nel, batch, flags))
+
+    def last_kernel_ms(self):
+        ms = C.c_double()
+        self._check(load().tg_tuple_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def stream(self):
+        return load().tg_tuple_stream(self._h)

@@ -1,0 +1,1011 @@
+// tg_tuple.hip -- the reference's own scattering operator on the GPU: element
+// tuples of the e-ph channel {el in, pn in, el out}, the Monte-Carlo
+// delta-shell integral per tuple, and the per-stage propagator over DG
+// (linear-form) element coefficients.
+//
+// Reference: ScatteringChannel<NDim,3>::precalculateIndices
+// (src/Physics/Scattering/ScatteringChannel.cpp:421-538), ::propagate
+// (:618-670) with scatteringIntegrationTypeB
+// (include/Physics/Scattering/IntegratorCore.hpp:260-357), the MC point matrix
+// (ScatteringChannel.cpp:677-688) and Function::applyInverseMass
+// (src/Geometry/Structured/FunctionSpace/Function.cpp:801-804).
+//
+// Design
+//   * Tuple list, built on the GPU in three passes over (e0, e2) element
+//     pairs (one workgroup per e0, e2 in chunks of the workgroup): energy
+//     window of the pair, umklapp vectors, phonon elements whose box meets
+//     k2 - k0 - G u, |det D_C| and the MC screening value.  Counting, then
+//     filling in (e0, e2, umklapp, e1) order, so the list is deterministic.
+//   * The MC point matrix is fixed between redraws, so everything that does
+//     not depend on the state is integrated once per redraw: with a phonon
+//     bath of occupation n the integrand is
+//         (1+n) f(k2) - n f(k0) - f(k0) f(k2),
+//     bilinear in the two electron elements' coefficients c0, c2.  Per tuple
+//     and output basis j of leg l the moments
+//         Q_l[j][a][b] = sum_m w phi_j(x_l) phi_a(x_0) phi_b(x_2)
+//     (phi_0 = 1, phi_j = x_{j-1}, w = prod(vol*dim!)/divisor) carry the whole
+//     operator.  Each tuple is stored twice, once in the list of its leg-0
+//     element and once in the list of its leg-2 element, with that leg's
+//     moments; basis 0 of both is the same integral.
+//   * Per stage one workgroup per element sums its two lists in a fixed order
+//     (deterministic, no atomics), applies the inverse mass and writes the
+//     element's coefficients for every member.  The moments are read once per
+//     member chunk; the other element's coefficients are L2-resident gathers.
+//   * The geometry (D, N, D_C^-1, constrained coordinates, inside tests) is
+//     evaluated with contraction off and the same operation order as the CPU
+//     restatement, so both make the same inside/outside decision per point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <cstdio>
+#include <new>
+#include <vector>
+
+#include "../../include/tortoise_gpu.h"
+#include "tg_mesh.hpp"
+
+struct tg_tuple_ctx {
+    int dim = 2;
+    int32_t nsec[3] = {1, 1, 1};
+    double origin[3] = {0, 0, 0};
+    double extent[3] = {1, 1, 1};
+    int64_t n = 0, nel = 0;
+    double vol = 0;
+    double omega0 = 0, amplitude = 1;
+    int32_t nu = 1, nmc = 50, rows = 4;
+    double min_det = 1e-9, threshold = 1e-9;
+    int32_t max_batch = 1;
+
+    double* d_eps = nullptr;     // nodal electron dispersion
+    double* d_pts = nullptr;     // MC point matrix [rows][nmc]
+    double minv[16] = {};        // LF inverse mass of one element (row-major NC x NC)
+    double* d_minv = nullptr;
+
+    int64_t ntup = 0;            // tuples (e0, e1, e2, uid)
+    int32_t* d_tup = nullptr;    // [ntup][4], ordered by (e0, e2, uid, e1)
+    int64_t* d_off0 = nullptr;   // tuples of element e as leg 0: [off0[e], off0[e+1])
+    int64_t* d_off2 = nullptr;   // ... as leg 2, positions in the leg-2 list
+    int32_t* d_oth0 = nullptr;   // leg-0 list: the tuple's e2
+    int32_t* d_oth2 = nullptr;   // leg-2 list: the tuple's e0
+    int64_t* d_pos2 = nullptr;   // tuple -> position in the leg-2 list
+    double* d_q0 = nullptr;      // leg-0 moments, [nq][ntup]
+    double* d_q2 = nullptr;      // leg-2 moments, [nq][ntup]
+    bool have_points = false;
+
+    double* d_f = nullptr;       // staging [max_batch][nel][dim+1]
+    double* d_out = nullptr;
+    double* d_bath = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    bool time_kernel = false;
+
+    hipStream_t stream = nullptr;
+    int device = 0;
+    char err[512] = {0};
+};
+
+namespace {
+
+constexpr int kEnumThreads = 256;
+constexpr int kApplyThreads = 256;
+constexpr int kApplyMembers = 8;       // members per pass over the moments
+
+tg_status fail(tg_tuple_ctx* c, tg_status code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+tg_status fail(tg_tuple_ctx* c, tg_status code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define TT_HIP(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail((ctx), TG_ERR_HIP, "%s failed: %s (%s:%d)", #call,                 \
+                        hipGetErrorString(_e), __FILE__, __LINE__);                        \
+    } while (0)
+
+template <typename T>
+tg_status dalloc(tg_tuple_ctx* c, T** p, size_t count) {
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, sizeof(T) * count) != hipSuccess) {
+        *p = nullptr;
+        return fail(c, TG_ERR_NOMEM, "hipMalloc of %zu bytes failed", sizeof(T) * count);
+    }
+    return TG_OK;
+}
+
+// ------------------------------------------------------------- geometry
+struct Geo {
+    int32_t nsec[3];
+    double origin[3], extent[3];
+    double omega0, vol;
+    int32_t nu, nuv, nmc, rows;
+    const double* eps;
+    const double* pts;
+};
+
+// Everything from here to the moment kernel decides which tuples and which MC
+// points count, so it is evaluated without contraction and in the operation
+// order of the CPU restatement.
+#pragma clang fp contract(off)
+
+template <int DIM>
+__host__ __device__ inline double node_coord(const Geo& g, int64_t node, int d) {
+    int64_t r = node;
+    for (int q = 0; q < d; ++q) r /= (g.nsec[q] + 1);
+    int64_t id = r % (g.nsec[d] + 1);
+    return g.origin[d] + (g.extent[d] / (double)g.nsec[d]) * (double)id;
+}
+
+template <int DIM>
+__device__ inline void elem_nodes(const Geo& g, int64_t e, int64_t v[DIM + 1]) {
+    tg::MeshDims md{DIM, {g.nsec[0], g.nsec[1], g.nsec[2]}};
+    int64_t w[4];
+    tg::element_nodes(md, e, w);
+#pragma unroll
+    for (int a = 0; a <= DIM; ++a) v[a] = w[a];
+}
+
+// Linear forms of the momentum coordinates and the energy on element e:
+// kf[d][0..DIM] = [k_d(v0), k_d(v1) - k_d(v0), ...] (ScatteringChannel.cpp:285-301).
+template <int DIM>
+__device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double kf[DIM][DIM + 1],
+                                  double ef[DIM + 1]) {
+    int64_t v[DIM + 1];
+    elem_nodes<DIM>(g, e, v);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        double k0 = node_coord<DIM>(g, v[0], d);
+        kf[d][0] = k0;
+#pragma unroll
+        for (int a = 1; a <= DIM; ++a) kf[d][a] = node_coord<DIM>(g, v[a], d) - k0;
+    }
+    if (phonon) {
+        ef[0] = g.omega0;
+#pragma unroll
+        for (int a = 1; a <= DIM; ++a) ef[a] = 0.0;
+    } else {
+        double e0 = g.eps[v[0]];
+        ef[0] = e0;
+#pragma unroll
+        for (int a = 1; a <= DIM; ++a) ef[a] = g.eps[v[a]] - e0;
+    }
+}
+
+template <int DIM>
+__host__ __device__ inline void umklapp_vec(int32_t nu, int32_t uid, int u[3]) {
+    int span = 2 * nu + 1;
+    u[0] = u[1] = u[2] = 0;
+#pragma unroll
+    for (int d = DIM - 1; d >= 0; --d) { u[d] = uid % span - nu; uid /= span; }
+}
+
+// D ((DIM+1) x 3DIM) and N of the tuple {e0 in, e1 in, e2 out}
+// (ScatteringChannel.cpp:285-352).
+template <int DIM>
+__device__ inline void tuple_DN(const Geo& g, const int64_t el[3], const int u[3], double D[DIM + 1][3 * DIM],
+                                double N[DIM + 1]) {
+    const double dir[3] = {1.0, 1.0, -1.0};
+#pragma unroll
+    for (int r = 0; r <= DIM; ++r) N[r] = 0.0;
+#pragma unroll
+    for (int leg = 0; leg < 3; ++leg) {
+        double kf[DIM][DIM + 1], ef[DIM + 1];
+        elem_forms<DIM>(g, el[leg], leg == 1, kf, ef);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) D[d][leg * DIM + a] = dir[leg] * kf[d][1 + a];
+            N[d] -= dir[leg] * kf[d][0];
+        }
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) D[DIM][leg * DIM + a] = dir[leg] * ef[1 + a];
+        N[DIM] -= dir[leg] * ef[0];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) N[d] -= (double)u[d] * g.extent[d];
+}
+
+// Inverse of the NC x NC matrix by Gauss-Jordan with partial pivoting;
+// returns the determinant, 0 when singular.  Row swaps are done by selects
+// so the arrays stay in registers.
+template <int NC>
+__host__ __device__ inline double invert_small(const double A[NC][NC], double inv[NC][NC]) {
+    double a[NC][2 * NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) a[r][c] = c < NC ? A[r][c] : (c - NC == r ? 1.0 : 0.0);
+    double det = 1.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        int p = c;
+        double best = fabs(a[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < NC; ++r) {
+            double v = fabs(a[r][c]);
+            if (v > best) { best = v; p = r; }
+        }
+        if (best == 0.0) return 0.0;
+        if (p != c) {
+#pragma unroll
+            for (int r = c + 1; r < NC; ++r)
+                if (r == p)
+#pragma unroll
+                    for (int q = 0; q < 2 * NC; ++q) { double t = a[r][q]; a[r][q] = a[c][q]; a[c][q] = t; }
+            det = -det;
+        }
+        double piv = a[c][c];
+        det *= piv;
+#pragma unroll
+        for (int q = 0; q < 2 * NC; ++q) a[c][q] /= piv;
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            if (r == c) continue;
+            double m = a[r][c];
+#pragma unroll
+            for (int q = 0; q < 2 * NC; ++q) a[r][q] -= m * a[c][q];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) inv[r][c] = a[r][NC + c];
+    return det;
+}
+
+template <int DIM>
+__device__ inline double dc_inverse(const double D[DIM + 1][3 * DIM], double inv[DIM + 1][DIM + 1]) {
+    constexpr int NC = DIM + 1, C0 = 2 * DIM - 1;
+    double DC[NC][NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) DC[r][q] = D[r][C0 + q];
+    return invert_small<NC>(DC, inv);
+}
+
+// Legs' points of MC sample m (IntegratorCore.hpp:283-320) and the inside
+// tests of the two constrained legs.
+template <int DIM>
+__device__ inline bool shell_point(const Geo& g, const double D[DIM + 1][3 * DIM], const double N[DIM + 1],
+                                   const double inv[DIM + 1][DIM + 1], int32_t m, double K[3][DIM]) {
+    constexpr int NC = DIM + 1, C0 = 2 * DIM - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) K[0][d] = g.pts[(size_t)d * g.nmc + m];
+#pragma unroll
+    for (int d = 0; d < DIM - 1; ++d) K[1][d] = g.pts[(size_t)(g.rows - (DIM - 1) + d) * g.nmc + m];
+    double rhs[NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < C0; ++c) s += D[r][c] * K[c / DIM][c % DIM];
+        rhs[r] = s - N[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s += inv[r][c] * rhs[c];
+        if (r == 0) K[1][DIM - 1] = -s;
+        else K[2][r - 1] = -s;
+    }
+    bool inside = true;
+#pragma unroll
+    for (int leg = 1; leg < 3; ++leg) {
+        double sum = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { inside &= K[leg][d] > 0.0; sum += K[leg][d]; }
+        inside &= sum < 1.0;
+    }
+    return inside;
+}
+
+__host__ __device__ inline double shell_divisor(int dim, double det, int32_t nmc) {
+    double w = fabs(det);
+    w *= (double)nmc * (double)tg::factorial(dim - 1);
+    w *= (double)tg::factorial(dim);
+    return w;
+}
+
+// Screening (ScatteringChannel.cpp:474-503 with the channel's points): true
+// when |det D_C| >= min_det and hits * c^3 * prod(vol*dim!) / divisor exceeds
+// the threshold; the value grows with the hit count, so the first passing
+// count decides.
+template <int DIM>
+__device__ inline bool tuple_passes(const Geo& g, const double D[DIM + 1][3 * DIM], const double N[DIM + 1],
+                                    double min_det, double threshold) {
+    double inv[DIM + 1][DIM + 1];
+    double det = dc_inverse<DIM>(D, inv);
+    if (!(fabs(det) >= min_det)) return false;
+    double c = sqrt(2.0);
+    double cpow = c * c * c;
+    double amp = 1.0;
+    for (int leg = 0; leg < 3; ++leg) amp *= g.vol * (double)tg::factorial(DIM);
+    double w = shell_divisor(DIM, det, g.nmc);
+    int64_t hits = 0;
+    for (int32_t m = 0; m < g.nmc; ++m) {
+        double K[3][DIM];
+        if (!shell_point<DIM>(g, D, N, inv, m, K)) continue;
+        ++hits;
+        if ((double)hits * cpow * amp / w > threshold) return true;
+    }
+    return false;
+}
+
+template <int DIM>
+__device__ inline void elem_box_erange(const Geo& g, int64_t e, double lo[DIM], double hi[DIM], double* elo,
+                                       double* ehi) {
+    int64_t v[DIM + 1];
+    elem_nodes<DIM>(g, e, v);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { lo[d] = 1e300; hi[d] = -1e300; }
+    *elo = 1e300;
+    *ehi = -1e300;
+#pragma unroll
+    for (int a = 0; a <= DIM; ++a) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            double x = node_coord<DIM>(g, v[a], d);
+            lo[d] = x < lo[d] ? x : lo[d];
+            hi[d] = x > hi[d] ? x : hi[d];
+        }
+        double en = g.eps[v[a]];
+        *elo = en < *elo ? en : *elo;
+        *ehi = en > *ehi ? en : *ehi;
+    }
+}
+
+// Walks the tuples of the pair (e0, e2) in (uid, e1) order; calls emit(e1, uid)
+// for each kept one.  Returns the count.
+template <int DIM, typename Emit>
+__device__ inline int32_t pair_tuples(const Geo& g, int64_t e0, int64_t e2, double min_det, double threshold,
+                                      Emit emit) {
+    constexpr int NES = DIM == 2 ? 2 : 6;
+    double lo0[DIM], hi0[DIM], lo2[DIM], hi2[DIM], el0, eh0, el2, eh2;
+    elem_box_erange<DIM>(g, e0, lo0, hi0, &el0, &eh0);
+    elem_box_erange<DIM>(g, e2, lo2, hi2, &el2, &eh2);
+    if (!(el0 + g.omega0 - eh2 <= 0.0 && eh0 + g.omega0 - el2 >= 0.0)) return 0;
+    int32_t cnt = 0;
+    for (int32_t uid = 0; uid < g.nuv; ++uid) {
+        int u[3];
+        umklapp_vec<DIM>(g.nu, uid, u);
+        double lo[DIM], hi[DIM];
+        int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
+        bool empty = false;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            lo[d] = lo2[d] - hi0[d] - u[d] * g.extent[d];
+            hi[d] = hi2[d] - lo0[d] - u[d] * g.extent[d];
+            double h = g.extent[d] / g.nsec[d];
+            double a = floor((lo[d] - g.origin[d]) / h), b = floor((hi[d] - g.origin[d]) / h);
+            empty |= b < 0.0 || a > g.nsec[d] - 1;
+            slo[d] = a < 0.0 ? 0 : (a > g.nsec[d] - 1 ? g.nsec[d] - 1 : (int)a);
+            shi[d] = b > g.nsec[d] - 1 ? g.nsec[d] - 1 : (b < 0.0 ? 0 : (int)b);
+        }
+        if (empty) continue;
+        int sc[3] = {slo[0], slo[1], slo[2]};
+        for (;;) {
+            int64_t sid = 0, stride = 1;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) { sid += stride * sc[d]; stride *= g.nsec[d]; }
+            for (int t = 0; t < NES; ++t) {
+                int64_t e1 = sid * NES + t;
+                double lo1[DIM], hi1[DIM], a1, b1;
+                elem_box_erange<DIM>(g, e1, lo1, hi1, &a1, &b1);
+                bool meet = true;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) meet &= lo1[d] <= hi[d] && hi1[d] >= lo[d];
+                if (!meet) continue;
+                int64_t el[3] = {e0, e1, e2};
+                double D[DIM + 1][3 * DIM], N[DIM + 1];
+                tuple_DN<DIM>(g, el, u, D, N);
+                if (!tuple_passes<DIM>(g, D, N, min_det, threshold)) continue;
+                emit(e1, uid);
+                ++cnt;
+            }
+            int d = 0;
+            while (d < DIM && ++sc[d] > shi[d]) { sc[d] = slo[d]; ++d; }
+            if (d == DIM) break;
+        }
+    }
+    return cnt;
+}
+
+__device__ inline int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t* total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kEnumThreads; off <<= 1) {
+        int64_t x = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += x;
+        __syncthreads();
+    }
+    int64_t incl = sh[tid];
+    *total = sh[kEnumThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// One workgroup per e0: tuple count of every e0 (write == false) or the
+// tuples themselves at base[e0] in (e2, uid, e1) order (write == true).
+template <int DIM>
+__global__ __launch_bounds__(kEnumThreads) void enumerate_kernel(Geo g, int64_t nel, double min_det,
+                                                                   double threshold, int64_t* counts,
+                                                                   const int64_t* base, int32_t* tup,
+                                                                   bool write) {
+    __shared__ int64_t sh[kEnumThreads];
+    const int64_t e0 = blockIdx.x;
+    int64_t pos = write ? base[e0] : 0;
+    int64_t total = 0;
+    for (int64_t c0 = 0; c0 < nel; c0 += kEnumThreads) {
+        const int64_t e2 = c0 + threadIdx.x;
+        int32_t cnt = 0;
+        if (e2 < nel) cnt = pair_tuples<DIM>(g, e0, e2, min_det, threshold, [](int64_t, int32_t) {});
+        if (!write) {
+            total += cnt;
+            continue;
+        }
+        int64_t chunk_total;
+        int64_t my = pos + block_exclusive_scan(cnt, sh, &chunk_total);
+        if (cnt > 0)
+            pair_tuples<DIM>(g, e0, e2, min_det, threshold, [&](int64_t e1, int32_t uid) {
+                tup[my * 4 + 0] = (int32_t)e0;
+                tup[my * 4 + 1] = (int32_t)e1;
+                tup[my * 4 + 2] = (int32_t)e2;
+                tup[my * 4 + 3] = uid;
+                ++my;
+            });
+        pos += chunk_total;
+    }
+    if (!write) {
+        int64_t sum_total;
+        (void)block_exclusive_scan(total, sh, &sum_total);
+        if (threadIdx.x == 0) counts[e0] = sum_total;
+    }
+}
+
+// Moments of one tuple (2D: 3 bases; 3D: 4).  q0 / q2 are the leg-0 and leg-2
+// lists, SoA [k][ntup] with k = (j*NC + a)*NC + b.  The MC points are the
+// same for every lane, so their loads are uniform.
+template <int DIM>
+__global__ __launch_bounds__(256) void moments_kernel(Geo g, const int32_t* __restrict__ tup, int64_t ntup,
+                                                        const int64_t* __restrict__ pos2,
+                                                        double* __restrict__ q0, double* __restrict__ q2) {
+    constexpr int NC = DIM + 1;
+    constexpr int NQ = NC * NC * NC;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntup) return;
+    const int64_t el[3] = {tup[t * 4], tup[t * 4 + 1], tup[t * 4 + 2]};
+    int u[3];
+    umklapp_vec<DIM>(g.nu, tup[t * 4 + 3], u);
+    double D[DIM + 1][3 * DIM], N[DIM + 1], inv[DIM + 1][DIM + 1];
+    tuple_DN<DIM>(g, el, u, D, N);
+    double det = dc_inverse<DIM>(D, inv);
+    double m0[NQ], m2[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) { m0[k] = 0.0; m2[k] = 0.0; }
+    for (int32_t m = 0; m < g.nmc; ++m) {
+        double K[3][DIM];
+        if (!shell_point<DIM>(g, D, N, inv, m, K)) continue;
+        double p0[NC], p2[NC];
+        p0[0] = 1.0;
+        p2[0] = 1.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { p0[1 + d] = K[0][d]; p2[1 + d] = K[2][d]; }
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                double pab = p0[a] * p2[b];
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    m0[(j * NC + a) * NC + b] += p0[j] * pab;
+                    m2[(j * NC + a) * NC + b] += p2[j] * pab;
+                }
+            }
+    }
+    double amp = 1.0;
+    for (int leg = 0; leg < 3; ++leg) amp *= g.vol * (double)tg::factorial(DIM);
+    double w = amp / shell_divisor(DIM, det, g.nmc);
+    const int64_t p = pos2[t];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        q0[(int64_t)k * ntup + t] = m0[k] * w;
+        q2[(int64_t)k * ntup + p] = m2[k] * w;
+    }
+}
+
+#pragma clang fp contract(on)
+
+// Leg-2 list: position of every tuple (stable counting sort on e2) and the
+// other element of each entry.
+__global__ void fill_leg_lists(const int32_t* __restrict__ tup, int64_t ntup, const int64_t* __restrict__ pos2,
+                               int32_t* __restrict__ oth0, int32_t* __restrict__ oth2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntup) return;
+    oth0[t] = tup[t * 4 + 2];
+    oth2[pos2[t]] = tup[t * 4];
+}
+
+// Sum of one element's entries of one leg list into acc[member][j].
+//   leg 0 (sign +1): c0 = own coefficients, c2 = the other element's;
+//   leg 2 (sign -1): c0 = the other's, c2 = own.
+//   G[a][b] = (1+n) d_a0 c2_b - n c0_a d_b0 - c0_a c2_b,  out_j += Q[j][a][b] G[a][b]
+template <int DIM, int LEG, int MB>
+__device__ inline void accumulate_leg(int64_t i0, int64_t i1, int64_t ntup, const int32_t* __restrict__ oth,
+                                      const double* __restrict__ q, const double* __restrict__ f, int64_t nel,
+                                      int mb, int nm, const double (*s_own)[DIM + 1], const double* s_n,
+                                      double acc[MB][DIM + 1]) {
+    constexpr int NC = DIM + 1;
+    constexpr int NQ = NC * NC * NC;
+    constexpr double sgn = LEG == 0 ? 1.0 : -1.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kApplyThreads) {
+        double Q[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) Q[k] = q[(int64_t)k * ntup + i];
+        const int64_t o = oth[i];
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            if (b >= nm) break;
+            double own[NC], other[NC];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                own[a] = s_own[b][a];
+                other[a] = f[((int64_t)(mb + b) * nel + o) * NC + a];
+            }
+            const double n = s_n[b];
+            double c0[NC], c2[NC];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                c0[a] = LEG == 0 ? own[a] : other[a];
+                c2[a] = LEG == 0 ? other[a] : own[a];
+            }
+            double G[NC][NC];
+#pragma unroll
+            for (int a = 0; a < NC; ++a)
+#pragma unroll
+                for (int bb = 0; bb < NC; ++bb) G[a][bb] = -c0[a] * c2[bb];
+#pragma unroll
+            for (int bb = 0; bb < NC; ++bb) G[0][bb] += (1.0 + n) * c2[bb];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) G[a][0] -= n * c0[a];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                double s = 0.0;
+#pragma unroll
+                for (int a = 0; a < NC; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < NC; ++bb) s += Q[(j * NC + a) * NC + bb] * G[a][bb];
+                acc[b][j] += sgn * s;
+            }
+        }
+    }
+}
+
+// Propagator: one workgroup per (element, member chunk).  Out coefficients of
+// element e for members [mb, mb+MB): amplitude * (sum of the element's leg-0
+// entries minus its leg-2 entries) times the inverse mass.
+template <int DIM>
+__global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
+    int64_t nel, int64_t ntup, const int64_t* __restrict__ off0, const int64_t* __restrict__ off2,
+    const int32_t* __restrict__ oth0, const int32_t* __restrict__ oth2, const double* __restrict__ q0,
+    const double* __restrict__ q2, const double* __restrict__ f, const double* __restrict__ bath,
+    int32_t batch, double amplitude, const double* __restrict__ minv_g, double* __restrict__ out) {
+    constexpr int NC = DIM + 1;
+    constexpr int MB = kApplyMembers;
+    const int64_t e = blockIdx.x;
+    const int mb = blockIdx.y * MB;
+    const int nm = min(MB, batch - mb);
+    __shared__ double s_own[MB][NC];
+    __shared__ double s_n[MB];
+    __shared__ double s_red[kApplyThreads / 64][MB * NC];
+    if (threadIdx.x < MB * NC) {
+        int b = threadIdx.x / NC, a = threadIdx.x % NC;
+        s_own[b][a] = b < nm ? f[((int64_t)(mb + b) * nel + e) * NC + a] : 0.0;
+    }
+    if (threadIdx.x < MB) s_n[threadIdx.x] = (int)threadIdx.x < nm ? bath[mb + threadIdx.x] : 0.0;
+    __syncthreads();
+    double acc[MB][NC];
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[b][j] = 0.0;
+    accumulate_leg<DIM, 0, MB>(off0[e], off0[e + 1], ntup, oth0, q0, f, nel, mb, nm, s_own, s_n, acc);
+    accumulate_leg<DIM, 2, MB>(off2[e], off2[e + 1], ntup, oth2, q2, f, nel, mb, nm, s_own, s_n, acc);
+    // fixed-order reduction: within waves by shuffles, then across waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            double v = acc[b][j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) s_red[wave][b * NC + j] = v;
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < nm * NC) {
+        const int b = threadIdx.x / NC, j = threadIdx.x % NC;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            double s = 0.0;
+            for (int w = 0; w < kApplyThreads / 64; ++w) s += s_red[w][b * NC + a];
+            v += s * minv_g[a * NC + j];
+        }
+        out[((int64_t)(mb + b) * nel + e) * NC + j] = amplitude * v;
+    }
+}
+
+Geo make_geo(const tg_tuple_ctx* c) {
+    Geo g{};
+    for (int d = 0; d < 3; ++d) {
+        g.nsec[d] = c->nsec[d];
+        g.origin[d] = c->origin[d];
+        g.extent[d] = c->extent[d];
+    }
+    g.omega0 = c->omega0;
+    g.vol = c->vol;
+    g.nu = c->nu;
+    g.nuv = 1;
+    for (int d = 0; d < c->dim; ++d) g.nuv *= 2 * c->nu + 1;
+    g.nmc = c->nmc;
+    g.rows = c->rows;
+    g.eps = c->d_eps;
+    g.pts = c->d_pts;
+    return g;
+}
+
+tg_status build_lists(tg_tuple_ctx* c) {
+    // leg-0 offsets from the (e0-ordered) tuples, leg-2 positions by a stable
+    // counting sort on e2 (host: the lists are built once per tuple list)
+    const int64_t nel = c->nel, T = c->ntup;
+    std::vector<int32_t> tup((size_t)T * 4);
+    if (T) TT_HIP(c, hipMemcpy(tup.data(), c->d_tup, sizeof(int32_t) * 4 * T, hipMemcpyDeviceToHost));
+    std::vector<int64_t> off0(nel + 1, 0), off2(nel + 1, 0), pos2(T > 0 ? T : 1);
+    for (int64_t t = 0; t < T; ++t) {
+        ++off0[tup[t * 4] + 1];
+        ++off2[tup[t * 4 + 2] + 1];
+    }
+    for (int64_t e = 0; e < nel; ++e) { off0[e + 1] += off0[e]; off2[e + 1] += off2[e]; }
+    std::vector<int64_t> fillp(off2.begin(), off2.end() - 1);
+    for (int64_t t = 0; t < T; ++t) pos2[t] = fillp[tup[t * 4 + 2]]++;
+    tg_status s;
+    if ((s = dalloc(c, &c->d_off0, nel + 1)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_off2, nel + 1)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_pos2, T)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_oth0, T)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_oth2, T)) != TG_OK) return s;
+    const int nq = (c->dim + 1) * (c->dim + 1) * (c->dim + 1);
+    if ((s = dalloc(c, &c->d_q0, (size_t)nq * T)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_q2, (size_t)nq * T)) != TG_OK) return s;
+    TT_HIP(c, hipMemcpy(c->d_off0, off0.data(), sizeof(int64_t) * (nel + 1), hipMemcpyHostToDevice));
+    TT_HIP(c, hipMemcpy(c->d_off2, off2.data(), sizeof(int64_t) * (nel + 1), hipMemcpyHostToDevice));
+    if (T) {
+        TT_HIP(c, hipMemcpy(c->d_pos2, pos2.data(), sizeof(int64_t) * T, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(fill_leg_lists, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_tup, T, c->d_pos2, c->d_oth0, c->d_oth2);
+        TT_HIP(c, hipGetLastError());
+    }
+    return TG_OK;
+}
+
+tg_status enumerate(tg_tuple_ctx* c) {
+    const int64_t nel = c->nel;
+    if (nel > 0x7fffffff) return fail(c, TG_ERR_ARG, "element count %lld exceeds int32", (long long)nel);
+    Geo g = make_geo(c);
+    int64_t* d_counts = nullptr;
+    int64_t* d_base = nullptr;
+    tg_status s;
+    if ((s = dalloc(c, &d_counts, nel)) != TG_OK) return s;
+    if ((s = dalloc(c, &d_base, nel)) != TG_OK) { (void)hipFree(d_counts); return s; }
+    auto cleanup = [&]() { (void)hipFree(d_counts); (void)hipFree(d_base); };
+    auto launch_enum = [&](const int64_t* b, int32_t* tp, bool write) {
+        if (c->dim == 2)
+            hipLaunchKernelGGL(enumerate_kernel<2>, dim3((unsigned)nel), dim3(kEnumThreads), 0, c->stream, g,
+                               nel, c->min_det, c->threshold, d_counts, b, tp, write);
+        else
+            hipLaunchKernelGGL(enumerate_kernel<3>, dim3((unsigned)nel), dim3(kEnumThreads), 0, c->stream, g,
+                               nel, c->min_det, c->threshold, d_counts, b, tp, write);
+    };
+    launch_enum(nullptr, nullptr, false);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { cleanup(); return fail(c, TG_ERR_HIP, "tuple count: %s", hipGetErrorString(e)); }
+    std::vector<int64_t> counts(nel), base(nel);
+    e = hipMemcpy(counts.data(), d_counts, sizeof(int64_t) * nel, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { cleanup(); return fail(c, TG_ERR_HIP, "tuple count copy: %s", hipGetErrorString(e)); }
+    int64_t total = 0;
+    for (int64_t i = 0; i < nel; ++i) { base[i] = total; total += counts[i]; }
+    if (total > ((int64_t)1 << 40)) { cleanup(); return fail(c, TG_ERR_ARG, "%lld tuples", (long long)total); }
+    if ((s = dalloc(c, &c->d_tup, (size_t)total * 4)) != TG_OK) { cleanup(); return s; }
+    e = hipMemcpy(d_base, base.data(), sizeof(int64_t) * nel, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_enum(d_base, c->d_tup, true);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, "tuple fill: %s", hipGetErrorString(e));
+    c->ntup = total;
+    return build_lists(c);
+}
+
+tg_status moments(tg_tuple_ctx* c) {
+    if (c->ntup == 0) return TG_OK;
+    Geo g = make_geo(c);
+    dim3 grid((unsigned)((c->ntup + 255) / 256));
+    if (c->dim == 2)
+        hipLaunchKernelGGL(moments_kernel<2>, grid, dim3(256), 0, c->stream, g, c->d_tup, c->ntup, c->d_pos2,
+                           c->d_q0, c->d_q2);
+    else
+        hipLaunchKernelGGL(moments_kernel<3>, grid, dim3(256), 0, c->stream, g, c->d_tup, c->ntup, c->d_pos2,
+                           c->d_q0, c->d_q2);
+    TT_HIP(c, hipGetLastError());
+    TT_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_bath, double* d_out,
+                           int32_t batch) {
+    dim3 grid((unsigned)c->nel, (unsigned)((batch + kApplyMembers - 1) / kApplyMembers));
+    if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (c->dim == 2)
+        hipLaunchKernelGGL(propagate_kernel<2>, grid, dim3(kApplyThreads), 0, c->stream, c->nel, c->ntup,
+                           c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_f, d_bath, batch,
+                           c->amplitude, c->d_minv, d_out);
+    else
+        hipLaunchKernelGGL(propagate_kernel<3>, grid, dim3(kApplyThreads), 0, c->stream, c->nel, c->ntup,
+                           c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_f, d_bath, batch,
+                           c->amplitude, c->d_minv, d_out);
+    TT_HIP(c, hipGetLastError());
+    if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
+    return TG_OK;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+extern "C" const char* tg_tuple_last_error(const tg_tuple_ctx* c) { return c ? c->err : "null context"; }
+
+extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
+    if (!c) return;
+    void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
+                    c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out) {
+    if (!out) return TG_ERR_ARG;
+    *out = nullptr;
+    if (!d) return TG_ERR_ARG;
+    tg_tuple_ctx* c = new (std::nothrow) tg_tuple_ctx();
+    if (!c) return TG_ERR_NOMEM;
+    auto bail = [&](tg_status s) { *out = c; return s; };   // caller reads the message, then destroys
+    if (d->dim != 2 && d->dim != 3) return bail(fail(c, TG_ERR_ARG, "dim %d: the tuple operator covers 2D and 3D", d->dim));
+    c->dim = d->dim;
+    c->n = 1;
+    c->nel = tg::factorial(d->dim);
+    for (int q = 0; q < 3; ++q) {
+        c->nsec[q] = q < d->dim ? d->nsec[q] : 1;
+        c->origin[q] = q < d->dim ? d->origin[q] : 0.0;
+        c->extent[q] = q < d->dim ? d->extent[q] : 1.0;
+    }
+    for (int q = 0; q < d->dim; ++q) {
+        if (d->nsec[q] < 1 || !(d->extent[q] > 0.0))
+            return bail(fail(c, TG_ERR_ARG, "axis %d: nsec %d, extent %g", q, d->nsec[q], d->extent[q]));
+        c->n *= d->nsec[q] + 1;
+        c->nel *= d->nsec[q];
+    }
+    if (!d->eps) return bail(fail(c, TG_ERR_ARG, "eps is null"));
+    if (d->num_mc < 1 || d->num_mc > (1 << 20)) return bail(fail(c, TG_ERR_ARG, "num_mc %d outside [1, 2^20]", d->num_mc));
+    if (d->n_umklapp < 0 || d->n_umklapp > 3) return bail(fail(c, TG_ERR_ARG, "n_umklapp %d outside [0, 3]", d->n_umklapp));
+    if (d->max_batch < 1) return bail(fail(c, TG_ERR_ARG, "max_batch %d < 1", d->max_batch));
+    c->omega0 = d->omega0;
+    c->amplitude = d->amplitude;
+    c->nu = d->n_umklapp;
+    c->nmc = d->num_mc;
+    c->rows = std::max(d->dim * 2 - 1, 2 * d->dim);
+    c->min_det = d->min_det;
+    c->threshold = d->threshold;
+    c->max_batch = d->max_batch;
+    {
+        double v = 1.0;
+        for (int q = 0; q < d->dim; ++q) v *= d->extent[q] / (double)d->nsec[q];
+        c->vol = std::fabs(v / (double)tg::factorial(d->dim));
+    }
+    // LF inverse mass (Mesh.cpp:95-123): M_ab = vol * dim! * int_ref phi_a phi_b
+    {
+        const int nc = d->dim + 1;
+        double M[4][4] = {}, Mi[4][4] = {};
+        const double f1 = 1.0 / tg::factorial(d->dim + 1), f2 = 1.0 / tg::factorial(d->dim + 2);
+        const double scale = c->vol * tg::factorial(d->dim);
+        for (int a = 0; a < nc; ++a)
+            for (int b = 0; b < nc; ++b) {
+                double m = (a == 0 && b == 0) ? 1.0 / tg::factorial(d->dim) : (a == 0 || b == 0) ? f1 : (a == b ? 2.0 * f2 : f2);
+                M[a][b] = scale * m;
+            }
+        if (nc == 3) {
+            double A[3][3], I3[3][3];
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) A[a][b] = M[a][b];
+            invert_small<3>(A, I3);
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) Mi[a][b] = I3[a][b];
+        } else {
+            invert_small<4>(M, Mi);
+        }
+        for (int a = 0; a < nc; ++a)
+            for (int b = 0; b < nc; ++b) c->minv[a * nc + b] = Mi[a][b];
+    }
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "stream/event: %s", hipGetErrorString(e)));
+    for (int64_t i = 0; i < c->n; ++i)
+        if (!std::isfinite(d->eps[i])) return bail(fail(c, TG_ERR_NONFINITE, "eps[%lld] is not finite", (long long)i));
+    const int nc = d->dim + 1;
+    tg_status s;
+    if ((s = dalloc(c, &c->d_eps, c->n)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_pts, (size_t)c->rows * c->nmc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_f, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_out, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_minv, 16)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_bath, c->max_batch)) != TG_OK) return bail(s);
+    e = hipMemcpy(c->d_eps, d->eps, sizeof(double) * c->n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_minv, c->minv, sizeof(double) * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    *out = c;
+    return TG_OK;
+}
+
+extern "C" int32_t tg_tuple_point_rows(int32_t dim, int32_t nlegs) {
+    return std::max(dim * (nlegs - 1) - 1, 2 * dim);
+}
+
+extern "C" tg_status tg_tuple_set_points(tg_tuple_ctx* c, const double* pts, int32_t flags) {
+    if (!c || !pts) return TG_ERR_ARG;
+    (void)flags;
+    for (int64_t i = 0; i < (int64_t)c->rows * c->nmc; ++i)
+        if (!std::isfinite(pts[i])) return fail(c, TG_ERR_NONFINITE, "pts[%lld] is not finite", (long long)i);
+    TT_HIP(c, hipMemcpy(c->d_pts, pts, sizeof(double) * c->rows * c->nmc, hipMemcpyHostToDevice));
+    tg_status s;
+    if (!c->d_tup && (s = enumerate(c)) != TG_OK) return s;
+    if ((s = moments(c)) != TG_OK) return s;
+    c->have_points = true;
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_set_indices(tg_tuple_ctx* c, const int32_t* idx, int64_t ntup) {
+    if (!c || (ntup > 0 && !idx) || ntup < 0) return TG_ERR_ARG;
+    int32_t nuv = 1;
+    for (int d = 0; d < c->dim; ++d) nuv *= 2 * c->nu + 1;
+    for (int64_t t = 0; t < ntup; ++t) {
+        for (int l = 0; l < 3; ++l)
+            if (idx[t * 4 + l] < 0 || idx[t * 4 + l] >= c->nel)
+                return fail(c, TG_ERR_ARG, "tuple %lld: element %d outside [0, %lld)", (long long)t, idx[t * 4 + l],
+                            (long long)c->nel);
+        if (idx[t * 4 + 3] < 0 || idx[t * 4 + 3] >= nuv)
+            return fail(c, TG_ERR_ARG, "tuple %lld: umklapp id %d outside [0, %d)", (long long)t, idx[t * 4 + 3], nuv);
+        if (t > 0 && idx[t * 4] < idx[(t - 1) * 4])
+            return fail(c, TG_ERR_ARG, "tuples must be ordered by their leg-0 element (tuple %lld)", (long long)t);
+    }
+    tg_status s;
+    if ((s = dalloc(c, &c->d_tup, (size_t)ntup * 4)) != TG_OK) return s;
+    if (ntup) TT_HIP(c, hipMemcpy(c->d_tup, idx, sizeof(int32_t) * 4 * ntup, hipMemcpyHostToDevice));
+    c->ntup = ntup;
+    if ((s = build_lists(c)) != TG_OK) return s;
+    if (c->have_points && (s = moments(c)) != TG_OK) return s;
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_count(const tg_tuple_ctx* c, int64_t* ntup) {
+    if (!c || !ntup) return TG_ERR_ARG;
+    *ntup = c->d_tup ? c->ntup : -1;
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_copy_indices(tg_tuple_ctx* c, int32_t* idx, int64_t cap) {
+    if (!c || !idx) return TG_ERR_ARG;
+    if (!c->d_tup) return fail(c, TG_ERR_STATE, "no tuple list yet (call tg_tuple_set_points)");
+    if (cap < c->ntup) return fail(c, TG_ERR_ARG, "cap %lld < %lld tuples", (long long)cap, (long long)c->ntup);
+    if (c->ntup) TT_HIP(c, hipMemcpy(idx, c->d_tup, sizeof(int32_t) * 4 * c->ntup, hipMemcpyDeviceToHost));
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_set_amplitude(tg_tuple_ctx* c, double amplitude) {
+    if (!c || !std::isfinite(amplitude)) return TG_ERR_ARG;
+    c->amplitude = amplitude;
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_propagate(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out,
+                                        int64_t nel, int32_t batch, int32_t flags) {
+    if (!c || !f || !bath_n || !out) return TG_ERR_ARG;
+    if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
+    if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
+    if (batch < 1 || batch > c->max_batch)
+        return fail(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
+    c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
+    const size_t len = (size_t)c->nel * (c->dim + 1) * batch;
+    if (flags & TG_PTR_DEVICE) {
+        tg_status s = propagate_device(c, f, bath_n, out, batch);
+        if (s != TG_OK) return s;
+    } else {
+        TT_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
+        TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
+        tg_status s = propagate_device(c, c->d_f, c->d_bath, c->d_out, batch);
+        if (s != TG_OK) return s;
+        TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+    }
+    TT_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->time_kernel) TT_HIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_last_kernel_ms(tg_tuple_ctx* c, double* ms) {
+    if (!c || !ms) return TG_ERR_ARG;
+    *ms = c->last_ms;
+    return TG_OK;
+}
+
+extern "C" void* tg_tuple_stream(const tg_tuple_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// .scattIndices file of a 3-leg channel (ScatteringChannel.cpp:422-459,520-537):
+// for each leg and axis the int nSecSplits, an unsigned long tuple count,
+// then per tuple the three element ids and the umklapp id, all native-endian.
+extern "C" tg_status tg_tuple_save_indices(tg_tuple_ctx* c, const char* path) {
+    if (!c || !path) return TG_ERR_ARG;
+    if (!c->d_tup) return fail(c, TG_ERR_STATE, "no tuple list yet");
+    std::vector<int32_t> idx((size_t)c->ntup * 4);
+    if (c->ntup) TT_HIP(c, hipMemcpy(idx.data(), c->d_tup, sizeof(int32_t) * 4 * c->ntup, hipMemcpyDeviceToHost));
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(c, TG_ERR_ARG, "cannot open %s for writing", path);
+    bool ok = true;
+    for (int l = 0; l < 3; ++l)
+        for (int d = 0; d < c->dim; ++d) ok &= std::fwrite(&c->nsec[d], sizeof(int32_t), 1, fp) == 1;
+    unsigned long total = (unsigned long)c->ntup;
+    ok &= std::fwrite(&total, sizeof(unsigned long), 1, fp) == 1;
+    if (c->ntup) ok &= std::fwrite(idx.data(), sizeof(int32_t) * 4, (size_t)c->ntup, fp) == (size_t)c->ntup;
+    ok &= std::fclose(fp) == 0;
+    return ok ? TG_OK : fail(c, TG_ERR_ARG, "short write to %s", path);
+}
+
+extern "C" tg_status tg_tuple_load_indices(tg_tuple_ctx* c, const char* path) {
+    if (!c || !path) return TG_ERR_ARG;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(c, TG_ERR_ARG, "cannot open %s", path);
+    bool same = true;
+    for (int l = 0; l < 3; ++l)
+        for (int d = 0; d < c->dim; ++d) {
+            int32_t ns = 0;
+            same &= std::fread(&ns, sizeof(int32_t), 1, fp) == 1 && ns == c->nsec[d];
+        }
+    unsigned long total = 0;
+    if (!same || std::fread(&total, sizeof(unsigned long), 1, fp) != 1) {
+        std::fclose(fp);
+        return fail(c, TG_ERR_ARG, "%s: mesh sections differ from this channel's", path);
+    }
+    std::vector<int32_t> idx((size_t)total * 4);
+    bool ok = total == 0 || std::fread(idx.data(), sizeof(int32_t) * 4, total, fp) == total;
+    std::fclose(fp);
+    if (!ok) return fail(c, TG_ERR_ARG, "%s: truncated tuple list", path);
+    return tg_tuple_set_indices(c, idx.data(), (int64_t)total);
+}

@@ -169,6 +169,106 @@ def cube_ee(steps=5, t_final=1.0):
             "number_drift_rel": drift}
 
 
+def reference_operator(budget_s=12.0):
+    """The reference's own operator (tuple list + MC delta-shell integral,
+    NMC = 50, umklapp {-1,0,1}^2) on the square zone: 32x32 sections (the
+    reference's CPU-runnable case) and 128x128 sections (the mesh of the
+    headline workload).  Timed: tuple enumeration, per-tuple moments,
+    propagate for 1 and 8 members; the CPU restatement of the reference
+    algorithm (per stage: every tuple's MC integral, OpenMP) at 32x32 on the
+    host cores; parity against it."""
+    import numpy as np
+    import torch
+
+    import oracle
+    import tortoise_amd as tg
+    from tortoise_amd import workloads as wl
+
+    res = {}
+    for ns in (32, 128):
+        w = wl.Workload(f"square{ns}", 2, (ns, ns))
+        pts = oracle.mc_points(2, 3, 50, 0)
+        op = tg.TupleOperator.create(w, max_batch=8)
+        t0 = time.perf_counter()
+        op.set_points(pts)
+        t_first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        op.set_points(oracle.mc_points(2, 3, 50, 1))
+        t_mom = time.perf_counter() - t0
+        T = op.count()
+        base = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+        line = {"sections": ns, "elements": op.nel, "tuples": T, "enumerate_and_moments_s": t_first,
+                "moments_s": t_mom}
+        stream = torch.cuda.ExternalStream(op.stream())
+        for B in (1, 8):
+            F = torch.tensor(np.stack([base] * B), device="cuda")
+            bath = torch.full((B,), w.bath_n, dtype=torch.float64, device="cuda")
+            out = torch.empty_like(F)
+            torch.cuda.synchronize()
+
+            def call(tk=False):
+                op.propagate_device(F.data_ptr(), bath.data_ptr(), out.data_ptr(), B, time_kernel=tk)
+            for _ in range(3):
+                call()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            reps = 20 if ns == 32 else 5
+            e0.record(stream)
+            for _ in range(reps):
+                call()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            kms = 0.0
+            for _ in range(reps):
+                call(True)
+                kms += op.last_kernel_ms()
+            kms /= reps
+            # compulsory bytes of one launch: both moment lists (27 doubles an
+            # entry, two entries a tuple) read once per 8-member chunk, the
+            # other-element indices, the states in and out
+            byts = T * 2 * (27 * 8 + 4) + 2 * B * op.nel * 3 * 8
+            line[f"b{B}"] = {"ms": ms, "evals_per_s": B * 1000.0 / ms, "kernel_ms": kms,
+                             "kernel_gbs": byts / (kms * 1e-3) / 1e9,
+                             "frac_hbm": byts / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            if ns == 32 and B == 1:
+                Fh = F[0].cpu().numpy()
+                Ig = out[0].cpu().numpy()
+        _log(f"tuple operator {ns}x{ns}: {T} tuples, enumerate+moments {t_first:.2f} s, moments "
+             f"{t_mom * 1e3:.1f} ms; propagate B=1 {line['b1']['ms']:.3f} ms "
+             f"({line['b1']['kernel_gbs']:.0f} GB/s), B=8 {line['b8']['ms']:.3f} ms "
+             f"({line['b8']['evals_per_s']:.0f} evals/s)")
+        if ns == 32:
+            idx = op.indices()
+            fph = np.zeros_like(Fh)
+            fph[:, 0] = w.bath_n
+            pts_now = oracle.mc_points(2, 3, 50, 1)       # the points the moments hold
+            args = (2, w.nsec, w.origin, w.extent, w.eps(), w.omega0, w.g ** 2, 1, pts_now, idx, Fh,
+                    fph)
+            cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            ref = oracle.mc_apply_eph(*args, nthreads=cores)
+            line["max_rel_diff_vs_cpu"] = float(np.abs(Ig - ref).max() / np.abs(ref).max())
+            times = []
+            t_start = time.perf_counter()
+            while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 20):
+                t0 = time.perf_counter()
+                oracle.mc_apply_eph(*args, nthreads=cores)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            line["cpu_reference_algorithm"] = {
+                "value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
+                "sample": f"CPU restatement of ScatteringChannel::propagate (every tuple's MC "
+                          f"integral per stage, {T} tuples, NMC 50), one member, median of "
+                          f"{len(times)} ({sum(times):.1f} s)"}
+            line["gpu_vs_cpu_b8"] = line["b8"]["evals_per_s"] * med
+            _log(f"  CPU reference algorithm {med:.3f} s/eval on {cores} threads; GPU/CPU "
+                 f"{line['gpu_vs_cpu_b8']:.0f}x (8 members); max rel diff {line['max_rel_diff_vs_cpu']:.1e}")
+        op.close()
+        res[f"square{ns}"] = line
+    return res
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes (the known-size reference for the
     FETCH_SIZE / WRITE_SIZE counters)."""
@@ -202,6 +302,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dp5", action="store_true", help="skip the full DP5 runs")
+    ap.add_argument("--no-reference-op", action="store_true",
+                    help="skip the reference tuple operator measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -330,6 +432,8 @@ def main():
     ctx.close()
     if not args.no_dp5:
         out["cube20_ee"] = cube_ee()
+    if not args.no_reference_op:
+        out["reference_operator"] = reference_operator()
     print(json.dumps(out), flush=True)
 
 

@@ -61,6 +61,7 @@ struct tg_tuple_ctx {
 
     double* d_eps = nullptr;     // nodal electron dispersion
     double* d_pts = nullptr;     // MC point matrix [rows][nmc]
+    double* d_etab = nullptr;    // element table (forms, boxes, energy ranges)
     double minv[16] = {};        // LF inverse mass of one element (row-major NC x NC)
     double* d_minv = nullptr;
 
@@ -76,6 +77,7 @@ struct tg_tuple_ctx {
     bool have_points = false;
 
     double* d_f = nullptr;       // staging [max_batch][nel][dim+1]
+    double* d_ft = nullptr;      // element-major copy of the states [nel][batch][dim+1]
     double* d_out = nullptr;
     double* d_bath = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -91,7 +93,7 @@ namespace {
 
 constexpr int kEnumThreads = 256;
 constexpr int kApplyThreads = 256;
-constexpr int kApplyMembers = 8;       // members per pass over the moments
+constexpr int kApplyMembers = 8;       // members per workgroup when batch > 1
 
 tg_status fail(tg_tuple_ctx* c, tg_status code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 tg_status fail(tg_tuple_ctx* c, tg_status code, const char* fmt, ...) {
@@ -131,7 +133,13 @@ struct Geo {
     int32_t nu, nuv, nmc, rows;
     const double* eps;
     const double* pts;
+    const double* etab;   // per element: electron kf, ef, box lo, box hi, energy range
 };
+
+// Element table record: DIM*(DIM+1) momentum form coefficients, DIM+1 energy
+// form coefficients, DIM box minima, DIM box maxima, energy min and max.
+template <int DIM>
+constexpr int etab_stride() { return DIM * (DIM + 1) + (DIM + 1) + 2 * DIM + 2; }
 
 // Everything from here to the moment kernel decides which tuples and which MC
 // points count, so it is evaluated without contraction and in the operation
@@ -158,8 +166,8 @@ __device__ inline void elem_nodes(const Geo& g, int64_t e, int64_t v[DIM + 1]) {
 // Linear forms of the momentum coordinates and the energy on element e:
 // kf[d][0..DIM] = [k_d(v0), k_d(v1) - k_d(v0), ...] (ScatteringChannel.cpp:285-301).
 template <int DIM>
-__device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double kf[DIM][DIM + 1],
-                                  double ef[DIM + 1]) {
+__device__ inline void elem_forms_compute(const Geo& g, int64_t e, double kf[DIM][DIM + 1],
+                                          double ef[DIM + 1]) {
     int64_t v[DIM + 1];
     elem_nodes<DIM>(g, e, v);
 #pragma unroll
@@ -169,16 +177,24 @@ __device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double k
 #pragma unroll
         for (int a = 1; a <= DIM; ++a) kf[d][a] = node_coord<DIM>(g, v[a], d) - k0;
     }
-    if (phonon) {
-        ef[0] = g.omega0;
+    double e0 = g.eps[v[0]];
+    ef[0] = e0;
 #pragma unroll
-        for (int a = 1; a <= DIM; ++a) ef[a] = 0.0;
-    } else {
-        double e0 = g.eps[v[0]];
-        ef[0] = e0;
+    for (int a = 1; a <= DIM; ++a) ef[a] = g.eps[v[a]] - e0;
+}
+
+// Forms of element e read back from the table; the phonon band has the same
+// momentum forms and the constant energy omega0.
+template <int DIM>
+__device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double kf[DIM][DIM + 1],
+                                  double ef[DIM + 1]) {
+    const double* r = g.etab + e * etab_stride<DIM>();
 #pragma unroll
-        for (int a = 1; a <= DIM; ++a) ef[a] = g.eps[v[a]] - e0;
-    }
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int a = 0; a <= DIM; ++a) kf[d][a] = r[d * (DIM + 1) + a];
+#pragma unroll
+    for (int a = 0; a <= DIM; ++a) ef[a] = phonon ? (a == 0 ? g.omega0 : 0.0) : r[DIM * (DIM + 1) + a];
 }
 
 template <int DIM>
@@ -346,12 +362,39 @@ __device__ inline bool tuple_passes(const Geo& g, const double D[DIM + 1][3 * DI
 template <int DIM>
 __device__ inline void elem_box_erange(const Geo& g, int64_t e, double lo[DIM], double hi[DIM], double* elo,
                                        double* ehi) {
+    const double* r = g.etab + e * etab_stride<DIM>() + DIM * (DIM + 1) + (DIM + 1);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { lo[d] = r[d]; hi[d] = r[DIM + d]; }
+    *elo = r[2 * DIM];
+    *ehi = r[2 * DIM + 1];
+}
+
+template <int DIM>
+__device__ inline void elem_erange(const Geo& g, int64_t e, double* elo, double* ehi) {
+    const double* r = g.etab + e * etab_stride<DIM>() + DIM * (DIM + 1) + (DIM + 1) + 2 * DIM;
+    *elo = r[0];
+    *ehi = r[1];
+}
+
+// One thread per element: the table record (forms, bounding box, energy range).
+template <int DIM>
+__global__ void elem_table_kernel(Geo g, int64_t nel, double* __restrict__ tab) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nel) return;
+    double kf[DIM][DIM + 1], ef[DIM + 1];
+    elem_forms_compute<DIM>(g, e, kf, ef);
+    double* r = tab + e * etab_stride<DIM>();
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int a = 0; a <= DIM; ++a) r[d * (DIM + 1) + a] = kf[d][a];
+#pragma unroll
+    for (int a = 0; a <= DIM; ++a) r[DIM * (DIM + 1) + a] = ef[a];
     int64_t v[DIM + 1];
     elem_nodes<DIM>(g, e, v);
+    double lo[DIM], hi[DIM], elo = 1e300, ehi = -1e300;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) { lo[d] = 1e300; hi[d] = -1e300; }
-    *elo = 1e300;
-    *ehi = -1e300;
 #pragma unroll
     for (int a = 0; a <= DIM; ++a) {
 #pragma unroll
@@ -361,9 +404,14 @@ __device__ inline void elem_box_erange(const Geo& g, int64_t e, double lo[DIM], 
             hi[d] = x > hi[d] ? x : hi[d];
         }
         double en = g.eps[v[a]];
-        *elo = en < *elo ? en : *elo;
-        *ehi = en > *ehi ? en : *ehi;
+        elo = en < elo ? en : elo;
+        ehi = en > ehi ? en : ehi;
     }
+    double* b = r + DIM * (DIM + 1) + (DIM + 1);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) { b[d] = lo[d]; b[DIM + d] = hi[d]; }
+    b[2 * DIM] = elo;
+    b[2 * DIM + 1] = ehi;
 }
 
 // Walks the tuples of the pair (e0, e2) in (uid, e1) order; calls emit(e1, uid)
@@ -373,9 +421,11 @@ __device__ inline int32_t pair_tuples(const Geo& g, int64_t e0, int64_t e2, doub
                                       Emit emit) {
     constexpr int NES = DIM == 2 ? 2 : 6;
     double lo0[DIM], hi0[DIM], lo2[DIM], hi2[DIM], el0, eh0, el2, eh2;
+    elem_erange<DIM>(g, e0, &el0, &eh0);
+    elem_erange<DIM>(g, e2, &el2, &eh2);
+    if (!(el0 + g.omega0 - eh2 <= 0.0 && eh0 + g.omega0 - el2 >= 0.0)) return 0;
     elem_box_erange<DIM>(g, e0, lo0, hi0, &el0, &eh0);
     elem_box_erange<DIM>(g, e2, lo2, hi2, &el2, &eh2);
-    if (!(el0 + g.omega0 - eh2 <= 0.0 && eh0 + g.omega0 - el2 >= 0.0)) return 0;
     int32_t cnt = 0;
     for (int32_t uid = 0; uid < g.nuv; ++uid) {
         int u[3];
@@ -539,115 +589,193 @@ __global__ void fill_leg_lists(const int32_t* __restrict__ tup, int64_t ntup, co
     oth2[pos2[t]] = tup[t * 4];
 }
 
-// Sum of one element's entries of one leg list into acc[member][j].
-//   leg 0 (sign +1): c0 = own coefficients, c2 = the other element's;
-//   leg 2 (sign -1): c0 = the other's, c2 = own.
-//   G[a][b] = (1+n) d_a0 c2_b - n c0_a d_b0 - c0_a c2_b,  out_j += Q[j][a][b] G[a][b]
-template <int DIM, int LEG, int MB>
-__device__ inline void accumulate_leg(int64_t i0, int64_t i1, int64_t ntup, const int32_t* __restrict__ oth,
-                                      const double* __restrict__ q, const double* __restrict__ f, int64_t nel,
-                                      int mb, int nm, const double (*s_own)[DIM + 1], const double* s_n,
-                                      double acc[MB][DIM + 1]) {
+// Propagator: one workgroup per (element, chunk of MB members).
+//
+// An element's entries (its leg-0 list, then its leg-2 list) are dealt to
+// SL slots in rounds: entry i of a list goes to slot (i - first) % SL, thread
+// h owns slot h and sums its entries in order for each member of the chunk.
+// The element's result is the sum of the slots in slot order, then
+// amplitude * Minv^T.  The order depends only on the lists, so a member of a
+// batch is bitwise equal to its single evaluation.  A round's moments are
+// staged in LDS (coalesced SoA reads, the next round's loads in flight during
+// the current round's arithmetic); in 2D each thread then keeps its entry's
+// moments in registers for all members of the chunk.  States are read from
+// an element-major copy ft[e][member][a], so the chunk's members of one
+// element are contiguous.  For an entry of the leg-0 list (sign +1)
+// c0 = own coefficients, c2 = the other element's; for the leg-2 list (sign
+// -1) the roles swap.  With
+//   G[a][b] = (1+n) d_a0 c2_b - n c0_a d_b0 - c0_a c2_b
+// the entry adds sign * sum_ab Q[j][a][b] G[a][b] to output basis j.
+// Explicit fma with contraction off: every chunk size rounds identically.
+#pragma clang fp contract(off)
+
+template <int DIM>
+constexpr int prop_slots() { return DIM == 2 ? 256 : 128; }
+
+template <int DIM>
+constexpr size_t prop_lds_bytes() {
+    constexpr int NC = DIM + 1, NQ = NC * NC * NC, SL = prop_slots<DIM>();
+    return sizeof(double) * (size_t)NQ * SL + sizeof(int32_t) * SL;
+}
+
+template <int DIM, typename QAt>
+__device__ inline void entry_terms(QAt qat, const double c0[DIM + 1], const double c2[DIM + 1], double n,
+                                   double np1, double out[DIM + 1]) {
     constexpr int NC = DIM + 1;
-    constexpr int NQ = NC * NC * NC;
-    constexpr double sgn = LEG == 0 ? 1.0 : -1.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += kApplyThreads) {
-        double Q[NQ];
+    double G[NC][NC];
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) Q[k] = q[(int64_t)k * ntup + i];
-        const int64_t o = oth[i];
+    for (int a = 0; a < NC; ++a)
 #pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            if (b >= nm) break;
-            double own[NC], other[NC];
+        for (int b = 0; b < NC; ++b) G[a][b] = -(c0[a] * c2[b]);
 #pragma unroll
-            for (int a = 0; a < NC; ++a) {
-                own[a] = s_own[b][a];
-                other[a] = f[((int64_t)(mb + b) * nel + o) * NC + a];
-            }
-            const double n = s_n[b];
-            double c0[NC], c2[NC];
+    for (int b = 0; b < NC; ++b) G[0][b] = fma(np1, c2[b], G[0][b]);
 #pragma unroll
-            for (int a = 0; a < NC; ++a) {
-                c0[a] = LEG == 0 ? own[a] : other[a];
-                c2[a] = LEG == 0 ? other[a] : own[a];
-            }
-            double G[NC][NC];
+    for (int a = 0; a < NC; ++a) G[a][0] = fma(-n, c0[a], G[a][0]);
 #pragma unroll
-            for (int a = 0; a < NC; ++a)
+    for (int j = 0; j < NC; ++j) {
+        double s = 0.0;
 #pragma unroll
-                for (int bb = 0; bb < NC; ++bb) G[a][bb] = -c0[a] * c2[bb];
+        for (int a = 0; a < NC; ++a)
 #pragma unroll
-            for (int bb = 0; bb < NC; ++bb) G[0][bb] += (1.0 + n) * c2[bb];
-#pragma unroll
-            for (int a = 0; a < NC; ++a) G[a][0] -= n * c0[a];
-#pragma unroll
-            for (int j = 0; j < NC; ++j) {
-                double s = 0.0;
-#pragma unroll
-                for (int a = 0; a < NC; ++a)
-#pragma unroll
-                    for (int bb = 0; bb < NC; ++bb) s += Q[(j * NC + a) * NC + bb] * G[a][bb];
-                acc[b][j] += sgn * s;
-            }
-        }
+            for (int b = 0; b < NC; ++b) s = fma(qat((j * NC + a) * NC + b), G[a][b], s);
+        out[j] = s;
     }
 }
 
-// Propagator: one workgroup per (element, member chunk).  Out coefficients of
-// element e for members [mb, mb+MB): amplitude * (sum of the element's leg-0
-// entries minus its leg-2 entries) times the inverse mass.
-template <int DIM>
+template <int DIM, int MB>
 __global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
     int64_t nel, int64_t ntup, const int64_t* __restrict__ off0, const int64_t* __restrict__ off2,
     const int32_t* __restrict__ oth0, const int32_t* __restrict__ oth2, const double* __restrict__ q0,
-    const double* __restrict__ q2, const double* __restrict__ f, const double* __restrict__ bath,
+    const double* __restrict__ q2, const double* __restrict__ ft, const double* __restrict__ bath,
     int32_t batch, double amplitude, const double* __restrict__ minv_g, double* __restrict__ out) {
     constexpr int NC = DIM + 1;
-    constexpr int MB = kApplyMembers;
+    constexpr int NQ = NC * NC * NC;
+    constexpr int SL = prop_slots<DIM>();
+    constexpr bool QREG = DIM == 2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* sQ = reinterpret_cast<double*>(smem_raw);                 // [NQ][SL]
+    int32_t* sO = reinterpret_cast<int32_t*>(sQ + (size_t)NQ * SL);   // [SL]
+    __shared__ double sOwn[MB][NC];
+    __shared__ double sN[MB];
+    __shared__ double sTot[MB][NC];
     const int64_t e = blockIdx.x;
     const int mb = blockIdx.y * MB;
     const int nm = min(MB, batch - mb);
-    __shared__ double s_own[MB][NC];
-    __shared__ double s_n[MB];
-    __shared__ double s_red[kApplyThreads / 64][MB * NC];
+    const int h = threadIdx.x;
+    const bool slot_thread = h < SL;
     if (threadIdx.x < MB * NC) {
-        int b = threadIdx.x / NC, a = threadIdx.x % NC;
-        s_own[b][a] = b < nm ? f[((int64_t)(mb + b) * nel + e) * NC + a] : 0.0;
+        const int b = threadIdx.x / NC, a = threadIdx.x % NC;
+        sOwn[b][a] = b < nm ? ft[((int64_t)e * batch + mb + b) * NC + a] : 0.0;
     }
-    if (threadIdx.x < MB) s_n[threadIdx.x] = (int)threadIdx.x < nm ? bath[mb + threadIdx.x] : 0.0;
-    __syncthreads();
+    if (threadIdx.x < MB) sN[threadIdx.x] = (int)threadIdx.x < nm ? bath[mb + threadIdx.x] : 0.0;
     double acc[MB][NC];
 #pragma unroll
     for (int b = 0; b < MB; ++b)
 #pragma unroll
-        for (int j = 0; j < NC; ++j) acc[b][j] = 0.0;
-    accumulate_leg<DIM, 0, MB>(off0[e], off0[e + 1], ntup, oth0, q0, f, nel, mb, nm, s_own, s_n, acc);
-    accumulate_leg<DIM, 2, MB>(off2[e], off2[e + 1], ntup, oth2, q2, f, nel, mb, nm, s_own, s_n, acc);
-    // fixed-order reduction: within waves by shuffles, then across waves
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int a = 0; a < NC; ++a) acc[b][a] = 0.0;
+    // rounds of SL entries: the leg-0 list, then the leg-2 list
+    constexpr int PER = NQ * SL / kApplyThreads;
+    static_assert(NQ * SL % kApplyThreads == 0, "staging must tile the workgroup");
+    const int64_t a0 = off0[e], len0 = off0[e + 1] - a0;
+    const int64_t a2 = off2[e], len2 = off2[e + 1] - a2;
+    const int64_t nr0 = (len0 + SL - 1) / SL, nr = nr0 + (len2 + SL - 1) / SL;
+    double stage[PER];
+    int32_t stage_o = 0;
+    auto fetch = [&](int64_t r) {
+        const bool l0 = r < nr0;
+        const int64_t base = l0 ? a0 + r * SL : a2 + (r - nr0) * SL;
+        const int cnt = (int)min((int64_t)SL, (l0 ? a0 + len0 : a2 + len2) - base);
+        const double* q = l0 ? q0 : q2;
 #pragma unroll
-    for (int b = 0; b < MB; ++b)
-#pragma unroll
-        for (int j = 0; j < NC; ++j) {
-            double v = acc[b][j];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-            if (lane == 0) s_red[wave][b * NC + j] = v;
+        for (int p = 0; p < PER; ++p) {
+            const int idx = threadIdx.x + p * kApplyThreads;
+            const int k = idx / SL, hh = idx % SL;
+            stage[p] = hh < cnt ? q[(int64_t)k * ntup + base + hh] : 0.0;
         }
-    __syncthreads();
-    if ((int)threadIdx.x < nm * NC) {
+        const int32_t* oth = l0 ? oth0 : oth2;
+        stage_o = (int)threadIdx.x < cnt ? oth[base + threadIdx.x] : 0;
+    };
+    if (nr > 0) fetch(0);
+    for (int64_t r = 0; r < nr; ++r) {
+        const bool l0 = r < nr0;
+        const int64_t base = l0 ? a0 + r * SL : a2 + (r - nr0) * SL;
+        const int cnt = (int)min((int64_t)SL, (l0 ? a0 + len0 : a2 + len2) - base);
+#pragma unroll
+        for (int p = 0; p < PER; ++p) sQ[threadIdx.x + p * kApplyThreads] = stage[p];
+        if (slot_thread) sO[threadIdx.x] = stage_o;
+        __syncthreads();
+        if (r + 1 < nr) fetch(r + 1);
+        if (slot_thread && h < cnt) {
+            const int64_t o = sO[h];
+            double Qr[QREG ? NQ : 1];
+            if constexpr (QREG) {
+#pragma unroll
+                for (int k = 0; k < NQ; ++k) Qr[k] = sQ[k * SL + h];
+            }
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                if (b < nm) {
+                    double own[NC], other[NC], c0[NC], c2[NC], t[NC];
+#pragma unroll
+                    for (int a = 0; a < NC; ++a) {
+                        own[a] = sOwn[b][a];
+                        other[a] = ft[((int64_t)o * batch + mb + b) * NC + a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < NC; ++a) {
+                        c0[a] = l0 ? own[a] : other[a];
+                        c2[a] = l0 ? other[a] : own[a];
+                    }
+                    if constexpr (QREG)
+                        entry_terms<DIM>([&](int k) { return Qr[k]; }, c0, c2, sN[b], 1.0 + sN[b], t);
+                    else
+                        entry_terms<DIM>([&](int k) { return sQ[k * SL + h]; }, c0, c2, sN[b], 1.0 + sN[b], t);
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) acc[b][j] = l0 ? acc[b][j] + t[j] : acc[b][j] - t[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // slot-ordered sum per (member, basis) through the (now free) staging area
+    double* sRed = sQ;   // [SL][NC]
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        if (b < nm) {
+            if (slot_thread)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) sRed[h * NC + j] = acc[b][j];
+            __syncthreads();
+            if (threadIdx.x < NC) {
+                double total = 0.0;
+                for (int hh = 0; hh < SL; ++hh) total += sRed[hh * NC + threadIdx.x];
+                sTot[b][threadIdx.x] = total;
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x < nm * NC) {
         const int b = threadIdx.x / NC, j = threadIdx.x % NC;
         double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < NC; ++a) {
-            double s = 0.0;
-            for (int w = 0; w < kApplyThreads / 64; ++w) s += s_red[w][b * NC + a];
-            v += s * minv_g[a * NC + j];
-        }
+        for (int a = 0; a < NC; ++a) v = fma(sTot[b][a], minv_g[a * NC + j], v);
         out[((int64_t)(mb + b) * nel + e) * NC + j] = amplitude * v;
     }
 }
+
+// ft[e][b][a] = f[b][e][a]
+__global__ void transpose_state_kernel(const double* __restrict__ f, double* __restrict__ ft, int64_t nel,
+                                       int32_t batch, int nc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)batch * nel * nc;
+    if (i >= total) return;
+    const int a = (int)(i % nc);
+    const int64_t be = i / nc;
+    const int64_t e = be % nel, b = be / nel;
+    ft[(e * batch + b) * nc + a] = f[i];
+}
+
+#pragma clang fp contract(on)
 
 Geo make_geo(const tg_tuple_ctx* c) {
     Geo g{};
@@ -665,6 +793,7 @@ Geo make_geo(const tg_tuple_ctx* c) {
     g.rows = c->rows;
     g.eps = c->d_eps;
     g.pts = c->d_pts;
+    g.etab = c->d_etab;
     return g;
 }
 
@@ -758,19 +887,39 @@ tg_status moments(tg_tuple_ctx* c) {
     return TG_OK;
 }
 
+template <int DIM, int MB>
+tg_status launch_propagate(tg_tuple_ctx* c, const double* d_ft, const double* d_bath, double* d_out, int32_t batch) {
+    constexpr size_t lds = prop_lds_bytes<DIM>();
+    static bool attr_set = false;   // per process; the attribute belongs to the function
+    if (!attr_set) {
+        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_kernel<DIM, MB>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    dim3 grid((unsigned)c->nel, (unsigned)((batch + MB - 1) / MB));
+    hipLaunchKernelGGL((propagate_kernel<DIM, MB>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
+                       c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_ft, d_bath, batch,
+                       c->amplitude, c->d_minv, d_out);
+    TT_HIP(c, hipGetLastError());
+    return TG_OK;
+}
+
 tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_bath, double* d_out,
                            int32_t batch) {
-    dim3 grid((unsigned)c->nel, (unsigned)((batch + kApplyMembers - 1) / kApplyMembers));
+    const int nc = c->dim + 1;
+    const int64_t total = (int64_t)batch * c->nel * nc;
     if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev0, c->stream));
-    if (c->dim == 2)
-        hipLaunchKernelGGL(propagate_kernel<2>, grid, dim3(kApplyThreads), 0, c->stream, c->nel, c->ntup,
-                           c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_f, d_bath, batch,
-                           c->amplitude, c->d_minv, d_out);
-    else
-        hipLaunchKernelGGL(propagate_kernel<3>, grid, dim3(kApplyThreads), 0, c->stream, c->nel, c->ntup,
-                           c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_f, d_bath, batch,
-                           c->amplitude, c->d_minv, d_out);
+    hipLaunchKernelGGL(transpose_state_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, d_f,
+                       c->d_ft, c->nel, batch, nc);
     TT_HIP(c, hipGetLastError());
+    tg_status s;
+    if (c->dim == 2)
+        s = batch == 1 ? launch_propagate<2, 1>(c, c->d_ft, d_bath, d_out, batch)
+                       : launch_propagate<2, kApplyMembers>(c, c->d_ft, d_bath, d_out, batch);
+    else
+        s = batch == 1 ? launch_propagate<3, 1>(c, c->d_ft, d_bath, d_out, batch)
+                       : launch_propagate<3, kApplyMembers>(c, c->d_ft, d_bath, d_out, batch);
+    if (s != TG_OK) return s;
     if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
     return TG_OK;
 }
@@ -783,7 +932,7 @@ extern "C" const char* tg_tuple_last_error(const tg_tuple_ctx* c) { return c ? c
 extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
     if (!c) return;
     void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
-                    c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv};
+                    c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -866,12 +1015,26 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
     if ((s = dalloc(c, &c->d_pts, (size_t)c->rows * c->nmc)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_f, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_out, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_ft, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_minv, 16)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_bath, c->max_batch)) != TG_OK) return bail(s);
     e = hipMemcpy(c->d_eps, d->eps, sizeof(double) * c->n, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->d_minv, c->minv, sizeof(double) * 16, hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    {
+        const int stride = d->dim == 2 ? etab_stride<2>() : etab_stride<3>();
+        if ((s = dalloc(c, &c->d_etab, (size_t)c->nel * stride)) != TG_OK) return bail(s);
+        Geo g = make_geo(c);
+        dim3 grid((unsigned)((c->nel + 255) / 256));
+        if (d->dim == 2)
+            hipLaunchKernelGGL(elem_table_kernel<2>, grid, dim3(256), 0, c->stream, g, c->nel, c->d_etab);
+        else
+            hipLaunchKernelGGL(elem_table_kernel<3>, grid, dim3(256), 0, c->stream, g, c->nel, c->d_etab);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "element table: %s", hipGetErrorString(e)));
+    }
     *out = c;
     return TG_OK;
 }

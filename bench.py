@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--no-dp5", action="store_true", help="skip the full DP5 runs")
     ap.add_argument("--no-reference-op", action="store_true",
                     help="skip the reference tuple operator measurements")
+    ap.add_argument("--only-reference-op", action="store_true",
+                    help="only the reference tuple operator measurements (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -321,6 +323,11 @@ def main():
 
     import tortoise_amd as tg
     from tortoise_amd import workloads as wl
+
+    if args.only_reference_op:
+        torch.cuda.init()
+        print(json.dumps({"reference_operator": reference_operator()}), flush=True)
+        return
 
     w = wl.SQUARE128_ENSEMBLE
     B = w.batch

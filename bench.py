@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP16_MFMA_PEAK_TFLOPS = 2500.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_gemm_traffic.json")
+TUPLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tuple_propagate_traffic.json")
 
 
 def _log(msg):
@@ -229,15 +230,27 @@ def reference_operator(budget_s=12.0):
             # entry, two entries a tuple) read once per 8-member chunk, the
             # other-element indices, the states in and out
             byts = T * 2 * (27 * 8 + 4) + 2 * B * op.nel * 3 * 8
+            traffic = None
+            try:
+                with open(TUPLE_TRAFFIC_FILE) as fh:
+                    tt = json.load(fh)
+                rec = tt["launches"].get(f"square{ns}_b{B}")
+                if rec and rec["tuples"] == T:
+                    traffic = rec["fetch_kib"] * 1024 * tt["fetch_scale"] + rec["write_kib"] * 1024 * tt["write_scale"]
+            except (OSError, KeyError, ValueError):
+                traffic = None
+            gbs = byts / (kms * 1e-3) / 1e9
             line[f"b{B}"] = {"ms": ms, "evals_per_s": B * 1000.0 / ms, "kernel_ms": kms,
-                             "kernel_gbs": byts / (kms * 1e-3) / 1e9,
-                             "frac_hbm": byts / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                                          "traffic": traffic, "bytes_per_launch": byts,
+                                          "kernel": f"propagate_kernel<2, {1 if B == 1 else 8}>"}}
             if ns == 32 and B == 1:
                 Fh = F[0].cpu().numpy()
                 Ig = out[0].cpu().numpy()
         _log(f"tuple operator {ns}x{ns}: {T} tuples, enumerate+moments {t_first:.2f} s, moments "
              f"{t_mom * 1e3:.1f} ms; propagate B=1 {line['b1']['ms']:.3f} ms "
-             f"({line['b1']['kernel_gbs']:.0f} GB/s), B=8 {line['b8']['ms']:.3f} ms "
+             f"({line['b1']['roofline']['achieved']:.0f} GB/s), B=8 {line['b8']['ms']:.3f} ms "
              f"({line['b8']['evals_per_s']:.0f} evals/s)")
         if ns == 32:
             idx = op.indices()

@@ -256,6 +256,18 @@ tg_status   tg_tuple_set_amplitude(tg_tuple_ctx* ctx, double amplitude);
 tg_status   tg_tuple_propagate(tg_tuple_ctx* ctx, const double* f, const double* bath_n, double* out,
                                int64_t nel, int32_t batch, int32_t flags);
 tg_status   tg_tuple_last_kernel_ms(tg_tuple_ctx* ctx, double* ms);
+/* MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137) with the tuple
+ * operator: DP5(4) (DormandPrince54.hpp:479-665) of the electron band's DG
+ * coefficients f [batch][nel][dim+1] from t0 to t1, per-member steps, the
+ * fermion error norm of Band::relativeError (Band.cpp:87-101) on DG data, the
+ * phonon band a bath of occupation bath_n[b].  Same outputs as
+ * tg_dp5_integrate (dense_out [batch][max_dense][nel][dim+1]).  The MC points
+ * are the ones last set (the caller redraws them per propagate, as
+ * MaterialTimeStatus.cpp:104 does).  flags: TG_PTR_DEVICE, TG_QUIET. */
+tg_status   tg_tuple_dp5(tg_tuple_ctx* ctx, double* f, int64_t nel, int32_t batch, double t0,
+                         double t1, double dense_dt, double rtol, double atol,
+                         const double* bath_n, int32_t flags, double* dense_out,
+                         int32_t max_dense, tg_step_stats* stats);
 void*       tg_tuple_stream(const tg_tuple_ctx* ctx);
 
 #ifdef __cplusplus

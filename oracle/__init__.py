@@ -122,6 +122,14 @@ def lib():
                                       C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
                                       _f64p, C.c_void_p, C.c_int32]
         L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+        L.or_relative_error_dg.restype = C.c_double
+        L.or_relative_error_dg.argtypes = [C.c_int, C.c_int64, _f64p, _f64p, C.c_double, C.c_double,
+                                           C.c_double, C.c_double]
+        L.or_dp5_mc_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                    C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, C.c_double, _f64p,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                    C.POINTER(DP5Stats)]
         L.or_ee_grid.argtypes = [C.c_int, _i32p, _f64p, C.c_double, C.POINTER(C.c_int32)]
         L.or_ee_apply.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, _f64p]
@@ -473,3 +481,29 @@ def mc_nodal_to_dg(dim, nsec, f):
     dg = np.zeros((nel, dim + 1))
     lib().or_mc_nodal_to_dg(dim, nsec, np.ascontiguousarray(f, np.float64), dg)
     return dg
+
+
+def relative_error_dg(dim, err, pop, stat=-1.0, rtol=1e-8, atol=1e-12, penalty=0.0):
+    """Band::relativeError on DG coefficients [nel][dim+1]."""
+    err = np.ascontiguousarray(err, np.float64)
+    pop = np.ascontiguousarray(pop, np.float64)
+    return lib().or_relative_error_dg(dim, err.shape[0], err, pop, stat, rtol, atol, penalty)
+
+
+def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0, t0,
+               t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8):
+    """DP5(4) of the electron band's DG state with the tuple operator."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    pts = np.ascontiguousarray(pts, np.float64)
+    idx = np.ascontiguousarray(idx, np.int32)
+    f = np.ascontiguousarray(f0, np.float64).copy()
+    dense = np.zeros((max(max_dense, 1),) + f.shape)
+    dense_t = np.zeros(max(max_dense, 1))
+    st = DP5Stats()
+    lib().or_dp5_mc_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
+                        amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], bath_n, f, t0, t_final,
+                        dense_dt, rtol, atol, dense.ctypes.data if max_dense else None,
+                        dense_t.ctypes.data if max_dense else None, max_dense, nthreads, C.byref(st))
+    return {"f": f, "dense": dense[:st.n_dense] if max_dense else None,
+            "dense_t": dense_t[:st.n_dense] if max_dense else None, "stats": st}

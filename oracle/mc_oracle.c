@@ -552,3 +552,82 @@ void or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg
     }
     free(conn);
 }
+
+/* ====================================================== DG error norm + DP5 */
+/* Band::relativeError (Band.cpp:87-101) on DG linear-form coefficients
+ * [nel][dim+1]: Function's pointwise operations act on the element-vertex
+ * values c0 (+ cj), and integrate(rhs) of the P1 result is exact per element.
+ * Fermion (stat = -1) or boson (+1) scale; the penalty term of states outside
+ * the physical range as in the reference. */
+double or_relative_error_dg(int dim, int64_t nel, const double* err, const double* pop, double stat,
+                            double rtol, double atol, double penalty) {
+    const int nc = dim + 1;
+    double s_r = 0.0, s_p = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        double sr = 0.0, sr2 = 0.0, sp = 0.0, sp2 = 0.0;
+        for (int a = 0; a < nc; ++a) {
+            double y = pop[e * nc] + (a ? pop[e * nc + a] : 0.0);
+            double x = err[e * nc] + (a ? err[e * nc + a] : 0.0);
+            double scale, pen;
+            if (stat == -1.0) {
+                double m = fabs(y) < fabs(1.0 - y) ? fabs(y) : fabs(1.0 - y);
+                scale = rtol * m + atol;
+                pen = ((y < 0.0 ? -y : 0.0) + (y - 1.0 > 0.0 ? y - 1.0 : 0.0)) / scale;
+            } else {
+                scale = rtol * y + atol;
+                pen = (y < 0.0 ? -y : 0.0) / scale;
+            }
+            double r = x / scale;
+            sr += r; sr2 += r * r; sp += pen; sp2 += pen * pen;
+        }
+        s_r += sr2 + sr * sr;
+        s_p += sp2 + sp * sp;
+    }
+    double denom = (double)((dim + 1) * (dim + 2)) * (double)nel;
+    return sqrt((s_r + penalty * s_p) / denom);
+}
+
+typedef struct mc_dp5_user {
+    int dim;
+    const int32_t* nsec;
+    const double *origin, *extent, *eps, *pts;
+    double omega0, amplitude, rtol, atol;
+    int32_t nu, nmc, nthreads;
+    const int32_t* idx;
+    int64_t ntup, nel;
+    double* fph;
+} mc_dp5_user;
+
+static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
+    (void)t;
+    const mc_dp5_user* p = (const mc_dp5_user*)u;
+    or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->omega0, p->amplitude, p->nu,
+                    p->pts, p->nmc, p->idx, p->ntup, y, p->fph, dydt, NULL, p->nthreads);
+}
+
+static double mc_dp5_norm(void* u, const double* err, const double* y5) {
+    const mc_dp5_user* p = (const mc_dp5_user*)u;
+    return or_relative_error_dg(p->dim, p->nel, err, y5, -1.0, p->rtol, p->atol, 0.0);
+}
+
+/* MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137) with the
+ * tuple operator: DP5(4) over the electron band's DG coefficients, the phonon
+ * band a bath of occupation bath_n (its constraint makes its error zero),
+ * fermion error norm without penalty (Band.hpp default). */
+void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                   const double* eps, double omega0, double amplitude, int32_t nu, const double* pts,
+                   int32_t nmc, const int32_t* idx, int64_t ntup, double bath_n, double* f, double t0,
+                   double t_final, double dense_dt, double rtol, double atol, double* dense_out,
+                   double* dense_t, int32_t max_dense, int32_t nthreads, or_dp5_stats* stats) {
+    mc_dp5_user u;
+    u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
+    u.omega0 = omega0; u.amplitude = amplitude; u.rtol = rtol; u.atol = atol;
+    u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
+    u.nel = or_num_elements(dim, nsec);
+    const int nc = dim + 1;
+    u.fph = (double*)calloc((size_t)u.nel * nc, sizeof(double));
+    for (int64_t e = 0; e < u.nel; ++e) u.fph[e * nc] = bath_n;
+    or_dp5_integrate(u.nel * nc, f, t0, t_final, dense_dt, mc_dp5_rhs, mc_dp5_norm, &u, dense_out,
+                     dense_t, max_dense, NULL, NULL, 0, stats);
+    free(u.fph);
+}

@@ -217,6 +217,16 @@ void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin,
                         const int32_t* idx, int64_t ntup, const double* f_el,
                         const double* f_ph, double* out_el, double* out_ph,
                         int32_t nthreads);
+/* Band::relativeError on DG coefficients [nel][dim+1]. */
+double  or_relative_error_dg(int dim, int64_t nel, const double* err, const double* pop,
+                             double stat, double rtol, double atol, double penalty);
+/* DP5(4) of the electron band's DG coefficients with the tuple operator. */
+void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                      const double* eps, double omega0, double amplitude, int32_t nu,
+                      const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                      double bath_n, double* f, double t0, double t_final, double dense_dt,
+                      double rtol, double atol, double* dense_out, double* dense_t,
+                      int32_t max_dense, int32_t nthreads, or_dp5_stats* stats);
 /* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
 void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
 

@@ -153,3 +153,31 @@ def test_set_indices_rejects_bad_lists():
     with pytest.raises(tg.TgError, match="MC points"):
         op.propagate(np.zeros((nel, 3)), 0.1)
     op.close()
+
+
+def test_tuple_dp5_matches_oracle_and_batches_bitwise():
+    """MaterialTimeStatus::propagate with the reference operator: DP5(4) of
+    the DG state on the device against the oracle's DP5 with the same tuple
+    operator (identical step sequence, dense outputs to 1e-10); a member of a
+    batch follows exactly its single run (bitwise)."""
+    print("[gpu] tuple dp5 8x8 t=5", flush=True)
+    w, eps, pts = _case(2, 8)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    op = tg.TupleOperator.create(w, max_batch=3)
+    op.set_points(pts)
+    F, bath = _state(w, 3, seed=5)
+    fg, dg, st = op.dp5(F[0], 0.0, 5.0, 0.5, 1e-8, 1e-12, bath[0], max_dense=10)
+    r = oracle.dp5_mc_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                          bath[0], F[0], 0.0, 5.0, 0.5, 1e-8, 1e-12, max_dense=10)
+    assert st["attempted"] == r["stats"].attempted and st["accepted"] == r["stats"].accepted
+    assert st["n_dense"] == r["stats"].n_dense == len(dg)
+    for a, b in zip(dg, r["dense"]):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    num = _number(w, fg).sum()
+    assert abs(num - _number(w, F[0]).sum()) <= 1e-12 * abs(num)
+    fb, db, sb = op.dp5(F, 0.0, 5.0, 0.5, 1e-8, 1e-12, bath, max_dense=10)
+    assert np.array_equal(fb[0], fg) and sb[0]["attempted"] == st["attempted"]
+    for b in (1, 2):
+        f1, _, s1 = op.dp5(F[b], 0.0, 5.0, 0.5, 1e-8, 1e-12, bath[b])
+        assert np.array_equal(fb[b], f1) and sb[b]["attempted"] == s1["attempted"]
+    op.close()

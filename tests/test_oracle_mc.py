@@ -259,3 +259,33 @@ def test_operator_vanishes_for_empty_band():
     out = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 0.01, 1, pts, idx, z,
                               fph)
     assert not out.any() and nel <= 72
+
+
+def test_dg_error_norm_equals_nodal_norm_for_continuous_fields():
+    """For continuous fields the DG norm is the nodal one (pinned by closed
+    forms in test_oracle.py): same vertex values, same exact P1 integral."""
+    w, eps = _mesh(6)
+    rng = np.random.default_rng(2)
+    n = (6 + 1) ** 2
+    err = rng.normal(size=n) * 1e-6
+    pop = rng.uniform(-0.1, 1.1, n)
+    conn = oracle.mesh_elements(2, [6, 6])
+    for stat, pen in ((-1.0, 0.0), (-1.0, 0.5), (1.0, 0.3)):
+        ref = oracle.relative_error(2, conn, err, pop, stat, 1e-6, 1e-9, pen)
+        dg = oracle.relative_error_dg(2, oracle.mc_nodal_to_dg(2, w.nsec, err),
+                                      oracle.mc_nodal_to_dg(2, w.nsec, pop), stat, 1e-6, 1e-9, pen)
+        assert abs(dg - ref) <= 1e-13 * ref
+
+
+def test_dp5_with_tuple_operator_conserves_number():
+    w, eps = _mesh(6)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    f0 = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    r = oracle.dp5_mc_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                          w.bath_n, f0, 0.0, 5.0, 0.5, 1e-8, 1e-12, max_dense=10)
+    assert r["stats"].accepted > 0 and not r["stats"].failed
+    num0 = _integrals(w, eps, f0).sum()
+    num1 = _integrals(w, eps, r["f"]).sum()
+    assert abs(num1 - num0) <= 1e-13 * abs(num0)
+    assert np.abs(r["f"] - f0).max() > 1e-8      # the state moved

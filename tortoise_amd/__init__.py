@@ -48,7 +48,7 @@ EXPORTS = [
     "tg_tuple_create", "tg_tuple_destroy", "tg_tuple_last_error", "tg_tuple_point_rows",
     "tg_tuple_set_points", "tg_tuple_count", "tg_tuple_copy_indices", "tg_tuple_set_indices",
     "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
-    "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream",
+    "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream", "tg_tuple_dp5",
 ]
 
 
@@ -175,6 +175,9 @@ def load():
         fn.argtypes = args
     L.tg_tuple_stream.restype = vp
     L.tg_tuple_stream.argtypes = [vp]
+    L.tg_tuple_dp5.restype = i32
+    L.tg_tuple_dp5.argtypes = [vp, vp, i64, i32, f64, f64, f64, f64, f64, vp, i32, vp, i32,
+                               C.POINTER(StepStats)]
     _lib = L
     return L
 
@@ -504,3 +507,22 @@ class TupleOperator:
 
     def stream(self):
         return load().tg_tuple_stream(self._h)
+
+    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n, max_dense=0, quiet=True):
+        """DP5(4) of DG states f0 [nel, dim+1] or [batch, nel, dim+1]; returns
+        (f_last, dense [(batch,) max_dense, nel, dim+1], stats)."""
+        f = np.ascontiguousarray(f0, np.float64).copy()
+        single = f.ndim == 2
+        fb = f[None] if single else f
+        batch = fb.shape[0]
+        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        dense = np.zeros((batch, max_dense) + fb.shape[1:]) if max_dense else None
+        st = (StepStats * batch)()
+        rc = load().tg_tuple_dp5(self._h, _p(fb), self.nel, batch, t0, t1, dense_dt, rtol, atol,
+                                 _p(bn), TG_QUIET if quiet else 0,
+                                 None if dense is None else _p(dense), max_dense, st)
+        self._check(rc)
+        stats = [{k: getattr(s, k) for k, _ in StepStats._fields_} for s in st]
+        if single:
+            return fb[0], (None if dense is None else dense[0][:stats[0]["n_dense"]]), stats[0]
+        return fb, dense, stats

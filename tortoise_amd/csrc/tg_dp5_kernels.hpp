@@ -1,0 +1,224 @@
+// tg_dp5_kernels.hpp -- device side of the batched Dormand-Prince 5(4)
+// driver, shared by the W-form driver (tg_dp5.hip) and the tuple operator's
+// (tg_tuple.hip): tableau, per-member controller state and the stage, error,
+// controller, dense-output and accept kernels.  The state of a member is a
+// flat vector of n doubles; member b at b*n.
+//
+// Restates Algorithms::dormandPrince54 (vetoer form,
+// include/Generics/Algorithms/DormandPrince54.hpp:479-665).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/tortoise_gpu.h"
+
+namespace tg {
+namespace {
+
+// Butcher tableau (DormandPrince54.hpp:503-511)
+constexpr double C2 = 1. / 5., C3 = 3. / 10., C4 = 4. / 5., C5 = 8. / 9.;
+constexpr double A21 = 1. / 5.;
+constexpr double A31 = 3. / 40., A32 = 9. / 40.;
+constexpr double A41 = 44. / 45., A42 = -56. / 15., A43 = 32. / 9.;
+constexpr double A51 = 19372. / 6561., A52 = -25360. / 2187., A53 = 64448. / 6561., A54 = -212. / 729.;
+constexpr double A61 = 9017. / 3168., A62 = -355. / 33., A63 = 46732. / 5247., A64 = 49. / 176.,
+                 A65 = -5103. / 18656.;
+constexpr double B1 = 35. / 384., B3 = 500. / 1113., B4 = 125. / 192., B5 = -2187. / 6784., B6 = 11. / 84.;
+constexpr double BH1 = 5179. / 57600., BH3 = 7571. / 16695., BH4 = 393. / 640.,
+                 BH5 = -92097. / 339200., BH6 = 187. / 2100., BH7 = 1. / 40.;
+
+struct StageArgs {
+    const double* k[7];   // stage derivatives (n x B each)
+    double a[7];          // tableau coefficients (multiplied by h per member)
+    int nk;
+};
+
+// out[:, b] = y[:, b] + sum_j (a_j h_b) k_j[:, b]; the product a_j*h_b is
+// formed first, as the reference does ((a_ij * h) * k_j).
+__global__ void stage_kernel(int64_t n, int B, const double* __restrict__ y, StageArgs s,
+                             const double* __restrict__ h, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const double hb = h[b];
+    double c[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) c[j] = s.a[j] * hb;
+    const int64_t base = (int64_t)b * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = y[base + i];
+        for (int j = 0; j < s.nk; ++j) v += c[j] * s.k[j][base + i];
+        out[base + i] = v;
+    }
+}
+
+// err = y5 - (y + sum_j (h_b bh_j) k_j), j over k1,k3..k7
+__global__ void error_kernel(int64_t n, int B, const double* __restrict__ y, StageArgs s,
+                             const double* __restrict__ h, const double* __restrict__ y5,
+                             double* __restrict__ err) {
+    const int b = blockIdx.y;
+    const double hb = h[b];
+    double c[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) c[j] = hb * s.a[j];
+    const int64_t base = (int64_t)b * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v = y[base + i];
+        for (int j = 0; j < s.nk; ++j) v += c[j] * s.k[j][base + i];
+        err[base + i] = y5[base + i] - v;
+    }
+}
+
+// Per-member controller state (DormandPrince54.hpp:527-543, 547-661), on the
+// device.  The accepted step's start, size and first dense time are kept for
+// the dense-output kernel.
+struct MemberDev {
+    double t, h, dense_time, olderror;
+    double step_t, step_h, step_dense_time;
+    int32_t rejected, failed, nonfinite, active;
+    int32_t step_idx, acc, dense_k0, dense_k1;
+    tg_step_stats st;
+};
+
+struct Ctl {
+    double t1, dense_dt, min_h, denom;
+    int32_t B, replay, h_cap, max_dense;
+    const double* sched;      // replay: step sizes, B x h_cap
+    const int32_t* nsched;    // replay: steps per member
+};
+
+__device__ inline bool member_running(const MemberDev& m, const Ctl& k, int b) {
+    if (k.replay) return m.step_idx < k.nsched[b] && m.t < k.t1;
+    return m.t < k.t1 && !m.failed;
+}
+
+// members still running take their step size, the others 0; clears the
+// "any member running" flag the controller sets
+__global__ void prep_kernel(MemberDev* __restrict__ md, Ctl k, double* __restrict__ h,
+                            int32_t* __restrict__ running) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *running = 0;
+    if (b >= k.B) return;
+    MemberDev m = md[b];
+    m.active = member_running(m, k, b) ? 1 : 0;
+    if (m.active && k.replay) m.h = k.sched[(int64_t)b * k.h_cap + m.step_idx];
+    h[b] = m.active ? m.h : 0.0;
+    md[b] = m;
+}
+
+// accept / reject per member (DormandPrince54.hpp:582-661); norms holds the
+// error-norm sums, err = sqrt(sum / denom) (Band.cpp:87-101)
+__global__ void controller_kernel(MemberDev* __restrict__ md, Ctl k,
+                                  const double* __restrict__ norms, int8_t* __restrict__ acc,
+                                  double* __restrict__ hlog, int32_t* __restrict__ running) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < k.B) {
+        const double minscale = 0.333333, maxscale = 3.0, safety = 0.98;
+        const double alpha = 0.7 / 5., beta = 0.4 / 5.;
+        MemberDev m = md[b];
+        m.acc = 0;
+        m.dense_k0 = m.dense_k1 = m.st.n_dense;
+        if (m.active) {
+            if (k.replay) ++m.step_idx;
+            m.st.attempted += 1;
+            m.st.evals += 6;
+            const double e = k.replay ? 0.0 : sqrt(norms[b] / k.denom);
+            const double h = m.h;
+            if (!isfinite(e)) {
+                m.failed = 1;
+                m.nonfinite = 1;
+            } else if (e <= 1.0) {
+                // dense outputs inside the step (:598-611)
+                m.step_t = m.t;
+                m.step_h = h;
+                m.step_dense_time = m.dense_time;
+                while (m.dense_time <= m.t + 1.0001 * h && m.dense_time <= k.t1 + .0001 * h) {
+                    m.st.n_dense += 1;
+                    m.dense_time += k.dense_dt;
+                }
+                m.dense_k1 = m.st.n_dense;
+                if (hlog && m.st.accepted < k.h_cap) hlog[(int64_t)b * k.h_cap + m.st.accepted] = h;
+                m.t += h;
+                m.acc = 1;
+                double scale;
+                if (e == 0.) {
+                    scale = maxscale;
+                } else {
+                    scale = safety * pow(1. / e, alpha) * pow(m.olderror, beta);
+                    scale = fmin(fmax(scale, minscale), maxscale);
+                }
+                m.h = m.rejected ? h * fmin(1.0, scale) : h * scale;
+                m.rejected = 0;
+                m.olderror = e;
+                m.st.accepted += 1;
+            } else {
+                double scale = safety * pow(1. / e, 1 / 5.);
+                scale = fmin(fmax(scale, minscale), maxscale);
+                m.h = h * scale;
+                m.rejected = 1;
+                m.st.rejected += 1;
+                if (m.h < k.min_h) m.failed = 1;
+            }
+        }
+        acc[b] = (int8_t)m.acc;
+        if (member_running(m, k, b)) atomicOr(&any, 1);
+        md[b] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(running, 1);
+}
+
+// dense outputs of the step just accepted: y(t_s) = y + sum_j (s h bt_j(s)) k_j
+// with s = (t_s - t)/h (DormandPrince54.hpp:601-608)
+__global__ void dense_kernel(int64_t n, const MemberDev* __restrict__ md, Ctl k,
+                             const double* __restrict__ y, const double* __restrict__ k1,
+                             const double* __restrict__ k3, const double* __restrict__ k4,
+                             const double* __restrict__ k5, const double* __restrict__ k6,
+                             const double* __restrict__ k7, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const MemberDev& m = md[b];
+    if (!m.acc || m.dense_k0 >= m.dense_k1 || m.dense_k0 >= k.max_dense) return;
+    const int64_t base = (int64_t)b * n;
+    const double t = m.step_t, h = m.step_h;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = base + i;
+        double td = m.step_dense_time;
+        for (int32_t slot = m.dense_k0; slot < m.dense_k1 && slot < k.max_dense; ++slot) {
+            const double s = (td - t) / h;
+            const double bt1 = -(435. * s * s * s - 1184. * s * s + 1098. * s - 384.) / (384.);
+            const double bt3 = 500. * s * (6. * s * s - 14. * s + 9.) / (1113.);
+            const double bt4 = -125. * s * (9. * s * s - 16. * s + 6.) / (192.);
+            const double bt5 = 729. * s * (35. * s * s - 64. * s + 26.) / (6784.);
+            const double bt6 = -11. * s * (3. * s - 2.) * (5. * s - 6.) / (84.);
+            const double bt7 = s * (s - 1.) * (5. * s - 3.) / (2.);
+            out[((int64_t)b * k.max_dense + slot) * n + i] =
+                y[q] + (s * h * bt1) * k1[q] + (s * h * bt3) * k3[q] + (s * h * bt4) * k4[q] +
+                (s * h * bt5) * k5[q] + (s * h * bt6) * k6[q] + (s * h * bt7) * k7[q];
+            td += k.dense_dt;
+        }
+    }
+}
+
+// accepted members: y <- y5, k1 <- k7 (FSAL)
+__global__ void accept_kernel(int64_t n, const int8_t* __restrict__ acc, double* __restrict__ y,
+                              const double* __restrict__ y5, double* __restrict__ k1,
+                              const double* __restrict__ k7) {
+    const int b = blockIdx.y;
+    if (!acc[b]) return;
+    const int64_t base = (int64_t)b * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        y[base + i] = y5[base + i];
+        k1[base + i] = k7[base + i];
+    }
+}
+
+}  // namespace
+}  // namespace tg

@@ -45,6 +45,7 @@
 #include <vector>
 
 #include "../../include/tortoise_gpu.h"
+#include "tg_dp5_kernels.hpp"
 #include "tg_mesh.hpp"
 
 struct tg_tuple_ctx {
@@ -83,6 +84,8 @@ struct tg_tuple_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     bool time_kernel = false;
+
+    void* dp5 = nullptr;         // DP5 work buffers (tg_tuple_dp5)
 
     hipStream_t stream = nullptr;
     int device = 0;
@@ -924,6 +927,174 @@ tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_b
     return TG_OK;
 }
 
+// ------------------------------------------------------------------ DP5
+// Band::relativeError (Band.cpp:87-101) of a fermion band on DG coefficients:
+// Function's pointwise operations act on the element-vertex values c0 (+ cj),
+// integrate(rhs) of the P1 result is exact per element:
+//   sum_e (sum_v r_v^2 + (sum_v r_v)^2),  r_v = err_v / (rtol min(|p_v|,|1-p_v|) + atol)
+// (penalty 0, the Band default); the controller divides by (dim+1)(dim+2) nel.
+constexpr int kRelThreads = 256;
+
+template <int NC>
+__global__ __launch_bounds__(kRelThreads) void dg_relerr_partial_kernel(int64_t nel, const double* __restrict__ err,
+                                                                          const double* __restrict__ pop, double rtol,
+                                                                          double atol, int nblk,
+                                                                          double* __restrict__ part) {
+    __shared__ double sw[kRelThreads / 64];
+    const int b = blockIdx.y;
+    const double* ev = err + (int64_t)b * nel * NC;
+    const double* pv = pop + (int64_t)b * nel * NC;
+    double s = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * kRelThreads + threadIdx.x; e < nel; e += (int64_t)nblk * kRelThreads) {
+        double sr = 0.0, sr2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            const double y = pv[e * NC] + (a ? pv[e * NC + a] : 0.0);
+            const double x = ev[e * NC] + (a ? ev[e * NC + a] : 0.0);
+            const double m = fabs(y) < fabs(1.0 - y) ? fabs(y) : fabs(1.0 - y);
+            const double r = x / (rtol * m + atol);
+            sr += r;
+            sr2 += r * r;
+        }
+        s += sr2 + sr * sr;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kRelThreads / 64; ++w) t += sw[w];
+        part[(int64_t)b * nblk + blockIdx.x] = t;
+    }
+}
+
+__global__ void dg_relerr_final_kernel(const double* __restrict__ part, int nblk, int B, double* __restrict__ norms) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double s = 0.0;
+    for (int i = 0; i < nblk; ++i) s += part[(int64_t)b * nblk + i];
+    norms[b] = s;
+}
+
+struct TupleDp5 {
+    int32_t cap = 0;
+    int64_t n = 0;
+    int nblk = 1;
+    double* k[7] = {};
+    double* ys = nullptr;
+    double* y5 = nullptr;
+    double* errv = nullptr;
+    double* y = nullptr;
+    double* h = nullptr;
+    int8_t* acc = nullptr;
+    tg::MemberDev* md = nullptr;
+    double* norms = nullptr;
+    double* part = nullptr;
+    int32_t* running = nullptr;
+    int32_t* running_host = nullptr;
+    double* dense = nullptr;
+    size_t dense_cap = 0;
+};
+
+void tuple_dp5_free(tg_tuple_ctx* c) {
+    TupleDp5* w = (TupleDp5*)c->dp5;
+    if (!w) return;
+    void* bufs[] = {w->ys, w->y5, w->errv, w->y, w->h, w->acc, w->md, w->norms, w->part, w->running, w->dense};
+    for (int j = 0; j < 7; ++j)
+        if (w->k[j]) (void)hipFree(w->k[j]);
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (w->running_host) (void)hipHostFree(w->running_host);
+    delete w;
+    c->dp5 = nullptr;
+}
+
+tg_status tuple_dp5_ensure(tg_tuple_ctx* c) {
+    if (c->dp5) return TG_OK;
+    TupleDp5* w = new (std::nothrow) TupleDp5();
+    if (!w) return fail(c, TG_ERR_NOMEM, "DP5 work");
+    c->dp5 = w;
+    w->cap = c->max_batch;
+    w->n = c->nel * (c->dim + 1);
+    w->nblk = (int)std::min<int64_t>(64, (c->nel + kRelThreads - 1) / kRelThreads);
+    const size_t sz = (size_t)w->n * w->cap;
+    tg_status s;
+    for (int j = 0; j < 7; ++j)
+        if ((s = dalloc(c, &w->k[j], sz)) != TG_OK) return s;
+    if ((s = dalloc(c, &w->ys, sz)) != TG_OK || (s = dalloc(c, &w->y5, sz)) != TG_OK ||
+        (s = dalloc(c, &w->errv, sz)) != TG_OK || (s = dalloc(c, &w->y, sz)) != TG_OK ||
+        (s = dalloc(c, &w->h, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->acc, (size_t)w->cap)) != TG_OK ||
+        (s = dalloc(c, &w->md, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->norms, (size_t)w->cap)) != TG_OK ||
+        (s = dalloc(c, &w->part, (size_t)w->cap * w->nblk)) != TG_OK || (s = dalloc(c, &w->running, 1)) != TG_OK)
+        return s;
+    TT_HIP(c, hipHostMalloc((void**)&w->running_host, sizeof(int32_t), hipHostMallocDefault));
+    return TG_OK;
+}
+
+// one attempted step for all members (kernel launches only; captured)
+tg_status tuple_attempt(tg_tuple_ctx* c, TupleDp5* w, int32_t B, const tg::Ctl& ctl, double rtol, double atol,
+                        double* d_dense) {
+    using namespace tg;
+    hipStream_t st = c->stream;
+    const int64_t n = w->n;
+    double** k = w->k;
+    const dim3 blk(256);
+    const dim3 grd((unsigned)std::max<int64_t>(1, (n + 1023) / 1024), B);
+    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out) -> tg_status {
+        StageArgs a{};
+        int j = 0;
+        for (double v : coefs) a.a[j++] = v;
+        for (int q = 0; q < nk; ++q) a.k[q] = k[q];
+        a.nk = nk;
+        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out);
+        TT_HIP(c, hipGetLastError());
+        return TG_OK;
+    };
+    auto eval = [&](const double* in, double* out) { return propagate_device(c, in, c->d_bath, out, B); };
+    const dim3 cgrd((unsigned)((B + 255) / 256));
+    hipLaunchKernelGGL(prep_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->h, w->running);
+    TT_HIP(c, hipGetLastError());
+    tg_status s;
+    // stages k2..k6, y5 and k7 (DormandPrince54.hpp:565-578)
+    if ((s = stage({A21}, 1, w->ys)) != TG_OK || (s = eval(w->ys, k[1])) != TG_OK) return s;
+    if ((s = stage({A31, A32}, 2, w->ys)) != TG_OK || (s = eval(w->ys, k[2])) != TG_OK) return s;
+    if ((s = stage({A41, A42, A43}, 3, w->ys)) != TG_OK || (s = eval(w->ys, k[3])) != TG_OK) return s;
+    if ((s = stage({A51, A52, A53, A54}, 4, w->ys)) != TG_OK || (s = eval(w->ys, k[4])) != TG_OK) return s;
+    if ((s = stage({A61, A62, A63, A64, A65}, 5, w->ys)) != TG_OK || (s = eval(w->ys, k[5])) != TG_OK) return s;
+    if ((s = stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5)) != TG_OK || (s = eval(w->y5, k[6])) != TG_OK) return s;
+    {   // err = y5 - y4 (:580)
+        StageArgs a{};
+        const double bh[7] = {BH1, 0.0, BH3, BH4, BH5, BH6, BH7};
+        for (int q = 0; q < 7; ++q) {
+            a.a[q] = bh[q];
+            a.k[q] = k[q];
+        }
+        a.nk = 7;
+        hipLaunchKernelGGL(error_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, w->y5, w->errv);
+        TT_HIP(c, hipGetLastError());
+    }
+    if (c->dim == 2)
+        hipLaunchKernelGGL(dg_relerr_partial_kernel<3>, dim3(w->nblk, B), dim3(kRelThreads), 0, st, c->nel, w->errv,
+                           w->y5, rtol, atol, w->nblk, w->part);
+    else
+        hipLaunchKernelGGL(dg_relerr_partial_kernel<4>, dim3(w->nblk, B), dim3(kRelThreads), 0, st, c->nel, w->errv,
+                           w->y5, rtol, atol, w->nblk, w->part);
+    hipLaunchKernelGGL(dg_relerr_final_kernel, cgrd, dim3(256), 0, st, w->part, w->nblk, B, w->norms);
+    hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc, (double*)nullptr,
+                       w->running);
+    TT_HIP(c, hipGetLastError());
+    if (d_dense) {
+        hipLaunchKernelGGL(dense_kernel, dim3((unsigned)std::max<int64_t>(1, (n + 1023) / 1024), B), blk, 0, st, n,
+                           w->md, ctl, w->y, k[0], k[2], k[3], k[4], k[5], k[6], d_dense);
+        TT_HIP(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(accept_kernel, grd, blk, 0, st, n, w->acc, w->y, w->y5, k[0], k[6]);
+    TT_HIP(c, hipGetLastError());
+    TT_HIP(c, hipMemcpyAsync(w->running_host, w->running, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    return TG_OK;
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -931,6 +1102,7 @@ extern "C" const char* tg_tuple_last_error(const tg_tuple_ctx* c) { return c ? c
 
 extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
     if (!c) return;
+    tuple_dp5_free(c);
     void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
                     c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft};
     for (void* p : bufs)
@@ -1171,4 +1343,132 @@ extern "C" tg_status tg_tuple_load_indices(tg_tuple_ctx* c, const char* path) {
     std::fclose(fp);
     if (!ok) return fail(c, TG_ERR_ARG, "%s: truncated tuple list", path);
     return tg_tuple_set_indices(c, idx.data(), (int64_t)total);
+}
+
+// MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137) with the
+// tuple operator: DP5(4) (DormandPrince54.hpp:479-665) of the electron band's
+// DG coefficients for `batch` members, each with its own steps; the phonon
+// band is a bath (its constraint makes its error zero).  One attempted step
+// is captured as a HIP graph and replayed one attempt ahead of the "any
+// member running" flag it reads, as in tg_dp5_integrate.
+extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32_t batch, double t0, double t1,
+                                  double dense_dt, double rtol, double atol, const double* bath_n, int32_t flags,
+                                  double* dense_out, int32_t max_dense, tg_step_stats* stats) {
+    using namespace tg;
+    if (!c || !f || !bath_n) return TG_ERR_ARG;
+    if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
+    if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
+    if (batch < 1 || batch > c->max_batch)
+        return fail(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
+    if (!(dense_dt > 0.0)) return fail(c, TG_ERR_ARG, "dense_dt must be > 0");
+    if (dense_out && max_dense < 1) return fail(c, TG_ERR_ARG, "max_dense < 1");
+    tg_status s;
+    if ((s = tuple_dp5_ensure(c)) != TG_OK) return s;
+    TupleDp5* w = (TupleDp5*)c->dp5;
+    const bool dev = flags & TG_PTR_DEVICE;
+    hipStream_t st = c->stream;
+    const size_t vec = (size_t)w->n * batch;
+    TT_HIP(c, hipMemcpyAsync(w->y, f, sizeof(double) * vec, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    double* d_dense = nullptr;
+    if (dense_out) {
+        if (dev) {
+            d_dense = dense_out;
+        } else {
+            const size_t need = (size_t)max_dense * vec;
+            if (w->dense_cap < need) {
+                if (w->dense) (void)hipFree(w->dense);
+                w->dense = nullptr;
+                w->dense_cap = 0;
+                if ((s = dalloc(c, &w->dense, need)) != TG_OK) return s;
+                w->dense_cap = need;
+            }
+            d_dense = w->dense;
+        }
+    }
+    Ctl ctl{};
+    ctl.t1 = t1;
+    ctl.dense_dt = dense_dt;
+    ctl.min_h = dense_dt / 10000.;
+    ctl.denom = (double)((c->dim + 1) * (c->dim + 2)) * (double)c->nel;
+    ctl.B = batch;
+    ctl.replay = 0;
+    ctl.h_cap = 1;
+    ctl.max_dense = d_dense ? max_dense : 0;
+    ctl.sched = nullptr;
+    ctl.nsched = nullptr;
+    std::vector<MemberDev> ms(batch);
+    for (auto& m : ms) {
+        std::memset(&m, 0, sizeof(m));
+        m.t = t0;
+        m.h = dense_dt;
+        m.dense_time = t0 + dense_dt;
+        m.olderror = 1.0;
+        m.st.evals = 1;
+    }
+    TT_HIP(c, hipMemcpyAsync(w->md, ms.data(), sizeof(MemberDev) * batch, hipMemcpyHostToDevice, st));
+    const bool timing = c->time_kernel;
+    c->time_kernel = false;   // no event records inside the graph
+    s = propagate_device(c, w->y, c->d_bath, w->k[0], batch);   // k1 = timeOp(y0)
+    if (s != TG_OK) { c->time_kernel = timing; return s; }
+    TT_HIP(c, hipStreamSynchronize(st));
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    TT_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    tg_status cap = tuple_attempt(c, w, batch, ctl, rtol, atol, d_dense);
+    c->time_kernel = timing;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (cap != TG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return cap;
+    }
+    if (ec != hipSuccess) return fail(c, TG_ERR_HIP, "graph capture of a DP5 attempt: %s", hipGetErrorString(ec));
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, TG_ERR_HIP, "graph instantiation: %s", hipGetErrorString(ei));
+
+    tg_status rc = TG_OK;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev)
+        if (rc == TG_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            rc = fail(c, TG_ERR_HIP, "event creation");
+    const int64_t max_attempts = 100000000;   // guards a controller that never ends
+    for (int64_t a = 0; rc == TG_OK; ++a) {
+        if (a >= max_attempts) {
+            rc = fail(c, TG_ERR_STATE, "more than %lld DP5 attempts", (long long)max_attempts);
+            break;
+        }
+        if (hipGraphLaunch(exec, st) != hipSuccess || hipEventRecord(ev[a & 1], st) != hipSuccess) {
+            rc = fail(c, TG_ERR_HIP, "graph launch of a DP5 attempt");
+            break;
+        }
+        if (a == 0) continue;
+        if (hipEventSynchronize(ev[(a - 1) & 1]) != hipSuccess) {
+            rc = fail(c, TG_ERR_HIP, "DP5 attempt failed on the device");
+            break;
+        }
+        if (*(volatile int32_t*)w->running_host == 0) break;
+    }
+    const hipError_t es = hipStreamSynchronize(st);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipGraphExecDestroy(exec);
+    if (rc != TG_OK) return rc;
+    if (es != hipSuccess) return fail(c, TG_ERR_HIP, "DP5 attempts: %s", hipGetErrorString(es));
+    TT_HIP(c, hipMemcpyAsync(ms.data(), w->md, sizeof(MemberDev) * batch, hipMemcpyDeviceToHost, st));
+    TT_HIP(c, hipMemcpyAsync(f, w->y, sizeof(double) * vec, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    if (dense_out && !dev)
+        TT_HIP(c, hipMemcpyAsync(dense_out, d_dense, sizeof(double) * (size_t)max_dense * vec, hipMemcpyDeviceToHost, st));
+    TT_HIP(c, hipStreamSynchronize(st));
+    if (!(flags & TG_QUIET) && batch == 1) printf("Number of adaptive steps: %lld\n", (long long)ms[0].st.attempted);
+    for (int b = 0; b < batch; ++b) {
+        ms[b].st.failed = ms[b].failed ? 1 : 0;
+        if (stats) stats[b] = ms[b].st;
+        if (ms[b].nonfinite) rc = TG_ERR_NONFINITE;
+        else if (ms[b].failed && rc == TG_OK) rc = TG_ERR_STEP_UNDERFLOW;
+    }
+    if (rc != TG_OK) fail(c, rc, rc == TG_ERR_NONFINITE ? "non-finite error norm" : "step size underflow");
+    return rc;
 }

@@ -6,7 +6,10 @@
 // propagation with dense output, and the number / energy sums of every dense
 // step.  Runs the collision operator on the GPU.
 //
-// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0]
+// With ref=1 the collision operator is the reference's own: element tuples
+// and the Monte-Carlo delta-shell integral per tuple on the DG coefficients.
+//
+// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0] [ref=0]
 #include <TORTOISE>
 
 #include <cmath>
@@ -20,6 +23,7 @@ int main(int argc, char** argv) {
     const Real tFinal = argc > 2 ? std::atof(argv[2]) : 100.0;
     const bool quiet = argc > 3 ? std::atoi(argv[3]) != 0 : true;
     const Real gEE = argc > 4 ? std::atof(argv[4]) : 0.0;
+    const bool ref = argc > 5 ? std::atoi(argv[5]) != 0 : false;
     const Real pi = M_PI;
 
     Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
@@ -27,7 +31,8 @@ int main(int argc, char** argv) {
     Material<2> mat("SquareParabolic", zone);
     mat.addBand("el:C", -1, fermion, mesh, [](Point<2> k) { return 0.5 * k.squaredNorm(); });
     mat.addBand("pn:opt", 0, boson, mesh, 0.05);
-    for (auto sc : mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 0)) {
+    mat.referenceOperator = ref;
+    for (auto sc : mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 1)) {
         sc->amplitude = 0.1 * 0.1;      // g^2
         sc->deltaBroadening = 0.01;
     }

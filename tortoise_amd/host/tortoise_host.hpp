@@ -301,8 +301,13 @@ public:
     Real deltaBroadening = 0.01;
     Real energyGridStep = 0.0;      // <= 0: deltaBroadening / 8
     Real deltaCutoff = 8.0;         // in units of deltaBroadening
-    int numMCPoints = 50;           // accepted for source compatibility; unused
-    Real minimumDeterminant = 1e-9; // idem
+    // the reference operator (Material::referenceOperator): MC points per
+    // tuple, screening thresholds and umklapp vectors {-n..n}^NDim
+    // (Material.hpp:89-91, ScatteringChannel.hpp:80-83)
+    int numMCPoints = 50;
+    Real minimumDeterminant = 1e-9;
+    Real scattElemeThreshold = 1e-9;
+    int umklappSteps = 0;
 };
 
 template <int NDim> class MaterialStatus;
@@ -316,9 +321,18 @@ public:
     std::deque<Band<NDim>> bands;                       // insertion order (StringMap.hpp:78-79)
     std::deque<ScatteringChannel<NDim>> channels;
 
+    // false: the W-form (nodal P1, broadened delta, assembled once);
+    // true: the reference's own operator -- element tuples and the Monte-Carlo
+    // delta-shell integral per tuple on the DG coefficients (tg_tuple_*)
+    bool referenceOperator = false;
+    uint64_t mcSeed = 0;     // start of the MC point stream (the reference uses std::rand)
+
     Material(std::string n, const Region<NDim>& r) : name(std::move(n)), region(&r) {}
     Material(const Material&) = delete;
-    ~Material() { if (ctx_) tg_ctx_destroy(ctx_); }
+    ~Material() {
+        if (ctx_) tg_ctx_destroy(ctx_);
+        if (tctx_) tg_tuple_destroy(tctx_);
+    }
 
     void addBand(const std::string& n, Real charge, Real stat, const Mesh<NDim>& m,
                  const std::function<Real(Point<NDim>)>& disp) {
@@ -360,6 +374,7 @@ public:
                     channels.push_back(ScatteringChannel<NDim>{});
                     channels.back().electronBand = e;
                     channels.back().phononBand = p;
+                    channels.back().umklappSteps = nUmklapp;
                     r.push_back(&channels.back());
                 }
         } else {
@@ -380,7 +395,6 @@ public:
                 r.push_back(&channels.back());
             }
         }
-        (void)nUmklapp;  // lattice momentum wraps periodically; no umklapp enumeration
         invalidate();
         return r;
     }
@@ -444,14 +458,113 @@ public:
     void check(tg_status s) const {
         if (s != TG_OK) throw std::runtime_error(std::string("tortoise_gpu: ") + tg_last_error(ctx_));
     }
+    void checkTuple(tg_status s) const {
+        if (s != TG_OK) throw std::runtime_error(std::string("tortoise_gpu: ") + tg_tuple_last_error(tctx_));
+    }
+
+    // The reference operator of the e-ph channel: tuple list built on first
+    // use (ScatteringChannel::precalculateIndices, ScatteringChannel.cpp:421-538)
+    // with the MC points of the first constructMCPoints().
+    tg_tuple_ctx* tupleBackend() {
+        if (tctx_) return tctx_;
+        const ScatteringChannel<NDim>* eph = nullptr;
+        for (const auto& c : channels)
+            if (c.legs == 3) eph = &c;
+        if (!eph) throw std::runtime_error("the reference operator needs an e-ph channel");
+        const Band<NDim>& el = (*this)[eph->electronBand];
+        const auto omega = (*this)[eph->phononBand].dispersion.toNodal();
+        for (Real w : omega)
+            if (std::fabs(w - omega[0]) > 1e-12 * std::fabs(omega[0]))
+                throw std::runtime_error("phonon band must be dispersionless");
+        eps_ = el.dispersion.toNodal();
+        tg_tuple_desc d{};
+        d.dim = NDim;
+        for (int q = 0; q < 3; ++q) {
+            d.nsec[q] = q < NDim ? el.mesh->nSecSplits[q] : 1;
+            d.origin[q] = q < NDim ? el.mesh->origin(q) : 0.0;
+            d.extent[q] = q < NDim ? el.mesh->extent(q) : 1.0;
+        }
+        d.eps = eps_.data();
+        d.omega0 = omega[0];
+        d.amplitude = eph->amplitude;
+        d.n_umklapp = eph->umklappSteps;
+        d.num_mc = eph->numMCPoints;
+        d.min_det = eph->minimumDeterminant;
+        d.threshold = eph->scattElemeThreshold;
+        d.max_batch = 1;
+        tg_status s = tg_tuple_create(&d, &tctx_);
+        if (s != TG_OK) {
+            std::string msg = tctx_ ? tg_tuple_last_error(tctx_) : "tg_tuple_create failed";
+            if (tctx_) tg_tuple_destroy(tctx_);
+            tctx_ = nullptr;
+            throw std::runtime_error("tortoise_gpu: " + msg);
+        }
+        tupleChannel_ = eph;
+        return tctx_;
+    }
+    // Material::constructMCPoints: new points for the e-ph channel (leg 0 in
+    // the reference simplex, the partially free leg in the (NDim-1)-simplex,
+    // ScatteringChannel.cpp:677-688, Point.hpp:44-68), from a splitmix64 stream.
+    void constructMCPoints() {
+        tg_tuple_ctx* t = tupleBackend();
+        const int nmc = tupleChannel_->numMCPoints;
+        const int rows = tg_tuple_point_rows(NDim, 3);
+        std::vector<Real> pts((size_t)rows * nmc, 0.0);
+        for (int m = 0; m < nmc; ++m) {
+            Real x[3];
+            simplexPoint(NDim, x);
+            for (int d = 0; d < NDim; ++d) pts[(size_t)d * nmc + m] = x[d];
+        }
+        if (NDim > 1)
+            for (int m = 0; m < nmc; ++m) {
+                Real x[3];
+                simplexPoint(NDim - 1, x);
+                for (int d = 0; d < NDim - 1; ++d) pts[(size_t)(rows - (NDim - 1) + d) * nmc + m] = x[d];
+            }
+        checkTuple(tg_tuple_set_points(t, pts.data(), 0));
+        havePoints_ = true;
+    }
+    bool haveMCPoints() const { return havePoints_; }
+    const ScatteringChannel<NDim>& tupleChannel() const { return *tupleChannel_; }
 
 private:
     tg_ctx* ctx_ = nullptr;
     Real ctx_bath_ = -1.0;
     std::vector<Real> eps_;
+    tg_tuple_ctx* tctx_ = nullptr;
+    const ScatteringChannel<NDim>* tupleChannel_ = nullptr;
+    bool havePoints_ = false;
+    uint64_t rngState_ = 0;
+    bool rngStarted_ = false;
     void invalidate() {
         if (ctx_) tg_ctx_destroy(ctx_);
         ctx_ = nullptr;
+        if (tctx_) tg_tuple_destroy(tctx_);
+        tctx_ = nullptr;
+        havePoints_ = false;
+    }
+    Real unitDraw() {
+        if (!rngStarted_) { rngState_ = mcSeed; rngStarted_ = true; }
+        uint64_t z = (rngState_ += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (Real)(z >> 11) * 0x1.0p-53;
+    }
+    void simplexPoint(int d, Real* x) {
+        for (int i = 0; i < d; ++i) x[i] = unitDraw();
+        if (d >= 2 && x[0] + x[1] > 1.0) { x[0] = 1.0 - x[0]; x[1] = 1.0 - x[1]; }
+        if (d == 3) {
+            if (x[1] + x[2] > 1.0) {
+                Real t = x[2];
+                x[2] = 1.0 - x[0] - x[1];
+                x[1] = 1.0 - t;
+            } else if (x[0] + x[1] + x[2] > 1.0) {
+                Real t = x[2];
+                x[2] = x[0] + x[1] + x[2] - 1.0;
+                x[0] = 1.0 - x[1] - t;
+            }
+        }
     }
     static bool match(const std::string& pat, const std::string& s) {
         auto star = pat.find('*');
@@ -491,6 +604,21 @@ public:
     // phonon bath is constrained, its derivative is zero.
     MaterialStatus propagate() const {
         MaterialStatus r(*material);
+        if (material->referenceOperator) {
+            // ScatteringChannel::propagate (ScatteringChannel.cpp:651-670): the
+            // tuple operator on the DG coefficients, inverse mass included
+            tg_tuple_ctx* t = material->tupleBackend();
+            if (!material->haveMCPoints()) material->constructMCPoints();
+            const auto& tch = material->tupleChannel();
+            const Function<NDim>& fe = (*this)[tch.electronBand];
+            Function<NDim> out(*fe.mesh);
+            const Real n = bathOccupation();
+            material->checkTuple(tg_tuple_set_amplitude(t, tch.amplitude));
+            material->checkTuple(tg_tuple_propagate(t, fe.vec.data(), &n, out.vec.data(),
+                                                    fe.mesh->numberElements, 1, 0));
+            r[tch.electronBand] = out;
+            return r;
+        }
         const auto& ch = material->channels.front();
         tg_ctx* ctx = material->backend(bathOccupation());
         const Function<NDim>& fe = (*this)[ch.electronBand];
@@ -541,6 +669,10 @@ public:
     // MaterialTimeStatus.cpp:102-137 -> DormandPrince54.hpp:479-665: adaptive
     // DP5(4) from the last entry to finalTime; every dense output is appended.
     void propagate(Real finalTime, Real denseStep) {
+        if (material->referenceOperator) {
+            propagateReference(finalTime, denseStep);
+            return;
+        }
         const MaterialStatus<NDim> start = valVec.back();
         const auto& ch = material->channels.front();
         const Band<NDim>& el = (*material)[ch.electronBand];
@@ -567,6 +699,38 @@ public:
     }
     // the name the reference's examples call (examples/main.cpp:99)
     void propagateDeterministic(Real finalTime, Real denseStep) { propagate(finalTime, denseStep); }
+
+private:
+    // MaterialTimeStatus.cpp:102-137 with the reference operator: new MC points
+    // (material->constructMCPoints()), then DP5(4) of the electron band's DG
+    // coefficients on the device (tg_tuple_dp5).
+    void propagateReference(Real finalTime, Real denseStep) {
+        const MaterialStatus<NDim> start = valVec.back();
+        tg_tuple_ctx* t = material->tupleBackend();
+        material->constructMCPoints();
+        const auto& ch = material->tupleChannel();
+        const Band<NDim>& el = (*material)[ch.electronBand];
+        std::vector<Real> f = start[ch.electronBand].vec;
+        const int64_t n = (int64_t)f.size();
+        const Real t0 = argVec.back();
+        const Real bath = start.bathOccupation();
+        const int maxDense = (int)std::floor((finalTime - t0) / denseStep + 1e-9) + 1;
+        std::vector<Real> dense((size_t)maxDense * n);
+        material->checkTuple(tg_tuple_set_amplitude(t, ch.amplitude));
+        tg_status s = tg_tuple_dp5(t, f.data(), el.mesh->numberElements, 1, t0, finalTime, denseStep,
+                                   el.relativeErrorTolerance, el.absoluteErrorTolerance, &bath,
+                                   quiet ? TG_QUIET : 0, dense.data(), maxDense, &lastStats);
+        if (s != TG_OK && s != TG_ERR_STEP_UNDERFLOW) material->checkTuple(s);
+        if (s == TG_ERR_STEP_UNDERFLOW) std::printf("The step size has become too small to be handled\n");
+        const int nd = lastStats.n_dense < maxDense ? lastStats.n_dense : maxDense;
+        for (int k = 0; k < nd; ++k) {
+            MaterialStatus<NDim> st = start;
+            std::copy(dense.begin() + (size_t)k * n, dense.begin() + (size_t)(k + 1) * n,
+                      st[ch.electronBand].vec.begin());
+            argVec.push_back(t0 + (k + 1) * denseStep);
+            valVec.push_back(std::move(st));
+        }
+    }
 };
 
 }  // namespace Tortoise

@@ -122,6 +122,9 @@ def lib():
                                       C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
                                       _f64p, C.c_void_p, C.c_int32]
         L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+        L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                      C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
+                                      _f64p, C.c_int32]
         L.or_relative_error_dg.restype = C.c_double
         L.or_relative_error_dg.argtypes = [C.c_int, C.c_int64, _f64p, _f64p, C.c_double, C.c_double,
                                            C.c_double, C.c_double]
@@ -507,3 +510,19 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
                         dense_t.ctypes.data if max_dense else None, max_dense, nthreads, C.byref(st))
     return {"f": f, "dense": dense[:st.n_dense] if max_dense else None,
             "dense_t": dense_t[:st.n_dense] if max_dense else None, "stats": st}
+
+
+def mc_rates_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
+                 nthreads=1):
+    """ScatteringChannel::scatteringRates of the electron band (DG, inverse-massed)."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    pts = np.ascontiguousarray(pts, np.float64)
+    f_el = np.ascontiguousarray(f_el, np.float64)
+    f_ph = np.ascontiguousarray(f_ph, np.float64)
+    out = np.zeros_like(f_el)
+    idx = np.ascontiguousarray(idx, np.int32)
+    lib().or_mc_rates_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
+                          amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out,
+                          nthreads)
+    return out

@@ -631,3 +631,74 @@ void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const dou
                      dense_t, max_dense, NULL, NULL, 0, stats);
     free(u.fph);
 }
+
+/* ============================================================ rates */
+/* ScatteringChannel::scatteringRates(status, band) (ScatteringChannel.cpp:544-612)
+ * for the electron band of the e-ph channel: for each leg the band occupies
+ * (0 and 2), singleScatteringRate replaces that leg's forms by
+ *   direct:        (leg==0 ? -1 : 1) ((1-dir) stat + (1+dir)) * 1
+ *   time-reversed: ((1+dir) stat + (1-dir)) * 1
+ * keeps the others as in the propagator, and subtracts dir * result of that
+ * leg; the legs' inverse-massed results are summed. */
+void or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                     const double* eps, double omega0, double amplitude, int32_t nu, const double* pts,
+                     int32_t nmc, const int32_t* idx, int64_t ntup, const double* f_el,
+                     const double* f_ph, double* out_el, int32_t nthreads) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    const int nc = dim + 1;
+    const int64_t nel = g.nel, len = nel * nc;
+    int nt = nthreads > 0 ? nthreads : 1;
+    double* acc = (double*)calloc((size_t)nt * len, sizeof(double));
+    double fullamp = amplitude;
+    for (int leg = 0; leg < 3; ++leg) fullamp *= g.vol * (double)mc_fact(dim);
+    static const double dir[3] = {1.0, 1.0, -1.0};
+    static const double stat[3] = {-1.0, 1.0, -1.0};
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int th = 0; th < nt; ++th) {
+        double* ae = acc + (size_t)th * len;
+        int64_t t0 = ntup * th / nt, t1 = ntup * (th + 1) / nt;
+        for (int64_t t = t0; t < t1; ++t) {
+            int64_t el[3] = {idx[t * 4], idx[t * 4 + 1], idx[t * 4 + 2]};
+            int u[3] = {0, 0, 0};
+            umklapp_vec(dim, nu, idx[t * 4 + 3], u);
+            double D[36], N[4];
+            tuple_DN(&g, el, u, D, N);
+            const double* c[3] = {f_el + el[0] * nc, f_ph + el[1] * nc, f_el + el[2] * nc};
+            for (int which = 0; which < 3; which += 2) {
+                double fa[12], fb[12], res[12];
+                for (int leg = 0; leg < 3; ++leg)
+                    for (int a = 0; a < nc; ++a) {
+                        double one = a == 0 ? 1.0 : 0.0;
+                        if (leg == which) {
+                            fa[leg * nc + a] = (leg == 0 ? -1.0 : 1.0) *
+                                               ((1.0 - dir[leg]) * stat[leg] + (1.0 + dir[leg])) * one;
+                            fb[leg * nc + a] = ((1.0 + dir[leg]) * stat[leg] + (1.0 - dir[leg])) * one;
+                        } else {
+                            fa[leg * nc + a] = (leg == 0 ? -1.0 : 1.0) * 0.5 *
+                                               ((1.0 - dir[leg]) * one +
+                                                ((1.0 - dir[leg]) * stat[leg] + (1.0 + dir[leg])) * c[leg][a]);
+                            fb[leg * nc + a] = 0.5 * ((1.0 + dir[leg]) * one +
+                                                      ((1.0 + dir[leg]) * stat[leg] + (1.0 - dir[leg])) * c[leg][a]);
+                        }
+                    }
+                if (or_mc_integrate(dim, 3, D, N, fa, fb, pts, nmc, res) != 0) continue;
+                for (int a = 0; a < nc; ++a) ae[el[which] * nc + a] -= dir[which] * fullamp * res[which * nc + a];
+            }
+        }
+    }
+    double minv[16];
+    or_mc_inverse_mass(dim, g.vol, minv);
+    for (int64_t e = 0; e < nel; ++e) {
+        double se[4] = {0, 0, 0, 0};
+        for (int th = 0; th < nt; ++th)
+            for (int a = 0; a < nc; ++a) se[a] += acc[(size_t)th * len + e * nc + a];
+        for (int b = 0; b < nc; ++b) {
+            double v = 0.0;
+            for (int a = 0; a < nc; ++a) v += se[a] * minv[a * nc + b];
+            out_el[e * nc + b] = v;
+        }
+    }
+    free(acc);
+    geo_free(&g);
+}

@@ -227,6 +227,11 @@ void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const 
                       double bath_n, double* f, double t0, double t_final, double dense_dt,
                       double rtol, double atol, double* dense_out, double* dense_t,
                       int32_t max_dense, int32_t nthreads, or_dp5_stats* stats);
+/* ScatteringChannel::scatteringRates of the electron band (legs 0 and 2). */
+void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                        const double* eps, double omega0, double amplitude, int32_t nu,
+                        const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                        const double* f_el, const double* f_ph, double* out_el, int32_t nthreads);
 /* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
 void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
 

@@ -181,3 +181,24 @@ def test_tuple_dp5_matches_oracle_and_batches_bitwise():
         f1, _, s1 = op.dp5(F[b], 0.0, 5.0, 0.5, 1e-8, 1e-12, bath[b])
         assert np.array_equal(fb[b], f1) and sb[b]["attempted"] == s1["attempted"]
     op.close()
+
+
+@pytest.mark.parametrize("dim,ns", [(2, 12), (3, 3)])
+def test_scattering_rates_match_oracle(dim, ns):
+    """ScatteringChannel::scatteringRates of the electron band (legs 0 and 2)
+    from the same per-tuple moments, against the oracle's restatement."""
+    print(f"[gpu] tuple rates {dim}D ns={ns}", flush=True)
+    w, eps, pts = _case(dim, ns)
+    idx = oracle.mc_tuples_eph(dim, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    op = tg.TupleOperator.create(w, max_batch=2)
+    op.set_points(pts)
+    F, bath = _state(w, 2, seed=11)
+    R = op.rates(F, bath)
+    for b in range(2):
+        fph = np.zeros_like(F[b])
+        fph[:, 0] = bath[b]
+        ref = oracle.mc_rates_eph(dim, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts,
+                                  idx, F[b], fph, nthreads=8)
+        assert np.abs(R[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
+        assert np.array_equal(op.rates(F[b], bath[b]), R[b])
+    op.close()

@@ -289,3 +289,55 @@ def test_dp5_with_tuple_operator_conserves_number():
     num1 = _integrals(w, eps, r["f"]).sum()
     assert abs(num1 - num0) <= 1e-13 * abs(num0)
     assert np.abs(r["f"] - f0).max() > 1e-8      # the state moved
+
+
+def _unit_moments(w, eps, pts, idx):
+    """Per element, the inverse-massed measure moments of legs 0 and 2 over
+    the tuple list (unit forms on every leg), from the integrator alone."""
+    X = oracle.mesh_nodes(2, list(w.nsec), list(w.origin), list(w.extent))
+    conn = oracle.mesh_elements(2, list(w.nsec))
+    nel = conn.shape[0]
+    vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
+    span = 3
+    U0 = np.zeros((nel, 3))
+    U2 = np.zeros((nel, 3))
+    one = np.zeros(9)
+    one[[0, 3, 6]] = 1.0
+    for a, b, c, uid in idx:
+        u = np.array([uid // span - 1, uid % span - 1])
+        D = np.zeros((3, 6))
+        N = np.zeros(3)
+        for leg, (e, dr, ph) in enumerate(((a, 1.0, False), (b, 1.0, True), (c, -1.0, False))):
+            k0 = X[conn[e, 0]]
+            D[:2, 2 * leg:2 * leg + 2] = dr * np.stack([X[conn[e, 1]] - k0, X[conn[e, 2]] - k0], 1)
+            N[:2] -= dr * k0
+            if not ph:
+                D[2, 2 * leg:2 * leg + 2] = dr * (eps[conn[e, 1:]] - eps[conn[e, 0]])
+            N[2] -= dr * (w.omega0 if ph else eps[conn[e, 0]])
+        N[:2] -= u * np.array(w.extent)
+        r = oracle.mc_integrate(2, 3, D, N, one, np.zeros(9), pts) * (vol * 2) ** 3 * w.g ** 2
+        U0[a] += r[0:3]
+        U2[c] += r[6:9]
+    Minv = oracle.mc_inverse_mass(2, vol)
+    return U0 @ Minv, U2 @ Minv
+
+
+def test_scattering_rates_of_constant_bands():
+    """scatteringRates with the rate leg's forms replaced by constants: for a
+    band of constant occupation c the rate is 2(n + c) U0 + 2(1 + n - c) U2,
+    U_l the measure moments of leg l (absorption needs a phonon, emission
+    leaves one; Pauli blocking by the partner state)."""
+    w, eps = _mesh(4)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    U0, U2 = _unit_moments(w, eps, pts, idx)
+    nel = U0.shape[0]
+    for c, n in ((0.0, 0.09), (0.3, 0.2)):
+        f = np.zeros((nel, 3))
+        f[:, 0] = c
+        fph = np.zeros((nel, 3))
+        fph[:, 0] = n
+        r = oracle.mc_rates_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                                f, fph)
+        expect = 2 * (n + c) * U0 + 2 * (1 + n - c) * U2
+        assert np.abs(r - expect).max() <= 1e-12 * np.abs(expect).max()

@@ -49,6 +49,7 @@ EXPORTS = [
     "tg_tuple_set_points", "tg_tuple_count", "tg_tuple_copy_indices", "tg_tuple_set_indices",
     "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
     "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream", "tg_tuple_dp5",
+    "tg_tuple_rates",
 ]
 
 
@@ -169,6 +170,7 @@ def load():
                        ("tg_tuple_load_indices", [vp, C.c_char_p]),
                        ("tg_tuple_set_amplitude", [vp, f64]),
                        ("tg_tuple_propagate", [vp, vp, vp, vp, i64, i32, i32]),
+                       ("tg_tuple_rates", [vp, vp, vp, vp, i64, i32, i32]),
                        ("tg_tuple_last_kernel_ms", [vp, C.POINTER(f64)])):
         fn = getattr(L, name)
         fn.restype = i32
@@ -492,6 +494,17 @@ class TupleOperator:
         out = np.zeros_like(fb)
         self._check(load().tg_tuple_propagate(self._h, _p(fb), _p(bn), _p(out), self.nel, batch,
                                               TG_TIME_KERNEL if time_kernel else 0))
+        return out[0] if single else out
+
+    def rates(self, f, bath_n):
+        """ScatteringChannel::scatteringRates of the electron band, same layout."""
+        f = np.ascontiguousarray(f, np.float64)
+        single = f.ndim == 2
+        fb = f[None] if single else f
+        batch = fb.shape[0]
+        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        out = np.zeros_like(fb)
+        self._check(load().tg_tuple_rates(self._h, _p(fb), _p(bn), _p(out), self.nel, batch, 0))
         return out[0] if single else out
 
     def propagate_device(self, f, bath_n, out, batch, time_kernel=False):

@@ -621,19 +621,41 @@ constexpr size_t prop_lds_bytes() {
     return sizeof(double) * (size_t)NQ * SL + sizeof(int32_t) * SL;
 }
 
-template <int DIM, typename QAt>
-__device__ inline void entry_terms(QAt qat, const double c0[DIM + 1], const double c2[DIM + 1], double n,
+// The bilinear weight of an entry.  Propagator (RATES = false):
+//   G[a][b] = (1+n) d_a0 c2_b - n c0_a d_b0 - c0_a c2_b.
+// Scattering rates (RATES = true, ScatteringChannel.cpp:544-612 with the
+// rate leg's forms replaced by constants; the factor 2 is applied at the end):
+//   leg 0 (own = c0):  G[a][b] = d_a0 (n d_b0 + c2_b)
+//   leg 2 (own = c2):  G[a][b] = d_b0 ((1+n) d_a0 - c0_a)
+template <int DIM, bool RATES, typename QAt>
+__device__ inline void entry_terms(QAt qat, bool leg0, const double c0[DIM + 1], const double c2[DIM + 1], double n,
                                    double np1, double out[DIM + 1]) {
     constexpr int NC = DIM + 1;
     double G[NC][NC];
+    if constexpr (!RATES) {
 #pragma unroll
-    for (int a = 0; a < NC; ++a)
+        for (int a = 0; a < NC; ++a)
 #pragma unroll
-        for (int b = 0; b < NC; ++b) G[a][b] = -(c0[a] * c2[b]);
+            for (int b = 0; b < NC; ++b) G[a][b] = -(c0[a] * c2[b]);
 #pragma unroll
-    for (int b = 0; b < NC; ++b) G[0][b] = fma(np1, c2[b], G[0][b]);
+        for (int b = 0; b < NC; ++b) G[0][b] = fma(np1, c2[b], G[0][b]);
 #pragma unroll
-    for (int a = 0; a < NC; ++a) G[a][0] = fma(-n, c0[a], G[a][0]);
+        for (int a = 0; a < NC; ++a) G[a][0] = fma(-n, c0[a], G[a][0]);
+    } else {
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) G[a][b] = 0.0;
+        if (leg0) {
+#pragma unroll
+            for (int b = 0; b < NC; ++b) G[0][b] = c2[b];
+            G[0][0] += n;
+        } else {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) G[a][0] = -c0[a];
+            G[0][0] += np1;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
         double s = 0.0;
@@ -645,7 +667,7 @@ __device__ inline void entry_terms(QAt qat, const double c0[DIM + 1], const doub
     }
 }
 
-template <int DIM, int MB>
+template <int DIM, int MB, bool RATES>
 __global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
     int64_t nel, int64_t ntup, const int64_t* __restrict__ off0, const int64_t* __restrict__ off2,
     const int32_t* __restrict__ oth0, const int32_t* __restrict__ oth2, const double* __restrict__ q0,
@@ -730,11 +752,12 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
                         c2[a] = l0 ? other[a] : own[a];
                     }
                     if constexpr (QREG)
-                        entry_terms<DIM>([&](int k) { return Qr[k]; }, c0, c2, sN[b], 1.0 + sN[b], t);
+                        entry_terms<DIM, RATES>([&](int k) { return Qr[k]; }, l0, c0, c2, sN[b], 1.0 + sN[b], t);
                     else
-                        entry_terms<DIM>([&](int k) { return sQ[k * SL + h]; }, c0, c2, sN[b], 1.0 + sN[b], t);
+                        entry_terms<DIM, RATES>([&](int k) { return sQ[k * SL + h]; }, l0, c0, c2, sN[b],
+                                                1.0 + sN[b], t);
 #pragma unroll
-                    for (int j = 0; j < NC; ++j) acc[b][j] = l0 ? acc[b][j] + t[j] : acc[b][j] - t[j];
+                    for (int j = 0; j < NC; ++j) acc[b][j] = (l0 || RATES) ? acc[b][j] + t[j] : acc[b][j] - t[j];
                 }
             }
         }
@@ -762,7 +785,7 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
         double v = 0.0;
 #pragma unroll
         for (int a = 0; a < NC; ++a) v = fma(sTot[b][a], minv_g[a * NC + j], v);
-        out[((int64_t)(mb + b) * nel + e) * NC + j] = amplitude * v;
+        out[((int64_t)(mb + b) * nel + e) * NC + j] = RATES ? 2.0 * (amplitude * v) : amplitude * v;
     }
 }
 
@@ -890,23 +913,24 @@ tg_status moments(tg_tuple_ctx* c) {
     return TG_OK;
 }
 
-template <int DIM, int MB>
+template <int DIM, int MB, bool RATES>
 tg_status launch_propagate(tg_tuple_ctx* c, const double* d_ft, const double* d_bath, double* d_out, int32_t batch) {
     constexpr size_t lds = prop_lds_bytes<DIM>();
     static bool attr_set = false;   // per process; the attribute belongs to the function
     if (!attr_set) {
-        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_kernel<DIM, MB>),
+        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_kernel<DIM, MB, RATES>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr_set = true;
     }
     dim3 grid((unsigned)c->nel, (unsigned)((batch + MB - 1) / MB));
-    hipLaunchKernelGGL((propagate_kernel<DIM, MB>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
+    hipLaunchKernelGGL((propagate_kernel<DIM, MB, RATES>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
                        c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_q0, c->d_q2, d_ft, d_bath, batch,
                        c->amplitude, c->d_minv, d_out);
     TT_HIP(c, hipGetLastError());
     return TG_OK;
 }
 
+template <bool RATES = false>
 tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_bath, double* d_out,
                            int32_t batch) {
     const int nc = c->dim + 1;
@@ -917,11 +941,11 @@ tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_b
     TT_HIP(c, hipGetLastError());
     tg_status s;
     if (c->dim == 2)
-        s = batch == 1 ? launch_propagate<2, 1>(c, c->d_ft, d_bath, d_out, batch)
-                       : launch_propagate<2, kApplyMembers>(c, c->d_ft, d_bath, d_out, batch);
+        s = batch == 1 ? launch_propagate<2, 1, RATES>(c, c->d_ft, d_bath, d_out, batch)
+                       : launch_propagate<2, kApplyMembers, RATES>(c, c->d_ft, d_bath, d_out, batch);
     else
-        s = batch == 1 ? launch_propagate<3, 1>(c, c->d_ft, d_bath, d_out, batch)
-                       : launch_propagate<3, kApplyMembers>(c, c->d_ft, d_bath, d_out, batch);
+        s = batch == 1 ? launch_propagate<3, 1, RATES>(c, c->d_ft, d_bath, d_out, batch)
+                       : launch_propagate<3, kApplyMembers, RATES>(c, c->d_ft, d_bath, d_out, batch);
     if (s != TG_OK) return s;
     if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
     return TG_OK;
@@ -1271,8 +1295,9 @@ extern "C" tg_status tg_tuple_set_amplitude(tg_tuple_ctx* c, double amplitude) {
     return TG_OK;
 }
 
-extern "C" tg_status tg_tuple_propagate(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out,
-                                        int64_t nel, int32_t batch, int32_t flags) {
+template <bool RATES>
+tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out, int64_t nel, int32_t batch,
+                      int32_t flags) {
     if (!c || !f || !bath_n || !out) return TG_ERR_ARG;
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
@@ -1281,18 +1306,28 @@ extern "C" tg_status tg_tuple_propagate(tg_tuple_ctx* c, const double* f, const 
     c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
     const size_t len = (size_t)c->nel * (c->dim + 1) * batch;
     if (flags & TG_PTR_DEVICE) {
-        tg_status s = propagate_device(c, f, bath_n, out, batch);
+        tg_status s = propagate_device<RATES>(c, f, bath_n, out, batch);
         if (s != TG_OK) return s;
     } else {
         TT_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
         TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
-        tg_status s = propagate_device(c, c->d_f, c->d_bath, c->d_out, batch);
+        tg_status s = propagate_device<RATES>(c, c->d_f, c->d_bath, c->d_out, batch);
         if (s != TG_OK) return s;
         TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
     }
     TT_HIP(c, hipStreamSynchronize(c->stream));
     if (c->time_kernel) TT_HIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
     return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_propagate(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out,
+                                        int64_t nel, int32_t batch, int32_t flags) {
+    return tuple_apply<false>(c, f, bath_n, out, nel, batch, flags);
+}
+
+extern "C" tg_status tg_tuple_rates(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out,
+                                    int64_t nel, int32_t batch, int32_t flags) {
+    return tuple_apply<true>(c, f, bath_n, out, nel, batch, flags);
 }
 
 extern "C" tg_status tg_tuple_last_kernel_ms(tg_tuple_ctx* c, double* ms) {

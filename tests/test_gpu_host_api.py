@@ -75,24 +75,32 @@ def test_host_example_ee_channel_matches_abi():
 def test_host_example_reference_operator_matches_abi():
     """Material::referenceOperator: MaterialTimeStatus::propagate draws the MC
     points (splitmix64 from seed 0, the reference's simplex folds) and runs
-    DP5 with the tuple operator on the DG state.  The same points, tuples and
-    integration through the C ABI give the same step count and final number."""
+    DP5 with the tuple operator on the DG state; Material::scatteringRates
+    gives the initial state's rates.  The same points, tuples and calls
+    through the C ABI give the same rates integral, step count and final
+    number."""
     assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
     print("[gpu] host example 16 sections t=2 reference operator", flush=True)
     r = subprocess.run([EXE, "16", "2", "1", "0", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.strip().splitlines()
-    steps = int(lines[0].split()[1])
-    t0_num = float(lines[2].split("|")[1])
+    rates_int = float(next(x for x in lines if x.startswith("rates integral")).split()[-1])
+    i0 = next(i for i, x in enumerate(lines) if x.startswith("steps"))
+    steps = int(lines[i0].split()[1])
+    t0_num = float(lines[i0 + 2].split("|")[1])
     num_end = float(lines[-1].split("|")[1])
     assert abs(num_end - t0_num) <= 1e-12 * t0_num
     w = wl.Workload("square16", 2, (16, 16))
     op = tg.TupleOperator.create(w, n_umklapp=1, num_mc=50, max_batch=1)
     op.set_points(oracle.mc_points(2, 3, 50, 0))
     f0 = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    vol = (2 * np.pi / 16) ** 2 / 2
+
+    def integral(g):
+        return vol * (g[:, 0] + (g[:, 1] + g[:, 2]) / 3).sum()
+    rr = op.rates(f0, w.bath_n)
+    assert abs(integral(rr) - rates_int) <= 1e-12 * abs(rates_int)
     f, dense, st = op.dp5(f0, 0.0, 2.0, 0.02, 1e-8, 1e-12, w.bath_n, max_dense=100)
     assert st["attempted"] == steps
-    vol = (2 * np.pi / 16) ** 2 / 2
-    num = vol * (f[:, 0] + (f[:, 1] + f[:, 2]) / 3).sum()
-    assert abs(num - num_end) <= 1e-12 * num_end
+    assert abs(integral(f) - num_end) <= 1e-12 * num_end
     op.close()

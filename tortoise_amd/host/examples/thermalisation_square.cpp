@@ -54,6 +54,12 @@ int main(int argc, char** argv) {
         return 0.2 * std::exp(-0.5 * std::pow((e - 0.9) / 0.03, 2));
     });
 
+    if (ref) {
+        // relaxation rates of the initial state (Material::scatteringRates, as
+        // the reference's examples/main.cpp:84-87 prints them)
+        Function<2> rates = mat.scatteringRates<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, start, "el:C");
+        std::printf("rates integral %.15e\n", rates.integrate());
+    }
     MaterialTimeStatus<2> dyn(start);
     dyn.quiet = quiet;
     dyn.propagateDeterministic(tFinal, tFinal / 100.0);

@@ -527,6 +527,15 @@ public:
     bool haveMCPoints() const { return havePoints_; }
     const ScatteringChannel<NDim>& tupleChannel() const { return *tupleChannel_; }
 
+    // Material.cpp:396-402 -> ScatteringChannel::scatteringRates(status, band)
+    // (ScatteringChannel.cpp:544-612): the e-ph channel's rates of `bandName`
+    // (the electron band) from the reference operator, summed over the legs
+    // the band occupies.  Channel patterns select the e-ph channel.
+    template <int NLegs>
+    Function<NDim> scatteringRates(const std::array<std::string, NLegs>& bandsNamesWildCards,
+                                   const std::array<int, NLegs>& legDirection,
+                                   const MaterialStatus<NDim>& status, const std::string& bandName);
+
 private:
     tg_ctx* ctx_ = nullptr;
     Real ctx_bath_ = -1.0;
@@ -645,6 +654,30 @@ public:
         return v.empty() ? 0.0 : v[0];
     }
 };
+
+template <int NDim>
+template <int NLegs>
+Function<NDim> Material<NDim>::scatteringRates(const std::array<std::string, NLegs>& bandsNamesWildCards,
+                                               const std::array<int, NLegs>& legDirection,
+                                               const MaterialStatus<NDim>& status, const std::string& bandName) {
+    static_assert(NLegs == 3, "rates are evaluated for the 3-leg e-ph channel");
+    if (!(legDirection[0] == in && legDirection[1] == in && legDirection[2] == out))
+        throw std::invalid_argument("e-ph channel must be {in, in, out}");
+    tg_tuple_ctx* t = tupleBackend();
+    const auto& ch = tupleChannel();
+    bool sel = false;
+    for (auto& e : matchListId(bandsNamesWildCards[0]))
+        for (auto& p : matchListId(bandsNamesWildCards[1]))
+            sel = sel || (e == ch.electronBand && p == ch.phononBand);
+    const Function<NDim>& fe = status[ch.electronBand];
+    Function<NDim> out(*fe.mesh);
+    if (!sel || bandName != ch.electronBand) return out;   // no selected channel acts on the band
+    if (!haveMCPoints()) constructMCPoints();
+    const Real n = status.bathOccupation();
+    checkTuple(tg_tuple_set_amplitude(t, ch.amplitude));
+    checkTuple(tg_tuple_rates(t, fe.vec.data(), &n, out.vec.data(), fe.mesh->numberElements, 1, 0));
+    return out;
+}
 
 // ------------------------------------------------------ MaterialTimeStatus
 template <int NDim>

@@ -114,25 +114,25 @@ def lib():
         L.or_mc_integrate.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
                                       C.c_int32, _f64p]
         L.or_mc_tuples_eph.restype = C.c_int64
-        L.or_mc_tuples_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_int32,
-                                       _f64p, C.c_int32, C.c_double, C.c_double, C.c_void_p,
-                                       C.c_int64]
+        L.or_mc_tuples_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+                                       C.c_int32, _f64p, C.c_int32, C.c_double, C.c_double,
+                                       C.c_void_p, C.c_int64]
         L.or_mc_inverse_mass.argtypes = [C.c_int, C.c_double, _f64p]
-        L.or_mc_apply_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
-                                      C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
-                                      _f64p, C.c_void_p, C.c_int32]
+        L.or_mc_apply_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+                                      C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
+                                      _f64p, _f64p, _f64p, C.c_void_p, C.c_int32]
         L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
-        L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
-                                      C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, _f64p, _f64p,
-                                      _f64p, C.c_int32]
+        L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+                                      C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
+                                      _f64p, _f64p, _f64p, C.c_int32]
         L.or_relative_error_dg.restype = C.c_double
         L.or_relative_error_dg.argtypes = [C.c_int, C.c_int64, _f64p, _f64p, C.c_double, C.c_double,
                                            C.c_double, C.c_double]
-        L.or_dp5_mc_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
-                                    C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64, C.c_double, _f64p,
-                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
-                                    C.POINTER(DP5Stats)]
+        L.or_dp5_mc_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+                                    C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
+                                    C.c_int32, C.c_double, _f64p, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int32,
+                                    C.c_int32, C.POINTER(DP5Stats)]
         L.or_ee_grid.argtypes = [C.c_int, _i32p, _f64p, C.c_double, C.POINTER(C.c_int32)]
         L.or_ee_apply.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, _f64p]
@@ -438,18 +438,27 @@ def mc_integrate(dim, nlegs, D, N, fa, fb, pts):
     return out
 
 
+def _eps_ph_ptr(eps_ph):
+    if eps_ph is None:
+        return None, None
+    a = np.ascontiguousarray(eps_ph, np.float64)
+    return a, a.ctypes.data
+
+
 def mc_tuples_eph(dim, nsec, origin, extent, eps, omega0, nu, pts, min_det=1e-9,
-                  threshold=1e-9):
-    """Tuple list int32 [T][4] = (e0, e1, e2, umklapp id) of the e-ph channel."""
+                  threshold=1e-9, eps_ph=None):
+    """Tuple list int32 [T][4] = (e0, e1, e2, umklapp id) of the e-ph channel
+    (phonon band: nodal energies eps_ph, or the constant omega0)."""
     nsec = np.ascontiguousarray(nsec, np.int32)
     org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
     eps = np.ascontiguousarray(eps, np.float64)
     pts = np.ascontiguousarray(pts, np.float64)
+    keep, ph = _eps_ph_ptr(eps_ph)
     L = lib()
-    n = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, omega0, nu, pts, pts.shape[1], min_det,
+    n = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
                            threshold, None, 0)
     idx = np.zeros((max(n, 1), 4), np.int32)
-    m = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, omega0, nu, pts, pts.shape[1], min_det,
+    m = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
                            threshold, idx.ctypes.data, n)
     assert m == n
     return idx[:n]
@@ -462,7 +471,7 @@ def mc_inverse_mass(dim, vol):
 
 
 def mc_apply_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
-                 nthreads=1, phonon_out=False):
+                 nthreads=1, phonon_out=False, eps_ph=None):
     """Electron (and optionally phonon) DG derivative of the e-ph channel."""
     nsec = np.ascontiguousarray(nsec, np.int32)
     org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
@@ -472,7 +481,8 @@ def mc_apply_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx
     out_el = np.zeros_like(f_el)
     out_ph = np.zeros_like(f_ph) if phonon_out else None
     idx = np.ascontiguousarray(idx, np.int32)
-    lib().or_mc_apply_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
+    keep, ph = _eps_ph_ptr(eps_ph)
+    lib().or_mc_apply_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
                           amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out_el,
                           None if out_ph is None else out_ph.ctypes.data, nthreads)
     return (out_el, out_ph) if phonon_out else out_el
@@ -494,8 +504,9 @@ def relative_error_dg(dim, err, pop, stat=-1.0, rtol=1e-8, atol=1e-12, penalty=0
 
 
 def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0, t0,
-               t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8):
-    """DP5(4) of the electron band's DG state with the tuple operator."""
+               t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8, dynamic=False, eps_ph=None):
+    """DP5(4) of the electron band's DG state with the tuple operator; with
+    dynamic=True the state is [electron DG | phonon DG] and bath_n unused."""
     nsec = np.ascontiguousarray(nsec, np.int32)
     org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
     pts = np.ascontiguousarray(pts, np.float64)
@@ -504,8 +515,10 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
     dense = np.zeros((max(max_dense, 1),) + f.shape)
     dense_t = np.zeros(max(max_dense, 1))
     st = DP5Stats()
-    lib().or_dp5_mc_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
-                        amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], bath_n, f, t0, t_final,
+    keep, ph = _eps_ph_ptr(eps_ph)
+    lib().or_dp5_mc_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+                        amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
+                        bath_n, f, t0, t_final,
                         dense_dt, rtol, atol, dense.ctypes.data if max_dense else None,
                         dense_t.ctypes.data if max_dense else None, max_dense, nthreads, C.byref(st))
     return {"f": f, "dense": dense[:st.n_dense] if max_dense else None,
@@ -513,7 +526,7 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
 
 
 def mc_rates_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
-                 nthreads=1):
+                 nthreads=1, eps_ph=None):
     """ScatteringChannel::scatteringRates of the electron band (DG, inverse-massed)."""
     nsec = np.ascontiguousarray(nsec, np.int32)
     org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
@@ -522,7 +535,8 @@ def mc_rates_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx
     f_ph = np.ascontiguousarray(f_ph, np.float64)
     out = np.zeros_like(f_el)
     idx = np.ascontiguousarray(idx, np.int32)
-    lib().or_mc_rates_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), omega0,
+    keep, ph = _eps_ph_ptr(eps_ph)
+    lib().or_mc_rates_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
                           amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out,
                           nthreads)
     return out

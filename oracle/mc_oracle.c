@@ -223,13 +223,15 @@ int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const 
 /* ==================================================== e-ph channel geometry */
 /* The channel {el in, pn in, el out} with every band on one structured mesh:
  * leg 0 and leg 2 on the electron band (P1 dispersion from nodal eps), leg 1
- * on a dispersionless phonon band (energy omega0). */
+ * on the phonon band (P1 dispersion from nodal eps_ph, or the constant omega0
+ * when eps_ph is NULL). */
 typedef struct mc_geo {
     int dim;
     const int32_t* nsec;
     const double* origin;
     const double* extent;
     const double* eps;
+    const double* eps_ph;    /* phonon nodal energies, or NULL: constant omega0 */
     double omega0;
     int64_t nel;
     int32_t* conn;
@@ -238,8 +240,9 @@ typedef struct mc_geo {
 } mc_geo;
 
 static void geo_init(mc_geo* g, int dim, const int32_t* nsec, const double* origin,
-                     const double* extent, const double* eps, double omega0) {
+                     const double* extent, const double* eps, const double* eps_ph, double omega0) {
     g->dim = dim; g->nsec = nsec; g->origin = origin; g->extent = extent; g->eps = eps;
+    g->eps_ph = eps_ph;
     g->omega0 = omega0;
     g->nel = or_num_elements(dim, nsec);
     int64_t n = or_num_nodes(dim, nsec);
@@ -261,13 +264,14 @@ static void elem_forms(const mc_geo* g, int64_t e, int phonon, double kf[3][4], 
         kf[d][0] = k0;
         for (int a = 1; a <= dim; ++a) kf[d][a] = g->nodes[(int64_t)v[a] * dim + d] - k0;
     }
-    if (phonon) {
+    if (phonon && !g->eps_ph) {
         ef[0] = g->omega0;
         for (int a = 1; a <= dim; ++a) ef[a] = 0.0;
     } else {
-        double e0 = g->eps[v[0]];
+        const double* en = phonon ? g->eps_ph : g->eps;
+        double e0 = en[v[0]];
         ef[0] = e0;
-        for (int a = 1; a <= dim; ++a) ef[a] = g->eps[v[a]] - e0;
+        for (int a = 1; a <= dim; ++a) ef[a] = en[v[a]] - e0;
     }
 }
 
@@ -307,11 +311,11 @@ static void elem_box(const mc_geo* g, int64_t e, double* lo, double* hi) {
         }
 }
 
-static void elem_erange(const mc_geo* g, int64_t e, double* lo, double* hi) {
+static void elem_erange(const mc_geo* g, const double* en, int64_t e, double* lo, double* hi) {
     const int32_t* v = g->conn + e * (g->dim + 1);
     *lo = 1e300; *hi = -1e300;
     for (int a = 0; a <= g->dim; ++a) {
-        double x = g->eps[v[a]];
+        double x = en[v[a]];
         if (x < *lo) *lo = x;
         if (x > *hi) *hi = x;
     }
@@ -351,19 +355,30 @@ static double tuple_screen(const mc_geo* g, const double* D, const double* N, co
  * (ScatteringChannel.cpp:474-503).  idx[t*4 + {0,1,2,3}] = e0, e1, e2, uid.
  * Returns the tuple count (idx may be NULL to count; at most cap written). */
 int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                         const double* eps, double omega0, int32_t nu, const double* pts,
-                         int32_t nmc, double min_det, double threshold, int32_t* idx, int64_t cap) {
+                         const double* eps, const double* eps_ph, double omega0, int32_t nu,
+                         const double* pts, int32_t nmc, double min_det, double threshold,
+                         int32_t* idx, int64_t cap) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
     const int64_t nel = g.nel;
     const int nes = mc_fact(dim);
     int32_t nuv = 1;
     for (int d = 0; d < dim; ++d) nuv *= 2 * nu + 1;
     double* box = (double*)malloc(sizeof(double) * nel * 2 * dim);
     double* er = (double*)malloc(sizeof(double) * nel * 2);
+    double* pr = (double*)malloc(sizeof(double) * nel * 2);   /* phonon energy ranges */
+    double pmin = omega0, pmax = omega0;
     for (int64_t e = 0; e < nel; ++e) {
         elem_box(&g, e, box + e * 2 * dim, box + e * 2 * dim + dim);
-        elem_erange(&g, e, er + 2 * e, er + 2 * e + 1);
+        elem_erange(&g, eps, e, er + 2 * e, er + 2 * e + 1);
+        if (eps_ph) {
+            elem_erange(&g, eps_ph, e, pr + 2 * e, pr + 2 * e + 1);
+            if (e == 0 || pr[2 * e] < pmin) pmin = pr[2 * e];
+            if (e == 0 || pr[2 * e + 1] > pmax) pmax = pr[2 * e + 1];
+        } else {
+            pr[2 * e] = omega0;
+            pr[2 * e + 1] = omega0;
+        }
     }
     int64_t* counts = (int64_t*)calloc((size_t)nel + 1, sizeof(int64_t));
     for (int pass = 0; pass < 2; ++pass) {
@@ -376,8 +391,10 @@ int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, con
             int64_t w = pass == 1 ? counts[e0] : 0, cnt = 0;
             const double* b0 = box + e0 * 2 * dim;
             for (int64_t e2 = 0; e2 < nel; ++e2) {
-                if (!(er[2 * e0] + omega0 - er[2 * e2 + 1] <= 0.0 &&
-                      er[2 * e0 + 1] + omega0 - er[2 * e2] >= 0.0))
+                /* energy window of the three legs (ScatteringChannel.hpp:212-222):
+                 * first with the phonon band's whole range, then per phonon element */
+                if (!(er[2 * e0] + pmin - er[2 * e2 + 1] <= 0.0 &&
+                      er[2 * e0 + 1] + pmax - er[2 * e2] >= 0.0))
                     continue;
                 const double* b2 = box + e2 * 2 * dim;
                 for (int32_t uid = 0; uid < nuv; ++uid) {
@@ -407,6 +424,9 @@ int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, con
                             for (int d = 0; d < dim; ++d)
                                 meet &= b1[d] <= hi[d] && b1[dim + d] >= lo[d];
                             if (!meet) continue;
+                            if (!(er[2 * e0] + pr[2 * e1] - er[2 * e2 + 1] <= 0.0 &&
+                                  er[2 * e0 + 1] + pr[2 * e1 + 1] - er[2 * e2] >= 0.0))
+                                continue;
                             int64_t el[3] = {e0, e1, e2};
                             double D[36], N[4];
                             tuple_DN(&g, el, u, D, N);
@@ -433,7 +453,7 @@ int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, con
     int64_t total = 0;
     if (idx) total = counts[nel];
     else for (int64_t e = 0; e < nel; ++e) total += counts[e + 1];
-    free(counts); free(box); free(er);
+    free(counts); free(box); free(er); free(pr);
     geo_free(&g);
     return total;
 }
@@ -470,12 +490,12 @@ void or_mc_inverse_mass(int dim, double vol, double* minv) {
  * and leg l receives dir_l * result_l.  Threads accumulate disjoint tuple
  * ranges; their sums are added in thread order. */
 void or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                     const double* eps, double omega0, double amplitude, int32_t nu,
-                     const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                     const double* eps, const double* eps_ph, double omega0, double amplitude,
+                     int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
                      const double* f_el, const double* f_ph, double* out_el, double* out_ph,
                      int32_t nthreads) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
     const int nc = dim + 1;
     const int64_t nel = g.nel, len = nel * nc;
     int nt = nthreads > 0 ? nthreads : 1;
@@ -589,8 +609,9 @@ double or_relative_error_dg(int dim, int64_t nel, const double* err, const doubl
 
 typedef struct mc_dp5_user {
     int dim;
+    int dynamic;             /* 1: the state holds the phonon band after the electron band */
     const int32_t* nsec;
-    const double *origin, *extent, *eps, *pts;
+    const double *origin, *extent, *eps, *eps_ph, *pts;
     double omega0, amplitude, rtol, atol;
     int32_t nu, nmc, nthreads;
     const int32_t* idx;
@@ -601,13 +622,28 @@ typedef struct mc_dp5_user {
 static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
     (void)t;
     const mc_dp5_user* p = (const mc_dp5_user*)u;
-    or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->omega0, p->amplitude, p->nu,
-                    p->pts, p->nmc, p->idx, p->ntup, y, p->fph, dydt, NULL, p->nthreads);
+    const int64_t len = p->nel * (p->dim + 1);
+    if (p->dynamic)
+        or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->eps_ph, p->omega0,
+                        p->amplitude, p->nu, p->pts, p->nmc, p->idx, p->ntup, y, y + len, dydt,
+                        dydt + len, p->nthreads);
+    else
+        or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->eps_ph, p->omega0,
+                        p->amplitude, p->nu, p->pts, p->nmc, p->idx, p->ntup, y, p->fph, dydt, NULL,
+                        p->nthreads);
 }
 
+/* MaterialStatus::relativeError (MaterialStatus.cpp:189-196): max over the
+ * bands, fermion electrons and (when dynamic) boson phonons */
 static double mc_dp5_norm(void* u, const double* err, const double* y5) {
     const mc_dp5_user* p = (const mc_dp5_user*)u;
-    return or_relative_error_dg(p->dim, p->nel, err, y5, -1.0, p->rtol, p->atol, 0.0);
+    double e = or_relative_error_dg(p->dim, p->nel, err, y5, -1.0, p->rtol, p->atol, 0.0);
+    if (p->dynamic) {
+        const int64_t len = p->nel * (p->dim + 1);
+        double q = or_relative_error_dg(p->dim, p->nel, err + len, y5 + len, 1.0, p->rtol, p->atol, 0.0);
+        if (q >= e) e = q;
+    }
+    return e;
 }
 
 /* MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137) with the
@@ -615,20 +651,22 @@ static double mc_dp5_norm(void* u, const double* err, const double* y5) {
  * band a bath of occupation bath_n (its constraint makes its error zero),
  * fermion error norm without penalty (Band.hpp default). */
 void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                   const double* eps, double omega0, double amplitude, int32_t nu, const double* pts,
-                   int32_t nmc, const int32_t* idx, int64_t ntup, double bath_n, double* f, double t0,
-                   double t_final, double dense_dt, double rtol, double atol, double* dense_out,
-                   double* dense_t, int32_t max_dense, int32_t nthreads, or_dp5_stats* stats) {
+                   const double* eps, const double* eps_ph, double omega0, double amplitude,
+                   int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                   int32_t dynamic, double bath_n, double* f, double t0, double t_final,
+                   double dense_dt, double rtol, double atol, double* dense_out, double* dense_t,
+                   int32_t max_dense, int32_t nthreads, or_dp5_stats* stats) {
     mc_dp5_user u;
     u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
+    u.eps_ph = eps_ph; u.dynamic = dynamic;
     u.omega0 = omega0; u.amplitude = amplitude; u.rtol = rtol; u.atol = atol;
     u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
     u.nel = or_num_elements(dim, nsec);
     const int nc = dim + 1;
     u.fph = (double*)calloc((size_t)u.nel * nc, sizeof(double));
     for (int64_t e = 0; e < u.nel; ++e) u.fph[e * nc] = bath_n;
-    or_dp5_integrate(u.nel * nc, f, t0, t_final, dense_dt, mc_dp5_rhs, mc_dp5_norm, &u, dense_out,
-                     dense_t, max_dense, NULL, NULL, 0, stats);
+    or_dp5_integrate(u.nel * nc * (dynamic ? 2 : 1), f, t0, t_final, dense_dt, mc_dp5_rhs,
+                     mc_dp5_norm, &u, dense_out, dense_t, max_dense, NULL, NULL, 0, stats);
     free(u.fph);
 }
 
@@ -641,11 +679,11 @@ void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const dou
  * keeps the others as in the propagator, and subtracts dir * result of that
  * leg; the legs' inverse-massed results are summed. */
 void or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                     const double* eps, double omega0, double amplitude, int32_t nu, const double* pts,
-                     int32_t nmc, const int32_t* idx, int64_t ntup, const double* f_el,
-                     const double* f_ph, double* out_el, int32_t nthreads) {
+                     const double* eps, const double* eps_ph, double omega0, double amplitude,
+                     int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
+                     const double* f_el, const double* f_ph, double* out_el, int32_t nthreads) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, omega0);
+    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
     const int nc = dim + 1;
     const int64_t nel = g.nel, len = nel * nc;
     int nt = nthreads > 0 ? nthreads : 1;

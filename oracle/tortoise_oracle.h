@@ -205,16 +205,16 @@ int     or_mc_integrate(int dim, int nlegs, const double* D, const double* N,
                         int32_t nmc, double* out);
 /* Tuples (e0, e1, e2, umklapp id) of the e-ph channel {el in, pn in, el out}. */
 int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin,
-                         const double* extent, const double* eps, double omega0,
-                         int32_t nu, const double* pts, int32_t nmc, double min_det,
-                         double threshold, int32_t* idx, int64_t cap);
+                         const double* extent, const double* eps, const double* eps_ph,
+                         double omega0, int32_t nu, const double* pts, int32_t nmc,
+                         double min_det, double threshold, int32_t* idx, int64_t cap);
 /* LF-basis inverse mass matrix of one element. */
 void    or_mc_inverse_mass(int dim, double vol, double* minv);
 /* Propagator of the e-ph channel over a tuple list, then the inverse mass. */
 void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin,
-                        const double* extent, const double* eps, double omega0,
-                        double amplitude, int32_t nu, const double* pts, int32_t nmc,
-                        const int32_t* idx, int64_t ntup, const double* f_el,
+                        const double* extent, const double* eps, const double* eps_ph,
+                        double omega0, double amplitude, int32_t nu, const double* pts,
+                        int32_t nmc, const int32_t* idx, int64_t ntup, const double* f_el,
                         const double* f_ph, double* out_el, double* out_ph,
                         int32_t nthreads);
 /* Band::relativeError on DG coefficients [nel][dim+1]. */
@@ -222,16 +222,18 @@ double  or_relative_error_dg(int dim, int64_t nel, const double* err, const doub
                              double stat, double rtol, double atol, double penalty);
 /* DP5(4) of the electron band's DG coefficients with the tuple operator. */
 void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                      const double* eps, double omega0, double amplitude, int32_t nu,
-                      const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
-                      double bath_n, double* f, double t0, double t_final, double dense_dt,
+                      const double* eps, const double* eps_ph, double omega0, double amplitude,
+                      int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                      int64_t ntup, int32_t dynamic, double bath_n, double* f, double t0,
+                      double t_final, double dense_dt,
                       double rtol, double atol, double* dense_out, double* dense_t,
                       int32_t max_dense, int32_t nthreads, or_dp5_stats* stats);
 /* ScatteringChannel::scatteringRates of the electron band (legs 0 and 2). */
 void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
-                        const double* eps, double omega0, double amplitude, int32_t nu,
-                        const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
-                        const double* f_el, const double* f_ph, double* out_el, int32_t nthreads);
+                        const double* eps, const double* eps_ph, double omega0, double amplitude,
+                        int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                        int64_t ntup, const double* f_el, const double* f_ph, double* out_el,
+                        int32_t nthreads);
 /* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
 void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
 

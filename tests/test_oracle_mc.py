@@ -145,8 +145,9 @@ def _mesh(ns, dim=2):
     return w, w.eps()
 
 
-def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9):
-    """Every (e0, e1, e2, u) of a small 2D mesh, D and N written out here."""
+def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9, eps_ph=None):
+    """Every (e0, e1, e2, u) of a small 2D mesh, D and N written out here
+    (phonon band: nodal energies eps_ph, or the constant omega0)."""
     dim = 2
     X = oracle.mesh_nodes(dim, list(w.nsec), list(w.origin), list(w.extent))
     conn = oracle.mesh_elements(dim, list(w.nsec))
@@ -159,24 +160,33 @@ def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9):
     ef = np.stack([eps[conn[:, 1]] - eps[conn[:, 0]], eps[conn[:, 2]] - eps[conn[:, 0]]], 1)
     e0v = eps[conn[:, 0]]
     emin, emax = eps[conn].min(1), eps[conn].max(1)
+    if eps_ph is None:
+        pf = np.zeros((nel, 2))
+        p0 = np.full(nel, w.omega0)
+        pmin = pmax = np.full(nel, w.omega0)
+    else:
+        pf = np.stack([eps_ph[conn[:, 1]] - eps_ph[conn[:, 0]], eps_ph[conn[:, 2]] - eps_ph[conn[:, 0]]], 1)
+        p0 = eps_ph[conn[:, 0]]
+        pmin, pmax = eps_ph[conn].min(1), eps_ph[conn].max(1)
     one = np.zeros(9)
     one[[0, 3, 6]] = math.sqrt(2.0)
     keep = set()
     for a in range(nel):
         for c in range(nel):
-            if not (emin[a] + w.omega0 - emax[c] <= 0 and emax[a] + w.omega0 - emin[c] >= 0):
-                continue
             for b in range(nel):
+                if not (emin[a] + pmin[b] - emax[c] <= 0 and emax[a] + pmax[b] - emin[c] >= 0):
+                    continue
                 for uid, u in enumerate(us):
                     D = np.zeros((3, 6))
                     D[:2, 0:2] = kf[a]
                     D[:2, 2:4] = kf[b]
                     D[:2, 4:6] = -kf[c]
                     D[2, 0:2] = ef[a]
+                    D[2, 2:4] = pf[b]
                     D[2, 4:6] = -ef[c]
                     N = np.zeros(3)
                     N[:2] = -(k0[a] + k0[b] - k0[c]) - np.array(u) * np.array(w.extent)
-                    N[2] = -(e0v[a] + w.omega0 - e0v[c])
+                    N[2] = -(e0v[a] + p0[b] - e0v[c])
                     DC = D[:, 3:6]
                     if abs(np.linalg.det(DC)) < min_det:
                         continue
@@ -341,3 +351,55 @@ def test_scattering_rates_of_constant_bands():
                                 f, fph)
         expect = 2 * (n + c) * U0 + 2 * (1 + n - c) * U2
         assert np.abs(r - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+def _acoustic(w):
+    """A dispersive phonon band: omega(k) = 0.05 + 0.02 |k|, nodal."""
+    X = oracle.mesh_nodes(2, list(w.nsec), list(w.origin), list(w.extent))
+    return 0.05 + 0.02 * np.sqrt((X ** 2).sum(1))
+
+
+def test_tuples_with_dispersive_phonons_match_brute_force():
+    w, eps = _mesh(4)
+    om = _acoustic(w)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts, eps_ph=om)
+    got = set(map(tuple, idx.tolist()))
+    assert len(got) == len(idx)
+    assert got == _brute_tuples(w, eps, 1, pts, eps_ph=om)
+    # a different phonon dispersion changes the list
+    base = set(map(tuple, oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1,
+                                               pts).tolist()))
+    assert got != base
+
+
+def test_dynamical_phonons_conserve_number_and_energy():
+    """Electrons and a dispersive phonon band that evolves: electron number
+    and total energy (electrons + phonons) balance to roundoff; phonons are
+    created and destroyed."""
+    w, eps = _mesh(8)
+    om = _acoustic(w)
+    pts = oracle.mc_points(2, 3, 50, seed=1)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts, eps_ph=om)
+    rng = np.random.default_rng(4)
+    fe = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f()) + 0.01 * rng.standard_normal((128, 3))
+    fp = oracle.mc_nodal_to_dg(2, w.nsec, 1.0 / np.expm1(om / 0.05))      # hot, non-uniform phonons
+    el, ph = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                                 fe, fp, phonon_out=True, eps_ph=om)
+    num = _integrals(w, eps, el)
+    assert abs(num.sum()) <= 1e-13 * np.abs(num).sum()
+    e_el = _integrals(w, eps, el, oracle.mc_nodal_to_dg(2, w.nsec, eps)).sum()
+    e_ph = _integrals(w, eps, ph, oracle.mc_nodal_to_dg(2, w.nsec, om)).sum()
+    assert abs(e_el + e_ph) <= 1e-12 * (abs(e_el) + abs(e_ph))
+    assert abs(_integrals(w, eps, ph).sum()) > 1e-6 * np.abs(_integrals(w, eps, ph)).sum()
+    # DP5 of both bands: number and total energy stay constant over the run
+    y0 = np.concatenate([fe, fp])
+    r = oracle.dp5_mc_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx, 0.0,
+                          y0, 0.0, 5.0, 0.5, 1e-8, 1e-12, dynamic=True, eps_ph=om)
+    y = r["f"]
+    def energy(s):
+        return (_integrals(w, eps, s[:128], oracle.mc_nodal_to_dg(2, w.nsec, eps)).sum() +
+                _integrals(w, eps, s[128:], oracle.mc_nodal_to_dg(2, w.nsec, om)).sum())
+    assert abs(_integrals(w, eps, y[:128]).sum() - _integrals(w, eps, y0[:128]).sum()) <= 1e-13
+    assert abs(energy(y) - energy(y0)) <= 1e-12 * abs(energy(y0))
+    assert np.abs(y[128:] - fp).max() > 1e-9      # the phonons moved

@@ -212,7 +212,9 @@ tg_status   tg_relative_error(tg_ctx* ctx, const double* err, const double* pop,
  *
  * State layout: member b, element e, linear-form coefficient a at
  * ptr[(b*nel + e)*(dim+1) + a] -- the reference Function's element view
- * [f(v0), f(v1)-f(v0), ...] (Function.hpp:326-344).
+ * [f(v0), f(v1)-f(v0), ...] (Function.hpp:326-344).  With dynamic_phonons the
+ * phonon band's block follows the electron band's for every member and
+ * bath_n may be NULL.
  * ================================================================== */
 typedef struct tg_tuple_ctx tg_tuple_ctx;
 
@@ -229,7 +231,10 @@ typedef struct tg_tuple_desc {
     double  min_det;        /* minimumDeterminant (Material default 1e-9) */
     double  threshold;      /* scattElemeThreshold (Material default 1e-9) */
     int32_t max_batch;      /* members per propagate call */
-    int32_t reserved;
+    int32_t dynamic_phonons;/* 0: the phonon band is a bath of occupation bath_n per member;
+                               1: it evolves -- states and outputs hold both bands,
+                               [member][band: electron, phonon][element][dim+1] */
+    const double* eps_ph;   /* phonon nodal energies (a P1 dispersion), or NULL: omega0 */
 } tg_tuple_desc;
 
 tg_status   tg_tuple_create(const tg_tuple_desc* desc, tg_tuple_ctx** out);

@@ -202,3 +202,67 @@ def test_scattering_rates_match_oracle(dim, ns):
         assert np.abs(R[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
         assert np.array_equal(op.rates(F[b], bath[b]), R[b])
     op.close()
+
+
+def _acoustic(w):
+    X = oracle.mesh_nodes(w.dim, list(w.nsec), list(w.origin), list(w.extent))
+    return 0.05 + 0.02 * np.sqrt((X ** 2).sum(1))
+
+
+def test_dispersive_phonon_bath_matches_oracle():
+    """A dispersive phonon band (omega = 0.05 + 0.02 |k|) held as a bath: the
+    phonon energy forms enter D, N and the energy windows."""
+    print("[gpu] tuple operator, dispersive phonons", flush=True)
+    w, eps, pts = _case(2, 12)
+    om = _acoustic(w)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts, eps_ph=om)
+    op = tg.TupleOperator.create(w, max_batch=2, eps_ph=om)
+    op.set_points(pts)
+    assert np.array_equal(op.indices(), idx)
+    F, bath = _state(w, 2, seed=21)
+    out = op.propagate(F, bath)
+    for b in range(2):
+        fph = np.zeros_like(F[b])
+        fph[:, 0] = bath[b]
+        ref = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                                  F[b], fph, nthreads=8, eps_ph=om)
+        assert np.abs(out[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
+    op.close()
+
+
+@pytest.mark.parametrize("dim,ns", [(2, 10), (3, 3)])
+def test_dynamical_phonons_match_oracle(dim, ns):
+    """The phonon band evolves: both bands' derivatives against the oracle,
+    batch members bitwise equal to single runs, and DP5 of both bands with
+    the max-over-bands error norm (identical step sequence)."""
+    print(f"[gpu] tuple operator, dynamical phonons {dim}D ns={ns}", flush=True)
+    w, eps, pts = _case(dim, ns)
+    om = _acoustic(w)
+    idx = oracle.mc_tuples_eph(dim, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts, eps_ph=om)
+    op = tg.TupleOperator.create(w, max_batch=3, eps_ph=om, dynamic=True)
+    op.set_points(pts)
+    assert np.array_equal(op.indices(), idx)
+    rng = np.random.default_rng(ns)
+    fe = oracle.mc_nodal_to_dg(dim, w.nsec, w.initial_f())
+    fp = oracle.mc_nodal_to_dg(dim, w.nsec, 1.0 / np.expm1(om / 0.05))
+    S = np.stack([np.stack([fe + 0.01 * rng.standard_normal(fe.shape),
+                            fp * (1 + 0.1 * rng.standard_normal(fp.shape))]) for _ in range(3)])
+    out = op.propagate(S)
+    for b in range(3):
+        el, ph = oracle.mc_apply_eph(dim, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts,
+                                     idx, S[b, 0], S[b, 1], nthreads=8, phonon_out=True, eps_ph=om)
+        assert np.abs(out[b, 0] - el).max() <= 1e-12 * np.abs(el).max(), b
+        assert np.abs(out[b, 1] - ph).max() <= 1e-12 * np.abs(ph).max(), b
+    for b in (0, 2):
+        assert np.array_equal(op.propagate(S[b]), out[b])
+    if dim == 2:
+        fg, dg, st = op.dp5(S[0], 0.0, 5.0, 0.5, 1e-8, 1e-12, max_dense=10)
+        r = oracle.dp5_mc_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx, 0.0,
+                              S[0].reshape(-1, dim + 1), 0.0, 5.0, 0.5, 1e-8, 1e-12, max_dense=10,
+                              dynamic=True, eps_ph=om)
+        assert st["attempted"] == r["stats"].attempted and st["accepted"] == r["stats"].accepted
+        for a, bref in zip(dg, r["dense"]):
+            assert np.abs(a.reshape(-1, dim + 1) - bref).max() <= 1e-10 * np.abs(bref).max()
+        fb, _, sb = op.dp5(S[:2], 0.0, 5.0, 0.5, 1e-8, 1e-12)
+        assert np.array_equal(fb[0], fg) and sb[0]["attempted"] == st["attempted"]
+    op.close()

@@ -96,7 +96,7 @@ class TupleDesc(C.Structure):
                 ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("omega0", C.c_double),
                 ("amplitude", C.c_double), ("n_umklapp", C.c_int32), ("num_mc", C.c_int32),
                 ("min_det", C.c_double), ("threshold", C.c_double), ("max_batch", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("dynamic_phonons", C.c_int32), ("eps_ph", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -405,10 +405,11 @@ class TupleOperator:
         self.dim = dim
         self.nel = nel
         self.max_batch = max_batch
+        self.dynamic = False
 
     @classmethod
     def create(cls, workload, n_umklapp=1, num_mc=50, min_det=1e-9, threshold=1e-9,
-               max_batch=1, amplitude=None, eps=None):
+               max_batch=1, amplitude=None, eps=None, eps_ph=None, dynamic=False):
         L = load()
         w = workload
         e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
@@ -426,6 +427,11 @@ class TupleOperator:
         d.min_det = min_det
         d.threshold = threshold
         d.max_batch = max_batch
+        d.dynamic_phonons = 1 if dynamic else 0
+        ph = None
+        if eps_ph is not None:
+            ph = np.ascontiguousarray(eps_ph, np.float64)
+            d.eps_ph = ph.ctypes.data_as(C.POINTER(C.c_double))
         h = C.c_void_p()
         rc = L.tg_tuple_create(C.byref(d), C.byref(h))
         if rc != TG_OK:
@@ -437,6 +443,7 @@ class TupleOperator:
         op = cls(h, w.dim, nel, max_batch)
         op.rows = L.tg_tuple_point_rows(w.dim, 3)
         op.num_mc = num_mc
+        op.dynamic = bool(dynamic)
         return op
 
     def _check(self, rc):
@@ -484,16 +491,20 @@ class TupleOperator:
     def set_amplitude(self, amplitude):
         self._check(load().tg_tuple_set_amplitude(self._h, amplitude))
 
-    def propagate(self, f, bath_n, time_kernel=False):
-        """f: [nel, dim+1] or [batch, nel, dim+1]; bath_n scalar or [batch]."""
+    def propagate(self, f, bath_n=None, time_kernel=False):
+        """f: [nel, dim+1] or [batch, nel, dim+1] (with dynamical phonons
+        [2, nel, dim+1] or [batch, 2, nel, dim+1]: electrons, then phonons);
+        bath_n scalar or [batch] (unused with dynamical phonons)."""
         f = np.ascontiguousarray(f, np.float64)
-        single = f.ndim == 2
+        single = f.ndim == (3 if self.dynamic else 2)
         fb = f[None] if single else f
         batch = fb.shape[0]
-        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        bn = None
+        if bath_n is not None:
+            bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
         out = np.zeros_like(fb)
-        self._check(load().tg_tuple_propagate(self._h, _p(fb), _p(bn), _p(out), self.nel, batch,
-                                              TG_TIME_KERNEL if time_kernel else 0))
+        self._check(load().tg_tuple_propagate(self._h, _p(fb), None if bn is None else _p(bn), _p(out),
+                                              self.nel, batch, TG_TIME_KERNEL if time_kernel else 0))
         return out[0] if single else out
 
     def rates(self, f, bath_n):
@@ -521,18 +532,21 @@ class TupleOperator:
     def stream(self):
         return load().tg_tuple_stream(self._h)
 
-    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n, max_dense=0, quiet=True):
-        """DP5(4) of DG states f0 [nel, dim+1] or [batch, nel, dim+1]; returns
-        (f_last, dense [(batch,) max_dense, nel, dim+1], stats)."""
+    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, max_dense=0, quiet=True):
+        """DP5(4) of DG states f0 [nel, dim+1] or [batch, nel, dim+1] (with
+        dynamical phonons [(batch,) 2, nel, dim+1]); returns (f_last,
+        dense [(batch,) max_dense, ...], stats)."""
         f = np.ascontiguousarray(f0, np.float64).copy()
-        single = f.ndim == 2
+        single = f.ndim == (3 if self.dynamic else 2)
         fb = f[None] if single else f
         batch = fb.shape[0]
-        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        bn = None
+        if bath_n is not None:
+            bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
         dense = np.zeros((batch, max_dense) + fb.shape[1:]) if max_dense else None
         st = (StepStats * batch)()
         rc = load().tg_tuple_dp5(self._h, _p(fb), self.nel, batch, t0, t1, dense_dt, rtol, atol,
-                                 _p(bn), TG_QUIET if quiet else 0,
+                                 None if bn is None else _p(bn), TG_QUIET if quiet else 0,
                                  None if dense is None else _p(dense), max_dense, st)
         self._check(rc)
         stats = [{k: getattr(s, k) for k, _ in StepStats._fields_} for s in st]

@@ -77,6 +77,17 @@ struct tg_tuple_ctx {
     double* d_q2 = nullptr;      // leg-2 moments, [nq][ntup]
     bool have_points = false;
 
+    // dynamical phonon band: three leg lists with two other elements each and
+    // three blocks of moments per entry
+    bool dyn = false;
+    double* d_eps_ph = nullptr;  // phonon nodal energies (dispersive band), or null
+    double pmin = 0, pmax = 0;
+    int64_t* d_doff[3] = {};
+    int32_t* d_dothA[3] = {};
+    int32_t* d_dothB[3] = {};
+    int64_t* d_dpos[3] = {};
+    double* d_dq[3] = {};
+
     double* d_f = nullptr;       // staging [max_batch][nel][dim+1]
     double* d_ft = nullptr;      // element-major copy of the states [nel][batch][dim+1]
     double* d_out = nullptr;
@@ -133,16 +144,22 @@ struct Geo {
     int32_t nsec[3];
     double origin[3], extent[3];
     double omega0, vol;
+    double pmin, pmax;    // phonon energy range over the zone
     int32_t nu, nuv, nmc, rows;
     const double* eps;
+    const double* eps_ph; // phonon nodal energies, or null: the constant omega0
     const double* pts;
-    const double* etab;   // per element: electron kf, ef, box lo, box hi, energy range
+    const double* etab;   // per element: momentum forms, energy forms, box, energy ranges
 };
 
-// Element table record: DIM*(DIM+1) momentum form coefficients, DIM+1 energy
-// form coefficients, DIM box minima, DIM box maxima, energy min and max.
+// Element table record: DIM*(DIM+1) momentum form coefficients, DIM+1
+// electron energy form coefficients, DIM box minima, DIM box maxima, the
+// electron energy min and max, DIM+1 phonon energy form coefficients, the
+// phonon energy min and max.
 template <int DIM>
-constexpr int etab_stride() { return DIM * (DIM + 1) + (DIM + 1) + 2 * DIM + 2; }
+constexpr int etab_stride() { return DIM * (DIM + 1) + (DIM + 1) + 2 * DIM + 2 + (DIM + 1) + 2; }
+template <int DIM>
+constexpr int etab_ph() { return DIM * (DIM + 1) + (DIM + 1) + 2 * DIM + 2; }
 
 // Everything from here to the moment kernel decides which tuples and which MC
 // points count, so it is evaluated without contraction and in the operation
@@ -187,7 +204,7 @@ __device__ inline void elem_forms_compute(const Geo& g, int64_t e, double kf[DIM
 }
 
 // Forms of element e read back from the table; the phonon band has the same
-// momentum forms and the constant energy omega0.
+// momentum forms and its own energy form.
 template <int DIM>
 __device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double kf[DIM][DIM + 1],
                                   double ef[DIM + 1]) {
@@ -197,7 +214,7 @@ __device__ inline void elem_forms(const Geo& g, int64_t e, bool phonon, double k
 #pragma unroll
         for (int a = 0; a <= DIM; ++a) kf[d][a] = r[d * (DIM + 1) + a];
 #pragma unroll
-    for (int a = 0; a <= DIM; ++a) ef[a] = phonon ? (a == 0 ? g.omega0 : 0.0) : r[DIM * (DIM + 1) + a];
+    for (int a = 0; a <= DIM; ++a) ef[a] = phonon ? r[etab_ph<DIM>() + a] : r[DIM * (DIM + 1) + a];
 }
 
 template <int DIM>
@@ -379,6 +396,13 @@ __device__ inline void elem_erange(const Geo& g, int64_t e, double* elo, double*
     *ehi = r[1];
 }
 
+template <int DIM>
+__device__ inline void elem_prange(const Geo& g, int64_t e, double* plo, double* phi) {
+    const double* r = g.etab + e * etab_stride<DIM>() + etab_ph<DIM>() + DIM + 1;
+    *plo = r[0];
+    *phi = r[1];
+}
+
 // One thread per element: the table record (forms, bounding box, energy range).
 template <int DIM>
 __global__ void elem_table_kernel(Geo g, int64_t nel, double* __restrict__ tab) {
@@ -415,6 +439,28 @@ __global__ void elem_table_kernel(Geo g, int64_t nel, double* __restrict__ tab) 
     for (int d = 0; d < DIM; ++d) { b[d] = lo[d]; b[DIM + d] = hi[d]; }
     b[2 * DIM] = elo;
     b[2 * DIM + 1] = ehi;
+    // phonon energy form and range (a P1 band, or the constant omega0)
+    double* q = r + etab_ph<DIM>();
+    if (g.eps_ph) {
+        const double p0 = g.eps_ph[v[0]];
+        q[0] = p0;
+        double plo = 1e300, phi = -1e300;
+#pragma unroll
+        for (int a = 0; a <= DIM; ++a) {
+            if (a) q[a] = g.eps_ph[v[a]] - p0;
+            const double en = g.eps_ph[v[a]];
+            plo = en < plo ? en : plo;
+            phi = en > phi ? en : phi;
+        }
+        q[DIM + 1] = plo;
+        q[DIM + 2] = phi;
+    } else {
+        q[0] = g.omega0;
+#pragma unroll
+        for (int a = 1; a <= DIM; ++a) q[a] = 0.0;
+        q[DIM + 1] = g.omega0;
+        q[DIM + 2] = g.omega0;
+    }
 }
 
 // Walks the tuples of the pair (e0, e2) in (uid, e1) order; calls emit(e1, uid)
@@ -426,7 +472,9 @@ __device__ inline int32_t pair_tuples(const Geo& g, int64_t e0, int64_t e2, doub
     double lo0[DIM], hi0[DIM], lo2[DIM], hi2[DIM], el0, eh0, el2, eh2;
     elem_erange<DIM>(g, e0, &el0, &eh0);
     elem_erange<DIM>(g, e2, &el2, &eh2);
-    if (!(el0 + g.omega0 - eh2 <= 0.0 && eh0 + g.omega0 - el2 >= 0.0)) return 0;
+    // energy window of the three legs (ScatteringChannel.hpp:212-222): first
+    // with the phonon band's whole range, then per phonon element
+    if (!(el0 + g.pmin - eh2 <= 0.0 && eh0 + g.pmax - el2 >= 0.0)) return 0;
     elem_box_erange<DIM>(g, e0, lo0, hi0, &el0, &eh0);
     elem_box_erange<DIM>(g, e2, lo2, hi2, &el2, &eh2);
     int32_t cnt = 0;
@@ -460,6 +508,9 @@ __device__ inline int32_t pair_tuples(const Geo& g, int64_t e0, int64_t e2, doub
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) meet &= lo1[d] <= hi[d] && hi1[d] >= lo[d];
                 if (!meet) continue;
+                double pl1, ph1;
+                elem_prange<DIM>(g, e1, &pl1, &ph1);
+                if (!(el0 + pl1 - eh2 <= 0.0 && eh0 + ph1 - el2 >= 0.0)) continue;
                 int64_t el[3] = {e0, e1, e2};
                 double D[DIM + 1][3 * DIM], N[DIM + 1];
                 tuple_DN<DIM>(g, el, u, D, N);
@@ -578,6 +629,59 @@ __global__ __launch_bounds__(256) void moments_kernel(Geo g, const int32_t* __re
         q0[(int64_t)k * ntup + t] = m0[k] * w;
         q2[(int64_t)k * ntup + p] = m2[k] * w;
     }
+}
+
+// Dynamical phonons: the integrand F2 + P F2 - F0 P - F0 F2 (fermion
+// electrons, boson phonons; the cubic terms of the two products cancel) is a
+// sum of pairwise products, so per leg l three blocks of moments carry it:
+//   block 0: phi_j(x_l) phi_a(x0) phi_b(x2)
+//   block 1: phi_j(x_l) phi_a(x1) phi_b(x2)
+//   block 2: phi_j(x_l) phi_a(x0) phi_b(x1)
+// at k = ((block*NC + j)*NC + a)*NC + b of the leg's list, entry pos[t].
+template <int DIM, int LEG>
+__global__ __launch_bounds__(256) void moments_dyn_kernel(Geo g, const int32_t* __restrict__ tup, int64_t ntup,
+                                                            const int64_t* __restrict__ pos, double* __restrict__ q) {
+    constexpr int NC = DIM + 1;
+    constexpr int NQ = NC * NC * NC;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntup) return;
+    const int64_t el[3] = {tup[t * 4], tup[t * 4 + 1], tup[t * 4 + 2]};
+    int u[3];
+    umklapp_vec<DIM>(g.nu, tup[t * 4 + 3], u);
+    double D[DIM + 1][3 * DIM], N[DIM + 1], inv[DIM + 1][DIM + 1];
+    tuple_DN<DIM>(g, el, u, D, N);
+    double det = dc_inverse<DIM>(D, inv);
+    double m[3 * NQ];
+#pragma unroll
+    for (int k = 0; k < 3 * NQ; ++k) m[k] = 0.0;
+    for (int32_t mm = 0; mm < g.nmc; ++mm) {
+        double K[3][DIM];
+        if (!shell_point<DIM>(g, D, N, inv, mm, K)) continue;
+        double p[3][NC];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            p[l][0] = 1.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) p[l][1 + d] = K[l][d];
+        }
+        constexpr int X[3] = {0, 1, 0}, Y[3] = {2, 2, 1};
+#pragma unroll
+        for (int blk = 0; blk < 3; ++blk)
+#pragma unroll
+            for (int a = 0; a < NC; ++a)
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    const double pab = p[X[blk]][a] * p[Y[blk]][b];
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) m[((blk * NC + j) * NC + a) * NC + b] += p[LEG][j] * pab;
+                }
+    }
+    double amp = 1.0;
+    for (int leg = 0; leg < 3; ++leg) amp *= g.vol * (double)tg::factorial(DIM);
+    const double w = amp / shell_divisor(DIM, det, g.nmc);
+    const int64_t P = LEG == 0 ? t : pos[t];
+#pragma unroll
+    for (int k = 0; k < 3 * NQ; ++k) q[(int64_t)k * ntup + P] = m[k] * w;
 }
 
 #pragma clang fp contract(on)
@@ -789,17 +893,204 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_kernel(
     }
 }
 
-// ft[e][b][a] = f[b][e][a]
+// ft[band][e][b][a] = f[b][band][e][a]: the members of one element are
+// contiguous (nbands = 1 for the phonon bath, 2 with a dynamical phonon band)
 __global__ void transpose_state_kernel(const double* __restrict__ f, double* __restrict__ ft, int64_t nel,
-                                       int32_t batch, int nc) {
+                                       int32_t batch, int nc, int nbands) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = (int64_t)batch * nel * nc;
+    const int64_t total = (int64_t)batch * nbands * nel * nc;
     if (i >= total) return;
     const int a = (int)(i % nc);
-    const int64_t be = i / nc;
-    const int64_t e = be % nel, b = be / nel;
-    ft[(e * batch + b) * nc + a] = f[i];
+    const int64_t r = i / nc;
+    const int64_t e = r % nel;
+    const int64_t bb = r / nel;
+    const int band = (int)(bb % nbands);
+    const int64_t b = bb / nbands;
+    ft[(((int64_t)band * nel + e) * batch + b) * nc + a] = f[i];
 }
+
+// The dynamical-phonon propagator: one workgroup per (element, chunk of MB
+// members) sums the element's leg-0 and leg-2 lists into its electron
+// coefficients and its leg-1 list into its phonon coefficients, slot-ordered
+// as the bath kernel (SL slots per round, thread h = slot h, members in the
+// inner loop, so a batch member is bitwise equal to its single evaluation).
+// Entry of leg l with coefficients c0 (leg 0), p (leg 1), c2 (leg 2):
+//   t_j = sum_ab Q0[j][a][b] (d_a0 c2_b - c0_a c2_b) + Q1[j][a][b] p_a c2_b - Q2[j][a][b] c0_a p_b
+// added with sign +, +, - for l = 0, 1, 2.  States are read from the
+// element-major copy ft[band][e][b][a]; out is [b][band][e][a].
+struct DynLists {
+    const int64_t* off[3];
+    const int32_t* othA[3];
+    const int32_t* othB[3];
+    const double* q[3];
+};
+
+template <int DIM>
+constexpr int dyn_slots() { return DIM == 2 ? 128 : 64; }
+
+template <int DIM>
+constexpr size_t dyn_lds_bytes() {
+    constexpr int NC = DIM + 1, NQ = 3 * NC * NC * NC, SL = dyn_slots<DIM>();
+    return sizeof(double) * (size_t)NQ * SL + 2 * sizeof(int32_t) * SL;
+}
+
+template <int DIM, int MB>
+__global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t nel, int64_t ntup, DynLists L,
+                                                                        const double* __restrict__ ft, int32_t batch,
+                                                                        double amplitude, const double* __restrict__ minv_g,
+                                                                        double* __restrict__ out) {
+    constexpr int NC = DIM + 1;
+    constexpr int NQ = 3 * NC * NC * NC;
+    constexpr int SL = dyn_slots<DIM>();
+    constexpr int PER = (NQ * SL + kApplyThreads - 1) / kApplyThreads;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* sQ = reinterpret_cast<double*>(smem_raw);                  // [NQ][SL]
+    int32_t* sA = reinterpret_cast<int32_t*>(sQ + (size_t)NQ * SL);    // [SL]
+    int32_t* sB = sA + SL;                                             // [SL]
+    __shared__ double sOwn[2][MB][NC];
+    __shared__ double sTot[2][MB][NC];
+    const int64_t e = blockIdx.x;
+    const int mb = blockIdx.y * MB;
+    const int nm = min(MB, batch - mb);
+    const int h = threadIdx.x;
+    const bool slot_thread = h < SL;
+    const double* fe = ft;                                  // electrons [e][b][a]
+    const double* fp = ft + (int64_t)nel * batch * NC;      // phonons
+    if (threadIdx.x < 2 * MB * NC) {
+        const int band = threadIdx.x / (MB * NC), r = threadIdx.x % (MB * NC);
+        const int b = r / NC, a = r % NC;
+        sOwn[band][b][a] = b < nm ? (band ? fp : fe)[((int64_t)e * batch + mb + b) * NC + a] : 0.0;
+    }
+    double acc[2][MB][NC];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+            for (int a = 0; a < NC; ++a) acc[q][b][a] = 0.0;
+    // rounds: the leg-0 list, the leg-2 list, then the leg-1 list
+    constexpr int ORDER[3] = {0, 2, 1};
+    int64_t first[3], len[3], nrl[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        first[i] = L.off[ORDER[i]][e];
+        len[i] = L.off[ORDER[i]][e + 1] - first[i];
+        nrl[i] = (len[i] + SL - 1) / SL;
+    }
+    const int64_t nr = nrl[0] + nrl[1] + nrl[2];
+    double stage[PER];
+    int32_t stage_a = 0, stage_b = 0;
+    auto locate = [&](int64_t r, int& li, int64_t& base, int& cnt) {
+        li = r < nrl[0] ? 0 : (r < nrl[0] + nrl[1] ? 1 : 2);
+        const int64_t rr = li == 0 ? r : (li == 1 ? r - nrl[0] : r - nrl[0] - nrl[1]);
+        base = first[li] + rr * SL;
+        cnt = (int)min((int64_t)SL, first[li] + len[li] - base);
+    };
+    auto fetch = [&](int64_t r) {
+        int li, cnt;
+        int64_t base;
+        locate(r, li, base, cnt);
+        const int leg = ORDER[li];
+        const double* q = L.q[leg];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int idx = threadIdx.x + p * kApplyThreads;
+            const int k = idx / SL, hh = idx % SL;
+            stage[p] = (idx < NQ * SL && hh < cnt) ? q[(int64_t)k * ntup + base + hh] : 0.0;
+        }
+        stage_a = (int)threadIdx.x < cnt ? L.othA[leg][base + threadIdx.x] : 0;
+        stage_b = (int)threadIdx.x < cnt ? L.othB[leg][base + threadIdx.x] : 0;
+    };
+    if (nr > 0) fetch(0);
+    for (int64_t r = 0; r < nr; ++r) {
+        int li, cnt;
+        int64_t base;
+        locate(r, li, base, cnt);
+        const int leg = ORDER[li];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int idx = threadIdx.x + p * kApplyThreads;
+            if (idx < NQ * SL) sQ[idx] = stage[p];
+        }
+        if (slot_thread) {
+            sA[threadIdx.x] = stage_a;
+            sB[threadIdx.x] = stage_b;
+        }
+        __syncthreads();
+        if (r + 1 < nr) fetch(r + 1);
+        if (slot_thread && h < cnt) {
+            const int64_t oa = sA[h], ob = sB[h];
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                if (b < nm) {
+                    double c0[NC], pp[NC], c2[NC];
+#pragma unroll
+                    for (int a = 0; a < NC; ++a) {
+                        const int64_t ia = (oa * batch + mb + b) * NC + a, ib = (ob * batch + mb + b) * NC + a;
+                        if (leg == 0) { c0[a] = sOwn[0][b][a]; pp[a] = fp[ia]; c2[a] = fe[ib]; }
+                        else if (leg == 1) { c0[a] = fe[ia]; pp[a] = sOwn[1][b][a]; c2[a] = fe[ib]; }
+                        else { c0[a] = fe[ia]; pp[a] = fp[ib]; c2[a] = sOwn[0][b][a]; }
+                    }
+                    double G0[NC][NC], G1[NC][NC], G2[NC][NC];
+#pragma unroll
+                    for (int a = 0; a < NC; ++a)
+#pragma unroll
+                        for (int bb = 0; bb < NC; ++bb) {
+                            G0[a][bb] = -(c0[a] * c2[bb]);
+                            G1[a][bb] = pp[a] * c2[bb];
+                            G2[a][bb] = -(c0[a] * pp[bb]);
+                        }
+#pragma unroll
+                    for (int bb = 0; bb < NC; ++bb) G0[0][bb] += c2[bb];
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int a = 0; a < NC; ++a)
+#pragma unroll
+                            for (int bb = 0; bb < NC; ++bb) {
+                                t = fma(sQ[(((0 * NC + j) * NC + a) * NC + bb) * SL + h], G0[a][bb], t);
+                                t = fma(sQ[(((1 * NC + j) * NC + a) * NC + bb) * SL + h], G1[a][bb], t);
+                                t = fma(sQ[(((2 * NC + j) * NC + a) * NC + bb) * SL + h], G2[a][bb], t);
+                            }
+                        if (leg == 1) acc[1][b][j] = acc[1][b][j] + t;
+                        else if (leg == 0) acc[0][b][j] = acc[0][b][j] + t;
+                        else acc[0][b][j] = acc[0][b][j] - t;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // slot-ordered sums per (band, member, basis) through the staging area
+    double* sRed = sQ;   // [SL][NC]
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            if (b < nm) {
+                if (slot_thread)
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) sRed[h * NC + j] = acc[q][b][j];
+                __syncthreads();
+                if (threadIdx.x < NC) {
+                    double total = 0.0;
+                    for (int hh = 0; hh < SL; ++hh) total += sRed[hh * NC + threadIdx.x];
+                    sTot[q][b][threadIdx.x] = total;
+                }
+                __syncthreads();
+            }
+        }
+    if (threadIdx.x < 2 * nm * NC) {
+        const int q = threadIdx.x / (nm * NC), r = threadIdx.x % (nm * NC);
+        const int b = r / NC, j = r % NC;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) v = fma(sTot[q][b][a], minv_g[a * NC + j], v);
+        out[(((int64_t)(mb + b) * 2 + q) * nel + e) * NC + j] = amplitude * v;
+    }
+}
+
 
 #pragma clang fp contract(on)
 
@@ -818,6 +1109,9 @@ Geo make_geo(const tg_tuple_ctx* c) {
     g.nmc = c->nmc;
     g.rows = c->rows;
     g.eps = c->d_eps;
+    g.eps_ph = c->d_eps_ph;
+    g.pmin = c->pmin;
+    g.pmax = c->pmax;
     g.pts = c->d_pts;
     g.etab = c->d_etab;
     return g;
@@ -853,6 +1147,40 @@ tg_status build_lists(tg_tuple_ctx* c) {
         hipLaunchKernelGGL(fill_leg_lists, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, c->stream,
                            c->d_tup, T, c->d_pos2, c->d_oth0, c->d_oth2);
         TT_HIP(c, hipGetLastError());
+    }
+    if (!c->dyn) return TG_OK;
+    // dynamical phonons: the leg-1 list too, and per list the two other
+    // elements in leg order (leg 0: e1, e2; leg 1: e0, e2; leg 2: e0, e1)
+    std::vector<int64_t> off1(nel + 1, 0), pos1(T > 0 ? T : 1), pos0(T > 0 ? T : 1);
+    for (int64_t t = 0; t < T; ++t) ++off1[tup[t * 4 + 1] + 1];
+    for (int64_t e = 0; e < nel; ++e) off1[e + 1] += off1[e];
+    std::vector<int64_t> fill1(off1.begin(), off1.end() - 1);
+    for (int64_t t = 0; t < T; ++t) {
+        pos1[t] = fill1[tup[t * 4 + 1]]++;
+        pos0[t] = t;
+    }
+    const std::vector<int64_t>* offs[3] = {&off0, &off1, &off2};
+    const std::vector<int64_t>* poss[3] = {&pos0, &pos1, &pos2};
+    const int nqd = 3 * nq;
+    for (int l = 0; l < 3; ++l) {
+        std::vector<int32_t> a((size_t)(T > 0 ? T : 1)), b((size_t)(T > 0 ? T : 1));
+        const int la = l == 0 ? 1 : 0, lb = l == 2 ? 1 : 2;
+        for (int64_t t = 0; t < T; ++t) {
+            const int64_t q = (*poss[l])[t];
+            a[q] = tup[t * 4 + la];
+            b[q] = tup[t * 4 + lb];
+        }
+        if ((s = dalloc(c, &c->d_doff[l], nel + 1)) != TG_OK) return s;
+        if ((s = dalloc(c, &c->d_dothA[l], T)) != TG_OK) return s;
+        if ((s = dalloc(c, &c->d_dothB[l], T)) != TG_OK) return s;
+        if ((s = dalloc(c, &c->d_dpos[l], T)) != TG_OK) return s;
+        if ((s = dalloc(c, &c->d_dq[l], (size_t)nqd * T)) != TG_OK) return s;
+        TT_HIP(c, hipMemcpy(c->d_doff[l], offs[l]->data(), sizeof(int64_t) * (nel + 1), hipMemcpyHostToDevice));
+        if (T) {
+            TT_HIP(c, hipMemcpy(c->d_dothA[l], a.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+            TT_HIP(c, hipMemcpy(c->d_dothB[l], b.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+            TT_HIP(c, hipMemcpy(c->d_dpos[l], poss[l]->data(), sizeof(int64_t) * T, hipMemcpyHostToDevice));
+        }
     }
     return TG_OK;
 }
@@ -902,6 +1230,16 @@ tg_status moments(tg_tuple_ctx* c) {
     if (c->ntup == 0) return TG_OK;
     Geo g = make_geo(c);
     dim3 grid((unsigned)((c->ntup + 255) / 256));
+    if (c->dyn) {
+#define TT_MDYN(D, L) hipLaunchKernelGGL((moments_dyn_kernel<D, L>), grid, dim3(256), 0, c->stream, g, c->d_tup, \
+                                         c->ntup, c->d_dpos[L], c->d_dq[L])
+        if (c->dim == 2) { TT_MDYN(2, 0); TT_MDYN(2, 1); TT_MDYN(2, 2); }
+        else { TT_MDYN(3, 0); TT_MDYN(3, 1); TT_MDYN(3, 2); }
+#undef TT_MDYN
+        TT_HIP(c, hipGetLastError());
+        TT_HIP(c, hipStreamSynchronize(c->stream));
+        return TG_OK;
+    }
     if (c->dim == 2)
         hipLaunchKernelGGL(moments_kernel<2>, grid, dim3(256), 0, c->stream, g, c->d_tup, c->ntup, c->d_pos2,
                            c->d_q0, c->d_q2);
@@ -930,16 +1268,49 @@ tg_status launch_propagate(tg_tuple_ctx* c, const double* d_ft, const double* d_
     return TG_OK;
 }
 
+template <int DIM, int MB>
+tg_status launch_dyn(tg_tuple_ctx* c, const double* d_ft, double* d_out, int32_t batch) {
+    constexpr size_t lds = dyn_lds_bytes<DIM>();
+    static bool attr_set = false;
+    if (!attr_set) {
+        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_dyn_kernel<DIM, MB>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    DynLists L{};
+    for (int l = 0; l < 3; ++l) {
+        L.off[l] = c->d_doff[l];
+        L.othA[l] = c->d_dothA[l];
+        L.othB[l] = c->d_dothB[l];
+        L.q[l] = c->d_dq[l];
+    }
+    dim3 grid((unsigned)c->nel, (unsigned)((batch + MB - 1) / MB));
+    hipLaunchKernelGGL((propagate_dyn_kernel<DIM, MB>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
+                       L, d_ft, batch, c->amplitude, c->d_minv, d_out);
+    TT_HIP(c, hipGetLastError());
+    return TG_OK;
+}
+
 template <bool RATES = false>
 tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_bath, double* d_out,
                            int32_t batch) {
     const int nc = c->dim + 1;
-    const int64_t total = (int64_t)batch * c->nel * nc;
+    const int nbands = c->dyn ? 2 : 1;
+    const int64_t total = (int64_t)batch * nbands * c->nel * nc;
     if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(transpose_state_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, d_f,
-                       c->d_ft, c->nel, batch, nc);
+                       c->d_ft, c->nel, batch, nc, nbands);
     TT_HIP(c, hipGetLastError());
     tg_status s;
+    if (c->dyn) {
+        if (c->dim == 2)
+            s = batch == 1 ? launch_dyn<2, 1>(c, c->d_ft, d_out, batch) : launch_dyn<2, kApplyMembers>(c, c->d_ft, d_out, batch);
+        else
+            s = batch == 1 ? launch_dyn<3, 1>(c, c->d_ft, d_out, batch) : launch_dyn<3, kApplyMembers>(c, c->d_ft, d_out, batch);
+        if (s != TG_OK) return s;
+        if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
+        return TG_OK;
+    }
     if (c->dim == 2)
         s = batch == 1 ? launch_propagate<2, 1, RATES>(c, c->d_ft, d_bath, d_out, batch)
                        : launch_propagate<2, kApplyMembers, RATES>(c, c->d_ft, d_bath, d_out, batch);
@@ -959,15 +1330,16 @@ tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_b
 // (penalty 0, the Band default); the controller divides by (dim+1)(dim+2) nel.
 constexpr int kRelThreads = 256;
 
-template <int NC>
+template <int NC, bool BOSON>
 __global__ __launch_bounds__(kRelThreads) void dg_relerr_partial_kernel(int64_t nel, const double* __restrict__ err,
-                                                                          const double* __restrict__ pop, double rtol,
+                                                                          const double* __restrict__ pop,
+                                                                          int64_t stride, int64_t offset, double rtol,
                                                                           double atol, int nblk,
                                                                           double* __restrict__ part) {
     __shared__ double sw[kRelThreads / 64];
     const int b = blockIdx.y;
-    const double* ev = err + (int64_t)b * nel * NC;
-    const double* pv = pop + (int64_t)b * nel * NC;
+    const double* ev = err + (int64_t)b * stride + offset;
+    const double* pv = pop + (int64_t)b * stride + offset;
     double s = 0.0;
     for (int64_t e = (int64_t)blockIdx.x * kRelThreads + threadIdx.x; e < nel; e += (int64_t)nblk * kRelThreads) {
         double sr = 0.0, sr2 = 0.0;
@@ -975,8 +1347,14 @@ __global__ __launch_bounds__(kRelThreads) void dg_relerr_partial_kernel(int64_t 
         for (int a = 0; a < NC; ++a) {
             const double y = pv[e * NC] + (a ? pv[e * NC + a] : 0.0);
             const double x = ev[e * NC] + (a ? ev[e * NC + a] : 0.0);
-            const double m = fabs(y) < fabs(1.0 - y) ? fabs(y) : fabs(1.0 - y);
-            const double r = x / (rtol * m + atol);
+            double scale;
+            if constexpr (BOSON) {
+                scale = rtol * y + atol;
+            } else {
+                const double m = fabs(y) < fabs(1.0 - y) ? fabs(y) : fabs(1.0 - y);
+                scale = rtol * m + atol;
+            }
+            const double r = x / scale;
             sr += r;
             sr2 += r * r;
         }
@@ -993,11 +1371,19 @@ __global__ __launch_bounds__(kRelThreads) void dg_relerr_partial_kernel(int64_t 
     }
 }
 
-__global__ void dg_relerr_final_kernel(const double* __restrict__ part, int nblk, int B, double* __restrict__ norms) {
+// per member: the error-norm sum, or the larger of the two bands' sums
+// (MaterialStatus::relativeError takes the max over bands, MaterialStatus.cpp:189-196)
+__global__ void dg_relerr_final_kernel(const double* __restrict__ part, const double* __restrict__ part2, int nblk,
+                                       int B, double* __restrict__ norms) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     double s = 0.0;
     for (int i = 0; i < nblk; ++i) s += part[(int64_t)b * nblk + i];
+    if (part2) {
+        double q = 0.0;
+        for (int i = 0; i < nblk; ++i) q += part2[(int64_t)b * nblk + i];
+        if (q >= s) s = q;
+    }
     norms[b] = s;
 }
 
@@ -1015,6 +1401,7 @@ struct TupleDp5 {
     tg::MemberDev* md = nullptr;
     double* norms = nullptr;
     double* part = nullptr;
+    double* part2 = nullptr;     // phonon band's partials (dynamical phonons)
     int32_t* running = nullptr;
     int32_t* running_host = nullptr;
     double* dense = nullptr;
@@ -1024,7 +1411,7 @@ struct TupleDp5 {
 void tuple_dp5_free(tg_tuple_ctx* c) {
     TupleDp5* w = (TupleDp5*)c->dp5;
     if (!w) return;
-    void* bufs[] = {w->ys, w->y5, w->errv, w->y, w->h, w->acc, w->md, w->norms, w->part, w->running, w->dense};
+    void* bufs[] = {w->ys, w->y5, w->errv, w->y, w->h, w->acc, w->md, w->norms, w->part, w->part2, w->running, w->dense};
     for (int j = 0; j < 7; ++j)
         if (w->k[j]) (void)hipFree(w->k[j]);
     for (void* p : bufs)
@@ -1040,7 +1427,7 @@ tg_status tuple_dp5_ensure(tg_tuple_ctx* c) {
     if (!w) return fail(c, TG_ERR_NOMEM, "DP5 work");
     c->dp5 = w;
     w->cap = c->max_batch;
-    w->n = c->nel * (c->dim + 1);
+    w->n = c->nel * (c->dim + 1) * (c->dyn ? 2 : 1);
     w->nblk = (int)std::min<int64_t>(64, (c->nel + kRelThreads - 1) / kRelThreads);
     const size_t sz = (size_t)w->n * w->cap;
     tg_status s;
@@ -1050,7 +1437,8 @@ tg_status tuple_dp5_ensure(tg_tuple_ctx* c) {
         (s = dalloc(c, &w->errv, sz)) != TG_OK || (s = dalloc(c, &w->y, sz)) != TG_OK ||
         (s = dalloc(c, &w->h, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->acc, (size_t)w->cap)) != TG_OK ||
         (s = dalloc(c, &w->md, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->norms, (size_t)w->cap)) != TG_OK ||
-        (s = dalloc(c, &w->part, (size_t)w->cap * w->nblk)) != TG_OK || (s = dalloc(c, &w->running, 1)) != TG_OK)
+        (s = dalloc(c, &w->part, (size_t)w->cap * w->nblk)) != TG_OK ||
+        (s = dalloc(c, &w->part2, (size_t)w->cap * w->nblk)) != TG_OK || (s = dalloc(c, &w->running, 1)) != TG_OK)
         return s;
     TT_HIP(c, hipHostMalloc((void**)&w->running_host, sizeof(int32_t), hipHostMallocDefault));
     return TG_OK;
@@ -1098,13 +1486,22 @@ tg_status tuple_attempt(tg_tuple_ctx* c, TupleDp5* w, int32_t B, const tg::Ctl& 
         hipLaunchKernelGGL(error_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, w->y5, w->errv);
         TT_HIP(c, hipGetLastError());
     }
-    if (c->dim == 2)
-        hipLaunchKernelGGL(dg_relerr_partial_kernel<3>, dim3(w->nblk, B), dim3(kRelThreads), 0, st, c->nel, w->errv,
-                           w->y5, rtol, atol, w->nblk, w->part);
-    else
-        hipLaunchKernelGGL(dg_relerr_partial_kernel<4>, dim3(w->nblk, B), dim3(kRelThreads), 0, st, c->nel, w->errv,
-                           w->y5, rtol, atol, w->nblk, w->part);
-    hipLaunchKernelGGL(dg_relerr_final_kernel, cgrd, dim3(256), 0, st, w->part, w->nblk, B, w->norms);
+    {
+        const int64_t band = c->nel * (c->dim + 1);
+#define TT_REL(NC, BOS, OFF, PART)                                                                              \
+    hipLaunchKernelGGL((dg_relerr_partial_kernel<NC, BOS>), dim3(w->nblk, B), dim3(kRelThreads), 0, st, c->nel, \
+                       w->errv, w->y5, w->n, OFF, rtol, atol, w->nblk, PART)
+        if (c->dim == 2) {
+            TT_REL(3, false, 0, w->part);
+            if (c->dyn) TT_REL(3, true, band, w->part2);
+        } else {
+            TT_REL(4, false, 0, w->part);
+            if (c->dyn) TT_REL(4, true, band, w->part2);
+        }
+#undef TT_REL
+    }
+    hipLaunchKernelGGL(dg_relerr_final_kernel, cgrd, dim3(256), 0, st, w->part, c->dyn ? w->part2 : (double*)nullptr,
+                       w->nblk, B, w->norms);
     hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc, (double*)nullptr,
                        w->running);
     TT_HIP(c, hipGetLastError());
@@ -1127,6 +1524,12 @@ extern "C" const char* tg_tuple_last_error(const tg_tuple_ctx* c) { return c ? c
 extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
     if (!c) return;
     tuple_dp5_free(c);
+    for (int l = 0; l < 3; ++l) {
+        void* dl[] = {c->d_doff[l], c->d_dothA[l], c->d_dothB[l], c->d_dpos[l], c->d_dq[l]};
+        for (void* p : dl)
+            if (p) (void)hipFree(p);
+    }
+    if (c->d_eps_ph) (void)hipFree(c->d_eps_ph);
     void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
                     c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft};
     for (void* p : bufs)
@@ -1171,6 +1574,7 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
     c->min_det = d->min_det;
     c->threshold = d->threshold;
     c->max_batch = d->max_batch;
+    c->dyn = d->dynamic_phonons != 0;
     {
         double v = 1.0;
         for (int q = 0; q < d->dim; ++q) v *= d->extent[q] / (double)d->nsec[q];
@@ -1205,15 +1609,30 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
     if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "stream/event: %s", hipGetErrorString(e)));
     for (int64_t i = 0; i < c->n; ++i)
         if (!std::isfinite(d->eps[i])) return bail(fail(c, TG_ERR_NONFINITE, "eps[%lld] is not finite", (long long)i));
+    c->pmin = c->pmax = d->omega0;
+    if (d->eps_ph) {
+        for (int64_t i = 0; i < c->n; ++i) {
+            if (!std::isfinite(d->eps_ph[i]))
+                return bail(fail(c, TG_ERR_NONFINITE, "eps_ph[%lld] is not finite", (long long)i));
+            if (i == 0 || d->eps_ph[i] < c->pmin) c->pmin = d->eps_ph[i];
+            if (i == 0 || d->eps_ph[i] > c->pmax) c->pmax = d->eps_ph[i];
+        }
+    }
+    const int nbands = c->dyn ? 2 : 1;
     const int nc = d->dim + 1;
     tg_status s;
     if ((s = dalloc(c, &c->d_eps, c->n)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_pts, (size_t)c->rows * c->nmc)) != TG_OK) return bail(s);
-    if ((s = dalloc(c, &c->d_f, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
-    if ((s = dalloc(c, &c->d_out, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
-    if ((s = dalloc(c, &c->d_ft, (size_t)c->max_batch * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_f, (size_t)c->max_batch * nbands * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_out, (size_t)c->max_batch * nbands * c->nel * nc)) != TG_OK) return bail(s);
+    if ((s = dalloc(c, &c->d_ft, (size_t)c->max_batch * nbands * c->nel * nc)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_minv, 16)) != TG_OK) return bail(s);
     if ((s = dalloc(c, &c->d_bath, c->max_batch)) != TG_OK) return bail(s);
+    if (d->eps_ph) {
+        if ((s = dalloc(c, &c->d_eps_ph, c->n)) != TG_OK) return bail(s);
+        e = hipMemcpy(c->d_eps_ph, d->eps_ph, sizeof(double) * c->n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(fail(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    }
     e = hipMemcpy(c->d_eps, d->eps, sizeof(double) * c->n, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->d_minv, c->minv, sizeof(double) * 16, hipMemcpyHostToDevice);
@@ -1298,19 +1717,21 @@ extern "C" tg_status tg_tuple_set_amplitude(tg_tuple_ctx* c, double amplitude) {
 template <bool RATES>
 tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out, int64_t nel, int32_t batch,
                       int32_t flags) {
-    if (!c || !f || !bath_n || !out) return TG_ERR_ARG;
+    if (!c || !f || (!bath_n && !c->dyn) || !out) return TG_ERR_ARG;
+    if (RATES && c->dyn) return fail(c, TG_ERR_STATE, "scattering rates are evaluated with a phonon bath");
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
         return fail(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
     c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
-    const size_t len = (size_t)c->nel * (c->dim + 1) * batch;
+    const size_t len = (size_t)c->nel * (c->dim + 1) * batch * (c->dyn ? 2 : 1);
     if (flags & TG_PTR_DEVICE) {
         tg_status s = propagate_device<RATES>(c, f, bath_n, out, batch);
         if (s != TG_OK) return s;
     } else {
         TT_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
-        TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
+        if (bath_n)
+            TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
         tg_status s = propagate_device<RATES>(c, c->d_f, c->d_bath, c->d_out, batch);
         if (s != TG_OK) return s;
         TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
@@ -1390,7 +1811,7 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
                                   double dense_dt, double rtol, double atol, const double* bath_n, int32_t flags,
                                   double* dense_out, int32_t max_dense, tg_step_stats* stats) {
     using namespace tg;
-    if (!c || !f || !bath_n) return TG_ERR_ARG;
+    if (!c || !f || (!bath_n && !c->dyn)) return TG_ERR_ARG;
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
@@ -1404,8 +1825,9 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     hipStream_t st = c->stream;
     const size_t vec = (size_t)w->n * batch;
     TT_HIP(c, hipMemcpyAsync(w->y, f, sizeof(double) * vec, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch,
-                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    if (bath_n)
+        TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch,
+                                 dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     double* d_dense = nullptr;
     if (dense_out) {
         if (dev) {

@@ -262,8 +262,9 @@ tg_status   tg_tuple_propagate(tg_tuple_ctx* ctx, const double* f, const double*
                                int64_t nel, int32_t batch, int32_t flags);
 /* ScatteringChannel::scatteringRates(status, band) (ScatteringChannel.cpp:544-612)
  * of the electron band: the sum over its legs (0 and 2) of the rate with that
- * leg's forms replaced by constants, after the inverse mass.  Same layout and
- * flags as tg_tuple_propagate. */
+ * leg's forms replaced by constants, after the inverse mass.  Same input and
+ * flags as tg_tuple_propagate; out holds the electron band only,
+ * [batch][nel][dim+1]. */
 tg_status   tg_tuple_rates(tg_tuple_ctx* ctx, const double* f, const double* bath_n, double* out,
                            int64_t nel, int32_t batch, int32_t flags);
 tg_status   tg_tuple_last_kernel_ms(tg_tuple_ctx* ctx, double* ms);

@@ -72,6 +72,13 @@ def test_host_example_ee_channel_matches_abi():
     assert np.linalg.norm(fp - f) > 1e-6 * np.linalg.norm(f)
 
 
+def _second_draw_seed(nmc):
+    """The host layer draws MC points from one splitmix64 stream: the rates
+    call takes the first draw (3 numbers per point in 2D), the propagate call
+    the second, which starts 3 * nmc increments later."""
+    return (3 * nmc * 0x9E3779B97F4A7C15) % 2 ** 64
+
+
 def test_host_example_reference_operator_matches_abi():
     """Material::referenceOperator: MaterialTimeStatus::propagate draws the MC
     points (splitmix64 from seed 0, the reference's simplex folds) and runs
@@ -89,6 +96,7 @@ def test_host_example_reference_operator_matches_abi():
     steps = int(lines[i0].split()[1])
     t0_num = float(lines[i0 + 2].split("|")[1])
     num_end = float(lines[-1].split("|")[1])
+    en_end = float(lines[-1].split("|")[2])
     assert abs(num_end - t0_num) <= 1e-12 * t0_num
     w = wl.Workload("square16", 2, (16, 16))
     op = tg.TupleOperator.create(w, n_umklapp=1, num_mc=50, max_batch=1)
@@ -100,7 +108,52 @@ def test_host_example_reference_operator_matches_abi():
         return vol * (g[:, 0] + (g[:, 1] + g[:, 2]) / 3).sum()
     rr = op.rates(f0, w.bath_n)
     assert abs(integral(rr) - rates_int) <= 1e-12 * abs(rates_int)
+    # MaterialTimeStatus::propagate redraws the points (the second draw)
+    op.set_points(oracle.mc_points(2, 3, 50, _second_draw_seed(50)))
     f, dense, st = op.dp5(f0, 0.0, 2.0, 0.02, 1e-8, 1e-12, w.bath_n, max_dense=100)
     assert st["attempted"] == steps
-    assert abs(integral(f) - num_end) <= 1e-12 * num_end
+    assert abs(integral(dense[-1]) - num_end) <= 1e-12 * num_end
+    # the electron energy is not conserved: it checks the dynamics themselves
+    conn = oracle.mesh_elements(2, [16, 16])
+    eps_dg = oracle.mc_nodal_to_dg(2, w.nsec, w.eps())
+    vol_m = np.linalg.inv(oracle.mc_inverse_mass(2, vol))
+    en = float(np.einsum("ea,ab,eb->", eps_dg, vol_m, dense[-1]))
+    assert abs(en - en_end) <= 1e-11 * abs(en_end)
+    op.close()
+
+
+def test_host_example_dynamical_phonons_matches_abi():
+    """Material::dynamicPhonons with the reference operator: the phonon band
+    evolves with the electrons.  Same step count, electron number and total
+    energy (electrons + phonons) as the C-ABI run; total energy conserved."""
+    assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
+    print("[gpu] host example 12 sections t=2 dynamical phonons", flush=True)
+    r = subprocess.run([EXE, "12", "2", "1", "0", "1", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    i0 = next(i for i, x in enumerate(lines) if x.startswith("steps"))
+    steps = int(lines[i0].split()[1])
+    e0_el = float(lines[i0 + 2].split("|")[2])
+    fin = next(x for x in lines if x.startswith("final")).split("|")
+    num_end, en_end = float(fin[1]), float(fin[2])
+    ph = next(x for x in lines if x.startswith("phonons")).split()
+    ph_num, ph_en = float(ph[1]), float(ph[3])
+    w = wl.Workload("square12", 2, (12, 12))
+    op = tg.TupleOperator.create(w, n_umklapp=1, num_mc=50, max_batch=1, dynamic=True)
+    op.set_points(oracle.mc_points(2, 3, 50, 0))    # the tuple list comes from the first draw
+    op.set_points(oracle.mc_points(2, 3, 50, _second_draw_seed(50)))
+    fe = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    fp = np.zeros_like(fe)
+    fp[:, 0] = w.bath_n
+    f, dense, st = op.dp5(np.stack([fe, fp]), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    assert st["attempted"] == steps
+    vol = (2 * np.pi / 12) ** 2 / 2
+
+    def integral(g):
+        return vol * (g[:, 0] + (g[:, 1] + g[:, 2]) / 3).sum()
+    last = dense[-1]                 # the dense output at t = 2, the host's last entry
+    assert abs(integral(last[0]) - num_end) <= 1e-12 * num_end
+    assert abs(integral(last[1]) - ph_num) <= 1e-12 * abs(ph_num)
+    # total energy: electrons (exact P1 x P1) plus phonons (omega0 x number)
+    assert abs((en_end + ph_en) - (e0_el + 0.05 * integral(fp))) <= 1e-10 * abs(e0_el)
     op.close()

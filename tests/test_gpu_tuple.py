@@ -266,3 +266,18 @@ def test_dynamical_phonons_match_oracle(dim, ns):
         fb, _, sb = op.dp5(S[:2], 0.0, 5.0, 0.5, 1e-8, 1e-12)
         assert np.array_equal(fb[0], fg) and sb[0]["attempted"] == st["attempted"]
     op.close()
+
+
+def test_scattering_rates_with_dynamical_phonons_match_oracle():
+    w, eps, pts = _case(2, 10)
+    om = _acoustic(w)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts, eps_ph=om)
+    op = tg.TupleOperator.create(w, max_batch=2, eps_ph=om, dynamic=True)
+    op.set_points(pts)
+    fe = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    fp = oracle.mc_nodal_to_dg(2, w.nsec, 1.0 / np.expm1(om / 0.05))
+    R = op.rates(np.stack([fe, fp]))
+    ref = oracle.mc_rates_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx,
+                              fe, fp, nthreads=8, eps_ph=om)
+    assert np.abs(R - ref).max() <= 1e-12 * np.abs(ref).max()
+    op.close()

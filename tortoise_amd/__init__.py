@@ -507,15 +507,18 @@ class TupleOperator:
                                               self.nel, batch, TG_TIME_KERNEL if time_kernel else 0))
         return out[0] if single else out
 
-    def rates(self, f, bath_n):
-        """ScatteringChannel::scatteringRates of the electron band, same layout."""
+    def rates(self, f, bath_n=None):
+        """ScatteringChannel::scatteringRates of the electron band: [(batch,) nel, dim+1]."""
         f = np.ascontiguousarray(f, np.float64)
-        single = f.ndim == 2
+        single = f.ndim == (3 if self.dynamic else 2)
         fb = f[None] if single else f
         batch = fb.shape[0]
-        bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
-        out = np.zeros_like(fb)
-        self._check(load().tg_tuple_rates(self._h, _p(fb), _p(bn), _p(out), self.nel, batch, 0))
+        bn = None
+        if bath_n is not None:
+            bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
+        out = np.zeros((batch, self.nel, self.dim + 1))
+        self._check(load().tg_tuple_rates(self._h, _p(fb), None if bn is None else _p(bn), _p(out),
+                                          self.nel, batch, 0))
         return out[0] if single else out
 
     def propagate_device(self, f, bath_n, out, batch, time_kernel=False):

@@ -20,7 +20,10 @@ namespace tg {
 constexpr int kPlainThreads = 256;
 constexpr int kPlainRows = 128;      // rows per workgroup (32 per wave)
 constexpr int kPlainCols = 128;      // columns per sweep (64 lanes x 2)
-constexpr int kPlainChunk = 2048;    // columns per workgroup
+#ifndef TG_PLAIN_CHUNK
+#define TG_PLAIN_CHUNK 2048
+#endif
+constexpr int kPlainChunk = TG_PLAIN_CHUNK;   // columns per workgroup
 constexpr int kLdAlign = 64;         // K row stride alignment in elements
 
 struct Buffer {

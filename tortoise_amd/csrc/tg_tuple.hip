@@ -934,7 +934,12 @@ constexpr size_t dyn_lds_bytes() {
     return sizeof(double) * (size_t)NQ * SL + 2 * sizeof(int32_t) * SL;
 }
 
-template <int DIM, int MB>
+// RATES: ScatteringChannel::scatteringRates of the electron band with the
+// rate leg's forms replaced by constants (factor 2 at the end):
+//   leg-0 entries:  G0[a][b] = d_a0 c2_b, G2[a][b] = d_a0 p_b        (2 (P + F2))
+//   leg-2 entries:  G0[a][b] = -c0_a d_b0, G1[a][b] = (d_a0 + p_a) d_b0  (2 (1 + P - F0))
+// the leg-1 list is skipped and out is the electron band only, [b][e][a].
+template <int DIM, int MB, bool RATES>
 __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t nel, int64_t ntup, DynLists L,
                                                                         const double* __restrict__ ft, int32_t batch,
                                                                         double amplitude, const double* __restrict__ minv_g,
@@ -977,6 +982,7 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t ne
         len[i] = L.off[ORDER[i]][e + 1] - first[i];
         nrl[i] = (len[i] + SL - 1) / SL;
     }
+    if (RATES) nrl[2] = 0;
     const int64_t nr = nrl[0] + nrl[1] + nrl[2];
     double stage[PER];
     int32_t stage_a = 0, stage_b = 0;
@@ -1032,16 +1038,37 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t ne
                         else { c0[a] = fe[ia]; pp[a] = fp[ib]; c2[a] = sOwn[0][b][a]; }
                     }
                     double G0[NC][NC], G1[NC][NC], G2[NC][NC];
+                    if constexpr (!RATES) {
 #pragma unroll
-                    for (int a = 0; a < NC; ++a)
+                        for (int a = 0; a < NC; ++a)
 #pragma unroll
-                        for (int bb = 0; bb < NC; ++bb) {
-                            G0[a][bb] = -(c0[a] * c2[bb]);
-                            G1[a][bb] = pp[a] * c2[bb];
-                            G2[a][bb] = -(c0[a] * pp[bb]);
+                            for (int bb = 0; bb < NC; ++bb) {
+                                G0[a][bb] = -(c0[a] * c2[bb]);
+                                G1[a][bb] = pp[a] * c2[bb];
+                                G2[a][bb] = -(c0[a] * pp[bb]);
+                            }
+#pragma unroll
+                        for (int bb = 0; bb < NC; ++bb) G0[0][bb] += c2[bb];
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < NC; ++a)
+#pragma unroll
+                            for (int bb = 0; bb < NC; ++bb) G0[a][bb] = G1[a][bb] = G2[a][bb] = 0.0;
+                        if (leg == 0) {
+#pragma unroll
+                            for (int bb = 0; bb < NC; ++bb) {
+                                G0[0][bb] = c2[bb];
+                                G2[0][bb] = pp[bb];
+                            }
+                        } else {
+#pragma unroll
+                            for (int a = 0; a < NC; ++a) {
+                                G0[a][0] = -c0[a];
+                                G1[a][0] = pp[a];
+                            }
+                            G1[0][0] += 1.0;
                         }
-#pragma unroll
-                    for (int bb = 0; bb < NC; ++bb) G0[0][bb] += c2[bb];
+                    }
 #pragma unroll
                     for (int j = 0; j < NC; ++j) {
                         double t = 0.0;
@@ -1054,7 +1081,7 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t ne
                                 t = fma(sQ[(((2 * NC + j) * NC + a) * NC + bb) * SL + h], G2[a][bb], t);
                             }
                         if (leg == 1) acc[1][b][j] = acc[1][b][j] + t;
-                        else if (leg == 0) acc[0][b][j] = acc[0][b][j] + t;
+                        else if (leg == 0 || RATES) acc[0][b][j] = acc[0][b][j] + t;
                         else acc[0][b][j] = acc[0][b][j] - t;
                     }
                 }
@@ -1081,13 +1108,14 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t ne
                 __syncthreads();
             }
         }
-    if (threadIdx.x < 2 * nm * NC) {
+    if (threadIdx.x < (RATES ? 1 : 2) * nm * NC) {
         const int q = threadIdx.x / (nm * NC), r = threadIdx.x % (nm * NC);
         const int b = r / NC, j = r % NC;
         double v = 0.0;
 #pragma unroll
         for (int a = 0; a < NC; ++a) v = fma(sTot[q][b][a], minv_g[a * NC + j], v);
-        out[(((int64_t)(mb + b) * 2 + q) * nel + e) * NC + j] = amplitude * v;
+        if (RATES) out[((int64_t)(mb + b) * nel + e) * NC + j] = 2.0 * (amplitude * v);
+        else out[(((int64_t)(mb + b) * 2 + q) * nel + e) * NC + j] = amplitude * v;
     }
 }
 
@@ -1268,12 +1296,12 @@ tg_status launch_propagate(tg_tuple_ctx* c, const double* d_ft, const double* d_
     return TG_OK;
 }
 
-template <int DIM, int MB>
+template <int DIM, int MB, bool RATES>
 tg_status launch_dyn(tg_tuple_ctx* c, const double* d_ft, double* d_out, int32_t batch) {
     constexpr size_t lds = dyn_lds_bytes<DIM>();
     static bool attr_set = false;
     if (!attr_set) {
-        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_dyn_kernel<DIM, MB>),
+        TT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&propagate_dyn_kernel<DIM, MB, RATES>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr_set = true;
     }
@@ -1285,7 +1313,7 @@ tg_status launch_dyn(tg_tuple_ctx* c, const double* d_ft, double* d_out, int32_t
         L.q[l] = c->d_dq[l];
     }
     dim3 grid((unsigned)c->nel, (unsigned)((batch + MB - 1) / MB));
-    hipLaunchKernelGGL((propagate_dyn_kernel<DIM, MB>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
+    hipLaunchKernelGGL((propagate_dyn_kernel<DIM, MB, RATES>), grid, dim3(kApplyThreads), lds, c->stream, c->nel, c->ntup,
                        L, d_ft, batch, c->amplitude, c->d_minv, d_out);
     TT_HIP(c, hipGetLastError());
     return TG_OK;
@@ -1304,9 +1332,11 @@ tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_b
     tg_status s;
     if (c->dyn) {
         if (c->dim == 2)
-            s = batch == 1 ? launch_dyn<2, 1>(c, c->d_ft, d_out, batch) : launch_dyn<2, kApplyMembers>(c, c->d_ft, d_out, batch);
+            s = batch == 1 ? launch_dyn<2, 1, RATES>(c, c->d_ft, d_out, batch)
+                           : launch_dyn<2, kApplyMembers, RATES>(c, c->d_ft, d_out, batch);
         else
-            s = batch == 1 ? launch_dyn<3, 1>(c, c->d_ft, d_out, batch) : launch_dyn<3, kApplyMembers>(c, c->d_ft, d_out, batch);
+            s = batch == 1 ? launch_dyn<3, 1, RATES>(c, c->d_ft, d_out, batch)
+                           : launch_dyn<3, kApplyMembers, RATES>(c, c->d_ft, d_out, batch);
         if (s != TG_OK) return s;
         if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
         return TG_OK;
@@ -1718,13 +1748,13 @@ template <bool RATES>
 tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out, int64_t nel, int32_t batch,
                       int32_t flags) {
     if (!c || !f || (!bath_n && !c->dyn) || !out) return TG_ERR_ARG;
-    if (RATES && c->dyn) return fail(c, TG_ERR_STATE, "scattering rates are evaluated with a phonon bath");
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
         return fail(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
     c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
     const size_t len = (size_t)c->nel * (c->dim + 1) * batch * (c->dyn ? 2 : 1);
+    const size_t olen = RATES ? (size_t)c->nel * (c->dim + 1) * batch : len;   // rates: the electron band
     if (flags & TG_PTR_DEVICE) {
         tg_status s = propagate_device<RATES>(c, f, bath_n, out, batch);
         if (s != TG_OK) return s;
@@ -1734,7 +1764,7 @@ tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, do
             TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
         tg_status s = propagate_device<RATES>(c, c->d_f, c->d_bath, c->d_out, batch);
         if (s != TG_OK) return s;
-        TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+        TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * olen, hipMemcpyDeviceToHost, c->stream));
     }
     TT_HIP(c, hipStreamSynchronize(c->stream));
     if (c->time_kernel) TT_HIP(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));

@@ -9,7 +9,9 @@
 // With ref=1 the collision operator is the reference's own: element tuples
 // and the Monte-Carlo delta-shell integral per tuple on the DG coefficients.
 //
-// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0] [ref=0]
+// With ref=1 dyn=1 the phonon band evolves instead of acting as a bath.
+//
+// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0] [ref=0] [dyn=0]
 #include <TORTOISE>
 
 #include <cmath>
@@ -24,6 +26,7 @@ int main(int argc, char** argv) {
     const bool quiet = argc > 3 ? std::atoi(argv[3]) != 0 : true;
     const Real gEE = argc > 4 ? std::atof(argv[4]) : 0.0;
     const bool ref = argc > 5 ? std::atoi(argv[5]) != 0 : false;
+    const bool dynPh = argc > 6 ? std::atoi(argv[6]) != 0 : false;
     const Real pi = M_PI;
 
     Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
@@ -32,6 +35,7 @@ int main(int argc, char** argv) {
     mat.addBand("el:C", -1, fermion, mesh, [](Point<2> k) { return 0.5 * k.squaredNorm(); });
     mat.addBand("pn:opt", 0, boson, mesh, 0.05);
     mat.referenceOperator = ref;
+    mat.dynamicPhonons = dynPh;
     for (auto sc : mat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 1)) {
         sc->amplitude = 0.1 * 0.1;      // g^2
         sc->deltaBroadening = 0.01;
@@ -73,5 +77,9 @@ int main(int argc, char** argv) {
     const Function<2>& fEnd = dyn.back()["el:C"];
     std::printf("final %.4f | %.15e | %.15e\n", dyn.argback(), fEnd.integrate(),
                 fEnd.integrate(mat["el:C"].dispersion));
+    if (mat.dynamicPhonons) {
+        const Function<2>& pEnd = dyn.back()["pn:opt"];
+        std::printf("phonons %.15e | %.15e\n", pEnd.integrate(), pEnd.integrate(mat["pn:opt"].dispersion));
+    }
     return 0;
 }

@@ -325,6 +325,9 @@ public:
     // true: the reference's own operator -- element tuples and the Monte-Carlo
     // delta-shell integral per tuple on the DG coefficients (tg_tuple_*)
     bool referenceOperator = false;
+    // with referenceOperator: the phonon band evolves (its derivative is
+    // returned and integrated) instead of acting as a bath
+    bool dynamicPhonons = false;
     uint64_t mcSeed = 0;     // start of the MC point stream (the reference uses std::rand)
 
     Material(std::string n, const Region<NDim>& r) : name(std::move(n)), region(&r) {}
@@ -472,10 +475,9 @@ public:
             if (c.legs == 3) eph = &c;
         if (!eph) throw std::runtime_error("the reference operator needs an e-ph channel");
         const Band<NDim>& el = (*this)[eph->electronBand];
-        const auto omega = (*this)[eph->phononBand].dispersion.toNodal();
-        for (Real w : omega)
-            if (std::fabs(w - omega[0]) > 1e-12 * std::fabs(omega[0]))
-                throw std::runtime_error("phonon band must be dispersionless");
+        omegaPh_ = (*this)[eph->phononBand].dispersion.toNodal();
+        bool flat = true;
+        for (Real w : omegaPh_) flat = flat && w == omegaPh_[0];
         eps_ = el.dispersion.toNodal();
         tg_tuple_desc d{};
         d.dim = NDim;
@@ -485,7 +487,9 @@ public:
             d.extent[q] = q < NDim ? el.mesh->extent(q) : 1.0;
         }
         d.eps = eps_.data();
-        d.omega0 = omega[0];
+        d.omega0 = omegaPh_[0];
+        d.eps_ph = flat ? nullptr : omegaPh_.data();   // a dispersive phonon band
+        d.dynamic_phonons = dynamicPhonons ? 1 : 0;
         d.amplitude = eph->amplitude;
         d.n_umklapp = eph->umklappSteps;
         d.num_mc = eph->numMCPoints;
@@ -541,6 +545,7 @@ private:
     Real ctx_bath_ = -1.0;
     std::vector<Real> eps_;
     tg_tuple_ctx* tctx_ = nullptr;
+    std::vector<Real> omegaPh_;
     const ScatteringChannel<NDim>* tupleChannel_ = nullptr;
     bool havePoints_ = false;
     uint64_t rngState_ = 0;
@@ -620,9 +625,23 @@ public:
             if (!material->haveMCPoints()) material->constructMCPoints();
             const auto& tch = material->tupleChannel();
             const Function<NDim>& fe = (*this)[tch.electronBand];
+            material->checkTuple(tg_tuple_set_amplitude(t, tch.amplitude));
+            if (material->dynamicPhonons) {
+                // both bands: [electrons | phonons]
+                const Function<NDim>& fp = (*this)[tch.phononBand];
+                std::vector<Real> y(fe.vec), o(2 * fe.vec.size());
+                y.insert(y.end(), fp.vec.begin(), fp.vec.end());
+                material->checkTuple(tg_tuple_propagate(t, y.data(), nullptr, o.data(),
+                                                        fe.mesh->numberElements, 1, 0));
+                Function<NDim> oe(*fe.mesh), op(*fp.mesh);
+                std::copy(o.begin(), o.begin() + fe.vec.size(), oe.vec.begin());
+                std::copy(o.begin() + fe.vec.size(), o.end(), op.vec.begin());
+                r[tch.electronBand] = oe;
+                r[tch.phononBand] = op;
+                return r;
+            }
             Function<NDim> out(*fe.mesh);
             const Real n = bathOccupation();
-            material->checkTuple(tg_tuple_set_amplitude(t, tch.amplitude));
             material->checkTuple(tg_tuple_propagate(t, fe.vec.data(), &n, out.vec.data(),
                                                     fe.mesh->numberElements, 1, 0));
             r[tch.electronBand] = out;
@@ -743,7 +762,10 @@ private:
         material->constructMCPoints();
         const auto& ch = material->tupleChannel();
         const Band<NDim>& el = (*material)[ch.electronBand];
+        const bool dyn = material->dynamicPhonons;
         std::vector<Real> f = start[ch.electronBand].vec;
+        const size_t ne = f.size();
+        if (dyn) f.insert(f.end(), start[ch.phononBand].vec.begin(), start[ch.phononBand].vec.end());
         const int64_t n = (int64_t)f.size();
         const Real t0 = argVec.back();
         const Real bath = start.bathOccupation();
@@ -751,15 +773,19 @@ private:
         std::vector<Real> dense((size_t)maxDense * n);
         material->checkTuple(tg_tuple_set_amplitude(t, ch.amplitude));
         tg_status s = tg_tuple_dp5(t, f.data(), el.mesh->numberElements, 1, t0, finalTime, denseStep,
-                                   el.relativeErrorTolerance, el.absoluteErrorTolerance, &bath,
-                                   quiet ? TG_QUIET : 0, dense.data(), maxDense, &lastStats);
+                                   el.relativeErrorTolerance, el.absoluteErrorTolerance,
+                                   dyn ? nullptr : &bath, quiet ? TG_QUIET : 0, dense.data(), maxDense,
+                                   &lastStats);
         if (s != TG_OK && s != TG_ERR_STEP_UNDERFLOW) material->checkTuple(s);
         if (s == TG_ERR_STEP_UNDERFLOW) std::printf("The step size has become too small to be handled\n");
         const int nd = lastStats.n_dense < maxDense ? lastStats.n_dense : maxDense;
         for (int k = 0; k < nd; ++k) {
             MaterialStatus<NDim> st = start;
-            std::copy(dense.begin() + (size_t)k * n, dense.begin() + (size_t)(k + 1) * n,
+            std::copy(dense.begin() + (size_t)k * n, dense.begin() + (size_t)k * n + ne,
                       st[ch.electronBand].vec.begin());
+            if (dyn)
+                std::copy(dense.begin() + (size_t)k * n + ne, dense.begin() + (size_t)(k + 1) * n,
+                          st[ch.phononBand].vec.begin());
             argVec.push_back(t0 + (k + 1) * denseStep);
             valVec.push_back(std::move(st));
         }

@@ -71,10 +71,16 @@ struct MixedData {
     int fold = 2;              // iterations per FP64 fold (0: FP32 partials)
     int waves = 4;             // waves per workgroup
     int depth = 2;             // iterations of register prefetch
+    bool persist = true;       // persistent balanced kernel (else the tiled grid kernel)
+    int ncu = 0;               // compute units of the device
+    int da = 2;                // persistent kernel: iterations of register prefetch
+    int sched_nw = 0;          // tile width the schedule below was built for (0: none)
+    int bins_per_xcd = 0;
+    int32_t* sched_off = nullptr;    // per bin: first entry in sched_panel (nbins + 1)
+    int32_t* sched_panel = nullptr;  // panels in bin order
     float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
-    double* fs = nullptr;      // f in sorted order, FP64, member-major (nt x n)
     bool rowcache = true;      // member rows fit in LDS
     int64_t work_items = 0;
 };
@@ -159,99 +165,84 @@ __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* 
 // consecutive and the 16-byte units of eight members that share a 128-byte
 // line of the packed operands meet in one L2.  With ROW the member's row of
 // f (caller order) is staged in LDS by coalesced reads and the energy-order
-// gather reads LDS; without it (rows larger than the LDS) the gather reads
-// global memory.
+// gathers read LDS; without it (rows larger than the LDS) they read global
+// memory.
 constexpr int kMemT = 1024;
 
 __device__ inline int member_of_block(int w, int nt) { return (w % 8) * (nt / 8) + w / 8; }
 
-// f (caller order) -> fs, the FP64 copy in energy order the GEMM operands
-// and the epilogue are built from
-template <bool ROW>
-__global__ void __launch_bounds__(kMemT)
-gather_member_kernel(const double* __restrict__ f, int64_t n, int B,
-                     const int32_t* __restrict__ perm, int nt, double* __restrict__ fs) {
-    extern __shared__ double row[];
-    const int b = member_of_block(blockIdx.x, nt);
-    if (b >= B) return;
-    const double* fb = f + (int64_t)b * n;
-    // UNR independent loads per thread in flight before any is used
+// stage row b of f (caller order) in LDS, UNR independent loads per thread in
+// flight before any is stored
+__device__ inline void stage_member_row(const double* __restrict__ fb, int64_t n, double* row) {
     constexpr int UNR = 8;
-    if constexpr (ROW) {
-        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * UNR) {
-            double v[UNR];
-#pragma unroll
-            for (int j = 0; j < UNR; ++j) {
-                const int64_t i = i0 + (int64_t)j * kMemT;
-                v[j] = i < n ? fb[i] : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < UNR; ++j) {
-                const int64_t i = i0 + (int64_t)j * kMemT;
-                if (i < n) row[i] = v[j];
-            }
-        }
-        __syncthreads();
-    }
-    double* out = fs + (int64_t)b * n;
-    for (int64_t p0 = threadIdx.x; p0 < n; p0 += (int64_t)kMemT * UNR) {
-        int32_t node[UNR];
-#pragma unroll
-        for (int j = 0; j < UNR; ++j) {
-            const int64_t p = p0 + (int64_t)j * kMemT;
-            node[j] = p < n ? perm[p] : 0;
-        }
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * UNR) {
         double v[UNR];
 #pragma unroll
-        for (int j = 0; j < UNR; ++j) v[j] = ROW ? row[node[j]] : fb[node[j]];
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t i = i0 + (int64_t)j * kMemT;
+            v[j] = i < n ? fb[i] : 0.0;
+        }
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
-            const int64_t p = p0 + (int64_t)j * kMemT;
-            if (p < n) out[p] = v[j];
+            const int64_t i = i0 + (int64_t)j * kMemT;
+            if (i < n) row[i] = v[j];
         }
-    }
-}
-
-// fs -> packed fp16 hi/lo of both operands z = f and z = g = 1 - f (the GEMM
-// streams one of them per (panel, member), see choose_operand_kernel).  A
-// workgroup transposes a tile of 64 members x 64 sorted nodes through LDS so
-// both the fs reads and the 16-byte unit writes are contiguous; members >= B
-// and nodes >= n pack as zero.
-constexpr int kPackT = 256;
-__global__ void __launch_bounds__(kPackT)
-pack_tile_kernel(const double* __restrict__ fs, int64_t n, int B, int64_t kgroups, int nt,
-                 u32x4* __restrict__ Fhi, u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi,
-                 u32x4* __restrict__ Glo) {
-    __shared__ double tile[64][65];
-    const int64_t p0 = (int64_t)blockIdx.x * 64;
-    const int b0 = blockIdx.y * 64;
-    const int t = threadIdx.x;
-    {
-        const int pos = t & 63;
-        const int64_t p = p0 + pos;
-        constexpr int R = 64 / (kPackT / 64);
-        double v[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int b = b0 + (t >> 6) + j * (kPackT / 64);
-            v[j] = (b < B && p < n) ? fs[(int64_t)b * n + p] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < R; ++j) tile[(t >> 6) + j * (kPackT / 64)][pos] = v[j];
     }
     __syncthreads();
+}
+
+// f (caller order) -> packed fp16 hi/lo of both GEMM operands z = f and
+// z = g = 1 - f in energy order (the GEMM streams one of them per (panel,
+// member)), and the operand choice of every panel.  Members >= B and nodes
+// >= n pack as zero.
+//
+// Operand of (panel, member): g = 1 - f when the panel's middle node is in
+// the filled part of the member's distribution (f > 1/2), else f.  The
+// products the epilogue needs come back as K z and K 1 - K z, so the
+// Pauli-blocked out-scattering K^T(1 - f) of filled nodes is a sum of small
+// terms instead of the difference of two large ones.
+template <bool ROW>
+__global__ void __launch_bounds__(kMemT)
+pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
+                   int64_t kgroups, int64_t kt, int npanel, int nt, u32x4* __restrict__ Fhi,
+                   u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo,
+                   uint8_t* __restrict__ useg) {
+    extern __shared__ double row[];
+    const int b = member_of_block(blockIdx.x, nt);
+    if (b >= nt) return;
+    const bool live = b < B;
+    const double* fb = f + (int64_t)(live ? b : 0) * n;
+    if constexpr (ROW) {
+        if (live) stage_member_row(fb, n, row);
+    }
+    for (int pn = threadIdx.x; pn < npanel; pn += kMemT) {
+        uint8_t v = 0;
+        if (live) {
+            // K rows and K^T rows index the same sorted nodes
+            const int64_t r = (int64_t)pn * kMT + kMT / 2;
+            int64_t p = r < kt ? r : r - kt;
+            if (p > n - 1) p = n - 1;
+            const int32_t node = perm[p];
+            v = (ROW ? row[node] : fb[node]) > 0.5 ? 1 : 0;
+        }
+        useg[(int64_t)pn * nt + b] = v;
+    }
     const double scale = (double)(1 << kSF);
-    const int ml = t & 63;
-    const int b = b0 + ml;
-    for (int kgl = t >> 6; kgl < 8; kgl += kPackT / 64) {
-        const int64_t kg = p0 / 8 + kgl;
-        if (kg >= kgroups) break;
-        const bool live = b < B;
-        half8 fh, fl, gh, gl;
+    for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
+        int32_t node[8];
+#pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int64_t p = kg * 8 + q;
-            const double x = tile[ml][kgl * 8 + q];
-            const double y = (live && p < n) ? 1.0 - x : 0.0;
+            node[q] = (live && p < n) ? perm[p] : -1;
+        }
+        half8 fh, fl, gh, gl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double x = 0.0, y = 0.0;
+            if (node[q] >= 0) {
+                x = ROW ? row[node[q]] : fb[node[q]];
+                y = 1.0 - x;
+            }
             _Float16 hi, lo;
             split_fp16(x, scale, hi, lo);
             fh[q] = hi;
@@ -265,28 +256,6 @@ pack_tile_kernel(const double* __restrict__ fs, int64_t n, int B, int64_t kgroup
         Ghi[kg * nt + b] = __builtin_bit_cast(u32x4, gh);
         Glo[kg * nt + b] = __builtin_bit_cast(u32x4, gl);
     }
-}
-
-// Operand of (panel, member): g = 1 - f when the panel's middle node is in
-// the filled part of the member's distribution (f > 1/2), else f.  The
-// products the epilogue needs come back as K z and K 1 - K z, so the
-// Pauli-blocked out-scattering K^T(1 - f) of filled nodes is a sum of small
-// terms instead of the difference of two large ones.
-__global__ void choose_operand_kernel(const double* __restrict__ f, int64_t n, int64_t kt, int B,
-                                      int npanel, const int32_t* __restrict__ perm,
-                                      uint8_t* __restrict__ useg, int nt) {
-    const int pn = blockIdx.x;
-    const int b = threadIdx.x;
-    if (pn >= npanel || b >= nt) return;
-    uint8_t v = 0;
-    if (b < B) {
-        // K rows and K^T rows index the same sorted nodes
-        const int64_t row = (int64_t)pn * kMT + kMT / 2;
-        int64_t p = row < kt ? row : row - kt;
-        if (p > n - 1) p = n - 1;
-        v = f[(int64_t)b * n + perm[p]] > 0.5 ? 1 : 0;
-    }
-    useg[(int64_t)pn * nt + b] = v;
 }
 
 // --------------------------------------------------------------------- GEMM
@@ -500,17 +469,168 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
         }
 }
 
+// Persistent variant: one workgroup of four waves per CU, each wave owning
+// 32 rows of the panel and all NW members of the tile (TN = NW/32 MFMA tiles
+// of 32 x 32).  A fragments go from global memory straight into the owning
+// wave's registers (no LDS), DA iterations ahead; the Z tile is the only
+// operand shared by the waves and goes through double-buffered LDS with one
+// barrier per iteration, also DA iterations ahead.
+//
+// The host balances the panels over the workgroups (longest band first onto
+// the least loaded bin, a fixed order, so every call runs the same
+// schedule).  A bin is nmt workgroups on one XCD that run the same panel
+// list for the nmt member tiles side by side, so a panel's A band is fetched
+// into that XCD's L2 once for all member tiles.  With nsplit = 1 and the
+// same FOLD the summation order is that of mixed_gemm_kernel.
+constexpr int kPerT = 256;
+
+template <int NW, int FOLD, int DA>
+__global__ void __launch_bounds__(kPerT, 1)
+mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
+                          const u32x4* __restrict__ Fhi, const u32x4* __restrict__ Flo,
+                          const u32x4* __restrict__ Ghi, const u32x4* __restrict__ Glo,
+                          const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
+                          const int2* __restrict__ krange, const int32_t* __restrict__ sched_off,
+                          const int32_t* __restrict__ sched_panel, int nmt, int bins_per_xcd,
+                          double* __restrict__ P) {
+    static_assert(FOLD > 0, "the persistent kernel folds into FP64");
+    constexpr int TN = NW / 32;
+    constexpr int Z_PER_T = 2 * kKG * NW / kPerT;
+    static_assert(Z_PER_T >= 1 && kPerT % NW == 0, "Z staging assumes one member per thread");
+    constexpr int U = DA % 2 == 0 ? DA : 2 * DA;   // unroll: register slot and LDS buffer fixed per step
+    __shared__ u32x4 sZ[2][2][kKG][NW];
+
+    const int w = blockIdx.x;
+    const int wi = w / 8;
+    const int mt = wi % nmt;
+    const int bin = (w % 8) * bins_per_xcd + wi / nmt;
+    if (wi / nmt >= bins_per_xcd) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col0 = mt * NW;
+    const int zcol = tid % NW;
+
+    for (int s = sched_off[bin]; s < sched_off[bin + 1]; ++s) {
+        const int panel = sched_panel[s];
+        const int64_t row0 = (int64_t)panel * kMT;
+        const int64_t arow = row0 + wave * 32 + (lane & 31);
+        const int2 kr = krange[panel];
+        const int N = kr.y - kr.x;
+        const bool my_g = useg[(int64_t)panel * nt + col0 + zcol] != 0;
+        const u32x4* Zhi = my_g ? Ghi : Fhi;
+        const u32x4* Zlo = my_g ? Glo : Flo;
+
+        floatx16 acc[TN];
+        double sum[TN][16];
+#pragma unroll
+        for (int u = 0; u < TN; ++u) {
+            acc[u] = floatx16{0};
+#pragma unroll
+            for (int e = 0; e < 16; ++e) sum[u][e] = 0.0;
+        }
+        if (N > 0) {
+            u32x4 ra[DA][4], rz[DA][Z_PER_T];
+            // A(k): lane's 16-byte units of k-steps 0 and 1 (hi, lo); Z(k): this
+            // thread's units of the shared tile; iteration indices clamp at the end
+            auto load_a = [&](u32x4 (&r)[4], int k) {
+                const int64_t kg = (int64_t)(kr.x + min(k, N - 1)) * kKG + (lane >> 5);
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    r[2 * ks] = Ahi[(kg + 2 * ks) * rows_pad + arow];
+                    r[2 * ks + 1] = Alo[(kg + 2 * ks) * rows_pad + arow];
+                }
+            };
+            auto load_z = [&](u32x4 (&r)[Z_PER_T], int k) {
+                const int64_t kg0 = (int64_t)(kr.x + min(k, N - 1)) * kKG;
+#pragma unroll
+                for (int j = 0; j < Z_PER_T; ++j) {
+                    const int q = tid + j * kPerT;
+                    const int which = q / (kKG * NW);
+                    const int kg = (q / NW) % kKG;
+                    r[j] = (which ? Zlo : Zhi)[(kg0 + kg) * nt + col0 + zcol];
+                }
+            };
+            auto store_z = [&](const u32x4 (&r)[Z_PER_T], int buf) {
+#pragma unroll
+                for (int j = 0; j < Z_PER_T; ++j) {
+                    const int q = tid + j * kPerT;
+                    sZ[buf][q / (kKG * NW)][(q / NW) % kKG][q % NW] = r[j];
+                }
+            };
+            // prologue: A(0..DA-1); Z(0) to LDS buffer 0, Z(1..DA) in registers
+#pragma unroll
+            for (int d = 0; d < DA; ++d) load_a(ra[d], d);
+            load_z(rz[0], 0);
+            store_z(rz[0], 0);
+#pragma unroll
+            for (int d = 1; d < DA; ++d) load_z(rz[d], d);
+            load_z(rz[0], DA);
+            __syncthreads();
+            for (int k0 = 0; k0 < N; k0 += U) {
+#pragma unroll
+                for (int uu = 0; uu < U; ++uu) {
+                    const int k = k0 + uu;
+                    if (k < N) {
+                        const int da = uu % DA, buf = uu & 1;
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            const int kg = 2 * ks + (lane >> 5);
+                            const half8 ah = __builtin_bit_cast(half8, ra[da][2 * ks]);
+                            const half8 al = __builtin_bit_cast(half8, ra[da][2 * ks + 1]);
+#pragma unroll
+                            for (int u = 0; u < TN; ++u) {
+                                const int col = u * 32 + (lane & 31);
+                                const half8 zh = __builtin_bit_cast(half8, sZ[buf][0][kg][col]);
+                                const half8 zl = __builtin_bit_cast(half8, sZ[buf][1][kg][col]);
+                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zh, acc[u], 0, 0, 0);
+                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zl, acc[u], 0, 0, 0);
+                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, zh, acc[u], 0, 0, 0);
+                            }
+                        }
+                        load_a(ra[da], k + DA);
+                        // Z(k+1) was loaded DA-1 iterations ago into slot (k+1) % DA
+                        const int dz = (uu + 1) % DA;
+                        store_z(rz[dz], (uu + 1) & 1);
+                        load_z(rz[dz], k + 1 + DA);
+                        if ((k + 1) % FOLD == 0 || k + 1 == N) {
+#pragma unroll
+                            for (int u = 0; u < TN; ++u) {
+#pragma unroll
+                                for (int e = 0; e < 16; ++e) sum[u][e] += (double)acc[u][e];
+                                acc[u] = floatx16{0};
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+        // C tile -> P[member][row]; lane holds member (lane & 31) of each
+        // 32-wide tile and rows 8*(e/4) + 4*(lane/32) + (e%4) of the wave's 32
+#pragma unroll
+        for (int u = 0; u < TN; ++u) {
+            const int col = col0 + u * 32 + (lane & 31);
+            double* dst = P + (int64_t)col * rows_pad + row0 + wave * 32 + 4 * (lane >> 5);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) dst[8 * (e / 4) + (e % 4)] = sum[u][e];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ epilogue
 // One workgroup per member: FP64 sum of the split partials, Pauli-blocked
 // assembly with the member's bath occupation, FP64 block sums of sum V I and
 // sum V |f(1-f)| for the number-defect removal, result in the caller's node
-// order.  With ROW the result is scattered into an LDS row and written out
-// by coalesced stores together with the correction; without it the scatter
-// goes to global memory and the correction is a second pass.
+// order.  With ROW the member's f is staged in LDS and every position
+// replaces the f of its node by I in place (the energy order is a
+// permutation, so each LDS slot is read and written by one thread only); the
+// row then goes out by coalesced stores together with the correction.
+// Without it the gather reads and the scatter writes global memory and the
+// correction is a second pass.
 template <bool ROW, typename PT>
 __global__ void __launch_bounds__(kMemT)
 finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
-                     int64_t kt, int B, const double* __restrict__ fs, const double* __restrict__ f,
+                     int64_t kt, int B, const double* __restrict__ f,
                      const int32_t* __restrict__ perm, const double* __restrict__ rk,
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
@@ -522,13 +642,14 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     if (b >= B) return;
     const double nb = bath[b];
     const double np1 = nb + 1.0;
-    const double* fsb = fs + (int64_t)b * n;
+    const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
+    if constexpr (ROW) stage_member_row(fb, n, row);
     double dI = 0.0, dw = 0.0;
     // UNR positions per thread with all their loads issued before use
     constexpr int UNR = 4;
     for (int64_t p0 = threadIdx.x; p0 < n; p0 += (int64_t)kMemT * UNR) {
-        double u[UNR], v[UNR], fi[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
+        double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
         int32_t node[UNR];
         bool gu[UNR], gv[UNR];
 #pragma unroll
@@ -543,7 +664,6 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             }
             gu[j] = useg[(p / kMT) * (int64_t)nt + b] != 0;
             gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
-            fi[j] = fsb[p];
             rkp[j] = rk[p];
             ckp[j] = ck[p];
             Vp[j] = V[p];
@@ -556,7 +676,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
             const double Kf = gu[j] ? rkp[j] - uu : uu, Kg = gu[j] ? uu : rkp[j] - uu;
             const double KTf = gv[j] ? ckp[j] - vv : vv, KTg = gv[j] ? vv : ckp[j] - vv;
-            const double x = fi[j];
+            const double x = ROW ? row[node[j]] : fb[node[j]];
             // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
             const double Ii = ((1.0 - x) * (np1 * Kf + nb * KTf) - x * (np1 * KTg + nb * Kg)) / Vp[j];
             dI += Vp[j] * Ii;
@@ -589,7 +709,6 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     }
     __syncthreads();
     if (conserve) sc = s_b;
-    const double* fb = f + (int64_t)b * n;
     constexpr int WUNR = 8;
     if (ROW || conserve) {
         for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * WUNR) {
@@ -628,15 +747,88 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     X(128, 4, 1, 1) X(128, 8, 1, 1) X(128, 8, 1, 2) X(128, 8, 1, 3) X(128, 8, 2, 2)             \
     X(256, 4, 0, 1) X(256, 4, 0, 2) X(256, 8, 0, 2)
 
-static bool config_known(int nw, int waves, int fold, int depth) {
+// the persistent configurations: (members, fold, prefetch depth)
+#define TG_PERSIST_CONFIGS(X) X(64, 2, 2) X(64, 2, 3) X(128, 1, 2) X(128, 2, 2) X(128, 2, 3) X(128, 4, 2)
+
+static bool config_known(bool persist, int nw, int waves, int fold, int depth) {
+    if (persist) {
+#define TG_X(a, c_, d_) if (nw == a && fold == c_ && depth == d_) return true;
+        TG_PERSIST_CONFIGS(TG_X)
+#undef TG_X
+        return false;
+    }
 #define TG_X(a, b, c_, d_) if (nw == a && waves == b && fold == c_ && depth == d_) return true;
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
     return false;
 }
 
+// Longest-processing-time schedule of the panels over the bins of the
+// persistent kernel: panels by decreasing band length (ties by index), each
+// onto the bin with the least work so far (ties to the lowest bin); the cost
+// of a panel is its iteration count plus one for its write-out.  Built on
+// the host, copied once; evaluations only launch.
+static tg_status build_schedule(tg_ctx* c, MixedData* m) {
+    if (m->sched_nw == m->nw) return TG_OK;
+    const int nmt = m->nt / m->nw;
+    const int per_xcd = std::max(1, m->ncu / 8);
+    m->bins_per_xcd = std::max(1, per_xcd / nmt);
+    const int nbins = 8 * m->bins_per_xcd;
+    std::vector<int> order(m->npanel);
+    for (int i = 0; i < m->npanel; ++i) order[i] = i;
+    auto cost = [&](int pn) { return (int64_t)(m->h_krange[pn].y - m->h_krange[pn].x) + 1; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<int64_t> load(nbins, 0);
+    std::vector<std::vector<int32_t>> lists(nbins);
+    for (int pn : order) {
+        int best = 0;
+        for (int b = 1; b < nbins; ++b)
+            if (load[b] < load[best]) best = b;
+        load[best] += cost(pn);
+        lists[best].push_back(pn);
+    }
+    std::vector<int32_t> off(nbins + 1, 0), panels;
+    panels.reserve(m->npanel);
+    for (int b = 0; b < nbins; ++b) {
+        panels.insert(panels.end(), lists[b].begin(), lists[b].end());
+        off[b + 1] = (int32_t)panels.size();
+    }
+    dev_free(&m->sched_off);
+    dev_free(&m->sched_panel);
+    TG_TRY(dev_alloc(c, &m->sched_off, (size_t)nbins + 1));
+    TG_TRY(dev_alloc(c, &m->sched_panel, (size_t)m->npanel));
+    TG_HIP(c, hipMemcpyAsync(m->sched_off, off.data(), sizeof(int32_t) * (nbins + 1), hipMemcpyHostToDevice,
+                             c->stream));
+    TG_HIP(c, hipMemcpyAsync(m->sched_panel, panels.data(), sizeof(int32_t) * m->npanel,
+                             hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    m->sched_nw = m->nw;
+    return TG_OK;
+}
+
+template <int NW, int FOLD, int DA>
+static void launch_persist_t(tg_ctx* c, MixedData* m) {
+    const int nmt = m->nt / NW;
+    hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
+                       dim3(kPerT), 0, c->stream, (const u32x4*)m->A_hi, (const u32x4*)m->A_lo,
+                       (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
+                       (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, m->sched_off,
+                       m->sched_panel, nmt, m->bins_per_xcd, (double*)m->P);
+}
+
 static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     if (m->nt % m->nw != 0) return set_error(c, TG_ERR_ARG, "tile width %d does not divide %d", m->nw, m->nt);
+    if (m->persist) {
+        if (m->sched_nw != m->nw || m->nsplit != 1)
+            return set_error(c, TG_ERR_STATE, "persistent GEMM schedule not built for tile %d", m->nw);
+#define TG_X(a, c_, d_) if (m->nw == a && m->fold == c_ && m->da == d_) launch_persist_t<a, c_, d_>(c, m); else
+        TG_PERSIST_CONFIGS(TG_X)
+#undef TG_X
+        return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d", m->nw,
+                         m->fold, m->da);
+        TG_CHECK_LAUNCH(c);
+        return TG_OK;
+    }
 #define TG_X(a, b, c_, d_)                                                        \
     if (m->nw == a && m->waves == b && m->fold == c_ && m->depth == d_) launch_gemm_t<a, b, c_, d_>(c, m); else
     TG_MIXED_CONFIGS(TG_X)
@@ -650,16 +842,13 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
 static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B) {
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
-    if (m->rowcache)
-        hipLaunchKernelGGL(gather_member_kernel<true>, dim3(m->nt), dim3(kMemT), sizeof(double) * n,
-                           c->stream, d_f, n, B, m->perm, m->nt, m->fs);
-    else
-        hipLaunchKernelGGL(gather_member_kernel<false>, dim3(m->nt), dim3(kMemT), 0, c->stream, d_f, n,
-                           B, m->perm, m->nt, m->fs);
-    TG_CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(pack_tile_kernel, dim3(ceil_div(kgroups * 8, 64), m->nt / 64), dim3(kPackT), 0,
-                       c->stream, m->fs, n, B, kgroups, m->nt, (u32x4*)m->F_hi, (u32x4*)m->F_lo,
-                       (u32x4*)m->G_hi, (u32x4*)m->G_lo);
+#define TG_PACK(ROW)                                                                                \
+    hipLaunchKernelGGL(pack_member_kernel<ROW>, dim3(m->nt), dim3(kMemT), ROW ? sizeof(double) * n : 0, \
+                       c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt,             \
+                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg)
+    if (m->rowcache) TG_PACK(true);
+    else TG_PACK(false);
+#undef TG_PACK
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }
@@ -671,8 +860,8 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
 #define TG_FINISH(ROW, PT)                                                                           \
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
-                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, m->fs,  \
-                       d_f, m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
+                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
+                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
                        d_I)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
@@ -698,17 +887,43 @@ static tg_status ensure_P(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
-// Autotuner: times every accuracy-safe tile configuration (FP64 fold, so no
-// K-split is needed for accuracy) with one and two K-splits on this band --
-// the GEMM and the epilogue, whose partial-sum reads grow with the split
-// count -- and keeps the fastest.  The operands are zeroed first; the time
-// does not depend on their values.
+// Autotuner: times every accuracy-safe configuration (FP64 fold, so no
+// K-split is needed for accuracy) -- the persistent kernels, and the tiled
+// grid kernels with one and two K-splits -- on this band, the GEMM and the
+// epilogue, whose partial-sum reads grow with the split count, and keeps the
+// fastest.  The operands are zeroed first; the time does not depend on their
+// values.
 constexpr int kTuneReps = 5;
 
+struct TuneCand {
+    bool persist;
+    int nw, waves, fold, depth, nsplit;
+};
+
+static void apply_cand(MixedData* m, const TuneCand& cd) {
+    m->persist = cd.persist;
+    m->nw = cd.nw;
+    m->fold = cd.fold;
+    m->nsplit = cd.nsplit;
+    if (cd.persist) m->da = cd.depth;
+    else {
+        m->waves = cd.waves;
+        m->depth = cd.depth;
+    }
+}
+
+static TuneCand current_cand(const MixedData* m) {
+    return TuneCand{m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth, m->nsplit};
+}
+
 static tg_status autotune(tg_ctx* c, MixedData* m) {
-    struct Cand { int nw, waves, fold, depth; };
-    std::vector<Cand> cands;
-#define TG_X(a, b, c_, d_) if (c_ > 0 && a <= m->nt) cands.push_back(Cand{a, b, c_, d_});
+    std::vector<TuneCand> cands;
+#define TG_X(a, c_, d_) if (a <= m->nt) cands.push_back(TuneCand{true, a, 4, c_, d_, 1});
+    TG_PERSIST_CONFIGS(TG_X)
+#undef TG_X
+#define TG_X(a, b, c_, d_)                                                        \
+    if (c_ > 0 && a <= m->nt)                                                     \
+        for (int ns = 1; ns <= 2; ++ns) cands.push_back(TuneCand{false, a, b, c_, d_, ns});
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
     if (cands.empty()) return TG_OK;
@@ -719,9 +934,10 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     TG_HIP(c, hipMemsetAsync(m->G_hi, 0, sizeof(uint4) * kgroups * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->G_lo, 0, sizeof(uint4) * kgroups * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->useg, 0, (size_t)m->npanel * m->nt, st));
-    TG_HIP(c, hipMemsetAsync(m->fs, 0, sizeof(double) * c->n * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->bath, 0, sizeof(double) * c->max_batch, st));
-    double* d_out = nullptr;
+    double *d_zero = nullptr, *d_out = nullptr;
+    TG_TRY(dev_alloc(c, &d_zero, (size_t)c->n * m->nt));
+    TG_HIP(c, hipMemsetAsync(d_zero, 0, sizeof(double) * c->n * m->nt, st));
     TG_TRY(dev_alloc(c, &d_out, (size_t)c->n * m->nt));
     const int Bt = std::min(m->nt, (int)c->max_batch);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -729,61 +945,51 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     TG_HIP(c, hipEventCreate(&e1));
     // the default keeps its place unless another candidate is >3% faster, so
     // near-equal candidates do not flip from one context to the next
-    const Cand dflt{m->nw, m->waves, m->fold, m->depth};
-    const int dflt_split = m->nsplit;
+    const TuneCand dflt = current_cand(m);
     float best = 0.0f, dflt_ms = 0.0f;
-    Cand pick = cands[0];
-    int pick_split = 1;
+    TuneCand pick = cands[0];
     tg_status status = TG_OK;
-    for (const Cand& cd : cands) {
-        for (int ns = 1; ns <= 2 && status == TG_OK; ++ns) {
-            m->nw = cd.nw;
-            m->waves = cd.waves;
-            m->fold = cd.fold;
-            m->depth = cd.depth;
-            m->nsplit = ns;
-            status = ensure_P(c, m);
-            if (status == TG_OK) status = launch_gemm(c, m);     // warm-up
-            if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
-            if (status != TG_OK) break;
-            if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
-            for (int r = 0; r < kTuneReps && status == TG_OK; ++r) {
-                status = launch_gemm(c, m);
-                if (status == TG_OK) status = launch_finish(c, m, m->fs, d_out, Bt, 0);
-            }
-            if (status != TG_OK) break;
-            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                status = set_error(c, TG_ERR_HIP, "event timing");
-                break;
-            }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            ms /= kTuneReps;
-            if (cd.nw == dflt.nw && cd.waves == dflt.waves && cd.fold == dflt.fold &&
-                cd.depth == dflt.depth && ns == dflt_split)
-                dflt_ms = ms;
-            if (best == 0.0f || ms < best) {
-                best = ms;
-                pick = cd;
-                pick_split = ns;
-            }
+    for (const TuneCand& cd : cands) {
+        apply_cand(m, cd);
+        if (cd.persist) status = build_schedule(c, m);
+        if (status == TG_OK) status = ensure_P(c, m);
+        if (status == TG_OK) status = launch_gemm(c, m);     // warm-up
+        if (status == TG_OK) status = launch_finish(c, m, d_zero, d_out, Bt, 0);
+        if (status != TG_OK) break;
+        if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
+        for (int r = 0; r < kTuneReps && status == TG_OK; ++r) {
+            status = launch_gemm(c, m);
+            if (status == TG_OK) status = launch_finish(c, m, d_zero, d_out, Bt, 0);
+        }
+        if (status != TG_OK) break;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            status = set_error(c, TG_ERR_HIP, "event timing");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= kTuneReps;
+        if (cd.persist == dflt.persist && cd.nw == dflt.nw && cd.fold == dflt.fold && cd.depth == dflt.depth &&
+            cd.nsplit == dflt.nsplit && (cd.persist || cd.waves == dflt.waves))
+            dflt_ms = ms;
+        if (best == 0.0f || ms < best) {
+            best = ms;
+            pick = cd;
         }
     }
     if (status == TG_OK && dflt_ms > 0.0f && best > 0.97f * dflt_ms) {
         best = dflt_ms;
         pick = dflt;
-        pick_split = dflt_split;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(st);
+    dev_free(&d_zero);
     dev_free(&d_out);
     TG_TRY(status);
-    m->nw = pick.nw;
-    m->waves = pick.waves;
-    m->fold = pick.fold;
-    m->depth = pick.depth;
-    m->nsplit = pick_split;
+    apply_cand(m, pick);
+    if (m->persist) TG_TRY(build_schedule(c, m));
+    TG_TRY(ensure_P(c, m));
     m->tuned_ms = best;
     return TG_OK;
 }
@@ -878,41 +1084,55 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
-    // tile width, waves, FP64 fold interval and K-splits
-    // default (kept by the autotuner unless beaten by >3%): 128 members, 8
-    // waves, FP64 fold every 2 iterations, 2 iterations of prefetch; one
-    // K-split, or more when the panels alone leave CUs idle
+    // configuration.  Default (kept by the autotuner unless beaten by >3%):
+    // the persistent kernel, 128 members per tile (64 for smaller batches),
+    // FP64 fold every 2 iterations, 2 iterations of prefetch.  The tiled grid
+    // kernel: 128 members, 8 waves, fold 2, prefetch 2, one K-split or two
+    // when the panels alone leave CUs idle.
+    {
+        hipDeviceProp_t prop;
+        TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
+        m->ncu = prop.multiProcessorCount;
+    }
     m->nw = std::min(128, m->nt);
     m->waves = m->nw == 128 ? 8 : 4;
     m->fold = 2;
     m->depth = 2;
-    {
-        const int nmt = m->nt / m->nw;
-        int ns = (int)std::ceil(512.0 / ((double)m->npanel * nmt));
-        m->nsplit = std::max(1, std::min(ns, 2));
-    }
+    m->da = 2;
+    m->persist = true;
+    m->nsplit = 1;
     // explicit overrides (measurements of the tile configuration) replace the autotuner
     bool fixed = false;
-    for (const char* v : {"TG_MIXED_NW", "TG_MIXED_FOLD", "TG_MIXED_WAVES", "TG_MIXED_DEPTH", "TG_MIXED_SPLITS"})
+    for (const char* v : {"TG_MIXED_NW", "TG_MIXED_FOLD", "TG_MIXED_WAVES", "TG_MIXED_DEPTH", "TG_MIXED_SPLITS",
+                          "TG_MIXED_PERSIST"})
         if (std::getenv(v)) fixed = true;
     if (const char* e = std::getenv("TG_MIXED_AUTOTUNE"))
         if (std::atoi(e) == 0) fixed = true;
+    // the waves and the K-split belong to the tiled kernel: setting either
+    // selects it unless TG_MIXED_PERSIST says otherwise
+    if (std::getenv("TG_MIXED_WAVES") || std::getenv("TG_MIXED_SPLITS")) m->persist = false;
+    if (const char* e = std::getenv("TG_MIXED_PERSIST")) m->persist = std::atoi(e) != 0;
     if (const char* e = std::getenv("TG_MIXED_NW")) {
         const int v = std::atoi(e);
         if ((v == 64 || v == 128 || v == 256) && v <= m->nt) m->nw = v;
     }
     if (const char* e = std::getenv("TG_MIXED_FOLD")) m->fold = std::atoi(e);
     if (const char* e = std::getenv("TG_MIXED_WAVES")) m->waves = std::atoi(e);
-    if (const char* e = std::getenv("TG_MIXED_DEPTH")) m->depth = std::atoi(e);
-    if (!config_known(m->nw, m->waves, m->fold, m->depth)) {
-        m->nw = std::min(64, m->nt);
-        m->waves = 4;
-        m->fold = 1;
-        m->depth = 2;
+    if (const char* e = std::getenv("TG_MIXED_DEPTH")) m->depth = m->da = std::atoi(e);
+    if (!config_known(m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth)) {
+        m->persist = true;
+        m->nw = std::min(128, m->nt);
+        m->fold = 2;
+        m->da = 2;
     }
-    if (const char* e = std::getenv("TG_MIXED_SPLITS")) {
-        const int v = std::atoi(e);
-        if (v >= 1 && v <= 64) m->nsplit = v;
+    if (!m->persist) {
+        const int nmt = m->nt / m->nw;
+        const int ns = (int)std::ceil(512.0 / ((double)m->npanel * nmt));
+        m->nsplit = std::max(1, std::min(ns, 2));
+        if (const char* e = std::getenv("TG_MIXED_SPLITS")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= 64) m->nsplit = v;
+        }
     }
     // member rows in LDS when they fit (with the epilogue's reduction scratch)
     {
@@ -923,7 +1143,7 @@ tg_status mixed_prepare(tg_ctx* c) {
         if (const char* e = std::getenv("TG_MIXED_ROWCACHE"))
             if (std::atoi(e) == 0) m->rowcache = false;
         if (m->rowcache) {
-            TG_HIP(c, hipFuncSetAttribute((const void*)gather_member_kernel<true>,
+            TG_HIP(c, hipFuncSetAttribute((const void*)pack_member_kernel<true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
             TG_HIP(c, hipFuncSetAttribute((const void*)finish_member_kernel<true, double>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
@@ -931,13 +1151,13 @@ tg_status mixed_prepare(tg_ctx* c) {
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
         }
     }
-    TG_TRY(dev_alloc(c, &m->fs, (size_t)n * m->nt));
     TG_TRY(dev_alloc(c, &m->F_hi, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->F_lo, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->G_hi, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->G_lo, (size_t)kgroups * m->nt));
     TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
+    if (m->persist) TG_TRY(build_schedule(c, m));
     if (!fixed) TG_TRY(autotune(c, m));
     TG_HIP(c, hipStreamSynchronize(st));
     return TG_OK;
@@ -952,8 +1172,9 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_scale = m->sa;
     out->mixed_tile = m->nw;
     out->mixed_fold = m->fold;
-    out->mixed_waves = m->waves;
-    out->mixed_depth = m->depth;
+    out->mixed_waves = m->persist ? kPerT / 64 : m->waves;
+    out->mixed_depth = m->persist ? m->da : m->depth;
+    out->mixed_persist = m->persist ? 1 : 0;
     out->mixed_tuned_ms = m->tuned_ms;
 }
 
@@ -968,11 +1189,12 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->G_lo);
     dev_free(&m->useg);
     dev_free(&m->krange);
+    dev_free(&m->sched_off);
+    dev_free(&m->sched_panel);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
     dev_free(&m->P);
     dev_free(&m->bath);
-    dev_free(&m->fs);
     delete m;
     c->mixed = nullptr;
 }
@@ -999,9 +1221,6 @@ tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
     TG_TRY(ensure_P(c, m));
     TG_TRY(launch_pack(c, m, d_f, B));
-    hipLaunchKernelGGL(choose_operand_kernel, dim3(m->npanel), dim3(m->nt), 0, c->stream, d_f, c->n,
-                       m->kt, B, m->npanel, m->perm, m->useg, m->nt);
-    TG_CHECK_LAUNCH(c);
     time_begin(c);
     TG_TRY(launch_gemm(c, m));
     time_end(c);

@@ -125,7 +125,8 @@ typedef struct tg_kernel_stats {
     int32_t mixed_waves;    /* waves per workgroup of the batched kernel */
     int32_t mixed_depth;    /* iterations of register prefetch of the batched kernel */
     float   mixed_tuned_ms; /* autotuner's time of the chosen configuration (0: set explicitly) */
-    int32_t mixed_persist;  /* 1: persistent kernel with a balanced panel schedule, 0: tiled grid */
+    int32_t mixed_persist;  /* persistent kernel with a balanced panel schedule, A fragments straight
+                               into registers (1) or A tile through LDS (2); 0: tiled grid */
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

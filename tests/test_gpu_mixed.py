@@ -131,8 +131,12 @@ TILE_CONFIGS = [(64, 4, 0, 1, 16), (64, 4, 1, 1, 1), (64, 4, 1, 2, 3), (64, 4, 1
                 (64, 4, 2, 2, 1), (128, 4, 0, 1, 8), (128, 4, 1, 1, 2), (128, 8, 1, 1, 1),
                 (128, 8, 1, 2, 2), (128, 8, 1, 3, 1), (128, 8, 2, 2, 3), (256, 4, 0, 1, 8),
                 (256, 4, 0, 2, 4), (256, 8, 0, 2, 8)]
-# the persistent kernel's configurations: (members per workgroup, fold, prefetch depth)
-PERSIST_CONFIGS = [(64, 2, 2), (64, 2, 3), (128, 1, 2), (128, 2, 2), (128, 2, 3), (128, 4, 2)]
+# the persistent kernel's configurations: (members per workgroup, fold,
+# prefetch depth, waves per workgroup, A staged through LDS)
+PERSIST_CONFIGS = [(64, 2, 2, 4, 0), (64, 2, 3, 4, 0), (128, 1, 2, 4, 0), (128, 2, 2, 4, 0),
+                   (128, 2, 3, 4, 0), (128, 4, 2, 4, 0), (128, 2, 2, 8, 0), (128, 2, 3, 8, 0),
+                   (128, 4, 2, 8, 0), (128, 4, 3, 8, 0), (64, 2, 2, 8, 0), (64, 2, 3, 8, 0),
+                   (128, 2, 2, 8, 1), (128, 2, 3, 8, 1), (128, 4, 2, 8, 1), (64, 2, 2, 8, 1)]
 
 
 def test_mixed_tile_configurations(monkeypatch):
@@ -171,7 +175,7 @@ def test_mixed_persistent_configurations(monkeypatch, mesh):
     w = wl.SQUARE32 if mesh == "square32" else wl.Workload("square64", 2, (64, 64))
     B = 256
     F, bath = _members(w, B, seed=13)
-    for k in ("TG_MIXED_WAVES", "TG_MIXED_SPLITS"):
+    for k in ("TG_MIXED_WAVES", "TG_MIXED_SPLITS", "TG_MIXED_AL"):
         monkeypatch.delenv(k, raising=False)
     ref = None
     tiled = {}
@@ -189,24 +193,26 @@ def test_mixed_persistent_configurations(monkeypatch, mesh):
         if ref is None:
             ref = ctx.apply(F[:, ::37], bath_n=bath[::37])
         ctx.close()
-    monkeypatch.delenv("TG_MIXED_WAVES")
     monkeypatch.delenv("TG_MIXED_SPLITS")
-    for nw, fold, depth in PERSIST_CONFIGS:
-        print(f"[gpu] {mesh} persistent {nw} fold {fold} depth {depth}", flush=True)
+    for nw, fold, depth, waves, al in PERSIST_CONFIGS:
+        print(f"[gpu] {mesh} persistent {nw} fold {fold} depth {depth} waves {waves} A-in-LDS {al}",
+              flush=True)
         monkeypatch.setenv("TG_MIXED_PERSIST", "1")
         monkeypatch.setenv("TG_MIXED_NW", str(nw))
         monkeypatch.setenv("TG_MIXED_FOLD", str(fold))
         monkeypatch.setenv("TG_MIXED_DEPTH", str(depth))
+        monkeypatch.setenv("TG_MIXED_WAVES", str(waves))
+        monkeypatch.setenv("TG_MIXED_AL", str(al))
         ctx = tg.Context.create(w, max_batch=B)
         ctx.assemble(mixed=True)
         st = ctx.kernel_stats()
         assert (st["mixed_persist"], st["mixed_tile"], st["mixed_fold"], st["mixed_depth"],
-                st["mixed_splits"]) == (1, nw, fold, depth, 1)
+                st["mixed_waves"], st["mixed_splits"]) == (1 + al, nw, fold, depth, waves, 1)
         Im = ctx.apply(F, bath_n=bath, mixed=True)
         for q, b in enumerate(range(0, B, 37)):
-            assert _rel(Im[:, b], ref[:, q]) <= 2e-6, (nw, fold, depth, b)
+            assert _rel(Im[:, b], ref[:, q]) <= 2e-6, (nw, fold, depth, waves, al, b)
         if fold in tiled:
-            assert np.array_equal(Im, tiled[fold]), (nw, fold, depth)
+            assert np.array_equal(Im, tiled[fold]), (nw, fold, depth, waves, al)
         # a partial batch runs the same schedule
         Ip = ctx.apply(F[:, :5], bath_n=bath[:5], mixed=True)
         assert np.array_equal(Ip, Im[:, :5])

@@ -20,13 +20,18 @@
 // kernel copies f into energy order and the epilogue scatters back.
 //
 // Packed layouts (16-byte units of eight fp16 values along the reduction k):
-//   A_{hi,lo}[kg][row]   kg = k/8, row in [0, rows_pad)
+//   A band: per panel its band only, iteration-major, each iteration one
+//     16 KB block [kg in 0..3][hi, lo][row in 0..127] (kg = k/8 within the
+//     iteration); the panel's blocks are contiguous from band_off[panel]
 //   Z_{hi,lo}[kg][col]   col = member in [0, nt)
-// A workgroup's LDS tile and every MFMA fragment are contiguous 16-byte
-// reads in these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1).
+// Every MFMA fragment and LDS tile is a contiguous 16-byte read per lane in
+// these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1), and a
+// panel streams its band as consecutive 16 KB blocks.  The dense layout
+// A_{hi,lo}[kg][row] exists only while the band is being measured.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <numeric>
 #include <vector>
 
 #include "tg_internal.hpp"
@@ -52,8 +57,11 @@ struct MixedData {
     int nt = 256;              // members per GEMM tile (padded batch)
     int sa = 0;                // A scale exponent
     int npanel = 0;
-    uint4* A_hi = nullptr;
+    uint4* A_hi = nullptr;     // dense packed A (during preparation only)
     uint4* A_lo = nullptr;
+    uint4* A_band = nullptr;   // band-major packed A (hi and lo)
+    int64_t* band_off = nullptr;    // per panel: first 16-byte unit of its band
+    int64_t band_units = 0;
     uint4* F_hi = nullptr;
     uint4* F_lo = nullptr;
     uint4* G_hi = nullptr;     // operand g = 1 - f
@@ -74,6 +82,7 @@ struct MixedData {
     bool persist = true;       // persistent balanced kernel (else the tiled grid kernel)
     int ncu = 0;               // compute units of the device
     int da = 2;                // persistent kernel: iterations of register prefetch
+    int al = 0;                // persistent kernel: A staged through LDS
     int sched_nw = 0;          // tile width the schedule below was built for (0: none)
     int bins_per_xcd = 0;
     int32_t* sched_off = nullptr;    // per bin: first entry in sched_panel (nbins + 1)
@@ -158,6 +167,23 @@ __global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* 
     s *= inv_scale;
     if (row < n) rk[row] = s;
     else ck[row - kt] = s;
+}
+
+// dense packed A -> band-major packed A: block (panel, iteration) holds the
+// k-groups 4 it .. 4 it + 3 of the panel's band, hi and lo, 128 rows each
+constexpr int kBandT = 2 * kKG * kMT;
+__global__ void __launch_bounds__(kBandT)
+band_pack_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo, int64_t rows_pad,
+                 const int2* __restrict__ krange, const int64_t* __restrict__ band_off,
+                 uint4* __restrict__ band) {
+    const int panel = blockIdx.x;
+    const int itl = blockIdx.y;
+    const int2 kr = krange[panel];
+    if (itl >= kr.y - kr.x) return;
+    const int t = threadIdx.x;
+    const int r = t % kMT, h = (t / kMT) % 2, kg = t / (2 * kMT);
+    const int64_t src = ((int64_t)(kr.x + itl) * kKG + kg) * rows_pad + (int64_t)panel * kMT + r;
+    band[band_off[panel] + (int64_t)itl * kBandT + t] = h ? Alo[src] : Ahi[src];
 }
 
 // Member-per-workgroup kernels.  Workgroup w serves member
@@ -283,17 +309,14 @@ template <>
 struct Partial<0> { typedef float type; };
 
 template <int NTH, int N>
-__device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict__ Ahi,
-                                        const u32x4* __restrict__ Alo, int64_t rows_pad,
-                                        int64_t row0, int64_t kg0, int tid) {
+__device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict__ Ab, int64_t itl, int tid) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int q = tid + j * NTH;
         const int which = q / (kKG * kMT);
         const int kg = (q / kMT) % kKG;
         const int r = q % kMT;
-        const u32x4* src = which ? Alo : Ahi;
-        ra[j] = src[(kg0 + kg) * rows_pad + row0 + r];
+        ra[j] = Ab[((itl * kKG + kg) * 2 + which) * kMT + r];
     }
 }
 
@@ -314,7 +337,7 @@ __device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* __restrict_
 
 template <int NW, int WAVES, int FOLD, int DEPTH>
 __global__ void __launch_bounds__(64 * WAVES)
-mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
+mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ band_off,
                   const u32x4* __restrict__ Fhi, const u32x4* __restrict__ Flo,
                   const u32x4* __restrict__ Ghi, const u32x4* __restrict__ Glo,
                   const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
@@ -346,6 +369,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
     const int col0 = mt * NW;
 
     const int2 kr = krange[panel];
+    const u32x4* Ab = Aband + band_off[panel];
     const int nit = kr.y - kr.x;
     const int it0 = kr.x + (int)((int64_t)nit * split / nsplit);
     const int it1 = kr.x + (int)((int64_t)nit * (split + 1) / nsplit);
@@ -382,7 +406,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
 #pragma unroll
         for (int d = 0; d < DEPTH; ++d) {
             const int il = min(it0 + d, last);
-            stage_a<NTH>(ra[d], Ahi, Alo, rows_pad, row0, (int64_t)il * kKG, tid);
+            stage_a<NTH>(ra[d], Ab, il - kr.x, tid);
             stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
         }
         int since = 0;
@@ -403,7 +427,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
                 __syncthreads();
                 {
                     const int il = min(it + DEPTH, last);
-                    stage_a<NTH>(ra[d], Ahi, Alo, rows_pad, row0, (int64_t)il * kKG, tid);
+                    stage_a<NTH>(ra[d], Ab, il - kr.x, tid);
                     stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
                 }
                 if (it < it1) {
@@ -469,12 +493,15 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
         }
 }
 
-// Persistent variant: one workgroup of four waves per CU, each wave owning
-// 32 rows of the panel and all NW members of the tile (TN = NW/32 MFMA tiles
-// of 32 x 32).  A fragments go from global memory straight into the owning
-// wave's registers (no LDS), DA iterations ahead; the Z tile is the only
-// operand shared by the waves and goes through double-buffered LDS with one
-// barrier per iteration, also DA iterations ahead.
+// Persistent variant: one workgroup of 4 MG waves per CU (MG waves per
+// SIMD).  Wave w owns 32 rows of the panel (row group w % 4) and NW / MG
+// members of the tile (member group w / 4), i.e. TN = NW / (32 MG) MFMA
+// tiles of 32 x 32.  A fragments go from global memory straight into the
+// wave's registers (no LDS), DA iterations ahead; the Z tile is the operand
+// all row groups share and goes through double-buffered LDS with one barrier
+// per iteration, also DA iterations ahead.  With MG = 2 a SIMD runs two
+// waves, so one wave's FP64 folds and memory waits overlap the other's
+// MFMAs.
 //
 // The host balances the panels over the workgroups (longest band first onto
 // the least loaded bin, a fixed order, so every call runs the same
@@ -482,11 +509,9 @@ mixed_gemm_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
 // list for the nmt member tiles side by side, so a panel's A band is fetched
 // into that XCD's L2 once for all member tiles.  With nsplit = 1 and the
 // same FOLD the summation order is that of mixed_gemm_kernel.
-constexpr int kPerT = 256;
-
-template <int NW, int FOLD, int DA>
-__global__ void __launch_bounds__(kPerT, 1)
-mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict__ Alo,
+template <int NW, int FOLD, int DA, int MG, bool AL>
+__global__ void __launch_bounds__(256 * MG, 1)
+mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ band_off,
                           const u32x4* __restrict__ Fhi, const u32x4* __restrict__ Flo,
                           const u32x4* __restrict__ Ghi, const u32x4* __restrict__ Glo,
                           const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
@@ -494,11 +519,18 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict
                           const int32_t* __restrict__ sched_panel, int nmt, int bins_per_xcd,
                           double* __restrict__ P) {
     static_assert(FOLD > 0, "the persistent kernel folds into FP64");
-    constexpr int TN = NW / 32;
-    constexpr int Z_PER_T = 2 * kKG * NW / kPerT;
-    static_assert(Z_PER_T >= 1 && kPerT % NW == 0, "Z staging assumes one member per thread");
-    constexpr int U = DA % 2 == 0 ? DA : 2 * DA;   // unroll: register slot and LDS buffer fixed per step
+    constexpr int NTH = 256 * MG;
+    constexpr int WN = NW / MG;                    // members per wave
+    constexpr int TN = WN / 32;
+    constexpr int Z_PER_T = 2 * kKG * NW / NTH;
+    static_assert(TN >= 1 && WN % 32 == 0, "member group narrower than one MFMA tile");
+    static_assert(Z_PER_T >= 1 && NTH % NW == 0, "Z staging assumes one member per thread");
+    // unroll U = lcm(2, DA, FOLD): register slot, LDS buffer and fold fixed per step
+    constexpr int U = std::lcm(std::lcm(2, DA), FOLD);
+    static_assert(U % DA == 0 && U % 2 == 0 && U % FOLD == 0, "unroll must hold whole prefetch, buffer and fold cycles");
+    constexpr int A_PER_T = 2 * kKG * kMT / NTH;
     __shared__ u32x4 sZ[2][2][kKG][NW];
+    __shared__ u32x4 sA[AL ? 2 : 1][kKG][2][AL ? kMT : 1];
 
     const int w = blockIdx.x;
     const int wi = w / 8;
@@ -507,13 +539,15 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict
     if (wi / nmt >= bins_per_xcd) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
+    const int rg = wave % 4, mg = wave / 4;
     const int col0 = mt * NW;
     const int zcol = tid % NW;
 
     for (int s = sched_off[bin]; s < sched_off[bin + 1]; ++s) {
         const int panel = sched_panel[s];
         const int64_t row0 = (int64_t)panel * kMT;
-        const int64_t arow = row0 + wave * 32 + (lane & 31);
+        const int arow = rg * 32 + (lane & 31);
+        const u32x4* Ab = Aband + band_off[panel];
         const int2 kr = krange[panel];
         const int N = kr.y - kr.x;
         const bool my_g = useg[(int64_t)panel * nt + col0 + zcol] != 0;
@@ -529,22 +563,33 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict
             for (int e = 0; e < 16; ++e) sum[u][e] = 0.0;
         }
         if (N > 0) {
-            u32x4 ra[DA][4], rz[DA][Z_PER_T];
+            u32x4 ra[AL ? 1 : DA][4], rz[DA][Z_PER_T];
+            // AL: this thread's units of the panel's 16 KB A block, staged in LDS
+            u32x4 rst[AL ? DA : 1][A_PER_T];
+            auto load_ast = [&](u32x4 (&r)[A_PER_T], int k) {
+                const u32x4* blk = Ab + (int64_t)min(k, N - 1) * (2 * kKG * kMT);
+#pragma unroll
+                for (int j = 0; j < A_PER_T; ++j) r[j] = blk[tid + j * NTH];
+            };
+            auto store_ast = [&](const u32x4 (&r)[A_PER_T], int buf) {
+#pragma unroll
+                for (int j = 0; j < A_PER_T; ++j) (&sA[buf][0][0][0])[tid + j * NTH] = r[j];
+            };
             // A(k): lane's 16-byte units of k-steps 0 and 1 (hi, lo); Z(k): this
             // thread's units of the shared tile; iteration indices clamp at the end
             auto load_a = [&](u32x4 (&r)[4], int k) {
-                const int64_t kg = (int64_t)(kr.x + min(k, N - 1)) * kKG + (lane >> 5);
+                const int64_t kg = (int64_t)min(k, N - 1) * kKG + (lane >> 5);
 #pragma unroll
                 for (int ks = 0; ks < 2; ++ks) {
-                    r[2 * ks] = Ahi[(kg + 2 * ks) * rows_pad + arow];
-                    r[2 * ks + 1] = Alo[(kg + 2 * ks) * rows_pad + arow];
+                    r[2 * ks] = Ab[((kg + 2 * ks) * 2) * kMT + arow];
+                    r[2 * ks + 1] = Ab[((kg + 2 * ks) * 2 + 1) * kMT + arow];
                 }
             };
             auto load_z = [&](u32x4 (&r)[Z_PER_T], int k) {
                 const int64_t kg0 = (int64_t)(kr.x + min(k, N - 1)) * kKG;
 #pragma unroll
                 for (int j = 0; j < Z_PER_T; ++j) {
-                    const int q = tid + j * kPerT;
+                    const int q = tid + j * NTH;
                     const int which = q / (kKG * NW);
                     const int kg = (q / NW) % kKG;
                     r[j] = (which ? Zlo : Zhi)[(kg0 + kg) * nt + col0 + zcol];
@@ -553,64 +598,89 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Ahi, const u32x4* __restrict
             auto store_z = [&](const u32x4 (&r)[Z_PER_T], int buf) {
 #pragma unroll
                 for (int j = 0; j < Z_PER_T; ++j) {
-                    const int q = tid + j * kPerT;
+                    const int q = tid + j * NTH;
                     sZ[buf][q / (kKG * NW)][(q / NW) % kKG][q % NW] = r[j];
                 }
             };
-            // prologue: A(0..DA-1); Z(0) to LDS buffer 0, Z(1..DA) in registers
+            // one iteration: MFMAs on A(k) (registers, slot k % DA) and Z(k) (LDS
+            // buffer k & 1); then A(k+DA) into the freed slot, Z(k+1) (loaded DA
+            // iterations ago) into the other LDS buffer and Z(k+1+DA) into its
+            // slot; the FP64 fold after every FOLD iterations.  uu = k mod U is a
+            // compile-time constant at every call, so slots, buffers and the fold
+            // position are fixed in the code.
+            auto step = [&](int k, int uu) {
+                const int da = uu % DA, buf = uu & 1;
 #pragma unroll
-            for (int d = 0; d < DA; ++d) load_a(ra[d], d);
+                for (int ks = 0; ks < 2; ++ks) {
+                    const int kg = 2 * ks + (lane >> 5);
+                    const half8 ah = __builtin_bit_cast(half8, AL ? sA[buf][kg][0][arow] : ra[AL ? 0 : da][2 * ks]);
+                    const half8 al = __builtin_bit_cast(half8, AL ? sA[buf][kg][1][arow] : ra[AL ? 0 : da][2 * ks + 1]);
+#pragma unroll
+                    for (int u = 0; u < TN; ++u) {
+                        const int col = mg * WN + u * 32 + (lane & 31);
+                        const half8 zh = __builtin_bit_cast(half8, sZ[buf][0][kg][col]);
+                        const half8 zl = __builtin_bit_cast(half8, sZ[buf][1][kg][col]);
+                        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zh, acc[u], 0, 0, 0);
+                        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zl, acc[u], 0, 0, 0);
+                        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, zh, acc[u], 0, 0, 0);
+                    }
+                }
+                const int dz = (uu + 1) % DA;
+                if constexpr (AL) {
+                    store_ast(rst[dz], (uu + 1) & 1);
+                    load_ast(rst[dz], k + 1 + DA);
+                } else {
+                    load_a(ra[da], k + DA);
+                }
+                store_z(rz[dz], (uu + 1) & 1);
+                load_z(rz[dz], k + 1 + DA);
+                if ((uu + 1) % FOLD == 0) {
+#pragma unroll
+                    for (int u = 0; u < TN; ++u) {
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) sum[u][e] += (double)acc[u][e];
+                        acc[u] = floatx16{0};
+                    }
+                }
+                __syncthreads();
+            };
+            // prologue: Z(0) to LDS buffer 0, then the loads of "iterations" -DA..-1
+            // in the order the loop issues them (A(j+DA), Z(j+1+DA)), so the wait
+            // counts at the loop head are the same from the prologue and from the
+            // back edge
+            if constexpr (AL) {
+                load_ast(rst[0], 0);
+                store_ast(rst[0], 0);
+            }
             load_z(rz[0], 0);
             store_z(rz[0], 0);
 #pragma unroll
-            for (int d = 1; d < DA; ++d) load_z(rz[d], d);
-            load_z(rz[0], DA);
+            for (int d = 0; d < DA; ++d) {
+                if constexpr (AL) load_ast(rst[(d + 1) % DA], d + 1);
+                else load_a(ra[d], d);
+                load_z(rz[(d + 1) % DA], d + 1);
+            }
             __syncthreads();
-            for (int k0 = 0; k0 < N; k0 += U) {
+            const int nfull = N / U * U;
+            for (int k0 = 0; k0 < nfull; k0 += U) {
 #pragma unroll
-                for (int uu = 0; uu < U; ++uu) {
-                    const int k = k0 + uu;
-                    if (k < N) {
-                        const int da = uu % DA, buf = uu & 1;
+                for (int uu = 0; uu < U; ++uu) step(k0 + uu, uu);
+            }
 #pragma unroll
-                        for (int ks = 0; ks < 2; ++ks) {
-                            const int kg = 2 * ks + (lane >> 5);
-                            const half8 ah = __builtin_bit_cast(half8, ra[da][2 * ks]);
-                            const half8 al = __builtin_bit_cast(half8, ra[da][2 * ks + 1]);
+            for (int uu = 0; uu + 1 < U; ++uu)
+                if (nfull + uu < N) step(nfull + uu, uu);
 #pragma unroll
-                            for (int u = 0; u < TN; ++u) {
-                                const int col = u * 32 + (lane & 31);
-                                const half8 zh = __builtin_bit_cast(half8, sZ[buf][0][kg][col]);
-                                const half8 zl = __builtin_bit_cast(half8, sZ[buf][1][kg][col]);
-                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zh, acc[u], 0, 0, 0);
-                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, zl, acc[u], 0, 0, 0);
-                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, zh, acc[u], 0, 0, 0);
-                            }
-                        }
-                        load_a(ra[da], k + DA);
-                        // Z(k+1) was loaded DA-1 iterations ago into slot (k+1) % DA
-                        const int dz = (uu + 1) % DA;
-                        store_z(rz[dz], (uu + 1) & 1);
-                        load_z(rz[dz], k + 1 + DA);
-                        if ((k + 1) % FOLD == 0 || k + 1 == N) {
+            for (int u = 0; u < TN; ++u) {
 #pragma unroll
-                            for (int u = 0; u < TN; ++u) {
-#pragma unroll
-                                for (int e = 0; e < 16; ++e) sum[u][e] += (double)acc[u][e];
-                                acc[u] = floatx16{0};
-                            }
-                        }
-                        __syncthreads();
-                    }
-                }
+                for (int e = 0; e < 16; ++e) sum[u][e] += (double)acc[u][e];
             }
         }
         // C tile -> P[member][row]; lane holds member (lane & 31) of each
         // 32-wide tile and rows 8*(e/4) + 4*(lane/32) + (e%4) of the wave's 32
 #pragma unroll
         for (int u = 0; u < TN; ++u) {
-            const int col = col0 + u * 32 + (lane & 31);
-            double* dst = P + (int64_t)col * rows_pad + row0 + wave * 32 + 4 * (lane >> 5);
+            const int col = col0 + mg * WN + u * 32 + (lane & 31);
+            double* dst = P + (int64_t)col * rows_pad + row0 + rg * 32 + 4 * (lane >> 5);
 #pragma unroll
             for (int e = 0; e < 16; ++e) dst[8 * (e / 4) + (e % 4)] = sum[u][e];
         }
@@ -735,7 +805,7 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     const int64_t nblocks = (int64_t)m->npanel * m->nsplit * nmt;
     const int64_t per = (nblocks + 7) / 8;
     hipLaunchKernelGGL((mixed_gemm_kernel<NW, WAVES, FOLD, DEPTH>), dim3((unsigned)(8 * per)), dim3(64 * WAVES),
-                       0, c->stream, (const u32x4*)m->A_hi, (const u32x4*)m->A_lo,
+                       0, c->stream, (const u32x4*)m->A_band, m->band_off,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, nmt, m->nsplit,
                        nblocks, per, (typename Partial<FOLD>::type*)m->P);
@@ -747,12 +817,17 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     X(128, 4, 1, 1) X(128, 8, 1, 1) X(128, 8, 1, 2) X(128, 8, 1, 3) X(128, 8, 2, 2)             \
     X(256, 4, 0, 1) X(256, 4, 0, 2) X(256, 8, 0, 2)
 
-// the persistent configurations: (members, fold, prefetch depth)
-#define TG_PERSIST_CONFIGS(X) X(64, 2, 2) X(64, 2, 3) X(128, 1, 2) X(128, 2, 2) X(128, 2, 3) X(128, 4, 2)
+// the persistent configurations: (members, fold, prefetch depth, waves per
+// SIMD, A staged through LDS instead of straight into registers)
+#define TG_PERSIST_CONFIGS(X)                                                                       \
+    X(64, 2, 2, 1, 0) X(64, 2, 3, 1, 0) X(128, 1, 2, 1, 0) X(128, 2, 2, 1, 0) X(128, 2, 3, 1, 0)         \
+    X(128, 4, 2, 1, 0) X(128, 2, 2, 2, 0) X(128, 2, 3, 2, 0) X(128, 4, 2, 2, 0) X(128, 4, 3, 2, 0)       \
+    X(64, 2, 2, 2, 0) X(64, 2, 3, 2, 0) X(128, 2, 2, 2, 1) X(128, 2, 3, 2, 1) X(128, 4, 2, 2, 1)         \
+    X(64, 2, 2, 2, 1)
 
-static bool config_known(bool persist, int nw, int waves, int fold, int depth) {
+static bool config_known(bool persist, int nw, int waves, int fold, int depth, int al) {
     if (persist) {
-#define TG_X(a, c_, d_) if (nw == a && fold == c_ && depth == d_) return true;
+#define TG_X(a, c_, d_, g_, z_) if (nw == a && fold == c_ && depth == d_ && waves == 4 * g_ && al == z_) return true;
         TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
         return false;
@@ -806,11 +881,11 @@ static tg_status build_schedule(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
-template <int NW, int FOLD, int DA>
+template <int NW, int FOLD, int DA, int MG, bool AL>
 static void launch_persist_t(tg_ctx* c, MixedData* m) {
     const int nmt = m->nt / NW;
-    hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
-                       dim3(kPerT), 0, c->stream, (const u32x4*)m->A_hi, (const u32x4*)m->A_lo,
+    hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA, MG, AL>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
+                       dim3(256 * MG), 0, c->stream, (const u32x4*)m->A_band, m->band_off,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, m->sched_off,
                        m->sched_panel, nmt, m->bins_per_xcd, (double*)m->P);
@@ -821,11 +896,13 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     if (m->persist) {
         if (m->sched_nw != m->nw || m->nsplit != 1)
             return set_error(c, TG_ERR_STATE, "persistent GEMM schedule not built for tile %d", m->nw);
-#define TG_X(a, c_, d_) if (m->nw == a && m->fold == c_ && m->da == d_) launch_persist_t<a, c_, d_>(c, m); else
+#define TG_X(a, c_, d_, g_, z_)                                                                   \
+    if (m->nw == a && m->fold == c_ && m->da == d_ && m->waves == 4 * g_ && m->al == z_)                  \
+        launch_persist_t<a, c_, d_, g_, z_ != 0>(c, m); else
         TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
-        return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d", m->nw,
-                         m->fold, m->da);
+        return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d, %d waves, A in LDS %d",
+                         m->nw, m->fold, m->da, m->waves, m->al);
         TG_CHECK_LAUNCH(c);
         return TG_OK;
     }
@@ -898,6 +975,7 @@ constexpr int kTuneReps = 5;
 struct TuneCand {
     bool persist;
     int nw, waves, fold, depth, nsplit;
+    int al;
 };
 
 static void apply_cand(MixedData* m, const TuneCand& cd) {
@@ -905,25 +983,24 @@ static void apply_cand(MixedData* m, const TuneCand& cd) {
     m->nw = cd.nw;
     m->fold = cd.fold;
     m->nsplit = cd.nsplit;
+    m->waves = cd.waves;
+    m->al = cd.al;
     if (cd.persist) m->da = cd.depth;
-    else {
-        m->waves = cd.waves;
-        m->depth = cd.depth;
-    }
+    else m->depth = cd.depth;
 }
 
 static TuneCand current_cand(const MixedData* m) {
-    return TuneCand{m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth, m->nsplit};
+    return TuneCand{m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth, m->nsplit, m->al};
 }
 
 static tg_status autotune(tg_ctx* c, MixedData* m) {
     std::vector<TuneCand> cands;
-#define TG_X(a, c_, d_) if (a <= m->nt) cands.push_back(TuneCand{true, a, 4, c_, d_, 1});
+#define TG_X(a, c_, d_, g_, z_) if (a <= m->nt) cands.push_back(TuneCand{true, a, 4 * g_, c_, d_, 1, z_});
     TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
 #define TG_X(a, b, c_, d_)                                                        \
     if (c_ > 0 && a <= m->nt)                                                     \
-        for (int ns = 1; ns <= 2; ++ns) cands.push_back(TuneCand{false, a, b, c_, d_, ns});
+        for (int ns = 1; ns <= 2; ++ns) cands.push_back(TuneCand{false, a, b, c_, d_, ns, 0});
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
     if (cands.empty()) return TG_OK;
@@ -970,7 +1047,7 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= kTuneReps;
         if (cd.persist == dflt.persist && cd.nw == dflt.nw && cd.fold == dflt.fold && cd.depth == dflt.depth &&
-            cd.nsplit == dflt.nsplit && (cd.persist || cd.waves == dflt.waves))
+            cd.nsplit == dflt.nsplit && cd.waves == dflt.waves && cd.al == dflt.al)
             dflt_ms = ms;
         if (best == 0.0f || ms < best) {
             best = ms;
@@ -1084,27 +1161,53 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
+    // band-major copy of A; the dense packing is freed once it is made
+    {
+        std::vector<int64_t> off(m->npanel);
+        int64_t units = 0;
+        int maxit = 0;
+        for (int pn = 0; pn < m->npanel; ++pn) {
+            off[pn] = units;
+            const int nit = m->h_krange[pn].y - m->h_krange[pn].x;
+            units += (int64_t)nit * kBandT;
+            maxit = std::max(maxit, nit);
+        }
+        m->band_units = units;
+        TG_TRY(dev_alloc(c, &m->band_off, (size_t)m->npanel));
+        TG_HIP(c, hipMemcpyAsync(m->band_off, off.data(), sizeof(int64_t) * m->npanel, hipMemcpyHostToDevice, st));
+        TG_TRY(dev_alloc(c, &m->A_band, (size_t)std::max<int64_t>(units, 1)));
+        if (maxit > 0) {
+            hipLaunchKernelGGL(band_pack_kernel, dim3(m->npanel, maxit), dim3(kBandT), 0, st, m->A_hi, m->A_lo,
+                               m->rows_pad, m->krange, m->band_off, m->A_band);
+            TG_CHECK_LAUNCH(c);
+        }
+        TG_HIP(c, hipStreamSynchronize(st));
+        dev_free(&m->A_hi);
+        dev_free(&m->A_lo);
+    }
     // configuration.  Default (kept by the autotuner unless beaten by >3%):
     // the persistent kernel, 128 members per tile (64 for smaller batches),
-    // FP64 fold every 2 iterations, 2 iterations of prefetch.  The tiled grid
-    // kernel: 128 members, 8 waves, fold 2, prefetch 2, one K-split or two
-    // when the panels alone leave CUs idle.
+    // 8 waves, A staged through LDS, FP64 fold every 2 iterations, 2
+    // iterations of prefetch.  The tiled grid kernel: 128 members (8 waves;
+    // 64 members: 4 waves), fold 2, prefetch 2, one K-split or two when the
+    // panels alone leave CUs idle.
     {
         hipDeviceProp_t prop;
         TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
         m->ncu = prop.multiProcessorCount;
     }
     m->nw = std::min(128, m->nt);
-    m->waves = m->nw == 128 ? 8 : 4;
+    m->waves = 8;
     m->fold = 2;
     m->depth = 2;
     m->da = 2;
     m->persist = true;
+    m->al = 1;
     m->nsplit = 1;
     // explicit overrides (measurements of the tile configuration) replace the autotuner
     bool fixed = false;
     for (const char* v : {"TG_MIXED_NW", "TG_MIXED_FOLD", "TG_MIXED_WAVES", "TG_MIXED_DEPTH", "TG_MIXED_SPLITS",
-                          "TG_MIXED_PERSIST"})
+                          "TG_MIXED_PERSIST", "TG_MIXED_AL"})
         if (std::getenv(v)) fixed = true;
     if (const char* e = std::getenv("TG_MIXED_AUTOTUNE"))
         if (std::atoi(e) == 0) fixed = true;
@@ -1112,6 +1215,7 @@ tg_status mixed_prepare(tg_ctx* c) {
     // selects it unless TG_MIXED_PERSIST says otherwise
     if (std::getenv("TG_MIXED_WAVES") || std::getenv("TG_MIXED_SPLITS")) m->persist = false;
     if (const char* e = std::getenv("TG_MIXED_PERSIST")) m->persist = std::atoi(e) != 0;
+    if (!m->persist && m->nw == 64) m->waves = 4;
     if (const char* e = std::getenv("TG_MIXED_NW")) {
         const int v = std::atoi(e);
         if ((v == 64 || v == 128 || v == 256) && v <= m->nt) m->nw = v;
@@ -1119,11 +1223,15 @@ tg_status mixed_prepare(tg_ctx* c) {
     if (const char* e = std::getenv("TG_MIXED_FOLD")) m->fold = std::atoi(e);
     if (const char* e = std::getenv("TG_MIXED_WAVES")) m->waves = std::atoi(e);
     if (const char* e = std::getenv("TG_MIXED_DEPTH")) m->depth = m->da = std::atoi(e);
-    if (!config_known(m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth)) {
+    if (const char* e = std::getenv("TG_MIXED_AL")) m->al = std::atoi(e) != 0 ? 1 : 0;
+    if (!m->persist) m->al = 0;
+    if (!config_known(m->persist, m->nw, m->waves, m->fold, m->persist ? m->da : m->depth, m->al)) {
         m->persist = true;
         m->nw = std::min(128, m->nt);
+        m->waves = 8;
         m->fold = 2;
         m->da = 2;
+        m->al = 1;
     }
     if (!m->persist) {
         const int nmt = m->nt / m->nw;
@@ -1172,9 +1280,9 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_scale = m->sa;
     out->mixed_tile = m->nw;
     out->mixed_fold = m->fold;
-    out->mixed_waves = m->persist ? kPerT / 64 : m->waves;
+    out->mixed_waves = m->waves;
     out->mixed_depth = m->persist ? m->da : m->depth;
-    out->mixed_persist = m->persist ? 1 : 0;
+    out->mixed_persist = m->persist ? 1 + m->al : 0;
     out->mixed_tuned_ms = m->tuned_ms;
 }
 
@@ -1183,6 +1291,8 @@ void mixed_destroy(tg_ctx* c) {
     if (!m) return;
     dev_free(&m->A_hi);
     dev_free(&m->A_lo);
+    dev_free(&m->A_band);
+    dev_free(&m->band_off);
     dev_free(&m->F_hi);
     dev_free(&m->F_lo);
     dev_free(&m->G_hi);

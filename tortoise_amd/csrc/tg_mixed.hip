@@ -23,7 +23,9 @@
 //   A band: per panel its band only, iteration-major, each iteration one
 //     16 KB block [kg in 0..3][hi, lo][row in 0..127] (kg = k/8 within the
 //     iteration); the panel's blocks are contiguous from band_off[panel]
-//   Z_{hi,lo}[kg][col]   col = member in [0, nt)
+//   Z_{hi,lo}[it][col][kgl]   it = kg/4, kgl = kg%4, col = member in [0, nt):
+//     a member's four k-groups of one iteration are 64 contiguous bytes, and
+//     a tile of members of one iteration is one contiguous block
 // Every MFMA fragment and LDS tile is a contiguous 16-byte read per lane in
 // these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1), and a
 // panel streams its band as consecutive 16 KB blocks.  The dense layout
@@ -97,13 +99,24 @@ struct MixedData {
 static MixedData* md_of(tg_ctx* c) { return (MixedData*)c->mixed; }
 
 // ------------------------------------------------------------------ packing
-// hi = s rounded to fp16 through FP32 (two hardware conversions); a double
-// rounding tie moves hi by one fp16 ulp, which the exact FP64 residual s - hi
-// absorbs, so hi + lo keeps its ~22 significant bits
+// FP64 -> fp16 through FP32 with the two hardware conversions.  The empty
+// asm statement keeps the FP32 value opaque: without it the compiler folds
+// the two conversions into one correctly rounded double -> half conversion,
+// which gfx950 has no instruction for and which it emulates with ~40 integer
+// operations.
+__device__ inline _Float16 to_fp16(double s) {
+    float t = (float)s;
+    __asm__ volatile("" : "+v"(t));
+    return (_Float16)t;
+}
+
+// hi = s rounded to fp16 through FP32; a double rounding tie moves hi by one
+// fp16 ulp, which the exact FP64 residual s - hi absorbs, so hi + lo keeps
+// its ~22 significant bits
 __device__ inline void split_fp16(double x, double scale, _Float16& hi, _Float16& lo) {
     const double s = x * scale;
-    hi = (_Float16)(float)s;
-    lo = (_Float16)(float)(s - (double)hi);
+    hi = to_fp16(s);
+    lo = to_fp16(s - (double)hi);
 }
 
 // A[r][q] from K stored in energy order: r < n -> K[r][q]; the K^T half
@@ -198,16 +211,15 @@ constexpr int kMemT = 1024;
 __device__ inline int member_of_block(int w, int nt) { return (w % 8) * (nt / 8) + w / 8; }
 
 // stage row b of f (caller order) in LDS, UNR independent loads per thread in
-// flight before any is stored
+// flight before any is stored.  Every load is unconditional (indices clamp to
+// the last node) so the loads issue back to back; only the stores are
+// guarded.
 __device__ inline void stage_member_row(const double* __restrict__ fb, int64_t n, double* row) {
     constexpr int UNR = 8;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * UNR) {
         double v[UNR];
 #pragma unroll
-        for (int j = 0; j < UNR; ++j) {
-            const int64_t i = i0 + (int64_t)j * kMemT;
-            v[j] = i < n ? fb[i] : 0.0;
-        }
+        for (int j = 0; j < UNR; ++j) v[j] = fb[min(i0 + (int64_t)j * kMemT, n - 1)];
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t i = i0 + (int64_t)j * kMemT;
@@ -255,20 +267,20 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
     }
     const double scale = (double)(1 << kSF);
     for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
+        // unconditional loads at clamped positions; padding and idle members
+        // select zero afterwards
         int32_t node[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int64_t p = kg * 8 + q;
-            node[q] = (live && p < n) ? perm[p] : -1;
-        }
+        for (int q = 0; q < 8; ++q) node[q] = perm[min(kg * 8 + q, n - 1)];
+        double xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) xv[q] = ROW ? row[node[q]] : fb[node[q]];
         half8 fh, fl, gh, gl;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            double x = 0.0, y = 0.0;
-            if (node[q] >= 0) {
-                x = ROW ? row[node[q]] : fb[node[q]];
-                y = 1.0 - x;
-            }
+            const bool valid = live && kg * 8 + q < n;
+            const double x = valid ? xv[q] : 0.0;
+            const double y = valid ? 1.0 - xv[q] : 0.0;
             _Float16 hi, lo;
             split_fp16(x, scale, hi, lo);
             fh[q] = hi;
@@ -277,10 +289,13 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
             gh[q] = hi;
             gl[q] = lo;
         }
-        Fhi[kg * nt + b] = __builtin_bit_cast(u32x4, fh);
-        Flo[kg * nt + b] = __builtin_bit_cast(u32x4, fl);
-        Ghi[kg * nt + b] = __builtin_bit_cast(u32x4, gh);
-        Glo[kg * nt + b] = __builtin_bit_cast(u32x4, gl);
+        // four neighbouring threads (k-groups of one iteration) write 64
+        // contiguous bytes of each operand
+        const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
+        Fhi[u] = __builtin_bit_cast(u32x4, fh);
+        Flo[u] = __builtin_bit_cast(u32x4, fl);
+        Ghi[u] = __builtin_bit_cast(u32x4, gh);
+        Glo[u] = __builtin_bit_cast(u32x4, gl);
     }
 }
 
@@ -320,18 +335,28 @@ __device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict_
     }
 }
 
+// Unit q of a Z tile (NW members, one iteration, hi and lo): consecutive q
+// are consecutive 16-byte units in memory (member-major, four k-groups per
+// member); in LDS the tile is k-group-major with rows of NW + 1 units, so
+// the MFMA fragments are contiguous reads and the transposing stores spread
+// over the banks.
+template <int NW>
+__device__ __forceinline__ void z_unit(int q, int& which, int& m, int& kgl) {
+    which = q / (kKG * NW);
+    const int rem = q % (kKG * NW);
+    m = rem / kKG;
+    kgl = rem % kKG;
+}
+
 template <int NTH, int NW, int N>
-__device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* __restrict__ Zhi,
-                                        const u32x4* __restrict__ Zlo, int nt, int col0,
-                                        int64_t kg0, int tid) {
+__device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* const (&zhi)[N],
+                                        const u32x4* const (&zlo)[N], int nt, int col0, int64_t it,
+                                        int tid) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const int q = tid + j * NTH;
-        const int which = q / (kKG * NW);
-        const int kg = (q / NW) % kKG;
-        const int col = q % NW;
-        const u32x4* src = which ? Zlo : Zhi;
-        rf[j] = src[(kg0 + kg) * nt + col0 + col];
+        int which, m, kgl;
+        z_unit<NW>(tid + j * NTH, which, m, kgl);
+        rf[j] = (which ? zlo[j] : zhi[j])[(it * nt + col0 + m) * kKG + kgl];
     }
 }
 
@@ -348,11 +373,10 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
     constexpr int WN = NW / WC;                  // members per wave
     constexpr int TN = WN / 32;                  // 32-wide member tiles per wave
     static_assert(TN >= 1 && WN % 32 == 0, "member tile too narrow for the wave grid");
-    static_assert(NTH % NW == 0, "each thread stages a single member");
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
     constexpr int F_PER_T = 2 * kKG * NW / NTH;
     __shared__ u32x4 sA[2][kKG][kMT];
-    __shared__ u32x4 sF[2][kKG][NW];
+    __shared__ u32x4 sF[2][kKG][NW + 1];
 
     const int64_t wg = blockIdx.x;
     const int64_t L = (wg % 8) * per_xcd + wg / 8;
@@ -389,10 +413,17 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
                 for (int e = 0; e < 16; ++e) sum[t][u][e] = 0.0;
     }
 
-    // every Z unit this thread stages belongs to member col0 + tid % NW
-    const bool my_g = useg[(int64_t)panel * nt + col0 + (tid % NW)] != 0;
-    const u32x4* Zhi = my_g ? Ghi : Fhi;
-    const u32x4* Zlo = my_g ? Glo : Flo;
+    // operand of the member of every Z unit this thread stages
+    const u32x4* Zhi[F_PER_T];
+    const u32x4* Zlo[F_PER_T];
+#pragma unroll
+    for (int j = 0; j < F_PER_T; ++j) {
+        int which, m, kgl;
+        z_unit<NW>(tid + j * NTH, which, m, kgl);
+        const bool g = useg[(int64_t)panel * nt + col0 + m] != 0;
+        Zhi[j] = g ? Ghi : Fhi;
+        Zlo[j] = g ? Glo : Flo;
+    }
     // DEPTH register buffers: iteration it is staged from buffer (it - it0) %
     // DEPTH, whose loads were issued DEPTH iterations earlier.  Every
     // iteration of the padded count issues its loads and LDS stores
@@ -407,7 +438,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
         for (int d = 0; d < DEPTH; ++d) {
             const int il = min(it0 + d, last);
             stage_a<NTH>(ra[d], Ab, il - kr.x, tid);
-            stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
+            stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, il, tid);
         }
         int since = 0;
         for (int itb = it0; itb < it0 + nit_pad; itb += DEPTH) {
@@ -422,13 +453,15 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
 #pragma unroll
                 for (int j = 0; j < F_PER_T; ++j) {
                     const int q = tid + j * NTH;
-                    sF[q / (kKG * NW)][(q / NW) % kKG][q % NW] = rf[d][j];
+                    int which, m, kgl;
+                    z_unit<NW>(q, which, m, kgl);
+                    sF[which][kgl][m] = rf[d][j];
                 }
                 __syncthreads();
                 {
                     const int il = min(it + DEPTH, last);
                     stage_a<NTH>(ra[d], Ab, il - kr.x, tid);
-                    stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, (int64_t)il * kKG, tid);
+                    stage_z<NTH, NW>(rf[d], Zhi, Zlo, nt, col0, il, tid);
                 }
                 if (it < it1) {
 #pragma unroll
@@ -524,12 +557,12 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     constexpr int TN = WN / 32;
     constexpr int Z_PER_T = 2 * kKG * NW / NTH;
     static_assert(TN >= 1 && WN % 32 == 0, "member group narrower than one MFMA tile");
-    static_assert(Z_PER_T >= 1 && NTH % NW == 0, "Z staging assumes one member per thread");
+    static_assert(Z_PER_T >= 1 && Z_PER_T * NTH == 2 * kKG * NW, "Z tile must split evenly over the threads");
     // unroll U = lcm(2, DA, FOLD): register slot, LDS buffer and fold fixed per step
     constexpr int U = std::lcm(std::lcm(2, DA), FOLD);
     static_assert(U % DA == 0 && U % 2 == 0 && U % FOLD == 0, "unroll must hold whole prefetch, buffer and fold cycles");
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
-    __shared__ u32x4 sZ[2][2][kKG][NW];
+    __shared__ u32x4 sZ[2][2][kKG][NW + 1];
     __shared__ u32x4 sA[AL ? 2 : 1][kKG][2][AL ? kMT : 1];
 
     const int w = blockIdx.x;
@@ -541,7 +574,6 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     const int lane = tid & 63, wave = tid >> 6;
     const int rg = wave % 4, mg = wave / 4;
     const int col0 = mt * NW;
-    const int zcol = tid % NW;
 
     for (int s = sched_off[bin]; s < sched_off[bin + 1]; ++s) {
         const int panel = sched_panel[s];
@@ -550,9 +582,17 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
         const u32x4* Ab = Aband + band_off[panel];
         const int2 kr = krange[panel];
         const int N = kr.y - kr.x;
-        const bool my_g = useg[(int64_t)panel * nt + col0 + zcol] != 0;
-        const u32x4* Zhi = my_g ? Ghi : Fhi;
-        const u32x4* Zlo = my_g ? Glo : Flo;
+        // operand of the member of every Z unit this thread stages
+        const u32x4* Zhi[Z_PER_T];
+        const u32x4* Zlo[Z_PER_T];
+#pragma unroll
+        for (int j = 0; j < Z_PER_T; ++j) {
+            int which, m, kgl;
+            z_unit<NW>(tid + j * NTH, which, m, kgl);
+            const bool g = useg[(int64_t)panel * nt + col0 + m] != 0;
+            Zhi[j] = g ? Ghi : Fhi;
+            Zlo[j] = g ? Glo : Flo;
+        }
 
         floatx16 acc[TN];
         double sum[TN][16];
@@ -586,20 +626,14 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
                 }
             };
             auto load_z = [&](u32x4 (&r)[Z_PER_T], int k) {
-                const int64_t kg0 = (int64_t)(kr.x + min(k, N - 1)) * kKG;
-#pragma unroll
-                for (int j = 0; j < Z_PER_T; ++j) {
-                    const int q = tid + j * NTH;
-                    const int which = q / (kKG * NW);
-                    const int kg = (q / NW) % kKG;
-                    r[j] = (which ? Zlo : Zhi)[(kg0 + kg) * nt + col0 + zcol];
-                }
+                stage_z<NTH, NW>(r, Zhi, Zlo, nt, col0, (int64_t)(kr.x + min(k, N - 1)), tid);
             };
             auto store_z = [&](const u32x4 (&r)[Z_PER_T], int buf) {
 #pragma unroll
                 for (int j = 0; j < Z_PER_T; ++j) {
-                    const int q = tid + j * NTH;
-                    sZ[buf][q / (kKG * NW)][(q / NW) % kKG][q % NW] = r[j];
+                    int which, m, kgl;
+                    z_unit<NW>(tid + j * NTH, which, m, kgl);
+                    sZ[buf][which][kgl][m] = r[j];
                 }
             };
             // one iteration: MFMAs on A(k) (registers, slot k % DA) and Z(k) (LDS
@@ -725,9 +759,10 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
-            u[j] = 0.0;
-            v[j] = 0.0;
-            for (int s = 0; s < nsplit; ++s) {
+            const PT* P0 = P + (int64_t)b * rows_pad;
+            u[j] = (double)P0[p];
+            v[j] = (double)P0[kt + p];
+            for (int s = 1; s < nsplit; ++s) {
                 const PT* Ps = P + ((int64_t)s * nt + b) * rows_pad;
                 u[j] += (double)Ps[p];
                 v[j] += (double)Ps[kt + p];
@@ -820,7 +855,7 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
 // the persistent configurations: (members, fold, prefetch depth, waves per
 // SIMD, A staged through LDS instead of straight into registers)
 #define TG_PERSIST_CONFIGS(X)                                                                       \
-    X(64, 2, 2, 1, 0) X(64, 2, 3, 1, 0) X(128, 1, 2, 1, 0) X(128, 2, 2, 1, 0) X(128, 2, 3, 1, 0)         \
+    X(64, 2, 2, 1, 0) X(64, 2, 3, 1, 0) X(128, 1, 2, 1, 0) X(128, 2, 2, 1, 0)                           \
     X(128, 4, 2, 1, 0) X(128, 2, 2, 2, 0) X(128, 2, 3, 2, 0) X(128, 4, 2, 2, 0) X(128, 4, 3, 2, 0)       \
     X(64, 2, 2, 2, 0) X(64, 2, 3, 2, 0) X(128, 2, 2, 2, 1) X(128, 2, 3, 2, 1) X(128, 4, 2, 2, 1)         \
     X(64, 2, 2, 2, 1)

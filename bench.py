@@ -295,6 +295,15 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
+def gemm_kernel_name(st):
+    """Template name of the batched GEMM kernel the tile configuration runs."""
+    if st["mixed_persist"]:
+        return (f"mixed_gemm_persist_kernel<{st['mixed_tile']}, {st['mixed_fold']}, {st['mixed_depth']}, "
+                f"{st['mixed_waves'] // 4}, {'true' if st['mixed_persist'] == 2 else 'false'}>")
+    return (f"mixed_gemm_kernel<{st['mixed_tile']}, {st['mixed_waves']}, {st['mixed_fold']}, "
+            f"{st['mixed_depth']}>")
+
+
 def measured_traffic(st):
     """HBM bytes per launch of the batched GEMM from the committed counter
     runs, when they were taken on the tile configuration in use."""
@@ -302,7 +311,8 @@ def measured_traffic(st):
         return None, "no counter profile"
     with open(TRAFFIC_FILE) as fh:
         t = json.load(fh)
-    cfg = [st["mixed_tile"], st["mixed_waves"], st["mixed_fold"], st["mixed_depth"], st["mixed_splits"]]
+    cfg = [st["mixed_persist"], st["mixed_tile"], st["mixed_waves"], st["mixed_fold"], st["mixed_depth"],
+           st["mixed_splits"]]
     if t.get("config") != cfg:
         return None, f"counter profile taken on tile configuration {t.get('config')}, running {cfg}"
     return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"], "measured"
@@ -402,8 +412,7 @@ def main():
     gbs_streamed = (8.0 * st["plain_band"] + 32.0 * n) / (ms_dual * 1e-3) / 1e9
     _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
          f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}; "
-         f"tile {st['mixed_tile']}x{st['mixed_waves']}w fold {st['mixed_fold']} depth "
-         f"{st['mixed_depth']} splits {st['mixed_splits']}")
+         f"kernel {gemm_kernel_name(st)} splits {st['mixed_splits']}")
     _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), dual kernel "
          f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s algorithmic, {gbs_streamed:.0f} GB/s streamed")
 
@@ -427,11 +436,12 @@ def main():
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP16_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP16_MFMA_PEAK_TFLOPS,
                      "traffic": traffic, "traffic_note": traffic_note,
-                     "kernel": "mixed_gemm_kernel", "kernel_ms": ms_gemm,
+                     "kernel": gemm_kernel_name(st), "kernel_ms": ms_gemm,
                      "flops_per_launch": flops_alg, "issued_tflops": issued,
                      "issued_frac": issued / FP16_MFMA_PEAK_TFLOPS,
                      "dense_equiv_tflops": dense_tflops,
-                     "tile": {"members": st["mixed_tile"], "waves": st["mixed_waves"],
+                     "tile": {"persistent": st["mixed_persist"] > 0, "a_in_lds": st["mixed_persist"] == 2,
+                              "members": st["mixed_tile"], "waves": st["mixed_waves"],
                               "fold": st["mixed_fold"], "depth": st["mixed_depth"],
                               "splits": st["mixed_splits"], "autotuned_ms": st["mixed_tuned_ms"]}},
         "plain_fp64": {"evals_per_s": evals_plain, "ms_per_eval": ms_plain,

@@ -136,7 +136,8 @@ TILE_CONFIGS = [(64, 4, 0, 1, 16), (64, 4, 1, 1, 1), (64, 4, 1, 2, 3), (64, 4, 1
 PERSIST_CONFIGS = [(64, 2, 2, 4, 0), (64, 2, 3, 4, 0), (128, 1, 2, 4, 0), (128, 2, 2, 4, 0),
                    (128, 4, 2, 4, 0), (128, 2, 2, 8, 0), (128, 2, 3, 8, 0),
                    (128, 4, 2, 8, 0), (128, 4, 3, 8, 0), (64, 2, 2, 8, 0), (64, 2, 3, 8, 0),
-                   (128, 2, 2, 8, 1), (128, 2, 3, 8, 1), (128, 4, 2, 8, 1), (64, 2, 2, 8, 1)]
+                   (128, 2, 2, 8, 1), (128, 2, 3, 8, 1), (128, 4, 2, 8, 1), (64, 2, 2, 8, 1),
+                   (128, 2, 1, 8, 1)]
 
 
 def test_mixed_tile_configurations(monkeypatch):

@@ -858,7 +858,7 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     X(64, 2, 2, 1, 0) X(64, 2, 3, 1, 0) X(128, 1, 2, 1, 0) X(128, 2, 2, 1, 0)                           \
     X(128, 4, 2, 1, 0) X(128, 2, 2, 2, 0) X(128, 2, 3, 2, 0) X(128, 4, 2, 2, 0) X(128, 4, 3, 2, 0)       \
     X(64, 2, 2, 2, 0) X(64, 2, 3, 2, 0) X(128, 2, 2, 2, 1) X(128, 2, 3, 2, 1) X(128, 4, 2, 2, 1)         \
-    X(64, 2, 2, 2, 1)
+    X(64, 2, 2, 2, 1) X(128, 2, 1, 2, 1)
 
 static bool config_known(bool persist, int nw, int waves, int fold, int depth, int al) {
     if (persist) {

@@ -125,6 +125,19 @@ def test_mixed_member_rows_outside_lds(monkeypatch):
     assert np.array_equal(Ja, Jb)
 
 
+def test_mixed_bath_changes_between_calls(sq32_mixed):
+    """The library keeps the last per-member bath on the device and uploads
+    only when it changes: a new bath, a smaller batch with the same leading
+    values, and a return to the first bath each give the FP64 operator."""
+    w, ctx, ok, K = sq32_mixed
+    F, bath = _members(w, 8, seed=17)
+    for bb, cols in ((bath, 8), (bath * 1.5, 8), (bath * 1.5, 5), (bath, 8), (bath, 3)):
+        Im = ctx.apply(F[:, :cols], bath_n=bb[:cols], mixed=True)
+        for b in range(cols):
+            Io = oracle.collision_apply(K, ok.V, bb[b], F[:, b])
+            assert _rel(Im[:, b], Io) <= 2e-6, (cols, b)
+
+
 # every tile configuration the library carries: (members per workgroup,
 # waves, FP64 fold interval, prefetch depth), each with a K-split count
 TILE_CONFIGS = [(64, 4, 0, 1, 16), (64, 4, 1, 1, 1), (64, 4, 1, 2, 3), (64, 4, 1, 3, 2),

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -92,6 +93,7 @@ struct MixedData {
     float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
+    std::vector<double> bath_set;   // host copy of what `bath` holds (empty: unknown)
     bool rowcache = true;      // member rows fit in LDS
     int64_t work_items = 0;
 };
@@ -1047,6 +1049,7 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
     TG_HIP(c, hipMemsetAsync(m->G_lo, 0, sizeof(uint4) * kgroups * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->useg, 0, (size_t)m->npanel * m->nt, st));
     TG_HIP(c, hipMemsetAsync(m->bath, 0, sizeof(double) * c->max_batch, st));
+    m->bath_set.clear();
     double *d_zero = nullptr, *d_out = nullptr;
     TG_TRY(dev_alloc(c, &d_zero, (size_t)c->n * m->nt));
     TG_HIP(c, hipMemsetAsync(d_zero, 0, sizeof(double) * c->n * m->nt, st));
@@ -1351,9 +1354,15 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
     std::vector<double> bath(B);
     for (int b = 0; b < B; ++b) bath[b] = h_bath ? h_bath[b] : c->bath_n;
+    // the device copy already holds these values (bit for bit): no copy and no
+    // stream synchronisation, so repeated evaluations keep the queue full
+    if (bath.size() <= m->bath_set.size() &&
+        std::memcmp(bath.data(), m->bath_set.data(), sizeof(double) * B) == 0)
+        return TG_OK;
     TG_HIP(c, hipMemcpyAsync(m->bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice,
                              c->stream));
     TG_HIP(c, hipStreamSynchronize(c->stream));
+    m->bath_set = bath;
     return TG_OK;
 }
 

@@ -23,9 +23,8 @@
 //   A band: per panel its band only, iteration-major, each iteration one
 //     16 KB block [kg in 0..3][hi, lo][row in 0..127] (kg = k/8 within the
 //     iteration); the panel's blocks are contiguous from band_off[panel]
-//   Z_{hi,lo}[it][col][kgl]   it = kg/4, kgl = kg%4, col = member in [0, nt):
-//     a member's four k-groups of one iteration are 64 contiguous bytes, and
-//     a tile of members of one iteration is one contiguous block
+//   Z_{hi,lo}[kg][col]   col = member in [0, nt): a tile of members of one
+//     k-group is a contiguous run, which is also its layout in LDS
 // Every MFMA fragment and LDS tile is a contiguous 16-byte read per lane in
 // these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1), and a
 // panel streams its band as consecutive 16 KB blocks.  The dense layout
@@ -65,6 +64,7 @@ struct MixedData {
     uint4* A_band = nullptr;   // band-major packed A (hi and lo)
     int64_t* band_off = nullptr;    // per panel: first 16-byte unit of its band
     int64_t band_units = 0;
+    uint4* Z_all = nullptr;    // F_hi, F_lo, G_hi, G_lo back to back (one allocation)
     uint4* F_hi = nullptr;
     uint4* F_lo = nullptr;
     uint4* G_hi = nullptr;     // operand g = 1 - f
@@ -291,9 +291,7 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
             gh[q] = hi;
             gl[q] = lo;
         }
-        // four neighbouring threads (k-groups of one iteration) write 64
-        // contiguous bytes of each operand
-        const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
+        const int64_t u = kg * nt + b;
         Fhi[u] = __builtin_bit_cast(u32x4, fh);
         Flo[u] = __builtin_bit_cast(u32x4, fl);
         Ghi[u] = __builtin_bit_cast(u32x4, gh);
@@ -337,17 +335,15 @@ __device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict_
     }
 }
 
-// Unit q of a Z tile (NW members, one iteration, hi and lo): consecutive q
-// are consecutive 16-byte units in memory (member-major, four k-groups per
-// member); in LDS the tile is k-group-major with rows of NW + 1 units, so
-// the MFMA fragments are contiguous reads and the transposing stores spread
-// over the banks.
+// Unit q of a Z tile (NW members, the four k-groups of one iteration, hi and
+// lo): consecutive q are consecutive members of one k-group, contiguous in
+// memory and in LDS.
 template <int NW>
 __device__ __forceinline__ void z_unit(int q, int& which, int& m, int& kgl) {
     which = q / (kKG * NW);
     const int rem = q % (kKG * NW);
-    m = rem / kKG;
-    kgl = rem % kKG;
+    kgl = rem / NW;
+    m = rem % NW;
 }
 
 template <int NTH, int NW, int N>
@@ -358,7 +354,7 @@ __device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* const (&zhi
     for (int j = 0; j < N; ++j) {
         int which, m, kgl;
         z_unit<NW>(tid + j * NTH, which, m, kgl);
-        rf[j] = (which ? zlo[j] : zhi[j])[(it * nt + col0 + m) * kKG + kgl];
+        rf[j] = (which ? zlo[j] : zhi[j])[(it * kKG + kgl) * nt + col0 + m];
     }
 }
 
@@ -378,7 +374,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
     constexpr int F_PER_T = 2 * kKG * NW / NTH;
     __shared__ u32x4 sA[2][kKG][kMT];
-    __shared__ u32x4 sF[2][kKG][NW + 1];
+    __shared__ u32x4 sF[2][kKG][NW];
 
     const int64_t wg = blockIdx.x;
     const int64_t L = (wg % 8) * per_xcd + wg / 8;
@@ -564,7 +560,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     constexpr int U = std::lcm(std::lcm(2, DA), FOLD);
     static_assert(U % DA == 0 && U % 2 == 0 && U % FOLD == 0, "unroll must hold whole prefetch, buffer and fold cycles");
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
-    __shared__ u32x4 sZ[2][2][kKG][NW + 1];
+    __shared__ u32x4 sZ[2][2][kKG][NW];
     __shared__ u32x4 sA[AL ? 2 : 1][kKG][2][AL ? kMT : 1];
 
     const int w = blockIdx.x;
@@ -855,7 +851,7 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
     X(256, 4, 0, 1) X(256, 4, 0, 2) X(256, 8, 0, 2)
 
 // the persistent configurations: (members, fold, prefetch depth, waves per
-// SIMD, A staged through LDS instead of straight into registers)
+// SIMD, A path: 0 straight into registers, 1 staged through LDS)
 #define TG_PERSIST_CONFIGS(X)                                                                       \
     X(64, 2, 2, 1, 0) X(64, 2, 3, 1, 0) X(128, 1, 2, 1, 0) X(128, 2, 2, 1, 0)                           \
     X(128, 4, 2, 1, 0) X(128, 2, 2, 2, 0) X(128, 2, 3, 2, 0) X(128, 4, 2, 2, 0) X(128, 4, 3, 2, 0)       \
@@ -918,10 +914,11 @@ static tg_status build_schedule(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
-template <int NW, int FOLD, int DA, int MG, bool AL>
+template <int NW, int FOLD, int DA, int MG, int AL>
 static void launch_persist_t(tg_ctx* c, MixedData* m) {
+    static_assert(AL == 0 || AL == 1, "A path");
     const int nmt = m->nt / NW;
-    hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA, MG, AL>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
+    hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA, MG, AL == 1>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
                        dim3(256 * MG), 0, c->stream, (const u32x4*)m->A_band, m->band_off,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, m->sched_off,
@@ -935,7 +932,7 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
             return set_error(c, TG_ERR_STATE, "persistent GEMM schedule not built for tile %d", m->nw);
 #define TG_X(a, c_, d_, g_, z_)                                                                   \
     if (m->nw == a && m->fold == c_ && m->da == d_ && m->waves == 4 * g_ && m->al == z_)                  \
-        launch_persist_t<a, c_, d_, g_, z_ != 0>(c, m); else
+        launch_persist_t<a, c_, d_, g_, z_>(c, m); else
         TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
         return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d, %d waves, A in LDS %d",
@@ -1069,6 +1066,13 @@ static tg_status autotune(tg_ctx* c, MixedData* m) {
         if (cd.persist) status = build_schedule(c, m);
         if (status == TG_OK) status = ensure_P(c, m);
         if (status == TG_OK) status = launch_gemm(c, m);     // warm-up
+        if (status != TG_OK && cd.persist) {
+            // a persistent configuration the device cannot launch (LDS or
+            // register limits) is skipped
+            (void)hipGetLastError();
+            status = TG_OK;
+            continue;
+        }
         if (status == TG_OK) status = launch_finish(c, m, d_zero, d_out, Bt, 0);
         if (status != TG_OK) break;
         if (hipEventRecord(e0, st) != hipSuccess) { status = set_error(c, TG_ERR_HIP, "event record"); break; }
@@ -1297,10 +1301,12 @@ tg_status mixed_prepare(tg_ctx* c) {
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
         }
     }
-    TG_TRY(dev_alloc(c, &m->F_hi, (size_t)kgroups * m->nt));
-    TG_TRY(dev_alloc(c, &m->F_lo, (size_t)kgroups * m->nt));
-    TG_TRY(dev_alloc(c, &m->G_hi, (size_t)kgroups * m->nt));
-    TG_TRY(dev_alloc(c, &m->G_lo, (size_t)kgroups * m->nt));
+    // the four operand arrays in one allocation, each kgroups x nt units
+    TG_TRY(dev_alloc(c, &m->Z_all, 4 * (size_t)kgroups * m->nt));
+    m->F_hi = m->Z_all;
+    m->F_lo = m->Z_all + (size_t)kgroups * m->nt;
+    m->G_hi = m->Z_all + 2 * (size_t)kgroups * m->nt;
+    m->G_lo = m->Z_all + 3 * (size_t)kgroups * m->nt;
     TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
     if (m->persist) TG_TRY(build_schedule(c, m));
@@ -1331,10 +1337,8 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->A_lo);
     dev_free(&m->A_band);
     dev_free(&m->band_off);
-    dev_free(&m->F_hi);
-    dev_free(&m->F_lo);
-    dev_free(&m->G_hi);
-    dev_free(&m->G_lo);
+    dev_free(&m->Z_all);
+    m->F_hi = m->F_lo = m->G_hi = m->G_lo = nullptr;
     dev_free(&m->useg);
     dev_free(&m->krange);
     dev_free(&m->sched_off);

@@ -16,17 +16,25 @@
 //   Ha_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2+q)(1 - f(2))   (gain)
 //   Hb_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2)(1 - f(2+q))   (loss)
 // are convolved with the Gaussian on the grid and read back at e1 - e3 for
-// every k1.  A workgroup owns 16 values of q and one member:
-//   - both spectra are accumulated in LDS as int64 fixed point from one sweep
+// every k1.  The read-back differences e(k1) - e(k1 - q) are the deposited
+// differences e(k2 + q) - e(k2) with k2 = k1 - q, so the convolution is only
+// needed on the bins the spectrum occupies.  A workgroup owns 16 values of q,
+// one member and one of the two spectra (gain or loss):
+//   - the spectrum is accumulated in LDS as int64 fixed point from one sweep
 //     over k2 (integer adds are exact, so the sum does not depend on their
 //     order; lanes of a wave that hit the same bin are summed by a segmented
-//     scan before one atomic), then converted to FP64 in place;
-//   - the convolution runs in FP64, 8 grid outputs per thread from a sliding
-//     register window of the spectrum;
+//     scan before one atomic), with its occupied bin range, then converted to
+//     FP64 in place;
+//   - the convolution runs in FP64 over the occupied range, 8 grid outputs
+//     per thread from a sliding register window, and overwrites the spectrum
+//     in place once every thread holds its outputs;
 //   - each thread keeps the sums of its k1 values in registers across the
 //     16 transfers and writes them once;
-// and a last kernel sums the workgroups' partials per k1 in a fixed order and
-// adds the result to the caller's output on every mesh node.
+// one spectrum per workgroup (M doubles of LDS) and at most 128 registers
+// let two workgroups share a CU, so one's convolution overlaps another's
+// deposit.  A last kernel sums the workgroups' partials per k1 in a fixed
+// order (gain tiles, then loss tiles) and adds the result to the caller's
+// output on every mesh node.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -40,6 +48,8 @@ constexpr int kEeAccMax = 32;   // k1 values per thread: P <= kEeT * kEeAccMax
 constexpr int kEeQTile = 16;    // momentum transfers per workgroup
 constexpr int kEeConvR = 8;     // grid outputs per thread in the convolution
 constexpr int kEeMaxJ = 256;    // largest Gaussian half-width in grid cells
+constexpr int kEeConvCh = 2;    // convolution chunks per thread: M <= kEeT * kEeConvR * kEeConvCh
+constexpr int kEeMaxM = kEeT * kEeConvR * kEeConvCh;
 
 struct EeData {
     int dim = 2;
@@ -56,7 +66,7 @@ struct EeData {
     double* amp = nullptr;       // A(q) (P)
     double* fp = nullptr;        // f on the lattice, cap x P
     int32_t* fexp = nullptr;     // fixed-point exponent per member
-    double* part = nullptr;      // cap x ntile x P partial sums
+    double* part = nullptr;      // cap x 2 x ntile x P partial sums (gain, loss)
 };
 
 struct EeLattice {
@@ -123,13 +133,12 @@ __device__ inline int ee_shift(const EeLattice& L, int a, int q, int sign) {
     return c0 + L.ns0 * (c1 + L.ns1 * c2);
 }
 
-// Adds the four fixed-point deposits of every lane to its bins m, m + 1 of
-// the two spectra.  Lanes of one wave that hit the same bin (consecutive
-// lanes with equal m: common when the energy difference does not depend on
-// the fastest lattice axis) are summed first by a segmented scan, so each
-// run costs one set of atomics.  Integer sums: the result is exact.
-__device__ inline void ee_deposit(int m, long long a0, long long a1, long long b0, long long b1,
-                                  long long* __restrict__ Ha, long long* __restrict__ Hb) {
+// Adds the two fixed-point deposits of every lane to its bins m, m + 1 of
+// the spectrum.  Lanes of one wave that hit the same bin (consecutive lanes
+// with equal m: common when the energy difference does not depend on the
+// fastest lattice axis) are summed first by a segmented scan, so each run
+// costs one pair of atomics.  Integer sums: the result is exact.
+__device__ inline void ee_deposit(int m, long long a0, long long a1, long long* __restrict__ H) {
     const int lane = threadIdx.x & 63;
     const int mprev = __shfl_up(m, 1, 64);
     int start = (lane == 0 || mprev != m) ? lane : 0;
@@ -141,53 +150,60 @@ __device__ inline void ee_deposit(int m, long long a0, long long a1, long long b
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const long long x0 = __shfl_up(a0, o, 64), x1 = __shfl_up(a1, o, 64);
-        const long long y0 = __shfl_up(b0, o, 64), y1 = __shfl_up(b1, o, 64);
         if (lane - o >= start) {
             a0 += x0;
             a1 += x1;
-            b0 += y0;
-            b1 += y1;
         }
     }
     const int mnext = __shfl_down(m, 1, 64);
     if ((lane == 63 || mnext != m) && m >= 0) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&Ha[m]), (unsigned long long)a0);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&Ha[m + 1]), (unsigned long long)a1);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&Hb[m]), (unsigned long long)b0);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&Hb[m + 1]), (unsigned long long)b1);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&H[m]), (unsigned long long)a0);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&H[m + 1]), (unsigned long long)a1);
     }
 }
 
-// S = G * H on the grid: kEeConvR consecutive outputs per thread, the taps
-// walked from +J to -J with a sliding window of H in registers (one LDS read
-// of H and one broadcast read of G per tap)
-__device__ inline void ee_convolve(const double* __restrict__ H, const double* __restrict__ g,
-                                   int M, int J, double* __restrict__ S) {
-    for (int m0 = threadIdx.x * kEeConvR; m0 < M; m0 += kEeT * kEeConvR) {
-        double out[kEeConvR], h[kEeConvR];
+// S = G * H on the bins [lo, hi], in place: every thread first computes its
+// outputs (kEeConvR consecutive bins per chunk, the taps walked from +J to
+// -J with a sliding window of H in registers; one LDS read of H and one
+// broadcast read of G per tap), then, after a barrier, overwrites H with them.
+// H is zero outside [lo, hi].
+__device__ inline void ee_convolve_inplace(double* __restrict__ H, const double* __restrict__ g, int lo,
+                                           int hi, int M, int J) {
+    double out[kEeConvCh][kEeConvR];
+#pragma unroll
+    for (int c = 0; c < kEeConvCh; ++c) {
+        const int m0 = lo + (threadIdx.x + c * kEeT) * kEeConvR;
+        double h[kEeConvR];
 #pragma unroll
         for (int r = 0; r < kEeConvR; ++r) {
-            out[r] = 0.0;
+            out[c][r] = 0.0;
             const int k = m0 + r - J;
-            h[r] = (k >= 0 && k < M) ? H[k] : 0.0;
+            h[r] = (m0 <= hi && k >= 0 && k < M) ? H[k] : 0.0;
         }
+        if (m0 > hi) continue;
         for (int j = J; j >= -J; --j) {
             const double gj = g[j + J];
 #pragma unroll
-            for (int r = 0; r < kEeConvR; ++r) out[r] += gj * h[r];
+            for (int r = 0; r < kEeConvR; ++r) out[c][r] += gj * h[r];
 #pragma unroll
             for (int r = 0; r + 1 < kEeConvR; ++r) h[r] = h[r + 1];
             const int k = m0 + kEeConvR - j;
             h[kEeConvR - 1] = (k >= 0 && k < M) ? H[k] : 0.0;
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kEeConvCh; ++c) {
+        const int m0 = lo + (threadIdx.x + c * kEeT) * kEeConvR;
 #pragma unroll
         for (int r = 0; r < kEeConvR; ++r)
-            if (m0 + r < M) S[m0 + r] = out[r];
+            if (m0 + r <= hi) H[m0 + r] = out[c][r];
     }
 }
 
+// grid (q tile, member, spectrum): spectrum 0 is the gain term, 1 the loss
 template <int ACC>
-__global__ void __launch_bounds__(kEeT)
+__global__ void __launch_bounds__(kEeT, 4)
 ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                const double* __restrict__ gtab, const double* __restrict__ ep,
                const int32_t* __restrict__ coords,
@@ -195,37 +211,43 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                const int32_t* __restrict__ fexp, int ntile, double* __restrict__ part) {
     extern __shared__ double lds[];
     __shared__ double g[2 * kEeMaxJ + 1];
-    long long* Ha = reinterpret_cast<long long*>(lds);       // gain spectrum, fixed point
-    long long* Hb = Ha + M;                                   // loss spectrum
-    double* S = lds + 2 * (size_t)M;                          // convolved spectrum
+    __shared__ int s_lo, s_hi;
+    long long* H = reinterpret_cast<long long*>(lds);       // spectrum, fixed point
+    double* Hd = lds;                                        // the same bins as FP64
     const int tid = threadIdx.x;
     const int b = blockIdx.y;
     const int tile = blockIdx.x;
+    const int gain = blockIdx.z == 0;
     const int q0 = tile * kEeQTile;
     const int q1 = min(q0 + kEeQTile, P);
     const double* fb = fp + (int64_t)b * P;
     const double scale = ldexp(1.0, fexp[b]);
     const double inv = ldexp(1.0, -fexp[b]);
     for (int j = tid; j <= 2 * J; j += kEeT) g[j] = gtab[j];
+    for (int m = tid; m < M; m += kEeT) H[m] = 0;
+    if (tid == 0) {
+        s_lo = M;
+        s_hi = -1;
+    }
 
-    // this thread's points p = tid + i*kEeT serve both as k2 (spectra) and as
+    // this thread's points p = tid + i*kEeT serve both as k2 (spectrum) and as
     // k1 (evaluation); only the sums of the k1 values live in registers
     double acc[ACC];
 #pragma unroll
     for (int i = 0; i < ACC; ++i) acc[i] = 0.0;
+    __syncthreads();
 
     for (int q = q0; q < q1; ++q) {
         const double aq = w * amp[q];
         const int qc = coords[q];
-        for (int m = tid; m < 2 * M; m += kEeT) Ha[m] = 0;
-        __syncthreads();
-        // both pair spectra of transfer q: k4 = k2 + q (every lane takes part
-        // in every iteration: the deposit scans across the wave)
+        // spectrum of transfer q: k4 = k2 + q (every lane takes part in every
+        // iteration: the deposit scans across the wave)
+        int mlo = M, mhi = -1;
 #pragma unroll 1
         for (int i = 0; i < ACC; ++i) {
             const int p2 = tid + i * kEeT;
             int m = -1;
-            long long a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+            long long a0 = 0, a1 = 0;
             if (p2 < P) {
                 const int p4 = ee_shift(L, coords[p2], qc, +1);
                 const double f2 = fb[p2], f4 = fb[p4];
@@ -233,45 +255,56 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                 const double fl = floor(y);
                 const double t = y - fl;
                 m = (int)fl + Mh;
-                const double Fa = w * f4 * (1.0 - f2);
-                const double Fb = w * f2 * (1.0 - f4);
-                a0 = llrint((1.0 - t) * Fa * scale);
-                a1 = llrint(t * Fa * scale);
-                b0 = llrint((1.0 - t) * Fb * scale);
-                b1 = llrint(t * Fb * scale);
+                const double F = gain ? w * f4 * (1.0 - f2) : w * f2 * (1.0 - f4);
+                a0 = llrint((1.0 - t) * F * scale);
+                a1 = llrint(t * F * scale);
+                mlo = min(mlo, m);
+                mhi = max(mhi, m);
             }
-            ee_deposit(m, a0, a1, b0, b1, Ha, Hb);
+            ee_deposit(m, a0, a1, H);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            mlo = min(mlo, __shfl_xor(mlo, o, 64));
+            mhi = max(mhi, __shfl_xor(mhi, o, 64));
+        }
+        if ((tid & 63) == 0) {
+            atomicMin(&s_lo, mlo);
+            atomicMax(&s_hi, mhi);
         }
         __syncthreads();
-        for (int pass = 0; pass < 2; ++pass) {
-            long long* Hi = pass == 0 ? Ha : Hb;
-            double* Hd = reinterpret_cast<double*>(Hi);
-            for (int m = tid; m < M; m += kEeT) Hd[m] = (double)Hi[m] * inv;
-            __syncthreads();
-            ee_convolve(Hd, g, M, J, S);
-            __syncthreads();
-            // pairs (1, 3) with k1 - k3 = q
-#pragma unroll
-            for (int i = 0; i < ACC; ++i) {
-                const int p1 = tid + i * kEeT;
-                if (p1 < P) {
-                    const int p3 = ee_shift(L, coords[p1], qc, -1);
-                    const double x = (ep[p1] - ep[p3]) / dw;
-                    const double fl = floor(x);
-                    const double t = x - fl;
-                    const int m = (int)fl + Mh;
-                    const double sv = (1.0 - t) * S[m] + t * S[m + 1];
-                    const double f1 = fb[p1], f3 = fb[p3];
-                    acc[i] += pass == 0 ? aq * ((1.0 - f1) * f3 * sv) : -aq * (f1 * (1.0 - f3) * sv);
-                }
+        // nonzero bins: [lo, hi] (deposits go to m and m + 1)
+        const int lo = s_lo, hi = s_hi + 1;
+        for (int m = lo + tid; m <= hi; m += kEeT) Hd[m] = (double)H[m] * inv;
+        __syncthreads();
+        ee_convolve_inplace(Hd, g, lo, hi, M, J);
+        __syncthreads();
+        // pairs (1, 3) with k1 - k3 = q: the bins read are within [lo, hi]
+#pragma unroll 1
+        for (int i = 0; i < ACC; ++i) {
+            const int p1 = tid + i * kEeT;
+            if (p1 < P) {
+                const int p3 = ee_shift(L, coords[p1], qc, -1);
+                const double x = (ep[p1] - ep[p3]) / dw;
+                const double fl = floor(x);
+                const double t = x - fl;
+                const int m = (int)fl + Mh;
+                const double sv = (1.0 - t) * Hd[m] + t * Hd[m + 1];
+                const double f1 = fb[p1], f3 = fb[p3];
+                acc[i] += gain ? aq * ((1.0 - f1) * f3 * sv) : -aq * (f1 * (1.0 - f3) * sv);
             }
-            __syncthreads();
         }
+        __syncthreads();
+        for (int m = lo + tid; m <= hi; m += kEeT) H[m] = 0;
+        if (tid == 0) {
+            s_lo = M;
+            s_hi = -1;
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < ACC; ++i) {
         const int p1 = tid + i * kEeT;
-        if (p1 < P) part[((int64_t)b * ntile + tile) * P + p1] = acc[i];
+        if (p1 < P) part[(((int64_t)b * 2 + blockIdx.z) * ntile + tile) * P + p1] = acc[i];
     }
 }
 
@@ -284,7 +317,7 @@ __global__ void ee_reduce_kernel(const double* __restrict__ part, int ntile, int
     if (i >= n) return;
     const int64_t p = p_of_node[i];
     double s = 0.0;
-    for (int t = 0; t < ntile; ++t) s += part[((int64_t)b * ntile + t) * P + p];
+    for (int t = 0; t < 2 * ntile; ++t) s += part[((int64_t)b * 2 * ntile + t) * P + p];
     out[(int64_t)b * n + i] += s;
 }
 
@@ -321,9 +354,9 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
                          kEeMaxJ);
     hipDeviceProp_t prop;
     TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
-    const size_t lds = sizeof(double) * 3 * (size_t)e->M;
-    if (lds + sizeof(double) * (2 * kEeMaxJ + 1) > (size_t)prop.sharedMemPerBlock)
-        return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds the LDS", e->M);
+    const size_t lds = sizeof(double) * (size_t)e->M;
+    if (e->M > kEeMaxM || lds + sizeof(double) * (2 * kEeMaxJ + 1) + 64 > (size_t)prop.sharedMemPerBlock)
+        return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds %d", e->M, kEeMaxM);
     for (const void* k : {(const void*)ee_main_kernel<8>, (const void*)ee_main_kernel<16>,
                           (const void*)ee_main_kernel<32>})
         TG_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -391,7 +424,7 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
     TG_TRY(dev_alloc(c, &e->coords, (size_t)P));
     TG_TRY(dev_alloc(c, &e->fp, (size_t)P * e->cap));
     TG_TRY(dev_alloc(c, &e->fexp, (size_t)e->cap));
-    TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap));
+    TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap * 2));
     TG_HIP(c, hipMemcpy(e->gtab, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->ep, ep.data(), sizeof(double) * P, hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->amp, amp.data(), sizeof(double) * P, hipMemcpyHostToDevice));
@@ -428,9 +461,9 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B) 
                        e->fp, e->fexp);
     TG_CHECK_LAUNCH(c);
     const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
-    const size_t lds = sizeof(double) * 3 * (size_t)e->M;
+    const size_t lds = sizeof(double) * (size_t)e->M;
 #define TG_EE_MAIN(ACC)                                                                              \
-    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
+    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B, 2), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
                        e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
                        e->ntile,                                                                     \
                        e->part)

@@ -134,13 +134,23 @@ __device__ inline int ee_shift(const EeLattice& L, int a, int q, int sign) {
 }
 
 // Adds the two fixed-point deposits of every lane to its bins m, m + 1 of
-// the spectrum.  Lanes of one wave that hit the same bin (consecutive lanes
+// the spectrum.  When lanes of one wave hit the same bin (consecutive lanes
 // with equal m: common when the energy difference does not depend on the
-// fastest lattice axis) are summed first by a segmented scan, so each run
-// costs one pair of atomics.  Integer sums: the result is exact.
+// fastest lattice axis) they are summed first by a segmented scan, so each
+// run costs one pair of atomics.  Integer sums: the result is exact and does
+// not depend on the path taken.
 __device__ inline void ee_deposit(int m, long long a0, long long a1, long long* __restrict__ H) {
     const int lane = threadIdx.x & 63;
     const int mprev = __shfl_up(m, 1, 64);
+    // no two neighbouring lanes share a bin (the usual case when the transfer
+    // has a component along the fastest axis): one pair of atomics per lane
+    if (__ballot(lane > 0 && mprev == m && m >= 0) == 0) {
+        if (m >= 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&H[m]), (unsigned long long)a0);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&H[m + 1]), (unsigned long long)a1);
+        }
+        return;
+    }
     int start = (lane == 0 || mprev != m) ? lane : 0;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {

@@ -104,19 +104,22 @@ void dp5_destroy(tg_ctx* c) {
     c->dp5 = nullptr;
 }
 
-// operator evaluation for the whole batch (bath set beforehand for the
-// batched path); kernel launches only
+// operator evaluation for the whole batch
+// (bath set beforehand for the batched path); kernel launches only.  On the
+// batched path members whose step is 0 this attempt (finished: gate = the
+// per-member step sizes) are skipped: the controller reads nothing of theirs.
 static tg_status eval_batch(tg_ctx* c, const double* d_in, double* d_out, int32_t B,
-                            const std::vector<double>& bath, int32_t flags) {
+                            const std::vector<double>& bath, int32_t flags, const double* gate) {
     const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
     if (flags & TG_PREC_MIXED) {
-        TG_TRY(mixed_eval_device(c, d_in, d_out, B, conserve));
+        TG_TRY(mixed_eval_device(c, d_in, d_out, B, conserve, gate));
     } else {
         for (int b = 0; b < B; ++b)
             TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, bath[b],
                                       conserve));
+        gate = nullptr;
     }
-    return ee_apply_device(c, d_in, d_out, B);
+    return ee_apply_device(c, d_in, d_out, B, gate);
 }
 
 // the device work of one attempted step (captured into a graph)
@@ -143,18 +146,18 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl,
     TG_CHECK_LAUNCH(c);
     // stages k2..k6 (DormandPrince54.hpp:565-574)
     TG_TRY(stage({A21}, 1, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[1], B, bath, flags));
+    TG_TRY(eval_batch(c, w->ys, k[1], B, bath, flags, w->h));
     TG_TRY(stage({A31, A32}, 2, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[2], B, bath, flags));
+    TG_TRY(eval_batch(c, w->ys, k[2], B, bath, flags, w->h));
     TG_TRY(stage({A41, A42, A43}, 3, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[3], B, bath, flags));
+    TG_TRY(eval_batch(c, w->ys, k[3], B, bath, flags, w->h));
     TG_TRY(stage({A51, A52, A53, A54}, 4, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[4], B, bath, flags));
+    TG_TRY(eval_batch(c, w->ys, k[4], B, bath, flags, w->h));
     TG_TRY(stage({A61, A62, A63, A64, A65}, 5, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[5], B, bath, flags));
+    TG_TRY(eval_batch(c, w->ys, k[5], B, bath, flags, w->h));
     // y5 (b2 = 0) and k7 (:576-578)
     TG_TRY(stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5));
-    TG_TRY(eval_batch(c, w->y5, k[6], B, bath, flags));
+    TG_TRY(eval_batch(c, w->y5, k[6], B, bath, flags, w->h));
     // err = y5 - y4 (:580)
     {
         StageArgs a{};
@@ -271,7 +274,7 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
         m.st.evals = 1;
     }
     TG_HIP(c, hipMemcpyAsync(w->md, ms.data(), sizeof(MemberDev) * B, hipMemcpyHostToDevice, st));
-    TG_TRY(eval_batch(c, w->y, w->k[0], B, bath, flags));  // k1 = timeOp(y0)
+    TG_TRY(eval_batch(c, w->y, w->k[0], B, bath, flags, nullptr));  // k1 = timeOp(y0)
     TG_HIP(c, hipStreamSynchronize(st));
 
     // the attempt as a graph; replayed one attempt ahead of the flag read

@@ -42,6 +42,7 @@ __global__ void stage_kernel(int64_t n, int B, const double* __restrict__ y, Sta
                              const double* __restrict__ h, double* __restrict__ out) {
     const int b = blockIdx.y;
     const double hb = h[b];
+    if (hb == 0.0) return;   // finished member: nothing of this attempt is read
     double c[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j) c[j] = s.a[j] * hb;
@@ -60,6 +61,7 @@ __global__ void error_kernel(int64_t n, int B, const double* __restrict__ y, Sta
                              double* __restrict__ err) {
     const int b = blockIdx.y;
     const double hb = h[b];
+    if (hb == 0.0) return;   // finished member: the controller skips it
     double c[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j) c[j] = hb * s.a[j];

@@ -98,9 +98,10 @@ __device__ inline int64_t lat_add(const EeLattice& L, int64_t a, int64_t b, int 
 __global__ void __launch_bounds__(kEeT)
 ee_prep_kernel(const double* __restrict__ f, int64_t n, int64_t P,
                const int32_t* __restrict__ node_of_p, double w, double* __restrict__ fp,
-               int32_t* __restrict__ fexp) {
+               int32_t* __restrict__ fexp, const double* __restrict__ gate) {
     __shared__ double red[kEeT / 64];
     const int b = blockIdx.x;
+    if (gate && gate[b] == 0.0) return;
     double m = 0.0;
     for (int64_t p = threadIdx.x; p < P; p += kEeT) {
         const double x = f[(int64_t)b * n + node_of_p[p]];
@@ -218,10 +219,12 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                const double* __restrict__ gtab, const double* __restrict__ ep,
                const int32_t* __restrict__ coords,
                const double* __restrict__ amp, const double* __restrict__ fp,
-               const int32_t* __restrict__ fexp, int ntile, double* __restrict__ part) {
+               const int32_t* __restrict__ fexp, int ntile, const double* __restrict__ gate,
+               double* __restrict__ part) {
     extern __shared__ double lds[];
     __shared__ double g[2 * kEeMaxJ + 1];
     __shared__ int s_lo, s_hi;
+    if (gate && gate[blockIdx.y] == 0.0) return;   // uniform over the workgroup
     long long* H = reinterpret_cast<long long*>(lds);       // spectrum, fixed point
     double* Hd = lds;                                        // the same bins as FP64
     const int tid = threadIdx.x;
@@ -321,10 +324,10 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
 // out[b][node] += sum over workgroups (fixed order) of the node's periodic point
 __global__ void ee_reduce_kernel(const double* __restrict__ part, int ntile, int64_t P,
                                  const int32_t* __restrict__ p_of_node, int64_t n,
-                                 double* __restrict__ out) {
+                                 const double* __restrict__ gate, double* __restrict__ out) {
     const int b = blockIdx.y;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (gate && gate[b] == 0.0)) return;
     const int64_t p = p_of_node[i];
     double s = 0.0;
     for (int t = 0; t < 2 * ntile; ++t) s += part[((int64_t)b * 2 * ntile + t) * P + p];
@@ -461,21 +464,21 @@ void ee_destroy(tg_ctx* c) {
 }
 
 // d_I[b] += I_ee(d_f[b]) for the B members (caller order, member-major);
-// kernel launches only
-tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B) {
+// kernel launches only; members with gate[b] == 0 are skipped (gate nullable)
+tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, const double* gate) {
     EeData* e = ee_of(c);
     if (!e) return TG_OK;
     if (B < 1 || B > e->cap) return set_error(c, TG_ERR_ARG, "e-e channel: batch %d outside [1, %d]", B, e->cap);
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(ee_prep_kernel, dim3(B), dim3(kEeT), 0, st, d_f, c->n, e->P, e->node_of_p, e->w,
-                       e->fp, e->fexp);
+                       e->fp, e->fexp, gate);
     TG_CHECK_LAUNCH(c);
     const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
     const size_t lds = sizeof(double) * (size_t)e->M;
 #define TG_EE_MAIN(ACC)                                                                              \
     hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B, 2), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
                        e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
-                       e->ntile,                                                                     \
+                       e->ntile, gate,                                                               \
                        e->part)
     if (e->P <= 8 * kEeT) TG_EE_MAIN(8);
     else if (e->P <= 16 * kEeT) TG_EE_MAIN(16);
@@ -483,7 +486,7 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B) 
 #undef TG_EE_MAIN
     TG_CHECK_LAUNCH(c);
     hipLaunchKernelGGL(ee_reduce_kernel, dim3(ceil_div(c->n, 256), B), dim3(256), 0, st, e->part,
-                       e->ntile, e->P, e->p_of_node, c->n, d_I);
+                       e->ntile, e->P, e->p_of_node, c->n, gate, d_I);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }

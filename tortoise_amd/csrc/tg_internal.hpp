@@ -181,7 +181,10 @@ tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);
 tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t batch);
-tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch, int conserve);
+// gate (nullable, device, one per member): members with gate[b] == 0 are
+// skipped (their outputs are not written)
+tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch, int conserve,
+                            const double* gate);
 // error-norm sums sum_e (sum r^2 + (sum r)^2) per member into out[batch]
 // (part: batch x relerr_blocks partials); kernel launches only
 int relerr_blocks(tg_ctx* c);
@@ -193,7 +196,8 @@ void dp5_destroy(tg_ctx* c);
 tg_status ee_prepare(tg_ctx* c, const double* h_eps);
 void ee_destroy(tg_ctx* c);
 // d_I[b] += I_ee(d_f[b]); no-op without the e-e channel; kernel launches only
-tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch);
+tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
+                          const double* gate = nullptr);
 void adjoint_destroy(tg_ctx* c);
 tg_status ensure_staging(tg_ctx* c, int32_t batch);
 // event bracket of the dominant kernel when TG_TIME_KERNEL is active

@@ -234,7 +234,8 @@ __device__ inline void stage_member_row(const double* __restrict__ fb, int64_t n
 // f (caller order) -> packed fp16 hi/lo of both GEMM operands z = f and
 // z = g = 1 - f in energy order (the GEMM streams one of them per (panel,
 // member)), and the operand choice of every panel.  Members >= B and nodes
-// >= n pack as zero.
+// >= n pack as zero; with a gate (the batched DP5's per-member step sizes)
+// members whose step is 0 are skipped.
 //
 // Operand of (panel, member): g = 1 - f when the panel's middle node is in
 // the filled part of the member's distribution (f > 1/2), else f.  The
@@ -246,10 +247,13 @@ __global__ void __launch_bounds__(kMemT)
 pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
                    int64_t kgroups, int64_t kt, int npanel, int nt, u32x4* __restrict__ Fhi,
                    u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo,
-                   uint8_t* __restrict__ useg) {
+                   uint8_t* __restrict__ useg, const double* __restrict__ gate) {
     extern __shared__ double row[];
     const int b = member_of_block(blockIdx.x, nt);
     if (b >= nt) return;
+    // a member whose step is 0 this attempt (finished) keeps its old columns:
+    // its products are not read
+    if (gate && b < B && gate[b] == 0.0) return;
     const bool live = b < B;
     const double* fb = f + (int64_t)(live ? b : 0) * n;
     if constexpr (ROW) {
@@ -728,7 +732,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
 // permutation, so each LDS slot is read and written by one thread only); the
 // row then goes out by coalesced stores together with the correction.
 // Without it the gather reads and the scatter writes global memory and the
-// correction is a second pass.
+// correction is a second pass.  Gated members (step 0) are skipped.
 template <bool ROW, typename PT>
 __global__ void __launch_bounds__(kMemT)
 finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
@@ -736,12 +740,13 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                      const int32_t* __restrict__ perm, const double* __restrict__ rk,
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
-                     double inv_scale, int conserve, double* __restrict__ I) {
+                     double inv_scale, int conserve, const double* __restrict__ gate,
+                     double* __restrict__ I) {
     extern __shared__ double row[];
     __shared__ double red[2][kMemT / 64];
     __shared__ double s_b;
     const int b = member_of_block(blockIdx.x, nt);
-    if (b >= B) return;
+    if (b >= B || (gate && gate[b] == 0.0)) return;
     const double nb = bath[b];
     const double np1 = nb + 1.0;
     const double* fb = f + (int64_t)b * n;
@@ -950,13 +955,13 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
-static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B) {
+static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, const double* gate) {
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
 #define TG_PACK(ROW)                                                                                \
     hipLaunchKernelGGL(pack_member_kernel<ROW>, dim3(m->nt), dim3(kMemT), ROW ? sizeof(double) * n : 0, \
                        c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt,             \
-                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg)
+                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg, gate)
     if (m->rowcache) TG_PACK(true);
     else TG_PACK(false);
 #undef TG_PACK
@@ -965,7 +970,7 @@ static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B) 
 }
 
 static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, double* d_I, int B,
-                               int conserve) {
+                               int conserve, const double* gate = nullptr) {
     const int64_t n = c->n;
     const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
@@ -973,7 +978,7 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
                        c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
                        m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
-                       d_I)
+                       gate, d_I)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
         else TG_FINISH(false, double);
@@ -1373,23 +1378,24 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 // one batched evaluation with the bath set by mixed_set_bath: kernel launches
 // only, so a caller may capture it in a graph once it has run once (buffers
 // are allocated on first use)
-tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, int conserve) {
+tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, int conserve,
+                            const double* gate) {
     MixedData* m = md_of(c);
     if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
     TG_TRY(ensure_P(c, m));
-    TG_TRY(launch_pack(c, m, d_f, B));
+    TG_TRY(launch_pack(c, m, d_f, B, gate));
     time_begin(c);
     TG_TRY(launch_gemm(c, m));
     time_end(c);
-    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve));
+    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate));
     return TG_OK;
 }
 
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B,
                              const double* h_bath, int conserve) {
     TG_TRY(mixed_set_bath(c, h_bath, B));
-    return mixed_eval_device(c, d_f, d_I, B, conserve);
+    return mixed_eval_device(c, d_f, d_I, B, conserve, nullptr);
 }
 
 }  // namespace tg

@@ -413,7 +413,7 @@ def main():
     _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
          f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}; "
          f"kernel {gemm_kernel_name(st)} splits {st['mixed_splits']}")
-    _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), dual kernel "
+    _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), stacked-band GEMV "
          f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s algorithmic, {gbs_streamed:.0f} GB/s streamed")
 
     out = {
@@ -445,7 +445,7 @@ def main():
                               "fold": st["mixed_fold"], "depth": st["mixed_depth"],
                               "splits": st["mixed_splits"], "autotuned_ms": st["mixed_tuned_ms"]}},
         "plain_fp64": {"evals_per_s": evals_plain, "ms_per_eval": ms_plain,
-                       "kernel": "plain_dual_kernel", "kernel_ms": ms_dual,
+                       "kernel": "stack_gemv_kernel", "kernel_ms": ms_dual,
                        "achieved_gbs": gbs_plain, "frac_hbm": gbs_plain / HBM_PEAK_GBS,
                        "streamed_gbs": gbs_streamed, "bytes_per_eval": 8.0 * st["nnz"] + 32.0 * n},
         "mixed_vs_plain": evals_mixed / evals_plain,

@@ -113,7 +113,7 @@ tg_status   tg_kernel_assemble(tg_ctx* ctx, int32_t flags);
 typedef struct tg_kernel_stats {
     int64_t n;              /* nodes */
     int64_t nnz;            /* structurally nonzero entries of K */
-    int64_t plain_band;     /* entries of K the FP64 one-pass kernel streams per evaluation */
+    int64_t plain_band;     /* entries of the stacked band [K; K^T] the plain FP64 evaluation streams */
     int64_t mixed_band;     /* entries of [K; K^T] the batched kernel streams per evaluation */
     int32_t mixed_splits;   /* reduction splits of the batched kernel */
     int32_t mixed_width;    /* members per batched tile (64, 128 or 256) */

@@ -8,8 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtortoise_gpu.so")
-SOURCES = ["tg_context.hip", "tg_assemble.hip", "tg_plain.hip", "tg_mixed.hip", "tg_dp5.hip",
-           "tg_adjoint.hip", "tg_ee.hip", "tg_tuple.hip"]
+SOURCES = ["tg_context.hip", "tg_assemble.hip", "tg_plain.hip", "tg_stack.hip", "tg_mixed.hip",
+           "tg_dp5.hip", "tg_adjoint.hip", "tg_ee.hip", "tg_tuple.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops off: hipcc must not form v_pk_{add,mul,fma}_f32 from float code.

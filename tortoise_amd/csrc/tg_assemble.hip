@@ -427,6 +427,8 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
         TG_HIP(c, hipStreamSynchronize(c->stream));
         // the projection/convolution scratch is only needed for assembly
         dev_free(&c->d_Q);
+        // 8. the stacked FP64 band [K; K^T] of the plain evaluation
+        TG_TRY(stack_prepare(c));
         c->assembled = true;
     }
     if ((flags & TG_PREC_MIXED) || c->max_batch > 1) TG_TRY(mixed_prepare(c));

@@ -322,6 +322,7 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     mixed_destroy(c);
+    stack_destroy(c);
     dp5_destroy(c);
     adjoint_destroy(c);
     ee_destroy(c);

@@ -99,6 +99,9 @@ struct tg_ctx {
     int32_t* d_cc_rows = nullptr;  // row blocks whose band covers the chunk, increasing
     int64_t band_elems = 0;        // K entries the one-pass kernel streams per evaluation
 
+    // stacked FP64 band of [K; K^T] for the plain evaluation (tg_stack.hip)
+    void* stack = nullptr;
+
     // mixed batched data (tg_mixed.hip)
     void* mixed = nullptr;
 
@@ -177,6 +180,16 @@ tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, doubl
                               double* energy);
 tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,
                                 int32_t batch, double rtol, double atol, double* h_out);
+// stacked FP64 band of [K; K^T] (tg_stack.hip): built at assembly, one plain
+// evaluation per call
+tg_status stack_prepare(tg_ctx* c);
+void stack_destroy(tg_ctx* c);
+tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve);
+// the plain evaluation through the one-pass dual kernel (tg_plain.hip)
+tg_status plain_dual_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
+                                  int conserve);
+// I -= d |f(1-f)| with d from the block sums dpart (number-defect removal)
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I);
 tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);

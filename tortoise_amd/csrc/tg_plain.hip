@@ -208,6 +208,13 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
         I[i] -= s * fabs(f[i] * (1.0 - f[i]));
 }
 
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I) {
+    hipLaunchKernelGGL(conserve_fix_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, dpart, nblk,
+                       c->n, d_f, d_I);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
 static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
     hipLaunchKernelGGL(gather_sorted_f_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
                        d_f, c->d_perm, c->n, c->d_fs);
@@ -220,8 +227,16 @@ static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
     return TG_OK;
 }
 
+// one plain FP64 evaluation: the stacked-band GEMV (tg_stack.hip)
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
                              int conserve) {
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
+    return stack_apply_device(c, d_f, d_I, bath_n, conserve);
+}
+
+// the same evaluation through the one-pass dual kernel (K's band read once)
+tg_status plain_dual_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
+                                  int conserve) {
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
     TG_TRY(launch_dual(c, c->d_K, d_f));
     const unsigned nb = ceil_div(c->n, kFinishNodes);
@@ -239,11 +254,7 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
                        c->d_cc_rows, c->n, c->ld, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s,
                        c->d_V_s, bath_n, d_I, nullptr, dpart);
     TG_CHECK_LAUNCH(c);
-    if (conserve) {
-        hipLaunchKernelGGL(conserve_fix_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                           dpart, (int)nb, c->n, d_f, d_I);
-        TG_CHECK_LAUNCH(c);
-    }
+    if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I));
     return TG_OK;
 }
 

@@ -185,9 +185,6 @@ tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_
 tg_status stack_prepare(tg_ctx* c);
 void stack_destroy(tg_ctx* c);
 tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve);
-// the plain evaluation through the one-pass dual kernel (tg_plain.hip)
-tg_status plain_dual_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
-                                  int conserve);
 // I -= d |f(1-f)| with d from the block sums dpart (number-defect removal)
 tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I);
 tg_status mixed_prepare(tg_ctx* c);

@@ -23,8 +23,10 @@
 //   A band: per panel its band only, iteration-major, each iteration one
 //     16 KB block [kg in 0..3][hi, lo][row in 0..127] (kg = k/8 within the
 //     iteration); the panel's blocks are contiguous from band_off[panel]
-//   Z_{hi,lo}[kg][col]   col = member in [0, nt): a tile of members of one
-//     k-group is a contiguous run, which is also its layout in LDS
+//   Z_{hi,lo}[it][col][kgl]   it = kg/4, kgl = kg%4, col = member in [0, nt):
+//     a member's four k-groups of one iteration are 64 contiguous bytes (the
+//     pack kernel writes whole 64-byte pieces), and a tile of members of one
+//     iteration is one contiguous block
 // Every MFMA fragment and LDS tile is a contiguous 16-byte read per lane in
 // these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1), and a
 // panel streams its band as consecutive 16 KB blocks.  The dense layout
@@ -295,7 +297,9 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
             gh[q] = hi;
             gl[q] = lo;
         }
-        const int64_t u = kg * nt + b;
+        // four neighbouring threads (the k-groups of one iteration) write 64
+        // contiguous bytes of each operand
+        const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
         Fhi[u] = __builtin_bit_cast(u32x4, fh);
         Flo[u] = __builtin_bit_cast(u32x4, fl);
         Ghi[u] = __builtin_bit_cast(u32x4, gh);
@@ -340,14 +344,16 @@ __device__ __forceinline__ void stage_a(u32x4 (&ra)[N], const u32x4* __restrict_
 }
 
 // Unit q of a Z tile (NW members, the four k-groups of one iteration, hi and
-// lo): consecutive q are consecutive members of one k-group, contiguous in
-// memory and in LDS.
+// lo): consecutive q are consecutive 16-byte units in memory (member-major,
+// four k-groups per member); in LDS the tile is k-group-major with rows of
+// NW + 1 units, so the MFMA fragments are contiguous reads and the
+// transposing stores spread over the banks.
 template <int NW>
 __device__ __forceinline__ void z_unit(int q, int& which, int& m, int& kgl) {
     which = q / (kKG * NW);
     const int rem = q % (kKG * NW);
-    kgl = rem / NW;
-    m = rem % NW;
+    m = rem / kKG;
+    kgl = rem % kKG;
 }
 
 template <int NTH, int NW, int N>
@@ -358,7 +364,7 @@ __device__ __forceinline__ void stage_z(u32x4 (&rf)[N], const u32x4* const (&zhi
     for (int j = 0; j < N; ++j) {
         int which, m, kgl;
         z_unit<NW>(tid + j * NTH, which, m, kgl);
-        rf[j] = (which ? zlo[j] : zhi[j])[(it * kKG + kgl) * nt + col0 + m];
+        rf[j] = (which ? zlo[j] : zhi[j])[(it * nt + col0 + m) * kKG + kgl];
     }
 }
 
@@ -378,7 +384,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
     constexpr int F_PER_T = 2 * kKG * NW / NTH;
     __shared__ u32x4 sA[2][kKG][kMT];
-    __shared__ u32x4 sF[2][kKG][NW];
+    __shared__ u32x4 sF[2][kKG][NW + 1];
 
     const int64_t wg = blockIdx.x;
     const int64_t L = (wg % 8) * per_xcd + wg / 8;
@@ -564,7 +570,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     constexpr int U = std::lcm(std::lcm(2, DA), FOLD);
     static_assert(U % DA == 0 && U % 2 == 0 && U % FOLD == 0, "unroll must hold whole prefetch, buffer and fold cycles");
     constexpr int A_PER_T = 2 * kKG * kMT / NTH;
-    __shared__ u32x4 sZ[2][2][kKG][NW];
+    __shared__ u32x4 sZ[2][2][kKG][NW + 1];
     __shared__ u32x4 sA[AL ? 2 : 1][kKG][2][AL ? kMT : 1];
 
     const int w = blockIdx.x;

@@ -234,30 +234,6 @@ tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     return stack_apply_device(c, d_f, d_I, bath_n, conserve);
 }
 
-// the same evaluation through the one-pass dual kernel (K's band read once)
-tg_status plain_dual_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
-                                  int conserve) {
-    if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
-    TG_TRY(launch_dual(c, c->d_K, d_f));
-    const unsigned nb = ceil_div(c->n, kFinishNodes);
-    double* dpart = nullptr;
-    if (conserve) {
-        if ((size_t)c->red_cap < 2 * (size_t)nb) {
-            dev_free(&c->d_red);
-            TG_TRY(dev_alloc(c, &c->d_red, 2 * (size_t)nb));
-            c->red_cap = 2 * (int32_t)nb;
-        }
-        dpart = c->d_red;
-    }
-    hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
-                       c->stream, 0, c->d_upart, c->d_vpart, c->d_rb_chunk, c->d_cc_off,
-                       c->d_cc_rows, c->n, c->ld, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s,
-                       c->d_V_s, bath_n, d_I, nullptr, dpart);
-    TG_CHECK_LAUNCH(c);
-    if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I));
-    return TG_OK;
-}
-
 // u = M f, v = M^T f (caller's node order) for M = K or dK/domega0 stored in
 // energy order (row/column sums at assembly, products of the adjoint)
 tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,

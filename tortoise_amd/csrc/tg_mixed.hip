@@ -214,23 +214,51 @@ constexpr int kMemT = 1024;
 
 __device__ inline int member_of_block(int w, int nt) { return (w % 8) * (nt / 8) + w / 8; }
 
-// stage row b of f (caller order) in LDS, UNR independent loads per thread in
-// flight before any is stored.  Every load is unconditional (indices clamp to
-// the last node) so the loads issue back to back; only the stores are
-// guarded.
+// A member row moves between global memory and LDS in 16-byte pieces; a row
+// that starts 8 bytes past a 16-byte boundary moves its first element alone,
+// and an odd remainder its last.
+__device__ inline int64_t row_head(const double* p) { return ((uintptr_t)p & 15) ? 1 : 0; }
+
+// Pieces per thread of the staging: every row that fits the LDS
+// (n <= 20352) is staged by one round of 1024 x 10 16-byte loads.
+constexpr int kStageUnr = 10;
+// k-groups of 8 positions per thread in the staged pack
+constexpr int kPackKG = 3;
+
+// stage row b of f (caller order) in LDS, n <= 2 kMemT kStageUnr: all of a
+// thread's loads are in flight before any is stored (indices clamp to the
+// last piece so the loads issue unconditionally; only the stores are
+// guarded).  There is no loop, so loads the caller issued before stay in
+// flight; the caller issues the barrier (lds_barrier keeps them in flight
+// across it too).
 __device__ inline void stage_member_row(const double* __restrict__ fb, int64_t n, double* row) {
-    constexpr int UNR = 8;
-    for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * UNR) {
-        double v[UNR];
+    const int64_t h = row_head(fb);
+    const int64_t np = (n - h) >> 1;
+    const double2* __restrict__ src = reinterpret_cast<const double2*>(fb + h);
+    if (np > 0) {
+        double2 v[kStageUnr];
 #pragma unroll
-        for (int j = 0; j < UNR; ++j) v[j] = fb[min(i0 + (int64_t)j * kMemT, n - 1)];
+        for (int j = 0; j < kStageUnr; ++j) v[j] = src[min((int64_t)threadIdx.x + j * kMemT, np - 1)];
 #pragma unroll
-        for (int j = 0; j < UNR; ++j) {
-            const int64_t i = i0 + (int64_t)j * kMemT;
-            if (i < n) row[i] = v[j];
+        for (int j = 0; j < kStageUnr; ++j) {
+            const int64_t q = (int64_t)threadIdx.x + j * kMemT;
+            if (q < np) {
+                row[h + 2 * q] = v[j].x;
+                row[h + 2 * q + 1] = v[j].y;
+            }
         }
     }
-    __syncthreads();
+    if (threadIdx.x == 0 && h) row[0] = fb[0];
+    if (threadIdx.x == kMemT - 1 && h + 2 * np < n) row[n - 1] = fb[n - 1];
+}
+
+// Workgroup barrier for LDS data only: waits for this wave's LDS stores, not
+// for its global loads, so loads issued before it stay in flight (the
+// compiler inserts the waits for them where their values are used).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // f (caller order) -> packed fp16 hi/lo of both GEMM operands z = f and
@@ -258,28 +286,17 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
     if (gate && b < B && gate[b] == 0.0) return;
     const bool live = b < B;
     const double* fb = f + (int64_t)(live ? b : 0) * n;
-    if constexpr (ROW) {
-        if (live) stage_member_row(fb, n, row);
-    }
-    for (int pn = threadIdx.x; pn < npanel; pn += kMemT) {
-        uint8_t v = 0;
-        if (live) {
-            // K rows and K^T rows index the same sorted nodes
-            const int64_t r = (int64_t)pn * kMT + kMT / 2;
-            int64_t p = r < kt ? r : r - kt;
-            if (p > n - 1) p = n - 1;
-            const int32_t node = perm[p];
-            v = (ROW ? row[node] : fb[node]) > 0.5 ? 1 : 0;
-        }
-        useg[(int64_t)pn * nt + b] = v;
-    }
     const double scale = (double)(1 << kSF);
-    for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
-        // unconditional loads at clamped positions; padding and idle members
-        // select zero afterwards
-        int32_t node[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) node[q] = perm[min(kg * 8 + q, n - 1)];
+    // operand choice of panel pn from the node at its middle (K rows and K^T
+    // rows index the same sorted nodes)
+    auto panel_node = [&](int pn) {
+        const int64_t r = (int64_t)pn * kMT + kMT / 2;
+        int64_t p = r < kt ? r : r - kt;
+        return perm[p > n - 1 ? n - 1 : p];
+    };
+    // k-group kg: 8 energy-order positions of both operands, from the nodes
+    // at those positions (padding and idle members pack as zero)
+    auto pack_group = [&](int64_t kg, const int32_t* node) {
         double xv[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) xv[q] = ROW ? row[node[q]] : fb[node[q]];
@@ -304,6 +321,40 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
         Flo[u] = __builtin_bit_cast(u32x4, fl);
         Ghi[u] = __builtin_bit_cast(u32x4, gh);
         Glo[u] = __builtin_bit_cast(u32x4, gl);
+    };
+    if constexpr (ROW) {
+        // The node indices of this thread's k-groups and of its panel load
+        // before the row is staged, so their latency overlaps the staging
+        // loads (a row that fits the LDS has kgroups <= kPackKG * kMemT and
+        // npanel <= kMemT, checked when the rows are sized).
+        int32_t node[kPackKG][8];
+#pragma unroll
+        for (int j = 0; j < kPackKG; ++j) {
+            const int64_t kg = min((int64_t)threadIdx.x + (int64_t)j * kMemT, kgroups - 1);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) node[j][q] = perm[min(kg * 8 + q, n - 1)];
+        }
+        const int pn = threadIdx.x;
+        const int32_t pnode = panel_node(pn < npanel ? pn : 0);
+        if (live) {
+            stage_member_row(fb, n, row);
+            lds_barrier();
+        }
+        if (pn < npanel) useg[(int64_t)pn * nt + b] = live && row[pnode] > 0.5 ? 1 : 0;
+#pragma unroll
+        for (int j = 0; j < kPackKG; ++j) {
+            const int64_t kg = (int64_t)threadIdx.x + (int64_t)j * kMemT;
+            if (kg < kgroups) pack_group(kg, node[j]);
+        }
+    } else {
+        for (int pn = threadIdx.x; pn < npanel; pn += kMemT)
+            useg[(int64_t)pn * nt + b] = live && fb[panel_node(pn)] > 0.5 ? 1 : 0;
+        for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
+            int32_t node[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) node[q] = perm[min(kg * 8 + q, n - 1)];
+            pack_group(kg, node);
+        }
     }
 }
 
@@ -757,14 +808,16 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     const double np1 = nb + 1.0;
     const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
-    if constexpr (ROW) stage_member_row(fb, n, row);
     double dI = 0.0, dw = 0.0;
-    // UNR positions per thread with all their loads issued before use
-    constexpr int UNR = 4;
-    for (int64_t p0 = threadIdx.x; p0 < n; p0 += (int64_t)kMemT * UNR) {
-        double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
-        int32_t node[UNR];
-        bool gu[UNR], gv[UNR];
+    // UNR positions per thread per round with all their loads issued before
+    // use; the trip count is the same for every thread (the barrier after the
+    // first round's loads needs all of them)
+    constexpr int UNR = 6;
+    double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
+    int32_t node[UNR];
+    bool gu[UNR], gv[UNR];
+    // the partial sums (streamed from HBM) ...
+    auto load_products = [&](int64_t p0) {
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
@@ -776,6 +829,13 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                 u[j] += (double)Ps[p];
                 v[j] += (double)Ps[kt + p];
             }
+        }
+    };
+    // ... and the per-position constants every member shares (L2-resident)
+    auto load_constants = [&](int64_t p0) {
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
             gu[j] = useg[(p / kMT) * (int64_t)nt + b] != 0;
             gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
             rkp[j] = rk[p];
@@ -783,6 +843,15 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             Vp[j] = V[p];
             node[j] = perm[p];
         }
+    };
+    auto finish_round = [&](int64_t p0) {
+        // every load of the round is used here, so the scheduler keeps them
+        // issued back to back instead of sinking each to its first use (one
+        // memory round trip per round, not one per position)
+#pragma unroll
+        for (int j = 0; j < UNR; ++j)
+            asm volatile("" ::"v"(u[j]), "v"(v[j]), "v"(rkp[j]), "v"(ckp[j]), "v"(Vp[j]), "v"(node[j]),
+                         "v"(gu[j]), "v"(gv[j]));
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             if (p0 + (int64_t)j * kMemT >= n) break;
@@ -800,6 +869,18 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             else
                 Ib[node[j]] = Ii;
         }
+    };
+    const int64_t step = (int64_t)kMemT * UNR;
+    // the first round's partial sums and the row staging are in flight together
+    load_products(threadIdx.x);
+    if constexpr (ROW) stage_member_row(fb, n, row);
+    load_constants(threadIdx.x);
+    if constexpr (ROW) lds_barrier();
+    finish_round(threadIdx.x);
+    for (int64_t base = step; base < n; base += step) {
+        load_products(base + threadIdx.x);
+        load_constants(base + threadIdx.x);
+        finish_round(base + threadIdx.x);
     }
     double sc = 0.0;
     if (conserve) {
@@ -823,22 +904,32 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     }
     __syncthreads();
     if (conserve) sc = s_b;
-    constexpr int WUNR = 8;
+    // the row (LDS, or the scattered I) goes out with 16-byte stores, minus
+    // the number correction sc |f (1 - f)|
     if (ROW || conserve) {
-        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * WUNR) {
-            double x[WUNR], y[WUNR];
+        auto out = [&](int64_t i, double y) {
+            if (!conserve) return y;
+            const double x = fb[i];
+            return y - sc * fabs(x * (1.0 - x));
+        };
+        constexpr int WUNR = 5;
+        const int64_t h = row_head(Ib);
+        const int64_t np = (n - h) >> 1;
+        double2* __restrict__ dst = reinterpret_cast<double2*>(Ib + h);
+        for (int64_t j0 = threadIdx.x; j0 < np; j0 += (int64_t)kMemT * WUNR) {
+            double2 y[WUNR];
 #pragma unroll
             for (int j = 0; j < WUNR; ++j) {
-                const int64_t i = min(i0 + (int64_t)j * kMemT, n - 1);
-                x[j] = conserve ? fb[i] : 0.0;
-                y[j] = ROW ? row[i] : Ib[i];
+                const int64_t i = h + 2 * min(j0 + (int64_t)j * kMemT, np - 1);
+                y[j].x = out(i, ROW ? row[i] : Ib[i]);
+                y[j].y = out(i + 1, ROW ? row[i + 1] : Ib[i + 1]);
             }
 #pragma unroll
-            for (int j = 0; j < WUNR; ++j) {
-                const int64_t i = i0 + (int64_t)j * kMemT;
-                if (i < n) Ib[i] = y[j] - sc * fabs(x[j] * (1.0 - x[j]));
-            }
+            for (int j = 0; j < WUNR; ++j)
+                if (j0 + (int64_t)j * kMemT < np) dst[j0 + (int64_t)j * kMemT] = y[j];
         }
+        if (threadIdx.x == 0 && h) Ib[0] = out(0, ROW ? row[0] : Ib[0]);
+        if (threadIdx.x == kMemT - 1 && h + 2 * np < n) Ib[n - 1] = out(n - 1, ROW ? row[n - 1] : Ib[n - 1]);
     }
 }
 
@@ -1300,7 +1391,11 @@ tg_status mixed_prepare(tg_ctx* c) {
         hipDeviceProp_t prop;
         TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
         const size_t rowbytes = sizeof(double) * (size_t)n;
-        m->rowcache = rowbytes + 1024 <= (size_t)prop.sharedMemPerBlock;
+        // the staged pack holds every k-group index and the panel index of a
+        // thread in registers
+        m->rowcache = rowbytes + 1024 <= (size_t)prop.sharedMemPerBlock &&
+                      n <= 2 * (int64_t)kMemT * kStageUnr &&
+                      m->kpad / 8 <= (int64_t)kPackKG * kMemT && m->npanel <= kMemT;
         if (const char* e = std::getenv("TG_MIXED_ROWCACHE"))
             if (std::atoi(e) == 0) m->rowcache = false;
         if (m->rowcache) {

@@ -16,25 +16,31 @@
 //   Ha_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2+q)(1 - f(2))   (gain)
 //   Hb_q(m) = sum_2 w h_m(e(2+q) - e(2)) f(2)(1 - f(2+q))   (loss)
 // are convolved with the Gaussian on the grid and read back at e1 - e3 for
-// every k1.  The read-back differences e(k1) - e(k1 - q) are the deposited
-// differences e(k2 + q) - e(k2) with k2 = k1 - q, so the convolution is only
-// needed on the bins the spectrum occupies.  A workgroup owns 16 values of q,
-// one member and one of the two spectra (gain or loss):
+// every k1.  Substituting 2 -> 2 + q shows Ha_{-q}(m) = Hb_q(-m) (bins
+// mirrored about 0; the grid and the Gaussian are symmetric), so only the
+// loss spectra are built: the convolved S_q = G * Hb_q serves
+//   - the loss term of q at k1:  -A(q) f1 (1 - f(k1-q)) S_q(e1 - e(k1-q)),
+//   - the gain term of -q at k1: +A(q) (1 - f1) f(k1+q) S_q(e(k1+q) - e1)
+// (A(-q) = A(q)).  Both read-back positions are deposited differences
+// e(k2 + q) - e(k2) (k2 = k1 - q and k2 = k1), so the convolution is only
+// needed on the bins the spectrum occupies.  A workgroup owns 16 values of q
+// and one member:
 //   - the spectrum is accumulated in LDS as int64 fixed point from one sweep
 //     over k2 (integer adds are exact, so the sum does not depend on their
 //     order; lanes of a wave that hit the same bin are summed by a segmented
 //     scan before one atomic), with its occupied bin range, then converted to
 //     FP64 in place;
 //   - the convolution runs in FP64 over the occupied range, 8 grid outputs
-//     per thread from a sliding register window, and overwrites the spectrum
-//     in place once every thread holds its outputs;
+//     per thread from a register window over the spectrum (zero guard bins
+//     on both sides, so no read is bounds-checked), and overwrites the
+//     spectrum in place once every thread holds its outputs;
 //   - each thread keeps the sums of its k1 values in registers across the
 //     16 transfers and writes them once;
 // one spectrum per workgroup (M doubles of LDS) and at most 128 registers
 // let two workgroups share a CU, so one's convolution overlaps another's
 // deposit.  A last kernel sums the workgroups' partials per k1 in a fixed
-// order (gain tiles, then loss tiles) and adds the result to the caller's
-// output on every mesh node.
+// order (tile by tile) and adds the result to the caller's output on every
+// mesh node.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -66,7 +72,7 @@ struct EeData {
     double* amp = nullptr;       // A(q) (P)
     double* fp = nullptr;        // f on the lattice, cap x P
     int32_t* fexp = nullptr;     // fixed-point exponent per member
-    double* part = nullptr;      // cap x 2 x ntile x P partial sums (gain, loss)
+    double* part = nullptr;      // cap x ntile x P partial sums
 };
 
 struct EeLattice {
@@ -175,31 +181,34 @@ __device__ inline void ee_deposit(int m, long long a0, long long a1, long long* 
 
 // S = G * H on the bins [lo, hi], in place: every thread first computes its
 // outputs (kEeConvR consecutive bins per chunk, the taps walked from +J to
-// -J with a sliding window of H in registers; one LDS read of H and one
-// broadcast read of G per tap), then, after a barrier, overwrites H with them.
-// H is zero outside [lo, hi].
-__device__ inline void ee_convolve_inplace(double* __restrict__ H, const double* __restrict__ g, int lo,
-                                           int hi, int M, int J) {
+// -J with a window of H in registers; one LDS read of H and one scalar load
+// of the tap per tap), then, after a barrier, overwrites H with them.  The
+// window is not shifted: the taps run in unrolled blocks of kEeConvR, and at
+// step s of a block output r reads slot (r + s) % kEeConvR, the slot the
+// step then refills.  H is zero outside [lo, hi], including its guard bins
+// H[-J .. -1] and H[M .. M + J + kEeConvR - 1], so no read is bounds-checked.
+__device__ inline void ee_convolve_inplace(double* __restrict__ H, const double* __restrict__ gtab,
+                                           int lo, int hi, int J) {
     double out[kEeConvCh][kEeConvR];
 #pragma unroll
     for (int c = 0; c < kEeConvCh; ++c) {
         const int m0 = lo + (threadIdx.x + c * kEeT) * kEeConvR;
+#pragma unroll
+        for (int r = 0; r < kEeConvR; ++r) out[c][r] = 0.0;
+        if (m0 > hi) continue;
         double h[kEeConvR];
 #pragma unroll
-        for (int r = 0; r < kEeConvR; ++r) {
-            out[c][r] = 0.0;
-            const int k = m0 + r - J;
-            h[r] = (m0 <= hi && k >= 0 && k < M) ? H[k] : 0.0;
-        }
-        if (m0 > hi) continue;
-        for (int j = J; j >= -J; --j) {
-            const double gj = g[j + J];
+        for (int r = 0; r < kEeConvR; ++r) h[r] = H[m0 + r - J];
+        for (int j0 = J; j0 >= -J; j0 -= kEeConvR) {
 #pragma unroll
-            for (int r = 0; r < kEeConvR; ++r) out[c][r] += gj * h[r];
+            for (int s = 0; s < kEeConvR; ++s) {
+                const int j = j0 - s;
+                if (j < -J) break;
+                const double gj = gtab[j + J];
 #pragma unroll
-            for (int r = 0; r + 1 < kEeConvR; ++r) h[r] = h[r + 1];
-            const int k = m0 + kEeConvR - j;
-            h[kEeConvR - 1] = (k >= 0 && k < M) ? H[k] : 0.0;
+                for (int r = 0; r < kEeConvR; ++r) out[c][r] += gj * h[(r + s) % kEeConvR];
+                h[s] = H[m0 + kEeConvR - j];
+            }
         }
     }
     __syncthreads();
@@ -212,7 +221,7 @@ __device__ inline void ee_convolve_inplace(double* __restrict__ H, const double*
     }
 }
 
-// grid (q tile, member, spectrum): spectrum 0 is the gain term, 1 the loss
+// grid (q tile, member)
 template <int ACC>
 __global__ void __launch_bounds__(kEeT, 4)
 ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
@@ -222,22 +231,20 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                const int32_t* __restrict__ fexp, int ntile, const double* __restrict__ gate,
                double* __restrict__ part) {
     extern __shared__ double lds[];
-    __shared__ double g[2 * kEeMaxJ + 1];
     __shared__ int s_lo, s_hi;
     if (gate && gate[blockIdx.y] == 0.0) return;   // uniform over the workgroup
-    long long* H = reinterpret_cast<long long*>(lds);       // spectrum, fixed point
-    double* Hd = lds;                                        // the same bins as FP64
+    // spectrum bins 0 .. M-1 with zero guard bins -J .. -1 and M .. M+J+kEeConvR-1
+    long long* H = reinterpret_cast<long long*>(lds) + J;   // fixed point
+    double* Hd = lds + J;                                    // the same bins as FP64
     const int tid = threadIdx.x;
     const int b = blockIdx.y;
     const int tile = blockIdx.x;
-    const int gain = blockIdx.z == 0;
     const int q0 = tile * kEeQTile;
     const int q1 = min(q0 + kEeQTile, P);
     const double* fb = fp + (int64_t)b * P;
     const double scale = ldexp(1.0, fexp[b]);
     const double inv = ldexp(1.0, -fexp[b]);
-    for (int j = tid; j <= 2 * J; j += kEeT) g[j] = gtab[j];
-    for (int m = tid; m < M; m += kEeT) H[m] = 0;
+    for (int m = tid - J; m < M + J + kEeConvR; m += kEeT) H[m] = 0;
     if (tid == 0) {
         s_lo = M;
         s_hi = -1;
@@ -253,8 +260,8 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
     for (int q = q0; q < q1; ++q) {
         const double aq = w * amp[q];
         const int qc = coords[q];
-        // spectrum of transfer q: k4 = k2 + q (every lane takes part in every
-        // iteration: the deposit scans across the wave)
+        // loss spectrum of transfer q: k4 = k2 + q (every lane takes part in
+        // every iteration: the deposit scans across the wave)
         int mlo = M, mhi = -1;
 #pragma unroll 1
         for (int i = 0; i < ACC; ++i) {
@@ -268,7 +275,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                 const double fl = floor(y);
                 const double t = y - fl;
                 m = (int)fl + Mh;
-                const double F = gain ? w * f4 * (1.0 - f2) : w * f2 * (1.0 - f4);
+                const double F = w * f2 * (1.0 - f4);
                 a0 = llrint((1.0 - t) * F * scale);
                 a1 = llrint(t * F * scale);
                 mlo = min(mlo, m);
@@ -289,21 +296,25 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
         const int lo = s_lo, hi = s_hi + 1;
         for (int m = lo + tid; m <= hi; m += kEeT) Hd[m] = (double)H[m] * inv;
         __syncthreads();
-        ee_convolve_inplace(Hd, g, lo, hi, M, J);
+        ee_convolve_inplace(Hd, gtab, lo, hi, J);
         __syncthreads();
-        // pairs (1, 3) with k1 - k3 = q: the bins read are within [lo, hi]
+        // k1 = p1: the loss term of q (k3 = k1 - q) and the gain term of -q
+        // (k3 = k1 + q), both read within [lo, hi]
 #pragma unroll 1
         for (int i = 0; i < ACC; ++i) {
             const int p1 = tid + i * kEeT;
             if (p1 < P) {
-                const int p3 = ee_shift(L, coords[p1], qc, -1);
-                const double x = (ep[p1] - ep[p3]) / dw;
-                const double fl = floor(x);
-                const double t = x - fl;
-                const int m = (int)fl + Mh;
-                const double sv = (1.0 - t) * Hd[m] + t * Hd[m + 1];
-                const double f1 = fb[p1], f3 = fb[p3];
-                acc[i] += gain ? aq * ((1.0 - f1) * f3 * sv) : -aq * (f1 * (1.0 - f3) * sv);
+                const int c1 = coords[p1];
+                const int pm = ee_shift(L, c1, qc, -1), pp = ee_shift(L, c1, qc, +1);
+                const double e1 = ep[p1];
+                const double x = (e1 - ep[pm]) / dw, z = (ep[pp] - e1) / dw;
+                const double fx = floor(x), fz = floor(z);
+                const double tx = x - fx, tz = z - fz;
+                const int mx = (int)fx + Mh, mz = (int)fz + Mh;
+                const double sx = (1.0 - tx) * Hd[mx] + tx * Hd[mx + 1];
+                const double sz = (1.0 - tz) * Hd[mz] + tz * Hd[mz + 1];
+                const double f1 = fb[p1], fm = fb[pm], fp_ = fb[pp];
+                acc[i] += aq * ((1.0 - f1) * fp_ * sz - f1 * (1.0 - fm) * sx);
             }
         }
         __syncthreads();
@@ -317,7 +328,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
 #pragma unroll
     for (int i = 0; i < ACC; ++i) {
         const int p1 = tid + i * kEeT;
-        if (p1 < P) part[(((int64_t)b * 2 + blockIdx.z) * ntile + tile) * P + p1] = acc[i];
+        if (p1 < P) part[((int64_t)b * ntile + tile) * P + p1] = acc[i];
     }
 }
 
@@ -330,7 +341,7 @@ __global__ void ee_reduce_kernel(const double* __restrict__ part, int ntile, int
     if (i >= n || (gate && gate[b] == 0.0)) return;
     const int64_t p = p_of_node[i];
     double s = 0.0;
-    for (int t = 0; t < 2 * ntile; ++t) s += part[((int64_t)b * 2 * ntile + t) * P + p];
+    for (int t = 0; t < ntile; ++t) s += part[((int64_t)b * ntile + t) * P + p];
     out[(int64_t)b * n + i] += s;
 }
 
@@ -367,8 +378,8 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
                          kEeMaxJ);
     hipDeviceProp_t prop;
     TG_HIP(c, hipGetDeviceProperties(&prop, c->device));
-    const size_t lds = sizeof(double) * (size_t)e->M;
-    if (e->M > kEeMaxM || lds + sizeof(double) * (2 * kEeMaxJ + 1) + 64 > (size_t)prop.sharedMemPerBlock)
+    const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
+    if (e->M > kEeMaxM || lds + 64 > (size_t)prop.sharedMemPerBlock)
         return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds %d", e->M, kEeMaxM);
     for (const void* k : {(const void*)ee_main_kernel<8>, (const void*)ee_main_kernel<16>,
                           (const void*)ee_main_kernel<32>})
@@ -437,7 +448,7 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
     TG_TRY(dev_alloc(c, &e->coords, (size_t)P));
     TG_TRY(dev_alloc(c, &e->fp, (size_t)P * e->cap));
     TG_TRY(dev_alloc(c, &e->fexp, (size_t)e->cap));
-    TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap * 2));
+    TG_TRY(dev_alloc(c, &e->part, (size_t)P * e->ntile * e->cap));
     TG_HIP(c, hipMemcpy(e->gtab, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->ep, ep.data(), sizeof(double) * P, hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(e->amp, amp.data(), sizeof(double) * P, hipMemcpyHostToDevice));
@@ -474,9 +485,9 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, 
                        e->fp, e->fexp, gate);
     TG_CHECK_LAUNCH(c);
     const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
-    const size_t lds = sizeof(double) * (size_t)e->M;
+    const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
 #define TG_EE_MAIN(ACC)                                                                              \
-    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B, 2), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
+    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
                        e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
                        e->ntile, gate,                                                               \
                        e->part)

@@ -224,7 +224,7 @@ __device__ inline void ee_convolve_inplace(double* __restrict__ H, const double*
 // grid (q tile, member)
 template <int ACC>
 __global__ void __launch_bounds__(kEeT, 4)
-ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
+ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w,
                const double* __restrict__ gtab, const double* __restrict__ ep,
                const int32_t* __restrict__ coords,
                const double* __restrict__ amp, const double* __restrict__ fp,
@@ -271,7 +271,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
             if (p2 < P) {
                 const int p4 = ee_shift(L, coords[p2], qc, +1);
                 const double f2 = fb[p2], f4 = fb[p4];
-                const double y = (ep[p4] - ep[p2]) / dw;
+                const double y = (ep[p4] - ep[p2]) * inv_dw;
                 const double fl = floor(y);
                 const double t = y - fl;
                 m = (int)fl + Mh;
@@ -299,7 +299,8 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
         ee_convolve_inplace(Hd, gtab, lo, hi, J);
         __syncthreads();
         // k1 = p1: the loss term of q (k3 = k1 - q) and the gain term of -q
-        // (k3 = k1 + q), both read within [lo, hi]
+        // (k3 = k1 + q), both read within [lo, hi] (x and z are computed by
+        // the same operations as the deposits' y at k2 = k1 - q and k2 = k1)
 #pragma unroll 1
         for (int i = 0; i < ACC; ++i) {
             const int p1 = tid + i * kEeT;
@@ -307,7 +308,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double dw, double w,
                 const int c1 = coords[p1];
                 const int pm = ee_shift(L, c1, qc, -1), pp = ee_shift(L, c1, qc, +1);
                 const double e1 = ep[p1];
-                const double x = (e1 - ep[pm]) / dw, z = (ep[pp] - e1) / dw;
+                const double x = (e1 - ep[pm]) * inv_dw, z = (ep[pp] - e1) * inv_dw;
                 const double fx = floor(x), fz = floor(z);
                 const double tx = x - fx, tz = z - fz;
                 const int mx = (int)fx + Mh, mz = (int)fz + Mh;
@@ -488,7 +489,7 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, 
     const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
 #define TG_EE_MAIN(ACC)                                                                              \
     hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
-                       e->M, e->J, e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
+                       e->M, e->J, 1.0 / e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
                        e->ntile, gate,                                                               \
                        e->part)
     if (e->P <= 8 * kEeT) TG_EE_MAIN(8);

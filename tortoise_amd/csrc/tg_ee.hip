@@ -80,24 +80,6 @@ struct EeLattice {
     int ns0, ns1, ns2;
 };
 
-__device__ inline int64_t lat_add(const EeLattice& L, int64_t a, int64_t b, int sign) {
-    int64_t a0 = a % L.ns0, b0 = b % L.ns0;
-    int64_t c0 = (a0 + sign * b0) % L.ns0;
-    if (c0 < 0) c0 += L.ns0;
-    if (L.dim == 1) return c0;
-    a /= L.ns0;
-    b /= L.ns0;
-    int64_t a1 = a % L.ns1, b1 = b % L.ns1;
-    int64_t c1 = (a1 + sign * b1) % L.ns1;
-    if (c1 < 0) c1 += L.ns1;
-    if (L.dim == 2) return c0 + (int64_t)L.ns0 * c1;
-    a /= L.ns1;
-    b /= L.ns1;
-    int64_t c2 = (a + sign * b) % L.ns2;
-    if (c2 < 0) c2 += L.ns2;
-    return c0 + (int64_t)L.ns0 * (c1 + (int64_t)L.ns1 * c2);
-}
-
 // f on the lattice per member and the fixed-point exponent E of its spectra:
 // every bin sum is bounded by P w m (1 + m), m = max|f|, and E puts that
 // bound at 2^61
@@ -129,15 +111,17 @@ ee_prep_kernel(const double* __restrict__ f, int64_t n, int64_t P,
 }
 
 // lattice coordinates are packed as c0 | c1 << 10 | c2 << 20 (at most 1023 per axis)
-// index of the point with coordinates a + sign*q (periodic), no divisions
-__device__ inline int ee_shift(const EeLattice& L, int a, int q, int sign) {
-    int c0 = (a & 1023) + sign * (q & 1023);
-    int c1 = ((a >> 10) & 1023) + sign * ((q >> 10) & 1023);
-    int c2 = (a >> 20) + sign * (q >> 20);
-    c0 += c0 < 0 ? L.ns0 : (c0 >= L.ns0 ? -L.ns0 : 0);
-    c1 += c1 < 0 ? L.ns1 : (c1 >= L.ns1 ? -L.ns1 : 0);
-    c2 += c2 < 0 ? L.ns2 : (c2 >= L.ns2 ? -L.ns2 : 0);
-    return c0 + L.ns0 * (c1 + L.ns1 * c2);
+// index of the point a + sign*q (periodic) from the linear indices p of a
+// and q of the transfer: p + sign*q, corrected on each axis that wraps
+// (sign +: c + q >= ns; sign -: c < q); no divisions or re-linearisation
+__device__ inline int ee_shift(const EeLattice& L, int p, int a, int q, int qc, int sign) {
+    const int c0 = a & 1023, c1 = (a >> 10) & 1023, c2 = a >> 20;
+    const int q0 = qc & 1023, q1 = (qc >> 10) & 1023, q2 = qc >> 20;
+    const int s1 = L.ns0 * L.ns1, s2 = s1 * L.ns2;
+    if (sign > 0)
+        return p + q - (c0 >= L.ns0 - q0 ? L.ns0 : 0) - (c1 >= L.ns1 - q1 ? s1 : 0) -
+               (c2 >= L.ns2 - q2 ? s2 : 0);
+    return p - q + (c0 < q0 ? L.ns0 : 0) + (c1 < q1 ? s1 : 0) + (c2 < q2 ? s2 : 0);
 }
 
 // Adds the two fixed-point deposits of every lane to its bins m, m + 1 of
@@ -269,7 +253,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w
             int m = -1;
             long long a0 = 0, a1 = 0;
             if (p2 < P) {
-                const int p4 = ee_shift(L, coords[p2], qc, +1);
+                const int p4 = ee_shift(L, p2, coords[p2], q, qc, +1);
                 const double f2 = fb[p2], f4 = fb[p4];
                 const double y = (ep[p4] - ep[p2]) * inv_dw;
                 const double fl = floor(y);
@@ -306,7 +290,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w
             const int p1 = tid + i * kEeT;
             if (p1 < P) {
                 const int c1 = coords[p1];
-                const int pm = ee_shift(L, c1, qc, -1), pp = ee_shift(L, c1, qc, +1);
+                const int pm = ee_shift(L, p1, c1, q, qc, -1), pp = ee_shift(L, p1, c1, q, qc, +1);
                 const double e1 = ep[p1];
                 const double x = (e1 - ep[pm]) * inv_dw, z = (ep[pp] - e1) * inv_dw;
                 const double fx = floor(x), fz = floor(z);

@@ -329,6 +329,10 @@ def main():
                     help="skip the reference tuple operator measurements")
     ap.add_argument("--only-reference-op", action="store_true",
                     help="only the reference tuple operator measurements (profiling runs)")
+    ap.add_argument("--only-ee", action="store_true",
+                    help="only the 20^3 e-ph + e-e ensemble line (profiling runs)")
+    ap.add_argument("--ee-t-final", type=float, default=1.0,
+                    help="DP5 end time of the 20^3 e-e ensemble run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -350,6 +354,10 @@ def main():
     if args.only_reference_op:
         torch.cuda.init()
         print(json.dumps({"reference_operator": reference_operator()}), flush=True)
+        return
+    if args.only_ee:
+        torch.cuda.init()
+        print(json.dumps({"cube20_ee": cube_ee(t_final=args.ee_t_final)}), flush=True)
         return
 
     w = wl.SQUARE128_ENSEMBLE
@@ -461,7 +469,7 @@ def main():
         out["mixed_vs_cpu"] = evals_mixed / cb["value"]
     ctx.close()
     if not args.no_dp5:
-        out["cube20_ee"] = cube_ee()
+        out["cube20_ee"] = cube_ee(t_final=args.ee_t_final)
     if not args.no_reference_op:
         out["reference_operator"] = reference_operator()
     print(json.dumps(out), flush=True)

@@ -138,13 +138,14 @@ tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double
 /* One collision-operator evaluation per member:
  *   I_b = [(1-f_b)(W_b f_b) - f_b (W_b^T (1-f_b))] / V,  W_b = (n_b+1) K + n_b K^T.
  * bath_n: per-member bath occupations (batch values) or NULL for the default.
- * Default path: FP64 one-pass K f / K^T f kernel (any batch, members in turn).
- * TG_PREC_MIXED: batched MFMA kernel (batch <= max_batch). */
+ * Default path: FP64 streaming pass over the stacked band [K; K^T] (any
+ * batch, members in turn).  TG_PREC_MIXED: batched MFMA kernel
+ * (batch <= max_batch). */
 tg_status   tg_collision_apply(tg_ctx* ctx, const double* f, double* I, int64_t n,
                                int32_t batch, const double* bath_n, int32_t flags);
 
 /* Duration in ms of the dominant kernel of the last tg_collision_apply made
- * with TG_TIME_KERNEL (plain: the one-pass K f / K^T f kernel; mixed: the
+ * with TG_TIME_KERNEL (plain: the stacked-band K f / K^T f pass; mixed: the
  * MFMA GEMM), from HIP events on the context's stream.  Waits for it. */
 tg_status   tg_last_kernel_ms(tg_ctx* ctx, double* ms);
 
@@ -155,7 +156,11 @@ tg_status   tg_last_kernel_ms(tg_ctx* ctx, double* ms);
  * major: member b's k-th output at dense_out + (b*max_dense + k)*n) receives
  * the dense outputs at t0 + k*dense_dt.  stats: batch entries (optional).
  * h_accepted (optional, batch x h_cap): the accepted step sizes of member b
- * in order at h_accepted + b*h_cap (stats[b].accepted of them, up to h_cap). */
+ * in order at h_accepted + b*h_cap (stats[b].accepted of them, up to h_cap).
+ * A step whose error norm is not finite (its stages overflowed) is rejected
+ * with the smallest step scale; a member whose step then falls below
+ * dense_dt/1e4 fails with TG_ERR_NONFINITE (TG_ERR_STEP_UNDERFLOW when the
+ * last norm was finite). */
 tg_status   tg_dp5_integrate(tg_ctx* ctx, double* f, int64_t n, int32_t batch,
                              double t0, double t1, double dense_dt, double rtol,
                              double atol, const double* bath_n, int32_t flags,

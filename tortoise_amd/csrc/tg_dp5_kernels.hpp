@@ -133,8 +133,18 @@ __global__ void controller_kernel(MemberDev* __restrict__ md, Ctl k,
             const double e = k.replay ? 0.0 : sqrt(norms[b] / k.denom);
             const double h = m.h;
             if (!isfinite(e)) {
-                m.failed = 1;
-                m.nonfinite = 1;
+                // a trial step so far outside the stability region that its
+                // stages overflow: rejected with the smallest scale (what
+                // DormandPrince54.hpp:646-649 gives for an infinite norm); a
+                // member that cannot recover fails by step underflow, reported
+                // as a non-finite error norm
+                m.h = h * minscale;
+                m.rejected = 1;
+                m.st.rejected += 1;
+                if (m.h < k.min_h) {
+                    m.failed = 1;
+                    m.nonfinite = 1;
+                }
             } else if (e <= 1.0) {
                 // dense outputs inside the step (:598-611)
                 m.step_t = m.t;

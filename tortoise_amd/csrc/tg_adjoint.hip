@@ -12,9 +12,11 @@
 //   mu^T dF/domega0 = mu^T F[W'](Y),  W' = (n+1) K' + n K'^T + n' (K + K^T),
 // with K' = dK/domega0 (same windows, sampled G') and n' = dn/domega0 given
 // by the caller.  Every product is one pass of the FP64 one-pass kernel over
-// K or K' (K Y, K^T Y in one read).  Forward states are recomputed from
-// checkpoints every `ckpt` steps; only the step-start states of one segment
-// and the seven stage states of one step are held at a time.
+// K or K' (K Y, K^T Y in one read).  With the e-e channel on, its operator
+// joins the forward stages and its vector-Jacobian product joins J^T mu
+// (tg_ee.hip); it depends on neither g nor omega0.  Forward states are
+// recomputed from checkpoints every `ckpt` steps; only the step-start states
+// of one segment and the seven stage states of one step are held at a time.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -221,6 +223,7 @@ struct AdjRun {
                                w->Y[s]);
             TG_CHECK_LAUNCH(c);
             TG_TRY(plain_apply_device(c, w->Y[s], w->k[s], nb, 0));
+            if (c->ee) TG_TRY(ee_apply_device(c, w->Y[s], w->k[s], 1));
         }
         return TG_OK;
     }
@@ -236,6 +239,8 @@ struct AdjRun {
         hipLaunchKernelGGL(vjp_out_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, c->d_V, nb,
                            w->uf, w->vf, c->d_rk, c->d_ck, w->ux, w->vx, w->uy, w->vy, w->nu);
         TG_CHECK_LAUNCH(c);
+        // the e-e channel adds its own product (it depends on neither g nor omega0)
+        if (c->ee) TG_TRY(ee_vjp_device(c, Yst, mu, w->nu));
         TG_TRY(plain_products_device(c, c->d_Kd, Yst, w->udf, w->vdf));
         const int nblk = (int)ceil_div(n, kAdjT);
         hipLaunchKernelGGL(param_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, Yst,
@@ -262,7 +267,6 @@ extern "C" tg_status tg_dp5_adjoint(tg_ctx* c, const double* f0, int64_t n, doub
     if (n_steps < 1 || ckpt < 1 || !(dense_dt > 0.0)) return set_error(c, TG_ERR_ARG, "bad steps");
     if (!(c->g != 0.0)) return set_error(c, TG_ERR_ARG, "g must be nonzero for dL/dg");
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "tg_kernel_assemble not called");
-    if (c->ee) return set_error(c, TG_ERR_STATE, "the adjoint covers the e-ph channel only (e-e is on)");
     TG_TRY(assemble_domega(c));
     const int32_t nck = n_steps / ckpt + 2;
     TG_TRY(adj_ensure(c, ckpt, nck));

@@ -73,6 +73,10 @@ struct EeData {
     double* fp = nullptr;        // f on the lattice, cap x P
     int32_t* fexp = nullptr;     // fixed-point exponent per member
     double* part = nullptr;      // cap x ntile x P partial sums
+    // vector-Jacobian product (one member): mu summed onto the lattice and
+    // the fixed-point exponent of the mu-weighted spectrum
+    double* mup = nullptr;       // P
+    int32_t* mexp = nullptr;     // 1
 };
 
 struct EeLattice {
@@ -205,15 +209,26 @@ __device__ inline void ee_convolve_inplace(double* __restrict__ H, const double*
     }
 }
 
-// grid (q tile, member)
-template <int ACC>
+// grid (q tile, member).  MODE 0: the operator.  MODES 1 and 2: the two
+// passes of the vector-Jacobian product lambda = J(f)^T mu (one member, mu on
+// the lattice in mup).  With W(1,2,3,4) the kernel of the sum (symmetric under
+// 1<->2, 3<->4 and under 12<->34),
+//   lambda_1 = sum_{2,3} W (mu3 + mu4 - mu1 - mu2) [f3 f4 (1-f2) + f2 (1-f3)(1-f4)],
+// which splits like the operator into pair spectra over k2 read back at k1:
+//   MODE 1 deposits the loss spectrum Hb_q (as MODE 0) and reads back
+//     (1 - f3)(mu3 - mu1) S_q(e1 - e3), k3 = k1 - q, and f3 (mu3 - mu1) S_q(e3 - e1),
+//     k3 = k1 + q (the gain spectrum of -q, mirrored);
+//   MODE 2 deposits Hbmu_q(m) = sum_2 w h_m(e4 - e2) (mu4 - mu2) f2 (1 - f4)
+//     (the mirror of -Habmu_{-q}) and reads back (1 - f3) S_q(e1 - e3), k3 = k1 - q,
+//     and -f3 S_q(e3 - e1), k3 = k1 + q.
+template <int ACC, int MODE>
 __global__ void __launch_bounds__(kEeT, 4)
 ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w,
                const double* __restrict__ gtab, const double* __restrict__ ep,
                const int32_t* __restrict__ coords,
                const double* __restrict__ amp, const double* __restrict__ fp,
                const int32_t* __restrict__ fexp, int ntile, const double* __restrict__ gate,
-               double* __restrict__ part) {
+               const double* __restrict__ mup, double* __restrict__ part) {
     extern __shared__ double lds[];
     __shared__ int s_lo, s_hi;
     if (gate && gate[blockIdx.y] == 0.0) return;   // uniform over the workgroup
@@ -259,7 +274,8 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w
                 const double fl = floor(y);
                 const double t = y - fl;
                 m = (int)fl + Mh;
-                const double F = w * f2 * (1.0 - f4);
+                double F = w * f2 * (1.0 - f4);
+                if constexpr (MODE == 2) F *= mup[p4] - mup[p2];
                 a0 = llrint((1.0 - t) * F * scale);
                 a1 = llrint(t * F * scale);
                 mlo = min(mlo, m);
@@ -299,7 +315,14 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w
                 const double sx = (1.0 - tx) * Hd[mx] + tx * Hd[mx + 1];
                 const double sz = (1.0 - tz) * Hd[mz] + tz * Hd[mz + 1];
                 const double f1 = fb[p1], fm = fb[pm], fp_ = fb[pp];
-                acc[i] += aq * ((1.0 - f1) * fp_ * sz - f1 * (1.0 - fm) * sx);
+                if constexpr (MODE == 0) {
+                    acc[i] += aq * ((1.0 - f1) * fp_ * sz - f1 * (1.0 - fm) * sx);
+                } else if constexpr (MODE == 1) {
+                    const double m1 = mup[p1];
+                    acc[i] += aq * ((1.0 - fm) * (mup[pm] - m1) * sx + fp_ * (mup[pp] - m1) * sz);
+                } else {
+                    acc[i] += aq * ((1.0 - fm) * sx - fp_ * sz);
+                }
             }
         }
         __syncthreads();
@@ -328,6 +351,64 @@ __global__ void ee_reduce_kernel(const double* __restrict__ part, int ntile, int
     double s = 0.0;
     for (int t = 0; t < ntile; ++t) s += part[((int64_t)b * ntile + t) * P + p];
     out[(int64_t)b * n + i] += s;
+}
+
+// mu (nodes) -> mup (lattice, one member): the fixed-order sum over the
+// nodes of each periodic point (a point on a face, edge or corner of the box
+// has 2, 4 or 8 nodes: coordinate 0 and nsec on each axis where it is 0);
+// and the fixed-point exponent of the mu-weighted spectrum, whose bin sums
+// are bounded by P w max|f| (1 + max|f|) 2 max|mup|.  One workgroup.
+__global__ void __launch_bounds__(kEeT)
+ee_vjp_prep_kernel(EeLattice L, int64_t P, const double* __restrict__ mu, const double* __restrict__ fp,
+                   double w, double* __restrict__ mup, int32_t* __restrict__ mexp) {
+    __shared__ double red[2][kEeT / 64];
+    const int s0 = L.ns0 + 1, s1 = (L.ns0 + 1) * (L.ns1 + 1);
+    double mf = 0.0, mm = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kEeT) {
+        const int c0 = (int)(p % L.ns0), c1 = (int)((p / L.ns0) % L.ns1), c2 = (int)(p / ((int64_t)L.ns0 * L.ns1));
+        const int n0 = c0 == 0 ? 2 : 1, n1 = (L.dim > 1 && c1 == 0) ? 2 : 1, n2 = (L.dim > 2 && c2 == 0) ? 2 : 1;
+        double sum = 0.0;
+        for (int a2 = 0; a2 < n2; ++a2)
+            for (int a1 = 0; a1 < n1; ++a1)
+                for (int a0 = 0; a0 < n0; ++a0)
+                    sum += mu[(c0 + a0 * L.ns0) + (int64_t)(c1 + a1 * L.ns1) * s0 + (int64_t)(c2 + a2 * L.ns2) * s1];
+        mup[p] = sum;
+        mm = fmax(mm, fabs(sum));
+        mf = fmax(mf, fabs(fp[p]));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mf = fmax(mf, __shfl_down(mf, o, 64));
+        mm = fmax(mm, __shfl_down(mm, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = mf;
+        red[1][threadIdx.x >> 6] = mm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < kEeT / 64; ++i) {
+            a = fmax(a, red[0][i]);
+            b = fmax(b, red[1][i]);
+        }
+        const double bound = (double)P * w * a * (1.0 + a) * 2.0 * b;
+        int32_t e = 61;
+        if (!isfinite(bound)) e = 0;
+        else if (bound > 0.0) e = (int32_t)floor(61.0 - log2(bound));
+        *mexp = max(-1000, min(1000, e));
+    }
+}
+
+// out[i] += sum over workgroups (fixed order) of the point's partials, on the
+// node that carries the point's value (node_of_p): the other nodes of a point
+// do not enter the operator
+__global__ void ee_vjp_reduce_kernel(const double* __restrict__ part, int ntile, int64_t P,
+                                     const int32_t* __restrict__ node_of_p, double* __restrict__ out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double s = 0.0;
+    for (int t = 0; t < ntile; ++t) s += part[(int64_t)t * P + p];
+    out[node_of_p[p]] += s;
 }
 
 static EeData* ee_of(tg_ctx* c) { return (EeData*)c->ee; }
@@ -366,8 +447,11 @@ tg_status ee_prepare(tg_ctx* c, const double* h_eps) {
     const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
     if (e->M > kEeMaxM || lds + 64 > (size_t)prop.sharedMemPerBlock)
         return set_error(c, TG_ERR_ARG, "e-e channel: energy grid of %d cells exceeds %d", e->M, kEeMaxM);
-    for (const void* k : {(const void*)ee_main_kernel<8>, (const void*)ee_main_kernel<16>,
-                          (const void*)ee_main_kernel<32>})
+    for (const void* k : {(const void*)ee_main_kernel<8, 0>, (const void*)ee_main_kernel<16, 0>,
+                          (const void*)ee_main_kernel<32, 0>, (const void*)ee_main_kernel<8, 1>,
+                          (const void*)ee_main_kernel<16, 1>, (const void*)ee_main_kernel<32, 1>,
+                          (const void*)ee_main_kernel<8, 2>, (const void*)ee_main_kernel<16, 2>,
+                          (const void*)ee_main_kernel<32, 2>})
         TG_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     e->ntile = (int32_t)((e->P + kEeQTile - 1) / kEeQTile);
     e->cap = c->max_batch;
@@ -455,6 +539,8 @@ void ee_destroy(tg_ctx* c) {
     dev_free(&e->fp);
     dev_free(&e->fexp);
     dev_free(&e->part);
+    dev_free(&e->mup);
+    dev_free(&e->mexp);
     delete e;
     c->ee = nullptr;
 }
@@ -472,10 +558,9 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, 
     const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
     const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
 #define TG_EE_MAIN(ACC)                                                                              \
-    hipLaunchKernelGGL(ee_main_kernel<ACC>, dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
+    hipLaunchKernelGGL((ee_main_kernel<ACC, 0>), dim3(e->ntile, B), dim3(kEeT), lds, st, L, (int)e->P, e->Mh, \
                        e->M, e->J, 1.0 / e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp, e->fexp,   \
-                       e->ntile, gate,                                                               \
-                       e->part)
+                       e->ntile, gate, nullptr, e->part)
     if (e->P <= 8 * kEeT) TG_EE_MAIN(8);
     else if (e->P <= 16 * kEeT) TG_EE_MAIN(16);
     else TG_EE_MAIN(32);
@@ -484,6 +569,42 @@ tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, 
     hipLaunchKernelGGL(ee_reduce_kernel, dim3(ceil_div(c->n, 256), B), dim3(256), 0, st, e->part,
                        e->ntile, e->P, e->p_of_node, c->n, gate, d_I);
     TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+// d_out += J_ee(f)^T mu for one member (nodes, FP64); kernel launches only
+tg_status ee_vjp_device(tg_ctx* c, const double* d_f, const double* d_mu, double* d_out) {
+    EeData* e = ee_of(c);
+    if (!e) return TG_OK;
+    hipStream_t st = c->stream;
+    if (!e->mup) {
+        TG_TRY(dev_alloc(c, &e->mup, (size_t)e->P));
+        TG_TRY(dev_alloc(c, &e->mexp, 1));
+    }
+    hipLaunchKernelGGL(ee_prep_kernel, dim3(1), dim3(kEeT), 0, st, d_f, c->n, e->P, e->node_of_p, e->w,
+                       e->fp, e->fexp, nullptr);
+    TG_CHECK_LAUNCH(c);
+    const EeLattice L{e->dim, e->ns[0], e->ns[1], e->ns[2]};
+    hipLaunchKernelGGL(ee_vjp_prep_kernel, dim3(1), dim3(kEeT), 0, st, L, e->P, d_mu, e->fp, e->w, e->mup,
+                       e->mexp);
+    TG_CHECK_LAUNCH(c);
+    const size_t lds = sizeof(double) * ((size_t)e->M + 2 * e->J + kEeConvR);
+#define TG_EE_VJP(ACC, MODE, EXP)                                                                     \
+    hipLaunchKernelGGL((ee_main_kernel<ACC, MODE>), dim3(e->ntile, 1), dim3(kEeT), lds, st, L, (int)e->P, \
+                       e->Mh, e->M, e->J, 1.0 / e->dw, e->w, e->gtab, e->ep, e->coords, e->amp, e->fp,   \
+                       EXP, e->ntile, nullptr, e->mup, e->part)
+#define TG_EE_VJP_PASS(MODE, EXP)                                                                     \
+    if (e->P <= 8 * kEeT) TG_EE_VJP(8, MODE, EXP);                                                    \
+    else if (e->P <= 16 * kEeT) TG_EE_VJP(16, MODE, EXP);                                             \
+    else TG_EE_VJP(32, MODE, EXP);                                                                     \
+    TG_CHECK_LAUNCH(c);                                                                                 \
+    hipLaunchKernelGGL(ee_vjp_reduce_kernel, dim3(ceil_div(e->P, 256)), dim3(256), 0, st, e->part,      \
+                       e->ntile, e->P, e->node_of_p, d_out);                                             \
+    TG_CHECK_LAUNCH(c);
+    TG_EE_VJP_PASS(1, e->fexp)
+    TG_EE_VJP_PASS(2, e->mexp)
+#undef TG_EE_VJP_PASS
+#undef TG_EE_VJP
     return TG_OK;
 }
 

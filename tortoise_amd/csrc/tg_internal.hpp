@@ -208,6 +208,9 @@ void ee_destroy(tg_ctx* c);
 // d_I[b] += I_ee(d_f[b]); no-op without the e-e channel; kernel launches only
 tg_status ee_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                           const double* gate = nullptr);
+// d_out += J_ee(f)^T mu for one member (nodes): the e-e operator's
+// vector-Jacobian product, for the adjoint (tg_ee.hip)
+tg_status ee_vjp_device(tg_ctx* c, const double* d_f, const double* d_mu, double* d_out);
 void adjoint_destroy(tg_ctx* c);
 tg_status ensure_staging(tg_ctx* c, int32_t batch);
 // event bracket of the dominant kernel when TG_TIME_KERNEL is active

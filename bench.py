@@ -161,13 +161,29 @@ def cube_ee(steps=5, t_final=1.0):
     wall = time.perf_counter() - t0
     drift = float(np.max(np.abs(V @ fT - V @ F) / np.abs(V @ F)))
     att = [s["attempted"] for s in st]
-    ctx.close()
     _log(f"cube20 e-ph + e-e: {B} members in {ms:.2f} ms per batched evaluation; DP5 to "
          f"t={t_final} in {wall:.1f} s ({min(att)}..{max(att)} attempts), number drift {drift:.1e}")
+    # gradient of L = |f(t_final) - f0|^2 / 2 wrt g for member 0 (FP64 plain path,
+    # e-e included) through the adjoint of its own accepted steps
+    from tortoise_amd import autograd as tga
+    f0 = F[:, 0].copy()
+    t0 = time.perf_counter()
+    _, dense, st0, steps = ctx.dp5(f0, 0.0, t_final, t_final / 100, w.rtol, w.atol,
+                                   bath_n=bath[0], max_dense=100, h_cap=1 << 16)
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, dLdg, _ = ctx.dp5_adjoint(f0, 0.0, t_final, t_final / 100, steps, dense[-1] - f0, bath[0],
+                                 tga.dbath_domega0(w.omega0, temp[0]), ckpt=32)
+    adj = time.perf_counter() - t0
+    ctx.close()
+    _log(f"cube20 adjoint, member 0: forward FP64 DP5 to t={t_final} in {fwd:.1f} s "
+         f"({st0['accepted']} steps), adjoint in {adj:.1f} s, dL/dg = {dLdg:.6e}")
     return {"workload": w.name, "members": B, "nodes": int(eps.size), "lattice_points": 8000,
             "ms_per_batched_eval": ms, "member_evals_per_s": B * 1000.0 / ms,
             "dp5_t_final": t_final, "dp5_wall_s": wall, "dp5_attempts_max": int(max(att)),
-            "number_drift_rel": drift}
+            "number_drift_rel": drift,
+            "adjoint_member0": {"loss": "|f(t_final) - f0|^2 / 2", "steps": int(st0["accepted"]),
+                                "forward_s": fwd, "adjoint_s": adj, "dL_dg": dLdg}}
 
 
 def reference_operator(budget_s=12.0):

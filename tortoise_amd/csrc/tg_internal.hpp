@@ -97,7 +97,7 @@ struct tg_ctx {
     int32_t n_work = 0;
     int32_t* d_cc_off = nullptr;   // per column chunk: offset into d_cc_rows (n_colchk + 1)
     int32_t* d_cc_rows = nullptr;  // row blocks whose band covers the chunk, increasing
-    int64_t band_elems = 0;        // K entries the one-pass kernel streams per evaluation
+    int64_t band_elems = 0;        // entries the plain evaluation streams (the stacked band once built)
 
     // stacked FP64 band of [K; K^T] for the plain evaluation (tg_stack.hip)
     void* stack = nullptr;

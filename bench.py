@@ -437,7 +437,7 @@ def main():
     _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
          f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}; "
          f"kernel {gemm_kernel_name(st)} splits {st['mixed_splits']}")
-    _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), stacked-band GEMV "
+    _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), K-band pass "
          f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s algorithmic, {gbs_streamed:.0f} GB/s streamed")
 
     out = {

@@ -113,7 +113,7 @@ tg_status   tg_kernel_assemble(tg_ctx* ctx, int32_t flags);
 typedef struct tg_kernel_stats {
     int64_t n;              /* nodes */
     int64_t nnz;            /* structurally nonzero entries of K */
-    int64_t plain_band;     /* entries of the stacked band [K; K^T] the plain FP64 evaluation streams */
+    int64_t plain_band;     /* entries of K's band (128-row panels) the plain FP64 evaluation streams */
     int64_t mixed_band;     /* entries of [K; K^T] the batched kernel streams per evaluation */
     int32_t mixed_splits;   /* reduction splits of the batched kernel */
     int32_t mixed_width;    /* members per batched tile (64, 128 or 256) */
@@ -138,14 +138,14 @@ tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double
 /* One collision-operator evaluation per member:
  *   I_b = [(1-f_b)(W_b f_b) - f_b (W_b^T (1-f_b))] / V,  W_b = (n_b+1) K + n_b K^T.
  * bath_n: per-member bath occupations (batch values) or NULL for the default.
- * Default path: FP64 streaming pass over the stacked band [K; K^T] (any
- * batch, members in turn).  TG_PREC_MIXED: batched MFMA kernel
+ * Default path: FP64 streaming pass over K's band giving K f and K^T f
+ * in one read (any batch, members in turn).  TG_PREC_MIXED: batched MFMA kernel
  * (batch <= max_batch). */
 tg_status   tg_collision_apply(tg_ctx* ctx, const double* f, double* I, int64_t n,
                                int32_t batch, const double* bath_n, int32_t flags);
 
 /* Duration in ms of the dominant kernel of the last tg_collision_apply made
- * with TG_TIME_KERNEL (plain: the stacked-band K f / K^T f pass; mixed: the
+ * with TG_TIME_KERNEL (plain: the K-band K f / K^T f pass; mixed: the
  * MFMA GEMM), from HIP events on the context's stream.  Waits for it. */
 tg_status   tg_last_kernel_ms(tg_ctx* ctx, double* ms);
 

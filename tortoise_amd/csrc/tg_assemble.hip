@@ -427,7 +427,7 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
         TG_HIP(c, hipStreamSynchronize(c->stream));
         // the projection/convolution scratch is only needed for assembly
         dev_free(&c->d_Q);
-        // 8. the stacked FP64 band [K; K^T] of the plain evaluation
+        // 8. the FP64 band of K of the plain evaluation
         TG_TRY(stack_prepare(c));
         c->assembled = true;
     }

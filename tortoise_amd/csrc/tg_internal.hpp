@@ -97,9 +97,9 @@ struct tg_ctx {
     int32_t n_work = 0;
     int32_t* d_cc_off = nullptr;   // per column chunk: offset into d_cc_rows (n_colchk + 1)
     int32_t* d_cc_rows = nullptr;  // row blocks whose band covers the chunk, increasing
-    int64_t band_elems = 0;        // entries the plain evaluation streams (the stacked band once built)
+    int64_t band_elems = 0;        // entries the plain evaluation streams (K's band once built)
 
-    // stacked FP64 band of [K; K^T] for the plain evaluation (tg_stack.hip)
+    // FP64 band of K for the plain evaluation (tg_stack.hip)
     void* stack = nullptr;
 
     // mixed batched data (tg_mixed.hip)
@@ -180,7 +180,7 @@ tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, doubl
                               double* energy);
 tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,
                                 int32_t batch, double rtol, double atol, double* h_out);
-// stacked FP64 band of [K; K^T] (tg_stack.hip): built at assembly, one plain
+// FP64 band of K (tg_stack.hip): built at assembly, one plain
 // evaluation per call
 tg_status stack_prepare(tg_ctx* c);
 void stack_destroy(tg_ctx* c);

@@ -227,7 +227,7 @@ static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
     return TG_OK;
 }
 
-// one plain FP64 evaluation: the stacked-band GEMV (tg_stack.hip)
+// one plain FP64 evaluation: one pass over K's band (tg_stack.hip)
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
                              int conserve) {
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");

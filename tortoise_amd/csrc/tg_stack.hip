@@ -1,29 +1,33 @@
-// tg_stack.hip -- plain FP64 collision operator as one streaming GEMV over
-// the stacked band of [K; K^T].
+// tg_stack.hip -- plain FP64 collision operator as one streaming pass over
+// the band of K that yields both products.
 //
-// The products the operator needs are u = K f and v = K^T f (f in energy
-// order).  Both are row products of the stacked matrix S = [K; K^T]
-// (2n x n, energy order; the K^T half starts at the 128-aligned row kt), so
-// every output row is one dot product and no partial sums cross rows:
+// The operator needs u = K f and v = K^T f (f in energy order).  K's band is
+// read once:
 //
-//   - S is stored once, in FP64, band-major: each 128-row panel keeps only the
+//   - K is stored once, in FP64, band-major: each 128-row panel keeps only the
 //     32-column iterations its rows touch (exact FP64 nonzero extents of K's
-//     rows and columns), each iteration as [column 0..31][row 0..127], so one
-//     column of a panel is 1 KB of contiguous memory;
+//     rows), each iteration as [column 0..31][row 0..127], so one column of a
+//     panel is 1 KB of contiguous memory;
 //   - a workgroup takes an item of at most kStackChunk iterations of one
 //     panel; lane l of a wave holds rows 2l and 2l+1 and streams columns
 //     (16 bytes per lane, kStackUnroll loads in flight), the four waves take
-//     the item's columns round robin, and their row sums are combined in LDS
-//     in wave order;
-//   - the finish kernel adds a position's item partials of its K panel (u)
-//     and of its K^T panel (v) in item order, applies the Pauli-blocked
-//     assembly with the member's bath and scatters to the caller's order.
+//     the item's columns round robin;
+//   - u: each lane accumulates its two rows over its columns, and the waves'
+//     row sums are combined in LDS in wave order (one partial per item and
+//     row);
+//   - v: the column's 128 products K[r][c] f[r] are summed across the wave at
+//     once (an 8-column butterfly: three exchange steps halve the columns a
+//     lane holds, three more finish one column's sum), one partial per item
+//     and column; a column is covered by the items of the few panels whose
+//     band reaches it;
+//   - the finish kernel adds a position's row partials (u) and column partials
+//     (v, from a per-iteration list of the items covering it) in item order,
+//     applies the Pauli-blocked assembly with the member's bath and scatters
+//     to the caller's order.
 // Every sum runs in a fixed order, so evaluations are bit-reproducible.
 //
-// It streams 8 bytes per stored band entry of S (about 1.6 times K's own
-// band) and writes no per-column partials; the dual kernel (tg_plain.hip)
-// reads K's band once but reduces column partials across workgroups, and
-// stays in use for products with other matrices (dK/domega0).
+// It streams 8 bytes per stored band entry of K and writes 8 bytes per item
+// row and per item column of partials (about 1% of the band).
 #include <algorithm>
 #include <vector>
 
@@ -36,9 +40,10 @@ constexpr int kStackIt = 32;        // columns per iteration
 constexpr int kStackChunk = 8;      // iterations per work item
 constexpr int kStackThreads = 256;  // four waves
 constexpr int kStackUnroll = 8;     // columns in flight per lane
+static_assert(kStackUnroll == 8, "the column butterfly sums 8 columns at a time");
 
 struct StackData {
-    int64_t kt = 0;                 // first row of the K^T half
+    int64_t kt = 0;                 // n rounded up to whole panels
     int32_t npanel = 0;
     int32_t nitems = 0;
     int64_t elems = 0;              // stored entries of S
@@ -48,7 +53,10 @@ struct StackData {
     int2* kr = nullptr;
     int4* items = nullptr;          // (panel, first iteration, end iteration, 0)
     int2* pitems = nullptr;         // per panel: [first item, end item)
-    double* part = nullptr;         // nitems x 128
+    double* part = nullptr;         // nitems x 128: row partials
+    double* vpart = nullptr;        // nitems x kStackChunk*32: column partials
+    int32_t* cit_off = nullptr;     // per 32-column iteration: [cit_off[I], cit_off[I+1]) ...
+    int32_t* cit = nullptr;         // ... items covering it, in item order
 };
 
 static StackData* sd_of(tg_ctx* c) { return (StackData*)c->stack; }
@@ -85,23 +93,8 @@ __global__ void stack_gather_kernel(const double* __restrict__ f, const int32_t*
     if (p < n) fs[p] = f[perm[p]];
 }
 
-// first / last nonzero row of every column of K (-1 when empty)
-__global__ void col_nz_extent_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                                     int32_t* __restrict__ first, int32_t* __restrict__ last) {
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    int32_t f = -1, l = -1;
-    for (int64_t i = 0; i < n; ++i)
-        if (K[i * ld + j] != 0.0) {
-            if (f < 0) f = (int32_t)i;
-            l = (int32_t)i;
-        }
-    first[j] = f;
-    last[j] = l;
-}
-
-// block (panel, local iteration): entries [column 0..31][row 0..127] of S
-__global__ void stack_pack_kernel(const double* __restrict__ K, int64_t n, int64_t ld, int64_t kt,
+// block (panel, local iteration): entries [column 0..31][row 0..127] of K
+__global__ void stack_pack_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
                                   const int2* __restrict__ kr, const int64_t* __restrict__ off,
                                   double* __restrict__ Sb) {
     const int panel = blockIdx.x;
@@ -113,20 +106,17 @@ __global__ void stack_pack_kernel(const double* __restrict__ K, int64_t n, int64
         const int col = t / kStackRows, row = t % kStackRows;
         const int64_t R = (int64_t)panel * kStackRows + row;
         const int64_t C = (int64_t)(r.x + itl) * kStackIt + col;
-        double x = 0.0;
-        if (C < n) {
-            if (R < n) x = K[R * ld + C];
-            else if (R >= kt && R - kt < n) x = K[C * ld + (R - kt)];
-        }
-        dst[t] = x;
+        dst[t] = (C < n && R < n) ? K[R * ld + C] : 0.0;
     }
 }
 
-// item partials: part[item][row] = sum over the item's columns of S[row][c] fs[c]
+// item partials: part[item][row] = sum over the item's columns of K[row][c] fs[c]
+// and vpart[item][c] = sum over the panel's rows of K[row][c] fs[row]
 __global__ void __launch_bounds__(kStackThreads)
 stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off,
                   const int2* __restrict__ kr, const int4* __restrict__ items, int64_t n,
-                  const double* __restrict__ fs, double* __restrict__ part) {
+                  const double* __restrict__ fs, double* __restrict__ part,
+                  double* __restrict__ vpart) {
     __shared__ double red[4][kStackRows];
     typedef double dbl2 __attribute__((ext_vector_type(2)));
     const int4 it = items[blockIdx.x];
@@ -137,6 +127,10 @@ stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off
     const double* base = Sb + off[panel] + (int64_t)(it.y - r.x) * kStackIt * kStackRows;
     const int ncol = (it.z - it.y) * kStackIt;
     const int64_t col0 = (int64_t)it.y * kStackIt;
+    const int64_t row0 = (int64_t)panel * kStackRows + 2 * lane;
+    const double fr0 = row0 < n ? fs[row0] : 0.0, fr1 = row0 + 1 < n ? fs[row0 + 1] : 0.0;
+    double* vp = vpart + (int64_t)blockIdx.x * (kStackChunk * kStackIt);
+    const int b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1;
     double a0 = 0.0, a1 = 0.0;
     // columns wave, wave + 4, ...: kStackUnroll loads issued before use
     for (int c0 = wave; c0 < ncol; c0 += 4 * kStackUnroll) {
@@ -149,13 +143,31 @@ stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off
             const int64_t C = col0 + c;
             fv[q] = C < n ? fs[C] : 0.0;
         }
+        double t[kStackUnroll];
 #pragma unroll
         for (int q = 0; q < kStackUnroll; ++q) {
             if (c0 + 4 * q < ncol) {
                 a0 += k[q].x * fv[q];
                 a1 += k[q].y * fv[q];
             }
+            t[q] = k[q].x * fr0 + k[q].y * fr1;
         }
+        // column sums over the wave: after the exchanges across lanes 32, 16
+        // and 8 apart, lane l holds column 4 b5 + 2 b4 + b3 summed over its
+        // group of 8 lanes; three more finish the sum over all 64
+        double t4[4], t2[2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            t4[j] = (b5 ? t[j + 4] : t[j]) + __shfl_xor(b5 ? t[j] : t[j + 4], 32, 64);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            t2[j] = (b4 ? t4[j + 2] : t4[j]) + __shfl_xor(b4 ? t4[j] : t4[j + 2], 16, 64);
+        double t1 = (b3 ? t2[1] : t2[0]) + __shfl_xor(b3 ? t2[0] : t2[1], 8, 64);
+        t1 += __shfl_xor(t1, 4, 64);
+        t1 += __shfl_xor(t1, 2, 64);
+        t1 += __shfl_xor(t1, 1, 64);
+        const int q = lane >> 3;
+        if ((lane & 7) == 0 && c0 + 4 * q < ncol) vp[c0 + 4 * q] = t1;
     }
     red[wave][2 * lane] = a0;
     red[wave][2 * lane + 1] = a1;
@@ -171,8 +183,10 @@ stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off
 // for the number-defect removal
 constexpr int kStackFinishT = 256;
 __global__ void __launch_bounds__(kStackFinishT)
-stack_finish_kernel(const double* __restrict__ part, const int2* __restrict__ pitems, int64_t n,
-                    int64_t kt, const int32_t* __restrict__ perm, const double* __restrict__ fs,
+stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
+                    const int2* __restrict__ pitems, const int4* __restrict__ items,
+                    const int32_t* __restrict__ cit_off, const int32_t* __restrict__ cit, int64_t n,
+                    const int32_t* __restrict__ perm, const double* __restrict__ fs,
                     const double* __restrict__ rk, const double* __restrict__ ck,
                     const double* __restrict__ V, double bath_n, double* __restrict__ out,
                     double* __restrict__ dpart) {
@@ -182,9 +196,11 @@ stack_finish_kernel(const double* __restrict__ part, const int2* __restrict__ pi
         double u = 0.0, v = 0.0;
         const int2 iu = pitems[p / kStackRows];
         for (int i = iu.x; i < iu.y; ++i) u += part[(int64_t)i * kStackRows + p % kStackRows];
-        const int64_t q = kt + p;
-        const int2 iv = pitems[q / kStackRows];
-        for (int i = iv.x; i < iv.y; ++i) v += part[(int64_t)i * kStackRows + q % kStackRows];
+        const int I = (int)(p / kStackIt);
+        for (int k = cit_off[I]; k < cit_off[I + 1]; ++k) {
+            const int i = cit[k];
+            v += vpart[(int64_t)i * (kStackChunk * kStackIt) + (I - items[i].y) * kStackIt + p % kStackIt];
+        }
         const double np1 = bath_n + 1.0;
         const double fi = fs[p];
         const double wf = np1 * u + bath_n * v;
@@ -218,36 +234,28 @@ stack_finish_kernel(const double* __restrict__ part, const int2* __restrict__ pi
     }
 }
 
-// builds the stacked band from the assembled K (energy order): extents,
-// band-major copy, work items
+// builds K's band from the assembled K (energy order): row extents,
+// band-major copy, work items and the items covering each column iteration
 tg_status stack_prepare(tg_ctx* c) {
     if (c->stack) return TG_OK;
     StackData* s = new StackData();
     c->stack = s;
     const int64_t n = c->n;
     s->kt = (n + kStackRows - 1) / kStackRows * kStackRows;
-    s->npanel = (int32_t)(2 * s->kt / kStackRows);
+    s->npanel = (int32_t)(s->kt / kStackRows);
     hipStream_t st = c->stream;
-    int32_t *d_rf = nullptr, *d_rl = nullptr, *d_cf = nullptr, *d_cl = nullptr;
+    int32_t *d_rf = nullptr, *d_rl = nullptr;
     TG_TRY(dev_alloc(c, &d_rf, (size_t)n));
     TG_TRY(dev_alloc(c, &d_rl, (size_t)n));
-    TG_TRY(dev_alloc(c, &d_cf, (size_t)n));
-    TG_TRY(dev_alloc(c, &d_cl, (size_t)n));
     hipLaunchKernelGGL(stack_row_extent_kernel, dim3(n), dim3(256), 0, st, c->d_K, n, c->ld, d_rf, d_rl);
-    hipLaunchKernelGGL(col_nz_extent_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->d_K, n, c->ld,
-                       d_cf, d_cl);
     TG_CHECK_LAUNCH(c);
-    std::vector<int32_t> rf(n), rl(n), cf(n), cl(n);
+    std::vector<int32_t> rf(n), rl(n);
     TG_HIP(c, hipMemcpyAsync(rf.data(), d_rf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     TG_HIP(c, hipMemcpyAsync(rl.data(), d_rl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipMemcpyAsync(cf.data(), d_cf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipMemcpyAsync(cl.data(), d_cl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     TG_HIP(c, hipStreamSynchronize(st));
     dev_free(&d_rf);
     dev_free(&d_rl);
-    dev_free(&d_cf);
-    dev_free(&d_cl);
-    // panel extents in iterations; a K^T row kt + j is column j of K
+    // panel extents in iterations
     s->h_kr.assign(s->npanel, int2{0, 0});
     std::vector<int64_t> off(s->npanel, 0);
     std::vector<int4> items;
@@ -258,17 +266,9 @@ tg_status stack_prepare(tg_ctx* c) {
         int32_t lo = INT32_MAX, hi = -1;
         for (int r = 0; r < kStackRows; ++r) {
             const int64_t R = (int64_t)pn * kStackRows + r;
-            int32_t f = -1, l = -1;
-            if (R < n) {
-                f = rf[R];
-                l = rl[R];
-            } else if (R >= s->kt && R - s->kt < n) {
-                f = cf[R - s->kt];
-                l = cl[R - s->kt];
-            }
-            if (f >= 0) {
-                lo = std::min(lo, f);
-                hi = std::max(hi, l);
+            if (R < n && rf[R] >= 0) {
+                lo = std::min(lo, rf[R]);
+                hi = std::max(hi, rl[R]);
             }
         }
         int2 k{0, 0};
@@ -283,6 +283,18 @@ tg_status stack_prepare(tg_ctx* c) {
         pitems[pn].y = (int32_t)items.size();
     }
     if (items.empty()) items.push_back(int4{0, 0, 0, 0});
+    // items covering each column iteration, in item order
+    const int64_t nit = (n + kStackIt - 1) / kStackIt;
+    std::vector<int32_t> cit_off(nit + 1, 0), cit;
+    for (const int4& q : items)
+        for (int I = q.y; I < q.z; ++I) ++cit_off[I + 1];
+    for (int64_t I = 0; I < nit; ++I) cit_off[I + 1] += cit_off[I];
+    cit.assign(std::max<int32_t>(cit_off[nit], 1), 0);
+    {
+        std::vector<int32_t> fill(cit_off.begin(), cit_off.end() - 1);
+        for (int32_t i = 0; i < (int32_t)items.size(); ++i)
+            for (int I = items[i].y; I < items[i].z; ++I) cit[fill[I]++] = i;
+    }
     s->elems = elems;
     s->nitems = (int32_t)items.size();
     c->band_elems = elems;
@@ -292,13 +304,18 @@ tg_status stack_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &s->items, items.size()));
     TG_TRY(dev_alloc(c, &s->pitems, (size_t)s->npanel));
     TG_TRY(dev_alloc(c, &s->part, (size_t)s->nitems * kStackRows));
+    TG_TRY(dev_alloc(c, &s->vpart, (size_t)s->nitems * kStackChunk * kStackIt));
+    TG_TRY(dev_alloc(c, &s->cit_off, (size_t)nit + 1));
+    TG_TRY(dev_alloc(c, &s->cit, cit.size()));
     TG_HIP(c, hipMemcpyAsync(s->off, off.data(), sizeof(int64_t) * s->npanel, hipMemcpyHostToDevice, st));
     TG_HIP(c, hipMemcpyAsync(s->kr, s->h_kr.data(), sizeof(int2) * s->npanel, hipMemcpyHostToDevice, st));
     TG_HIP(c, hipMemcpyAsync(s->items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, st));
     TG_HIP(c, hipMemcpyAsync(s->pitems, pitems.data(), sizeof(int2) * s->npanel, hipMemcpyHostToDevice, st));
+    TG_HIP(c, hipMemcpyAsync(s->cit_off, cit_off.data(), sizeof(int32_t) * (nit + 1), hipMemcpyHostToDevice, st));
+    TG_HIP(c, hipMemcpyAsync(s->cit, cit.data(), sizeof(int32_t) * cit.size(), hipMemcpyHostToDevice, st));
     if (maxit > 0) {
         hipLaunchKernelGGL(stack_pack_kernel, dim3(s->npanel, maxit), dim3(256), 0, st, c->d_K, n, c->ld,
-                           s->kt, s->kr, s->off, s->Sb);
+                           s->kr, s->off, s->Sb);
         TG_CHECK_LAUNCH(c);
     }
     TG_HIP(c, hipStreamSynchronize(st));
@@ -314,6 +331,9 @@ void stack_destroy(tg_ctx* c) {
     dev_free(&s->items);
     dev_free(&s->pitems);
     dev_free(&s->part);
+    dev_free(&s->vpart);
+    dev_free(&s->cit_off);
+    dev_free(&s->cit);
     delete s;
     c->stack = nullptr;
 }
@@ -321,14 +341,14 @@ void stack_destroy(tg_ctx* c) {
 // one plain FP64 evaluation (caller order in and out); kernel launches only
 tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve) {
     StackData* s = sd_of(c);
-    if (!s) return set_error(c, TG_ERR_STATE, "stacked band not prepared");
+    if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, d_f, c->d_perm,
                        c->n, c->d_fs);
     TG_CHECK_LAUNCH(c);
     time_begin(c);
     hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, s->Sb, s->off, s->kr,
-                       s->items, c->n, c->d_fs, s->part);
+                       s->items, c->n, c->d_fs, s->part, s->vpart);
     time_end(c);
     TG_CHECK_LAUNCH(c);
     const unsigned nb = ceil_div(c->n, kStackFinishT);
@@ -341,8 +361,9 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
         }
         dpart = c->d_red;
     }
-    hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, s->part, s->pitems, c->n,
-                       s->kt, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s, bath_n, d_I, dpart);
+    hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, s->part, s->vpart, s->pitems,
+                       s->items, s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s,
+                       bath_n, d_I, dpart);
     TG_CHECK_LAUNCH(c);
     if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I));
     return TG_OK;

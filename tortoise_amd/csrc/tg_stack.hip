@@ -56,7 +56,8 @@ struct StackData {
     double* part = nullptr;         // nitems x 128: row partials
     double* vpart = nullptr;        // nitems x kStackChunk*32: column partials
     int32_t* cit_off = nullptr;     // per 32-column iteration: [cit_off[I], cit_off[I+1]) ...
-    int32_t* cit = nullptr;         // ... items covering it, in item order
+    int32_t* cit = nullptr;         // ... offsets in vpart of the iteration's first column in the
+                                    //     items covering it, in item order
 };
 
 static StackData* sd_of(tg_ctx* c) { return (StackData*)c->stack; }
@@ -181,11 +182,11 @@ stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off
 // per sorted position: u, v from the item partials (item order), Pauli
 // assembly, scatter to the caller's order; block sums of V I and V |f(1-f)|
 // for the number-defect removal
-constexpr int kStackFinishT = 256;
+constexpr int kStackFinishT = 64;
 __global__ void __launch_bounds__(kStackFinishT)
 stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
-                    const int2* __restrict__ pitems, const int4* __restrict__ items,
-                    const int32_t* __restrict__ cit_off, const int32_t* __restrict__ cit, int64_t n,
+                    const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
+                    const int32_t* __restrict__ cit, int64_t n,
                     const int32_t* __restrict__ perm, const double* __restrict__ fs,
                     const double* __restrict__ rk, const double* __restrict__ ck,
                     const double* __restrict__ V, double bath_n, double* __restrict__ out,
@@ -193,13 +194,30 @@ stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ 
     const int64_t p = blockIdx.x * (int64_t)kStackFinishT + threadIdx.x;
     double dI = 0.0, dw = 0.0;
     if (p < n) {
+        // partials in item order, four loads in flight at a time
         double u = 0.0, v = 0.0;
         const int2 iu = pitems[p / kStackRows];
-        for (int i = iu.x; i < iu.y; ++i) u += part[(int64_t)i * kStackRows + p % kStackRows];
+        for (int i = iu.x; i < iu.y; i += 4) {
+            double x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                x[j] = i + j < iu.y ? part[(int64_t)(i + j) * kStackRows + p % kStackRows] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (i + j < iu.y) u += x[j];
+        }
         const int I = (int)(p / kStackIt);
-        for (int k = cit_off[I]; k < cit_off[I + 1]; ++k) {
-            const int i = cit[k];
-            v += vpart[(int64_t)i * (kStackChunk * kStackIt) + (I - items[i].y) * kStackIt + p % kStackIt];
+        const int k1 = cit_off[I + 1];
+        for (int k = cit_off[I]; k < k1; k += 4) {
+            int o[4];
+            double x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[j] = k + j < k1 ? cit[k + j] : -1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[j] = o[j] >= 0 ? vpart[(int64_t)o[j] + p % kStackIt] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k + j < k1) v += x[j];
         }
         const double np1 = bath_n + 1.0;
         const double fi = fs[p];
@@ -293,7 +311,8 @@ tg_status stack_prepare(tg_ctx* c) {
     {
         std::vector<int32_t> fill(cit_off.begin(), cit_off.end() - 1);
         for (int32_t i = 0; i < (int32_t)items.size(); ++i)
-            for (int I = items[i].y; I < items[i].z; ++I) cit[fill[I]++] = i;
+            for (int I = items[i].y; I < items[i].z; ++I)
+                cit[fill[I]++] = i * (kStackChunk * kStackIt) + (I - items[i].y) * kStackIt;
     }
     s->elems = elems;
     s->nitems = (int32_t)items.size();
@@ -362,7 +381,7 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
         dpart = c->d_red;
     }
     hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, s->part, s->vpart, s->pitems,
-                       s->items, s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s,
+                       s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s,
                        bath_n, d_I, dpart);
     TG_CHECK_LAUNCH(c);
     if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I));

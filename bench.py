@@ -186,6 +186,67 @@ def cube_ee(steps=5, t_final=1.0):
                                 "forward_s": fwd, "adjoint_s": adj, "dL_dg": dLdg}}
 
 
+def square64_gradients(T=0.02):
+    """The differentiation workload: 64x64 sections (4225 nodes), target f(50)
+    generated at g = 0.1, omega0 = 0.05; from g = 0.12, omega0 = 0.045 the
+    gradients of L = |Wb f(50) - target|^2 (Wb: 40 energy bins of width 0.05,
+    node volumes as weights) wrt g, omega0 and f0 through the adjoint of the
+    accepted steps (checkpoints every 32), with dL/dg checked against a
+    central difference (h = 1e-6 g) of the same fixed-step map."""
+    import numpy as np
+
+    import tortoise_amd as tg
+    from tortoise_amd import autograd as tga
+    from tortoise_amd import workloads as wl
+
+    w = wl.SQUARE64
+    t1, dt = 50.0, 0.5
+    nd = int(round(t1 / dt))
+
+    def ctx_at(g, om):
+        c = tg.Context.create(w, max_batch=1, g=g, omega0=om, bath_n=tga.bath_occupation(om, T))
+        c.assemble()
+        return c
+
+    f0 = w.initial_f()
+    tgt_ctx = ctx_at(0.1, 0.05)
+    _, dense_t, _ = tgt_ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd)
+    V = tgt_ctx.node_volumes()
+    eps = w.eps()
+    edges = np.linspace(0.0, 2.0, 41)
+    Wb = np.stack([np.where((eps >= a) & (eps < b), V, 0.0) for a, b in zip(edges[:-1], edges[1:])])
+    target = Wb @ dense_t[-1]
+    tgt_ctx.close()
+    g0, w0 = 0.12, 0.045
+    ctx = ctx_at(g0, w0)
+    t0 = time.perf_counter()
+    _, dense, st, steps = ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd, h_cap=1 << 14)
+    fwd = time.perf_counter() - t0
+    d = Wb @ dense[-1] - target
+    t0 = time.perf_counter()
+    g_f0, g_g, g_w = ctx.dp5_adjoint(f0, 0.0, t1, dt, steps, 2.0 * (Wb.T @ d),
+                                     tga.bath_occupation(w0, T), tga.dbath_domega0(w0, T), ckpt=32)
+    adj = time.perf_counter() - t0
+    ctx.close()
+
+    def L_at(g):
+        c = ctx_at(g, w0)
+        _, dn = c.dp5_replay(f0, 0.0, t1, dt, steps, max_dense=nd)
+        c.close()
+        r = Wb @ dn[-1] - target
+        return float(r @ r)
+
+    hg = 1e-6 * g0
+    fd_g = (L_at(g0 + hg) - L_at(g0 - hg)) / (2 * hg)
+    rel = abs(g_g - fd_g) / abs(fd_g)
+    _log(f"square64 gradients: FP64 DP5 to t={t1} in {fwd:.2f} s ({st['accepted']} steps), adjoint "
+         f"(g, omega0, f0; checkpoints every 32) in {adj:.2f} s; dL/dg = {g_g:.6e} (central FD "
+         f"rel diff {rel:.1e}), dL/domega0 = {g_w:.6e}")
+    return {"workload": "square64_gradients", "nodes": int(eps.size), "steps": int(st["accepted"]),
+            "forward_s": fwd, "adjoint_s": adj, "dL_dg": g_g, "dL_domega0": g_w,
+            "dL_df0_norm": float(np.linalg.norm(g_f0)), "dL_dg_rel_diff_vs_central_fd": rel}
+
+
 def reference_operator(budget_s=12.0):
     """The reference's own operator (tuple list + MC delta-shell integral,
     NMC = 50, umklapp {-1,0,1}^2) on the square zone: 32x32 sections (the
@@ -347,6 +408,8 @@ def main():
                     help="only the reference tuple operator measurements (profiling runs)")
     ap.add_argument("--only-ee", action="store_true",
                     help="only the 20^3 e-ph + e-e ensemble line (profiling runs)")
+    ap.add_argument("--only-gradients", action="store_true",
+                    help="only the 64x64 gradient line (adjoint of DP5)")
     ap.add_argument("--ee-t-final", type=float, default=1.0,
                     help="DP5 end time of the 20^3 e-e ensemble run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -370,6 +433,10 @@ def main():
     if args.only_reference_op:
         torch.cuda.init()
         print(json.dumps({"reference_operator": reference_operator()}), flush=True)
+        return
+    if args.only_gradients:
+        torch.cuda.init()
+        print(json.dumps({"square64_gradients": square64_gradients()}), flush=True)
         return
     if args.only_ee:
         torch.cuda.init()
@@ -486,6 +553,7 @@ def main():
     ctx.close()
     if not args.no_dp5:
         out["cube20_ee"] = cube_ee(t_final=args.ee_t_final)
+        out["square64_gradients"] = square64_gradients()
     if not args.no_reference_op:
         out["reference_operator"] = reference_operator()
     print(json.dumps(out), flush=True)

@@ -354,6 +354,22 @@ def reference_operator(budget_s=12.0):
             line["gpu_vs_cpu_b8"] = line["b8"]["evals_per_s"] * med
             _log(f"  CPU reference algorithm {med:.3f} s/eval on {cores} threads; GPU/CPU "
                  f"{line['gpu_vs_cpu_b8']:.0f}x (8 members); max rel diff {line['max_rel_diff_vs_cpu']:.1e}")
+            # the reference's CPU-runnable propagation: DP5 to t = 100 (rtol 1e-8,
+            # atol 1e-12, dense step 1), one member, with the tuple operator and
+            # with the W-form on the same mesh
+            t0 = time.perf_counter()
+            _, _, stt = op.dp5(base, 0.0, 100.0, 1.0, 1e-8, 1e-12, bath_n=w.bath_n)
+            t_tuple = time.perf_counter() - t0
+            cw = tg.Context.create(wl.SQUARE32, max_batch=1)
+            cw.assemble()
+            t0 = time.perf_counter()
+            _, _, stw = cw.dp5(wl.SQUARE32.initial_f(), 0.0, 100.0, 1.0, 1e-8, 1e-12)
+            t_wform = time.perf_counter() - t0
+            cw.close()
+            line["dp5_t100"] = {"tuple_operator": {"wall_s": t_tuple, "attempts": int(stt["attempted"])},
+                                "w_form": {"wall_s": t_wform, "attempts": int(stw["attempted"])}}
+            _log(f"  DP5 to t=100 (rtol 1e-8): tuple operator {t_tuple:.2f} s ({stt['attempted']} "
+                 f"attempts), W-form {t_wform:.3f} s ({stw['attempted']} attempts)")
         op.close()
         res[f"square{ns}"] = line
     return res

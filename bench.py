@@ -436,7 +436,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(members shard across GPUs without per-stage communication, see DESIGN.md)",
+              "(ensemble members would shard across GPUs with no per-stage communication)",
               file=sys.stderr)
         sys.exit(2)
 

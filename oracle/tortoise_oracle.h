@@ -141,7 +141,7 @@ void or_dp5_collision(int dim, const int32_t* nsec, int64_t n, const double* K,
 /* Fixed-step replay of a recorded attempt sequence (h_log/acc_log): the same
  * stage arithmetic as or_dp5_integrate with the controller taken out, so a
  * second precision can be compared on identical steps.  Writes the state
- * after the last accepted step to y and dense outputs like the driver. */
+ * after the last accepted step to y and dense outputs like or_dp5_integrate. */
 void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
                              const double* K, const double* V, double bath_n,
                              double* f, double t0, double t_final,

@@ -11,8 +11,9 @@
 //   mu^T dF/dg      = (2/g) mu^T F(Y)            (K scales as g^2)
 //   mu^T dF/domega0 = mu^T F[W'](Y),  W' = (n+1) K' + n K'^T + n' (K + K^T),
 // with K' = dK/domega0 (same windows, sampled G') and n' = dn/domega0 given
-// by the caller.  Every product is one pass of the FP64 one-pass kernel over
-// K or K' (K Y, K^T Y in one read).  With the e-e channel on, its operator
+// by the caller.  Every product with K is one pass over K's band
+// (tg_stack.hip), every product with K' one pass of the dual kernel over its
+// band (K Y and K^T Y in one read either way).  With the e-e channel on, its operator
 // joins the forward stages and its vector-Jacobian product joins J^T mu
 // (tg_ee.hip); it depends on neither g nor omega0.  Forward states are
 // recomputed from checkpoints every `ckpt` steps; only the step-start states
@@ -230,12 +231,12 @@ struct AdjRun {
     // nu = J(Y)^T mu, accumulate the parameter products
     tg_status vjp(const double* Yst, const double* mu) {
         const int64_t n = c->n;
-        TG_TRY(plain_products_device(c, c->d_K, Yst, w->uf, w->vf));
+        TG_TRY(stack_products_device(c, Yst, w->uf, w->vf));
         hipLaunchKernelGGL(vjp_prep_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, Yst,
                            c->d_V, w->x, w->y);
         TG_CHECK_LAUNCH(c);
-        TG_TRY(plain_products_device(c, c->d_K, w->x, w->ux, w->vx));
-        TG_TRY(plain_products_device(c, c->d_K, w->y, w->uy, w->vy));
+        TG_TRY(stack_products_device(c, w->x, w->ux, w->vx));
+        TG_TRY(stack_products_device(c, w->y, w->uy, w->vy));
         hipLaunchKernelGGL(vjp_out_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, c->d_V, nb,
                            w->uf, w->vf, c->d_rk, c->d_ck, w->ux, w->vx, w->uy, w->vy, w->nu);
         TG_CHECK_LAUNCH(c);

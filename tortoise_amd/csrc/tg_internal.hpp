@@ -183,6 +183,8 @@ tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_
 // FP64 band of K (tg_stack.hip): built at assembly, one plain
 // evaluation per call
 tg_status stack_prepare(tg_ctx* c);
+// u = K f, v = K^T f (caller order) from the same pass
+tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
 void stack_destroy(tg_ctx* c);
 tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve);
 // I -= d |f(1-f)| with d from the block sums dpart (number-defect removal)

@@ -183,6 +183,54 @@ stack_gemv_kernel(const double* __restrict__ Sb, const int64_t* __restrict__ off
 // assembly, scatter to the caller's order; block sums of V I and V |f(1-f)|
 // for the number-defect removal
 constexpr int kStackFinishT = 64;
+
+// u = (K fs)[p], v = (K^T fs)[p] from the item partials, in item order, four
+// loads in flight at a time
+__device__ inline void stack_sums(const double* __restrict__ part, const double* __restrict__ vpart,
+                                  const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
+                                  const int32_t* __restrict__ cit, int64_t p, double& u, double& v) {
+    u = 0.0;
+    v = 0.0;
+    const int2 iu = pitems[p / kStackRows];
+    for (int i = iu.x; i < iu.y; i += 4) {
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            x[j] = i + j < iu.y ? part[(int64_t)(i + j) * kStackRows + p % kStackRows] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i + j < iu.y) u += x[j];
+    }
+    const int I = (int)(p / kStackIt);
+    const int k1 = cit_off[I + 1];
+    for (int k = cit_off[I]; k < k1; k += 4) {
+        int o[4];
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = k + j < k1 ? cit[k + j] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = o[j] >= 0 ? vpart[(int64_t)o[j] + p % kStackIt] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) v += x[j];
+    }
+}
+
+// the products alone, in the caller's order: u[perm[p]], v[perm[p]]
+__global__ void __launch_bounds__(256)
+stack_products_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
+                      const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
+                      const int32_t* __restrict__ cit, int64_t n, const int32_t* __restrict__ perm,
+                      double* __restrict__ u_out, double* __restrict__ v_out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double u, v;
+    stack_sums(part, vpart, pitems, cit_off, cit, p, u, v);
+    const int32_t node = perm[p];
+    u_out[node] = u;
+    v_out[node] = v;
+}
+
 __global__ void __launch_bounds__(kStackFinishT)
 stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
                     const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
@@ -194,31 +242,8 @@ stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ 
     const int64_t p = blockIdx.x * (int64_t)kStackFinishT + threadIdx.x;
     double dI = 0.0, dw = 0.0;
     if (p < n) {
-        // partials in item order, four loads in flight at a time
-        double u = 0.0, v = 0.0;
-        const int2 iu = pitems[p / kStackRows];
-        for (int i = iu.x; i < iu.y; i += 4) {
-            double x[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                x[j] = i + j < iu.y ? part[(int64_t)(i + j) * kStackRows + p % kStackRows] : 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (i + j < iu.y) u += x[j];
-        }
-        const int I = (int)(p / kStackIt);
-        const int k1 = cit_off[I + 1];
-        for (int k = cit_off[I]; k < k1; k += 4) {
-            int o[4];
-            double x[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) o[j] = k + j < k1 ? cit[k + j] : -1;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) x[j] = o[j] >= 0 ? vpart[(int64_t)o[j] + p % kStackIt] : 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (k + j < k1) v += x[j];
-        }
+        double u, v;
+        stack_sums(part, vpart, pitems, cit_off, cit, p, u, v);
         const double np1 = bath_n + 1.0;
         const double fi = fs[p];
         const double wf = np1 * u + bath_n * v;
@@ -355,6 +380,24 @@ void stack_destroy(tg_ctx* c) {
     dev_free(&s->cit);
     delete s;
     c->stack = nullptr;
+}
+
+// u = K f, v = K^T f (caller order in and out) from one pass over K's band;
+// kernel launches only
+tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v) {
+    StackData* s = sd_of(c);
+    if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, d_f, c->d_perm,
+                       c->n, c->d_fs);
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, s->Sb, s->off, s->kr,
+                       s->items, c->n, c->d_fs, s->part, s->vpart);
+    TG_CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(stack_products_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, s->part, s->vpart,
+                       s->pitems, s->cit_off, s->cit, c->n, c->d_perm, d_u, d_v);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
 }
 
 // one plain FP64 evaluation (caller order in and out); kernel launches only

@@ -127,6 +127,10 @@ typedef struct tg_kernel_stats {
     float   mixed_tuned_ms; /* autotuner's time of the chosen configuration (0: set explicitly) */
     int32_t mixed_persist;  /* persistent kernel with a balanced panel schedule, A fragments straight
                                into registers (1) or A tile through LDS (2); 0: tiled grid */
+    int32_t mixed_skip;     /* 1: panels whose fp16 operands are zero for every member are left out of
+                               the persistent kernel (their products are exactly 0) */
+    int64_t mixed_last_band;/* entries of [K; K^T] the last batched evaluation streamed (its panels
+                               with work; mixed_band when nothing was left out) */
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

@@ -268,3 +268,39 @@ def test_mixed_dp5_error_vs_fp64_on_same_steps(sq32_mixed):
         assert _rel(dm[b, -1], dr[b, -1]) <= 1e-6, (b, _rel(dm[b, -1], dr[b, -1]))
         n0 = np.dot(V, F[:, b])
         assert abs(np.dot(V, dm[b, -1]) - n0) <= 1e-12 * n0
+
+
+def test_mixed_zero_operand_panels_left_out_exactly(monkeypatch):
+    """Panels whose fp16 operands are zero for every member (the empty top of
+    cold distributions, where f is below the split's range) are left out of
+    the persistent GEMM and read as 0: the results are bit for bit those with
+    every panel computed, on the cold 256-member ensemble (most panels left
+    out) and on hot members (none left out), and the batched DP5 takes the
+    same steps."""
+    w = wl.SQUARE128_ENSEMBLE
+    F, bath = _members(w, 256, seed=0)
+    eps = w.eps()
+    H = np.stack([1.0 / (np.exp((eps - 3.0) / (1.0 + 0.01 * b)) + 1.0) for b in range(64)], axis=1)
+    for k, v in [("TG_MIXED_PERSIST", "1"), ("TG_MIXED_NW", "128"), ("TG_MIXED_WAVES", "8"),
+                 ("TG_MIXED_FOLD", "2"), ("TG_MIXED_DEPTH", "2"), ("TG_MIXED_AL", "1"),
+                 ("TG_MIXED_AUTOTUNE", "0")]:
+        monkeypatch.setenv(k, v)
+    out = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("TG_MIXED_SKIP", skip)
+        print(f"[gpu] mixed square128 B=256, zero-operand panels skipped={skip}", flush=True)
+        ctx = tg.Context.create(w, max_batch=256)
+        ctx.assemble(mixed=True)
+        Ic = ctx.apply(F, bath_n=bath, mixed=True)
+        st_cold = ctx.kernel_stats()
+        Ih = ctx.apply(H, bath_n=bath[:64], mixed=True)
+        st_hot = ctx.kernel_stats()
+        _, _, dst = ctx.dp5(F[:, :32], 0.0, 20.0, 0.2, 1e-6, 1e-12, bath_n=bath[:32], mixed=True)
+        out[skip] = (Ic, Ih, [s["attempted"] for s in dst], st_cold, st_hot)
+        ctx.close()
+    a, b = out["1"], out["0"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    assert a[3]["mixed_skip"] == 1 and b[3]["mixed_skip"] == 0
+    # the cold ensemble leaves most of the band out, the hot members none of it
+    assert a[3]["mixed_last_band"] < 0.6 * a[3]["mixed_band"], a[3]
+    assert a[4]["mixed_last_band"] == a[4]["mixed_band"], a[4]

@@ -92,6 +92,19 @@ struct MixedData {
     int bins_per_xcd = 0;
     int32_t* sched_off = nullptr;    // per bin: first entry in sched_panel (nbins + 1)
     int32_t* sched_panel = nullptr;  // panels in bin order
+    // zero-operand panels: which iterations hold a nonzero f (g) operand of a
+    // live member, which panels some member runs with f (g); from them the
+    // panels with work and their schedule, rebuilt on the device per evaluation
+    bool skip = true;
+    int32_t nit = 0;                 // iterations of 32 k over the padded k range
+    uint8_t* nzF = nullptr;          // nit
+    uint8_t* nzG = nullptr;          // nit
+    uint8_t* useF = nullptr;         // npanel
+    uint8_t* useG = nullptr;         // npanel
+    int32_t* lpt_order = nullptr;    // panels by decreasing band length
+    int32_t* dsched_off = nullptr;   // nbins + 1
+    int32_t* dsched_panel = nullptr; // npanel
+    int64_t* dstat = nullptr;        // iterations the last schedule holds
     float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
@@ -277,7 +290,9 @@ __global__ void __launch_bounds__(kMemT)
 pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
                    int64_t kgroups, int64_t kt, int npanel, int nt, u32x4* __restrict__ Fhi,
                    u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo,
-                   uint8_t* __restrict__ useg, const double* __restrict__ gate) {
+                   uint8_t* __restrict__ useg, const double* __restrict__ gate,
+                   uint8_t* __restrict__ nzF, uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF,
+                   uint8_t* __restrict__ useG) {
     extern __shared__ double row[];
     const int b = member_of_block(blockIdx.x, nt);
     if (b >= nt) return;
@@ -317,10 +332,18 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
         // four neighbouring threads (the k-groups of one iteration) write 64
         // contiguous bytes of each operand
         const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
-        Fhi[u] = __builtin_bit_cast(u32x4, fh);
-        Flo[u] = __builtin_bit_cast(u32x4, fl);
-        Ghi[u] = __builtin_bit_cast(u32x4, gh);
-        Glo[u] = __builtin_bit_cast(u32x4, gl);
+        const u32x4 fh4 = __builtin_bit_cast(u32x4, fh), fl4 = __builtin_bit_cast(u32x4, fl);
+        const u32x4 gh4 = __builtin_bit_cast(u32x4, gh), gl4 = __builtin_bit_cast(u32x4, gl);
+        Fhi[u] = fh4;
+        Flo[u] = fl4;
+        Ghi[u] = gh4;
+        Glo[u] = gl4;
+        // iterations holding a nonzero operand (every writer stores the same 1)
+        if (nzF) {
+            const u32x4 fo = fh4 | fl4, go = gh4 | gl4;
+            if ((fo.x | fo.y | fo.z | fo.w) != 0u) nzF[kg / kKG] = 1;
+            if ((go.x | go.y | go.z | go.w) != 0u) nzG[kg / kKG] = 1;
+        }
     };
     if constexpr (ROW) {
         // The node indices of this thread's k-groups and of its panel load
@@ -340,15 +363,22 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
             stage_member_row(fb, n, row);
             lds_barrier();
         }
-        if (pn < npanel) useg[(int64_t)pn * nt + b] = live && row[pnode] > 0.5 ? 1 : 0;
+        if (pn < npanel) {
+            const uint8_t v = live && row[pnode] > 0.5 ? 1 : 0;
+            useg[(int64_t)pn * nt + b] = v;
+            if (useF && live) (v ? useG : useF)[pn] = 1;
+        }
 #pragma unroll
         for (int j = 0; j < kPackKG; ++j) {
             const int64_t kg = (int64_t)threadIdx.x + (int64_t)j * kMemT;
             if (kg < kgroups) pack_group(kg, node[j]);
         }
     } else {
-        for (int pn = threadIdx.x; pn < npanel; pn += kMemT)
-            useg[(int64_t)pn * nt + b] = live && fb[panel_node(pn)] > 0.5 ? 1 : 0;
+        for (int pn = threadIdx.x; pn < npanel; pn += kMemT) {
+            const uint8_t v = live && fb[panel_node(pn)] > 0.5 ? 1 : 0;
+            useg[(int64_t)pn * nt + b] = v;
+            if (useF && live) (v ? useG : useF)[pn] = 1;
+        }
         for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
             int32_t node[8];
 #pragma unroll
@@ -814,8 +844,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     // first round's loads needs all of them)
     constexpr int UNR = 6;
     double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
-    int32_t node[UNR];
-    bool gu[UNR], gv[UNR];
+    int32_t node[UNR], gu[UNR], gv[UNR];   // gu, gv: the panels' operand bytes
     // the partial sums (streamed from HBM) ...
     auto load_products = [&](int64_t p0) {
 #pragma unroll
@@ -829,6 +858,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                 u[j] += (double)Ps[p];
                 v[j] += (double)Ps[kt + p];
             }
+
         }
     };
     // ... and the per-position constants every member shares (L2-resident)
@@ -836,8 +866,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
-            gu[j] = useg[(p / kMT) * (int64_t)nt + b] != 0;
-            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b] != 0;
+            gu[j] = useg[(p / kMT) * (int64_t)nt + b];
+            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b];
             rkp[j] = rk[p];
             ckp[j] = ck[p];
             Vp[j] = V[p];
@@ -855,10 +885,14 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             if (p0 + (int64_t)j * kMemT >= n) break;
-            const double uu = u[j] * inv_scale, vv = v[j] * inv_scale;
+            // operand byte: bit 0 = g (z = 1 - f) rather than f, bit 1 = the panel
+            // was left out of the GEMM (all operands zero: its product is 0)
+            const double uu = (gu[j] & 2) ? 0.0 : u[j] * inv_scale;
+            const double vv = (gv[j] & 2) ? 0.0 : v[j] * inv_scale;
             // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
-            const double Kf = gu[j] ? rkp[j] - uu : uu, Kg = gu[j] ? uu : rkp[j] - uu;
-            const double KTf = gv[j] ? ckp[j] - vv : vv, KTg = gv[j] ? vv : ckp[j] - vv;
+            const bool gfu = gu[j] & 1, gfv = gv[j] & 1;
+            const double Kf = gfu ? rkp[j] - uu : uu, Kg = gfu ? uu : rkp[j] - uu;
+            const double KTf = gfv ? ckp[j] - vv : vv, KTg = gfv ? vv : ckp[j] - vv;
             const double x = ROW ? row[node[j]] : fb[node[j]];
             // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
             const double Ii = ((1.0 - x) * (np1 * Kf + nb * KTf) - x * (np1 * KTg + nb * Kg)) / Vp[j];
@@ -930,6 +964,129 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
         }
         if (threadIdx.x == 0 && h) Ib[0] = out(0, ROW ? row[0] : Ib[0]);
         if (threadIdx.x == kMemT - 1 && h + 2 * np < n) Ib[n - 1] = out(n - 1, ROW ? row[n - 1] : Ib[n - 1]);
+    }
+}
+
+// ------------------------------------------------------- panels with work
+// A panel whose k range holds, for every member, only zero fp16 operand
+// units (f and 1 - f below the split's range, e.g. the empty top and the
+// filled bottom of a cold distribution) has C = A Z = 0 exactly: it is left
+// out of the GEMM and its products read as 0.  One workgroup decides which
+// panels have work (some iteration of its k range holds a nonzero operand
+// of a kind some member runs it with), deals them in decreasing band length
+// to the bins in snake order (the k-th panel of pass s to bin k on even
+// passes, nbins - 1 - k on odd ones: every complete pass gives each bin one
+// panel, so a bin's slots and offsets have closed forms), and clears the
+// flags for the next evaluation.  Dynamic LDS: npanel ints.
+constexpr int kSchedT = 1024;
+__global__ void __launch_bounds__(kSchedT)
+active_schedule_kernel(int npanel, int nbins, int nit, int nt, const int2* __restrict__ krange,
+                       const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
+                       uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF, uint8_t* __restrict__ useG,
+                       uint8_t* __restrict__ useg, int32_t* __restrict__ sched_off,
+                       int32_t* __restrict__ sched_panel, int64_t* __restrict__ stat) {
+    extern __shared__ int32_t flag[];   // per position of `order`: 1 if that panel has work,
+                                        // then the total; then per panel (actp); then the
+                                        // iteration flags (sF, sG, bytes)
+    int32_t* actp = flag + npanel + 1;
+    uint8_t* sF = reinterpret_cast<uint8_t*>(actp + npanel);
+    uint8_t* sG = sF + nit;
+    __shared__ int32_t part[kSchedT];
+    __shared__ unsigned long long iters;
+    const int tid = threadIdx.x;
+    if (tid == 0) iters = 0;
+    for (int i = tid; i < nit; i += kSchedT) {
+        sF[i] = nzF[i];
+        sG[i] = nzG[i];
+    }
+    __syncthreads();
+    for (int sidx = tid; sidx < npanel; sidx += kSchedT) {
+        const int pn = order[sidx];
+        const int2 kr = krange[pn];
+        const bool uf = useF[pn] != 0, ug = useG[pn] != 0;
+        int a = 0;
+        for (int it = kr.x; it < kr.y; ++it) a |= (uf ? sF[it] : 0) | (ug ? sG[it] : 0);
+        a = a != 0;
+        flag[sidx] = a;
+        actp[pn] = a;
+        if (a) atomicAdd(&iters, (unsigned long long)(kr.y - kr.x));
+    }
+    __syncthreads();
+    // bit 1 of every member's operand byte of a panel: left out of the GEMM
+    // (rewritten for every member, also those the pack skipped); 16 bytes at a
+    // time (nt is a multiple of 16, so a piece lies in one panel), eight
+    // pieces per thread in flight
+    {
+        u32x4* ug4 = reinterpret_cast<u32x4*>(useg);
+        const int64_t npc = (int64_t)npanel * (nt / 16);
+        const int per = nt / 16;
+        for (int64_t i0 = tid; i0 < npc; i0 += (int64_t)kSchedT * 8) {
+            u32x4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = ug4[min(i0 + (int64_t)j * kSchedT, npc - 1)];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t i = i0 + (int64_t)j * kSchedT;
+                if (i < npc) {
+                    const uint32_t set = actp[i / per] ? 0u : 0x02020202u;
+                    u32x4 w;
+                    w.x = (v[j].x & 0x01010101u) | set;
+                    w.y = (v[j].y & 0x01010101u) | set;
+                    w.z = (v[j].z & 0x01010101u) | set;
+                    w.w = (v[j].w & 0x01010101u) | set;
+                    ug4[i] = w;
+                }
+            }
+        }
+    }
+    // exclusive ranks over `order`: thread t counts its contiguous chunk
+    const int chunk = (npanel + kSchedT - 1) / kSchedT;
+    const int s0 = min(tid * chunk, npanel), s1 = min(s0 + chunk, npanel);
+    int cnt = 0;
+    for (int sidx = s0; sidx < s1; ++sidx) cnt += flag[sidx];
+    // block-wide exclusive scan of the counts: within each wave by
+    // shuffles, then across the four waves
+    const int lane = tid & 63, wv = tid >> 6;
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) part[wv] = inc;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < wv; ++w) wbase += part[w];   // part[0 .. kSchedT/64) hold the wave totals
+    if (tid == kSchedT - 1) flag[npanel] = wbase + inc;   // total, in the spare slot past the flags
+    const int excl = wbase + inc - cnt;
+    __syncthreads();
+    part[tid] = excl;
+    __syncthreads();
+    const int A = flag[npanel];
+    const int full = A / nbins, rem = A % nbins;
+    auto bin_off = [&](int b) {
+        int pb = 0;
+        if (rem) pb = (full % 2 == 0) ? min(b, rem) : max(0, b - (nbins - rem));
+        return full * b + pb;
+    };
+    int r = part[tid];
+    for (int sidx = s0; sidx < s1; ++sidx) {
+        if (!flag[sidx]) continue;
+        const int pass = r / nbins, k = r % nbins;
+        const int b = (pass % 2 == 0) ? k : nbins - 1 - k;
+        sched_panel[bin_off(b) + pass] = order[sidx];
+        ++r;
+    }
+    for (int b = tid; b <= nbins; b += kSchedT) sched_off[b] = bin_off(b);
+    if (tid == 0) stat[0] = (int64_t)iters;
+    // the flags were read above (before the first barrier): clear them
+    for (int i = tid; i < nit; i += kSchedT) {
+        nzF[i] = 0;
+        nzG[i] = 0;
+    }
+    for (int i = tid; i < npanel; i += kSchedT) {
+        useF[i] = 0;
+        useG[i] = 0;
     }
 }
 
@@ -1017,24 +1174,25 @@ static tg_status build_schedule(tg_ctx* c, MixedData* m) {
 }
 
 template <int NW, int FOLD, int DA, int MG, int AL>
-static void launch_persist_t(tg_ctx* c, MixedData* m) {
+static void launch_persist_t(tg_ctx* c, MixedData* m, bool dyn) {
     static_assert(AL == 0 || AL == 1, "A path");
     const int nmt = m->nt / NW;
     hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA, MG, AL == 1>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
                        dim3(256 * MG), 0, c->stream, (const u32x4*)m->A_band, m->band_off,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
-                       (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, m->sched_off,
-                       m->sched_panel, nmt, m->bins_per_xcd, (double*)m->P);
+                       (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange,
+                       dyn ? m->dsched_off : m->sched_off, dyn ? m->dsched_panel : m->sched_panel, nmt,
+                       m->bins_per_xcd, (double*)m->P);
 }
 
-static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
+static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false) {
     if (m->nt % m->nw != 0) return set_error(c, TG_ERR_ARG, "tile width %d does not divide %d", m->nw, m->nt);
     if (m->persist) {
         if (m->sched_nw != m->nw || m->nsplit != 1)
             return set_error(c, TG_ERR_STATE, "persistent GEMM schedule not built for tile %d", m->nw);
 #define TG_X(a, c_, d_, g_, z_)                                                                   \
     if (m->nw == a && m->fold == c_ && m->da == d_ && m->waves == 4 * g_ && m->al == z_)                  \
-        launch_persist_t<a, c_, d_, g_, z_>(c, m); else
+        launch_persist_t<a, c_, d_, g_, z_>(c, m, dyn); else
         TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
         return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d, %d waves, A in LDS %d",
@@ -1052,13 +1210,16 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m) {
     return TG_OK;
 }
 
-static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, const double* gate) {
+static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, const double* gate,
+                             bool dyn = false) {
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
 #define TG_PACK(ROW)                                                                                \
     hipLaunchKernelGGL(pack_member_kernel<ROW>, dim3(m->nt), dim3(kMemT), ROW ? sizeof(double) * n : 0, \
                        c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt,             \
-                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg, gate)
+                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg, gate, \
+                       dyn ? m->nzF : nullptr, dyn ? m->nzG : nullptr, dyn ? m->useF : nullptr,        \
+                       dyn ? m->useG : nullptr)
     if (m->rowcache) TG_PACK(true);
     else TG_PACK(false);
 #undef TG_PACK
@@ -1416,6 +1577,32 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
     if (m->persist) TG_TRY(build_schedule(c, m));
+    // panels with work: flags cleared here once, then by the schedule kernel
+    // after each use; the panels by decreasing band length for the dealing
+    if (const char* e = std::getenv("TG_MIXED_SKIP"))
+        if (std::atoi(e) == 0) m->skip = false;
+    {
+        m->nit = (int32_t)(m->kpad / kKS);
+        TG_TRY(dev_alloc(c, &m->nzF, (size_t)m->nit));
+        TG_TRY(dev_alloc(c, &m->nzG, (size_t)m->nit));
+        TG_TRY(dev_alloc(c, &m->useF, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->useG, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->lpt_order, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->dsched_off, (size_t)std::max(m->ncu, 8) + 1));
+        TG_TRY(dev_alloc(c, &m->dsched_panel, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->dstat, 1));
+        TG_HIP(c, hipMemsetAsync(m->dstat, 0xff, sizeof(int64_t), st));   // -1: no evaluation yet
+        TG_HIP(c, hipMemsetAsync(m->nzF, 0, (size_t)m->nit, st));
+        TG_HIP(c, hipMemsetAsync(m->nzG, 0, (size_t)m->nit, st));
+        TG_HIP(c, hipMemsetAsync(m->useF, 0, (size_t)m->npanel, st));
+        TG_HIP(c, hipMemsetAsync(m->useG, 0, (size_t)m->npanel, st));
+        std::vector<int32_t> order(m->npanel);
+        for (int i = 0; i < m->npanel; ++i) order[i] = i;
+        auto cost = [&](int pn) { return (int64_t)(m->h_krange[pn].y - m->h_krange[pn].x) + 1; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        TG_HIP(c, hipMemcpyAsync(m->lpt_order, order.data(), sizeof(int32_t) * m->npanel,
+                                 hipMemcpyHostToDevice, st));
+    }
     if (!fixed) TG_TRY(autotune(c, m));
     TG_HIP(c, hipStreamSynchronize(st));
     return TG_OK;
@@ -1434,6 +1621,14 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_depth = m->persist ? m->da : m->depth;
     out->mixed_persist = m->persist ? 1 + m->al : 0;
     out->mixed_tuned_ms = m->tuned_ms;
+    out->mixed_skip = m->persist && m->skip ? 1 : 0;
+    out->mixed_last_band = out->mixed_band;
+    if (out->mixed_skip && m->dstat) {
+        int64_t it = -1;
+        if (hipMemcpyAsync(&it, m->dstat, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess && it >= 0)
+            out->mixed_last_band = it * (int64_t)kMT * kKS;
+    }
 }
 
 void mixed_destroy(tg_ctx* c) {
@@ -1449,6 +1644,14 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->krange);
     dev_free(&m->sched_off);
     dev_free(&m->sched_panel);
+    dev_free(&m->nzF);
+    dev_free(&m->nzG);
+    dev_free(&m->useF);
+    dev_free(&m->useG);
+    dev_free(&m->lpt_order);
+    dev_free(&m->dsched_off);
+    dev_free(&m->dsched_panel);
+    dev_free(&m->dstat);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
     dev_free(&m->P);
@@ -1485,9 +1688,19 @@ tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B
     if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
     TG_TRY(ensure_P(c, m));
-    TG_TRY(launch_pack(c, m, d_f, B, gate));
+    // panels with all-zero operands leave the persistent GEMM
+    const bool dyn = m->persist && m->skip;
+    TG_TRY(launch_pack(c, m, d_f, B, gate, dyn));
+    if (dyn) {
+        const int nbins = 8 * m->bins_per_xcd;
+        hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
+                           sizeof(int32_t) * (2 * m->npanel + 1) + 2 * (size_t)m->nit,
+                           c->stream, m->npanel, nbins, m->nit, m->nt, m->krange, m->lpt_order, m->nzF,
+                           m->nzG, m->useF, m->useG, m->useg, m->dsched_off, m->dsched_panel, m->dstat);
+        TG_CHECK_LAUNCH(c);
+    }
     time_begin(c);
-    TG_TRY(launch_gemm(c, m));
+    TG_TRY(launch_gemm(c, m, dyn));
     time_end(c);
     TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate));
     return TG_OK;

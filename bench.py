@@ -405,7 +405,7 @@ def measured_traffic(st):
     with open(TRAFFIC_FILE) as fh:
         t = json.load(fh)
     cfg = [st["mixed_persist"], st["mixed_tile"], st["mixed_waves"], st["mixed_fold"], st["mixed_depth"],
-           st["mixed_splits"]]
+           st["mixed_splits"], st["mixed_skip"]]
     if t.get("config") != cfg:
         return None, f"counter profile taken on tile configuration {t.get('config')}, running {cfg}"
     return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"], "measured"
@@ -508,17 +508,23 @@ def main():
     # two products K F and K^T F over the structural nonzeros of K for B
     # members, 4 nnz B flops.  Dense-equivalent (4 n^2 B, zeros counted) and
     # issued (three fp16 products over the streamed band) rates beside it.
-    flops_alg = 4.0 * st["nnz"] * B
+    # the launch leaves out the panels whose fp16 operands are zero for every
+    # member (their products are exactly 0): the work counted is that of the
+    # panels it computes, the nonzeros scaled by the executed fraction of the band
+    st_run = ctx.kernel_stats()
+    exec_frac = st_run["mixed_last_band"] / st["mixed_band"] if st["mixed_band"] else 1.0
+    flops_alg = 4.0 * st["nnz"] * B * exec_frac
     tflops = flops_alg / (ms_gemm * 1e-3) / 1e12
     dense_tflops = 4.0 * n * n * B / (ms_gemm * 1e-3) / 1e12
-    issued = 3 * 2.0 * st["mixed_band"] * st["mixed_width"] / (ms_gemm * 1e-3) / 1e12
+    issued = 3 * 2.0 * st_run["mixed_last_band"] * st["mixed_width"] / (ms_gemm * 1e-3) / 1e12
     traffic, traffic_note = measured_traffic(st)
     # plain FP64 pass: algorithmic bytes = the nonzeros of K once (8 nnz) plus
     # f in, I out and two n-vectors (32 n); streamed = the stored band
     gbs_plain = (8.0 * st["nnz"] + 32.0 * n) / (ms_dual * 1e-3) / 1e9
     gbs_streamed = (8.0 * st["plain_band"] + 32.0 * n) / (ms_dual * 1e-3) / 1e9
     _log(f"mixed: {ms_mixed:.4f} ms/batch ({evals_mixed:.0f} evals/s), GEMM {ms_gemm:.4f} ms "
-         f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued; max rel err {rel:.2e}; "
+         f"= {tflops:.1f} TFLOP/s algorithmic, {issued:.1f} TFLOP/s issued ({exec_frac:.0%} of the band "
+         f"computed); max rel err {rel:.2e}; "
          f"kernel {gemm_kernel_name(st)} splits {st['mixed_splits']}")
     _log(f"plain FP64: {ms_plain:.4f} ms/eval ({evals_plain:.0f} evals/s), K-band pass "
          f"{ms_dual:.4f} ms = {gbs_plain:.0f} GB/s algorithmic, {gbs_streamed:.0f} GB/s streamed")
@@ -544,7 +550,8 @@ def main():
                      "unit": "TFLOP/s", "frac": tflops / FP16_MFMA_PEAK_TFLOPS,
                      "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": gemm_kernel_name(st), "kernel_ms": ms_gemm,
-                     "flops_per_launch": flops_alg, "issued_tflops": issued,
+                     "flops_per_launch": flops_alg, "executed_band_fraction": exec_frac,
+                     "flops_per_launch_all_panels": 4.0 * st["nnz"] * B, "issued_tflops": issued,
                      "issued_frac": issued / FP16_MFMA_PEAK_TFLOPS,
                      "dense_equiv_tflops": dense_tflops,
                      "tile": {"persistent": st["mixed_persist"] > 0, "a_in_lds": st["mixed_persist"] == 2,

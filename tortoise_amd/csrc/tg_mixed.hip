@@ -845,20 +845,33 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     constexpr int UNR = 6;
     double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
     int32_t node[UNR], gu[UNR], gv[UNR];   // gu, gv: the panels' operand bytes
-    // the partial sums (streamed from HBM) ...
+    // the partial sums (streamed from HBM), read only for the panels the GEMM
+    // computed: a panel left out (bit 1 of its operand byte) was not written
+    // and its product is 0, so its bytes need not move ...
     auto load_products = [&](int64_t p0) {
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
             const PT* P0 = P + (int64_t)b * rows_pad;
-            u[j] = (double)P0[p];
-            v[j] = (double)P0[kt + p];
-            for (int s = 1; s < nsplit; ++s) {
-                const PT* Ps = P + ((int64_t)s * nt + b) * rows_pad;
-                u[j] += (double)Ps[p];
-                v[j] += (double)Ps[kt + p];
+            u[j] = 0.0;
+            v[j] = 0.0;
+            if (!(gu[j] & 2)) {
+                u[j] = (double)P0[p];
+                for (int s = 1; s < nsplit; ++s) u[j] += (double)P[((int64_t)s * nt + b) * rows_pad + p];
             }
-
+            if (!(gv[j] & 2)) {
+                v[j] = (double)P0[kt + p];
+                for (int s = 1; s < nsplit; ++s) v[j] += (double)P[((int64_t)s * nt + b) * rows_pad + kt + p];
+            }
+        }
+    };
+    // ... after the operand bytes, which say which panels those are ...
+    auto load_bytes = [&](int64_t p0) {
+#pragma unroll
+        for (int j = 0; j < UNR; ++j) {
+            const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
+            gu[j] = useg[(p / kMT) * (int64_t)nt + b];
+            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b];
         }
     };
     // ... and the per-position constants every member shares (L2-resident)
@@ -866,8 +879,6 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
-            gu[j] = useg[(p / kMT) * (int64_t)nt + b];
-            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b];
             rkp[j] = rk[p];
             ckp[j] = ck[p];
             Vp[j] = V[p];
@@ -887,8 +898,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             if (p0 + (int64_t)j * kMemT >= n) break;
             // operand byte: bit 0 = g (z = 1 - f) rather than f, bit 1 = the panel
             // was left out of the GEMM (all operands zero: its product is 0)
-            const double uu = (gu[j] & 2) ? 0.0 : u[j] * inv_scale;
-            const double vv = (gv[j] & 2) ? 0.0 : v[j] * inv_scale;
+            const double uu = u[j] * inv_scale;
+            const double vv = v[j] * inv_scale;
             // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
             const bool gfu = gu[j] & 1, gfv = gv[j] & 1;
             const double Kf = gfu ? rkp[j] - uu : uu, Kg = gfu ? uu : rkp[j] - uu;
@@ -906,12 +917,14 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     };
     const int64_t step = (int64_t)kMemT * UNR;
     // the first round's partial sums and the row staging are in flight together
-    load_products(threadIdx.x);
+    load_bytes(threadIdx.x);
     if constexpr (ROW) stage_member_row(fb, n, row);
+    load_products(threadIdx.x);
     load_constants(threadIdx.x);
     if constexpr (ROW) lds_barrier();
     finish_round(threadIdx.x);
     for (int64_t base = step; base < n; base += step) {
+        load_bytes(base + threadIdx.x);
         load_products(base + threadIdx.x);
         load_constants(base + threadIdx.x);
         finish_round(base + threadIdx.x);

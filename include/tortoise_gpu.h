@@ -50,6 +50,8 @@ typedef int32_t tg_status;
 #define TG_CONSERVE     0x4   /* remove the particle-number defect of one evaluation */
 #define TG_QUIET        0x8   /* (dp5) no per-step console line */
 #define TG_TIME_KERNEL  0x10  /* (apply) bracket the dominant kernel with HIP events */
+#define TG_BATH_SCHEDULE 0x20 /* (dp5) the bath follows the schedule set by tg_set_bath_schedule /
+                                 tg_tuple_set_bath_schedule; bath_n is not read */
 
 typedef struct tg_problem_desc {
     int32_t dim;            /* 2 or 3 */
@@ -84,7 +86,31 @@ typedef struct tg_step_stats {
     int64_t evals;          /* collision-operator evaluations of this member */
     int32_t failed;         /* 1: step underflow */
     int32_t n_dense;        /* dense outputs produced */
+    double  bath_energy;    /* energy ledger of the phonon bath: the energy it received over the
+                               accepted steps, i.e. minus the e-ph channel's electron energy flux
+                               (sum_i V_i eps_i I_i, FP64 per stage) integrated with the step's
+                               weights, so E_electrons + bath_energy stays at its initial value */
 } tg_step_stats;
+
+/* Time-dependent bath: the phonon band constrained to a uniform population
+ * n_b(t) per member (Band::constrained / constrainedPopulation,
+ * include/Physics/Material/Band.hpp:70-74, enforced at every stage time by
+ * MaterialStatus::applyConstraints, src/Physics/Status/MaterialStatus.cpp:198-206).
+ * Member b's occupation is sampled at t_start + k*dt, k = 0..n_samples-1, at
+ * samples + b*n_samples; between samples it is linear, outside it holds the
+ * end values.  The step vetoer (MaterialTimeStatus::timeStepVetoer,
+ * src/Physics/Status/MaterialTimeStatus.cpp:60-100) reads the band's error
+ * tolerances atol and rtol (Band::absoluteErrorTolerance /
+ * relativeErrorTolerance, Band.hpp:77-78). */
+typedef struct tg_bath_schedule {
+    const double* samples;  /* host memory, batch x n_samples */
+    int32_t n_samples;      /* >= 2 */
+    int32_t batch;          /* members covered (a DP5 call with TG_BATH_SCHEDULE takes at most this many) */
+    double  t_start;
+    double  dt;             /* > 0 */
+    double  atol;
+    double  rtol;
+} tg_bath_schedule;
 
 /* ---------------------------------------------------------------- setup */
 /* Number of lattice nodes of a structured mesh (no context needed). */
@@ -170,6 +196,16 @@ tg_status   tg_dp5_integrate(tg_ctx* ctx, double* f, int64_t n, int32_t batch,
                              double atol, const double* bath_n, int32_t flags,
                              double* dense_out, int32_t max_dense,
                              tg_step_stats* stats, double* h_accepted, int32_t h_cap);
+
+/* Sets (or, with NULL, clears) the time-dependent bath the DP5 entry points
+ * use when TG_BATH_SCHEDULE is in their flags: every stage of member b
+ * evaluates the operator with n_b(t) at the stage time, and the step vetoer
+ * caps the steps (the vetoer runs in every adaptive DP5; without a schedule
+ * it returns the step unchanged up to t + h - t, as the reference's does
+ * with no constrained band).  Replaces MaterialStatus::applyConstraints and
+ * MaterialTimeStatus::timeStepVetoer (MaterialStatus.cpp:198-206,
+ * MaterialTimeStatus.cpp:60-100) for a constrained phonon bath. */
+tg_status   tg_set_bath_schedule(tg_ctx* ctx, const tg_bath_schedule* sched);
 
 /* The same stage arithmetic on a given step sequence: member b takes the
  * n_steps[b] steps h_steps[b*h_cap ...] (all accepted, no error control),
@@ -285,11 +321,22 @@ tg_status   tg_tuple_last_kernel_ms(tg_tuple_ctx* ctx, double* ms);
  * phonon band a bath of occupation bath_n[b].  Same outputs as
  * tg_dp5_integrate (dense_out [batch][max_dense][nel][dim+1]).  The MC points
  * are the ones last set (the caller redraws them per propagate, as
- * MaterialTimeStatus.cpp:104 does).  flags: TG_PTR_DEVICE, TG_QUIET. */
+ * MaterialTimeStatus.cpp:104 does).  flags: TG_PTR_DEVICE, TG_QUIET,
+ * TG_BATH_SCHEDULE.  h_accepted (optional, batch x h_cap): the accepted step
+ * sizes of member b at h_accepted + b*h_cap, as in tg_dp5_integrate. */
 tg_status   tg_tuple_dp5(tg_tuple_ctx* ctx, double* f, int64_t nel, int32_t batch, double t0,
                          double t1, double dense_dt, double rtol, double atol,
                          const double* bath_n, int32_t flags, double* dense_out,
-                         int32_t max_dense, tg_step_stats* stats);
+                         int32_t max_dense, tg_step_stats* stats, double* h_accepted,
+                         int32_t h_cap);
+/* tg_set_bath_schedule for the tuple operator's bath (channels with a phonon
+ * bath; not with dynamic_phonons).  tg_tuple_dp5 with TG_BATH_SCHEDULE in
+ * flags evaluates every stage with the member's n_b(t) at the stage time and
+ * its step vetoer reads the schedule.  The bath ledger of tg_tuple_dp5
+ * (stats[b].bath_energy) integrates minus the electron energy flux
+ * int eps(k) I(k) dk of the DG result (zero with dynamic_phonons, where the
+ * phonon band is part of the state). */
+tg_status   tg_tuple_set_bath_schedule(tg_tuple_ctx* ctx, const tg_bath_schedule* sched);
 void*       tg_tuple_stream(const tg_tuple_ctx* ctx);
 
 #ifdef __cplusplus

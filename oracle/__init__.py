@@ -92,6 +92,37 @@ def lib():
                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_int, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                        C.c_void_p, C.c_int64, C.POINTER(DP5Stats)]
+        L.or_dp5_collision_ex.argtypes = [C.c_int, _i32p, C.c_int64, _f64p, _f64p, C.c_void_p,
+                                          C.c_double, C.c_void_p, C.c_int32, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, _f64p, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                          C.POINTER(DP5Stats), C.POINTER(C.c_double)]
+        L.or_dp5_collision_replay_ex.argtypes = [C.c_int, _i32p, C.c_int64, _f64p, _f64p, C.c_void_p,
+                                                 C.c_double, C.c_void_p, C.c_int32, C.c_double,
+                                                 C.c_double, _f64p, C.c_double, C.c_double,
+                                                 C.c_double, C.c_int, _f64p, _i8p, C.c_int64,
+                                                 C.c_void_p, C.c_void_p, C.c_int32,
+                                                 C.POINTER(C.c_double)]
+        L.or_dp5_mc_eph_replay.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
+                                           C.c_double, C.c_double, C.c_int32, _f64p, C.c_int32,
+                                           _i32p, C.c_int64, C.c_int32, C.c_double, C.c_void_p,
+                                           C.c_int32, C.c_double, C.c_double, _f64p, C.c_double,
+                                           C.c_double, C.c_double, _f64p, C.c_int64, C.c_void_p,
+                                           C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+        L.or_sched_value.restype = C.c_double
+        L.or_sched_value.argtypes = [_f64p, C.c_int32, C.c_double, C.c_double, C.c_double]
+        L.or_step_vetoer.restype = C.c_double
+        L.or_step_vetoer.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.or_energy_dg.restype = C.c_double
+        L.or_energy_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p]
+        L.or_dp5_mc_eph_ex.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+                                       C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
+                                       C.c_int32, C.c_double, C.c_void_p, C.c_int32, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, _f64p, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                       C.c_void_p, C.c_int32, C.c_int32, C.POINTER(DP5Stats),
+                                       C.POINTER(C.c_double)]
         L.or_dp5_collision_replay.argtypes = [C.c_int, _i32p, C.c_int64, _f64p, _f64p, C.c_double,
                                               _f64p, C.c_double, C.c_double, C.c_double, C.c_int,
                                               _f64p, _i8p, C.c_int64, C.c_void_p, C.c_void_p,
@@ -315,8 +346,41 @@ def dp5_integrate(rhs, norm, y0, t0, t_final, dense_dt, max_dense=100000, log_ca
                 stats=st)
 
 
+class BathSchedule:
+    """Samples of a uniform constrained bath n(t) at t_start + k dt (linear
+    between samples, held outside) and the band's error tolerances for the
+    step vetoer (Band::constrainedPopulation, MaterialTimeStatus::timeStepVetoer)."""
+
+    def __init__(self, samples, t_start, dt, atol=1e-3, rtol=1e-3):
+        self.samples = np.ascontiguousarray(samples, np.float64)
+        self.t_start, self.dt, self.atol, self.rtol = float(t_start), float(dt), float(atol), float(rtol)
+
+    def _struct(self):
+        return _BathSchedC(self.samples.ctypes.data_as(C.POINTER(C.c_double)), self.samples.size,
+                           self.t_start, self.dt, self.atol, self.rtol)
+
+    def value(self, t):
+        return lib().or_sched_value(self.samples, self.samples.size, self.t_start, self.dt, t)
+
+
+class _BathSchedC(C.Structure):
+    _fields_ = [("s", C.POINTER(C.c_double)), ("ns", C.c_int32), ("t_start", C.c_double),
+                ("dt", C.c_double), ("atol", C.c_double), ("rtol", C.c_double)]
+
+
+def step_vetoer(sched, t, h, dense_dt):
+    """MaterialTimeStatus::timeStepVetoer on timesToTest(t, h); sched None: no constrained band."""
+    if sched is None:
+        return lib().or_step_vetoer(None, t, h, dense_dt)
+    cs = sched._struct()
+    return lib().or_step_vetoer(C.cast(C.pointer(cs), C.c_void_p), t, h, dense_dt)
+
+
 def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol,
-                  conserve=False, max_dense=100000, log_cap=1 << 20):
+                  conserve=False, max_dense=100000, log_cap=1 << 20, sched=None, eps=None):
+    """Adaptive DP5 on the W-form.  sched (BathSchedule): time-dependent bath
+    instead of bath_n.  eps (nodal energies): also return the bath energy
+    ledger ("ledger": the energy the bath received)."""
     f = np.array(f0, np.float64, copy=True)
     n = f.size
     dense = np.zeros((max_dense, n)) if max_dense else None
@@ -324,30 +388,48 @@ def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol
     hl = np.zeros(log_cap)
     al = np.zeros(log_cap, np.int8)
     st = DP5Stats()
-    lib().or_dp5_collision(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
-                           np.ascontiguousarray(V, np.float64), bath_n, f, t0, t_final, dense_dt,
-                           rtol, atol, int(bool(conserve)), _ptr(dense), _ptr(dt), max_dense,
-                           _ptr(hl), _ptr(al), log_cap, C.byref(st))
+    led = C.c_double(0.0)
+    e = None if eps is None else np.ascontiguousarray(eps, np.float64)
+    sp = None if sched is None else sched.samples
+    lib().or_dp5_collision_ex(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
+                              np.ascontiguousarray(V, np.float64), _ptr(e), bath_n, _ptr(sp),
+                              0 if sched is None else sp.size,
+                              0.0 if sched is None else sched.t_start,
+                              1.0 if sched is None else sched.dt,
+                              0.0 if sched is None else sched.atol,
+                              0.0 if sched is None else sched.rtol, f, t0, t_final, dense_dt,
+                              rtol, atol, int(bool(conserve)), _ptr(dense), _ptr(dt), max_dense,
+                              _ptr(hl), _ptr(al), log_cap, C.byref(st),
+                              C.byref(led) if e is not None else None)
     nd = min(st.n_dense, max_dense)
     na = min(st.attempted, log_cap)
     return dict(y=f, dense=None if dense is None else dense[:nd], dense_t=dt[:nd], h=hl[:na],
-                accepted=al[:na].astype(bool), stats=st)
+                accepted=al[:na].astype(bool), stats=st, ledger=led.value)
 
 
 def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, accepted,
-                         conserve=False, max_dense=100000):
+                         conserve=False, max_dense=100000, sched=None, eps=None):
+    """Fixed-step DP5 on the recorded attempts (h, accepted); sched: a
+    BathSchedule instead of bath_n; with eps the bath energy ledger of the
+    replayed steps is returned too ("ledger")."""
     f = np.array(f0, np.float64, copy=True)
     n = f.size
     dense = np.zeros((max_dense, n))
     dt = np.zeros(max_dense)
     h = np.ascontiguousarray(h, np.float64)
     acc = np.ascontiguousarray(np.asarray(accepted).astype(np.int8))
-    lib().or_dp5_collision_replay(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
-                                  np.ascontiguousarray(V, np.float64), bath_n, f, t0, t_final,
-                                  dense_dt, int(bool(conserve)), h, acc, h.size, _ptr(dense),
-                                  _ptr(dt), max_dense)
+    led = C.c_double(0.0)
+    e = None if eps is None else np.ascontiguousarray(eps, np.float64)
+    sp = None if sched is None else sched.samples
+    lib().or_dp5_collision_replay_ex(dim, _nsec(nsec), n, np.ascontiguousarray(K, np.float64),
+                                     np.ascontiguousarray(V, np.float64), _ptr(e), bath_n, _ptr(sp),
+                                     0 if sched is None else sp.size,
+                                     0.0 if sched is None else sched.t_start,
+                                     1.0 if sched is None else sched.dt, f, t0, t_final,
+                                     dense_dt, int(bool(conserve)), h, acc, h.size, _ptr(dense),
+                                     _ptr(dt), max_dense, C.byref(led) if e is not None else None)
     nd = int(np.count_nonzero(dt))
-    return dict(y=f, dense=dense[:nd], dense_t=dt[:nd])
+    return dict(y=f, dense=dense[:nd], dense_t=dt[:nd], ledger=led.value)
 
 
 def collision_vjp(K, V, bath_n, f, mu):
@@ -503,10 +585,19 @@ def relative_error_dg(dim, err, pop, stat=-1.0, rtol=1e-8, atol=1e-12, penalty=0
     return lib().or_relative_error_dg(dim, err.shape[0], err, pop, stat, rtol, atol, penalty)
 
 
+def energy_dg(dim, nsec, extent, eps, f):
+    """int eps f dk of DG coefficients [nel][dim+1] (exact per element)."""
+    return lib().or_energy_dg(dim, np.ascontiguousarray(nsec, np.int32), _pad3(extent, 1.0),
+                              np.ascontiguousarray(eps, np.float64), np.ascontiguousarray(f, np.float64))
+
+
 def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0, t0,
-               t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8, dynamic=False, eps_ph=None):
+               t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8, dynamic=False, eps_ph=None,
+               sched=None):
     """DP5(4) of the electron band's DG state with the tuple operator; with
-    dynamic=True the state is [electron DG | phonon DG] and bath_n unused."""
+    dynamic=True the state is [electron DG | phonon DG] and bath_n unused.
+    sched (BathSchedule): time-dependent bath.  The result holds the bath
+    energy ledger ("ledger")."""
     nsec = np.ascontiguousarray(nsec, np.int32)
     org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
     pts = np.ascontiguousarray(pts, np.float64)
@@ -516,13 +607,49 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
     dense_t = np.zeros(max(max_dense, 1))
     st = DP5Stats()
     keep, ph = _eps_ph_ptr(eps_ph)
-    lib().or_dp5_mc_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
-                        amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
-                        bath_n, f, t0, t_final,
-                        dense_dt, rtol, atol, dense.ctypes.data if max_dense else None,
-                        dense_t.ctypes.data if max_dense else None, max_dense, nthreads, C.byref(st))
+    led = C.c_double(0.0)
+    sp = None if sched is None else sched.samples
+    lib().or_dp5_mc_eph_ex(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+                           amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
+                           bath_n, _ptr(sp), 0 if sched is None else sp.size,
+                           0.0 if sched is None else sched.t_start,
+                           1.0 if sched is None else sched.dt,
+                           0.0 if sched is None else sched.atol,
+                           0.0 if sched is None else sched.rtol, f, t0, t_final,
+                           dense_dt, rtol, atol, dense.ctypes.data if max_dense else None,
+                           dense_t.ctypes.data if max_dense else None, max_dense, nthreads,
+                           C.byref(st), C.byref(led))
     return {"f": f, "dense": dense[:st.n_dense] if max_dense else None,
-            "dense_t": dense_t[:st.n_dense] if max_dense else None, "stats": st}
+            "dense_t": dense_t[:st.n_dense] if max_dense else None, "stats": st,
+            "ledger": led.value}
+
+
+def dp5_mc_eph_replay(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0,
+                      t0, t_final, dense_dt, steps, max_dense=0, nthreads=8, dynamic=False,
+                      eps_ph=None, sched=None):
+    """Fixed-step DP5 of the tuple operator on the accepted steps `steps`;
+    returns the final state, dense outputs and the bath energy ledger."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    pts = np.ascontiguousarray(pts, np.float64)
+    idx = np.ascontiguousarray(idx, np.int32)
+    steps = np.ascontiguousarray(steps, np.float64)
+    f = np.ascontiguousarray(f0, np.float64).copy()
+    dense = np.zeros((max(max_dense, 1),) + f.shape)
+    dense_t = np.zeros(max(max_dense, 1))
+    keep, ph = _eps_ph_ptr(eps_ph)
+    led = C.c_double(0.0)
+    sp = None if sched is None else sched.samples
+    lib().or_dp5_mc_eph_replay(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+                               amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
+                               bath_n, _ptr(sp), 0 if sched is None else sp.size,
+                               0.0 if sched is None else sched.t_start,
+                               1.0 if sched is None else sched.dt, f, t0, t_final, dense_dt, steps,
+                               steps.size, dense.ctypes.data if max_dense else None,
+                               dense_t.ctypes.data if max_dense else None, max_dense, nthreads,
+                               C.byref(led))
+    nd = int(np.count_nonzero(dense_t)) if max_dense else 0
+    return {"f": f, "dense": dense[:nd] if max_dense else None, "ledger": led.value}
 
 
 def mc_rates_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,

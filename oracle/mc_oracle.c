@@ -617,12 +617,46 @@ typedef struct mc_dp5_user {
     const int32_t* idx;
     int64_t ntup, nel;
     double* fph;
+    const or_bath_sched* bs; /* non-NULL: the bath at the stage time */
 } mc_dp5_user;
 
+/* int eps f over the mesh for DG coefficients f: per element the exact
+ * P1 x P1 integral vol/((d+1)(d+2)) (sum_v eps_v f_v + sum_v eps_v sum_v f_v)
+ * with the element-vertex values (eps at the element's nodes) */
+double or_energy_dg(int dim, const int32_t* nsec, const double* extent, const double* eps,
+                    const double* f) {
+    const int64_t nel = or_num_elements(dim, nsec);
+    const int nc = dim + 1;
+    int32_t* conn = (int32_t*)malloc(sizeof(int32_t) * nel * nc);
+    or_mesh_elements(dim, nsec, conn);
+    double tot = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        double sei = 0.0, se = 0.0, si = 0.0;
+        for (int a = 0; a < nc; ++a) {
+            const double ev = eps[conn[e * nc + a]];
+            const double x = f[e * nc] + (a ? f[e * nc + a] : 0.0);
+            sei += ev * x;
+            se += ev;
+            si += x;
+        }
+        tot += sei + se * si;
+    }
+    free(conn);
+    return tot * or_element_volume(dim, nsec, extent) / (double)((dim + 1) * (dim + 2));
+}
+
+static double mc_dp5_flux(void* u, const double* dydt) {
+    const mc_dp5_user* p = (const mc_dp5_user*)u;
+    return or_energy_dg(p->dim, p->nsec, p->extent, p->eps, dydt);
+}
+
 static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
-    (void)t;
     const mc_dp5_user* p = (const mc_dp5_user*)u;
     const int64_t len = p->nel * (p->dim + 1);
+    if (p->bs) {
+        const double nb = or_sched_value(p->bs->s, p->bs->ns, p->bs->t_start, p->bs->dt, t);
+        for (int64_t e = 0; e < p->nel; ++e) p->fph[e * (p->dim + 1)] = nb;
+    }
     if (p->dynamic)
         or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->eps_ph, p->omega0,
                         p->amplitude, p->nu, p->pts, p->nmc, p->idx, p->ntup, y, y + len, dydt,
@@ -656,17 +690,64 @@ void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const dou
                    int32_t dynamic, double bath_n, double* f, double t0, double t_final,
                    double dense_dt, double rtol, double atol, double* dense_out, double* dense_t,
                    int32_t max_dense, int32_t nthreads, or_dp5_stats* stats) {
+    or_dp5_mc_eph_ex(dim, nsec, origin, extent, eps, eps_ph, omega0, amplitude, nu, pts, nmc, idx, ntup,
+                     dynamic, bath_n, NULL, 0, 0.0, 1.0, 0.0, 0.0, f, t0, t_final, dense_dt, rtol, atol,
+                     dense_out, dense_t, max_dense, nthreads, stats, NULL);
+}
+
+void or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                      const double* eps, const double* eps_ph, double omega0, double amplitude,
+                      int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                      int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
+                      int32_t ns, double t_start, double sdt, double satol, double srtol,
+                      double* f, double t0, double t_final, double dense_dt,
+                      double rtol, double atol, double* dense_out, double* dense_t,
+                      int32_t max_dense, int32_t nthreads, or_dp5_stats* stats, double* ledger) {
     mc_dp5_user u;
+    memset(&u, 0, sizeof(u));
+    or_bath_sched bs = {sched, ns, t_start, sdt, satol, srtol};
     u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
     u.eps_ph = eps_ph; u.dynamic = dynamic;
     u.omega0 = omega0; u.amplitude = amplitude; u.rtol = rtol; u.atol = atol;
     u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
     u.nel = or_num_elements(dim, nsec);
+    u.bs = (sched && !dynamic) ? &bs : NULL;
     const int nc = dim + 1;
     u.fph = (double*)calloc((size_t)u.nel * nc, sizeof(double));
     for (int64_t e = 0; e < u.nel; ++e) u.fph[e * nc] = bath_n;
-    or_dp5_integrate(u.nel * nc * (dynamic ? 2 : 1), f, t0, t_final, dense_dt, mc_dp5_rhs,
-                     mc_dp5_norm, &u, dense_out, dense_t, max_dense, NULL, NULL, 0, stats);
+    if (ledger) *ledger = 0.0;
+    or_dp5_integrate_ex(u.nel * nc * (dynamic ? 2 : 1), f, t0, t_final, dense_dt, mc_dp5_rhs, mc_dp5_norm, &u,
+                        u.bs, (ledger && !dynamic) ? mc_dp5_flux : NULL, ledger, dense_out, dense_t, max_dense,
+                        NULL, NULL, 0, stats);
+    free(u.fph);
+}
+
+void or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                          const double* eps, const double* eps_ph, double omega0, double amplitude,
+                          int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                          int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
+                          int32_t ns, double t_start, double sdt, double* f, double t0,
+                          double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
+                          double* dense_out, double* dense_t, int32_t max_dense, int32_t nthreads,
+                          double* ledger) {
+    mc_dp5_user u;
+    memset(&u, 0, sizeof(u));
+    or_bath_sched bs = {sched, ns, t_start, sdt, 0.0, 0.0};
+    u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
+    u.eps_ph = eps_ph; u.dynamic = dynamic;
+    u.omega0 = omega0; u.amplitude = amplitude;
+    u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
+    u.nel = or_num_elements(dim, nsec);
+    u.bs = (sched && !dynamic) ? &bs : NULL;
+    const int nc = dim + 1;
+    u.fph = (double*)calloc((size_t)u.nel * nc, sizeof(double));
+    for (int64_t e = 0; e < u.nel; ++e) u.fph[e * nc] = bath_n;
+    int8_t* acc = (int8_t*)malloc((size_t)(n_steps > 0 ? n_steps : 1));
+    memset(acc, 1, (size_t)(n_steps > 0 ? n_steps : 1));
+    or_dp5_replay_ex(u.nel * nc * (dynamic ? 2 : 1), f, t0, t_final, dense_dt, mc_dp5_rhs, &u,
+                     (ledger && !dynamic) ? mc_dp5_flux : NULL, ledger, h_steps, acc, n_steps, dense_out,
+                     dense_t, max_dense);
+    free(acc);
     free(u.fph);
 }
 

@@ -459,8 +459,10 @@ double or_relative_error(int dim, int64_t nel, const int32_t* conn,
 
 /* ======================================================== DP5(4) driver */
 /* Butcher tableau, controller and dense output of DormandPrince54.hpp:503-511,
- * :532-543, :588-662, :598-611.  enforceValue and the step vetoer are the
- * identity here (no constrained bands in the W-form). */
+ * :532-543, :588-662, :598-611.  The step vetoer (:562-564, :640-642) is
+ * MaterialTimeStatus::timeStepVetoer (MaterialTimeStatus.cpp:60-100); a
+ * constrained bath enters through the stage times the rhs receives
+ * (enforceValue at t + c_i h, :565-578). */
 #define DP_C2 (1. / 5.)
 #define DP_C3 (3. / 10.)
 #define DP_C4 (4. / 5.)
@@ -527,11 +529,68 @@ static void dp5_dense(const dp5_work* w, const double* y, double h, double s, do
                  (s * h * bt5) * w->k5[i] + (s * h * bt6) * w->k6[i] + (s * h * bt7) * w->k7[i];
 }
 
+/* piecewise-linear samples, held at the ends (the device's sched_value) */
+double or_sched_value(const double* s, int32_t ns, double t_start, double dt, double t) {
+    const double u = (t - t_start) / dt;
+    if (!(u > 0.0)) return s[0];
+    if (u >= (double)(ns - 1)) return s[ns - 1];
+    int k = (int)floor(u);
+    if (k > ns - 2) k = ns - 2;
+    const double w = u - (double)k;
+    return fma(w, s[k + 1] - s[k], s[k]);
+}
+
+/* MaterialTimeStatus.cpp:60-100 for one uniform constrained band: sample
+ * times t0 + dense_dt, t0 + 2 dense_dt, ... run through the five intervals of
+ * timesToTest; a sample further from its interval's average than
+ * (1 + 0.3)|L - R| + atol + rtol |average| is the critical time. */
+double or_step_vetoer(const or_bath_sched* bs, double t, double h, double dense_dt) {
+    const double times[6] = {t, t + DP_C2 * h, t + DP_C3 * h, t + DP_C4 * h, t + DP_C5 * h, t + h};
+    double crit = times[5];
+    if (bs) {
+        const double var = 1. + 0.3;
+        double cur = times[0] + dense_dt;
+        double L = or_sched_value(bs->s, bs->ns, bs->t_start, bs->dt, times[0]);
+        double R = or_sched_value(bs->s, bs->ns, bs->t_start, bs->dt, times[1]);
+        int found = 0;
+        for (int i = 0; i < 5 && !found; ++i) {
+            if (i != 0) {
+                L = R;
+                R = or_sched_value(bs->s, bs->ns, bs->t_start, bs->dt, times[i + 1]);
+            }
+            const double avg = 0.5 * (L + R);
+            const double acc = var * fabs(L - R) + bs->atol + bs->rtol * fabs(avg);
+            while (cur < times[i + 1]) {
+                const double v = or_sched_value(bs->s, bs->ns, bs->t_start, bs->dt, cur);
+                if (fabs(v - avg) / acc > 1.) {
+                    crit = cur;
+                    found = 1;
+                    break;
+                }
+                const double next = cur + dense_dt;
+                if (!(next > cur)) break;
+                cur = next;
+            }
+        }
+    }
+    return crit - times[0];
+}
+
 void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
                       double dense_dt, or_rhs_fn rhs, or_norm_fn norm, void* user,
                       double* dense_out, double* dense_t, int32_t max_dense,
                       double* h_log, int8_t* acc_log, int64_t log_cap,
                       or_dp5_stats* stats) {
+    or_dp5_integrate_ex(n, y, t0, t_final, dense_dt, rhs, norm, user, NULL, NULL, NULL, dense_out,
+                        dense_t, max_dense, h_log, acc_log, log_cap, stats);
+}
+
+void or_dp5_integrate_ex(int64_t n, double* y, double t0, double t_final,
+                         double dense_dt, or_rhs_fn rhs, or_norm_fn norm, void* user,
+                         const or_bath_sched* bs, or_flux_fn flux, double* ledger,
+                         double* dense_out, double* dense_t, int32_t max_dense,
+                         double* h_log, int8_t* acc_log, int64_t log_cap,
+                         or_dp5_stats* stats) {
     dp5_work w;
     dp5_alloc(&w, n);
     or_dp5_stats st;
@@ -542,8 +601,11 @@ void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
     const double alpha = 0.7 / 5., beta = 0.4 / 5.;
     double olderror = 1.0;
     int notfailed = 1, rejected = 0;
+    double e1 = 0.0;   /* flux of k1 (FSAL with k7) */
     rhs(user, t, y, w.k1);
+    if (flux) e1 = flux(user, w.k1);
     st.evals = 1;
+    h = fmin(h, or_step_vetoer(bs, t, h, dense_dt));
     while (t < t_final && notfailed) {
         dp5_attempt(&w, y, t, h, rhs, user);
         st.evals += 6;
@@ -562,7 +624,15 @@ void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
                 ++st.n_dense;
                 dense_time += dense_dt;
             }
+            if (flux) {
+                const double e3 = flux(user, w.k3), e4 = flux(user, w.k4), e5 = flux(user, w.k5);
+                const double e6 = flux(user, w.k6), e7 = flux(user, w.k7);
+                const double de = (h * B1) * e1 + (h * B3) * e3 + (h * B4) * e4 + (h * B5) * e5 + (h * B6) * e6;
+                *ledger -= de;
+                e1 = e7;
+            }
             memcpy(y, w.y5, sizeof(double) * n);
+            const double hstep = h;
             t += h;
             double* tmp = w.k1; w.k1 = w.k7; w.k7 = tmp;
             double scale;
@@ -572,8 +642,9 @@ void or_dp5_integrate(int64_t n, double* y, double t0, double t_final,
                 scale = safety * pow(1. / en, alpha) * pow(olderror, beta);
                 scale = fmin(fmax(scale, minscale), maxscale);
             }
-            if (rejected) h *= fmin(1.0, scale);
-            else h *= scale;
+            if (rejected) h = hstep * fmin(1.0, scale);
+            else h = hstep * scale;
+            h = fmin(h, or_step_vetoer(bs, t, h, dense_dt));
             rejected = 0;
             olderror = en;
             ++st.accepted;
@@ -598,15 +669,25 @@ typedef struct coll_user {
     const double* K;
     const double* V;
     double bath_n;
+    const or_bath_sched* bs;   /* non-NULL: the bath at the stage time */
+    const double* eps;         /* ledger weights V_i eps_i come from V and eps */
     int conserve;
     int32_t* conn;
     double rtol, atol;
 } coll_user;
 
 static void coll_rhs(void* u, double t, const double* y, double* dydt) {
-    (void)t;
     coll_user* c = (coll_user*)u;
-    or_collision_apply(c->n, c->K, c->V, c->bath_n, y, dydt, c->conserve);
+    const double nb = c->bs ? or_sched_value(c->bs->s, c->bs->ns, c->bs->t_start, c->bs->dt, t) : c->bath_n;
+    or_collision_apply(c->n, c->K, c->V, nb, y, dydt, c->conserve);
+}
+
+/* electron energy flux sum_i V_i eps_i dydt_i (sequential) */
+static double coll_flux(void* u, const double* dydt) {
+    coll_user* c = (coll_user*)u;
+    double s = 0.0;
+    for (int64_t i = 0; i < c->n; ++i) s += c->V[i] * c->eps[i] * dydt[i];
+    return s;
 }
 
 static double coll_norm(void* u, const double* err, const double* y5) {
@@ -620,14 +701,33 @@ void or_dp5_collision(int dim, const int32_t* nsec, int64_t n, const double* K,
                       int conserve, double* dense_out, double* dense_t,
                       int32_t max_dense, double* h_log, int8_t* acc_log,
                       int64_t log_cap, or_dp5_stats* stats) {
+    or_dp5_collision_ex(dim, nsec, n, K, V, NULL, bath_n, NULL, 0, 0.0, 1.0, 0.0, 0.0, f, t0, t_final,
+                        dense_dt, rtol, atol, conserve, dense_out, dense_t, max_dense, h_log, acc_log,
+                        log_cap, stats, NULL);
+}
+
+void or_dp5_collision_ex(int dim, const int32_t* nsec, int64_t n, const double* K,
+                         const double* V, const double* eps, double bath_n,
+                         const double* sched, int32_t ns, double t_start, double sdt,
+                         double satol, double srtol, double* f, double t0,
+                         double t_final, double dense_dt, double rtol, double atol,
+                         int conserve, double* dense_out, double* dense_t,
+                         int32_t max_dense, double* h_log, int8_t* acc_log,
+                         int64_t log_cap, or_dp5_stats* stats, double* ledger) {
     coll_user c;
+    memset(&c, 0, sizeof(c));
+    or_bath_sched bs = {sched, ns, t_start, sdt, satol, srtol};
     c.dim = dim; c.n = n; c.nel = or_num_elements(dim, nsec);
     c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    c.bs = sched ? &bs : NULL;
+    c.eps = eps;
     c.rtol = rtol; c.atol = atol;
     c.conn = (int32_t*)malloc(sizeof(int32_t) * c.nel * (dim + 1));
     or_mesh_elements(dim, nsec, c.conn);
-    or_dp5_integrate(n, f, t0, t_final, dense_dt, coll_rhs, coll_norm, &c, dense_out,
-                     dense_t, max_dense, h_log, acc_log, log_cap, stats);
+    if (ledger) *ledger = 0.0;
+    or_dp5_integrate_ex(n, f, t0, t_final, dense_dt, coll_rhs, coll_norm, &c, c.bs,
+                        (ledger && eps) ? coll_flux : NULL, ledger, dense_out, dense_t, max_dense,
+                        h_log, acc_log, log_cap, stats);
     free(c.conn);
 }
 
@@ -638,19 +738,34 @@ void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
                              const int8_t* acc_log, int64_t n_attempts,
                              double* dense_out, double* dense_t,
                              int32_t max_dense) {
-    (void)dim; (void)nsec;
-    coll_user c;
-    memset(&c, 0, sizeof(c));
-    c.n = n; c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    or_dp5_collision_replay_ex(dim, nsec, n, K, V, NULL, bath_n, NULL, 0, 0.0, 1.0, f, t0, t_final,
+                               dense_dt, conserve, h_log, acc_log, n_attempts, dense_out, dense_t,
+                               max_dense, NULL);
+}
+
+/* the fixed-step driver shared by the replays: accepted attempts of
+ * (h_log, acc_log) in order, dense outputs and, with flux, the ledger */
+void or_dp5_replay_ex(int64_t n, double* f, double t0, double t_final, double dense_dt,
+                      or_rhs_fn rhs, void* user, or_flux_fn flux, double* ledger,
+                      const double* h_log, const int8_t* acc_log, int64_t n_attempts,
+                      double* dense_out, double* dense_t, int32_t max_dense) {
     dp5_work w;
     dp5_alloc(&w, n);
     double t = t0, dense_time = t0 + dense_dt;
     int32_t nd = 0;
-    coll_rhs(&c, t, f, w.k1);
+    rhs(user, t, f, w.k1);
+    double e1 = flux ? flux(user, w.k1) : 0.0;
+    if (ledger) *ledger = 0.0;
     for (int64_t a = 0; a < n_attempts && t < t_final; ++a) {
         if (!acc_log[a]) continue;
         double h = h_log[a];
-        dp5_attempt(&w, f, t, h, coll_rhs, &c);
+        dp5_attempt(&w, f, t, h, rhs, user);
+        if (flux) {
+            const double e3 = flux(user, w.k3), e4 = flux(user, w.k4), e5 = flux(user, w.k5);
+            const double e6 = flux(user, w.k6), e7 = flux(user, w.k7);
+            *ledger -= (h * B1) * e1 + (h * B3) * e3 + (h * B4) * e4 + (h * B5) * e5 + (h * B6) * e6;
+            e1 = e7;
+        }
         while (dense_time <= t + 1.0001 * h && dense_time <= t_final + .0001 * h) {
             double s = (dense_time - t) / h;
             if (nd < max_dense && dense_out) {
@@ -665,6 +780,25 @@ void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
         double* tmp = w.k1; w.k1 = w.k7; w.k7 = tmp;
     }
     dp5_free(&w);
+}
+
+void or_dp5_collision_replay_ex(int dim, const int32_t* nsec, int64_t n,
+                                const double* K, const double* V, const double* eps,
+                                double bath_n, const double* sched, int32_t ns,
+                                double t_start, double sdt, double* f, double t0,
+                                double t_final, double dense_dt, int conserve,
+                                const double* h_log, const int8_t* acc_log,
+                                int64_t n_attempts, double* dense_out, double* dense_t,
+                                int32_t max_dense, double* ledger) {
+    (void)dim; (void)nsec;
+    coll_user c;
+    memset(&c, 0, sizeof(c));
+    or_bath_sched bs = {sched, ns, t_start, sdt, 0.0, 0.0};
+    c.n = n; c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    c.bs = sched ? &bs : NULL;
+    c.eps = eps;
+    or_dp5_replay_ex(n, f, t0, t_final, dense_dt, coll_rhs, &c, (ledger && eps) ? coll_flux : NULL, ledger,
+                     h_log, acc_log, n_attempts, dense_out, dense_t, max_dense);
 }
 
 /* ============================================================ adjoint */

@@ -118,6 +118,36 @@ typedef struct or_dp5_stats {
     int32_t n_dense;      /* dense outputs written */
 } or_dp5_stats;
 
+/* Time-dependent bath of a constrained phonon band (uniform population):
+ * samples s[0..ns) at t_start + k*dt, linear between them, held outside
+ * (Band::constrainedPopulation, Band.hpp:70-74). */
+typedef struct or_bath_sched {
+    const double* s;
+    int32_t ns;
+    double t_start, dt;
+    double atol, rtol;      /* the constrained band's error tolerances (vetoer) */
+} or_bath_sched;
+
+double or_sched_value(const double* s, int32_t ns, double t_start, double dt, double t);
+/* MaterialTimeStatus::timeStepVetoer (MaterialTimeStatus.cpp:60-100) with
+ * timesToTest = {t, t+c2 h, t+c3 h, t+c4 h, t+c5 h, t+h}; bs NULL: no
+ * constrained band (returns (t+h) - t). */
+double or_step_vetoer(const or_bath_sched* bs, double t, double h, double dense_dt);
+
+/* Electron energy flux functional of a derivative (bath energy ledger). */
+typedef double (*or_flux_fn)(void* user, const double* dydt);
+
+/* or_dp5_integrate with the step vetoer of a constrained bath (bs, NULL for
+ * none: the vetoer then leaves h at (t+h)-t as the reference's does) and,
+ * when flux != NULL, the bath energy ledger: *ledger -= sum_j (h b_j) flux(k_j)
+ * over the accepted steps. */
+void or_dp5_integrate_ex(int64_t n, double* y, double t0, double t_final,
+                         double dense_dt, or_rhs_fn rhs, or_norm_fn norm, void* user,
+                         const or_bath_sched* bs, or_flux_fn flux, double* ledger,
+                         double* dense_out, double* dense_t, int32_t max_dense,
+                         double* h_log, int8_t* acc_log, int64_t log_cap,
+                         or_dp5_stats* stats);
+
 /* Integrates y' = rhs(t,y) from t0 to t_final.  Dense outputs at t0+k*dense_dt
  * (k>=1) are written to dense_out[(k-1)*n ...] (capacity max_dense) with
  * their times in dense_t.  y is overwritten with the last accepted y5.
@@ -138,6 +168,19 @@ void or_dp5_collision(int dim, const int32_t* nsec, int64_t n, const double* K,
                       int32_t max_dense, double* h_log, int8_t* acc_log,
                       int64_t log_cap, or_dp5_stats* stats);
 
+/* or_dp5_collision with a time-dependent bath (sched NULL: constant bath_n;
+ * else ns samples, t_start, sdt, and the band's satol, srtol) and the bath
+ * energy ledger (energy received by the bath, with eps the nodal energies:
+ * -sum_j (h b_j) sum_i V_i eps_i k_j,i over the accepted steps). */
+void or_dp5_collision_ex(int dim, const int32_t* nsec, int64_t n, const double* K,
+                         const double* V, const double* eps, double bath_n,
+                         const double* sched, int32_t ns, double t_start, double sdt,
+                         double satol, double srtol, double* f, double t0,
+                         double t_final, double dense_dt, double rtol, double atol,
+                         int conserve, double* dense_out, double* dense_t,
+                         int32_t max_dense, double* h_log, int8_t* acc_log,
+                         int64_t log_cap, or_dp5_stats* stats, double* ledger);
+
 /* Fixed-step replay of a recorded attempt sequence (h_log/acc_log): the same
  * stage arithmetic as or_dp5_integrate with the controller taken out, so a
  * second precision can be compared on identical steps.  Writes the state
@@ -149,6 +192,25 @@ void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
                              const int8_t* acc_log, int64_t n_attempts,
                              double* dense_out, double* dense_t,
                              int32_t max_dense);
+
+/* Fixed-step DP5 over the accepted attempts of (h_log, acc_log) with the
+ * same stage arithmetic and dense output as or_dp5_integrate_ex; with flux
+ * the bath energy ledger of those steps. */
+void or_dp5_replay_ex(int64_t n, double* f, double t0, double t_final, double dense_dt,
+                      or_rhs_fn rhs, void* user, or_flux_fn flux, double* ledger,
+                      const double* h_log, const int8_t* acc_log, int64_t n_attempts,
+                      double* dense_out, double* dense_t, int32_t max_dense);
+
+/* or_dp5_collision_replay with a time-dependent bath (sched NULL: bath_n)
+ * and, with eps and ledger, the bath energy ledger of the replayed steps. */
+void or_dp5_collision_replay_ex(int dim, const int32_t* nsec, int64_t n,
+                                const double* K, const double* V, const double* eps,
+                                double bath_n, const double* sched, int32_t ns,
+                                double t_start, double sdt, double* f, double t0,
+                                double t_final, double dense_dt, int conserve,
+                                const double* h_log, const int8_t* acc_log,
+                                int64_t n_attempts, double* dense_out, double* dense_t,
+                                int32_t max_dense, double* ledger);
 
 /* ------------------------------------------------------------ adjoint */
 /* Derivative of K with respect to omega0 (the sampled G' table, same
@@ -228,6 +290,29 @@ void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const 
                       double t_final, double dense_dt,
                       double rtol, double atol, double* dense_out, double* dense_t,
                       int32_t max_dense, int32_t nthreads, or_dp5_stats* stats);
+/* or_dp5_mc_eph with a time-dependent bath (sched NULL: constant bath_n) and
+ * the bath energy ledger (-int eps I dt of the electron band; 0 when dynamic). */
+void    or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                         const double* eps, const double* eps_ph, double omega0, double amplitude,
+                         int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                         int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
+                         int32_t ns, double t_start, double sdt, double satol, double srtol,
+                         double* f, double t0, double t_final, double dense_dt,
+                         double rtol, double atol, double* dense_out, double* dense_t,
+                         int32_t max_dense, int32_t nthreads, or_dp5_stats* stats, double* ledger);
+/* or_dp5_mc_eph_ex on a given step sequence (all accepted): the tuple
+ * operator's fixed-step replay with the ledger. */
+void    or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, const double* extent,
+                             const double* eps, const double* eps_ph, double omega0, double amplitude,
+                             int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
+                             int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
+                             int32_t ns, double t_start, double sdt, double* f, double t0,
+                             double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
+                             double* dense_out, double* dense_t, int32_t max_dense, int32_t nthreads,
+                             double* ledger);
+/* Electron energy int eps f dk of DG coefficients [nel][dim+1] (exact P1 x P1). */
+double  or_energy_dg(int dim, const int32_t* nsec, const double* extent, const double* eps,
+                     const double* f);
 /* ScatteringChannel::scatteringRates of the electron band (legs 0 and 2). */
 void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
                         const double* eps, const double* eps_ph, double omega0, double amplitude,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 using namespace Tortoise;
 
@@ -57,6 +58,22 @@ int main() {
     MaterialStatus<2> s(mat);
     s.setToEquilibrium(0.02, 0.5, 0.0);
     CHECK(std::fabs(s.bathOccupation() - 1.0 / std::expm1(0.05 / 0.02)) < 1e-15);
+    // a non-uniform phonon population is refused, not replaced by one value
+    MaterialStatus<2> s2 = s;
+    s2["pn:o"] = Function<2>(m, [](Point<2> k) { return 0.1 + 0.01 * k(0); });
+    bool threw = false;
+    try { (void)s2.bathOccupation(); } catch (const std::runtime_error&) { threw = true; }
+    CHECK(threw);
+    // a constrained electron band is refused before any device work
+    mat["el:C"].constrained = true;
+    mat["el:C"].constrainedPopulation = [&m](Real) { return Function<2>(m, 0.5); };
+    MaterialTimeStatus<2> ts(s);
+    threw = false;
+    try { ts.propagate(1.0, 0.1); } catch (const std::runtime_error& e) {
+        threw = std::string(e.what()).find("constrained electron band") != std::string::npos;
+    }
+    CHECK(threw);
+    mat["el:C"].constrained = false;
     if (fails == 0) std::printf("ok\n");
     return fails ? 1 : 0;
 }

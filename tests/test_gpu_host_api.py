@@ -1,5 +1,6 @@
 """The C++ host-layer example runs the GPU operator through the Tortoise::
 API and agrees with the same integration driven through the C ABI."""
+import math
 import os
 import subprocess
 
@@ -157,3 +158,50 @@ def test_host_example_dynamical_phonons_matches_abi():
     # total energy: electrons (exact P1 x P1) plus phonons (omega0 x number)
     assert abs((en_end + ph_en) - (e0_el + 0.05 * integral(fp))) <= 1e-10 * abs(e0_el)
     op.close()
+
+
+def test_host_example_pulsed_bath_matches_abi():
+    """Band::constrained with a time-dependent uniform population: the host
+    layer samples constrainedPopulation (64 per dense step) into the device
+    bath schedule; the same samples through the C ABI give the same run."""
+    assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
+    print("[gpu] host example 16 sections t=10 pulsed bath", flush=True)
+    r = subprocess.run([EXE, "16", "10", "1", "0", "0", "0", "1.5"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    steps = int(lines[0].split()[1])
+    fin = next(x for x in lines if x.startswith("final")).split("|")
+    num_end, en_end = float(fin[1]), float(fin[2])
+    bath_end = float(next(x for x in lines if x.startswith("bath at the end")).split()[-1])
+    w = wl.Workload("square16", 2, (16, 16), t_final=10.0, rtol=1e-8, atol=1e-12)
+    # the example's pulse, n_BE + A (1 - x^2)^2 on |x| < 1, sampled with the
+    # same IEEE operations (no transcendental beyond expm1 of the constant)
+    nbe, tp, width = 1.0 / math.expm1(0.05 / 0.02), 10.0 / 4, 10.0 / 10
+
+    def pulse(t):
+        x = (t - tp) / width
+        q = 1.0 - x * x
+        return nbe + np.where(np.abs(x) < 1.0, 1.5 * q * q, 0.0)
+
+    dt = (10.0 / 100.0) / 64
+    ns = int(np.ceil(2.0 * 10.0 / dt)) + 2
+    samples = pulse(np.arange(ns) * dt)
+    ctx = tg.Context.create(w, max_batch=1)
+    ctx.assemble()
+    ctx.set_bath_schedule(samples, 0.0, dt, atol=1e-3, rtol=1e-3)
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100, schedule=True)
+    assert st["attempted"] == steps
+    V = ctx.node_volumes()
+    assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
+    conn = oracle.mesh_elements(2, [16, 16])
+    eps = w.eps()
+    vol = (2 * np.pi / 16) ** 2 / 2
+    a, b = dense[-1][conn], eps[conn]
+    en = vol / 12 * ((a * b).sum() + (a.sum(1) * b.sum(1)).sum())
+    assert abs(en - en_end) <= 1e-10 * abs(en)
+    # the pulse did change the run, and the last dense state's phonon band is
+    # the constraint at t = 10 (enforceValue at the dense time)
+    _, dc, _ = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100)
+    assert np.abs(dc[-1] - dense[-1]).max() > 1e-6
+    assert bath_end == pulse(np.array([10.0]))[0]

@@ -148,8 +148,9 @@ def test_set_indices_rejects_bad_lists():
         op.set_indices(np.array([[0, 1, nel, 0]], np.int32))
     with pytest.raises(tg.TgError, match="umklapp"):
         op.set_indices(np.array([[0, 1, 2, 9]], np.int32))
-    with pytest.raises(tg.TgError, match="ordered"):
-        op.set_indices(np.array([[3, 1, 2, 0], [2, 1, 2, 0]], np.int32))
+    # a list out of leg-0 order is accepted and kept in leg-0 order
+    op.set_indices(np.array([[3, 1, 2, 0], [2, 1, 2, 0]], np.int32))
+    assert np.array_equal(op.indices(), [[2, 1, 2, 0], [3, 1, 2, 0]])
     with pytest.raises(tg.TgError, match="MC points"):
         op.propagate(np.zeros((nel, 3)), 0.1)
     op.close()

@@ -37,6 +37,7 @@ TG_PREC_MIXED = 0x2
 TG_CONSERVE = 0x4
 TG_QUIET = 0x8
 TG_TIME_KERNEL = 0x10
+TG_BATH_SCHEDULE = 0x20
 
 # every symbol include/tortoise_gpu.h declares
 EXPORTS = [
@@ -49,7 +50,7 @@ EXPORTS = [
     "tg_tuple_set_points", "tg_tuple_count", "tg_tuple_copy_indices", "tg_tuple_set_indices",
     "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
     "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream", "tg_tuple_dp5",
-    "tg_tuple_rates",
+    "tg_tuple_rates", "tg_set_bath_schedule", "tg_tuple_set_bath_schedule",
 ]
 
 
@@ -86,10 +87,30 @@ class KernelStats(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("attempted", C.c_int64), ("accepted", C.c_int64), ("rejected", C.c_int64),
-                ("evals", C.c_int64), ("failed", C.c_int32), ("n_dense", C.c_int32)]
+                ("evals", C.c_int64), ("failed", C.c_int32), ("n_dense", C.c_int32),
+                ("bath_energy", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BathScheduleDesc(C.Structure):
+    _fields_ = [("samples", C.POINTER(C.c_double)), ("n_samples", C.c_int32),
+                ("batch", C.c_int32), ("t_start", C.c_double), ("dt", C.c_double),
+                ("atol", C.c_double), ("rtol", C.c_double)]
+
+
+def _schedule_desc(samples, t_start, dt, atol, rtol):
+    """samples [ns] or [batch, ns] -> (descriptor, array kept alive)."""
+    a = np.ascontiguousarray(samples, np.float64)
+    if a.ndim == 1:
+        a = a[None]
+    d = BathScheduleDesc()
+    d.samples = a.ctypes.data_as(C.POINTER(C.c_double))
+    d.n_samples = a.shape[1]
+    d.batch = a.shape[0]
+    d.t_start, d.dt, d.atol, d.rtol = t_start, dt, atol, rtol
+    return d, a
 
 
 class TupleDesc(C.Structure):
@@ -178,9 +199,13 @@ def load():
         fn.argtypes = args
     L.tg_tuple_stream.restype = vp
     L.tg_tuple_stream.argtypes = [vp]
+    L.tg_set_bath_schedule.restype = i32
+    L.tg_set_bath_schedule.argtypes = [vp, C.POINTER(BathScheduleDesc)]
+    L.tg_tuple_set_bath_schedule.restype = i32
+    L.tg_tuple_set_bath_schedule.argtypes = [vp, C.POINTER(BathScheduleDesc)]
     L.tg_tuple_dp5.restype = i32
     L.tg_tuple_dp5.argtypes = [vp, vp, i64, i32, f64, f64, f64, f64, f64, vp, i32, vp, i32,
-                               C.POINTER(StepStats)]
+                               C.POINTER(StepStats), vp, i32]
     _lib = L
     return L
 
@@ -310,10 +335,22 @@ class Context:
                                              _p(out), 0))
         return out[0] if err.ndim == 1 else out
 
+    def set_bath_schedule(self, samples=None, t_start=0.0, dt=1.0, atol=1e-3, rtol=1e-3):
+        """Time-dependent bath n_b(t): samples [ns] or [batch, ns] at
+        t_start + k dt (tg_set_bath_schedule); samples None clears it."""
+        if samples is None:
+            self._check(load().tg_set_bath_schedule(self._h, None))
+            return
+        d, keep = _schedule_desc(samples, t_start, dt, atol, rtol)
+        self._check(load().tg_set_bath_schedule(self._h, C.byref(d)))
+        del keep
+
     def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, mixed=False, conserve=False,
-            max_dense=0, quiet=True, raise_on_fail=True, h_cap=0):
+            max_dense=0, quiet=True, raise_on_fail=True, h_cap=0, schedule=False):
         """Returns (f_last, dense[(batch,) max_dense, n], stats) and, with
-        h_cap > 0, the accepted step sizes per member as a 4th element."""
+        h_cap > 0, the accepted step sizes per member as a 4th element.
+        schedule=True: the bath follows set_bath_schedule (TG_BATH_SCHEDULE).
+        stats[b]["bath_energy"] is the bath's energy ledger."""
         f0 = np.ascontiguousarray(f0, np.float64)
         batch = 1 if f0.ndim == 1 else f0.shape[1]
         f = np.ascontiguousarray(f0.T if f0.ndim == 2 else f0).copy()
@@ -323,7 +360,7 @@ class Context:
                                                               np.float64)
         hl = np.zeros((batch, h_cap)) if h_cap else None
         flags = ((TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
-                 | (TG_QUIET if quiet else 0))
+                 | (TG_QUIET if quiet else 0) | (TG_BATH_SCHEDULE if schedule else 0))
         rc = load().tg_dp5_integrate(self._h, _p(f), self.n, batch, t0, t1, dense_dt, rtol, atol,
                                      None if bn is None else _p(bn), flags,
                                      None if dense is None else _p(dense), max_dense, st,
@@ -341,8 +378,9 @@ class Context:
         return out + ((steps,) if steps is not None else ())
 
     def dp5_replay(self, f0, t0, t1, dense_dt, steps, bath_n=None, mixed=False, conserve=False,
-                   max_dense=0):
-        """Fixed-step DP5 on per-member accepted step lists; returns (f_last, dense)."""
+                   max_dense=0, schedule=False):
+        """Fixed-step DP5 on per-member accepted step lists; returns (f_last, dense).
+        schedule=True: the bath follows set_bath_schedule."""
         f0 = np.ascontiguousarray(f0, np.float64)
         batch = 1 if f0.ndim == 1 else f0.shape[1]
         if f0.ndim == 1:
@@ -357,7 +395,8 @@ class Context:
         dense = np.zeros((batch, max_dense, self.n)) if max_dense else None
         bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
                                                               np.float64)
-        flags = (TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+        flags = ((TG_PREC_MIXED if mixed else 0) | (TG_CONSERVE if conserve else 0)
+                 | (TG_BATH_SCHEDULE if schedule else 0))
         self._check(load().tg_dp5_replay(self._h, _p(f), self.n, batch, t0, t1, dense_dt, _p(H),
                                          _p(ns), cap, None if bn is None else _p(bn), flags,
                                          None if dense is None else _p(dense), max_dense))
@@ -536,10 +575,22 @@ class TupleOperator:
     def stream(self):
         return load().tg_tuple_stream(self._h)
 
-    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, max_dense=0, quiet=True):
+    def set_bath_schedule(self, samples=None, t_start=0.0, dt=1.0, atol=1e-3, rtol=1e-3):
+        """Time-dependent bath (tg_tuple_set_bath_schedule); samples None clears it."""
+        if samples is None:
+            self._check(load().tg_tuple_set_bath_schedule(self._h, None))
+            return
+        d, keep = _schedule_desc(samples, t_start, dt, atol, rtol)
+        self._check(load().tg_tuple_set_bath_schedule(self._h, C.byref(d)))
+        del keep
+
+    def dp5(self, f0, t0, t1, dense_dt, rtol, atol, bath_n=None, max_dense=0, quiet=True,
+            schedule=False, h_cap=0):
         """DP5(4) of DG states f0 [nel, dim+1] or [batch, nel, dim+1] (with
         dynamical phonons [(batch,) 2, nel, dim+1]); returns (f_last,
-        dense [(batch,) max_dense, ...], stats)."""
+        dense [(batch,) max_dense, ...], stats) and, with h_cap > 0, the
+        accepted step sizes per member.  schedule=True: the bath follows
+        set_bath_schedule."""
         f = np.ascontiguousarray(f0, np.float64).copy()
         single = f.ndim == (3 if self.dynamic else 2)
         fb = f[None] if single else f
@@ -549,11 +600,19 @@ class TupleOperator:
             bn = np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)), np.float64)
         dense = np.zeros((batch, max_dense) + fb.shape[1:]) if max_dense else None
         st = (StepStats * batch)()
+        flags = (TG_QUIET if quiet else 0) | (TG_BATH_SCHEDULE if schedule else 0)
+        hl = np.zeros((batch, h_cap)) if h_cap else None
         rc = load().tg_tuple_dp5(self._h, _p(fb), self.nel, batch, t0, t1, dense_dt, rtol, atol,
-                                 None if bn is None else _p(bn), TG_QUIET if quiet else 0,
-                                 None if dense is None else _p(dense), max_dense, st)
+                                 None if bn is None else _p(bn), flags,
+                                 None if dense is None else _p(dense), max_dense, st,
+                                 None if hl is None else _p(hl), h_cap)
         self._check(rc)
         stats = [{k: getattr(s, k) for k, _ in StepStats._fields_} for s in st]
+        steps = None
+        if hl is not None:
+            steps = [hl[b, : min(stats[b]["accepted"], h_cap)].copy() for b in range(batch)]
         if single:
-            return fb[0], (None if dense is None else dense[0][:stats[0]["n_dense"]]), stats[0]
-        return fb, dense, stats
+            out = (fb[0], (None if dense is None else dense[0][:stats[0]["n_dense"]]), stats[0])
+            return out + ((steps[0],) if steps is not None else ())
+        out = (fb, dense, stats)
+        return out + ((steps,) if steps is not None else ())

@@ -32,5 +32,18 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host_example(verbose: bool = False) -> str:
+    """The C++ host-layer example (tortoise_amd/host), linked against the
+    library just built: its GPU tests run the binary."""
+    cmd = ["make", "-B", "-C", os.path.join(HERE, "host")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the host-layer example failed")
+    return os.path.join(HERE, "host", "examples", "thermalisation_square")
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

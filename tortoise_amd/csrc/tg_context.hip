@@ -288,13 +288,19 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     for (int64_t i = 0; i < c->n; ++i) c->h_perm[i] = (int32_t)i;
     std::stable_sort(c->h_perm.begin(), c->h_perm.end(),
                      [&](int32_t a, int32_t b) { return d->eps[a] < d->eps[b]; });
-    std::vector<double> Vs(c->n);
-    for (int64_t p = 0; p < c->n; ++p) Vs[p] = V[c->h_perm[p]];
+    std::vector<double> Vs(c->n), Es(c->n);
+    for (int64_t p = 0; p < c->n; ++p) {
+        Vs[p] = V[c->h_perm[p]];
+        Es[p] = d->eps[c->h_perm[p]];
+    }
     if ((s = dev_alloc(c, &c->d_perm, c->n)) != TG_OK) return fail(s);
     if ((s = dev_alloc(c, &c->d_V_s, c->n)) != TG_OK) return fail(s);
+    if ((s = dev_alloc(c, &c->d_eps_s, c->n)) != TG_OK) return fail(s);
     if ((s = dev_alloc(c, &c->d_fs, c->n)) != TG_OK) return fail(s);
+    if ((s = dev_alloc(c, &c->d_bath, (size_t)c->max_batch)) != TG_OK) return fail(s);
     e = hipMemcpy(c->d_perm, c->h_perm.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_V_s, Vs.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_eps_s, Es.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
 
     // energy grid (same rule as the CPU restatement: e0 = min - de)
@@ -344,6 +350,9 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_vpart);
     dev_free(&c->d_perm);
     dev_free(&c->d_V_s);
+    dev_free(&c->d_eps_s);
+    dev_free(&c->d_bath);
+    dev_free(&c->d_bsched);
     dev_free(&c->d_rk_s);
     dev_free(&c->d_ck_s);
     dev_free(&c->d_fs);
@@ -425,5 +434,36 @@ extern "C" tg_status tg_last_kernel_ms(tg_ctx* c, double* ms) {
     float v = 0.f;
     TG_HIP(c, hipEventElapsedTime(&v, c->ev_k0, c->ev_k1));
     *ms = v;
+    return TG_OK;
+}
+
+extern "C" tg_status tg_set_bath_schedule(tg_ctx* c, const tg_bath_schedule* s) {
+    if (!c) return TG_ERR_ARG;
+    if (!s) {
+        c->h_bsched.clear();
+        c->bs_ns = c->bs_batch = 0;
+        return TG_OK;
+    }
+    if (!s->samples || s->n_samples < 2 || s->batch < 1 || s->batch > c->max_batch)
+        return set_error(c, TG_ERR_ARG, "bath schedule needs samples, n_samples >= 2 and batch in [1, max_batch]");
+    if (!(s->dt > 0.0) || !std::isfinite(s->t_start) || !std::isfinite(s->dt) || !(s->atol >= 0.0) ||
+        !(s->rtol >= 0.0))
+        return set_error(c, TG_ERR_ARG, "bath schedule needs dt > 0, finite t_start and atol, rtol >= 0");
+    if (!(s->atol > 0.0) && !(s->rtol > 0.0))
+        return set_error(c, TG_ERR_ARG, "bath schedule: atol and rtol both 0 make the vetoer divide by 0");
+    const size_t cnt = (size_t)s->batch * s->n_samples;
+    for (size_t i = 0; i < cnt; ++i)
+        if (!std::isfinite(s->samples[i]))
+            return set_error(c, TG_ERR_NONFINITE, "bath schedule sample %zu is not finite", i);
+    dev_free(&c->d_bsched);
+    TG_TRY(dev_alloc(c, &c->d_bsched, cnt));
+    TG_HIP(c, hipMemcpy(c->d_bsched, s->samples, sizeof(double) * cnt, hipMemcpyHostToDevice));
+    c->h_bsched.assign(s->samples, s->samples + cnt);
+    c->bs_ns = s->n_samples;
+    c->bs_batch = s->batch;
+    c->bs_t0 = s->t_start;
+    c->bs_dt = s->dt;
+    c->bs_atol = s->atol;
+    c->bs_rtol = s->rtol;
     return TG_OK;
 }

@@ -6,8 +6,14 @@
 // step size, PI-controller memory and reject flag; an attempted step runs the
 // six stage evaluations for all unfinished members at once (one batched
 // operator call per stage), then each member is accepted or rejected on its
-// own error norm (Band::relativeError, Band.cpp:87-101).  enforceValue and the
-// step vetoer are the identity (no constrained bands in this operator).
+// own error norm (Band::relativeError, Band.cpp:87-101).  The phonon band is
+// a bath: constant, or constrained to a per-member occupation n_b(t)
+// (tg_set_bath_schedule), which enforceValue writes at every stage time
+// (MaterialStatus::applyConstraints, MaterialStatus.cpp:198-206) and the
+// step vetoer reads (MaterialTimeStatus::timeStepVetoer,
+// MaterialTimeStatus.cpp:60-100).  The e-ph epilogue of every stage returns
+// the member's electron energy flux; the controller integrates it with the
+// step weights into the bath's energy ledger.
 //
 // The whole attempt runs on the device and is captured once as a HIP graph:
 //   prep_kernel        members still running take their step size (0 for the
@@ -48,6 +54,7 @@ struct Dp5Work {
     size_t sched_cap = 0;
     double* hlog = nullptr;   // accepted step sizes (B x h_cap)
     size_t hlog_cap = 0;
+    double* flux = nullptr;   // e-ph electron energy flux per stage (k1..k7) and member: 7 x cap
     int32_t* running = nullptr;        // device flag
     int32_t* running_host = nullptr;   // pinned copy
     double* dense = nullptr;  // device dense buffer when the caller's is host memory
@@ -76,6 +83,7 @@ static tg_status dp5_ensure(tg_ctx* c, int32_t B) {
     TG_TRY(dev_alloc(c, &w->norms, (size_t)w->cap));
     TG_TRY(dev_alloc(c, &w->npart, (size_t)w->cap * relerr_blocks(c)));
     TG_TRY(dev_alloc(c, &w->nsched, (size_t)w->cap));
+    TG_TRY(dev_alloc(c, &w->flux, (size_t)7 * w->cap));
     TG_TRY(dev_alloc(c, &w->running, (size_t)1));
     TG_HIP(c, hipHostMalloc((void**)&w->running_host, sizeof(int32_t), hipHostMallocDefault));
     return TG_OK;
@@ -96,6 +104,7 @@ void dp5_destroy(tg_ctx* c) {
     dev_free(&w->npart);
     dev_free(&w->sched);
     dev_free(&w->nsched);
+    dev_free(&w->flux);
     dev_free(&w->hlog);
     dev_free(&w->running);
     if (w->running_host) (void)hipHostFree(w->running_host);
@@ -104,60 +113,66 @@ void dp5_destroy(tg_ctx* c) {
     c->dp5 = nullptr;
 }
 
-// operator evaluation for the whole batch
-// (bath set beforehand for the batched path); kernel launches only.  On the
-// batched path members whose step is 0 this attempt (finished: gate = the
+// operator evaluation for the whole batch with the per-member baths in
+// d_bath (device); kernel launches only.  flux (device, one per member)
+// receives the e-ph electron energy flux of each member.  On the batched
+// path members whose step is 0 this attempt (finished: gate = the
 // per-member step sizes) are skipped: the controller reads nothing of theirs.
 static tg_status eval_batch(tg_ctx* c, const double* d_in, double* d_out, int32_t B,
-                            const std::vector<double>& bath, int32_t flags, const double* gate) {
+                            const double* d_bath, int32_t flags, const double* gate, double* flux) {
     const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
     if (flags & TG_PREC_MIXED) {
-        TG_TRY(mixed_eval_device(c, d_in, d_out, B, conserve, gate));
+        TG_TRY(mixed_eval_device(c, d_in, d_out, B, conserve, gate, flux));
     } else {
         for (int b = 0; b < B; ++b)
-            TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, bath[b],
-                                      conserve));
+            TG_TRY(plain_apply_device(c, d_in + (int64_t)b * c->n, d_out + (int64_t)b * c->n, 0.0, conserve,
+                                      d_bath + b, flux + b));
         gate = nullptr;
     }
     return ee_apply_device(c, d_in, d_out, B, gate);
 }
 
 // the device work of one attempted step (captured into a graph)
-static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl,
-                         const std::vector<double>& bath, int32_t flags, double rtol, double atol,
-                         double* d_dense, double* d_hlog) {
+static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, double* d_bath,
+                         int32_t flags, double rtol, double atol, double* d_dense, double* d_hlog) {
     hipStream_t st = c->stream;
     const int64_t n = c->n;
     double** k = w->k;
     const dim3 blk(256);
     const dim3 grd(ceil_div(n, 256 * 4) > 0 ? ceil_div(n, 256 * 4) : 1, B);
-    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out) -> tg_status {
+    // stage state y + h sum a_j k_j at the stage time t + cs h (the bath
+    // schedule, if any, is evaluated there)
+    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out, double cs) -> tg_status {
         StageArgs a{};
         int j = 0;
         for (double v : coefs) a.a[j++] = v;
         for (int q = 0; q < nk; ++q) a.k[q] = k[q];
         a.nk = nk;
-        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out);
+        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out, ctl.bath, w->md, cs,
+                           d_bath);
         TG_CHECK_LAUNCH(c);
         return TG_OK;
+    };
+    auto eval = [&](const double* in, int s) {
+        return eval_batch(c, in, k[s], B, d_bath, flags, w->h, w->flux + (int64_t)s * w->cap);
     };
     const dim3 cgrd(ceil_div(B, 256));
     hipLaunchKernelGGL(prep_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->h, w->running);
     TG_CHECK_LAUNCH(c);
     // stages k2..k6 (DormandPrince54.hpp:565-574)
-    TG_TRY(stage({A21}, 1, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[1], B, bath, flags, w->h));
-    TG_TRY(stage({A31, A32}, 2, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[2], B, bath, flags, w->h));
-    TG_TRY(stage({A41, A42, A43}, 3, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[3], B, bath, flags, w->h));
-    TG_TRY(stage({A51, A52, A53, A54}, 4, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[4], B, bath, flags, w->h));
-    TG_TRY(stage({A61, A62, A63, A64, A65}, 5, w->ys));
-    TG_TRY(eval_batch(c, w->ys, k[5], B, bath, flags, w->h));
-    // y5 (b2 = 0) and k7 (:576-578)
-    TG_TRY(stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5));
-    TG_TRY(eval_batch(c, w->y5, k[6], B, bath, flags, w->h));
+    TG_TRY(stage({A21}, 1, w->ys, C2));
+    TG_TRY(eval(w->ys, 1));
+    TG_TRY(stage({A31, A32}, 2, w->ys, C3));
+    TG_TRY(eval(w->ys, 2));
+    TG_TRY(stage({A41, A42, A43}, 3, w->ys, C4));
+    TG_TRY(eval(w->ys, 3));
+    TG_TRY(stage({A51, A52, A53, A54}, 4, w->ys, C5));
+    TG_TRY(eval(w->ys, 4));
+    TG_TRY(stage({A61, A62, A63, A64, A65}, 5, w->ys, 1.0));
+    TG_TRY(eval(w->ys, 5));
+    // y5 (b2 = 0) and k7 (:576-578), both at t + h
+    TG_TRY(stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5, 1.0));
+    TG_TRY(eval(w->y5, 6));
     // err = y5 - y4 (:580)
     {
         StageArgs a{};
@@ -210,9 +225,32 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
     hipStream_t st = c->stream;
     const size_t vec = (size_t)n * B;
 
+    // the bath: constant per member, or the schedule's n_b(t) (enforceValue at
+    // t0 for k1 = timeOp(t0, enforce(t0, y0)), DormandPrince54.hpp:548)
+    const bool sched_on = (flags & TG_BATH_SCHEDULE) != 0;
+    if (sched_on && c->bs_ns < 2) return set_error(c, TG_ERR_STATE, "TG_BATH_SCHEDULE without tg_set_bath_schedule");
+    if (sched_on && B > c->bs_batch)
+        return set_error(c, TG_ERR_ARG, "batch %d exceeds the %d members of the bath schedule", B, c->bs_batch);
+    BathSched bs{};
+    if (sched_on) {
+        bs.tab = c->h_bsched.data();   // host copy for the host-side calls below
+        bs.ns = c->bs_ns;
+        bs.t_start = c->bs_t0;
+        bs.dt = c->bs_dt;
+        bs.atol = c->bs_atol;
+        bs.rtol = c->bs_rtol;
+    }
     std::vector<double> bath(B);
-    for (int b = 0; b < B; ++b) bath[b] = bath_n ? bath_n[b] : c->bath_n;
-    if (flags & TG_PREC_MIXED) TG_TRY(mixed_set_bath(c, bath.data(), B));
+    for (int b = 0; b < B; ++b)
+        bath[b] = sched_on ? sched_value(bs.tab + (size_t)b * bs.ns, bs.ns, bs.t_start, bs.dt, t0)
+                           : (bath_n ? bath_n[b] : c->bath_n);
+    double* d_bath = c->d_bath;
+    if (flags & TG_PREC_MIXED) {
+        TG_TRY(mixed_set_bath(c, bath.data(), B));
+        d_bath = mixed_bath_device(c);
+    } else {
+        TG_HIP(c, hipMemcpyAsync(c->d_bath, bath.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    }
 
     TG_HIP(c, hipMemcpyAsync(w->y, f, sizeof(double) * vec,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
@@ -264,17 +302,25 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
     ctl.max_dense = d_dense ? max_dense : 0;
     ctl.sched = replay ? w->sched : nullptr;
     ctl.nsched = w->nsched;
+    ctl.bath = bs;
+    ctl.bath.tab = sched_on ? c->d_bsched : nullptr;
+    ctl.flux = w->flux;
+    ctl.flux_stride = w->cap;
     std::vector<MemberDev> ms(B);
-    for (auto& m : ms) {
+    for (int b = 0; b < B; ++b) {
+        MemberDev& m = ms[b];
         std::memset(&m, 0, sizeof(m));
         m.t = t0;
         m.h = dense_dt;
+        // the first step is vetted too (DormandPrince54.hpp:562-564)
+        if (!replay)
+            m.h = std::fmin(m.h, step_vetoer(bs, sched_on ? bs.tab + (size_t)b * bs.ns : nullptr, t0, m.h, dense_dt));
         m.dense_time = t0 + dense_dt;
         m.olderror = 1.0;
         m.st.evals = 1;
     }
     TG_HIP(c, hipMemcpyAsync(w->md, ms.data(), sizeof(MemberDev) * B, hipMemcpyHostToDevice, st));
-    TG_TRY(eval_batch(c, w->y, w->k[0], B, bath, flags, nullptr));  // k1 = timeOp(y0)
+    TG_TRY(eval_batch(c, w->y, w->k[0], B, d_bath, flags, nullptr, w->flux));  // k1 = timeOp(y0)
     TG_HIP(c, hipStreamSynchronize(st));
 
     // the attempt as a graph; replayed one attempt ahead of the flag read
@@ -283,7 +329,7 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
     TG_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const bool timing = c->time_kernel;
     c->time_kernel = false;   // no event records inside the graph
-    tg_status cap = attempt(c, w, B, ctl, bath, flags, rtol, atol, d_dense, h_acc ? w->hlog : nullptr);
+    tg_status cap = attempt(c, w, B, ctl, d_bath, flags, rtol, atol, d_dense, h_acc ? w->hlog : nullptr);
     c->time_kernel = timing;
     const hipError_t ec = hipStreamEndCapture(st, &graph);
     if (cap != TG_OK) {
@@ -337,9 +383,11 @@ static tg_status dp5_run(tg_ctx* c, double* f, int64_t n, int32_t B, double t0, 
         TG_HIP(c, hipMemcpyAsync(h_acc, w->hlog, sizeof(double) * (size_t)B * h_cap,
                                  hipMemcpyDeviceToHost, st));
     TG_HIP(c, hipStreamSynchronize(st));
+    if (sched_on && (flags & TG_PREC_MIXED)) mixed_bath_forget(c);   // the stages rewrote the device baths
     if (!quiet && B == 1) printf("Number of adaptive steps: %lld\n", (long long)ms[0].st.attempted);
     for (int b = 0; b < B; ++b) {
         ms[b].st.failed = ms[b].failed ? 1 : 0;
+        ms[b].st.bath_energy = ms[b].ledger;
         if (stats) stats[b] = ms[b].st;
         if (ms[b].nonfinite) rc = TG_ERR_NONFINITE;
         else if (ms[b].failed && rc == TG_OK) rc = TG_ERR_STEP_UNDERFLOW;

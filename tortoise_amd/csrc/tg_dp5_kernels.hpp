@@ -36,13 +36,114 @@ struct StageArgs {
     int nk;
 };
 
+// Time-dependent bath of a constrained phonon band: member b's occupation
+// n_b(t) is given by samples at t_start + k dt, piecewise linear between them
+// and held at the end values outside.  This is the device form of
+// Band::constrainedPopulation (include/Physics/Material/Band.hpp:70-74) for a
+// uniform population, which MaterialStatus::applyConstraints
+// (src/Physics/Status/MaterialStatus.cpp:198-206) writes into the state at
+// every stage time.
+struct BathSched {
+    const double* tab;    // [B][ns] samples (device memory in kernels); null: constant bath
+    int32_t ns;           // samples per member (>= 2)
+    double t_start, dt;   // time of sample 0, sample spacing
+    double atol, rtol;    // the constrained band's absolute / relative error tolerances
+};
+
+// n_b(t) from member b's samples s[0..ns); the same operations on host and
+// device (no contraction: the checker restates them operation by operation)
+__host__ __device__ inline double sched_value(const double* s, int ns, double t_start, double dt,
+                                              double t) {
+#pragma clang fp contract(off)
+    const double u = (t - t_start) / dt;
+    if (!(u > 0.0)) return s[0];
+    if (u >= (double)(ns - 1)) return s[ns - 1];
+    int k = (int)floor(u);
+    if (k > ns - 2) k = ns - 2;
+    const double w = u - (double)k;
+    return fma(w, s[k + 1] - s[k], s[k]);
+}
+
+// MaterialTimeStatus::timeStepVetoer (src/Physics/Status/MaterialTimeStatus.cpp:60-100)
+// for one member, called with timesToTest = {t, t + c2 h, t + c3 h, t + c4 h,
+// t + c5 h, t + h} as DormandPrince54.hpp:563-564,640-642 build it.  The
+// constrained population is sampled at t + dense_dt, t + 2 dense_dt, ...;
+// the first sample inside interval i that leaves the interval average by
+// more than 1.3 |L - R| + atol + rtol |average| (L, R: the population at the
+// interval's ends) caps the step at that sample's time.  Without a
+// constrained band (s null) the cap is timesToTest.back() - timesToTest[0],
+// as in the reference.  The population is uniform, so the max over nodes of
+// the reference's nodal Function arithmetic is this scalar expression.
+__host__ __device__ inline double step_vetoer(const BathSched& bs, const double* s, double t, double h,
+                                              double dense_dt) {
+#pragma clang fp contract(off)
+    const double times[6] = {t, t + C2 * h, t + C3 * h, t + C4 * h, t + C5 * h, t + h};
+    double crit = times[5];
+    if (s) {
+        const double var = 1. + 0.3;
+        double cur = times[0] + dense_dt;
+        double L = sched_value(s, bs.ns, bs.t_start, bs.dt, times[0]);
+        double R = sched_value(s, bs.ns, bs.t_start, bs.dt, times[1]);
+        bool found = false;
+        for (int i = 0; i < 5 && !found; ++i) {
+            if (i != 0) {
+                L = R;
+                R = sched_value(s, bs.ns, bs.t_start, bs.dt, times[i + 1]);
+            }
+            const double avg = 0.5 * (L + R);
+            const double acc = var * fabs(L - R) + bs.atol + bs.rtol * fabs(avg);
+            while (cur < times[i + 1]) {
+                const double v = sched_value(s, bs.ns, bs.t_start, bs.dt, cur);
+                if (fabs(v - avg) / acc > 1.) {
+                    crit = cur;
+                    found = true;
+                    break;
+                }
+                const double next = cur + dense_dt;
+                if (!(next > cur)) break;   // dense_dt below the resolution of t: no further samples
+                cur = next;
+            }
+        }
+    }
+    return crit - times[0];
+}
+
+// Per-member controller state (DormandPrince54.hpp:527-543, 547-661), on the
+// device.  The accepted step's start, size and first dense time are kept for
+// the dense-output kernel.  ledger: energy the phonon bath has received (the
+// e-ph channel's electron energy flux, integrated with the step weights).
+struct MemberDev {
+    double t, h, dense_time, olderror;
+    double step_t, step_h, step_dense_time;
+    double ledger;
+    int32_t rejected, failed, nonfinite, active;
+    int32_t step_idx, acc, dense_k0, dense_k1;
+    tg_step_stats st;
+};
+
 // out[:, b] = y[:, b] + sum_j (a_j h_b) k_j[:, b]; the product a_j*h_b is
-// formed first, as the reference does ((a_ij * h) * k_j).
+// formed first, as the reference does ((a_ij * h) * k_j), and every product
+// and sum is rounded separately, left to right, as the reference's vector
+// expression is (no fused multiply-adds: y5 - y4 is a difference of nearly
+// equal vectors, and the error norm divides it by tolerances far below the
+// resolution of filled states, so the rounding pattern is part of what the
+// step controller sees; the stage, error and dense kernels round exactly
+// like DormandPrince54.hpp:565-608).  With a bath
+// schedule, block 0 of each member also writes the member's bath occupation
+// at the stage time t + c h (enforceValue at the stage time,
+// DormandPrince54.hpp:565-578) for the operator evaluation that follows.
 __global__ void stage_kernel(int64_t n, int B, const double* __restrict__ y, StageArgs s,
-                             const double* __restrict__ h, double* __restrict__ out) {
+                             const double* __restrict__ h, double* __restrict__ out,
+                             BathSched bs = BathSched{}, const MemberDev* __restrict__ md = nullptr,
+                             double cs = 0.0, double* __restrict__ bath = nullptr) {
+#pragma clang fp contract(off)
     const int b = blockIdx.y;
     const double hb = h[b];
     if (hb == 0.0) return;   // finished member: nothing of this attempt is read
+    if (bs.tab && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double ts = md[b].t + cs * hb;
+        bath[b] = sched_value(bs.tab + (int64_t)b * bs.ns, bs.ns, bs.t_start, bs.dt, ts);
+    }
     double c[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j) c[j] = s.a[j] * hb;
@@ -59,6 +160,7 @@ __global__ void stage_kernel(int64_t n, int B, const double* __restrict__ y, Sta
 __global__ void error_kernel(int64_t n, int B, const double* __restrict__ y, StageArgs s,
                              const double* __restrict__ h, const double* __restrict__ y5,
                              double* __restrict__ err) {
+#pragma clang fp contract(off)
     const int b = blockIdx.y;
     const double hb = h[b];
     if (hb == 0.0) return;   // finished member: the controller skips it
@@ -74,22 +176,14 @@ __global__ void error_kernel(int64_t n, int B, const double* __restrict__ y, Sta
     }
 }
 
-// Per-member controller state (DormandPrince54.hpp:527-543, 547-661), on the
-// device.  The accepted step's start, size and first dense time are kept for
-// the dense-output kernel.
-struct MemberDev {
-    double t, h, dense_time, olderror;
-    double step_t, step_h, step_dense_time;
-    int32_t rejected, failed, nonfinite, active;
-    int32_t step_idx, acc, dense_k0, dense_k1;
-    tg_step_stats st;
-};
-
 struct Ctl {
     double t1, dense_dt, min_h, denom;
     int32_t B, replay, h_cap, max_dense;
     const double* sched;      // replay: step sizes, B x h_cap
     const int32_t* nsched;    // replay: steps per member
+    BathSched bath;           // time-dependent bath (tab null: constant)
+    double* flux;             // null, or per stage s = 0..6 (k1..k7) and member the e-ph
+    int64_t flux_stride;      //   electron energy flux at flux[s * flux_stride + b]
 };
 
 __device__ inline bool member_running(const MemberDev& m, const Ctl& k, int b) {
@@ -166,9 +260,23 @@ __global__ void controller_kernel(MemberDev* __restrict__ md, Ctl k,
                     scale = fmin(fmax(scale, minscale), maxscale);
                 }
                 m.h = m.rejected ? h * fmin(1.0, scale) : h * scale;
+                // the vetoer caps the next step (:640-642)
+                if (!k.replay)
+                    m.h = fmin(m.h, step_vetoer(k.bath, k.bath.tab ? k.bath.tab + (int64_t)b * k.bath.ns : nullptr,
+                                                m.t, m.h, k.dense_dt));
                 m.rejected = 0;
                 m.olderror = e;
                 m.st.accepted += 1;
+                if (k.flux) {
+                    // bath ledger: minus the electron energy the step's weights
+                    // integrate (y5 = y + sum_j (h b_j) k_j), then FSAL k1 <- k7
+                    double* ef = k.flux;
+                    const int64_t S = k.flux_stride;
+                    const double de = (h * B1) * ef[b] + (h * B3) * ef[2 * S + b] + (h * B4) * ef[3 * S + b] +
+                                      (h * B5) * ef[4 * S + b] + (h * B6) * ef[5 * S + b];
+                    m.ledger -= de;
+                    ef[b] = ef[6 * S + b];
+                }
             } else {
                 double scale = safety * pow(1. / e, 1 / 5.);
                 scale = fmin(fmax(scale, minscale), maxscale);
@@ -193,6 +301,7 @@ __global__ void dense_kernel(int64_t n, const MemberDev* __restrict__ md, Ctl k,
                              const double* __restrict__ k3, const double* __restrict__ k4,
                              const double* __restrict__ k5, const double* __restrict__ k6,
                              const double* __restrict__ k7, double* __restrict__ out) {
+#pragma clang fp contract(off)
     const int b = blockIdx.y;
     const MemberDev& m = md[b];
     if (!m.acc || m.dense_k0 >= m.dense_k1 || m.dense_k0 >= k.max_dense) return;

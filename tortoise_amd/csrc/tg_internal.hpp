@@ -84,6 +84,14 @@ struct tg_ctx {
     int32_t* d_perm = nullptr;     // sorted position -> node
     std::vector<int32_t> h_perm;
     double* d_V_s = nullptr;       // V in sorted order
+    double* d_eps_s = nullptr;     // eps in sorted order (energy fluxes)
+    double* d_bath = nullptr;      // per-member bath occupations of the plain FP64 evaluations
+                                   // (max_batch, device)
+    // time-dependent bath (tg_set_bath_schedule): batch x ns samples
+    std::vector<double> h_bsched;
+    double* d_bsched = nullptr;
+    int32_t bs_ns = 0, bs_batch = 0;
+    double bs_t0 = 0, bs_dt = 0, bs_atol = 0, bs_rtol = 0;
     double* d_rk_s = nullptr;      // K 1 in sorted order
     double* d_ck_s = nullptr;      // K^T 1 in sorted order
     double* d_fs = nullptr;        // scratch: one state gathered to sorted order
@@ -170,8 +178,10 @@ void dev_free(T** p) {
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ---- internal entry points between translation units
+// one plain FP64 evaluation; the bath is d_bath[0] (device) when given, else
+// bath_n; d_flux (device, nullable) receives sum_i V_i eps_i I_i
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
-                             int conserve);
+                             int conserve, const double* d_bath = nullptr, double* d_flux = nullptr);
 tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
                                 double* d_v);
 tg_status assemble_domega(tg_ctx* c);
@@ -186,17 +196,26 @@ tg_status stack_prepare(tg_ctx* c);
 // u = K f, v = K^T f (caller order) from the same pass
 tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
 void stack_destroy(tg_ctx* c);
-tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve);
-// I -= d |f(1-f)| with d from the block sums dpart (number-defect removal)
-tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I);
+tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve,
+                             const double* d_bath, double* d_flux);
+// from the block sums dpart (4 per block): with conserve I -= s |f(1-f)|
+// (number-defect removal); with flux the energy flux of the result
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I,
+                              int conserve, double* flux);
 tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);
 tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t batch);
 // gate (nullable, device, one per member): members with gate[b] == 0 are
 // skipped (their outputs are not written)
+// flux (nullable, device, one per member) receives each member's e-ph
+// electron energy flux sum_i V_i eps_i I_i
 tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch, int conserve,
-                            const double* gate);
+                            const double* gate, double* flux = nullptr);
+// the device array of per-member bath occupations the batched epilogue reads
+// (a DP5 with a bath schedule writes it per stage; the host copy is then stale)
+double* mixed_bath_device(tg_ctx* c);
+void mixed_bath_forget(tg_ctx* c);
 // error-norm sums sum_e (sum r^2 + (sum r)^2) per member into out[batch]
 // (part: batch x relerr_blocks partials); kernel launches only
 int relerr_blocks(tg_ctx* c);

@@ -78,6 +78,7 @@ struct MixedData {
     double* rk_s = nullptr;    // split-matrix K 1 (sorted order)
     double* ck_s = nullptr;    // split-matrix K^T 1 (sorted order)
     const double* V_s = nullptr;    // node volumes (sorted order, owned by the context)
+    const double* eps_s = nullptr;  // node energies (sorted order, owned by the context)
     uint8_t* P = nullptr;      // nsplit x nt x rows_pad partial sums (FP64, FP32 if fold == 0)
     int nsplit = 1;
     int nw = 128;              // members per workgroup tile
@@ -813,7 +814,9 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
 // ------------------------------------------------------------------ epilogue
 // One workgroup per member: FP64 sum of the split partials, Pauli-blocked
 // assembly with the member's bath occupation, FP64 block sums of sum V I and
-// sum V |f(1-f)| for the number-defect removal, result in the caller's node
+// sum V |f(1-f)| for the number-defect removal (and of sum V eps I and
+// sum V eps |f(1-f)| for the member's energy flux, flux[b] = the e-ph
+// electron energy flux of the corrected I), result in the caller's node
 // order.  With ROW the member's f is staged in LDS and every position
 // replaces the f of its node by I in place (the energy order is a
 // permutation, so each LDS slot is read and written by one thread only); the
@@ -828,9 +831,10 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
                      double inv_scale, int conserve, const double* __restrict__ gate,
-                     double* __restrict__ I) {
+                     double* __restrict__ I, const double* __restrict__ eps_s,
+                     double* __restrict__ flux) {
     extern __shared__ double row[];
-    __shared__ double red[2][kMemT / 64];
+    __shared__ double red[4][kMemT / 64];
     __shared__ double s_b;
     const int b = member_of_block(blockIdx.x, nt);
     if (b >= B || (gate && gate[b] == 0.0)) return;
@@ -838,12 +842,12 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     const double np1 = nb + 1.0;
     const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
-    double dI = 0.0, dw = 0.0;
+    double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
     // UNR positions per thread per round with all their loads issued before
     // use; the trip count is the same for every thread (the barrier after the
     // first round's loads needs all of them)
     constexpr int UNR = 6;
-    double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR];
+    double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR], Ep[UNR];
     int32_t node[UNR], gu[UNR], gv[UNR];   // gu, gv: the panels' operand bytes
     // the partial sums (streamed from HBM), read only for the panels the GEMM
     // computed: a panel left out (bit 1 of its operand byte) was not written
@@ -883,6 +887,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             ckp[j] = ck[p];
             Vp[j] = V[p];
             node[j] = perm[p];
+            if (flux) Ep[j] = eps_s[p];
         }
     };
     auto finish_round = [&](int64_t p0) {
@@ -907,8 +912,13 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             const double x = ROW ? row[node[j]] : fb[node[j]];
             // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
             const double Ii = ((1.0 - x) * (np1 * Kf + nb * KTf) - x * (np1 * KTg + nb * Kg)) / Vp[j];
-            dI += Vp[j] * Ii;
-            dw += Vp[j] * fabs(x * (1.0 - x));
+            const double vi = Vp[j] * Ii, vw = Vp[j] * fabs(x * (1.0 - x));
+            dI += vi;
+            dw += vw;
+            if (flux) {
+                eI += Ep[j] * vi;
+                ew += Ep[j] * vw;
+            }
             if constexpr (ROW)
                 row[node[j]] = Ii;
             else
@@ -930,23 +940,30 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
         finish_round(base + threadIdx.x);
     }
     double sc = 0.0;
-    if (conserve) {
+    if (conserve || flux) {
         for (int o = 32; o > 0; o >>= 1) {
             dI += __shfl_down(dI, o, 64);
             dw += __shfl_down(dw, o, 64);
+            eI += __shfl_down(eI, o, 64);
+            ew += __shfl_down(ew, o, 64);
         }
         if ((threadIdx.x & 63) == 0) {
             red[0][threadIdx.x >> 6] = dI;
             red[1][threadIdx.x >> 6] = dw;
+            red[2][threadIdx.x >> 6] = eI;
+            red[3][threadIdx.x >> 6] = ew;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double a = 0.0, c = 0.0;
+            double a = 0.0, c = 0.0, ea = 0.0, ec = 0.0;
             for (int w = 0; w < kMemT / 64; ++w) {
                 a += red[0][w];
                 c += red[1][w];
+                ea += red[2][w];
+                ec += red[3][w];
             }
-            s_b = c > 0.0 ? a / c : 0.0;
+            s_b = (conserve && c > 0.0) ? a / c : 0.0;
+            if (flux) flux[b] = ea - s_b * ec;
         }
     }
     __syncthreads();
@@ -1241,7 +1258,7 @@ static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, 
 }
 
 static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, double* d_I, int B,
-                               int conserve, const double* gate = nullptr) {
+                               int conserve, const double* gate = nullptr, double* flux = nullptr) {
     const int64_t n = c->n;
     const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
@@ -1249,7 +1266,7 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
                        c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
                        m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
-                       gate, d_I)
+                       gate, d_I, m->eps_s, flux)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
         else TG_FINISH(false, double);
@@ -1436,6 +1453,7 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
     TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
     m->V_s = c->d_V_s;
+    m->eps_s = c->d_eps_s;
     hipLaunchKernelGGL(split_rowsum_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st,
                        m->A_hi, m->A_lo, m->rows_pad, kgroups, n, m->kt, 1.0 / scale, m->rk_s,
                        m->ck_s);
@@ -1696,7 +1714,7 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 // only, so a caller may capture it in a graph once it has run once (buffers
 // are allocated on first use)
 tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B, int conserve,
-                            const double* gate) {
+                            const double* gate, double* flux) {
     MixedData* m = md_of(c);
     if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
     if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
@@ -1715,8 +1733,18 @@ tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B
     time_begin(c);
     TG_TRY(launch_gemm(c, m, dyn));
     time_end(c);
-    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate));
+    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate, flux));
     return TG_OK;
+}
+
+double* mixed_bath_device(tg_ctx* c) {
+    MixedData* m = md_of(c);
+    return m ? m->bath : nullptr;
+}
+
+void mixed_bath_forget(tg_ctx* c) {
+    MixedData* m = md_of(c);
+    if (m) m->bath_set.clear();
 }
 
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B,

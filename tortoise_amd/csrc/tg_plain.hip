@@ -1,13 +1,15 @@
-// tg_plain.hip -- plain FP64 collision operator, one pass over K.
-//
-// Per evaluation the kernel streams K (n x ld FP64, row-major) from HBM once
-// and produces both u = K f and v = K^T f:
+// tg_plain.hip -- FP64 products u = M f and v = M^T f from one pass over a
+// band matrix M stored n x ld (row-major, energy order): the row and column
+// sums of K at assembly and the products with dK/domega0 in the adjoint.  The
+// collision operator's own FP64 evaluation streams K's compact band
+// (tg_stack.hip); this file also holds the C entry point tg_collision_apply
+// and the number-defect / energy-flux reduction its finish kernels share.
 //
 // K is stored with rows and columns in energy order, which makes the
 // emission kernel a band: every 128-row block has a contiguous range of
 // nonzero columns, and only (row block, 2048-column chunk) pairs that meet
 // it are launched; sweeps of a launched chunk outside the range publish
-// zeros without reading K.
+// zeros without reading M.
 //
 //   phase 1 (plain_dual_kernel): workgroup (row block r, chunk c) covers
 //     rows [128 r, 128 r + 128) x cols [2048 c, 2048 c + 2048).  Each of the 4
@@ -17,15 +19,10 @@
 //     flight together.  Row partials of u stay in registers for the whole
 //     chunk and are reduced across the wave once per chunk; column partials
 //     of v are reduced across the 4 waves in LDS once per sweep.
-//       upart[c][i] = sum_{j in chunk c} K_ij f_j
-//       vpart[r][j] = sum_{i in rowblock r} K_ij f_i
+//       upart[c][i] = sum_{j in chunk c} M_ij f_j
+//       vpart[r][j] = sum_{i in rowblock r} M_ij f_i
 //   phase 2 (plain_finish_kernel): per node, sum the partials in index order
-//     (deterministic) and apply the Pauli-blocked assembly
-//       I_i = [(1-f_i)((n+1)u_i + n v_i) - f_i((n+1)ck_i + n rk_i - (n+1)v_i - n u_i)] / V_i
-//     with rk = K 1 and ck = K^T 1 precomputed in FP64.
-//
-// HBM traffic per evaluation: 8 x (band entries of K) + 32 n (f, I, two
-// vectors) + the partial buffers; the dense form would be 8 n^2 + 32 n.
+//     (deterministic) and write u and v in the caller's node order.
 #include "tg_internal.hpp"
 
 namespace tg {
@@ -126,21 +123,18 @@ __global__ void gather_sorted_f_kernel(const double* __restrict__ f, const int32
     if (p < n) fs[p] = f[perm[p]];
 }
 
-// mode 0: collision operator; mode 1: raw products u -> out0, v -> out1.
-// Thread (tx, ty) sums slice ty of node p's partials: the column chunks of
-// its row block (u) and the row blocks covering its column chunk (v).
-// Outputs go to the caller's node order perm[p].
+// Raw products u -> out0, v -> out1 (caller order perm[p]).  Thread (tx, ty)
+// sums slice ty of node p's partials: the column chunks of its row block (u)
+// and the row blocks covering its column chunk (v).
 constexpr int kFinishNodes = 64;
 constexpr int kFinishSlices = 4;
 
 __global__ void __launch_bounds__(kFinishNodes* kFinishSlices)
-plain_finish_kernel(int mode, const double* __restrict__ upart, const double* __restrict__ vpart,
+plain_finish_kernel(const double* __restrict__ upart, const double* __restrict__ vpart,
                     const int2* __restrict__ rb_chunk, const int32_t* __restrict__ cc_off,
                     const int32_t* __restrict__ cc_rows, int64_t n, int64_t ld,
-                    const int32_t* __restrict__ perm, const double* __restrict__ fs,
-                    const double* __restrict__ rk, const double* __restrict__ ck,
-                    const double* __restrict__ V, double bath_n, double* __restrict__ out0,
-                    double* __restrict__ out1, double* __restrict__ dpart) {
+                    const int32_t* __restrict__ perm, double* __restrict__ out0,
+                    double* __restrict__ out1) {
     __shared__ double su[kFinishSlices][kFinishNodes];
     __shared__ double svv[kFinishSlices][kFinishNodes];
     const int tx = threadIdx.x % kFinishNodes;
@@ -157,60 +151,44 @@ plain_finish_kernel(int mode, const double* __restrict__ upart, const double* __
     su[ty][tx] = u;
     svv[ty][tx] = v;
     __syncthreads();
-    double dI = 0.0, dw = 0.0;
     if (ty == 0 && p < n) {
         u = (su[0][tx] + su[1][tx]) + (su[2][tx] + su[3][tx]);
         v = (svv[0][tx] + svv[1][tx]) + (svv[2][tx] + svv[3][tx]);
         const int64_t node = perm[p];
-        if (mode == 1) {
-            out0[node] = u;
-            out1[node] = v;
-        } else {
-            const double np1 = bath_n + 1.0;
-            const double fi = fs[p];
-            const double wf = np1 * u + bath_n * v;
-            const double wt1 = np1 * ck[p] + bath_n * rk[p];
-            const double wtf = np1 * v + bath_n * u;
-            const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
-            out0[node] = Ii;
-            dI = V[p] * Ii;
-            dw = V[p] * fabs(fi * (1.0 - fi));
-        }
-    }
-    if (dpart) {
-        // per-block partials of sum V I and sum V |f(1-f)|, wave 0 only
-        if (ty == 0) {
-            for (int o = 32; o > 0; o >>= 1) {
-                dI += __shfl_down(dI, o, 64);
-                dw += __shfl_down(dw, o, 64);
-            }
-            if (tx == 0) {
-                dpart[2 * blockIdx.x] = dI;
-                dpart[2 * blockIdx.x + 1] = dw;
-            }
-        }
+        out0[node] = u;
+        out1[node] = v;
     }
 }
 
-// Removes the number defect d = sum V I with weights |f(1-f)| (nonnegative
-// also where a state leaves [0,1]): I_i -= d |f_i(1-f_i)| / sum_k V_k |f_k(1-f_k)|.
+// Per-block partials dpart[4 b + {0,1,2,3}] = sums of V I, V |f(1-f)|,
+// V eps I and V eps |f(1-f)| (fixed order) give
+//   - with conserve, the number-defect removal I_i -= s |f_i(1-f_i)|,
+//     s = sum V I / sum V |f(1-f)| (weights nonnegative also where a state
+//     leaves [0,1]);
+//   - with flux, the e-ph electron energy flux of the corrected I:
+//     sum V eps I - s sum V eps |f(1-f)| (one value, written by thread 0).
 __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
-                                    const double* __restrict__ f, double* __restrict__ I) {
-    double d = 0.0, w = 0.0;
+                                    const double* __restrict__ f, double* __restrict__ I,
+                                    int conserve, double* __restrict__ flux) {
+    double d = 0.0, w = 0.0, ei = 0.0, ew = 0.0;
     for (int b = 0; b < nblk; ++b) {
-        d += dpart[2 * b];
-        w += dpart[2 * b + 1];
+        d += dpart[4 * b];
+        w += dpart[4 * b + 1];
+        ei += dpart[4 * b + 2];
+        ew += dpart[4 * b + 3];
     }
-    if (!(w > 0.0)) return;
-    const double s = d / w;
+    const double s = (conserve && w > 0.0) ? d / w : 0.0;
+    if (flux && blockIdx.x == 0 && threadIdx.x == 0) *flux = ei - s * ew;
+    if (s == 0.0) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         I[i] -= s * fabs(f[i] * (1.0 - f[i]));
 }
 
-tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I) {
-    hipLaunchKernelGGL(conserve_fix_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, dpart, nblk,
-                       c->n, d_f, d_I);
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I,
+                              int conserve, double* flux) {
+    hipLaunchKernelGGL(conserve_fix_kernel, dim3(conserve ? ceil_div(c->n, 256) : 1), dim3(256), 0, c->stream,
+                       dpart, nblk, c->n, d_f, d_I, conserve, flux);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }
@@ -229,9 +207,9 @@ static tg_status launch_dual(tg_ctx* c, const double* Kmat, const double* d_f) {
 
 // one plain FP64 evaluation: one pass over K's band (tg_stack.hip)
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
-                             int conserve) {
+                             int conserve, const double* d_bath, double* d_flux) {
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
-    return stack_apply_device(c, d_f, d_I, bath_n, conserve);
+    return stack_apply_device(c, d_f, d_I, bath_n, conserve, d_bath, d_flux);
 }
 
 // u = M f, v = M^T f (caller's node order) for M = K or dK/domega0 stored in
@@ -241,9 +219,8 @@ tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f
     TG_TRY(launch_dual(c, Kmat, d_f));
     const unsigned nb = ceil_div(c->n, kFinishNodes);
     hipLaunchKernelGGL(plain_finish_kernel, dim3(nb), dim3(kFinishNodes * kFinishSlices), 0,
-                       c->stream, 1, c->d_upart, c->d_vpart, c->d_rb_chunk, c->d_cc_off,
-                       c->d_cc_rows, c->n, c->ld, c->d_perm, c->d_fs, nullptr, nullptr, nullptr,
-                       0.0, d_u, d_v, nullptr);
+                       c->stream, c->d_upart, c->d_vpart, c->d_rb_chunk, c->d_cc_off,
+                       c->d_cc_rows, c->n, c->ld, c->d_perm, d_u, d_v);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }
@@ -277,7 +254,8 @@ extern "C" tg_status tg_collision_apply(tg_ctx* c, const double* f, double* I, i
     } else {
         for (int b = 0; b < batch; ++b) {
             const double nb = bath_n ? bath_n[b] : c->bath_n;
-            TG_TRY(plain_apply_device(c, d_f + (int64_t)b * n, d_I + (int64_t)b * n, nb, conserve));
+            TG_TRY(plain_apply_device(c, d_f + (int64_t)b * n, d_I + (int64_t)b * n, nb, conserve, nullptr,
+                                      nullptr));
         }
     }
     c->time_kernel = false;

@@ -231,48 +231,49 @@ stack_products_kernel(const double* __restrict__ part, const double* __restrict_
     v_out[node] = v;
 }
 
+// One position per thread: the operator with the member's bath (bath_p[0]
+// when given, else bath_n) in the caller's order; with dpart, per-block sums
+// of V I, V |f(1-f)|, V eps I and V eps |f(1-f)| for the number-defect
+// removal and the energy flux (conserve_fix_kernel).
 __global__ void __launch_bounds__(kStackFinishT)
 stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
                     const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
                     const int32_t* __restrict__ cit, int64_t n,
                     const int32_t* __restrict__ perm, const double* __restrict__ fs,
                     const double* __restrict__ rk, const double* __restrict__ ck,
-                    const double* __restrict__ V, double bath_n, double* __restrict__ out,
+                    const double* __restrict__ V, const double* __restrict__ eps_s, double bath_n,
+                    const double* __restrict__ bath_p, double* __restrict__ out,
                     double* __restrict__ dpart) {
     const int64_t p = blockIdx.x * (int64_t)kStackFinishT + threadIdx.x;
-    double dI = 0.0, dw = 0.0;
+    const double nb = bath_p ? bath_p[0] : bath_n;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (p < n) {
         double u, v;
         stack_sums(part, vpart, pitems, cit_off, cit, p, u, v);
-        const double np1 = bath_n + 1.0;
+        const double np1 = nb + 1.0;
         const double fi = fs[p];
-        const double wf = np1 * u + bath_n * v;
-        const double wt1 = np1 * ck[p] + bath_n * rk[p];
-        const double wtf = np1 * v + bath_n * u;
+        const double wf = np1 * u + nb * v;
+        const double wt1 = np1 * ck[p] + nb * rk[p];
+        const double wtf = np1 * v + nb * u;
         const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
         out[perm[p]] = Ii;
-        dI = V[p] * Ii;
-        dw = V[p] * fabs(fi * (1.0 - fi));
+        acc[0] = V[p] * Ii;
+        acc[1] = V[p] * fabs(fi * (1.0 - fi));
+        acc[2] = eps_s[p] * acc[0];
+        acc[3] = eps_s[p] * acc[1];
     }
     if (dpart) {
-        __shared__ double red[2][kStackFinishT / 64];
-        for (int o = 32; o > 0; o >>= 1) {
-            dI += __shfl_down(dI, o, 64);
-            dw += __shfl_down(dw, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            red[0][threadIdx.x >> 6] = dI;
-            red[1][threadIdx.x >> 6] = dw;
+        __shared__ double red[4][kStackFinishT / 64];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_down(acc[q], o, 64);
+            if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = acc[q];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double a = 0.0, b = 0.0;
-            for (int w = 0; w < kStackFinishT / 64; ++w) {
-                a += red[0][w];
-                b += red[1][w];
-            }
-            dpart[2 * blockIdx.x] = a;
-            dpart[2 * blockIdx.x + 1] = b;
+        if (threadIdx.x < 4) {
+            double a = 0.0;
+            for (int w = 0; w < kStackFinishT / 64; ++w) a += red[threadIdx.x][w];
+            dpart[4 * blockIdx.x + threadIdx.x] = a;
         }
     }
 }
@@ -400,8 +401,11 @@ tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, doubl
     return TG_OK;
 }
 
-// one plain FP64 evaluation (caller order in and out); kernel launches only
-tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve) {
+// one plain FP64 evaluation (caller order in and out); kernel launches only.
+// The bath is d_bath[0] (device) when given, else bath_n; d_flux (device,
+// nullable) receives the e-ph electron energy flux sum_i V_i eps_i I_i.
+tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve,
+                             const double* d_bath, double* d_flux) {
     StackData* s = sd_of(c);
     if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
     hipStream_t st = c->stream;
@@ -415,19 +419,19 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     TG_CHECK_LAUNCH(c);
     const unsigned nb = ceil_div(c->n, kStackFinishT);
     double* dpart = nullptr;
-    if (conserve) {
-        if ((size_t)c->red_cap < 2 * (size_t)nb) {
+    if (conserve || d_flux) {
+        if ((size_t)c->red_cap < 4 * (size_t)nb) {
             dev_free(&c->d_red);
-            TG_TRY(dev_alloc(c, &c->d_red, 2 * (size_t)nb));
-            c->red_cap = 2 * (int32_t)nb;
+            TG_TRY(dev_alloc(c, &c->d_red, 4 * (size_t)nb));
+            c->red_cap = 4 * (int32_t)nb;
         }
         dpart = c->d_red;
     }
     hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, s->part, s->vpart, s->pitems,
                        s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s,
-                       bath_n, d_I, dpart);
+                       c->d_eps_s, bath_n, d_bath, d_I, dpart);
     TG_CHECK_LAUNCH(c);
-    if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I));
+    if (dpart) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I, conserve, d_flux));
     return TG_OK;
 }
 

@@ -98,6 +98,12 @@ struct tg_tuple_ctx {
 
     void* dp5 = nullptr;         // DP5 work buffers (tg_tuple_dp5)
 
+    // time-dependent bath (tg_tuple_set_bath_schedule): batch x ns samples
+    std::vector<double> h_bsched;
+    double* d_bsched = nullptr;
+    int32_t bs_ns = 0, bs_batch = 0;
+    double bs_t0 = 0, bs_dt = 0, bs_atol = 0, bs_rtol = 0;
+
     hipStream_t stream = nullptr;
     int device = 0;
     char err[512] = {0};
@@ -1417,6 +1423,53 @@ __global__ void dg_relerr_final_kernel(const double* __restrict__ part, const do
     norms[b] = s;
 }
 
+// Electron energy flux of DG derivatives I (LF coefficients): per member
+// sum_e int_e eps I = vol / ((dim+1)(dim+2)) sum_e (sum_v eps_v I_v + sum_v eps_v sum_v I_v),
+// the exact P1 x P1 integral on each simplex with the element-vertex values
+// (eps from the element table's energy form).  Block partials in a fixed
+// order, then the final sum times the volume factor.
+template <int DIM>
+__global__ __launch_bounds__(kRelThreads) void dg_flux_partial_kernel(int64_t nel, const double* __restrict__ I,
+                                                                        int64_t stride, const double* __restrict__ etab,
+                                                                        int nblk, double* __restrict__ part) {
+    constexpr int NC = DIM + 1;
+    __shared__ double sw[kRelThreads / 64];
+    const int b = blockIdx.y;
+    const double* iv = I + (int64_t)b * stride;
+    double s = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * kRelThreads + threadIdx.x; e < nel; e += (int64_t)nblk * kRelThreads) {
+        const double* ef = etab + e * etab_stride<DIM>() + DIM * (DIM + 1);
+        double sei = 0.0, se = 0.0, si = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            const double ev = ef[0] + (a ? ef[a] : 0.0);
+            const double x = iv[e * NC] + (a ? iv[e * NC + a] : 0.0);
+            sei += ev * x;
+            se += ev;
+            si += x;
+        }
+        s += sei + se * si;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kRelThreads / 64; ++w) t += sw[w];
+        part[(int64_t)b * nblk + blockIdx.x] = t;
+    }
+}
+
+__global__ void dg_flux_final_kernel(const double* __restrict__ part, int nblk, int B, double scale,
+                                     const double* __restrict__ h, double* __restrict__ flux) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || (h && h[b] == 0.0)) return;
+    double s = 0.0;
+    for (int i = 0; i < nblk; ++i) s += part[(int64_t)b * nblk + i];
+    flux[b] = scale * s;
+}
+
 struct TupleDp5 {
     int32_t cap = 0;
     int64_t n = 0;
@@ -1432,6 +1485,10 @@ struct TupleDp5 {
     double* norms = nullptr;
     double* part = nullptr;
     double* part2 = nullptr;     // phonon band's partials (dynamical phonons)
+    double* fpart = nullptr;     // energy-flux partials (cap x nblk)
+    double* flux = nullptr;      // electron energy flux per stage and member (7 x cap)
+    double* hlog = nullptr;      // accepted step sizes (cap x hlog_cap)
+    size_t hlog_cap = 0;
     int32_t* running = nullptr;
     int32_t* running_host = nullptr;
     double* dense = nullptr;
@@ -1441,7 +1498,8 @@ struct TupleDp5 {
 void tuple_dp5_free(tg_tuple_ctx* c) {
     TupleDp5* w = (TupleDp5*)c->dp5;
     if (!w) return;
-    void* bufs[] = {w->ys, w->y5, w->errv, w->y, w->h, w->acc, w->md, w->norms, w->part, w->part2, w->running, w->dense};
+    void* bufs[] = {w->ys,    w->y5,   w->errv,  w->y,    w->h,       w->acc,  w->md,
+                    w->norms, w->part, w->part2, w->fpart, w->flux, w->running, w->dense, w->hlog};
     for (int j = 0; j < 7; ++j)
         if (w->k[j]) (void)hipFree(w->k[j]);
     for (void* p : bufs)
@@ -1468,43 +1526,68 @@ tg_status tuple_dp5_ensure(tg_tuple_ctx* c) {
         (s = dalloc(c, &w->h, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->acc, (size_t)w->cap)) != TG_OK ||
         (s = dalloc(c, &w->md, (size_t)w->cap)) != TG_OK || (s = dalloc(c, &w->norms, (size_t)w->cap)) != TG_OK ||
         (s = dalloc(c, &w->part, (size_t)w->cap * w->nblk)) != TG_OK ||
-        (s = dalloc(c, &w->part2, (size_t)w->cap * w->nblk)) != TG_OK || (s = dalloc(c, &w->running, 1)) != TG_OK)
+        (s = dalloc(c, &w->part2, (size_t)w->cap * w->nblk)) != TG_OK ||
+        (s = dalloc(c, &w->fpart, (size_t)w->cap * w->nblk)) != TG_OK ||
+        (s = dalloc(c, &w->flux, (size_t)7 * w->cap)) != TG_OK || (s = dalloc(c, &w->running, 1)) != TG_OK)
         return s;
     TT_HIP(c, hipHostMalloc((void**)&w->running_host, sizeof(int32_t), hipHostMallocDefault));
     return TG_OK;
 }
 
+// the electron energy flux of the derivative k (electron band) of every
+// running member into flux[0..B) (kernel launches only)
+tg_status tuple_flux(tg_tuple_ctx* c, TupleDp5* w, const double* k, int32_t B, const double* h, double* flux) {
+    const dim3 grid(w->nblk, B);
+    if (c->dim == 2)
+        hipLaunchKernelGGL(dg_flux_partial_kernel<2>, grid, dim3(kRelThreads), 0, c->stream, c->nel, k, w->n,
+                           c->d_etab, w->nblk, w->fpart);
+    else
+        hipLaunchKernelGGL(dg_flux_partial_kernel<3>, grid, dim3(kRelThreads), 0, c->stream, c->nel, k, w->n,
+                           c->d_etab, w->nblk, w->fpart);
+    const double scale = c->vol / (double)((c->dim + 1) * (c->dim + 2));
+    hipLaunchKernelGGL(dg_flux_final_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, c->stream, w->fpart,
+                       w->nblk, B, scale, h, flux);
+    TT_HIP(c, hipGetLastError());
+    return TG_OK;
+}
+
 // one attempted step for all members (kernel launches only; captured)
 tg_status tuple_attempt(tg_tuple_ctx* c, TupleDp5* w, int32_t B, const tg::Ctl& ctl, double rtol, double atol,
-                        double* d_dense) {
+                        double* d_dense, double* d_hlog) {
     using namespace tg;
     hipStream_t st = c->stream;
     const int64_t n = w->n;
     double** k = w->k;
     const dim3 blk(256);
     const dim3 grd((unsigned)std::max<int64_t>(1, (n + 1023) / 1024), B);
-    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out) -> tg_status {
+    // stage state at t + cs h (the bath schedule, if any, is evaluated there)
+    auto stage = [&](std::initializer_list<double> coefs, int nk, double* out, double cs) -> tg_status {
         StageArgs a{};
         int j = 0;
         for (double v : coefs) a.a[j++] = v;
         for (int q = 0; q < nk; ++q) a.k[q] = k[q];
         a.nk = nk;
-        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out);
+        hipLaunchKernelGGL(stage_kernel, grd, blk, 0, st, n, B, w->y, a, w->h, out, ctl.bath, w->md, cs,
+                           c->d_bath);
         TT_HIP(c, hipGetLastError());
         return TG_OK;
     };
-    auto eval = [&](const double* in, double* out) { return propagate_device(c, in, c->d_bath, out, B); };
+    auto eval = [&](const double* in, int q) {
+        tg_status r = propagate_device(c, in, c->d_bath, k[q], B);
+        if (r == TG_OK && ctl.flux) r = tuple_flux(c, w, k[q], B, w->h, w->flux + (int64_t)q * w->cap);
+        return r;
+    };
     const dim3 cgrd((unsigned)((B + 255) / 256));
     hipLaunchKernelGGL(prep_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->h, w->running);
     TT_HIP(c, hipGetLastError());
     tg_status s;
     // stages k2..k6, y5 and k7 (DormandPrince54.hpp:565-578)
-    if ((s = stage({A21}, 1, w->ys)) != TG_OK || (s = eval(w->ys, k[1])) != TG_OK) return s;
-    if ((s = stage({A31, A32}, 2, w->ys)) != TG_OK || (s = eval(w->ys, k[2])) != TG_OK) return s;
-    if ((s = stage({A41, A42, A43}, 3, w->ys)) != TG_OK || (s = eval(w->ys, k[3])) != TG_OK) return s;
-    if ((s = stage({A51, A52, A53, A54}, 4, w->ys)) != TG_OK || (s = eval(w->ys, k[4])) != TG_OK) return s;
-    if ((s = stage({A61, A62, A63, A64, A65}, 5, w->ys)) != TG_OK || (s = eval(w->ys, k[5])) != TG_OK) return s;
-    if ((s = stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5)) != TG_OK || (s = eval(w->y5, k[6])) != TG_OK) return s;
+    if ((s = stage({A21}, 1, w->ys, C2)) != TG_OK || (s = eval(w->ys, 1)) != TG_OK) return s;
+    if ((s = stage({A31, A32}, 2, w->ys, C3)) != TG_OK || (s = eval(w->ys, 2)) != TG_OK) return s;
+    if ((s = stage({A41, A42, A43}, 3, w->ys, C4)) != TG_OK || (s = eval(w->ys, 3)) != TG_OK) return s;
+    if ((s = stage({A51, A52, A53, A54}, 4, w->ys, C5)) != TG_OK || (s = eval(w->ys, 4)) != TG_OK) return s;
+    if ((s = stage({A61, A62, A63, A64, A65}, 5, w->ys, 1.0)) != TG_OK || (s = eval(w->ys, 5)) != TG_OK) return s;
+    if ((s = stage({B1, 0.0, B3, B4, B5, B6}, 6, w->y5, 1.0)) != TG_OK || (s = eval(w->y5, 6)) != TG_OK) return s;
     {   // err = y5 - y4 (:580)
         StageArgs a{};
         const double bh[7] = {BH1, 0.0, BH3, BH4, BH5, BH6, BH7};
@@ -1532,7 +1615,7 @@ tg_status tuple_attempt(tg_tuple_ctx* c, TupleDp5* w, int32_t B, const tg::Ctl& 
     }
     hipLaunchKernelGGL(dg_relerr_final_kernel, cgrd, dim3(256), 0, st, w->part, c->dyn ? w->part2 : (double*)nullptr,
                        w->nblk, B, w->norms);
-    hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc, (double*)nullptr,
+    hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc, d_hlog,
                        w->running);
     TT_HIP(c, hipGetLastError());
     if (d_dense) {
@@ -1560,6 +1643,7 @@ extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
             if (p) (void)hipFree(p);
     }
     if (c->d_eps_ph) (void)hipFree(c->d_eps_ph);
+    if (c->d_bsched) (void)hipFree(c->d_bsched);
     void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
                     c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft};
     for (void* p : bufs)
@@ -1712,8 +1796,25 @@ extern "C" tg_status tg_tuple_set_indices(tg_tuple_ctx* c, const int32_t* idx, i
                             (long long)c->nel);
         if (idx[t * 4 + 3] < 0 || idx[t * 4 + 3] >= nuv)
             return fail(c, TG_ERR_ARG, "tuple %lld: umklapp id %d outside [0, %d)", (long long)t, idx[t * 4 + 3], nuv);
-        if (t > 0 && idx[t * 4] < idx[(t - 1) * 4])
-            return fail(c, TG_ERR_ARG, "tuples must be ordered by their leg-0 element (tuple %lld)", (long long)t);
+    }
+    // The leg lists are built from a list ordered by the leg-0 element.  The
+    // reference writes one full sweep per umklapp vector
+    // (ScatteringChannel.cpp:473-532), so a list in any other order is put in
+    // leg-0 order by a stable counting sort (tuples of one e0 keep their
+    // order; the propagate sums depend on the order only through rounding).
+    std::vector<int32_t> sorted;
+    bool ordered = true;
+    for (int64_t t = 1; t < ntup && ordered; ++t) ordered = idx[t * 4] >= idx[(t - 1) * 4];
+    if (!ordered) {
+        std::vector<int64_t> start((size_t)c->nel + 1, 0);
+        for (int64_t t = 0; t < ntup; ++t) ++start[(size_t)idx[t * 4] + 1];
+        for (int64_t e = 0; e < c->nel; ++e) start[(size_t)e + 1] += start[(size_t)e];
+        sorted.resize((size_t)ntup * 4);
+        for (int64_t t = 0; t < ntup; ++t) {
+            const int64_t d = start[(size_t)idx[t * 4]]++;
+            for (int q = 0; q < 4; ++q) sorted[(size_t)d * 4 + q] = idx[t * 4 + q];
+        }
+        idx = sorted.data();
     }
     tg_status s;
     if ((s = dalloc(c, &c->d_tup, (size_t)ntup * 4)) != TG_OK) return s;
@@ -1735,6 +1836,35 @@ extern "C" tg_status tg_tuple_copy_indices(tg_tuple_ctx* c, int32_t* idx, int64_
     if (!c->d_tup) return fail(c, TG_ERR_STATE, "no tuple list yet (call tg_tuple_set_points)");
     if (cap < c->ntup) return fail(c, TG_ERR_ARG, "cap %lld < %lld tuples", (long long)cap, (long long)c->ntup);
     if (c->ntup) TT_HIP(c, hipMemcpy(idx, c->d_tup, sizeof(int32_t) * 4 * c->ntup, hipMemcpyDeviceToHost));
+    return TG_OK;
+}
+
+extern "C" tg_status tg_tuple_set_bath_schedule(tg_tuple_ctx* c, const tg_bath_schedule* s) {
+    if (!c) return TG_ERR_ARG;
+    if (!s) {
+        c->h_bsched.clear();
+        c->bs_ns = c->bs_batch = 0;
+        return TG_OK;
+    }
+    if (c->dyn) return fail(c, TG_ERR_STATE, "the phonon band evolves (dynamic_phonons): it has no bath");
+    if (!s->samples || s->n_samples < 2 || s->batch < 1 || s->batch > c->max_batch)
+        return fail(c, TG_ERR_ARG, "bath schedule needs samples, n_samples >= 2 and batch in [1, max_batch]");
+    if (!(s->dt > 0.0) || !std::isfinite(s->t_start) || !std::isfinite(s->dt) || !(s->atol >= 0.0) ||
+        !(s->rtol >= 0.0) || (!(s->atol > 0.0) && !(s->rtol > 0.0)))
+        return fail(c, TG_ERR_ARG, "bath schedule needs dt > 0, finite t_start, atol, rtol >= 0 not both 0");
+    const size_t cnt = (size_t)s->batch * s->n_samples;
+    for (size_t i = 0; i < cnt; ++i)
+        if (!std::isfinite(s->samples[i])) return fail(c, TG_ERR_NONFINITE, "bath schedule sample %zu is not finite", i);
+    tg_status r;
+    if ((r = dalloc(c, &c->d_bsched, cnt)) != TG_OK) return r;
+    TT_HIP(c, hipMemcpy(c->d_bsched, s->samples, sizeof(double) * cnt, hipMemcpyHostToDevice));
+    c->h_bsched.assign(s->samples, s->samples + cnt);
+    c->bs_ns = s->n_samples;
+    c->bs_batch = s->batch;
+    c->bs_t0 = s->t_start;
+    c->bs_dt = s->dt;
+    c->bs_atol = s->atol;
+    c->bs_rtol = s->rtol;
     return TG_OK;
 }
 
@@ -1839,15 +1969,22 @@ extern "C" tg_status tg_tuple_load_indices(tg_tuple_ctx* c, const char* path) {
 // member running" flag it reads, as in tg_dp5_integrate.
 extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32_t batch, double t0, double t1,
                                   double dense_dt, double rtol, double atol, const double* bath_n, int32_t flags,
-                                  double* dense_out, int32_t max_dense, tg_step_stats* stats) {
+                                  double* dense_out, int32_t max_dense, tg_step_stats* stats,
+                                  double* h_accepted, int32_t h_cap) {
     using namespace tg;
-    if (!c || !f || (!bath_n && !c->dyn)) return TG_ERR_ARG;
+    const bool sched_on = (flags & TG_BATH_SCHEDULE) != 0;
+    if (h_accepted && h_cap < 1) return TG_ERR_ARG;
+    if (!c || !f || (!bath_n && !c->dyn && !sched_on)) return TG_ERR_ARG;
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
         return fail(c, TG_ERR_ARG, "batch %d outside [1, max_batch=%d]", batch, c->max_batch);
     if (!(dense_dt > 0.0)) return fail(c, TG_ERR_ARG, "dense_dt must be > 0");
     if (dense_out && max_dense < 1) return fail(c, TG_ERR_ARG, "max_dense < 1");
+    if (sched_on && (c->dyn || c->bs_ns < 2))
+        return fail(c, TG_ERR_STATE, "TG_BATH_SCHEDULE needs a bath channel and tg_tuple_set_bath_schedule");
+    if (sched_on && batch > c->bs_batch)
+        return fail(c, TG_ERR_ARG, "batch %d exceeds the %d members of the bath schedule", batch, c->bs_batch);
     tg_status s;
     if ((s = tuple_dp5_ensure(c)) != TG_OK) return s;
     TupleDp5* w = (TupleDp5*)c->dp5;
@@ -1855,9 +1992,23 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     hipStream_t st = c->stream;
     const size_t vec = (size_t)w->n * batch;
     TT_HIP(c, hipMemcpyAsync(w->y, f, sizeof(double) * vec, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    if (bath_n)
+    // the bath: constant per member, or the schedule's n_b(t0) for k1
+    BathSched bs{};
+    std::vector<double> bath0;
+    if (sched_on) {
+        bs.tab = c->h_bsched.data();
+        bs.ns = c->bs_ns;
+        bs.t_start = c->bs_t0;
+        bs.dt = c->bs_dt;
+        bs.atol = c->bs_atol;
+        bs.rtol = c->bs_rtol;
+        bath0.resize(batch);
+        for (int b = 0; b < batch; ++b) bath0[b] = sched_value(bs.tab + (size_t)b * bs.ns, bs.ns, bs.t_start, bs.dt, t0);
+        TT_HIP(c, hipMemcpyAsync(c->d_bath, bath0.data(), sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    } else if (bath_n) {
         TT_HIP(c, hipMemcpyAsync(c->d_bath, bath_n, sizeof(double) * batch,
                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    }
     double* d_dense = nullptr;
     if (dense_out) {
         if (dev) {
@@ -1881,15 +2032,31 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     ctl.denom = (double)((c->dim + 1) * (c->dim + 2)) * (double)c->nel;
     ctl.B = batch;
     ctl.replay = 0;
-    ctl.h_cap = 1;
+    ctl.h_cap = h_accepted ? h_cap : 1;
     ctl.max_dense = d_dense ? max_dense : 0;
+    if (h_accepted) {
+        const size_t need = (size_t)batch * h_cap;
+        if (w->hlog_cap < need) {
+            if ((s = dalloc(c, &w->hlog, need)) != TG_OK) return s;
+            w->hlog_cap = need;
+        }
+        // entries past a member's accepted count keep the caller's values
+        TT_HIP(c, hipMemcpyAsync(w->hlog, h_accepted, sizeof(double) * need, hipMemcpyHostToDevice, st));
+    }
     ctl.sched = nullptr;
     ctl.nsched = nullptr;
+    ctl.bath = bs;
+    ctl.bath.tab = sched_on ? c->d_bsched : nullptr;
+    ctl.flux = c->dyn ? nullptr : w->flux;   // with dynamical phonons there is no bath
+    ctl.flux_stride = w->cap;
     std::vector<MemberDev> ms(batch);
-    for (auto& m : ms) {
+    for (int b = 0; b < batch; ++b) {
+        MemberDev& m = ms[b];
         std::memset(&m, 0, sizeof(m));
         m.t = t0;
-        m.h = dense_dt;
+        // the first step is vetted too (DormandPrince54.hpp:562-564)
+        m.h = std::fmin(dense_dt, step_vetoer(bs, sched_on ? bs.tab + (size_t)b * bs.ns : nullptr, t0, dense_dt,
+                                              dense_dt));
         m.dense_time = t0 + dense_dt;
         m.olderror = 1.0;
         m.st.evals = 1;
@@ -1898,13 +2065,14 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     const bool timing = c->time_kernel;
     c->time_kernel = false;   // no event records inside the graph
     s = propagate_device(c, w->y, c->d_bath, w->k[0], batch);   // k1 = timeOp(y0)
+    if (s == TG_OK && ctl.flux) s = tuple_flux(c, w, w->k[0], batch, nullptr, w->flux);
     if (s != TG_OK) { c->time_kernel = timing; return s; }
     TT_HIP(c, hipStreamSynchronize(st));
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     TT_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    tg_status cap = tuple_attempt(c, w, batch, ctl, rtol, atol, d_dense);
+    tg_status cap = tuple_attempt(c, w, batch, ctl, rtol, atol, d_dense, h_accepted ? w->hlog : nullptr);
     c->time_kernel = timing;
     const hipError_t ec = hipStreamEndCapture(st, &graph);
     if (cap != TG_OK) {
@@ -1948,10 +2116,13 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     TT_HIP(c, hipMemcpyAsync(f, w->y, sizeof(double) * vec, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     if (dense_out && !dev)
         TT_HIP(c, hipMemcpyAsync(dense_out, d_dense, sizeof(double) * (size_t)max_dense * vec, hipMemcpyDeviceToHost, st));
+    if (h_accepted)
+        TT_HIP(c, hipMemcpyAsync(h_accepted, w->hlog, sizeof(double) * (size_t)batch * h_cap, hipMemcpyDeviceToHost, st));
     TT_HIP(c, hipStreamSynchronize(st));
     if (!(flags & TG_QUIET) && batch == 1) printf("Number of adaptive steps: %lld\n", (long long)ms[0].st.attempted);
     for (int b = 0; b < batch; ++b) {
         ms[b].st.failed = ms[b].failed ? 1 : 0;
+        ms[b].st.bath_energy = ms[b].ledger;
         if (stats) stats[b] = ms[b].st;
         if (ms[b].nonfinite) rc = TG_ERR_NONFINITE;
         else if (ms[b].failed && rc == TG_OK) rc = TG_ERR_STEP_UNDERFLOW;

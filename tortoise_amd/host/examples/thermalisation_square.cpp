@@ -11,7 +11,12 @@
 //
 // With ref=1 dyn=1 the phonon band evolves instead of acting as a bath.
 //
-// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0] [ref=0] [dyn=0]
+// With pulse=A > 0 the phonon band is constrained to a pulsed bath,
+// n(t) = n_BE + A (1 - x^2)^2 for |x| < 1, x = (t - t_final/4) / (t_final/10),
+// the way the reference's examples/main.cpp:93-94 drives a band
+// (Band::constrained and constrainedPopulation).
+//
+// usage: thermalisation_square [sections=32] [t_final=100] [quiet=1] [g_ee=0] [ref=0] [dyn=0] [pulse=0]
 #include <TORTOISE>
 
 #include <cmath>
@@ -27,6 +32,7 @@ int main(int argc, char** argv) {
     const Real gEE = argc > 4 ? std::atof(argv[4]) : 0.0;
     const bool ref = argc > 5 ? std::atoi(argv[5]) != 0 : false;
     const bool dynPh = argc > 6 ? std::atoi(argv[6]) != 0 : false;
+    const Real pulse = argc > 7 ? std::atof(argv[7]) : 0.0;
     const Real pi = M_PI;
 
     Region<2> zone(Point<2>({-pi, -pi}), {Point<2>({2 * pi, 0.0}), Point<2>({0.0, 2 * pi})});
@@ -64,6 +70,15 @@ int main(int argc, char** argv) {
         Function<2> rates = mat.scatteringRates<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, start, "el:C");
         std::printf("rates integral %.15e\n", rates.integrate());
     }
+    if (pulse > 0.0) {
+        const Real nBE = 1.0 / std::expm1(0.05 / T), tp = tFinal / 4, w = tFinal / 10;
+        mat["pn:opt"].constrained = true;
+        mat["pn:opt"].constrainedPopulation = [&mesh, nBE, tp, w, pulse](Real t) {
+            const Real x = (t - tp) / w;
+            const Real q = 1.0 - x * x;
+            return Function<2>(mesh, nBE + (std::fabs(x) < 1.0 ? pulse * q * q : 0.0));
+        };
+    }
     MaterialTimeStatus<2> dyn(start);
     dyn.quiet = quiet;
     dyn.propagateDeterministic(tFinal, tFinal / 100.0);
@@ -77,6 +92,7 @@ int main(int argc, char** argv) {
     const Function<2>& fEnd = dyn.back()["el:C"];
     std::printf("final %.4f | %.15e | %.15e\n", dyn.argback(), fEnd.integrate(),
                 fEnd.integrate(mat["el:C"].dispersion));
+    if (pulse > 0.0) std::printf("bath at the end %.15e\n", dyn.back().bathOccupation());
     if (mat.dynamicPhonons) {
         const Function<2>& pEnd = dyn.back()["pn:opt"];
         std::printf("phonons %.15e | %.15e\n", pEnd.integrate(), pEnd.integrate(mat["pn:opt"].dispersion));

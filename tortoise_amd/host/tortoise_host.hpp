@@ -277,6 +277,11 @@ public:
     Real absoluteErrorTolerance = 1.e-3;         // Band.hpp:77-82
     Real relativeErrorTolerance = 1.e-3;
     Real penalisationUnphysicalPopulation = 0.;
+    // The device holds a constrained (uniform) population as samples of
+    // constrainedPopulation(t), linear in between: samples per dense step of
+    // a propagation (a backend parameter; the reference calls the function
+    // itself at every stage time, MaterialStatus.cpp:198-206).
+    int constraintSamplesPerDenseStep = 64;
     Band(std::string n, Real ch, Real st, const Mesh<NDim>& m, const Function<NDim>& disp)
         : name(std::move(n)), charge(ch), statistics(st), mesh(&m), dispersion(disp) {}
 };
@@ -666,11 +671,22 @@ public:
                                           b.relativeErrorTolerance, b.absoluteErrorTolerance, &out, 0));
         return out;
     }
-    // occupation of the phonon bath: the (uniform) population of the phonon band
+    // occupation of the phonon bath: the population of the phonon band, which
+    // the backend's bath forms need uniform (a non-uniform one is refused,
+    // not replaced by one of its values)
     Real bathOccupation() const {
         const auto& ch = material->channels.front();
-        const auto v = (*this)[ch.phononBand].toNodal();
-        return v.empty() ? 0.0 : v[0];
+        return uniformValue((*this)[ch.phononBand], "phonon band population");
+    }
+    // the value of a uniform population; throws when its nodal values differ
+    static Real uniformValue(const Function<NDim>& f, const char* what) {
+        const auto v = f.toNodal();
+        if (v.empty()) return 0.0;
+        for (Real x : v)
+            if (!(std::fabs(x - v[0]) <= 1e-12 * std::fabs(v[0])))
+                throw std::runtime_error(std::string("tortoise_gpu: the ") + what +
+                                         " is not uniform; the backend's phonon bath takes one occupation");
+        return v[0];
     }
 };
 
@@ -692,8 +708,16 @@ Function<NDim> Material<NDim>::scatteringRates(const std::array<std::string, NLe
     Function<NDim> out(*fe.mesh);
     if (!sel || bandName != ch.electronBand) return out;   // no selected channel acts on the band
     if (!haveMCPoints()) constructMCPoints();
-    const Real n = status.bathOccupation();
     checkTuple(tg_tuple_set_amplitude(t, ch.amplitude));
+    if (dynamicPhonons) {
+        // the state holds both bands, [electrons | phonons]
+        const Function<NDim>& fp = status[ch.phononBand];
+        std::vector<Real> y(fe.vec);
+        y.insert(y.end(), fp.vec.begin(), fp.vec.end());
+        checkTuple(tg_tuple_rates(t, y.data(), nullptr, out.vec.data(), fe.mesh->numberElements, 1, 0));
+        return out;
+    }
+    const Real n = status.bathOccupation();
     checkTuple(tg_tuple_rates(t, fe.vec.data(), &n, out.vec.data(), fe.mesh->numberElements, 1, 0));
     return out;
 }
@@ -720,6 +744,9 @@ public:
 
     // MaterialTimeStatus.cpp:102-137 -> DormandPrince54.hpp:479-665: adaptive
     // DP5(4) from the last entry to finalTime; every dense output is appended.
+    // A constrained phonon band (Band::constrained, a uniform population
+    // n(t)) is the bath the device evaluates at every stage time, and the
+    // step vetoer reads it (MaterialStatus.cpp:198-206, MaterialTimeStatus.cpp:60-100).
     void propagate(Real finalTime, Real denseStep) {
         if (material->referenceOperator) {
             propagateReference(finalTime, denseStep);
@@ -728,16 +755,24 @@ public:
         const MaterialStatus<NDim> start = valVec.back();
         const auto& ch = material->channels.front();
         const Band<NDim>& el = (*material)[ch.electronBand];
-        tg_ctx* ctx = material->backend(start.bathOccupation());
+        const Band<NDim>& pb = (*material)[ch.phononBand];
+        checkConstraints(el, pb);
+        const Real t0 = argVec.back();
+        tg_ctx* ctx = material->backend(pb.constrained ? constrainedValue(pb, t0) : start.bathOccupation());
+        int32_t flags = quiet ? TG_QUIET : 0;
+        if (pb.constrained) {
+            const Schedule sc = sampleConstraint(pb, t0, finalTime, denseStep);
+            tg_bath_schedule d = sc.desc(pb);
+            material->check(tg_set_bath_schedule(ctx, &d));
+            flags |= TG_BATH_SCHEDULE;
+        }
         std::vector<Real> f = start[ch.electronBand].toNodal();
         const int64_t n = (int64_t)f.size();
-        const Real t0 = argVec.back();
         const int maxDense = (int)std::floor((finalTime - t0) / denseStep + 1e-9) + 1;
         std::vector<Real> dense((size_t)maxDense * n);
         tg_status s = tg_dp5_integrate(ctx, f.data(), n, 1, t0, finalTime, denseStep,
                                        el.relativeErrorTolerance, el.absoluteErrorTolerance, nullptr,
-                                       quiet ? TG_QUIET : 0, dense.data(), maxDense, &lastStats,
-                                       nullptr, 0);
+                                       flags, dense.data(), maxDense, &lastStats, nullptr, 0);
         if (s != TG_OK && s != TG_ERR_STEP_UNDERFLOW) material->check(s);
         if (s == TG_ERR_STEP_UNDERFLOW) std::printf("The step size has become too small to be handled\n");
         const int nd = lastStats.n_dense < maxDense ? lastStats.n_dense : maxDense;
@@ -745,7 +780,9 @@ public:
             MaterialStatus<NDim> st = start;
             st[ch.electronBand] = Function<NDim>::fromNodal(
                 *el.mesh, std::vector<Real>(dense.begin() + (size_t)k * n, dense.begin() + (size_t)(k + 1) * n));
-            argVec.push_back(t0 + (k + 1) * denseStep);
+            const Real td = t0 + (k + 1) * denseStep;
+            if (pb.constrained) st[ch.phononBand] = pb.constrainedPopulation(td);   // enforceValue at td
+            argVec.push_back(td);
             valVec.push_back(std::move(st));
         }
     }
@@ -753,6 +790,50 @@ public:
     void propagateDeterministic(Real finalTime, Real denseStep) { propagate(finalTime, denseStep); }
 
 private:
+    // samples of a constrained uniform population from t0 over twice the
+    // propagation span (the last step may end past finalTime; beyond the
+    // samples the population holds its last value)
+    struct Schedule {
+        std::vector<Real> v;
+        Real t0 = 0, dt = 1;
+        tg_bath_schedule desc(const Band<NDim>& b) const {
+            tg_bath_schedule d{};
+            d.samples = v.data();
+            d.n_samples = (int32_t)v.size();
+            d.batch = 1;
+            d.t_start = t0;
+            d.dt = dt;
+            d.atol = b.absoluteErrorTolerance;
+            d.rtol = b.relativeErrorTolerance;
+            return d;
+        }
+    };
+    static Real constrainedValue(const Band<NDim>& b, Real t) {
+        return MaterialStatus<NDim>::uniformValue(b.constrainedPopulation(t), "constrained phonon population");
+    }
+    static Schedule sampleConstraint(const Band<NDim>& b, Real t0, Real t1, Real denseStep) {
+        if (!b.constrainedPopulation) throw std::runtime_error("tortoise_gpu: constrained band without a constrainedPopulation");
+        Schedule sc;
+        sc.t0 = t0;
+        sc.dt = denseStep / (Real)(b.constraintSamplesPerDenseStep > 0 ? b.constraintSamplesPerDenseStep : 1);
+        const int64_t ns = (int64_t)std::ceil(2.0 * (t1 - t0) / sc.dt) + 2;
+        if (ns > (int64_t)1 << 24) throw std::runtime_error("tortoise_gpu: too many constraint samples");
+        sc.v.resize((size_t)ns);
+        for (int64_t k = 0; k < ns; ++k) sc.v[(size_t)k] = constrainedValue(b, t0 + (Real)k * sc.dt);
+        return sc;
+    }
+    // what the backend cannot honour is refused, not ignored
+    void checkConstraints(const Band<NDim>& el, const Band<NDim>& pb) const {
+        if (el.constrained)
+            throw std::runtime_error("tortoise_gpu: a constrained electron band is not supported by the backend");
+        if (pb.constrained && material->dynamicPhonons)
+            throw std::runtime_error(
+                "tortoise_gpu: a constrained phonon band cannot evolve (dynamicPhonons); use it as the bath");
+        for (const auto& b : material->bands)
+            if (b.constrained && b.name != el.name && b.name != pb.name)
+                throw std::runtime_error("tortoise_gpu: constrained band " + b.name + " is not on the backend's channel");
+    }
+
     // MaterialTimeStatus.cpp:102-137 with the reference operator: new MC points
     // (material->constructMCPoints()), then DP5(4) of the electron band's DG
     // coefficients on the device (tg_tuple_dp5).
@@ -762,20 +843,29 @@ private:
         material->constructMCPoints();
         const auto& ch = material->tupleChannel();
         const Band<NDim>& el = (*material)[ch.electronBand];
+        const Band<NDim>& pb = (*material)[ch.phononBand];
+        checkConstraints(el, pb);
         const bool dyn = material->dynamicPhonons;
         std::vector<Real> f = start[ch.electronBand].vec;
         const size_t ne = f.size();
         if (dyn) f.insert(f.end(), start[ch.phononBand].vec.begin(), start[ch.phononBand].vec.end());
         const int64_t n = (int64_t)f.size();
         const Real t0 = argVec.back();
-        const Real bath = start.bathOccupation();
+        const Real bath = (dyn || pb.constrained) ? 0.0 : start.bathOccupation();
+        int32_t flags = quiet ? TG_QUIET : 0;
+        if (pb.constrained) {
+            const Schedule sc = sampleConstraint(pb, t0, finalTime, denseStep);
+            tg_bath_schedule d = sc.desc(pb);
+            material->checkTuple(tg_tuple_set_bath_schedule(t, &d));
+            flags |= TG_BATH_SCHEDULE;
+        }
         const int maxDense = (int)std::floor((finalTime - t0) / denseStep + 1e-9) + 1;
         std::vector<Real> dense((size_t)maxDense * n);
         material->checkTuple(tg_tuple_set_amplitude(t, ch.amplitude));
         tg_status s = tg_tuple_dp5(t, f.data(), el.mesh->numberElements, 1, t0, finalTime, denseStep,
                                    el.relativeErrorTolerance, el.absoluteErrorTolerance,
-                                   dyn ? nullptr : &bath, quiet ? TG_QUIET : 0, dense.data(), maxDense,
-                                   &lastStats);
+                                   (dyn || pb.constrained) ? nullptr : &bath, flags, dense.data(), maxDense,
+                                   &lastStats, nullptr, 0);
         if (s != TG_OK && s != TG_ERR_STEP_UNDERFLOW) material->checkTuple(s);
         if (s == TG_ERR_STEP_UNDERFLOW) std::printf("The step size has become too small to be handled\n");
         const int nd = lastStats.n_dense < maxDense ? lastStats.n_dense : maxDense;
@@ -786,7 +876,9 @@ private:
             if (dyn)
                 std::copy(dense.begin() + (size_t)k * n + ne, dense.begin() + (size_t)(k + 1) * n,
                           st[ch.phononBand].vec.begin());
-            argVec.push_back(t0 + (k + 1) * denseStep);
+            const Real td = t0 + (k + 1) * denseStep;
+            if (pb.constrained) st[ch.phononBand] = pb.constrainedPopulation(td);   // enforceValue at td
+            argVec.push_back(td);
             valVec.push_back(std::move(st));
         }
     }

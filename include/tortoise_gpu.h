@@ -174,6 +174,22 @@ tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double
 tg_status   tg_collision_apply(tg_ctx* ctx, const double* f, double* I, int64_t n,
                                int32_t batch, const double* bath_n, int32_t flags);
 
+/* tg_collision_apply (plain FP64 path) that allocates nothing: no
+ * hipMalloc, hipFree or allocating library call and no stream
+ * synchronisation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  No counterpart in the reference.  f and I
+ * are device memory (TG_PTR_DEVICE is required); bath_n is host memory (batch
+ * values) or NULL; TG_CONSERVE is honoured; TG_PREC_MIXED, TG_TIME_KERNEL and
+ * contexts with the e-e channel are refused.  Its scratch (the gathered state
+ * and the partial sums) is a 16 MiB device array per GPU, held until the
+ * process ends: a context whose band needs more (beyond about 128 x 128
+ * sections) is refused with TG_ERR_ARG.  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other streams or contexts)
+ * race; calls on one stream are ordered.  The work is enqueued on the
+ * context's stream and the call returns without waiting. */
+tg_status   tg_collision_apply_noalloc(tg_ctx* ctx, const double* f, double* I, int64_t n,
+                                       int32_t batch, const double* bath_n, int32_t flags);
+
 /* Duration in ms of the dominant kernel of the last tg_collision_apply made
  * with TG_TIME_KERNEL (plain: the K-band K f / K^T f pass; mixed: the
  * MFMA GEMM), from HIP events on the context's stream.  Waits for it. */

@@ -51,6 +51,7 @@ EXPORTS = [
     "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
     "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream", "tg_tuple_dp5",
     "tg_tuple_rates", "tg_set_bath_schedule", "tg_tuple_set_bath_schedule",
+    "tg_collision_apply_noalloc",
 ]
 
 
@@ -165,6 +166,8 @@ def load():
     L.tg_kernel_copy_rows.argtypes = [vp, i64, i64, vp]
     L.tg_collision_apply.restype = i32
     L.tg_collision_apply.argtypes = [vp, vp, vp, i64, i32, vp, i32]
+    L.tg_collision_apply_noalloc.restype = i32
+    L.tg_collision_apply_noalloc.argtypes = [vp, vp, vp, i64, i32, vp, i32]
     L.tg_dp5_integrate.restype = i32
     L.tg_dp5_integrate.argtypes = [vp, vp, i64, i32, f64, f64, f64, f64, f64, vp, i32, vp, i32,
                                    vp, vp, i32]
@@ -432,6 +435,19 @@ class Context:
         self._check(load().tg_collision_apply(self._h, C.c_void_p(f_ptr), C.c_void_p(I_ptr),
                                               self.n, batch, None if bn is None else _p(bn),
                                               flags))
+
+
+def collision_apply_noalloc(ctx, f_ptr, I_ptr, batch=1, bath_host=None, conserve=False,
+                            flags=None):
+    """tg_collision_apply_noalloc on device pointers; returns the status code
+    (no exception, so callers can check refusals)."""
+    L = load()
+    bn = None
+    if bath_host is not None:
+        bn = np.ascontiguousarray(np.broadcast_to(bath_host, (batch,)), np.float64)
+    fl = (TG_PTR_DEVICE | (TG_CONSERVE if conserve else 0)) if flags is None else flags
+    return L.tg_collision_apply_noalloc(ctx._h, C.c_void_p(f_ptr), C.c_void_p(I_ptr), ctx.n, batch,
+                                        None if bn is None else _p(bn), fl)
 
 
 class TupleOperator:

@@ -253,6 +253,10 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     };
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e)));
+    {
+        const tg_status sn = noalloc_init(c);   // the device's scratch address, once per device
+        if (sn != TG_OK) return fail(sn);
+    }
     e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     c->stream = c->own_stream;

@@ -196,6 +196,9 @@ tg_status stack_prepare(tg_ctx* c);
 // u = K f, v = K^T f (caller order) from the same pass
 tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
 void stack_destroy(tg_ctx* c);
+// records this device's address of the allocation-free evaluation's scratch
+// array (tg_collision_apply_noalloc); called at context creation
+tg_status noalloc_init(tg_ctx* c);
 tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve,
                              const double* d_bath, double* d_flux);
 // from the block sums dpart (4 per block): with conserve I -= s |f(1-f)|

@@ -29,6 +29,7 @@
 // It streams 8 bytes per stored band entry of K and writes 8 bytes per item
 // row and per item column of partials (about 1% of the band).
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "tg_internal.hpp"
@@ -435,4 +436,94 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     return TG_OK;
 }
 
+// ------------------------------------------------ allocation-free variant
+// tg_collision_apply_noalloc runs the same kernels as the plain evaluation
+// with their scratch (the gathered state, the row and column partials, the
+// number-defect partials) in one file-scope device array instead of buffers
+// the library allocates.  Each GPU holds its own copy of the array for the
+// life of the process; the host reaches it only through the device address
+// hipGetSymbolAddress returns (recorded per device when a context is created
+// on it): the array's name in host code is not a device address.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;   // 16 MiB per device
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+static double* g_scratch[kNoAllocMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// called by tg_ctx_create with the context's device current
+tg_status noalloc_init(tg_ctx* c) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device < 0 || c->device >= kNoAllocMaxDevices)
+        return set_error(c, TG_ERR_ARG, "device %d beyond the %d the scratch table holds", c->device,
+                         kNoAllocMaxDevices);
+    if (g_scratch[c->device]) return TG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(c, TG_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[c->device] = (double*)p;
+    return TG_OK;
+}
+
+static size_t round256(size_t doubles) { return (doubles + 31) / 32 * 32; }
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" tg_status tg_collision_apply_noalloc(tg_ctx* c, const double* f, double* I, int64_t n, int32_t batch,
+                                                const double* bath_n, int32_t flags) {
+    if (!c || !f || !I) return TG_ERR_ARG;
+    if (n != c->n)
+        return set_error(c, TG_ERR_ARG, "n=%lld but the mesh has %lld nodes", (long long)n, (long long)c->n);
+    if (batch < 1) return set_error(c, TG_ERR_ARG, "batch < 1");
+    if (!(flags & TG_PTR_DEVICE))
+        return set_error(c, TG_ERR_ARG, "the allocation-free evaluation takes device pointers (TG_PTR_DEVICE)");
+    if (flags & (TG_PREC_MIXED | TG_TIME_KERNEL))
+        return set_error(c, TG_ERR_ARG, "the allocation-free evaluation is the plain FP64 one (no TG_PREC_MIXED, "
+                                        "no TG_TIME_KERNEL)");
+    if (c->ee) return set_error(c, TG_ERR_STATE, "the allocation-free evaluation has no e-e channel");
+    if (!c->assembled) return set_error(c, TG_ERR_STATE, "tg_kernel_assemble not called");
+    StackData* sd = sd_of(c);
+    if (!sd) return set_error(c, TG_ERR_STATE, "band of K not prepared");
+    int dev = -1;
+    TG_HIP(c, hipGetDevice(&dev));
+    if (dev != c->device) return set_error(c, TG_ERR_STATE, "current device %d is not the context's %d", dev, c->device);
+    double* base = g_scratch[dev];
+    if (!base) return set_error(c, TG_ERR_STATE, "no scratch address recorded for device %d", dev);
+    // scratch layout: gathered state | row partials | column partials | block sums
+    const unsigned nb = ceil_div(n, kStackFinishT);
+    const size_t o_fs = 0;
+    const size_t o_part = o_fs + round256((size_t)n);
+    const size_t o_vpart = o_part + round256((size_t)sd->nitems * kStackRows);
+    const size_t o_dpart = o_vpart + round256((size_t)sd->nitems * kStackChunk * kStackIt);
+    const size_t need = o_dpart + round256(4 * (size_t)nb);
+    if (need * sizeof(double) > kNoAllocScratchBytes)
+        return set_error(c, TG_ERR_ARG, "this mesh needs %zu bytes of scratch; the allocation-free evaluation has %zu",
+                         need * sizeof(double), kNoAllocScratchBytes);
+    double* fs = base + o_fs;
+    double* part = base + o_part;
+    double* vpart = base + o_vpart;
+    double* dpart = base + o_dpart;
+    const int conserve = (flags & TG_CONSERVE) ? 1 : 0;
+    hipStream_t st = c->stream;
+    for (int b = 0; b < batch; ++b) {
+        const double* fb = f + (int64_t)b * n;
+        double* Ib = I + (int64_t)b * n;
+        const double nbath = bath_n ? bath_n[b] : c->bath_n;
+        hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, fb, c->d_perm, n, fs);
+        TG_CHECK_LAUNCH(c);
+        hipLaunchKernelGGL(stack_gemv_kernel, dim3(sd->nitems), dim3(kStackThreads), 0, st, sd->Sb, sd->off, sd->kr,
+                           sd->items, n, fs, part, vpart);
+        TG_CHECK_LAUNCH(c);
+        hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, part, vpart, sd->pitems,
+                           sd->cit_off, sd->cit, n, c->d_perm, fs, c->d_rk_s, c->d_ck_s, c->d_V_s, c->d_eps_s, nbath,
+                           (const double*)nullptr, Ib, conserve ? dpart : (double*)nullptr);
+        TG_CHECK_LAUNCH(c);
+        if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, fb, Ib, 1, nullptr));
+    }
+    return TG_OK;
+}
+
+namespace tg {
 }  // namespace tg

@@ -56,38 +56,56 @@ def kernel_ms(ctx, call, reps):
     return tot / reps
 
 
-def cpu_baseline(ctx, w, f, budget_s=15.0):
-    """Oracle W-form (one pass over K, OpenMP) on the host cores, one member."""
+def _cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_wform_band(ctx, w, f, bath_n, budget_s=10.0):
+    """The W-form on the host cores over K's band in energy order (oracle,
+    FP64, OpenMP): the same algorithmic bytes as the GPU's plain evaluation
+    (the nonzeros of K once, plus the n-vectors).  One member; median of the
+    timed evaluations after one warm-up; parity against the GPU's FP64."""
     import numpy as np
 
     import oracle
 
     n = ctx.n
     K = ctx.kernel_rows(0, n)
-    V = ctx.node_volumes()
-    oracle.collision_apply(K, V, w.bath_n, f)  # warm-up
+    band = oracle.KernelBand(lambda idx: K[idx], w.eps(), ctx.node_volumes())
+    del K
+    cores = _cores()
+    scratch = np.empty((cores + 2) * n)
+    ref = band.apply(bath_n, f, nthreads=cores, scratch=scratch)   # warm-up
     times = []
     t_start = time.perf_counter()
-    while len(times) < 5 or time.perf_counter() - t_start < budget_s:
+    while len(times) < 5 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
         t0 = time.perf_counter()
-        oracle.collision_apply(K, V, w.bath_n, f)
+        band.apply(bath_n, f, nthreads=cores, scratch=scratch)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    gpu = ctx.apply(f, bath_n=bath_n)
+    rel = float(np.linalg.norm(gpu - ref) / np.linalg.norm(ref))
+    gbs = (8.0 * band.entries + 32.0 * n) / med / 1e9
+    _log(f"CPU W-form over K's band ({band.entries} entries): {med * 1e3:.2f} ms/eval on {cores} threads "
+         f"({gbs:.1f} GB/s); GPU FP64 vs it {rel:.1e}")
     return {"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 W-form evaluation of one member, n={n}, median of "
-                      f"{len(times)} evaluations ({sum(times):.1f} s)"}
+            "band_entries": band.entries, "gbs": gbs, "gpu_fp64_rel_diff": rel,
+            "sample": f"oracle FP64 W-form over K's band (energy order, {band.entries} entries), one member "
+                      f"of the 256-member ensemble, n={n}, median of {len(times)} after a warm-up "
+                      f"({sum(times):.1f} s)"}
 
 
 def dp5_batched(ctx, w, F, bath, check=(0, 128, 255)):
     """The batched workload integrated: all members with DP5 to t_final (per-
     member adaptive steps, rtol 1e-6, number-defect removal), the attempt
     graph on the device.  Error of f(t_final) against FP64 replays of the same
-    accepted steps for a few members; particle-number drift for all."""
+    accepted steps for a few members; particle-number drift for all; energy
+    drift of electrons + the bath's FP64 ledger for all."""
     import numpy as np
 
     V = ctx.node_volumes()
-    n0 = V @ F
+    E = V * w.eps()
+    n0, e0 = V @ F, E @ F
     t0 = time.perf_counter()
     fT, _, st, steps = ctx.dp5(F, 0.0, w.t_final, w.t_final / 100, w.rtol, w.atol, bath_n=bath,
                                mixed=True, conserve=True, h_cap=8192)
@@ -99,13 +117,16 @@ def dp5_batched(ctx, w, F, bath, check=(0, 128, 255)):
         fr, _ = ctx.dp5_replay(F[:, b], 0.0, w.t_final, w.t_final / 100, steps[b], bath_n=bath[b])
         errs.append(float(np.linalg.norm(fT[:, b] - fr) / np.linalg.norm(fr)))
     drift = float(np.max(np.abs(V @ fT - n0) / np.abs(n0)))
+    ledger = np.array([s["bath_energy"] for s in st])
+    edrift = float(np.max(np.abs(E @ fT + ledger - e0) / np.abs(e0)))
     _log(f"dp5 batched: {F.shape[1]} members to t={w.t_final} in {wall:.3f} s, attempts per member "
          f"{att.min()}..{att.max()}, f(t_final) rel err vs FP64 replay {max(errs):.2e}, "
-         f"number drift {drift:.1e}")
+         f"number drift {drift:.1e}, energy drift (electrons + bath ledger) {edrift:.1e}")
     return {"members": int(F.shape[1]), "t_final": w.t_final, "rtol": w.rtol, "wall_s": wall,
             "attempts_min": int(att.min()), "attempts_max": int(att.max()),
             "member_evals_per_s": evals / wall, "f_final_rel_err_vs_fp64": max(errs),
-            "checked_members": list(check), "number_drift_rel": drift}
+            "checked_members": list(check), "number_drift_rel": drift,
+            "energy_drift_rel": edrift, "bath_energy_member0": float(ledger[0])}, fT
 
 
 def dp5_plain(ctx, w, f0, bath_n):
@@ -117,14 +138,17 @@ def dp5_plain(ctx, w, f0, bath_n):
 
     p = wl.SQUARE128
     V = ctx.node_volumes()
+    E = V * p.eps()
     t0 = time.perf_counter()
     fT, _, st = ctx.dp5(f0, 0.0, p.t_final, p.t_final / 100, p.rtol, p.atol, bath_n=bath_n)
     wall = time.perf_counter() - t0
     drift = float(abs(V @ fT - V @ f0) / abs(V @ f0))
+    edrift = float(abs(E @ fT + st["bath_energy"] - E @ f0) / abs(E @ f0))
     _log(f"dp5 plain: t={p.t_final} rtol {p.rtol} in {wall:.3f} s, {st['attempted']} attempts, "
-         f"number drift {drift:.1e}")
+         f"number drift {drift:.1e}, energy drift (electrons + bath ledger) {edrift:.1e}")
     return {"t_final": p.t_final, "rtol": p.rtol, "wall_s": wall, "attempts": st["attempted"],
-            "evals_per_s": st["evals"] / wall, "number_drift_rel": drift}
+            "evals_per_s": st["evals"] / wall, "number_drift_rel": drift, "energy_drift_rel": edrift,
+            "bath_energy": st["bath_energy"]}
 
 
 def cube_ee(steps=5, t_final=1.0):
@@ -247,14 +271,48 @@ def square64_gradients(T=0.02):
             "dL_df0_norm": float(np.linalg.norm(g_f0)), "dL_dg_rel_diff_vs_central_fd": rel}
 
 
-def reference_operator(budget_s=12.0):
+def hat_moments(w, out):
+    """<phi_i, I> of a DG result [nel][3] (LF coefficients, inverse mass
+    applied) on the nodal hat functions."""
+    import numpy as np
+
+    import oracle
+
+    vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
+    m = out @ np.linalg.inv(oracle.mc_inverse_mass(2, vol))
+    conn = oracle.mesh_elements(2, list(w.nsec))
+    H = np.zeros((w.nsec[0] + 1) * (w.nsec[1] + 1))
+    np.add.at(H, conn[:, 0], m[:, 0] - m[:, 1] - m[:, 2])
+    np.add.at(H, conn[:, 1], m[:, 1])
+    np.add.at(H, conn[:, 2], m[:, 2])
+    return H
+
+
+def wform_vs_reference(w, op, f, bath_n, Hw):
+    """The W-form's weak form against the reference operator's on one state:
+    relative L2 of the hat moments <phi_i, I> and the ratio of the electron
+    cooling powers sum_i eps_i <phi_i, I> (W-form / reference)."""
+    import numpy as np
+
+    import oracle
+
+    Ht = hat_moments(w, op.propagate(oracle.mc_nodal_to_dg(2, w.nsec, f), bath_n))
+    eps = w.eps()
+    return {"hat_moments_rel_l2": float(np.linalg.norm(Hw - Ht) / np.linalg.norm(Ht)),
+            "cooling_power_wform": float(eps @ Hw), "cooling_power_reference": float(eps @ Ht),
+            "cooling_power_ratio": float((eps @ Hw) / (eps @ Ht))}
+
+
+def reference_operator(budget_s=12.0, cpu_budget_s=30.0, member0=None):
     """The reference's own operator (tuple list + MC delta-shell integral,
     NMC = 50, umklapp {-1,0,1}^2) on the square zone: 32x32 sections (the
     reference's CPU-runnable case) and 128x128 sections (the mesh of the
     headline workload).  Timed: tuple enumeration, per-tuple moments,
     propagate for 1 and 8 members; the CPU restatement of the reference
-    algorithm (per stage: every tuple's MC integral, OpenMP) at 32x32 on the
-    host cores; parity against it."""
+    algorithm (per stage: every tuple's MC integral, OpenMP) on the host cores
+    at both sizes, with parity of the GPU against it.  member0 = (f, bath,
+    W-form hat moments) of the ensemble's member 0 on the 128x128 mesh: the
+    W-form's discrepancy from the reference operator on that state."""
     import numpy as np
     import torch
 
@@ -322,38 +380,50 @@ def reference_operator(budget_s=12.0):
                                           "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                                           "traffic": traffic, "bytes_per_launch": byts,
                                           "kernel": f"propagate_kernel<2, {1 if B == 1 else 8}>"}}
-            if ns == 32 and B == 1:
+            if B == 1:
                 Fh = F[0].cpu().numpy()
                 Ig = out[0].cpu().numpy()
         _log(f"tuple operator {ns}x{ns}: {T} tuples, enumerate+moments {t_first:.2f} s, moments "
              f"{t_mom * 1e3:.1f} ms; propagate B=1 {line['b1']['ms']:.3f} ms "
              f"({line['b1']['roofline']['achieved']:.0f} GB/s), B=8 {line['b8']['ms']:.3f} ms "
              f"({line['b8']['evals_per_s']:.0f} evals/s)")
+        # the CPU restatement of the reference algorithm on the GPU-enumerated
+        # list: parity and time per evaluation (one warm-up, then the median)
+        idx = op.indices()
+        fph = np.zeros_like(Fh)
+        fph[:, 0] = w.bath_n
+        pts_now = oracle.mc_points(2, 3, 50, 1)       # the points the moments hold
+        args = (2, w.nsec, w.origin, w.extent, w.eps(), w.omega0, w.g ** 2, 1, pts_now, idx, Fh, fph)
+        cores = _cores()
+        t0 = time.perf_counter()
+        ref = oracle.mc_apply_eph(*args, nthreads=cores)
+        warm = time.perf_counter() - t0
+        line["max_rel_diff_vs_cpu"] = float(np.abs(Ig - ref).max() / np.abs(ref).max())
+        times = []
+        limit = budget_s if ns == 32 else cpu_budget_s
+        t_start = time.perf_counter()
+        while limit > 0 and (len(times) < 3 or (time.perf_counter() - t_start + warm < limit
+                                                 and len(times) < 20)):
+            t0 = time.perf_counter()
+            oracle.mc_apply_eph(*args, nthreads=cores)
+            times.append(time.perf_counter() - t0)
+        med = float(np.median(times)) if times else warm
+        del idx
+        line["cpu_reference_algorithm"] = {
+            "value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"CPU restatement of ScatteringChannel::propagate (every tuple's MC integral "
+                      f"per stage, {T} tuples of the {ns}x{ns}-section mesh, NMC 50), one member, "
+                      f"median of {len(times)} after a warm-up ({sum(times) + warm:.1f} s)"}
+        line["gpu_vs_cpu_b8"] = line["b8"]["evals_per_s"] * med
+        _log(f"  CPU reference algorithm {med:.3f} s/eval on {cores} threads; GPU/CPU "
+             f"{line['gpu_vs_cpu_b8']:.0f}x (8 members); max rel diff {line['max_rel_diff_vs_cpu']:.1e}")
+        if ns == 128 and member0 is not None:
+            f0, b0, Hw = member0
+            line["wform_vs_reference_member0"] = wform_vs_reference(w, op, f0, b0, Hw)
+            d = line["wform_vs_reference_member0"]
+            _log(f"  W-form vs reference operator, member 0: cooling power ratio "
+                 f"{d['cooling_power_ratio']:.3f}, hat moments rel L2 {d['hat_moments_rel_l2']:.3f}")
         if ns == 32:
-            idx = op.indices()
-            fph = np.zeros_like(Fh)
-            fph[:, 0] = w.bath_n
-            pts_now = oracle.mc_points(2, 3, 50, 1)       # the points the moments hold
-            args = (2, w.nsec, w.origin, w.extent, w.eps(), w.omega0, w.g ** 2, 1, pts_now, idx, Fh,
-                    fph)
-            cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-            ref = oracle.mc_apply_eph(*args, nthreads=cores)
-            line["max_rel_diff_vs_cpu"] = float(np.abs(Ig - ref).max() / np.abs(ref).max())
-            times = []
-            t_start = time.perf_counter()
-            while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 20):
-                t0 = time.perf_counter()
-                oracle.mc_apply_eph(*args, nthreads=cores)
-                times.append(time.perf_counter() - t0)
-            med = float(np.median(times))
-            line["cpu_reference_algorithm"] = {
-                "value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
-                "sample": f"CPU restatement of ScatteringChannel::propagate (every tuple's MC "
-                          f"integral per stage, {T} tuples, NMC 50), one member, median of "
-                          f"{len(times)} ({sum(times):.1f} s)"}
-            line["gpu_vs_cpu_b8"] = line["b8"]["evals_per_s"] * med
-            _log(f"  CPU reference algorithm {med:.3f} s/eval on {cores} threads; GPU/CPU "
-                 f"{line['gpu_vs_cpu_b8']:.0f}x (8 members); max rel diff {line['max_rel_diff_vs_cpu']:.1e}")
             # the reference's CPU-runnable propagation: DP5 to t = 100 (rtol 1e-8,
             # atol 1e-12, dense step 1), one member, with the tuple operator and
             # with the W-form on the same mesh
@@ -365,6 +435,11 @@ def reference_operator(budget_s=12.0):
             t0 = time.perf_counter()
             _, _, stw = cw.dp5(wl.SQUARE32.initial_f(), 0.0, 100.0, 1.0, 1e-8, 1e-12)
             t_wform = time.perf_counter() - t0
+            # the W-form's discrepancy from the reference operator on the
+            # initial state of this configuration
+            f32 = wl.SQUARE32.initial_f()
+            Hw32 = cw.node_volumes() * cw.apply(f32)
+            line["wform_vs_reference_initial"] = wform_vs_reference(w, op, f32, w.bath_n, Hw32)
             cw.close()
             line["dp5_t100"] = {"tuple_operator": {"wall_s": t_tuple, "attempts": int(stt["attempted"])},
                                 "w_form": {"wall_s": t_wform, "attempts": int(stw["attempted"])}}
@@ -566,19 +641,65 @@ def main():
         "max_rel_err_vs_fp64": rel,
         "wall_s_timed": wall,
     }
+    out["value_note"] = ("the t = 0 ensemble; panels whose fp16 operands are zero for every member are "
+                         "left out of the GEMM (exactly: their products are 0); value_all_panels and "
+                         "value_t_final are the same evaluation with every panel computed and on the "
+                         "ensemble's state at t_final")
+    # the same batched evaluation with every panel computed (a context built
+    # with TG_MIXED_SKIP=0): the state-independent rate
+    os.environ["TG_MIXED_SKIP"] = "0"
+    try:
+        ctx_all = tg.Context.create(w, max_batch=B)
+        ctx_all.assemble(mixed=True)
+    finally:
+        del os.environ["TG_MIXED_SKIP"]
+
+    def all_call(time_kernel=False):
+        ctx_all.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
+                             time_kernel=time_kernel)
+
+    ms_all, _ = time_loop(torch, ctx_all, all_call, args.steps, args.warmup)
+    ms_gemm_all = kernel_ms(ctx_all, lambda: all_call(True), 10)
+    ctx_all.close()
+    out["value_all_panels"] = B * 1000.0 / ms_all
+    out["all_panels"] = {"ms_per_step": ms_all, "gemm_ms": ms_gemm_all,
+                         "gemm_tflops": 4.0 * st["nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
+    _log(f"all panels: {ms_all:.4f} ms/batch ({out['value_all_panels']:.0f} evals/s), GEMM {ms_gemm_all:.4f} ms")
+    # W-form hat moments of member 0 (FP64) for the comparison with the
+    # reference operator at this mesh
+    I0 = ctx.apply(Fh[:, 0], bath_n=bath[0])
+    member0 = (Fh[:, 0].copy(), float(bath[0]), ctx.node_volumes() * I0)
     if not args.no_dp5:
-        out["dp5_batched"] = dp5_batched(ctx, w, Fh, bath)
+        out["dp5_batched"], fT = dp5_batched(ctx, w, Fh, bath)
         out["dp5_plain"] = dp5_plain(ctx, w, Fh[:, 0].copy(), bath[0])
+        # the headline evaluation on the ensemble's state at t_final
+        Fd.copy_(torch.from_numpy(np.ascontiguousarray(fT.T)))
+        torch.cuda.synchronize()
+        ms_tf, _ = time_loop(torch, ctx, mixed_call, args.steps, args.warmup)
+        st_tf = ctx.kernel_stats()
+        out["value_t_final"] = B * 1000.0 / ms_tf
+        out["t_final_state"] = {"ms_per_step": ms_tf, "t": w.t_final,
+                                "executed_band_fraction": st_tf["mixed_last_band"] / st["mixed_band"]}
+        _log(f"state at t={w.t_final}: {ms_tf:.4f} ms/batch ({out['value_t_final']:.0f} evals/s), "
+             f"{out['t_final_state']['executed_band_fraction']:.0%} of the band computed")
     if not args.no_cpu:
-        cb = cpu_baseline(ctx, w, Fh[:, 0].copy())
-        out["cpu_baseline"] = cb
-        out["mixed_vs_cpu"] = evals_mixed / cb["value"]
+        out["cpu_wform_band"] = cpu_wform_band(ctx, w, Fh[:, 0].copy(), float(bath[0]))
+        out["mixed_vs_cpu_wform_band"] = evals_mixed / out["cpu_wform_band"]["value"]
+        out["cpu_baseline"] = out["cpu_wform_band"]
     ctx.close()
     if not args.no_dp5:
         out["cube20_ee"] = cube_ee(t_final=args.ee_t_final)
         out["square64_gradients"] = square64_gradients()
     if not args.no_reference_op:
-        out["reference_operator"] = reference_operator()
+        ro = reference_operator(cpu_budget_s=0.0 if args.no_cpu else 30.0, member0=member0)
+        out["reference_operator"] = ro
+        if not args.no_cpu:
+            # the reference CPU path on the headline mesh: the reference's own
+            # algorithm (tuple list, MC integral per tuple) restated in C
+            out["cpu_baseline"] = ro["square128"]["cpu_reference_algorithm"]
+            out["mixed_vs_cpu"] = evals_mixed / out["cpu_baseline"]["value"]
+    if "mixed_vs_cpu" not in out and "cpu_baseline" in out:
+        out["mixed_vs_cpu"] = evals_mixed / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

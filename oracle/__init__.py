@@ -110,6 +110,9 @@ def lib():
                                            C.c_int32, C.c_double, C.c_double, _f64p, C.c_double,
                                            C.c_double, C.c_double, _f64p, C.c_int64, C.c_void_p,
                                            C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+        _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.or_collision_apply_band.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                              _i32p, C.c_double, _f64p, _f64p, _f64p, C.c_int]
         L.or_sched_value.restype = C.c_double
         L.or_sched_value.argtypes = [_f64p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.or_step_vetoer.restype = C.c_double
@@ -300,6 +303,49 @@ def collision_apply(K, V, bath_n, f, conserve=False):
     lib().or_collision_apply(f.size, K, np.ascontiguousarray(V, np.float64), bath_n, f, out,
                              int(bool(conserve)))
     return out
+
+
+class KernelBand:
+    """K's band in energy order (the CPU W-form that streams the same bytes as
+    the GPU's plain evaluation).  Built from a dense K (node order) in row
+    blocks: perm sorts the nodes by energy (stable), row p keeps its columns
+    from the first to the last nonzero."""
+
+    def __init__(self, K_rows, eps, V, block=1024):
+        n = eps.size
+        self.n = n
+        self.perm = np.ascontiguousarray(np.argsort(eps, kind="stable"), np.int32)
+        lo = np.zeros(n, np.int32)
+        ln = np.zeros(n, np.int64)
+        chunks = []
+        rk = np.zeros(n)
+        ck = np.zeros(n)
+        for p0 in range(0, n, block):
+            rows = K_rows(self.perm[p0:p0 + block])          # node-order rows
+            Ks = rows[:, self.perm]
+            rk[p0:p0 + len(rows)] = Ks.sum(1)
+            ck += Ks.sum(0)
+            for q in range(len(Ks)):
+                nz = np.flatnonzero(Ks[q])
+                if nz.size:
+                    lo[p0 + q] = nz[0]
+                    ln[p0 + q] = nz[-1] + 1 - nz[0]
+                    chunks.append(Ks[q, nz[0]:nz[-1] + 1].copy())
+        self.lo = lo
+        self.rowptr = np.concatenate([[0], np.cumsum(ln)]).astype(np.int64)
+        self.vals = np.ascontiguousarray(np.concatenate(chunks) if chunks else np.zeros(1))
+        self.rk, self.ck = rk, ck
+        self.Vs = np.ascontiguousarray(V[self.perm], np.float64)
+        self.entries = int(self.rowptr[-1])
+
+    def apply(self, bath_n, f, nthreads=1, scratch=None):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.empty_like(f)
+        if scratch is None:
+            scratch = np.empty((nthreads + 2) * self.n)
+        lib().or_collision_apply_band(self.n, self.rowptr, self.lo, self.vals, self.rk, self.ck, self.Vs,
+                                      self.perm, bath_n, f, out, scratch, nthreads)
+        return out
 
 
 def conservation(V, eps, f):

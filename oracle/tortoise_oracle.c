@@ -416,6 +416,49 @@ void or_collision_apply(int64_t n, const double* K, const double* V,
     free(u); free(v); free(rk); free(ck);
 }
 
+/* The same operator from K's band in energy order: row p (node perm[p]) holds
+ * the columns [lo[p], lo[p] + rowptr[p+1] - rowptr[p]) contiguously at
+ * vals + rowptr[p].  One pass over the band gives u = K f and v = K^T f
+ * (rows dealt to threads in blocks, v summed from per-thread partials in
+ * thread order); rk = K 1 and ck = K^T 1 (energy order) come precomputed.
+ * scratch: nthreads * n doubles. */
+void or_collision_apply_band(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
+                             const double* rk, const double* ck, const double* Vs, const int32_t* perm,
+                             double bath_n, const double* f, double* I, double* scratch, int nthreads) {
+    const int nt = nthreads > 0 ? nthreads : 1;
+    double* fs = scratch + (size_t)nt * n;       /* f in energy order */
+    double* u = fs + n;
+    for (int64_t p = 0; p < n; ++p) fs[p] = f[perm[p]];
+#pragma omp parallel num_threads(nt)
+    {
+        const int th = omp_get_thread_num();
+        double* vp = scratch + (size_t)th * n;
+        for (int64_t j = 0; j < n; ++j) vp[j] = 0.0;
+        const int64_t p0 = n * th / nt, p1 = n * (th + 1) / nt;
+        for (int64_t p = p0; p < p1; ++p) {
+            const double* r = vals + rowptr[p];
+            const int64_t len = rowptr[p + 1] - rowptr[p];
+            const int32_t c0 = lo[p];
+            const double fp = fs[p];
+            double su = 0.0;
+            for (int64_t q = 0; q < len; ++q) {
+                su += r[q] * fs[c0 + q];
+                vp[c0 + q] += r[q] * fp;
+            }
+            u[p] = su;
+        }
+    }
+    const double np1 = bath_n + 1.0;
+    for (int64_t p = 0; p < n; ++p) {
+        double v = 0.0;
+        for (int t = 0; t < nt; ++t) v += scratch[(size_t)t * n + p];
+        const double wf = np1 * u[p] + bath_n * v;
+        const double wt1 = np1 * ck[p] + bath_n * rk[p];
+        const double wtf = np1 * v + bath_n * u[p];
+        I[perm[p]] = ((1.0 - fs[p]) * wf - fs[p] * (wt1 - wtf)) / Vs[p];
+    }
+}
+
 void or_conservation(int64_t n, const double* V, const double* eps,
                      const double* f, double* number, double* energy) {
     double s = 0.0, e = 0.0;

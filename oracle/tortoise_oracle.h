@@ -94,6 +94,14 @@ void or_kernel_row(int64_t n, int64_t i, const int32_t* off, const int32_t* len,
 void or_collision_apply(int64_t n, const double* K, const double* V,
                         double bath_n, const double* f, double* I, int conserve);
 
+/* or_collision_apply from K's band in energy order (the CPU form that reads
+ * the same bytes as the GPU's plain evaluation): row p (node perm[p]) holds
+ * columns [lo[p], lo[p] + rowptr[p+1] - rowptr[p]) at vals + rowptr[p]; rk,
+ * ck, Vs in energy order; scratch (nthreads + 2) * n doubles. */
+void or_collision_apply_band(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
+                             const double* rk, const double* ck, const double* Vs, const int32_t* perm,
+                             double bath_n, const double* f, double* I, double* scratch, int nthreads);
+
 /* sum_i V_i f_i and sum_i V_i eps_i f_i */
 void or_conservation(int64_t n, const double* V, const double* eps,
                      const double* f, double* number, double* energy);

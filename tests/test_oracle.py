@@ -369,3 +369,18 @@ def test_dp5_collision_conserves_number(k8):
     rp = oracle.dp5_collision_replay(2, kern.nsec, K, kern.V, 0.0894, f0, 0.0, 20.0, 0.2,
                                      r["h"], r["accepted"])
     assert np.array_equal(rp["dense"], r["dense"]) and np.array_equal(rp["y"], r["y"])
+
+
+def test_band_form_equals_dense_operator(k8):
+    """The CPU W-form over K's band in energy order (rows from the first to
+    the last nonzero) gives the dense operator's I for any bath."""
+    kern, K, eps = k8
+    band = oracle.KernelBand(lambda idx: K[idx], eps, kern.V, block=7)
+    assert band.entries >= np.count_nonzero(K)
+    rng = np.random.default_rng(1)
+    f = rng.uniform(0, 1, eps.size)
+    for n in (0.0, 0.3):
+        a = oracle.collision_apply(K, kern.V, n, f)
+        for nt in (1, 3):
+            b = band.apply(n, f, nthreads=nt)
+            assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()

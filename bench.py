@@ -493,6 +493,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dp5", action="store_true", help="skip the full DP5 runs")
+    ap.add_argument("--no-all-panels", action="store_true",
+                    help="skip the all-panels context (profiling runs of the headline)")
     ap.add_argument("--no-reference-op", action="store_true",
                     help="skip the reference tuple operator measurements")
     ap.add_argument("--only-reference-op", action="store_true",
@@ -647,24 +649,26 @@ def main():
                          "ensemble's state at t_final")
     # the same batched evaluation with every panel computed (a context built
     # with TG_MIXED_SKIP=0): the state-independent rate
-    os.environ["TG_MIXED_SKIP"] = "0"
-    try:
-        ctx_all = tg.Context.create(w, max_batch=B)
-        ctx_all.assemble(mixed=True)
-    finally:
-        del os.environ["TG_MIXED_SKIP"]
+    if not args.no_all_panels:
+        os.environ["TG_MIXED_SKIP"] = "0"
+        try:
+            ctx_all = tg.Context.create(w, max_batch=B)
+            ctx_all.assemble(mixed=True)
+        finally:
+            del os.environ["TG_MIXED_SKIP"]
 
-    def all_call(time_kernel=False):
-        ctx_all.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
-                             time_kernel=time_kernel)
+        def all_call(time_kernel=False):
+            ctx_all.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
+                                 time_kernel=time_kernel)
 
-    ms_all, _ = time_loop(torch, ctx_all, all_call, args.steps, args.warmup)
-    ms_gemm_all = kernel_ms(ctx_all, lambda: all_call(True), 10)
-    ctx_all.close()
-    out["value_all_panels"] = B * 1000.0 / ms_all
-    out["all_panels"] = {"ms_per_step": ms_all, "gemm_ms": ms_gemm_all,
-                         "gemm_tflops": 4.0 * st["nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
-    _log(f"all panels: {ms_all:.4f} ms/batch ({out['value_all_panels']:.0f} evals/s), GEMM {ms_gemm_all:.4f} ms")
+        ms_all, _ = time_loop(torch, ctx_all, all_call, args.steps, args.warmup)
+        ms_gemm_all = kernel_ms(ctx_all, lambda: all_call(True), 10)
+        ctx_all.close()
+        out["value_all_panels"] = B * 1000.0 / ms_all
+        out["all_panels"] = {"ms_per_step": ms_all, "gemm_ms": ms_gemm_all,
+                             "gemm_tflops": 4.0 * st["nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
+        _log(f"all panels: {ms_all:.4f} ms/batch ({out['value_all_panels']:.0f} evals/s), "
+             f"GEMM {ms_gemm_all:.4f} ms")
     # W-form hat moments of member 0 (FP64) for the comparison with the
     # reference operator at this mesh
     I0 = ctx.apply(Fh[:, 0], bath_n=bath[0])

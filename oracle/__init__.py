@@ -113,6 +113,9 @@ def lib():
         _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.or_collision_apply_band.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                               _i32p, C.c_double, _f64p, _f64p, _f64p, C.c_int]
+        L.or_dp5_replay_ex.argtypes = [C.c_int64, _f64p, C.c_double, C.c_double, C.c_double, RHS_FN,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _i8p, C.c_int64,
+                                       C.c_void_p, C.c_void_p, C.c_int32]
         L.or_sched_value.restype = C.c_double
         L.or_sched_value.argtypes = [_f64p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.or_step_vetoer.restype = C.c_double
@@ -420,6 +423,26 @@ def step_vetoer(sched, t, h, dense_dt):
         return lib().or_step_vetoer(None, t, h, dense_dt)
     cs = sched._struct()
     return lib().or_step_vetoer(C.cast(C.pointer(cs), C.c_void_p), t, h, dense_dt)
+
+
+def dp5_replay(rhs, y0, t0, t_final, dense_dt, steps, max_dense=100000):
+    """Fixed-step DP5 with a Python rhs(t, y) -> dydt on the accepted steps
+    `steps` (the stage arithmetic and dense output of dp5_integrate)."""
+    y = np.array(y0, np.float64, copy=True)
+    n = y.size
+
+    def _rhs(_u, t, yp, dp):
+        np.ctypeslib.as_array(dp, shape=(n,))[:] = rhs(t, np.ctypeslib.as_array(yp, shape=(n,)).copy())
+
+    cr = RHS_FN(_rhs)
+    h = np.ascontiguousarray(steps, np.float64)
+    acc = np.ones(h.size, np.int8)
+    dense = np.zeros((max_dense, n))
+    dt = np.zeros(max_dense)
+    lib().or_dp5_replay_ex(n, y, t0, t_final, dense_dt, cr, None, None, None, h, acc, h.size,
+                           _ptr(dense), _ptr(dt), max_dense)
+    nd = int(np.count_nonzero(dt))
+    return dict(y=y, dense=dense[:nd], dense_t=dt[:nd])
 
 
 def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol,

@@ -270,6 +270,26 @@ def test_mixed_dp5_error_vs_fp64_on_same_steps(sq32_mixed):
         assert abs(np.dot(V, dm[b, -1]) - n0) <= 1e-12 * n0
 
 
+def test_mixed_dp5_device_states_equal_host_states(sq32_mixed):
+    """The same batched DP5 with the states in device memory (TG_PTR_DEVICE,
+    integrated in place) gives the host-pointer call's states, statistics and
+    steps bit for bit."""
+    import torch
+
+    w, ctx, ok, K = sq32_mixed
+    F, bath = _members(w, 8, seed=4)
+    print("[gpu] mixed dp5 square32 B=8 t=50, device states", flush=True)
+    fm, _, stats, steps = ctx.dp5(F, 0.0, 50.0, 1.0, 1e-6, 1e-12, bath_n=bath, mixed=True, conserve=True,
+                                  h_cap=4096)
+    X = torch.from_numpy(np.ascontiguousarray(F.T)).to("cuda")
+    sd, stepd = ctx.dp5_device(X.data_ptr(), 8, 0.0, 50.0, 1.0, 1e-6, 1e-12, bath_n=bath, mixed=True,
+                               conserve=True, h_cap=4096)
+    torch.cuda.synchronize()
+    assert np.array_equal(X.cpu().numpy().T, fm)
+    assert sd == stats
+    assert all(np.array_equal(a, b) for a, b in zip(stepd, steps))
+
+
 def test_mixed_zero_operand_panels_left_out_exactly(monkeypatch):
     """Panels whose fp16 operands are zero for every member (the empty top of
     cold distributions, where f is below the split's range) are left out of

@@ -380,6 +380,27 @@ class Context:
         out = (f.T, dense, stats)
         return out + ((steps,) if steps is not None else ())
 
+    def dp5_device(self, f_ptr, batch, t0, t1, dense_dt, rtol, atol, bath_n=None, mixed=False,
+                   conserve=False, h_cap=0, schedule=False, raise_on_fail=True):
+        """DP5 of device-resident states: f_ptr (device address, batch x n,
+        member-major) is integrated in place (TG_PTR_DEVICE).  Returns the
+        per-member stats and, with h_cap > 0, the accepted step sizes."""
+        st = (StepStats * batch)()
+        bn = None if bath_n is None else np.ascontiguousarray(np.broadcast_to(bath_n, (batch,)),
+                                                              np.float64)
+        hl = np.zeros((batch, h_cap)) if h_cap else None
+        flags = (TG_PTR_DEVICE | TG_QUIET | (TG_PREC_MIXED if mixed else 0)
+                 | (TG_CONSERVE if conserve else 0) | (TG_BATH_SCHEDULE if schedule else 0))
+        rc = load().tg_dp5_integrate(self._h, C.c_void_p(f_ptr), self.n, batch, t0, t1, dense_dt, rtol,
+                                     atol, None if bn is None else _p(bn), flags, None, 0, st,
+                                     None if hl is None else _p(hl), h_cap)
+        if rc != TG_OK and raise_on_fail:
+            self._check(rc)
+        stats = [s.as_dict() for s in st]
+        if hl is None:
+            return stats
+        return stats, [hl[b, : min(stats[b]["accepted"], h_cap)].copy() for b in range(batch)]
+
     def dp5_replay(self, f0, t0, t1, dense_dt, steps, bath_n=None, mixed=False, conserve=False,
                    max_dense=0, schedule=False):
         """Fixed-step DP5 on per-member accepted step lists; returns (f_last, dense).

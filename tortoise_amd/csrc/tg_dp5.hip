@@ -25,6 +25,11 @@
 //   dense_kernel       dense outputs of the accepted steps
 //   accept_kernel      y <- y5, k1 <- k7 (FSAL)
 //   a copy of the "any member running" flag to pinned host memory.
+// On the batched mixed-precision path (member rows in LDS, no e-e channel)
+// the stage, pack, epilogue and error work between two GEMMs is one fused
+// member pass (Dp5PassArgs): stage 2 + pack, GEMM, then five times finish
+// k_s + next stage + pack, GEMM, and finish k7 + err + the error-norm sums;
+// the attempt is then 13 kernels and one schedule per GEMM instead of 37.
 // The host replays the graph one attempt ahead of the flag it reads, so it
 // never waits for a decision and at most one masked attempt runs at the end.
 #include <cmath>
@@ -132,6 +137,9 @@ static tg_status eval_batch(tg_ctx* c, const double* d_in, double* d_out, int32_
     return ee_apply_device(c, d_in, d_out, B, gate);
 }
 
+static tg_status finish_attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, double* d_dense,
+                                double* d_hlog);
+
 // the device work of one attempted step (captured into a graph)
 static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, double* d_bath,
                          int32_t flags, double rtol, double atol, double* d_dense, double* d_hlog) {
@@ -159,6 +167,55 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, doubl
     const dim3 cgrd(ceil_div(B, 256));
     hipLaunchKernelGGL(prep_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->h, w->running);
     TG_CHECK_LAUNCH(c);
+    if ((flags & TG_PREC_MIXED) && mixed_dp5_fusable(c)) {
+        // the same steps as below with the vector work between GEMMs fused
+        // into one member pass each (Dp5PassArgs); no err vector
+        Dp5PassArgs p{};
+        p.y = w->y;
+        p.h = w->h;
+        p.md = w->md;
+        p.bs = ctl.bath;
+        p.bath = d_bath;
+        p.conserve = (flags & TG_CONSERVE) ? 1 : 0;
+        p.conn = c->d_conn;
+        p.nel = c->nel;
+        p.dim = c->dim;
+        // finish k_s (index s-1 of k) evaluated at `fin`, then the next stage
+        // state y + sum (a_j h) k_j over the k's in `terms`, k_s last
+        auto pass = [&](int mode, int s_fin, const double* fin, std::initializer_list<int> terms,
+                        std::initializer_list<double> ca, double* ys, double cs) -> tg_status {
+            p.fin = fin;
+            p.kout = s_fin > 0 ? k[s_fin - 1] : nullptr;
+            p.flux = s_fin > 0 ? w->flux + (int64_t)(s_fin - 1) * w->cap : nullptr;
+            int j = 0;
+            for (int t : terms) p.kg[j++] = k[t];
+            p.ng = j;
+            j = 0;
+            for (double v : ca) p.ca[j++] = v;
+            p.ys = ys;
+            p.cs = cs;
+            TG_TRY(mixed_dp5_pass(c, mode, p, B));
+            return mode == kDp5Last ? TG_OK : mixed_dp5_gemm(c);
+        };
+        // stages k2..k6 (DormandPrince54.hpp:565-574)
+        TG_TRY(pass(kDp5Stage, 0, nullptr, {0}, {A21}, w->ys, C2));
+        TG_TRY(pass(kDp5Mid, 2, w->ys, {0}, {A31, A32}, w->ys, C3));
+        TG_TRY(pass(kDp5Mid, 3, w->ys, {0, 1}, {A41, A42, A43}, w->ys, C4));
+        TG_TRY(pass(kDp5Mid, 4, w->ys, {0, 1, 2}, {A51, A52, A53, A54}, w->ys, C5));
+        TG_TRY(pass(kDp5Mid, 5, w->ys, {0, 1, 2, 3}, {A61, A62, A63, A64, A65}, w->ys, 1.0));
+        // y5 (b2 = 0) and the y4 prefix, then k7 (:576-578), both at t + h
+        const double cb[5] = {BH1, BH3, BH4, BH5, BH6};
+        for (int q = 0; q < 5; ++q) p.cb[q] = cb[q];
+        p.y4 = w->err;
+        TG_TRY(pass(kDp5Mid, 6, w->ys, {0, 2, 3, 4}, {B1, B3, B4, B5, B6}, w->y5, 1.0));
+        // k7, err = y5 - y4 (:580) and the error-norm sums
+        p.c7 = BH7;
+        p.rtol = rtol;
+        p.atol = atol;
+        p.norms = ctl.replay ? nullptr : w->norms;
+        TG_TRY(pass(kDp5Last, 7, w->y5, {}, {}, nullptr, 1.0));
+        return finish_attempt(c, w, B, ctl, d_dense, d_hlog);
+    }
     // stages k2..k6 (DormandPrince54.hpp:565-574)
     TG_TRY(stage({A21}, 1, w->ys, C2));
     TG_TRY(eval(w->ys, 1));
@@ -186,6 +243,18 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, doubl
         TG_CHECK_LAUNCH(c);
     }
     if (!ctl.replay) TG_TRY(relerr_sums_device(c, w->err, w->y5, B, rtol, atol, w->npart, w->norms));
+    return finish_attempt(c, w, B, ctl, d_dense, d_hlog);
+}
+
+// controller, dense outputs, accept and the running flag (after the norms)
+static tg_status finish_attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, double* d_dense,
+                                double* d_hlog) {
+    hipStream_t st = c->stream;
+    const int64_t n = c->n;
+    double** k = w->k;
+    const dim3 blk(256);
+    const dim3 grd(ceil_div(n, 256 * 4) > 0 ? ceil_div(n, 256 * 4) : 1, B);
+    const dim3 cgrd(ceil_div(B, 256));
     hipLaunchKernelGGL(controller_kernel, cgrd, dim3(256), 0, st, w->md, ctl, w->norms, w->acc,
                        d_hlog, w->running);
     TG_CHECK_LAUNCH(c);

@@ -16,7 +16,6 @@
 #include "../../include/tortoise_gpu.h"
 
 namespace tg {
-namespace {
 
 // Butcher tableau (DormandPrince54.hpp:503-511)
 constexpr double C2 = 1. / 5., C3 = 3. / 10., C4 = 4. / 5., C5 = 8. / 9.;
@@ -120,6 +119,52 @@ struct MemberDev {
     int32_t step_idx, acc, dense_k0, dense_k1;
     tg_step_stats st;
 };
+
+// One fused member pass of the batched DP5 on the mixed-precision path
+// (tg_mixed.hip, mixed_dp5_pass): the vector work between two GEMMs of an
+// attempted step for every member at once.
+//   kDp5Stage  first stage of the attempt: the stage state ys = y + sum_j
+//              (a_j h) k_j (ng terms) is formed in LDS, written out, packed
+//   kDp5Mid    the stage derivative k_s = I(fin) is finished (epilogue,
+//              number-defect removal, energy flux) and written out; the next
+//              stage state y + sum_j (a_j h) k_j + (a_s h) k_s is formed with
+//              k_s still in registers, written out and packed; before y5 the
+//              pass also writes the y4 prefix y + sum_j (bh_j h) k_j + (bh6 h) k6
+//   kDp5Last   k7 = I(y5) is finished and written out; err = y5 - (y4 prefix
+//              + (bh7 h) k7) and r = err / (rtol min(|y5|, |1 - y5|) + atol)
+//              go to LDS, and the member's error-norm sum (Band.cpp:87-101)
+//              is taken over the elements
+// Sums run left to right in ascending j with every product rounded
+// separately, as DormandPrince54.hpp:565-580 evaluates them; terms whose
+// coefficient is zero (a72 = b2 = bh2 = 0) are left out.
+enum { kDp5Stage = 0, kDp5Mid = 1, kDp5Last = 2 };
+
+struct Dp5PassArgs {
+    const double* y;          // the attempt's start state (n x B)
+    const double* kg[6];      // stage derivatives of the next stage's sum read from memory, ascending j
+    double ca[7];             // their coefficients, then the finished k's (kDp5Mid)
+    double cb[7];             // y4-prefix coefficients, same order (y4 written)
+    int32_t ng;               // entries of kg
+    double* ys;               // next stage state (kDp5Stage, kDp5Mid)
+    double* y4;               // y4 prefix: written by the kDp5Mid pass before y5 (else null), read by kDp5Last
+    double c7;                // kDp5Last: bh7
+    const double* fin;        // kDp5Mid, kDp5Last: the state the finished k is evaluated at
+    double* kout;             // the finished k
+    double* flux;             // its e-ph electron energy flux per member (null: none)
+    const double* h;          // per-member step of the attempt; 0: finished member, skipped
+    const MemberDev* md;      // member times (bath schedule)
+    BathSched bs;             // bath schedule (tab null: constant bath)
+    double cs;                // next stage's time fraction: its bath is n_b(t + cs h)
+    double* bath;             // per-member bath: read for the finished k, rewritten for the next stage
+    double rtol, atol;        // kDp5Last: error-norm tolerances
+    double* norms;            // kDp5Last: error-norm sum per member (null: no norm, replay)
+    const int32_t* conn;      // mesh elements, nel x (dim + 1)
+    int64_t nel;
+    int32_t dim;
+    int32_t conserve;         // number-defect removal
+};
+
+namespace {
 
 // out[:, b] = y[:, b] + sum_j (a_j h_b) k_j[:, b]; the product a_j*h_b is
 // formed first, as the reference does ((a_ij * h) * k_j), and every product

@@ -215,6 +215,14 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t batch);
 // electron energy flux sum_i V_i eps_i I_i
 tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch, int conserve,
                             const double* gate, double* flux = nullptr);
+// Fused passes of the batched DP5 on the mixed path (available when member
+// rows fit the LDS and there is no e-e channel): mixed_dp5_pass runs one
+// Dp5PassArgs pass (tg_dp5_kernels.hpp) for every member, mixed_dp5_gemm the
+// GEMM of the operands the last pass packed; kernel launches only
+struct Dp5PassArgs;
+bool mixed_dp5_fusable(tg_ctx* c);
+tg_status mixed_dp5_pass(tg_ctx* c, int mode, const Dp5PassArgs& a, int32_t batch);
+tg_status mixed_dp5_gemm(tg_ctx* c);
 // the device array of per-member bath occupations the batched epilogue reads
 // (a DP5 with a bath schedule writes it per stage; the host copy is then stale)
 double* mixed_bath_device(tg_ctx* c);

@@ -38,6 +38,7 @@
 #include <numeric>
 #include <vector>
 
+#include "tg_dp5_kernels.hpp"
 #include "tg_internal.hpp"
 
 namespace tg {
@@ -275,25 +276,119 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// f (caller order) -> packed fp16 hi/lo of both GEMM operands z = f and
-// z = g = 1 - f in energy order (the GEMM streams one of them per (panel,
-// member)), and the operand choice of every panel.  Members >= B and nodes
-// >= n pack as zero; with a gate (the batched DP5's per-member step sizes)
-// members whose step is 0 are skipped.
-//
 // Operand of (panel, member): g = 1 - f when the panel's middle node is in
 // the filled part of the member's distribution (f > 1/2), else f.  The
 // products the epilogue needs come back as K z and K 1 - K z, so the
 // Pauli-blocked out-scattering K^T(1 - f) of filled nodes is a sum of small
 // terms instead of the difference of two large ones.
+struct PackOut {
+    u32x4* Fhi;
+    u32x4* Flo;
+    u32x4* Ghi;
+    u32x4* Glo;
+    uint8_t* useg;          // npanel x nt operand choice
+    uint8_t* nzF;           // null, or per iteration: holds a nonzero f (g) operand
+    uint8_t* nzG;
+    uint8_t* useF;          // per panel: some live member runs it with f (g)
+    uint8_t* useG;
+};
+
+// the caller-order node at the middle of panel pn (K rows and K^T rows index
+// the same sorted nodes)
+__device__ inline int32_t panel_node(int pn, int64_t kt, int64_t n, const int32_t* __restrict__ perm) {
+    const int64_t r = (int64_t)pn * kMT + kMT / 2;
+    const int64_t p = r < kt ? r : r - kt;
+    return perm[p > n - 1 ? n - 1 : p];
+}
+
+// k-group kg of member b: the values xv of its 8 energy-order positions
+// (padding and idle members pack as zero) -> fp16 hi/lo of f and g = 1 - f
+__device__ __forceinline__ void pack_group(int64_t kg, const double (&xv)[8], bool live, int64_t n, int b, int nt,
+                                           const PackOut& o) {
+    const double scale = (double)(1 << kSF);
+    half8 fh, fl, gh, gl;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const bool valid = live && kg * 8 + q < n;
+        const double x = valid ? xv[q] : 0.0;
+        const double y = valid ? 1.0 - xv[q] : 0.0;
+        _Float16 hi, lo;
+        split_fp16(x, scale, hi, lo);
+        fh[q] = hi;
+        fl[q] = lo;
+        split_fp16(y, scale, hi, lo);
+        gh[q] = hi;
+        gl[q] = lo;
+    }
+    // four neighbouring threads (the k-groups of one iteration) write 64
+    // contiguous bytes of each operand
+    const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
+    const u32x4 fh4 = __builtin_bit_cast(u32x4, fh), fl4 = __builtin_bit_cast(u32x4, fl);
+    const u32x4 gh4 = __builtin_bit_cast(u32x4, gh), gl4 = __builtin_bit_cast(u32x4, gl);
+    o.Fhi[u] = fh4;
+    o.Flo[u] = fl4;
+    o.Ghi[u] = gh4;
+    o.Glo[u] = gl4;
+    // iterations holding a nonzero operand (every writer stores the same 1)
+    if (o.nzF) {
+        const u32x4 fo = fh4 | fl4, go = gh4 | gl4;
+        if ((fo.x | fo.y | fo.z | fo.w) != 0u) o.nzF[kg / kKG] = 1;
+        if ((go.x | go.y | go.z | go.w) != 0u) o.nzG[kg / kKG] = 1;
+    }
+}
+
+// Node indices of this thread's k-groups and of its panel for the packing
+// from an LDS row (a row that fits the LDS has kgroups <= kPackKG * kMemT and
+// npanel <= kMemT, checked when the rows are sized); loaded ahead of the row
+// so their latency overlaps whatever the row needs.
+struct PackNodes {
+    int32_t node[kPackKG][8];
+    int32_t pnode;
+};
+
+__device__ __forceinline__ void load_pack_nodes(PackNodes& pk, int64_t n, int64_t kgroups, int64_t kt, int npanel,
+                                                const int32_t* __restrict__ perm) {
+#pragma unroll
+    for (int j = 0; j < kPackKG; ++j) {
+        const int64_t kg = min((int64_t)threadIdx.x + (int64_t)j * kMemT, kgroups - 1);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pk.node[j][q] = perm[min(kg * 8 + q, n - 1)];
+    }
+    const int pn = threadIdx.x;
+    pk.pnode = panel_node(pn < npanel ? pn : 0, kt, n, perm);
+}
+
+// member b's operands and operand choices from its f in LDS `row` (caller
+// order, complete: the caller has passed a barrier since writing it)
+__device__ __forceinline__ void pack_from_row(const double* row, const PackNodes& pk, bool live, int b, int64_t n,
+                                              int64_t kgroups, int npanel, int nt, const PackOut& o) {
+    const int pn = threadIdx.x;
+    if (pn < npanel) {
+        const uint8_t v = live && row[pk.pnode] > 0.5 ? 1 : 0;
+        o.useg[(int64_t)pn * nt + b] = v;
+        if (o.useF && live) (v ? o.useG : o.useF)[pn] = 1;
+    }
+#pragma unroll
+    for (int j = 0; j < kPackKG; ++j) {
+        const int64_t kg = (int64_t)threadIdx.x + (int64_t)j * kMemT;
+        if (kg < kgroups) {
+            double xv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) xv[q] = row[pk.node[j][q]];
+            pack_group(kg, xv, live, n, b, nt, o);
+        }
+    }
+}
+
+// f (caller order) -> packed fp16 hi/lo of both GEMM operands z = f and
+// z = g = 1 - f in energy order (the GEMM streams one of them per (panel,
+// member)), and the operand choice of every panel.  Members >= B and nodes
+// >= n pack as zero; with a gate (the batched DP5's per-member step sizes)
+// members whose step is 0 are skipped.
 template <bool ROW>
 __global__ void __launch_bounds__(kMemT)
 pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t* __restrict__ perm,
-                   int64_t kgroups, int64_t kt, int npanel, int nt, u32x4* __restrict__ Fhi,
-                   u32x4* __restrict__ Flo, u32x4* __restrict__ Ghi, u32x4* __restrict__ Glo,
-                   uint8_t* __restrict__ useg, const double* __restrict__ gate,
-                   uint8_t* __restrict__ nzF, uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF,
-                   uint8_t* __restrict__ useG) {
+                   int64_t kgroups, int64_t kt, int npanel, int nt, PackOut o, const double* __restrict__ gate) {
     extern __shared__ double row[];
     const int b = member_of_block(blockIdx.x, nt);
     if (b >= nt) return;
@@ -302,89 +397,25 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
     if (gate && b < B && gate[b] == 0.0) return;
     const bool live = b < B;
     const double* fb = f + (int64_t)(live ? b : 0) * n;
-    const double scale = (double)(1 << kSF);
-    // operand choice of panel pn from the node at its middle (K rows and K^T
-    // rows index the same sorted nodes)
-    auto panel_node = [&](int pn) {
-        const int64_t r = (int64_t)pn * kMT + kMT / 2;
-        int64_t p = r < kt ? r : r - kt;
-        return perm[p > n - 1 ? n - 1 : p];
-    };
-    // k-group kg: 8 energy-order positions of both operands, from the nodes
-    // at those positions (padding and idle members pack as zero)
-    auto pack_group = [&](int64_t kg, const int32_t* node) {
-        double xv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) xv[q] = ROW ? row[node[q]] : fb[node[q]];
-        half8 fh, fl, gh, gl;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const bool valid = live && kg * 8 + q < n;
-            const double x = valid ? xv[q] : 0.0;
-            const double y = valid ? 1.0 - xv[q] : 0.0;
-            _Float16 hi, lo;
-            split_fp16(x, scale, hi, lo);
-            fh[q] = hi;
-            fl[q] = lo;
-            split_fp16(y, scale, hi, lo);
-            gh[q] = hi;
-            gl[q] = lo;
-        }
-        // four neighbouring threads (the k-groups of one iteration) write 64
-        // contiguous bytes of each operand
-        const int64_t u = ((kg / kKG) * nt + b) * kKG + kg % kKG;
-        const u32x4 fh4 = __builtin_bit_cast(u32x4, fh), fl4 = __builtin_bit_cast(u32x4, fl);
-        const u32x4 gh4 = __builtin_bit_cast(u32x4, gh), gl4 = __builtin_bit_cast(u32x4, gl);
-        Fhi[u] = fh4;
-        Flo[u] = fl4;
-        Ghi[u] = gh4;
-        Glo[u] = gl4;
-        // iterations holding a nonzero operand (every writer stores the same 1)
-        if (nzF) {
-            const u32x4 fo = fh4 | fl4, go = gh4 | gl4;
-            if ((fo.x | fo.y | fo.z | fo.w) != 0u) nzF[kg / kKG] = 1;
-            if ((go.x | go.y | go.z | go.w) != 0u) nzG[kg / kKG] = 1;
-        }
-    };
     if constexpr (ROW) {
-        // The node indices of this thread's k-groups and of its panel load
-        // before the row is staged, so their latency overlaps the staging
-        // loads (a row that fits the LDS has kgroups <= kPackKG * kMemT and
-        // npanel <= kMemT, checked when the rows are sized).
-        int32_t node[kPackKG][8];
-#pragma unroll
-        for (int j = 0; j < kPackKG; ++j) {
-            const int64_t kg = min((int64_t)threadIdx.x + (int64_t)j * kMemT, kgroups - 1);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) node[j][q] = perm[min(kg * 8 + q, n - 1)];
-        }
-        const int pn = threadIdx.x;
-        const int32_t pnode = panel_node(pn < npanel ? pn : 0);
+        PackNodes pk;
+        load_pack_nodes(pk, n, kgroups, kt, npanel, perm);
         if (live) {
             stage_member_row(fb, n, row);
             lds_barrier();
         }
-        if (pn < npanel) {
-            const uint8_t v = live && row[pnode] > 0.5 ? 1 : 0;
-            useg[(int64_t)pn * nt + b] = v;
-            if (useF && live) (v ? useG : useF)[pn] = 1;
-        }
-#pragma unroll
-        for (int j = 0; j < kPackKG; ++j) {
-            const int64_t kg = (int64_t)threadIdx.x + (int64_t)j * kMemT;
-            if (kg < kgroups) pack_group(kg, node[j]);
-        }
+        pack_from_row(row, pk, live, b, n, kgroups, npanel, nt, o);
     } else {
         for (int pn = threadIdx.x; pn < npanel; pn += kMemT) {
-            const uint8_t v = live && fb[panel_node(pn)] > 0.5 ? 1 : 0;
-            useg[(int64_t)pn * nt + b] = v;
-            if (useF && live) (v ? useG : useF)[pn] = 1;
+            const uint8_t v = live && fb[panel_node(pn, kt, n, perm)] > 0.5 ? 1 : 0;
+            o.useg[(int64_t)pn * nt + b] = v;
+            if (o.useF && live) (v ? o.useG : o.useF)[pn] = 1;
         }
         for (int64_t kg = threadIdx.x; kg < kgroups; kg += kMemT) {
-            int32_t node[8];
+            double xv[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) node[q] = perm[min(kg * 8 + q, n - 1)];
-            pack_group(kg, node);
+            for (int q = 0; q < 8; ++q) xv[q] = fb[perm[min(kg * 8 + q, n - 1)]];
+            pack_group(kg, xv, live, n, b, nt, o);
         }
     }
 }
@@ -823,26 +854,21 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
 // row then goes out by coalesced stores together with the correction.
 // Without it the gather reads and the scatter writes global memory and the
 // correction is a second pass.  Gated members (step 0) are skipped.
+// The energy-order part of the epilogue for member b: the products of the
+// computed panels (P), the member's f (staged in LDS `row` with ROW, else
+// read from fb) and the per-position constants give I at every position;
+// I replaces f in `row` (ROW) or goes to Ib.  Returns the FP64 partial sums
+// of V I and V |f(1-f)| (and, with want_e, of eps V I and eps V |f(1-f)|).
 template <bool ROW, typename PT>
-__global__ void __launch_bounds__(kMemT)
-finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
-                     int64_t kt, int B, const double* __restrict__ f,
-                     const int32_t* __restrict__ perm, const double* __restrict__ rk,
-                     const double* __restrict__ ck, const double* __restrict__ V,
-                     const double* __restrict__ bath, const uint8_t* __restrict__ useg,
-                     double inv_scale, int conserve, const double* __restrict__ gate,
-                     double* __restrict__ I, const double* __restrict__ eps_s,
-                     double* __restrict__ flux) {
-    extern __shared__ double row[];
-    __shared__ double red[4][kMemT / 64];
-    __shared__ double s_b;
-    const int b = member_of_block(blockIdx.x, nt);
-    if (b >= B || (gate && gate[b] == 0.0)) return;
-    const double nb = bath[b];
+__device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad,
+                                              int64_t n, int64_t kt, int b, const double* __restrict__ fb,
+                                              const int32_t* __restrict__ perm, const double* __restrict__ rk,
+                                              const double* __restrict__ ck, const double* __restrict__ V,
+                                              double nb, const uint8_t* __restrict__ useg, double inv_scale,
+                                              const double* __restrict__ eps_s, bool want_e, double* row,
+                                              double* __restrict__ Ib, double& dI, double& dw, double& eI,
+                                              double& ew) {
     const double np1 = nb + 1.0;
-    const double* fb = f + (int64_t)b * n;
-    double* Ib = I + (int64_t)b * n;
-    double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
     // UNR positions per thread per round with all their loads issued before
     // use; the trip count is the same for every thread (the barrier after the
     // first round's loads needs all of them)
@@ -887,7 +913,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             ckp[j] = ck[p];
             Vp[j] = V[p];
             node[j] = perm[p];
-            if (flux) Ep[j] = eps_s[p];
+            if (want_e) Ep[j] = eps_s[p];
         }
     };
     auto finish_round = [&](int64_t p0) {
@@ -915,7 +941,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
             const double vi = Vp[j] * Ii, vw = Vp[j] * fabs(x * (1.0 - x));
             dI += vi;
             dw += vw;
-            if (flux) {
+            if (want_e) {
                 eI += Ep[j] * vi;
                 ew += Ep[j] * vw;
             }
@@ -939,31 +965,62 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
         load_constants(base + threadIdx.x);
         finish_round(base + threadIdx.x);
     }
+}
+
+// Workgroup sums of the four epilogue partials (every thread calls it; the
+// result is valid in thread 0).  red: 4 x (kMemT / 64) LDS doubles.
+__device__ inline void block_sum4(double& a, double& b, double& c, double& d, double (*red)[kMemT / 64]) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+        c += __shfl_down(c, o, 64);
+        d += __shfl_down(d, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+        red[2][threadIdx.x >> 6] = c;
+        red[3][threadIdx.x >> 6] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = b = c = d = 0.0;
+        for (int w = 0; w < kMemT / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+            c += red[2][w];
+            d += red[3][w];
+        }
+    }
+}
+
+template <bool ROW, typename PT>
+__global__ void __launch_bounds__(kMemT)
+finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
+                     int64_t kt, int B, const double* __restrict__ f,
+                     const int32_t* __restrict__ perm, const double* __restrict__ rk,
+                     const double* __restrict__ ck, const double* __restrict__ V,
+                     const double* __restrict__ bath, const uint8_t* __restrict__ useg,
+                     double inv_scale, int conserve, const double* __restrict__ gate,
+                     double* __restrict__ I, const double* __restrict__ eps_s,
+                     double* __restrict__ flux) {
+    extern __shared__ double row[];
+    __shared__ double red[4][kMemT / 64];
+    __shared__ double s_b;
+    const int b = member_of_block(blockIdx.x, nt);
+    if (b >= B || (gate && gate[b] == 0.0)) return;
+    const double nb = bath[b];
+    const double* fb = f + (int64_t)b * n;
+    double* Ib = I + (int64_t)b * n;
+    double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
+    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, inv_scale, eps_s,
+                           flux != nullptr, row, Ib, dI, dw, eI, ew);
     double sc = 0.0;
     if (conserve || flux) {
-        for (int o = 32; o > 0; o >>= 1) {
-            dI += __shfl_down(dI, o, 64);
-            dw += __shfl_down(dw, o, 64);
-            eI += __shfl_down(eI, o, 64);
-            ew += __shfl_down(ew, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            red[0][threadIdx.x >> 6] = dI;
-            red[1][threadIdx.x >> 6] = dw;
-            red[2][threadIdx.x >> 6] = eI;
-            red[3][threadIdx.x >> 6] = ew;
-        }
-        __syncthreads();
+        block_sum4(dI, dw, eI, ew, red);
         if (threadIdx.x == 0) {
-            double a = 0.0, c = 0.0, ea = 0.0, ec = 0.0;
-            for (int w = 0; w < kMemT / 64; ++w) {
-                a += red[0][w];
-                c += red[1][w];
-                ea += red[2][w];
-                ec += red[3][w];
-            }
-            s_b = (conserve && c > 0.0) ? a / c : 0.0;
-            if (flux) flux[b] = ea - s_b * ec;
+            s_b = (conserve && dw > 0.0) ? dI / dw : 0.0;
+            if (flux) flux[b] = eI - s_b * ew;
         }
     }
     __syncthreads();
@@ -994,6 +1051,156 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
         }
         if (threadIdx.x == 0 && h) Ib[0] = out(0, ROW ? row[0] : Ib[0]);
         if (threadIdx.x == kMemT - 1 && h + 2 * np < n) Ib[n - 1] = out(n - 1, ROW ? row[n - 1] : Ib[n - 1]);
+    }
+}
+
+// ------------------------------------------------- fused passes of the DP5
+// What a fused pass needs of the packed operator (by value).
+template <typename PT>
+struct MixedView {
+    const PT* P;
+    int nsplit, nt, B, npanel;
+    int64_t rows_pad, n, kt, kgroups;
+    const int32_t* perm;
+    const double* rk;
+    const double* ck;
+    const double* V;
+    const double* eps_s;
+    const uint8_t* useg;
+    double inv_scale;
+    PackOut o;
+};
+
+// Positions per thread per sweep of a pass's caller-order loop; all loads of
+// a sweep are issued before the first result is stored (the stores could
+// alias them as far as the compiler knows).
+constexpr int kPassUnr = 3;
+
+// One member per workgroup (member_of_block), member rows in LDS; the passes
+// are described with Dp5PassArgs.  The GEMM that follows a kDp5Stage or
+// kDp5Mid pass reads the operands it packs; a kDp5Mid or kDp5Last pass reads
+// the partial sums of the GEMM before it.  A kDp5Mid pass may write ys over
+// fin: each position is read (for the number-defect removal) before the same
+// thread writes it, and the LDS copy of fin is complete before any write.
+template <int MODE, typename PT>
+__global__ void __launch_bounds__(kMemT)
+dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
+#pragma clang fp contract(off)
+    extern __shared__ double row[];
+    __shared__ double red[4][kMemT / 64];
+    __shared__ double s_b;
+    const int b = member_of_block(blockIdx.x, v.nt);
+    if (b >= v.nt) return;
+    const bool live = b < v.B;
+    if (MODE == kDp5Last && !live) return;
+    const double hb = live ? a.h[b] : 0.0;
+    if (live && hb == 0.0) return;   // finished member: nothing of this attempt is read
+    const int64_t n = v.n;
+    const int64_t base = (int64_t)b * n;
+    if (MODE != kDp5Stage && live) {
+        // k_s = I(fin): the epilogue over the energy order, I in place of fin in LDS
+        double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
+        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, b, a.fin + base, v.perm, v.rk, v.ck,
+                                v.V, a.bath[b], v.useg, v.inv_scale, v.eps_s, a.flux != nullptr, row, nullptr,
+                                dI, dw, eI, ew);
+        block_sum4(dI, dw, eI, ew, red);
+        if (threadIdx.x == 0) {
+            s_b = (a.conserve && dw > 0.0) ? dI / dw : 0.0;
+            if (a.flux) a.flux[b] = eI - s_b * ew;
+        }
+        __syncthreads();
+    }
+    // the next stage's bath (every thread has read this one's above)
+    if (MODE != kDp5Last && live && a.bs.tab && threadIdx.x == 0) {
+        const double ts = a.md[b].t + a.cs * hb;
+        a.bath[b] = sched_value(a.bs.tab + (int64_t)b * a.bs.ns, a.bs.ns, a.bs.t_start, a.bs.dt, ts);
+    }
+    PackNodes pk;
+    if (MODE != kDp5Last) load_pack_nodes(pk, n, v.kgroups, v.kt, v.npanel, v.perm);
+    if (live) {
+        const double sc = MODE != kDp5Stage ? s_b : 0.0;
+        double c[7], cb[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            c[j] = a.ca[j] * hb;
+            cb[j] = a.cb[j] * hb;
+        }
+        const double c7 = hb * a.c7;
+        const int ng = a.ng;
+        const bool want_y4 = MODE == kDp5Mid && a.y4 != nullptr;
+        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * kPassUnr) {
+            double yv[kPassUnr], fv[kPassUnr], kv[6][kPassUnr], y4v[kPassUnr];
+#pragma unroll
+            for (int u = 0; u < kPassUnr; ++u) {
+                const int64_t i = base + min(i0 + (int64_t)u * kMemT, n - 1);
+                if (MODE != kDp5Last) {
+                    yv[u] = a.y[i];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+                        if (j < ng) kv[j][u] = a.kg[j][i];
+                }
+                if (MODE != kDp5Stage && (a.conserve || MODE == kDp5Last)) fv[u] = a.fin[i];
+                if (MODE == kDp5Last) y4v[u] = a.y4[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kPassUnr; ++u) {
+                const int64_t p = i0 + (int64_t)u * kMemT;
+                if (p >= n) break;
+                const int64_t i = base + p;
+                double kn = 0.0;
+                if (MODE != kDp5Stage) {
+                    kn = row[p];
+                    if (a.conserve) {
+                        const double x = fv[u];
+                        kn = kn - sc * fabs(x * (1.0 - x));
+                    }
+                    a.kout[i] = kn;
+                }
+                if (MODE == kDp5Last) {
+                    const double y5 = fv[u];
+                    const double err = y5 - (y4v[u] + c7 * kn);
+                    const double ay = fabs(y5), a1 = fabs(1.0 - y5);
+                    const double scale = a.rtol * (ay < a1 ? ay : a1) + a.atol;
+                    row[p] = err / scale;
+                } else {
+                    double s = yv[u];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+                        if (j < ng) s = s + c[j] * kv[j][u];
+                    if (MODE == kDp5Mid) s = s + c[ng] * kn;
+                    a.ys[i] = s;
+                    row[p] = s;
+                    if (want_y4) {
+                        double w = yv[u];
+#pragma unroll
+                        for (int j = 0; j < 6; ++j)
+                            if (j < ng) w = w + cb[j] * kv[j][u];
+                        a.y4[i] = w + cb[ng] * kn;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (MODE != kDp5Last) {
+        pack_from_row(row, pk, live, b, n, v.kgroups, v.npanel, v.nt, v.o);
+    } else {
+        if (!a.norms) return;
+        // sum over elements of sum_a r_a^2 + (sum_a r_a)^2 (the P1 integral of
+        // r^2 up to the factor the controller divides by)
+        const int nv = a.dim + 1;
+        double s = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0;
+        for (int64_t e = threadIdx.x; e < a.nel; e += kMemT) {
+            double sr = 0.0, sr2 = 0.0;
+            for (int q = 0; q < nv; ++q) {
+                const double r = row[a.conn[e * nv + q]];
+                sr += r;
+                sr2 += r * r;
+            }
+            s += sr2 + sr * sr;
+        }
+        block_sum4(s, z1, z2, z3, red);
+        if (threadIdx.x == 0) a.norms[b] = s;
     }
 }
 
@@ -1240,16 +1447,28 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false) {
     return TG_OK;
 }
 
+static PackOut pack_out(MixedData* m, bool dyn) {
+    PackOut o;
+    o.Fhi = (u32x4*)m->F_hi;
+    o.Flo = (u32x4*)m->F_lo;
+    o.Ghi = (u32x4*)m->G_hi;
+    o.Glo = (u32x4*)m->G_lo;
+    o.useg = m->useg;
+    o.nzF = dyn ? m->nzF : nullptr;
+    o.nzG = dyn ? m->nzG : nullptr;
+    o.useF = dyn ? m->useF : nullptr;
+    o.useG = dyn ? m->useG : nullptr;
+    return o;
+}
+
 static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, const double* gate,
                              bool dyn = false) {
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
+    const PackOut o = pack_out(m, dyn);
 #define TG_PACK(ROW)                                                                                \
     hipLaunchKernelGGL(pack_member_kernel<ROW>, dim3(m->nt), dim3(kMemT), ROW ? sizeof(double) * n : 0, \
-                       c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt,             \
-                       (u32x4*)m->F_hi, (u32x4*)m->F_lo, (u32x4*)m->G_hi, (u32x4*)m->G_lo, m->useg, gate, \
-                       dyn ? m->nzF : nullptr, dyn ? m->nzG : nullptr, dyn ? m->useF : nullptr,        \
-                       dyn ? m->useG : nullptr)
+                       c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt, o, gate)
     if (m->rowcache) TG_PACK(true);
     else TG_PACK(false);
 #undef TG_PACK
@@ -1597,6 +1816,14 @@ tg_status mixed_prepare(tg_ctx* c) {
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
             TG_HIP(c, hipFuncSetAttribute((const void*)finish_member_kernel<true, float>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
+            const void* passes[] = {(const void*)dp5_member_kernel<kDp5Stage, double>,
+                                    (const void*)dp5_member_kernel<kDp5Mid, double>,
+                                    (const void*)dp5_member_kernel<kDp5Last, double>,
+                                    (const void*)dp5_member_kernel<kDp5Stage, float>,
+                                    (const void*)dp5_member_kernel<kDp5Mid, float>,
+                                    (const void*)dp5_member_kernel<kDp5Last, float>};
+            for (const void* k : passes)
+                TG_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rowbytes));
         }
     }
     // the four operand arrays in one allocation, each kgroups x nt units
@@ -1710,6 +1937,78 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
     return TG_OK;
 }
 
+// the panels with work and their schedule from the flags the pack set
+static tg_status launch_schedule(tg_ctx* c, MixedData* m) {
+    const int nbins = 8 * m->bins_per_xcd;
+    hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
+                       sizeof(int32_t) * (2 * m->npanel + 1) + 2 * (size_t)m->nit, c->stream, m->npanel, nbins,
+                       m->nit, m->nt, m->krange, m->lpt_order, m->nzF, m->nzG, m->useF, m->useG, m->useg,
+                       m->dsched_off, m->dsched_panel, m->dstat);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+template <int MODE, typename PT>
+static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, bool dyn) {
+    MixedView<PT> v;
+    v.P = (const PT*)m->P;
+    v.nsplit = m->nsplit;
+    v.nt = m->nt;
+    v.B = B;
+    v.npanel = m->npanel;
+    v.rows_pad = m->rows_pad;
+    v.n = c->n;
+    v.kt = m->kt;
+    v.kgroups = m->kpad / 8;
+    v.perm = m->perm;
+    v.rk = m->rk_s;
+    v.ck = m->ck_s;
+    v.V = m->V_s;
+    v.eps_s = m->eps_s;
+    v.useg = m->useg;
+    v.inv_scale = std::ldexp(1.0, -(m->sa + kSF));
+    v.o = pack_out(m, dyn);
+    hipLaunchKernelGGL((dp5_member_kernel<MODE, PT>), dim3(m->nt), dim3(kMemT), sizeof(double) * (size_t)c->n,
+                       c->stream, a, v);
+}
+
+bool mixed_dp5_fusable(tg_ctx* c) {
+    MixedData* m = md_of(c);
+    return m && m->rowcache && c->ee == nullptr;
+}
+
+// one fused member pass of the batched DP5 (Dp5PassArgs); kernel launches only
+tg_status mixed_dp5_pass(tg_ctx* c, int mode, const Dp5PassArgs& a, int32_t B) {
+    MixedData* m = md_of(c);
+    if (!mixed_dp5_fusable(c)) return set_error(c, TG_ERR_STATE, "fused DP5 passes need member rows in LDS and no e-e channel");
+    if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
+    if (a.ng < 0 || a.ng > 6) return set_error(c, TG_ERR_ARG, "fused DP5 pass with %d stage terms", a.ng);
+    TG_TRY(ensure_P(c, m));
+    const bool dyn = m->persist && m->skip;
+#define TG_PASS(MODE)                                                              \
+    if (m->fold) launch_pass_t<MODE, double>(c, m, a, B, dyn);                     \
+    else launch_pass_t<MODE, float>(c, m, a, B, dyn);
+    switch (mode) {
+    case kDp5Stage: TG_PASS(kDp5Stage) break;
+    case kDp5Mid: TG_PASS(kDp5Mid) break;
+    case kDp5Last: TG_PASS(kDp5Last) break;
+    default: return set_error(c, TG_ERR_ARG, "fused DP5 pass mode %d", mode);
+    }
+#undef TG_PASS
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
+}
+
+// the GEMM of the operands the last kDp5Stage / kDp5Mid pass packed (with its
+// panel schedule); kernel launches only
+tg_status mixed_dp5_gemm(tg_ctx* c) {
+    MixedData* m = md_of(c);
+    if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
+    const bool dyn = m->persist && m->skip;
+    if (dyn) TG_TRY(launch_schedule(c, m));
+    return launch_gemm(c, m, dyn);
+}
+
 // one batched evaluation with the bath set by mixed_set_bath: kernel launches
 // only, so a caller may capture it in a graph once it has run once (buffers
 // are allocated on first use)
@@ -1722,14 +2021,7 @@ tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B
     // panels with all-zero operands leave the persistent GEMM
     const bool dyn = m->persist && m->skip;
     TG_TRY(launch_pack(c, m, d_f, B, gate, dyn));
-    if (dyn) {
-        const int nbins = 8 * m->bins_per_xcd;
-        hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
-                           sizeof(int32_t) * (2 * m->npanel + 1) + 2 * (size_t)m->nit,
-                           c->stream, m->npanel, nbins, m->nit, m->nt, m->krange, m->lpt_order, m->nzF,
-                           m->nzG, m->useF, m->useG, m->useg, m->dsched_off, m->dsched_panel, m->dstat);
-        TG_CHECK_LAUNCH(c);
-    }
+    if (dyn) TG_TRY(launch_schedule(c, m));
     time_begin(c);
     TG_TRY(launch_gemm(c, m, dyn));
     time_end(c);

@@ -8,6 +8,7 @@ sum_i V_i I_i is at FP64 roundoff.
 """
 import numpy as np
 import pytest
+import torch  # before the library: the process then has one HIP runtime (torch's)
 
 import oracle
 import tortoise_amd as tg
@@ -274,8 +275,6 @@ def test_mixed_dp5_device_states_equal_host_states(sq32_mixed):
     """The same batched DP5 with the states in device memory (TG_PTR_DEVICE,
     integrated in place) gives the host-pointer call's states, statistics and
     steps bit for bit."""
-    import torch
-
     w, ctx, ok, K = sq32_mixed
     F, bath = _members(w, 8, seed=4)
     print("[gpu] mixed dp5 square32 B=8 t=50, device states", flush=True)

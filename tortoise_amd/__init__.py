@@ -130,7 +130,13 @@ def library_path():
 
 
 def load():
-    """Loads libtortoise_gpu.so; raises if it has not been built."""
+    """Loads libtortoise_gpu.so; raises if it has not been built.
+
+    In a process that also uses torch on the GPU, import torch first: torch's
+    libraries ask for `libamdhip64.so` by a name the ROCm runtime this library
+    links (`libamdhip64.so.7`) does not answer to, so torch loaded second
+    brings a second HIP runtime into the process, which then sees no device.
+    Loaded first, torch's runtime is the one both use."""
     global _lib
     if _lib is not None:
         return _lib

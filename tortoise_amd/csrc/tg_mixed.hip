@@ -103,6 +103,7 @@ struct MixedData {
     uint8_t* nzG = nullptr;          // nit
     uint8_t* useF = nullptr;         // npanel
     uint8_t* useG = nullptr;         // npanel
+    uint8_t* actp = nullptr;         // npanel: 1 if the last schedule computed the panel
     int32_t* lpt_order = nullptr;    // panels by decreasing band length
     int32_t* dsched_off = nullptr;   // nbins + 1
     int32_t* dsched_panel = nullptr; // npanel
@@ -864,7 +865,8 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
                                               int64_t n, int64_t kt, int b, const double* __restrict__ fb,
                                               const int32_t* __restrict__ perm, const double* __restrict__ rk,
                                               const double* __restrict__ ck, const double* __restrict__ V,
-                                              double nb, const uint8_t* __restrict__ useg, double inv_scale,
+                                              double nb, const uint8_t* __restrict__ useg,
+                                              const uint8_t* __restrict__ actp, double inv_scale,
                                               const double* __restrict__ eps_s, bool want_e, double* row,
                                               double* __restrict__ Ib, double& dI, double& dw, double& eI,
                                               double& ew) {
@@ -895,13 +897,19 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
             }
         }
     };
-    // ... after the operand bytes, which say which panels those are ...
+    // ... after the operand bytes and the panel flags (actp, null when every
+    // panel is computed), which say which panels those are ...
     auto load_bytes = [&](int64_t p0) {
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
-            gu[j] = useg[(p / kMT) * (int64_t)nt + b];
-            gv[j] = useg[((kt + p) / kMT) * (int64_t)nt + b];
+            const int64_t pu = p / kMT, pv = (kt + p) / kMT;
+            gu[j] = useg[pu * nt + b];
+            gv[j] = useg[pv * nt + b];
+            if (actp) {
+                gu[j] |= actp[pu] ? 0 : 2;
+                gv[j] |= actp[pv] ? 0 : 2;
+            }
         }
     };
     // ... and the per-position constants every member shares (L2-resident)
@@ -927,8 +935,8 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             if (p0 + (int64_t)j * kMemT >= n) break;
-            // operand byte: bit 0 = g (z = 1 - f) rather than f, bit 1 = the panel
-            // was left out of the GEMM (all operands zero: its product is 0)
+            // bit 0 = g (z = 1 - f) rather than f, bit 1 = the panel was left
+            // out of the GEMM (all operands zero: its product is 0)
             const double uu = u[j] * inv_scale;
             const double vv = v[j] * inv_scale;
             // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
@@ -1001,7 +1009,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                      const int32_t* __restrict__ perm, const double* __restrict__ rk,
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
-                     double inv_scale, int conserve, const double* __restrict__ gate,
+                     const uint8_t* __restrict__ actp, double inv_scale, int conserve,
+                     const double* __restrict__ gate,
                      double* __restrict__ I, const double* __restrict__ eps_s,
                      double* __restrict__ flux) {
     extern __shared__ double row[];
@@ -1013,8 +1022,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
     double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
-    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, inv_scale, eps_s,
-                           flux != nullptr, row, Ib, dI, dw, eI, ew);
+    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, actp, inv_scale,
+                           eps_s, flux != nullptr, row, Ib, dI, dw, eI, ew);
     double sc = 0.0;
     if (conserve || flux) {
         block_sum4(dI, dw, eI, ew, red);
@@ -1067,6 +1076,7 @@ struct MixedView {
     const double* V;
     const double* eps_s;
     const uint8_t* useg;
+    const uint8_t* actp;
     double inv_scale;
     PackOut o;
 };
@@ -1101,8 +1111,8 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         // k_s = I(fin): the epilogue over the energy order, I in place of fin in LDS
         double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
         finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, b, a.fin + base, v.perm, v.rk, v.ck,
-                                v.V, a.bath[b], v.useg, v.inv_scale, v.eps_s, a.flux != nullptr, row, nullptr,
-                                dI, dw, eI, ew);
+                                v.V, a.bath[b], v.useg, v.actp, v.inv_scale, v.eps_s, a.flux != nullptr, row,
+                                nullptr, dI, dw, eI, ew);
         block_sum4(dI, dw, eI, ew, red);
         if (threadIdx.x == 0) {
             s_b = (a.conserve && dw > 0.0) ? dI / dw : 0.0;
@@ -1210,103 +1220,98 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
 // filled bottom of a cold distribution) has C = A Z = 0 exactly: it is left
 // out of the GEMM and its products read as 0.  One workgroup decides which
 // panels have work (some iteration of its k range holds a nonzero operand
-// of a kind some member runs it with), deals them in decreasing band length
-// to the bins in snake order (the k-th panel of pass s to bin k on even
-// passes, nbins - 1 - k on odd ones: every complete pass gives each bin one
-// panel, so a bin's slots and offsets have closed forms), and clears the
-// flags for the next evaluation.  Dynamic LDS: npanel ints.
+// of a kind some member runs it with: a difference of prefix counts of the
+// iteration flags), flags them in actp for the epilogue, deals them in
+// decreasing band length to the bins in snake order (the k-th panel of pass
+// s to bin k on even passes, nbins - 1 - k on odd ones: every complete pass
+// gives each bin one panel, so a bin's slots and offsets have closed forms),
+// and clears the flags for the next evaluation.  Dynamic LDS:
+// 2 (nit + 1) + npanel + 1 ints.
 constexpr int kSchedT = 1024;
+
+// exclusive prefix of one int per thread over the workgroup; *total gets the
+// sum (wtot: kSchedT / 64 ints of LDS)
+__device__ inline int block_exclusive_scan(int v, int* wtot, int* total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    int wbase = 0, all = 0;
+    for (int w = 0; w < kSchedT / 64; ++w) {
+        if (w < wv) wbase += wtot[w];
+        all += wtot[w];
+    }
+    __syncthreads();   // wtot may be reused after this
+    *total = all;
+    return wbase + inc - v;
+}
+
 __global__ void __launch_bounds__(kSchedT)
-active_schedule_kernel(int npanel, int nbins, int nit, int nt, const int2* __restrict__ krange,
+active_schedule_kernel(int npanel, int nbins, int nit, const int2* __restrict__ krange,
                        const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
                        uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF, uint8_t* __restrict__ useG,
-                       uint8_t* __restrict__ useg, int32_t* __restrict__ sched_off,
+                       uint8_t* __restrict__ actp, int32_t* __restrict__ sched_off,
                        int32_t* __restrict__ sched_panel, int64_t* __restrict__ stat) {
-    extern __shared__ int32_t flag[];   // per position of `order`: 1 if that panel has work,
-                                        // then the total; then per panel (actp); then the
-                                        // iteration flags (sF, sG, bytes)
-    int32_t* actp = flag + npanel + 1;
-    uint8_t* sF = reinterpret_cast<uint8_t*>(actp + npanel);
-    uint8_t* sG = sF + nit;
-    __shared__ int32_t part[kSchedT];
+    extern __shared__ int32_t sh[];
+    int32_t* cF = sh;                 // nit + 1: iterations before it holding a nonzero f operand
+    int32_t* cG = cF + nit + 1;       // nit + 1: the same for g
+    int32_t* flag = cG + nit + 1;     // npanel: 1 if the panel at that position of `order` has work
+    __shared__ int wtot[kSchedT / 64];
     __shared__ unsigned long long iters;
     const int tid = threadIdx.x;
     if (tid == 0) iters = 0;
-    for (int i = tid; i < nit; i += kSchedT) {
-        sF[i] = nzF[i];
-        sG[i] = nzG[i];
+    // prefix counts of the iteration flags: thread t counts its contiguous chunk
+    {
+        const int chunk = (nit + kSchedT - 1) / kSchedT;
+        const int i0 = min(tid * chunk, nit), i1 = min(i0 + chunk, nit);
+        int nf = 0, ng = 0;
+        for (int i = i0; i < i1; ++i) {
+            nf += nzF[i] != 0;
+            ng += nzG[i] != 0;
+        }
+        int tf, tgv;
+        int pf = block_exclusive_scan(nf, wtot, &tf);
+        int pg = block_exclusive_scan(ng, wtot, &tgv);
+        for (int i = i0; i < i1; ++i) {
+            cF[i] = pf;
+            cG[i] = pg;
+            pf += nzF[i] != 0;
+            pg += nzG[i] != 0;
+        }
+        if (tid == 0) {
+            cF[nit] = tf;
+            cG[nit] = tgv;
+        }
     }
     __syncthreads();
     for (int sidx = tid; sidx < npanel; sidx += kSchedT) {
         const int pn = order[sidx];
         const int2 kr = krange[pn];
         const bool uf = useF[pn] != 0, ug = useG[pn] != 0;
-        int a = 0;
-        for (int it = kr.x; it < kr.y; ++it) a |= (uf ? sF[it] : 0) | (ug ? sG[it] : 0);
-        a = a != 0;
-        flag[sidx] = a;
-        actp[pn] = a;
+        const bool a = (uf && cF[kr.y] > cF[kr.x]) || (ug && cG[kr.y] > cG[kr.x]);
+        flag[sidx] = a ? 1 : 0;
+        actp[pn] = a ? 1 : 0;
         if (a) atomicAdd(&iters, (unsigned long long)(kr.y - kr.x));
     }
     __syncthreads();
-    // bit 1 of every member's operand byte of a panel: left out of the GEMM
-    // (rewritten for every member, also those the pack skipped); 16 bytes at a
-    // time (nt is a multiple of 16, so a piece lies in one panel), eight
-    // pieces per thread in flight
-    {
-        u32x4* ug4 = reinterpret_cast<u32x4*>(useg);
-        const int64_t npc = (int64_t)npanel * (nt / 16);
-        const int per = nt / 16;
-        for (int64_t i0 = tid; i0 < npc; i0 += (int64_t)kSchedT * 8) {
-            u32x4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = ug4[min(i0 + (int64_t)j * kSchedT, npc - 1)];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int64_t i = i0 + (int64_t)j * kSchedT;
-                if (i < npc) {
-                    const uint32_t set = actp[i / per] ? 0u : 0x02020202u;
-                    u32x4 w;
-                    w.x = (v[j].x & 0x01010101u) | set;
-                    w.y = (v[j].y & 0x01010101u) | set;
-                    w.z = (v[j].z & 0x01010101u) | set;
-                    w.w = (v[j].w & 0x01010101u) | set;
-                    ug4[i] = w;
-                }
-            }
-        }
-    }
-    // exclusive ranks over `order`: thread t counts its contiguous chunk
+    // ranks over `order`: thread t counts its contiguous chunk
     const int chunk = (npanel + kSchedT - 1) / kSchedT;
     const int s0 = min(tid * chunk, npanel), s1 = min(s0 + chunk, npanel);
     int cnt = 0;
     for (int sidx = s0; sidx < s1; ++sidx) cnt += flag[sidx];
-    // block-wide exclusive scan of the counts: within each wave by
-    // shuffles, then across the four waves
-    const int lane = tid & 63, wv = tid >> 6;
-    int inc = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += y;
-    }
-    if (lane == 63) part[wv] = inc;
-    __syncthreads();
-    int wbase = 0;
-    for (int w = 0; w < wv; ++w) wbase += part[w];   // part[0 .. kSchedT/64) hold the wave totals
-    if (tid == kSchedT - 1) flag[npanel] = wbase + inc;   // total, in the spare slot past the flags
-    const int excl = wbase + inc - cnt;
-    __syncthreads();
-    part[tid] = excl;
-    __syncthreads();
-    const int A = flag[npanel];
+    int A;
+    int r = block_exclusive_scan(cnt, wtot, &A);
     const int full = A / nbins, rem = A % nbins;
     auto bin_off = [&](int b) {
         int pb = 0;
         if (rem) pb = (full % 2 == 0) ? min(b, rem) : max(0, b - (nbins - rem));
         return full * b + pb;
     };
-    int r = part[tid];
     for (int sidx = s0; sidx < s1; ++sidx) {
         if (!flag[sidx]) continue;
         const int pass = r / nbins, k = r % nbins;
@@ -1477,14 +1482,16 @@ static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, 
 }
 
 static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, double* d_I, int B,
-                               int conserve, const double* gate = nullptr, double* flux = nullptr) {
+                               int conserve, const double* gate = nullptr, double* flux = nullptr,
+                               bool dyn = false) {
     const int64_t n = c->n;
     const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
 #define TG_FINISH(ROW, PT)                                                                           \
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
                        c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
-                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, inv_scale, conserve,  \
+                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, dyn ? m->actp : nullptr, \
+                       inv_scale, conserve,                                                          \
                        gate, d_I, m->eps_s, flux)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
@@ -1845,6 +1852,8 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_TRY(dev_alloc(c, &m->nzG, (size_t)m->nit));
         TG_TRY(dev_alloc(c, &m->useF, (size_t)m->npanel));
         TG_TRY(dev_alloc(c, &m->useG, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->actp, (size_t)m->npanel));
+        TG_HIP(c, hipMemsetAsync(m->actp, 1, (size_t)m->npanel, st));
         TG_TRY(dev_alloc(c, &m->lpt_order, (size_t)m->npanel));
         TG_TRY(dev_alloc(c, &m->dsched_off, (size_t)std::max(m->ncu, 8) + 1));
         TG_TRY(dev_alloc(c, &m->dsched_panel, (size_t)m->npanel));
@@ -1907,6 +1916,7 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->useF);
     dev_free(&m->useG);
     dev_free(&m->lpt_order);
+    dev_free(&m->actp);
     dev_free(&m->dsched_off);
     dev_free(&m->dsched_panel);
     dev_free(&m->dstat);
@@ -1941,8 +1951,8 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 static tg_status launch_schedule(tg_ctx* c, MixedData* m) {
     const int nbins = 8 * m->bins_per_xcd;
     hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
-                       sizeof(int32_t) * (2 * m->npanel + 1) + 2 * (size_t)m->nit, c->stream, m->npanel, nbins,
-                       m->nit, m->nt, m->krange, m->lpt_order, m->nzF, m->nzG, m->useF, m->useG, m->useg,
+                       sizeof(int32_t) * (2 * ((size_t)m->nit + 1) + m->npanel), c->stream, m->npanel, nbins,
+                       m->nit, m->krange, m->lpt_order, m->nzF, m->nzG, m->useF, m->useG, m->actp,
                        m->dsched_off, m->dsched_panel, m->dstat);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
@@ -1966,6 +1976,7 @@ static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, 
     v.V = m->V_s;
     v.eps_s = m->eps_s;
     v.useg = m->useg;
+    v.actp = dyn ? m->actp : nullptr;
     v.inv_scale = std::ldexp(1.0, -(m->sa + kSF));
     v.o = pack_out(m, dyn);
     hipLaunchKernelGGL((dp5_member_kernel<MODE, PT>), dim3(m->nt), dim3(kMemT), sizeof(double) * (size_t)c->n,
@@ -2025,7 +2036,7 @@ tg_status mixed_eval_device(tg_ctx* c, const double* d_f, double* d_I, int32_t B
     time_begin(c);
     TG_TRY(launch_gemm(c, m, dyn));
     time_end(c);
-    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate, flux));
+    TG_TRY(launch_finish(c, m, d_f, d_I, B, conserve, gate, flux, dyn));
     return TG_OK;
 }
 

@@ -242,7 +242,7 @@ struct AdjRun {
         TG_CHECK_LAUNCH(c);
         // the e-e channel adds its own product (it depends on neither g nor omega0)
         if (c->ee) TG_TRY(ee_vjp_device(c, Yst, mu, w->nu));
-        TG_TRY(plain_products_device(c, c->d_Kd, Yst, w->udf, w->vdf));
+        TG_TRY(stack_products_device(c, Yst, w->udf, w->vdf, true));
         const int nblk = (int)ceil_div(n, kAdjT);
         hipLaunchKernelGGL(param_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, Yst,
                            c->d_V, nb, dnb, w->uf, w->vf, c->d_rk, c->d_ck, w->udf, w->vdf,

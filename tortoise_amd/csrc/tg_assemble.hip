@@ -11,12 +11,20 @@
 // piece between knots and grid points.  The amplitude convention (g^2, full
 // momentum measure, no 2*pi) follows ScatteringChannel.cpp:399-415.
 //
+// K itself is never stored as an n x n array: its entries
+// K_ij = g^2 <R_i, Q_j> (tg_kentry.hpp) are evaluated by the kernels that
+// build the bands the evaluations stream (tg_stack.hip for FP64,
+// tg_mixed.hip for the fp16 split), from R and Q, which stay resident
+// (n x maxw and n x (maxw + nm - 1) doubles).
+//
 // Kernels (all setup, run once per context):
 //   proj_window_kernel  node -> [off, len) of touched hats        (gather)
 //   proj_values_kernel  node -> R row                              (gather)
 //   conv_kernel         node -> Q_j = R_j * g (sampled Gaussian)   (window conv)
-//   kernel_entries      (i, j) -> K_ij = g^2 <R_i, Q_j>            (dense FP64)
-//   K 1 and K^T 1 come from the plain one-pass kernel with f = 1
+//   k_extent_kernel     row p -> exact nonzero extent of K's energy-ordered
+//                       row and column, nonzero count (entries where the
+//                       windows meet)
+//   K 1 and K^T 1 come from the FP64 band pass with f = 1
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -178,61 +186,83 @@ __global__ void conv_kernel(const double* __restrict__ R, const int32_t* __restr
     Q[j * (int64_t)qw + p] = s;
 }
 
-// K_ij = amp * sum_b R_i(b) Q_j(b) over the overlap of the two windows,
-// stored in energy order: row p holds node i = perm[p], column q node
-// j = perm[q].  Block = 256 columns of one row; the row's window is in LDS.
-constexpr int kKThreads = 256;
-__global__ void kernel_entries(const double* __restrict__ R, const double* __restrict__ Q,
-                               const int32_t* __restrict__ off, const int32_t* __restrict__ len,
-                               int maxw, int qw, int m_lo, int nm, int64_t n, int64_t ld,
-                               const int32_t* __restrict__ perm, double amp,
-                               double* __restrict__ K) {
+// Exact nonzero extents of K in energy order, from the entries themselves:
+// block p takes row p (node perm[p], its window in LDS), tests every column's
+// window and evaluates the entries where the windows meet; an entry counts
+// when it is nonzero in FP64 or, with split_scale > 0, when its fp16 hi/lo
+// split at that scale is (the batched path's band).  Row p gets its first
+// and last such column and their count, column q its first and last row
+// (atomic min / max: the result does not depend on their order).
+constexpr int kExtT = 256;
+__global__ void __launch_bounds__(kExtT)
+k_extent_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, double split_scale,
+                int32_t* __restrict__ rf, int32_t* __restrict__ rl, int32_t* __restrict__ cf,
+                int32_t* __restrict__ cl, unsigned long long* __restrict__ cnt) {
     extern __shared__ double sRi[];
-    const int64_t prow = blockIdx.y;
-    const int64_t i = perm[prow];
-    const int offi = off[i], leni = len[i];
-    for (int w = threadIdx.x; w < leni; w += blockDim.x) sRi[w] = R[i * (int64_t)maxw + w];
-    __syncthreads();
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= ld) return;
-    double s = 0.0;
-    if (q < n) {
-        const int64_t j = perm[q];
-        const int qoff = off[j] + m_lo;
-        const int qlen = len[j] + nm - 1;
-        int b0 = offi > qoff ? offi : qoff;
-        int b1 = (offi + leni < qoff + qlen) ? offi + leni : qoff + qlen;
-        const double* Qj = Q + j * (int64_t)qw;
-        for (int b = b0; b < b1; ++b) s += sRi[b - offi] * Qj[b - qoff];
-        s *= amp;
-    }
-    K[prow * ld + q] = s;  // padding columns hold exact zeros
-}
-
-// first / last nonzero column of every (sorted) row of K; -1 when empty
-__global__ void row_nz_extent_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                                     int32_t* __restrict__ first, int32_t* __restrict__ last) {
-    __shared__ int32_t sf[4], sl[4];
+    __shared__ int32_t sf[kExtT / 64], sl[kExtT / 64];
+    __shared__ unsigned long long sc[kExtT / 64];
     const int64_t p = blockIdx.x;
-    int32_t f = 0x7fffffff, l = -1;
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x)
-        if (K[p * ld + q] != 0.0) {
-            f = q < f ? (int32_t)q : f;
-            l = (int32_t)q;
-        }
+    const int64_t i = perm[p];
+    const int offi = k.off[i], leni = k.len[i];
+    for (int w = threadIdx.x; w < leni; w += blockDim.x) sRi[w] = k.R[i * (int64_t)k.maxw + w];
+    __syncthreads();
+    int32_t f = INT32_MAX, l = -1;
+    unsigned long long cc = 0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+        const int64_t j = perm[q];
+        if (!k_windows_meet(k, offi, leni, j)) continue;
+        const double v = k_entry_w(k, sRi, offi, leni, j);
+        const bool nz = split_scale > 0.0 ? split_nonzero(v, split_scale) : v != 0.0;
+        if (!nz) continue;
+        f = min(f, (int32_t)q);
+        l = (int32_t)q;
+        ++cc;
+        atomicMin(&cf[q], (int32_t)p);
+        atomicMax(&cl[q], (int32_t)p);
+    }
     for (int o = 32; o > 0; o >>= 1) {
         f = min(f, __shfl_down(f, o, 64));
         l = max(l, __shfl_down(l, o, 64));
+        cc += __shfl_down(cc, o, 64);
     }
     if ((threadIdx.x & 63) == 0) {
         sf[threadIdx.x >> 6] = f;
         sl[threadIdx.x >> 6] = l;
+        sc[threadIdx.x >> 6] = cc;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        first[p] = min(min(sf[0], sf[1]), min(sf[2], sf[3]));
-        last[p] = max(max(sl[0], sl[1]), max(sl[2], sl[3]));
+        for (int w = 1; w < kExtT / 64; ++w) {
+            f = min(f, sf[w]);
+            l = max(l, sl[w]);
+            cc += sc[w];
+        }
+        rf[p] = l < 0 ? -1 : f;
+        rl[p] = l;
+        if (cnt) cnt[p] = cc;
     }
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ a, int64_t n, int32_t v) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void fill_f64_kernel(double* __restrict__ a, int64_t n, double v) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// natural-order rows of K from the entries: out[r][j] = K_{row0 + r, j}
+__global__ void copy_rows_kernel(KSrc k, int64_t n, int64_t row0, double* __restrict__ out) {
+    extern __shared__ double sRi[];
+    const int64_t r = blockIdx.x;
+    const int64_t i = row0 + r;
+    const int offi = k.off[i], leni = k.len[i];
+    for (int w = threadIdx.x; w < leni; w += blockDim.x) sRi[w] = k.R[i * (int64_t)k.maxw + w];
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x)
+        out[r * n + j] = k_windows_meet(k, offi, leni, j) ? k_entry_w(k, sRi, offi, leni, j) : 0.0;
 }
 
 __global__ void gather_kernel(const double* __restrict__ src, const int32_t* __restrict__ perm,
@@ -241,88 +271,52 @@ __global__ void gather_kernel(const double* __restrict__ src, const int32_t* __r
     if (p < n) dst[p] = src[perm[p]];
 }
 
-// natural-order rows of K from the sorted storage: out[r][j] = K[pinv r'][pinv j]
-__global__ void copy_rows_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                                 const int32_t* __restrict__ pinv, int64_t row0,
-                                 double* __restrict__ out) {
-    const int64_t r = blockIdx.y;
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    out[r * n + j] = K[(int64_t)pinv[row0 + r] * ld + pinv[j]];
+// K's entries with the convolved windows Q
+KSrc ksrc_of(const tg_ctx* c, const double* Q) {
+    return KSrc{c->d_R, Q, c->d_off, c->d_len, c->maxw, c->qw, c->m_lo, c->nm, c->g * c->g};
 }
 
-// nonzero count per row of K (for throughput accounting)
-__global__ void nnz_rows_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                                unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long sh[4];
-    const int64_t i = blockIdx.x;
-    unsigned long long c = 0;
-    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) c += K[i * ld + j] != 0.0;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) out[i] = sh[0] + sh[1] + sh[2] + sh[3];
-}
-
-// Row-block column ranges of the sorted K, the (row block, chunk) work list
-// and, per column chunk, the row blocks whose range covers it.
-tg_status build_band(tg_ctx* c) {
-    int32_t *d_first = nullptr, *d_last = nullptr;
-    TG_TRY(dev_alloc(c, &d_first, (size_t)c->n));
-    TG_TRY(dev_alloc(c, &d_last, (size_t)c->n));
-    hipLaunchKernelGGL(row_nz_extent_kernel, dim3(c->n), dim3(256), 0, c->stream, c->d_K, c->n,
-                       c->ld, d_first, d_last);
+// Exact extents of K in energy order (host copies), optionally of the fp16
+// split at split_scale, and the nonzero count; kernel scan over all pairs
+// whose windows meet.
+tg_status k_extents(tg_ctx* c, double split_scale, std::vector<int32_t>& rf, std::vector<int32_t>& rl,
+                    std::vector<int32_t>& cf, std::vector<int32_t>& cl, int64_t* nnz) {
+    const int64_t n = c->n;
+    int32_t *d_rf = nullptr, *d_rl = nullptr, *d_cf = nullptr, *d_cl = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    TG_TRY(dev_alloc(c, &d_rf, (size_t)n));
+    TG_TRY(dev_alloc(c, &d_rl, (size_t)n));
+    TG_TRY(dev_alloc(c, &d_cf, (size_t)n));
+    TG_TRY(dev_alloc(c, &d_cl, (size_t)n));
+    if (nnz) TG_TRY(dev_alloc(c, &d_cnt, (size_t)n));
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_cf, n, INT32_MAX);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_cl, n, -1);
+    hipLaunchKernelGGL(k_extent_kernel, dim3((unsigned)n), dim3(kExtT), sizeof(double) * c->maxw, st,
+                       ksrc_of(c, c->d_Q), c->d_perm, n, split_scale, d_rf, d_rl, d_cf, d_cl, d_cnt);
     TG_CHECK_LAUNCH(c);
-    std::vector<int32_t> first(c->n), last(c->n);
-    TG_HIP(c, hipMemcpyAsync(first.data(), d_first, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
-    TG_HIP(c, hipMemcpyAsync(last.data(), d_last, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
-    TG_HIP(c, hipStreamSynchronize(c->stream));
-    dev_free(&d_first);
-    dev_free(&d_last);
-    c->h_rb_col.assign(c->n_rowblk, int2{0, 0});
-    std::vector<int2> rb_chunk(c->n_rowblk, int2{0, -1});
-    std::vector<int2> work;
-    std::vector<std::vector<int32_t>> cover(c->n_colchk);
-    c->band_elems = 0;
-    for (int32_t r = 0; r < c->n_rowblk; ++r) {
-        int32_t lo = 0x7fffffff, hi = -1;
-        const int64_t p1 = std::min<int64_t>(c->n, (int64_t)(r + 1) * kPlainRows);
-        for (int64_t p = (int64_t)r * kPlainRows; p < p1; ++p)
-            if (last[p] >= 0) {
-                lo = std::min(lo, first[p]);
-                hi = std::max(hi, last[p]);
-            }
-        if (hi < 0) continue;
-        // whole sweeps: the kernel reads 128-column segments
-        const int32_t c0 = lo / kPlainCols * kPlainCols;
-        const int32_t c1 = std::min<int64_t>(c->ld, (hi / kPlainCols + 1) * kPlainCols);
-        c->h_rb_col[r] = int2{c0, c1};
-        rb_chunk[r] = int2{c0 / kPlainChunk, (c1 - 1) / kPlainChunk};
-        for (int32_t ch = rb_chunk[r].x; ch <= rb_chunk[r].y; ++ch) {
-            work.push_back(int2{r, ch});
-            cover[ch].push_back(r);
-        }
-        c->band_elems += (p1 - (int64_t)r * kPlainRows) * (int64_t)(c1 - c0);
+    rf.resize(n);
+    rl.resize(n);
+    cf.resize(n);
+    cl.resize(n);
+    TG_HIP(c, hipMemcpyAsync(rf.data(), d_rf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    TG_HIP(c, hipMemcpyAsync(rl.data(), d_rl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    TG_HIP(c, hipMemcpyAsync(cf.data(), d_cf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    TG_HIP(c, hipMemcpyAsync(cl.data(), d_cl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> cnt(nnz ? n : 0);
+    if (nnz) TG_HIP(c, hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, st));
+    TG_HIP(c, hipStreamSynchronize(st));
+    for (int64_t q = 0; q < n; ++q)
+        if (cl[q] < 0) cf[q] = -1;
+    if (nnz) {
+        *nnz = 0;
+        for (auto v : cnt) *nnz += (int64_t)v;
     }
-    std::vector<int32_t> cc_off(c->n_colchk + 1, 0), cc_rows;
-    for (int32_t ch = 0; ch < c->n_colchk; ++ch) {
-        cc_off[ch + 1] = cc_off[ch] + (int32_t)cover[ch].size();
-        cc_rows.insert(cc_rows.end(), cover[ch].begin(), cover[ch].end());
-    }
-    if (cc_rows.empty()) cc_rows.push_back(0);
-    if (work.empty()) work.push_back(int2{0, 0});
-    c->n_work = (int32_t)work.size();
-    TG_TRY(dev_alloc(c, &c->d_rb_col, (size_t)c->n_rowblk));
-    TG_TRY(dev_alloc(c, &c->d_rb_chunk, (size_t)c->n_rowblk));
-    TG_TRY(dev_alloc(c, &c->d_work, work.size()));
-    TG_TRY(dev_alloc(c, &c->d_cc_off, cc_off.size()));
-    TG_TRY(dev_alloc(c, &c->d_cc_rows, cc_rows.size()));
-    TG_HIP(c, hipMemcpyAsync(c->d_rb_col, c->h_rb_col.data(), sizeof(int2) * c->n_rowblk, hipMemcpyHostToDevice, c->stream));
-    TG_HIP(c, hipMemcpyAsync(c->d_rb_chunk, rb_chunk.data(), sizeof(int2) * c->n_rowblk, hipMemcpyHostToDevice, c->stream));
-    TG_HIP(c, hipMemcpyAsync(c->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, c->stream));
-    TG_HIP(c, hipMemcpyAsync(c->d_cc_off, cc_off.data(), sizeof(int32_t) * cc_off.size(), hipMemcpyHostToDevice, c->stream));
-    TG_HIP(c, hipMemcpyAsync(c->d_cc_rows, cc_rows.data(), sizeof(int32_t) * cc_rows.size(), hipMemcpyHostToDevice, c->stream));
-    TG_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&d_rf);
+    dev_free(&d_rl);
+    dev_free(&d_cf);
+    dev_free(&d_cl);
+    dev_free(&d_cnt);
     return TG_OK;
 }
 
@@ -379,56 +373,33 @@ extern "C" tg_status tg_kernel_assemble(tg_ctx* c, int32_t flags) {
                            c->d_R, c->d_len, c->maxw, c->d_gm, c->nm, c->qw, c->n, c->d_Q);
         TG_CHECK_LAUNCH(c);
 
-        // 5. K (dense FP64, row stride ld, zero padding)
-        TG_TRY(dev_alloc(c, &c->d_K, (size_t)c->n * c->ld));
-        hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
-                           sizeof(double) * c->maxw, c->stream, c->d_R, c->d_Q, c->d_off, c->d_len,
-                           c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->d_perm, c->g * c->g,
-                           c->d_K);
-        TG_CHECK_LAUNCH(c);
-
-        // 6. band of the sorted K and the work list of the one-pass kernel
-        c->n_rowblk = (int32_t)ceil_div(c->n, kPlainRows);
-        c->n_colchk = (int32_t)ceil_div(c->ld, kPlainChunk);
-        TG_TRY(build_band(c));
-        TG_TRY(dev_alloc(c, &c->d_upart, (size_t)c->n_colchk * c->n));
-        TG_TRY(dev_alloc(c, &c->d_vpart, (size_t)c->n_rowblk * c->ld));
-        // 7. K 1 and K^T 1 through the same one-pass kernel (f = 1)
+        // 5. exact nonzero extents of K in energy order and its nonzero count
+        TG_TRY(k_extents(c, 0.0, c->h_rf, c->h_rl, c->h_cf, c->h_cl, &c->nnz));
+        // 6. the FP64 band of K of the plain evaluation, entries from R and Q
+        TG_TRY(stack_prepare(c));
+        // 7. K 1 and K^T 1 through the same band pass (f = 1)
         TG_TRY(dev_alloc(c, &c->d_rk, c->n));
         TG_TRY(dev_alloc(c, &c->d_ck, c->n));
         TG_TRY(dev_alloc(c, &c->d_rk_s, c->n));
         TG_TRY(dev_alloc(c, &c->d_ck_s, c->n));
         {
-            std::vector<double> ones(c->n, 1.0);
-            TG_HIP(c, hipMemcpyAsync(c->d_Q, ones.data(), sizeof(double) * c->n,
-                                     hipMemcpyHostToDevice, c->stream));
-            TG_TRY(plain_products_device(c, c->d_K, c->d_Q, c->d_rk, c->d_ck));
-            hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                               c->d_rk, c->d_perm, c->n, c->d_rk_s);
-            hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                               c->d_ck, c->d_perm, c->n, c->d_ck_s);
-            TG_CHECK_LAUNCH(c);
-            TG_HIP(c, hipStreamSynchronize(c->stream));
+            double* d_one = nullptr;
+            TG_TRY(dev_alloc(c, &d_one, (size_t)c->n));
+            hipLaunchKernelGGL(fill_f64_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, d_one, c->n,
+                               1.0);
+            tg_status sp = stack_products_device(c, d_one, c->d_rk, c->d_ck);
+            if (sp == TG_OK) {
+                hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                                   c->d_rk, c->d_perm, c->n, c->d_rk_s);
+                hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream,
+                                   c->d_ck, c->d_perm, c->n, c->d_ck_s);
+                if (hipGetLastError() != hipSuccess) sp = set_error(c, TG_ERR_HIP, "row sums of K");
+            }
+            if (sp == TG_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+                sp = set_error(c, TG_ERR_HIP, "row sums of K");
+            dev_free(&d_one);
+            TG_TRY(sp);
         }
-        {
-            unsigned long long* d_cnt = nullptr;
-            TG_TRY(dev_alloc(c, &d_cnt, (size_t)c->n));
-            hipLaunchKernelGGL(nnz_rows_kernel, dim3(c->n), dim3(256), 0, c->stream, c->d_K, c->n,
-                               c->ld, d_cnt);
-            TG_CHECK_LAUNCH(c);
-            std::vector<unsigned long long> cnt(c->n);
-            TG_HIP(c, hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned long long) * c->n,
-                                     hipMemcpyDeviceToHost, c->stream));
-            TG_HIP(c, hipStreamSynchronize(c->stream));
-            dev_free(&d_cnt);
-            c->nnz = 0;
-            for (auto v : cnt) c->nnz += (int64_t)v;
-        }
-        TG_HIP(c, hipStreamSynchronize(c->stream));
-        // the projection/convolution scratch is only needed for assembly
-        dev_free(&c->d_Q);
-        // 8. the FP64 band of K of the plain evaluation
-        TG_TRY(stack_prepare(c));
         c->assembled = true;
     }
     if ((flags & TG_PREC_MIXED) || c->max_batch > 1) TG_TRY(mixed_prepare(c));
@@ -440,24 +411,16 @@ extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows,
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
     if (row0 < 0 || nrows < 0 || row0 + nrows > c->n) return set_error(c, TG_ERR_ARG, "rows out of range");
     if (nrows == 0) return TG_OK;
-    // K is stored in energy order: gather natural rows/columns through pinv
-    std::vector<int32_t> pinv(c->n);
-    for (int64_t p = 0; p < c->n; ++p) pinv[c->h_perm[p]] = (int32_t)p;
-    int32_t* d_pinv = nullptr;
+    // rows computed from the projection data (K is never stored densely)
     double* d_out = nullptr;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nrows, (int64_t)(1ll << 27) / c->n));
-    TG_TRY(dev_alloc(c, &d_pinv, (size_t)c->n));
-    tg_status s = dev_alloc(c, &d_out, (size_t)chunk * c->n);
-    if (s != TG_OK) {
-        dev_free(&d_pinv);
-        return s;
-    }
-    hipError_t e = hipMemcpyAsync(d_pinv, pinv.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice,
-                                  c->stream);
+    TG_TRY(dev_alloc(c, &d_out, (size_t)chunk * c->n));
+    const KSrc k = ksrc_of(c, c->d_Q);
+    hipError_t e = hipSuccess;
     for (int64_t r0 = 0; r0 < nrows && e == hipSuccess; r0 += chunk) {
         const int64_t nr = std::min(chunk, nrows - r0);
-        hipLaunchKernelGGL(copy_rows_kernel, dim3(ceil_div(c->n, 256), nr), dim3(256), 0, c->stream,
-                           c->d_K, c->n, c->ld, d_pinv, row0 + r0, d_out);
+        hipLaunchKernelGGL(copy_rows_kernel, dim3((unsigned)nr), dim3(256), sizeof(double) * c->maxw, c->stream, k,
+                           c->n, row0 + r0, d_out);
         e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(out + r0 * c->n, d_out, sizeof(double) * nr * c->n,
@@ -465,7 +428,6 @@ extern "C" tg_status tg_kernel_copy_rows(tg_ctx* c, int64_t row0, int64_t nrows,
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     dev_free(&d_out);
-    dev_free(&d_pinv);
     if (e != hipSuccess) return set_error(c, TG_ERR_HIP, "copy rows: %s", hipGetErrorString(e));
     return TG_OK;
 }
@@ -475,7 +437,7 @@ namespace tg {
 // dK/domega0: the same projection windows with the sampled derivative table
 // G'(x) = -x/sigma^2 G(x) (x = m dE + omega0), then K' 1 and K'^T 1.
 tg_status assemble_domega(tg_ctx* c) {
-    if (c->d_Kd) return TG_OK;
+    if (c->kd_ready) return TG_OK;
     if (!c->assembled) return set_error(c, TG_ERR_STATE, "kernel not assembled");
     std::vector<double> gd(c->nm);
     const double norm = 1.0 / (c->sigma * std::sqrt(2.0 * M_PI));
@@ -486,28 +448,28 @@ tg_status assemble_domega(tg_ctx* c) {
     }
     double* d_gd = nullptr;
     double* d_Qd = nullptr;
+    double* d_one = nullptr;
     TG_TRY(dev_alloc(c, &d_gd, gd.size()));
     TG_TRY(dev_alloc(c, &d_Qd, (size_t)c->n * c->qw));
+    TG_TRY(dev_alloc(c, &d_one, (size_t)c->n));
     TG_HIP(c, hipMemcpyAsync(d_gd, gd.data(), sizeof(double) * gd.size(), hipMemcpyHostToDevice,
                              c->stream));
     hipLaunchKernelGGL(conv_kernel, dim3(ceil_div(c->qw, 128), c->n), dim3(128), 0, c->stream,
                        c->d_R, c->d_len, c->maxw, d_gd, c->nm, c->qw, c->n, d_Qd);
     TG_CHECK_LAUNCH(c);
-    TG_TRY(dev_alloc(c, &c->d_Kd, (size_t)c->n * c->ld));
-    hipLaunchKernelGGL(kernel_entries, dim3(ceil_div(c->ld, kKThreads), c->n), dim3(kKThreads),
-                       sizeof(double) * c->maxw, c->stream, c->d_R, d_Qd, c->d_off, c->d_len,
-                       c->maxw, c->qw, c->m_lo, c->nm, c->n, c->ld, c->d_perm, c->g * c->g,
-                       c->d_Kd);
-    TG_CHECK_LAUNCH(c);
+    // dK/domega0 in K's band layout (its nonzeros lie inside K's: the same
+    // windows, and a window pair whose K entry is 0 has Q_j = 0 on the overlap)
+    TG_TRY(stack_prepare_domega(c, d_Qd));
     TG_TRY(dev_alloc(c, &c->d_rkd, c->n));
     TG_TRY(dev_alloc(c, &c->d_ckd, c->n));
-    std::vector<double> ones(c->n, 1.0);
-    TG_HIP(c, hipMemcpyAsync(d_Qd, ones.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
-                             c->stream));
-    TG_TRY(plain_products_device(c, c->d_Kd, d_Qd, c->d_rkd, c->d_ckd));
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, d_one, c->n, 1.0);
+    TG_CHECK_LAUNCH(c);
+    TG_TRY(stack_products_device(c, d_one, c->d_rkd, c->d_ckd, true));
     TG_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&d_one);
     dev_free(&d_Qd);
     dev_free(&d_gd);
+    c->kd_ready = true;
     return TG_OK;
 }
 

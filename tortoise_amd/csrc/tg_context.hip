@@ -238,7 +238,6 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     double v = 1.0;
     for (int q = 0; q < c->dim; ++q) v *= c->extent[q] / (double)c->nsec[q];
     c->vol = std::fabs(v / (double)factorial(c->dim));
-    c->ld = (c->n + kLdAlign - 1) / kLdAlign * kLdAlign;
     c->g = d->g;
     c->omega0 = d->omega0;
     c->sigma = d->sigma;
@@ -336,7 +335,6 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dp5_destroy(c);
     adjoint_destroy(c);
     ee_destroy(c);
-    dev_free(&c->d_Kd);
     dev_free(&c->d_rkd);
     dev_free(&c->d_ckd);
     dev_free(&c->d_eps);
@@ -347,11 +345,8 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_R);
     dev_free(&c->d_Q);
     dev_free(&c->d_gm);
-    dev_free(&c->d_K);
     dev_free(&c->d_rk);
     dev_free(&c->d_ck);
-    dev_free(&c->d_upart);
-    dev_free(&c->d_vpart);
     dev_free(&c->d_perm);
     dev_free(&c->d_V_s);
     dev_free(&c->d_eps_s);
@@ -360,11 +355,6 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_rk_s);
     dev_free(&c->d_ck_s);
     dev_free(&c->d_fs);
-    dev_free(&c->d_rb_chunk);
-    dev_free(&c->d_rb_col);
-    dev_free(&c->d_work);
-    dev_free(&c->d_cc_off);
-    dev_free(&c->d_cc_rows);
     dev_free(&c->d_f);
     dev_free(&c->d_I);
     dev_free(&c->d_red);

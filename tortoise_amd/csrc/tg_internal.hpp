@@ -10,21 +10,9 @@
 #include <vector>
 
 #include "../../include/tortoise_gpu.h"
+#include "tg_kentry.hpp"
 
 namespace tg {
-
-// Plain FP64 one-pass kernel geometry: a workgroup of 256 threads covers
-// kPlainRows rows x kPlainCols columns per sweep (each thread two adjacent
-// columns, the four waves interleaved over rows) and sweeps kPlainChunk
-// columns before reducing its per-row partial sums.
-constexpr int kPlainThreads = 256;
-constexpr int kPlainRows = 128;      // rows per workgroup (32 per wave)
-constexpr int kPlainCols = 128;      // columns per sweep (64 lanes x 2)
-#ifndef TG_PLAIN_CHUNK
-#define TG_PLAIN_CHUNK 2048
-#endif
-constexpr int kPlainChunk = TG_PLAIN_CHUNK;   // columns per workgroup
-constexpr int kLdAlign = 64;         // K row stride alignment in elements
 
 struct Buffer {
     void* p = nullptr;
@@ -43,7 +31,6 @@ struct tg_ctx {
     double extent[3] = {1, 1, 1};
     int64_t n = 0;       // nodes
     int64_t nel = 0;     // elements
-    int64_t ld = 0;      // K row stride (elements)
     double vol = 0;      // element volume
     double g = 0, omega0 = 0, sigma = 0, bath_n = 0, de = 0, cutoff = 8;
     int32_t max_batch = 1;
@@ -63,22 +50,20 @@ struct tg_ctx {
     int32_t* d_off = nullptr;      // projection window start per node
     int32_t* d_len = nullptr;      // projection window length per node
     double* d_R = nullptr;         // n x maxw
-    double* d_Q = nullptr;         // n x qw
+    double* d_Q = nullptr;         // n x qw (kept: K's entries are computed from R and Q where
+                                   // needed, K itself is never stored densely)
     double* d_gm = nullptr;        // nm
-    double* d_K = nullptr;         // n x ld, FP64
     double* d_rk = nullptr;        // K 1
     double* d_ck = nullptr;        // K^T 1
     int64_t nnz = 0;               // structurally nonzero entries of K
-    double* d_Kd = nullptr;        // dK/domega0 (adjoint only), n x ld
+    // exact nonzero extents of K in energy order (-1 when empty): row p's
+    // columns [h_rf[p], h_rl[p]], column q's rows [h_cf[q], h_cl[q]]
+    std::vector<int32_t> h_rf, h_rl, h_cf, h_cl;
+    bool kd_ready = false;         // dK/domega0's band exists (adjoint)
     double* d_rkd = nullptr;       // dK/domega0 1
     double* d_ckd = nullptr;       // (dK/domega0)^T 1
     void* adj = nullptr;           // adjoint work (tg_adjoint.hip)
     bool assembled = false;
-
-    // plain FP64 apply scratch
-    int32_t n_rowblk = 0, n_colchk = 0;
-    double* d_upart = nullptr;     // n_colchk x n
-    double* d_vpart = nullptr;     // n_rowblk x ld
 
     // energy order: K is stored with rows and columns sorted by node energy
     int32_t* d_perm = nullptr;     // sorted position -> node
@@ -95,17 +80,7 @@ struct tg_ctx {
     double* d_rk_s = nullptr;      // K 1 in sorted order
     double* d_ck_s = nullptr;      // K^T 1 in sorted order
     double* d_fs = nullptr;        // scratch: one state gathered to sorted order
-    // band of the sorted K: row block r has nonzero columns in
-    // [rb_col[r].x, rb_col[r].y) (column units); the one-pass kernel runs over
-    // the (row block, column chunk) pairs that intersect it
-    std::vector<int2> h_rb_col;
-    int2* d_rb_chunk = nullptr;    // per row block: [first chunk, last chunk]
-    int2* d_rb_col = nullptr;      // per row block: [first col, end col)
-    int2* d_work = nullptr;        // (row block, chunk) pairs
-    int32_t n_work = 0;
-    int32_t* d_cc_off = nullptr;   // per column chunk: offset into d_cc_rows (n_colchk + 1)
-    int32_t* d_cc_rows = nullptr;  // row blocks whose band covers the chunk, increasing
-    int64_t band_elems = 0;        // entries the plain evaluation streams (K's band once built)
+    int64_t band_elems = 0;        // entries the plain evaluation streams (K's band)
 
     // FP64 band of K for the plain evaluation (tg_stack.hip)
     void* stack = nullptr;
@@ -182,19 +157,27 @@ inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) /
 // bath_n; d_flux (device, nullable) receives sum_i V_i eps_i I_i
 tg_status plain_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n,
                              int conserve, const double* d_bath = nullptr, double* d_flux = nullptr);
-tg_status plain_products_device(tg_ctx* c, const double* Kmat, const double* d_f, double* d_u,
-                                double* d_v);
+// K's entries from the projection data with the convolved windows Q (the
+// context's d_Q for K, the derivative table's for dK/domega0)
+KSrc ksrc_of(const tg_ctx* c, const double* Q);
+// exact nonzero extents of K in energy order (rows: first / last column;
+// columns: first / last row; -1 when empty) and optionally the nonzero
+// count; with split_scale > 0 of the fp16 hi/lo split at that scale
+tg_status k_extents(tg_ctx* c, double split_scale, std::vector<int32_t>& rf, std::vector<int32_t>& rl,
+                    std::vector<int32_t>& cf, std::vector<int32_t>& cl, int64_t* nnz);
 tg_status assemble_domega(tg_ctx* c);
-tg_status build_band(tg_ctx* c);
 tg_status conservation_device(tg_ctx* c, const double* d_f, int32_t batch, double* number,
                               double* energy);
 tg_status relative_error_device(tg_ctx* c, const double* d_err, const double* d_pop,
                                 int32_t batch, double rtol, double atol, double* h_out);
-// FP64 band of K (tg_stack.hip): built at assembly, one plain
-// evaluation per call
+// FP64 band of K (tg_stack.hip): built at assembly from the row extents
+// h_rf / h_rl and the entries, one plain evaluation per call
 tg_status stack_prepare(tg_ctx* c);
-// u = K f, v = K^T f (caller order) from the same pass
-tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v);
+// dK/domega0 in the same band layout (entries from Qd)
+tg_status stack_prepare_domega(tg_ctx* c, const double* d_Qd);
+// u = M f, v = M^T f (caller order) from the same pass over M = K, or
+// M = dK/domega0 when domega
+tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v, bool domega = false);
 void stack_destroy(tg_ctx* c);
 // records this device's address of the allocation-free evaluation's scratch
 // array (tg_collision_apply_noalloc); called at context creation

@@ -29,8 +29,8 @@
 //     iteration is one contiguous block
 // Every MFMA fragment and LDS tile is a contiguous 16-byte read per lane in
 // these layouts (lanes 0-31: k-group 2s, lanes 32-63: k-group 2s+1), and a
-// panel streams its band as consecutive 16 KB blocks.  The dense layout
-// A_{hi,lo}[kg][row] exists only while the band is being measured.
+// panel streams its band as consecutive 16 KB blocks.  The band is packed
+// straight from K's entries (tg_kentry.hpp): no dense copy of A exists.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -62,8 +62,6 @@ struct MixedData {
     int nt = 256;              // members per GEMM tile (padded batch)
     int sa = 0;                // A scale exponent
     int npanel = 0;
-    uint4* A_hi = nullptr;     // dense packed A (during preparation only)
-    uint4* A_lo = nullptr;
     uint4* A_band = nullptr;   // band-major packed A (hi and lo)
     int64_t* band_off = nullptr;    // per panel: first 16-byte unit of its band
     int64_t band_units = 0;
@@ -119,104 +117,62 @@ struct MixedData {
 static MixedData* md_of(tg_ctx* c) { return (MixedData*)c->mixed; }
 
 // ------------------------------------------------------------------ packing
-// FP64 -> fp16 through FP32 with the two hardware conversions.  The empty
-// asm statement keeps the FP32 value opaque: without it the compiler folds
-// the two conversions into one correctly rounded double -> half conversion,
-// which gfx950 has no instruction for and which it emulates with ~40 integer
-// operations.
-__device__ inline _Float16 to_fp16(double s) {
-    float t = (float)s;
-    __asm__ volatile("" : "+v"(t));
-    return (_Float16)t;
-}
-
-// hi = s rounded to fp16 through FP32; a double rounding tie moves hi by one
-// fp16 ulp, which the exact FP64 residual s - hi absorbs, so hi + lo keeps
-// its ~22 significant bits
-__device__ inline void split_fp16(double x, double scale, _Float16& hi, _Float16& lo) {
-    const double s = x * scale;
-    hi = to_fp16(s);
-    lo = to_fp16(s - (double)hi);
-}
-
-// A[r][q] from K stored in energy order: r < n -> K[r][q]; the K^T half
-// starts at the panel-aligned row kt: kt <= r < kt + n -> K[q][r-kt]
-__global__ void pack_A_kernel(const double* __restrict__ K, int64_t n, int64_t ld, int64_t kt,
-                              int64_t rows_pad, int64_t kgroups, double scale,
-                              uint4* __restrict__ Ahi, uint4* __restrict__ Alo) {
-    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t kg = blockIdx.y;
-    if (row >= rows_pad || kg >= kgroups) return;
-    union { half8 v; uint4 u; } h, l;
-    for (int q = 0; q < 8; ++q) {
-        const int64_t col = kg * 8 + q;
-        double x = 0.0;
-        if (col < n) {
-            if (row < n) x = K[row * ld + col];
-            else if (row >= kt && row < kt + n) x = K[col * ld + (row - kt)];
-        }
-        _Float16 a, b;
-        split_fp16(x, scale, a, b);
-        h.v[q] = a;
-        l.v[q] = b;
-    }
-    Ahi[kg * rows_pad + row] = h.u;
-    Alo[kg * rows_pad + row] = l.u;
-}
-
-// first / last nonzero column (sorted order) of every A row; -1 when empty
-__global__ void row_extent_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                                  int64_t rows_pad, int64_t kgroups, int32_t* __restrict__ first,
-                                  int32_t* __restrict__ last) {
-    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (row >= rows_pad) return;
-    int32_t f = -1, l = -1;
-    for (int64_t kg = 0; kg < kgroups; ++kg) {
-        uint4 a = Ahi[kg * rows_pad + row], b = Alo[kg * rows_pad + row];
-        if ((a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w) != 0u) {
-            if (f < 0) f = (int32_t)kg;
-            l = (int32_t)kg;
-        }
-    }
-    first[row] = f;
-    last[row] = l;
-}
-
-// exact FP64 row sums of the split matrix: rows [0, n) give K 1, rows
-// [kt, kt + n) give K^T 1 (both in sorted order)
-__global__ void split_rowsum_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo,
-                                    int64_t rows_pad, int64_t kgroups, int64_t n, int64_t kt,
-                                    double inv_scale, double* __restrict__ rk,
-                                    double* __restrict__ ck) {
-    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (row >= rows_pad || (row >= n && row < kt) || row >= kt + n) return;
-    double s = 0.0;
-    for (int64_t kg = 0; kg < kgroups; ++kg) {
-        union { half8 v; uint4 u; } h, l;
-        h.u = Ahi[kg * rows_pad + row];
-        l.u = Alo[kg * rows_pad + row];
-        for (int q = 0; q < 8; ++q) s += (double)h.v[q] + (double)l.v[q];
-    }
-    s *= inv_scale;
-    if (row < n) rk[row] = s;
-    else ck[row - kt] = s;
-}
-
-// dense packed A -> band-major packed A: block (panel, iteration) holds the
-// k-groups 4 it .. 4 it + 3 of the panel's band, hi and lo, 128 rows each
+// Band-major packed A from K's entries (tg_kentry.hpp): block (panel,
+// iteration) holds the k-groups 4 it .. 4 it + 3 of the panel's band, hi and
+// lo, 128 rows each.  A[r][q] = K[r][q] for r < n; the K^T half starts at
+// the panel-aligned row kt: A[kt + r][q] = K[q][r].  Thread (row, k-group)
+// evaluates its 8 entries and writes their hi and lo units.
 constexpr int kBandT = 2 * kKG * kMT;
-__global__ void __launch_bounds__(kBandT)
-band_pack_kernel(const uint4* __restrict__ Ahi, const uint4* __restrict__ Alo, int64_t rows_pad,
+__global__ void __launch_bounds__(kKG * kMT)
+band_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, int64_t kt, double scale,
                  const int2* __restrict__ krange, const int64_t* __restrict__ band_off,
                  uint4* __restrict__ band) {
     const int panel = blockIdx.x;
     const int itl = blockIdx.y;
     const int2 kr = krange[panel];
     if (itl >= kr.y - kr.x) return;
-    const int t = threadIdx.x;
-    const int r = t % kMT, h = (t / kMT) % 2, kg = t / (2 * kMT);
-    const int64_t src = ((int64_t)(kr.x + itl) * kKG + kg) * rows_pad + (int64_t)panel * kMT + r;
-    band[band_off[panel] + (int64_t)itl * kBandT + t] = h ? Alo[src] : Ahi[src];
+    const int r = threadIdx.x % kMT, kg = threadIdx.x / kMT;
+    const int64_t row = (int64_t)panel * kMT + r;
+    const int64_t col0 = ((int64_t)(kr.x + itl) * kKG + kg) * 8;
+    union { half8 v; uint4 u; } h, l;
+    for (int q = 0; q < 8; ++q) {
+        const int64_t col = col0 + q;
+        double x = 0.0;
+        if (col < n) {
+            if (row < n) x = k_entry(k, perm[row], perm[col]);
+            else if (row >= kt && row < kt + n) x = k_entry(k, perm[col], perm[row - kt]);
+        }
+        _Float16 a, b;
+        split_fp16(x, scale, a, b);
+        h.v[q] = a;
+        l.v[q] = b;
+    }
+    uint4* dst = band + band_off[panel] + (int64_t)itl * kBandT;
+    dst[(kg * 2 + 0) * kMT + r] = h.u;
+    dst[(kg * 2 + 1) * kMT + r] = l.u;
+}
+
+// exact FP64 row sums of the split matrix over each row's band (ascending
+// columns): rows [0, n) give K 1, rows [kt, kt + n) give K^T 1 (sorted order)
+__global__ void band_rowsum_kernel(const uint4* __restrict__ band, const int64_t* __restrict__ band_off,
+                                   const int2* __restrict__ krange, int64_t rows_pad, int64_t n, int64_t kt,
+                                   double inv_scale, double* __restrict__ rk, double* __restrict__ ck) {
+    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (row >= rows_pad || (row >= n && row < kt) || row >= kt + n) return;
+    const int panel = (int)(row / kMT), r = (int)(row % kMT);
+    const int2 kr = krange[panel];
+    const uint4* b0 = band + band_off[panel];
+    double s = 0.0;
+    for (int it = 0; it < kr.y - kr.x; ++it)
+        for (int kg = 0; kg < kKG; ++kg) {
+            union { half8 v; uint4 u; } h, l;
+            h.u = b0[(int64_t)it * kBandT + (kg * 2 + 0) * kMT + r];
+            l.u = b0[(int64_t)it * kBandT + (kg * 2 + 1) * kMT + r];
+            for (int q = 0; q < 8; ++q) s += (double)h.v[q] + (double)l.v[q];
+        }
+    s *= inv_scale;
+    if (row < n) rk[row] = s;
+    else ck[row - kt] = s;
 }
 
 // Member-per-workgroup kernels.  Workgroup w serves member
@@ -1669,44 +1625,27 @@ tg_status mixed_prepare(tg_ctx* c) {
     m->sa = kmax > 0.0 ? (int)std::floor(std::log2(16384.0 / kmax)) : 0;
     const double scale = std::ldexp(1.0, m->sa);
 
-    TG_TRY(dev_alloc(c, &m->A_hi, (size_t)kgroups * m->rows_pad));
-    TG_TRY(dev_alloc(c, &m->A_lo, (size_t)kgroups * m->rows_pad));
-    hipLaunchKernelGGL(pack_A_kernel, dim3(ceil_div(m->rows_pad, 256), kgroups), dim3(256), 0, st,
-                       c->d_K, n, c->ld, m->kt, m->rows_pad, kgroups, scale, m->A_hi, m->A_lo);
-    TG_CHECK_LAUNCH(c);
-
-    // split-matrix row sums and sorted node volumes
-    TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
-    TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
-    m->V_s = c->d_V_s;
-    m->eps_s = c->d_eps_s;
-    hipLaunchKernelGGL(split_rowsum_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st,
-                       m->A_hi, m->A_lo, m->rows_pad, kgroups, n, m->kt, 1.0 / scale, m->rk_s,
-                       m->ck_s);
-    TG_CHECK_LAUNCH(c);
-
-    // band extent of every panel
-    int32_t *d_first = nullptr, *d_last = nullptr;
-    TG_TRY(dev_alloc(c, &d_first, (size_t)m->rows_pad));
-    TG_TRY(dev_alloc(c, &d_last, (size_t)m->rows_pad));
-    hipLaunchKernelGGL(row_extent_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st,
-                       m->A_hi, m->A_lo, m->rows_pad, kgroups, d_first, d_last);
-    TG_CHECK_LAUNCH(c);
-    std::vector<int32_t> first(m->rows_pad), last(m->rows_pad);
-    TG_HIP(c, hipMemcpyAsync(first.data(), d_first, sizeof(int32_t) * m->rows_pad, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipMemcpyAsync(last.data(), d_last, sizeof(int32_t) * m->rows_pad, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipStreamSynchronize(st));
-    dev_free(&d_first);
-    dev_free(&d_last);
+    // band extent of every panel: the exact extents of K's rows (A rows < n)
+    // and columns (the K^T half) where the fp16 split is nonzero, in k-groups
+    std::vector<int32_t> rf, rl, cf, cl;
+    TG_TRY(k_extents(c, scale, rf, rl, cf, cl, nullptr));
     m->h_krange.resize(m->npanel);
     int64_t items = 0;
     for (int pn = 0; pn < m->npanel; ++pn) {
         int32_t lo = INT32_MAX, hi = -1;
         for (int r = 0; r < kMT; ++r) {
             const int64_t row = (int64_t)pn * kMT + r;
-            if (first[row] >= 0) {
-                lo = std::min(lo, first[row]);
-                hi = std::max(hi, last[row]);
+            int32_t f = -1, l = -1;
+            if (row < n) {
+                f = rf[row];
+                l = rl[row];
+            } else if (row >= m->kt && row < m->kt + n) {
+                f = cf[row - m->kt];
+                l = cl[row - m->kt];
+            }
+            if (l >= 0) {
+                lo = std::min(lo, f / 8);      // columns -> k-groups
+                hi = std::max(hi, l / 8);
             }
         }
         int2 kr;
@@ -1723,7 +1662,8 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
-    // band-major copy of A; the dense packing is freed once it is made
+    // band-major packed A from K's entries, then the split-matrix row sums
+    // over it; sorted node volumes and energies
     {
         std::vector<int64_t> off(m->npanel);
         int64_t units = 0;
@@ -1739,13 +1679,19 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_HIP(c, hipMemcpyAsync(m->band_off, off.data(), sizeof(int64_t) * m->npanel, hipMemcpyHostToDevice, st));
         TG_TRY(dev_alloc(c, &m->A_band, (size_t)std::max<int64_t>(units, 1)));
         if (maxit > 0) {
-            hipLaunchKernelGGL(band_pack_kernel, dim3(m->npanel, maxit), dim3(kBandT), 0, st, m->A_hi, m->A_lo,
-                               m->rows_pad, m->krange, m->band_off, m->A_band);
+            hipLaunchKernelGGL(band_pack_kernel, dim3(m->npanel, maxit), dim3(kKG * kMT), 0, st,
+                               ksrc_of(c, c->d_Q), c->d_perm, n, m->kt, scale, m->krange, m->band_off,
+                               m->A_band);
             TG_CHECK_LAUNCH(c);
         }
+        TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
+        TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
+        m->V_s = c->d_V_s;
+        m->eps_s = c->d_eps_s;
+        hipLaunchKernelGGL(band_rowsum_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st, m->A_band,
+                           m->band_off, m->krange, m->rows_pad, n, m->kt, 1.0 / scale, m->rk_s, m->ck_s);
+        TG_CHECK_LAUNCH(c);
         TG_HIP(c, hipStreamSynchronize(st));
-        dev_free(&m->A_hi);
-        dev_free(&m->A_lo);
     }
     // configuration.  Default (kept by the autotuner unless beaten by >3%):
     // the persistent kernel, 128 members per tile (64 for smaller batches),
@@ -1901,8 +1847,6 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
 void mixed_destroy(tg_ctx* c) {
     MixedData* m = md_of(c);
     if (!m) return;
-    dev_free(&m->A_hi);
-    dev_free(&m->A_lo);
     dev_free(&m->A_band);
     dev_free(&m->band_off);
     dev_free(&m->Z_all);

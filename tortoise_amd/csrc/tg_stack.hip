@@ -47,9 +47,11 @@ struct StackData {
     int64_t kt = 0;                 // n rounded up to whole panels
     int32_t npanel = 0;
     int32_t nitems = 0;
+    int32_t maxit = 0;              // most iterations of a panel
     int64_t elems = 0;              // stored entries of S
     std::vector<int2> h_kr;         // per panel: [first, end) iteration
     double* Sb = nullptr;           // band-major S
+    double* Sd = nullptr;           // dK/domega0 in the same layout (adjoint; null until needed)
     int64_t* off = nullptr;         // per panel: first entry
     int2* kr = nullptr;
     int4* items = nullptr;          // (panel, first iteration, end iteration, 0)
@@ -63,31 +65,6 @@ struct StackData {
 
 static StackData* sd_of(tg_ctx* c) { return (StackData*)c->stack; }
 
-// first / last nonzero column of every row of K (-1 when empty)
-__global__ void stack_row_extent_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
-                                        int32_t* __restrict__ first, int32_t* __restrict__ last) {
-    __shared__ int32_t sf, sl;
-    const int64_t i = blockIdx.x;
-    if (threadIdx.x == 0) {
-        sf = INT32_MAX;
-        sl = -1;
-    }
-    __syncthreads();
-    int32_t f = INT32_MAX, l = -1;
-    for (int64_t j = threadIdx.x; j < n; j += blockDim.x)
-        if (K[i * ld + j] != 0.0) {
-            f = min(f, (int32_t)j);
-            l = max(l, (int32_t)j);
-        }
-    atomicMin(&sf, f);
-    atomicMax(&sl, l);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        first[i] = sl < 0 ? -1 : sf;
-        last[i] = sl;
-    }
-}
-
 // fs[p] = f[perm[p]]
 __global__ void stack_gather_kernel(const double* __restrict__ f, const int32_t* __restrict__ perm,
                                     int64_t n, double* __restrict__ fs) {
@@ -95,8 +72,9 @@ __global__ void stack_gather_kernel(const double* __restrict__ f, const int32_t*
     if (p < n) fs[p] = f[perm[p]];
 }
 
-// block (panel, local iteration): entries [column 0..31][row 0..127] of K
-__global__ void stack_pack_kernel(const double* __restrict__ K, int64_t n, int64_t ld,
+// block (panel, local iteration): entries [column 0..31][row 0..127] of the
+// energy-ordered matrix whose entries k gives (K, or dK/domega0)
+__global__ void stack_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n,
                                   const int2* __restrict__ kr, const int64_t* __restrict__ off,
                                   double* __restrict__ Sb) {
     const int panel = blockIdx.x;
@@ -108,7 +86,7 @@ __global__ void stack_pack_kernel(const double* __restrict__ K, int64_t n, int64
         const int col = t / kStackRows, row = t % kStackRows;
         const int64_t R = (int64_t)panel * kStackRows + row;
         const int64_t C = (int64_t)(r.x + itl) * kStackIt + col;
-        dst[t] = (C < n && R < n) ? K[R * ld + C] : 0.0;
+        dst[t] = (C < n && R < n) ? k_entry(k, perm[R], perm[C]) : 0.0;
     }
 }
 
@@ -279,8 +257,9 @@ stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ 
     }
 }
 
-// builds K's band from the assembled K (energy order): row extents,
-// band-major copy, work items and the items covering each column iteration
+// builds K's band (energy order) from the exact row extents h_rf / h_rl:
+// band-major entries (computed from the projection data), work items and
+// the items covering each column iteration
 tg_status stack_prepare(tg_ctx* c) {
     if (c->stack) return TG_OK;
     StackData* s = new StackData();
@@ -289,17 +268,10 @@ tg_status stack_prepare(tg_ctx* c) {
     s->kt = (n + kStackRows - 1) / kStackRows * kStackRows;
     s->npanel = (int32_t)(s->kt / kStackRows);
     hipStream_t st = c->stream;
-    int32_t *d_rf = nullptr, *d_rl = nullptr;
-    TG_TRY(dev_alloc(c, &d_rf, (size_t)n));
-    TG_TRY(dev_alloc(c, &d_rl, (size_t)n));
-    hipLaunchKernelGGL(stack_row_extent_kernel, dim3(n), dim3(256), 0, st, c->d_K, n, c->ld, d_rf, d_rl);
-    TG_CHECK_LAUNCH(c);
-    std::vector<int32_t> rf(n), rl(n);
-    TG_HIP(c, hipMemcpyAsync(rf.data(), d_rf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipMemcpyAsync(rl.data(), d_rl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    TG_HIP(c, hipStreamSynchronize(st));
-    dev_free(&d_rf);
-    dev_free(&d_rl);
+    if ((int64_t)c->h_rf.size() != n || (int64_t)c->h_rl.size() != n)
+        return set_error(c, TG_ERR_STATE, "row extents of K not computed");
+    const std::vector<int32_t>& rf = c->h_rf;
+    const std::vector<int32_t>& rl = c->h_rl;
     // panel extents in iterations
     s->h_kr.assign(s->npanel, int2{0, 0});
     std::vector<int64_t> off(s->npanel, 0);
@@ -359,12 +331,29 @@ tg_status stack_prepare(tg_ctx* c) {
     TG_HIP(c, hipMemcpyAsync(s->pitems, pitems.data(), sizeof(int2) * s->npanel, hipMemcpyHostToDevice, st));
     TG_HIP(c, hipMemcpyAsync(s->cit_off, cit_off.data(), sizeof(int32_t) * (nit + 1), hipMemcpyHostToDevice, st));
     TG_HIP(c, hipMemcpyAsync(s->cit, cit.data(), sizeof(int32_t) * cit.size(), hipMemcpyHostToDevice, st));
+    s->maxit = maxit;
     if (maxit > 0) {
-        hipLaunchKernelGGL(stack_pack_kernel, dim3(s->npanel, maxit), dim3(256), 0, st, c->d_K, n, c->ld,
-                           s->kr, s->off, s->Sb);
+        hipLaunchKernelGGL(stack_pack_kernel, dim3(s->npanel, maxit), dim3(256), 0, st, ksrc_of(c, c->d_Q),
+                           c->d_perm, n, s->kr, s->off, s->Sb);
         TG_CHECK_LAUNCH(c);
     }
     TG_HIP(c, hipStreamSynchronize(st));
+    return TG_OK;
+}
+
+// dK/domega0 in the band layout of K (entries from the derivative table's
+// convolved windows Qd)
+tg_status stack_prepare_domega(tg_ctx* c, const double* d_Qd) {
+    StackData* s = sd_of(c);
+    if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
+    if (s->Sd) return TG_OK;
+    TG_TRY(dev_alloc(c, &s->Sd, (size_t)std::max<int64_t>(s->elems, 1)));
+    if (s->maxit > 0) {
+        hipLaunchKernelGGL(stack_pack_kernel, dim3(s->npanel, s->maxit), dim3(256), 0, c->stream, ksrc_of(c, d_Qd),
+                           c->d_perm, c->n, s->kr, s->off, s->Sd);
+        TG_CHECK_LAUNCH(c);
+    }
+    TG_HIP(c, hipStreamSynchronize(c->stream));
     return TG_OK;
 }
 
@@ -372,6 +361,7 @@ void stack_destroy(tg_ctx* c) {
     StackData* s = sd_of(c);
     if (!s) return;
     dev_free(&s->Sb);
+    dev_free(&s->Sd);
     dev_free(&s->off);
     dev_free(&s->kr);
     dev_free(&s->items);
@@ -384,17 +374,18 @@ void stack_destroy(tg_ctx* c) {
     c->stack = nullptr;
 }
 
-// u = K f, v = K^T f (caller order in and out) from one pass over K's band;
-// kernel launches only
-tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v) {
+// u = M f, v = M^T f (caller order in and out) from one pass over the band
+// of M = K (or dK/domega0 with domega); kernel launches only
+tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, double* d_v, bool domega) {
     StackData* s = sd_of(c);
     if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
+    if (domega && !s->Sd) return set_error(c, TG_ERR_STATE, "band of dK/domega0 not prepared");
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, d_f, c->d_perm,
                        c->n, c->d_fs);
     TG_CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, s->Sb, s->off, s->kr,
-                       s->items, c->n, c->d_fs, s->part, s->vpart);
+    hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, domega ? s->Sd : s->Sb,
+                       s->off, s->kr, s->items, c->n, c->d_fs, s->part, s->vpart);
     TG_CHECK_LAUNCH(c);
     hipLaunchKernelGGL(stack_products_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, s->part, s->vpart,
                        s->pitems, s->cit_off, s->cit, c->n, c->d_perm, d_u, d_v);

@@ -297,6 +297,17 @@ typedef struct tg_tuple_desc {
                                1: it evolves -- states and outputs hold both bands,
                                [member][band: electron, phonon][element][dim+1] */
     const double* eps_ph;   /* phonon nodal energies (a P1 dispersion), or NULL: omega0 */
+    /* A general zone (replaces Region<NDim> / Mesh<NDim>'s parallelepipeds,
+     * include/Geometry/Structured/DirectSpace/Region.hpp:29-34,
+     * src/Geometry/Structured/DirectSpace/Mesh.cpp:268-281): the mesh spans
+     * origin + sum_c t_c sides[c] (t in [0,1]^dim; sides[c][d] is component
+     * d of side c, Mesh::gVec = Region::gVec * diag(relative extent)), node
+     * index i_c along side c at origin + sum_c (i_c / nsec[c]) sides[c]; the
+     * umklapp vectors are sum_c u_c zone[c] (zone[c] = Region::gVec column
+     * c, ScatteringChannel.cpp:330-334).  All-zero sides: the axis-aligned
+     * box origin + extent, which is also the zone. */
+    double  sides[3][3];
+    double  zone[3][3];
 } tg_tuple_desc;
 
 tg_status   tg_tuple_create(const tg_tuple_desc* desc, tg_tuple_ctx** out);

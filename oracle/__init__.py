@@ -104,7 +104,7 @@ def lib():
                                                  C.c_double, C.c_int, _f64p, _i8p, C.c_int64,
                                                  C.c_void_p, C.c_void_p, C.c_int32,
                                                  C.POINTER(C.c_double)]
-        L.or_dp5_mc_eph_replay.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
+        L.or_dp5_mc_eph_replay.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p,
                                            C.c_double, C.c_double, C.c_int32, _f64p, C.c_int32,
                                            _i32p, C.c_int64, C.c_int32, C.c_double, C.c_void_p,
                                            C.c_int32, C.c_double, C.c_double, _f64p, C.c_double,
@@ -122,7 +122,7 @@ def lib():
         L.or_step_vetoer.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.or_energy_dg.restype = C.c_double
         L.or_energy_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p]
-        L.or_dp5_mc_eph_ex.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+        L.or_dp5_mc_eph_ex.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                        C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                        C.c_int32, C.c_double, C.c_void_p, C.c_int32, C.c_double,
                                        C.c_double, C.c_double, C.c_double, _f64p, C.c_double,
@@ -151,21 +151,21 @@ def lib():
         L.or_mc_integrate.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
                                       C.c_int32, _f64p]
         L.or_mc_tuples_eph.restype = C.c_int64
-        L.or_mc_tuples_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+        L.or_mc_tuples_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                        C.c_int32, _f64p, C.c_int32, C.c_double, C.c_double,
                                        C.c_void_p, C.c_int64]
         L.or_mc_inverse_mass.argtypes = [C.c_int, C.c_double, _f64p]
-        L.or_mc_apply_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+        L.or_mc_apply_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                       C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                       _f64p, _f64p, _f64p, C.c_void_p, C.c_int32]
         L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
-        L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+        L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                       C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                       _f64p, _f64p, _f64p, C.c_int32]
         L.or_relative_error_dg.restype = C.c_double
         L.or_relative_error_dg.argtypes = [C.c_int, C.c_int64, _f64p, _f64p, C.c_double, C.c_double,
                                            C.c_double, C.c_double]
-        L.or_dp5_mc_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
+        L.or_dp5_mc_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                     C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                     C.c_int32, C.c_double, _f64p, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int32,
@@ -596,20 +596,36 @@ def _eps_ph_ptr(eps_ph):
     return a, a.ctypes.data
 
 
+def zone_geometry(dim, origin, extent, sides=None, zone=None):
+    """Geometry block of the tuple-operator functions (21 doubles): the
+    origin, the mesh's side vectors (side c: node index i_c along it at
+    origin + sum_c (i_c / nsec_c) side_c), the zone's side vectors (the
+    umklapp basis).  Default: the axis-aligned box origin + extent, which is
+    also the zone; with sides and no zone the mesh covers the zone."""
+    g = np.zeros(21)
+    g[:dim] = np.asarray(origin, np.float64)[:dim]
+    S = np.diag(np.asarray(extent, np.float64)[:dim]) if sides is None else np.asarray(sides, np.float64)
+    Z = S if zone is None else np.asarray(zone, np.float64)
+    for c in range(dim):
+        g[3 + 3 * c: 3 + 3 * c + dim] = S[c][:dim]
+        g[12 + 3 * c: 12 + 3 * c + dim] = Z[c][:dim]
+    return g
+
+
 def mc_tuples_eph(dim, nsec, origin, extent, eps, omega0, nu, pts, min_det=1e-9,
-                  threshold=1e-9, eps_ph=None):
+                  threshold=1e-9, eps_ph=None, sides=None, zone=None):
     """Tuple list int32 [T][4] = (e0, e1, e2, umklapp id) of the e-ph channel
     (phonon band: nodal energies eps_ph, or the constant omega0)."""
     nsec = np.ascontiguousarray(nsec, np.int32)
-    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
     eps = np.ascontiguousarray(eps, np.float64)
     pts = np.ascontiguousarray(pts, np.float64)
     keep, ph = _eps_ph_ptr(eps_ph)
     L = lib()
-    n = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
+    n = L.or_mc_tuples_eph(dim, nsec, geo, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
                            threshold, None, 0)
     idx = np.zeros((max(n, 1), 4), np.int32)
-    m = L.or_mc_tuples_eph(dim, nsec, org, ext, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
+    m = L.or_mc_tuples_eph(dim, nsec, geo, eps, ph, omega0, nu, pts, pts.shape[1], min_det,
                            threshold, idx.ctypes.data, n)
     assert m == n
     return idx[:n]
@@ -622,10 +638,10 @@ def mc_inverse_mass(dim, vol):
 
 
 def mc_apply_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
-                 nthreads=1, phonon_out=False, eps_ph=None):
+                 nthreads=1, phonon_out=False, eps_ph=None, sides=None, zone=None):
     """Electron (and optionally phonon) DG derivative of the e-ph channel."""
     nsec = np.ascontiguousarray(nsec, np.int32)
-    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
     pts = np.ascontiguousarray(pts, np.float64)
     f_el = np.ascontiguousarray(f_el, np.float64)
     f_ph = np.ascontiguousarray(f_ph, np.float64)
@@ -633,7 +649,7 @@ def mc_apply_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx
     out_ph = np.zeros_like(f_ph) if phonon_out else None
     idx = np.ascontiguousarray(idx, np.int32)
     keep, ph = _eps_ph_ptr(eps_ph)
-    lib().or_mc_apply_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+    lib().or_mc_apply_eph(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), ph, omega0,
                           amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out_el,
                           None if out_ph is None else out_ph.ctypes.data, nthreads)
     return (out_el, out_ph) if phonon_out else out_el
@@ -654,21 +670,22 @@ def relative_error_dg(dim, err, pop, stat=-1.0, rtol=1e-8, atol=1e-12, penalty=0
     return lib().or_relative_error_dg(dim, err.shape[0], err, pop, stat, rtol, atol, penalty)
 
 
-def energy_dg(dim, nsec, extent, eps, f):
+def energy_dg(dim, nsec, extent, eps, f, sides=None):
     """int eps f dk of DG coefficients [nel][dim+1] (exact per element)."""
-    return lib().or_energy_dg(dim, np.ascontiguousarray(nsec, np.int32), _pad3(extent, 1.0),
+    geo = zone_geometry(dim, [0.0] * dim, extent, sides, None)
+    return lib().or_energy_dg(dim, np.ascontiguousarray(nsec, np.int32), geo,
                               np.ascontiguousarray(eps, np.float64), np.ascontiguousarray(f, np.float64))
 
 
 def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0, t0,
                t_final, dense_dt, rtol, atol, max_dense=0, nthreads=8, dynamic=False, eps_ph=None,
-               sched=None):
+               sched=None, sides=None, zone=None):
     """DP5(4) of the electron band's DG state with the tuple operator; with
     dynamic=True the state is [electron DG | phonon DG] and bath_n unused.
     sched (BathSchedule): time-dependent bath.  The result holds the bath
     energy ledger ("ledger")."""
     nsec = np.ascontiguousarray(nsec, np.int32)
-    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
     pts = np.ascontiguousarray(pts, np.float64)
     idx = np.ascontiguousarray(idx, np.int32)
     f = np.ascontiguousarray(f0, np.float64).copy()
@@ -678,7 +695,7 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
     keep, ph = _eps_ph_ptr(eps_ph)
     led = C.c_double(0.0)
     sp = None if sched is None else sched.samples
-    lib().or_dp5_mc_eph_ex(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+    lib().or_dp5_mc_eph_ex(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), ph, omega0,
                            amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
                            bath_n, _ptr(sp), 0 if sched is None else sp.size,
                            0.0 if sched is None else sched.t_start,
@@ -695,11 +712,11 @@ def dp5_mc_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, 
 
 def dp5_mc_eph_replay(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, bath_n, f0,
                       t0, t_final, dense_dt, steps, max_dense=0, nthreads=8, dynamic=False,
-                      eps_ph=None, sched=None):
+                      eps_ph=None, sched=None, sides=None, zone=None):
     """Fixed-step DP5 of the tuple operator on the accepted steps `steps`;
     returns the final state, dense outputs and the bath energy ledger."""
     nsec = np.ascontiguousarray(nsec, np.int32)
-    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
     pts = np.ascontiguousarray(pts, np.float64)
     idx = np.ascontiguousarray(idx, np.int32)
     steps = np.ascontiguousarray(steps, np.float64)
@@ -709,7 +726,7 @@ def dp5_mc_eph_replay(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts
     keep, ph = _eps_ph_ptr(eps_ph)
     led = C.c_double(0.0)
     sp = None if sched is None else sched.samples
-    lib().or_dp5_mc_eph_replay(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+    lib().or_dp5_mc_eph_replay(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), ph, omega0,
                                amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], 1 if dynamic else 0,
                                bath_n, _ptr(sp), 0 if sched is None else sp.size,
                                0.0 if sched is None else sched.t_start,
@@ -722,17 +739,17 @@ def dp5_mc_eph_replay(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts
 
 
 def mc_rates_eph(dim, nsec, origin, extent, eps, omega0, amplitude, nu, pts, idx, f_el, f_ph,
-                 nthreads=1, eps_ph=None):
+                 nthreads=1, eps_ph=None, sides=None, zone=None):
     """ScatteringChannel::scatteringRates of the electron band (DG, inverse-massed)."""
     nsec = np.ascontiguousarray(nsec, np.int32)
-    org, ext = _pad3(origin, 0.0), _pad3(extent, 1.0)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
     pts = np.ascontiguousarray(pts, np.float64)
     f_el = np.ascontiguousarray(f_el, np.float64)
     f_ph = np.ascontiguousarray(f_ph, np.float64)
     out = np.zeros_like(f_el)
     idx = np.ascontiguousarray(idx, np.int32)
     keep, ph = _eps_ph_ptr(eps_ph)
-    lib().or_mc_rates_eph(dim, nsec, org, ext, np.ascontiguousarray(eps, np.float64), ph, omega0,
+    lib().or_mc_rates_eph(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), ph, omega0,
                           amplitude, nu, pts, pts.shape[1], idx, idx.shape[0], f_el, f_ph, out,
                           nthreads)
     return out

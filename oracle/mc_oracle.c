@@ -228,8 +228,12 @@ int or_mc_integrate(int dim, int nlegs, const double* D, const double* N, const 
 typedef struct mc_geo {
     int dim;
     const int32_t* nsec;
-    const double* origin;
-    const double* extent;
+    double origin[3];
+    double extent[3];        /* axis-aligned mesh: its box sides */
+    double dG[3][3];         /* dG[d][c]: component d of mesh side c over nsec[c] */
+    double dGinv[3][3];
+    double zone[3][3];       /* zone[d][c]: component d of zone side c (umklapp basis) */
+    int axis;                /* the mesh sides lie along the axes */
     const double* eps;
     const double* eps_ph;    /* phonon nodal energies, or NULL: constant omega0 */
     double omega0;
@@ -239,18 +243,77 @@ typedef struct mc_geo {
     double vol;
 } mc_geo;
 
-static void geo_init(mc_geo* g, int dim, const int32_t* nsec, const double* origin,
-                     const double* extent, const double* eps, const double* eps_ph, double omega0) {
-    g->dim = dim; g->nsec = nsec; g->origin = origin; g->extent = extent; g->eps = eps;
+/* Geometry block geo[21]: origin (3), then the mesh sides (side c at
+ * geo[3 + 3c], Mesh::gVec columns, Mesh.cpp:268-281), then the zone sides
+ * (zone side c at geo[12 + 3c], Region::gVec columns: the umklapp basis,
+ * ScatteringChannel.cpp:330-334).  An axis-aligned box has side c =
+ * extent_c e_c.  Returns the element volume. */
+static double geo_setup(mc_geo* g, int dim, const int32_t* nsec, const double* geo) {
+    g->axis = 1;
+    for (int d = 0; d < 3; ++d) {
+        g->origin[d] = d < dim ? geo[d] : 0.0;
+        g->extent[d] = 1.0;
+        for (int c = 0; c < 3; ++c) {
+            const double sd = (d < dim && c < dim) ? geo[3 + 3 * c + d] : 0.0;
+            g->dG[d][c] = (d < dim && c < dim) ? sd / (double)nsec[c] : 0.0;
+            g->zone[d][c] = (d < dim && c < dim) ? geo[12 + 3 * c + d] : 0.0;
+            g->dGinv[d][c] = 0.0;
+            if (d != c && sd != 0.0) g->axis = 0;
+        }
+    }
+    for (int d = 0; d < dim; ++d) {
+        const double sd = geo[3 + 3 * d + d];
+        if (!(sd > 0.0)) g->axis = 0;
+        g->extent[d] = sd;
+    }
+    double det;
+    if (dim == 1) {
+        det = g->dG[0][0];
+        g->dGinv[0][0] = 1.0 / det;
+    } else if (dim == 2) {
+        det = g->dG[0][0] * g->dG[1][1] - g->dG[0][1] * g->dG[1][0];
+        g->dGinv[0][0] = g->dG[1][1] / det;
+        g->dGinv[0][1] = -g->dG[0][1] / det;
+        g->dGinv[1][0] = -g->dG[1][0] / det;
+        g->dGinv[1][1] = g->dG[0][0] / det;
+    } else {
+        double A[9], Ai[9];
+        for (int d = 0; d < 3; ++d)
+            for (int c = 0; c < 3; ++c) A[3 * d + c] = g->dG[d][c];
+        det = or_mc_invert(3, A, Ai);
+        for (int d = 0; d < 3; ++d)
+            for (int c = 0; c < 3; ++c) g->dGinv[d][c] = Ai[3 * d + c];
+    }
+    /* element volume: the box product, else |det(section sides)| / dim! */
+    if (g->axis) return or_element_volume(dim, nsec, g->extent);
+    return fabs(det) / (double)mc_fact(dim);
+}
+
+static void geo_init(mc_geo* g, int dim, const int32_t* nsec, const double* geo, const double* eps,
+                     const double* eps_ph, double omega0) {
+    g->dim = dim; g->nsec = nsec; g->eps = eps;
     g->eps_ph = eps_ph;
     g->omega0 = omega0;
+    g->vol = geo_setup(g, dim, nsec, geo);
     g->nel = or_num_elements(dim, nsec);
     int64_t n = or_num_nodes(dim, nsec);
     g->conn = (int32_t*)malloc(sizeof(int32_t) * g->nel * (dim + 1));
     g->nodes = (double*)malloc(sizeof(double) * n * dim);
     or_mesh_elements(dim, nsec, g->conn);
-    or_mesh_nodes(dim, nsec, origin, extent, g->nodes);
-    g->vol = or_element_volume(dim, nsec, extent);
+    /* node coordinate d: origin + sum_c dG[d][c] i_c in axis order (on an
+     * axis-aligned mesh the other axes add exact zeros) */
+    for (int64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < dim; ++d) {
+            int64_t r = i;
+            double x = g->origin[d];
+            for (int c = 0; c < dim; ++c) {
+                const int64_t id = r % (nsec[c] + 1);
+                r /= (nsec[c] + 1);
+                x = x + g->dG[d][c] * (double)id;
+            }
+            g->nodes[i * dim + d] = x;
+        }
+    }
 }
 static void geo_free(mc_geo* g) { free(g->conn); free(g->nodes); }
 
@@ -296,7 +359,9 @@ static void tuple_DN(const mc_geo* g, const int64_t* el, const int* u, double* D
         for (int a = 0; a < dim; ++a) D[dim * ncol + leg * dim + a] = dir[leg] * ef[1 + a];
         N[dim] -= dir[leg] * ef[0];
     }
-    for (int d = 0; d < dim; ++d) N[d] -= (double)u[d] * g->extent[d];
+    /* N_d -= sum_c u_c zone[d][c], one term at a time (ScatteringChannel.cpp:330-334) */
+    for (int d = 0; d < dim; ++d)
+        for (int c = 0; c < dim; ++c) N[d] -= (double)u[c] * g->zone[d][c];
 }
 
 static void elem_box(const mc_geo* g, int64_t e, double* lo, double* hi) {
@@ -354,12 +419,12 @@ static double tuple_screen(const mc_geo* g, const double* D, const double* N, co
  * |det D_C| >= min_det and the screening value exceeds `threshold`
  * (ScatteringChannel.cpp:474-503).  idx[t*4 + {0,1,2,3}] = e0, e1, e2, uid.
  * Returns the tuple count (idx may be NULL to count; at most cap written). */
-int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* geo,
                          const double* eps, const double* eps_ph, double omega0, int32_t nu,
                          const double* pts, int32_t nmc, double min_det, double threshold,
                          int32_t* idx, int64_t cap) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
+    geo_init(&g, dim, nsec, geo, eps, eps_ph, omega0);
     const int64_t nel = g.nel;
     const int nes = mc_fact(dim);
     int32_t nuv = 1;
@@ -403,11 +468,42 @@ int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin, con
                     /* k1 = k2 - k0 - G u */
                     double lo[3], hi[3];
                     int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, empty = 0;
+                    /* sections that can hold k1: on an axis-aligned mesh the
+                     * box's own section range, else the reduced coordinates of
+                     * its corners, one section wider (a superset) */
+                    double smin[3], smax[3];
                     for (int d = 0; d < dim; ++d) {
-                        lo[d] = b2[d] - b0[dim + d] - u[d] * extent[d];
-                        hi[d] = b2[dim + d] - b0[d] - u[d] * extent[d];
-                        double h = extent[d] / nsec[d];
-                        double a = floor((lo[d] - origin[d]) / h), b = floor((hi[d] - origin[d]) / h);
+                        double U = 0.0;
+                        for (int c = 0; c < dim; ++c) U = U + (double)u[c] * g.zone[d][c];
+                        lo[d] = b2[d] - b0[dim + d] - U;
+                        hi[d] = b2[dim + d] - b0[d] - U;
+                        if (g.axis) {
+                            double h = g.extent[d] / nsec[d];
+                            smin[d] = floor((lo[d] - g.origin[d]) / h);
+                            smax[d] = floor((hi[d] - g.origin[d]) / h);
+                        } else {
+                            smin[d] = 1e300;
+                            smax[d] = -1e300;
+                        }
+                    }
+                    if (!g.axis) {
+                        for (int cm = 0; cm < (1 << dim); ++cm) {
+                            double x[3];
+                            for (int d = 0; d < dim; ++d) x[d] = ((cm >> d) & 1 ? hi[d] : lo[d]) - g.origin[d];
+                            for (int c = 0; c < dim; ++c) {
+                                double sr = 0.0;
+                                for (int d = 0; d < dim; ++d) sr = sr + g.dGinv[c][d] * x[d];
+                                if (sr < smin[c]) smin[c] = sr;
+                                if (sr > smax[c]) smax[c] = sr;
+                            }
+                        }
+                        for (int c = 0; c < dim; ++c) {
+                            smin[c] = floor(smin[c]) - 1.0;
+                            smax[c] = floor(smax[c]) + 1.0;
+                        }
+                    }
+                    for (int d = 0; d < dim; ++d) {
+                        double a = smin[d], b = smax[d];
                         if (b < 0.0 || a > nsec[d] - 1) { empty = 1; break; }
                         slo[d] = a < 0.0 ? 0 : (int)a;
                         shi[d] = b > nsec[d] - 1 ? nsec[d] - 1 : (int)b;
@@ -489,13 +585,13 @@ void or_mc_inverse_mass(int dim, double vol, double* minv) {
  *   time-reversed (1 - f(k0)) * (1 + n(k1)) * f(k2)
  * and leg l receives dir_l * result_l.  Threads accumulate disjoint tuple
  * ranges; their sums are added in thread order. */
-void or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void or_mc_apply_eph(int dim, const int32_t* nsec, const double* geo,
                      const double* eps, const double* eps_ph, double omega0, double amplitude,
                      int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
                      const double* f_el, const double* f_ph, double* out_el, double* out_ph,
                      int32_t nthreads) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
+    geo_init(&g, dim, nsec, geo, eps, eps_ph, omega0);
     const int nc = dim + 1;
     const int64_t nel = g.nel, len = nel * nc;
     int nt = nthreads > 0 ? nthreads : 1;
@@ -611,7 +707,7 @@ typedef struct mc_dp5_user {
     int dim;
     int dynamic;             /* 1: the state holds the phonon band after the electron band */
     const int32_t* nsec;
-    const double *origin, *extent, *eps, *eps_ph, *pts;
+    const double *geo, *eps, *eps_ph, *pts;
     double omega0, amplitude, rtol, atol;
     int32_t nu, nmc, nthreads;
     const int32_t* idx;
@@ -623,7 +719,7 @@ typedef struct mc_dp5_user {
 /* int eps f over the mesh for DG coefficients f: per element the exact
  * P1 x P1 integral vol/((d+1)(d+2)) (sum_v eps_v f_v + sum_v eps_v sum_v f_v)
  * with the element-vertex values (eps at the element's nodes) */
-double or_energy_dg(int dim, const int32_t* nsec, const double* extent, const double* eps,
+double or_energy_dg(int dim, const int32_t* nsec, const double* geo, const double* eps,
                     const double* f) {
     const int64_t nel = or_num_elements(dim, nsec);
     const int nc = dim + 1;
@@ -642,12 +738,13 @@ double or_energy_dg(int dim, const int32_t* nsec, const double* extent, const do
         tot += sei + se * si;
     }
     free(conn);
-    return tot * or_element_volume(dim, nsec, extent) / (double)((dim + 1) * (dim + 2));
+    mc_geo g;
+    return tot * geo_setup(&g, dim, nsec, geo) / (double)((dim + 1) * (dim + 2));
 }
 
 static double mc_dp5_flux(void* u, const double* dydt) {
     const mc_dp5_user* p = (const mc_dp5_user*)u;
-    return or_energy_dg(p->dim, p->nsec, p->extent, p->eps, dydt);
+    return or_energy_dg(p->dim, p->nsec, p->geo, p->eps, dydt);
 }
 
 static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
@@ -658,11 +755,11 @@ static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
         for (int64_t e = 0; e < p->nel; ++e) p->fph[e * (p->dim + 1)] = nb;
     }
     if (p->dynamic)
-        or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->eps_ph, p->omega0,
+        or_mc_apply_eph(p->dim, p->nsec, p->geo, p->eps, p->eps_ph, p->omega0,
                         p->amplitude, p->nu, p->pts, p->nmc, p->idx, p->ntup, y, y + len, dydt,
                         dydt + len, p->nthreads);
     else
-        or_mc_apply_eph(p->dim, p->nsec, p->origin, p->extent, p->eps, p->eps_ph, p->omega0,
+        or_mc_apply_eph(p->dim, p->nsec, p->geo, p->eps, p->eps_ph, p->omega0,
                         p->amplitude, p->nu, p->pts, p->nmc, p->idx, p->ntup, y, p->fph, dydt, NULL,
                         p->nthreads);
 }
@@ -684,18 +781,18 @@ static double mc_dp5_norm(void* u, const double* err, const double* y5) {
  * tuple operator: DP5(4) over the electron band's DG coefficients, the phonon
  * band a bath of occupation bath_n (its constraint makes its error zero),
  * fermion error norm without penalty (Band.hpp default). */
-void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void or_dp5_mc_eph(int dim, const int32_t* nsec, const double* geo,
                    const double* eps, const double* eps_ph, double omega0, double amplitude,
                    int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
                    int32_t dynamic, double bath_n, double* f, double t0, double t_final,
                    double dense_dt, double rtol, double atol, double* dense_out, double* dense_t,
                    int32_t max_dense, int32_t nthreads, or_dp5_stats* stats) {
-    or_dp5_mc_eph_ex(dim, nsec, origin, extent, eps, eps_ph, omega0, amplitude, nu, pts, nmc, idx, ntup,
+    or_dp5_mc_eph_ex(dim, nsec, geo, eps, eps_ph, omega0, amplitude, nu, pts, nmc, idx, ntup,
                      dynamic, bath_n, NULL, 0, 0.0, 1.0, 0.0, 0.0, f, t0, t_final, dense_dt, rtol, atol,
                      dense_out, dense_t, max_dense, nthreads, stats, NULL);
 }
 
-void or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* geo,
                       const double* eps, const double* eps_ph, double omega0, double amplitude,
                       int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                       int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
@@ -706,7 +803,7 @@ void or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const 
     mc_dp5_user u;
     memset(&u, 0, sizeof(u));
     or_bath_sched bs = {sched, ns, t_start, sdt, satol, srtol};
-    u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
+    u.dim = dim; u.nsec = nsec; u.geo = geo; u.eps = eps; u.pts = pts;
     u.eps_ph = eps_ph; u.dynamic = dynamic;
     u.omega0 = omega0; u.amplitude = amplitude; u.rtol = rtol; u.atol = atol;
     u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
@@ -722,7 +819,7 @@ void or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const 
     free(u.fph);
 }
 
-void or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* geo,
                           const double* eps, const double* eps_ph, double omega0, double amplitude,
                           int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                           int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
@@ -733,7 +830,7 @@ void or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, co
     mc_dp5_user u;
     memset(&u, 0, sizeof(u));
     or_bath_sched bs = {sched, ns, t_start, sdt, 0.0, 0.0};
-    u.dim = dim; u.nsec = nsec; u.origin = origin; u.extent = extent; u.eps = eps; u.pts = pts;
+    u.dim = dim; u.nsec = nsec; u.geo = geo; u.eps = eps; u.pts = pts;
     u.eps_ph = eps_ph; u.dynamic = dynamic;
     u.omega0 = omega0; u.amplitude = amplitude;
     u.nu = nu; u.nmc = nmc; u.nthreads = nthreads; u.idx = idx; u.ntup = ntup;
@@ -759,12 +856,12 @@ void or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, co
  *   time-reversed: ((1+dir) stat + (1-dir)) * 1
  * keeps the others as in the propagator, and subtracts dir * result of that
  * leg; the legs' inverse-massed results are summed. */
-void or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void or_mc_rates_eph(int dim, const int32_t* nsec, const double* geo,
                      const double* eps, const double* eps_ph, double omega0, double amplitude,
                      int32_t nu, const double* pts, int32_t nmc, const int32_t* idx, int64_t ntup,
                      const double* f_el, const double* f_ph, double* out_el, int32_t nthreads) {
     mc_geo g;
-    geo_init(&g, dim, nsec, origin, extent, eps, eps_ph, omega0);
+    geo_init(&g, dim, nsec, geo, eps, eps_ph, omega0);
     const int nc = dim + 1;
     const int64_t nel = g.nel, len = nel * nc;
     int nt = nthreads > 0 ? nthreads : 1;

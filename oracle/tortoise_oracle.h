@@ -273,16 +273,18 @@ double  or_mc_invert(int n, const double* A, double* inv);
 int     or_mc_integrate(int dim, int nlegs, const double* D, const double* N,
                         const double* fa, const double* fb, const double* pts,
                         int32_t nmc, double* out);
+/* The tuple-operator functions below take the zone as a geometry block
+ * geo[21]: origin (3), the mesh sides (side c at geo[3 + 3c]; node index i_c
+ * along side c at origin + sum_c (i_c / nsec[c]) side c), the zone sides
+ * (zone side c at geo[12 + 3c]; umklapp vectors sum_c u_c zone side c). */
 /* Tuples (e0, e1, e2, umklapp id) of the e-ph channel {el in, pn in, el out}. */
-int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* origin,
-                         const double* extent, const double* eps, const double* eps_ph,
+int64_t or_mc_tuples_eph(int dim, const int32_t* nsec, const double* geo, const double* eps, const double* eps_ph,
                          double omega0, int32_t nu, const double* pts, int32_t nmc,
                          double min_det, double threshold, int32_t* idx, int64_t cap);
 /* LF-basis inverse mass matrix of one element. */
 void    or_mc_inverse_mass(int dim, double vol, double* minv);
 /* Propagator of the e-ph channel over a tuple list, then the inverse mass. */
-void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin,
-                        const double* extent, const double* eps, const double* eps_ph,
+void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* geo, const double* eps, const double* eps_ph,
                         double omega0, double amplitude, int32_t nu, const double* pts,
                         int32_t nmc, const int32_t* idx, int64_t ntup, const double* f_el,
                         const double* f_ph, double* out_el, double* out_ph,
@@ -291,7 +293,7 @@ void    or_mc_apply_eph(int dim, const int32_t* nsec, const double* origin,
 double  or_relative_error_dg(int dim, int64_t nel, const double* err, const double* pop,
                              double stat, double rtol, double atol, double penalty);
 /* DP5(4) of the electron band's DG coefficients with the tuple operator. */
-void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* geo,
                       const double* eps, const double* eps_ph, double omega0, double amplitude,
                       int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                       int64_t ntup, int32_t dynamic, double bath_n, double* f, double t0,
@@ -300,7 +302,7 @@ void    or_dp5_mc_eph(int dim, const int32_t* nsec, const double* origin, const 
                       int32_t max_dense, int32_t nthreads, or_dp5_stats* stats);
 /* or_dp5_mc_eph with a time-dependent bath (sched NULL: constant bath_n) and
  * the bath energy ledger (-int eps I dt of the electron band; 0 when dynamic). */
-void    or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void    or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* geo,
                          const double* eps, const double* eps_ph, double omega0, double amplitude,
                          int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                          int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
@@ -310,7 +312,7 @@ void    or_dp5_mc_eph_ex(int dim, const int32_t* nsec, const double* origin, con
                          int32_t max_dense, int32_t nthreads, or_dp5_stats* stats, double* ledger);
 /* or_dp5_mc_eph_ex on a given step sequence (all accepted): the tuple
  * operator's fixed-step replay with the ledger. */
-void    or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void    or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* geo,
                              const double* eps, const double* eps_ph, double omega0, double amplitude,
                              int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                              int64_t ntup, int32_t dynamic, double bath_n, const double* sched,
@@ -322,7 +324,7 @@ void    or_dp5_mc_eph_replay(int dim, const int32_t* nsec, const double* origin,
 double  or_energy_dg(int dim, const int32_t* nsec, const double* extent, const double* eps,
                      const double* f);
 /* ScatteringChannel::scatteringRates of the electron band (legs 0 and 2). */
-void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* origin, const double* extent,
+void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* geo,
                         const double* eps, const double* eps_ph, double omega0, double amplitude,
                         int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                         int64_t ntup, const double* f_el, const double* f_ph, double* out_el,

@@ -74,6 +74,30 @@ int main() {
     }
     CHECK(threw);
     mat["el:C"].constrained = false;
+    // the hexagonal zone of the reference's example (examples/main.cpp:37-39):
+    // a skewed region, a sub-mesh of it, node positions, element volume, and
+    // the W-form backend's refusal (it meshes axis-aligned zones)
+    const Real h3 = std::sqrt(3.0) / 2.0;
+    Region<2> hex(Point<2>({0.0, 0.0}), {Point<2>({1.0, 0.0}), Point<2>({-0.5, h3})});
+    CHECK(!hex.axisAligned());
+    Mesh<2> hm(hex, Point<2>({0.1, 0.1}), Point<2>({0.3, 0.3}), {3, 3});
+    CHECK(!hm.axisAligned);
+    // node (i, j) = origin + (i/3) side0 + (j/3) side1, origin = 0.1 (g0 + g1), side_a = 0.3 g_a
+    const Point<2> o = hm.node(0), p = hm.node(2 + 4 * 1);
+    CHECK(std::fabs(o(0) - 0.1 * (1.0 - 0.5)) < 1e-15 && std::fabs(o(1) - 0.1 * h3) < 1e-15);
+    CHECK(std::fabs(p(0) - (o(0) + 0.2 * 1.0 + 0.1 * -0.5)) < 1e-15 && std::fabs(p(1) - (o(1) + 0.1 * h3)) < 1e-15);
+    CHECK(std::fabs(hm.elemVolume - 0.09 * h3 / 9.0 / 2.0) < 1e-16);   // |det(0.3 g)| / 9 / 2
+    // the integral of 1 over the sub-mesh is its area
+    CHECK(std::fabs(Function<2>(hm, 1.0).integrate() - 0.09 * h3) < 1e-14);
+    Material<2> hmat("hex", hex);
+    hmat.addBand("el:C", -1, fermion, hm, [](Point<2> k) { return 0.5 * k.squaredNorm(); });
+    hmat.addBand("pn:o", 0, boson, hm, 0.05);
+    hmat.addScatteringChannelsWildcards<3>({"el:*", "pn:*", "el:*"}, {in, in, out}, 1);
+    threw = false;
+    try { (void)hmat.backend(0.1, 1); } catch (const std::runtime_error& e) {
+        threw = std::string(e.what()).find("axis-aligned") != std::string::npos;
+    }
+    CHECK(threw);
     if (fails == 0) std::printf("ok\n");
     return fails ? 1 : 0;
 }

@@ -73,6 +73,47 @@ def test_tuple_list_and_propagate_match_oracle(dim, ns):
     op.close()
 
 
+HEX_SIDES = np.array([[1.0, 0.0], [-0.5, np.sqrt(3.0) / 2.0]])   # examples/main.cpp:37-39
+
+
+@pytest.mark.parametrize("ns,nu", [(12, 1), (16, 2)])
+def test_hexagonal_zone_matches_oracle(ns, nu):
+    """The hexagonal zone of the reference's propagation example (a Region
+    from the origin with sides (1, 0) and (-1/2, sqrt(3)/2), meshed whole,
+    umklapp vectors over its sides up to the second neighbouring zones as the
+    example's channels use): the GPU's tuple list is the oracle's and one
+    evaluation agrees to 1e-12 of max|out|; number balances to 1e-13."""
+    print(f"[gpu] tuple operator hexagonal zone ns={ns} nu={nu}", flush=True)
+    w = wl.Workload(f"hex{ns}", 2, (ns, ns))
+    i0, i1 = np.meshgrid(np.arange(ns + 1), np.arange(ns + 1), indexing="xy")
+    X = np.outer(i0.ravel() / ns, HEX_SIDES[0]) + np.outer(i1.ravel() / ns, HEX_SIDES[1])
+    c = 0.5 * (HEX_SIDES[0] + HEX_SIDES[1])
+    eps = 1.2 * ((X - c) ** 2).sum(1)
+    pts = oracle.mc_points(2, 3, 50, 7)
+    idx_o = oracle.mc_tuples_eph(2, w.nsec, (0.0, 0.0), None, eps, w.omega0, nu, pts, sides=HEX_SIDES)
+    op = tg.TupleOperator.create(w, n_umklapp=nu, max_batch=2, eps=eps, origin=(0.0, 0.0), sides=HEX_SIDES)
+    op.set_points(pts)
+    idx_g = op.indices()
+    assert idx_g.shape == idx_o.shape and np.array_equal(idx_g, idx_o)
+    rng = np.random.default_rng(ns)
+    f = 0.5 + 0.4 * np.tanh((0.4 - eps) / 0.1)
+    base = oracle.mc_nodal_to_dg(2, w.nsec, f)
+    F = np.stack([base + 0.01 * rng.standard_normal(base.shape) for _ in range(2)])
+    bath = np.array([0.08, 0.15])
+    out = op.propagate(F, bath)
+    vol = abs(np.linalg.det(HEX_SIDES)) / (ns * ns) / 2.0
+    M = np.linalg.inv(oracle.mc_inverse_mass(2, vol))
+    for b in range(2):
+        fph = np.zeros_like(F[b])
+        fph[:, 0] = bath[b]
+        ref = oracle.mc_apply_eph(2, w.nsec, (0.0, 0.0), None, eps, w.omega0, w.g ** 2, nu, pts, idx_o,
+                                  F[b], fph, nthreads=8, sides=HEX_SIDES)
+        assert np.abs(out[b] - ref).max() <= 1e-12 * np.abs(ref).max(), b
+        num = out[b] @ M[:, 0]
+        assert abs(num.sum()) <= 1e-13 * np.abs(num).sum()
+    op.close()
+
+
 def test_tuple_config0_square32():
     """The reference's own CPU-runnable case: 32 x 32 sections (2048
     triangles), NMC = 50, umklapp {-1,0,1}^2."""

@@ -145,14 +145,19 @@ def _mesh(ns, dim=2):
     return w, w.eps()
 
 
-def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9, eps_ph=None):
+def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9, eps_ph=None, X=None, vol=None,
+                  zone=None):
     """Every (e0, e1, e2, u) of a small 2D mesh, D and N written out here
-    (phonon band: nodal energies eps_ph, or the constant omega0)."""
+    (phonon band: nodal energies eps_ph, or the constant omega0).  X, vol,
+    zone: the node coordinates, element volume and zone sides (rows) of a
+    general mesh; by default the workload's box."""
     dim = 2
-    X = oracle.mesh_nodes(dim, list(w.nsec), list(w.origin), list(w.extent))
+    if X is None:
+        X = oracle.mesh_nodes(dim, list(w.nsec), list(w.origin), list(w.extent))
+        vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
+        zone = np.diag(np.array(w.extent, np.float64))
     conn = oracle.mesh_elements(dim, list(w.nsec))
     nel = conn.shape[0]
-    vol = abs(np.prod(np.array(w.extent) / np.array(w.nsec))) / 2
     span = 2 * nu + 1
     us = [(i - nu, j - nu) for i in range(span) for j in range(span)]
     kf = np.stack([X[conn[:, 1]] - X[conn[:, 0]], X[conn[:, 2]] - X[conn[:, 0]]], axis=2)  # [e][d][a]
@@ -185,7 +190,7 @@ def _brute_tuples(w, eps, nu, pts, min_det=1e-9, threshold=1e-9, eps_ph=None):
                     D[2, 2:4] = pf[b]
                     D[2, 4:6] = -ef[c]
                     N = np.zeros(3)
-                    N[:2] = -(k0[a] + k0[b] - k0[c]) - np.array(u) * np.array(w.extent)
+                    N[:2] = -(k0[a] + k0[b] - k0[c]) - (u[0] * zone[0] + u[1] * zone[1])
                     N[2] = -(e0v[a] + p0[b] - e0v[c])
                     DC = D[:, 3:6]
                     if abs(np.linalg.det(DC)) < min_det:
@@ -207,6 +212,93 @@ def test_tuples_match_brute_force_small_mesh():
     # ordered by (e0, e2, umklapp id, e1)
     key = idx[:, 0].astype(np.int64) * 10 ** 9 + idx[:, 2] * 10 ** 5 + idx[:, 3] * 10 ** 3 + idx[:, 1]
     assert (np.diff(key) > 0).all()
+
+
+# the hexagonal zone of the reference's propagation example (examples/main.cpp:37-39):
+# a Region from the origin with sides (1, 0) and (-1/2, sqrt(3)/2)
+HEX_SIDES = np.array([[1.0, 0.0], [-0.5, math.sqrt(3.0) / 2.0]])
+
+
+def _zone_nodes(nsec, origin, sides):
+    """Node coordinates of the mesh origin + sum_c (i_c / nsec_c) side_c
+    (index 0 fastest, the reference's vertex numbering)."""
+    i0, i1 = np.meshgrid(np.arange(nsec[0] + 1), np.arange(nsec[1] + 1), indexing="xy")
+    i0, i1 = i0.ravel(), i1.ravel()
+    return np.asarray(origin) + np.outer(i0 / nsec[0], sides[0]) + np.outer(i1 / nsec[1], sides[1])
+
+
+def _hex_case(ns=4):
+    nsec = (ns, ns)
+    X = _zone_nodes(nsec, (0.0, 0.0), HEX_SIDES)
+    c = 0.5 * (HEX_SIDES[0] + HEX_SIDES[1])
+    eps = 1.2 * ((X - c) ** 2).sum(1)
+    w = wl.Workload("hex", 2, nsec)
+    vol = abs(np.linalg.det(HEX_SIDES)) / (ns * ns) / 2.0
+    return w, nsec, X, eps, vol
+
+
+def test_tuples_match_brute_force_hexagonal_zone():
+    """The whole hexagonal zone meshed with 4 x 4 sections (skewed
+    triangles, umklapp vectors along the zone's sides): the list equals brute
+    force over every element triple and umklapp vector, D and N built here
+    from the skewed vertices."""
+    w, nsec, X, eps, vol = _hex_case(4)
+    pts = oracle.mc_points(2, 3, 50, seed=0)
+    idx = oracle.mc_tuples_eph(2, nsec, (0.0, 0.0), None, eps, w.omega0, 1, pts, sides=HEX_SIDES)
+    got = set(map(tuple, idx.tolist()))
+    assert len(got) == len(idx) and len(got) > 100
+    brute = _brute_tuples(w, eps, 1, pts, X=X, vol=vol, zone=HEX_SIDES)
+    assert got == brute
+
+
+def test_hexagonal_zone_operator_balances_number_and_energy():
+    """Number and electron + phonon energy of the operator's DG output are
+    zero to roundoff on the skewed mesh (every tuple moves one electron and
+    one phonon of energy omega0)."""
+    w, nsec, X, eps, vol = _hex_case(8)
+    pts = oracle.mc_points(2, 3, 50, seed=1)
+    idx = oracle.mc_tuples_eph(2, nsec, (0.0, 0.0), None, eps, w.omega0, 1, pts, sides=HEX_SIDES)
+    conn = oracle.mesh_elements(2, list(nsec))
+    rng = np.random.default_rng(3)
+    f = 0.5 + 0.4 * np.tanh((0.4 - eps) / 0.1)
+    fdg = oracle.mc_nodal_to_dg(2, nsec, f) + 0.01 * rng.standard_normal((conn.shape[0], 3))
+    fph = np.zeros_like(fdg)
+    fph[:, 0] = 0.09
+    el, ph = oracle.mc_apply_eph(2, nsec, (0.0, 0.0), None, eps, w.omega0, w.g ** 2, 1, pts, idx, fdg,
+                                 fph, phonon_out=True, sides=HEX_SIDES)
+    M = np.linalg.inv(oracle.mc_inverse_mass(2, vol))
+    one = np.zeros_like(fdg)
+    one[:, 0] = 1.0
+    ef = oracle.mc_nodal_to_dg(2, nsec, eps)          # the P1 dispersion as DG forms
+    n_el = np.einsum("ea,ab,eb->", one, M, el)
+    n_scale = np.einsum("ea,ab,eb->", one, M, np.abs(el))
+    assert abs(n_el) <= 1e-12 * n_scale
+    e_el = np.einsum("ea,ab,eb->", ef, M, el) + w.omega0 * np.einsum("ea,ab,eb->", one, M, ph)
+    e_scale = np.einsum("ea,ab,eb->", np.abs(ef), M, np.abs(el))
+    assert abs(e_el) <= 1e-12 * e_scale
+
+
+def test_rotated_square_zone_gives_the_box_operator():
+    """Rotating the square zone and its isotropic band about the band's
+    centre moves every vertex but changes no physics: the skewed path gives
+    the axis-aligned path's tuple list and operator."""
+    w, eps = _mesh(8)
+    th = 0.37
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    sides = (R @ np.diag(np.array(w.extent, np.float64))).T     # row c: R (extent_c e_c)
+    origin = R @ np.array(w.origin, np.float64)
+    pts = oracle.mc_points(2, 3, 50, seed=2)
+    idx0 = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    idx1 = oracle.mc_tuples_eph(2, w.nsec, origin, None, eps, w.omega0, 1, pts, sides=sides)
+    assert set(map(tuple, idx0.tolist())) == set(map(tuple, idx1.tolist()))
+    fdg = oracle.mc_nodal_to_dg(2, w.nsec, w.initial_f())
+    fph = np.zeros_like(fdg)
+    fph[:, 0] = 0.0894
+    el0 = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx0,
+                              fdg, fph)
+    el1 = oracle.mc_apply_eph(2, w.nsec, origin, None, eps, w.omega0, w.g ** 2, 1, pts, idx1, fdg,
+                              fph, sides=sides)
+    assert np.linalg.norm(el1 - el0) <= 1e-11 * np.linalg.norm(el0)
 
 
 def _operator_case(ns=8, seed=0, nthreads=1, amp=None):

@@ -119,7 +119,8 @@ class TupleDesc(C.Structure):
                 ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("omega0", C.c_double),
                 ("amplitude", C.c_double), ("n_umklapp", C.c_int32), ("num_mc", C.c_int32),
                 ("min_det", C.c_double), ("threshold", C.c_double), ("max_batch", C.c_int32),
-                ("dynamic_phonons", C.c_int32), ("eps_ph", C.POINTER(C.c_double))]
+                ("dynamic_phonons", C.c_int32), ("eps_ph", C.POINTER(C.c_double)),
+                ("sides", (C.c_double * 3) * 3), ("zone", (C.c_double * 3) * 3)]
 
 
 _lib = None
@@ -492,7 +493,12 @@ class TupleOperator:
 
     @classmethod
     def create(cls, workload, n_umklapp=1, num_mc=50, min_det=1e-9, threshold=1e-9,
-               max_batch=1, amplitude=None, eps=None, eps_ph=None, dynamic=False):
+               max_batch=1, amplitude=None, eps=None, eps_ph=None, dynamic=False, origin=None,
+               sides=None, zone=None):
+        """sides (dim x dim, row c = side c) with origin: a general
+        parallelepiped mesh (the reference's Mesh over a Region); zone (row c
+        = zone side c): the umklapp basis, by default the mesh's sides.
+        Without sides: the workload's axis-aligned box."""
         L = load()
         w = workload
         e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
@@ -502,6 +508,14 @@ class TupleOperator:
             d.nsec[q] = w.nsec[q] if q < w.dim else 1
             d.origin[q] = w.origin[q] if q < w.dim else 0.0
             d.extent[q] = w.extent[q] if q < w.dim else 1.0
+        if sides is not None:
+            S = np.asarray(sides, np.float64)
+            Z = S if zone is None else np.asarray(zone, np.float64)
+            for c in range(w.dim):
+                d.origin[c] = float(origin[c]) if origin is not None else w.origin[c]
+                for q in range(w.dim):
+                    d.sides[c][q] = S[c][q]
+                    d.zone[c][q] = Z[c][q]
         d.eps = e.ctypes.data_as(C.POINTER(C.c_double))
         d.omega0 = w.omega0
         d.amplitude = w.g ** 2 if amplitude is None else amplitude

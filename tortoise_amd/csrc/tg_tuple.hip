@@ -53,6 +53,9 @@ struct tg_tuple_ctx {
     int32_t nsec[3] = {1, 1, 1};
     double origin[3] = {0, 0, 0};
     double extent[3] = {1, 1, 1};
+    double dG[3][3] = {}, dGinv[3][3] = {}, zone[3][3] = {};   // Geo's affine mesh and zone
+    int32_t axis = 1;
+    double gen_vol = 0;         // |det dG| / dim!
     int64_t n = 0, nel = 0;
     double vol = 0;
     double omega0 = 0, amplitude = 1;
@@ -149,6 +152,11 @@ tg_status dalloc(tg_tuple_ctx* c, T** p, size_t count) {
 struct Geo {
     int32_t nsec[3];
     double origin[3], extent[3];
+    // affine mesh: node index i at origin + sum_c dG[.][c] i_c (dG[d][c]:
+    // component d of side c over nsec[c]); umklapp u at sum_c u_c zone[.][c];
+    // axis: sides along the axes (the box origin + extent, dG diagonal)
+    double dG[3][3], dGinv[3][3], zone[3][3];
+    int32_t axis;
     double omega0, vol;
     double pmin, pmax;    // phonon energy range over the zone
     int32_t nu, nuv, nmc, rows;
@@ -172,12 +180,27 @@ constexpr int etab_ph() { return DIM * (DIM + 1) + (DIM + 1) + 2 * DIM + 2; }
 // order of the CPU restatement.
 #pragma clang fp contract(off)
 
+// coordinate d of a node: origin + sum_c dG[d][c] i_c, the terms added in
+// axis order (on an axis-aligned mesh the other axes add exact zeros: the
+// box value origin + (extent / nsec) i)
 template <int DIM>
 __host__ __device__ inline double node_coord(const Geo& g, int64_t node, int d) {
     int64_t r = node;
-    for (int q = 0; q < d; ++q) r /= (g.nsec[q] + 1);
-    int64_t id = r % (g.nsec[d] + 1);
-    return g.origin[d] + (g.extent[d] / (double)g.nsec[d]) * (double)id;
+    double x = g.origin[d];
+    for (int c = 0; c < DIM; ++c) {
+        const int64_t id = r % (g.nsec[c] + 1);
+        r /= (g.nsec[c] + 1);
+        x = x + g.dG[d][c] * (double)id;
+    }
+    return x;
+}
+
+// the umklapp vector's component d: sum_c u_c zone[d][c]
+template <int DIM>
+__host__ __device__ inline double umklapp_comp(const Geo& g, const int u[3], int d) {
+    double s = 0.0;
+    for (int c = 0; c < DIM; ++c) s = s + (double)u[c] * g.zone[d][c];
+    return s;
 }
 
 template <int DIM>
@@ -253,8 +276,11 @@ __device__ inline void tuple_DN(const Geo& g, const int64_t el[3], const int u[3
         for (int a = 0; a < DIM; ++a) D[DIM][leg * DIM + a] = dir[leg] * ef[1 + a];
         N[DIM] -= dir[leg] * ef[0];
     }
+    // N_d -= sum_c u_c zone[d][c], one term at a time (ScatteringChannel.cpp:330-334)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) N[d] -= (double)u[d] * g.extent[d];
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) N[d] -= (double)u[c] * g.zone[d][c];
 }
 
 // Inverse of the NC x NC matrix by Gauss-Jordan with partial pivoting;
@@ -490,12 +516,48 @@ __device__ inline int32_t pair_tuples(const Geo& g, int64_t e0, int64_t e2, doub
         double lo[DIM], hi[DIM];
         int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
         bool empty = false;
+        // k1 = k2 - k0 - G u lies in the box [lo, hi]; the sections that can
+        // hold it: on an axis-aligned mesh the box's own section range, else
+        // the reduced coordinates of the box's corners, one section wider
+        // (a superset: the element boxes and the tuple tests decide)
+        double smin[3], smax[3];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            lo[d] = lo2[d] - hi0[d] - u[d] * g.extent[d];
-            hi[d] = hi2[d] - lo0[d] - u[d] * g.extent[d];
-            double h = g.extent[d] / g.nsec[d];
-            double a = floor((lo[d] - g.origin[d]) / h), b = floor((hi[d] - g.origin[d]) / h);
+            const double U = umklapp_comp<DIM>(g, u, d);
+            lo[d] = lo2[d] - hi0[d] - U;
+            hi[d] = hi2[d] - lo0[d] - U;
+            if (g.axis) {
+                double h = g.extent[d] / g.nsec[d];
+                smin[d] = floor((lo[d] - g.origin[d]) / h);
+                smax[d] = floor((hi[d] - g.origin[d]) / h);
+            } else {
+                smin[d] = 1e300;
+                smax[d] = -1e300;
+            }
+        }
+        if (!g.axis) {
+            for (int cm = 0; cm < (1 << DIM); ++cm) {
+                double x[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = ((cm >> d) & 1 ? hi[d] : lo[d]) - g.origin[d];
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) s = s + g.dGinv[c][d] * x[d];
+                    smin[c] = s < smin[c] ? s : smin[c];
+                    smax[c] = s > smax[c] ? s : smax[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                smin[c] = floor(smin[c]) - 1.0;
+                smax[c] = floor(smax[c]) + 1.0;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double a = smin[d], b = smax[d];
             empty |= b < 0.0 || a > g.nsec[d] - 1;
             slo[d] = a < 0.0 ? 0 : (a > g.nsec[d] - 1 ? g.nsec[d] - 1 : (int)a);
             shi[d] = b > g.nsec[d] - 1 ? g.nsec[d] - 1 : (b < 0.0 ? 0 : (int)b);
@@ -1134,7 +1196,13 @@ Geo make_geo(const tg_tuple_ctx* c) {
         g.nsec[d] = c->nsec[d];
         g.origin[d] = c->origin[d];
         g.extent[d] = c->extent[d];
+        for (int e = 0; e < 3; ++e) {
+            g.dG[d][e] = c->dG[d][e];
+            g.dGinv[d][e] = c->dGinv[d][e];
+            g.zone[d][e] = c->zone[d][e];
+        }
     }
+    g.axis = c->axis;
     g.omega0 = c->omega0;
     g.vol = c->vol;
     g.nu = c->nu;
@@ -1665,16 +1733,67 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
     c->dim = d->dim;
     c->n = 1;
     c->nel = tg::factorial(d->dim);
+    // the zone: an axis-aligned box (all-zero sides) or a general
+    // parallelepiped mesh with its own zone sides
+    bool general = false;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) general |= d->sides[a][b] != 0.0;
     for (int q = 0; q < 3; ++q) {
         c->nsec[q] = q < d->dim ? d->nsec[q] : 1;
         c->origin[q] = q < d->dim ? d->origin[q] : 0.0;
         c->extent[q] = q < d->dim ? d->extent[q] : 1.0;
     }
     for (int q = 0; q < d->dim; ++q) {
-        if (d->nsec[q] < 1 || !(d->extent[q] > 0.0))
+        if (d->nsec[q] < 1 || (!general && !(d->extent[q] > 0.0)))
             return bail(fail(c, TG_ERR_ARG, "axis %d: nsec %d, extent %g", q, d->nsec[q], d->extent[q]));
         c->n *= d->nsec[q] + 1;
         c->nel *= d->nsec[q];
+    }
+    {
+        // dG[d][c] = component d of side c / nsec[c]; zone[d][c] likewise of zone side c
+        const int D = d->dim;
+        double S[3][3] = {}, Z[3][3] = {};
+        for (int cc = 0; cc < D; ++cc)
+            for (int q = 0; q < D; ++q) {
+                S[q][cc] = general ? d->sides[cc][q] : (q == cc ? d->extent[q] : 0.0);
+                Z[q][cc] = general ? d->zone[cc][q] : (q == cc ? d->extent[q] : 0.0);
+            }
+        c->axis = 1;
+        for (int q = 0; q < D; ++q)
+            for (int cc = 0; cc < D; ++cc) {
+                c->dG[q][cc] = S[q][cc] / (double)c->nsec[cc];
+                c->zone[q][cc] = Z[q][cc];
+                if (q != cc && S[q][cc] != 0.0) c->axis = 0;
+            }
+        for (int q = 0; q < D; ++q)
+            if (c->axis && !(S[q][q] > 0.0)) c->axis = 0;
+        if (c->axis)
+            for (int q = 0; q < D; ++q) c->extent[q] = S[q][q];
+        // determinant and inverse of dG (section volume, reduced coordinates)
+        double det;
+        if (D == 2) {
+            det = c->dG[0][0] * c->dG[1][1] - c->dG[0][1] * c->dG[1][0];
+        } else {
+            det = c->dG[0][0] * (c->dG[1][1] * c->dG[2][2] - c->dG[1][2] * c->dG[2][1]) -
+                  c->dG[0][1] * (c->dG[1][0] * c->dG[2][2] - c->dG[1][2] * c->dG[2][0]) +
+                  c->dG[0][2] * (c->dG[1][0] * c->dG[2][1] - c->dG[1][1] * c->dG[2][0]);
+        }
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+            return bail(fail(c, TG_ERR_ARG, "the mesh sides are degenerate (determinant %g)", det));
+        double A[3][3] = {}, Ai[3][3] = {};
+        for (int q = 0; q < D; ++q)
+            for (int cc = 0; cc < D; ++cc) A[q][cc] = c->dG[q][cc];
+        if (D == 2) {
+            double in2[2][2], a2[2][2] = {{A[0][0], A[0][1]}, {A[1][0], A[1][1]}};
+            invert_small<2>(a2, in2);
+            for (int q = 0; q < 2; ++q)
+                for (int cc = 0; cc < 2; ++cc) Ai[q][cc] = in2[q][cc];
+        } else {
+            invert_small<3>(A, Ai);
+        }
+        for (int q = 0; q < D; ++q)
+            for (int cc = 0; cc < D; ++cc) c->dGinv[q][cc] = Ai[q][cc];
+        c->gen_vol = std::fabs(det) / (double)tg::factorial(D);
     }
     if (!d->eps) return bail(fail(c, TG_ERR_ARG, "eps is null"));
     if (d->num_mc < 1 || d->num_mc > (1 << 20)) return bail(fail(c, TG_ERR_ARG, "num_mc %d outside [1, 2^20]", d->num_mc));
@@ -1689,10 +1808,12 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
     c->threshold = d->threshold;
     c->max_batch = d->max_batch;
     c->dyn = d->dynamic_phonons != 0;
-    {
+    if (c->axis) {
         double v = 1.0;
-        for (int q = 0; q < d->dim; ++q) v *= d->extent[q] / (double)d->nsec[q];
+        for (int q = 0; q < d->dim; ++q) v *= c->extent[q] / (double)d->nsec[q];
         c->vol = std::fabs(v / (double)tg::factorial(d->dim));
+    } else {
+        c->vol = c->gen_vol;   // |det(section sides)| / dim!
     }
     // LF inverse mass (Mesh.cpp:95-123): M_ab = vol * dim! * int_ref phi_a phi_b
     {

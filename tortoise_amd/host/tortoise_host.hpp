@@ -74,18 +74,31 @@ Real dot(const Point<NDim>& a, const Point<NDim>& b) { Real s = 0; for (int i = 
 inline int factorial(int d) { int r = 1; for (int i = 2; i <= d; ++i) r *= i; return r; }
 
 // ------------------------------------------------------------------ Region
+// The parallelepiped origin + sum_a t_a gVec[a], t in [0,1]^NDim
+// (Region.hpp:29-34): the zone whose sides the umklapp vectors combine.
 template <int NDim>
 class Region {
 public:
     Point<NDim> origin;
-    std::array<Point<NDim>, NDim> gVec;
-    // The backend meshes axis-aligned boxes: gVec must be diagonal.
+    std::array<Point<NDim>, NDim> gVec;   // gVec[a]: side a (Region::gVec column a)
     Region(const Point<NDim>& t_origin, const std::array<Point<NDim>, NDim>& t_gVec)
         : origin(t_origin), gVec(t_gVec) {
+        if (!(std::fabs(sideDeterminant(gVec)) > 0.0))
+            throw std::invalid_argument("Region: the sides are degenerate");
+    }
+    static Real sideDeterminant(const std::array<Point<NDim>, NDim>& v) {
+        if constexpr (NDim == 1) return v[0](0);
+        else if constexpr (NDim == 2) return v[0](0) * v[1](1) - v[1](0) * v[0](1);
+        else
+            return v[0](0) * (v[1](1) * v[2](2) - v[2](1) * v[1](2)) -
+                   v[1](0) * (v[0](1) * v[2](2) - v[2](1) * v[0](2)) +
+                   v[2](0) * (v[0](1) * v[1](2) - v[1](1) * v[0](2));
+    }
+    bool axisAligned() const {
         for (int a = 0; a < NDim; ++a)
             for (int d = 0; d < NDim; ++d)
-                if (a != d && gVec[a](d) != 0.0)
-                    throw std::invalid_argument("Region: only axis-aligned zones are supported");
+                if (a != d && gVec[a](d) != 0.0) return false;
+        return true;
     }
 };
 
@@ -97,22 +110,40 @@ public:
     const int nElementSection = factorial(NDim);
     const Region<NDim>* region;
     Point<NDim> origin;          // physical origin of the mesh
-    Point<NDim> extent;          // physical edge lengths
+    std::array<Point<NDim>, NDim> sides;   // its side vectors (Mesh::gVec columns)
+    Point<NDim> extent;          // axis-aligned mesh: the sides' lengths along the axes
+    bool axisAligned = true;
     std::array<int, NDim> nSecSplits;
     int numberSections, numberElements, dataVectorDim;
     Real elemVolume;
 
-    // Mesh.cpp:268-293: origin = region.origin + gVec * relOrigin; gVec scaled by relSize
+    // Mesh.cpp:268-293: origin = region.origin + gVec * relOrigin, the sides
+    // gVec * diag(relSize) (terms in axis order: on an axis-aligned region
+    // the other sides add exact zeros)
     Mesh(const Region<NDim>& t_region, const Point<NDim>& relOrigin, const Point<NDim>& relSize,
          const std::array<int, NDim>& t_nSecSplits)
         : region(&t_region), nSecSplits(t_nSecSplits) {
         numberSections = 1;
-        Real vol = 1.0;
         for (int d = 0; d < NDim; ++d) {
-            origin(d) = t_region.origin(d) + t_region.gVec[d](d) * relOrigin(d);
-            extent(d) = t_region.gVec[d](d) * relSize(d);
+            Real o = t_region.origin(d);
+            for (int a = 0; a < NDim; ++a) o = o + t_region.gVec[a](d) * relOrigin(a);
+            origin(d) = o;
             numberSections *= nSecSplits[d];
-            vol *= extent(d) / nSecSplits[d];
+        }
+        for (int a = 0; a < NDim; ++a) sides[a] = t_region.gVec[a] * relSize(a);
+        axisAligned = t_region.axisAligned();
+        Real vol = 1.0;
+        if (axisAligned) {
+            for (int d = 0; d < NDim; ++d) {
+                extent(d) = sides[d](d);
+                vol *= extent(d) / nSecSplits[d];
+            }
+        } else {
+            vol = Region<NDim>::sideDeterminant(sides);
+            for (int d = 0; d < NDim; ++d) {
+                extent(d) = 0.0;
+                vol /= nSecSplits[d];
+            }
         }
         numberElements = numberSections * nElementSection;
         dataVectorDim = numberElements * nVertElement;
@@ -126,12 +157,18 @@ public:
         for (int d = 0; d < NDim; ++d) n *= nSecSplits[d] + 1;
         return n;
     }
+    // origin + sum_a (sides[a] / nSecSplits[a]) i_a, terms in axis order
     Point<NDim> node(int64_t id) const {
+        std::array<int64_t, NDim> idx;
+        for (int a = 0; a < NDim; ++a) {
+            idx[a] = id % (nSecSplits[a] + 1);
+            id /= (nSecSplits[a] + 1);
+        }
         Point<NDim> p;
         for (int d = 0; d < NDim; ++d) {
-            int64_t i = id % (nSecSplits[d] + 1);
-            id /= (nSecSplits[d] + 1);
-            p(d) = origin(d) + (extent(d) / nSecSplits[d]) * (Real)i;
+            Real x = origin(d);
+            for (int a = 0; a < NDim; ++a) x = x + (sides[a](d) / nSecSplits[a]) * (Real)idx[a];
+            p(d) = x;
         }
         return p;
     }
@@ -434,6 +471,9 @@ public:
         for (Real w : omega)
             if (std::fabs(w - omega[0]) > 1e-12 * std::fabs(omega[0]))
                 throw std::runtime_error("phonon band must be dispersionless");
+        if (!el.mesh->axisAligned)
+            throw std::runtime_error("the W-form backend meshes axis-aligned zones; the reference operator "
+                                     "(tupleBackend) takes general ones");
         eps_ = el.dispersion.toNodal();
         tg_problem_desc d{};
         d.dim = NDim;
@@ -491,6 +531,14 @@ public:
             d.origin[q] = q < NDim ? el.mesh->origin(q) : 0.0;
             d.extent[q] = q < NDim ? el.mesh->extent(q) : 1.0;
         }
+        // the mesh's sides and the region's (the umklapp basis,
+        // ScatteringChannel.cpp:330-334): a sub-mesh of a region moves by the
+        // region's sides, not its own
+        for (int a = 0; a < NDim; ++a)
+            for (int q = 0; q < NDim; ++q) {
+                d.sides[a][q] = el.mesh->sides[a](q);
+                d.zone[a][q] = el.mesh->region->gVec[a](q);
+            }
         d.eps = eps_.data();
         d.omega0 = omegaPh_[0];
         d.eps_ph = flat ? nullptr : omegaPh_.data();   // a dispersive phonon band

@@ -513,6 +513,8 @@ def main():
                     help="only the reference tuple operator measurements (profiling runs)")
     ap.add_argument("--only-ee", action="store_true",
                     help="only the 20^3 e-ph + e-e ensemble line (profiling runs)")
+    ap.add_argument("--only-dp5", action="store_true",
+                    help="only the assembly and the DP5 runs on the 128x128 ensemble (profiling runs)")
     ap.add_argument("--only-gradients", action="store_true",
                     help="only the 64x64 gradient line (adjoint of DP5)")
     ap.add_argument("--ee-t-final", type=float, default=1.0,
@@ -569,6 +571,13 @@ def main():
     Fd = torch.from_numpy(F).to("cuda")
     Id = torch.empty_like(Fd)
     torch.cuda.synchronize()
+    if args.only_dp5:
+        Fh = F.T.copy()
+        res = {"dp5_batched": dp5_batched(ctx, w, Fh, bath)[0],
+               "dp5_plain": dp5_plain(ctx, w, Fh[:, 0].copy(), bath[0])}
+        ctx.close()
+        print(json.dumps(res), flush=True)
+        return
 
     def mixed_call(time_kernel=False):
         ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,

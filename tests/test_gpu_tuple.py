@@ -114,6 +114,19 @@ def test_hexagonal_zone_matches_oracle(ns, nu):
     op.close()
 
 
+def test_degenerate_sides_and_zone_refused():
+    """Degenerate mesh sides are refused; so is a degenerate zone when umklapp
+    vectors are asked for (they would repeat tuples), not without them."""
+    w = wl.Workload("hex4", 2, (4, 4))
+    eps = np.zeros(25)
+    flat = np.array([[1.0, 0.0], [2.0, 0.0]])
+    with pytest.raises(RuntimeError, match="mesh sides are degenerate"):
+        tg.TupleOperator.create(w, n_umklapp=0, eps=eps, origin=(0.0, 0.0), sides=flat)
+    with pytest.raises(RuntimeError, match="zone sides are degenerate"):
+        tg.TupleOperator.create(w, n_umklapp=1, eps=eps, origin=(0.0, 0.0), sides=HEX_SIDES, zone=flat)
+    tg.TupleOperator.create(w, n_umklapp=0, eps=eps, origin=(0.0, 0.0), sides=HEX_SIDES, zone=flat).close()
+
+
 def test_tuple_config0_square32():
     """The reference's own CPU-runnable case: 32 x 32 sections (2048
     triangles), NMC = 50, umklapp {-1,0,1}^2."""

@@ -1780,6 +1780,17 @@ extern "C" tg_status tg_tuple_create(const tg_tuple_desc* d, tg_tuple_ctx** out)
         }
         if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
             return bail(fail(c, TG_ERR_ARG, "the mesh sides are degenerate (determinant %g)", det));
+        // umklapp vectors are combinations of the zone's sides: a degenerate
+        // zone would repeat tuples under different umklapp ids
+        if (d->n_umklapp > 0) {
+            const double zdet = D == 2 ? Z[0][0] * Z[1][1] - Z[0][1] * Z[1][0]
+                                       : Z[0][0] * (Z[1][1] * Z[2][2] - Z[1][2] * Z[2][1]) -
+                                             Z[0][1] * (Z[1][0] * Z[2][2] - Z[1][2] * Z[2][0]) +
+                                             Z[0][2] * (Z[1][0] * Z[2][1] - Z[1][1] * Z[2][0]);
+            if (!(std::fabs(zdet) > 0.0) || !std::isfinite(zdet))
+                return bail(fail(c, TG_ERR_ARG, "the zone sides are degenerate (determinant %g) with n_umklapp %d",
+                                 zdet, d->n_umklapp));
+        }
         double A[3][3] = {}, Ai[3][3] = {};
         for (int q = 0; q < D; ++q)
             for (int cc = 0; cc < D; ++cc) A[q][cc] = c->dG[q][cc];

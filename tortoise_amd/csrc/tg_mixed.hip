@@ -618,7 +618,9 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
 // the least loaded bin, a fixed order, so every call runs the same
 // schedule).  A bin is nmt workgroups on one XCD that run the same panel
 // list for the nmt member tiles side by side, so a panel's A band is fetched
-// into that XCD's L2 once for all member tiles.  With nsplit = 1 and the
+// into that XCD's L2 once for all member tiles.  Bin b runs on XCD b % 8:
+// consecutive bins of a schedule (the longest panels first) spread over the
+// eight L2s, also when only a few bins have work.  With nsplit = 1 and the
 // same FOLD the summation order is that of mixed_gemm_kernel.
 template <int NW, int FOLD, int DA, int MG, bool AL>
 __global__ void __launch_bounds__(256 * MG, 1)
@@ -646,7 +648,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     const int w = blockIdx.x;
     const int wi = w / 8;
     const int mt = wi % nmt;
-    const int bin = (w % 8) * bins_per_xcd + wi / nmt;
+    const int bin = (wi / nmt) * 8 + w % 8;
     if (wi / nmt >= bins_per_xcd) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -822,15 +824,18 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
                                               const int32_t* __restrict__ perm, const double* __restrict__ rk,
                                               const double* __restrict__ ck, const double* __restrict__ V,
                                               double nb, const uint8_t* __restrict__ useg,
-                                              const uint8_t* __restrict__ actp, double inv_scale,
-                                              const double* __restrict__ eps_s, bool want_e, double* row,
-                                              double* __restrict__ Ib, double& dI, double& dw, double& eI,
-                                              double& ew) {
+                                              const uint8_t* __restrict__ actp, int npanel, uint8_t* sflag,
+                                              double inv_scale, const double* __restrict__ eps_s, bool want_e,
+                                              double* row, double* __restrict__ Ib, double& dI, double& dw,
+                                              double& eI, double& ew) {
     const double np1 = nb + 1.0;
     // UNR positions per thread per round with all their loads issued before
     // use; the trip count is the same for every thread (the barrier after the
     // first round's loads needs all of them)
-    constexpr int UNR = 6;
+#ifndef TG_FIN_UNR
+#define TG_FIN_UNR 4
+#endif
+    constexpr int UNR = TG_FIN_UNR;
     double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR], Ep[UNR];
     int32_t node[UNR], gu[UNR], gv[UNR];   // gu, gv: the panels' operand bytes
     // the partial sums (streamed from HBM), read only for the panels the GEMM
@@ -854,17 +859,26 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
         }
     };
     // ... after the operand bytes and the panel flags (actp, null when every
-    // panel is computed), which say which panels those are ...
+    // panel is computed), which say which panels those are.  With ROW one
+    // byte per panel (the member's operand choice in bit 0, bit 1 for a panel
+    // left out) is staged in LDS (sflag) once, so a round waits for one
+    // memory trip, not two ...
+    auto panel_byte = [&](int64_t pn) {
+        int g = useg[pn * nt + b];
+        if (actp) g |= actp[pn] ? 0 : 2;
+        return g;
+    };
     auto load_bytes = [&](int64_t p0) {
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             const int64_t p = min(p0 + (int64_t)j * kMemT, n - 1);
             const int64_t pu = p / kMT, pv = (kt + p) / kMT;
-            gu[j] = useg[pu * nt + b];
-            gv[j] = useg[pv * nt + b];
-            if (actp) {
-                gu[j] |= actp[pu] ? 0 : 2;
-                gv[j] |= actp[pv] ? 0 : 2;
+            if constexpr (ROW) {
+                gu[j] = sflag[pu];
+                gv[j] = sflag[pv];
+            } else {
+                gu[j] = panel_byte(pu);
+                gv[j] = panel_byte(pv);
             }
         }
     };
@@ -916,12 +930,22 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
         }
     };
     const int64_t step = (int64_t)kMemT * UNR;
-    // the first round's partial sums and the row staging are in flight together
-    load_bytes(threadIdx.x);
-    if constexpr (ROW) stage_member_row(fb, n, row);
-    load_products(threadIdx.x);
-    load_constants(threadIdx.x);
-    if constexpr (ROW) lds_barrier();
+    if constexpr (ROW) {
+        // the panel bytes, the first round's constants and the row staging
+        // are in flight together; the partial sums follow the barrier
+        int g = 0;
+        if (threadIdx.x < npanel) g = panel_byte(threadIdx.x);
+        load_constants(threadIdx.x);
+        stage_member_row(fb, n, row);
+        if (threadIdx.x < npanel) sflag[threadIdx.x] = (uint8_t)g;
+        lds_barrier();
+        load_bytes(threadIdx.x);
+        load_products(threadIdx.x);
+    } else {
+        load_bytes(threadIdx.x);
+        load_products(threadIdx.x);
+        load_constants(threadIdx.x);
+    }
     finish_round(threadIdx.x);
     for (int64_t base = step; base < n; base += step) {
         load_bytes(base + threadIdx.x);
@@ -965,21 +989,22 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                      const int32_t* __restrict__ perm, const double* __restrict__ rk,
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
-                     const uint8_t* __restrict__ actp, double inv_scale, int conserve,
+                     const uint8_t* __restrict__ actp, int npanel, double inv_scale, int conserve,
                      const double* __restrict__ gate,
                      double* __restrict__ I, const double* __restrict__ eps_s,
                      double* __restrict__ flux) {
     extern __shared__ double row[];
     __shared__ double red[4][kMemT / 64];
     __shared__ double s_b;
+    __shared__ uint8_t sflag[ROW ? kMemT : 1];
     const int b = member_of_block(blockIdx.x, nt);
     if (b >= B || (gate && gate[b] == 0.0)) return;
     const double nb = bath[b];
     const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
     double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
-    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, actp, inv_scale,
-                           eps_s, flux != nullptr, row, Ib, dI, dw, eI, ew);
+    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, actp, npanel, sflag,
+                           inv_scale, eps_s, flux != nullptr, row, Ib, dI, dw, eI, ew);
     double sc = 0.0;
     if (conserve || flux) {
         block_sum4(dI, dw, eI, ew, red);
@@ -1055,6 +1080,7 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     extern __shared__ double row[];
     __shared__ double red[4][kMemT / 64];
     __shared__ double s_b;
+    __shared__ uint8_t sflag[kMemT];
     const int b = member_of_block(blockIdx.x, v.nt);
     if (b >= v.nt) return;
     const bool live = b < v.B;
@@ -1067,8 +1093,8 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         // k_s = I(fin): the epilogue over the energy order, I in place of fin in LDS
         double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
         finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, b, a.fin + base, v.perm, v.rk, v.ck,
-                                v.V, a.bath[b], v.useg, v.actp, v.inv_scale, v.eps_s, a.flux != nullptr, row,
-                                nullptr, dI, dw, eI, ew);
+                                v.V, a.bath[b], v.useg, v.actp, v.npanel, sflag, v.inv_scale, v.eps_s,
+                                a.flux != nullptr, row, nullptr, dI, dw, eI, ew);
         block_sum4(dI, dw, eI, ew, red);
         if (threadIdx.x == 0) {
             s_b = (a.conserve && dw > 0.0) ? dI / dw : 0.0;
@@ -1447,7 +1473,7 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
                        c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
                        m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, dyn ? m->actp : nullptr, \
-                       inv_scale, conserve,                                                          \
+                       m->npanel, inv_scale, conserve,                                               \
                        gate, d_I, m->eps_s, flux)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
@@ -1757,7 +1783,7 @@ tg_status mixed_prepare(tg_ctx* c) {
         const size_t rowbytes = sizeof(double) * (size_t)n;
         // the staged pack holds every k-group index and the panel index of a
         // thread in registers
-        m->rowcache = rowbytes + 1024 <= (size_t)prop.sharedMemPerBlock &&
+        m->rowcache = rowbytes + 2048 <= (size_t)prop.sharedMemPerBlock &&
                       n <= 2 * (int64_t)kMemT * kStageUnr &&
                       m->kpad / 8 <= (int64_t)kPackKG * kMemT && m->npanel <= kMemT;
         if (const char* e = std::getenv("TG_MIXED_ROWCACHE"))

@@ -170,6 +170,7 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, doubl
     if ((flags & TG_PREC_MIXED) && mixed_dp5_fusable(c)) {
         // the same steps as below with the vector work between GEMMs fused
         // into one member pass each (Dp5PassArgs); no err vector
+        TG_TRY(mixed_dp5_compact(c, w->h, B));
         Dp5PassArgs p{};
         p.y = w->y;
         p.h = w->h;

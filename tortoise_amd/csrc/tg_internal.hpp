@@ -206,6 +206,9 @@ struct Dp5PassArgs;
 bool mixed_dp5_fusable(tg_ctx* c);
 tg_status mixed_dp5_pass(tg_ctx* c, int mode, const Dp5PassArgs& a, int32_t batch);
 tg_status mixed_dp5_gemm(tg_ctx* c);
+// before the passes of an attempt: running members (h[b] != 0) take the
+// leading columns, and the GEMM computes only the member tiles they fill
+tg_status mixed_dp5_compact(tg_ctx* c, const double* h, int32_t batch);
 // the device array of per-member bath occupations the batched epilogue reads
 // (a DP5 with a bath schedule writes it per stage; the host copy is then stale)
 double* mixed_bath_device(tg_ctx* c);

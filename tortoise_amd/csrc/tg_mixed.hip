@@ -106,6 +106,8 @@ struct MixedData {
     int32_t* dsched_off = nullptr;   // nbins + 1
     int32_t* dsched_panel = nullptr; // npanel
     int64_t* dstat = nullptr;        // iterations the last schedule holds
+    int32_t* dslot = nullptr;        // nt: a fused DP5 attempt's column of each member (-1: not running)
+    int32_t* dncol = nullptr;        // 1: the columns running members take (leading ones)
     float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
@@ -317,12 +319,12 @@ __device__ __forceinline__ void load_pack_nodes(PackNodes& pk, int64_t n, int64_
 
 // member b's operands and operand choices from its f in LDS `row` (caller
 // order, complete: the caller has passed a barrier since writing it)
-__device__ __forceinline__ void pack_from_row(const double* row, const PackNodes& pk, bool live, int b, int64_t n,
+__device__ __forceinline__ void pack_from_row(const double* row, const PackNodes& pk, bool live, int col, int64_t n,
                                               int64_t kgroups, int npanel, int nt, const PackOut& o) {
     const int pn = threadIdx.x;
     if (pn < npanel) {
         const uint8_t v = live && row[pk.pnode] > 0.5 ? 1 : 0;
-        o.useg[(int64_t)pn * nt + b] = v;
+        o.useg[(int64_t)pn * nt + col] = v;
         if (o.useF && live) (v ? o.useG : o.useF)[pn] = 1;
     }
 #pragma unroll
@@ -332,7 +334,7 @@ __device__ __forceinline__ void pack_from_row(const double* row, const PackNodes
             double xv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) xv[q] = row[pk.node[j][q]];
-            pack_group(kg, xv, live, n, b, nt, o);
+            pack_group(kg, xv, live, n, col, nt, o);
         }
     }
 }
@@ -445,7 +447,8 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
                   const u32x4* __restrict__ Ghi, const u32x4* __restrict__ Glo,
                   const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
                   const int2* __restrict__ krange, int nmt, int nsplit, int64_t nblocks,
-                  int64_t per_xcd, typename Partial<FOLD>::type* __restrict__ P) {
+                  int64_t per_xcd, const int32_t* __restrict__ ncol,
+                  typename Partial<FOLD>::type* __restrict__ P) {
     constexpr int NTH = 64 * WAVES;
     constexpr int WC = WAVES / 2;                // waves along the members
     constexpr int WN = NW / WC;                  // members per wave
@@ -460,6 +463,7 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
     const int64_t L = (wg % 8) * per_xcd + wg / 8;
     if (L >= nblocks) return;
     const int mt = (int)(L % nmt);
+    if (ncol && mt * NW >= *ncol) return;   // a member tile with no running member
     const int64_t ps = L / nmt;
     const int split = (int)(ps % nsplit);
     const int panel = (int)(ps / nsplit);
@@ -622,6 +626,16 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
 // consecutive bins of a schedule (the longest panels first) spread over the
 // eight L2s, also when only a few bins have work.  With nsplit = 1 and the
 // same FOLD the summation order is that of mixed_gemm_kernel.
+// The member tiles a persistent launch computes: all nmt, or with ncol (the
+// columns running members fill, from the fused DP5's compaction) the tiles
+// those columns reach.  Each XCD's workgroups then regroup into bins of that
+// many, so fewer running tiles give more bins and shorter panel lists.
+__device__ inline int active_tiles(const int32_t* ncol, int nw, int nmt) {
+    if (!ncol) return nmt;
+    const int t = (*ncol + nw - 1) / nw;
+    return t < nmt ? t : nmt;
+}
+
 template <int NW, int FOLD, int DA, int MG, bool AL>
 __global__ void __launch_bounds__(256 * MG, 1)
 mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ band_off,
@@ -630,7 +644,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
                           const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
                           const int2* __restrict__ krange, const int32_t* __restrict__ sched_off,
                           const int32_t* __restrict__ sched_panel, int nmt, int bins_per_xcd,
-                          double* __restrict__ P) {
+                          const int32_t* __restrict__ ncol, double* __restrict__ P) {
     static_assert(FOLD > 0, "the persistent kernel folds into FP64");
     constexpr int NTH = 256 * MG;
     constexpr int WN = NW / MG;                    // members per wave
@@ -645,11 +659,15 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     __shared__ u32x4 sZ[2][2][kKG][NW + 1];
     __shared__ u32x4 sA[AL ? 2 : 1][kKG][2][AL ? kMT : 1];
 
+    // workgroup w runs on XCD w % 8; the T workgroups of a bin (one per
+    // member tile) share an XCD, so a panel's A band is read into one L2
+    const int T = active_tiles(ncol, NW, nmt);
+    if (T == 0) return;
     const int w = blockIdx.x;
     const int wi = w / 8;
-    const int mt = wi % nmt;
-    const int bin = (wi / nmt) * 8 + w % 8;
-    if (wi / nmt >= bins_per_xcd) return;
+    const int mt = wi % T;
+    const int bin = (wi / T) * 8 + w % 8;
+    if (wi / T >= bins_per_xcd * nmt / T) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int rg = wave % 4, mg = wave / 4;
@@ -1058,6 +1076,7 @@ struct MixedView {
     const double* eps_s;
     const uint8_t* useg;
     const uint8_t* actp;
+    const int32_t* slot;   // null: member b's column is b
     double inv_scale;
     PackOut o;
 };
@@ -1087,12 +1106,16 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     if (MODE == kDp5Last && !live) return;
     const double hb = live ? a.h[b] : 0.0;
     if (live && hb == 0.0) return;   // finished member: nothing of this attempt is read
+    // with slots the running members take the leading columns of the
+    // operands and partial sums (padding members have none)
+    if (v.slot && !live) return;
+    const int col = v.slot ? v.slot[b] : b;
     const int64_t n = v.n;
     const int64_t base = (int64_t)b * n;
     if (MODE != kDp5Stage && live) {
         // k_s = I(fin): the epilogue over the energy order, I in place of fin in LDS
         double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
-        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, b, a.fin + base, v.perm, v.rk, v.ck,
+        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, col, a.fin + base, v.perm, v.rk, v.ck,
                                 v.V, a.bath[b], v.useg, v.actp, v.npanel, sflag, v.inv_scale, v.eps_s,
                                 a.flux != nullptr, row, nullptr, dI, dw, eI, ew);
         block_sum4(dI, dw, eI, ew, red);
@@ -1175,7 +1198,7 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     }
     __syncthreads();
     if constexpr (MODE != kDp5Last) {
-        pack_from_row(row, pk, live, b, n, v.kgroups, v.npanel, v.nt, v.o);
+        pack_from_row(row, pk, live, col, n, v.kgroups, v.npanel, v.nt, v.o);
     } else {
         if (!a.norms) return;
         // sum over elements of sum_a r_a^2 + (sum_a r_a)^2 (the P1 integral of
@@ -1233,13 +1256,37 @@ __device__ inline int block_exclusive_scan(int v, int* wtot, int* total) {
     return wbase + inc - v;
 }
 
+// The columns of a fused DP5 attempt: member b < B that runs (step h[b]
+// != 0) takes column slot[b], the running members in member order from 0;
+// ncol[0] = their count.  Member tiles past it are not computed, so the
+// batched GEMM's work shrinks as members finish.  Columns are computed
+// independently: a member's results do not depend on its column.
 __global__ void __launch_bounds__(kSchedT)
-active_schedule_kernel(int npanel, int nbins, int nit, const int2* __restrict__ krange,
+compact_kernel(const double* __restrict__ h, int B, int nt, int32_t* __restrict__ slot, int32_t* __restrict__ ncol) {
+    __shared__ int wtot[kSchedT / 64];
+    int base = 0;
+    for (int b0 = 0; b0 < nt; b0 += kSchedT) {
+        const int b = b0 + (int)threadIdx.x;
+        const int run = (b < B && h[b] != 0.0) ? 1 : 0;
+        int tot;
+        const int r = block_exclusive_scan(run, wtot, &tot);
+        if (b < nt) slot[b] = run ? base + r : -1;
+        base += tot;
+    }
+    if (threadIdx.x == 0) ncol[0] = base;
+}
+
+__global__ void __launch_bounds__(kSchedT)
+active_schedule_kernel(int npanel, int wg_per_xcd, const int32_t* __restrict__ ncol, int nw, int nmt, int nit,
+                       const int2* __restrict__ krange,
                        const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
                        uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF, uint8_t* __restrict__ useG,
                        uint8_t* __restrict__ actp, int32_t* __restrict__ sched_off,
                        int32_t* __restrict__ sched_panel, int64_t* __restrict__ stat) {
     extern __shared__ int32_t sh[];
+    // bins of the GEMM launch that reads this schedule (active_tiles)
+    const int T = active_tiles(ncol, nw, nmt);
+    const int nbins = 8 * (wg_per_xcd / (T > 0 ? T : 1));
     int32_t* cF = sh;                 // nit + 1: iterations before it holding a nonzero f operand
     int32_t* cG = cF + nit + 1;       // nit + 1: the same for g
     int32_t* flag = cG + nit + 1;     // npanel: 1 if the panel at that position of `order` has work
@@ -1316,7 +1363,7 @@ active_schedule_kernel(int npanel, int nbins, int nit, const int2* __restrict__ 
 
 // ------------------------------------------------------------------- host
 template <int NW, int WAVES, int FOLD, int DEPTH>
-static void launch_gemm_t(tg_ctx* c, MixedData* m) {
+static void launch_gemm_t(tg_ctx* c, MixedData* m, const int32_t* ncol) {
     const int nmt = m->nt / NW;
     const int64_t nblocks = (int64_t)m->npanel * m->nsplit * nmt;
     const int64_t per = (nblocks + 7) / 8;
@@ -1324,7 +1371,7 @@ static void launch_gemm_t(tg_ctx* c, MixedData* m) {
                        0, c->stream, (const u32x4*)m->A_band, m->band_off,
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange, nmt, m->nsplit,
-                       nblocks, per, (typename Partial<FOLD>::type*)m->P);
+                       nblocks, per, ncol, (typename Partial<FOLD>::type*)m->P);
 }
 
 // the tile configurations the library carries: (members, waves, fold, prefetch depth)
@@ -1398,7 +1445,7 @@ static tg_status build_schedule(tg_ctx* c, MixedData* m) {
 }
 
 template <int NW, int FOLD, int DA, int MG, int AL>
-static void launch_persist_t(tg_ctx* c, MixedData* m, bool dyn) {
+static void launch_persist_t(tg_ctx* c, MixedData* m, bool dyn, const int32_t* ncol) {
     static_assert(AL == 0 || AL == 1, "A path");
     const int nmt = m->nt / NW;
     hipLaunchKernelGGL((mixed_gemm_persist_kernel<NW, FOLD, DA, MG, AL == 1>), dim3((unsigned)(8 * m->bins_per_xcd * nmt)),
@@ -1406,17 +1453,18 @@ static void launch_persist_t(tg_ctx* c, MixedData* m, bool dyn) {
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange,
                        dyn ? m->dsched_off : m->sched_off, dyn ? m->dsched_panel : m->sched_panel, nmt,
-                       m->bins_per_xcd, (double*)m->P);
+                       m->bins_per_xcd, ncol, (double*)m->P);
 }
 
-static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false) {
+// ncol (device, optional): member tiles from *ncol on are not computed
+static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false, const int32_t* ncol = nullptr) {
     if (m->nt % m->nw != 0) return set_error(c, TG_ERR_ARG, "tile width %d does not divide %d", m->nw, m->nt);
     if (m->persist) {
         if (m->sched_nw != m->nw || m->nsplit != 1)
             return set_error(c, TG_ERR_STATE, "persistent GEMM schedule not built for tile %d", m->nw);
 #define TG_X(a, c_, d_, g_, z_)                                                                   \
     if (m->nw == a && m->fold == c_ && m->da == d_ && m->waves == 4 * g_ && m->al == z_)                  \
-        launch_persist_t<a, c_, d_, g_, z_>(c, m, dyn); else
+        launch_persist_t<a, c_, d_, g_, z_>(c, m, dyn, ncol); else
         TG_PERSIST_CONFIGS(TG_X)
 #undef TG_X
         return set_error(c, TG_ERR_ARG, "no persistent kernel for tile %d members, fold %d, depth %d, %d waves, A in LDS %d",
@@ -1425,7 +1473,7 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false) {
         return TG_OK;
     }
 #define TG_X(a, b, c_, d_)                                                        \
-    if (m->nw == a && m->waves == b && m->fold == c_ && m->depth == d_) launch_gemm_t<a, b, c_, d_>(c, m); else
+    if (m->nw == a && m->waves == b && m->fold == c_ && m->depth == d_) launch_gemm_t<a, b, c_, d_>(c, m, ncol); else
     TG_MIXED_CONFIGS(TG_X)
 #undef TG_X
     return set_error(c, TG_ERR_ARG, "no kernel for tile %d members, %d waves, fold %d, depth %d", m->nw,
@@ -1827,10 +1875,14 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_TRY(dev_alloc(c, &m->actp, (size_t)m->npanel));
         TG_HIP(c, hipMemsetAsync(m->actp, 1, (size_t)m->npanel, st));
         TG_TRY(dev_alloc(c, &m->lpt_order, (size_t)m->npanel));
-        TG_TRY(dev_alloc(c, &m->dsched_off, (size_t)std::max(m->ncu, 8) + 1));
+        // one entry per workgroup of any persistent launch (every workgroup
+        // is a bin when one member tile runs) and one
+        TG_TRY(dev_alloc(c, &m->dsched_off, (size_t)std::max(m->ncu, 8) + 8 * (size_t)m->nt + 1));
         TG_TRY(dev_alloc(c, &m->dsched_panel, (size_t)m->npanel));
         TG_TRY(dev_alloc(c, &m->dstat, 1));
         TG_HIP(c, hipMemsetAsync(m->dstat, 0xff, sizeof(int64_t), st));   // -1: no evaluation yet
+        TG_TRY(dev_alloc(c, &m->dslot, (size_t)m->nt));
+        TG_TRY(dev_alloc(c, &m->dncol, 1));
         TG_HIP(c, hipMemsetAsync(m->nzF, 0, (size_t)m->nit, st));
         TG_HIP(c, hipMemsetAsync(m->nzG, 0, (size_t)m->nit, st));
         TG_HIP(c, hipMemsetAsync(m->useF, 0, (size_t)m->npanel, st));
@@ -1890,6 +1942,8 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->dsched_off);
     dev_free(&m->dsched_panel);
     dev_free(&m->dstat);
+    dev_free(&m->dslot);
+    dev_free(&m->dncol);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
     dev_free(&m->P);
@@ -1918,10 +1972,12 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 }
 
 // the panels with work and their schedule from the flags the pack set
-static tg_status launch_schedule(tg_ctx* c, MixedData* m) {
-    const int nbins = 8 * m->bins_per_xcd;
+// ncol (device, optional): the columns running members fill (the GEMM that
+// reads the schedule gets the same)
+static tg_status launch_schedule(tg_ctx* c, MixedData* m, const int32_t* ncol = nullptr) {
     hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
-                       sizeof(int32_t) * (2 * ((size_t)m->nit + 1) + m->npanel), c->stream, m->npanel, nbins,
+                       sizeof(int32_t) * (2 * ((size_t)m->nit + 1) + m->npanel), c->stream, m->npanel,
+                       m->bins_per_xcd * (m->nt / m->nw), ncol, m->nw, m->nt / m->nw,
                        m->nit, m->krange, m->lpt_order, m->nzF, m->nzG, m->useF, m->useG, m->actp,
                        m->dsched_off, m->dsched_panel, m->dstat);
     TG_CHECK_LAUNCH(c);
@@ -1947,6 +2003,7 @@ static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, 
     v.eps_s = m->eps_s;
     v.useg = m->useg;
     v.actp = dyn ? m->actp : nullptr;
+    v.slot = m->dslot;
     v.inv_scale = std::ldexp(1.0, -(m->sa + kSF));
     v.o = pack_out(m, dyn);
     hipLaunchKernelGGL((dp5_member_kernel<MODE, PT>), dim3(m->nt), dim3(kMemT), sizeof(double) * (size_t)c->n,
@@ -1986,8 +2043,22 @@ tg_status mixed_dp5_gemm(tg_ctx* c) {
     MixedData* m = md_of(c);
     if (!m) return set_error(c, TG_ERR_STATE, "batched path not prepared");
     const bool dyn = m->persist && m->skip;
-    if (dyn) TG_TRY(launch_schedule(c, m));
-    return launch_gemm(c, m, dyn);
+    // the persistent GEMM regroups its bins for the running tiles only with
+    // the schedule built for them (dyn); the tiled kernel skips empty tiles
+    const int32_t* ncol = (dyn || !m->persist) ? m->dncol : nullptr;
+    if (dyn) TG_TRY(launch_schedule(c, m, ncol));
+    return launch_gemm(c, m, dyn, ncol);
+}
+
+// the columns of this attempt's running members (h: the per-member step
+// sizes after the attempt's prep); kernel launch only
+tg_status mixed_dp5_compact(tg_ctx* c, const double* h, int32_t B) {
+    MixedData* m = md_of(c);
+    if (!m || !m->dslot) return set_error(c, TG_ERR_STATE, "batched path not prepared");
+    if (B < 1 || B > m->nt) return set_error(c, TG_ERR_ARG, "batch %d exceeds the packed width %d", B, m->nt);
+    hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(kSchedT), 0, c->stream, h, B, m->nt, m->dslot, m->dncol);
+    TG_CHECK_LAUNCH(c);
+    return TG_OK;
 }
 
 // one batched evaluation with the bath set by mixed_set_bath: kernel launches

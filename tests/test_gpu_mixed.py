@@ -289,6 +289,34 @@ def test_mixed_dp5_device_states_equal_host_states(sq32_mixed):
     assert all(np.array_equal(a, b) for a, b in zip(stepd, steps))
 
 
+def test_mixed_dp5_members_independent_of_the_batch():
+    """A member's batched DP5 does not depend on the batch around it: in a
+    256-member batch the running members are repacked into the leading
+    operand columns every attempt (so a member's column and member tile
+    change as others finish, and the GEMM's bins regroup when one tile is
+    left), yet members 0, 130 and 255 integrated in a batch of three give the
+    same states, statistics and steps bit for bit."""
+    w = wl.SQUARE32
+    B = 256
+    F, bath = _members(w, B, seed=21)
+    print("[gpu] mixed dp5 square32 B=256 vs B=3, t=60", flush=True)
+    ctx = tg.Context.create(w, max_batch=B)
+    ctx.assemble(mixed=True)
+    fm, _, stats, steps = ctx.dp5(F, 0.0, 60.0, 0.6, 1e-6, 1e-12, bath_n=bath, mixed=True, conserve=True,
+                                  h_cap=4096)
+    att = np.array([s["attempted"] for s in stats])
+    # members finish at different attempts, so the columns move during the run
+    assert att.max() > att.min()
+    pick = [0, 130, 255]
+    fs, _, st3, steps3 = ctx.dp5(F[:, pick], 0.0, 60.0, 0.6, 1e-6, 1e-12, bath_n=bath[pick], mixed=True,
+                                 conserve=True, h_cap=4096)
+    for q, b in enumerate(pick):
+        assert np.array_equal(fs[:, q], fm[:, b]), b
+        assert st3[q] == stats[b], b
+        assert np.array_equal(steps3[q], steps[b]), b
+    ctx.close()
+
+
 def test_mixed_zero_operand_panels_left_out_exactly(monkeypatch):
     """Panels whose fp16 operands are zero for every member (the empty top of
     cold distributions, where f is below the split's range) are left out of

@@ -44,6 +44,10 @@ typedef int32_t tg_status;
 #define TG_ERR_STATE          -5   /* call out of order (e.g. apply before assemble) */
 #define TG_ERR_NOMEM          -6   /* device allocation failed */
 
+/* occupation factors (tg_problem_desc.occupation) */
+#define TG_OCC_NODAL 0
+#define TG_OCC_HAT   1
+
 /* flags */
 #define TG_PTR_DEVICE   0x1   /* f / I / out pointers are device memory */
 #define TG_PREC_MIXED   0x2   /* batched fp16 hi+lo split kernel, FP32 MFMA, FP64 epilogue */
@@ -66,7 +70,17 @@ typedef struct tg_problem_desc {
     double  de;             /* energy-grid spacing of the overlap quadrature (<= 0: sigma/8) */
     double  cutoff;         /* Gaussian cut-off in units of sigma (<= 0: 8) */
     int32_t max_batch;      /* largest batch the context will be asked for (>= 1) */
-    int32_t reserved;
+    /* Occupation factors of the test function's node i (the (1 - f_i) of the
+     * gain and the f_i of the loss):
+     *   TG_OCC_NODAL (0): the nodal value f_i (the lumped W-form);
+     *   TG_OCC_HAT   (1): the hat average fbar_i = int phi_i f / int phi_i of the
+     *                     P1 interpolant (consistent over lumped P1 mass).
+     * The reference integrates these P1 products exactly per element tuple
+     * (include/Physics/Scattering/IntegratorCore.hpp:298-316); lumping them at
+     * the node adds an energy diffusion of the size of the hat's energy spread,
+     * which the hat average removes.  The hat form does not conserve the
+     * particle number exactly by itself: pair it with TG_CONSERVE. */
+    int32_t occupation;
     /* electron-electron (4-leg) channel on the periodic lattice of the box
      * (off when ee_g == 0): amplitude ee_g^2 / (|q|^2 + ee_kappa2)^2, energy
      * delta a Gaussian of width ee_sigma (cut at `cutoff` widths) projected on
@@ -166,7 +180,9 @@ tg_status   tg_kernel_copy_rows(tg_ctx* ctx, int64_t row0, int64_t nrows, double
 
 /* -------------------------------------------------------- the hot path */
 /* One collision-operator evaluation per member:
- *   I_b = [(1-f_b)(W_b f_b) - f_b (W_b^T (1-f_b))] / V,  W_b = (n_b+1) K + n_b K^T.
+ *   I_b = [(1-x_b)(W_b f_b) - x_b (W_b^T (1-f_b))] / V,  W_b = (n_b+1) K + n_b K^T,
+ * x_b = f_b (TG_OCC_NODAL) or its hat average (TG_OCC_HAT).  With TG_CONSERVE
+ * the number defect sum_i V_i I_i is removed as I -= s |x (1 - x)|.
  * bath_n: per-member bath occupations (batch values) or NULL for the default.
  * Default path: FP64 streaming pass over K's band giving K f and K^T f
  * in one read (any batch, members in turn).  TG_PREC_MIXED: batched MFMA kernel

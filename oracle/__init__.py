@@ -79,6 +79,19 @@ def lib():
         L.or_kernel_row.argtypes = [C.c_int64, C.c_int64, _i32p, _i32p, _f64p, C.c_int32, _f64p,
                                     C.c_int32, C.c_int32, C.c_double, _f64p]
         L.or_collision_apply.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, _f64p, C.c_int]
+        L.or_hat_create.restype = C.c_void_p
+        L.or_hat_create.argtypes = [C.c_int, _i32p]
+        L.or_hat_free.argtypes = [C.c_void_p]
+        L.or_hat_apply.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.or_hat_apply_t.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.or_hat_stencil.restype = C.c_int64
+        L.or_hat_stencil.argtypes = [C.c_int, _i32p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_collision_apply_occ.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, C.c_void_p, _f64p,
+                                             C.c_int]
+        L.or_collision_general_occ.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, C.c_double, _f64p,
+                                               C.c_void_p, _f64p]
+        L.or_collision_vjp_occ.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, C.c_void_p, _f64p,
+                                           _f64p]
         L.or_conservation.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.or_relative_error.restype = C.c_double
@@ -113,6 +126,9 @@ def lib():
         _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.or_collision_apply_band.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                               _i32p, C.c_double, _f64p, _f64p, _f64p, C.c_int]
+        L.or_collision_apply_band_occ.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                                  _i32p, C.c_double, _f64p, C.c_void_p, C.c_int, _f64p,
+                                                  _f64p, C.c_int]
         L.or_dp5_replay_ex.argtypes = [C.c_int64, _f64p, C.c_double, C.c_double, C.c_double, RHS_FN,
                                        C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _i8p, C.c_int64,
                                        C.c_void_p, C.c_void_p, C.c_int32]
@@ -142,6 +158,10 @@ def lib():
                                      C.c_double, _f64p, C.c_double, C.c_double, C.c_double, _f64p,
                                      C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.or_dp5_adjoint_occ.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                         C.c_double, C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double,
+                                         _f64p, C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
         L.or_mc_rows.restype = C.c_int32
         L.or_mc_rows.argtypes = [C.c_int, C.c_int]
         L.or_mc_points.argtypes = [C.c_int, C.c_int, C.c_int32, C.c_uint64, _f64p]
@@ -298,13 +318,59 @@ class Kernel:
         return K
 
 
+# ------------------------------------------------- hat-averaged occupation
+OCC_CONSERVE = 1   # bits of the DP5 wrappers' mode argument (OR_CONSERVE, OR_OCC_HAT)
+OCC_HAT = 2
+
+
+class HatStencil:
+    """fbar = S f, fbar_i = int phi_i f / int phi_i of the P1 interpolant on the
+    structured mesh (consistent mass over lumped mass)."""
+
+    def __init__(self, dim, nsec):
+        self.dim, self.nsec = dim, list(nsec)
+        self.n = num_nodes(dim, nsec)
+        self.h = lib().or_hat_create(dim, _nsec(nsec))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.or_hat_free(self.h)
+            self.h = None
+
+    def apply(self, f):
+        out = np.zeros(self.n)
+        lib().or_hat_apply(self.h, np.ascontiguousarray(f, np.float64), out)
+        return out
+
+    def apply_t(self, z):
+        out = np.zeros(self.n)
+        lib().or_hat_apply_t(self.h, np.ascontiguousarray(z, np.float64), out)
+        return out
+
+    def csr(self):
+        nnz = lib().or_hat_stencil(self.dim, _nsec(self.nsec), None, None, None)
+        ptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(nnz, np.int32)
+        w = np.zeros(nnz)
+        lib().or_hat_stencil(self.dim, _nsec(self.nsec), _ptr(ptr), _ptr(col), _ptr(w))
+        return ptr, col, w
+
+
+def _occ_mode(conserve, occupation):
+    if occupation not in ("nodal", "hat"):
+        raise ValueError(f"occupation must be 'nodal' or 'hat', not {occupation!r}")
+    return (OCC_CONSERVE if conserve else 0) | (OCC_HAT if occupation == "hat" else 0)
+
+
 # ---------------------------------------------------------------- collision
-def collision_apply(K, V, bath_n, f, conserve=False):
+def collision_apply(K, V, bath_n, f, conserve=False, hat=None):
+    """The W-form; hat (a HatStencil): the test function's occupation factors
+    are the hat averages of f instead of the nodal values."""
     K = np.ascontiguousarray(K, np.float64)
     f = np.ascontiguousarray(f, np.float64)
     out = np.zeros_like(f)
-    lib().or_collision_apply(f.size, K, np.ascontiguousarray(V, np.float64), bath_n, f, out,
-                             int(bool(conserve)))
+    lib().or_collision_apply_occ(f.size, K, np.ascontiguousarray(V, np.float64), bath_n, f,
+                                 None if hat is None else hat.h, out, int(bool(conserve)))
     return out
 
 
@@ -341,13 +407,14 @@ class KernelBand:
         self.Vs = np.ascontiguousarray(V[self.perm], np.float64)
         self.entries = int(self.rowptr[-1])
 
-    def apply(self, bath_n, f, nthreads=1, scratch=None):
+    def apply(self, bath_n, f, nthreads=1, scratch=None, hat=None, conserve=False):
         f = np.ascontiguousarray(f, np.float64)
         out = np.empty_like(f)
         if scratch is None:
-            scratch = np.empty((nthreads + 2) * self.n)
-        lib().or_collision_apply_band(self.n, self.rowptr, self.lo, self.vals, self.rk, self.ck, self.Vs,
-                                      self.perm, bath_n, f, out, scratch, nthreads)
+            scratch = np.empty((nthreads + 3) * self.n)
+        lib().or_collision_apply_band_occ(self.n, self.rowptr, self.lo, self.vals, self.rk, self.ck, self.Vs,
+                                          self.perm, bath_n, f, None if hat is None else hat.h,
+                                          int(bool(conserve)), out, scratch, nthreads)
         return out
 
 
@@ -446,7 +513,8 @@ def dp5_replay(rhs, y0, t0, t_final, dense_dt, steps, max_dense=100000):
 
 
 def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol,
-                  conserve=False, max_dense=100000, log_cap=1 << 20, sched=None, eps=None):
+                  conserve=False, max_dense=100000, log_cap=1 << 20, sched=None, eps=None,
+                  occupation="nodal"):
     """Adaptive DP5 on the W-form.  sched (BathSchedule): time-dependent bath
     instead of bath_n.  eps (nodal energies): also return the bath energy
     ledger ("ledger": the energy the bath received)."""
@@ -467,7 +535,7 @@ def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol
                               1.0 if sched is None else sched.dt,
                               0.0 if sched is None else sched.atol,
                               0.0 if sched is None else sched.rtol, f, t0, t_final, dense_dt,
-                              rtol, atol, int(bool(conserve)), _ptr(dense), _ptr(dt), max_dense,
+                              rtol, atol, _occ_mode(conserve, occupation), _ptr(dense), _ptr(dt), max_dense,
                               _ptr(hl), _ptr(al), log_cap, C.byref(st),
                               C.byref(led) if e is not None else None)
     nd = min(st.n_dense, max_dense)
@@ -477,7 +545,7 @@ def dp5_collision(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, rtol, atol
 
 
 def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, accepted,
-                         conserve=False, max_dense=100000, sched=None, eps=None):
+                         conserve=False, max_dense=100000, sched=None, eps=None, occupation="nodal"):
     """Fixed-step DP5 on the recorded attempts (h, accepted); sched: a
     BathSchedule instead of bath_n; with eps the bath energy ledger of the
     replayed steps is returned too ("ledger")."""
@@ -495,33 +563,34 @@ def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, 
                                      0 if sched is None else sp.size,
                                      0.0 if sched is None else sched.t_start,
                                      1.0 if sched is None else sched.dt, f, t0, t_final,
-                                     dense_dt, int(bool(conserve)), h, acc, h.size, _ptr(dense),
+                                     dense_dt, _occ_mode(conserve, occupation), h, acc, h.size, _ptr(dense),
                                      _ptr(dt), max_dense, C.byref(led) if e is not None else None)
     nd = int(np.count_nonzero(dt))
     return dict(y=f, dense=dense[:nd], dense_t=dt[:nd], ledger=led.value)
 
 
-def collision_vjp(K, V, bath_n, f, mu):
+def collision_vjp(K, V, bath_n, f, mu, hat=None):
     out = np.zeros_like(np.asarray(f, np.float64))
-    lib().or_collision_vjp(out.size, np.ascontiguousarray(K, np.float64),
-                           np.ascontiguousarray(V, np.float64), bath_n,
-                           np.ascontiguousarray(f, np.float64), np.ascontiguousarray(mu, np.float64),
-                           out)
+    lib().or_collision_vjp_occ(out.size, np.ascontiguousarray(K, np.float64),
+                               np.ascontiguousarray(V, np.float64), bath_n,
+                               np.ascontiguousarray(f, np.float64), None if hat is None else hat.h,
+                               np.ascontiguousarray(mu, np.float64), out)
     return out
 
 
-def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL_df, ckpt=32):
+def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL_df, ckpt=32, hat=None):
     """Discrete adjoint of the fixed-step DP5 with dense output at t_final.
     Returns (dL/df0, dL/dg, dL/domega0)."""
     n = len(f0)
     out = np.zeros(n)
     dg, dw = C.c_double(), C.c_double()
     steps = np.ascontiguousarray(steps, np.float64)
-    lib().or_dp5_adjoint(n, np.ascontiguousarray(K, np.float64), np.ascontiguousarray(Kd, np.float64),
-                         np.ascontiguousarray(V, np.float64), bath_n, dbath, g,
-                         np.ascontiguousarray(f0, np.float64), t0, t_final, dense_dt, steps,
-                         steps.size, ckpt, np.ascontiguousarray(dL_df, np.float64), out,
-                         C.byref(dg), C.byref(dw))
+    lib().or_dp5_adjoint_occ(n, np.ascontiguousarray(K, np.float64), np.ascontiguousarray(Kd, np.float64),
+                             np.ascontiguousarray(V, np.float64), bath_n, dbath, g,
+                             None if hat is None else hat.h,
+                             np.ascontiguousarray(f0, np.float64), t0, t_final, dense_dt, steps,
+                             steps.size, ckpt, np.ascontiguousarray(dL_df, np.float64), out,
+                             C.byref(dg), C.byref(dw))
     return out, dg.value, dw.value
 
 

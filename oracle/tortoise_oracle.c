@@ -115,6 +115,91 @@ void or_node_volumes(int dim, const int32_t* nsec, const double* extent, double*
     free(conn);
 }
 
+/* ============================================== hat-averaged occupation
+ * fbar_i = int phi_i f / int phi_i for the P1 interpolant f of nodal values:
+ * the consistent mass matrix over the lumped one.  On a simplex with d+1
+ * vertices int phi_a phi_b = vol (1 + delta_ab) / ((d+1)(d+2)) and
+ * int phi_a = vol/(d+1), so row i of the stencil collects, over the elements
+ * e holding i, (1 + delta_ib) / ((d+2) |star(i)|) for each vertex b of e. */
+or_hat* or_hat_create(int dim, const int32_t* nsec) {
+    const int64_t n = or_num_nodes(dim, nsec), nel = or_num_elements(dim, nsec);
+    const int nv = dim + 1;
+    int32_t* conn = (int32_t*)malloc(sizeof(int32_t) * nel * nv);
+    or_mesh_elements(dim, nsec, conn);
+    /* dense accumulation per node over a small neighbour list */
+    int32_t* star = (int32_t*)calloc(n, sizeof(int32_t));
+    for (int64_t e = 0; e < nel; ++e)
+        for (int a = 0; a < nv; ++a) ++star[conn[e * nv + a]];
+    /* at most (2^dim - 1) * 2 + 1 distinct vertices share an element with a node */
+    const int cap = dim == 2 ? 7 : 27;
+    int32_t* nb = (int32_t*)malloc(sizeof(int32_t) * n * cap);
+    double* wt = (double*)calloc(n * cap, sizeof(double));
+    int32_t* cnt = (int32_t*)calloc(n, sizeof(int32_t));
+    for (int64_t e = 0; e < nel; ++e)
+        for (int a = 0; a < nv; ++a) {
+            const int32_t i = conn[e * nv + a];
+            const double scale = 1.0 / ((double)(dim + 2) * (double)star[i]);
+            for (int b = 0; b < nv; ++b) {
+                const int32_t m = conn[e * nv + b];
+                const double w = (b == a ? 2.0 : 1.0) * scale;
+                int k = 0;
+                while (k < cnt[i] && nb[i * cap + k] != m) ++k;
+                if (k == cnt[i]) { nb[i * cap + k] = m; ++cnt[i]; }
+                wt[i * cap + k] += w;
+            }
+        }
+    or_hat* h = (or_hat*)malloc(sizeof(or_hat));
+    h->n = n;
+    h->ptr = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    h->ptr[0] = 0;
+    for (int64_t i = 0; i < n; ++i) h->ptr[i + 1] = h->ptr[i] + cnt[i];
+    h->col = (int32_t*)malloc(sizeof(int32_t) * h->ptr[n]);
+    h->w = (double*)malloc(sizeof(double) * h->ptr[n]);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < cnt[i]; ++k) {
+            h->col[h->ptr[i] + k] = nb[i * cap + k];
+            h->w[h->ptr[i] + k] = wt[i * cap + k];
+        }
+    free(conn); free(star); free(nb); free(wt); free(cnt);
+    return h;
+}
+
+/* Number-defect weight of the hat form: node i's largest gross rate at its
+ * own occupation x, (|1-x| (W 1)_i + |x| (W^T 1)_i) / V_i, with
+ * W = (n+1) K + n K^T, rk = K 1 and ck = K^T 1. */
+double or_hat_defect_weight(double x, double np1, double nb, double rk, double ck, double V) {
+    return (fabs(1.0 - x) * (np1 * rk + nb * ck) + fabs(x) * (np1 * ck + nb * rk)) / V;
+}
+
+void or_hat_free(or_hat* h) {
+    if (!h) return;
+    free(h->ptr); free(h->col); free(h->w); free(h);
+}
+
+void or_hat_apply(const or_hat* h, const double* f, double* fbar) {
+    for (int64_t i = 0; i < h->n; ++i) {
+        double s = 0.0;
+        for (int64_t k = h->ptr[i]; k < h->ptr[i + 1]; ++k) s += h->w[k] * f[h->col[k]];
+        fbar[i] = s;
+    }
+}
+
+void or_hat_apply_t(const or_hat* h, const double* z, double* out) {
+    for (int64_t i = 0; i < h->n; ++i) out[i] = 0.0;
+    for (int64_t i = 0; i < h->n; ++i)
+        for (int64_t k = h->ptr[i]; k < h->ptr[i + 1]; ++k) out[h->col[k]] += h->w[k] * z[i];
+}
+
+int64_t or_hat_stencil(int dim, const int32_t* nsec, int64_t* ptr, int32_t* col, double* w) {
+    or_hat* h = or_hat_create(dim, nsec);
+    const int64_t nnz = h->ptr[h->n];
+    if (ptr) memcpy(ptr, h->ptr, sizeof(int64_t) * (h->n + 1));
+    if (col) memcpy(col, h->col, sizeof(int32_t) * nnz);
+    if (w) memcpy(w, h->w, sizeof(double) * nnz);
+    or_hat_free(h);
+    return nnz;
+}
+
 /* ======================================================= energy projection */
 void or_egrid_make(const double* eps, int64_t n, double de, or_egrid* g) {
     double lo = eps[0], hi = eps[0];
@@ -364,6 +449,21 @@ void or_kernel_full(int64_t n, const int32_t* off, const int32_t* len,
 /* ========================================================= collision */
 void or_collision_apply(int64_t n, const double* K, const double* V,
                         double bath_n, const double* f, double* I, int conserve) {
+    or_collision_apply_occ(n, K, V, bath_n, f, NULL, I, conserve);
+}
+
+/* The occupation factors of the node the row belongs to (the test function's
+ * node: (1 - f_i) of the gain, f_i of the loss) are x = fbar with a hat
+ * stencil, else the nodal f; the products with K use the nodal f. */
+void or_collision_apply_occ(int64_t n, const double* K, const double* V, double bath_n,
+                            const double* f, const or_hat* hat, double* I, int conserve) {
+    double* xb = NULL;
+    const double* x = f;
+    if (hat) {
+        xb = (double*)malloc(sizeof(double) * n);
+        or_hat_apply(hat, f, xb);
+        x = xb;
+    }
     double* u = (double*)calloc(n, sizeof(double));
     double* rk = (double*)calloc(n, sizeof(double));
     double* v = (double*)calloc(n, sizeof(double));
@@ -402,18 +502,20 @@ void or_collision_apply(int64_t n, const double* K, const double* V,
         double wf = np1 * u[i] + bath_n * v[i];        /* (W f)_i   */
         double wt1 = np1 * ck[i] + bath_n * rk[i];     /* (W^T 1)_i */
         double wtf = np1 * v[i] + bath_n * u[i];       /* (W^T f)_i */
-        I[i] = ((1.0 - f[i]) * wf - f[i] * (wt1 - wtf)) / V[i];
+        I[i] = ((1.0 - x[i]) * wf - x[i] * (wt1 - wtf)) / V[i];
+        if (hat) xb[i] = or_hat_defect_weight(x[i], np1, bath_n, rk[i], ck[i], V[i]);   /* x[i] is not read again */
     }
     if (conserve) {
+        /* weights: the hat form's (in xb), else |f(1-f)| */
         double d = 0.0, wsum = 0.0;
         for (int64_t i = 0; i < n; ++i) {
             d += V[i] * I[i];
-            wsum += V[i] * fabs(f[i] * (1.0 - f[i]));
+            wsum += V[i] * (hat ? xb[i] : fabs(x[i] * (1.0 - x[i])));
         }
         if (wsum > 0.0)
-            for (int64_t i = 0; i < n; ++i) I[i] -= d * fabs(f[i] * (1.0 - f[i])) / wsum;
+            for (int64_t i = 0; i < n; ++i) I[i] -= d * (hat ? xb[i] : fabs(x[i] * (1.0 - x[i]))) / wsum;
     }
-    free(u); free(v); free(rk); free(ck);
+    free(u); free(v); free(rk); free(ck); free(xb);
 }
 
 /* The same operator from K's band in energy order: row p (node perm[p]) holds
@@ -425,9 +527,24 @@ void or_collision_apply(int64_t n, const double* K, const double* V,
 void or_collision_apply_band(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
                              const double* rk, const double* ck, const double* Vs, const int32_t* perm,
                              double bath_n, const double* f, double* I, double* scratch, int nthreads) {
+    or_collision_apply_band_occ(n, rowptr, lo, vals, rk, ck, Vs, perm, bath_n, f, NULL, 0, I, scratch, nthreads);
+}
+
+/* ... with the occupation factors of or_collision_apply_occ (hat NULL: the
+ * nodal f) and the number-defect removal; scratch (nthreads + 3) * n */
+void or_collision_apply_band_occ(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
+                                 const double* rk, const double* ck, const double* Vs, const int32_t* perm,
+                                 double bath_n, const double* f, const or_hat* hat, int conserve, double* I,
+                                 double* scratch, int nthreads) {
     const int nt = nthreads > 0 ? nthreads : 1;
     double* fs = scratch + (size_t)nt * n;       /* f in energy order */
     double* u = fs + n;
+    double* xb = u + n;                          /* occupation factors, caller order */
+    const double* x = f;
+    if (hat) {
+        or_hat_apply(hat, f, xb);
+        x = xb;
+    }
     for (int64_t p = 0; p < n; ++p) fs[p] = f[perm[p]];
 #pragma omp parallel num_threads(nt)
     {
@@ -455,7 +572,20 @@ void or_collision_apply_band(int64_t n, const int64_t* rowptr, const int32_t* lo
         const double wf = np1 * u[p] + bath_n * v;
         const double wt1 = np1 * ck[p] + bath_n * rk[p];
         const double wtf = np1 * v + bath_n * u[p];
-        I[perm[p]] = ((1.0 - fs[p]) * wf - fs[p] * (wt1 - wtf)) / Vs[p];
+        const double xp = x[perm[p]];
+        I[perm[p]] = ((1.0 - xp) * wf - xp * (wt1 - wtf)) / Vs[p];
+        if (hat) xb[perm[p]] = or_hat_defect_weight(xp, np1, bath_n, rk[p], ck[p], Vs[p]);
+    }
+    if (conserve) {
+        /* the same weights as or_collision_apply_occ, sums in energy order */
+        double d = 0.0, wsum = 0.0;
+        for (int64_t p = 0; p < n; ++p) {
+            const double xp = x[perm[p]];
+            d += Vs[p] * I[perm[p]];
+            wsum += Vs[p] * (hat ? xb[perm[p]] : fabs(xp * (1.0 - xp)));
+        }
+        if (wsum > 0.0)
+            for (int64_t i = 0; i < n; ++i) I[i] -= d * (hat ? xb[i] : fabs(x[i] * (1.0 - x[i]))) / wsum;
     }
 }
 
@@ -715,6 +845,7 @@ typedef struct coll_user {
     const or_bath_sched* bs;   /* non-NULL: the bath at the stage time */
     const double* eps;         /* ledger weights V_i eps_i come from V and eps */
     int conserve;
+    const or_hat* hat;         /* non-NULL: hat-averaged occupation factors */
     int32_t* conn;
     double rtol, atol;
 } coll_user;
@@ -722,7 +853,7 @@ typedef struct coll_user {
 static void coll_rhs(void* u, double t, const double* y, double* dydt) {
     coll_user* c = (coll_user*)u;
     const double nb = c->bs ? or_sched_value(c->bs->s, c->bs->ns, c->bs->t_start, c->bs->dt, t) : c->bath_n;
-    or_collision_apply(c->n, c->K, c->V, nb, y, dydt, c->conserve);
+    or_collision_apply_occ(c->n, c->K, c->V, nb, y, c->hat, dydt, c->conserve);
 }
 
 /* electron energy flux sum_i V_i eps_i dydt_i (sequential) */
@@ -761,7 +892,9 @@ void or_dp5_collision_ex(int dim, const int32_t* nsec, int64_t n, const double* 
     memset(&c, 0, sizeof(c));
     or_bath_sched bs = {sched, ns, t_start, sdt, satol, srtol};
     c.dim = dim; c.n = n; c.nel = or_num_elements(dim, nsec);
-    c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve & OR_CONSERVE;
+    or_hat* hat = (conserve & OR_OCC_HAT) ? or_hat_create(dim, nsec) : NULL;
+    c.hat = hat;
     c.bs = sched ? &bs : NULL;
     c.eps = eps;
     c.rtol = rtol; c.atol = atol;
@@ -772,6 +905,7 @@ void or_dp5_collision_ex(int dim, const int32_t* nsec, int64_t n, const double* 
                         (ledger && eps) ? coll_flux : NULL, ledger, dense_out, dense_t, max_dense,
                         h_log, acc_log, log_cap, stats);
     free(c.conn);
+    or_hat_free(hat);
 }
 
 void or_dp5_collision_replay(int dim, const int32_t* nsec, int64_t n,
@@ -833,15 +967,17 @@ void or_dp5_collision_replay_ex(int dim, const int32_t* nsec, int64_t n,
                                 const double* h_log, const int8_t* acc_log,
                                 int64_t n_attempts, double* dense_out, double* dense_t,
                                 int32_t max_dense, double* ledger) {
-    (void)dim; (void)nsec;
     coll_user c;
     memset(&c, 0, sizeof(c));
     or_bath_sched bs = {sched, ns, t_start, sdt, 0.0, 0.0};
-    c.n = n; c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve;
+    c.n = n; c.K = K; c.V = V; c.bath_n = bath_n; c.conserve = conserve & OR_CONSERVE;
+    or_hat* hat = (conserve & OR_OCC_HAT) ? or_hat_create(dim, nsec) : NULL;
+    c.hat = hat;
     c.bs = sched ? &bs : NULL;
     c.eps = eps;
     or_dp5_replay_ex(n, f, t0, t_final, dense_dt, coll_rhs, &c, (ledger && eps) ? coll_flux : NULL, ledger,
                      h_log, acc_log, n_attempts, dense_out, dense_t, max_dense);
+    or_hat_free(hat);
 }
 
 /* ============================================================ adjoint */
@@ -885,7 +1021,15 @@ static void dual_products(int64_t n, const double* K, int nv, const double* cons
 
 void or_collision_general(int64_t n, const double* K, const double* V, double alpha,
                           double beta, const double* f, double* I) {
+    or_collision_general_occ(n, K, V, alpha, beta, f, NULL, I);
+}
+
+void or_collision_general_occ(int64_t n, const double* K, const double* V, double alpha,
+                              double beta, const double* f, const or_hat* hat, double* I) {
     double* buf = (double*)malloc(sizeof(double) * 6 * n);
+    double* x = buf + 5 * n;
+    if (hat) or_hat_apply(hat, f, x);
+    else memcpy(x, f, sizeof(double) * n);
     double *one = buf, *u = buf + n, *v = buf + 2 * n, *rk = buf + 3 * n, *ck = buf + 4 * n;
     for (int64_t i = 0; i < n; ++i) one[i] = 1.0;
     const double* xs[2] = {f, one};
@@ -896,7 +1040,7 @@ void or_collision_general(int64_t n, const double* K, const double* V, double al
         double wf = alpha * u[i] + beta * v[i];
         double wt1 = alpha * ck[i] + beta * rk[i];
         double wtf = alpha * v[i] + beta * u[i];
-        I[i] = ((1.0 - f[i]) * wf - f[i] * (wt1 - wtf)) / V[i];
+        I[i] = ((1.0 - x[i]) * wf - x[i] * (wt1 - wtf)) / V[i];
     }
     free(buf);
 }
@@ -905,15 +1049,25 @@ void or_collision_general(int64_t n, const double* K, const double* V, double al
  * out_k = -m_k [(W f)_k + (W^T 1)_k - (W^T f)_k] + (W^T x)_k + (W y)_k */
 void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n,
                       const double* f, const double* mu, double* out) {
-    double* buf = (double*)malloc(sizeof(double) * 12 * n);
+    or_collision_vjp_occ(n, K, V, bath_n, f, NULL, mu, out);
+}
+
+/* With a hat stencil S the occupation factors are fbar = S f, and the term
+ * -m (W f + W^T 1 - W^T f) reaches f through S^T. */
+void or_collision_vjp_occ(int64_t n, const double* K, const double* V, double bath_n,
+                          const double* f, const or_hat* hat, const double* mu, double* out) {
+    double* buf = (double*)malloc(sizeof(double) * 14 * n);
     double *one = buf, *x = buf + n, *y = buf + 2 * n;
     double *uf = buf + 3 * n, *vf = buf + 4 * n, *u1 = buf + 5 * n, *v1 = buf + 6 * n;
     double *ux = buf + 7 * n, *vx = buf + 8 * n, *uy = buf + 9 * n, *vy = buf + 10 * n;
+    double *ob = buf + 11 * n, *z = buf + 12 * n, *zt = buf + 13 * n;
+    if (hat) or_hat_apply(hat, f, ob);
+    else memcpy(ob, f, sizeof(double) * n);
     for (int64_t i = 0; i < n; ++i) {
         double m = mu[i] / V[i];
         one[i] = 1.0;
-        x[i] = m * (1.0 - f[i]);
-        y[i] = m * f[i];
+        x[i] = m * (1.0 - ob[i]);
+        y[i] = m * ob[i];
     }
     const double* xs[4] = {f, one, x, y};
     double* us[4] = {uf, u1, ux, uy};
@@ -927,8 +1081,12 @@ void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n
         double wtf = np1 * vf[k] + bath_n * uf[k];
         double wtx = np1 * vx[k] + bath_n * ux[k];
         double wy = np1 * uy[k] + bath_n * vy[k];
-        out[k] = -m * (wf + wt1 - wtf) + wtx + wy;
+        z[k] = -m * (wf + wt1 - wtf);
+        out[k] = wtx + wy;
     }
+    if (hat) or_hat_apply_t(hat, z, zt);
+    else memcpy(zt, z, sizeof(double) * n);
+    for (int64_t k = 0; k < n; ++k) out[k] += zt[k];
     free(buf);
 }
 
@@ -936,10 +1094,11 @@ typedef struct adj_ctx {
     int64_t n;
     const double *K, *Kd, *V;
     double bath, dbath, g;
+    const or_hat* hat;
 } adj_ctx;
 
 static void adj_rhs(const adj_ctx* a, const double* y, double* out) {
-    or_collision_general(a->n, a->K, a->V, a->bath + 1.0, a->bath, y, out);
+    or_collision_general_occ(a->n, a->K, a->V, a->bath + 1.0, a->bath, y, a->hat, out);
 }
 
 /* mu^T dF/dtheta at state y: (dL/dg, dL/domega0) contributions */
@@ -952,8 +1111,8 @@ static void adj_param(const adj_ctx* a, const double* y, const double* mu, doubl
     for (int64_t i = 0; i < n; ++i) s += mu[i] * t1[i];
     *dg += 2.0 * s / a->g;
     /* W' = (n+1) Kd + n Kd^T + n' (K + K^T) */
-    or_collision_general(n, a->Kd, a->V, a->bath + 1.0, a->bath, y, t1);
-    or_collision_general(n, a->K, a->V, a->dbath, a->dbath, y, t2);
+    or_collision_general_occ(n, a->Kd, a->V, a->bath + 1.0, a->bath, y, a->hat, t1);
+    or_collision_general_occ(n, a->K, a->V, a->dbath, a->dbath, y, a->hat, t2);
     s = 0.0;
     for (int64_t i = 0; i < n; ++i) s += mu[i] * (t1[i] + t2[i]);
     *dw += s;
@@ -987,7 +1146,16 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
                     int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
                     double* dL_domega0) {
-    adj_ctx a = {n, K, Kd, V, bath_n, dbath, g};
+    or_dp5_adjoint_occ(n, K, Kd, V, bath_n, dbath, g, NULL, f0, t0, t_final, dense_dt, h_steps, n_steps,
+                       ckpt, dL_df, dL_df0, dL_dg, dL_domega0);
+}
+
+void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const double* V,
+                        double bath_n, double dbath, double g, const or_hat* hat, const double* f0,
+                        double t0, double t_final, double dense_dt, const double* h_steps,
+                        int64_t n_steps, int32_t ckpt, const double* dL_df, double* dL_df0,
+                        double* dL_dg, double* dL_domega0) {
+    adj_ctx a = {n, K, Kd, V, bath_n, dbath, g, hat};
     /* forward: find the step holding the last dense output and keep checkpoints */
     int64_t nck = n_steps / ckpt + 2;
     double* ck = (double*)malloc(sizeof(double) * n * nck);
@@ -1045,7 +1213,7 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                 for (int j = 0; j < 7; ++j)
                     for (int64_t i = 0; i < n; ++i) mu[j][i] = (s * h * bt[j]) * dL_df[i];
                 /* k7 = F(Y7), Y7 = y + sum_i (h b_i) k_i */
-                or_collision_vjp(n, K, V, bath_n, Y[6], mu[6], nu);
+                or_collision_vjp_occ(n, K, V, bath_n, Y[6], hat, mu[6], nu);
                 adj_param(&a, Y[6], mu[6], &dg, &dw);
                 for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
                 for (int j = 0; j < 6; ++j)
@@ -1056,13 +1224,13 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     for (int64_t i = 0; i < n; ++i) mu[j][i] = (TA[6][j] * h) * lam[i];
             }
             for (int st = 5; st >= 1; --st) {
-                or_collision_vjp(n, K, V, bath_n, Y[st], mu[st], nu);
+                or_collision_vjp_occ(n, K, V, bath_n, Y[st], hat, mu[st], nu);
                 adj_param(&a, Y[st], mu[st], &dg, &dw);
                 for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
                 for (int j = 0; j < st; ++j)
                     for (int64_t i = 0; i < n; ++i) mu[j][i] += (TA[st][j] * h) * nu[i];
             }
-            or_collision_vjp(n, K, V, bath_n, Y[0], mu[0], nu);
+            or_collision_vjp_occ(n, K, V, bath_n, Y[0], hat, mu[0], nu);
             adj_param(&a, Y[0], mu[0], &dg, &dw);
             for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
         }

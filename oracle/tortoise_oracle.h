@@ -87,12 +87,46 @@ void or_kernel_row(int64_t n, int64_t i, const int32_t* off, const int32_t* len,
                    const double* R, int32_t maxw, const double* gm,
                    int32_t m_lo, int32_t nm, double amp, double* Krow);
 
+/* ----------------------------------------------- hat-averaged occupation */
+/* fbar_i = int phi_i f / int phi_i of the P1 interpolant (consistent mass over
+ * lumped mass) as a CSR stencil on the structured mesh: fbar = S f.  New:
+ * the reference integrates the P1 products exactly per element tuple
+ * (IntegratorCore.hpp:298-316); the W-form takes the test function's
+ * occupation factor as this average instead of the nodal value. */
+typedef struct or_hat {
+    int64_t n;
+    int64_t* ptr;
+    int32_t* col;
+    double* w;
+} or_hat;
+or_hat* or_hat_create(int dim, const int32_t* nsec);
+void    or_hat_free(or_hat* h);
+void    or_hat_apply(const or_hat* h, const double* f, double* fbar);   /* S f */
+void    or_hat_apply_t(const or_hat* h, const double* z, double* out);  /* S^T z */
+/* copies the stencil out (NULL pointers: count only); returns its nonzeros */
+int64_t or_hat_stencil(int dim, const int32_t* nsec, int64_t* ptr, int32_t* col, double* w);
+
+/* Number-defect weight of the hat form: node i's largest gross rate at its
+ * own occupation x, (|1-x| (W 1)_i + |x| (W^T 1)_i) / V_i with
+ * W = (n+1) K + n K^T, rk = K 1, ck = K^T 1 (the nodal form keeps
+ * |f(1-f)|). */
+double  or_hat_defect_weight(double x, double np1, double nb, double rk, double ck, double V);
+
+/* bits of the `conserve` argument of the DP5 wrappers below */
+#define OR_CONSERVE 1   /* number-defect removal */
+#define OR_OCC_HAT  2   /* hat-averaged occupation factors */
+
 /* ---------------------------------------------------------- collision */
 /* I = [(1-f) (W f) - f (W^T 1 - W^T f)] / V,  W = (n+1)K + n K^T.
  * K row-major n x n.  If conserve != 0 the number defect sum_i V_i I_i is
  * removed with weights |f_i(1-f_i)| (the same rule the GPU paths use). */
 void or_collision_apply(int64_t n, const double* K, const double* V,
                         double bath_n, const double* f, double* I, int conserve);
+/* The same with the occupation factors of node i (1 - x_i of the gain, x_i of
+ * the loss) x = S f when hat != NULL; the number-defect removal then weighs
+ * node i with or_hat_defect_weight. */
+void or_collision_apply_occ(int64_t n, const double* K, const double* V, double bath_n,
+                            const double* f, const or_hat* hat, double* I, int conserve);
 
 /* or_collision_apply from K's band in energy order (the CPU form that reads
  * the same bytes as the GPU's plain evaluation): row p (node perm[p]) holds
@@ -101,6 +135,12 @@ void or_collision_apply(int64_t n, const double* K, const double* V,
 void or_collision_apply_band(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
                              const double* rk, const double* ck, const double* Vs, const int32_t* perm,
                              double bath_n, const double* f, double* I, double* scratch, int nthreads);
+/* ... with hat occupation factors (hat NULL: nodal) and the number-defect
+ * removal; scratch (nthreads + 3) * n doubles */
+void or_collision_apply_band_occ(int64_t n, const int64_t* rowptr, const int32_t* lo, const double* vals,
+                                 const double* rk, const double* ck, const double* Vs, const int32_t* perm,
+                                 double bath_n, const double* f, const or_hat* hat, int conserve, double* I,
+                                 double* scratch, int nthreads);
 
 /* sum_i V_i f_i and sum_i V_i eps_i f_i */
 void or_conservation(int64_t n, const double* V, const double* eps,
@@ -230,10 +270,14 @@ void or_kernel_domega_full(int64_t n, const int32_t* off, const int32_t* len,
 /* Collision operator with a general W = alpha K + beta K^T (no correction). */
 void or_collision_general(int64_t n, const double* K, const double* V, double alpha,
                           double beta, const double* f, double* I);
+void or_collision_general_occ(int64_t n, const double* K, const double* V, double alpha,
+                              double beta, const double* f, const or_hat* hat, double* I);
 
 /* Vector-Jacobian product of the operator at f: out = J(f)^T mu. */
 void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n,
                       const double* f, const double* mu, double* out);
+void or_collision_vjp_occ(int64_t n, const double* K, const double* V, double bath_n,
+                          const double* f, const or_hat* hat, const double* mu, double* out);
 
 /* Discrete adjoint of the fixed-step DP5 (same step sequence as
  * or_dp5_collision_replay, all accepted) with dense output at t_final:
@@ -245,6 +289,11 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
                     int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
                     double* dL_domega0);
+void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const double* V,
+                        double bath_n, double dbath, double g, const or_hat* hat, const double* f0,
+                        double t0, double t_final, double dense_dt, const double* h_steps,
+                        int64_t n_steps, int32_t ckpt, const double* dL_df, double* dL_df0,
+                        double* dL_dg, double* dL_domega0);
 
 /* ------------------------------------------------------- e-e channel */
 /* Electron-electron (4-leg) operator on the periodic lattice of the mesh box

@@ -1,0 +1,182 @@
+"""Pins of the oracle's hat-averaged occupation factors.
+
+The W-form takes the occupation factor of the test function's node (1 - f_i
+of the gain, f_i of the loss) as a single number per node.  With the nodal
+value f_i this lumps the P1 product over the hat's support; the reference
+integrates (1 - f(k)) and f(k) over the element tuples exactly
+(IntegratorCore.hpp:298-316, ScatteringChannel.cpp:618-650).  The hat average
+fbar_i = int phi_i f / int phi_i keeps the hat's mean occupation and removes
+the spurious energy diffusion of the lumping (the cross-check below).
+
+Pins:
+  * fbar against a quadrature that is exact for the quadratic phi_i f (edge
+    midpoints in 2D, the 4-point degree-2 rule in 3D), independent of the
+    closed-form mass matrix the stencil is built from;
+  * constants are kept, the P1 mass is kept, a linear field is kept at the
+    interior nodes (their stars are centrally symmetric);
+  * the operator's vector-Jacobian product and the DP5 adjoint with hat
+    factors against central differences;
+  * physics: at 16x16 sections the nodal W-form gives the electron cooling
+    power with the wrong sign against the reference operator (the oracle's
+    tuple restatement), the hat-averaged form within 25 %.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tortoise_amd import workloads as wl
+
+PI = math.pi
+
+
+def _quadrature_average(dim, nsec, extent, f):
+    """int phi_i f / int phi_i by a quadrature rule exact for quadratics."""
+    conn = oracle.mesh_elements(dim, list(nsec))
+    vol = oracle.element_volume(dim, list(nsec), list(extent))
+    if dim == 2:
+        # edge midpoints, weight 1/3 each
+        pts = np.array([[0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
+        wq = np.full(3, 1.0 / 3.0)
+    else:
+        a, b = 0.5854101966249685, 0.1381966011250105
+        pts = np.full((4, 4), b)
+        np.fill_diagonal(pts, a)
+        wq = np.full(4, 0.25)
+    n = oracle.num_nodes(dim, nsec)
+    num = np.zeros(n)
+    den = np.zeros(n)
+    fv = f[conn]                              # nel x (dim+1)
+    for lam, w in zip(pts, wq):
+        fq = fv @ lam                         # f at the point of every element
+        for a_ in range(dim + 1):
+            np.add.at(num, conn[:, a_], vol * w * lam[a_] * fq)
+            np.add.at(den, conn[:, a_], vol * w * lam[a_])
+    return num / den
+
+
+@pytest.mark.parametrize("dim,nsec", [(2, (7, 5)), (3, (3, 4, 2))])
+def test_hat_average_matches_quadrature(dim, nsec):
+    rng = np.random.default_rng(1)
+    extent = [2 * PI] * dim
+    n = oracle.num_nodes(dim, nsec)
+    f = rng.uniform(-0.5, 1.5, n)
+    S = oracle.HatStencil(dim, nsec)
+    fb = S.apply(f)
+    ref = _quadrature_average(dim, nsec, extent, f)
+    assert np.abs(fb - ref).max() <= 1e-14 * np.abs(ref).max()
+    # S^T is the transpose of S
+    z = rng.normal(size=n)
+    assert abs(z @ fb - S.apply_t(z) @ f) <= 1e-13 * np.abs(z).sum()
+
+
+@pytest.mark.parametrize("dim,nsec", [(2, (6, 6)), (3, (3, 3, 3))])
+def test_hat_average_invariants(dim, nsec):
+    rng = np.random.default_rng(2)
+    extent = [2 * PI] * dim
+    n = oracle.num_nodes(dim, nsec)
+    S = oracle.HatStencil(dim, nsec)
+    assert np.abs(S.apply(np.full(n, 0.3)) - 0.3).max() <= 1e-15
+    V = oracle.node_volumes(dim, list(nsec), extent)
+    f = rng.uniform(0.0, 1.0, n)
+    assert abs(V @ S.apply(f) - V @ f) <= 1e-13 * (V @ f)
+    X = oracle.mesh_nodes(dim, list(nsec), [-PI] * dim, extent)
+    lin = 0.3 + X @ np.arange(1, dim + 1)
+    idx = np.ravel_multi_index(np.meshgrid(*[np.arange(1, s) for s in reversed(nsec)], indexing="ij"),
+                               [s + 1 for s in reversed(nsec)]).ravel()
+    assert np.abs(S.apply(lin)[idx] - lin[idx]).max() <= 1e-13 * np.abs(lin).max()
+    ptr, col, w = S.csr()
+    assert ptr[-1] == col.size and np.all(w > 0)
+    assert np.diff(ptr).max() == (7 if dim == 2 else 15)
+
+
+@pytest.fixture(scope="module")
+def small():
+    nsec = (8, 8)
+    X = oracle.mesh_nodes(2, list(nsec), [-PI, -PI], [2 * PI, 2 * PI])
+    eps = 0.5 * (X ** 2).sum(1)
+    k = oracle.Kernel(2, list(nsec), [-PI, -PI], [2 * PI, 2 * PI], eps, 0.12, 0.045, 0.01)
+    return k, eps, k.full(), oracle.HatStencil(2, nsec)
+
+
+def test_hat_vjp_against_fd(small):
+    k, eps, K, S = small
+    rng = np.random.default_rng(3)
+    f = rng.uniform(0.05, 0.95, eps.size)
+    mu = rng.normal(size=f.size)
+    dirv = rng.normal(size=f.size)
+    h = 1e-6
+    Ip = oracle.collision_apply(K, k.V, 0.3, f + h * dirv, hat=S)
+    Im = oracle.collision_apply(K, k.V, 0.3, f - h * dirv, hat=S)
+    fd = mu @ (Ip - Im) / (2 * h)
+    ad = oracle.collision_vjp(K, k.V, 0.3, f, mu, hat=S) @ dirv
+    assert abs(fd - ad) <= 1e-7 * abs(ad)
+    # the hat term matters: the nodal VJP differs
+    assert abs(oracle.collision_vjp(K, k.V, 0.3, f, mu) @ dirv - ad) > 1e-3 * abs(ad)
+
+
+def test_hat_dp5_adjoint_against_fd(small):
+    k, eps, K, S = small
+    T = 0.02
+    bath = lambda w: 1.0 / math.expm1(w / T)
+    f0 = 1 / (np.exp((eps - 0.5) / 0.02) + 1) + 0.2 * np.exp(-0.5 * ((eps - 0.9) / 0.03) ** 2)
+    run = oracle.dp5_collision(2, k.nsec, K, k.V, bath(0.045), f0, 0.0, 4.0, 0.4, 1e-8, 1e-12,
+                               max_dense=100, occupation="hat")
+    steps = run["h"][run["accepted"]]
+    w8 = np.linspace(0.5, 1.5, eps.size)
+
+    def final(f):
+        r = oracle.dp5_collision_replay(2, k.nsec, K, k.V, bath(0.045), f, 0.0, 4.0, 0.4, steps,
+                                        np.ones(len(steps), bool), occupation="hat")
+        return r["dense"][-1]
+
+    fT = final(f0)
+    gf0, gg, _ = oracle.dp5_adjoint(K, k.domega_full(), k.V, bath(0.045), 0.0, 0.12, f0, 0.0, 4.0, 0.4,
+                                    steps, 2 * w8 * fT, ckpt=3, hat=S)
+    loss = lambda f: float(w8 @ (f * f))
+    for i in (5, 40, 70):
+        e = np.zeros_like(f0)
+        e[i] = 1e-6
+        fdi = (loss(final(f0 + e)) - loss(final(f0 - e))) / 2e-6
+        assert abs(gf0[i] - fdi) <= 1e-5 * max(abs(fdi), 1e-3 * np.abs(gf0).max()), (i, gf0[i], fdi)
+
+
+def _hat_moments(dim, nsec, extent, out):
+    vol = oracle.element_volume(dim, list(nsec), list(extent))
+    m = out @ np.linalg.inv(oracle.mc_inverse_mass(dim, vol))
+    conn = oracle.mesh_elements(dim, list(nsec))
+    H = np.zeros(oracle.num_nodes(dim, nsec))
+    np.add.at(H, conn[:, 0], m[:, 0] - m[:, 1:].sum(1))
+    for a in range(1, dim + 1):
+        np.add.at(H, conn[:, a], m[:, a])
+    return H
+
+
+def test_hat_occupation_fixes_the_cooling_power_at_16_sections():
+    """Electron cooling power sum_i eps_i <phi_i, I> of a hot Fermi-Dirac
+    state (T = 1, mu = 0.5) against cold phonons (n = 0.5), 16x16 sections:
+    the reference operator (tuple restatement, NMC 300) against the W-form
+    with sigma = 0.0025, nodal and hat-averaged factors (the latter with the
+    number-defect removal, as the DP5 runs it)."""
+    w = wl.Workload("sq16", 2, (16, 16))
+    eps = w.eps()
+    f = 1.0 / (np.exp((eps - 0.5) / 1.0) + 1.0)
+    nb = 0.5
+    pts = oracle.mc_points(2, 3, 300, 0)
+    idx = oracle.mc_tuples_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, 1, pts)
+    fdg = oracle.mc_nodal_to_dg(2, w.nsec, f)
+    fph = np.zeros_like(fdg)
+    fph[:, 0] = nb
+    out = oracle.mc_apply_eph(2, w.nsec, w.origin, w.extent, eps, w.omega0, w.g ** 2, 1, pts, idx, fdg, fph)
+    p_ref = eps @ _hat_moments(2, w.nsec, w.extent, out)
+    k = oracle.Kernel(2, list(w.nsec), list(w.origin), list(w.extent), eps, w.g, w.omega0, 0.0025)
+    K = k.full()
+    S = oracle.HatStencil(2, w.nsec)
+    p_nodal = eps @ (k.V * oracle.collision_apply(K, k.V, nb, f))
+    I_hat = oracle.collision_apply(K, k.V, nb, f, conserve=True, hat=S)
+    assert abs(k.V @ I_hat) <= 1e-13 * np.abs(k.V * I_hat).sum()
+    p_hat = eps @ (k.V * I_hat)
+    assert p_ref < 0
+    assert p_nodal / p_ref < 0, (p_nodal, p_ref)
+    assert 0.75 <= p_hat / p_ref <= 1.25, (p_hat, p_ref)

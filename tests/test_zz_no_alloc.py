@@ -58,6 +58,28 @@ def test_noalloc_matches_apply_on_two_meshes(gpu_available):
     c2.close()
 
 
+def test_noalloc_hat_occupation_matches_apply(gpu_available):
+    """With hat-averaged occupation factors the allocation-free evaluation
+    keeps them in its scratch too: bitwise equal to tg_collision_apply, on
+    a 3D mesh then a smaller 2D one."""
+    cube = wl.Workload("cube6", 3, (6, 6, 6), sigma=0.02)
+    small = wl.Workload("square16", 2, (16, 16))
+    print("[gpu] no-alloc evaluation with hat factors, cube6 then square16", flush=True)
+    c1 = tg.Context.create(cube, max_batch=2, occupation="hat")
+    c1.assemble()
+    c2 = tg.Context.create(small, max_batch=1, occupation="hat")
+    c2.assemble()
+    F1, b1 = _members(cube, 2, 3)
+    for conserve in (False, True):
+        a, b = _run(c1, F1, b1, conserve)
+        assert np.array_equal(a, b)
+    F2, b2 = _members(small, 1, 4)
+    a, b = _run(c2, F2, b2, True)
+    assert np.array_equal(a, b)
+    c1.close()
+    c2.close()
+
+
 def test_noalloc_refuses_what_it_does_not_do(gpu_available):
     import torch
 

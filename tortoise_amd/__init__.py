@@ -35,6 +35,9 @@ TG_ERR_NOMEM = -6
 TG_PTR_DEVICE = 0x1
 TG_PREC_MIXED = 0x2
 TG_CONSERVE = 0x4
+TG_OCC_NODAL = 0
+TG_OCC_HAT = 1
+OCCUPATIONS = {"nodal": TG_OCC_NODAL, "hat": TG_OCC_HAT}
 TG_QUIET = 0x8
 TG_TIME_KERNEL = 0x10
 TG_BATH_SCHEDULE = 0x20
@@ -67,7 +70,7 @@ class ProblemDesc(C.Structure):
         ("extent", C.c_double * 3), ("eps", C.POINTER(C.c_double)), ("g", C.c_double),
         ("omega0", C.c_double), ("sigma", C.c_double), ("bath_n", C.c_double),
         ("de", C.c_double), ("cutoff", C.c_double), ("max_batch", C.c_int32),
-        ("reserved", C.c_int32), ("ee_g", C.c_double), ("ee_kappa2", C.c_double),
+        ("occupation", C.c_int32), ("ee_g", C.c_double), ("ee_kappa2", C.c_double),
         ("ee_sigma", C.c_double), ("ee_dw", C.c_double),
     ]
 
@@ -234,7 +237,12 @@ class Context:
 
     @classmethod
     def create(cls, workload, max_batch=None, eps=None, g=None, omega0=None, sigma=None,
-               bath_n=None, de=0.0, cutoff=0.0, ee_g=None, ee_dw=0.0):
+               bath_n=None, de=0.0, cutoff=0.0, ee_g=None, ee_dw=0.0, occupation="nodal"):
+        """occupation: the test node's occupation factors, "nodal" (f_i, the
+        lumped W-form) or "hat" (the hat average of the P1 f; pair it with
+        conserve=True for exact number conservation)."""
+        if occupation not in OCCUPATIONS:
+            raise ValueError(f"occupation must be one of {sorted(OCCUPATIONS)}, not {occupation!r}")
         L = load()
         w = workload
         e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
@@ -256,6 +264,7 @@ class Context:
         d.ee_kappa2 = w.ee_kappa2
         d.ee_sigma = w.ee_sigma
         d.ee_dw = ee_dw
+        d.occupation = OCCUPATIONS[occupation]
         h = C.c_void_p()
         rc = L.tg_ctx_create(C.byref(d), C.byref(h))
         if rc != TG_OK:
@@ -265,6 +274,7 @@ class Context:
             raise TgError(rc, msg)
         ctx = cls(h, e.size, w)
         ctx.eps = e
+        ctx.occupation = occupation
         return ctx
 
     def _check(self, rc):

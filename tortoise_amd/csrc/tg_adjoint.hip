@@ -52,13 +52,15 @@ __global__ void vjp_prep_kernel(int64_t n, const double* __restrict__ mu,
     y[i] = m * f[i];
 }
 
+// with z (hat occupation factors) the first term goes to z (it reaches f
+// through S^T) and out gets the rest
 __global__ void vjp_out_kernel(int64_t n, const double* __restrict__ mu,
                                const double* __restrict__ V, double nb,
                                const double* __restrict__ uf, const double* __restrict__ vf,
                                const double* __restrict__ rk, const double* __restrict__ ck,
                                const double* __restrict__ ux, const double* __restrict__ vx,
                                const double* __restrict__ uy, const double* __restrict__ vy,
-                               double* __restrict__ out) {
+                               double* __restrict__ out, double* __restrict__ z) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double np1 = nb + 1.0;
@@ -68,7 +70,31 @@ __global__ void vjp_out_kernel(int64_t n, const double* __restrict__ mu,
     const double wtf = np1 * vf[k] + nb * uf[k];
     const double wtx = np1 * vx[k] + nb * ux[k];
     const double wy = np1 * uy[k] + nb * vy[k];
-    out[k] = -m * (wf + wt1 - wtf) + wtx + wy;
+    const double zk = -m * (wf + wt1 - wtf);
+    if (z) {
+        z[k] = zk;
+        out[k] = wtx + wy;
+    } else {
+        out[k] = zk + wtx + wy;
+    }
+}
+
+// out_k += (S^T z)_k = sum_i S_ik z_i for the hat stencil S (the stencil is
+// symmetric in its pattern and its multiplicities, S_ik = mult_ik / D_i with
+// D_i the multiplicity sum of row i)
+__global__ void hat_transpose_kernel(int64_t n, int dim, const uint32_t* __restrict__ hs,
+                                     const double* __restrict__ z, double* __restrict__ out) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] += hat_value_dim(dim, hs, k, [&](int64_t i) { return z[i] / (double)hat_den(dim, hs, i); }) *
+              (double)hat_den(dim, hs, k);
+}
+
+// occupation factors S f of one state (caller order)
+__global__ void hat_apply_kernel(int64_t n, int dim, const uint32_t* __restrict__ hs, const double* __restrict__ f,
+                                 double* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = hat_value_dim(dim, hs, i, [&](int64_t j) { return f[j]; });
 }
 
 // per-block partials of mu^T F(Y) and mu^T F[W'](Y)
@@ -136,6 +162,7 @@ struct AdjWork {
     double *nu = nullptr, *lam = nullptr, *x = nullptr, *y = nullptr;
     double *uf = nullptr, *vf = nullptr, *ux = nullptr, *vx = nullptr, *uy = nullptr,
            *vy = nullptr, *udf = nullptr, *vdf = nullptr;
+    double *xbar = nullptr, *z = nullptr;   // hat occupation factors of a stage state, S^T's input
     double* seg = nullptr;    // (ckpt+1) x n step-start states of a segment
     double* ckp = nullptr;    // nck x n checkpoints
     double* part = nullptr;
@@ -151,7 +178,7 @@ void adjoint_destroy(tg_ctx* c) {
         dev_free(&w->mu[s]);
     }
     for (double** p : {&w->nu, &w->lam, &w->x, &w->y, &w->uf, &w->vf, &w->ux, &w->vx, &w->uy,
-                       &w->vy, &w->udf, &w->vdf, &w->seg, &w->ckp, &w->part, &w->acc})
+                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z, &w->seg, &w->ckp, &w->part, &w->acc})
         dev_free(p);
     delete w;
     c->adj = nullptr;
@@ -173,7 +200,7 @@ static tg_status adj_ensure(tg_ctx* c, int32_t ckpt, int32_t nck) {
         TG_TRY(dev_alloc(c, &w->mu[s], n));
     }
     for (double** p : {&w->nu, &w->lam, &w->x, &w->y, &w->uf, &w->vf, &w->ux, &w->vx, &w->uy,
-                       &w->vy, &w->udf, &w->vdf})
+                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z})
         TG_TRY(dev_alloc(c, p, n));
     TG_TRY(dev_alloc(c, &w->seg, n * (ckpt + 1)));
     TG_TRY(dev_alloc(c, &w->ckp, n * nck));
@@ -229,22 +256,38 @@ struct AdjRun {
         return TG_OK;
     }
     // nu = J(Y)^T mu, accumulate the parameter products
+    // With hat occupation factors xbar = S Y replaces Y in the factors of
+    // the rows' nodes, and -m (W Y + W^T 1 - W^T Y) reaches Y through S^T.
     tg_status vjp(const double* Yst, const double* mu) {
         const int64_t n = c->n;
+        const bool hat = c->occ == TG_OCC_HAT;
+        const double* occ = Yst;
+        if (hat) {
+            hipLaunchKernelGGL(hat_apply_kernel, dim3(grid()), dim3(256), 0, c->stream, n, c->dim, c->d_hat, Yst,
+                               w->xbar);
+            TG_CHECK_LAUNCH(c);
+            occ = w->xbar;
+        }
         TG_TRY(stack_products_device(c, Yst, w->uf, w->vf));
-        hipLaunchKernelGGL(vjp_prep_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, Yst,
+        hipLaunchKernelGGL(vjp_prep_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, occ,
                            c->d_V, w->x, w->y);
         TG_CHECK_LAUNCH(c);
         TG_TRY(stack_products_device(c, w->x, w->ux, w->vx));
         TG_TRY(stack_products_device(c, w->y, w->uy, w->vy));
         hipLaunchKernelGGL(vjp_out_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, c->d_V, nb,
-                           w->uf, w->vf, c->d_rk, c->d_ck, w->ux, w->vx, w->uy, w->vy, w->nu);
+                           w->uf, w->vf, c->d_rk, c->d_ck, w->ux, w->vx, w->uy, w->vy, w->nu,
+                           hat ? w->z : nullptr);
         TG_CHECK_LAUNCH(c);
+        if (hat) {
+            hipLaunchKernelGGL(hat_transpose_kernel, dim3(grid()), dim3(256), 0, c->stream, n, c->dim, c->d_hat,
+                               w->z, w->nu);
+            TG_CHECK_LAUNCH(c);
+        }
         // the e-e channel adds its own product (it depends on neither g nor omega0)
         if (c->ee) TG_TRY(ee_vjp_device(c, Yst, mu, w->nu));
         TG_TRY(stack_products_device(c, Yst, w->udf, w->vdf, true));
         const int nblk = (int)ceil_div(n, kAdjT);
-        hipLaunchKernelGGL(param_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, Yst,
+        hipLaunchKernelGGL(param_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, occ,
                            c->d_V, nb, dnb, w->uf, w->vf, c->d_rk, c->d_ck, w->udf, w->vdf,
                            c->d_rkd, c->d_ckd, w->part);
         TG_CHECK_LAUNCH(c);

@@ -245,6 +245,12 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     c->de = d->de > 0.0 ? d->de : d->sigma / 8.0;
     c->cutoff = d->cutoff > 0.0 ? d->cutoff : 8.0;
     c->max_batch = d->max_batch;
+    if (d->occupation != TG_OCC_NODAL && d->occupation != TG_OCC_HAT) {
+        c->err = "occupation must be TG_OCC_NODAL or TG_OCC_HAT";
+        *out = c;
+        return TG_ERR_ARG;
+    }
+    c->occ = d->occupation;
 
     auto fail = [&](tg_status s) {
         *out = c;  // keep the context so the caller can read the error
@@ -285,6 +291,15 @@ extern "C" tg_status tg_ctx_create(const tg_problem_desc* d, tg_ctx** out) {
     if (e == hipSuccess)
         e = hipMemcpy(c->d_conn, conn.data(), sizeof(int32_t) * conn.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    if (c->occ == TG_OCC_HAT) {
+        std::vector<uint32_t> hs;
+        if (!hat_build(md, c->n, c->nel, conn, hs))
+            return fail(set_error(c, TG_ERR_ARG, "mesh too large for the hat stencil"));
+        if ((s = dev_alloc(c, &c->d_hat, hs.size())) != TG_OK) return fail(s);
+        if ((s = dev_alloc(c, &c->d_xb, c->n)) != TG_OK) return fail(s);
+        e = hipMemcpy(c->d_hat, hs.data(), sizeof(uint32_t) * hs.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(set_error(c, TG_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    }
 
     // energy order of the nodes (stable: equal energies keep the node order)
     c->h_perm.resize(c->n);
@@ -340,6 +355,8 @@ extern "C" void tg_ctx_destroy(tg_ctx* c) {
     dev_free(&c->d_eps);
     dev_free(&c->d_V);
     dev_free(&c->d_conn);
+    dev_free(&c->d_hat);
+    dev_free(&c->d_xb);
     dev_free(&c->d_off);
     dev_free(&c->d_len);
     dev_free(&c->d_R);

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/tortoise_gpu.h"
+#include "tg_hat.hpp"
 #include "tg_kentry.hpp"
 
 namespace tg {
@@ -34,6 +35,11 @@ struct tg_ctx {
     double vol = 0;      // element volume
     double g = 0, omega0 = 0, sigma = 0, bath_n = 0, de = 0, cutoff = 8;
     int32_t max_batch = 1;
+    // occupation factors of the test node: TG_OCC_NODAL or TG_OCC_HAT; with
+    // TG_OCC_HAT d_hat holds the packed hat stencil (tg_hat.hpp)
+    int32_t occ = TG_OCC_NODAL;
+    uint32_t* d_hat = nullptr;
+    double* d_xb = nullptr;        // scratch: one state's occupation factors (caller order)
 
     // energy grid and projection
     double e0 = 0;
@@ -184,10 +190,11 @@ void stack_destroy(tg_ctx* c);
 tg_status noalloc_init(tg_ctx* c);
 tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double bath_n, int conserve,
                              const double* d_bath, double* d_flux);
-// from the block sums dpart (4 per block): with conserve I -= s |f(1-f)|
-// (number-defect removal); with flux the energy flux of the result
-tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I,
-                              int conserve, double* flux);
+// from the block sums dpart (4 per block): with conserve I -= s w
+// (number-defect removal; w = |f(1-f)|, or the hat form's weights d_wt when
+// given); with flux the energy flux of the result
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, const double* d_wt,
+                              double* d_I, int conserve, double* flux);
 tg_status mixed_prepare(tg_ctx* c);
 tg_status mixed_apply_device(tg_ctx* c, const double* d_f, double* d_I, int32_t batch,
                              const double* h_bath, int conserve);

@@ -76,6 +76,8 @@ struct MixedData {
     const int32_t* perm = nullptr;  // sorted position -> node (owned by the context)
     double* rk_s = nullptr;    // split-matrix K 1 (sorted order)
     double* ck_s = nullptr;    // split-matrix K^T 1 (sorted order)
+    double* rk_c = nullptr;    // the same in the caller's order (the hat form's defect weights)
+    double* ck_c = nullptr;
     const double* V_s = nullptr;    // node volumes (sorted order, owned by the context)
     const double* eps_s = nullptr;  // node energies (sorted order, owned by the context)
     uint8_t* P = nullptr;      // nsplit x nt x rows_pad partial sums (FP64, FP32 if fold == 0)
@@ -112,6 +114,8 @@ struct MixedData {
     size_t P_cap = 0;          // bytes
     double* bath = nullptr;    // per-member bath occupation (device)
     std::vector<double> bath_set;   // host copy of what `bath` holds (empty: unknown)
+    double* xb = nullptr;      // hat-averaged occupation factors of the packed states (nt x n,
+                               // member-major; null with nodal factors)
     bool rowcache = true;      // member rows fit in LDS
     int64_t work_items = 0;
 };
@@ -250,7 +254,20 @@ struct PackOut {
     uint8_t* nzG;
     uint8_t* useF;          // per panel: some live member runs it with f (g)
     uint8_t* useG;
+    // hat-averaged occupation factors of the packed state, member-major n
+    // values per member, from the stencil hs (null: nodal factors, not written)
+    const uint32_t* hs;
+    double* xb;
+    int dim;
 };
+
+// member b's occupation factors x = S f from its state in LDS `row` (caller
+// order, complete) into o.xb
+__device__ __forceinline__ void hat_from_row(const double* row, int64_t n, const PackOut& o, int b) {
+    double* __restrict__ xb = o.xb + (int64_t)b * n;
+    for (int64_t i = threadIdx.x; i < n; i += kMemT)
+        xb[i] = hat_value_dim(o.dim, o.hs, i, [&](int64_t j) { return row[j]; });
+}
 
 // the caller-order node at the middle of panel pn (K rows and K^T rows index
 // the same sorted nodes)
@@ -364,7 +381,13 @@ pack_member_kernel(const double* __restrict__ f, int64_t n, int B, const int32_t
             lds_barrier();
         }
         pack_from_row(row, pk, live, b, n, kgroups, npanel, nt, o);
+        if (o.xb && live) hat_from_row(row, n, o, b);
     } else {
+        if (o.xb && live) {
+            double* __restrict__ xb = o.xb + (int64_t)b * n;
+            for (int64_t i = threadIdx.x; i < n; i += kMemT)
+                xb[i] = hat_value_dim(o.dim, o.hs, i, [&](int64_t j) { return fb[j]; });
+        }
         for (int pn = threadIdx.x; pn < npanel; pn += kMemT) {
             const uint8_t v = live && fb[panel_node(pn, kt, n, perm)] > 0.5 ? 1 : 0;
             o.useg[(int64_t)pn * nt + b] = v;
@@ -844,8 +867,8 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
                                               double nb, const uint8_t* __restrict__ useg,
                                               const uint8_t* __restrict__ actp, int npanel, uint8_t* sflag,
                                               double inv_scale, const double* __restrict__ eps_s, bool want_e,
-                                              double* row, double* __restrict__ Ib, double& dI, double& dw,
-                                              double& eI, double& ew) {
+                                              bool hatw, double* row, double* __restrict__ Ib, double& dI,
+                                              double& dw, double& eI, double& ew) {
     const double np1 = nb + 1.0;
     // UNR positions per thread per round with all their loads issued before
     // use; the trip count is the same for every thread (the barrier after the
@@ -934,7 +957,9 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
             const double x = ROW ? row[node[j]] : fb[node[j]];
             // (1-f)(W f) - f (W^T g),  W = (n+1) K + n K^T
             const double Ii = ((1.0 - x) * (np1 * Kf + nb * KTf) - x * (np1 * KTg + nb * Kg)) / Vp[j];
-            const double vi = Vp[j] * Ii, vw = Vp[j] * fabs(x * (1.0 - x));
+            // number-defect weights: |x(1-x)|, or the hat form's
+            const double wj = hatw ? hat_defect_weight(x, np1, nb, rkp[j], ckp[j], Vp[j]) : fabs(x * (1.0 - x));
+            const double vi = Vp[j] * Ii, vw = Vp[j] * wj;
             dI += vi;
             dw += vw;
             if (want_e) {
@@ -1010,7 +1035,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
                      const uint8_t* __restrict__ actp, int npanel, double inv_scale, int conserve,
                      const double* __restrict__ gate,
                      double* __restrict__ I, const double* __restrict__ eps_s,
-                     double* __restrict__ flux) {
+                     double* __restrict__ flux, const double* __restrict__ rkc,
+                     const double* __restrict__ ckc, const double* __restrict__ Vc) {
     extern __shared__ double row[];
     __shared__ double red[4][kMemT / 64];
     __shared__ double s_b;
@@ -1021,8 +1047,10 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     const double* fb = f + (int64_t)b * n;
     double* Ib = I + (int64_t)b * n;
     double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
+    // f holds the hat form's occupation factors when rkc is given (the hat
+    // form's defect weights need the caller-order K 1, K^T 1 and V)
     finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, actp, npanel, sflag,
-                           inv_scale, eps_s, flux != nullptr, row, Ib, dI, dw, eI, ew);
+                           inv_scale, eps_s, flux != nullptr, rkc != nullptr, row, Ib, dI, dw, eI, ew);
     double sc = 0.0;
     if (conserve || flux) {
         block_sum4(dI, dw, eI, ew, red);
@@ -1039,7 +1067,8 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
         auto out = [&](int64_t i, double y) {
             if (!conserve) return y;
             const double x = fb[i];
-            return y - sc * fabs(x * (1.0 - x));
+            const double w = rkc ? hat_defect_weight(x, nb + 1.0, nb, rkc[i], ckc[i], Vc[i]) : fabs(x * (1.0 - x));
+            return y - sc * w;
         };
         constexpr int WUNR = 5;
         const int64_t h = row_head(Ib);
@@ -1062,6 +1091,15 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     }
 }
 
+// a[perm[p]] = as[p], b[perm[p]] = bs[p] (energy order to the caller's)
+__global__ void scatter_pair_kernel(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ as,
+                                    const double* __restrict__ bs, double* __restrict__ a, double* __restrict__ b) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    a[perm[p]] = as[p];
+    b[perm[p]] = bs[p];
+}
+
 // ------------------------------------------------- fused passes of the DP5
 // What a fused pass needs of the packed operator (by value).
 template <typename PT>
@@ -1079,6 +1117,10 @@ struct MixedView {
     const int32_t* slot;   // null: member b's column is b
     double inv_scale;
     PackOut o;
+    // the hat form's defect weights (with o.xb): K 1, K^T 1 and V in the caller's order
+    const double* rkc;
+    const double* ckc;
+    const double* Vc;
 };
 
 // Positions per thread per sweep of a pass's caller-order loop; all loads of
@@ -1112,12 +1154,18 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     const int col = v.slot ? v.slot[b] : b;
     const int64_t n = v.n;
     const int64_t base = (int64_t)b * n;
+    // the bath of the finished k (thread 0 rewrites it for the next stage
+    // behind a barrier below)
+    const double nbk = live ? a.bath[b] : 0.0;
     if (MODE != kDp5Stage && live) {
         // k_s = I(fin): the epilogue over the energy order, I in place of fin in LDS
         double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
-        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, col, a.fin + base, v.perm, v.rk, v.ck,
-                                v.V, a.bath[b], v.useg, v.actp, v.npanel, sflag, v.inv_scale, v.eps_s,
-                                a.flux != nullptr, row, nullptr, dI, dw, eI, ew);
+        // the occupation factors of fin: the hat averages the previous pass
+        // wrote, or fin itself
+        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, col, (v.o.xb ? v.o.xb : a.fin) + base,
+                                v.perm, v.rk, v.ck,
+                                v.V, nbk, v.useg, v.actp, v.npanel, sflag, v.inv_scale, v.eps_s,
+                                a.flux != nullptr, v.o.xb != nullptr, row, nullptr, dI, dw, eI, ew);
         block_sum4(dI, dw, eI, ew, red);
         if (threadIdx.x == 0) {
             s_b = (a.conserve && dw > 0.0) ? dI / dw : 0.0;
@@ -1144,7 +1192,7 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         const int ng = a.ng;
         const bool want_y4 = MODE == kDp5Mid && a.y4 != nullptr;
         for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * kPassUnr) {
-            double yv[kPassUnr], fv[kPassUnr], kv[6][kPassUnr], y4v[kPassUnr];
+            double yv[kPassUnr], fv[kPassUnr], xv[kPassUnr], kv[6][kPassUnr], y4v[kPassUnr];
 #pragma unroll
             for (int u = 0; u < kPassUnr; ++u) {
                 const int64_t i = base + min(i0 + (int64_t)u * kMemT, n - 1);
@@ -1155,6 +1203,8 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
                         if (j < ng) kv[j][u] = a.kg[j][i];
                 }
                 if (MODE != kDp5Stage && (a.conserve || MODE == kDp5Last)) fv[u] = a.fin[i];
+                // the number-defect weights |x(1-x)| use the occupation factors
+                if (MODE != kDp5Stage && a.conserve) xv[u] = v.o.xb ? v.o.xb[i] : fv[u];
                 if (MODE == kDp5Last) y4v[u] = a.y4[i];
             }
 #pragma unroll
@@ -1166,8 +1216,10 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
                 if (MODE != kDp5Stage) {
                     kn = row[p];
                     if (a.conserve) {
-                        const double x = fv[u];
-                        kn = kn - sc * fabs(x * (1.0 - x));
+                        const double x = xv[u];
+                        const double w = v.o.xb ? hat_defect_weight(x, nbk + 1.0, nbk, v.rkc[p], v.ckc[p], v.Vc[p])
+                                                : fabs(x * (1.0 - x));
+                        kn = kn - sc * w;
                     }
                     a.kout[i] = kn;
                 }
@@ -1199,6 +1251,9 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     __syncthreads();
     if constexpr (MODE != kDp5Last) {
         pack_from_row(row, pk, live, col, n, v.kgroups, v.npanel, v.nt, v.o);
+        // the next pass's finish reads these (every read of this member's
+        // factors in this pass is behind the barrier above)
+        if (v.o.xb && live) hat_from_row(row, n, v.o, b);
     } else {
         if (!a.norms) return;
         // sum over elements of sum_a r_a^2 + (sum_a r_a)^2 (the P1 integral of
@@ -1482,7 +1537,7 @@ static tg_status launch_gemm(tg_ctx* c, MixedData* m, bool dyn = false, const in
     return TG_OK;
 }
 
-static PackOut pack_out(MixedData* m, bool dyn) {
+static PackOut pack_out(MixedData* m, bool dyn, const uint32_t* hat_stencil, int hat_dim) {
     PackOut o;
     o.Fhi = (u32x4*)m->F_hi;
     o.Flo = (u32x4*)m->F_lo;
@@ -1493,6 +1548,9 @@ static PackOut pack_out(MixedData* m, bool dyn) {
     o.nzG = dyn ? m->nzG : nullptr;
     o.useF = dyn ? m->useF : nullptr;
     o.useG = dyn ? m->useG : nullptr;
+    o.hs = m->xb ? hat_stencil : nullptr;
+    o.xb = m->xb;
+    o.dim = hat_dim;
     return o;
 }
 
@@ -1500,7 +1558,7 @@ static tg_status launch_pack(tg_ctx* c, MixedData* m, const double* d_f, int B, 
                              bool dyn = false) {
     const int64_t n = c->n;
     const int64_t kgroups = m->kpad / 8;
-    const PackOut o = pack_out(m, dyn);
+    const PackOut o = pack_out(m, dyn, c->d_hat, c->dim);
 #define TG_PACK(ROW)                                                                                \
     hipLaunchKernelGGL(pack_member_kernel<ROW>, dim3(m->nt), dim3(kMemT), ROW ? sizeof(double) * n : 0, \
                        c->stream, d_f, n, B, m->perm, kgroups, m->kt, m->npanel, m->nt, o, gate)
@@ -1517,12 +1575,15 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     const int64_t n = c->n;
     const size_t rowbytes = m->rowcache ? sizeof(double) * (size_t)n : 0;
     const double inv_scale = std::ldexp(1.0, -(m->sa + kSF));
+    // the occupation factors of the rows' nodes: the hat averages the pack
+    // wrote, or the nodal state
+    const double* xf = m->xb ? m->xb : d_f;
 #define TG_FINISH(ROW, PT)                                                                           \
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
-                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, d_f,    \
+                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, xf,     \
                        m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, dyn ? m->actp : nullptr, \
                        m->npanel, inv_scale, conserve,                                               \
-                       gate, d_I, m->eps_s, flux)
+                       gate, d_I, m->eps_s, flux, m->xb ? m->rk_c : nullptr, m->ck_c, c->d_V)
     if (m->fold) {
         if (m->rowcache) TG_FINISH(true, double);
         else TG_FINISH(false, double);
@@ -1760,10 +1821,15 @@ tg_status mixed_prepare(tg_ctx* c) {
         }
         TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
         TG_TRY(dev_alloc(c, &m->ck_s, (size_t)n));
+        TG_TRY(dev_alloc(c, &m->rk_c, (size_t)n));
+        TG_TRY(dev_alloc(c, &m->ck_c, (size_t)n));
         m->V_s = c->d_V_s;
         m->eps_s = c->d_eps_s;
         hipLaunchKernelGGL(band_rowsum_kernel, dim3(ceil_div(m->rows_pad, 256)), dim3(256), 0, st, m->A_band,
                            m->band_off, m->krange, m->rows_pad, n, m->kt, 1.0 / scale, m->rk_s, m->ck_s);
+        TG_CHECK_LAUNCH(c);
+        hipLaunchKernelGGL(scatter_pair_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, m->perm, m->rk_s,
+                           m->ck_s, m->rk_c, m->ck_c);
         TG_CHECK_LAUNCH(c);
         TG_HIP(c, hipStreamSynchronize(st));
     }
@@ -1861,6 +1927,11 @@ tg_status mixed_prepare(tg_ctx* c) {
     m->G_lo = m->Z_all + 3 * (size_t)kgroups * m->nt;
     TG_TRY(dev_alloc(c, &m->useg, (size_t)m->npanel * m->nt));
     TG_TRY(dev_alloc(c, &m->bath, (size_t)c->max_batch));
+    if (c->occ == TG_OCC_HAT) {
+        TG_TRY(dev_alloc(c, &m->xb, (size_t)n * m->nt));
+        // defined values for the autotuner's epilogue runs on zero operands
+        TG_HIP(c, hipMemsetAsync(m->xb, 0, sizeof(double) * (size_t)n * m->nt, c->stream));
+    }
     if (m->persist) TG_TRY(build_schedule(c, m));
     // panels with work: flags cleared here once, then by the schedule kernel
     // after each use; the panels by decreasing band length for the dealing
@@ -1930,6 +2001,7 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->Z_all);
     m->F_hi = m->F_lo = m->G_hi = m->G_lo = nullptr;
     dev_free(&m->useg);
+    dev_free(&m->xb);
     dev_free(&m->krange);
     dev_free(&m->sched_off);
     dev_free(&m->sched_panel);
@@ -1946,6 +2018,8 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->dncol);
     dev_free(&m->rk_s);
     dev_free(&m->ck_s);
+    dev_free(&m->rk_c);
+    dev_free(&m->ck_c);
     dev_free(&m->P);
     dev_free(&m->bath);
     delete m;
@@ -2005,7 +2079,10 @@ static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, 
     v.actp = dyn ? m->actp : nullptr;
     v.slot = m->dslot;
     v.inv_scale = std::ldexp(1.0, -(m->sa + kSF));
-    v.o = pack_out(m, dyn);
+    v.o = pack_out(m, dyn, c->d_hat, c->dim);
+    v.rkc = m->rk_c;
+    v.ckc = m->ck_c;
+    v.Vc = c->d_V;
     hipLaunchKernelGGL((dp5_member_kernel<MODE, PT>), dim3(m->nt), dim3(kMemT), sizeof(double) * (size_t)c->n,
                        c->stream, a, v);
 }

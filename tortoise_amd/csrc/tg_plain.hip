@@ -6,16 +6,17 @@
 
 namespace tg {
 
-// Per-block partials dpart[4 b + {0,1,2,3}] = sums of V I, V |f(1-f)|,
-// V eps I and V eps |f(1-f)| (fixed order) give
-//   - with conserve, the number-defect removal I_i -= s |f_i(1-f_i)|,
-//     s = sum V I / sum V |f(1-f)| (weights nonnegative also where a state
-//     leaves [0,1]);
+// Per-block partials dpart[4 b + {0,1,2,3}] = sums of V I, V w, V eps I
+// and V eps w (fixed order), with weights w = |f(1-f)| (nodal factors) or
+// the hat form's weights wt (caller order), give
+//   - with conserve, the number-defect removal I_i -= s w_i,
+//     s = sum V I / sum V w (weights nonnegative also where a state leaves
+//     [0,1]);
 //   - with flux, the e-ph electron energy flux of the corrected I:
-//     sum V eps I - s sum V eps |f(1-f)| (one value, written by thread 0).
+//     sum V eps I - s sum V eps w (one value, written by thread 0).
 __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
-                                    const double* __restrict__ f, double* __restrict__ I,
-                                    int conserve, double* __restrict__ flux) {
+                                    const double* __restrict__ f, const double* __restrict__ wt,
+                                    double* __restrict__ I, int conserve, double* __restrict__ flux) {
     double d = 0.0, w = 0.0, ei = 0.0, ew = 0.0;
     for (int b = 0; b < nblk; ++b) {
         d += dpart[4 * b];
@@ -28,13 +29,13 @@ __global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, 
     if (s == 0.0) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        I[i] -= s * fabs(f[i] * (1.0 - f[i]));
+        I[i] -= s * (wt ? wt[i] : fabs(f[i] * (1.0 - f[i])));
 }
 
-tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, double* d_I,
-                              int conserve, double* flux) {
+tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, const double* d_wt,
+                              double* d_I, int conserve, double* flux) {
     hipLaunchKernelGGL(conserve_fix_kernel, dim3(conserve ? ceil_div(c->n, 256) : 1), dim3(256), 0, c->stream,
-                       dpart, nblk, c->n, d_f, d_I, conserve, flux);
+                       dpart, nblk, c->n, d_f, d_wt, d_I, conserve, flux);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
 }

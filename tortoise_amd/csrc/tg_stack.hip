@@ -66,10 +66,15 @@ struct StackData {
 static StackData* sd_of(tg_ctx* c) { return (StackData*)c->stack; }
 
 // fs[p] = f[perm[p]]
+// fs = f in energy order; with a hat stencil hs also the occupation factors
+// xb = S f in the caller's order (thread p computes node p)
 __global__ void stack_gather_kernel(const double* __restrict__ f, const int32_t* __restrict__ perm,
-                                    int64_t n, double* __restrict__ fs) {
+                                    int64_t n, double* __restrict__ fs, int dim,
+                                    const uint32_t* __restrict__ hs, double* __restrict__ xb) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p < n) fs[p] = f[perm[p]];
+    if (p >= n) return;
+    fs[p] = f[perm[p]];
+    if (hs) xb[p] = hat_value_dim(dim, hs, p, [&](int64_t j) { return f[j]; });
 }
 
 // block (panel, local iteration): entries [column 0..31][row 0..127] of the
@@ -211,14 +216,18 @@ stack_products_kernel(const double* __restrict__ part, const double* __restrict_
 }
 
 // One position per thread: the operator with the member's bath (bath_p[0]
-// when given, else bath_n) in the caller's order; with dpart, per-block sums
-// of V I, V |f(1-f)|, V eps I and V eps |f(1-f)| for the number-defect
-// removal and the energy flux (conserve_fix_kernel).
+// when given, else bath_n) in the caller's order; the occupation factors of
+// the row's node are xb (caller order; hat form) when given, else the nodal
+// f; with dpart, per-block sums of V I, V w, V eps I and V eps w for the
+// number-defect removal with weights w and the energy flux
+// (conserve_fix_kernel): w = |f(1-f)|, or with xb the hat form's
+// hat_defect_weight, which replaces the node's entry of xb.
 __global__ void __launch_bounds__(kStackFinishT)
 stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ vpart,
                     const int2* __restrict__ pitems, const int32_t* __restrict__ cit_off,
                     const int32_t* __restrict__ cit, int64_t n,
                     const int32_t* __restrict__ perm, const double* __restrict__ fs,
+                    double* __restrict__ xb,
                     const double* __restrict__ rk, const double* __restrict__ ck,
                     const double* __restrict__ V, const double* __restrict__ eps_s, double bath_n,
                     const double* __restrict__ bath_p, double* __restrict__ out,
@@ -230,14 +239,22 @@ stack_finish_kernel(const double* __restrict__ part, const double* __restrict__ 
         double u, v;
         stack_sums(part, vpart, pitems, cit_off, cit, p, u, v);
         const double np1 = nb + 1.0;
-        const double fi = fs[p];
+        const double fi = xb ? xb[perm[p]] : fs[p];
         const double wf = np1 * u + nb * v;
         const double wt1 = np1 * ck[p] + nb * rk[p];
         const double wtf = np1 * v + nb * u;
         const double Ii = ((1.0 - fi) * wf - fi * (wt1 - wtf)) / V[p];
         out[perm[p]] = Ii;
         acc[0] = V[p] * Ii;
-        acc[1] = V[p] * fabs(fi * (1.0 - fi));
+        if (xb) {
+            // the hat form's number-defect weight, kept for the removal in
+            // the slot of this node's factor (read above, not read again)
+            const double w = hat_defect_weight(fi, np1, nb, rk[p], ck[p], V[p]);
+            xb[perm[p]] = w;
+            acc[1] = V[p] * w;
+        } else {
+            acc[1] = V[p] * fabs(fi * (1.0 - fi));
+        }
         acc[2] = eps_s[p] * acc[0];
         acc[3] = eps_s[p] * acc[1];
     }
@@ -382,7 +399,7 @@ tg_status stack_products_device(tg_ctx* c, const double* d_f, double* d_u, doubl
     if (domega && !s->Sd) return set_error(c, TG_ERR_STATE, "band of dK/domega0 not prepared");
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, d_f, c->d_perm,
-                       c->n, c->d_fs);
+                       c->n, c->d_fs, c->dim, (const uint32_t*)nullptr, (double*)nullptr);
     TG_CHECK_LAUNCH(c);
     hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, domega ? s->Sd : s->Sb,
                        s->off, s->kr, s->items, c->n, c->d_fs, s->part, s->vpart);
@@ -401,8 +418,9 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
     StackData* s = sd_of(c);
     if (!s) return set_error(c, TG_ERR_STATE, "band of K not prepared");
     hipStream_t st = c->stream;
+    const bool hat = c->occ == TG_OCC_HAT;
     hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(c->n, 256)), dim3(256), 0, st, d_f, c->d_perm,
-                       c->n, c->d_fs);
+                       c->n, c->d_fs, c->dim, hat ? (const uint32_t*)c->d_hat : nullptr, c->d_xb);
     TG_CHECK_LAUNCH(c);
     time_begin(c);
     hipLaunchKernelGGL(stack_gemv_kernel, dim3(s->nitems), dim3(kStackThreads), 0, st, s->Sb, s->off, s->kr,
@@ -420,10 +438,10 @@ tg_status stack_apply_device(tg_ctx* c, const double* d_f, double* d_I, double b
         dpart = c->d_red;
     }
     hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, s->part, s->vpart, s->pitems,
-                       s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, c->d_rk_s, c->d_ck_s, c->d_V_s,
-                       c->d_eps_s, bath_n, d_bath, d_I, dpart);
+                       s->cit_off, s->cit, c->n, c->d_perm, c->d_fs, hat ? c->d_xb : nullptr,
+                       c->d_rk_s, c->d_ck_s, c->d_V_s, c->d_eps_s, bath_n, d_bath, d_I, dpart);
     TG_CHECK_LAUNCH(c);
-    if (dpart) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, d_I, conserve, d_flux));
+    if (dpart) TG_TRY(conserve_fix_device(c, dpart, (int)nb, d_f, hat ? c->d_xb : nullptr, d_I, conserve, d_flux));
     return TG_OK;
 }
 
@@ -482,10 +500,13 @@ extern "C" tg_status tg_collision_apply_noalloc(tg_ctx* c, const double* f, doub
     if (dev != c->device) return set_error(c, TG_ERR_STATE, "current device %d is not the context's %d", dev, c->device);
     double* base = g_scratch[dev];
     if (!base) return set_error(c, TG_ERR_STATE, "no scratch address recorded for device %d", dev);
-    // scratch layout: gathered state | row partials | column partials | block sums
+    // scratch layout: gathered state | occupation factors | row partials |
+    // column partials | block sums
     const unsigned nb = ceil_div(n, kStackFinishT);
+    const bool hat = c->occ == TG_OCC_HAT;
     const size_t o_fs = 0;
-    const size_t o_part = o_fs + round256((size_t)n);
+    const size_t o_xb = o_fs + round256((size_t)n);
+    const size_t o_part = o_xb + (hat ? round256((size_t)n) : 0);
     const size_t o_vpart = o_part + round256((size_t)sd->nitems * kStackRows);
     const size_t o_dpart = o_vpart + round256((size_t)sd->nitems * kStackChunk * kStackIt);
     const size_t need = o_dpart + round256(4 * (size_t)nb);
@@ -493,6 +514,7 @@ extern "C" tg_status tg_collision_apply_noalloc(tg_ctx* c, const double* f, doub
         return set_error(c, TG_ERR_ARG, "this mesh needs %zu bytes of scratch; the allocation-free evaluation has %zu",
                          need * sizeof(double), kNoAllocScratchBytes);
     double* fs = base + o_fs;
+    double* xb = hat ? base + o_xb : nullptr;
     double* part = base + o_part;
     double* vpart = base + o_vpart;
     double* dpart = base + o_dpart;
@@ -502,16 +524,18 @@ extern "C" tg_status tg_collision_apply_noalloc(tg_ctx* c, const double* f, doub
         const double* fb = f + (int64_t)b * n;
         double* Ib = I + (int64_t)b * n;
         const double nbath = bath_n ? bath_n[b] : c->bath_n;
-        hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, fb, c->d_perm, n, fs);
+        hipLaunchKernelGGL(stack_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, fb, c->d_perm, n, fs,
+                           c->dim, hat ? (const uint32_t*)c->d_hat : nullptr, xb);
         TG_CHECK_LAUNCH(c);
         hipLaunchKernelGGL(stack_gemv_kernel, dim3(sd->nitems), dim3(kStackThreads), 0, st, sd->Sb, sd->off, sd->kr,
                            sd->items, n, fs, part, vpart);
         TG_CHECK_LAUNCH(c);
         hipLaunchKernelGGL(stack_finish_kernel, dim3(nb), dim3(kStackFinishT), 0, st, part, vpart, sd->pitems,
-                           sd->cit_off, sd->cit, n, c->d_perm, fs, c->d_rk_s, c->d_ck_s, c->d_V_s, c->d_eps_s, nbath,
-                           (const double*)nullptr, Ib, conserve ? dpart : (double*)nullptr);
+                           sd->cit_off, sd->cit, n, c->d_perm, fs, xb, c->d_rk_s, c->d_ck_s,
+                           c->d_V_s, c->d_eps_s, nbath, (const double*)nullptr, Ib,
+                           conserve ? dpart : (double*)nullptr);
         TG_CHECK_LAUNCH(c);
-        if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, fb, Ib, 1, nullptr));
+        if (conserve) TG_TRY(conserve_fix_device(c, dpart, (int)nb, fb, xb, Ib, 1, nullptr));
     }
     return TG_OK;
 }

@@ -371,6 +371,12 @@ public:
     // returned and integrated) instead of acting as a bath
     bool dynamicPhonons = false;
     uint64_t mcSeed = 0;     // start of the MC point stream (the reference uses std::rand)
+    // W-form occupation factors of the test node: true, the hat average of the
+    // P1 occupation (its energy relaxation follows the reference operator's,
+    // which integrates those P1 products exactly), with the number defect
+    // removed; false, the nodal value (the lumped form)
+    bool hatOccupation = true;
+    int32_t formFlags() const { return hatOccupation ? TG_CONSERVE : 0; }
 
     Material(std::string n, const Region<NDim>& r) : name(std::move(n)), region(&r) {}
     Material(const Material&) = delete;
@@ -490,6 +496,7 @@ public:
         d.de = ch.energyGridStep;
         d.cutoff = ch.deltaCutoff;
         d.max_batch = max_batch;
+        d.occupation = hatOccupation ? TG_OCC_HAT : TG_OCC_NODAL;
         if (ee) {
             d.ee_g = std::sqrt(ee->amplitude);
             d.ee_kappa2 = ee->screening;
@@ -704,7 +711,8 @@ public:
         tg_ctx* ctx = material->backend(bathOccupation());
         const Function<NDim>& fe = (*this)[ch.electronBand];
         std::vector<Real> f = fe.toNodal(), I(f.size());
-        material->check(tg_collision_apply(ctx, f.data(), I.data(), (int64_t)f.size(), 1, nullptr, 0));
+        material->check(tg_collision_apply(ctx, f.data(), I.data(), (int64_t)f.size(), 1, nullptr,
+                                           material->formFlags()));
         r[ch.electronBand] = Function<NDim>::fromNodal(*fe.mesh, I);
         return r;
     }
@@ -807,7 +815,7 @@ public:
         checkConstraints(el, pb);
         const Real t0 = argVec.back();
         tg_ctx* ctx = material->backend(pb.constrained ? constrainedValue(pb, t0) : start.bathOccupation());
-        int32_t flags = quiet ? TG_QUIET : 0;
+        int32_t flags = (quiet ? TG_QUIET : 0) | material->formFlags();
         if (pb.constrained) {
             const Schedule sc = sampleConstraint(pb, t0, finalTime, denseStep);
             tg_bath_schedule d = sc.desc(pb);

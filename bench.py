@@ -21,6 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the W-form's occupation factors of the headline contexts: the nodal values
+# (the lumped form: positivity-preserving, conserving by itself); the
+# hat-averaged form is measured beside it (hat_form_line), every evaluation
+# with the number-defect removal the DP5 runs use
+OCC = "nodal"
 FP16_MFMA_PEAK_TFLOPS = 2500.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_gemm_traffic.json")
 TUPLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tuple_propagate_traffic.json")
@@ -73,24 +78,26 @@ def cpu_wform_band(ctx, w, f, bath_n, budget_s=10.0):
     K = ctx.kernel_rows(0, n)
     band = oracle.KernelBand(lambda idx: K[idx], w.eps(), ctx.node_volumes())
     del K
+    hat = oracle.HatStencil(w.dim, w.nsec) if OCC == "hat" else None
     cores = _cores()
-    scratch = np.empty((cores + 2) * n)
-    ref = band.apply(bath_n, f, nthreads=cores, scratch=scratch)   # warm-up
+    scratch = np.empty((cores + 3) * n)
+    ref = band.apply(bath_n, f, nthreads=cores, scratch=scratch, hat=hat, conserve=True)   # warm-up
     times = []
     t_start = time.perf_counter()
     while len(times) < 5 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
         t0 = time.perf_counter()
-        band.apply(bath_n, f, nthreads=cores, scratch=scratch)
+        band.apply(bath_n, f, nthreads=cores, scratch=scratch, hat=hat, conserve=True)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    gpu = ctx.apply(f, bath_n=bath_n)
+    gpu = ctx.apply(f, bath_n=bath_n, conserve=True)
     rel = float(np.linalg.norm(gpu - ref) / np.linalg.norm(ref))
     gbs = (8.0 * band.entries + 32.0 * n) / med / 1e9
     _log(f"CPU W-form over K's band ({band.entries} entries): {med * 1e3:.2f} ms/eval on {cores} threads "
          f"({gbs:.1f} GB/s); GPU FP64 vs it {rel:.1e}")
     return {"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "port",
             "band_entries": band.entries, "gbs": gbs, "gpu_fp64_rel_diff": rel,
-            "sample": f"oracle FP64 W-form over K's band (energy order, {band.entries} entries), one member "
+            "sample": f"oracle FP64 W-form ({OCC} occupation factors, number-defect removal) over K's band "
+                      f"(energy order, {band.entries} entries), one member "
                       f"of the 256-member ensemble, n={n}, median of {len(times)} after a warm-up "
                       f"({sum(times):.1f} s)"}
 
@@ -126,19 +133,22 @@ def dp5_batched(ctx, w, F, bath, check=(0, 128, 255)):
     evals = int(sum(s["evals"] for s in st))
     errs = []
     for b in check:
-        fr, _ = ctx.dp5_replay(F[:, b], 0.0, w.t_final, w.t_final / 100, steps[b], bath_n=bath[b])
+        fr, _ = ctx.dp5_replay(F[:, b], 0.0, w.t_final, w.t_final / 100, steps[b], bath_n=bath[b],
+                               conserve=True)
         errs.append(float(np.linalg.norm(fT[:, b] - fr) / np.linalg.norm(fr)))
     drift = float(np.max(np.abs(V @ fT - n0) / np.abs(n0)))
+    fmin = float(fT.min())
     ledger = np.array([s["bath_energy"] for s in st])
     edrift = float(np.max(np.abs(E @ fT + ledger - e0) / np.abs(e0)))
     _log(f"dp5 batched: {F.shape[1]} members to t={w.t_final} in {wall:.3f} s, attempts per member "
          f"{att.min()}..{att.max()}, f(t_final) rel err vs FP64 replay {max(errs):.2e}, "
-         f"number drift {drift:.1e}, energy drift (electrons + bath ledger) {edrift:.1e}")
+         f"number drift {drift:.1e}, energy drift (electrons + bath ledger) {edrift:.1e}, min f {fmin:.2e}")
     return {"members": int(F.shape[1]), "t_final": w.t_final, "rtol": w.rtol, "wall_s": wall,
             "attempts_min": int(att.min()), "attempts_max": int(att.max()),
             "member_evals_per_s": evals / wall, "f_final_rel_err_vs_fp64": max(errs),
             "checked_members": list(check), "number_drift_rel": drift,
-            "energy_drift_rel": edrift, "bath_energy_member0": float(ledger[0])}, fT
+            "energy_drift_rel": edrift, "bath_energy_member0": float(ledger[0]),
+            "min_f_t_final": fmin}, fT
 
 
 def dp5_plain(ctx, w, f0, bath_n):
@@ -152,7 +162,7 @@ def dp5_plain(ctx, w, f0, bath_n):
     V = ctx.node_volumes()
     E = V * p.eps()
     t0 = time.perf_counter()
-    fT, _, st = ctx.dp5(f0, 0.0, p.t_final, p.t_final / 100, p.rtol, p.atol, bath_n=bath_n)
+    fT, _, st = ctx.dp5(f0, 0.0, p.t_final, p.t_final / 100, p.rtol, p.atol, bath_n=bath_n, conserve=True)
     wall = time.perf_counter() - t0
     drift = float(abs(V @ fT - V @ f0) / abs(V @ f0))
     edrift = float(abs(E @ fT + st["bath_energy"] - E @ f0) / abs(E @ f0))
@@ -181,13 +191,13 @@ def cube_ee(steps=5, t_final=1.0):
     F = np.stack([wl.initial_distribution(eps, w.mu, temp[b], centre[b], w.pump_width, amp[b])
                   for b in range(B)], axis=1)
     bath = np.array([wl.bose_einstein(w.omega0, t) for t in temp])
-    ctx = tg.Context.create(w, max_batch=B)
+    ctx = tg.Context.create(w, max_batch=B, occupation=OCC)
     ctx.assemble(mixed=True)
     Fd = torch.from_numpy(np.ascontiguousarray(F.T)).to("cuda")
     Id = torch.empty_like(Fd)
 
     def call():
-        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath)
+        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath, conserve=True)
 
     ms, _ = time_loop(torch, ctx, call, steps, 1)
     V = ctx.node_volumes()
@@ -205,11 +215,11 @@ def cube_ee(steps=5, t_final=1.0):
     f0 = F[:, 0].copy()
     t0 = time.perf_counter()
     _, dense, st0, steps = ctx.dp5(f0, 0.0, t_final, t_final / 100, w.rtol, w.atol,
-                                   bath_n=bath[0], max_dense=100, h_cap=1 << 16)
+                                   bath_n=bath[0], max_dense=100, h_cap=1 << 16, conserve=True)
     fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
     _, dLdg, _ = ctx.dp5_adjoint(f0, 0.0, t_final, t_final / 100, steps, dense[-1] - f0, bath[0],
-                                 tga.dbath_domega0(w.omega0, temp[0]), ckpt=32)
+                                 tga.dbath_domega0(w.omega0, temp[0]), ckpt=32, conserve=True)
     adj = time.perf_counter() - t0
     ctx.close()
     _log(f"cube20 adjoint, member 0: forward FP64 DP5 to t={t_final} in {fwd:.1f} s "
@@ -222,13 +232,55 @@ def cube_ee(steps=5, t_final=1.0):
                                 "forward_s": fwd, "adjoint_s": adj, "dL_dg": dLdg}}
 
 
+def hat_form_line(w, F, bath, steps, warmup):
+    """The headline workload with the hat-averaged occupation factors
+    (TG_OCC_HAT, number-defect removal): batched and plain evaluation rates,
+    the 256-member DP5 to t_final (error against FP64 replays, drifts, the
+    smallest occupation at t_final: the hat form does not keep f >= 0)."""
+    import numpy as np
+    import torch
+
+    import tortoise_amd as tg
+
+    B = F.shape[1]
+    ctx = tg.Context.create(w, max_batch=B, occupation="hat")
+    ctx.assemble(mixed=True)
+    Fd = torch.from_numpy(np.ascontiguousarray(F.T)).to("cuda")
+    Id = torch.empty_like(Fd)
+
+    def mixed_call():
+        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath, conserve=True)
+
+    def plain_call():
+        ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=1, bath_host=bath[:1], conserve=True)
+
+    ms_mixed, _ = time_loop(torch, ctx, mixed_call, steps, warmup)
+    ms_plain, _ = time_loop(torch, ctx, plain_call, steps, warmup)
+    Im = Id.cpu().numpy().copy()
+    pick = [0, 128, 255]
+    Ip = ctx.apply(F[:, pick], bath_n=bath[pick], conserve=True)
+    rel = max(float(np.linalg.norm(Im[b] - Ip[:, q]) / np.linalg.norm(Ip[:, q])) for q, b in enumerate(pick))
+    try:
+        d5, _ = dp5_batched(ctx, w, F, bath)
+    except tg.TgError as e:
+        d5 = {"failed": str(e)}
+    ctx.close()
+    _log(f"hat form: batched {ms_mixed:.4f} ms ({B * 1000.0 / ms_mixed:.0f} evals/s), plain FP64 "
+         f"{ms_plain:.4f} ms, max rel err {rel:.2e}")
+    return {"occupation_factors": "hat", "number_defect_removal": True, "members": B,
+            "evals_per_s": B * 1000.0 / ms_mixed, "ms_per_step": ms_mixed,
+            "plain_fp64_evals_per_s": 1000.0 / ms_plain, "max_rel_err_vs_fp64": rel, "dp5_batched": d5}
+
+
 def square64_gradients(T=0.02):
     """The differentiation workload: 64x64 sections (4225 nodes), target f(50)
     generated at g = 0.1, omega0 = 0.05; from g = 0.12, omega0 = 0.045 the
     gradients of L = |Wb f(50) - target|^2 (Wb: 40 energy bins of width 0.05,
     node volumes as weights) wrt g, omega0 and f0 through the adjoint of the
     accepted steps (checkpoints every 32), with dL/dg checked against a
-    central difference (h = 1e-6 g) of the same fixed-step map."""
+    central difference (h = 1e-6 g) of the same fixed-step map; the operator
+    is the measured form (occupation factors OCC, number-defect removal in
+    every stage, differentiated with it)."""
     import numpy as np
 
     import tortoise_amd as tg
@@ -240,13 +292,14 @@ def square64_gradients(T=0.02):
     nd = int(round(t1 / dt))
 
     def ctx_at(g, om):
-        c = tg.Context.create(w, max_batch=1, g=g, omega0=om, bath_n=tga.bath_occupation(om, T))
+        c = tg.Context.create(w, max_batch=1, g=g, omega0=om, bath_n=tga.bath_occupation(om, T),
+                              occupation=OCC)
         c.assemble()
         return c
 
     f0 = w.initial_f()
     tgt_ctx = ctx_at(0.1, 0.05)
-    _, dense_t, _ = tgt_ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd)
+    _, dense_t, _ = tgt_ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd, conserve=True)
     V = tgt_ctx.node_volumes()
     eps = w.eps()
     edges = np.linspace(0.0, 2.0, 41)
@@ -256,18 +309,19 @@ def square64_gradients(T=0.02):
     g0, w0 = 0.12, 0.045
     ctx = ctx_at(g0, w0)
     t0 = time.perf_counter()
-    _, dense, st, steps = ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd, h_cap=1 << 14)
+    _, dense, st, steps = ctx.dp5(f0, 0.0, t1, dt, 1e-8, 1e-12, max_dense=nd, h_cap=1 << 14, conserve=True)
     fwd = time.perf_counter() - t0
     d = Wb @ dense[-1] - target
     t0 = time.perf_counter()
     g_f0, g_g, g_w = ctx.dp5_adjoint(f0, 0.0, t1, dt, steps, 2.0 * (Wb.T @ d),
-                                     tga.bath_occupation(w0, T), tga.dbath_domega0(w0, T), ckpt=32)
+                                     tga.bath_occupation(w0, T), tga.dbath_domega0(w0, T), ckpt=32,
+                                     conserve=True)
     adj = time.perf_counter() - t0
     ctx.close()
 
     def L_at(g):
         c = ctx_at(g, w0)
-        _, dn = c.dp5_replay(f0, 0.0, t1, dt, steps, max_dense=nd)
+        _, dn = c.dp5_replay(f0, 0.0, t1, dt, steps, max_dense=nd, conserve=True)
         c.close()
         r = Wb @ dn[-1] - target
         return float(r @ r)
@@ -300,9 +354,10 @@ def hat_moments(w, out):
     return H
 
 
-def wform_vs_reference(w, op, f, bath_n, Hw):
-    """The W-form's weak form against the reference operator's on one state:
-    relative L2 of the hat moments <phi_i, I> and the ratio of the electron
+def wform_vs_reference(w, op, f, bath_n, Hws):
+    """The W-form's weak form against the reference operator's on one state,
+    for each form in Hws (occupation factors -> the W-form's hat moments
+    <phi_i, I>): relative L2 of the hat moments and the ratio of the electron
     cooling powers sum_i eps_i <phi_i, I> (W-form / reference)."""
     import numpy as np
 
@@ -310,9 +365,12 @@ def wform_vs_reference(w, op, f, bath_n, Hw):
 
     Ht = hat_moments(w, op.propagate(oracle.mc_nodal_to_dg(2, w.nsec, f), bath_n))
     eps = w.eps()
-    return {"hat_moments_rel_l2": float(np.linalg.norm(Hw - Ht) / np.linalg.norm(Ht)),
-            "cooling_power_wform": float(eps @ Hw), "cooling_power_reference": float(eps @ Ht),
-            "cooling_power_ratio": float((eps @ Hw) / (eps @ Ht))}
+    out = {"cooling_power_reference": float(eps @ Ht)}
+    for form, Hw in Hws.items():
+        out[form] = {"hat_moments_rel_l2": float(np.linalg.norm(Hw - Ht) / np.linalg.norm(Ht)),
+                     "cooling_power_wform": float(eps @ Hw),
+                     "cooling_power_ratio": float((eps @ Hw) / (eps @ Ht))}
+    return out
 
 
 def reference_operator(budget_s=12.0, cpu_budget_s=30.0, member0=None):
@@ -433,8 +491,9 @@ def reference_operator(budget_s=12.0, cpu_budget_s=30.0, member0=None):
             f0, b0, Hw = member0
             line["wform_vs_reference_member0"] = wform_vs_reference(w, op, f0, b0, Hw)
             d = line["wform_vs_reference_member0"]
-            _log(f"  W-form vs reference operator, member 0: cooling power ratio "
-                 f"{d['cooling_power_ratio']:.3f}, hat moments rel L2 {d['hat_moments_rel_l2']:.3f}")
+            _log("  W-form vs reference operator, member 0: " + ", ".join(
+                f"{k} factors: cooling power ratio {v['cooling_power_ratio']:.3f}, hat moments rel L2 "
+                f"{v['hat_moments_rel_l2']:.3f}" for k, v in d.items() if isinstance(v, dict)))
         if ns == 32:
             # the reference's CPU-runnable propagation: DP5 to t = 100 (rtol 1e-8,
             # atol 1e-12, dense step 1), one member, with the tuple operator and
@@ -442,17 +501,22 @@ def reference_operator(budget_s=12.0, cpu_budget_s=30.0, member0=None):
             t0 = time.perf_counter()
             _, _, stt = op.dp5(base, 0.0, 100.0, 1.0, 1e-8, 1e-12, bath_n=w.bath_n)
             t_tuple = time.perf_counter() - t0
-            cw = tg.Context.create(wl.SQUARE32, max_batch=1)
+            cw = tg.Context.create(wl.SQUARE32, max_batch=1, occupation=OCC)
             cw.assemble()
             t0 = time.perf_counter()
-            _, _, stw = cw.dp5(wl.SQUARE32.initial_f(), 0.0, 100.0, 1.0, 1e-8, 1e-12)
+            _, _, stw = cw.dp5(wl.SQUARE32.initial_f(), 0.0, 100.0, 1.0, 1e-8, 1e-12, conserve=True)
             t_wform = time.perf_counter() - t0
             # the W-form's discrepancy from the reference operator on the
-            # initial state of this configuration
+            # initial state of this configuration, both forms
             f32 = wl.SQUARE32.initial_f()
-            Hw32 = cw.node_volumes() * cw.apply(f32)
-            line["wform_vs_reference_initial"] = wform_vs_reference(w, op, f32, w.bath_n, Hw32)
+            Hw32 = {OCC: cw.node_volumes() * cw.apply(f32, conserve=True)}
             cw.close()
+            other = "hat" if OCC == "nodal" else "nodal"
+            cn = tg.Context.create(wl.SQUARE32, max_batch=1, occupation=other)
+            cn.assemble()
+            Hw32[other] = cn.node_volumes() * cn.apply(f32, conserve=True)
+            cn.close()
+            line["wform_vs_reference_initial"] = wform_vs_reference(w, op, f32, w.bath_n, Hw32)
             line["dp5_t100"] = {"tuple_operator": {"wall_s": t_tuple, "attempts": int(stt["attempted"])},
                                 "w_form": {"wall_s": t_wform, "attempts": int(stw["attempted"])}}
             _log(f"  DP5 to t=100 (rtol 1e-8): tuple operator {t_tuple:.2f} s ({stt['attempted']} "
@@ -553,7 +617,7 @@ def main():
     w = wl.SQUARE128_ENSEMBLE
     B = w.batch
     torch.cuda.init()
-    ctx = tg.Context.create(w, max_batch=B)
+    ctx = tg.Context.create(w, max_batch=B, occupation=OCC)
     t0 = time.perf_counter()
     ctx.assemble(mixed=True)
     torch.cuda.synchronize()
@@ -581,11 +645,11 @@ def main():
 
     def mixed_call(time_kernel=False):
         ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
-                         time_kernel=time_kernel)
+                         time_kernel=time_kernel, conserve=True)
 
     def plain_call(time_kernel=False):
         ctx.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=1, bath_host=bath[:1],
-                         time_kernel=time_kernel)
+                         time_kernel=time_kernel, conserve=True)
 
     ms_mixed, wall = time_loop(torch, ctx, mixed_call, args.steps, args.warmup)
     ms_gemm = kernel_ms(ctx, lambda: mixed_call(True), 10)
@@ -596,7 +660,7 @@ def main():
     Im = Id.cpu().numpy().copy()
     Fh = F.T.copy()
     pick = [0, 63, 128, 255]
-    Ip = ctx.apply(Fh[:, pick], bath_n=bath[pick])
+    Ip = ctx.apply(Fh[:, pick], bath_n=bath[pick], conserve=True)
     rel = max(float(np.linalg.norm(Im[b] - Ip[:, q]) / np.linalg.norm(Ip[:, q]))
               for q, b in enumerate(pick))
 
@@ -642,6 +706,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "nodes": n, "members": B,
                    "mesh": "128x128 sections, 32768 P1 triangles",
+                   "occupation_factors": OCC, "number_defect_removal": True,
                    "kernel_nnz": st["nnz"], "streamed_band": st["mixed_band"],
                    "splits": st["mixed_splits"]},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP16_MFMA_PEAK_TFLOPS,
@@ -673,14 +738,14 @@ def main():
     if not args.no_all_panels:
         os.environ["TG_MIXED_SKIP"] = "0"
         try:
-            ctx_all = tg.Context.create(w, max_batch=B)
+            ctx_all = tg.Context.create(w, max_batch=B, occupation=OCC)
             ctx_all.assemble(mixed=True)
         finally:
             del os.environ["TG_MIXED_SKIP"]
 
         def all_call(time_kernel=False):
             ctx_all.apply_device(Fd.data_ptr(), Id.data_ptr(), batch=B, mixed=True, bath_host=bath,
-                                 time_kernel=time_kernel)
+                                 time_kernel=time_kernel, conserve=True)
 
         ms_all, _ = time_loop(torch, ctx_all, all_call, args.steps, args.warmup)
         ms_gemm_all = kernel_ms(ctx_all, lambda: all_call(True), 10)
@@ -690,10 +755,17 @@ def main():
                              "gemm_tflops": 4.0 * st["nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
         _log(f"all panels: {ms_all:.4f} ms/batch ({out['value_all_panels']:.0f} evals/s), "
              f"GEMM {ms_gemm_all:.4f} ms")
-    # W-form hat moments of member 0 (FP64) for the comparison with the
-    # reference operator at this mesh
-    I0 = ctx.apply(Fh[:, 0], bath_n=bath[0])
-    member0 = (Fh[:, 0].copy(), float(bath[0]), ctx.node_volumes() * I0)
+    # W-form hat moments of member 0 (FP64), with the measured (hat) and the
+    # nodal occupation factors, for the comparison with the reference
+    # operator at this mesh
+    other = "hat" if OCC == "nodal" else "nodal"
+    I0 = ctx.apply(Fh[:, 0], bath_n=bath[0], conserve=True)
+    cn = tg.Context.create(w, max_batch=1, occupation=other)
+    cn.assemble()
+    I0n = cn.apply(Fh[:, 0], bath_n=bath[0], conserve=True)
+    cn.close()
+    V0 = ctx.node_volumes()
+    member0 = (Fh[:, 0].copy(), float(bath[0]), {OCC: V0 * I0, other: V0 * I0n})
     if not args.no_dp5:
         out["dp5_batched"], fT = dp5_batched(ctx, w, Fh, bath)
         out["dp5_plain"] = dp5_plain(ctx, w, Fh[:, 0].copy(), bath[0])
@@ -707,6 +779,8 @@ def main():
                                 "executed_band_fraction": st_tf["mixed_last_band"] / st["mixed_band"]}
         _log(f"state at t={w.t_final}: {ms_tf:.4f} ms/batch ({out['value_t_final']:.0f} evals/s), "
              f"{out['t_final_state']['executed_band_fraction']:.0%} of the band computed")
+    if not args.no_dp5:
+        out["hat_form"] = hat_form_line(w, Fh, bath, args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_wform_band"] = cpu_wform_band(ctx, w, Fh[:, 0].copy(), float(bath[0]))
         out["mixed_vs_cpu_wform_band"] = evals_mixed / out["cpu_wform_band"]["value"]

@@ -254,13 +254,15 @@ tg_status   tg_dp5_replay(tg_ctx* ctx, double* f, int64_t n, int32_t batch, doub
  * values) and dL_dparams = {dL/dg, dL/domega0}.  dbath_domega0 is the
  * derivative of the bath occupation with respect to omega0 (e.g. of
  * n_BE(omega0, T) at fixed T).  Forward states are recomputed from
- * checkpoints every ckpt steps.  No reference counterpart (the reference has
- * no differentiation); the forward map it differentiates replaces
+ * checkpoints every ckpt steps.  flags: TG_CONSERVE (every stage evaluation
+ * removes the number defect, as tg_dp5_replay with TG_CONSERVE does, and the
+ * gradients include that removal).  No reference counterpart (the reference
+ * has no differentiation); the forward map it differentiates replaces
  * MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137). */
 tg_status   tg_dp5_adjoint(tg_ctx* ctx, const double* f0, int64_t n, double t0, double t1,
                            double dense_dt, const double* h_steps, int32_t n_steps,
                            int32_t ckpt, double bath_n, double dbath_domega0,
-                           const double* dL_df, double* dL_df0, double* dL_dparams);
+                           const double* dL_df, double* dL_df0, double* dL_dparams, int32_t flags);
 
 /* FP64 sums per member: number = sum_i V_i f_i, energy = sum_i V_i eps_i f_i. */
 tg_status   tg_conservation(tg_ctx* ctx, const double* f, int64_t n, int32_t batch,

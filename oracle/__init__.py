@@ -92,6 +92,8 @@ def lib():
                                                C.c_void_p, _f64p]
         L.or_collision_vjp_occ.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, C.c_void_p, _f64p,
                                            _f64p]
+        L.or_collision_vjp_ex.argtypes = [C.c_int64, _f64p, _f64p, C.c_double, _f64p, C.c_void_p, C.c_int,
+                                          _f64p, _f64p]
         L.or_conservation.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.or_relative_error.restype = C.c_double
@@ -159,9 +161,9 @@ def lib():
                                      C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
         L.or_dp5_adjoint_occ.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.c_double, C.c_double,
-                                         C.c_double, C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double,
-                                         _f64p, C.c_int64, C.c_int32, _f64p, _f64p, C.POINTER(C.c_double),
-                                         C.POINTER(C.c_double)]
+                                         C.c_double, C.c_void_p, C.c_int, _f64p, C.c_double, C.c_double,
+                                         C.c_double, _f64p, C.c_int64, C.c_int32, _f64p, _f64p,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_mc_rows.restype = C.c_int32
         L.or_mc_rows.argtypes = [C.c_int, C.c_int]
         L.or_mc_points.argtypes = [C.c_int, C.c_int, C.c_int32, C.c_uint64, _f64p]
@@ -569,16 +571,17 @@ def dp5_collision_replay(dim, nsec, K, V, bath_n, f0, t0, t_final, dense_dt, h, 
     return dict(y=f, dense=dense[:nd], dense_t=dt[:nd], ledger=led.value)
 
 
-def collision_vjp(K, V, bath_n, f, mu, hat=None):
+def collision_vjp(K, V, bath_n, f, mu, hat=None, conserve=False):
     out = np.zeros_like(np.asarray(f, np.float64))
-    lib().or_collision_vjp_occ(out.size, np.ascontiguousarray(K, np.float64),
-                               np.ascontiguousarray(V, np.float64), bath_n,
-                               np.ascontiguousarray(f, np.float64), None if hat is None else hat.h,
-                               np.ascontiguousarray(mu, np.float64), out)
+    lib().or_collision_vjp_ex(out.size, np.ascontiguousarray(K, np.float64),
+                              np.ascontiguousarray(V, np.float64), bath_n,
+                              np.ascontiguousarray(f, np.float64), None if hat is None else hat.h,
+                              int(bool(conserve)), np.ascontiguousarray(mu, np.float64), out)
     return out
 
 
-def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL_df, ckpt=32, hat=None):
+def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL_df, ckpt=32, hat=None,
+                conserve=False):
     """Discrete adjoint of the fixed-step DP5 with dense output at t_final.
     Returns (dL/df0, dL/dg, dL/domega0)."""
     n = len(f0)
@@ -587,7 +590,7 @@ def dp5_adjoint(K, Kd, V, bath_n, dbath, g, f0, t0, t_final, dense_dt, steps, dL
     steps = np.ascontiguousarray(steps, np.float64)
     lib().or_dp5_adjoint_occ(n, np.ascontiguousarray(K, np.float64), np.ascontiguousarray(Kd, np.float64),
                              np.ascontiguousarray(V, np.float64), bath_n, dbath, g,
-                             None if hat is None else hat.h,
+                             None if hat is None else hat.h, int(bool(conserve)),
                              np.ascontiguousarray(f0, np.float64), t0, t_final, dense_dt, steps,
                              steps.size, ckpt, np.ascontiguousarray(dL_df, np.float64), out,
                              C.byref(dg), C.byref(dw))

@@ -164,11 +164,12 @@ or_hat* or_hat_create(int dim, const int32_t* nsec) {
     return h;
 }
 
-/* Number-defect weight of the hat form: node i's largest gross rate at its
- * own occupation x, (|1-x| (W 1)_i + |x| (W^T 1)_i) / V_i, with
- * W = (n+1) K + n K^T, rk = K 1 and ck = K^T 1. */
+/* Number-defect weight of the hat form: the gross rate node i would have at
+ * its own occupation x if every other node were half filled,
+ * ((1-x) (W 1)_i + x (W^T 1)_i) / V_i, with W = (n+1) K + n K^T,
+ * rk = K 1 and ck = K^T 1 (linear in x). */
 double or_hat_defect_weight(double x, double np1, double nb, double rk, double ck, double V) {
-    return (fabs(1.0 - x) * (np1 * rk + nb * ck) + fabs(x) * (np1 * ck + nb * rk)) / V;
+    return ((1.0 - x) * (np1 * rk + nb * ck) + x * (np1 * ck + nb * rk)) / V;
 }
 
 void or_hat_free(or_hat* h) {
@@ -1056,38 +1057,90 @@ void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n
  * -m (W f + W^T 1 - W^T f) reaches f through S^T. */
 void or_collision_vjp_occ(int64_t n, const double* K, const double* V, double bath_n,
                           const double* f, const or_hat* hat, const double* mu, double* out) {
-    double* buf = (double*)malloc(sizeof(double) * 14 * n);
-    double *one = buf, *x = buf + n, *y = buf + 2 * n;
-    double *uf = buf + 3 * n, *vf = buf + 4 * n, *u1 = buf + 5 * n, *v1 = buf + 6 * n;
-    double *ux = buf + 7 * n, *vx = buf + 8 * n, *uy = buf + 9 * n, *vy = buf + 10 * n;
-    double *ob = buf + 11 * n, *z = buf + 12 * n, *zt = buf + 13 * n;
-    if (hat) or_hat_apply(hat, f, ob);
-    else memcpy(ob, f, sizeof(double) * n);
+    or_collision_vjp_ex(n, K, V, bath_n, f, hat, 0, mu, out);
+}
+
+/* The number-defect removal's pieces at state f: occupation factors x, the
+ * products uf = K f, vf = K^T f, rk = K 1, ck = K^T 1, the uncorrected
+ * I0, the weights w (0 without conserve), s = sum V I0 / sum V w and, for a
+ * cotangent mu, c = sum mu w / sum V w.  buf: 8 n doubles, laid out as
+ * x | uf | vf | rk | ck | I0 | w | scratch. */
+static void defect_pieces(int64_t n, const double* K, const double* V, double bath_n, const double* f,
+                          const or_hat* hat, int conserve, const double* mu, double* buf, double* s,
+                          double* c) {
+    double *x = buf, *uf = buf + n, *vf = buf + 2 * n, *rk = buf + 3 * n, *ck = buf + 4 * n;
+    double *I0 = buf + 5 * n, *w = buf + 6 * n, *one = buf + 7 * n;
+    if (hat) or_hat_apply(hat, f, x);
+    else memcpy(x, f, sizeof(double) * n);
+    for (int64_t i = 0; i < n; ++i) one[i] = 1.0;
+    const double* xs[2] = {f, one};
+    double* us[2] = {uf, rk};
+    double* vs[2] = {vf, ck};
+    dual_products(n, K, 2, xs, us, vs);
+    const double np1 = bath_n + 1.0;
+    double D = 0.0, Z = 0.0, M = 0.0;
     for (int64_t i = 0; i < n; ++i) {
-        double m = mu[i] / V[i];
-        one[i] = 1.0;
+        const double wf = np1 * uf[i] + bath_n * vf[i];
+        const double wt1 = np1 * ck[i] + bath_n * rk[i];
+        const double wtf = np1 * vf[i] + bath_n * uf[i];
+        I0[i] = ((1.0 - x[i]) * wf - x[i] * (wt1 - wtf)) / V[i];
+        w[i] = !conserve ? 0.0
+               : hat ? or_hat_defect_weight(x[i], np1, bath_n, rk[i], ck[i], V[i])
+                     : fabs(x[i] * (1.0 - x[i]));
+        D += V[i] * I0[i];
+        Z += V[i] * w[i];
+        if (mu) M += mu[i] * w[i];
+    }
+    *s = (conserve && Z > 0.0) ? D / Z : 0.0;
+    *c = (conserve && Z > 0.0) ? M / Z : 0.0;
+}
+
+/* d w_i / d x_i of the removal's weights */
+static double defect_dweight(const or_hat* hat, double x, double np1, double nb, double rk, double ck, double V) {
+    if (hat) return ((np1 * ck + nb * rk) - (np1 * rk + nb * ck)) / V;
+    const double p = x * (1.0 - x);
+    return p > 0.0 ? 1.0 - 2.0 * x : (p < 0.0 ? 2.0 * x - 1.0 : 0.0);
+}
+
+/* With conserve the operator is I = I0 - s w (s = sum V I0 / sum V w), and
+ *   mu^T dI = mu'^T dI0 - s mu'^T dw,  mu' = mu - c V,  c = mu^T w / sum V w. */
+void or_collision_vjp_ex(int64_t n, const double* K, const double* V, double bath_n, const double* f,
+                         const or_hat* hat, int conserve, const double* mu, double* out) {
+    double* pc = (double*)malloc(sizeof(double) * 8 * n);
+    double sdef = 0.0, cdef = 0.0;
+    defect_pieces(n, K, V, bath_n, f, hat, conserve, mu, pc, &sdef, &cdef);
+    const double *ob = pc, *uf = pc + n, *vf = pc + 2 * n, *u1 = pc + 3 * n, *v1 = pc + 4 * n;
+    double* buf = (double*)malloc(sizeof(double) * 9 * n);
+    double *x = buf, *y = buf + n, *mup = buf + 2 * n;
+    double *ux = buf + 3 * n, *vx = buf + 4 * n, *uy = buf + 5 * n, *vy = buf + 6 * n;
+    double *z = buf + 7 * n, *zt = buf + 8 * n;
+    for (int64_t i = 0; i < n; ++i) {
+        mup[i] = mu[i] - cdef * V[i];
+        const double m = mup[i] / V[i];
         x[i] = m * (1.0 - ob[i]);
         y[i] = m * ob[i];
     }
-    const double* xs[4] = {f, one, x, y};
-    double* us[4] = {uf, u1, ux, uy};
-    double* vs[4] = {vf, v1, vx, vy};
-    dual_products(n, K, 4, xs, us, vs);
+    const double* xs[2] = {x, y};
+    double* us[2] = {ux, uy};
+    double* vs[2] = {vx, vy};
+    dual_products(n, K, 2, xs, us, vs);
     const double np1 = bath_n + 1.0;
     for (int64_t k = 0; k < n; ++k) {
-        double m = mu[k] / V[k];
-        double wf = np1 * uf[k] + bath_n * vf[k];
-        double wt1 = np1 * v1[k] + bath_n * u1[k];
-        double wtf = np1 * vf[k] + bath_n * uf[k];
-        double wtx = np1 * vx[k] + bath_n * ux[k];
-        double wy = np1 * uy[k] + bath_n * vy[k];
+        const double m = mup[k] / V[k];
+        const double wf = np1 * uf[k] + bath_n * vf[k];
+        const double wt1 = np1 * v1[k] + bath_n * u1[k];
+        const double wtf = np1 * vf[k] + bath_n * uf[k];
+        const double wtx = np1 * vx[k] + bath_n * ux[k];
+        const double wy = np1 * uy[k] + bath_n * vy[k];
         z[k] = -m * (wf + wt1 - wtf);
+        if (conserve) z[k] -= sdef * mup[k] * defect_dweight(hat, ob[k], np1, bath_n, u1[k], v1[k], V[k]);
         out[k] = wtx + wy;
     }
     if (hat) or_hat_apply_t(hat, z, zt);
     else memcpy(zt, z, sizeof(double) * n);
     for (int64_t k = 0; k < n; ++k) out[k] += zt[k];
     free(buf);
+    free(pc);
 }
 
 typedef struct adj_ctx {
@@ -1095,28 +1148,48 @@ typedef struct adj_ctx {
     const double *K, *Kd, *V;
     double bath, dbath, g;
     const or_hat* hat;
+    int conserve;
 } adj_ctx;
 
 static void adj_rhs(const adj_ctx* a, const double* y, double* out) {
-    or_collision_general_occ(a->n, a->K, a->V, a->bath + 1.0, a->bath, y, a->hat, out);
+    or_collision_apply_occ(a->n, a->K, a->V, a->bath, y, a->hat, out, a->conserve);
 }
 
-/* mu^T dF/dtheta at state y: (dL/dg, dL/domega0) contributions */
+/* mu^T dF/dtheta at state y: (dL/dg, dL/domega0) contributions.  With the
+ * number-defect removal F = I0 - s w:  mu^T dF = mu'^T dI0 - s mu'^T dw
+ * (mu' = mu - c V); the hat form's weights scale as g^2 and move with omega0
+ * through K 1, K^T 1 and the bath, the nodal weights depend on neither. */
 static void adj_param(const adj_ctx* a, const double* y, const double* mu, double* dg, double* dw) {
     int64_t n = a->n;
-    double* t1 = (double*)malloc(sizeof(double) * 2 * n);
-    double* t2 = t1 + n;
-    adj_rhs(a, y, t1);
-    double s = 0.0;
-    for (int64_t i = 0; i < n; ++i) s += mu[i] * t1[i];
-    *dg += 2.0 * s / a->g;
+    double* pc = (double*)malloc(sizeof(double) * 8 * n);
+    double sdef = 0.0, cdef = 0.0;
+    defect_pieces(n, a->K, a->V, a->bath, y, a->hat, a->conserve, mu, pc, &sdef, &cdef);
+    const double *x = pc, *rk = pc + 3 * n, *ck = pc + 4 * n, *I0 = pc + 5 * n, *w = pc + 6 * n;
+    double* t = (double*)malloc(sizeof(double) * 5 * n);
+    double *mup = t, *t1 = t + n, *t2 = t + 2 * n, *rkd = t + 3 * n, *ckd = t + 4 * n;
+    for (int64_t i = 0; i < n; ++i) mup[i] = mu[i] - cdef * a->V[i];
+    double sg = 0.0;
+    for (int64_t i = 0; i < n; ++i) sg += mup[i] * (I0[i] - (a->hat ? sdef * w[i] : 0.0));
+    *dg += 2.0 * sg / a->g;
     /* W' = (n+1) Kd + n Kd^T + n' (K + K^T) */
     or_collision_general_occ(n, a->Kd, a->V, a->bath + 1.0, a->bath, y, a->hat, t1);
     or_collision_general_occ(n, a->K, a->V, a->dbath, a->dbath, y, a->hat, t2);
-    s = 0.0;
-    for (int64_t i = 0; i < n; ++i) s += mu[i] * (t1[i] + t2[i]);
-    *dw += s;
-    free(t1);
+    double sw = 0.0;
+    for (int64_t i = 0; i < n; ++i) sw += mup[i] * (t1[i] + t2[i]);
+    if (a->conserve && a->hat) {
+        for (int64_t i = 0; i < n; ++i) { rkd[i] = 0.0; ckd[i] = 0.0; }
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < n; ++j) { rkd[i] += a->Kd[i * n + j]; ckd[j] += a->Kd[i * n + j]; }
+        const double np1 = a->bath + 1.0, nb = a->bath, dn = a->dbath;
+        for (int64_t i = 0; i < n; ++i) {
+            const double dA = np1 * rkd[i] + nb * ckd[i] + dn * (rk[i] + ck[i]);
+            const double dB = np1 * ckd[i] + nb * rkd[i] + dn * (ck[i] + rk[i]);
+            sw -= sdef * mup[i] * ((1.0 - x[i]) * dA + x[i] * dB) / a->V[i];
+        }
+    }
+    *dw += sw;
+    free(t);
+    free(pc);
 }
 
 static const double TA[7][6] = {
@@ -1146,16 +1219,16 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
                     int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
                     double* dL_domega0) {
-    or_dp5_adjoint_occ(n, K, Kd, V, bath_n, dbath, g, NULL, f0, t0, t_final, dense_dt, h_steps, n_steps,
+    or_dp5_adjoint_occ(n, K, Kd, V, bath_n, dbath, g, NULL, 0, f0, t0, t_final, dense_dt, h_steps, n_steps,
                        ckpt, dL_df, dL_df0, dL_dg, dL_domega0);
 }
 
 void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const double* V,
-                        double bath_n, double dbath, double g, const or_hat* hat, const double* f0,
-                        double t0, double t_final, double dense_dt, const double* h_steps,
-                        int64_t n_steps, int32_t ckpt, const double* dL_df, double* dL_df0,
-                        double* dL_dg, double* dL_domega0) {
-    adj_ctx a = {n, K, Kd, V, bath_n, dbath, g, hat};
+                        double bath_n, double dbath, double g, const or_hat* hat, int conserve,
+                        const double* f0, double t0, double t_final, double dense_dt,
+                        const double* h_steps, int64_t n_steps, int32_t ckpt, const double* dL_df,
+                        double* dL_df0, double* dL_dg, double* dL_domega0) {
+    adj_ctx a = {n, K, Kd, V, bath_n, dbath, g, hat, conserve};
     /* forward: find the step holding the last dense output and keep checkpoints */
     int64_t nck = n_steps / ckpt + 2;
     double* ck = (double*)malloc(sizeof(double) * n * nck);
@@ -1213,7 +1286,7 @@ void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const doub
                 for (int j = 0; j < 7; ++j)
                     for (int64_t i = 0; i < n; ++i) mu[j][i] = (s * h * bt[j]) * dL_df[i];
                 /* k7 = F(Y7), Y7 = y + sum_i (h b_i) k_i */
-                or_collision_vjp_occ(n, K, V, bath_n, Y[6], hat, mu[6], nu);
+                or_collision_vjp_ex(n, K, V, bath_n, Y[6], hat, conserve, mu[6], nu);
                 adj_param(&a, Y[6], mu[6], &dg, &dw);
                 for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
                 for (int j = 0; j < 6; ++j)
@@ -1224,13 +1297,13 @@ void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const doub
                     for (int64_t i = 0; i < n; ++i) mu[j][i] = (TA[6][j] * h) * lam[i];
             }
             for (int st = 5; st >= 1; --st) {
-                or_collision_vjp_occ(n, K, V, bath_n, Y[st], hat, mu[st], nu);
+                or_collision_vjp_ex(n, K, V, bath_n, Y[st], hat, conserve, mu[st], nu);
                 adj_param(&a, Y[st], mu[st], &dg, &dw);
                 for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
                 for (int j = 0; j < st; ++j)
                     for (int64_t i = 0; i < n; ++i) mu[j][i] += (TA[st][j] * h) * nu[i];
             }
-            or_collision_vjp_occ(n, K, V, bath_n, Y[0], hat, mu[0], nu);
+            or_collision_vjp_ex(n, K, V, bath_n, Y[0], hat, conserve, mu[0], nu);
             adj_param(&a, Y[0], mu[0], &dg, &dw);
             for (int64_t i = 0; i < n; ++i) lam[i] += nu[i];
         }

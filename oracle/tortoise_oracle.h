@@ -106,10 +106,10 @@ void    or_hat_apply_t(const or_hat* h, const double* z, double* out);  /* S^T z
 /* copies the stencil out (NULL pointers: count only); returns its nonzeros */
 int64_t or_hat_stencil(int dim, const int32_t* nsec, int64_t* ptr, int32_t* col, double* w);
 
-/* Number-defect weight of the hat form: node i's largest gross rate at its
- * own occupation x, (|1-x| (W 1)_i + |x| (W^T 1)_i) / V_i with
- * W = (n+1) K + n K^T, rk = K 1, ck = K^T 1 (the nodal form keeps
- * |f(1-f)|). */
+/* Number-defect weight of the hat form: the gross rate node i would have at
+ * its own occupation x if every other node were half filled,
+ * ((1-x) (W 1)_i + x (W^T 1)_i) / V_i with W = (n+1) K + n K^T,
+ * rk = K 1, ck = K^T 1 (the nodal form keeps |f(1-f)|). */
 double  or_hat_defect_weight(double x, double np1, double nb, double rk, double ck, double V);
 
 /* bits of the `conserve` argument of the DP5 wrappers below */
@@ -278,6 +278,11 @@ void or_collision_vjp(int64_t n, const double* K, const double* V, double bath_n
                       const double* f, const double* mu, double* out);
 void or_collision_vjp_occ(int64_t n, const double* K, const double* V, double bath_n,
                           const double* f, const or_hat* hat, const double* mu, double* out);
+/* ... of the operator with the number-defect removal when conserve != 0
+ * (I = I0 - s w, s = sum V I0 / sum V w; the weights of
+ * or_collision_apply_occ) */
+void or_collision_vjp_ex(int64_t n, const double* K, const double* V, double bath_n, const double* f,
+                         const or_hat* hat, int conserve, const double* mu, double* out);
 
 /* Discrete adjoint of the fixed-step DP5 (same step sequence as
  * or_dp5_collision_replay, all accepted) with dense output at t_final:
@@ -289,11 +294,13 @@ void or_dp5_adjoint(int64_t n, const double* K, const double* Kd, const double* 
                     double t_final, double dense_dt, const double* h_steps, int64_t n_steps,
                     int32_t ckpt, const double* dL_df, double* dL_df0, double* dL_dg,
                     double* dL_domega0);
+/* ... with hat occupation factors (hat NULL: nodal) and, with conserve, the
+ * number-defect removal in every stage evaluation */
 void or_dp5_adjoint_occ(int64_t n, const double* K, const double* Kd, const double* V,
-                        double bath_n, double dbath, double g, const or_hat* hat, const double* f0,
-                        double t0, double t_final, double dense_dt, const double* h_steps,
-                        int64_t n_steps, int32_t ckpt, const double* dL_df, double* dL_df0,
-                        double* dL_dg, double* dL_domega0);
+                        double bath_n, double dbath, double g, const or_hat* hat, int conserve,
+                        const double* f0, double t0, double t_final, double dense_dt,
+                        const double* h_steps, int64_t n_steps, int32_t ckpt, const double* dL_df,
+                        double* dL_df0, double* dL_dg, double* dL_domega0);
 
 /* ------------------------------------------------------- e-e channel */
 /* Electron-electron (4-leg) operator on the periodic lattice of the mesh box

@@ -8,7 +8,8 @@ products with K still use the nodal f.  Checks: one FP64 evaluation (2D and
 evaluation against FP64, the LDS-row and global-memory epilogues bit for bit,
 the adaptive FP64 DP5 against the oracle's (identical steps), the batched
 mixed DP5 against FP64 replays with number and energy (electrons + bath
-ledger) conservation, and the adjoint against the oracle adjoint.
+ledger) conservation, and the adjoint (with and without the number-defect
+removal in its forward map) against the oracle adjoint.
 Tolerances as in the nodal tests: 1e-13 (FP64), 2e-6 (mixed evaluation),
 1e-12 (DP5 dense outputs), 1e-6 (mixed DP5 vs replay), 1e-10 (adjoint).
 """
@@ -147,19 +148,25 @@ def test_hat_mixed_dp5_vs_fp64_replay(sq32_hat):
     assert _rel(dr[0, -1], ro["dense"][-1]) <= 1e-12
 
 
-def test_hat_adjoint_matches_oracle():
+@pytest.mark.parametrize("occ,conserve", [("hat", False), ("hat", True), ("nodal", True)])
+def test_adjoint_matches_oracle_adjoint(occ, conserve):
+    """The GPU adjoint with hat factors and/or the number-defect removal in
+    every stage against the oracle's adjoint of the same fixed-step map."""
     w = wl.Workload("square32_adj", 2, (32, 32), g=0.12, omega0=0.045, t_final=4.0)
-    print("[gpu] hat adjoint vs oracle, square32 t=4", flush=True)
+    print(f"[gpu] adjoint vs oracle, square32 t=4, {occ} factors, conserve={conserve}", flush=True)
     T = 0.02
     bath, dbath = tga.bath_occupation(w.omega0, T), tga.dbath_domega0(w.omega0, T)
-    ctx = tg.Context.create(w, max_batch=1, bath_n=bath, occupation="hat")
+    ctx = tg.Context.create(w, max_batch=1, bath_n=bath, occupation=occ)
     ctx.assemble()
     f0 = w.initial_f()
-    _, _, _, steps = ctx.dp5(f0, 0.0, 4.0, 0.04, 1e-8, 1e-12, bath_n=bath, max_dense=100, h_cap=4096)
+    _, _, _, steps = ctx.dp5(f0, 0.0, 4.0, 0.04, 1e-8, 1e-12, bath_n=bath, max_dense=100, h_cap=4096,
+                             conserve=conserve)
     dL = np.random.default_rng(0).normal(size=ctx.n)
-    g_f0, g_g, g_w = ctx.dp5_adjoint(f0, 0.0, 4.0, 0.04, steps, dL, bath, dbath, ckpt=5)
+    g_f0, g_g, g_w = ctx.dp5_adjoint(f0, 0.0, 4.0, 0.04, steps, dL, bath, dbath, ckpt=5, conserve=conserve)
     ok = oracle.Kernel(2, w.nsec, w.origin, w.extent, w.eps(), w.g, w.omega0, w.sigma)
     o_f0, o_g, o_w = oracle.dp5_adjoint(ok.full(), ok.domega_full(), ok.V, bath, dbath, w.g, f0,
-                                        0.0, 4.0, 0.04, steps, dL, ckpt=7, hat=oracle.HatStencil(2, w.nsec))
+                                        0.0, 4.0, 0.04, steps, dL, ckpt=7,
+                                        hat=oracle.HatStencil(2, w.nsec) if occ == "hat" else None,
+                                        conserve=conserve)
     assert np.linalg.norm(g_f0 - o_f0) <= 1e-10 * np.linalg.norm(o_f0)
     assert abs(g_g - o_g) <= 1e-10 * abs(o_g) and abs(g_w - o_w) <= 1e-10 * abs(o_w)

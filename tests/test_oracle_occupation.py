@@ -15,7 +15,8 @@ Pins:
   * constants are kept, the P1 mass is kept, a linear field is kept at the
     interior nodes (their stars are centrally symmetric);
   * the operator's vector-Jacobian product and the DP5 adjoint with hat
-    factors against central differences;
+    factors, with and without the number-defect removal, against central
+    differences;
   * physics: at 16x16 sections the nodal W-form gives the electron cooling
     power with the wrong sign against the reference operator (the oracle's
     tuple restatement), the hat-averaged form within 25 %.
@@ -180,3 +181,70 @@ def test_hat_occupation_fixes_the_cooling_power_at_16_sections():
     assert p_ref < 0
     assert p_nodal / p_ref < 0, (p_nodal, p_ref)
     assert 0.75 <= p_hat / p_ref <= 1.25, (p_hat, p_ref)
+
+
+@pytest.mark.parametrize("use_hat", [True, False])
+def test_vjp_with_defect_removal_against_fd(small, use_hat):
+    k, eps, K, S = small
+    hat = S if use_hat else None
+    rng = np.random.default_rng(4)
+    f = rng.uniform(0.05, 0.95, eps.size)
+    mu = rng.normal(size=f.size)
+    dirv = rng.normal(size=f.size)
+    h = 1e-6
+    Ip = oracle.collision_apply(K, k.V, 0.3, f + h * dirv, conserve=True, hat=hat)
+    Im = oracle.collision_apply(K, k.V, 0.3, f - h * dirv, conserve=True, hat=hat)
+    fd = mu @ (Ip - Im) / (2 * h)
+    ad = oracle.collision_vjp(K, k.V, 0.3, f, mu, hat=hat, conserve=True) @ dirv
+    assert abs(fd - ad) <= 1e-7 * abs(ad)
+    # the hat form's removal matters (the VJP without it differs); the nodal
+    # form conserves by itself, so sum V I0 has a zero gradient and the
+    # removal changes nothing
+    d = abs(oracle.collision_vjp(K, k.V, 0.3, f, mu, hat=hat) @ dirv - ad)
+    assert (d > 1e-6 * abs(ad)) if use_hat else (d <= 1e-10 * abs(ad))
+
+
+def test_hat_dp5_adjoint_with_defect_removal_against_fd():
+    """Gradients wrt g, omega0 (bath n_BE(omega0, T)) and f0 of a loss of the
+    fixed-step DP5 map with hat factors and the number-defect removal in
+    every stage, against central differences of that map."""
+    T = 0.02
+    bath = lambda w: 1.0 / math.expm1(w / T)
+    dbath = lambda w: -math.exp(w / T) / (T * math.expm1(w / T) ** 2)
+    nsec = (8, 8)
+    X = oracle.mesh_nodes(2, list(nsec), [-PI, -PI], [2 * PI, 2 * PI])
+    eps = 0.5 * (X ** 2).sum(1)
+    S = oracle.HatStencil(2, nsec)
+
+    def kern(g, w):
+        return oracle.Kernel(2, list(nsec), [-PI, -PI], [2 * PI, 2 * PI], eps, g, w, 0.01)
+
+    g0, w0 = 0.12, 0.045
+    k = kern(g0, w0)
+    f0 = 1 / (np.exp((eps - 0.5) / 0.02) + 1) + 0.2 * np.exp(-0.5 * ((eps - 0.9) / 0.03) ** 2)
+    run = oracle.dp5_collision(2, nsec, k.full(), k.V, bath(w0), f0, 0.0, 4.0, 0.4, 1e-8, 1e-12,
+                               conserve=True, max_dense=100, occupation="hat")
+    steps = run["h"][run["accepted"]]
+    wl8 = np.linspace(0.5, 1.5, eps.size)
+
+    def final(g, w, f):
+        kk = kern(g, w)
+        r = oracle.dp5_collision_replay(2, nsec, kk.full(), kk.V, bath(w), f, 0.0, 4.0, 0.4, steps,
+                                        np.ones(len(steps), bool), conserve=True, occupation="hat")
+        return r["dense"][-1]
+
+    loss = lambda f: float(wl8 @ (f * f))
+    fT = final(g0, w0, f0)
+    gf0, gg, gw = oracle.dp5_adjoint(k.full(), k.domega_full(), k.V, bath(w0), dbath(w0), g0, f0, 0.0, 4.0,
+                                     0.4, steps, 2 * wl8 * fT, ckpt=3, hat=S, conserve=True)
+    hg = 1e-6 * g0
+    fdg = (loss(final(g0 + hg, w0, f0)) - loss(final(g0 - hg, w0, f0))) / (2 * hg)
+    assert abs(gg - fdg) <= 1e-5 * abs(fdg), (gg, fdg)
+    hw = 1e-6 * w0
+    fdw = (loss(final(g0, w0 + hw, f0)) - loss(final(g0, w0 - hw, f0))) / (2 * hw)
+    assert abs(gw - fdw) <= 1e-5 * abs(fdw), (gw, fdw)
+    for i in (5, 40, 70):
+        e = np.zeros_like(f0)
+        e[i] = 1e-6
+        fdi = (loss(final(g0, w0, f0 + e)) - loss(final(g0, w0, f0 - e))) / 2e-6
+        assert abs(gf0[i] - fdi) <= 1e-5 * max(abs(fdi), 1e-3 * np.abs(gf0).max()), (i, gf0[i], fdi)

@@ -184,7 +184,7 @@ def load():
     L.tg_dp5_replay.restype = i32
     L.tg_dp5_replay.argtypes = [vp, vp, i64, i32, f64, f64, f64, vp, vp, i32, vp, i32, vp, i32]
     L.tg_dp5_adjoint.restype = i32
-    L.tg_dp5_adjoint.argtypes = [vp, vp, i64, f64, f64, f64, vp, i32, i32, f64, f64, vp, vp, vp]
+    L.tg_dp5_adjoint.argtypes = [vp, vp, i64, f64, f64, f64, vp, i32, i32, f64, f64, vp, vp, vp, i32]
     L.tg_conservation.restype = i32
     L.tg_conservation.argtypes = [vp, vp, i64, i32, vp, vp, i32]
     L.tg_relative_error.restype = i32
@@ -445,8 +445,10 @@ class Context:
             return f, (None if dense is None else dense[0])
         return f.T, dense
 
-    def dp5_adjoint(self, f0, t0, t1, dense_dt, steps, dL_df, bath_n, dbath_domega0, ckpt=32):
-        """Gradients of L(f(t1)) through the fixed-step DP5: (dL/df0, dL/dg, dL/domega0)."""
+    def dp5_adjoint(self, f0, t0, t1, dense_dt, steps, dL_df, bath_n, dbath_domega0, ckpt=32,
+                    conserve=False):
+        """Gradients of L(f(t1)) through the fixed-step DP5: (dL/df0, dL/dg, dL/domega0).
+        conserve: the forward map removes the number defect in every stage."""
         f0 = np.ascontiguousarray(f0, np.float64)
         steps = np.ascontiguousarray(steps, np.float64)
         g = np.ascontiguousarray(dL_df, np.float64)
@@ -454,7 +456,7 @@ class Context:
         par = np.zeros(2)
         self._check(load().tg_dp5_adjoint(self._h, _p(f0), self.n, t0, t1, dense_dt, _p(steps),
                                           steps.size, ckpt, bath_n, dbath_domega0, _p(g), _p(out),
-                                          _p(par)))
+                                          _p(par), TG_CONSERVE if conserve else 0))
         return out, par[0], par[1]
 
     # ---- hot path, device pointers (e.g. torch tensors' data_ptr())

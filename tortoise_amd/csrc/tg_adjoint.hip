@@ -10,6 +10,9 @@
 //     m = mu / V,  x = m (1 - Y),  y = m Y,  W = (n+1) K + n K^T,
 //   mu^T dF/dg      = (2/g) mu^T F(Y)            (K scales as g^2)
 //   mu^T dF/domega0 = mu^T F[W'](Y),  W' = (n+1) K' + n K'^T + n' (K + K^T),
+// (with hat occupation factors x = S Y replaces Y in the factors of the rows'
+// nodes, and with the number-defect removal F = I0 - s w adds its own terms:
+// vjp below),
 // with K' = dK/domega0 (same windows, sampled G') and n' = dn/domega0 given
 // by the caller.  Every product with K is one pass over K's band
 // (tg_stack.hip), every product with K' one pass of the dual kernel over its
@@ -53,14 +56,17 @@ __global__ void vjp_prep_kernel(int64_t n, const double* __restrict__ mu,
 }
 
 // with z (hat occupation factors) the first term goes to z (it reaches f
-// through S^T) and out gets the rest
+// through S^T) and out gets the rest.  With the number-defect removal (sc:
+// s, c on the device) the term -s mu'_k dw_k/dx_k joins the first one (mu
+// is then mu' = mu - c V, x the occupation factors).
 __global__ void vjp_out_kernel(int64_t n, const double* __restrict__ mu,
                                const double* __restrict__ V, double nb,
                                const double* __restrict__ uf, const double* __restrict__ vf,
                                const double* __restrict__ rk, const double* __restrict__ ck,
                                const double* __restrict__ ux, const double* __restrict__ vx,
                                const double* __restrict__ uy, const double* __restrict__ vy,
-                               double* __restrict__ out, double* __restrict__ z) {
+                               double* __restrict__ out, double* __restrict__ z,
+                               const double* __restrict__ sc, const double* __restrict__ x, bool hat) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double np1 = nb + 1.0;
@@ -70,7 +76,17 @@ __global__ void vjp_out_kernel(int64_t n, const double* __restrict__ mu,
     const double wtf = np1 * vf[k] + nb * uf[k];
     const double wtx = np1 * vx[k] + nb * ux[k];
     const double wy = np1 * uy[k] + nb * vy[k];
-    const double zk = -m * (wf + wt1 - wtf);
+    double zk = -m * (wf + wt1 - wtf);
+    if (sc) {
+        double dw;
+        if (hat) {
+            dw = ((np1 * ck[k] + nb * rk[k]) - (np1 * rk[k] + nb * ck[k])) / V[k];
+        } else {
+            const double p = x[k] * (1.0 - x[k]);
+            dw = p > 0.0 ? 1.0 - 2.0 * x[k] : (p < 0.0 ? 2.0 * x[k] - 1.0 : 0.0);
+        }
+        zk -= sc[0] * mu[k] * dw;
+    }
     if (z) {
         z[k] = zk;
         out[k] = wtx + wy;
@@ -97,8 +113,74 @@ __global__ void hat_apply_kernel(int64_t n, int dim, const uint32_t* __restrict_
     if (i < n) out[i] = hat_value_dim(dim, hs, i, [&](int64_t j) { return f[j]; });
 }
 
-// per-block partials of mu^T F(Y) and mu^T F[W'](Y)
 constexpr int kAdjT = 256;
+
+// The number-defect removal at a stage state: per-block partials of
+// D = sum V I0, Z = sum V w and M = sum mu w (I0 the operator before the
+// removal, w its weights: the hat form's, or |x(1-x)|)
+__global__ void defect_partial_kernel(int64_t n, const double* __restrict__ mu, const double* __restrict__ x,
+                                      const double* __restrict__ V, double nb, const double* __restrict__ uf,
+                                      const double* __restrict__ vf, const double* __restrict__ rk,
+                                      const double* __restrict__ ck, bool hat, double* __restrict__ part) {
+    __shared__ double sh[3][kAdjT / 64];
+    const int64_t i = blockIdx.x * (int64_t)kAdjT + threadIdx.x;
+    double a = 0.0, b = 0.0, e = 0.0;
+    if (i < n) {
+        const double np1 = nb + 1.0, xi = x[i];
+        const double I0 = ((1.0 - xi) * (np1 * uf[i] + nb * vf[i]) -
+                           xi * (np1 * ck[i] + nb * rk[i] - np1 * vf[i] - nb * uf[i])) / V[i];
+        const double w = hat ? hat_defect_weight(xi, np1, nb, rk[i], ck[i], V[i]) : fabs(xi * (1.0 - xi));
+        a = V[i] * I0;
+        b = V[i] * w;
+        e = mu[i] * w;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+        e += __shfl_down(e, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = b;
+        sh[2][threadIdx.x >> 6] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0, se = 0.0;
+        for (int w = 0; w < kAdjT / 64; ++w) {
+            sa += sh[0][w];
+            sb += sh[1][w];
+            se += sh[2][w];
+        }
+        part[3 * blockIdx.x] = sa;
+        part[3 * blockIdx.x + 1] = sb;
+        part[3 * blockIdx.x + 2] = se;
+    }
+}
+
+// sc = {s, c}: s = D / Z (the removal's scale), c = M / Z (fixed order, one thread)
+__global__ void defect_scale_kernel(const double* __restrict__ part, int nblk, double* __restrict__ sc) {
+    double D = 0.0, Z = 0.0, M = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+        D += part[3 * k];
+        Z += part[3 * k + 1];
+        M += part[3 * k + 2];
+    }
+    sc[0] = Z > 0.0 ? D / Z : 0.0;
+    sc[1] = Z > 0.0 ? M / Z : 0.0;
+}
+
+// mu' = mu - c V
+__global__ void mu_shift_kernel(int64_t n, const double* __restrict__ mu, const double* __restrict__ V,
+                                const double* __restrict__ sc, double* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = mu[i] - sc[1] * V[i];
+}
+
+// per-block partials of mu^T F(Y) and mu^T F[W'](Y)
+// With the hat form's number-defect removal (sc non-null) F = I0 - s w and
+// mu is mu': the weights scale as g^2 and move with omega0 through K 1,
+// K^T 1 and the bath (the nodal weights depend on neither).
 __global__ void param_partial_kernel(int64_t n, const double* __restrict__ mu,
                                      const double* __restrict__ f, const double* __restrict__ V,
                                      double nb, double dnb, const double* __restrict__ uf,
@@ -106,7 +188,8 @@ __global__ void param_partial_kernel(int64_t n, const double* __restrict__ mu,
                                      const double* __restrict__ ck, const double* __restrict__ udf,
                                      const double* __restrict__ vdf,
                                      const double* __restrict__ rkd,
-                                     const double* __restrict__ ckd, double* __restrict__ part) {
+                                     const double* __restrict__ ckd, double* __restrict__ part,
+                                     const double* __restrict__ sc) {
     __shared__ double sh[2][kAdjT / 64];
     const int64_t i = blockIdx.x * (int64_t)kAdjT + threadIdx.x;
     double a = 0.0, b = 0.0;
@@ -120,6 +203,13 @@ __global__ void param_partial_kernel(int64_t n, const double* __restrict__ mu,
                            fi * (dnb * (ck[i] + rk[i]) - dnb * (vf[i] + uf[i]))) / V[i];
         a = mu[i] * Fi;
         b = mu[i] * (Fd + Fs);
+        if (sc) {
+            const double w = hat_defect_weight(fi, np1, nb, rk[i], ck[i], V[i]);
+            const double dA = np1 * rkd[i] + nb * ckd[i] + dnb * (rk[i] + ck[i]);
+            const double dB = np1 * ckd[i] + nb * rkd[i] + dnb * (ck[i] + rk[i]);
+            a -= sc[0] * mu[i] * w;
+            b -= sc[0] * mu[i] * ((1.0 - fi) * dA + fi * dB) / V[i];
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_down(a, o, 64);
@@ -163,6 +253,9 @@ struct AdjWork {
     double *uf = nullptr, *vf = nullptr, *ux = nullptr, *vx = nullptr, *uy = nullptr,
            *vy = nullptr, *udf = nullptr, *vdf = nullptr;
     double *xbar = nullptr, *z = nullptr;   // hat occupation factors of a stage state, S^T's input
+    double* mup = nullptr;    // mu - c V (number-defect removal)
+    double* dpart = nullptr;  // defect partials, 3 per block
+    double* sc = nullptr;     // s, c of the removal
     double* seg = nullptr;    // (ckpt+1) x n step-start states of a segment
     double* ckp = nullptr;    // nck x n checkpoints
     double* part = nullptr;
@@ -178,7 +271,8 @@ void adjoint_destroy(tg_ctx* c) {
         dev_free(&w->mu[s]);
     }
     for (double** p : {&w->nu, &w->lam, &w->x, &w->y, &w->uf, &w->vf, &w->ux, &w->vx, &w->uy,
-                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z, &w->seg, &w->ckp, &w->part, &w->acc})
+                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z, &w->mup, &w->seg, &w->ckp, &w->part,
+                       &w->acc, &w->dpart, &w->sc})
         dev_free(p);
     delete w;
     c->adj = nullptr;
@@ -200,8 +294,10 @@ static tg_status adj_ensure(tg_ctx* c, int32_t ckpt, int32_t nck) {
         TG_TRY(dev_alloc(c, &w->mu[s], n));
     }
     for (double** p : {&w->nu, &w->lam, &w->x, &w->y, &w->uf, &w->vf, &w->ux, &w->vx, &w->uy,
-                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z})
+                       &w->vy, &w->udf, &w->vdf, &w->xbar, &w->z, &w->mup})
         TG_TRY(dev_alloc(c, p, n));
+    TG_TRY(dev_alloc(c, &w->dpart, 3 * (size_t)ceil_div(c->n, kAdjT)));
+    TG_TRY(dev_alloc(c, &w->sc, 2));
     TG_TRY(dev_alloc(c, &w->seg, n * (ckpt + 1)));
     TG_TRY(dev_alloc(c, &w->ckp, n * nck));
     TG_TRY(dev_alloc(c, &w->part, 2 * (size_t)ceil_div(c->n, kAdjT)));
@@ -223,6 +319,7 @@ struct AdjRun {
     tg_ctx* c;
     AdjWork* w;
     double nb, dnb, g;
+    int conserve;
     unsigned grid() const { return ceil_div(c->n, 256); }
 
     tg_status lincomb(const double* base, std::initializer_list<std::pair<double, const double*>> t,
@@ -250,7 +347,7 @@ struct AdjRun {
             hipLaunchKernelGGL(lincomb_kernel, dim3(grid()), dim3(256), 0, c->stream, c->n, y, a,
                                w->Y[s]);
             TG_CHECK_LAUNCH(c);
-            TG_TRY(plain_apply_device(c, w->Y[s], w->k[s], nb, 0));
+            TG_TRY(plain_apply_device(c, w->Y[s], w->k[s], nb, conserve));
             if (c->ee) TG_TRY(ee_apply_device(c, w->Y[s], w->k[s], 1));
         }
         return TG_OK;
@@ -269,6 +366,22 @@ struct AdjRun {
             occ = w->xbar;
         }
         TG_TRY(stack_products_device(c, Yst, w->uf, w->vf));
+        // with the number-defect removal I = I0 - s w:
+        //   mu^T dI = mu'^T dI0 - s mu'^T dw,  mu' = mu - c V,  c = mu^T w / sum V w
+        const double* sc = nullptr;
+        if (conserve) {
+            const int nblk = (int)ceil_div(n, kAdjT);
+            hipLaunchKernelGGL(defect_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, occ, c->d_V, nb,
+                               w->uf, w->vf, c->d_rk, c->d_ck, hat, w->dpart);
+            TG_CHECK_LAUNCH(c);
+            hipLaunchKernelGGL(defect_scale_kernel, dim3(1), dim3(1), 0, c->stream, w->dpart, nblk, w->sc);
+            TG_CHECK_LAUNCH(c);
+            hipLaunchKernelGGL(mu_shift_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, c->d_V, w->sc,
+                               w->mup);
+            TG_CHECK_LAUNCH(c);
+            mu = w->mup;
+            sc = w->sc;
+        }
         hipLaunchKernelGGL(vjp_prep_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, occ,
                            c->d_V, w->x, w->y);
         TG_CHECK_LAUNCH(c);
@@ -276,7 +389,7 @@ struct AdjRun {
         TG_TRY(stack_products_device(c, w->y, w->uy, w->vy));
         hipLaunchKernelGGL(vjp_out_kernel, dim3(grid()), dim3(256), 0, c->stream, n, mu, c->d_V, nb,
                            w->uf, w->vf, c->d_rk, c->d_ck, w->ux, w->vx, w->uy, w->vy, w->nu,
-                           hat ? w->z : nullptr);
+                           hat ? w->z : nullptr, sc, occ, hat);
         TG_CHECK_LAUNCH(c);
         if (hat) {
             hipLaunchKernelGGL(hat_transpose_kernel, dim3(grid()), dim3(256), 0, c->stream, n, c->dim, c->d_hat,
@@ -289,7 +402,7 @@ struct AdjRun {
         const int nblk = (int)ceil_div(n, kAdjT);
         hipLaunchKernelGGL(param_partial_kernel, dim3(nblk), dim3(kAdjT), 0, c->stream, n, mu, occ,
                            c->d_V, nb, dnb, w->uf, w->vf, c->d_rk, c->d_ck, w->udf, w->vdf,
-                           c->d_rkd, c->d_ckd, w->part);
+                           c->d_rkd, c->d_ckd, w->part, hat ? sc : nullptr);
         TG_CHECK_LAUNCH(c);
         hipLaunchKernelGGL(param_accumulate_kernel, dim3(1), dim3(1), 0, c->stream, w->part, nblk,
                            2.0 / g, w->acc);
@@ -305,7 +418,7 @@ using namespace tg;
 extern "C" tg_status tg_dp5_adjoint(tg_ctx* c, const double* f0, int64_t n, double t0, double t1,
                                     double dense_dt, const double* h_steps, int32_t n_steps,
                                     int32_t ckpt, double bath_n, double dbath_domega0,
-                                    const double* dL_df, double* dL_df0, double* dL_dparams) {
+                                    const double* dL_df, double* dL_df0, double* dL_dparams, int32_t flags) {
     if (!c || !f0 || !h_steps || !dL_df || !dL_df0 || !dL_dparams) return TG_ERR_ARG;
     if (n != c->n) return set_error(c, TG_ERR_ARG, "n mismatch");
     if (n_steps < 1 || ckpt < 1 || !(dense_dt > 0.0)) return set_error(c, TG_ERR_ARG, "bad steps");
@@ -315,7 +428,8 @@ extern "C" tg_status tg_dp5_adjoint(tg_ctx* c, const double* f0, int64_t n, doub
     const int32_t nck = n_steps / ckpt + 2;
     TG_TRY(adj_ensure(c, ckpt, nck));
     AdjWork* w = (AdjWork*)c->adj;
-    AdjRun R{c, w, bath_n, dbath_domega0, c->g};
+    if (flags & ~TG_CONSERVE) return set_error(c, TG_ERR_ARG, "tg_dp5_adjoint takes TG_CONSERVE only");
+    AdjRun R{c, w, bath_n, dbath_domega0, c->g, (flags & TG_CONSERVE) ? 1 : 0};
     hipStream_t st = c->stream;
     const size_t vb = sizeof(double) * n;
 

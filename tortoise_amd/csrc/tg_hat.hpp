@@ -65,17 +65,19 @@ __device__ __forceinline__ uint32_t hat_den(int dim, const uint32_t* __restrict_
 // Number-defect weight of the hat form.  The hat factors break the pairwise
 // symmetry that makes the nodal W-form conserve the particle number, so with
 // TG_CONSERVE the defect d = sum_i V_i I_i is removed as I_i -= s w_i,
-// s = d / sum_i V_i w_i, with w_i the largest gross rate node i can have at
-// its own occupation x_i:
-//   w_i = (|1 - x_i| (W 1)_i + |x_i| (W^T 1)_i) / V_i,  W = (n+1) K + n K^T.
-// It is smooth in the state (the DP5 sees no kinks inside [0,1]) and scales
-// with the node's rates, so every node's rates move by a similar small
-// fraction.  (Weights that vanish with x_i (1 - x_i) act mostly on the edge
-// of the occupied region, where the hat average exceeds the nodal value by
-// orders of magnitude, and make the DP5 stiff.)  rk = K 1, ck = K^T 1.
+// s = d / sum_i V_i w_i, with w_i the gross rate node i would have at its
+// own occupation x_i if every other node were half filled:
+//   w_i = ((1 - x_i) (W 1)_i + x_i (W^T 1)_i) / V_i,  W = (n+1) K + n K^T.
+// It is linear in the state (no kinks for the DP5), positive for x_i in
+// [0,1] and well beyond, and scales with the node's rates, so every node's
+// rates move by a similar small fraction.  Weights that vanish with
+// x_i (1 - x_i) concentrate the removal where the state is partially
+// occupied; on distributions the mesh does not resolve that makes s large
+// and the DP5 stiff (steps underflow on the 128 x 128-section ensemble).
+// rk = K 1, ck = K^T 1.
 __host__ __device__ inline double hat_defect_weight(double x, double np1, double nb, double rk, double ck,
                                                     double V) {
-    return (fabs(1.0 - x) * (np1 * rk + nb * ck) + fabs(x) * (np1 * ck + nb * rk)) / V;
+    return ((1.0 - x) * (np1 * rk + nb * ck) + x * (np1 * ck + nb * rk)) / V;
 }
 
 // Builds the packed stencil of the structured mesh (host); false when the

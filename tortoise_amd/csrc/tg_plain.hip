@@ -14,27 +14,41 @@ namespace tg {
 //     [0,1]);
 //   - with flux, the e-ph electron energy flux of the corrected I:
 //     sum V eps I - s sum V eps w (one value, written by thread 0).
-__global__ void conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
-                                    const double* __restrict__ f, const double* __restrict__ wt,
-                                    double* __restrict__ I, int conserve, double* __restrict__ flux) {
-    double d = 0.0, w = 0.0, ei = 0.0, ew = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        d += dpart[4 * b];
-        w += dpart[4 * b + 1];
-        ei += dpart[4 * b + 2];
-        ew += dpart[4 * b + 3];
+constexpr int kFixT = 256;
+
+__global__ void __launch_bounds__(kFixT)
+conserve_fix_kernel(const double* __restrict__ dpart, int nblk, int64_t n,
+                    const double* __restrict__ f, const double* __restrict__ wt,
+                    double* __restrict__ I, int conserve, double* __restrict__ flux) {
+    // every block sums the partials the same way: strided per thread, then
+    // a fixed tree over the lanes and waves
+    __shared__ double red[4][kFixT / 64];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += kFixT)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] += dpart[4 * b + q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        for (int o = 32; o > 0; o >>= 1) a[q] += __shfl_down(a[q], o, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = a[q];
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a[q] = 0.0;
+        for (int w = 0; w < kFixT / 64; ++w) a[q] += red[q][w];
+    }
+    const double d = a[0], w = a[1], ei = a[2], ew = a[3];
     const double s = (conserve && w > 0.0) ? d / w : 0.0;
     if (flux && blockIdx.x == 0 && threadIdx.x == 0) *flux = ei - s * ew;
     if (s == 0.0) return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = blockIdx.x * (int64_t)kFixT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFixT)
         I[i] -= s * (wt ? wt[i] : fabs(f[i] * (1.0 - f[i])));
 }
 
 tg_status conserve_fix_device(tg_ctx* c, const double* dpart, int nblk, const double* d_f, const double* d_wt,
                               double* d_I, int conserve, double* flux) {
-    hipLaunchKernelGGL(conserve_fix_kernel, dim3(conserve ? ceil_div(c->n, 256) : 1), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(conserve_fix_kernel, dim3(conserve ? ceil_div(c->n, kFixT) : 1), dim3(kFixT), 0, c->stream,
                        dpart, nblk, c->n, d_f, d_wt, d_I, conserve, flux);
     TG_CHECK_LAUNCH(c);
     return TG_OK;

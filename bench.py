@@ -173,11 +173,12 @@ def dp5_plain(ctx, w, f0, bath_n):
             "bath_energy": st["bath_energy"]}
 
 
-def cube_ee(steps=5, t_final=1.0):
+def cube_ee(steps=5, t_final=50.0, adj_t_final=1.0):
     """The 3D workload with both channels: 20^3 sections (48000 tetrahedra,
     9261 nodes, 8000 periodic points), 16 members, e-ph on the batched mixed
     path plus the e-e channel (V(q) = 1/(|q|^2 + 0.09), sigma_ee = 0.02).
-    Batched evaluation time, then a DP5 run to t_final with number drift."""
+    Batched evaluation time, then the ensemble's DP5 run to t_final with
+    number drift, and member 0's adjoint gradient wrt g over [0, adj_t_final]."""
     import numpy as np
     import torch
 
@@ -214,21 +215,21 @@ def cube_ee(steps=5, t_final=1.0):
     from tortoise_amd import autograd as tga
     f0 = F[:, 0].copy()
     t0 = time.perf_counter()
-    _, dense, st0, steps = ctx.dp5(f0, 0.0, t_final, t_final / 100, w.rtol, w.atol,
+    _, dense, st0, steps = ctx.dp5(f0, 0.0, adj_t_final, adj_t_final / 100, w.rtol, w.atol,
                                    bath_n=bath[0], max_dense=100, h_cap=1 << 16, conserve=True)
     fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, dLdg, _ = ctx.dp5_adjoint(f0, 0.0, t_final, t_final / 100, steps, dense[-1] - f0, bath[0],
+    _, dLdg, _ = ctx.dp5_adjoint(f0, 0.0, adj_t_final, adj_t_final / 100, steps, dense[-1] - f0, bath[0],
                                  tga.dbath_domega0(w.omega0, temp[0]), ckpt=32, conserve=True)
     adj = time.perf_counter() - t0
     ctx.close()
-    _log(f"cube20 adjoint, member 0: forward FP64 DP5 to t={t_final} in {fwd:.1f} s "
+    _log(f"cube20 adjoint, member 0: forward FP64 DP5 to t={adj_t_final} in {fwd:.1f} s "
          f"({st0['accepted']} steps), adjoint in {adj:.1f} s, dL/dg = {dLdg:.6e}")
     return {"workload": w.name, "members": B, "nodes": int(eps.size), "lattice_points": 8000,
             "ms_per_batched_eval": ms, "member_evals_per_s": B * 1000.0 / ms,
             "dp5_t_final": t_final, "dp5_wall_s": wall, "dp5_attempts_max": int(max(att)),
             "number_drift_rel": drift,
-            "adjoint_member0": {"loss": "|f(t_final) - f0|^2 / 2", "steps": int(st0["accepted"]),
+            "adjoint_member0": {"loss": "|f(t) - f0|^2 / 2", "t": adj_t_final, "steps": int(st0["accepted"]),
                                 "forward_s": fwd, "adjoint_s": adj, "dL_dg": dLdg}}
 
 
@@ -581,8 +582,10 @@ def main():
                     help="only the assembly and the DP5 runs on the 128x128 ensemble (profiling runs)")
     ap.add_argument("--only-gradients", action="store_true",
                     help="only the 64x64 gradient line (adjoint of DP5)")
-    ap.add_argument("--ee-t-final", type=float, default=1.0,
+    ap.add_argument("--ee-t-final", type=float, default=50.0,
                     help="DP5 end time of the 20^3 e-e ensemble run")
+    ap.add_argument("--ee-adjoint-t-final", type=float, default=1.0,
+                    help="end time of the 20^3 member-0 adjoint gradient")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -611,7 +614,8 @@ def main():
         return
     if args.only_ee:
         torch.cuda.init()
-        print(json.dumps({"cube20_ee": cube_ee(t_final=args.ee_t_final)}), flush=True)
+        print(json.dumps({"cube20_ee": cube_ee(t_final=args.ee_t_final, adj_t_final=args.ee_adjoint_t_final)}),
+              flush=True)
         return
 
     w = wl.SQUARE128_ENSEMBLE
@@ -787,7 +791,7 @@ def main():
         out["cpu_baseline"] = out["cpu_wform_band"]
     ctx.close()
     if not args.no_dp5:
-        out["cube20_ee"] = cube_ee(t_final=args.ee_t_final)
+        out["cube20_ee"] = cube_ee(t_final=args.ee_t_final, adj_t_final=args.ee_adjoint_t_final)
         out["square64_gradients"] = square64_gradients()
     if not args.no_reference_op:
         ro = reference_operator(cpu_budget_s=0.0 if args.no_cpu else 30.0, member0=member0)

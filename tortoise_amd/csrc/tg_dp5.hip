@@ -183,9 +183,14 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, doubl
         p.dim = c->dim;
         // finish k_s (index s-1 of k) evaluated at `fin`, then the next stage
         // state y + sum (a_j h) k_j over the k's in `terms`, k_s last
+        // cfin: the coefficients of fin's own sum over the first of `terms`
+        // (the previous pass's), so the pass recomputes fin instead of reading it
         auto pass = [&](int mode, int s_fin, const double* fin, std::initializer_list<int> terms,
-                        std::initializer_list<double> ca, double* ys, double cs) -> tg_status {
+                        std::initializer_list<double> ca, double* ys, double cs,
+                        std::initializer_list<double> cfin = {}) -> tg_status {
             p.fin = fin;
+            p.nfin = 0;
+            for (double v : cfin) p.cfin[p.nfin++] = v;
             p.kout = s_fin > 0 ? k[s_fin - 1] : nullptr;
             p.flux = s_fin > 0 ? w->flux + (int64_t)(s_fin - 1) * w->cap : nullptr;
             int j = 0;
@@ -200,10 +205,11 @@ static tg_status attempt(tg_ctx* c, Dp5Work* w, int32_t B, const Ctl& ctl, doubl
         };
         // stages k2..k6 (DormandPrince54.hpp:565-574)
         TG_TRY(pass(kDp5Stage, 0, nullptr, {0}, {A21}, w->ys, C2));
-        TG_TRY(pass(kDp5Mid, 2, w->ys, {0}, {A31, A32}, w->ys, C3));
-        TG_TRY(pass(kDp5Mid, 3, w->ys, {0, 1}, {A41, A42, A43}, w->ys, C4));
-        TG_TRY(pass(kDp5Mid, 4, w->ys, {0, 1, 2}, {A51, A52, A53, A54}, w->ys, C5));
-        TG_TRY(pass(kDp5Mid, 5, w->ys, {0, 1, 2, 3}, {A61, A62, A63, A64, A65}, w->ys, 1.0));
+        TG_TRY(pass(kDp5Mid, 2, w->ys, {0}, {A31, A32}, w->ys, C3, {A21}));
+        TG_TRY(pass(kDp5Mid, 3, w->ys, {0, 1}, {A41, A42, A43}, w->ys, C4, {A31, A32}));
+        TG_TRY(pass(kDp5Mid, 4, w->ys, {0, 1, 2}, {A51, A52, A53, A54}, w->ys, C5, {A41, A42, A43}));
+        TG_TRY(pass(kDp5Mid, 5, w->ys, {0, 1, 2, 3}, {A61, A62, A63, A64, A65}, w->ys, 1.0,
+                    {A51, A52, A53, A54}));
         // y5 (b2 = 0) and the y4 prefix, then k7 (:576-578), both at t + h
         const double cb[5] = {BH1, BH3, BH4, BH5, BH6};
         for (int q = 0; q < 5; ++q) p.cb[q] = cb[q];

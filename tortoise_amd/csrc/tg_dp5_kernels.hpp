@@ -149,6 +149,9 @@ struct Dp5PassArgs {
     double* y4;               // y4 prefix: written by the kDp5Mid pass before y5 (else null), read by kDp5Last
     double c7;                // kDp5Last: bh7
     const double* fin;        // kDp5Mid, kDp5Last: the state the finished k is evaluated at
+    int32_t nfin;             // kDp5Mid: > 0 when fin = y + sum_{j < nfin} (cfin_j h) kg[j] with the
+    double cfin[6];           //   terms this pass reads (the previous pass's sum, recomputed in the same
+                              //   order instead of reading fin again for the number-defect weights)
     double* kout;             // the finished k
     double* flux;             // its e-ph electron energy flux per member (null: none)
     const double* h;          // per-member step of the attempt; 0: finished member, skipped

@@ -1182,12 +1182,14 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
     if (MODE != kDp5Last) load_pack_nodes(pk, n, v.kgroups, v.kt, v.npanel, v.perm);
     if (live) {
         const double sc = MODE != kDp5Stage ? s_b : 0.0;
-        double c[7], cb[7];
+        double c[7], cb[7], cf[6];
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
             c[j] = a.ca[j] * hb;
             cb[j] = a.cb[j] * hb;
         }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) cf[j] = a.cfin[j] * hb;
         const double c7 = hb * a.c7;
         const int ng = a.ng;
         const bool want_y4 = MODE == kDp5Mid && a.y4 != nullptr;
@@ -1202,9 +1204,11 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
                     for (int j = 0; j < 6; ++j)
                         if (j < ng) kv[j][u] = a.kg[j][i];
                 }
-                if (MODE != kDp5Stage && (a.conserve || MODE == kDp5Last)) fv[u] = a.fin[i];
-                // the number-defect weights |x(1-x)| use the occupation factors
-                if (MODE != kDp5Stage && a.conserve) xv[u] = v.o.xb ? v.o.xb[i] : fv[u];
+                // fin for the number-defect weights |x(1-x)| (nodal factors; the
+                // hat factors come from o.xb) or for y5 in the last pass
+                const bool need_fin = MODE == kDp5Last || (MODE == kDp5Mid && a.conserve && !v.o.xb && a.nfin == 0);
+                if (need_fin) fv[u] = a.fin[i];
+                if (MODE != kDp5Stage && a.conserve && v.o.xb) xv[u] = v.o.xb[i];
                 if (MODE == kDp5Last) y4v[u] = a.y4[i];
             }
 #pragma unroll
@@ -1216,6 +1220,18 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
                 if (MODE != kDp5Stage) {
                     kn = row[p];
                     if (a.conserve) {
+                        if (!v.o.xb) {
+                            if (MODE == kDp5Mid && a.nfin > 0) {
+                                // fin recomputed bit for bit: the previous pass's sum
+                                double x = yv[u];
+#pragma unroll
+                                for (int j = 0; j < 6; ++j)
+                                    if (j < a.nfin) x = x + cf[j] * kv[j][u];
+                                xv[u] = x;
+                            } else {
+                                xv[u] = fv[u];
+                            }
+                        }
                         const double x = xv[u];
                         const double w = v.o.xb ? hat_defect_weight(x, nbk + 1.0, nbk, v.rkc[p], v.ckc[p], v.Vc[p])
                                                 : fabs(x * (1.0 - x));

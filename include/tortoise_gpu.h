@@ -171,6 +171,8 @@ typedef struct tg_kernel_stats {
                                the persistent kernel (their products are exactly 0) */
     int64_t mixed_last_band;/* entries of [K; K^T] the last batched evaluation streamed (its panels
                                with work; mixed_band when nothing was left out) */
+    int32_t mixed_cuts;     /* panels whose band the persistent kernel computes as two work units
+                               (summed into two partial-sum slots the epilogue adds) */
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

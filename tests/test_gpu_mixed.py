@@ -184,13 +184,15 @@ def test_mixed_tile_configurations(monkeypatch):
 @pytest.mark.parametrize("mesh", ["square32", "square64"])
 def test_mixed_persistent_configurations(monkeypatch, mesh):
     """The persistent kernel (balanced panel schedule) in each configuration
-    gives the FP64 operator to the same tolerance, and with one K-split and
-    the same fold interval it sums in the order of the tiled kernel: the two
-    results are equal bit for bit."""
+    gives the FP64 operator to the same tolerance, and with whole panels as
+    work units (no cut bands), one K-split and the same fold interval it sums
+    in the order of the tiled kernel: the two results are equal bit for bit.
+    With cut bands (here every band longer than 8 iterations computed as two
+    chunks) it gives the FP64 operator to the same tolerance."""
     w = wl.SQUARE32 if mesh == "square32" else wl.Workload("square64", 2, (64, 64))
     B = 256
     F, bath = _members(w, B, seed=13)
-    for k in ("TG_MIXED_WAVES", "TG_MIXED_SPLITS", "TG_MIXED_AL"):
+    for k in ("TG_MIXED_WAVES", "TG_MIXED_SPLITS", "TG_MIXED_AL", "TG_MIXED_CUT"):
         monkeypatch.delenv(k, raising=False)
     ref = None
     tiled = {}
@@ -209,6 +211,18 @@ def test_mixed_persistent_configurations(monkeypatch, mesh):
             ref = ctx.apply(F[:, ::37], bath_n=bath[::37])
         ctx.close()
     monkeypatch.delenv("TG_MIXED_SPLITS")
+    # cut bands, default tile configuration
+    monkeypatch.setenv("TG_MIXED_PERSIST", "1")
+    monkeypatch.setenv("TG_MIXED_CUT", "8")
+    print(f"[gpu] {mesh} persistent, bands cut at 8 iterations", flush=True)
+    ctx = tg.Context.create(w, max_batch=B)
+    ctx.assemble(mixed=True)
+    assert ctx.kernel_stats()["mixed_cuts"] > 0
+    Ic = ctx.apply(F, bath_n=bath, mixed=True)
+    for q, b in enumerate(range(0, B, 37)):
+        assert _rel(Ic[:, b], ref[:, q]) <= 2e-6, b
+    ctx.close()
+    monkeypatch.setenv("TG_MIXED_CUT", "0")
     for nw, fold, depth, waves, al in PERSIST_CONFIGS:
         print(f"[gpu] {mesh} persistent {nw} fold {fold} depth {depth} waves {waves} A-in-LDS {al}",
               flush=True)
@@ -222,7 +236,7 @@ def test_mixed_persistent_configurations(monkeypatch, mesh):
         ctx.assemble(mixed=True)
         st = ctx.kernel_stats()
         assert (st["mixed_persist"], st["mixed_tile"], st["mixed_fold"], st["mixed_depth"],
-                st["mixed_waves"], st["mixed_splits"]) == (1 + al, nw, fold, depth, waves, 1)
+                st["mixed_waves"], st["mixed_splits"], st["mixed_cuts"]) == (1 + al, nw, fold, depth, waves, 1, 0)
         Im = ctx.apply(F, bath_n=bath, mixed=True)
         for q, b in enumerate(range(0, B, 37)):
             assert _rel(Im[:, b], ref[:, q]) <= 2e-6, (nw, fold, depth, waves, al, b)
@@ -351,3 +365,40 @@ def test_mixed_zero_operand_panels_left_out_exactly(monkeypatch):
     # the cold ensemble leaves most of the band out, the hot members none of it
     assert a[3]["mixed_last_band"] < 0.6 * a[3]["mixed_band"], a[3]
     assert a[4]["mixed_last_band"] == a[4]["mixed_band"], a[4]
+
+
+def test_mixed_cut_bands(monkeypatch):
+    """Bands cut in two work units (here every band longer than 6
+    iterations): each chunk is summed on its own into a partial-sum slot and
+    the epilogue adds the slots in order, so the result does not depend on
+    which chunks the schedule leaves out (bit for bit the same with every
+    unit computed), matches the FP64 operator to the mixed tolerance, and the
+    batched DP5 takes the same steps with and without leaving units out."""
+    w = wl.Workload("square64", 2, (64, 64))
+    B = 256
+    F, bath = _members(w, B, seed=21)
+    for k, v in [("TG_MIXED_PERSIST", "1"), ("TG_MIXED_NW", "128"), ("TG_MIXED_WAVES", "8"),
+                 ("TG_MIXED_FOLD", "2"), ("TG_MIXED_DEPTH", "2"), ("TG_MIXED_AL", "1"),
+                 ("TG_MIXED_CUT", "6")]:
+        monkeypatch.setenv(k, v)
+    out = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("TG_MIXED_SKIP", skip)
+        print(f"[gpu] mixed square64 B=256, bands cut at 6 iterations, skip={skip}", flush=True)
+        ctx = tg.Context.create(w, max_batch=B)
+        ctx.assemble(mixed=True)
+        st = ctx.kernel_stats()
+        I = ctx.apply(F, bath_n=bath, mixed=True, conserve=True)
+        st_run = ctx.kernel_stats()
+        _, _, dst = ctx.dp5(F[:, :16], 0.0, 10.0, 0.1, 1e-6, 1e-12, bath_n=bath[:16], mixed=True, conserve=True)
+        if skip == "1":
+            Ip = ctx.apply(F[:, ::51], bath_n=bath[::51], conserve=True)
+            for q, b in enumerate(range(0, B, 51)):
+                assert _rel(I[:, b], Ip[:, q]) <= 2e-6, b
+        out[skip] = (I, [s["attempted"] for s in dst], st, st_run)
+        ctx.close()
+    a, b = out["1"], out["0"]
+    assert a[2]["mixed_cuts"] > 0 and a[2]["mixed_cuts"] == b[2]["mixed_cuts"]
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    # the cold ensemble leaves units out
+    assert a[3]["mixed_last_band"] < a[3]["mixed_band"], a[3]

@@ -34,6 +34,7 @@ constexpr int kHatMultBits = 6;
 // global).  The products are summed in slot order, the same in every kernel.
 template <int S, typename Load>
 __device__ __forceinline__ double hat_value(const uint32_t* __restrict__ row, Load load) {
+#pragma clang fp contract(off)
     uint32_t w[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) w[k] = row[k];
@@ -77,6 +78,7 @@ __device__ __forceinline__ uint32_t hat_den(int dim, const uint32_t* __restrict_
 // rk = K 1, ck = K^T 1.
 __host__ __device__ inline double hat_defect_weight(double x, double np1, double nb, double rk, double ck,
                                                     double V) {
+#pragma clang fp contract(off)
     return ((1.0 - x) * (np1 * rk + nb * ck) + x * (np1 * ck + nb * rk)) / V;
 }
 

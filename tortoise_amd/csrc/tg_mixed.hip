@@ -53,6 +53,9 @@ constexpr int kKS = 32;         // reduction depth per iteration (two MFMA k-ste
 constexpr int kKG = kKS / 8;    // 16-byte k-groups per iteration
 constexpr int kMixThreads = 256;
 constexpr int kSF = 12;         // F scale exponent: f in [-8, 8] stays inside fp16
+// a persistent work unit is (panel << 1) | chunk: a panel's band is one
+// chunk or two (partial-sum slots 0 and 1)
+constexpr int kMaxChunks = 2;
 
 struct MixedData {
     int64_t rows = 0;          // 2n
@@ -80,8 +83,18 @@ struct MixedData {
     double* ck_c = nullptr;
     const double* V_s = nullptr;    // node volumes (sorted order, owned by the context)
     const double* eps_s = nullptr;  // node energies (sorted order, owned by the context)
-    uint8_t* P = nullptr;      // nsplit x nt x rows_pad partial sums (FP64, FP32 if fold == 0)
-    int nsplit = 1;
+    uint8_t* P = nullptr;      // slots x nt x rows_pad partial sums (FP64, FP32 if fold == 0)
+    int nsplit = 1;            // K-splits of the tiled kernel (its slots)
+    // the persistent kernel's work units: a band longer than cut_min
+    // iterations is cut in two chunks (partial-sum slots 0 and 1)
+    int cut_min = 0;
+    std::vector<int32_t> h_cut;      // per panel: iterations of chunk 0 (the band length when not cut)
+    int32_t* cut = nullptr;
+    int nunit = 0;                   // units: (panel << 1) | chunk
+    int pslots = 1;                  // slots of the persistent kernel (2 when some band is cut)
+    uint8_t* allmask = nullptr;      // per panel: the slots a schedule of every unit writes
+    int2* srange = nullptr;          // per unit: [first iteration, count] of its whole chunk
+    int2* durange = nullptr;         // per unit: the trimmed range the last schedule gave it
     int nw = 128;              // members per workgroup tile
     int fold = 2;              // iterations per FP64 fold (0: FP32 partials)
     int waves = 4;             // waves per workgroup
@@ -103,10 +116,10 @@ struct MixedData {
     uint8_t* nzG = nullptr;          // nit
     uint8_t* useF = nullptr;         // npanel
     uint8_t* useG = nullptr;         // npanel
-    uint8_t* actp = nullptr;         // npanel: 1 if the last schedule computed the panel
-    int32_t* lpt_order = nullptr;    // panels by decreasing band length
+    uint8_t* actp = nullptr;         // npanel: the slots (chunks) the last schedule computed
+    int32_t* lpt_order = nullptr;    // units by decreasing length
     int32_t* dsched_off = nullptr;   // nbins + 1
-    int32_t* dsched_panel = nullptr; // npanel
+    int32_t* dsched_panel = nullptr; // nunit
     int64_t* dstat = nullptr;        // iterations the last schedule holds
     int32_t* dslot = nullptr;        // nt: a fused DP5 attempt's column of each member (-1: not running)
     int32_t* dncol = nullptr;        // 1: the columns running members take (leading ones)
@@ -121,6 +134,18 @@ struct MixedData {
 };
 
 static MixedData* md_of(tg_ctx* c) { return (MixedData*)c->mixed; }
+
+// partial-sum slots of the configured kernel, and the per-panel slot masks
+// the epilogue reads (the last schedule's, the static ones of a schedule of
+// every unit, or null: every panel has all slots of the tiled kernel)
+static int slots(const MixedData* m) { return m->persist ? m->pslots : m->nsplit; }
+static const uint8_t* slot_masks(const MixedData* m, bool dyn) {
+    return m->persist ? (dyn ? m->actp : m->allmask) : nullptr;
+}
+// dynamic LDS of active_schedule_kernel
+static size_t sched_lds(const MixedData* m) {
+    return sizeof(int32_t) * (2 * ((size_t)m->nit + 1) + m->npanel + m->nunit);
+}
 
 // ------------------------------------------------------------------ packing
 // Band-major packed A from K's entries (tg_kentry.hpp): block (panel,
@@ -641,14 +666,24 @@ mixed_gemm_kernel(const u32x4* __restrict__ Aband, const int64_t* __restrict__ b
 // waves, so one wave's FP64 folds and memory waits overlap the other's
 // MFMAs.
 //
-// The host balances the panels over the workgroups (longest band first onto
-// the least loaded bin, a fixed order, so every call runs the same
-// schedule).  A bin is nmt workgroups on one XCD that run the same panel
-// list for the nmt member tiles side by side, so a panel's A band is fetched
-// into that XCD's L2 once for all member tiles.  Bin b runs on XCD b % 8:
-// consecutive bins of a schedule (the longest panels first) spread over the
-// eight L2s, also when only a few bins have work.  With nsplit = 1 and the
-// same FOLD the summation order is that of mixed_gemm_kernel.
+// The work units are chunks of the panels' bands: a band is one chunk, or
+// (optionally) two cut at a fixed iteration set once per context, each
+// summed on its own and written to partial-sum slot 0 or 1; the epilogue
+// adds the slots in slot order.  A unit computes the iterations urange gives
+// it: its whole chunk, or (schedules that leave out zero operands) the part
+// from the first to the last iteration holding a nonzero operand of a kind
+// its panel uses, the start moved back to the chunk's fold grid.  The
+// iterations left out add products of zero operands, exactly 0 to the FP32
+// sums, and the FP64 folds fall on the same iterations, so the result is
+// bit for bit that of the whole chunk.
+// The host balances the units over the workgroups (longest first onto the
+// least loaded bin, a fixed order, so every call runs the same schedule).
+// A bin is nmt workgroups on one XCD that run the same unit list for the
+// nmt member tiles side by side, so a panel's A band is fetched into that
+// XCD's L2 once for all member tiles.  Bin b runs on XCD b % 8: consecutive
+// bins of a schedule (the longest units first) spread over the eight L2s,
+// also when only a few bins have work.  With whole panels and the same FOLD
+// the summation order is that of mixed_gemm_kernel with one split.
 // The member tiles a persistent launch computes: all nmt, or with ncol (the
 // columns running members fill, from the fused DP5's compaction) the tiles
 // those columns reach.  Each XCD's workgroups then regroup into bins of that
@@ -667,7 +702,8 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
                           const uint8_t* __restrict__ useg, int64_t rows_pad, int nt,
                           const int2* __restrict__ krange, const int32_t* __restrict__ sched_off,
                           const int32_t* __restrict__ sched_panel, int nmt, int bins_per_xcd,
-                          const int32_t* __restrict__ ncol, double* __restrict__ P) {
+                          const int32_t* __restrict__ ncol, const int2* __restrict__ urange,
+                          int64_t slot_stride, double* __restrict__ P) {
     static_assert(FOLD > 0, "the persistent kernel folds into FP64");
     constexpr int NTH = 256 * MG;
     constexpr int WN = NW / MG;                    // members per wave
@@ -697,12 +733,16 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
     const int col0 = mt * NW;
 
     for (int s = sched_off[bin]; s < sched_off[bin + 1]; ++s) {
-        const int panel = sched_panel[s];
+        // work unit: chunk ch of the panel, urange[unit].y iterations from
+        // iteration urange[unit].x of its band, summed into slot ch
+        const int unit = sched_panel[s];
+        const int panel = unit >> 1, ch = unit & 1;
         const int64_t row0 = (int64_t)panel * kMT;
         const int arow = rg * 32 + (lane & 31);
-        const u32x4* Ab = Aband + band_off[panel];
         const int2 kr = krange[panel];
-        const int N = kr.y - kr.x;
+        const int2 ur = urange[unit];
+        const int k0 = ur.x, N = ur.y;
+        const u32x4* Ab = Aband + band_off[panel] + (int64_t)k0 * (2 * kKG * kMT);
         // operand of the member of every Z unit this thread stages
         const u32x4* Zhi[Z_PER_T];
         const u32x4* Zlo[Z_PER_T];
@@ -747,7 +787,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
                 }
             };
             auto load_z = [&](u32x4 (&r)[Z_PER_T], int k) {
-                stage_z<NTH, NW>(r, Zhi, Zlo, nt, col0, (int64_t)(kr.x + min(k, N - 1)), tid);
+                stage_z<NTH, NW>(r, Zhi, Zlo, nt, col0, (int64_t)(kr.x + k0 + min(k, N - 1)), tid);
             };
             auto store_z = [&](const u32x4 (&r)[Z_PER_T], int buf) {
 #pragma unroll
@@ -835,7 +875,7 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
 #pragma unroll
         for (int u = 0; u < TN; ++u) {
             const int col = col0 + mg * WN + u * 32 + (lane & 31);
-            double* dst = P + (int64_t)col * rows_pad + row0 + rg * 32 + 4 * (lane >> 5);
+            double* dst = P + ch * slot_stride + (int64_t)col * rows_pad + row0 + rg * 32 + 4 * (lane >> 5);
 #pragma unroll
             for (int e = 0; e < 16; ++e) dst[8 * (e / 4) + (e % 4)] = sum[u][e];
         }
@@ -860,15 +900,18 @@ mixed_gemm_persist_kernel(const u32x4* __restrict__ Aband, const int64_t* __rest
 // I replaces f in `row` (ROW) or goes to Ib.  Returns the FP64 partial sums
 // of V I and V |f(1-f)| (and, with want_e, of eps V I and eps V |f(1-f)|).
 template <bool ROW, typename PT>
-__device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad,
+__device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nslot, int nt, int64_t rows_pad,
                                               int64_t n, int64_t kt, int b, const double* __restrict__ fb,
                                               const int32_t* __restrict__ perm, const double* __restrict__ rk,
                                               const double* __restrict__ ck, const double* __restrict__ V,
                                               double nb, const uint8_t* __restrict__ useg,
-                                              const uint8_t* __restrict__ actp, int npanel, uint8_t* sflag,
+                                              const uint8_t* __restrict__ pmask, int npanel, uint8_t* sflag,
                                               double inv_scale, const double* __restrict__ eps_s, bool want_e,
                                               bool hatw, double* row, double* __restrict__ Ib, double& dI,
                                               double& dw, double& eI, double& ew) {
+    // no contraction into FMAs: every kernel that inlines this (LDS-row and
+    // global-memory epilogues, the fused DP5 passes) rounds the same way
+#pragma clang fp contract(off)
     const double np1 = nb + 1.0;
     // UNR positions per thread per round with all their loads issued before
     // use; the trip count is the same for every thread (the barrier after the
@@ -877,11 +920,15 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
 #define TG_FIN_UNR 4
 #endif
     constexpr int UNR = TG_FIN_UNR;
-    double u[UNR], v[UNR], rkp[UNR], ckp[UNR], Vp[UNR], Ep[UNR];
+    double u[UNR], v[UNR], u1[UNR], v1[UNR], rkp[UNR], ckp[UNR], Vp[UNR], Ep[UNR];
     int32_t node[UNR], gu[UNR], gv[UNR];   // gu, gv: the panels' operand bytes
-    // the partial sums (streamed from HBM), read only for the panels the GEMM
-    // computed: a panel left out (bit 1 of its operand byte) was not written
-    // and its product is 0, so its bytes need not move ...
+    // the partial sums (streamed from HBM), read only from the slots the GEMM
+    // wrote: bit 1 (2) of the panel byte for slot 0 (1); a panel left out
+    // has neither, its product is 0 and its bytes need not move.  Slot 1
+    // goes to registers of its own, so every load of a round is issued before
+    // the first sum.  Slots 2 and up (tiled kernel with more K-splits) add
+    // into slot 1's value ...
+    const int64_t slot = (int64_t)nt * rows_pad;
     auto load_products = [&](int64_t p0) {
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
@@ -889,24 +936,26 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
             const PT* P0 = P + (int64_t)b * rows_pad;
             u[j] = 0.0;
             v[j] = 0.0;
-            if (!(gu[j] & 2)) {
-                u[j] = (double)P0[p];
-                for (int s = 1; s < nsplit; ++s) u[j] += (double)P[((int64_t)s * nt + b) * rows_pad + p];
-            }
-            if (!(gv[j] & 2)) {
-                v[j] = (double)P0[kt + p];
-                for (int s = 1; s < nsplit; ++s) v[j] += (double)P[((int64_t)s * nt + b) * rows_pad + kt + p];
+            u1[j] = 0.0;
+            v1[j] = 0.0;
+            if (gu[j] & 2) u[j] = (double)P0[p];
+            if (gu[j] & 4) u1[j] = (double)P0[slot + p];
+            if (gv[j] & 2) v[j] = (double)P0[kt + p];
+            if (gv[j] & 4) v1[j] = (double)P0[slot + kt + p];
+            for (int s = 2; s < nslot; ++s) {
+                if (gu[j] & 2) u1[j] += (double)P0[s * slot + p];
+                if (gv[j] & 2) v1[j] += (double)P0[s * slot + kt + p];
             }
         }
     };
-    // ... after the operand bytes and the panel flags (actp, null when every
-    // panel is computed), which say which panels those are.  With ROW one
-    // byte per panel (the member's operand choice in bit 0, bit 1 for a panel
-    // left out) is staged in LDS (sflag) once, so a round waits for one
-    // memory trip, not two ...
+    // ... after the operand bytes and the panel slot masks (pmask: bit s for
+    // slot s; null when every panel has all nslot slots), which say which
+    // panels those are.  With ROW one byte per panel (the member's operand
+    // choice in bit 0, the slots in bits 1 and 2) is staged in LDS (sflag)
+    // once, so a round waits for one memory trip, not two ...
     auto panel_byte = [&](int64_t pn) {
         int g = useg[pn * nt + b];
-        if (actp) g |= actp[pn] ? 0 : 2;
+        g |= pmask ? (pmask[pn] & 3) << 1 : (nslot > 1 ? 6 : 2);
         return g;
     };
     auto load_bytes = [&](int64_t p0) {
@@ -941,15 +990,14 @@ __device__ __forceinline__ void finish_energy(const PT* __restrict__ P, int nspl
         // memory round trip per round, not one per position)
 #pragma unroll
         for (int j = 0; j < UNR; ++j)
-            asm volatile("" ::"v"(u[j]), "v"(v[j]), "v"(rkp[j]), "v"(ckp[j]), "v"(Vp[j]), "v"(node[j]),
-                         "v"(gu[j]), "v"(gv[j]));
+            asm volatile("" ::"v"(u[j]), "v"(v[j]), "v"(u1[j]), "v"(v1[j]), "v"(rkp[j]), "v"(ckp[j]), "v"(Vp[j]),
+                         "v"(node[j]), "v"(gu[j]), "v"(gv[j]));
 #pragma unroll
         for (int j = 0; j < UNR; ++j) {
             if (p0 + (int64_t)j * kMemT >= n) break;
-            // bit 0 = g (z = 1 - f) rather than f, bit 1 = the panel was left
-            // out of the GEMM (all operands zero: its product is 0)
-            const double uu = u[j] * inv_scale;
-            const double vv = v[j] * inv_scale;
+            // bit 0 = g (z = 1 - f) rather than f; the slots in slot order
+            const double uu = (u[j] + u1[j]) * inv_scale;
+            const double vv = (v[j] + v1[j]) * inv_scale;
             // u = K z, v = K^T z' with z, z' = f or g = 1 - f per panel
             const bool gfu = gu[j] & 1, gfv = gv[j] & 1;
             const double Kf = gfu ? rkp[j] - uu : uu, Kg = gfu ? uu : rkp[j] - uu;
@@ -1027,16 +1075,17 @@ __device__ inline void block_sum4(double& a, double& b, double& c, double& d, do
 
 template <bool ROW, typename PT>
 __global__ void __launch_bounds__(kMemT)
-finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_pad, int64_t n,
+finish_member_kernel(const PT* __restrict__ P, int nslot, int nt, int64_t rows_pad, int64_t n,
                      int64_t kt, int B, const double* __restrict__ f,
                      const int32_t* __restrict__ perm, const double* __restrict__ rk,
                      const double* __restrict__ ck, const double* __restrict__ V,
                      const double* __restrict__ bath, const uint8_t* __restrict__ useg,
-                     const uint8_t* __restrict__ actp, int npanel, double inv_scale, int conserve,
+                     const uint8_t* __restrict__ pmask, int npanel, double inv_scale, int conserve,
                      const double* __restrict__ gate,
                      double* __restrict__ I, const double* __restrict__ eps_s,
                      double* __restrict__ flux, const double* __restrict__ rkc,
                      const double* __restrict__ ckc, const double* __restrict__ Vc) {
+#pragma clang fp contract(off)
     extern __shared__ double row[];
     __shared__ double red[4][kMemT / 64];
     __shared__ double s_b;
@@ -1049,7 +1098,7 @@ finish_member_kernel(const PT* __restrict__ P, int nsplit, int nt, int64_t rows_
     double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
     // f holds the hat form's occupation factors when rkc is given (the hat
     // form's defect weights need the caller-order K 1, K^T 1 and V)
-    finish_energy<ROW, PT>(P, nsplit, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, actp, npanel, sflag,
+    finish_energy<ROW, PT>(P, nslot, nt, rows_pad, n, kt, b, fb, perm, rk, ck, V, nb, useg, pmask, npanel, sflag,
                            inv_scale, eps_s, flux != nullptr, rkc != nullptr, row, Ib, dI, dw, eI, ew);
     double sc = 0.0;
     if (conserve || flux) {
@@ -1105,7 +1154,7 @@ __global__ void scatter_pair_kernel(int64_t n, const int32_t* __restrict__ perm,
 template <typename PT>
 struct MixedView {
     const PT* P;
-    int nsplit, nt, B, npanel;
+    int nslot, nt, B, npanel;
     int64_t rows_pad, n, kt, kgroups;
     const int32_t* perm;
     const double* rk;
@@ -1113,7 +1162,7 @@ struct MixedView {
     const double* V;
     const double* eps_s;
     const uint8_t* useg;
-    const uint8_t* actp;
+    const uint8_t* pmask;   // per-panel slot masks (null: every panel has nslot slots)
     const int32_t* slot;   // null: member b's column is b
     double inv_scale;
     PackOut o;
@@ -1162,9 +1211,9 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         double dI = 0.0, dw = 0.0, eI = 0.0, ew = 0.0;
         // the occupation factors of fin: the hat averages the previous pass
         // wrote, or fin itself
-        finish_energy<true, PT>(v.P, v.nsplit, v.nt, v.rows_pad, n, v.kt, col, (v.o.xb ? v.o.xb : a.fin) + base,
+        finish_energy<true, PT>(v.P, v.nslot, v.nt, v.rows_pad, n, v.kt, col, (v.o.xb ? v.o.xb : a.fin) + base,
                                 v.perm, v.rk, v.ck,
-                                v.V, nbk, v.useg, v.actp, v.npanel, sflag, v.inv_scale, v.eps_s,
+                                v.V, nbk, v.useg, v.pmask, v.npanel, sflag, v.inv_scale, v.eps_s,
                                 a.flux != nullptr, v.o.xb != nullptr, row, nullptr, dI, dw, eI, ew);
         block_sum4(dI, dw, eI, ew, red);
         if (threadIdx.x == 0) {
@@ -1294,15 +1343,16 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
 // A panel whose k range holds, for every member, only zero fp16 operand
 // units (f and 1 - f below the split's range, e.g. the empty top and the
 // filled bottom of a cold distribution) has C = A Z = 0 exactly: it is left
-// out of the GEMM and its products read as 0.  One workgroup decides which
-// panels have work (some iteration of its k range holds a nonzero operand
-// of a kind some member runs it with: a difference of prefix counts of the
-// iteration flags), flags them in actp for the epilogue, deals them in
-// decreasing band length to the bins in snake order (the k-th panel of pass
-// s to bin k on even passes, nbins - 1 - k on odd ones: every complete pass
-// gives each bin one panel, so a bin's slots and offsets have closed forms),
-// and clears the flags for the next evaluation.  Dynamic LDS:
-// 2 (nit + 1) + npanel + 1 ints.
+// out of the GEMM and its products read as 0; so is either chunk of a cut
+// band on its own.  One workgroup decides which work units have work (some
+// iteration of the unit's k range holds a nonzero operand of a kind some
+// member runs the panel with: a difference of prefix counts of the
+// iteration flags), writes each panel's slot mask (bit c: chunk c computed)
+// for the epilogue, deals the units in decreasing length to the bins in
+// snake order (the k-th unit of pass s to bin k on even passes,
+// nbins - 1 - k on odd ones: every complete pass gives each bin one unit, so
+// a bin's slots and offsets have closed forms), and clears the flags for the
+// next evaluation.  Dynamic LDS: 2 (nit + 1) + npanel + nunit ints.
 constexpr int kSchedT = 1024;
 
 // exclusive prefix of one int per thread over the workgroup; *total gets the
@@ -1348,11 +1398,11 @@ compact_kernel(const double* __restrict__ h, int B, int nt, int32_t* __restrict_
 }
 
 __global__ void __launch_bounds__(kSchedT)
-active_schedule_kernel(int npanel, int wg_per_xcd, const int32_t* __restrict__ ncol, int nw, int nmt, int nit,
-                       const int2* __restrict__ krange,
-                       const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
+active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __restrict__ ncol, int nw, int nmt,
+                       int nit, int fold, const int2* __restrict__ krange, const int32_t* __restrict__ cut,
+                       int2* __restrict__ urange, const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
                        uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF, uint8_t* __restrict__ useG,
-                       uint8_t* __restrict__ actp, int32_t* __restrict__ sched_off,
+                       uint8_t* __restrict__ pmask, int32_t* __restrict__ sched_off,
                        int32_t* __restrict__ sched_panel, int64_t* __restrict__ stat) {
     extern __shared__ int32_t sh[];
     // bins of the GEMM launch that reads this schedule (active_tiles)
@@ -1360,7 +1410,8 @@ active_schedule_kernel(int npanel, int wg_per_xcd, const int32_t* __restrict__ n
     const int nbins = 8 * (wg_per_xcd / (T > 0 ? T : 1));
     int32_t* cF = sh;                 // nit + 1: iterations before it holding a nonzero f operand
     int32_t* cG = cF + nit + 1;       // nit + 1: the same for g
-    int32_t* flag = cG + nit + 1;     // npanel: 1 if the panel at that position of `order` has work
+    int32_t* msk = cG + nit + 1;      // npanel: the panel's slot mask
+    int32_t* flag = msk + npanel;     // nunit: 1 if the unit at that position of `order` has work
     __shared__ int wtot[kSchedT / 64];
     __shared__ unsigned long long iters;
     const int tid = threadIdx.x;
@@ -1389,19 +1440,64 @@ active_schedule_kernel(int npanel, int wg_per_xcd, const int32_t* __restrict__ n
         }
     }
     __syncthreads();
-    for (int sidx = tid; sidx < npanel; sidx += kSchedT) {
-        const int pn = order[sidx];
+    for (int pn = tid; pn < npanel; pn += kSchedT) {
         const int2 kr = krange[pn];
+        const int k1 = kr.x + cut[pn];
         const bool uf = useF[pn] != 0, ug = useG[pn] != 0;
-        const bool a = (uf && cF[kr.y] > cF[kr.x]) || (ug && cG[kr.y] > cG[kr.x]);
-        flag[sidx] = a ? 1 : 0;
-        actp[pn] = a ? 1 : 0;
-        if (a) atomicAdd(&iters, (unsigned long long)(kr.y - kr.x));
+        // iterations [a, e) with a nonzero operand of a kind the panel uses:
+        // the first (last) one is where the prefix count first (last) steps
+        auto first_nz = [&](const int32_t* cnt, int a, int e) {
+            int lo = a, hi = e - 1;   // cnt[lo + 1] > cnt[a] holds at hi
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cnt[mid + 1] > cnt[a]) hi = mid;
+                else lo = mid + 1;
+            }
+            return lo;
+        };
+        auto last_nz = [&](const int32_t* cnt, int a, int e) {
+            int lo = a, hi = e - 1;   // cnt[hi] < cnt[e] holds at lo
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cnt[mid] < cnt[e]) lo = mid;
+                else hi = mid - 1;
+            }
+            return lo;
+        };
+        int m = 0;
+        unsigned long long it = 0;
+        auto unit = [&](int c, int a, int e) {
+            const bool wf = uf && a < e && cF[e] > cF[a], wg = ug && a < e && cG[e] > cG[a];
+            if (!wf && !wg) return;
+            int f0 = e, l0 = a;
+            if (wf) {
+                f0 = min(f0, first_nz(cF, a, e));
+                l0 = max(l0, last_nz(cF, a, e));
+            }
+            if (wg) {
+                f0 = min(f0, first_nz(cG, a, e));
+                l0 = max(l0, last_nz(cG, a, e));
+            }
+            const int s0 = a + (f0 - a) / fold * fold;
+            urange[2 * pn + c] = make_int2(s0 - kr.x, l0 + 1 - s0);
+            m |= 1 << c;
+            it += l0 + 1 - s0;
+        };
+        unit(0, kr.x, k1);
+        unit(1, k1, kr.y);
+        msk[pn] = m;
+        pmask[pn] = (uint8_t)m;
+        if (it) atomicAdd(&iters, it);
+    }
+    __syncthreads();
+    for (int sidx = tid; sidx < nunit; sidx += kSchedT) {
+        const int u = order[sidx];
+        flag[sidx] = (msk[u >> 1] >> (u & 1)) & 1;
     }
     __syncthreads();
     // ranks over `order`: thread t counts its contiguous chunk
-    const int chunk = (npanel + kSchedT - 1) / kSchedT;
-    const int s0 = min(tid * chunk, npanel), s1 = min(s0 + chunk, npanel);
+    const int chunk = (nunit + kSchedT - 1) / kSchedT;
+    const int s0 = min(tid * chunk, nunit), s1 = min(s0 + chunk, nunit);
     int cnt = 0;
     for (int sidx = s0; sidx < s1; ++sidx) cnt += flag[sidx];
     int A;
@@ -1472,44 +1568,70 @@ static bool config_known(bool persist, int nw, int waves, int fold, int depth, i
     return false;
 }
 
-// Longest-processing-time schedule of the panels over the bins of the
-// persistent kernel: panels by decreasing band length (ties by index), each
-// onto the bin with the least work so far (ties to the lowest bin); the cost
-// of a panel is its iteration count plus one for its write-out.  Built on
-// the host, copied once; evaluations only launch.
+// The persistent kernel's work units (a panel's band, or its two chunks
+// when it is longer than cut_min iterations: the first half, rounded down,
+// and the rest), by decreasing length (ties by unit index), with the slot
+// mask every panel has when all units are computed.
+static void unit_list(const MixedData* m, std::vector<int32_t>& units, std::vector<int64_t>& cost,
+                      std::vector<uint8_t>& mask) {
+    units.clear();
+    cost.assign(2 * (size_t)m->npanel, 0);
+    mask.assign(m->npanel, 0);
+    for (int pn = 0; pn < m->npanel; ++pn) {
+        const int N = m->h_krange[pn].y - m->h_krange[pn].x, c = m->h_cut[pn];
+        if (c > 0) {
+            units.push_back(2 * pn);
+            cost[2 * pn] = c + 1;
+            mask[pn] |= 1;
+        }
+        if (N - c > 0) {
+            units.push_back(2 * pn + 1);
+            cost[2 * pn + 1] = N - c + 1;
+            mask[pn] |= 2;
+        }
+    }
+    std::stable_sort(units.begin(), units.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+}
+
+// Longest-processing-time schedule of the units over the bins of the
+// persistent kernel: units by decreasing length, each onto the bin with the
+// least work so far (ties to the lowest bin); the cost of a unit is its
+// iteration count plus one for its write-out.  Built on the host, copied
+// once; evaluations only launch.
 static tg_status build_schedule(tg_ctx* c, MixedData* m) {
     if (m->sched_nw == m->nw) return TG_OK;
     const int nmt = m->nt / m->nw;
     const int per_xcd = std::max(1, m->ncu / 8);
     m->bins_per_xcd = std::max(1, per_xcd / nmt);
     const int nbins = 8 * m->bins_per_xcd;
-    std::vector<int> order(m->npanel);
-    for (int i = 0; i < m->npanel; ++i) order[i] = i;
-    auto cost = [&](int pn) { return (int64_t)(m->h_krange[pn].y - m->h_krange[pn].x) + 1; };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<int32_t> order;
+    std::vector<int64_t> cost;
+    std::vector<uint8_t> mask;
+    unit_list(m, order, cost, mask);
     std::vector<int64_t> load(nbins, 0);
     std::vector<std::vector<int32_t>> lists(nbins);
-    for (int pn : order) {
+    for (int u : order) {
         int best = 0;
         for (int b = 1; b < nbins; ++b)
             if (load[b] < load[best]) best = b;
-        load[best] += cost(pn);
-        lists[best].push_back(pn);
+        load[best] += cost[u];
+        lists[best].push_back(u);
     }
-    std::vector<int32_t> off(nbins + 1, 0), panels;
-    panels.reserve(m->npanel);
+    std::vector<int32_t> off(nbins + 1, 0), units;
+    units.reserve(order.size());
     for (int b = 0; b < nbins; ++b) {
-        panels.insert(panels.end(), lists[b].begin(), lists[b].end());
-        off[b + 1] = (int32_t)panels.size();
+        units.insert(units.end(), lists[b].begin(), lists[b].end());
+        off[b + 1] = (int32_t)units.size();
     }
     dev_free(&m->sched_off);
     dev_free(&m->sched_panel);
     TG_TRY(dev_alloc(c, &m->sched_off, (size_t)nbins + 1));
-    TG_TRY(dev_alloc(c, &m->sched_panel, (size_t)m->npanel));
+    TG_TRY(dev_alloc(c, &m->sched_panel, std::max<size_t>(units.size(), 1)));
     TG_HIP(c, hipMemcpyAsync(m->sched_off, off.data(), sizeof(int32_t) * (nbins + 1), hipMemcpyHostToDevice,
                              c->stream));
-    TG_HIP(c, hipMemcpyAsync(m->sched_panel, panels.data(), sizeof(int32_t) * m->npanel,
-                             hipMemcpyHostToDevice, c->stream));
+    if (!units.empty())
+        TG_HIP(c, hipMemcpyAsync(m->sched_panel, units.data(), sizeof(int32_t) * units.size(),
+                                 hipMemcpyHostToDevice, c->stream));
     TG_HIP(c, hipStreamSynchronize(c->stream));
     m->sched_nw = m->nw;
     return TG_OK;
@@ -1524,7 +1646,8 @@ static void launch_persist_t(tg_ctx* c, MixedData* m, bool dyn, const int32_t* n
                        (const u32x4*)m->F_hi, (const u32x4*)m->F_lo, (const u32x4*)m->G_hi,
                        (const u32x4*)m->G_lo, m->useg, m->rows_pad, m->nt, m->krange,
                        dyn ? m->dsched_off : m->sched_off, dyn ? m->dsched_panel : m->sched_panel, nmt,
-                       m->bins_per_xcd, ncol, (double*)m->P);
+                       m->bins_per_xcd, ncol, dyn ? m->durange : m->srange, (int64_t)m->nt * m->rows_pad,
+                       (double*)m->P);
 }
 
 // ncol (device, optional): member tiles from *ncol on are not computed
@@ -1594,10 +1717,11 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
     // the occupation factors of the rows' nodes: the hat averages the pack
     // wrote, or the nodal state
     const double* xf = m->xb ? m->xb : d_f;
+    const uint8_t* pmask = slot_masks(m, dyn);
 #define TG_FINISH(ROW, PT)                                                                           \
     hipLaunchKernelGGL((finish_member_kernel<ROW, PT>), dim3(m->nt), dim3(kMemT), ROW ? rowbytes : 0, \
-                       c->stream, (const PT*)m->P, m->nsplit, m->nt, m->rows_pad, n, m->kt, B, xf,     \
-                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, dyn ? m->actp : nullptr, \
+                       c->stream, (const PT*)m->P, slots(m), m->nt, m->rows_pad, n, m->kt, B, xf,    \
+                       m->perm, m->rk_s, m->ck_s, m->V_s, m->bath, m->useg, pmask,                  \
                        m->npanel, inv_scale, conserve,                                               \
                        gate, d_I, m->eps_s, flux, m->xb ? m->rk_c : nullptr, m->ck_c, c->d_V)
     if (m->fold) {
@@ -1614,7 +1738,7 @@ static tg_status launch_finish(tg_ctx* c, MixedData* m, const double* d_f, doubl
 
 static tg_status ensure_P(tg_ctx* c, MixedData* m) {
     const size_t psize = m->fold ? sizeof(double) : sizeof(float);
-    const size_t need = (size_t)m->nsplit * m->nt * m->rows_pad * psize;
+    const size_t need = (size_t)slots(m) * m->nt * m->rows_pad * psize;
     if (m->P_cap < need) {
         dev_free(&m->P);
         m->P_cap = 0;
@@ -1813,6 +1937,23 @@ tg_status mixed_prepare(tg_ctx* c) {
     TG_TRY(dev_alloc(c, &m->krange, (size_t)m->npanel));
     TG_HIP(c, hipMemcpyAsync(m->krange, m->h_krange.data(), sizeof(int2) * m->npanel,
                              hipMemcpyHostToDevice, st));
+    // the persistent kernel's chunks: with TG_MIXED_CUT = L > 0 bands
+    // longer than L iterations are cut in two, so the units spread over more
+    // bins when only some panels have work.  Off by default: on the 128 x
+    // 128-section ensemble (B = 256) cuts at 24 iterations gave 9% less GEMM
+    // time with a quarter of the band computed and 20% more with all of it
+    // (the second slot's writes and reads, A bands fetched into two L2s).
+    if (const char* e = std::getenv("TG_MIXED_CUT")) m->cut_min = std::max(0, std::atoi(e));
+    m->h_cut.resize(m->npanel);
+    m->pslots = 1;
+    for (int pn = 0; pn < m->npanel; ++pn) {
+        const int N = m->h_krange[pn].y - m->h_krange[pn].x;
+        const bool two = m->cut_min > 0 && N > m->cut_min;
+        m->h_cut[pn] = two ? N / 2 : N;
+        if (two) m->pslots = 2;
+    }
+    TG_TRY(dev_alloc(c, &m->cut, (size_t)m->npanel));
+    TG_HIP(c, hipMemcpyAsync(m->cut, m->h_cut.data(), sizeof(int32_t) * m->npanel, hipMemcpyHostToDevice, st));
     // band-major packed A from K's entries, then the split-matrix row sums
     // over it; sorted node volumes and energies
     {
@@ -1950,7 +2091,7 @@ tg_status mixed_prepare(tg_ctx* c) {
     }
     if (m->persist) TG_TRY(build_schedule(c, m));
     // panels with work: flags cleared here once, then by the schedule kernel
-    // after each use; the panels by decreasing band length for the dealing
+    // after each use; the units by decreasing length for the dealing
     if (const char* e = std::getenv("TG_MIXED_SKIP"))
         if (std::atoi(e) == 0) m->skip = false;
     {
@@ -1959,13 +2100,37 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_TRY(dev_alloc(c, &m->nzG, (size_t)m->nit));
         TG_TRY(dev_alloc(c, &m->useF, (size_t)m->npanel));
         TG_TRY(dev_alloc(c, &m->useG, (size_t)m->npanel));
+        std::vector<int32_t> order;
+        std::vector<int64_t> cost;
+        std::vector<uint8_t> mask;
+        unit_list(m, order, cost, mask);
+        m->nunit = (int)order.size();
         TG_TRY(dev_alloc(c, &m->actp, (size_t)m->npanel));
-        TG_HIP(c, hipMemsetAsync(m->actp, 1, (size_t)m->npanel, st));
-        TG_TRY(dev_alloc(c, &m->lpt_order, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->allmask, (size_t)m->npanel));
+        TG_HIP(c, hipMemcpyAsync(m->allmask, mask.data(), (size_t)m->npanel, hipMemcpyHostToDevice, st));
+        TG_HIP(c, hipMemcpyAsync(m->actp, mask.data(), (size_t)m->npanel, hipMemcpyHostToDevice, st));
+        TG_TRY(dev_alloc(c, &m->lpt_order, std::max<size_t>(order.size(), 1)));
+        {
+            std::vector<int2> sr(2 * (size_t)m->npanel, make_int2(0, 0));
+            for (int pn = 0; pn < m->npanel; ++pn) {
+                const int N = m->h_krange[pn].y - m->h_krange[pn].x, cc = m->h_cut[pn];
+                sr[2 * pn] = make_int2(0, cc);
+                sr[2 * pn + 1] = make_int2(cc, N - cc);
+            }
+            TG_TRY(dev_alloc(c, &m->srange, sr.size()));
+            TG_TRY(dev_alloc(c, &m->durange, sr.size()));
+            TG_HIP(c, hipMemcpyAsync(m->srange, sr.data(), sizeof(int2) * sr.size(), hipMemcpyHostToDevice, st));
+            TG_HIP(c, hipMemcpyAsync(m->durange, sr.data(), sizeof(int2) * sr.size(), hipMemcpyHostToDevice, st));
+        }
+        if (!order.empty())
+            TG_HIP(c, hipMemcpyAsync(m->lpt_order, order.data(), sizeof(int32_t) * order.size(),
+                                     hipMemcpyHostToDevice, st));
         // one entry per workgroup of any persistent launch (every workgroup
         // is a bin when one member tile runs) and one
         TG_TRY(dev_alloc(c, &m->dsched_off, (size_t)std::max(m->ncu, 8) + 8 * (size_t)m->nt + 1));
-        TG_TRY(dev_alloc(c, &m->dsched_panel, (size_t)m->npanel));
+        TG_TRY(dev_alloc(c, &m->dsched_panel, std::max<size_t>(order.size(), 1)));
+        // the schedule's LDS (64 KB): without it, every unit is computed
+        if (sched_lds(m) > 65536) m->skip = false;
         TG_TRY(dev_alloc(c, &m->dstat, 1));
         TG_HIP(c, hipMemsetAsync(m->dstat, 0xff, sizeof(int64_t), st));   // -1: no evaluation yet
         TG_TRY(dev_alloc(c, &m->dslot, (size_t)m->nt));
@@ -1974,12 +2139,7 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_HIP(c, hipMemsetAsync(m->nzG, 0, (size_t)m->nit, st));
         TG_HIP(c, hipMemsetAsync(m->useF, 0, (size_t)m->npanel, st));
         TG_HIP(c, hipMemsetAsync(m->useG, 0, (size_t)m->npanel, st));
-        std::vector<int32_t> order(m->npanel);
-        for (int i = 0; i < m->npanel; ++i) order[i] = i;
-        auto cost = [&](int pn) { return (int64_t)(m->h_krange[pn].y - m->h_krange[pn].x) + 1; };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
-        TG_HIP(c, hipMemcpyAsync(m->lpt_order, order.data(), sizeof(int32_t) * m->npanel,
-                                 hipMemcpyHostToDevice, st));
+        TG_HIP(c, hipStreamSynchronize(st));
     }
     if (!fixed) TG_TRY(autotune(c, m));
     TG_HIP(c, hipStreamSynchronize(st));
@@ -2001,6 +2161,10 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_tuned_ms = m->tuned_ms;
     out->mixed_skip = m->persist && m->skip ? 1 : 0;
     out->mixed_last_band = out->mixed_band;
+    out->mixed_cuts = 0;
+    if (m->persist)
+        for (int pn = 0; pn < m->npanel; ++pn)
+            out->mixed_cuts += m->h_cut[pn] < m->h_krange[pn].y - m->h_krange[pn].x ? 1 : 0;
     if (out->mixed_skip && m->dstat) {
         int64_t it = -1;
         if (hipMemcpyAsync(&it, m->dstat, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
@@ -2027,6 +2191,10 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->useG);
     dev_free(&m->lpt_order);
     dev_free(&m->actp);
+    dev_free(&m->allmask);
+    dev_free(&m->cut);
+    dev_free(&m->srange);
+    dev_free(&m->durange);
     dev_free(&m->dsched_off);
     dev_free(&m->dsched_panel);
     dev_free(&m->dstat);
@@ -2065,10 +2233,10 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 // ncol (device, optional): the columns running members fill (the GEMM that
 // reads the schedule gets the same)
 static tg_status launch_schedule(tg_ctx* c, MixedData* m, const int32_t* ncol = nullptr) {
-    hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT),
-                       sizeof(int32_t) * (2 * ((size_t)m->nit + 1) + m->npanel), c->stream, m->npanel,
-                       m->bins_per_xcd * (m->nt / m->nw), ncol, m->nw, m->nt / m->nw,
-                       m->nit, m->krange, m->lpt_order, m->nzF, m->nzG, m->useF, m->useG, m->actp,
+    hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT), sched_lds(m), c->stream, m->npanel,
+                       m->nunit, m->bins_per_xcd * (m->nt / m->nw), ncol, m->nw, m->nt / m->nw,
+                       m->nit, m->fold, m->krange, m->cut, m->durange, m->lpt_order, m->nzF, m->nzG, m->useF,
+                       m->useG, m->actp,
                        m->dsched_off, m->dsched_panel, m->dstat);
     TG_CHECK_LAUNCH(c);
     return TG_OK;
@@ -2078,7 +2246,7 @@ template <int MODE, typename PT>
 static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, bool dyn) {
     MixedView<PT> v;
     v.P = (const PT*)m->P;
-    v.nsplit = m->nsplit;
+    v.nslot = slots(m);
     v.nt = m->nt;
     v.B = B;
     v.npanel = m->npanel;
@@ -2092,7 +2260,7 @@ static void launch_pass_t(tg_ctx* c, MixedData* m, const Dp5PassArgs& a, int B, 
     v.V = m->V_s;
     v.eps_s = m->eps_s;
     v.useg = m->useg;
-    v.actp = dyn ? m->actp : nullptr;
+    v.pmask = slot_masks(m, dyn);
     v.slot = m->dslot;
     v.inv_scale = std::ldexp(1.0, -(m->sa + kSF));
     v.o = pack_out(m, dyn, c->d_hat, c->dim);

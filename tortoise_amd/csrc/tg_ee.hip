@@ -30,10 +30,11 @@
 //     order; lanes of a wave that hit the same bin are summed by a segmented
 //     scan before one atomic), with its occupied bin range, then converted to
 //     FP64 in place;
-//   - the convolution runs in FP64 over the occupied range, 8 grid outputs
+//   - the convolution runs in FP64 over the occupied range, 9 grid outputs
 //     per thread from a register window over the spectrum (zero guard bins
-//     on both sides, so no read is bounds-checked), and overwrites the
-//     spectrum in place once every thread holds its outputs;
+//     on both sides, so no read is bounds-checked; an odd count keeps the
+//     wave's LDS reads free of bank conflicts), and overwrites the spectrum
+//     in place once every thread holds its outputs;
 //   - each thread keeps the sums of its k1 values in registers across the
 //     16 transfers and writes them once;
 // one spectrum per workgroup (M doubles of LDS) and at most 128 registers
@@ -52,10 +53,12 @@ namespace tg {
 constexpr int kEeT = 512;       // threads of the main kernel
 constexpr int kEeAccMax = 32;   // k1 values per thread: P <= kEeT * kEeAccMax
 constexpr int kEeQTile = 16;    // momentum transfers per workgroup
-constexpr int kEeConvR = 8;     // grid outputs per thread in the convolution
+// grid outputs per thread in the convolution: odd, so the lanes of a wave,
+// kEeConvR doubles apart, read LDS without bank conflicts (with 8 they are
+// 64 bytes apart and 16 lanes share each bank pair)
+constexpr int kEeConvR = 9;
 constexpr int kEeMaxJ = 256;    // largest Gaussian half-width in grid cells
-constexpr int kEeConvCh = 2;    // convolution chunks per thread: M <= kEeT * kEeConvR * kEeConvCh
-constexpr int kEeMaxM = kEeT * kEeConvR * kEeConvCh;
+constexpr int kEeMaxM = 8192;    // largest energy grid (the convolution's chunks per thread cover it)
 
 struct EeData {
     int dim = 2;
@@ -175,12 +178,14 @@ __device__ inline void ee_deposit(int m, long long a0, long long a1, long long* 
 // step s of a block output r reads slot (r + s) % kEeConvR, the slot the
 // step then refills.  H is zero outside [lo, hi], including its guard bins
 // H[-J .. -1] and H[M .. M + J + kEeConvR - 1], so no read is bounds-checked.
+template <int NT>
 __device__ inline void ee_convolve_inplace(double* __restrict__ H, const double* __restrict__ gtab,
                                            int lo, int hi, int J) {
-    double out[kEeConvCh][kEeConvR];
+    constexpr int CH = (kEeMaxM + NT * kEeConvR - 1) / (NT * kEeConvR);   // chunks per thread
+    double out[CH][kEeConvR];
 #pragma unroll
-    for (int c = 0; c < kEeConvCh; ++c) {
-        const int m0 = lo + (threadIdx.x + c * kEeT) * kEeConvR;
+    for (int c = 0; c < CH; ++c) {
+        const int m0 = lo + (threadIdx.x + c * NT) * kEeConvR;
 #pragma unroll
         for (int r = 0; r < kEeConvR; ++r) out[c][r] = 0.0;
         if (m0 > hi) continue;
@@ -194,15 +199,15 @@ __device__ inline void ee_convolve_inplace(double* __restrict__ H, const double*
                 if (j < -J) break;
                 const double gj = gtab[j + J];
 #pragma unroll
-                for (int r = 0; r < kEeConvR; ++r) out[c][r] += gj * h[(r + s) % kEeConvR];
+                for (int r = 0; r < kEeConvR; ++r) out[c][r] = fma(gj, h[(r + s) % kEeConvR], out[c][r]);
                 h[s] = H[m0 + kEeConvR - j];
             }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kEeConvCh; ++c) {
-        const int m0 = lo + (threadIdx.x + c * kEeT) * kEeConvR;
+    for (int c = 0; c < CH; ++c) {
+        const int m0 = lo + (threadIdx.x + c * NT) * kEeConvR;
 #pragma unroll
         for (int r = 0; r < kEeConvR; ++r)
             if (m0 + r <= hi) H[m0 + r] = out[c][r];
@@ -296,7 +301,7 @@ ee_main_kernel(EeLattice L, int P, int Mh, int M, int J, double inv_dw, double w
         const int lo = s_lo, hi = s_hi + 1;
         for (int m = lo + tid; m <= hi; m += kEeT) Hd[m] = (double)H[m] * inv;
         __syncthreads();
-        ee_convolve_inplace(Hd, gtab, lo, hi, J);
+        ee_convolve_inplace<kEeT>(Hd, gtab, lo, hi, J);
         __syncthreads();
         // k1 = p1: the loss term of q (k3 = k1 - q) and the gain term of -q
         // (k3 = k1 + q), both read within [lo, hi] (x and z are computed by

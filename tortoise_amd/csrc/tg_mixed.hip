@@ -1174,8 +1174,11 @@ struct MixedView {
 
 // Positions per thread per sweep of a pass's caller-order loop; all loads of
 // a sweep are issued before the first result is stored (the stores could
-// alias them as far as the compiler knows).
-constexpr int kPassUnr = 3;
+// alias them as far as the compiler knows).  One: with the stage terms and
+// the pack's node indices live, more positions spill registers (35 VGPRs at
+// three, 13 at one), and the 256-member DP5 on 128 x 128 sections ran 7%
+// faster with one.
+constexpr int kPassUnr = 1;
 
 // One member per workgroup (member_of_block), member rows in LDS; the passes
 // are described with Dp5PassArgs.  The GEMM that follows a kDp5Stage or

@@ -17,6 +17,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tortoise_amd", "host", "examples", "thermalisation_square")
 
 
+def _host_form(w, **kw):
+    """A C-ABI context with the host layer's W-form: hat-averaged occupation
+    factors, whose DP5 runs with the number-defect removal (TG_CONSERVE)."""
+    ctx = tg.Context.create(w, max_batch=1, occupation="hat", **kw)
+    ctx.assemble()
+    return ctx
+
+
 def test_host_example_matches_abi():
     assert os.path.exists(EXE), "build with `make -C tortoise_amd/host`"
     print("[gpu] host example 16 sections t=10", flush=True)
@@ -31,9 +39,8 @@ def test_host_example_matches_abi():
     assert abs(num_end - t0_num) <= 1e-12 * t0_num
     # same integration through the C ABI
     w = wl.Workload("square16", 2, (16, 16), t_final=10.0, rtol=1e-8, atol=1e-12)
-    ctx = tg.Context.create(w, max_batch=1)
-    ctx.assemble()
-    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100)
+    ctx = _host_form(w)
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100, conserve=True)
     assert st["attempted"] == steps
     V = ctx.node_volumes()
     assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
@@ -60,16 +67,14 @@ def test_host_example_ee_channel_matches_abi():
     num_end = float(fin[1])
     w = wl.Workload("square16ee", 2, (16, 16), t_final=2.0, rtol=1e-8, atol=1e-12,
                     ee_g=1.0, ee_kappa2=0.09, ee_sigma=0.05)
-    ctx = tg.Context.create(w, max_batch=1)
-    ctx.assemble()
-    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    ctx = _host_form(w)
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100, conserve=True)
     assert st["attempted"] == steps
     V = ctx.node_volumes()
     assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
     # without e-e the same run differs
-    cp = tg.Context.create(w, max_batch=1, ee_g=0.0)
-    cp.assemble()
-    fp, _, _ = cp.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100)
+    cp = _host_form(w, ee_g=0.0)
+    fp, _, _ = cp.dp5(w.initial_f(), 0.0, 2.0, 0.02, 1e-8, 1e-12, max_dense=100, conserve=True)
     assert np.linalg.norm(fp - f) > 1e-6 * np.linalg.norm(f)
 
 
@@ -187,10 +192,10 @@ def test_host_example_pulsed_bath_matches_abi():
     dt = (10.0 / 100.0) / 64
     ns = int(np.ceil(2.0 * 10.0 / dt)) + 2
     samples = pulse(np.arange(ns) * dt)
-    ctx = tg.Context.create(w, max_batch=1)
-    ctx.assemble()
+    ctx = _host_form(w)
     ctx.set_bath_schedule(samples, 0.0, dt, atol=1e-3, rtol=1e-3)
-    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100, schedule=True)
+    f, dense, st = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100, schedule=True,
+                           conserve=True)
     assert st["attempted"] == steps
     V = ctx.node_volumes()
     assert abs(np.dot(V, dense[-1]) - num_end) <= 1e-12 * num_end
@@ -202,6 +207,6 @@ def test_host_example_pulsed_bath_matches_abi():
     assert abs(en - en_end) <= 1e-10 * abs(en)
     # the pulse did change the run, and the last dense state's phonon band is
     # the constraint at t = 10 (enforceValue at the dense time)
-    _, dc, _ = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100)
+    _, dc, _ = ctx.dp5(w.initial_f(), 0.0, 10.0, 0.1, 1e-8, 1e-12, max_dense=100, conserve=True)
     assert np.abs(dc[-1] - dense[-1]).max() > 1e-6
     assert bath_end == pulse(np.array([10.0]))[0]

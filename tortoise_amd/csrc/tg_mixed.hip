@@ -1172,13 +1172,6 @@ struct MixedView {
     const double* Vc;
 };
 
-// Positions per thread per sweep of a pass's caller-order loop; all loads of
-// a sweep are issued before the first result is stored (the stores could
-// alias them as far as the compiler knows).  One: with the stage terms and
-// the pack's node indices live, more positions spill registers (35 VGPRs at
-// three, 13 at one), and the 256-member DP5 on 128 x 128 sections ran 7%
-// faster with one.
-constexpr int kPassUnr = 1;
 
 // One member per workgroup (member_of_block), member rows in LDS; the passes
 // are described with Dp5PassArgs.  The GEMM that follows a kDp5Stage or
@@ -1230,8 +1223,6 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         const double ts = a.md[b].t + a.cs * hb;
         a.bath[b] = sched_value(a.bs.tab + (int64_t)b * a.bs.ns, a.bs.ns, a.bs.t_start, a.bs.dt, ts);
     }
-    PackNodes pk;
-    if (MODE != kDp5Last) load_pack_nodes(pk, n, v.kgroups, v.kt, v.npanel, v.perm);
     if (live) {
         const double sc = MODE != kDp5Stage ? s_b : 0.0;
         double c[7], cb[7], cf[6];
@@ -1245,77 +1236,88 @@ dp5_member_kernel(Dp5PassArgs a, MixedView<PT> v) {
         const double c7 = hb * a.c7;
         const int ng = a.ng;
         const bool want_y4 = MODE == kDp5Mid && a.y4 != nullptr;
-        for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kMemT * kPassUnr) {
-            double yv[kPassUnr], fv[kPassUnr], xv[kPassUnr], kv[6][kPassUnr], y4v[kPassUnr];
+        // fin for the number-defect weights |x(1-x)| (nodal factors; the hat
+        // factors come from o.xb) or for y5 in the last pass
+        const bool need_fin = MODE == kDp5Last || (MODE == kDp5Mid && a.conserve && !v.o.xb && a.nfin == 0);
+        // one position per thread per sweep, the next sweep's loads issued
+        // before this sweep's arithmetic and stores (a store never writes a
+        // position a later load reads: ys may overwrite fin only at the
+        // position just read)
+        struct Sweep {
+            double y, f, x, k[6], y4;
+        };
+        auto load = [&](Sweep& w, int64_t p) {
+            const int64_t i = base + p;
+            if (MODE != kDp5Last) {
+                w.y = a.y[i];
 #pragma unroll
-            for (int u = 0; u < kPassUnr; ++u) {
-                const int64_t i = base + min(i0 + (int64_t)u * kMemT, n - 1);
-                if (MODE != kDp5Last) {
-                    yv[u] = a.y[i];
-#pragma unroll
-                    for (int j = 0; j < 6; ++j)
-                        if (j < ng) kv[j][u] = a.kg[j][i];
-                }
-                // fin for the number-defect weights |x(1-x)| (nodal factors; the
-                // hat factors come from o.xb) or for y5 in the last pass
-                const bool need_fin = MODE == kDp5Last || (MODE == kDp5Mid && a.conserve && !v.o.xb && a.nfin == 0);
-                if (need_fin) fv[u] = a.fin[i];
-                if (MODE != kDp5Stage && a.conserve && v.o.xb) xv[u] = v.o.xb[i];
-                if (MODE == kDp5Last) y4v[u] = a.y4[i];
+                for (int j = 0; j < 6; ++j)
+                    if (j < ng) w.k[j] = a.kg[j][i];
             }
+            if (need_fin) w.f = a.fin[i];
+            if (MODE != kDp5Stage && a.conserve && v.o.xb) w.x = v.o.xb[i];
+            if (MODE == kDp5Last) w.y4 = a.y4[i];
+        };
+        auto step = [&](Sweep& w, int64_t p) {
+            const int64_t i = base + p;
+            double kn = 0.0;
+            if (MODE != kDp5Stage) {
+                kn = row[p];
+                if (a.conserve) {
+                    if (!v.o.xb) {
+                        if (MODE == kDp5Mid && a.nfin > 0) {
+                            // fin recomputed bit for bit: the previous pass's sum
+                            double x = w.y;
 #pragma unroll
-            for (int u = 0; u < kPassUnr; ++u) {
-                const int64_t p = i0 + (int64_t)u * kMemT;
-                if (p >= n) break;
-                const int64_t i = base + p;
-                double kn = 0.0;
-                if (MODE != kDp5Stage) {
-                    kn = row[p];
-                    if (a.conserve) {
-                        if (!v.o.xb) {
-                            if (MODE == kDp5Mid && a.nfin > 0) {
-                                // fin recomputed bit for bit: the previous pass's sum
-                                double x = yv[u];
-#pragma unroll
-                                for (int j = 0; j < 6; ++j)
-                                    if (j < a.nfin) x = x + cf[j] * kv[j][u];
-                                xv[u] = x;
-                            } else {
-                                xv[u] = fv[u];
-                            }
+                            for (int j = 0; j < 6; ++j)
+                                if (j < a.nfin) x = x + cf[j] * w.k[j];
+                            w.x = x;
+                        } else {
+                            w.x = w.f;
                         }
-                        const double x = xv[u];
-                        const double w = v.o.xb ? hat_defect_weight(x, nbk + 1.0, nbk, v.rkc[p], v.ckc[p], v.Vc[p])
-                                                : fabs(x * (1.0 - x));
-                        kn = kn - sc * w;
                     }
-                    a.kout[i] = kn;
+                    const double x = w.x;
+                    const double wt = v.o.xb ? hat_defect_weight(x, nbk + 1.0, nbk, v.rkc[p], v.ckc[p], v.Vc[p])
+                                             : fabs(x * (1.0 - x));
+                    kn = kn - sc * wt;
                 }
-                if (MODE == kDp5Last) {
-                    const double y5 = fv[u];
-                    const double err = y5 - (y4v[u] + c7 * kn);
-                    const double ay = fabs(y5), a1 = fabs(1.0 - y5);
-                    const double scale = a.rtol * (ay < a1 ? ay : a1) + a.atol;
-                    row[p] = err / scale;
-                } else {
-                    double s = yv[u];
+                a.kout[i] = kn;
+            }
+            if (MODE == kDp5Last) {
+                const double y5 = w.f;
+                const double err = y5 - (w.y4 + c7 * kn);
+                const double ay = fabs(y5), a1 = fabs(1.0 - y5);
+                const double scale = a.rtol * (ay < a1 ? ay : a1) + a.atol;
+                row[p] = err / scale;
+            } else {
+                double sum = w.y;
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    if (j < ng) sum = sum + c[j] * w.k[j];
+                if (MODE == kDp5Mid) sum = sum + c[ng] * kn;
+                a.ys[i] = sum;
+                row[p] = sum;
+                if (want_y4) {
+                    double y4 = w.y;
 #pragma unroll
                     for (int j = 0; j < 6; ++j)
-                        if (j < ng) s = s + c[j] * kv[j][u];
-                    if (MODE == kDp5Mid) s = s + c[ng] * kn;
-                    a.ys[i] = s;
-                    row[p] = s;
-                    if (want_y4) {
-                        double w = yv[u];
-#pragma unroll
-                        for (int j = 0; j < 6; ++j)
-                            if (j < ng) w = w + cb[j] * kv[j][u];
-                        a.y4[i] = w + cb[ng] * kn;
-                    }
+                        if (j < ng) y4 = y4 + cb[j] * w.k[j];
+                    a.y4[i] = y4 + cb[ng] * kn;
                 }
             }
+        };
+        Sweep cur, nxt;
+        int64_t p = threadIdx.x;
+        if (p < n) load(cur, p);
+        for (; p < n; p += kMemT) {
+            const bool more = p + kMemT < n;
+            if (more) load(nxt, p + kMemT);
+            step(cur, p);
+            if (more) cur = nxt;
         }
     }
+    PackNodes pk;
+    if (MODE != kDp5Last) load_pack_nodes(pk, n, v.kgroups, v.kt, v.npanel, v.perm);
     __syncthreads();
     if constexpr (MODE != kDp5Last) {
         pack_from_row(row, pk, live, col, n, v.kgroups, v.npanel, v.nt, v.o);

@@ -671,15 +671,17 @@ def main():
     evals_mixed = B * 1000.0 / ms_mixed
     evals_plain = 1000.0 / ms_plain
     # dominant kernel: the MFMA GEMM.  Algorithmic work of one launch = the
-    # two products K F and K^T F over the structural nonzeros of K for B
-    # members, 4 nnz B flops.  Dense-equivalent (4 n^2 B, zeros counted) and
-    # issued (three fp16 products over the streamed band) rates beside it.
-    # the launch leaves out the panels whose fp16 operands are zero for every
-    # member (their products are exactly 0): the work counted is that of the
-    # panels it computes, the nonzeros scaled by the executed fraction of the band
+    # two products K F and K^T F over the nonzeros of K for B members, two
+    # flops per nonzero of [K; K^T] and member: 4 nnz B with every panel
+    # computed.  The launch leaves out the iterations whose fp16 operands are
+    # zero for every member (their products are exactly 0); the work counted
+    # is exact: two flops per member for each nonzero of [K; K^T] in the
+    # iterations it computes (mixed_last_nnz).  Dense-equivalent
+    # (4 n^2 B, zeros counted) and issued (three fp16 products over the
+    # computed band) rates beside it.
     st_run = ctx.kernel_stats()
     exec_frac = st_run["mixed_last_band"] / st["mixed_band"] if st["mixed_band"] else 1.0
-    flops_alg = 4.0 * st["nnz"] * B * exec_frac
+    flops_alg = 2.0 * st_run["mixed_last_nnz"] * B
     tflops = flops_alg / (ms_gemm * 1e-3) / 1e12
     dense_tflops = 4.0 * n * n * B / (ms_gemm * 1e-3) / 1e12
     issued = 3 * 2.0 * st_run["mixed_last_band"] * st["mixed_width"] / (ms_gemm * 1e-3) / 1e12
@@ -718,7 +720,8 @@ def main():
                      "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": gemm_kernel_name(st), "kernel_ms": ms_gemm,
                      "flops_per_launch": flops_alg, "executed_band_fraction": exec_frac,
-                     "flops_per_launch_all_panels": 4.0 * st["nnz"] * B, "issued_tflops": issued,
+                     "executed_nonzeros": st_run["mixed_last_nnz"], "band_nonzeros": st["mixed_nnz"],
+                     "flops_per_launch_all_panels": 2.0 * st["mixed_nnz"] * B, "issued_tflops": issued,
                      "issued_frac": issued / FP16_MFMA_PEAK_TFLOPS,
                      "dense_equiv_tflops": dense_tflops,
                      "tile": {"persistent": st["mixed_persist"] > 0, "a_in_lds": st["mixed_persist"] == 2,
@@ -733,10 +736,11 @@ def main():
         "max_rel_err_vs_fp64": rel,
         "wall_s_timed": wall,
     }
-    out["value_note"] = ("the t = 0 ensemble; panels whose fp16 operands are zero for every member are "
-                         "left out of the GEMM (exactly: their products are 0); value_all_panels and "
-                         "value_t_final are the same evaluation with every panel computed and on the "
-                         "ensemble's state at t_final")
+    out["value_note"] = ("the t = 0 ensemble; panels and iterations whose fp16 operands are zero for every "
+                         "member are left out of the GEMM (exactly: their products are 0); value_all_panels "
+                         "and value_t_final are the same evaluation with every panel computed and on the "
+                         "ensemble's state at t_final; roofline.achieved counts two flops per member for "
+                         "each nonzero of [K; K^T] in the iterations computed")
     # the same batched evaluation with every panel computed (a context built
     # with TG_MIXED_SKIP=0): the state-independent rate
     if not args.no_all_panels:
@@ -756,7 +760,7 @@ def main():
         ctx_all.close()
         out["value_all_panels"] = B * 1000.0 / ms_all
         out["all_panels"] = {"ms_per_step": ms_all, "gemm_ms": ms_gemm_all,
-                             "gemm_tflops": 4.0 * st["nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
+                             "gemm_tflops": 2.0 * st["mixed_nnz"] * B / (ms_gemm_all * 1e-3) / 1e12}
         _log(f"all panels: {ms_all:.4f} ms/batch ({out['value_all_panels']:.0f} evals/s), "
              f"GEMM {ms_gemm_all:.4f} ms")
     # W-form hat moments of member 0 (FP64), with the measured (hat) and the

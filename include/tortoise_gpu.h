@@ -173,6 +173,10 @@ typedef struct tg_kernel_stats {
                                with work; mixed_band when nothing was left out) */
     int32_t mixed_cuts;     /* panels whose band the persistent kernel computes as two work units
                                (summed into two partial-sum slots the epilogue adds) */
+    int64_t mixed_nnz;      /* nonzero entries of [K; K^T] in the batched kernel's band (2 nnz less
+                               the tail entries beyond the extents of the fp16 split) */
+    int64_t mixed_last_nnz; /* of those, the entries in the iterations the last batched launch
+                               computed (mixed_nnz when nothing was left out) */
 } tg_kernel_stats;
 
 /* Sizes of the assembled kernel (for throughput and roofline accounting). */

@@ -51,6 +51,26 @@ def test_mixed_matches_fp64(sq32_mixed):
         assert _rel(Im[:, b], Io) <= 2e-6, (b, _rel(Im[:, b], Io))
 
 
+def test_mixed_band_nonzero_count(sq32_mixed):
+    """The nonzeros of [K; K^T] the library counts per band block for the
+    roofline: the band holds every nonzero of K but the tail entries beyond
+    the extents of the fp16 split (entries whose split is zero, a few in
+    ten thousand here), so the count is 2 nnz(K) (the oracle's count of K's
+    nonzeros, whose structural zeros the library's K shares) less those;
+    with no operand zero for every member every iteration is computed and the
+    launch's count is the total."""
+    w, ctx, ok, K = sq32_mixed
+    st = ctx.kernel_stats()
+    assert st["nnz"] == int(np.count_nonzero(K))
+    assert (1.0 - 1e-3) * 2 * st["nnz"] <= st["mixed_nnz"] <= 2 * st["nnz"], (st["mixed_nnz"], 2 * st["nnz"])
+    eps = w.eps()
+    H = np.stack([1.0 / (np.exp((eps - 3.0) / (1.0 + 0.1 * b)) + 1.0) for b in range(8)], axis=1)
+    ctx.apply(H, bath_n=np.full(8, 0.1), mixed=True)
+    st = ctx.kernel_stats()
+    if st["mixed_skip"]:
+        assert st["mixed_last_nnz"] == st["mixed_nnz"] and st["mixed_last_band"] == st["mixed_band"]
+
+
 def test_mixed_partial_batch_and_conservation(sq32_mixed):
     w, ctx, ok, K = sq32_mixed
     F, bath = _members(w, 3, seed=5)
@@ -365,6 +385,10 @@ def test_mixed_zero_operand_panels_left_out_exactly(monkeypatch):
     # the cold ensemble leaves most of the band out, the hot members none of it
     assert a[3]["mixed_last_band"] < 0.6 * a[3]["mixed_band"], a[3]
     assert a[4]["mixed_last_band"] == a[4]["mixed_band"], a[4]
+    # the nonzeros of [K; K^T] the launch computed: fewer on the cold
+    # ensemble, all of them on the hot members and with every unit computed
+    assert a[3]["mixed_last_nnz"] < 0.6 * a[3]["mixed_nnz"]
+    assert a[4]["mixed_last_nnz"] == a[4]["mixed_nnz"] and b[3]["mixed_last_nnz"] == b[3]["mixed_nnz"]
 
 
 def test_mixed_cut_bands(monkeypatch):

@@ -83,7 +83,8 @@ class KernelStats(C.Structure):
                 ("mixed_tile", C.c_int32), ("mixed_fold", C.c_int32),
                 ("mixed_waves", C.c_int32), ("mixed_depth", C.c_int32),
                 ("mixed_tuned_ms", C.c_float), ("mixed_persist", C.c_int32),
-                ("mixed_skip", C.c_int32), ("mixed_last_band", C.c_int64), ("mixed_cuts", C.c_int32)]
+                ("mixed_skip", C.c_int32), ("mixed_last_band", C.c_int64), ("mixed_cuts", C.c_int32),
+                ("mixed_nnz", C.c_int64), ("mixed_last_nnz", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -120,7 +120,10 @@ struct MixedData {
     int32_t* lpt_order = nullptr;    // units by decreasing length
     int32_t* dsched_off = nullptr;   // nbins + 1
     int32_t* dsched_panel = nullptr; // nunit
-    int64_t* dstat = nullptr;        // iterations the last schedule holds
+    int64_t* dstat = nullptr;        // the last schedule's iterations and nonzeros of [K; K^T]
+    int64_t* nnz_pref = nullptr;     // exclusive prefix of K's nonzeros over the band's blocks
+    int32_t* blk_first = nullptr;    // per panel: its first block of the band
+    int64_t band_nnz = 0;           // nonzeros of [K; K^T] in the band
     int32_t* dslot = nullptr;        // nt: a fused DP5 attempt's column of each member (-1: not running)
     int32_t* dncol = nullptr;        // 1: the columns running members take (leading ones)
     float tuned_ms = 0.0f;     // GEMM + epilogue time of the configuration the autotuner picked (0: not tuned)
@@ -157,7 +160,8 @@ constexpr int kBandT = 2 * kKG * kMT;
 __global__ void __launch_bounds__(kKG * kMT)
 band_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, int64_t kt, double scale,
                  const int2* __restrict__ krange, const int64_t* __restrict__ band_off,
-                 uint4* __restrict__ band) {
+                 uint4* __restrict__ band, int32_t* __restrict__ blk_nnz) {
+    __shared__ int wsum[kBandT / 2 / 64];
     const int panel = blockIdx.x;
     const int itl = blockIdx.y;
     const int2 kr = krange[panel];
@@ -166,6 +170,7 @@ band_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, int64_t kt
     const int64_t row = (int64_t)panel * kMT + r;
     const int64_t col0 = ((int64_t)(kr.x + itl) * kKG + kg) * 8;
     union { half8 v; uint4 u; } h, l;
+    int nz = 0;
     for (int q = 0; q < 8; ++q) {
         const int64_t col = col0 + q;
         double x = 0.0;
@@ -173,6 +178,7 @@ band_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, int64_t kt
             if (row < n) x = k_entry(k, perm[row], perm[col]);
             else if (row >= kt && row < kt + n) x = k_entry(k, perm[col], perm[row - kt]);
         }
+        nz += x != 0.0;
         _Float16 a, b;
         split_fp16(x, scale, a, b);
         h.v[q] = a;
@@ -181,6 +187,15 @@ band_pack_kernel(KSrc k, const int32_t* __restrict__ perm, int64_t n, int64_t kt
     uint4* dst = band + band_off[panel] + (int64_t)itl * kBandT;
     dst[(kg * 2 + 0) * kMT + r] = h.u;
     dst[(kg * 2 + 1) * kMT + r] = l.u;
+    // K's (structural) nonzeros in this block
+    for (int o = 32; o > 0; o >>= 1) nz += __shfl_down(nz, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = nz;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBandT / 2 / 64; ++w) t += wsum[w];
+        blk_nnz[band_off[panel] / kBandT + itl] = t;
+    }
 }
 
 // exact FP64 row sums of the split matrix over each row's band (ascending
@@ -1405,6 +1420,7 @@ compact_kernel(const double* __restrict__ h, int B, int nt, int32_t* __restrict_
 __global__ void __launch_bounds__(kSchedT)
 active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __restrict__ ncol, int nw, int nmt,
                        int nit, int fold, const int2* __restrict__ krange, const int32_t* __restrict__ cut,
+                       const int32_t* __restrict__ blk_first, const int64_t* __restrict__ nnz_pref,
                        int2* __restrict__ urange, const int32_t* __restrict__ order, uint8_t* __restrict__ nzF,
                        uint8_t* __restrict__ nzG, uint8_t* __restrict__ useF, uint8_t* __restrict__ useG,
                        uint8_t* __restrict__ pmask, int32_t* __restrict__ sched_off,
@@ -1418,9 +1434,12 @@ active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __r
     int32_t* msk = cG + nit + 1;      // npanel: the panel's slot mask
     int32_t* flag = msk + npanel;     // nunit: 1 if the unit at that position of `order` has work
     __shared__ int wtot[kSchedT / 64];
-    __shared__ unsigned long long iters;
+    __shared__ unsigned long long iters, nnzs;
     const int tid = threadIdx.x;
-    if (tid == 0) iters = 0;
+    if (tid == 0) {
+        iters = 0;
+        nnzs = 0;
+    }
     // prefix counts of the iteration flags: thread t counts its contiguous chunk
     {
         const int chunk = (nit + kSchedT - 1) / kSchedT;
@@ -1470,7 +1489,7 @@ active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __r
             return lo;
         };
         int m = 0;
-        unsigned long long it = 0;
+        unsigned long long it = 0, nz = 0;
         auto unit = [&](int c, int a, int e) {
             const bool wf = uf && a < e && cF[e] > cF[a], wg = ug && a < e && cG[e] > cG[a];
             if (!wf && !wg) return;
@@ -1487,12 +1506,17 @@ active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __r
             urange[2 * pn + c] = make_int2(s0 - kr.x, l0 + 1 - s0);
             m |= 1 << c;
             it += l0 + 1 - s0;
+            const int64_t g = blk_first[pn] + (s0 - kr.x);
+            nz += nnz_pref[g + l0 + 1 - s0] - nnz_pref[g];
         };
         unit(0, kr.x, k1);
         unit(1, k1, kr.y);
         msk[pn] = m;
         pmask[pn] = (uint8_t)m;
-        if (it) atomicAdd(&iters, it);
+        if (it) {
+            atomicAdd(&iters, it);
+            atomicAdd(&nnzs, nz);
+        }
     }
     __syncthreads();
     for (int sidx = tid; sidx < nunit; sidx += kSchedT) {
@@ -1521,7 +1545,10 @@ active_schedule_kernel(int npanel, int nunit, int wg_per_xcd, const int32_t* __r
         ++r;
     }
     for (int b = tid; b <= nbins; b += kSchedT) sched_off[b] = bin_off(b);
-    if (tid == 0) stat[0] = (int64_t)iters;
+    if (tid == 0) {
+        stat[0] = (int64_t)iters;
+        stat[1] = (int64_t)nnzs;
+    }
     // the flags were read above (before the first barrier): clear them
     for (int i = tid; i < nit; i += kSchedT) {
         nzF[i] = 0;
@@ -1975,10 +2002,13 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_TRY(dev_alloc(c, &m->band_off, (size_t)m->npanel));
         TG_HIP(c, hipMemcpyAsync(m->band_off, off.data(), sizeof(int64_t) * m->npanel, hipMemcpyHostToDevice, st));
         TG_TRY(dev_alloc(c, &m->A_band, (size_t)std::max<int64_t>(units, 1)));
+        const int64_t nblk = units / kBandT;
+        int32_t* d_cnt = nullptr;
+        TG_TRY(dev_alloc(c, &d_cnt, (size_t)std::max<int64_t>(nblk, 1)));
         if (maxit > 0) {
             hipLaunchKernelGGL(band_pack_kernel, dim3(m->npanel, maxit), dim3(kKG * kMT), 0, st,
                                ksrc_of(c, c->d_Q), c->d_perm, n, m->kt, scale, m->krange, m->band_off,
-                               m->A_band);
+                               m->A_band, d_cnt);
             TG_CHECK_LAUNCH(c);
         }
         TG_TRY(dev_alloc(c, &m->rk_s, (size_t)n));
@@ -1993,6 +2023,25 @@ tg_status mixed_prepare(tg_ctx* c) {
         hipLaunchKernelGGL(scatter_pair_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, m->perm, m->rk_s,
                            m->ck_s, m->rk_c, m->ck_c);
         TG_CHECK_LAUNCH(c);
+        // K's nonzeros per block (counted by the pack) and their prefix over
+        // the band: the schedule sums them over the iterations a launch computes
+        std::vector<int32_t> cnt(std::max<int64_t>(nblk, 1), 0);
+        {
+            hipError_t e = hipSuccess;
+            if (nblk > 0) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            dev_free(&d_cnt);
+            TG_HIP(c, e);
+        }
+        std::vector<int64_t> pref((size_t)nblk + 1, 0);
+        for (int64_t g = 0; g < nblk; ++g) pref[g + 1] = pref[g] + cnt[g];
+        m->band_nnz = pref[nblk];
+        std::vector<int32_t> first(m->npanel);
+        for (int pn = 0; pn < m->npanel; ++pn) first[pn] = (int32_t)(off[pn] / kBandT);
+        TG_TRY(dev_alloc(c, &m->nnz_pref, pref.size()));
+        TG_TRY(dev_alloc(c, &m->blk_first, (size_t)m->npanel));
+        TG_HIP(c, hipMemcpyAsync(m->nnz_pref, pref.data(), sizeof(int64_t) * pref.size(), hipMemcpyHostToDevice, st));
+        TG_HIP(c, hipMemcpyAsync(m->blk_first, first.data(), sizeof(int32_t) * m->npanel, hipMemcpyHostToDevice, st));
         TG_HIP(c, hipStreamSynchronize(st));
     }
     // configuration.  Default (kept by the autotuner unless beaten by >3%):
@@ -2136,8 +2185,8 @@ tg_status mixed_prepare(tg_ctx* c) {
         TG_TRY(dev_alloc(c, &m->dsched_panel, std::max<size_t>(order.size(), 1)));
         // the schedule's LDS (64 KB): without it, every unit is computed
         if (sched_lds(m) > 65536) m->skip = false;
-        TG_TRY(dev_alloc(c, &m->dstat, 1));
-        TG_HIP(c, hipMemsetAsync(m->dstat, 0xff, sizeof(int64_t), st));   // -1: no evaluation yet
+        TG_TRY(dev_alloc(c, &m->dstat, 2));
+        TG_HIP(c, hipMemsetAsync(m->dstat, 0xff, 2 * sizeof(int64_t), st));   // -1: no evaluation yet
         TG_TRY(dev_alloc(c, &m->dslot, (size_t)m->nt));
         TG_TRY(dev_alloc(c, &m->dncol, 1));
         TG_HIP(c, hipMemsetAsync(m->nzF, 0, (size_t)m->nit, st));
@@ -2166,15 +2215,19 @@ void mixed_stats(tg_ctx* c, tg_kernel_stats* out) {
     out->mixed_tuned_ms = m->tuned_ms;
     out->mixed_skip = m->persist && m->skip ? 1 : 0;
     out->mixed_last_band = out->mixed_band;
+    out->mixed_nnz = m->band_nnz;
+    out->mixed_last_nnz = m->band_nnz;
     out->mixed_cuts = 0;
     if (m->persist)
         for (int pn = 0; pn < m->npanel; ++pn)
             out->mixed_cuts += m->h_cut[pn] < m->h_krange[pn].y - m->h_krange[pn].x ? 1 : 0;
     if (out->mixed_skip && m->dstat) {
-        int64_t it = -1;
-        if (hipMemcpyAsync(&it, m->dstat, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
-            hipStreamSynchronize(c->stream) == hipSuccess && it >= 0)
-            out->mixed_last_band = it * (int64_t)kMT * kKS;
+        int64_t st[2] = {-1, -1};
+        if (hipMemcpyAsync(st, m->dstat, sizeof(st), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess && st[0] >= 0) {
+            out->mixed_last_band = st[0] * (int64_t)kMT * kKS;
+            out->mixed_last_nnz = st[1];
+        }
     }
 }
 
@@ -2203,6 +2256,8 @@ void mixed_destroy(tg_ctx* c) {
     dev_free(&m->dsched_off);
     dev_free(&m->dsched_panel);
     dev_free(&m->dstat);
+    dev_free(&m->nnz_pref);
+    dev_free(&m->blk_first);
     dev_free(&m->dslot);
     dev_free(&m->dncol);
     dev_free(&m->rk_s);
@@ -2240,7 +2295,8 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
 static tg_status launch_schedule(tg_ctx* c, MixedData* m, const int32_t* ncol = nullptr) {
     hipLaunchKernelGGL(active_schedule_kernel, dim3(1), dim3(kSchedT), sched_lds(m), c->stream, m->npanel,
                        m->nunit, m->bins_per_xcd * (m->nt / m->nw), ncol, m->nw, m->nt / m->nw,
-                       m->nit, m->fold, m->krange, m->cut, m->durange, m->lpt_order, m->nzF, m->nzG, m->useF,
+                       m->nit, m->fold, m->krange, m->cut, m->blk_first, m->nnz_pref, m->durange, m->lpt_order,
+                       m->nzF, m->nzG, m->useF,
                        m->useG, m->actp,
                        m->dsched_off, m->dsched_panel, m->dstat);
     TG_CHECK_LAUNCH(c);

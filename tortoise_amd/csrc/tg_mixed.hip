@@ -106,10 +106,11 @@ struct MixedData {
     int sched_nw = 0;          // tile width the schedule below was built for (0: none)
     int bins_per_xcd = 0;
     int32_t* sched_off = nullptr;    // per bin: first entry in sched_panel (nbins + 1)
-    int32_t* sched_panel = nullptr;  // panels in bin order
-    // zero-operand panels: which iterations hold a nonzero f (g) operand of a
-    // live member, which panels some member runs with f (g); from them the
-    // panels with work and their schedule, rebuilt on the device per evaluation
+    int32_t* sched_panel = nullptr;  // work units in bin order
+    // zero operands: which iterations hold a nonzero f (g) operand of a live
+    // member, which panels some member runs with f (g); from them the units
+    // with work, their iteration ranges and their schedule, rebuilt on the
+    // device per evaluation
     bool skip = true;
     int32_t nit = 0;                 // iterations of 32 k over the padded k range
     uint8_t* nzF = nullptr;          // nit
@@ -2289,7 +2290,8 @@ tg_status mixed_set_bath(tg_ctx* c, const double* h_bath, int32_t B) {
     return TG_OK;
 }
 
-// the panels with work and their schedule from the flags the pack set
+// the units with work, their iteration ranges and their schedule from the
+// flags the pack set
 // ncol (device, optional): the columns running members fill (the GEMM that
 // reads the schedule gets the same)
 static tg_status launch_schedule(tg_ctx* c, MixedData* m, const int32_t* ncol = nullptr) {

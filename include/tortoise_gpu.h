@@ -335,6 +335,21 @@ typedef struct tg_tuple_desc {
 } tg_tuple_desc;
 
 tg_status   tg_tuple_create(const tg_tuple_desc* desc, tg_tuple_ctx** out);
+/* The 4-leg channel {el in, el in, el out, el out} of the electron band:
+ * ScatteringChannel<NDim,4> (src/Physics/Scattering/ScatteringChannel.cpp:716-722,
+ * created by Material::addScatteringChannelsWildcards<4>,
+ * src/Physics/Material/Material.cpp:304-347) with fermion statistics on every
+ * leg.  desc as for tg_tuple_create (omega0 and eps_ph are not read;
+ * dynamic_phonons must be 0).  functAmplitude (ScatteringChannel.hpp:70, at
+ * the legs' barycentres): 1 for kappa <= 0, the reference default
+ * (ScatteringChannel.cpp:144); else the squared screened Coulomb potential
+ * 1/(|q|^2 + kappa^2)^2, q = barycentre of leg 2 - barycentre of leg 0.
+ * On such a context the point matrix has tg_tuple_point_rows(dim, 4) rows,
+ * tuples are [n][5] = e0, e1, e2, e3, uid (any order; the built list is in
+ * (e0, e1, e2, uid, e3) order), tg_tuple_propagate ignores bath_n, and
+ * tg_tuple_rates, tg_tuple_dp5 and tg_tuple_set_bath_schedule return
+ * TG_ERR_STATE. */
+tg_status   tg_tuple4_create(const tg_tuple_desc* desc, double kappa, tg_tuple_ctx** out);
 void        tg_tuple_destroy(tg_tuple_ctx* ctx);
 const char* tg_tuple_last_error(const tg_tuple_ctx* ctx);
 /* rows of the MC point matrix of a channel: max(dim*(nlegs-1)-1, 2*dim) */

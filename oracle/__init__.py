@@ -181,6 +181,15 @@ def lib():
                                       C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                       _f64p, _f64p, _f64p, C.c_void_p, C.c_int32]
         L.or_mc_nodal_to_dg.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+        L.or_mc_tuples_ee.restype = C.c_int64
+        L.or_mc_tuples_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32,
+                                      C.c_double, C.c_double, C.c_int32, C.c_void_p, C.c_int64]
+        L.or_mc_tuple_ee.restype = C.c_int
+        L.or_mc_tuple_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32,
+                                     C.c_double, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.or_mc_apply_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32,
+                                     C.c_double, C.c_double, C.c_double, _i32p, C.c_int64, _f64p,
+                                     C.c_int32, _f64p]
         L.or_mc_rates_eph.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_void_p, C.c_double,
                                       C.c_double, C.c_int32, _f64p, C.c_int32, _i32p, C.c_int64,
                                       _f64p, _f64p, _f64p, C.c_int32]
@@ -701,6 +710,57 @@ def mc_tuples_eph(dim, nsec, origin, extent, eps, omega0, nu, pts, min_det=1e-9,
                            threshold, idx.ctypes.data, n)
     assert m == n
     return idx[:n]
+
+
+def mc_tuples_ee(dim, nsec, origin, extent, eps, nu, pts, min_det=1e-9, threshold=1e-9,
+                 brute=False, sides=None, zone=None):
+    """Tuple list int32 [T][5] = (e0, e1, e2, e3, umklapp id) of the 4-leg
+    channel {el in, el in, el out, el out} (mc_oracle.c); brute: every leg-3
+    element tested, without the box and energy pruning."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
+    eps = np.ascontiguousarray(eps, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    L = lib()
+    n = L.or_mc_tuples_ee(dim, nsec, geo, eps, nu, pts, pts.shape[1], min_det, threshold,
+                          int(brute), None, 0)
+    idx = np.zeros((max(n, 1), 5), np.int32)
+    m = L.or_mc_tuples_ee(dim, nsec, geo, eps, nu, pts, pts.shape[1], min_det, threshold,
+                          int(brute), idx.ctypes.data, n)
+    assert m == n
+    return idx[:n]
+
+
+def mc_tuple_ee(dim, nsec, origin, extent, eps, nu, pts, tup, c, min_det=1e-9, sides=None,
+                zone=None):
+    """One 4-leg tuple's integral: (res [4][dim+1], D, N, ok) for the legs'
+    DG coefficient blocks c [4][dim+1] (no amplitude, no leg signs)."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
+    pts = np.ascontiguousarray(pts, np.float64)
+    res = np.zeros((4, dim + 1))
+    D = np.zeros((dim + 1, 4 * dim))
+    N = np.zeros(dim + 1)
+    rc = lib().or_mc_tuple_ee(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), nu, pts,
+                              pts.shape[1], min_det, np.ascontiguousarray(tup, np.int32),
+                              np.ascontiguousarray(c, np.float64), res, D, N)
+    return res, D, N, rc == 0
+
+
+def mc_apply_ee(dim, nsec, origin, extent, eps, nu, pts, idx, f, amplitude=1.0, kappa=0.0,
+                min_det=1e-9, sides=None, zone=None):
+    """DG derivative [batch][nel][dim+1] of the 4-leg channel over a tuple
+    list, after the inverse mass."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
+    pts = np.ascontiguousarray(pts, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    batch = f.shape[0] if f.ndim == 3 else 1
+    out = np.zeros_like(f)
+    idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 5)
+    lib().or_mc_apply_ee(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), nu, pts,
+                         pts.shape[1], min_det, amplitude, kappa, idx, idx.shape[0], f, batch, out)
+    return out
 
 
 def mc_inverse_mass(dim, vol):

@@ -918,3 +918,335 @@ void or_mc_rates_eph(int dim, const int32_t* nsec, const double* geo,
     free(acc);
     geo_free(&g);
 }
+
+/* ================================================ 4-leg channel (e-e) */
+/* The channel {el in, el in, el out, el out} on the electron band: the
+ * reference's ScatteringChannel<NDim,4> (ScatteringChannel.cpp:716-722) with
+ * legDirection {+1, +1, -1, -1} and fermion statistics on every leg.
+ * functAmplitude (ScatteringChannel.hpp:70, evaluated at the legs'
+ * barycentres by constructAmplitude, ScatteringChannel.cpp:399-415) is 1 for
+ * kappa <= 0 (the reference default, ScatteringChannel.cpp:144), else the
+ * squared screened Coulomb potential 1/(|q|^2 + kappa^2)^2 with q the
+ * barycentre of leg 2 minus that of leg 0. */
+static const double k_dir4[4] = {1.0, 1.0, -1.0, -1.0};
+
+/* D and N of a 4-leg tuple (ScatteringChannel.cpp:285-352). */
+static void tuple4_DN(const mc_geo* g, const int64_t* el, const int* u, double* D, double* N) {
+    const int dim = g->dim, ncol = 4 * dim;
+    for (int r = 0; r <= dim; ++r) N[r] = 0.0;
+    for (int leg = 0; leg < 4; ++leg) {
+        double kf[3][4], ef[4];
+        elem_forms(g, el[leg], 0, kf, ef);
+        for (int d = 0; d < dim; ++d) {
+            for (int a = 0; a < dim; ++a) D[d * ncol + leg * dim + a] = k_dir4[leg] * kf[d][1 + a];
+            N[d] -= k_dir4[leg] * kf[d][0];
+        }
+        for (int a = 0; a < dim; ++a) D[dim * ncol + leg * dim + a] = k_dir4[leg] * ef[1 + a];
+        N[dim] -= k_dir4[leg] * ef[0];
+    }
+    for (int d = 0; d < dim; ++d)
+        for (int c = 0; c < dim; ++c) N[d] -= (double)u[c] * g->zone[d][c];
+}
+
+/* D_C^-1 of a 4-leg tuple; returns det D_C (IntegratorCore.hpp:153-155) */
+static double tuple4_dc(int dim, const double* D, double* inv) {
+    const int nc = dim + 1;
+    double DC[16];
+    for (int r = 0; r < nc; ++r)
+        for (int q = 0; q < nc; ++q) DC[r * nc + q] = D[r * 4 * dim + 3 * dim - 1 + q];
+    return or_mc_invert(nc, DC, inv);
+}
+
+/* Screening value of a 4-leg tuple: tuple_screen with c^4 and four volumes;
+ * -1 when |det D_C| < min_det. */
+static double tuple4_screen(const mc_geo* g, const double* D, const double* N, const double* pts,
+                            int32_t nmc, double min_det) {
+    const int dim = g->dim;
+    double inv[16];
+    double det = tuple4_dc(dim, D, inv);
+    if (!(fabs(det) >= min_det)) return -1.0;
+    int64_t hits = 0;
+    for (int32_t m = 0; m < nmc; ++m) {
+        double K[4][3];
+        hits += shell_point(dim, 4, D, N, inv, pts, nmc, m, K);
+    }
+    double c = dim == 1 ? 1.0 : sqrt(2.0);
+    double cpow = c * c * c * c;
+    double amp = 1.0;
+    for (int leg = 0; leg < 4; ++leg) amp *= g->vol * (double)mc_fact(dim);
+    return (double)hits * cpow * amp / shell_divisor(dim, 4, det, nmc);
+}
+
+/* Sections that can hold a point of the box [lo, hi]: the box's own section
+ * range on an axis-aligned mesh, else the reduced coordinates of its corners
+ * one section wider.  Returns 0 when none. */
+static int box_sections(const mc_geo* g, const double* lo, const double* hi, int* slo, int* shi) {
+    const int dim = g->dim;
+    double smin[3], smax[3];
+    for (int d = 0; d < dim; ++d) {
+        if (g->axis) {
+            double h = g->extent[d] / g->nsec[d];
+            smin[d] = floor((lo[d] - g->origin[d]) / h);
+            smax[d] = floor((hi[d] - g->origin[d]) / h);
+        } else {
+            smin[d] = 1e300;
+            smax[d] = -1e300;
+        }
+    }
+    if (!g->axis) {
+        for (int cm = 0; cm < (1 << dim); ++cm) {
+            double x[3];
+            for (int d = 0; d < dim; ++d) x[d] = ((cm >> d) & 1 ? hi[d] : lo[d]) - g->origin[d];
+            for (int c = 0; c < dim; ++c) {
+                double sr = 0.0;
+                for (int d = 0; d < dim; ++d) sr = sr + g->dGinv[c][d] * x[d];
+                if (sr < smin[c]) smin[c] = sr;
+                if (sr > smax[c]) smax[c] = sr;
+            }
+        }
+        for (int c = 0; c < dim; ++c) {
+            smin[c] = floor(smin[c]) - 1.0;
+            smax[c] = floor(smax[c]) + 1.0;
+        }
+    }
+    for (int d = 0; d < dim; ++d) {
+        double a = smin[d], b = smax[d];
+        if (b < 0.0 || a > g->nsec[d] - 1) return 0;
+        slo[d] = a < 0.0 ? 0 : (int)a;
+        shi[d] = b > g->nsec[d] - 1 ? g->nsec[d] - 1 : (int)b;
+    }
+    return 1;
+}
+
+/* Tuples of the 4-leg channel, ordered by (e0, e1, e2, umklapp id, e3):
+ * every leg-3 element whose box meets k0 + k1 - k2 + G u and whose energy
+ * window admits energy conservation (ScatteringChannel.hpp:212-222), kept when
+ * |det D_C| >= min_det and the screening value exceeds `threshold`
+ * (ScatteringChannel.cpp:474-503).  brute != 0 tests every leg-3 element
+ * without the box and energy pruning (the same list: both prunings keep every
+ * tuple whose delta shell meets the simplices).  idx[t*5 + {0..4}] = e0, e1,
+ * e2, e3, uid.  Returns the tuple count (idx may be NULL; at most cap written). */
+int64_t or_mc_tuples_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                        const double* pts, int32_t nmc, double min_det, double threshold, int32_t brute,
+                        int32_t* idx, int64_t cap) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, geo, eps, NULL, 0.0);
+    const int64_t nel = g.nel;
+    const int nes = mc_fact(dim);
+    int32_t nuv = 1;
+    for (int d = 0; d < dim; ++d) nuv *= 2 * nu + 1;
+    double* box = (double*)malloc(sizeof(double) * nel * 2 * dim);
+    double* er = (double*)malloc(sizeof(double) * nel * 2);
+    double emin = 0.0, emax = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        elem_box(&g, e, box + e * 2 * dim, box + e * 2 * dim + dim);
+        elem_erange(&g, eps, e, er + 2 * e, er + 2 * e + 1);
+        if (e == 0 || er[2 * e] < emin) emin = er[2 * e];
+        if (e == 0 || er[2 * e + 1] > emax) emax = er[2 * e + 1];
+    }
+    const int64_t npair = nel * nel;
+    int64_t* counts = (int64_t*)calloc((size_t)npair + 1, sizeof(int64_t));
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            if (!idx) break;
+            for (int64_t p = 0; p < npair; ++p) counts[p + 1] += counts[p];
+        }
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t p = 0; p < npair; ++p) {
+            const int64_t e0 = p / nel, e1 = p % nel;
+            int64_t w = pass == 1 ? counts[p] : 0, cnt = 0;
+            const double* b0 = box + e0 * 2 * dim;
+            const double* b1 = box + e1 * 2 * dim;
+            for (int64_t e2 = 0; e2 < nel; ++e2) {
+                /* energy window with leg 3 over the band's whole range */
+                if (!brute && !(er[2 * e0] + er[2 * e1] - er[2 * e2 + 1] - emax <= 0.0 &&
+                                er[2 * e0 + 1] + er[2 * e1 + 1] - er[2 * e2] - emin >= 0.0))
+                    continue;
+                const double* b2 = box + e2 * 2 * dim;
+                for (int32_t uid = 0; uid < nuv; ++uid) {
+                    int u[3] = {0, 0, 0};
+                    umklapp_vec(dim, nu, uid, u);
+                    int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
+                    if (brute) {
+                        for (int d = 0; d < dim; ++d) shi[d] = nsec[d] - 1;
+                    } else {
+                        /* k3 = k0 + k1 - k2 + G u */
+                        double lo[3], hi[3];
+                        for (int d = 0; d < dim; ++d) {
+                            double U = 0.0;
+                            for (int c = 0; c < dim; ++c) U = U + (double)u[c] * g.zone[d][c];
+                            lo[d] = b0[d] + b1[d] - b2[dim + d] + U;
+                            hi[d] = b0[dim + d] + b1[dim + d] - b2[d] + U;
+                        }
+                        if (!box_sections(&g, lo, hi, slo, shi)) continue;
+                        int sc[3] = {slo[0], dim > 1 ? slo[1] : 0, dim > 2 ? slo[2] : 0};
+                        for (;;) {
+                            int64_t sid = 0, stride = 1;
+                            for (int d = 0; d < dim; ++d) { sid += stride * sc[d]; stride *= nsec[d]; }
+                            for (int t = 0; t < nes; ++t) {
+                                int64_t e3 = sid * nes + t;
+                                const double* b3 = box + e3 * 2 * dim;
+                                int meet = 1;
+                                for (int d = 0; d < dim; ++d) meet &= b3[d] <= hi[d] && b3[dim + d] >= lo[d];
+                                if (!meet) continue;
+                                if (!(er[2 * e0] + er[2 * e1] - er[2 * e2 + 1] - er[2 * e3 + 1] <= 0.0 &&
+                                      er[2 * e0 + 1] + er[2 * e1 + 1] - er[2 * e2] - er[2 * e3] >= 0.0))
+                                    continue;
+                                int64_t el[4] = {e0, e1, e2, e3};
+                                double D[48], N[4];
+                                tuple4_DN(&g, el, u, D, N);
+                                if (!(tuple4_screen(&g, D, N, pts, nmc, min_det) > threshold)) continue;
+                                if (pass == 1 && w < cap) {
+                                    int32_t* o = idx + w * 5;
+                                    o[0] = (int32_t)e0; o[1] = (int32_t)e1; o[2] = (int32_t)e2;
+                                    o[3] = (int32_t)e3; o[4] = uid;
+                                }
+                                ++w;
+                                ++cnt;
+                            }
+                            int d = 0;
+                            while (d < dim && ++sc[d] > shi[d]) { sc[d] = slo[d]; ++d; }
+                            if (d == dim) break;
+                        }
+                        continue;
+                    }
+                    for (int64_t e3 = 0; e3 < nel; ++e3) {
+                        int64_t el[4] = {e0, e1, e2, e3};
+                        double D[48], N[4];
+                        tuple4_DN(&g, el, u, D, N);
+                        if (!(tuple4_screen(&g, D, N, pts, nmc, min_det) > threshold)) continue;
+                        if (pass == 1 && w < cap) {
+                            int32_t* o = idx + w * 5;
+                            o[0] = (int32_t)e0; o[1] = (int32_t)e1; o[2] = (int32_t)e2;
+                            o[3] = (int32_t)e3; o[4] = uid;
+                        }
+                        ++w;
+                        ++cnt;
+                    }
+                }
+            }
+            if (pass == 0) counts[p + 1] = cnt;
+        }
+    }
+    int64_t total = 0;
+    if (idx) total = counts[npair];
+    else for (int64_t p = 0; p < npair; ++p) total += counts[p + 1];
+    free(counts); free(box); free(er);
+    geo_free(&g);
+    return total;
+}
+
+/* constructAmplitude (ScatteringChannel.cpp:399-415) of a 4-leg tuple:
+ * amplitude * prod(vol * dim!) * functAmplitude(barycentres). */
+static double tuple4_amplitude(const mc_geo* g, const int64_t* el, double amplitude, double kappa) {
+    const int dim = g->dim;
+    double a = amplitude;
+    for (int leg = 0; leg < 4; ++leg) a *= g->vol * (double)mc_fact(dim);
+    if (kappa > 0.0) {
+        double q2 = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            double c0 = 0.0, c2 = 0.0;
+            for (int v = 0; v <= dim; ++v) {
+                c0 += g->nodes[(int64_t)g->conn[el[0] * (dim + 1) + v] * dim + d];
+                c2 += g->nodes[(int64_t)g->conn[el[2] * (dim + 1) + v] * dim + d];
+            }
+            const double q = (c2 - c0) / (double)(dim + 1);
+            q2 += q * q;
+        }
+        const double v = 1.0 / (q2 + kappa * kappa);
+        a *= v * v;
+    }
+    return a;
+}
+
+/* The forms of a 4-leg tuple for DG coefficient blocks c[leg] (the electron
+ * band at each leg's element), ScatteringChannel.cpp:628-640 with statistics
+ * -1: direct -f0 f1 (1 - f2)(1 - f3), time reversed (1 - f0)(1 - f1) f2 f3. */
+static void tuple4_forms(int dim, const double* const* c, double* fa, double* fb) {
+    const int nc = dim + 1;
+    for (int leg = 0; leg < 4; ++leg)
+        for (int a = 0; a < nc; ++a) {
+            const double one = a == 0 ? 1.0 : 0.0, x = c[leg][a];
+            if (k_dir4[leg] > 0.0) {
+                fa[leg * nc + a] = leg == 0 ? -x : x;
+                fb[leg * nc + a] = one - x;
+            } else {
+                fa[leg * nc + a] = one - x;
+                fb[leg * nc + a] = x;
+            }
+        }
+}
+
+/* The integral of one 4-leg tuple (singleScatteringPropagator without the
+ * amplitude and the leg signs, ScatteringChannel.cpp:618-650): res[4*(dim+1)]
+ * for the states of the four legs c (dim+1 coefficients each); D and N
+ * returned when not NULL.  Returns -1 when |det D_C| < min_det (res zero). */
+int or_mc_tuple_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                   const double* pts, int32_t nmc, double min_det, const int32_t* tup,
+                   const double* c, double* res, double* D_out, double* N_out) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, geo, eps, NULL, 0.0);
+    const int nc = dim + 1;
+    int64_t el[4] = {tup[0], tup[1], tup[2], tup[3]};
+    int u[3] = {0, 0, 0};
+    umklapp_vec(dim, nu, tup[4], u);
+    double D[48], N[4], inv[16];
+    tuple4_DN(&g, el, u, D, N);
+    if (D_out) memcpy(D_out, D, sizeof(double) * (dim + 1) * 4 * dim);
+    if (N_out) memcpy(N_out, N, sizeof(double) * (dim + 1));
+    for (int i = 0; i < 4 * nc; ++i) res[i] = 0.0;
+    int rc = -1;
+    if (fabs(tuple4_dc(dim, D, inv)) >= min_det) {
+        const double* cl[4] = {c, c + nc, c + 2 * nc, c + 3 * nc};
+        double fa[16], fb[16];
+        tuple4_forms(dim, cl, fa, fb);
+        rc = or_mc_integrate(dim, 4, D, N, fa, fb, pts, nmc, res);
+    }
+    geo_free(&g);
+    return rc;
+}
+
+/* ScatteringChannel<NDim,4>::propagate (ScatteringChannel.cpp:651-670) over a
+ * tuple list [ntup][5] for `batch` members f [batch][nel][dim+1], then
+ * applyInverseMass (Function.cpp:801-804): out [batch][nel][dim+1].  Tuples
+ * with |det D_C| < min_det are skipped (:621-622).  Per element the
+ * contributions are summed in tuple order, leg by leg. */
+void or_mc_apply_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                    const double* pts, int32_t nmc, double min_det, double amplitude, double kappa,
+                    const int32_t* idx, int64_t ntup, const double* f, int32_t batch, double* out) {
+    mc_geo g;
+    geo_init(&g, dim, nsec, geo, eps, NULL, 0.0);
+    const int nc = dim + 1;
+    const int64_t nel = g.nel, len = nel * nc;
+    double minv[16];
+    or_mc_inverse_mass(dim, g.vol, minv);
+#pragma omp parallel for schedule(static)
+    for (int32_t b = 0; b < batch; ++b) {
+        const double* fb_ = f + (size_t)b * len;
+        double* acc = (double*)calloc((size_t)len, sizeof(double));
+        for (int64_t t = 0; t < ntup; ++t) {
+            const int32_t* tp = idx + t * 5;
+            int64_t el[4] = {tp[0], tp[1], tp[2], tp[3]};
+            int u[3] = {0, 0, 0};
+            umklapp_vec(dim, nu, tp[4], u);
+            double D[48], N[4], inv[16];
+            tuple4_DN(&g, el, u, D, N);
+            if (!(fabs(tuple4_dc(dim, D, inv)) >= min_det)) continue;
+            const double* cl[4] = {fb_ + el[0] * nc, fb_ + el[1] * nc, fb_ + el[2] * nc, fb_ + el[3] * nc};
+            double fa[16], fbv[16], res[16];
+            tuple4_forms(dim, cl, fa, fbv);
+            or_mc_integrate(dim, 4, D, N, fa, fbv, pts, nmc, res);
+            const double amp = tuple4_amplitude(&g, el, amplitude, kappa);
+            for (int leg = 0; leg < 4; ++leg)
+                for (int a = 0; a < nc; ++a) acc[el[leg] * nc + a] += k_dir4[leg] * (amp * res[leg * nc + a]);
+        }
+        for (int64_t e = 0; e < nel; ++e)
+            for (int q = 0; q < nc; ++q) {
+                double v = 0.0;
+                for (int a = 0; a < nc; ++a) v += acc[e * nc + a] * minv[a * nc + q];
+                out[(size_t)b * len + e * nc + q] = v;
+            }
+        free(acc);
+    }
+    geo_free(&g);
+}

@@ -385,6 +385,19 @@ void    or_mc_rates_eph(int dim, const int32_t* nsec, const double* geo,
                         int32_t nu, const double* pts, int32_t nmc, const int32_t* idx,
                         int64_t ntup, const double* f_el, const double* f_ph, double* out_el,
                         int32_t nthreads);
+/* Tuples (e0, e1, e2, e3, umklapp id) of the 4-leg channel {el in, el in,
+ * el out, el out}; brute: no box or energy pruning of leg 3. */
+int64_t or_mc_tuples_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                        const double* pts, int32_t nmc, double min_det, double threshold,
+                        int32_t brute, int32_t* idx, int64_t cap);
+/* One 4-leg tuple's integral res[4*(dim+1)] (no amplitude, no leg signs). */
+int     or_mc_tuple_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                       const double* pts, int32_t nmc, double min_det, const int32_t* tup,
+                       const double* c, double* res, double* D_out, double* N_out);
+/* Propagator of the 4-leg channel over a tuple list, then the inverse mass. */
+void    or_mc_apply_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                       const double* pts, int32_t nmc, double min_det, double amplitude, double kappa,
+                       const int32_t* idx, int64_t ntup, const double* f, int32_t batch, double* out);
 /* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
 void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
 

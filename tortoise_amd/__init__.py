@@ -54,7 +54,7 @@ EXPORTS = [
     "tg_tuple_save_indices", "tg_tuple_load_indices", "tg_tuple_set_amplitude",
     "tg_tuple_propagate", "tg_tuple_last_kernel_ms", "tg_tuple_stream", "tg_tuple_dp5",
     "tg_tuple_rates", "tg_set_bath_schedule", "tg_tuple_set_bath_schedule",
-    "tg_collision_apply_noalloc",
+    "tg_collision_apply_noalloc", "tg_tuple4_create",
 ]
 
 
@@ -192,6 +192,8 @@ def load():
     L.tg_relative_error.argtypes = [vp, vp, vp, i64, i32, f64, f64, vp, i32]
     L.tg_tuple_create.restype = i32
     L.tg_tuple_create.argtypes = [C.POINTER(TupleDesc), C.POINTER(vp)]
+    L.tg_tuple4_create.restype = i32
+    L.tg_tuple4_create.argtypes = [C.POINTER(TupleDesc), f64, C.POINTER(vp)]
     L.tg_tuple_destroy.restype = None
     L.tg_tuple_destroy.argtypes = [vp]
     L.tg_tuple_last_error.restype = C.c_char_p
@@ -503,15 +505,22 @@ class TupleOperator:
         self.nel = nel
         self.max_batch = max_batch
         self.dynamic = False
+        self.nlegs = 3
 
     @classmethod
     def create(cls, workload, n_umklapp=1, num_mc=50, min_det=1e-9, threshold=1e-9,
                max_batch=1, amplitude=None, eps=None, eps_ph=None, dynamic=False, origin=None,
-               sides=None, zone=None):
+               sides=None, zone=None, legs=3, kappa=0.0):
         """sides (dim x dim, row c = side c) with origin: a general
         parallelepiped mesh (the reference's Mesh over a Region); zone (row c
         = zone side c): the umklapp basis, by default the mesh's sides.
-        Without sides: the workload's axis-aligned box."""
+        Without sides: the workload's axis-aligned box.  legs=4: the channel
+        {el in, el in, el out, el out} of the electron band
+        (ScatteringChannel<NDim,4>, tg_tuple4_create) with functAmplitude 1
+        (kappa <= 0) or 1/(|q|^2 + kappa^2)^2; tuples are [n][5] and
+        propagate takes no bath."""
+        if legs not in (3, 4):
+            raise ValueError("legs must be 3 or 4")
         L = load()
         w = workload
         e = np.ascontiguousarray(w.eps() if eps is None else eps, np.float64)
@@ -543,7 +552,10 @@ class TupleOperator:
             ph = np.ascontiguousarray(eps_ph, np.float64)
             d.eps_ph = ph.ctypes.data_as(C.POINTER(C.c_double))
         h = C.c_void_p()
-        rc = L.tg_tuple_create(C.byref(d), C.byref(h))
+        if legs == 4:
+            rc = L.tg_tuple4_create(C.byref(d), float(kappa), C.byref(h))
+        else:
+            rc = L.tg_tuple_create(C.byref(d), C.byref(h))
         if rc != TG_OK:
             msg = L.tg_tuple_last_error(h).decode() if h.value else "tg_tuple_create failed"
             if h.value:
@@ -551,7 +563,8 @@ class TupleOperator:
             raise TgError(rc, msg)
         nel = int(np.prod(w.nsec[:w.dim])) * (2 if w.dim == 2 else 6)
         op = cls(h, w.dim, nel, max_batch)
-        op.rows = L.tg_tuple_point_rows(w.dim, 3)
+        op.rows = L.tg_tuple_point_rows(w.dim, legs)
+        op.nlegs = legs
         op.num_mc = num_mc
         op.dynamic = bool(dynamic)
         return op
@@ -584,12 +597,12 @@ class TupleOperator:
 
     def indices(self):
         n = self.count()
-        idx = np.zeros((max(n, 1), 4), np.int32)
+        idx = np.zeros((max(n, 1), self.nlegs + 1), np.int32)
         self._check(load().tg_tuple_copy_indices(self._h, _p(idx), max(n, 0)))
         return idx[:max(n, 0)]
 
     def set_indices(self, idx):
-        idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 4)
+        idx = np.ascontiguousarray(idx, np.int32).reshape(-1, self.nlegs + 1)
         self._check(load().tg_tuple_set_indices(self._h, _p(idx), idx.shape[0]))
 
     def save_indices(self, path):

@@ -101,6 +101,15 @@ struct tg_tuple_ctx {
 
     void* dp5 = nullptr;         // DP5 work buffers (tg_tuple_dp5)
 
+    // 4-leg channel (tg_tuple4_create): tuples [ntup][5] in d_tup, the
+    // element entries (tuple << 2 | leg) from d_goff[e], results per
+    // (tuple, member)
+    int nlegs = 3;
+    double kappa = 0;
+    int64_t* d_goff = nullptr;
+    int32_t* d_gent = nullptr;
+    double* d_res4 = nullptr;
+
     // time-dependent bath (tg_tuple_set_bath_schedule): batch x ns samples
     std::vector<double> h_bsched;
     double* d_bsched = nullptr;
@@ -1188,6 +1197,386 @@ __global__ __launch_bounds__(kApplyThreads) void propagate_dyn_kernel(int64_t ne
 }
 
 
+// ------------------------------------------------------ 4-leg channel
+// The channel {el in, el in, el out, el out} of the electron band
+// (ScatteringChannel<NDim,4>, ScatteringChannel.cpp:716-722): legs 0 and 1
+// free, leg 2 partially free, leg 3 constrained (the NLEGS == 4 branch of
+// IntegratorCore.hpp:283-357).  The integrand is cubic in the states of four
+// elements, so per-tuple moments would be (DIM+1)^4 per output; instead every
+// evaluation sums the MC points directly, one thread per (tuple, member),
+// recomputing the status-independent geometry as the reference does per tuple
+// (ScatteringChannel.cpp:618-650).  Each element then adds its (tuple, leg)
+// entries in tuple order and applies the inverse mass.  Same operation order
+// and no contraction, as the CPU restatement.
+#pragma clang fp contract(off)
+
+__device__ inline double dir4(int leg) { return leg < 2 ? 1.0 : -1.0; }
+
+// D ((DIM+1) x 4DIM) and N of the tuple (ScatteringChannel.cpp:285-352)
+template <int DIM>
+__device__ inline void tuple4_DN(const Geo& g, const int64_t el[4], const int u[3], double D[DIM + 1][4 * DIM],
+                                 double N[DIM + 1]) {
+#pragma unroll
+    for (int r = 0; r <= DIM; ++r) N[r] = 0.0;
+#pragma unroll
+    for (int leg = 0; leg < 4; ++leg) {
+        double kf[DIM][DIM + 1], ef[DIM + 1];
+        elem_forms<DIM>(g, el[leg], false, kf, ef);
+        const double s = dir4(leg);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) D[d][leg * DIM + a] = s * kf[d][1 + a];
+            N[d] -= s * kf[d][0];
+        }
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) D[DIM][leg * DIM + a] = s * ef[1 + a];
+        N[DIM] -= s * ef[0];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) N[d] -= (double)u[c] * g.zone[d][c];
+}
+
+template <int DIM>
+__device__ inline double dc4_inverse(const double D[DIM + 1][4 * DIM], double inv[DIM + 1][DIM + 1]) {
+    constexpr int NC = DIM + 1, C0 = 3 * DIM - 1;
+    double DC[NC][NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) DC[r][q] = D[r][C0 + q];
+    return invert_small<NC>(DC, inv);
+}
+
+// Legs' points of MC sample m: legs 0 and 1 from the top 2 DIM rows, leg 2's
+// free coordinates from the bottom DIM - 1 rows, then
+// alpha = -D_C^-1 (D_F K_F - N) gives leg 2's last coordinate and leg 3;
+// true when both constrained legs lie inside their simplices.
+template <int DIM>
+__device__ inline bool shell4_point(const Geo& g, const double D[DIM + 1][4 * DIM], const double N[DIM + 1],
+                                    const double inv[DIM + 1][DIM + 1], int32_t m, double K[4][DIM]) {
+    constexpr int NC = DIM + 1, C0 = 3 * DIM - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        K[0][d] = g.pts[(size_t)d * g.nmc + m];
+        K[1][d] = g.pts[(size_t)(DIM + d) * g.nmc + m];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM - 1; ++d) K[2][d] = g.pts[(size_t)(g.rows - (DIM - 1) + d) * g.nmc + m];
+    double rhs[NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < C0; ++c) s += D[r][c] * K[c / DIM][c % DIM];
+        rhs[r] = s - N[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s += inv[r][c] * rhs[c];
+        if (r == 0) K[2][DIM - 1] = -s;
+        else K[3][r - 1] = -s;
+    }
+    bool inside = true;
+#pragma unroll
+    for (int leg = 2; leg < 4; ++leg) {
+        double sum = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { inside &= K[leg][d] > 0.0; sum += K[leg][d]; }
+        inside &= sum < 1.0;
+    }
+    return inside;
+}
+
+// |det D_C| NMC (DIM-1)! (DIM!)^2 (IntegratorCore.hpp:350-356)
+template <int DIM>
+__device__ inline double shell4_divisor(double det, int32_t nmc) {
+    double w = fabs(det);
+    w *= (double)nmc * (double)tg::factorial(DIM - 1);
+    w *= (double)tg::factorial(DIM);
+    w *= (double)tg::factorial(DIM);
+    return w;
+}
+
+// Screening of a 4-leg tuple: |det D_C| >= min_det and
+// hits c^4 prod(vol DIM!) / divisor > threshold (c = sqrt 2, the modal
+// order-0 form, ScatteringChannel.cpp:465-503)
+template <int DIM>
+__device__ inline bool tuple4_passes(const Geo& g, const double D[DIM + 1][4 * DIM], const double N[DIM + 1],
+                                     double min_det, double threshold) {
+    double inv[DIM + 1][DIM + 1];
+    const double det = dc4_inverse<DIM>(D, inv);
+    if (!(fabs(det) >= min_det)) return false;
+    int64_t hits = 0;
+    for (int32_t m = 0; m < g.nmc; ++m) {
+        double K[4][DIM];
+        hits += shell4_point<DIM>(g, D, N, inv, m, K) ? 1 : 0;
+    }
+    const double c = sqrt(2.0);
+    const double cpow = c * c * c * c;
+    double amp = 1.0;
+    for (int leg = 0; leg < 4; ++leg) amp *= g.vol * (double)tg::factorial(DIM);
+    return (double)hits * cpow * amp / shell4_divisor<DIM>(det, g.nmc) > threshold;
+}
+
+// The tuples of the element pair (e0, e1) in (e2, uid, e3) order: the energy
+// window of the four legs (ScatteringChannel.hpp:212-222; first with leg 3
+// over the band's range [emin, emax]) and the leg-3 elements whose box meets
+// k0 + k1 - k2 + G u.  Calls emit(e2, e3, uid) for each kept one.
+template <int DIM, typename Emit>
+__device__ inline int32_t pair4_tuples(const Geo& g, int64_t nel, int64_t e0, int64_t e1, double emin, double emax,
+                                       double min_det, double threshold, Emit emit) {
+    constexpr int NES = DIM == 2 ? 2 : 6;
+    double lo0[DIM], hi0[DIM], lo1[DIM], hi1[DIM], el0, eh0, el1, eh1;
+    elem_box_erange<DIM>(g, e0, lo0, hi0, &el0, &eh0);
+    elem_box_erange<DIM>(g, e1, lo1, hi1, &el1, &eh1);
+    int32_t cnt = 0;
+    for (int64_t e2 = 0; e2 < nel; ++e2) {
+        double lo2[DIM], hi2[DIM], el2, eh2;
+        elem_box_erange<DIM>(g, e2, lo2, hi2, &el2, &eh2);
+        if (!(el0 + el1 - eh2 - emax <= 0.0 && eh0 + eh1 - el2 - emin >= 0.0)) continue;
+        for (int32_t uid = 0; uid < g.nuv; ++uid) {
+            int u[3];
+            umklapp_vec<DIM>(g.nu, uid, u);
+            double lo[DIM], hi[DIM], smin[3], smax[3];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const double U = umklapp_comp<DIM>(g, u, d);
+                lo[d] = lo0[d] + lo1[d] - hi2[d] + U;
+                hi[d] = hi0[d] + hi1[d] - lo2[d] + U;
+                if (g.axis) {
+                    const double h = g.extent[d] / g.nsec[d];
+                    smin[d] = floor((lo[d] - g.origin[d]) / h);
+                    smax[d] = floor((hi[d] - g.origin[d]) / h);
+                } else {
+                    smin[d] = 1e300;
+                    smax[d] = -1e300;
+                }
+            }
+            if (!g.axis) {
+                for (int cm = 0; cm < (1 << DIM); ++cm) {
+                    double x[DIM];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) x[d] = ((cm >> d) & 1 ? hi[d] : lo[d]) - g.origin[d];
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) s = s + g.dGinv[c][d] * x[d];
+                        smin[c] = s < smin[c] ? s : smin[c];
+                        smax[c] = s > smax[c] ? s : smax[c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    smin[c] = floor(smin[c]) - 1.0;
+                    smax[c] = floor(smax[c]) + 1.0;
+                }
+            }
+            int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0};
+            bool empty = false;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const double a = smin[d], b = smax[d];
+                empty |= b < 0.0 || a > g.nsec[d] - 1;
+                slo[d] = a < 0.0 ? 0 : (a > g.nsec[d] - 1 ? g.nsec[d] - 1 : (int)a);
+                shi[d] = b > g.nsec[d] - 1 ? g.nsec[d] - 1 : (b < 0.0 ? 0 : (int)b);
+            }
+            if (empty) continue;
+            int sc[3] = {slo[0], slo[1], slo[2]};
+            for (;;) {
+                int64_t sid = 0, stride = 1;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) { sid += stride * sc[d]; stride *= g.nsec[d]; }
+                for (int t = 0; t < NES; ++t) {
+                    const int64_t e3 = sid * NES + t;
+                    double lo3[DIM], hi3[DIM], el3, eh3;
+                    elem_box_erange<DIM>(g, e3, lo3, hi3, &el3, &eh3);
+                    bool meet = true;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) meet &= lo3[d] <= hi[d] && hi3[d] >= lo[d];
+                    if (!meet) continue;
+                    if (!(el0 + el1 - eh2 - eh3 <= 0.0 && eh0 + eh1 - el2 - el3 >= 0.0)) continue;
+                    const int64_t el[4] = {e0, e1, e2, e3};
+                    double D[DIM + 1][4 * DIM], N[DIM + 1];
+                    tuple4_DN<DIM>(g, el, u, D, N);
+                    if (!tuple4_passes<DIM>(g, D, N, min_det, threshold)) continue;
+                    emit(e2, e3, uid);
+                    ++cnt;
+                }
+                int d = 0;
+                while (d < DIM && ++sc[d] > shi[d]) { sc[d] = slo[d]; ++d; }
+                if (d == DIM) break;
+            }
+        }
+    }
+    return cnt;
+}
+
+// One thread per element pair p = e0 nel + e1: pass 0 counts its tuples into
+// cnt[p], pass 1 writes them from off[p] as [e0, e1, e2, e3, uid].
+template <int DIM>
+__global__ void __launch_bounds__(kEnumThreads)
+enum4_kernel(Geo g, int64_t nel, double emin, double emax, double min_det, double threshold, int pass,
+             int64_t* __restrict__ cnt, const int64_t* __restrict__ off, int32_t* __restrict__ tup) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nel * nel) return;
+    const int64_t e0 = p / nel, e1 = p % nel;
+    if (pass == 0) {
+        cnt[p] = pair4_tuples<DIM>(g, nel, e0, e1, emin, emax, min_det, threshold, [](int64_t, int64_t, int32_t) {});
+    } else {
+        int64_t w = off[p];
+        pair4_tuples<DIM>(g, nel, e0, e1, emin, emax, min_det, threshold, [&](int64_t e2, int64_t e3, int32_t uid) {
+            int32_t* o = tup + w * 5;
+            o[0] = (int32_t)e0;
+            o[1] = (int32_t)e1;
+            o[2] = (int32_t)e2;
+            o[3] = (int32_t)e3;
+            o[4] = uid;
+            ++w;
+        });
+    }
+}
+
+// One thread per (tuple, member), member fastest: the tuple's MC integral
+// (scatteringIntegrationTypeB) times constructAmplitude
+// (ScatteringChannel.cpp:399-415) and the leg's sign, written as
+// res[(t batch + b)][leg][DIM+1]; zero when |det D_C| < min_det (:621-622).
+// functAmplitude: 1 for kappa <= 0, else 1/(|q|^2 + kappa^2)^2 with q the
+// barycentre of leg 2 minus that of leg 0.
+template <int DIM>
+__global__ void __launch_bounds__(kApplyThreads)
+tuple4_kernel(Geo g, int64_t ntup, int64_t nel, int32_t batch, const int32_t* __restrict__ tup,
+              const double* __restrict__ f, double amplitude, double kappa, double min_det,
+              double* __restrict__ res) {
+    constexpr int NC = DIM + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntup * batch) return;
+    const int64_t t = i / batch;
+    const int32_t b = (int32_t)(i % batch);
+    const int32_t* tp = tup + t * 5;
+    const int64_t el[4] = {tp[0], tp[1], tp[2], tp[3]};
+    int u[3];
+    umklapp_vec<DIM>(g.nu, tp[4], u);
+    double D[DIM + 1][4 * DIM], N[DIM + 1], inv[NC][NC];
+    tuple4_DN<DIM>(g, el, u, D, N);
+    const double det = dc4_inverse<DIM>(D, inv);
+    double acc[4][NC];
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) acc[l][a] = 0.0;
+    double* r = res + (size_t)i * 4 * NC;
+    if (!(fabs(det) >= min_det)) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int a = 0; a < NC; ++a) r[l * NC + a] = 0.0;
+        return;
+    }
+    // the forms (ScatteringChannel.cpp:628-640, fermion statistics): direct
+    // -f0 f1 (1 - f2)(1 - f3), time reversed (1 - f0)(1 - f1) f2 f3
+    double fa[4][NC], fb[4][NC];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const double* c = f + ((size_t)b * nel + el[l]) * NC;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            const double one = a == 0 ? 1.0 : 0.0, x = c[a];
+            if (l < 2) {
+                fa[l][a] = l == 0 ? -x : x;
+                fb[l][a] = one - x;
+            } else {
+                fa[l][a] = one - x;
+                fb[l][a] = x;
+            }
+        }
+    }
+    for (int32_t m = 0; m < g.nmc; ++m) {
+        double K[4][DIM];
+        if (!shell4_point<DIM>(g, D, N, inv, m, K)) continue;
+        double pa = 1.0, pb = 1.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            double va = fa[l][0], vb = fb[l][0];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                va += fa[l][1 + d] * K[l][d];
+                vb += fb[l][1 + d] * K[l][d];
+            }
+            pa *= va;
+            pb *= vb;
+        }
+        const double val = pa + pb;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            acc[l][0] += val;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) acc[l][1 + d] += val * K[l][d];
+        }
+    }
+    const double w = shell4_divisor<DIM>(det, g.nmc);
+    double amp = amplitude;
+    for (int l = 0; l < 4; ++l) amp *= g.vol * (double)tg::factorial(DIM);
+    if (kappa > 0.0) {
+        int64_t v0[DIM + 1], v2[DIM + 1];
+        elem_nodes<DIM>(g, el[0], v0);
+        elem_nodes<DIM>(g, el[2], v2);
+        double q2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            double c0 = 0.0, c2 = 0.0;
+#pragma unroll
+            for (int a = 0; a <= DIM; ++a) {
+                c0 += node_coord<DIM>(g, v0[a], d);
+                c2 += node_coord<DIM>(g, v2[a], d);
+            }
+            const double q = (c2 - c0) / (double)(DIM + 1);
+            q2 += q * q;
+        }
+        const double vq = 1.0 / (q2 + kappa * kappa);
+        amp *= vq * vq;
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) r[l * NC + a] = dir4(l) * (amp * (acc[l][a] / w));
+}
+
+// One thread per (element, member), member fastest: the element's entries
+// (tuple << 2 | leg, in tuple order) summed, then the inverse mass
+// (Function.cpp:801-804) into out [batch][nel][DIM+1].
+template <int DIM>
+__global__ void __launch_bounds__(kApplyThreads)
+gather4_kernel(int64_t nel, int32_t batch, const int64_t* __restrict__ goff, const int32_t* __restrict__ gent,
+               const double* __restrict__ res, const double* __restrict__ minv_g, double* __restrict__ out) {
+    constexpr int NC = DIM + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nel * batch) return;
+    const int64_t e = i / batch;
+    const int32_t b = (int32_t)(i % batch);
+    double acc[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) acc[a] = 0.0;
+    for (int64_t k = goff[e]; k < goff[e + 1]; ++k) {
+        const int32_t ent = gent[k];
+        const double* r = res + ((size_t)(ent >> 2) * batch + b) * 4 * NC + (ent & 3) * NC;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) acc[a] += r[a];
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) v += acc[a] * minv_g[a * NC + q];
+        out[((size_t)b * nel + e) * NC + q] = v;
+    }
+}
+
 #pragma clang fp contract(on)
 
 Geo make_geo(const tg_tuple_ctx* c) {
@@ -1326,6 +1715,113 @@ tg_status enumerate(tg_tuple_ctx* c) {
     if (e != hipSuccess) return fail(c, TG_ERR_HIP, "tuple fill: %s", hipGetErrorString(e));
     c->ntup = total;
     return build_lists(c);
+}
+
+// The element entries of a 4-leg list: for element e the (tuple << 2 | leg)
+// of every leg that lands on it, in tuple order then leg order, and the
+// per-(tuple, member) result buffer.
+tg_status build_lists4(tg_tuple_ctx* c, const int32_t* idx) {
+    const int64_t nel = c->nel, ntup = c->ntup;
+    if (ntup >= ((int64_t)1 << 29)) return fail(c, TG_ERR_ARG, "%lld tuples: at most 2^29 - 1", (long long)ntup);
+    std::vector<int64_t> off((size_t)nel + 1, 0);
+    for (int64_t t = 0; t < ntup; ++t)
+        for (int l = 0; l < 4; ++l) ++off[(size_t)idx[t * 5 + l] + 1];
+    for (int64_t e = 0; e < nel; ++e) off[(size_t)e + 1] += off[(size_t)e];
+    std::vector<int32_t> ent((size_t)std::max<int64_t>(4 * ntup, 1));
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int64_t t = 0; t < ntup; ++t)
+        for (int l = 0; l < 4; ++l) ent[(size_t)pos[(size_t)idx[t * 5 + l]]++] = (int32_t)((t << 2) | l);
+    tg_status s;
+    if ((s = dalloc(c, &c->d_goff, (size_t)nel + 1)) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_gent, ent.size())) != TG_OK) return s;
+    if ((s = dalloc(c, &c->d_res4, (size_t)std::max<int64_t>(ntup, 1) * c->max_batch * 4 * (c->dim + 1))) != TG_OK)
+        return s;
+    TT_HIP(c, hipMemcpy(c->d_goff, off.data(), sizeof(int64_t) * (nel + 1), hipMemcpyHostToDevice));
+    TT_HIP(c, hipMemcpy(c->d_gent, ent.data(), sizeof(int32_t) * ent.size(), hipMemcpyHostToDevice));
+    return TG_OK;
+}
+
+// The 4-leg tuple list on the GPU: one thread per element pair (e0, e1),
+// counting then filling, so the list is in (e0, e1, e2, uid, e3) order.
+tg_status enumerate4(tg_tuple_ctx* c) {
+    const int64_t nel = c->nel;
+    if (nel > 46340) return fail(c, TG_ERR_ARG, "%lld elements: the 4-leg pair grid needs nel^2 < 2^31", (long long)nel);
+    Geo g = make_geo(c);
+    // the band's energy range from the element table
+    const int stride = c->dim == 2 ? etab_stride<2>() : etab_stride<3>();
+    std::vector<double> tab((size_t)nel * stride);
+    TT_HIP(c, hipMemcpy(tab.data(), c->d_etab, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
+    const int eoff = c->dim * (c->dim + 1) + (c->dim + 1) + 2 * c->dim;
+    double emin = 0.0, emax = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        const double lo = tab[(size_t)e * stride + eoff], hi = tab[(size_t)e * stride + eoff + 1];
+        if (e == 0 || lo < emin) emin = lo;
+        if (e == 0 || hi > emax) emax = hi;
+    }
+    const int64_t np = nel * nel;
+    int64_t* d_cnt = nullptr;
+    int64_t* d_off = nullptr;
+    tg_status s;
+    if ((s = dalloc(c, &d_cnt, np)) != TG_OK) return s;
+    if ((s = dalloc(c, &d_off, np)) != TG_OK) { (void)hipFree(d_cnt); return s; }
+    auto cleanup = [&]() { (void)hipFree(d_cnt); (void)hipFree(d_off); };
+    const dim3 grid((unsigned)((np + kEnumThreads - 1) / kEnumThreads));
+    auto launch = [&](int pass, int32_t* tp) {
+        if (c->dim == 2)
+            hipLaunchKernelGGL(enum4_kernel<2>, grid, dim3(kEnumThreads), 0, c->stream, g, nel, emin, emax, c->min_det,
+                               c->threshold, pass, d_cnt, d_off, tp);
+        else
+            hipLaunchKernelGGL(enum4_kernel<3>, grid, dim3(kEnumThreads), 0, c->stream, g, nel, emin, emax, c->min_det,
+                               c->threshold, pass, d_cnt, d_off, tp);
+    };
+    launch(0, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { cleanup(); return fail(c, TG_ERR_HIP, "4-leg tuple count: %s", hipGetErrorString(e)); }
+    std::vector<int64_t> cnt((size_t)np), off((size_t)np);
+    e = hipMemcpy(cnt.data(), d_cnt, sizeof(int64_t) * np, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { cleanup(); return fail(c, TG_ERR_HIP, "4-leg count copy: %s", hipGetErrorString(e)); }
+    int64_t total = 0;
+    for (int64_t p = 0; p < np; ++p) { off[(size_t)p] = total; total += cnt[(size_t)p]; }
+    if (total >= ((int64_t)1 << 29)) { cleanup(); return fail(c, TG_ERR_ARG, "%lld tuples: at most 2^29 - 1", (long long)total); }
+    if ((s = dalloc(c, &c->d_tup, (size_t)total * 5)) != TG_OK) { cleanup(); return s; }
+    e = hipMemcpy(d_off, off.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch(1, c->d_tup);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, "4-leg tuple fill: %s", hipGetErrorString(e));
+    c->ntup = total;
+    std::vector<int32_t> idx((size_t)std::max<int64_t>(total, 1) * 5);
+    if (total) TT_HIP(c, hipMemcpy(idx.data(), c->d_tup, sizeof(int32_t) * 5 * total, hipMemcpyDeviceToHost));
+    return build_lists4(c, idx.data());
+}
+
+// One 4-leg evaluation of `batch` members from device states d_f into d_out.
+tg_status propagate4_device(tg_tuple_ctx* c, const double* d_f, double* d_out, int32_t batch) {
+    Geo g = make_geo(c);
+    if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev0, c->stream));
+    const int64_t nt = c->ntup * batch, ne = c->nel * batch;
+    const dim3 gt((unsigned)std::max<int64_t>((nt + kApplyThreads - 1) / kApplyThreads, 1));
+    const dim3 ge((unsigned)((ne + kApplyThreads - 1) / kApplyThreads));
+    if (c->dim == 2) {
+        if (nt)
+            hipLaunchKernelGGL(tuple4_kernel<2>, gt, dim3(kApplyThreads), 0, c->stream, g, c->ntup, c->nel, batch,
+                               c->d_tup, d_f, c->amplitude, c->kappa, c->min_det, c->d_res4);
+        hipLaunchKernelGGL(gather4_kernel<2>, ge, dim3(kApplyThreads), 0, c->stream, c->nel, batch, c->d_goff,
+                           c->d_gent, c->d_res4, c->d_minv, d_out);
+    } else {
+        if (nt)
+            hipLaunchKernelGGL(tuple4_kernel<3>, gt, dim3(kApplyThreads), 0, c->stream, g, c->ntup, c->nel, batch,
+                               c->d_tup, d_f, c->amplitude, c->kappa, c->min_det, c->d_res4);
+        hipLaunchKernelGGL(gather4_kernel<3>, ge, dim3(kApplyThreads), 0, c->stream, c->nel, batch, c->d_goff,
+                           c->d_gent, c->d_res4, c->d_minv, d_out);
+    }
+    TT_HIP(c, hipGetLastError());
+    if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev1, c->stream));
+    return TG_OK;
 }
 
 tg_status moments(tg_tuple_ctx* c) {
@@ -1713,7 +2209,8 @@ extern "C" void tg_tuple_destroy(tg_tuple_ctx* c) {
     if (c->d_eps_ph) (void)hipFree(c->d_eps_ph);
     if (c->d_bsched) (void)hipFree(c->d_bsched);
     void* bufs[] = {c->d_eps, c->d_pts, c->d_tup, c->d_off0, c->d_off2, c->d_oth0, c->d_oth2, c->d_pos2,
-                    c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft};
+                    c->d_q0,  c->d_q2,  c->d_f,   c->d_out,  c->d_bath, c->d_minv, c->d_etab, c->d_ft,
+                    c->d_goff, c->d_gent, c->d_res4};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1904,6 +2401,21 @@ extern "C" int32_t tg_tuple_point_rows(int32_t dim, int32_t nlegs) {
     return std::max(dim * (nlegs - 1) - 1, 2 * dim);
 }
 
+extern "C" tg_status tg_tuple4_create(const tg_tuple_desc* d, double kappa, tg_tuple_ctx** out) {
+    if (!out) return TG_ERR_ARG;
+    *out = nullptr;
+    if (!d) return TG_ERR_ARG;
+    tg_status s = tg_tuple_create(d, out);
+    if (s != TG_OK) return s;
+    tg_tuple_ctx* c = *out;
+    if (d->dynamic_phonons) return fail(c, TG_ERR_ARG, "the 4-leg channel has no phonon band (dynamic_phonons must be 0)");
+    if (!std::isfinite(kappa)) return fail(c, TG_ERR_ARG, "kappa is not finite");
+    c->nlegs = 4;
+    c->kappa = kappa;
+    c->rows = tg_tuple_point_rows(c->dim, 4);
+    return dalloc(c, &c->d_pts, (size_t)c->rows * c->nmc);
+}
+
 extern "C" tg_status tg_tuple_set_points(tg_tuple_ctx* c, const double* pts, int32_t flags) {
     if (!c || !pts) return TG_ERR_ARG;
     (void)flags;
@@ -1911,6 +2423,12 @@ extern "C" tg_status tg_tuple_set_points(tg_tuple_ctx* c, const double* pts, int
         if (!std::isfinite(pts[i])) return fail(c, TG_ERR_NONFINITE, "pts[%lld] is not finite", (long long)i);
     TT_HIP(c, hipMemcpy(c->d_pts, pts, sizeof(double) * c->rows * c->nmc, hipMemcpyHostToDevice));
     tg_status s;
+    if (c->nlegs == 4) {
+        // no moments: every evaluation integrates the points directly
+        if (!c->d_tup && (s = enumerate4(c)) != TG_OK) return s;
+        c->have_points = true;
+        return TG_OK;
+    }
     if (!c->d_tup && (s = enumerate(c)) != TG_OK) return s;
     if ((s = moments(c)) != TG_OK) return s;
     c->have_points = true;
@@ -1921,6 +2439,23 @@ extern "C" tg_status tg_tuple_set_indices(tg_tuple_ctx* c, const int32_t* idx, i
     if (!c || (ntup > 0 && !idx) || ntup < 0) return TG_ERR_ARG;
     int32_t nuv = 1;
     for (int d = 0; d < c->dim; ++d) nuv *= 2 * c->nu + 1;
+    if (c->nlegs == 4) {
+        // [ntup][5] in any order: each element sums its entries in list order
+        for (int64_t t = 0; t < ntup; ++t) {
+            for (int l = 0; l < 4; ++l)
+                if (idx[t * 5 + l] < 0 || idx[t * 5 + l] >= c->nel)
+                    return fail(c, TG_ERR_ARG, "tuple %lld: element %d outside [0, %lld)", (long long)t, idx[t * 5 + l],
+                                (long long)c->nel);
+            if (idx[t * 5 + 4] < 0 || idx[t * 5 + 4] >= nuv)
+                return fail(c, TG_ERR_ARG, "tuple %lld: umklapp id %d outside [0, %d)", (long long)t, idx[t * 5 + 4], nuv);
+        }
+        if (ntup >= ((int64_t)1 << 29)) return fail(c, TG_ERR_ARG, "%lld tuples: at most 2^29 - 1", (long long)ntup);
+        tg_status s;
+        if ((s = dalloc(c, &c->d_tup, (size_t)ntup * 5)) != TG_OK) return s;
+        if (ntup) TT_HIP(c, hipMemcpy(c->d_tup, idx, sizeof(int32_t) * 5 * ntup, hipMemcpyHostToDevice));
+        c->ntup = ntup;
+        return build_lists4(c, idx);
+    }
     for (int64_t t = 0; t < ntup; ++t) {
         for (int l = 0; l < 3; ++l)
             if (idx[t * 4 + l] < 0 || idx[t * 4 + l] >= c->nel)
@@ -1967,12 +2502,14 @@ extern "C" tg_status tg_tuple_copy_indices(tg_tuple_ctx* c, int32_t* idx, int64_
     if (!c || !idx) return TG_ERR_ARG;
     if (!c->d_tup) return fail(c, TG_ERR_STATE, "no tuple list yet (call tg_tuple_set_points)");
     if (cap < c->ntup) return fail(c, TG_ERR_ARG, "cap %lld < %lld tuples", (long long)cap, (long long)c->ntup);
-    if (c->ntup) TT_HIP(c, hipMemcpy(idx, c->d_tup, sizeof(int32_t) * 4 * c->ntup, hipMemcpyDeviceToHost));
+    const size_t rec = (size_t)c->nlegs + 1;
+    if (c->ntup) TT_HIP(c, hipMemcpy(idx, c->d_tup, sizeof(int32_t) * rec * c->ntup, hipMemcpyDeviceToHost));
     return TG_OK;
 }
 
 extern "C" tg_status tg_tuple_set_bath_schedule(tg_tuple_ctx* c, const tg_bath_schedule* s) {
     if (!c) return TG_ERR_ARG;
+    if (c->nlegs == 4) return fail(c, TG_ERR_STATE, "the 4-leg channel has no bath");
     if (!s) {
         c->h_bsched.clear();
         c->bs_ns = c->bs_batch = 0;
@@ -2009,7 +2546,8 @@ extern "C" tg_status tg_tuple_set_amplitude(tg_tuple_ctx* c, double amplitude) {
 template <bool RATES>
 tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, double* out, int64_t nel, int32_t batch,
                       int32_t flags) {
-    if (!c || !f || (!bath_n && !c->dyn) || !out) return TG_ERR_ARG;
+    if (!c || !f || (!bath_n && !c->dyn && c->nlegs == 3) || !out) return TG_ERR_ARG;
+    if (RATES && c->nlegs == 4) return fail(c, TG_ERR_STATE, "scattering rates: 3-leg channels only");
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
@@ -2017,7 +2555,17 @@ tg_status tuple_apply(tg_tuple_ctx* c, const double* f, const double* bath_n, do
     c->time_kernel = (flags & TG_TIME_KERNEL) != 0;
     const size_t len = (size_t)c->nel * (c->dim + 1) * batch * (c->dyn ? 2 : 1);
     const size_t olen = RATES ? (size_t)c->nel * (c->dim + 1) * batch : len;   // rates: the electron band
-    if (flags & TG_PTR_DEVICE) {
+    if (c->nlegs == 4) {
+        if (flags & TG_PTR_DEVICE) {
+            tg_status s = propagate4_device(c, f, out, batch);
+            if (s != TG_OK) return s;
+        } else {
+            TT_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
+            tg_status s = propagate4_device(c, c->d_f, c->d_out, batch);
+            if (s != TG_OK) return s;
+            TT_HIP(c, hipMemcpyAsync(out, c->d_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+        }
+    } else if (flags & TG_PTR_DEVICE) {
         tg_status s = propagate_device<RATES>(c, f, bath_n, out, batch);
         if (s != TG_OK) return s;
     } else {
@@ -2051,22 +2599,23 @@ extern "C" tg_status tg_tuple_last_kernel_ms(tg_tuple_ctx* c, double* ms) {
 
 extern "C" void* tg_tuple_stream(const tg_tuple_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
-// .scattIndices file of a 3-leg channel (ScatteringChannel.cpp:422-459,520-537):
+// .scattIndices file of a channel (ScatteringChannel.cpp:422-459,520-537):
 // for each leg and axis the int nSecSplits, an unsigned long tuple count,
-// then per tuple the three element ids and the umklapp id, all native-endian.
+// then per tuple the legs' element ids and the umklapp id, all native-endian.
 extern "C" tg_status tg_tuple_save_indices(tg_tuple_ctx* c, const char* path) {
     if (!c || !path) return TG_ERR_ARG;
     if (!c->d_tup) return fail(c, TG_ERR_STATE, "no tuple list yet");
-    std::vector<int32_t> idx((size_t)c->ntup * 4);
-    if (c->ntup) TT_HIP(c, hipMemcpy(idx.data(), c->d_tup, sizeof(int32_t) * 4 * c->ntup, hipMemcpyDeviceToHost));
+    const size_t rec = (size_t)c->nlegs + 1;
+    std::vector<int32_t> idx((size_t)c->ntup * rec);
+    if (c->ntup) TT_HIP(c, hipMemcpy(idx.data(), c->d_tup, sizeof(int32_t) * rec * c->ntup, hipMemcpyDeviceToHost));
     FILE* fp = std::fopen(path, "wb");
     if (!fp) return fail(c, TG_ERR_ARG, "cannot open %s for writing", path);
     bool ok = true;
-    for (int l = 0; l < 3; ++l)
+    for (int l = 0; l < c->nlegs; ++l)
         for (int d = 0; d < c->dim; ++d) ok &= std::fwrite(&c->nsec[d], sizeof(int32_t), 1, fp) == 1;
     unsigned long total = (unsigned long)c->ntup;
     ok &= std::fwrite(&total, sizeof(unsigned long), 1, fp) == 1;
-    if (c->ntup) ok &= std::fwrite(idx.data(), sizeof(int32_t) * 4, (size_t)c->ntup, fp) == (size_t)c->ntup;
+    if (c->ntup) ok &= std::fwrite(idx.data(), sizeof(int32_t) * rec, (size_t)c->ntup, fp) == (size_t)c->ntup;
     ok &= std::fclose(fp) == 0;
     return ok ? TG_OK : fail(c, TG_ERR_ARG, "short write to %s", path);
 }
@@ -2076,7 +2625,7 @@ extern "C" tg_status tg_tuple_load_indices(tg_tuple_ctx* c, const char* path) {
     FILE* fp = std::fopen(path, "rb");
     if (!fp) return fail(c, TG_ERR_ARG, "cannot open %s", path);
     bool same = true;
-    for (int l = 0; l < 3; ++l)
+    for (int l = 0; l < c->nlegs; ++l)
         for (int d = 0; d < c->dim; ++d) {
             int32_t ns = 0;
             same &= std::fread(&ns, sizeof(int32_t), 1, fp) == 1 && ns == c->nsec[d];
@@ -2086,8 +2635,9 @@ extern "C" tg_status tg_tuple_load_indices(tg_tuple_ctx* c, const char* path) {
         std::fclose(fp);
         return fail(c, TG_ERR_ARG, "%s: mesh sections differ from this channel's", path);
     }
-    std::vector<int32_t> idx((size_t)total * 4);
-    bool ok = total == 0 || std::fread(idx.data(), sizeof(int32_t) * 4, total, fp) == total;
+    const size_t rec = (size_t)c->nlegs + 1;
+    std::vector<int32_t> idx((size_t)total * rec);
+    bool ok = total == 0 || std::fread(idx.data(), sizeof(int32_t) * rec, total, fp) == total;
     std::fclose(fp);
     if (!ok) return fail(c, TG_ERR_ARG, "%s: truncated tuple list", path);
     return tg_tuple_set_indices(c, idx.data(), (int64_t)total);
@@ -2103,6 +2653,7 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
                                   double dense_dt, double rtol, double atol, const double* bath_n, int32_t flags,
                                   double* dense_out, int32_t max_dense, tg_step_stats* stats,
                                   double* h_accepted, int32_t h_cap) {
+    if (c && c->nlegs == 4) return fail(c, TG_ERR_STATE, "tg_tuple_dp5: 3-leg channels only");
     using namespace tg;
     const bool sched_on = (flags & TG_BATH_SCHEDULE) != 0;
     if (h_accepted && h_cap < 1) return TG_ERR_ARG;

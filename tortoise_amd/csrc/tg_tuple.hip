@@ -1547,33 +1547,70 @@ tuple4_kernel(Geo g, int64_t ntup, int64_t nel, int32_t batch, const int32_t* __
         for (int a = 0; a < NC; ++a) r[l * NC + a] = dir4(l) * (amp * (acc[l][a] / w));
 }
 
-// One thread per (element, member), member fastest: the element's entries
-// (tuple << 2 | leg, in tuple order) summed, then the inverse mass
+// One 64-lane workgroup per (element, member), b = blockIdx % batch: the
+// element's entries (tuple << 2 | leg, in tuple order) are loaded 256 at a
+// time into LDS by all lanes, then lane a < DIM+1 adds coefficient a of them
+// in entry order (the order of the sequential sum), then the inverse mass
 // (Function.cpp:801-804) into out [batch][nel][DIM+1].
+constexpr int kGather4Chunk = 256;
 template <int DIM>
-__global__ void __launch_bounds__(kApplyThreads)
+__global__ void __launch_bounds__(64)
 gather4_kernel(int64_t nel, int32_t batch, const int64_t* __restrict__ goff, const int32_t* __restrict__ gent,
                const double* __restrict__ res, const double* __restrict__ minv_g, double* __restrict__ out) {
     constexpr int NC = DIM + 1;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nel * batch) return;
-    const int64_t e = i / batch;
-    const int32_t b = (int32_t)(i % batch);
-    double acc[NC];
+    __shared__ double sh[kGather4Chunk][NC];
+    const int64_t e = blockIdx.x / batch;
+    const int32_t b = (int32_t)(blockIdx.x % batch);
+    const int lane = threadIdx.x;
+    const int64_t k_end = goff[e + 1];
+    constexpr int U = kGather4Chunk / 64;
+    // this lane's entries of the next chunk, loaded while the current one is summed
+    double nxt[U][NC];
+    auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int a = 0; a < NC; ++a) acc[a] = 0.0;
-    for (int64_t k = goff[e]; k < goff[e + 1]; ++k) {
-        const int32_t ent = gent[k];
-        const double* r = res + ((size_t)(ent >> 2) * batch + b) * 4 * NC + (ent & 3) * NC;
+        for (int u = 0; u < U; ++u) {
+            const int64_t k = k0 + u * 64 + lane;
+            if (k < k_end) {
+                const int32_t ent = gent[k];
+                const double* r = res + ((size_t)(ent >> 2) * batch + b) * 4 * NC + (ent & 3) * NC;
 #pragma unroll
-        for (int a = 0; a < NC; ++a) acc[a] += r[a];
+                for (int a = 0; a < NC; ++a) nxt[u][a] = r[a];
+            }
+        }
+    };
+    double acc = 0.0;
+    int64_t k0 = goff[e];
+    if (k0 < k_end) load(k0);
+    for (; k0 < k_end; k0 += kGather4Chunk) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u * 64 + lane < k_end)
+#pragma unroll
+                for (int a = 0; a < NC; ++a) sh[u * 64 + lane][a] = nxt[u][a];
+        __syncthreads();
+        if (k0 + kGather4Chunk < k_end) load(k0 + kGather4Chunk);
+        if (lane < NC) {
+            const int n = (int)min<int64_t>(kGather4Chunk, k_end - k0);
+            int j = 0;
+            // reads issued ahead, adds in entry order
+            for (; j + 16 <= n; j += 16) {
+                double v[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) v[q] = sh[j + q][lane];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc += v[q];
+            }
+            for (; j < n; ++j) acc += sh[j][lane];
+        }
+        __syncthreads();
     }
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
+    if (lane < NC) sh[0][lane] = acc;
+    __syncthreads();
+    if (lane < NC) {
         double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < NC; ++a) v += acc[a] * minv_g[a * NC + q];
-        out[((size_t)b * nel + e) * NC + q] = v;
+        for (int a = 0; a < NC; ++a) v += sh[0][a] * minv_g[a * NC + lane];
+        out[((size_t)b * nel + e) * NC + lane] = v;
     }
 }
 
@@ -1805,18 +1842,18 @@ tg_status propagate4_device(tg_tuple_ctx* c, const double* d_f, double* d_out, i
     if (c->time_kernel) TT_HIP(c, hipEventRecord(c->ev0, c->stream));
     const int64_t nt = c->ntup * batch, ne = c->nel * batch;
     const dim3 gt((unsigned)std::max<int64_t>((nt + kApplyThreads - 1) / kApplyThreads, 1));
-    const dim3 ge((unsigned)((ne + kApplyThreads - 1) / kApplyThreads));
+    const dim3 ge((unsigned)ne);   // one 64-lane workgroup per (element, member)
     if (c->dim == 2) {
         if (nt)
             hipLaunchKernelGGL(tuple4_kernel<2>, gt, dim3(kApplyThreads), 0, c->stream, g, c->ntup, c->nel, batch,
                                c->d_tup, d_f, c->amplitude, c->kappa, c->min_det, c->d_res4);
-        hipLaunchKernelGGL(gather4_kernel<2>, ge, dim3(kApplyThreads), 0, c->stream, c->nel, batch, c->d_goff,
+        hipLaunchKernelGGL(gather4_kernel<2>, ge, dim3(64), 0, c->stream, c->nel, batch, c->d_goff,
                            c->d_gent, c->d_res4, c->d_minv, d_out);
     } else {
         if (nt)
             hipLaunchKernelGGL(tuple4_kernel<3>, gt, dim3(kApplyThreads), 0, c->stream, g, c->ntup, c->nel, batch,
                                c->d_tup, d_f, c->amplitude, c->kappa, c->min_det, c->d_res4);
-        hipLaunchKernelGGL(gather4_kernel<3>, ge, dim3(kApplyThreads), 0, c->stream, c->nel, batch, c->d_goff,
+        hipLaunchKernelGGL(gather4_kernel<3>, ge, dim3(64), 0, c->stream, c->nel, batch, c->d_goff,
                            c->d_gent, c->d_res4, c->d_minv, d_out);
     }
     TT_HIP(c, hipGetLastError());

@@ -346,9 +346,9 @@ tg_status   tg_tuple_create(const tg_tuple_desc* desc, tg_tuple_ctx** out);
  * 1/(|q|^2 + kappa^2)^2, q = barycentre of leg 2 - barycentre of leg 0.
  * On such a context the point matrix has tg_tuple_point_rows(dim, 4) rows,
  * tuples are [n][5] = e0, e1, e2, e3, uid (any order; the built list is in
- * (e0, e1, e2, uid, e3) order), tg_tuple_propagate ignores bath_n, and
- * tg_tuple_rates, tg_tuple_dp5 and tg_tuple_set_bath_schedule return
- * TG_ERR_STATE. */
+ * (e0, e1, e2, uid, e3) order), tg_tuple_propagate and tg_tuple_dp5 ignore
+ * bath_n (the DP5 bath ledger stays zero), and tg_tuple_rates and
+ * tg_tuple_set_bath_schedule return TG_ERR_STATE. */
 tg_status   tg_tuple4_create(const tg_tuple_desc* desc, double kappa, tg_tuple_ctx** out);
 void        tg_tuple_destroy(tg_tuple_ctx* ctx);
 const char* tg_tuple_last_error(const tg_tuple_ctx* ctx);

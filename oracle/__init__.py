@@ -187,6 +187,10 @@ def lib():
         L.or_mc_tuple_ee.restype = C.c_int
         L.or_mc_tuple_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32,
                                      C.c_double, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.or_dp5_mc_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32, C.c_double,
+                                   C.c_double, C.c_double, _i32p, C.c_int64, _f64p, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int32,
+                                   C.c_void_p]
         L.or_mc_apply_ee.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int32, _f64p, C.c_int32,
                                      C.c_double, C.c_double, C.c_double, _i32p, C.c_int64, _f64p,
                                      C.c_int32, _f64p]
@@ -761,6 +765,25 @@ def mc_apply_ee(dim, nsec, origin, extent, eps, nu, pts, idx, f, amplitude=1.0, 
     lib().or_mc_apply_ee(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), nu, pts,
                          pts.shape[1], min_det, amplitude, kappa, idx, idx.shape[0], f, batch, out)
     return out
+
+
+def dp5_mc_ee(dim, nsec, origin, extent, eps, nu, pts, idx, f0, t0, t_final, dense_dt, rtol, atol,
+              amplitude=1.0, kappa=0.0, min_det=1e-9, max_dense=0, sides=None, zone=None):
+    """DP5(4) of the electron band's DG state [nel][dim+1] with the 4-leg channel."""
+    nsec = np.ascontiguousarray(nsec, np.int32)
+    geo = zone_geometry(dim, origin, extent, sides, zone)
+    pts = np.ascontiguousarray(pts, np.float64)
+    idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 5)
+    f = np.ascontiguousarray(f0, np.float64).copy()
+    dense = np.zeros((max(max_dense, 1),) + f.shape)
+    dense_t = np.zeros(max(max_dense, 1))
+    st = DP5Stats()
+    lib().or_dp5_mc_ee(dim, nsec, geo, np.ascontiguousarray(eps, np.float64), nu, pts, pts.shape[1], min_det,
+                       amplitude, kappa, idx, idx.shape[0], f, t0, t_final, dense_dt, rtol, atol,
+                       dense.ctypes.data if max_dense else None, dense_t.ctypes.data if max_dense else None,
+                       max_dense, C.byref(st))
+    return {"f": f, "dense": dense[:st.n_dense] if max_dense else None,
+            "dense_t": dense_t[:st.n_dense] if max_dense else None, "stats": st}
 
 
 def mc_inverse_mass(dim, vol):

@@ -714,6 +714,8 @@ typedef struct mc_dp5_user {
     int64_t ntup, nel;
     double* fph;
     const or_bath_sched* bs; /* non-NULL: the bath at the stage time */
+    int legs4;               /* 1: the 4-leg channel (idx [ntup][5]) */
+    double min_det, kappa;
 } mc_dp5_user;
 
 /* int eps f over the mesh for DG coefficients f: per element the exact
@@ -750,6 +752,11 @@ static double mc_dp5_flux(void* u, const double* dydt) {
 static void mc_dp5_rhs(void* u, double t, const double* y, double* dydt) {
     const mc_dp5_user* p = (const mc_dp5_user*)u;
     const int64_t len = p->nel * (p->dim + 1);
+    if (p->legs4) {
+        or_mc_apply_ee(p->dim, p->nsec, p->geo, p->eps, p->nu, p->pts, p->nmc, p->min_det, p->amplitude,
+                       p->kappa, p->idx, p->ntup, y, 1, dydt);
+        return;
+    }
     if (p->bs) {
         const double nb = or_sched_value(p->bs->s, p->bs->ns, p->bs->t_start, p->bs->dt, t);
         for (int64_t e = 0; e < p->nel; ++e) p->fph[e * (p->dim + 1)] = nb;
@@ -1249,4 +1256,23 @@ void or_mc_apply_ee(int dim, const int32_t* nsec, const double* geo, const doubl
         free(acc);
     }
     geo_free(&g);
+}
+
+/* MaterialTimeStatus::propagate (MaterialTimeStatus.cpp:102-137) with the
+ * 4-leg channel alone: DP5(4) over the electron band's DG coefficients with
+ * the fermion error norm (Band.cpp:87-101). */
+void or_dp5_mc_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                  const double* pts, int32_t nmc, double min_det, double amplitude, double kappa,
+                  const int32_t* idx, int64_t ntup, double* f, double t0, double t_final, double dense_dt,
+                  double rtol, double atol, double* dense_out, double* dense_t, int32_t max_dense,
+                  or_dp5_stats* stats) {
+    mc_dp5_user u;
+    memset(&u, 0, sizeof(u));
+    u.dim = dim; u.nsec = nsec; u.geo = geo; u.eps = eps; u.pts = pts;
+    u.amplitude = amplitude; u.rtol = rtol; u.atol = atol;
+    u.nu = nu; u.nmc = nmc; u.idx = idx; u.ntup = ntup;
+    u.nel = or_num_elements(dim, nsec);
+    u.legs4 = 1; u.min_det = min_det; u.kappa = kappa;
+    or_dp5_integrate_ex(u.nel * (dim + 1), f, t0, t_final, dense_dt, mc_dp5_rhs, mc_dp5_norm, &u, NULL, NULL,
+                        NULL, dense_out, dense_t, max_dense, NULL, NULL, 0, stats);
 }

@@ -398,6 +398,12 @@ int     or_mc_tuple_ee(int dim, const int32_t* nsec, const double* geo, const do
 void    or_mc_apply_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
                        const double* pts, int32_t nmc, double min_det, double amplitude, double kappa,
                        const int32_t* idx, int64_t ntup, const double* f, int32_t batch, double* out);
+/* DP5(4) of the electron band's DG state with the 4-leg channel alone. */
+void    or_dp5_mc_ee(int dim, const int32_t* nsec, const double* geo, const double* eps, int32_t nu,
+                     const double* pts, int32_t nmc, double min_det, double amplitude, double kappa,
+                     const int32_t* idx, int64_t ntup, double* f, double t0, double t_final,
+                     double dense_dt, double rtol, double atol, double* dense_out, double* dense_t,
+                     int32_t max_dense, or_dp5_stats* stats);
 /* Nodal values -> DG linear-form coefficients [nel][dim+1]. */
 void    or_mc_nodal_to_dg(int dim, const int32_t* nsec, const double* f, double* dg);
 

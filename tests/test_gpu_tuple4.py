@@ -12,7 +12,10 @@ Bar:
   - a member of a batch is bitwise equal to its single evaluation;
   - a list in another order (one sweep per umklapp vector, as the reference
     writes it) and a .scattIndices round trip give the oracle's result on
-    that list.
+    that list;
+  - DP5 with the 4-leg operator: the same attempted and accepted steps as the
+    oracle's DP5, dense outputs to 1e-10, particle number to 1e-12, members
+    of a batch bitwise equal to single runs.
 """
 import numpy as np
 import pytest
@@ -90,8 +93,35 @@ def test_tuple4_reference_order_and_file_round_trip(tmp_path):
     op2.load_indices(path)
     op2.set_points(pts)
     assert np.array_equal(op2.propagate(F), out)
-    for bad in (lambda: op.rates(F), lambda: op.dp5(F[0], 0.0, 1.0, 0.1, 1e-6, 1e-9, bath_n=0.1)):
-        with pytest.raises(tg.TgError):
-            bad()
+    with pytest.raises(tg.TgError):
+        op.rates(F)
     op.close()
     op2.close()
+
+
+def test_tuple4_dp5_matches_oracle_and_batches_bitwise():
+    print("[gpu] 4-leg tuple dp5 2D ns=3 t=2", flush=True)
+    dim, ns, nu, amp = 2, 3, 1, 0.02
+    w = wl.Workload("ee4_2d3", dim, (ns, ns))
+    eps = w.eps()
+    pts = oracle.mc_points(dim, 4, 50, 29)
+    idx = oracle.mc_tuples_ee(dim, w.nsec, w.origin, w.extent, eps, nu, pts)
+    op = tg.TupleOperator.create(w, n_umklapp=nu, max_batch=2, amplitude=amp, legs=4)
+    op.set_points(pts)
+    F = _states(w, 2, seed=5)
+    fg, dg, st = op.dp5(F[0], 0.0, 2.0, 0.2, 1e-8, 1e-12, max_dense=10)
+    r = oracle.dp5_mc_ee(dim, w.nsec, w.origin, w.extent, eps, nu, pts, idx, F[0], 0.0, 2.0, 0.2, 1e-8, 1e-12,
+                         amplitude=amp, max_dense=10)
+    print(f"[gpu]   attempted {st['attempted']} accepted {st['accepted']}", flush=True)
+    assert st["attempted"] == r["stats"].attempted and st["accepted"] == r["stats"].accepted
+    assert st["n_dense"] == r["stats"].n_dense == len(dg)
+    for a, b in zip(dg, r["dense"]):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    assert np.abs(fg - F[0]).max() > 1e-3
+    num0, num1 = _number(w, F[0]).sum(), _number(w, fg).sum()
+    assert abs(num1 - num0) <= 1e-12 * abs(num0)
+    fb, _, sb = op.dp5(F, 0.0, 2.0, 0.2, 1e-8, 1e-12, max_dense=10)
+    assert np.array_equal(fb[0], fg) and sb[0]["attempted"] == st["attempted"]
+    f1, _, _ = op.dp5(F[1], 0.0, 2.0, 0.2, 1e-8, 1e-12)
+    assert np.array_equal(fb[1], f1)
+    op.close()

@@ -1929,6 +1929,7 @@ tg_status launch_dyn(tg_tuple_ctx* c, const double* d_ft, double* d_out, int32_t
 template <bool RATES = false>
 tg_status propagate_device(tg_tuple_ctx* c, const double* d_f, const double* d_bath, double* d_out,
                            int32_t batch) {
+    if (!RATES && c->nlegs == 4) return propagate4_device(c, d_f, d_out, batch);
     const int nc = c->dim + 1;
     const int nbands = c->dyn ? 2 : 1;
     const int64_t total = (int64_t)batch * nbands * c->nel * nc;
@@ -2690,11 +2691,10 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
                                   double dense_dt, double rtol, double atol, const double* bath_n, int32_t flags,
                                   double* dense_out, int32_t max_dense, tg_step_stats* stats,
                                   double* h_accepted, int32_t h_cap) {
-    if (c && c->nlegs == 4) return fail(c, TG_ERR_STATE, "tg_tuple_dp5: 3-leg channels only");
     using namespace tg;
     const bool sched_on = (flags & TG_BATH_SCHEDULE) != 0;
     if (h_accepted && h_cap < 1) return TG_ERR_ARG;
-    if (!c || !f || (!bath_n && !c->dyn && !sched_on)) return TG_ERR_ARG;
+    if (!c || !f || (!bath_n && !c->dyn && !sched_on && c->nlegs == 3)) return TG_ERR_ARG;
     if (!c->have_points) return fail(c, TG_ERR_STATE, "no MC points yet (call tg_tuple_set_points)");
     if (nel != c->nel) return fail(c, TG_ERR_ARG, "nel=%lld, mesh has %lld elements", (long long)nel, (long long)c->nel);
     if (batch < 1 || batch > c->max_batch)
@@ -2767,7 +2767,8 @@ extern "C" tg_status tg_tuple_dp5(tg_tuple_ctx* c, double* f, int64_t nel, int32
     ctl.nsched = nullptr;
     ctl.bath = bs;
     ctl.bath.tab = sched_on ? c->d_bsched : nullptr;
-    ctl.flux = c->dyn ? nullptr : w->flux;   // with dynamical phonons there is no bath
+    // with dynamical phonons or four legs there is no bath
+    ctl.flux = (c->dyn || c->nlegs == 4) ? nullptr : w->flux;
     ctl.flux_stride = w->cap;
     std::vector<MemberDev> ms(batch);
     for (int b = 0; b < batch; ++b) {

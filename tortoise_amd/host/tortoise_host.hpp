@@ -523,8 +523,14 @@ public:
     tg_tuple_ctx* tupleBackend() {
         if (tctx_) return tctx_;
         const ScatteringChannel<NDim>* eph = nullptr;
-        for (const auto& c : channels)
-            if (c.legs == 3) eph = &c;
+        for (const auto& c : channels) {
+            if (c.legs == 4)
+                throw std::runtime_error(
+                    "the reference operator evaluates one e-ph channel; a 4-leg channel in tuple form is "
+                    "tg_tuple4_create, which this layer does not combine with the e-ph channel");
+            if (eph) throw std::runtime_error("the reference operator evaluates one e-ph channel");
+            eph = &c;
+        }
         if (!eph) throw std::runtime_error("the reference operator needs an e-ph channel");
         const Band<NDim>& el = (*this)[eph->electronBand];
         omegaPh_ = (*this)[eph->phononBand].dispersion.toNodal();
